@@ -1,0 +1,420 @@
+"""Write the golden input/output vectors that pin the oracle's FP32/FP64 rounding.
+
+Run ONLY in the build container, where the reference checkout exists:
+
+    PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py
+
+It imports the reference package (``meep_torch`` from /root/reference) for the
+functions whose floating-point trajectories no closed form pins -- the FP32/FP64
+state after N steps of ``step_tm``/``step_te``/``step_3d``/``step_lean``, the
+DFT accumulators of the flux/field monitors, the autograd gradient of a flux
+objective, and the ``Simulation`` flux of the GOLDEN_PLAIN scene -- and stores
+inputs + outputs as small ``.npz`` fixtures next to this file.  Nothing from the
+reference is copied: the fixtures are data (arrays), the reference never travels.
+
+Cases (all CPU, torch threads = 8):
+  pml        cpml_profiles (b, a) at several (n, npml, offset), f32 + f64
+  tm_small   2D TM 64x48, npml 8, waveguide + blob, point/line source, flux
+             lines (x- and y-normal), DFT-field monitors, 400 steps, plus a
+             chunked (checkpoint_every=150) rerun, f32 + f64
+  te_small   2D TE 64x48 analogue with FluxMonitorTE / FluxMonitorTEY
+  c1         C1 full size: 256x256 TM waveguide, 16-cell PML, 2000 steps,
+             50-frequency flux line; also times the reference CPU run
+  d3_small   3D 40x36x32, npml 6, point + plane sources, flux + DFT planes,
+             step_3d (f32, f64) and step_lean (f32)
+  grad_tm    autograd dJ/deps through run_tm(checkpoint_every=200), 48x48,
+             1400 steps, flux objective, f64 + f32
+  grad_3d    run_3d_adjoint eps gradient (30x24x20, 90 steps, chunk 16), f64
+  filters    conic/gaussian filter, tanh projection, DesignRegion.paint, f64
+  golden_plain  mp.Simulation GOLDEN_PLAIN cell (100x60 TM guide), f32 + f64
+"""
+import hashlib
+import json
+import math
+import os
+import sys
+import time
+
+sys.dont_write_bytecode = True
+REF = "/root/reference"
+if REF not in sys.path:
+    sys.path.insert(0, REF)
+
+import numpy as np  # noqa: E402
+import torch  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+torch.set_num_threads(8)
+
+
+def _np(t):
+    return t.detach().cpu().numpy().copy()
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, f"{name}.npz")
+    out = {}
+    for k, v in arrays.items():
+        if isinstance(v, torch.Tensor):
+            v = _np(v)
+        out[k] = np.asarray(v)
+    np.savez_compressed(path, **out)
+    return path
+
+
+def splitmix_uniform(n, seed):
+    """Counter-based U[0,1) floats (splitmix64); the same generator lives in
+    the package's synthetic-input module so the GPU box regenerates inputs."""
+    out = np.empty(n, dtype=np.float64)
+    mask = (1 << 64) - 1
+    for i in range(n):
+        z = (seed + (i + 1) * 0x9E3779B97F4A7C15) & mask
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        z = z ^ (z >> 31)
+        out[i] = (z >> 11) * (1.0 / (1 << 53))
+    return out
+
+
+# --------------------------------------------------------------------- pml
+def case_pml():
+    from meep_torch.core.pml import cpml_profiles
+    combos = [(256, 16, 0.0), (255, 16, 0.5), (64, 8, 0.0), (63, 8, 0.5),
+              (2048, 32, 0.0), (2047, 32, 0.5), (100, (10, 0), 0.0),
+              (99, (0, 7), 0.5), (40, 6, 0.0), (39, 6, 0.5)]
+    arrays = {}
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        for ci, (n, npml, off) in enumerate(combos):
+            dx = 0.05
+            dt = 0.5 * dx
+            b, a, k = cpml_profiles(n, npml, off, dt, dx, dtype=dtype)
+            arrays[f"{tag}_{ci}_b"] = b
+            arrays[f"{tag}_{ci}_a"] = a
+            arrays[f"{tag}_{ci}_k"] = k
+    meta = [[n, list(npml) if isinstance(npml, tuple) else npml, off]
+            for n, npml, off in combos]
+    arrays["combos_json"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    save("pml", **arrays)
+
+
+# --------------------------------------------------------------- 2D helpers
+def _eps2d(nx, ny, dtype):
+    eps = torch.ones(nx, ny, dtype=torch.float64)
+    eps[:, 20:28] = 12.0                       # horizontal guide
+    x = torch.arange(nx, dtype=torch.float64)[:, None]
+    y = torch.arange(ny, dtype=torch.float64)[None, :]
+    blob = torch.exp(-((x - 40.0) ** 2 + (y - 12.0) ** 2) / (2 * 3.0 ** 2))
+    eps = eps + 3.0 * blob
+    return eps.to(dtype)
+
+
+def case_tm_small():
+    from meep_torch.core.coeffs import build_tm_coeffs
+    from meep_torch.core.grid import Grid
+    from meep_torch.core.monitors import DFTFieldsMonitor, FluxMonitorTM
+    from meep_torch.core.run import run_tm
+    from meep_torch.core.sources import GaussianPulse, LineSource
+    from meep_torch.core.stepper_tm import init_tm_state, step_tm
+
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        g = Grid(nx=64, ny=48, dx=0.05, courant=0.5, dtype=dtype, device="cpu")
+        eps = _eps2d(64, 48, dtype)
+        m = build_tm_coeffs(g, eps_inf=eps, npml=8)
+        freqs = [0.3, 0.4, 0.5, 0.6, 0.7]
+        out = {"eps": eps}
+        for sname, src in (
+                ("point", LineSource(g, ix=12, j0=24, j1=25,
+                                     profile=GaussianPulse(0.5, 1.0))),
+                ("line", LineSource(g, ix=12, j0=16, j1=32,
+                                    profile=GaussianPulse(0.5, 1.0)))):
+            out[f"{sname}_mask"] = src.mask
+            out[f"{sname}_amp"] = np.array([src.amplitude(t) for t in range(400)])
+            for ck in (0, 150):
+                mons = [FluxMonitorTM(g, freqs, x_normal=True, idx=50, lo=10, hi=38),
+                        FluxMonitorTM(g, freqs, x_normal=False, idx=36, lo=10, hi=54),
+                        DFTFieldsMonitor(g, [0.45, 0.55], component="Ez"),
+                        DFTFieldsMonitor(g, [0.5], component="Hx")]
+                st, acc = run_tm(init_tm_state(g), m, 400, source=src,
+                                 monitors=mons, checkpoint_every=ck)
+                p = f"{sname}_ck{ck}"
+                for f in st._fields:
+                    v = getattr(st, f)
+                    if isinstance(v, torch.Tensor):
+                        out[f"{p}_{f}"] = v
+                for i, a in enumerate(acc):
+                    out[f"{p}_acc{i}"] = a
+                out[f"{p}_fluxx"] = mons[0].flux(acc[0], g.dx)
+                out[f"{p}_fluxy"] = mons[1].flux(acc[1], g.dx)
+            # early states (debug aid): after 1 and 10 steps, point/line source
+            st = init_tm_state(g)
+            for t in range(10):
+                st = step_tm(st, m, src.amplitude(t), src.mask)
+                if t in (0, 9):
+                    out[f"{sname}_s{t + 1}_Ez"] = st.Ez
+                    out[f"{sname}_s{t + 1}_Hx"] = st.Hx
+                    out[f"{sname}_s{t + 1}_Hy"] = st.Hy
+        # a state that starts from a nonzero, inconsistent Ez/Dz pair
+        st0 = init_tm_state(g)
+        x = torch.arange(64, dtype=torch.float64)[:, None]
+        y = torch.arange(48, dtype=torch.float64)[None, :]
+        G = torch.exp(-((x - 30.0) ** 2 + (y - 24.0) ** 2) / (2 * 4.0 ** 2)).to(dtype)
+        st0 = st0._replace(Ez=G.clone(), Dz=(1.5 * G).clone())
+        st, _ = run_tm(st0, m, 50)
+        out["blob_Ez0"] = G
+        for f in ("Ez", "Dz", "Hx", "Hy", "psi_Ezy", "psi_Ezx", "psi_Hyx", "psi_Hxy"):
+            out[f"blob_{f}"] = getattr(st, f)
+        save(f"tm_small_{tag}", **out)
+
+
+def case_te_small():
+    from meep_torch.core.coeffs import build_te_coeffs
+    from meep_torch.core.grid import Grid
+    from meep_torch.core.monitors import DFTFieldsMonitor, FluxMonitorTE, FluxMonitorTEY
+    from meep_torch.core.run import run_te
+    from meep_torch.core.sources import GaussianPulse, LineSource
+    from meep_torch.core.stepper_te import init_te_state
+
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        g = Grid(nx=64, ny=48, dx=0.05, courant=0.5, dtype=dtype, device="cpu")
+        eps = _eps2d(64, 48, dtype)
+        m = build_te_coeffs(g, eps_inf=eps, npml=8)
+        freqs = [0.3, 0.4, 0.5, 0.6, 0.7]
+        src = LineSource(g, ix=12, j0=16, j1=32, profile=GaussianPulse(0.5, 1.0))
+        src.mask = src.mask[:, :-1].contiguous()          # Jy on the Ey grid
+        out = {"eps": eps, "mask": src.mask,
+               "amp": np.array([src.amplitude(t) for t in range(400)])}
+        for ck in (0, 150):
+            mons = [FluxMonitorTE(g, freqs, ix=50, lo=10, hi=38),
+                    FluxMonitorTEY(g, freqs, iy=36, lo=10, hi=54),
+                    DFTFieldsMonitor(g, [0.45, 0.55], component="Hz"),
+                    DFTFieldsMonitor(g, [0.5], component="Ey")]
+            st, acc = run_te(init_te_state(g), m, 400, source=src, monitors=mons,
+                             checkpoint_every=ck)
+            p = f"ck{ck}"
+            for f in st._fields:
+                v = getattr(st, f)
+                if isinstance(v, torch.Tensor):
+                    out[f"{p}_{f}"] = v
+            for i, a in enumerate(acc):
+                out[f"{p}_acc{i}"] = a
+            out[f"{p}_fluxx"] = mons[0].flux(acc[0], g.dx)
+            out[f"{p}_fluxy"] = mons[1].flux(acc[1], g.dx)
+        save(f"te_small_{tag}", **out)
+
+
+def c1_inputs(dtype):
+    """C1: 256x256, dx=0.05, Courant 0.5, eps=12 rows [123,133), npml 16,
+    point Ez source at (24, 128), GaussianPulse(0.15, 0.1), flux line x=232,
+    rows [96, 160), 50 freqs in [0.1, 0.2]."""
+    eps = torch.ones(256, 256, dtype=dtype)
+    eps[:, 123:133] = 12.0
+    return eps
+
+
+def case_c1():
+    from meep_torch.core.coeffs import build_tm_coeffs
+    from meep_torch.core.grid import Grid
+    from meep_torch.core.monitors import FluxMonitorTM
+    from meep_torch.core.run import run_tm
+    from meep_torch.core.sources import GaussianPulse, LineSource
+    from meep_torch.core.stepper_tm import init_tm_state
+
+    timing = {}
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        g = Grid(nx=256, ny=256, dx=0.05, courant=0.5, dtype=dtype, device="cpu")
+        eps = c1_inputs(dtype)
+        m = build_tm_coeffs(g, eps_inf=eps, npml=16)
+        src = LineSource(g, ix=24, j0=128, j1=129, profile=GaussianPulse(0.15, 0.1))
+        freqs = np.linspace(0.1, 0.2, 50).tolist()
+        mon = FluxMonitorTM(g, freqs, x_normal=True, idx=232, lo=96, hi=160)
+        t0 = time.perf_counter()
+        st, (acc,) = run_tm(init_tm_state(g), m, 2000, source=src, monitors=[mon])
+        dtw = time.perf_counter() - t0
+        timing[tag] = dtw
+        save(f"c1_{tag}", eps=eps, Ez=st.Ez, Hx=st.Hx, Hy=st.Hy,
+             acc=acc, flux=mon.flux(acc, g.dx), wall_s=np.array(dtw),
+             threads=np.array(torch.get_num_threads()))
+    return timing
+
+
+# ----------------------------------------------------------------------- 3D
+def _eps3d(nx, ny, nz, dtype):
+    eps = torch.full((nx, ny, nz), 2.1, dtype=torch.float64)
+    eps[:, :, nz // 2:] = 1.0
+    eps[:, ny // 2 - 5:ny // 2 + 5, nz // 2 - 3:nz // 2 + 3] = 12.1
+    return eps.to(dtype)
+
+
+def case_3d_small():
+    from meep_torch.core.monitors_3d import (DFTFieldsMonitor3D, FluxMonitor3D,
+                                             PlaneSource, PointSource3D, run_3d)
+    from meep_torch.core.sources import GaussianPulse
+    from meep_torch.core.stepper_3d import build_3d_coeffs, init_3d_state
+    from meep_torch.core.stepper_3d_lean import build_lean, run_3d_lean
+
+    nx, ny, nz, npml, dx, n = 32, 28, 24, 5, 0.05, 200
+    prof = GaussianPulse(1.0, 2.0, cutoff=2.0)
+    freqs = [0.8, 1.0, 1.2, 1.4]
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        eps = _eps3d(nx, ny, nz, dtype)
+        c = build_3d_coeffs(nx, ny, nz, dx, eps=eps, npml=npml, dtype=dtype)
+        out = {"eps": eps}
+        srcs = [("point", PointSource3D(c.dt, "Ez", 8, ny // 2, nz // 2, prof))]
+        if dtype == torch.float64:
+            srcs.append(("plane", PlaneSource(nx, ny, nz, c.dt, "Ey", 0, 8, prof,
+                                              lo=npml, hi=min(nx, ny, nz) - npml)))
+        for sname, src in srcs:
+            mons = [FluxMonitor3D(c.dt, freqs, normal="x", idx=24, ap=npml + 1,
+                                  dtype=dtype),
+                    FluxMonitor3D(c.dt, freqs, normal="z", idx=14, ap=npml + 1,
+                                  dtype=dtype),
+                    DFTFieldsMonitor3D(c.dt, [1.1], "Ez", plane=(2, 12),
+                                       dtype=dtype),
+                    DFTFieldsMonitor3D(c.dt, [1.0], "Hy", plane=(0, 20),
+                                       dtype=dtype)]
+            st, acc = run_3d(init_3d_state(nx, ny, nz, npml=npml, dtype=dtype),
+                             c, n, source=src, monitors=mons)
+            for f in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+                out[f"{sname}_{f}"] = getattr(st, f)
+            out[f"{sname}_psi_Ez_x"] = st.psi_Ez_x
+            for i, a in enumerate(acc):
+                out[f"{sname}_acc{i}"] = a
+            out[f"{sname}_flux0"] = mons[0].flux(acc[0], dx)
+            out[f"{sname}_flux1"] = mons[1].flux(acc[1], dx)
+            out[f"{sname}_amp"] = np.array([src.amplitude(t) for t in range(n)])
+        if dtype == torch.float32:
+            cl = build_lean(nx, ny, nz, dx, eps, npml, dtype=dtype, device="cpu")
+            src = PointSource3D(cl.dt, "Ez", 8, ny // 2, nz // 2, prof)
+            mon = FluxMonitor3D(cl.dt, freqs, normal="x", idx=24, ap=npml + 1,
+                                dtype=dtype)
+            view, acc = run_3d_lean(cl, n, source=src, monitors=[mon])
+            for f in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+                out[f"lean_{f}"] = getattr(view, f)
+            out["lean_acc0"] = acc[0]
+        save(f"d3_small_{tag}", **out)
+
+
+# ------------------------------------------------------------------ grads
+def _blob_eps(theta, nx, ny, dtype, cx=24.0, cy=24.0, r=7.0, edge=1.0):
+    x = torch.arange(nx, dtype=dtype)[:, None]
+    y = torch.arange(ny, dtype=dtype)[None, :]
+    dist = torch.sqrt((x - cx) ** 2 + (y - cy) ** 2)
+    disk = 0.5 * (1.0 - torch.tanh((dist - r) / edge))
+    return 1.0 + theta * disk
+
+
+def case_grad_tm():
+    from meep_torch.core.coeffs import build_tm_coeffs
+    from meep_torch.core.grid import Grid
+    from meep_torch.core.monitors import FluxMonitorTM
+    from meep_torch.core.run import run_tm
+    from meep_torch.core.sources import GaussianPulse, LineSource
+    from meep_torch.core.stepper_tm import init_tm_state
+
+    for dtype, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+        g = Grid(nx=48, ny=48, dx=0.05, dtype=dtype, device="cpu")
+        eps = _blob_eps(0.8, 48, 48, dtype).detach().clone().requires_grad_(True)
+        m = build_tm_coeffs(g, eps_inf=eps, npml=6)
+        src = LineSource(g, ix=8, j0=18, j1=30, profile=GaussianPulse(1.0, 0.4),
+                         amp_func=lambda y: 5.0)
+        mon = FluxMonitorTM(g, [0.9, 1.0, 1.1], x_normal=True, idx=34, lo=16, hi=32)
+        _, (acc,) = run_tm(init_tm_state(g), m, 1400, source=src, monitors=[mon],
+                           checkpoint_every=200)
+        J = mon.flux(acc, g.dx).sum()
+        J.backward()
+        save(f"grad_tm_{tag}", eps=eps.detach(), mask=src.mask, J=J.detach(),
+             acc=acc.detach(), grad=eps.grad,
+             amp=np.array([src.amplitude(t) for t in range(1400)]))
+
+
+def case_grad_3d():
+    from meep_torch.core.adjoint3d import run_3d_adjoint
+    from meep_torch.core.monitors_3d import DFTFieldsMonitor3D, PointSource3D
+    from meep_torch.core.sources import GaussianPulse
+    from meep_torch.core.stepper_3d import build_3d_coeffs, init_3d_state
+
+    nx, ny, nz = 30, 24, 20
+    for dtype, tag in ((torch.float64, "f64"),):
+        eps = torch.full((nx, ny, nz), 1.5, dtype=dtype)
+        eps[12:18, 9:15, 8:12] = 4.0
+        eps = eps.requires_grad_(True)
+        c = build_3d_coeffs(nx, ny, nz, 0.05, eps=eps, npml=6, dtype=dtype)
+        src = PointSource3D(c.dt, "Ez", nx // 2, ny // 2, nz // 2 - 3,
+                            GaussianPulse(0.6, 0.25))
+        mon = DFTFieldsMonitor3D(c.dt, [0.6], "Ez", plane=(0, nx - 10), dtype=dtype)
+        st0 = init_3d_state(nx, ny, nz, npml=6, dtype=dtype)
+
+        def loss_fn(accs):
+            E2 = accs[0][0, 0] ** 2 + accs[0][0, 1] ** 2
+            return E2.sum()
+
+        L, accs = run_3d_adjoint(c, 90, 16, src, [mon], loss_fn, st0)
+        save(f"grad_3d_{tag}", eps=eps.detach(), L=np.array(L), acc=accs[0],
+             grad=eps.grad)
+
+
+# ---------------------------------------------------------------- filters
+def case_filters():
+    from meep_torch.adjoint.filters import conic_filter, gaussian_filter, tanh_projection
+    from meep_torch.adjoint.problem import DesignRegion, MaterialGrid
+
+    rho = torch.tensor(splitmix_uniform(32 * 24, 7).reshape(32, 24), dtype=torch.float64)
+    cf = conic_filter(rho, 0.2, 0.05)
+    gf = gaussian_filter(rho, 0.1, 0.05)
+    tp = tanh_projection(cf, 8.0, 0.5)
+    dr = DesignRegion(MaterialGrid((32, 24), 2.1, 12.1), ix0=10, ix1=42, iy0=5, iy1=29)
+    base = torch.ones(60, 40, dtype=torch.float64)
+    painted = dr.paint(base, tp)
+    dr2 = DesignRegion(MaterialGrid((32, 24), 2.1, 12.1), ix0=4, ix1=52, iy0=3, iy1=39)
+    painted2 = dr2.paint(base, tp)
+    save("filters", rho=rho, conic=cf, gauss=gf, tanh=tp, painted=painted,
+         painted2=painted2)
+
+
+# ----------------------------------------------------------- golden plain
+def case_golden_plain():
+    import meep_torch.mp as mp
+    out = {}
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        sim = mp.Simulation(
+            cell_size=mp.Vector3(10, 6), resolution=10,
+            geometry=[mp.Block(center=mp.Vector3(), size=mp.Vector3(mp.inf, 1, mp.inf),
+                               material=mp.Medium(epsilon=12))],
+            sources=[mp.Source(mp.GaussianSource(0.15, fwidth=0.1), component=mp.Ez,
+                               center=mp.Vector3(-3, 0), size=mp.Vector3(0, 2))],
+            boundary_layers=[mp.PML(1.0)], dtype=dtype)
+        fl = sim.add_flux(0.15, 0.1, 3, mp.FluxRegion(center=mp.Vector3(3, 0),
+                                                      size=mp.Vector3(0, 4)))
+        sim.run(until=40)
+        out[f"{tag}_flux"] = np.array(mp.get_fluxes(fl))
+        out[f"{tag}_freqs"] = np.array(mp.get_flux_freqs(fl))
+        out[f"{tag}_eps"] = sim.get_array(component=mp.Dielectric)
+        out[f"{tag}_Ez"] = sim.get_array(component=mp.Ez)
+    save("golden_plain", **out)
+
+
+def main():
+    which = sys.argv[1:] or ["pml", "tm", "te", "c1", "3d", "grad_tm", "grad_3d",
+                             "filters", "golden_plain"]
+    t_all = time.perf_counter()
+    timing = {}
+    for w in which:
+        t0 = time.perf_counter()
+        {"pml": case_pml, "tm": case_tm_small, "te": case_te_small,
+         "c1": lambda: timing.update(c1=case_c1()), "3d": case_3d_small,
+         "grad_tm": case_grad_tm, "grad_3d": case_grad_3d,
+         "filters": case_filters, "golden_plain": case_golden_plain}[w]()
+        print(f"[golden] {w}: {time.perf_counter() - t0:.1f}s", flush=True)
+    man = {}
+    for fn in sorted(os.listdir(HERE)):
+        if fn.endswith(".npz"):
+            with open(os.path.join(HERE, fn), "rb") as fh:
+                man[fn] = hashlib.sha256(fh.read()).hexdigest()
+    with open(os.path.join(HERE, "MANIFEST.json"), "w") as fh:
+        json.dump({"sha256": man, "torch": torch.__version__,
+                   "threads": torch.get_num_threads(), "timing": timing}, fh, indent=1)
+    print(f"[golden] total {time.perf_counter() - t_all:.1f}s")
+
+
+if __name__ == "__main__":
+    main()
