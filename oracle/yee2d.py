@@ -1,0 +1,340 @@
+"""2D TM/TE restatement (CPU, torch eager) -- test infrastructure only.
+
+Operation order follows the reference so FP32 results round identically:
+  profiles      meep_torch/core/pml.py:17-59
+  TM coeffs     meep_torch/core/coeffs.py:134-291 (eps, npml, y_pmc subset)
+  TE coeffs     meep_torch/core/coeffs.py:357-503 (eps, npml subset)
+  TM step       meep_torch/core/stepper_tm.py:78-254 (no poles/sigma/mirrors)
+  TE step       meep_torch/core/stepper_te.py:73-190 (no poles/sigma/tensor eps)
+  sources       meep_torch/core/sources.py:26-91
+  monitors      meep_torch/core/monitors.py:11-155
+  run loop      meep_torch/core/run.py:10-126
+"""
+import math
+from dataclasses import dataclass
+
+import torch
+
+
+# ------------------------------------------------------------------ profiles
+def _sides(npml):
+    if isinstance(npml, (tuple, list)):
+        return int(npml[0]), int(npml[1])
+    return int(npml), int(npml)
+
+
+def cpml_profiles(n, npml, offset, dt, dx, dtype=torch.float64):
+    """CFS-PML (b, a, kappa) at x = i + offset (meep_torch/core/pml.py:17-59).
+
+    sigma = -(m+1) ln(R0) / (2 n dx) u^m with m = 3, R0 = 1e-6; alpha = 0.05 (1-u)
+    inside the layer; kappa = 1; b = exp(-(sigma + alpha) dt);
+    a = sigma (b - 1) / (sigma + alpha) where sigma > 0.  The high wall is at
+    n - 1 + 2 offset.
+    """
+    m, R0, alpha_max = 3, 1e-6, 0.05
+    lo, hi = _sides(npml)
+    pos = torch.arange(n, dtype=dtype) + offset
+    u = torch.zeros(n, dtype=dtype)
+    sig = torch.zeros_like(u)
+    if lo > 0:
+        ul = torch.clamp((lo - pos) / lo, min=0.0)
+        u = u + ul
+        sig = sig + (-(m + 1) * math.log(R0) / (2.0 * lo * dx)) * ul ** m
+    if hi > 0:
+        wall = n - 1 + 2.0 * offset
+        uh = torch.clamp((pos - (wall - hi)) / hi, min=0.0)
+        u = u + uh
+        sig = sig + (-(m + 1) * math.log(R0) / (2.0 * hi * dx)) * uh ** m
+    u = torch.clamp(u, max=1.0)
+    alpha = torch.where(u > 0, alpha_max * (1.0 - u), torch.zeros_like(u))
+    kap = torch.ones_like(u)
+    b = torch.exp(-(sig / kap + alpha) * dt)
+    den = kap * (sig + kap * alpha)
+    a = torch.where(sig > 0, sig * (b - 1.0) / den.clamp(min=1e-300), torch.zeros_like(b))
+    return b, a, kap
+
+
+# -------------------------------------------------------------------- coeffs
+@dataclass
+class Coef2D:
+    dt: float
+    dx: float
+    eps: torch.Tensor          # TM: eps (nx, ny); TE: cell eps (nx, ny)
+    eps_x: torch.Tensor = None  # TE: (nx-1, ny)
+    eps_y: torch.Tensor = None  # TE: (nx, ny-1)
+    y_pmc: bool = False
+    # profiles as broadcastable tensors (same slicing as the reference)
+    b_hy: torch.Tensor = None
+    a_hy: torch.Tensor = None
+    b_hx: torch.Tensor = None
+    a_hx: torch.Tensor = None
+    b_ex: torch.Tensor = None
+    a_ex: torch.Tensor = None
+    b_ey: torch.Tensor = None
+    a_ey: torch.Tensor = None
+
+
+def tm_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, y_pmc=False):
+    """meep_torch/core/coeffs.py:134-291 restricted to eps + CPML (+ y_pmc)."""
+    dt = courant * dx
+    eps = eps.to(dtype)
+    if isinstance(npml, (tuple, list)):
+        px, py = _sides(npml[0]), _sides(npml[1])
+    else:
+        px = py = (int(npml), int(npml))
+
+    def prof(n, off, pair):
+        return cpml_profiles(n, pair, off, dt, dx, dtype)
+
+    b_hy, a_hy, _ = prof(ny - 1, 0.5, (0, 0) if y_pmc else py)
+    b_hx, a_hx, _ = prof(nx - 1, 0.5, px)
+    b_exf, a_exf, _ = prof(nx, 0.0, px)
+    b_eyf, a_eyf, _ = prof(ny, 0.0, (0, 0) if y_pmc else py)
+    if y_pmc:
+        b_ey, a_ey = b_eyf, a_eyf
+    else:
+        b_ey, a_ey = b_eyf[1:-1], a_eyf[1:-1]
+    return Coef2D(dt=dt, dx=dx, eps=eps, y_pmc=y_pmc,
+                  b_hy=b_hy[None, :], a_hy=a_hy[None, :],
+                  b_hx=b_hx[:, None], a_hx=a_hx[:, None],
+                  b_ex=b_exf[1:-1][:, None], a_ex=a_exf[1:-1][:, None],
+                  b_ey=b_ey[None, :], a_ey=a_ey[None, :])
+
+
+def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32):
+    """meep_torch/core/coeffs.py:357-503 restricted to eps + CPML; eps_x/eps_y
+    are the two-point averages of coeffs.py:347-355."""
+    dt = courant * dx
+    eps = eps.to(dtype)
+    eps_x = 0.5 * (eps[:-1, :] + eps[1:, :])
+    eps_y = 0.5 * (eps[:, :-1] + eps[:, 1:])
+    if isinstance(npml, (tuple, list)):
+        px, py = _sides(npml[0]), _sides(npml[1])
+    else:
+        px = py = (int(npml), int(npml))
+
+    def prof(n, off, pair):
+        return cpml_profiles(n, pair, off, dt, dx, dtype)
+
+    b_hx, a_hx, _ = prof(nx - 1, 0.5, px)
+    b_hy, a_hy, _ = prof(ny - 1, 0.5, py)
+    b_eyf, a_eyf, _ = prof(ny, 0.0, py)
+    b_exf, a_exf, _ = prof(nx, 0.0, px)
+    return Coef2D(dt=dt, dx=dx, eps=eps, eps_x=eps_x, eps_y=eps_y,
+                  b_hx=b_hx[:, None], a_hx=a_hx[:, None],
+                  b_hy=b_hy[None, :], a_hy=a_hy[None, :],
+                  b_ey=b_eyf[1:-1][None, :], a_ey=a_eyf[1:-1][None, :],
+                  b_ex=b_exf[1:-1][:, None], a_ex=a_exf[1:-1][:, None])
+
+
+# --------------------------------------------------------------------- state
+def tm_state(nx, ny, dtype=torch.float32, y_pmc=False):
+    """Field dict with the TMState shapes of stepper_tm.py:12-71."""
+    z = lambda *s: torch.zeros(*s, dtype=dtype)
+    ny_e = ny if y_pmc else ny - 2
+    return {"Ez": z(nx, ny), "Hx": z(nx, ny - 1), "Hy": z(nx - 1, ny), "Dz": z(nx, ny),
+            "psi_Ezy": z(nx, ny - 1), "psi_Ezx": z(nx - 1, ny),
+            "psi_Hyx": z(nx - 2, ny_e), "psi_Hxy": z(nx - 2, ny_e)}
+
+
+def te_state(nx, ny, dtype=torch.float32):
+    """Field dict with the TEState shapes of stepper_te.py:11-49."""
+    z = lambda *s: torch.zeros(*s, dtype=dtype)
+    return {"Hz": z(nx - 1, ny - 1), "Ex": z(nx - 1, ny), "Ey": z(nx, ny - 1),
+            "Dx": z(nx - 1, ny), "Dy": z(nx, ny - 1),
+            "psi_Eyx": z(nx - 1, ny - 1), "psi_Exy": z(nx - 1, ny - 1),
+            "psi_Hzy": z(nx - 1, ny - 2), "psi_Hzx": z(nx - 2, ny - 1)}
+
+
+# ---------------------------------------------------------------------- steps
+def tm_step(s, c, coef=0.0, mask=None):
+    """One TM step: B half (stepper_tm.py:85-92,126-128), D half
+    (stepper_tm.py:134-165, PEC or y_pmc), soft source (stepper_tm.py:197-200:
+    Dz -= (dt*amp) * mask, the scalar formed in double first), then
+    Ez = Dz / eps (stepper_tm.py:226-227).  ``coef`` is dt*amp."""
+    dx, dt = c.dx, c.dt
+    Ez = s["Ez"]
+    g = (Ez[:, 1:] - Ez[:, :-1]) / dx
+    pzy = c.b_hy * s["psi_Ezy"] + c.a_hy * g
+    Hx = s["Hx"] - dt * (g + pzy)
+    g2 = (Ez[1:, :] - Ez[:-1, :]) / dx
+    pzx = c.b_hx * s["psi_Ezx"] + c.a_hx * g2
+    Hy = s["Hy"] + dt * (g2 + pzx)
+
+    if c.y_pmc:
+        Hxp = torch.nn.functional.pad(Hx, (1, 1))
+        ys, pad_y = slice(None), (0, 0)
+    else:
+        Hxp, ys, pad_y = Hx, slice(1, -1), (1, 1)
+    dHx = (Hxp[1:-1, 1:] - Hxp[1:-1, :-1]) / dx
+    dHy = (Hy[1:, ys] - Hy[:-1, ys]) / dx
+    pyx = c.b_ex * s["psi_Hyx"] + c.a_ex * dHy
+    pxy = c.b_ey * s["psi_Hxy"] + c.a_ey * dHx
+    curl = dHy + pyx - dHx - pxy
+    Dz = s["Dz"] + dt * torch.nn.functional.pad(curl, pad_y + (1, 1))
+    if mask is not None:
+        Dz = Dz - coef * mask
+    out = dict(s)
+    out.update(Hx=Hx, Hy=Hy, Dz=Dz, Ez=Dz / c.eps, psi_Ezy=pzy, psi_Ezx=pzx,
+               psi_Hyx=pyx, psi_Hxy=pxy)
+    return out
+
+
+def te_step(s, c, coef=0.0, mask=None):
+    """One TE step entered at the D half (stepper_te.py:178-190): D from curl
+    Hz (stepper_te.py:80-103), source on Dy (stepper_te.py:131-134), E = D/eps
+    (stepper_te.py:160-168), then Hz from curl E (stepper_te.py:107-117)."""
+    dx, dt = c.dx, c.dt
+    Hz = s["Hz"]
+    gy = (Hz[:, 1:] - Hz[:, :-1]) / dx
+    phy = c.b_ey * s["psi_Hzy"] + c.a_ey * gy
+    cx = torch.nn.functional.pad(gy + phy, (1, 1))
+    gx = (Hz[1:, :] - Hz[:-1, :]) / dx
+    phx = c.b_ex * s["psi_Hzx"] + c.a_ex * gx
+    cy = torch.nn.functional.pad(gx + phx, (0, 0, 1, 1))
+    Dx = s["Dx"] + dt * cx
+    Dy = s["Dy"] - dt * cy
+    if mask is not None:
+        Dy = Dy - coef * mask
+    Ex = Dx / c.eps_x
+    Ey = Dy / c.eps_y
+    dEy = (Ey[1:, :] - Ey[:-1, :]) / dx
+    dEx = (Ex[:, 1:] - Ex[:, :-1]) / dx
+    pyx = c.b_hx * s["psi_Eyx"] + c.a_hx * dEy
+    pxy = c.b_hy * s["psi_Exy"] + c.a_hy * dEx
+    Hz2 = Hz - dt * (dEy + pyx - dEx - pxy)
+    out = dict(s)
+    out.update(Dx=Dx, Dy=Dy, Ex=Ex, Ey=Ey, Hz=Hz2, psi_Hzy=phy, psi_Hzx=phx,
+               psi_Eyx=pyx, psi_Exy=pxy)
+    return out
+
+
+# -------------------------------------------------------------------- sources
+def gaussian_pulse(fcen, fwidth, cutoff=5.0):
+    """meep_torch/core/sources.py:26-36: t -> cos(2 pi f (t-t0)) exp(-(t-t0)^2/(2w^2)),
+    w = 1/fwidth, t0 = cutoff*w."""
+    w = 1.0 / fwidth
+    t0 = cutoff * w
+
+    def amp(t):
+        return math.cos(2 * math.pi * fcen * (t - t0)) * math.exp(-((t - t0) ** 2) / (2 * w ** 2))
+    return amp
+
+
+def line_mask(nx, ny, ix, j0, j1, dx, dtype=torch.float32, weights=None):
+    """meep_torch/core/sources.py:49-78: unit (or per-cell) weights on
+    x = ix, j in [j0, j1), scaled by dx/dx^2 for a segment and 1/dx^2 for a
+    single cell."""
+    m = torch.zeros(nx, ny, dtype=dtype)
+    if weights is None:
+        m[ix, j0:j1] = 1.0
+    else:
+        m[ix, j0:j1] = torch.as_tensor(weights, dtype=dtype)
+    per = dx if (j1 - j0) > 1 else 1.0
+    return m * (per / dx ** 2)
+
+
+# ------------------------------------------------------------------ monitors
+def _phasor(freqs, dtype):
+    """2*pi*f as a grid-dtype tensor (monitors.py:65,86)."""
+    return 2 * math.pi * torch.tensor(freqs, dtype=dtype)
+
+
+def flux_line_sample(kind, s, idx, lo, hi, freqs, dt, t_idx, dtype):
+    """Per-step (F, 4, L) DFT sample of a flux line (monitors.py:71-95,
+    113-122, 141-151): dt * [cos(w tE) E, -sin(w tE) E, cos(w tH) H, -sin(w tH) H].
+
+    kind: 'tm_x' | 'tm_y' | 'te_x' | 'te_y'.
+    """
+    if kind == "tm_x":
+        E = s["Ez"][idx, lo:hi]
+        H = 0.5 * (s["Hy"][idx - 1, lo:hi] + s["Hy"][idx, lo:hi])
+        tE, tH = (t_idx + 1.0) * dt, (t_idx + 0.5) * dt
+    elif kind == "tm_y":
+        E = s["Ez"][lo:hi, idx]
+        H = 0.5 * (s["Hx"][lo:hi, idx - 1] + s["Hx"][lo:hi, idx])
+        tE, tH = (t_idx + 1.0) * dt, (t_idx + 0.5) * dt
+    elif kind == "te_x":
+        E = s["Ey"][idx, lo:hi]
+        H = 0.5 * (s["Hz"][idx - 1, lo:hi] + s["Hz"][idx, lo:hi])
+        tE, tH = (t_idx + 0.5) * dt, (t_idx + 1.0) * dt
+    else:
+        E = s["Ex"][lo:hi, idx]
+        H = 0.5 * (s["Hz"][lo:hi, idx - 1] + s["Hz"][lo:hi, idx])
+        tE, tH = (t_idx + 0.5) * dt, (t_idx + 1.0) * dt
+    w = _phasor(freqs, dtype)[:, None]
+    return dt * torch.stack([torch.cos(w * tE) * E, -torch.sin(w * tE) * E,
+                             torch.cos(w * tH) * H, -torch.sin(w * tH) * H], dim=1)
+
+
+def flux_from_acc(kind, acc, dx):
+    """monitors.py:97-100,124-126,153-155: -/+ 1/2 sum(ReE ReH + ImE ImH) dx."""
+    ReE, ImE, ReH, ImH = acc.unbind(1)
+    s = (ReE * ReH + ImE * ImH).sum(-1) * dx
+    return {"tm_x": -0.5, "tm_y": 0.5, "te_x": 0.5, "te_y": -0.5}[kind] * s
+
+
+def dft_field_sample(field, comp, freqs, dt, t_idx, dtype):
+    """monitors.py:40-46: dt * [cos(w t) F, -sin(w t) F] with t = (t+1)dt for
+    E components, (t+1/2)dt for H components (TM timing on TE runs too)."""
+    off = 1.0 if comp.startswith("E") else 0.5
+    t = (t_idx + off) * dt
+    w = _phasor(freqs, dtype)[:, None, None]
+    return dt * torch.stack([torch.cos(w * t) * field, -torch.sin(w * t) * field], dim=1)
+
+
+# ------------------------------------------------------------------- runner
+def run2d(mode, state, c, n_steps, amps=None, mask=None, monitors=(), checkpoint_every=0):
+    """meep_torch/core/run.py:10-63,86-97: chunked forward run.  Per chunk the
+    monitor samples are summed in step order starting from the first sample;
+    chunk sums are added to the running total.  ``amps`` are the per-step
+    source amplitudes (the profile at (t+1/2)dt, sources.py:90-91).
+
+    monitors: list of dicts {"type": "flux", "kind", "idx", "lo", "hi", "freqs"}
+    or {"type": "dft", "comp", "freqs"} or {"type": "probe", "ix", "iy"}.
+    """
+    step = tm_step if mode == "tm" else te_step
+    dtype = state[next(iter(state))].dtype
+    acc = [None] * len(monitors)
+    chunk = n_steps if checkpoint_every <= 0 else checkpoint_every
+    t0 = 0
+    s = state
+    while t0 < n_steps:
+        k = min(chunk, n_steps - t0)
+        part = [None] * len(monitors)
+        for t in range(t0, t0 + k):
+            if amps is not None:
+                s = step(s, c, c.dt * amps[t], mask)
+            else:
+                s = step(s, c)
+            for i, mon in enumerate(monitors):
+                if mon["type"] == "flux":
+                    smp = flux_line_sample(mon["kind"], s, mon["idx"], mon["lo"], mon["hi"],
+                                           mon["freqs"], c.dt, t, dtype)
+                elif mon["type"] == "dft":
+                    smp = dft_field_sample(s[mon["comp"]], mon["comp"], mon["freqs"], c.dt, t, dtype)
+                else:
+                    smp = s["Ez"][mon["ix"], mon["iy"]].clone()[None]
+                if mon["type"] == "probe":
+                    part[i] = smp if part[i] is None else torch.cat([part[i], smp])
+                else:
+                    part[i] = smp if part[i] is None else part[i] + smp
+        for i, mon in enumerate(monitors):
+            if acc[i] is None:
+                acc[i] = part[i]
+            elif mon["type"] == "probe":
+                acc[i] = torch.cat([acc[i], part[i]])
+            else:
+                acc[i] = acc[i] + part[i]
+        t0 += k
+    return s, acc
+
+
+# ---------------------------------------------------------------- invariants
+def tm_yee_invariant(s_prev, s_next, eps, dx):
+    """Discrete TM energy conserved by the leapfrog in a lossless PEC box
+    (tests/analytic/test_energy.py:29-41): 1/2 sum eps Ez(n)^2 +
+    1/2 sum Hx(n-1/2) Hx(n+1/2) + 1/2 sum Hy(n-1/2) Hy(n+1/2)."""
+    e = 0.5 * (eps * s_prev["Ez"] ** 2).sum()
+    h = 0.5 * (s_prev["Hx"] * s_next["Hx"]).sum() + 0.5 * (s_prev["Hy"] * s_next["Hy"]).sum()
+    return float((e + h) * dx * dx)

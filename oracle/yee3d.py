@@ -1,0 +1,205 @@
+"""3D Yee restatement (CPU, torch eager) -- test infrastructure only.
+
+Follows meep_torch/core/stepper_3d.py:150-266 (CPML coefficients, sigma = 0),
+stepper_3d.py:363-445 (CPML step), stepper_3d.py:448-481 (sources),
+stepper_3d_lean.py:174-296 (in-place lean form) and monitors_3d.py:18-210.
+"""
+import math
+
+import torch
+
+from .yee2d import cpml_profiles
+
+
+def avg(v, ax):
+    """Two-point average along ``ax`` (stepper_3d.py:70-75)."""
+    n = v.shape[ax]
+    return 0.5 * (v.narrow(ax, 0, n - 1) + v.narrow(ax, 1, n - 1))
+
+
+def diff(v, ax):
+    """Forward difference along ``ax`` (stepper_3d.py:78-83)."""
+    n = v.shape[ax]
+    return v.narrow(ax, 1, n - 1) - v.narrow(ax, 0, n - 1)
+
+
+def _along(v, ax):
+    sh = [1, 1, 1]
+    sh[ax] = v.shape[0]
+    return v.reshape(sh)
+
+
+def coeffs3d(nx, ny, nz, dx, eps, npml, dtype=torch.float32):
+    """Per-axis CPML profiles: H side length n-1 at offset 1/2, E side length n
+    at offset 0 trimmed to [1:-1] (stepper_3d.py:182-195); dt = dx/(2 sqrt 3)."""
+    dt = 0.5 * dx / math.sqrt(3.0)
+    pro = {}
+    for ax, n in enumerate((nx, ny, nz)):
+        b, a, _ = cpml_profiles(n - 1, npml, 0.5, dt, dx, dtype)
+        pro[("H", ax)] = (_along(b, ax), _along(a, ax))
+        b, a, _ = cpml_profiles(n, npml, 0.0, dt, dx, dtype)
+        pro[("E", ax)] = (_along(b[1:-1], ax), _along(a[1:-1], ax))
+    eps = eps.to(dtype)
+    return {"dt": dt, "dx": dx, "eps": eps, "pro": pro,
+            "eps_c": [avg(eps, 0), avg(eps, 1), avg(eps, 2)],
+            "inv_eps": [1.0 / avg(eps, 0), 1.0 / avg(eps, 1), 1.0 / avg(eps, 2)]}
+
+
+PSI = {  # name: (field differentiated, axis, side)  -- stepper_3d.py:47-58
+    "psi_Ez_y": ("Ez", 1, "H"), "psi_Ey_z": ("Ey", 2, "H"),
+    "psi_Ex_z": ("Ex", 2, "H"), "psi_Ez_x": ("Ez", 0, "H"),
+    "psi_Ey_x": ("Ey", 0, "H"), "psi_Ex_y": ("Ex", 1, "H"),
+    "psi_Hz_y": ("Hz", 1, "E"), "psi_Hy_z": ("Hy", 2, "E"),
+    "psi_Hx_z": ("Hx", 2, "E"), "psi_Hz_x": ("Hz", 0, "E"),
+    "psi_Hy_x": ("Hy", 0, "E"), "psi_Hx_y": ("Hx", 1, "E"),
+}
+COMP_SHAPE = {"Ex": (-1, 0, 0), "Ey": (0, -1, 0), "Ez": (0, 0, -1),
+              "Hx": (0, -1, -1), "Hy": (-1, 0, -1), "Hz": (-1, -1, 0)}
+
+
+def state3d(nx, ny, nz, dtype=torch.float32):
+    z = lambda sh: torch.zeros(*sh, dtype=dtype)
+    st = {}
+    for c, d in COMP_SHAPE.items():
+        st[c] = z((nx + d[0], ny + d[1], nz + d[2]))
+    for name, (f, ax, _) in PSI.items():
+        sh = list(st[f].shape)
+        sh[ax] -= 1
+        st[name] = z(sh)
+    return st
+
+
+def _ce(s, c, name):
+    """d/dx + psi with the psi recursion psi' = b psi + a d (stepper_3d.py:368-370)."""
+    f, ax, side = PSI[name]
+    d = diff(s[f], ax) / c["dx"]
+    b, a = c["pro"][(side, ax)]
+    p = b * s[name] + a * d
+    s[name] = p
+    return d + p
+
+
+def _trim(t, ax):
+    return t.narrow(ax, 1, t.shape[ax] - 2)
+
+
+def _pad(t, axes):
+    pad = [0] * 6
+    for ax in axes:
+        pad[2 * (2 - ax)] = pad[2 * (2 - ax) + 1] = 1
+    return torch.nn.functional.pad(t, pad)
+
+
+def step3d(st, c, amp=0.0, src=None, lean=False):
+    """One CPML leapfrog step.  H from curl E, then E from curl H with the
+    PEC tangential trim (stepper_3d.py:363-437).  E update:
+      functional form  E = (E*eps_c + dt*curl) / eps_c   (d1 = 1, d2 = dt)
+      lean form        E = E + (curl * inv_eps) * dt      (stepper_3d_lean.py:260)
+    Point source (comp, i, j, k): functional ADDs (dt*amp) * (1/eps_c) at the
+    node (stepper_3d.py:475-481); lean SUBTRACTS (dt*amp) * inv_eps
+    (stepper_3d_lean.py:294-296).  Plane (comp, mask): E -= (dt*amp)*mask/eps_c
+    (stepper_3d.py:452-461) / E -= mask*(dt*amp)*inv_eps (lean)."""
+    s = dict(st)
+    dt = c["dt"]
+    s["Hx"] = st["Hx"] - dt * (_ce(s, c, "psi_Ez_y") - _ce(s, c, "psi_Ey_z"))
+    s["Hy"] = st["Hy"] - dt * (_ce(s, c, "psi_Ex_z") - _ce(s, c, "psi_Ez_x"))
+    s["Hz"] = st["Hz"] - dt * (_ce(s, c, "psi_Ey_x") - _ce(s, c, "psi_Ex_y"))
+    # E: curl terms (each psi updated on its full extent, then trimmed)
+    for comp, (pa, fa), (pb, fb), axes in (
+            ("Ex", ("psi_Hz_y", 2), ("psi_Hy_z", 1), (1, 2)),
+            ("Ey", ("psi_Hx_z", 0), ("psi_Hz_x", 2), (0, 2)),
+            ("Ez", ("psi_Hy_x", 1), ("psi_Hx_y", 0), (0, 1))):
+        ta = _trim(_ce(s, c, pa), fa)
+        tb = _trim(_ce(s, c, pb), fb)
+        curl = _pad(ta - tb, axes)
+        k = "xyz".index(comp[1])
+        if lean:
+            s[comp] = st[comp] + curl * c["inv_eps"][k] * dt
+        else:
+            e = c["eps_c"][k]
+            s[comp] = (st[comp] * e + dt * curl) / e
+    if src is not None and amp != 0.0:
+        comp = src[0]
+        k = "xyz".index(comp[1])
+        if len(src) == 4:
+            i, j, kk = src[1:]
+            E = s[comp].clone()
+            if lean:
+                E[i, j, kk] = E[i, j, kk] - dt * amp * c["inv_eps"][k][i, j, kk]
+            else:
+                E[i, j, kk] = E[i, j, kk] + dt * amp / c["eps_c"][k][i, j, kk]
+            s[comp] = E
+        else:
+            mask = src[1].to(s[comp].dtype)
+            if lean:
+                s[comp] = s[comp] - mask * (dt * amp) * c["inv_eps"][k]
+            else:
+                s[comp] = s[comp] - dt * amp * mask / c["eps_c"][k]
+    return s
+
+
+def flux_plane_sample(st, normal, idx, freqs, dt, t_idx, ap=0, win=None):
+    """monitors_3d.py:124-181: 8 phasor parts of (Ea, Ha, Eb, Hb) on an
+    axis-normal plane, cropped to the common shape and aperture."""
+    if normal == "z":
+        Ea, Ha = st["Ex"][:, :, idx], 0.5 * (st["Hy"][:, :, idx - 1] + st["Hy"][:, :, idx])
+        Eb, Hb = st["Ey"][:, :, idx], 0.5 * (st["Hx"][:, :, idx - 1] + st["Hx"][:, :, idx])
+    elif normal == "x":
+        Ea, Ha = st["Ey"][idx], 0.5 * (st["Hz"][idx - 1] + st["Hz"][idx])
+        Eb, Hb = st["Ez"][idx], 0.5 * (st["Hy"][idx - 1] + st["Hy"][idx])
+    else:
+        Ea, Ha = st["Ez"][:, idx], 0.5 * (st["Hx"][:, idx - 1] + st["Hx"][:, idx])
+        Eb, Hb = st["Ex"][:, idx], 0.5 * (st["Hz"][:, idx - 1] + st["Hz"][:, idx])
+    ts = [Ea, Ha, Eb, Hb]
+    m0 = min(t.shape[0] for t in ts)
+    m1 = min(t.shape[1] for t in ts)
+    l0, h0, l1, h1 = win if win is not None else (ap, m0 - ap, ap, m1 - ap)
+    l0, l1, h0, h1 = max(0, l0), max(0, l1), min(m0, h0), min(m1, h1)
+    Ea, Ha, Eb, Hb = (t[l0:h0, l1:h1] for t in ts)
+    dtype = Ea.dtype
+    w = (2 * math.pi * torch.tensor(freqs, dtype=dtype))[:, None, None]
+    tE, tH = (t_idx + 1.0) * dt, (t_idx + 0.5) * dt
+    parts = [torch.cos(w * tE) * Ea, -torch.sin(w * tE) * Ea,
+             torch.cos(w * tH) * Ha, -torch.sin(w * tH) * Ha,
+             torch.cos(w * tE) * Eb, -torch.sin(w * tE) * Eb,
+             torch.cos(w * tH) * Hb, -torch.sin(w * tH) * Hb]
+    return dt * torch.cat([p.unsqueeze(1) for p in parts], dim=1)
+
+
+def flux_plane(acc, dx):
+    """monitors_3d.py:183-189: 1/2 sum(Re a E Re a H + ... - b pair) dx^2."""
+    p = acc.unbind(1)
+    sa = p[0] * p[2] + p[1] * p[3]
+    sb = p[4] * p[6] + p[5] * p[7]
+    return 0.5 * (sa - sb).sum(tuple(range(1, sa.ndim))) * dx * dx
+
+
+def dft_plane_sample(st, comp, plane, freq, dt, t_idx):
+    """monitors_3d.py:81-94 (single frequency; the reference's reshape admits
+    only one): dt * [cos(wt) F, -sin(wt) F] shaped (1, 2, 1, ...)."""
+    F = st[comp]
+    if plane is not None:
+        sl = [slice(None)] * 3
+        sl[plane[0]] = plane[1]
+        F = F[tuple(sl)]
+    off = 1.0 if comp.startswith("E") else 0.5
+    t = (t_idx + off) * dt
+    w = (2 * math.pi * torch.tensor([freq], dtype=F.dtype)).reshape([1, 1] + [1] * F.ndim)
+    return dt * torch.stack([torch.cos(w * t) * F, -torch.sin(w * t) * F], dim=1)
+
+
+def run3d(st, c, n_steps, amps=None, src=None, monitors=(), lean=False):
+    """monitors_3d.py:192-210 / stepper_3d_lean.py:321-348: sequential sums.
+    monitors: {"type": "flux", "normal", "idx", "freqs", "ap", "win"} or
+    {"type": "dft", "comp", "plane", "freq"}."""
+    acc = [None] * len(monitors)
+    for t in range(n_steps):
+        st = step3d(st, c, amps[t] if amps is not None else 0.0, src, lean=lean)
+        for i, m in enumerate(monitors):
+            if m["type"] == "flux":
+                smp = flux_plane_sample(st, m["normal"], m["idx"], m["freqs"], c["dt"], t,
+                                        m.get("ap", 0), m.get("win"))
+            else:
+                smp = dft_plane_sample(st, m["comp"], m.get("plane"), m["freq"], c["dt"], t)
+            acc[i] = smp if acc[i] is None else acc[i] + smp
+    return st, acc
