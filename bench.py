@@ -1,0 +1,195 @@
+"""FDTD benchmark: Yee cell-updates/s and HBM-roofline fraction on one MI355X.
+
+Default workload (C2): 2D TE 2048x2048, FP32, seeded random-density eps on the
+centred 512x512 (Gaussian-smoothed, eps 2.1..12.1), 32-cell CPML, Jy line
+source, 10000 Yee steps, 4 flux lines x 100 frequencies accumulated on the fly.
+One bench "step" is one full C2 run (10000 Yee steps); `value` is whole-job
+cell-updates/s with inputs resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--workload c2|c3|c1] [--no-cpu-baseline]
+
+Prints one JSON line on stdout (progress goes to stderr).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "meep-torch_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import torch  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+ALG_BYTES = {"c1": 32.0, "c2": 29.0, "c3": 58.0}   # FP32 algorithmic bytes per cell-update
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_c2(steps_override=None):
+    from meep_torch_amd import synth
+    from meep_torch_amd.core.coeffs import build_te_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTE, FluxMonitorTEY
+    from meep_torch_amd.core.sources import GaussianPulse, LineSource
+    from meep_torch_amd.core.stepper_te import init_te_state
+    P = synth.C2
+    n = P["n"]
+    g = Grid(nx=n, ny=n, dx=P["dx"], dtype=torch.float32, device="cuda")
+    eps = synth.c2_eps(device="cuda")
+    m = build_te_coeffs(g, eps_inf=eps, npml=P["npml"])
+    src = LineSource(g, ix=P["src_ix"], j0=P["src_lo"], j1=P["src_hi"],
+                     profile=GaussianPulse(P["fcen"], P["fwidth"]))
+    src.mask = src.mask[:, :-1].contiguous()
+    fr = synth.c2_freqs()
+    mons = [FluxMonitorTE(g, fr, ix=i, lo=P["line_lo"], hi=P["line_hi"]) for i in P["lines_x"]]
+    mons += [FluxMonitorTEY(g, fr, iy=j, lo=P["line_lo"], hi=P["line_hi"]) for j in P["lines_y"]]
+    steps = steps_override or P["steps"]
+
+    def run():
+        from meep_torch_amd.core.run import run_te
+        return run_te(init_te_state(g), m, steps, source=src, monitors=mons)
+
+    return run, n * n, steps, eps
+
+
+def build_c3(steps_override=None):
+    from meep_torch_amd import synth
+    from meep_torch_amd.core.monitors_3d import FluxMonitor3D, PointSource3D, run_3d
+    from meep_torch_amd.core.sources import GaussianPulse
+    from meep_torch_amd.core.stepper_3d import build_3d_coeffs, init_3d_state
+    P = synth.C3
+    n = P["n"]
+    eps = synth.c3_eps(n).cuda()
+    c = build_3d_coeffs(n, n, n, P["dx"], eps=eps, npml=P["npml"], dtype=torch.float32, device="cuda")
+    src = PointSource3D(c.dt, "Ez", P["npml"] + 8, n // 2, n // 2, GaussianPulse(P["fcen"], P["fwidth"]))
+    fr = [P["fcen"] - 0.1 + 0.2 * i / (P["nfreq"] - 1) for i in range(P["nfreq"])]
+    mons = [FluxMonitor3D(c.dt, fr, normal="x", idx=i, ap=P["npml"] + 1, dtype=torch.float32)
+            for i in P["planes"]]
+    steps = steps_override or P["steps"]
+
+    def run():
+        st = init_3d_state(n, n, n, npml=P["npml"], dtype=torch.float32, device="cuda")
+        return run_3d(st, c, steps, source=src, monitors=mons)
+
+    return run, n ** 3, steps, eps
+
+
+def build_c1(steps_override=None):
+    from meep_torch_amd import synth
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTM
+    from meep_torch_amd.core.run import run_tm
+    from meep_torch_amd.core.sources import GaussianPulse, LineSource
+    from meep_torch_amd.core.stepper_tm import init_tm_state
+    g = Grid(nx=256, ny=256, dx=0.05, dtype=torch.float32, device="cuda")
+    m = build_tm_coeffs(g, eps_inf=synth.c1_eps().cuda(), npml=16)
+    src = LineSource(g, ix=24, j0=128, j1=129, profile=GaussianPulse(0.15, 0.1))
+    fr = [0.1 + 0.1 * i / 49 for i in range(50)]
+    mon = FluxMonitorTM(g, fr, x_normal=True, idx=232, lo=96, hi=160)
+    steps = steps_override or 2000
+
+    def run():
+        return run_tm(init_tm_state(g), m, steps, source=src, monitors=[mon])
+
+    return run, 256 * 256, steps, None
+
+
+def cpu_baseline_c2(budget_s=15.0):
+    """The oracle (reference algorithm restated, torch CPU) on a bounded sample
+    of C2: full 2048x2048 grid with its 4 flux lines, as many Yee steps as fit
+    in ~budget_s seconds, on all host cores."""
+    from oracle import yee2d
+    from meep_torch_amd import synth
+    P = synth.C2
+    n = P["n"]
+    torch.set_num_threads(os.cpu_count() or 1)
+    eps = synth.c2_eps(device="cpu")
+    c = yee2d.te_coeffs(n, n, P["dx"], eps, P["npml"], dtype=torch.float32)
+    mask = yee2d.line_mask(n, n, P["src_ix"], P["src_lo"], P["src_hi"], P["dx"])[:, :-1].contiguous()
+    amp = yee2d.gaussian_pulse(P["fcen"], P["fwidth"])
+    fr = synth.c2_freqs()
+    mons = [{"type": "flux", "kind": "te_x", "idx": i, "lo": P["line_lo"], "hi": P["line_hi"],
+             "freqs": fr} for i in P["lines_x"]]
+    mons += [{"type": "flux", "kind": "te_y", "idx": j, "lo": P["line_lo"], "hi": P["line_hi"],
+              "freqs": fr} for j in P["lines_y"]]
+    st = yee2d.te_state(n, n)
+    amps = [amp((t + 0.5) * c.dt) for t in range(100000)]
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < budget_s or steps < 2:
+        st, _ = yee2d.run2d("te", st, c, 1, amps=amps[steps:steps + 1], mask=mask, monitors=mons)
+        steps += 1
+    dt = time.perf_counter() - t0
+    return {"value": n * n * steps / dt, "unit": "cell-updates/s", "cores": torch.get_num_threads(),
+            "kind": "port",
+            "sample": f"C2 full 2048x2048 grid + 4x100-freq flux lines, {steps} Yee steps "
+                      f"({dt:.1f} s), oracle (torch CPU eager)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--yee-steps", type=int, default=0, help="override Yee steps per run (tuning)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the FDTD path shards by spatial slabs across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    builder = {"c1": build_c1, "c2": build_c2, "c3": build_c3}[args.workload]
+    run, cells, ysteps, _ = builder(args.yee_steps or None)
+    log(f"[bench] {args.workload}: {cells} cells x {ysteps} Yee steps per bench step")
+    for i in range(args.warmup):
+        run()
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record()
+    for i in range(args.steps):
+        run()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    log(f"[bench] timed {args.steps} runs: wall {wall:.3f} s, events {gpu_ms / 1e3:.3f} s")
+    updates = cells * ysteps * args.steps
+    value = updates / wall
+    alg = ALG_BYTES[args.workload]
+    per_step_s = gpu_ms / 1e3 / (ysteps * args.steps)
+    achieved = alg * cells / per_step_s / 1e9
+    line = {
+        "metric": "Yee cell-updates/s (FP32) with % of MI355X HBM roofline",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": {"c1": "C1 2D TM 256x256 16-cell PML 2000 steps 50-freq flux line",
+                                "c2": "C2 2D TE 2048x2048 32-cell CPML 10000 steps 4 flux lines x 100 freqs",
+                                "c3": "C3 3D 384^3 12-cell CPML 2000 steps 2 flux planes x 64 freqs"
+                                }[args.workload],
+                   "yee_steps_per_step": ysteps, "cells": cells},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "one Yee step (all kernels of the step, HIP events)",
+                     "alg_bytes_per_cell_update": alg},
+    }
+    if not args.no_cpu_baseline and args.workload == "c2":
+        log("[bench] cpu baseline (oracle) ...")
+        line["cpu_baseline"] = cpu_baseline_c2()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

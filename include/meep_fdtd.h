@@ -1,0 +1,161 @@
+/*
+ * meep_fdtd.h -- C ABI of the MI355X FDTD time-stepping engine (libmeepfdtd.so).
+ *
+ * Plain C: device pointers, sizes and doubles; no torch types.  All pointers
+ * marked "device" address HIP device memory of the run's dtype (MT_F32 or
+ * MT_F64).  Arrays use the reference's tensor shapes, row-major, last axis
+ * contiguous, so a caller hands over the storage of the reference state
+ * tensors unchanged.  Every entry point returns 0 on success and a negative
+ * code on error; mt_last_error() then returns a message (thread-local).
+ * All work is enqueued on `stream` (a hipStream_t, NULL = default stream);
+ * no entry point synchronises the device.
+ */
+#ifndef MEEP_FDTD_H
+#define MEEP_FDTD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MT_F32 0
+#define MT_F64 1
+
+#define MT_OK 0
+#define MT_EINVAL (-1)
+#define MT_EHIP (-2)
+#define MT_ENOTIMPL (-3)
+
+#define MT_MAX_TERMS 4
+
+/* One 1D CPML profile (b, a) of length n.  Indices outside the shells
+ * [0, lo_end) and [hi_begin, n) must hold exactly b = 1, a = 0 (the
+ * reference's interior identity, meep_torch/core/pml.py:53-58), so the
+ * kernels skip the psi recursion there. */
+typedef struct {
+  const void* b; /* device, length n */
+  const void* a; /* device, length n */
+  int n;
+  int lo_end;
+  int hi_begin;
+} mt_pml_axis;
+
+/* 2D TM problem + state (meep_torch/core/coeffs.py:21-74 TMCoeffs and
+ * meep_torch/core/stepper_tm.py:12-34 TMState, eps/CPML/y_pmc subset).
+ * Profiles follow the reference naming: hy (ny-1, half-y), hx (nx-1, half-x),
+ * ex (nx-2, integer-x interior), ey (ny-2 interior, or ny under y_pmc). */
+typedef struct {
+  int nx, ny, dtype, y_pmc;
+  double dt, dx;
+  const void* eps;                       /* device (nx, ny) eps_inf */
+  mt_pml_axis hy, hx, ex, ey;
+  void *Ez, *Dz, *Hx, *Hy;               /* device (nx,ny) (nx,ny) (nx,ny-1) (nx-1,ny) */
+  void *psi_Ezy, *psi_Ezx;               /* device (nx,ny-1) (nx-1,ny) */
+  void *psi_Hyx, *psi_Hxy;               /* device (nx-2, ny_e), ny_e = ny-2 or ny (y_pmc) */
+} mt_tm2d;
+
+/* 2D TE problem + state (coeffs.py:294-345 TECoeffs, stepper_te.py:11-24). */
+typedef struct {
+  int nx, ny, dtype, pad_;
+  double dt, dx;
+  const void *eps_x, *eps_y;             /* device (nx-1,ny) (nx,ny-1) */
+  mt_pml_axis hx, hy, ey, ex;            /* nx-1, ny-1, ny-2, nx-2 */
+  void *Hz, *Ex, *Ey, *Dx, *Dy;          /* (nx-1,ny-1) (nx-1,ny) (nx,ny-1) (nx-1,ny) (nx,ny-1) */
+  void *psi_Eyx, *psi_Exy;               /* (nx-1,ny-1) each */
+  void *psi_Hzy, *psi_Hzx;               /* (nx-1,ny-2) (nx-2,ny-1) */
+} mt_te2d;
+
+/* Soft source on a 2D field: dense weights on the bounding box
+ * [i0,i1) x [j0,j1) of the reference's mask (sources.py:49-78); at step s the
+ * field loses (coef[s] cast to dtype) * w (stepper_tm.py:197-200,
+ * stepper_te.py:131-134).  coef is a HOST array of k doubles = dt * amp(t). */
+typedef struct {
+  int present;
+  int i0, i1, j0, j1;
+  const void* w;                          /* device (i1-i0, j1-j0) */
+} mt_source2d;
+
+/* 3D problem + state (stepper_3d.py:28-67 Coeffs3D / State3D, CPML path,
+ * sigma = 0).  lean = 1 selects the in-place lean stepper's E update and
+ * source sign (stepper_3d_lean.py:234-296). */
+typedef struct {
+  int nx, ny, nz, dtype, lean, pad_;
+  double dt, dx;
+  const void* eps;                        /* device (nx,ny,nz) cell-centred */
+  mt_pml_axis h[3];                       /* per axis, length n-1 (offset 1/2) */
+  mt_pml_axis e[3];                       /* per axis, length n-2 (interior) */
+  void *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;      /* reference State3D shapes */
+  /* psi, reference shapes (stepper_3d.py:47-58), in this order:
+   * Ez_y Ey_z Ex_z Ez_x Ey_x Ex_y Hz_y Hy_z Hx_z Hz_x Hy_x Hx_y */
+  void* psi[12];
+} mt_yee3d;
+
+/* 3D soft source: kind 1 = point at (i0, j0, k0), kind 2 = mask given as dense
+ * weights w on the box [i0,i1) x [j0,j1) x [k0,k1) of the component grid.
+ * comp: 0 Ex, 1 Ey, 2 Ez.  Functional form adds (dt*amp) * (1/eps_c) at a
+ * point (stepper_3d.py:475-481) and subtracts ((dt*amp) * w) / eps_c for a
+ * mask (stepper_3d.py:452-461); lean form subtracts (dt*amp) * inv_eps and
+ * (w * (dt*amp)) * inv_eps (stepper_3d_lean.py:285-296).  A step whose
+ * coefficient is exactly 0 injects nothing (stepper_3d.py:449). */
+typedef struct {
+  int kind, comp;
+  int i0, i1, j0, j1, k0, k1;
+  const void* w;                          /* device (i1-i0, j1-j0, k1-k0), kind 2 */
+} mt_source3d;
+
+/* A DFT monitor: up to MT_MAX_TERMS strided 2D field views sampled after
+ * every step.  Term value = a[e] or 0.5*(a[e]+b[e]) (H averaged onto the E
+ * nodes).  Per step t the accumulator gains
+ *   dt*(cos(w*t_q) v),  dt*(-sin(w*t_q) v)        (monitors.py:84-95)
+ * in slots 2q, 2q+1 of acc (nfreq, 2*n_terms, n0, n1), summed in step order.
+ * t_q = (t + 1/2) dt if t_half else (t + 1) dt, rounded to dtype.
+ * kind 1 = probe: single element stacked per step into out[t - step0]. */
+typedef struct {
+  const void* a;
+  const void* b;                          /* NULL: no averaging */
+  long long s0, s1;                       /* element strides */
+  int t_half, pad_;
+} mt_dft_term;
+
+typedef struct {
+  int kind;                               /* 0 DFT (reduce "sum"), 1 probe (reduce "stack") */
+  int n_terms, n0, n1, nfreq, pad_;
+  mt_dft_term term[MT_MAX_TERMS];
+  const void* omega;                      /* device, nfreq values of 2*pi*f in dtype */
+  void* acc;                              /* device accumulators (DFT) or (k,) stack (probe) */
+} mt_monitor;
+
+/* Bytes of staging workspace one call needs for `stage_steps` buffered steps. */
+long long mt_stage_bytes(const mt_monitor* mons, int n_mons, int stage_steps, int dtype);
+
+/* Advance a 2D TM state k steps starting at global step index step0.
+ * Replaces the per-step loop of meep_torch/core/run.py:10-40 (_run_chunk)
+ * driving step_tm (meep_torch/core/stepper_tm.py:234-254) and the monitor
+ * samples of meep_torch/core/monitors.py:11-100.  Monitor accumulators are
+ * summed in place (the caller zeroes them at a chunk start). */
+int mt_tm2d_run(const mt_tm2d* p, const mt_source2d* src, const double* coef,
+                const mt_monitor* mons, int n_mons, void* stage, long long stage_bytes,
+                int stage_steps, int step0, int k, void* stream);
+
+/* Same for 2D TE: _run_chunk driving step_te (meep_torch/core/stepper_te.py:178-190)
+ * with FluxMonitorTE/FluxMonitorTEY/DFTFieldsMonitor (monitors.py:24-155). */
+int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const double* coef,
+                const mt_monitor* mons, int n_mons, void* stage, long long stage_bytes,
+                int stage_steps, int step0, int k, void* stream);
+
+/* Advance a 3D state k steps: replaces run_3d (meep_torch/core/monitors_3d.py:192-210)
+ * over step_3d (stepper_3d.py:303-445) or, with lean = 1, run_3d_lean's loop
+ * (stepper_3d_lean.py:307-349), with FluxMonitor3D / DFTFieldsMonitor3D. */
+int mt_yee3d_run(const mt_yee3d* p, const mt_source3d* src, const double* coef,
+                 const mt_monitor* mons, int n_mons, void* stage, long long stage_bytes,
+                 int stage_steps, int step0, int k, void* stream);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* mt_last_error(void);
+
+/* ABI version (bumped on any struct layout change). */
+int mt_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MEEP_FDTD_H */

@@ -1,0 +1,51 @@
+// Shared helpers for the FDTD engine kernels (gfx950 / CDNA4, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "meep_fdtd.h"
+
+namespace mt {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_launch(const char* what);
+
+// CPML profile on the device side: psi recursion only inside the shells.
+template <typename T>
+struct Pml {
+  const T* b;
+  const T* a;
+  int lo_end, hi_begin;
+  __device__ __forceinline__ bool in(int i) const { return i < lo_end || i >= hi_begin; }
+};
+
+template <typename T>
+inline Pml<T> pml_of(const mt_pml_axis& ax) {
+  return Pml<T>{static_cast<const T*>(ax.b), static_cast<const T*>(ax.a), ax.lo_end, ax.hi_begin};
+}
+
+int check_axis(const mt_pml_axis& ax, int n, const char* name);
+
+// Monitor support shared by the 2D and 3D steppers (dft.hip).
+struct MonitorPlan {
+  const mt_monitor* mons;
+  int n_mons;
+  void* stage;
+  long long stage_bytes;
+  int stage_steps;
+  int dtype;
+};
+int monitors_validate(const MonitorPlan& mp);
+// Gather the values of every term for step slot `slot` (after the step).
+int monitors_sample(const MonitorPlan& mp, int slot, int t_rel, hipStream_t s);
+// Fold `n_filled` staged steps whose first global step index is t0 into the accumulators.
+int monitors_flush(const MonitorPlan& mp, int n_filled, int t0, double dt, hipStream_t s);
+
+inline dim3 grid2(int n_fast, int n_slow, dim3 blk) {
+  return dim3((n_fast + blk.x - 1) / blk.x, (n_slow + blk.y - 1) / blk.y, 1);
+}
+
+}  // namespace mt

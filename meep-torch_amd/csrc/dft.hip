@@ -1,0 +1,283 @@
+// On-the-fly DFT accumulation for flux lines / planes and field monitors.
+//
+// After every step a gather kernel writes each monitor term's sample value
+// (E, or the two-point H average 0.5*(a+b)) into a staging ring of
+// `stage_steps` slots.  Every `stage_steps` steps (and at the end of a call) a
+// phase-table kernel evaluates cos/sin(w*t) once per (frequency, step) and a
+// flush kernel folds the staged samples into the accumulators, one thread per
+// (term, element), looping over frequencies and then over the staged steps in
+// step order -- the same left-to-right summation as the reference's
+// per-step `partials + smp` (meep_torch/core/run.py:29-33), so FP32 spectra
+// round like the reference's.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "common.h"
+
+namespace mt {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(MT_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return MT_OK;
+}
+
+int check_axis(const mt_pml_axis& ax, int n, const char* name) {
+  if (ax.n != n) return fail(MT_EINVAL, std::string("profile ") + name + ": length " +
+                                 std::to_string(ax.n) + " != " + std::to_string(n));
+  if (n > 0 && (!ax.b || !ax.a)) return fail(MT_EINVAL, std::string("profile ") + name + ": null");
+  if (ax.lo_end < 0 || ax.hi_begin > n || ax.lo_end > ax.hi_begin)
+    return fail(MT_EINVAL, std::string("profile ") + name + ": bad shell bounds");
+  return MT_OK;
+}
+
+namespace {
+
+constexpr int kMaxTermsAll = 32;  // terms over all monitors of one call
+
+struct GatherTerm {
+  const void* a;
+  const void* b;
+  void* out;        // staging base for this term (slot stride = n) or probe stack
+  long long s0, s1;
+  int n0, n1;
+  long long slot_stride;  // elements between consecutive slots
+};
+
+struct GatherArgs {
+  int n_terms;
+  int prefix[kMaxTermsAll + 1];
+  GatherTerm t[kMaxTermsAll];
+};
+
+template <typename T>
+__global__ void gather_kernel(GatherArgs g, int slot) {
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= g.prefix[g.n_terms]) return;
+  int q = 0;
+  while (tid >= g.prefix[q + 1]) ++q;
+  const GatherTerm& t = g.t[q];
+  int e = tid - g.prefix[q];
+  int i0 = e / t.n1, i1 = e - i0 * t.n1;
+  long long off = (long long)i0 * t.s0 + (long long)i1 * t.s1;
+  T v = static_cast<const T*>(t.a)[off];
+  if (t.b) v = T(0.5) * (v + static_cast<const T*>(t.b)[off]);
+  static_cast<T*>(t.out)[(long long)slot * t.slot_stride + e] = v;
+}
+
+// table[(f * S + s) * 4 + {0,1,2,3}] = cos(w tE), sin(w tE), cos(w tH), sin(w tH)
+template <typename T>
+__global__ void phase_kernel(const T* omega, int nfreq, int n_filled, int t0, double dt, T* table) {
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= nfreq * n_filled) return;
+  int f = tid / n_filled, s = tid - f * n_filled;
+  double tE = ((double)(t0 + s) + 1.0) * dt;
+  double tH = ((double)(t0 + s) + 0.5) * dt;
+  T w = omega[f];
+  T pe = w * static_cast<T>(tE);
+  T ph = w * static_cast<T>(tH);
+  T* o = table + (long long)(f * n_filled + s) * 4;
+  o[0] = cos(pe);
+  o[1] = sin(pe);
+  o[2] = cos(ph);
+  o[3] = sin(ph);
+}
+
+// One thread per (term, element); accumulator layout (nfreq, 2*n_terms, n).
+template <typename T, int CH>
+__global__ void flush_kernel(const T* stage, const T* table, T* acc, int n_terms, int n, int nfreq,
+                             int n_filled, int half_mask, T dt) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int q = blockIdx.y;
+  if (e >= n) return;
+  const T* st = stage + (long long)q * n + e;  // slot stride = n_terms * n
+  long long slot_stride = (long long)n_terms * n;
+  int toff = ((half_mask >> q) & 1) ? 2 : 0;
+  for (int s0 = 0; s0 < n_filled; s0 += CH) {
+    int m = n_filled - s0 < CH ? n_filled - s0 : CH;
+    T v[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) v[u] = u < m ? st[(long long)(s0 + u) * slot_stride] : T(0);
+    for (int f = 0; f < nfreq; ++f) {
+      T* are = acc + ((long long)f * 2 * n_terms + 2 * q) * n + e;
+      T* aim = are + n;
+      T re = *are, im = *aim;
+      const T* tb = table + ((long long)f * n_filled + s0) * 4 + toff;
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        if (u < m) {
+          T c = tb[u * 4 + 0];
+          T sn = tb[u * 4 + 1];
+          re = re + dt * (c * v[u]);
+          im = im + dt * ((-sn) * v[u]);
+        }
+      }
+      *are = re;
+      *aim = im;
+    }
+  }
+}
+
+template <typename T>
+size_t esz() { return sizeof(T); }
+
+long long mon_elems(const mt_monitor& m) { return (long long)m.n0 * m.n1; }
+
+}  // namespace
+
+int monitors_validate(const MonitorPlan& mp) {
+  int total = 0;
+  for (int i = 0; i < mp.n_mons; ++i) {
+    const mt_monitor& m = mp.mons[i];
+    if (m.kind != 0 && m.kind != 1) return fail(MT_EINVAL, "monitor: unknown kind");
+    if (m.n_terms < 1 || m.n_terms > MT_MAX_TERMS) return fail(MT_EINVAL, "monitor: bad n_terms");
+    if (m.n0 < 0 || m.n1 < 0) return fail(MT_EINVAL, "monitor: bad extents");
+    if (!m.acc) return fail(MT_EINVAL, "monitor: null accumulator");
+    if (m.kind == 0 && (m.nfreq < 1 || !m.omega)) return fail(MT_EINVAL, "monitor: no frequencies");
+    if (m.kind == 1 && (m.n_terms != 1 || m.n0 * m.n1 != 1))
+      return fail(MT_EINVAL, "probe monitor must have one single-element term");
+    for (int q = 0; q < m.n_terms; ++q)
+      if (!m.term[q].a) return fail(MT_EINVAL, "monitor: null field view");
+    total += m.n_terms;
+  }
+  if (total > kMaxTermsAll) return fail(MT_EINVAL, "too many monitor terms in one call (max 32)");
+  long long need = mt_stage_bytes(mp.mons, mp.n_mons, mp.stage_steps, mp.dtype);
+  if (need > mp.stage_bytes) return fail(MT_EINVAL, "staging workspace too small");
+  if (need > 0 && !mp.stage) return fail(MT_EINVAL, "staging workspace missing");
+  return MT_OK;
+}
+
+template <typename T>
+static int sample_t(const MonitorPlan& mp, int slot, int t_rel, hipStream_t s) {
+  GatherArgs g{};
+  int nt = 0;
+  char* base = static_cast<char*>(mp.stage);
+  long long off = 0;
+  for (int i = 0; i < mp.n_mons; ++i) {
+    const mt_monitor& m = mp.mons[i];
+    long long n = mon_elems(m);
+    for (int q = 0; q < m.n_terms; ++q) {
+      GatherTerm& t = g.t[nt];
+      t.a = m.term[q].a;
+      t.b = m.term[q].b;
+      t.s0 = m.term[q].s0;
+      t.s1 = m.term[q].s1;
+      t.n0 = m.n0;
+      t.n1 = m.n1;
+      if (m.kind == 1) {
+        t.out = static_cast<T*>(m.acc);
+        t.slot_stride = 1;
+      } else {
+        // staging layout per monitor: [slot][term][elem]
+        t.out = reinterpret_cast<T*>(base + off) + (long long)q * n;
+        t.slot_stride = (long long)m.n_terms * n;
+      }
+      g.prefix[nt + 1] = g.prefix[nt] + (int)n;
+      ++nt;
+    }
+    // staging layout per DFT monitor: [slot][term][elem] then its phase table
+    if (m.kind == 0)
+      off += ((long long)mp.stage_steps * m.n_terms * n + (long long)m.nfreq * mp.stage_steps * 4) * sizeof(T);
+  }
+  g.n_terms = nt;
+  int total = g.prefix[nt];
+  if (total == 0) return MT_OK;
+  // probes write at the call-relative step index, DFT terms at the ring slot
+  bool any_probe = false;
+  for (int i = 0; i < mp.n_mons; ++i) any_probe |= mp.mons[i].kind == 1;
+  if (!any_probe) {
+    hipLaunchKernelGGL(gather_kernel<T>, dim3((total + 255) / 256), dim3(256), 0, s, g, slot);
+    return check_launch("monitor gather");
+  }
+  // probes and DFT terms use different slot indices: launch the two sets apart
+  GatherArgs gp{}, gd{};
+  int np = 0, nd = 0;
+  int ti = 0;
+  for (int i = 0; i < mp.n_mons; ++i) {
+    const mt_monitor& m = mp.mons[i];
+    for (int q = 0; q < m.n_terms; ++q, ++ti) {
+      GatherArgs& dst = m.kind == 1 ? gp : gd;
+      int& c = m.kind == 1 ? np : nd;
+      dst.t[c] = g.t[ti];
+      dst.prefix[c + 1] = dst.prefix[c] + (g.prefix[ti + 1] - g.prefix[ti]);
+      ++c;
+    }
+  }
+  gp.n_terms = np;
+  gd.n_terms = nd;
+  if (nd) {
+    int tot = gd.prefix[nd];
+    if (tot) hipLaunchKernelGGL(gather_kernel<T>, dim3((tot + 255) / 256), dim3(256), 0, s, gd, slot);
+  }
+  if (np) hipLaunchKernelGGL(gather_kernel<T>, dim3((gp.prefix[np] + 255) / 256), dim3(256), 0, s, gp, t_rel);
+  return check_launch("monitor gather");
+}
+
+int monitors_sample(const MonitorPlan& mp, int slot, int t_rel, hipStream_t s) {
+  if (mp.n_mons == 0) return MT_OK;
+  return mp.dtype == MT_F64 ? sample_t<double>(mp, slot, t_rel, s) : sample_t<float>(mp, slot, t_rel, s);
+}
+
+template <typename T>
+static int flush_t(const MonitorPlan& mp, int n_filled, int t0, double dt, hipStream_t s) {
+  char* base = static_cast<char*>(mp.stage);
+  long long off = 0;
+  for (int i = 0; i < mp.n_mons; ++i) {
+    const mt_monitor& m = mp.mons[i];
+    if (m.kind != 0) continue;
+    long long n = mon_elems(m);
+    T* stage = reinterpret_cast<T*>(base + off);
+    off += (long long)mp.stage_steps * m.n_terms * n * sizeof(T);
+    T* table = reinterpret_cast<T*>(base + off);
+    off += (long long)m.nfreq * mp.stage_steps * 4 * sizeof(T);
+    if (n == 0) continue;
+    int nt = m.nfreq * n_filled;
+    hipLaunchKernelGGL(phase_kernel<T>, dim3((nt + 255) / 256), dim3(256), 0, s,
+                       static_cast<const T*>(m.omega), m.nfreq, n_filled, t0, dt, table);
+    int half_mask = 0;
+    for (int q = 0; q < m.n_terms; ++q) half_mask |= (m.term[q].t_half ? 1 : 0) << q;
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)m.n_terms);
+    hipLaunchKernelGGL((flush_kernel<T, 8>), grid, dim3(256), 0, s, (const T*)stage, (const T*)table,
+                       static_cast<T*>(m.acc), m.n_terms, (int)n, m.nfreq, n_filled, half_mask,
+                       static_cast<T>(dt));
+    int rc = check_launch("monitor flush");
+    if (rc) return rc;
+  }
+  return MT_OK;
+}
+
+int monitors_flush(const MonitorPlan& mp, int n_filled, int t0, double dt, hipStream_t s) {
+  if (mp.n_mons == 0 || n_filled <= 0) return MT_OK;
+  return mp.dtype == MT_F64 ? flush_t<double>(mp, n_filled, t0, dt, s)
+                            : flush_t<float>(mp, n_filled, t0, dt, s);
+}
+
+}  // namespace mt
+
+extern "C" long long mt_stage_bytes(const mt_monitor* mons, int n_mons, int stage_steps, int dtype) {
+  long long es = dtype == MT_F64 ? 8 : 4;
+  long long total = 0;
+  for (int i = 0; i < n_mons; ++i) {
+    const mt_monitor& m = mons[i];
+    if (m.kind != 0) continue;
+    long long n = (long long)m.n0 * m.n1;
+    total += (long long)stage_steps * m.n_terms * n * es;
+    total += (long long)m.nfreq * stage_steps * 4 * es;
+  }
+  return total;
+}
+
+extern "C" const char* mt_last_error(void) { return mt::g_err.c_str(); }
+
+extern "C" int mt_abi_version(void) { return 1; }

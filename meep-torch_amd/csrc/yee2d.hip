@@ -1,0 +1,338 @@
+// 2D TM / TE Yee leapfrog with CPML shells and soft-source injection.
+//
+// Each step is two passes over the grid (H half, then E half for TM; D half
+// then B half for TE).  Arithmetic follows the reference op by op so that FP32
+// rounds exactly like its PyTorch stepper: derivatives are true divisions by
+// dx, the psi recursion is b*psi + a*d, curls are summed left to right, E is a
+// true division D/eps, and the build disables FMA contraction.  Cells outside
+// the CPML shells skip the psi recursion (there b = 1, a = 0, psi stays 0, so
+// the result is unchanged).  Fields keep the reference shapes (last axis y
+// contiguous); threads run along y for coalesced access.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "common.h"
+
+namespace mt {
+namespace {
+
+constexpr int BX = 64;  // threads along y (contiguous)
+constexpr int BY = 4;   // threads along x
+
+template <typename T>
+struct TmK {
+  int nx, ny, y_pmc;
+  T dt, dx;
+  const T* eps;
+  Pml<T> hy, hx, ex, ey;
+  T *Ez, *Dz, *Hx, *Hy, *pEzy, *pEzx, *pHyx, *pHxy;
+  int ny_e;
+};
+
+// meep_torch/core/stepper_tm.py:85-92,126-128 (B half, H from curl E)
+template <typename T>
+__global__ __launch_bounds__(BX* BY) void tm_b_kernel(TmK<T> p) {
+  int j = blockIdx.x * BX + threadIdx.x;
+  int i = blockIdx.y * BY + threadIdx.y;
+  if (i >= p.nx || j >= p.ny) return;
+  const int ny = p.ny;
+  T ez = p.Ez[(long long)i * ny + j];
+  if (j < ny - 1) {
+    long long h = (long long)i * (ny - 1) + j;
+    T d = (p.Ez[(long long)i * ny + j + 1] - ez) / p.dx;
+    T curl = d;
+    if (p.hy.in(j)) {
+      T ps = p.hy.b[j] * p.pEzy[h] + p.hy.a[j] * d;
+      p.pEzy[h] = ps;
+      curl = d + ps;
+    }
+    p.Hx[h] = p.Hx[h] - p.dt * curl;
+  }
+  if (i < p.nx - 1) {
+    long long h = (long long)i * ny + j;
+    T d = (p.Ez[(long long)(i + 1) * ny + j] - ez) / p.dx;
+    T curl = d;
+    if (p.hx.in(i)) {
+      T ps = p.hx.b[i] * p.pEzx[h] + p.hx.a[i] * d;
+      p.pEzx[h] = ps;
+      curl = d + ps;
+    }
+    p.Hy[h] = p.Hy[h] + p.dt * curl;
+  }
+}
+
+// meep_torch/core/stepper_tm.py:130-166 (D half), :197-200 (source), :226-227 (Ez)
+template <typename T>
+__global__ __launch_bounds__(BX* BY) void tm_d_kernel(TmK<T> p, mt_source2d src, T coef) {
+  int j = blockIdx.x * BX + threadIdx.x;
+  int i = blockIdx.y * BY + threadIdx.y;
+  if (i >= p.nx || j >= p.ny) return;
+  const int nx = p.nx, ny = p.ny;
+  long long c = (long long)i * ny + j;
+  T dz = p.Dz[c];
+  bool live = i >= 1 && i < nx - 1 && (p.y_pmc || (j >= 1 && j < ny - 1));
+  if (live) {
+    // H = 0 beyond a PMC wall (stepper_tm.py:135-137)
+    T hx1 = (j < ny - 1) ? p.Hx[(long long)i * (ny - 1) + j] : T(0);
+    T hx0 = (j >= 1) ? p.Hx[(long long)i * (ny - 1) + j - 1] : T(0);
+    T dHx = (hx1 - hx0) / p.dx;
+    T dHy = (p.Hy[(long long)i * ny + j] - p.Hy[(long long)(i - 1) * ny + j]) / p.dx;
+    int ii = i - 1, jj = p.y_pmc ? j : j - 1;
+    long long q = (long long)ii * p.ny_e + jj;
+    T a1 = dHy;
+    if (p.ex.in(ii)) {
+      T ps = p.ex.b[ii] * p.pHyx[q] + p.ex.a[ii] * dHy;
+      p.pHyx[q] = ps;
+      a1 = dHy + ps;
+    }
+    T curl = a1 - dHx;
+    if (p.ey.in(jj)) {
+      T ps = p.ey.b[jj] * p.pHxy[q] + p.ey.a[jj] * dHx;
+      p.pHxy[q] = ps;
+      curl = curl - ps;
+    }
+    dz = dz + p.dt * curl;
+  }
+  if (src.present && i >= src.i0 && i < src.i1 && j >= src.j0 && j < src.j1) {
+    const T* w = static_cast<const T*>(src.w);
+    dz = dz - coef * w[(long long)(i - src.i0) * (src.j1 - src.j0) + (j - src.j0)];
+  }
+  p.Dz[c] = dz;
+  p.Ez[c] = dz / p.eps[c];
+}
+
+template <typename T>
+struct TeK {
+  int nx, ny;
+  T dt, dx;
+  const T *eps_x, *eps_y;
+  Pml<T> hx, hy, ey, ex;
+  T *Hz, *Ex, *Ey, *Dx, *Dy, *pEyx, *pExy, *pHzy, *pHzx;
+};
+
+// meep_torch/core/stepper_te.py:80-103 (D half), :131-134 (source on Dy),
+// :160-168 (E = D / eps)
+template <typename T>
+__global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src, T coef) {
+  int j = blockIdx.x * BX + threadIdx.x;
+  int i = blockIdx.y * BY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx || j >= ny) return;
+  if (i < nx - 1) {  // Dx (nx-1, ny)
+    long long c = (long long)i * ny + j;
+    T dxv = p.Dx[c];
+    if (j >= 1 && j < ny - 1) {
+      T d = (p.Hz[(long long)i * (ny - 1) + j] - p.Hz[(long long)i * (ny - 1) + j - 1]) / p.dx;
+      T curl = d;
+      int jj = j - 1;
+      if (p.ey.in(jj)) {
+        long long q = (long long)i * (ny - 2) + jj;
+        T ps = p.ey.b[jj] * p.pHzy[q] + p.ey.a[jj] * d;
+        p.pHzy[q] = ps;
+        curl = d + ps;
+      }
+      dxv = dxv + p.dt * curl;
+    }
+    p.Dx[c] = dxv;
+    p.Ex[c] = dxv / p.eps_x[c];
+  }
+  if (j < ny - 1) {  // Dy (nx, ny-1)
+    long long c = (long long)i * (ny - 1) + j;
+    T dyv = p.Dy[c];
+    if (i >= 1 && i < nx - 1) {
+      T d = (p.Hz[(long long)i * (ny - 1) + j] - p.Hz[(long long)(i - 1) * (ny - 1) + j]) / p.dx;
+      T curl = d;
+      int ii = i - 1;
+      if (p.ex.in(ii)) {
+        long long q = (long long)ii * (ny - 1) + j;
+        T ps = p.ex.b[ii] * p.pHzx[q] + p.ex.a[ii] * d;
+        p.pHzx[q] = ps;
+        curl = d + ps;
+      }
+      dyv = dyv - p.dt * curl;
+    }
+    if (src.present && i >= src.i0 && i < src.i1 && j >= src.j0 && j < src.j1) {
+      const T* w = static_cast<const T*>(src.w);
+      dyv = dyv - coef * w[(long long)(i - src.i0) * (src.j1 - src.j0) + (j - src.j0)];
+    }
+    p.Dy[c] = dyv;
+    p.Ey[c] = dyv / p.eps_y[c];
+  }
+}
+
+// meep_torch/core/stepper_te.py:106-117 (B half, Hz from curl E)
+template <typename T>
+__global__ __launch_bounds__(BX* BY) void te_b_kernel(TeK<T> p) {
+  int j = blockIdx.x * BX + threadIdx.x;
+  int i = blockIdx.y * BY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx - 1 || j >= ny - 1) return;
+  long long h = (long long)i * (ny - 1) + j;
+  T dEy = (p.Ey[(long long)(i + 1) * (ny - 1) + j] - p.Ey[h]) / p.dx;
+  T dEx = (p.Ex[(long long)i * ny + j + 1] - p.Ex[(long long)i * ny + j]) / p.dx;
+  T a1 = dEy;
+  if (p.hx.in(i)) {
+    T ps = p.hx.b[i] * p.pEyx[h] + p.hx.a[i] * dEy;
+    p.pEyx[h] = ps;
+    a1 = dEy + ps;
+  }
+  T curl = a1 - dEx;
+  if (p.hy.in(j)) {
+    T ps = p.hy.b[j] * p.pExy[h] + p.hy.a[j] * dEx;
+    p.pExy[h] = ps;
+    curl = curl - ps;
+  }
+  p.Hz[h] = p.Hz[h] - p.dt * curl;
+}
+
+int check_src(const mt_source2d* s, int n0, int n1) {
+  if (!s || !s->present) return MT_OK;
+  if (s->i0 < 0 || s->j0 < 0 || s->i1 > n0 || s->j1 > n1 || s->i0 >= s->i1 || s->j0 >= s->j1 || !s->w)
+    return fail(MT_EINVAL, "source box outside the field");
+  return MT_OK;
+}
+
+template <typename T>
+int tm_run_t(const mt_tm2d* d, const mt_source2d* src, const double* coef, const MonitorPlan& mp,
+             int step0, int k, hipStream_t s) {
+  TmK<T> p;
+  p.nx = d->nx;
+  p.ny = d->ny;
+  p.y_pmc = d->y_pmc;
+  p.dt = static_cast<T>(d->dt);
+  p.dx = static_cast<T>(d->dx);
+  p.eps = static_cast<const T*>(d->eps);
+  p.hy = pml_of<T>(d->hy);
+  p.hx = pml_of<T>(d->hx);
+  p.ex = pml_of<T>(d->ex);
+  p.ey = pml_of<T>(d->ey);
+  p.Ez = static_cast<T*>(d->Ez);
+  p.Dz = static_cast<T*>(d->Dz);
+  p.Hx = static_cast<T*>(d->Hx);
+  p.Hy = static_cast<T*>(d->Hy);
+  p.pEzy = static_cast<T*>(d->psi_Ezy);
+  p.pEzx = static_cast<T*>(d->psi_Ezx);
+  p.pHyx = static_cast<T*>(d->psi_Hyx);
+  p.pHxy = static_cast<T*>(d->psi_Hxy);
+  p.ny_e = d->y_pmc ? d->ny : d->ny - 2;
+  mt_source2d sv{};
+  if (src && src->present) sv = *src;
+  dim3 blk(BX, BY), grd = grid2(p.ny, p.nx, blk);
+  int S = mp.stage_steps > 0 ? mp.stage_steps : 1;
+  int filled = 0, first = step0;
+  for (int s_ = 0; s_ < k; ++s_) {
+    hipLaunchKernelGGL(tm_b_kernel<T>, grd, blk, 0, s, p);
+    T c = sv.present ? static_cast<T>(coef[s_]) : T(0);
+    hipLaunchKernelGGL(tm_d_kernel<T>, grd, blk, 0, s, p, sv, c);
+    int rc = check_launch("tm2d step");
+    if (rc) return rc;
+    if (mp.n_mons) {
+      if ((rc = monitors_sample(mp, filled, s_, s))) return rc;
+      if (++filled == S || s_ == k - 1) {
+        if ((rc = monitors_flush(mp, filled, first, d->dt, s))) return rc;
+        first += filled;
+        filled = 0;
+      }
+    }
+  }
+  return MT_OK;
+}
+
+template <typename T>
+int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const MonitorPlan& mp,
+             int step0, int k, hipStream_t s) {
+  TeK<T> p;
+  p.nx = d->nx;
+  p.ny = d->ny;
+  p.dt = static_cast<T>(d->dt);
+  p.dx = static_cast<T>(d->dx);
+  p.eps_x = static_cast<const T*>(d->eps_x);
+  p.eps_y = static_cast<const T*>(d->eps_y);
+  p.hx = pml_of<T>(d->hx);
+  p.hy = pml_of<T>(d->hy);
+  p.ey = pml_of<T>(d->ey);
+  p.ex = pml_of<T>(d->ex);
+  p.Hz = static_cast<T*>(d->Hz);
+  p.Ex = static_cast<T*>(d->Ex);
+  p.Ey = static_cast<T*>(d->Ey);
+  p.Dx = static_cast<T*>(d->Dx);
+  p.Dy = static_cast<T*>(d->Dy);
+  p.pEyx = static_cast<T*>(d->psi_Eyx);
+  p.pExy = static_cast<T*>(d->psi_Exy);
+  p.pHzy = static_cast<T*>(d->psi_Hzy);
+  p.pHzx = static_cast<T*>(d->psi_Hzx);
+  mt_source2d sv{};
+  if (src && src->present) sv = *src;
+  dim3 blk(BX, BY), grd = grid2(p.ny, p.nx, blk);
+  int S = mp.stage_steps > 0 ? mp.stage_steps : 1;
+  int filled = 0, first = step0;
+  for (int s_ = 0; s_ < k; ++s_) {
+    T c = sv.present ? static_cast<T>(coef[s_]) : T(0);
+    hipLaunchKernelGGL(te_d_kernel<T>, grd, blk, 0, s, p, sv, c);
+    hipLaunchKernelGGL(te_b_kernel<T>, grd, blk, 0, s, p);
+    int rc = check_launch("te2d step");
+    if (rc) return rc;
+    if (mp.n_mons) {
+      if ((rc = monitors_sample(mp, filled, s_, s))) return rc;
+      if (++filled == S || s_ == k - 1) {
+        if ((rc = monitors_flush(mp, filled, first, d->dt, s))) return rc;
+        first += filled;
+        filled = 0;
+      }
+    }
+  }
+  return MT_OK;
+}
+
+}  // namespace
+}  // namespace mt
+
+using namespace mt;
+
+extern "C" int mt_tm2d_run(const mt_tm2d* p, const mt_source2d* src, const double* coef,
+                           const mt_monitor* mons, int n_mons, void* stage, long long stage_bytes,
+                           int stage_steps, int step0, int k, void* stream) {
+  if (!p) return fail(MT_EINVAL, "null problem");
+  if (p->nx < 3 || p->ny < 3) return fail(MT_EINVAL, "grid must be at least 3x3");
+  if (k < 0) return fail(MT_EINVAL, "negative step count");
+  if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");
+  if (!p->eps || !p->Ez || !p->Dz || !p->Hx || !p->Hy || !p->psi_Ezy || !p->psi_Ezx || !p->psi_Hyx ||
+      !p->psi_Hxy)
+    return fail(MT_EINVAL, "null field pointer");
+  int rc;
+  if ((rc = check_axis(p->hy, p->ny - 1, "hy")) || (rc = check_axis(p->hx, p->nx - 1, "hx")) ||
+      (rc = check_axis(p->ex, p->nx - 2, "ex")) ||
+      (rc = check_axis(p->ey, p->y_pmc ? p->ny : p->ny - 2, "ey")))
+    return rc;
+  if ((rc = check_src(src, p->nx, p->ny))) return rc;
+  if (src && src->present && !coef && k > 0) return fail(MT_EINVAL, "source without coefficients");
+  MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
+  if (n_mons && (rc = monitors_validate(mp))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return p->dtype == MT_F64 ? tm_run_t<double>(p, src, coef, mp, step0, k, s)
+                            : tm_run_t<float>(p, src, coef, mp, step0, k, s);
+}
+
+extern "C" int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const double* coef,
+                           const mt_monitor* mons, int n_mons, void* stage, long long stage_bytes,
+                           int stage_steps, int step0, int k, void* stream) {
+  if (!p) return fail(MT_EINVAL, "null problem");
+  if (p->nx < 3 || p->ny < 3) return fail(MT_EINVAL, "grid must be at least 3x3");
+  if (k < 0) return fail(MT_EINVAL, "negative step count");
+  if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");
+  if (!p->eps_x || !p->eps_y || !p->Hz || !p->Ex || !p->Ey || !p->Dx || !p->Dy || !p->psi_Eyx ||
+      !p->psi_Exy || !p->psi_Hzy || !p->psi_Hzx)
+    return fail(MT_EINVAL, "null field pointer");
+  int rc;
+  if ((rc = check_axis(p->hx, p->nx - 1, "hx")) || (rc = check_axis(p->hy, p->ny - 1, "hy")) ||
+      (rc = check_axis(p->ey, p->ny - 2, "ey")) || (rc = check_axis(p->ex, p->nx - 2, "ex")))
+    return rc;
+  if ((rc = check_src(src, p->nx, p->ny - 1))) return rc;
+  if (src && src->present && !coef && k > 0) return fail(MT_EINVAL, "source without coefficients");
+  MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
+  if (n_mons && (rc = monitors_validate(mp))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return p->dtype == MT_F64 ? te_run_t<double>(p, src, coef, mp, step0, k, s)
+                            : te_run_t<float>(p, src, coef, mp, step0, k, s);
+}

@@ -1,0 +1,161 @@
+"""Host side of the 2D engine: turn a meep_torch-style (state, coeffs, source,
+monitors) chunk into one mt_tm2d_run / mt_te2d_run call.
+
+The state tensors are updated in place (callers clone first to keep the
+reference's functional semantics).  Monitor accumulators for the chunk are
+fresh zero buffers; run.py adds chunk results exactly as the reference folds
+chunk partials (meep_torch/core/run.py:56-63)."""
+import ctypes
+
+import torch
+
+from .. import _engine as E
+from .monitors import omega_of
+
+STAGE_STEPS = 32
+STAGE_BUDGET = 1 << 29  # bytes of staging per call
+
+
+def source_plan(source, field_shape, device, dtype):
+    """Reference source protocol (run.py:17-25): .amplitude(t), .mask,
+    optional .h_masks (TFSF, not carried)."""
+    if source is None:
+        return None
+    if getattr(source, "h_masks", None):
+        raise NotImplementedError("TFSF magnetic source (h_masks) is not supported by the HIP stepper")
+    if getattr(source, "per_step_mask", False):
+        raise NotImplementedError(f"{type(source).__name__}: per-step masks are not supported")
+    mask = source.mask
+    if tuple(mask.shape) != tuple(field_shape):
+        raise ValueError(f"source mask shape {tuple(mask.shape)} != field shape {tuple(field_shape)}")
+    box = E.mask_box(mask.to(device=device, dtype=dtype))
+    return {"source": source, "box": box}
+
+
+def coef_array(plan, dt, step0, k):
+    """dt * amp(t) for t in [step0, step0+k), in double, in step order."""
+    if plan is None:
+        return None
+    src = plan["source"]
+    vals = [dt * src.amplitude(t) for t in range(step0, step0 + k)]
+    return (ctypes.c_double * max(1, k))(*vals)
+
+
+def source_struct(plan):
+    s = E.Source2D()
+    if plan is None or plan["box"] is None:
+        s.present = 0
+        return s, None
+    (i0, j0), (i1, j1), w = plan["box"]
+    s.present, s.i0, s.i1, s.j0, s.j1 = 1, i0, i1, j0, j1
+    s.w = E.ptr(w)
+    return s, w
+
+
+def monitor_structs(fields, monitors, k, dtype, device):
+    """Engine descriptors for known monitor classes; output buffers per chunk."""
+    arr = (E.Monitor * max(1, len(monitors)))()
+    outs, keep = [], []
+    for i, mon in enumerate(monitors):
+        if not hasattr(mon, "engine_terms"):
+            raise NotImplementedError(
+                f"monitor {type(mon).__name__} has no engine descriptor (unknown monitor class)")
+        kind, terms = mon.engine_terms(fields)
+        m = arr[i]
+        if len(terms) > E.MAX_TERMS:
+            raise ValueError("too many terms in one monitor")
+        m.n_terms = len(terms)
+        for q, (a, b, half) in enumerate(terms):
+            pa, s0, s1, n0, n1 = E.view2d(a)
+            if q == 0:
+                m.n0, m.n1 = n0, n1
+            elif (n0, n1) != (m.n0, m.n1):
+                raise ValueError("monitor terms must share one extent")
+            m.term[q].a = pa
+            m.term[q].b = E.view2d(b)[0] if b is not None else None
+            m.term[q].s0, m.term[q].s1 = s0, s1
+            m.term[q].t_half = int(half)
+        if kind == "probe":
+            m.kind = 1
+            out = torch.zeros(k, dtype=dtype, device=device)
+        else:
+            m.kind = 0
+            om = omega_of(mon).to(device=device, dtype=dtype)
+            keep.append(om)
+            m.nfreq = om.numel()
+            m.omega = om.data_ptr()
+            out = torch.zeros(m.nfreq, 2 * m.n_terms, m.n0 * m.n1, dtype=dtype, device=device)
+        m.acc = out.data_ptr()
+        outs.append((mon, kind, out))
+    return arr, outs, keep
+
+
+def finish_outputs(outs, fields):
+    res = []
+    for mon, kind, out in outs:
+        res.append(out if kind == "probe" else out.reshape(mon.out_shape(fields)))
+    return res
+
+
+def stage_alloc(L, arr, n, dtype_code, device):
+    if n == 0:
+        return 0, None, 0
+    per_step = L.mt_stage_bytes(arr, n, 1, dtype_code)
+    S = STAGE_STEPS
+    if per_step > 0:
+        S = max(1, min(STAGE_STEPS, STAGE_BUDGET // per_step))
+    nbytes = L.mt_stage_bytes(arr, n, S, dtype_code)
+    buf = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
+    return S, buf, nbytes
+
+
+def _axis(coeffs, name):
+    b = getattr(coeffs, "b_" + name)
+    a = getattr(coeffs, "a_" + name)
+    return E.pml_axis(b, a, coeffs.shells[name] if coeffs.shells else None)
+
+
+def run_chunk(mode, fields, coeffs, step0, k, plan, monitors):
+    """Advance ``fields`` (dict of device tensors, reference shapes) k steps."""
+    L = E.lib()
+    ref = fields["Ez"] if mode == "tm" else fields["Hz"]
+    dtype, device = ref.dtype, ref.device
+    code = E.dtype_code(dtype)
+    E.require_device(*fields.values())
+    if mode == "tm":
+        nx, ny = fields["Ez"].shape
+        d = E.TM2D()
+        d.nx, d.ny, d.dtype, d.y_pmc = nx, ny, code, int(bool(coeffs.y_pmc))
+        d.dt, d.dx = float(coeffs.dt), float(coeffs.dx)
+        eps = coeffs.eps_inf.to(device=device, dtype=dtype).contiguous()
+        d.eps = eps.data_ptr()
+        d.hy, d.hx, d.ex, d.ey = (_axis(coeffs, n) for n in ("hy", "hx", "ex", "ey"))
+        for f in ("Ez", "Dz", "Hx", "Hy", "psi_Ezy", "psi_Ezx", "psi_Hyx", "psi_Hxy"):
+            setattr(d, f, fields[f].data_ptr())
+        keep_c = [eps]
+    else:
+        nx, ny = fields["Ey"].shape[0], fields["Ex"].shape[1]
+        d = E.TE2D()
+        d.nx, d.ny, d.dtype = nx, ny, code
+        d.dt, d.dx = float(coeffs.dt), float(coeffs.dx)
+        ex_ = coeffs.eps_x.to(device=device, dtype=dtype).contiguous()
+        ey_ = coeffs.eps_y.to(device=device, dtype=dtype).contiguous()
+        d.eps_x, d.eps_y = ex_.data_ptr(), ey_.data_ptr()
+        d.hx, d.hy, d.ey, d.ex = (_axis(coeffs, n) for n in ("hx", "hy", "ey", "ex"))
+        for f in ("Hz", "Ex", "Ey", "Dx", "Dy", "psi_Eyx", "psi_Exy", "psi_Hzy", "psi_Hzx"):
+            setattr(d, f, fields[f].data_ptr())
+        keep_c = [ex_, ey_]
+    for f, t in fields.items():
+        if not t.is_contiguous():
+            raise ValueError(f"state field {f} must be contiguous")
+    src, wkeep = source_struct(plan)
+    coef = coef_array(plan, coeffs.dt, step0, k) if src.present else None
+    arr, outs, keep = monitor_structs(fields, list(monitors), k, dtype, device)
+    n = len(monitors)
+    S, stage, nbytes = stage_alloc(L, arr, n, code, device)
+    fn = L.mt_tm2d_run if mode == "tm" else L.mt_te2d_run
+    rc = fn(ctypes.byref(d), ctypes.byref(src), coef, arr, n,
+            E.ptr(stage) if stage is not None else None, nbytes, S, step0, k, E.stream_handle())
+    E.check(rc, f"mt_{mode}2d_run")
+    del keep_c, wkeep, keep
+    return finish_outputs(outs, fields)

@@ -1,0 +1,103 @@
+"""Host side of the 3D engine: one mt_yee3d_run call per run.
+
+Field dict keys are the State3D names (reference shapes); psi arrays are the
+full-grid reference tensors.  Sources follow the reference tuples
+(stepper_3d.py:448-481): (comp, i, j, k) point, (comp, mask) mask,
+(comp, ix, plane) plane-indexed."""
+import ctypes
+
+import torch
+
+from .. import _engine as E
+from .engine2d import stage_alloc
+from .stepper_3d import COMPS, PSI_NAMES
+
+_COMP = {"Ex": 0, "Ey": 1, "Ez": 2}
+
+
+def _source(src, fields, lean, dtype, device):
+    s = E.Source3D()
+    if src is None:
+        return s, None
+    comp = src[0]
+    if comp not in _COMP:
+        raise ValueError(f"3D source component must be Ex/Ey/Ez, got {comp!r}")
+    s.comp = _COMP[comp]
+    shape = tuple(fields[comp].shape)
+    if len(src) == 4:
+        i, j, k = (int(v) for v in src[1:])
+        for v, n in zip((i, j, k), shape):
+            if not 0 <= v < n:
+                raise IndexError(f"point source {src} outside the {comp} grid {shape}")
+        s.kind = 1
+        s.i0, s.i1, s.j0, s.j1, s.k0, s.k1 = i, i + 1, j, j + 1, k, k + 1
+        return s, None
+    if len(src) == 2:
+        mask = src[1].to(device=device, dtype=dtype)
+        if tuple(mask.shape) != shape:
+            raise ValueError(f"source mask shape {tuple(mask.shape)} != {comp} grid {shape}")
+        box = E.mask_box(mask)
+    elif len(src) == 3:
+        ix, plane = int(src[1]), src[2].to(device=device, dtype=dtype)
+        mask = torch.zeros(shape, dtype=dtype, device=device)
+        # functional plane-indexed form ADDS (stepper_3d.py:462-474): negate
+        # the weights so the kernel's subtraction adds exactly
+        mask[ix] = plane if lean else -plane
+        box = E.mask_box(mask)
+    else:
+        raise ValueError(f"unrecognised 3D source tuple of length {len(src)}")
+    if box is None:
+        return E.Source3D(), None
+    (i0, j0, k0), (i1, j1, k1), w = box
+    s.kind = 2
+    s.i0, s.i1, s.j0, s.j1, s.k0, s.k1 = i0, i1, j0, j1, k0, k1
+    s.w = w.data_ptr()
+    return s, w
+
+
+def _axis(c, key):
+    b, a, _ = c.cpml[key]
+    return E.pml_axis(b, a, c.shells[key] if c.shells else None)
+
+
+def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0):
+    """Advance the field dict n_steps in place; returns monitor outputs."""
+    from .engine2d import finish_outputs, monitor_structs
+    L = E.lib()
+    if c.cpml is None:
+        raise NotImplementedError("3D run without CPML is not supported by the HIP stepper")
+    ref = fields["Ex"]
+    dtype, device = ref.dtype, ref.device
+    code = E.dtype_code(dtype)
+    E.require_device(*fields.values())
+    for f, t in fields.items():
+        if not t.is_contiguous():
+            raise ValueError(f"state field {f} must be contiguous")
+    nx, ny, nz = fields["Ey"].shape[0], fields["Ex"].shape[1], fields["Ex"].shape[2]
+    d = E.Yee3D()
+    d.nx, d.ny, d.nz, d.dtype, d.lean = nx, ny, nz, code, int(bool(lean))
+    d.dt, d.dx = float(c.dt), float(c.dx)
+    eps = c.eps.to(device=device, dtype=dtype).contiguous()
+    d.eps = eps.data_ptr()
+    for a, (kh, ke) in enumerate((("hx", "ex"), ("hy", "ey"), ("hz", "ez"))):
+        d.h[a] = _axis(c, kh)
+        d.e[a] = _axis(c, ke)
+    for comp in COMPS:
+        setattr(d, comp, fields[comp].data_ptr())
+    for q, name in enumerate(PSI_NAMES):
+        d.psi[q] = fields[name].data_ptr()
+    src_t = source.src if source is not None else None
+    s, wkeep = _source(src_t, fields, lean, dtype, device)
+    coef = None
+    if s.kind:
+        vals = [c.dt * source.amplitude(t) for t in range(step0, step0 + n_steps)]
+        coef = (ctypes.c_double * max(1, n_steps))(*vals)
+    arr, outs, keep = monitor_structs(fields, list(monitors), n_steps, dtype, device)
+    n = len(monitors)
+    S, stage, nbytes = stage_alloc(L, arr, n, code, device)
+    rc = L.mt_yee3d_run(ctypes.byref(d), ctypes.byref(s), coef, arr, n,
+                        E.ptr(stage) if stage is not None else None, nbytes, S, step0, n_steps,
+                        E.stream_handle())
+    E.check(rc, "mt_yee3d_run")
+    del eps, wkeep, keep
+    return finish_outputs(outs, fields)

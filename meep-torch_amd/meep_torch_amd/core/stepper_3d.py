@@ -1,0 +1,176 @@
+"""3D Yee problem/state types and single step (meep_torch/core/stepper_3d.py).
+
+Coeffs3D / State3D keep the reference fields and shapes (State3D psi arrays are
+full-grid; the kernels touch them only inside the CPML shells).  The HIP
+stepper runs the CPML path with sigma = 0 (the configuration the reference's
+Simulation and lean stepper use); the matched-absorber path, material
+conductivity, ADE poles and PMC walls raise NotImplementedError."""
+import math
+from collections.abc import Mapping
+from typing import NamedTuple
+
+import torch
+
+from .pml import cpml_profiles, side_counts
+from .._engine import shell_bounds
+
+
+class Coeffs3D(NamedTuple):
+    dt: float
+    dx: float
+    eps: torch.Tensor
+    damp: Mapping
+    cpml: dict | None
+    pols: dict | None = None
+    pmc: tuple = ()
+    shells: dict | None = None
+
+
+class State3D(NamedTuple):
+    Ex: torch.Tensor
+    Ey: torch.Tensor
+    Ez: torch.Tensor
+    Hx: torch.Tensor
+    Hy: torch.Tensor
+    Hz: torch.Tensor
+    psi_Ez_y: torch.Tensor | None = None
+    psi_Ey_z: torch.Tensor | None = None
+    psi_Ex_z: torch.Tensor | None = None
+    psi_Ez_x: torch.Tensor | None = None
+    psi_Ey_x: torch.Tensor | None = None
+    psi_Ex_y: torch.Tensor | None = None
+    psi_Hz_y: torch.Tensor | None = None
+    psi_Hy_z: torch.Tensor | None = None
+    psi_Hx_z: torch.Tensor | None = None
+    psi_Hz_x: torch.Tensor | None = None
+    psi_Hy_x: torch.Tensor | None = None
+    psi_Hx_y: torch.Tensor | None = None
+    Px_cur: torch.Tensor | None = None
+    Px_prev: torch.Tensor | None = None
+    Py_cur: torch.Tensor | None = None
+    Py_prev: torch.Tensor | None = None
+    Pz_cur: torch.Tensor | None = None
+    Pz_prev: torch.Tensor | None = None
+
+
+PSI_NAMES = ("psi_Ez_y", "psi_Ey_z", "psi_Ex_z", "psi_Ez_x", "psi_Ey_x", "psi_Ex_y",
+             "psi_Hz_y", "psi_Hy_z", "psi_Hx_z", "psi_Hz_x", "psi_Hy_x", "psi_Hx_y")
+COMPS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+def _avg(v, axis):
+    n = v.shape[axis]
+    return 0.5 * (v.narrow(axis, 0, n - 1) + v.narrow(axis, 1, n - 1))
+
+
+def _bc(v, ax):
+    sh = [1, 1, 1]
+    sh[ax] = v.shape[0]
+    return v.reshape(sh)
+
+
+def _axis_sides3(v):
+    if isinstance(v, (tuple, list)):
+        return tuple(side_counts(p) for p in v)
+    return (side_counts(v),) * 3
+
+
+class _Damp(Mapping):
+    """Per-component (d1, d2) conductivity pairs of stepper_3d.py:226-235,
+    built on first access (all-ones / all-dt at sigma = 0)."""
+
+    def __init__(self, eps, dt):
+        self._eps, self._dt, self._cache = eps, dt, {}
+
+    def _pair(self, key):
+        e = self._eps
+        s = torch.zeros_like(e)
+        dt = self._dt
+        avg = {"ex": lambda v: _avg(v, 0), "ey": lambda v: _avg(v, 1), "ez": lambda v: _avg(v, 2),
+               "hx": lambda v: _avg(_avg(v, 1), 2), "hy": lambda v: _avg(_avg(v, 0), 2),
+               "hz": lambda v: _avg(_avg(v, 0), 1)}[key](s)
+        return ((1 - avg * dt / 2) / (1 + avg * dt / 2), dt / (1 + avg * dt / 2))
+
+    def __getitem__(self, key):
+        if key not in self._cache:
+            self._cache[key] = self._pair(key)
+        return self._cache[key]
+
+    def __iter__(self):
+        return iter(("ex", "ey", "ez", "hx", "hy", "hz"))
+
+    def __len__(self):
+        return 6
+
+
+def build_3d_coeffs(nx: int, ny: int, nz: int, dx: float, eps=None, npml=0, n_abs=0,
+                    absorber_axes=(), sigma_d=None, sigma_b=None, poles=(), pole_masks=None,
+                    dtype=torch.float64, device="cuda", absorber_m: int = 2,
+                    absorber_R0: float = 1e-15, pmc_axes=()) -> Coeffs3D:
+    """stepper_3d.py:150-266, CPML subset: dt = 0.5 dx / sqrt(3); per axis an
+    H-side profile of length n-1 at offset 1/2 and an E-side profile of
+    length n at offset 0 trimmed to [1:-1]."""
+    if any(l or h for l, h in _axis_sides3(n_abs)):
+        raise NotImplementedError("3D matched-sigma absorber (n_abs) is not supported by the HIP stepper")
+    if sigma_d is not None or sigma_b is not None:
+        raise NotImplementedError("3D sigma_d/sigma_b are not supported by the HIP stepper")
+    if len(poles):
+        raise NotImplementedError("3D ADE poles are not supported by the HIP stepper")
+    if len(pmc_axes):
+        raise NotImplementedError("3D PMC walls are not supported by the HIP stepper")
+    npml_ax = _axis_sides3(npml)
+    if not any(l or h for l, h in npml_ax):
+        raise NotImplementedError("3D run without CPML (absorber-path arithmetic) is not supported")
+    dt = 0.5 * dx / math.sqrt(3.0)
+    e = (eps if isinstance(eps, torch.Tensor) else torch.ones(nx, ny, nz, dtype=dtype))
+    e = e.to(dtype=dtype, device=device)
+    dims = (nx, ny, nz)
+    cpml, shells = {}, {}
+    for ax, kh, ke in ((0, "hx", "ex"), (1, "hy", "ey"), (2, "hz", "ez")):
+        n = dims[ax]
+        b, a, k = cpml_profiles(n - 1, npml_ax[ax], 0.5, dt, dx, dtype=dtype)
+        shells[kh] = shell_bounds(b, a)
+        cpml[kh] = tuple(_bc(v.to(device), ax) for v in (b, a, k))
+        b, a, k = cpml_profiles(n, npml_ax[ax], 0.0, dt, dx, dtype=dtype)
+        b, a, k = b[1:-1].contiguous(), a[1:-1].contiguous(), k[1:-1].contiguous()
+        shells[ke] = shell_bounds(b, a)
+        cpml[ke] = tuple(_bc(v.to(device), ax) for v in (b, a, k))
+    return Coeffs3D(dt=dt, dx=dx, eps=e, damp=_Damp(e, dt), cpml=cpml, pols=None, pmc=(),
+                    shells=shells)
+
+
+def init_3d_state(nx: int, ny: int, nz: int, npml: int = 0, n_poles: int = 0,
+                  dtype=torch.float64, device="cuda") -> State3D:
+    """stepper_3d.py:269-292."""
+    if n_poles:
+        raise NotImplementedError("3D ADE poles are not supported by the HIP stepper")
+    z = lambda *s: torch.zeros(*s, dtype=dtype, device=device)
+    psi = {}
+    if npml > 0:
+        psi = dict(
+            psi_Ez_y=z(nx, ny - 1, nz - 1), psi_Ey_z=z(nx, ny - 1, nz - 1),
+            psi_Ex_z=z(nx - 1, ny, nz - 1), psi_Ez_x=z(nx - 1, ny, nz - 1),
+            psi_Ey_x=z(nx - 1, ny - 1, nz), psi_Ex_y=z(nx - 1, ny - 1, nz),
+            psi_Hz_y=z(nx - 1, ny - 2, nz), psi_Hy_z=z(nx - 1, ny, nz - 2),
+            psi_Hx_z=z(nx, ny - 1, nz - 2), psi_Hz_x=z(nx - 2, ny - 1, nz),
+            psi_Hy_x=z(nx - 2, ny, nz - 1), psi_Hx_y=z(nx, ny - 2, nz - 1))
+    return State3D(z(nx - 1, ny, nz), z(nx, ny - 1, nz), z(nx, ny, nz - 1),
+                   z(nx, ny - 1, nz - 1), z(nx - 1, ny, nz - 1), z(nx - 1, ny - 1, nz), **psi)
+
+
+class _OneStep:
+    def __init__(self, amp, src):
+        self._amp, self.src = float(amp), src
+
+    def amplitude(self, t):
+        return self._amp
+
+
+def step_3d(state: State3D, c: Coeffs3D, src_amp: float = 0.0, src=None) -> State3D:
+    """One leapfrog step (stepper_3d.py:303-445) on the HIP engine.  Unlike the
+    reference's point-source path, which writes into the new state's tensor in
+    place (stepper_3d.py:476-481), the input state is never modified."""
+    from .engine3d import run_fields
+    work = {f: getattr(state, f).clone() for f in COMPS + PSI_NAMES}
+    run_fields(work, c, 1, _OneStep(src_amp, src) if src is not None else None, [], lean=False)
+    return state._replace(**work)

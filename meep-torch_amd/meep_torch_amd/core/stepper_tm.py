@@ -1,0 +1,77 @@
+"""2D TM state and single step (meep_torch/core/stepper_tm.py:12-71, 234-254).
+
+TMState keeps the reference fields and shapes.  step_tm advances one step on
+the HIP engine (functional: the input state is not modified)."""
+from typing import NamedTuple
+
+import torch
+
+from .coeffs import TMCoeffs
+from .grid import Grid
+
+
+class TMState(NamedTuple):
+    Ez: torch.Tensor       # (nx, ny)   E_z at time n
+    Hx: torch.Tensor       # (nx, ny-1) at n-1/2
+    Hy: torch.Tensor       # (nx-1, ny) at n-1/2
+    Dz: torch.Tensor       # (nx, ny)
+    P_cur: torch.Tensor    # (n_poles, nx, ny)
+    P_prev: torch.Tensor
+    psi_Ezy: torch.Tensor  # (nx, ny-1)
+    psi_Ezx: torch.Tensor  # (nx-1, ny)
+    psi_Hyx: torch.Tensor  # (nx-2, ny_e)
+    psi_Hxy: torch.Tensor  # (nx-2, ny_e)
+    Bx: torch.Tensor | None = None
+    By: torch.Tensor | None = None
+    Mx_cur: torch.Tensor | None = None
+    Mx_prev: torch.Tensor | None = None
+    My_cur: torch.Tensor | None = None
+    My_prev: torch.Tensor | None = None
+    Hx0: torch.Tensor | None = None
+    Hy0: torch.Tensor | None = None
+
+
+FIELDS = ("Ez", "Dz", "Hx", "Hy", "psi_Ezy", "psi_Ezx", "psi_Hyx", "psi_Hxy")
+
+
+def init_tm_state(grid: Grid, n_poles: int = 0, y_pmc: bool = False, n_mpoles: int = 0,
+                  y_mirror: int = 0, x_mirror: int = 0, y_rot2: int = 0, rot4: int = 0) -> TMState:
+    """stepper_tm.py:37-71 (psi_H rows: ny-2 with PEC walls, ny under y_pmc)."""
+    if n_poles or n_mpoles or y_mirror or x_mirror or y_rot2 or rot4:
+        raise NotImplementedError(
+            "init_tm_state: poles / magnetic poles / symmetry folds are not supported by the HIP stepper")
+    nx, ny = grid.nx, grid.ny
+    ny_e = ny if y_pmc else ny - 2
+    return TMState(
+        Ez=grid.zeros(nx, ny), Hx=grid.zeros(nx, ny - 1), Hy=grid.zeros(nx - 1, ny),
+        Dz=grid.zeros(nx, ny), P_cur=grid.zeros(0, nx, ny), P_prev=grid.zeros(0, nx, ny),
+        psi_Ezy=grid.zeros(nx, ny - 1), psi_Ezx=grid.zeros(nx - 1, ny),
+        psi_Hyx=grid.zeros(nx - 2, ny_e), psi_Hxy=grid.zeros(nx - 2, ny_e))
+
+
+def _check(state: TMState, m: TMCoeffs):
+    if state.P_cur.shape[0] or state.Bx is not None or state.Hx0 is not None:
+        raise NotImplementedError("TM state with poles / magnetic poles / mirror rows")
+    nx, ny = state.Ez.shape
+    ny_e = ny if m.y_pmc else ny - 2
+    if tuple(state.psi_Hxy.shape) != (nx - 2, ny_e):
+        raise ValueError(f"psi_Hxy shape {tuple(state.psi_Hxy.shape)} does not match y_pmc={m.y_pmc}")
+
+
+def step_tm(state: TMState, m: TMCoeffs, src_amp=0.0, src_mask: torch.Tensor | None = None,
+            src_h_amp: float = 0.0, src_hy_mask: torch.Tensor | None = None,
+            src_hx_mask: torch.Tensor | None = None) -> TMState:
+    """One TM step in meep's phase order (B, source(B), eh(H), D, source(D),
+    pols, eh(E)) on the HIP engine."""
+    if src_hy_mask is not None or src_hx_mask is not None:
+        raise NotImplementedError("TFSF magnetic source masks are not supported by the HIP stepper")
+    from .run import _Fixed
+    from . import engine2d
+    _check(state, m)
+    work = {f: getattr(state, f).clone() for f in FIELDS}
+    plan = None
+    if src_mask is not None:
+        plan = engine2d.source_plan(_Fixed(src_amp, src_mask), work["Dz"].shape,
+                                    work["Dz"].device, work["Dz"].dtype)
+    engine2d.run_chunk("tm", work, m, 0, 1, plan, [])
+    return state._replace(**work)

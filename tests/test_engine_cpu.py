@@ -1,0 +1,103 @@
+"""CPU-side checks of the engine boundary: the C-ABI library loads and exports
+every entry point include/meep_fdtd.h declares; host logic (profiles, shell
+bounds, source boxes, feature-flag errors) behaves like the reference."""
+import ctypes
+import os
+import re
+
+import pytest
+import torch
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "meep_fdtd.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w ]+?\*?\s*\b(mt_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from meep_torch_amd import _engine
+    L = _engine.lib()
+    names = _declared()
+    assert {"mt_tm2d_run", "mt_te2d_run", "mt_yee3d_run", "mt_last_error"} <= set(names)
+    for n in names:
+        assert hasattr(L, n), f"libmeepfdtd.so does not export {n}"
+    assert L.mt_abi_version() == _engine.ABI_VERSION
+
+
+def test_error_path_without_gpu_work():
+    """Invalid descriptors are rejected on the host before any launch."""
+    from meep_torch_amd import _engine as E
+    L = E.lib()
+    d = E.TM2D()
+    d.nx, d.ny = 2, 2
+    rc = L.mt_tm2d_run(ctypes.byref(d), None, None, None, 0, None, 0, 0, 0, 1, None)
+    assert rc != 0 and b"3x3" in L.mt_last_error()
+    rc = L.mt_tm2d_run(None, None, None, None, 0, None, 0, 0, 0, 1, None)
+    assert rc != 0
+
+
+def test_shell_bounds_match_profile():
+    from meep_torch_amd._engine import shell_bounds
+    from meep_torch_amd.core.pml import cpml_profiles
+    b, a, _ = cpml_profiles(255, 16, 0.5, 0.025, 0.05, dtype=torch.float32)
+    lo, hi = shell_bounds(b, a)
+    assert torch.all(b[lo:hi] == 1) and torch.all(a[lo:hi] == 0)
+    assert b[lo - 1] != 1 and b[hi] != 1
+    b, a, _ = cpml_profiles(50, 0, 0.0, 0.025, 0.05)
+    assert shell_bounds(b, a) == (0, 50)
+
+
+def test_package_profiles_bitexact_vs_golden():
+    import json
+    from conftest import golden
+    from meep_torch_amd.core.pml import cpml_profiles
+    g = golden("pml")
+    for ci, (n, npml, off) in enumerate(json.loads(bytes(g["combos_json"]).decode())):
+        npml = tuple(npml) if isinstance(npml, list) else npml
+        for tag, dt in (("f32", torch.float32), ("f64", torch.float64)):
+            b, a, k = cpml_profiles(n, npml, off, 0.025, 0.05, dtype=dt)
+            assert torch.equal(b, torch.from_numpy(g[f"{tag}_{ci}_b"]))
+            assert torch.equal(a, torch.from_numpy(g[f"{tag}_{ci}_a"]))
+
+
+def test_mask_box():
+    from meep_torch_amd._engine import mask_box
+    m = torch.zeros(10, 8)
+    m[3, 2:6] = 2.0
+    (i0, j0), (i1, j1), w = mask_box(m)
+    assert (i0, i1, j0, j1) == (3, 4, 2, 6) and torch.equal(w, m[3:4, 2:6])
+    assert mask_box(torch.zeros(4, 4)) is None
+
+
+@pytest.mark.parametrize("kw,flag", [({"chi3": 0.1}, "chi3"), ({"poles": [(1.0, 0.1, 1.0)]}, "poles"),
+                                     ({"sigma_d": 0.1}, "sigma"), ({"y_mirror": 1}, "symmetry")])
+def test_unsupported_flags_raise(kw, flag):
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    g = Grid(nx=16, ny=12, dx=0.05, dtype=torch.float32, device="cpu")
+    with pytest.raises(NotImplementedError, match=flag):
+        build_tm_coeffs(g, npml=2, **kw)
+
+
+def test_run_requires_hip_device():
+    """No CPU fallback: CPU tensors are refused loudly."""
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.run import run_tm
+    from meep_torch_amd.core.stepper_tm import init_tm_state
+    g = Grid(nx=16, ny=12, dx=0.05, dtype=torch.float32, device="cpu")
+    with pytest.raises(RuntimeError, match="HIP engine"):
+        run_tm(init_tm_state(g), build_tm_coeffs(g, npml=2), 3)
+
+
+def test_grid_validation():
+    from meep_torch_amd.core.grid import Grid
+    with pytest.raises(ValueError):
+        Grid(nx=2, ny=8, dx=0.1)
+    with pytest.raises(ValueError):
+        Grid(nx=8, ny=8, dx=0.1, courant=0.8)
+    assert Grid(nx=8, ny=8, dx=0.1, courant=0.5, device="cpu").dt == 0.05
