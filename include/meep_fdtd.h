@@ -51,6 +51,7 @@ typedef struct {
   void *Ez, *Dz, *Hx, *Hy;               /* device (nx,ny) (nx,ny) (nx,ny-1) (nx-1,ny) */
   void *psi_Ezy, *psi_Ezx;               /* device (nx,ny-1) (nx-1,ny) */
   void *psi_Hyx, *psi_Hxy;               /* device (nx-2, ny_e), ny_e = ny-2 or ny (y_pmc) */
+  void* tape;                            /* NULL, or device (k, nx, ny): Ez after each step of the call */
 } mt_tm2d;
 
 /* 2D TE problem + state (coeffs.py:294-345 TECoeffs, stepper_te.py:11-24). */
@@ -148,6 +149,29 @@ int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const double* coef,
 int mt_yee3d_run(const mt_yee3d* p, const mt_source3d* src, const double* coef,
                  const mt_monitor* mons, int n_mons, void* stage, long long stage_bytes,
                  int stage_steps, int step0, int k, void* stream);
+
+/* Adjoint state of a 2D TM run: cotangents of every TMState field (same
+ * shapes), scratch for the transposed differences, and the eps gradient. */
+typedef struct {
+  void *aEz, *aDz, *aHx, *aHy;
+  void *apsi_Ezy, *apsi_Ezx, *apsi_Hyx, *apsi_Hxy;
+  void *gHy, *gHx;                        /* scratch (nx, ny) each */
+  void *gEx, *gEy;                        /* scratch (nx-1, ny) and (nx, ny-1) */
+  void* eps_grad;                         /* device (nx, ny), accumulated */
+} mt_tm2d_adj;
+
+/* Reverse-time adjoint of k TM steps [step0, step0+k): on entry the mt_tm2d_adj
+ * state holds the cotangent of the state after step step0+k-1, on return the
+ * cotangent of the state before step step0; eps_grad gains dL/deps of the k
+ * steps.  `tape` (k, nx, ny) holds the forward Ez after each step (recorded by
+ * mt_tm2d_run with p->tape set).  Monitor descriptors are those of the forward
+ * run with term views pointing into the ADJOINT fields and `acc` holding the
+ * accumulator cotangent dL/dacc; each step injects
+ *   dt*(cos(w t) dacc_re - sin(w t) dacc_im)  (halved onto both rows of an
+ * averaged term).  Replaces PyTorch autograd through run_tm's checkpointed
+ * chunks (meep_torch/core/run.py:99-120, eps reached via stepper_tm.py:226-227). */
+int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const void* tape,
+                    const mt_monitor* mons, int n_mons, int step0, int k, void* stream);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* mt_last_error(void);

@@ -94,39 +94,33 @@ __global__ void phase_kernel(const T* omega, int nfreq, int n_filled, int t0, do
   o[3] = sin(ph);
 }
 
-// One thread per (term, element); accumulator layout (nfreq, 2*n_terms, n).
-template <typename T, int CH>
+// One thread per (element, frequency, term): the accumulator pair is read and
+// written once per flush, the staged samples are read coalesced along the
+// element axis and the phase row (one frequency per block) is uniform across
+// the block.  Accumulator layout (nfreq, 2*n_terms, n).
+template <typename T>
 __global__ void flush_kernel(const T* stage, const T* table, T* acc, int n_terms, int n, int nfreq,
                              int n_filled, int half_mask, T dt) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
-  int q = blockIdx.y;
+  int f = blockIdx.y;
+  int q = blockIdx.z;
   if (e >= n) return;
   const T* st = stage + (long long)q * n + e;  // slot stride = n_terms * n
   long long slot_stride = (long long)n_terms * n;
   int toff = ((half_mask >> q) & 1) ? 2 : 0;
-  for (int s0 = 0; s0 < n_filled; s0 += CH) {
-    int m = n_filled - s0 < CH ? n_filled - s0 : CH;
-    T v[CH];
-#pragma unroll
-    for (int u = 0; u < CH; ++u) v[u] = u < m ? st[(long long)(s0 + u) * slot_stride] : T(0);
-    for (int f = 0; f < nfreq; ++f) {
-      T* are = acc + ((long long)f * 2 * n_terms + 2 * q) * n + e;
-      T* aim = are + n;
-      T re = *are, im = *aim;
-      const T* tb = table + ((long long)f * n_filled + s0) * 4 + toff;
-#pragma unroll
-      for (int u = 0; u < CH; ++u) {
-        if (u < m) {
-          T c = tb[u * 4 + 0];
-          T sn = tb[u * 4 + 1];
-          re = re + dt * (c * v[u]);
-          im = im + dt * ((-sn) * v[u]);
-        }
-      }
-      *are = re;
-      *aim = im;
-    }
+  T* are = acc + ((long long)f * 2 * n_terms + 2 * q) * n + e;
+  T* aim = are + n;
+  T re = *are, im = *aim;
+  const T* tb = table + (long long)f * n_filled * 4 + toff;
+  for (int s = 0; s < n_filled; ++s) {
+    T v = st[(long long)s * slot_stride];
+    T c = tb[s * 4 + 0];
+    T sn = tb[s * 4 + 1];
+    re = re + dt * (c * v);
+    im = im + dt * ((-sn) * v);
   }
+  *are = re;
+  *aim = im;
 }
 
 template <typename T>
@@ -247,8 +241,9 @@ static int flush_t(const MonitorPlan& mp, int n_filled, int t0, double dt, hipSt
                        static_cast<const T*>(m.omega), m.nfreq, n_filled, t0, dt, table);
     int half_mask = 0;
     for (int q = 0; q < m.n_terms; ++q) half_mask |= (m.term[q].t_half ? 1 : 0) << q;
-    dim3 grid((unsigned)((n + 255) / 256), (unsigned)m.n_terms);
-    hipLaunchKernelGGL((flush_kernel<T, 8>), grid, dim3(256), 0, s, (const T*)stage, (const T*)table,
+    int blk = n >= 256 ? 256 : 64;
+    dim3 grid((unsigned)((n + blk - 1) / blk), (unsigned)m.nfreq, (unsigned)m.n_terms);
+    hipLaunchKernelGGL((flush_kernel<T>), grid, dim3(blk), 0, s, (const T*)stage, (const T*)table,
                        static_cast<T*>(m.acc), m.n_terms, (int)n, m.nfreq, n_filled, half_mask,
                        static_cast<T>(dt));
     int rc = check_launch("monitor flush");
@@ -280,4 +275,4 @@ extern "C" long long mt_stage_bytes(const mt_monitor* mons, int n_mons, int stag
 
 extern "C" const char* mt_last_error(void) { return mt::g_err.c_str(); }
 
-extern "C" int mt_abi_version(void) { return 1; }
+extern "C" int mt_abi_version(void) { return 2; }

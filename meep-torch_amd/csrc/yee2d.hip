@@ -28,6 +28,7 @@ struct TmK {
   Pml<T> hy, hx, ex, ey;
   T *Ez, *Dz, *Hx, *Hy, *pEzy, *pEzx, *pHyx, *pHxy;
   int ny_e;
+  T* tape;  // optional (k, nx, ny) record of Ez after each step
 };
 
 // meep_torch/core/stepper_tm.py:85-92,126-128 (B half, H from curl E)
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(BX* BY) void tm_b_kernel(TmK<T> p) {
 
 // meep_torch/core/stepper_tm.py:130-166 (D half), :197-200 (source), :226-227 (Ez)
 template <typename T>
-__global__ __launch_bounds__(BX* BY) void tm_d_kernel(TmK<T> p, mt_source2d src, T coef) {
+__global__ __launch_bounds__(BX* BY) void tm_d_kernel(TmK<T> p, mt_source2d src, T coef, int s_rel) {
   int j = blockIdx.x * BX + threadIdx.x;
   int i = blockIdx.y * BY + threadIdx.y;
   if (i >= p.nx || j >= p.ny) return;
@@ -99,7 +100,9 @@ __global__ __launch_bounds__(BX* BY) void tm_d_kernel(TmK<T> p, mt_source2d src,
     dz = dz - coef * w[(long long)(i - src.i0) * (src.j1 - src.j0) + (j - src.j0)];
   }
   p.Dz[c] = dz;
-  p.Ez[c] = dz / p.eps[c];
+  T ez = dz / p.eps[c];
+  p.Ez[c] = ez;
+  if (p.tape) p.tape[(long long)s_rel * nx * ny + c] = ez;
 }
 
 template <typename T>
@@ -216,6 +219,7 @@ int tm_run_t(const mt_tm2d* d, const mt_source2d* src, const double* coef, const
   p.pHyx = static_cast<T*>(d->psi_Hyx);
   p.pHxy = static_cast<T*>(d->psi_Hxy);
   p.ny_e = d->y_pmc ? d->ny : d->ny - 2;
+  p.tape = static_cast<T*>(d->tape);
   mt_source2d sv{};
   if (src && src->present) sv = *src;
   dim3 blk(BX, BY), grd = grid2(p.ny, p.nx, blk);
@@ -224,7 +228,7 @@ int tm_run_t(const mt_tm2d* d, const mt_source2d* src, const double* coef, const
   for (int s_ = 0; s_ < k; ++s_) {
     hipLaunchKernelGGL(tm_b_kernel<T>, grd, blk, 0, s, p);
     T c = sv.present ? static_cast<T>(coef[s_]) : T(0);
-    hipLaunchKernelGGL(tm_d_kernel<T>, grd, blk, 0, s, p, sv, c);
+    hipLaunchKernelGGL(tm_d_kernel<T>, grd, blk, 0, s, p, sv, c, s_);
     int rc = check_launch("tm2d step");
     if (rc) return rc;
     if (mp.n_mons) {

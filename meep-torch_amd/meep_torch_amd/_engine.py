@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -30,7 +30,14 @@ class TM2D(ctypes.Structure):
                 ("hy", PmlAxis), ("hx", PmlAxis), ("ex", PmlAxis), ("ey", PmlAxis),
                 ("Ez", c_void_p), ("Dz", c_void_p), ("Hx", c_void_p), ("Hy", c_void_p),
                 ("psi_Ezy", c_void_p), ("psi_Ezx", c_void_p),
-                ("psi_Hyx", c_void_p), ("psi_Hxy", c_void_p)]
+                ("psi_Hyx", c_void_p), ("psi_Hxy", c_void_p), ("tape", c_void_p)]
+
+
+class TM2DAdj(ctypes.Structure):
+    _fields_ = [("aEz", c_void_p), ("aDz", c_void_p), ("aHx", c_void_p), ("aHy", c_void_p),
+                ("apsi_Ezy", c_void_p), ("apsi_Ezx", c_void_p), ("apsi_Hyx", c_void_p),
+                ("apsi_Hxy", c_void_p), ("gHy", c_void_p), ("gHx", c_void_p), ("gEx", c_void_p),
+                ("gEy", c_void_p), ("eps_grad", c_void_p)]
 
 
 class TE2D(ctypes.Structure):
@@ -97,6 +104,9 @@ def lib():
     L.mt_te2d_run.argtypes = [P(TE2D), P(Source2D), P(c_double)] + common
     L.mt_yee3d_run.restype = c_int
     L.mt_yee3d_run.argtypes = [P(Yee3D), P(Source3D), P(c_double)] + common
+    L.mt_tm2d_adjoint.restype = c_int
+    L.mt_tm2d_adjoint.argtypes = [P(TM2D), P(TM2DAdj), c_void_p, P(Monitor), c_int, c_int, c_int,
+                                  c_void_p]
     if L.mt_abi_version() != ABI_VERSION:
         raise RuntimeError("meep_torch_amd: libmeepfdtd.so ABI mismatch; rebuild")
     _lib = L

@@ -1,6 +1,172 @@
-"""Hand-written time-reversed adjoint for run_tm (placeholder until the
-adjoint kernels land)."""
+"""Hand-written time-reversed adjoint for run_tm behind torch.autograd.Function.
+
+Replaces the reference's autograd through torch.utils.checkpoint chunks
+(meep_torch/core/run.py:63-118).  Forward: the HIP stepper runs the whole
+interval, saving the full TM state on the device every K steps (K =
+checkpoint_every, or 200).  Backward: for each chunk, last first, the chunk is
+recomputed from its checkpoint while a tape records Ez after every step, then
+mt_tm2d_adjoint sweeps the chunk backwards, injecting the monitor cotangents
+dL/dacc and accumulating dL/deps = sum_t -(dL/dEz_t) Ez_t / eps.  The gradient
+reaches whatever graph produced eps_inf (filters, projection, paint) through
+autograd as usual.
+
+Monitor accumulators are summed exactly as the forward-only run sums them (one
+sequential sum per reference chunk), so forward values are identical with or
+without gradients.  Supported: eps_inf as the only grad-requiring coefficient;
+TM mode; every monitor class of the 2D engine.  Gradients with respect to the
+initial state are returned for the fields (psi: inside the CPML shells)."""
+import ctypes
+
+import torch
+
+from .. import _engine as E
+from . import engine2d
+from .stepper_tm import FIELDS
+
+DEFAULT_K = 200
+
+
+def _adj_monitors(adj_fields, monitors, gaccs, dtype, device):
+    """Monitor descriptors whose term views point into the adjoint fields and
+    whose acc holds dL/dacc (skips monitors the loss does not reach)."""
+    used = [(m, g) for m, g in zip(monitors, gaccs) if g is not None]
+    arr = (E.Monitor * max(1, len(used)))()
+    keep = []
+    for i, (mon, g) in enumerate(used):
+        kind, terms = mon.engine_terms(adj_fields)
+        m = arr[i]
+        m.n_terms = len(terms)
+        for q, (a, b, half) in enumerate(terms):
+            pa, s0, s1, n0, n1 = E.view2d(a)
+            if q == 0:
+                m.n0, m.n1 = n0, n1
+            m.term[q].a = pa
+            m.term[q].b = E.view2d(b)[0] if b is not None else None
+            m.term[q].s0, m.term[q].s1 = s0, s1
+            m.term[q].t_half = int(half)
+        gc = g.detach().to(device=device, dtype=dtype).contiguous()
+        keep.append(gc)
+        if kind == "probe":
+            m.kind = 1
+        else:
+            m.kind = 0
+            from .monitors import omega_of
+            om = omega_of(mon).to(device=device, dtype=dtype).contiguous()
+            keep.append(om)
+            m.nfreq = om.numel()
+            m.omega = om.data_ptr()
+        m.acc = gc.data_ptr()
+    return arr, len(used), keep
+
+
+class _TMRun(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, eps, cfg, *state_in):
+        coeffs, plan, monitors, n_steps, ck = (cfg["coeffs"], cfg["plan"], cfg["monitors"],
+                                               cfg["n_steps"], cfg["ck"])
+        cdet = coeffs._replace(eps_inf=eps.detach())
+        work = {f: t.detach().clone().contiguous() for f, t in zip(FIELDS, state_in)}
+        K = ck if ck > 0 else DEFAULT_K
+        acc = [None] * len(monitors)
+        ckpts = []
+        step0 = 0
+        raw = None
+        while step0 < n_steps:
+            k = min(K, n_steps - step0)
+            ckpts.append((step0, k, {f: work[f].clone() for f in FIELDS}))
+            # one sequential sum per reference chunk: keep summing into the
+            # same buffers unless checkpoint_every starts a new chunk
+            reuse = raw if (ck <= 0 and raw is not None) else None
+            raw = engine2d.run_chunk("tm", work, cdet, step0, k, plan, monitors, reuse=reuse, raw=True)
+            if ck > 0 or step0 + k >= n_steps:
+                outs = engine2d.finish_outputs(raw, work)
+                for i, (mon, out) in enumerate(zip(monitors, outs)):
+                    if acc[i] is None:
+                        acc[i] = out
+                    elif mon.reduce == "sum":
+                        acc[i] = acc[i] + out
+                    else:
+                        acc[i] = torch.cat([acc[i], out])
+            elif any(kind == "probe" for _, kind, _ in raw):
+                raise NotImplementedError("PointProbe with gradients needs checkpoint_every > 0")
+            step0 += k
+        ctx.cfg = cfg
+        ctx.cdet = cdet
+        ctx.ckpts = ckpts
+        ctx.save_for_backward(eps)
+        ctx.n_state = len(state_in)
+        return (*[work[f] for f in FIELDS], *acc)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        (eps,) = ctx.saved_tensors
+        cfg, cdet = ctx.cfg, ctx.cdet
+        monitors, plan = cfg["monitors"], cfg["plan"]
+        nf = len(FIELDS)
+        gstate, gacc = grads[:nf], list(grads[nf:])
+        ref = ctx.ckpts[0][2]["Ez"]
+        dtype, device = ref.dtype, ref.device
+        nx, ny = ref.shape
+        adj = {f: (g.detach().to(dtype).clone().contiguous() if g is not None
+                   else torch.zeros_like(ctx.ckpts[0][2][f])) for f, g in zip(FIELDS, gstate)}
+        scratch = {"gHy": torch.empty(nx, ny, dtype=dtype, device=device),
+                   "gHx": torch.empty(nx, ny, dtype=dtype, device=device),
+                   "gEx": torch.empty(nx - 1, ny, dtype=dtype, device=device),
+                   "gEy": torch.empty(nx, ny - 1, dtype=dtype, device=device)}
+        eps_grad = torch.zeros(nx, ny, dtype=dtype, device=device)
+        # probe cotangents are per step: slice them per chunk
+        L = E.lib()
+        a = E.TM2DAdj()
+        for f in FIELDS:
+            setattr(a, "a" + f, adj[f].data_ptr())
+        for f, t in scratch.items():
+            setattr(a, f, t.data_ptr())
+        a.eps_grad = eps_grad.data_ptr()
+        d, d_keep = _descriptor(cdet, adj, dtype, device)
+        for step0, k, st in reversed(ctx.ckpts):
+            work = {f: t.clone() for f, t in st.items()}
+            tape = torch.empty(k, nx, ny, dtype=dtype, device=device)
+            engine2d.run_chunk("tm", work, cdet, step0, k, plan, [], tape=tape)
+            g_chunk = []
+            for mon, g in zip(monitors, gacc):
+                if g is not None and mon.reduce != "sum":
+                    g = g[step0:step0 + k]
+                g_chunk.append(g)
+            arr, n_used, keep = _adj_monitors(adj, monitors, g_chunk, dtype, device)
+            rc = L.mt_tm2d_adjoint(ctypes.byref(d), ctypes.byref(a), E.ptr(tape), arr, n_used,
+                                   step0, k, E.stream_handle())
+            E.check(rc, "mt_tm2d_adjoint")
+            del keep, tape
+        del d_keep
+        g_state = [adj[f] for f in FIELDS][:ctx.n_state]
+        return (eps_grad.to(eps.dtype), None, *g_state)
+
+
+def _descriptor(c, fields, dtype, device):
+    d = E.TM2D()
+    nx, ny = fields["Ez"].shape
+    d.nx, d.ny, d.dtype, d.y_pmc = nx, ny, E.dtype_code(dtype), int(bool(c.y_pmc))
+    d.dt, d.dx = float(c.dt), float(c.dx)
+    eps = c.eps_inf.to(device=device, dtype=dtype).contiguous()
+    d.eps = eps.data_ptr()
+    d.hy, d.hx, d.ex, d.ey = (engine2d._axis(c, n) for n in ("hy", "hx", "ex", "ey"))
+    return d, eps
 
 
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
-    raise NotImplementedError("gradient through run_tm/run_te: adjoint kernels not built yet")
+    if mode != "tm":
+        raise NotImplementedError("gradients through run_te are not supported (TM adjoint only)")
+    for name, t in coeffs._asdict().items():
+        if torch.is_tensor(t) and t.requires_grad and name != "eps_inf":
+            raise NotImplementedError(f"gradient with respect to coefficient '{name}' is not supported")
+    from .run import _validate
+    _validate("tm", state, coeffs)
+    eps = coeffs.eps_inf
+    E.require_device(eps)
+    plan = engine2d.source_plan(source, state.Dz.shape, state.Dz.device, state.Dz.dtype)
+    cfg = {"coeffs": coeffs, "plan": plan, "monitors": list(monitors), "n_steps": n_steps,
+           "ck": checkpoint_every}
+    outs = _TMRun.apply(eps, cfg, *[getattr(state, f) for f in FIELDS])
+    nf = len(FIELDS)
+    new_state = state._replace(**dict(zip(FIELDS, outs[:nf])))
+    return new_state, list(outs[nf:])

@@ -12,7 +12,7 @@ import torch
 from .. import _engine as E
 from .monitors import omega_of
 
-STAGE_STEPS = 32
+STAGE_STEPS = 64
 STAGE_BUDGET = 1 << 29  # bytes of staging per call
 
 
@@ -52,8 +52,9 @@ def source_struct(plan):
     return s, w
 
 
-def monitor_structs(fields, monitors, k, dtype, device):
-    """Engine descriptors for known monitor classes; output buffers per chunk."""
+def monitor_structs(fields, monitors, k, dtype, device, reuse=None):
+    """Engine descriptors for known monitor classes; output buffers per chunk
+    (``reuse``: accumulator buffers of a previous call to keep summing into)."""
     arr = (E.Monitor * max(1, len(monitors)))()
     outs, keep = [], []
     for i, mon in enumerate(monitors):
@@ -84,7 +85,11 @@ def monitor_structs(fields, monitors, k, dtype, device):
             keep.append(om)
             m.nfreq = om.numel()
             m.omega = om.data_ptr()
-            out = torch.zeros(m.nfreq, 2 * m.n_terms, m.n0 * m.n1, dtype=dtype, device=device)
+            if reuse is not None:
+                out = reuse[i][2]
+                assert out.is_contiguous() and out.numel() == m.nfreq * 2 * m.n_terms * m.n0 * m.n1
+            else:
+                out = torch.zeros(m.nfreq, 2 * m.n_terms, m.n0 * m.n1, dtype=dtype, device=device)
         m.acc = out.data_ptr()
         outs.append((mon, kind, out))
     return arr, outs, keep
@@ -115,8 +120,12 @@ def _axis(coeffs, name):
     return E.pml_axis(b, a, coeffs.shells[name] if coeffs.shells else None)
 
 
-def run_chunk(mode, fields, coeffs, step0, k, plan, monitors):
-    """Advance ``fields`` (dict of device tensors, reference shapes) k steps."""
+def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=None,
+              raw=False):
+    """Advance ``fields`` (dict of device tensors, reference shapes) k steps.
+    tape: optional (k, nx, ny) tensor receiving Ez after every step (TM).
+    reuse: raw outputs of a previous call whose accumulators keep summing.
+    raw: return the raw (monitor, kind, buffer) triples instead of shaped outputs."""
     L = E.lib()
     ref = fields["Ez"] if mode == "tm" else fields["Hz"]
     dtype, device = ref.dtype, ref.device
@@ -132,6 +141,9 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors):
         d.hy, d.hx, d.ex, d.ey = (_axis(coeffs, n) for n in ("hy", "hx", "ex", "ey"))
         for f in ("Ez", "Dz", "Hx", "Hy", "psi_Ezy", "psi_Ezx", "psi_Hyx", "psi_Hxy"):
             setattr(d, f, fields[f].data_ptr())
+        if tape is not None:
+            assert tape.is_contiguous() and tape.shape == (k, nx, ny) and tape.dtype == dtype
+            d.tape = tape.data_ptr()
         keep_c = [eps]
     else:
         nx, ny = fields["Ey"].shape[0], fields["Ex"].shape[1]
@@ -150,7 +162,7 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors):
             raise ValueError(f"state field {f} must be contiguous")
     src, wkeep = source_struct(plan)
     coef = coef_array(plan, coeffs.dt, step0, k) if src.present else None
-    arr, outs, keep = monitor_structs(fields, list(monitors), k, dtype, device)
+    arr, outs, keep = monitor_structs(fields, list(monitors), k, dtype, device, reuse)
     n = len(monitors)
     S, stage, nbytes = stage_alloc(L, arr, n, code, device)
     fn = L.mt_tm2d_run if mode == "tm" else L.mt_te2d_run
@@ -158,4 +170,4 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors):
             E.ptr(stage) if stage is not None else None, nbytes, S, step0, k, E.stream_handle())
     E.check(rc, f"mt_{mode}2d_run")
     del keep_c, wkeep, keep
-    return finish_outputs(outs, fields)
+    return outs if raw else finish_outputs(outs, fields)
