@@ -154,6 +154,10 @@ def main():
         run()
         torch.cuda.synchronize()
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    from meep_torch_amd import tune
+    if tune.tune.last:
+        log("[bench] autotuner ms per variant (-1 = two-pass):",
+            {k: round(v, 3) for k, v in tune.tune.last.items()}, "chosen:", dict(tune._CACHE))
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

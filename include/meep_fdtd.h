@@ -52,6 +52,13 @@ typedef struct {
   void *psi_Ezy, *psi_Ezx;               /* device (nx,ny-1) (nx-1,ny) */
   void *psi_Hyx, *psi_Hxy;               /* device (nx-2, ny_e), ny_e = ny-2 or ny (y_pmc) */
   void* tape;                            /* NULL, or device (k, nx, ny): Ez after each step of the call */
+  /* Temporally blocked path: variant >= 0 selects a tile shape / steps per
+   * launch (mt_tiled_variants); -1 runs two passes per step.  alt[] are
+   * second buffers of the same shapes as Ez Dz Hx Hy psi_Ezy psi_Ezx psi_Hyx
+   * psi_Hxy holding an identical copy on entry (ping-pong targets); results
+   * always end in the primary buffers. */
+  int variant, pad_;
+  void* alt[8];
 } mt_tm2d;
 
 /* 2D TE problem + state (coeffs.py:294-345 TECoeffs, stepper_te.py:11-24). */
@@ -63,6 +70,10 @@ typedef struct {
   void *Hz, *Ex, *Ey, *Dx, *Dy;          /* (nx-1,ny-1) (nx-1,ny) (nx,ny-1) (nx-1,ny) (nx,ny-1) */
   void *psi_Eyx, *psi_Exy;               /* (nx-1,ny-1) each */
   void *psi_Hzy, *psi_Hzx;               /* (nx-1,ny-2) (nx-2,ny-1) */
+  /* Tiled path as for mt_tm2d; alt[] shadows Hz Dx Dy psi_Eyx psi_Exy
+   * psi_Hzy psi_Hzx (alt[7] unused).  Ex/Ey are written at the end of the call. */
+  int variant, pad2_;
+  void* alt[8];
 } mt_te2d;
 
 /* Soft source on a 2D field: dense weights on the bounding box
@@ -114,7 +125,15 @@ typedef struct {
   const void* a;
   const void* b;                          /* NULL: no averaging */
   long long s0, s1;                       /* element strides */
-  int t_half, pad_;
+  int t_half;
+  /* Semantic form of the same view, used by the tiled 2D steppers (which
+   * sample from on-chip tiles, not from global arrays): field id (TM: 0 Ez,
+   * 1 Hx, 2 Hy, 3 Dz; TE: 0 Hz, 1 Ex, 2 Ey, 3 Dx, 4 Dy), line = 1 row
+   * (fixed first index `fix`, second index lo + e), 2 column (fixed second
+   * index, first index lo + e), 3 whole field (row-major), 4 single cell
+   * (fix, lo); averaged terms average positions fix-1 and fix.
+   * line = 0: no semantic form (the per-half-step steppers are used). */
+  int field, line, fix, lo, pad_;
 } mt_dft_term;
 
 typedef struct {
@@ -124,6 +143,12 @@ typedef struct {
   const void* omega;                      /* device, nfreq values of 2*pi*f in dtype */
   void* acc;                              /* device accumulators (DFT) or (k,) stack (probe) */
 } mt_monitor;
+
+/* Tile-shape variants of the temporally blocked 2D steppers for `dtype`:
+ * returns their number; for 0 <= v < count fills steps[v] (Yee steps per
+ * launch) and tx[v], ty[v] (owned tile rows/columns) when the arrays are
+ * non-NULL.  Used by the host-side autotuner. */
+int mt_tiled_variants(int dtype, int* steps, int* tx, int* ty);
 
 /* Bytes of staging workspace one call needs for `stage_steps` buffered steps. */
 long long mt_stage_bytes(const mt_monitor* mons, int n_mons, int stage_steps, int dtype);

@@ -15,6 +15,7 @@
 #include <string>
 
 #include "common.h"
+#include "tiled2d.h"
 
 namespace mt {
 
@@ -218,6 +219,41 @@ static int sample_t(const MonitorPlan& mp, int slot, int t_rel, hipStream_t s) {
   return check_launch("monitor gather");
 }
 
+bool build_taps(const MonitorPlan& mp, int slot0, int t_rel0, int esz, TapSet* out) {
+  out->n = 0;
+  char* base = static_cast<char*>(mp.stage);
+  long long off = 0;
+  for (int i = 0; i < mp.n_mons; ++i) {
+    const mt_monitor& m = mp.mons[i];
+    long long n = mon_elems(m);
+    for (int q = 0; q < m.n_terms; ++q) {
+      const mt_dft_term& tm = m.term[q];
+      if (tm.line < 1 || tm.line > 4 || out->n >= kMaxTaps) return false;
+      Tap& t = out->t[out->n++];
+      t.field = tm.field;
+      t.line = tm.line;
+      t.fix = tm.fix;
+      t.lo = tm.lo;
+      t.n = (int)n;
+      t.avg = tm.b != nullptr;
+      t.n0 = tm.line == 3 ? tm.fix : 0;  // whole field: fix = rows, lo = row length
+      t.n1 = tm.line == 3 ? tm.lo : 0;
+      if (m.kind == 1) {
+        t.out = m.acc;
+        t.slot_stride = 1;
+        t.slot0 = t_rel0;
+      } else {
+        t.out = base + off + (long long)q * n * esz;
+        t.slot_stride = (long long)m.n_terms * n;
+        t.slot0 = slot0;
+      }
+    }
+    if (m.kind == 0)
+      off += ((long long)mp.stage_steps * m.n_terms * n + (long long)m.nfreq * mp.stage_steps * 4) * esz;
+  }
+  return true;
+}
+
 int monitors_sample(const MonitorPlan& mp, int slot, int t_rel, hipStream_t s) {
   if (mp.n_mons == 0) return MT_OK;
   return mp.dtype == MT_F64 ? sample_t<double>(mp, slot, t_rel, s) : sample_t<float>(mp, slot, t_rel, s);
@@ -275,4 +311,4 @@ extern "C" long long mt_stage_bytes(const mt_monitor* mons, int n_mons, int stag
 
 extern "C" const char* mt_last_error(void) { return mt::g_err.c_str(); }
 
-extern "C" int mt_abi_version(void) { return 2; }
+extern "C" int mt_abi_version(void) { return 4; }

@@ -1,0 +1,600 @@
+// Temporally blocked 2D TM / TE steppers: NS Yee steps per launch on
+// overlapped LDS tiles.
+//
+// A workgroup owns a TX x TY block of the (i, j) cell lattice.  It loads the
+// block plus a halo (NS+1 cells on the low sides, NS on the high sides) of
+// the evolving fields into LDS, advances NS steps there, and writes back only
+// the cells it owns.  Per half-step the stencil reaches one cell (D/E halves
+// towards lower indices, H/B halves towards higher ones), so after NS steps
+// every owned cell, and the one row/column below it that averaged H monitor
+// samples need, is exact.  Halo cells are recomputed redundantly; cells whose
+// neighbours fall outside the tile compute throw-away values that never reach
+// an owned cell.  Reads come from one set of global buffers and writes go to
+// the other (ping-pong), so tiles never see each other's partial results.
+//
+// Arithmetic per cell is the per-half-step kernels' (yee2d.hip), op for op,
+// so results are bit-identical to them and to the reference's FP32 stepper.
+// Tiles whose halo touches a CPML shell also carry the four psi arrays in LDS
+// (template PML = true); interior tiles carry none and run in a separate
+// launch.  Monitor samples are taken inside the tile after each step by the
+// tile that owns the sampled cell and written to the staging ring consumed by
+// the DFT flush (dft.hip).
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "tiled2d.h"
+
+#include <algorithm>
+
+namespace mt {
+namespace {
+
+constexpr int BLK = 512;  // threads per tile workgroup
+
+template <typename T>
+__device__ __forceinline__ T dget(const T* p, long long idx, bool ok) {
+  return ok ? p[idx] : T(0);
+}
+
+// a / b for the constant grid step b with y = RN(1/b): q0 = a y, then one
+// FMA correction with the exact remainder a - q0 b gives the correctly
+// rounded quotient (Markstein's theorem; checked exhaustively against IEEE
+// division for |a| in [2^-100, 2^100] and several b).  Outside that range
+// the true division is used.
+template <typename T>
+__device__ __forceinline__ T div_c(T a, T b, T y) {
+  T aa = a < T(0) ? -a : a;
+  if (aa >= T(0x1p-100) && aa <= T(0x1p100)) {
+    T q0 = a * y;
+    T r = fma(-q0, b, a);
+    return fma(r, y, q0);
+  }
+  return a / b;
+}
+
+// Tile (tr, tc) of this workgroup: interior launches cover the rectangle
+// [r0, r1) x [c0, c1); PML launches enumerate the ring of tiles around it.
+__device__ __forceinline__ void tile_of(bool pml, const TileMap& m, int* tr, int* tc) {
+  if (!pml) {
+    *tr = m.r0 + blockIdx.y;
+    *tc = m.c0 + blockIdx.x;
+    return;
+  }
+  int b = blockIdx.x;
+  if (b < m.r0 * m.C) {
+    *tr = b / m.C;
+    *tc = b % m.C;
+    return;
+  }
+  b -= m.r0 * m.C;
+  if (b < (m.R - m.r1) * m.C) {
+    *tr = m.r1 + b / m.C;
+    *tc = b % m.C;
+    return;
+  }
+  b -= (m.R - m.r1) * m.C;
+  int per = m.c0 + (m.C - m.c1);
+  *tr = m.r0 + b / per;
+  int k = b % per;
+  *tc = k < m.c0 ? k : m.c1 + (k - m.c0);
+}
+
+// ---------------------------------------------------------------- taps
+// Write this tile's share of every monitor term after sub-step `s`.
+// FIELD(id, li, lj) returns the LDS value of field `id` at local cell.
+template <typename T, int RX, int RY, typename F>
+__device__ void write_taps(const TapSet& taps, int s, int I0, int J0, int TXo, int TYo, int a0, int b0,
+                           F field) {
+  const int tid = threadIdx.x;
+  for (int q = 0; q < taps.n; ++q) {
+    const Tap& t = taps.t[q];
+    T* out = static_cast<T*>(t.out) + (long long)(t.slot0 + s) * t.slot_stride;
+    if (t.line == 1) {  // row: fixed i = fix, j = lo + e
+      if (t.fix < I0 || t.fix >= I0 + TXo) continue;
+      int jlo = max(t.lo, J0), jhi = min(t.lo + t.n, J0 + TYo);
+      for (int j = jlo + tid; j < jhi; j += blockDim.x) {
+        int li = t.fix - a0, lj = j - b0;
+        T v = field(t.field, li, lj);
+        if (t.avg) v = T(0.5) * (field(t.field, li - 1, lj) + v);
+        out[j - t.lo] = v;
+      }
+    } else if (t.line == 2) {  // column: fixed j = fix, i = lo + e
+      if (t.fix < J0 || t.fix >= J0 + TYo) continue;
+      int ilo = max(t.lo, I0), ihi = min(t.lo + t.n, I0 + TXo);
+      for (int i = ilo + tid; i < ihi; i += blockDim.x) {
+        int li = i - a0, lj = t.fix - b0;
+        T v = field(t.field, li, lj);
+        if (t.avg) v = T(0.5) * (field(t.field, li, lj - 1) + v);
+        out[i - t.lo] = v;
+      }
+    } else if (t.line == 3) {  // whole field, row-major with row length t.n1
+      int ihi = min(I0 + TXo, t.n0), jhi = min(J0 + TYo, t.n1);
+      int w = jhi - J0;
+      if (I0 >= ihi || w <= 0) continue;
+      int cnt = (ihi - I0) * w;
+      for (int c = tid; c < cnt; c += blockDim.x) {
+        int i = I0 + c / w, j = J0 + c % w;
+        out[(long long)i * t.n1 + j] = field(t.field, i - a0, j - b0);
+      }
+    } else if (t.line == 4) {  // single cell (fix, lo); stacked per step
+      if (tid == 0 && t.fix >= I0 && t.fix < I0 + TXo && t.lo >= J0 && t.lo < J0 + TYo)
+        out[0] = field(t.field, t.fix - a0, t.lo - b0);
+    }
+  }
+}
+
+// ------------------------------------------------------------------- TE
+template <typename T, int TX, int TY, int NS, bool PML>
+__global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int nsub, int write_e) {
+  constexpr int RX = TX + 2 * NS + 1, RY = TY + 2 * NS + 1, RC = RX * RY;
+  constexpr int PER = (RC + BLK - 1) / BLK;
+  __shared__ T sHz[RC], sDx[RC], sDy[RC], sEx[RC], sEy[RC], sKx[RC], sKy[RC];
+  __shared__ T sP[PML ? 4 * RC : 1];
+  T* sPzy = sP;            // psi_Hzy: Dx update, y shell (index (i, j-1))
+  T* sPzx = sP + RC;       // psi_Hzx: Dy update, x shell (index (i-1, j))
+  T* sPyx = sP + 2 * RC;   // psi_Eyx: Hz update, x shell (index (i, j))
+  T* sPxy = sP + 3 * RC;   // psi_Exy: Hz update, y shell (index (i, j))
+  const int nx = p.nx, ny = p.ny;
+  int tr, tc;
+  tile_of(PML, p.map, &tr, &tc);
+  const int I0 = tr * TX, J0 = tc * TY;
+  const int a0 = I0 - NS - 1, b0 = J0 - NS - 1;
+  const int tid = threadIdx.x;
+  const T dt = p.dt, dx = p.dx, rdx = p.rdx;
+  {
+    T vHz[PER], vDx[PER], vDy[PER], vKx[PER], vKy[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      int c = tid + u * BLK;
+      int i = a0 + c / RY, j = b0 + c % RY;
+      bool in = c < RC;
+      bool hz = in && i >= 0 && i < nx - 1 && j >= 0 && j < ny - 1;
+      bool ex = in && i >= 0 && i < nx - 1 && j >= 0 && j < ny;
+      bool ey = in && i >= 0 && i < nx && j >= 0 && j < ny - 1;
+      vHz[u] = dget(p.Hz, (long long)i * (ny - 1) + j, hz);
+      vDx[u] = dget(p.Dx, (long long)i * ny + j, ex);
+      vDy[u] = dget(p.Dy, (long long)i * (ny - 1) + j, ey);
+      vKx[u] = ex ? p.eps_x[(long long)i * ny + j] : T(1);
+      vKy[u] = ey ? p.eps_y[(long long)i * (ny - 1) + j] : T(1);
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      int c = tid + u * BLK;
+      if (c < RC) {
+        sHz[c] = vHz[u];
+        sDx[c] = vDx[u];
+        sDy[c] = vDy[u];
+        sKx[c] = vKx[u];
+        sKy[c] = vKy[u];
+      }
+    }
+  }
+  if (PML) {
+    for (int c = tid; c < RC; c += BLK) {
+      int i = a0 + c / RY, j = b0 + c % RY;
+      bool hz = i >= 0 && i < nx - 1 && j >= 0 && j < ny - 1;
+      bool ex = i >= 0 && i < nx - 1 && j >= 0 && j < ny;
+      bool ey = i >= 0 && i < nx && j >= 0 && j < ny - 1;
+      sPzy[c] = dget(p.pHzy, (long long)i * (ny - 2) + (j - 1), ex && j >= 1 && j < ny - 1 && p.ey.in(j - 1));
+      sPzx[c] = dget(p.pHzx, (long long)(i - 1) * (ny - 1) + j, ey && i >= 1 && i < nx - 1 && p.ex.in(i - 1));
+      sPyx[c] = dget(p.pEyx, (long long)i * (ny - 1) + j, hz && p.hx.in(i));
+      sPxy[c] = dget(p.pExy, (long long)i * (ny - 1) + j, hz && p.hy.in(j));
+    }
+  }
+  __syncthreads();
+  for (int s = 0; s < nsub; ++s) {
+    const T coef = p.coef[s];
+    // D half (stepper_te.py:80-103,131-134,160-168)
+    int li = tid / RY, lj = tid % RY;
+#pragma unroll
+    for (int u = 0; u < PER; ++u, li += BLK / RY, lj += BLK % RY) {
+      if (lj >= RY) {
+        lj -= RY;
+        ++li;
+      }
+      int c = li * RY + lj;
+      if (c >= RC) break;
+      int i = a0 + li, j = b0 + lj;
+      if (!PML || (i >= 0 && i < nx - 1 && j >= 0 && j < ny)) {
+        T dxv = sDx[c];
+        if ((!PML || (j >= 1 && j < ny - 1)) && lj >= 1) {
+          T d = div_c(sHz[c] - sHz[c - 1], dx, rdx);
+          T curl = d;
+          if (PML && p.ey.in(j - 1)) {
+            T ps = p.ey.b[j - 1] * sPzy[c] + p.ey.a[j - 1] * d;
+            sPzy[c] = ps;
+            curl = d + ps;
+          }
+          dxv = dxv + dt * curl;
+        }
+        sDx[c] = dxv;
+        sEx[c] = dxv / sKx[c];
+      }
+      if (!PML || (i >= 0 && i < nx && j >= 0 && j < ny - 1)) {
+        T dyv = sDy[c];
+        if ((!PML || (i >= 1 && i < nx - 1)) && li >= 1) {
+          T d = div_c(sHz[c] - sHz[c - RY], dx, rdx);
+          T curl = d;
+          if (PML && p.ex.in(i - 1)) {
+            T ps = p.ex.b[i - 1] * sPzx[c] + p.ex.a[i - 1] * d;
+            sPzx[c] = ps;
+            curl = d + ps;
+          }
+          dyv = dyv - dt * curl;
+        }
+        if (p.src.present && i >= p.src.i0 && i < p.src.i1 && j >= p.src.j0 && j < p.src.j1)
+          dyv = dyv - coef * static_cast<const T*>(p.src.w)[(long long)(i - p.src.i0) * (p.src.j1 - p.src.j0) +
+                                                             (j - p.src.j0)];
+        sDy[c] = dyv;
+        sEy[c] = dyv / sKy[c];
+      }
+    }
+    __syncthreads();
+    // B half (stepper_te.py:106-117)
+    li = tid / RY;
+    lj = tid % RY;
+#pragma unroll
+    for (int u = 0; u < PER; ++u, li += BLK / RY, lj += BLK % RY) {
+      if (lj >= RY) {
+        lj -= RY;
+        ++li;
+      }
+      int c = li * RY + lj;
+      if (c >= RC) break;
+      int i = a0 + li, j = b0 + lj;
+      if ((!PML || (i >= 0 && i < nx - 1 && j >= 0 && j < ny - 1)) && li < RX - 1 && lj < RY - 1) {
+        T dEy = div_c(sEy[c + RY] - sEy[c], dx, rdx);
+        T dEx = div_c(sEx[c + 1] - sEx[c], dx, rdx);
+        T a1 = dEy;
+        if (PML && p.hx.in(i)) {
+          T ps = p.hx.b[i] * sPyx[c] + p.hx.a[i] * dEy;
+          sPyx[c] = ps;
+          a1 = dEy + ps;
+        }
+        T curl = a1 - dEx;
+        if (PML && p.hy.in(j)) {
+          T ps = p.hy.b[j] * sPxy[c] + p.hy.a[j] * dEx;
+          sPxy[c] = ps;
+          curl = curl - ps;
+        }
+        sHz[c] = sHz[c] - dt * curl;
+      }
+    }
+    __syncthreads();
+    if (taps.n) {
+      write_taps<T, RX, RY>(taps, s, I0, J0, TX, TY, a0, b0, [&](int f, int li, int lj) -> T {
+        int c = li * RY + lj;
+        return f == 0 ? sHz[c] : f == 1 ? sEx[c] : f == 2 ? sEy[c] : f == 3 ? sDx[c] : sDy[c];
+      });
+      __syncthreads();
+    }
+  }
+  // write back the owned block
+  for (int c = tid; c < TX * TY; c += BLK) {
+    int i = I0 + c / TY, j = J0 + c % TY;
+    int lc = (i - a0) * RY + (j - b0);
+    if (i < nx - 1 && j < ny - 1) {
+      long long h = (long long)i * (ny - 1) + j;
+      p.oHz[h] = sHz[lc];
+      if (PML) {
+        if (p.hx.in(i)) p.oEyx[h] = sPyx[lc];
+        if (p.hy.in(j)) p.oExy[h] = sPxy[lc];
+      }
+    }
+    if (i < nx - 1 && j < ny) {
+      long long h = (long long)i * ny + j;
+      p.oDx[h] = sDx[lc];
+      if (write_e) p.Ex[h] = sEx[lc];
+      if (PML && j >= 1 && j < ny - 1 && p.ey.in(j - 1)) p.oHzy[(long long)i * (ny - 2) + (j - 1)] = sPzy[lc];
+    }
+    if (i < nx && j < ny - 1) {
+      long long h = (long long)i * (ny - 1) + j;
+      p.oDy[h] = sDy[lc];
+      if (write_e) p.Ey[h] = sEy[lc];
+      if (PML && i >= 1 && i < nx - 1 && p.ex.in(i - 1)) p.oHzx[(long long)(i - 1) * (ny - 1) + j] = sPzx[lc];
+    }
+  }
+}
+
+// ------------------------------------------------------------------- TM
+template <typename T, int TX, int TY, int NS, bool PML>
+__global__ __launch_bounds__(BLK) void tm_tiled(TmTiled<T> p, TapSet taps, int nsub, int tape_s0) {
+  constexpr int RX = TX + 2 * NS + 1, RY = TY + 2 * NS + 1, RC = RX * RY;
+  constexpr int PER = (RC + BLK - 1) / BLK;
+  __shared__ T sEz[RC], sDz[RC], sHx[RC], sHy[RC], sK[RC];
+  __shared__ T sP[PML ? 4 * RC : 1];
+  T* sPzy = sP;            // psi_Ezy: Hx update, y shell (index (i, j))
+  T* sPzx = sP + RC;       // psi_Ezx: Hy update, x shell (index (i, j))
+  T* sPyx = sP + 2 * RC;   // psi_Hyx: Dz update, x shell (index (i-1, jj))
+  T* sPxy = sP + 3 * RC;   // psi_Hxy: Dz update, y shell (index (i-1, jj))
+  const int nx = p.nx, ny = p.ny, ny_e = p.ny_e;
+  int tr, tc;
+  tile_of(PML, p.map, &tr, &tc);
+  const int I0 = tr * TX, J0 = tc * TY;
+  const int a0 = I0 - NS - 1, b0 = J0 - NS - 1;
+  const int tid = threadIdx.x;
+  const T dt = p.dt, dx = p.dx, rdx = p.rdx;
+  const int joff = p.y_pmc ? 0 : 1;
+  {
+    T vEz[PER], vDz[PER], vHx[PER], vHy[PER], vK[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      int c = tid + u * BLK;
+      int i = a0 + c / RY, j = b0 + c % RY;
+      bool ez = c < RC && i >= 0 && i < nx && j >= 0 && j < ny;
+      vEz[u] = dget(p.Ez, (long long)i * ny + j, ez);
+      vDz[u] = dget(p.Dz, (long long)i * ny + j, ez);
+      vHx[u] = dget(p.Hx, (long long)i * (ny - 1) + j, ez && j < ny - 1);
+      vHy[u] = dget(p.Hy, (long long)i * ny + j, ez && i < nx - 1);
+      vK[u] = ez ? p.eps[(long long)i * ny + j] : T(1);
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      int c = tid + u * BLK;
+      if (c < RC) {
+        sEz[c] = vEz[u];
+        sDz[c] = vDz[u];
+        sHx[c] = vHx[u];
+        sHy[c] = vHy[u];
+        sK[c] = vK[u];
+      }
+    }
+  }
+  if (PML) {
+    for (int c = tid; c < RC; c += BLK) {
+      int i = a0 + c / RY, j = b0 + c % RY;
+      bool ez = i >= 0 && i < nx && j >= 0 && j < ny;
+      bool hx = ez && j < ny - 1;
+      bool hy = ez && i < nx - 1;
+      sPzy[c] = dget(p.pEzy, (long long)i * (ny - 1) + j, hx && p.hy.in(j));
+      sPzx[c] = dget(p.pEzx, (long long)i * ny + j, hy && p.hx.in(i));
+      int jj = j - joff;
+      bool live = ez && i >= 1 && i < nx - 1 && jj >= 0 && jj < ny_e;
+      long long q = (long long)(i - 1) * ny_e + jj;
+      sPyx[c] = dget(p.pHyx, q, live && p.ex.in(i - 1));
+      sPxy[c] = dget(p.pHxy, q, live && p.ey.in(jj));
+    }
+  }
+  __syncthreads();
+  for (int s = 0; s < nsub; ++s) {
+    const T coef = p.coef[s];
+    // B half (stepper_tm.py:85-92,126-128)
+    int li = tid / RY, lj = tid % RY;
+#pragma unroll
+    for (int u = 0; u < PER; ++u, li += BLK / RY, lj += BLK % RY) {
+      if (lj >= RY) {
+        lj -= RY;
+        ++li;
+      }
+      int c = li * RY + lj;
+      if (c >= RC) break;
+      int i = a0 + li, j = b0 + lj;
+      if (PML && (i < 0 || i >= nx || j < 0 || j >= ny)) continue;
+      T ez = sEz[c];
+      if ((!PML || j < ny - 1) && lj < RY - 1) {
+        T d = div_c(sEz[c + 1] - ez, dx, rdx);
+        T curl = d;
+        if (PML && p.hy.in(j)) {
+          T ps = p.hy.b[j] * sPzy[c] + p.hy.a[j] * d;
+          sPzy[c] = ps;
+          curl = d + ps;
+        }
+        sHx[c] = sHx[c] - dt * curl;
+      }
+      if ((!PML || i < nx - 1) && li < RX - 1) {
+        T d = div_c(sEz[c + RY] - ez, dx, rdx);
+        T curl = d;
+        if (PML && p.hx.in(i)) {
+          T ps = p.hx.b[i] * sPzx[c] + p.hx.a[i] * d;
+          sPzx[c] = ps;
+          curl = d + ps;
+        }
+        sHy[c] = sHy[c] + dt * curl;
+      }
+    }
+    __syncthreads();
+    // D half (stepper_tm.py:130-166,197-200,226-227)
+    li = tid / RY;
+    lj = tid % RY;
+#pragma unroll
+    for (int u = 0; u < PER; ++u, li += BLK / RY, lj += BLK % RY) {
+      if (lj >= RY) {
+        lj -= RY;
+        ++li;
+      }
+      int c = li * RY + lj;
+      if (c >= RC) break;
+      int i = a0 + li, j = b0 + lj;
+      if (PML && (i < 0 || i >= nx || j < 0 || j >= ny)) continue;
+      T dz = sDz[c];
+      bool live = !PML || (i >= 1 && i < nx - 1 && (p.y_pmc || (j >= 1 && j < ny - 1)));
+      if (live && li >= 1 && lj >= 1) {
+        T hx1 = (!PML || j < ny - 1) ? sHx[c] : T(0);
+        T hx0 = (!PML || j >= 1) ? sHx[c - 1] : T(0);
+        T dHx = div_c(hx1 - hx0, dx, rdx);
+        T dHy = div_c(sHy[c] - sHy[c - RY], dx, rdx);
+        int ii = i - 1, jj = j - joff;
+        T a1 = dHy;
+        if (PML && p.ex.in(ii)) {
+          T ps = p.ex.b[ii] * sPyx[c] + p.ex.a[ii] * dHy;
+          sPyx[c] = ps;
+          a1 = dHy + ps;
+        }
+        T curl = a1 - dHx;
+        if (PML && p.ey.in(jj)) {
+          T ps = p.ey.b[jj] * sPxy[c] + p.ey.a[jj] * dHx;
+          sPxy[c] = ps;
+          curl = curl - ps;
+        }
+        dz = dz + dt * curl;
+      }
+      if (p.src.present && i >= p.src.i0 && i < p.src.i1 && j >= p.src.j0 && j < p.src.j1)
+        dz = dz - coef * static_cast<const T*>(p.src.w)[(long long)(i - p.src.i0) * (p.src.j1 - p.src.j0) +
+                                                        (j - p.src.j0)];
+      sDz[c] = dz;
+      sEz[c] = dz / sK[c];
+    }
+    __syncthreads();
+    if (p.tape) {
+      for (int c = tid; c < TX * TY; c += BLK) {
+        int i = I0 + c / TY, j = J0 + c % TY;
+        if (i < nx && j < ny)
+          p.tape[(long long)(tape_s0 + s) * nx * ny + (long long)i * ny + j] = sEz[(i - a0) * RY + (j - b0)];
+      }
+    }
+    if (taps.n) {
+      write_taps<T, RX, RY>(taps, s, I0, J0, TX, TY, a0, b0, [&](int f, int li, int lj) -> T {
+        int c = li * RY + lj;
+        return f == 0 ? sEz[c] : f == 1 ? sHx[c] : f == 2 ? sHy[c] : sDz[c];
+      });
+      __syncthreads();
+    }
+  }
+  for (int c = tid; c < TX * TY; c += BLK) {
+    int i = I0 + c / TY, j = J0 + c % TY;
+    if (i >= nx || j >= ny) continue;
+    int lc = (i - a0) * RY + (j - b0);
+    long long e = (long long)i * ny + j;
+    p.oEz[e] = sEz[lc];
+    p.oDz[e] = sDz[lc];
+    if (j < ny - 1) {
+      long long h = (long long)i * (ny - 1) + j;
+      p.oHx[h] = sHx[lc];
+      if (PML && p.hy.in(j)) p.oEzy[h] = sPzy[lc];
+    }
+    if (i < nx - 1) {
+      p.oHy[e] = sHy[lc];
+      if (PML && p.hx.in(i)) p.oEzx[e] = sPzx[lc];
+    }
+    if (PML) {
+      int jj = j - joff;
+      if (i >= 1 && i < nx - 1 && jj >= 0 && jj < ny_e) {
+        long long q = (long long)(i - 1) * ny_e + jj;
+        if (p.ex.in(i - 1)) p.oHyx[q] = sPyx[lc];
+        if (p.ey.in(jj)) p.oHxy[q] = sPxy[lc];
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+// Interior tiles: the rectangle of tiles whose whole LDS region avoids every
+// CPML shell; PML tiles: the ring around it.
+template <int TX, int TY, int NS>
+TileMap make_map(int nx, int ny, int in_i0, int in_i1, int in_j0, int in_j1) {
+  TileMap m;
+  m.R = (nx + TX - 1) / TX;
+  m.C = (ny + TY - 1) / TY;
+  auto lo = [](int edge, int t, int ns) { return (edge + ns + 1 + t - 1) / t; };
+  auto hi = [](int edge, int t, int ns) { int v = edge - t - ns; return v < 0 ? 0 : v / t + 1; };
+  m.r0 = std::min(m.R, lo(in_i0, TX, NS));
+  m.r1 = std::max(m.r0, std::min(m.R, hi(in_i1, TX, NS)));
+  m.c0 = std::min(m.C, lo(in_j0, TY, NS));
+  m.c1 = std::max(m.c0, std::min(m.C, hi(in_j1, TY, NS)));
+  if (m.r1 == m.r0 || m.c1 == m.c0) m.r0 = m.r1 = m.c0 = m.c1 = 0;
+  return m;
+}
+
+template <typename P, typename K1, typename K2>
+int launch_pair(const TileMap& m, K1 kin, K2 kpml, hipStream_t s) {
+  int n_in = (m.r1 - m.r0) * (m.c1 - m.c0);
+  int n_ring = m.R * m.C - n_in;
+  if (n_in) kin(dim3(m.c1 - m.c0, m.r1 - m.r0));
+  if (n_ring) kpml(dim3(n_ring));
+  return MT_OK;
+}
+
+template <typename T, int TX, int TY, int NS>
+int launch_te_tiled(TeTiled<T> p, const TapSet& taps, int nsub, int write_e, hipStream_t s) {
+  p.map = make_map<TX, TY, NS>(p.nx, p.ny, p.in_i0, p.in_i1, p.in_j0, p.in_j1);
+  launch_pair<T>(
+      p.map,
+      [&](dim3 g) { hipLaunchKernelGGL((te_tiled<T, TX, TY, NS, false>), g, dim3(BLK), 0, s, p, taps, nsub, write_e); },
+      [&](dim3 g) { hipLaunchKernelGGL((te_tiled<T, TX, TY, NS, true>), g, dim3(BLK), 0, s, p, taps, nsub, write_e); },
+      s);
+  return check_launch("te2d tiled step");
+}
+
+template <typename T, int TX, int TY, int NS>
+int launch_tm_tiled(TmTiled<T> p, const TapSet& taps, int nsub, int tape_s0, hipStream_t s) {
+  p.map = make_map<TX, TY, NS>(p.nx, p.ny, p.in_i0, p.in_i1, p.in_j0, p.in_j1);
+  launch_pair<T>(
+      p.map,
+      [&](dim3 g) { hipLaunchKernelGGL((tm_tiled<T, TX, TY, NS, false>), g, dim3(BLK), 0, s, p, taps, nsub, tape_s0); },
+      [&](dim3 g) { hipLaunchKernelGGL((tm_tiled<T, TX, TY, NS, true>), g, dim3(BLK), 0, s, p, taps, nsub, tape_s0); },
+      s);
+  return check_launch("tm2d tiled step");
+}
+
+// Tile-shape variants the autotuner chooses from (index = variant id).
+// (TX, TY, NS): owned rows, owned columns, Yee steps per launch.  FP64 tiles
+// are smaller so LDS (fields + psi) fits one workgroup.
+#define MT_VARIANTS_F32(X, T) \
+  X(T, 16, 64, 1) X(T, 16, 64, 2) X(T, 16, 64, 4) X(T, 32, 64, 1) X(T, 32, 64, 2) X(T, 32, 64, 4) X(T, 8, 128, 2)
+#define MT_VARIANTS_F64(X, T) \
+  X(T, 8, 64, 1) X(T, 8, 64, 2) X(T, 8, 64, 4) X(T, 16, 64, 1) X(T, 16, 64, 2) X(T, 16, 32, 2) X(T, 8, 128, 1)
+static const int kVarF32[7][3] = {{16, 64, 1}, {16, 64, 2}, {16, 64, 4}, {32, 64, 1},
+                                  {32, 64, 2}, {32, 64, 4}, {8, 128, 2}};
+static const int kVarF64[7][3] = {{8, 64, 1}, {8, 64, 2}, {8, 64, 4}, {16, 64, 1},
+                                  {16, 64, 2}, {16, 32, 2}, {8, 128, 1}};
+
+int te_tiled_dispatch_f32(int variant, const TeTiled<float>& p, const TapSet& taps, int nsub, int write_e,
+                          hipStream_t s) {
+  int v = 0;
+#define X(T, A, B, C) \
+  if (variant == v++) return launch_te_tiled<T, A, B, C>(p, taps, nsub, write_e, s);
+  MT_VARIANTS_F32(X, float)
+#undef X
+  return fail(MT_EINVAL, "unknown tiled variant");
+}
+
+int te_tiled_dispatch_f64(int variant, const TeTiled<double>& p, const TapSet& taps, int nsub, int write_e,
+                          hipStream_t s) {
+  int v = 0;
+#define X(T, A, B, C) \
+  if (variant == v++) return launch_te_tiled<T, A, B, C>(p, taps, nsub, write_e, s);
+  MT_VARIANTS_F64(X, double)
+#undef X
+  return fail(MT_EINVAL, "unknown tiled variant");
+}
+
+int tm_tiled_dispatch_f32(int variant, const TmTiled<float>& p, const TapSet& taps, int nsub, int tape_s0,
+                          hipStream_t s) {
+  int v = 0;
+#define X(T, A, B, C) \
+  if (variant == v++) return launch_tm_tiled<T, A, B, C>(p, taps, nsub, tape_s0, s);
+  MT_VARIANTS_F32(X, float)
+#undef X
+  return fail(MT_EINVAL, "unknown tiled variant");
+}
+
+int tm_tiled_dispatch_f64(int variant, const TmTiled<double>& p, const TapSet& taps, int nsub, int tape_s0,
+                          hipStream_t s) {
+  int v = 0;
+#define X(T, A, B, C) \
+  if (variant == v++) return launch_tm_tiled<T, A, B, C>(p, taps, nsub, tape_s0, s);
+  MT_VARIANTS_F64(X, double)
+#undef X
+  return fail(MT_EINVAL, "unknown tiled variant");
+}
+
+int tiled_variant_count() { return 7; }
+
+int tiled_variant_steps(int variant, int dtype) {
+  if (variant < 0 || variant >= 7) return 0;
+  return dtype == MT_F64 ? kVarF64[variant][2] : kVarF32[variant][2];
+}
+
+}  // namespace mt
+
+extern "C" int mt_tiled_variants(int dtype, int* steps, int* tx, int* ty) {
+  const int(*tab)[3] = dtype == MT_F64 ? mt::kVarF64 : mt::kVarF32;
+  for (int v = 0; v < 7; ++v) {
+    if (steps) steps[v] = tab[v][2];
+    if (tx) tx[v] = tab[v][0];
+    if (ty) ty[v] = tab[v][1];
+  }
+  return 7;
+}

@@ -10,9 +10,11 @@
 // contiguous); threads run along y for coalesced access.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 
 #include "common.h"
+#include "tiled2d.h"
 
 namespace mt {
 namespace {
@@ -289,6 +291,164 @@ int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const
   return MT_OK;
 }
 
+template <typename T, typename P>
+int tm_run_tiled(const mt_tm2d* d, const mt_source2d* src, const double* coef, const MonitorPlan& mp,
+                 int step0, int k, hipStream_t s) {
+  TmTiled<T> p{};
+  p.nx = d->nx;
+  p.ny = d->ny;
+  p.y_pmc = d->y_pmc;
+  p.ny_e = d->y_pmc ? d->ny : d->ny - 2;
+  p.dt = static_cast<T>(d->dt);
+  p.dx = static_cast<T>(d->dx);
+  p.rdx = T(1) / p.dx;
+  p.eps = static_cast<const T*>(d->eps);
+  p.hy = pml_of<T>(d->hy);
+  p.hx = pml_of<T>(d->hx);
+  p.ex = pml_of<T>(d->ex);
+  p.ey = pml_of<T>(d->ey);
+  const int joff = d->y_pmc ? 0 : 1;
+  p.in_i0 = std::max(d->hx.lo_end, d->ex.lo_end + 1) + 1;
+  p.in_i1 = std::min(d->hx.hi_begin, d->ex.hi_begin + 1) - 1;
+  p.in_j0 = std::max(d->hy.lo_end, d->ey.lo_end + joff) + 1;
+  p.in_j1 = std::min(d->hy.hi_begin, d->ey.hi_begin + joff) - 1;
+  T* A[8] = {static_cast<T*>(d->Ez), static_cast<T*>(d->Dz), static_cast<T*>(d->Hx), static_cast<T*>(d->Hy),
+             static_cast<T*>(d->psi_Ezy), static_cast<T*>(d->psi_Ezx), static_cast<T*>(d->psi_Hyx),
+             static_cast<T*>(d->psi_Hxy)};
+  T* B[8];
+  for (int q = 0; q < 8; ++q) B[q] = static_cast<T*>(d->alt[q]);
+  size_t nbytes[8] = {(size_t)d->nx * d->ny, (size_t)d->nx * d->ny, (size_t)d->nx * (d->ny - 1),
+                      (size_t)(d->nx - 1) * d->ny, (size_t)d->nx * (d->ny - 1), (size_t)(d->nx - 1) * d->ny,
+                      (size_t)(d->nx - 2) * p.ny_e, (size_t)(d->nx - 2) * p.ny_e};
+  p.tape = static_cast<T*>(d->tape);
+  if (src && src->present) p.src = *src;
+  const int NS = tiled_variant_steps(d->variant, d->dtype);
+  const int S = mp.n_mons ? mp.stage_steps : k;
+  int done = 0, slot = 0, first = step0, cur = 0, rc;
+  while (done < k) {
+    int nsub = std::min(NS, k - done);
+    if (mp.n_mons) nsub = std::min(nsub, S - slot);
+    T** in = cur ? B : A;
+    T** out = cur ? A : B;
+    p.Ez = in[0], p.Dz = in[1], p.Hx = in[2], p.Hy = in[3];
+    p.pEzy = in[4], p.pEzx = in[5], p.pHyx = in[6], p.pHxy = in[7];
+    p.oEz = out[0], p.oDz = out[1], p.oHx = out[2], p.oHy = out[3];
+    p.oEzy = out[4], p.oEzx = out[5], p.oHyx = out[6], p.oHxy = out[7];
+    for (int u = 0; u < kMaxSub; ++u) p.coef[u] = (p.src.present && u < nsub) ? static_cast<T>(coef[done + u]) : T(0);
+    TapSet taps;
+    taps.n = 0;
+    if (mp.n_mons && !build_taps(mp, slot, done, sizeof(T), &taps)) return fail(MT_EINVAL, "tap build failed");
+    rc = P::tm(d->variant, p, taps, nsub, done, s);
+    if (rc) return rc;
+    cur ^= 1;
+    done += nsub;
+    slot += nsub;
+    if (mp.n_mons && (slot == S || done == k)) {
+      if ((rc = monitors_flush(mp, slot, first, d->dt, s))) return rc;
+      first += slot;
+      slot = 0;
+    }
+  }
+  if (cur)
+    for (int q = 0; q < 8; ++q)
+      if (hipMemcpyAsync(A[q], B[q], nbytes[q] * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return fail(MT_EHIP, "tiled TM: copy back failed");
+  return MT_OK;
+}
+
+template <typename T, typename P>
+int te_run_tiled(const mt_te2d* d, const mt_source2d* src, const double* coef, const MonitorPlan& mp,
+                 int step0, int k, hipStream_t s) {
+  TeTiled<T> p{};
+  p.nx = d->nx;
+  p.ny = d->ny;
+  p.dt = static_cast<T>(d->dt);
+  p.dx = static_cast<T>(d->dx);
+  p.rdx = T(1) / p.dx;
+  p.eps_x = static_cast<const T*>(d->eps_x);
+  p.eps_y = static_cast<const T*>(d->eps_y);
+  p.hx = pml_of<T>(d->hx);
+  p.hy = pml_of<T>(d->hy);
+  p.ey = pml_of<T>(d->ey);
+  p.ex = pml_of<T>(d->ex);
+  p.in_i0 = std::max(d->hx.lo_end, d->ex.lo_end + 1) + 1;
+  p.in_i1 = std::min(d->hx.hi_begin, d->ex.hi_begin + 1) - 1;
+  p.in_j0 = std::max(d->hy.lo_end, d->ey.lo_end + 1) + 1;
+  p.in_j1 = std::min(d->hy.hi_begin, d->ey.hi_begin + 1) - 1;
+  T* A[7] = {static_cast<T*>(d->Hz), static_cast<T*>(d->Dx), static_cast<T*>(d->Dy),
+             static_cast<T*>(d->psi_Eyx), static_cast<T*>(d->psi_Exy), static_cast<T*>(d->psi_Hzy),
+             static_cast<T*>(d->psi_Hzx)};
+  T* B[7];
+  for (int q = 0; q < 7; ++q) B[q] = static_cast<T*>(d->alt[q]);
+  const size_t nx = d->nx, ny = d->ny;
+  size_t nel[7] = {(nx - 1) * (ny - 1), (nx - 1) * ny, nx * (ny - 1), (nx - 1) * (ny - 1), (nx - 1) * (ny - 1),
+                   (nx - 1) * (ny - 2), (nx - 2) * (ny - 1)};
+  p.Ex = static_cast<T*>(d->Ex);
+  p.Ey = static_cast<T*>(d->Ey);
+  if (src && src->present) p.src = *src;
+  const int NS = tiled_variant_steps(d->variant, d->dtype);
+  const int S = mp.n_mons ? mp.stage_steps : k;
+  int done = 0, slot = 0, first = step0, cur = 0, rc;
+  while (done < k) {
+    int nsub = std::min(NS, k - done);
+    if (mp.n_mons) nsub = std::min(nsub, S - slot);
+    T** in = cur ? B : A;
+    T** out = cur ? A : B;
+    p.Hz = in[0], p.Dx = in[1], p.Dy = in[2], p.pEyx = in[3], p.pExy = in[4], p.pHzy = in[5], p.pHzx = in[6];
+    p.oHz = out[0], p.oDx = out[1], p.oDy = out[2], p.oEyx = out[3], p.oExy = out[4], p.oHzy = out[5],
+    p.oHzx = out[6];
+    for (int u = 0; u < kMaxSub; ++u) p.coef[u] = (p.src.present && u < nsub) ? static_cast<T>(coef[done + u]) : T(0);
+    TapSet taps;
+    taps.n = 0;
+    if (mp.n_mons && !build_taps(mp, slot, done, sizeof(T), &taps)) return fail(MT_EINVAL, "tap build failed");
+    rc = P::te(d->variant, p, taps, nsub, done + nsub == k ? 1 : 0, s);
+    if (rc) return rc;
+    cur ^= 1;
+    done += nsub;
+    slot += nsub;
+    if (mp.n_mons && (slot == S || done == k)) {
+      if ((rc = monitors_flush(mp, slot, first, d->dt, s))) return rc;
+      first += slot;
+      slot = 0;
+    }
+  }
+  if (cur)
+    for (int q = 0; q < 7; ++q)
+      if (hipMemcpyAsync(A[q], B[q], nel[q] * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return fail(MT_EHIP, "tiled TE: copy back failed");
+  return MT_OK;
+}
+
+struct DispF32 {
+  static int tm(int v, const TmTiled<float>& p, const TapSet& t, int n, int s0, hipStream_t s) {
+    return tm_tiled_dispatch_f32(v, p, t, n, s0, s);
+  }
+  static int te(int v, const TeTiled<float>& p, const TapSet& t, int n, int we, hipStream_t s) {
+    return te_tiled_dispatch_f32(v, p, t, n, we, s);
+  }
+};
+struct DispF64 {
+  static int tm(int v, const TmTiled<double>& p, const TapSet& t, int n, int s0, hipStream_t s) {
+    return tm_tiled_dispatch_f64(v, p, t, n, s0, s);
+  }
+  static int te(int v, const TeTiled<double>& p, const TapSet& t, int n, int we, hipStream_t s) {
+    return te_tiled_dispatch_f64(v, p, t, n, we, s);
+  }
+};
+
+bool taps_ok(const MonitorPlan& mp) {
+  for (int i = 0; i < mp.n_mons; ++i)
+    for (int q = 0; q < mp.mons[i].n_terms; ++q)
+      if (mp.mons[i].term[q].line == 0) return false;
+  return true;
+}
+
+bool alt_ok(void* const* alt, int n) {
+  for (int q = 0; q < n; ++q)
+    if (!alt[q]) return false;
+  return true;
+}
+
 }  // namespace
 }  // namespace mt
 
@@ -314,6 +474,13 @@ extern "C" int mt_tm2d_run(const mt_tm2d* p, const mt_source2d* src, const doubl
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p->variant >= 0) {
+    if (p->variant >= tiled_variant_count()) return fail(MT_EINVAL, "unknown tiled variant");
+    if (!alt_ok(p->alt, 8)) return fail(MT_EINVAL, "tiled path needs the alt buffers");
+    if (taps_ok(mp))
+      return p->dtype == MT_F64 ? tm_run_tiled<double, DispF64>(p, src, coef, mp, step0, k, s)
+                                : tm_run_tiled<float, DispF32>(p, src, coef, mp, step0, k, s);
+  }
   return p->dtype == MT_F64 ? tm_run_t<double>(p, src, coef, mp, step0, k, s)
                             : tm_run_t<float>(p, src, coef, mp, step0, k, s);
 }
@@ -337,6 +504,13 @@ extern "C" int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const doubl
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p->variant >= 0) {
+    if (p->variant >= tiled_variant_count()) return fail(MT_EINVAL, "unknown tiled variant");
+    if (!alt_ok(p->alt, 7)) return fail(MT_EINVAL, "tiled path needs the alt buffers");
+    if (taps_ok(mp))
+      return p->dtype == MT_F64 ? te_run_tiled<double, DispF64>(p, src, coef, mp, step0, k, s)
+                                : te_run_tiled<float, DispF32>(p, src, coef, mp, step0, k, s);
+  }
   return p->dtype == MT_F64 ? te_run_t<double>(p, src, coef, mp, step0, k, s)
                             : te_run_t<float>(p, src, coef, mp, step0, k, s);
 }

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 2
+ABI_VERSION = 4
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -30,7 +30,8 @@ class TM2D(ctypes.Structure):
                 ("hy", PmlAxis), ("hx", PmlAxis), ("ex", PmlAxis), ("ey", PmlAxis),
                 ("Ez", c_void_p), ("Dz", c_void_p), ("Hx", c_void_p), ("Hy", c_void_p),
                 ("psi_Ezy", c_void_p), ("psi_Ezx", c_void_p),
-                ("psi_Hyx", c_void_p), ("psi_Hxy", c_void_p), ("tape", c_void_p)]
+                ("psi_Hyx", c_void_p), ("psi_Hxy", c_void_p), ("tape", c_void_p),
+                ("variant", c_int), ("pad_", c_int), ("alt", c_void_p * 8)]
 
 
 class TM2DAdj(ctypes.Structure):
@@ -46,7 +47,8 @@ class TE2D(ctypes.Structure):
                 ("hx", PmlAxis), ("hy", PmlAxis), ("ey", PmlAxis), ("ex", PmlAxis),
                 ("Hz", c_void_p), ("Ex", c_void_p), ("Ey", c_void_p), ("Dx", c_void_p),
                 ("Dy", c_void_p), ("psi_Eyx", c_void_p), ("psi_Exy", c_void_p),
-                ("psi_Hzy", c_void_p), ("psi_Hzx", c_void_p)]
+                ("psi_Hzy", c_void_p), ("psi_Hzx", c_void_p),
+                ("variant", c_int), ("pad2_", c_int), ("alt", c_void_p * 8)]
 
 
 class Source2D(ctypes.Structure):
@@ -70,7 +72,7 @@ class Source3D(ctypes.Structure):
 
 class DftTerm(ctypes.Structure):
     _fields_ = [("a", c_void_p), ("b", c_void_p), ("s0", c_ll), ("s1", c_ll), ("t_half", c_int),
-                ("pad_", c_int)]
+                ("field", c_int), ("line", c_int), ("fix", c_int), ("lo", c_int), ("pad_", c_int)]
 
 
 class Monitor(ctypes.Structure):
@@ -104,6 +106,8 @@ def lib():
     L.mt_te2d_run.argtypes = [P(TE2D), P(Source2D), P(c_double)] + common
     L.mt_yee3d_run.restype = c_int
     L.mt_yee3d_run.argtypes = [P(Yee3D), P(Source3D), P(c_double)] + common
+    L.mt_tiled_variants.restype = c_int
+    L.mt_tiled_variants.argtypes = [c_int, P(c_int), P(c_int), P(c_int)]
     L.mt_tm2d_adjoint.restype = c_int
     L.mt_tm2d_adjoint.argtypes = [P(TM2D), P(TM2DAdj), c_void_p, P(Monitor), c_int, c_int, c_int,
                                   c_void_p]

@@ -36,7 +36,8 @@ def _adj_monitors(adj_fields, monitors, gaccs, dtype, device):
         kind, terms = mon.engine_terms(adj_fields)
         m = arr[i]
         m.n_terms = len(terms)
-        for q, (a, b, half) in enumerate(terms):
+        for q, term in enumerate(terms):
+            a, b, half = term[:3]
             pa, s0, s1, n0, n1 = E.view2d(a)
             if q == 0:
                 m.n0, m.n1 = n0, n1

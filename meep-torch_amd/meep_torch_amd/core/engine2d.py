@@ -66,7 +66,9 @@ def monitor_structs(fields, monitors, k, dtype, device, reuse=None):
         if len(terms) > E.MAX_TERMS:
             raise ValueError("too many terms in one monitor")
         m.n_terms = len(terms)
-        for q, (a, b, half) in enumerate(terms):
+        for q, term in enumerate(terms):
+            a, b, half = term[:3]
+            sem = term[3] if len(term) > 3 else None
             pa, s0, s1, n0, n1 = E.view2d(a)
             if q == 0:
                 m.n0, m.n1 = n0, n1
@@ -76,6 +78,8 @@ def monitor_structs(fields, monitors, k, dtype, device, reuse=None):
             m.term[q].b = E.view2d(b)[0] if b is not None else None
             m.term[q].s0, m.term[q].s1 = s0, s1
             m.term[q].t_half = int(half)
+            if sem is not None:
+                m.term[q].field, m.term[q].line, m.term[q].fix, m.term[q].lo = sem
         if kind == "probe":
             m.kind = 1
             out = torch.zeros(k, dtype=dtype, device=device)
@@ -120,8 +124,12 @@ def _axis(coeffs, name):
     return E.pml_axis(b, a, coeffs.shells[name] if coeffs.shells else None)
 
 
+TM_ORDER = ("Ez", "Dz", "Hx", "Hy", "psi_Ezy", "psi_Ezx", "psi_Hyx", "psi_Hxy")
+TE_ORDER = ("Hz", "Dx", "Dy", "psi_Eyx", "psi_Exy", "psi_Hzy", "psi_Hzx")
+
+
 def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=None,
-              raw=False):
+              raw=False, variant=None, alt=None):
     """Advance ``fields`` (dict of device tensors, reference shapes) k steps.
     tape: optional (k, nx, ny) tensor receiving Ez after every step (TM).
     reuse: raw outputs of a previous call whose accumulators keep summing.
@@ -160,6 +168,16 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
     for f, t in fields.items():
         if not t.is_contiguous():
             raise ValueError(f"state field {f} must be contiguous")
+    if variant is None:
+        from .. import tune
+        variant = tune.variant_for(mode, fields, coeffs, plan, monitors, k)
+    d.variant = int(variant)
+    if variant >= 0:
+        order = TM_ORDER if mode == "tm" else TE_ORDER
+        if alt is None:
+            alt = {f: fields[f].clone() for f in order}
+        for q, f in enumerate(order):
+            d.alt[q] = alt[f].data_ptr()
     src, wkeep = source_struct(plan)
     coef = coef_array(plan, coeffs.dt, step0, k) if src.present else None
     arr, outs, keep = monitor_structs(fields, list(monitors), k, dtype, device, reuse)

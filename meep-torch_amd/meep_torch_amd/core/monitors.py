@@ -21,6 +21,14 @@ def _line(dim: int, lo: int, hi: int):
     return r.start, len(r)
 
 
+TM_IDS = {"Ez": 0, "Hx": 1, "Hy": 2, "Dz": 3}
+TE_IDS = {"Hz": 0, "Ex": 1, "Ey": 2, "Dx": 3, "Dy": 4}
+
+
+def _ids(f):
+    return TM_IDS if "Ez" in f else TE_IDS
+
+
 class PointProbe:
     reduce = "stack"
 
@@ -28,7 +36,9 @@ class PointProbe:
         self.ix, self.iy = ix, iy
 
     def engine_terms(self, f):
-        return "probe", [(f["Ez"][self.ix, self.iy].reshape(1, 1), None, 0)]
+        nx, ny = f["Ez"].shape
+        sem = (0, 4, self.ix % nx, self.iy % ny)
+        return "probe", [(f["Ez"][self.ix, self.iy].reshape(1, 1), None, 0, sem)]
 
 
 class DFTFieldsMonitor:
@@ -44,7 +54,9 @@ class DFTFieldsMonitor:
     def engine_terms(self, f):
         F = f[self.component]
         half = 0 if self.component.startswith("E") else 1
-        return "dft", [(F.reshape(1, -1), None, half)]
+        fid = _ids(f).get(self.component)
+        sem = (fid, 3, F.shape[0], F.shape[1]) if fid is not None else None
+        return "dft", [(F.reshape(1, -1), None, half, sem)]
 
     def out_shape(self, f):
         return (self.freqs.numel(), 2) + tuple(f[self.component].shape)
@@ -66,13 +78,16 @@ class FluxMonitorTM:
 
     def engine_terms(self, f):
         i, lo, hi = self.idx, self.lo, self.hi
+        nx, ny = f["Ez"].shape
         if self.x_normal:
             E = f["Ez"][i, lo:hi]
             Ha, Hb = f["Hy"][i - 1, lo:hi], f["Hy"][i, lo:hi]
-        else:
-            E = f["Ez"][lo:hi, i]
-            Ha, Hb = f["Hx"][lo:hi, i - 1], f["Hx"][lo:hi, i]
-        return "dft", [(E, None, 0), (Ha, Hb, 1)]
+            l0 = _line(ny, lo, hi)[0]
+            return "dft", [(E, None, 0, (0, 1, i, l0)), (Ha, Hb, 1, (2, 1, i, l0))]
+        E = f["Ez"][lo:hi, i]
+        Ha, Hb = f["Hx"][lo:hi, i - 1], f["Hx"][lo:hi, i]
+        l0 = _line(nx, lo, hi)[0]
+        return "dft", [(E, None, 0, (0, 2, i, l0)), (Ha, Hb, 1, (1, 2, i, l0))]
 
     def out_shape(self, f):
         n = (f["Ez"].shape[1] if self.x_normal else f["Ez"].shape[0])
@@ -96,8 +111,9 @@ class FluxMonitorTE:
 
     def engine_terms(self, f):
         i, lo, hi = self.ix, self.lo, self.hi
-        return "dft", [(f["Ey"][i, lo:hi], None, 1),
-                       (f["Hz"][i - 1, lo:hi], f["Hz"][i, lo:hi], 0)]
+        l0 = _line(f["Ey"].shape[1], lo, hi)[0]
+        return "dft", [(f["Ey"][i, lo:hi], None, 1, (2, 1, i, l0)),
+                       (f["Hz"][i - 1, lo:hi], f["Hz"][i, lo:hi], 0, (0, 1, i, l0))]
 
     def out_shape(self, f):
         return (self.freqs.numel(), 4, _line(f["Ey"].shape[1], self.lo, self.hi)[1])
@@ -119,8 +135,9 @@ class FluxMonitorTEY:
 
     def engine_terms(self, f):
         j, lo, hi = self.iy, self.lo, self.hi
-        return "dft", [(f["Ex"][lo:hi, j], None, 1),
-                       (f["Hz"][lo:hi, j - 1], f["Hz"][lo:hi, j], 0)]
+        l0 = _line(f["Ex"].shape[0], lo, hi)[0]
+        return "dft", [(f["Ex"][lo:hi, j], None, 1, (1, 2, j, l0)),
+                       (f["Hz"][lo:hi, j - 1], f["Hz"][lo:hi, j], 0, (0, 2, j, l0))]
 
     def out_shape(self, f):
         return (self.freqs.numel(), 4, _line(f["Ex"].shape[0], self.lo, self.hi)[1])
