@@ -102,6 +102,54 @@ def build_c1(steps_override=None):
     return run, 256 * 256, steps, None
 
 
+def build_c4(n_var=64, steps=4000, ck=200, seed=4):
+    """C4: n_var TM design variants, N ~ U{512..1536} (splitmix), 128x128
+    density (U[0,1)) -> conic filter (r = 4 cells) -> tanh projection (beta 8)
+    -> eps 2.1..12.1 painted at the centre of an eps-2.1 cell, 24-cell CPML,
+    line source at x = npml + 16, FOM = sum of a 20-frequency output flux line.
+    One bench step = one design iteration: batched forward + batched adjoint."""
+    from meep_torch_amd import synth
+    from meep_torch_amd.adjoint.filters import conic_filter, tanh_projection
+    from meep_torch_amd.batch import BatchedTM
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTM
+    from meep_torch_amd.core.sources import GaussianPulse, LineSource
+    u = synth.splitmix_uniform(n_var, seed)
+    sizes = [512 + int(x * 1025) for x in u]
+    npml, fr = 24, [0.4 + 0.2 * i / 19 for i in range(20)]
+    probs, bases, mons = [], [], []
+    for v, n in enumerate(sizes):
+        g = Grid(nx=n, ny=n, dx=0.05, dtype=torch.float32, device="cuda")
+        base = torch.full((n, n), 2.1, dtype=torch.float32, device="cuda")
+        m = build_tm_coeffs(g, eps_inf=base, npml=npml)
+        src = LineSource(g, ix=npml + 16, j0=npml, j1=n - npml, profile=GaussianPulse(0.5, 0.2))
+        mon = FluxMonitorTM(g, fr, x_normal=True, idx=n - npml - 16, lo=npml, hi=n - npml)
+        probs.append((m, src, [mon]))
+        bases.append(base)
+        mons.append(mon)
+    rho = [torch.tensor(synth.splitmix_uniform(128 * 128, 100 + v).reshape(128, 128),
+                        dtype=torch.float32, device="cuda") for v in range(n_var)]
+    bt = BatchedTM(probs, steps, checkpoint_every=ck)
+    cells = sum(n * n for n in sizes)
+
+    def eps_of(v, r):
+        n = sizes[v]
+        e = bases[v].clone()
+        o = (n - 128) // 2
+        e[o:o + 128, o:o + 128] = 2.1 + 10.0 * tanh_projection(conic_filter(r, 0.2, 0.05), 8.0, 0.5)
+        return e
+
+    def iteration():
+        rs = [r.clone().requires_grad_(True) for r in rho]
+        out = bt.apply([eps_of(v, r) for v, r in enumerate(rs)])
+        fom = sum(mons[v].flux(o[0], 0.05).sum() for v, o in enumerate(out))
+        fom.backward()
+        return fom, rs
+
+    return iteration, cells, steps, sizes
+
+
 def cpu_baseline_c2(budget_s=15.0):
     """The oracle (reference algorithm restated, torch CPU) on a bounded sample
     of C2: full 2048x2048 grid with its 4 flux lines, as many Yee steps as fit
@@ -139,7 +187,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--c4-variants", type=int, default=64)
     ap.add_argument("--yee-steps", type=int, default=0, help="override Yee steps per run (tuning)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -147,6 +196,8 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the FDTD path shards by spatial slabs across GPUs, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c4":
+        return bench_c4(args)
     builder = {"c1": build_c1, "c2": build_c2, "c3": build_c3}[args.workload]
     run, cells, ysteps, _ = builder(args.yee_steps or None)
     log(f"[bench] {args.workload}: {cells} cells x {ysteps} Yee steps per bench step")
@@ -193,6 +244,32 @@ def main():
         log("[bench] cpu baseline (oracle) ...")
         line["cpu_baseline"] = cpu_baseline_c2()
     print(json.dumps(line), flush=True)
+
+
+def bench_c4(args):
+    steps = args.yee_steps or 4000
+    it, cells, ysteps, sizes = build_c4(args.c4_variants, steps)
+    log(f"[bench] c4: {len(sizes)} variants, {cells} cells, {ysteps} steps, sizes {min(sizes)}..{max(sizes)}")
+    for i in range(args.warmup):
+        it()
+        torch.cuda.synchronize()
+        log(f"[bench] c4 warmup {i + 1} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        fom, rs = it()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    upd = cells * ysteps * 3 * args.steps   # forward + recompute + adjoint sweeps
+    print(json.dumps({
+        "metric": "batched design iterations/s (forward + adjoint, all variants)",
+        "value": args.steps / wall, "unit": "design-iterations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"C4 batched TM fwd+adjoint, {len(sizes)} variants N in "
+                               f"[{min(sizes)},{max(sizes)}], {ysteps} steps, ckpt 200",
+                   "cells_per_variant_sum": cells},
+        "sweep_cell_updates_per_s": upd / wall, "fom": float(fom)}), flush=True)
 
 
 if __name__ == "__main__":

@@ -198,6 +198,49 @@ typedef struct {
 int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const void* tape,
                     const mt_monitor* mons, int n_mons, int step0, int k, void* stream);
 
+/* ---- Batched TM design variants (uneven grid sizes, one persistent launch) ----
+ * Each item is a full 2D TM problem with its own state (primary buffers plus
+ * alt[] ping-pong buffers), source, per-step coefficient table on the device
+ * (coef_tab[t] = dt*amp(t) for every global step t of the run, dtype of the
+ * problem) and monitors with their staging workspace.  All items share the
+ * dtype.  The plan copies per-variant descriptors into `dev_ws` (device
+ * memory of mt_tm2d_batch_workspace bytes) once; afterwards every call only
+ * passes scalars, so the launches can be captured in a graph and replayed. */
+typedef struct {
+  mt_tm2d prob;
+  mt_source2d src;
+  const void* coef_tab;
+  const mt_monitor* mons;
+  int n_mons, pad_;
+  void* stage;
+  long long stage_bytes;
+} mt_tm2d_batch_item;
+
+typedef struct mt_batch mt_batch;       /* opaque host-side plan */
+
+long long mt_tm2d_batch_workspace(int nvar, int dtype);
+
+/* Build the plan (tile `variant` of mt_tiled_variants; stage_steps = DFT
+ * staging ring length shared by all variants).  Returns NULL on error. */
+mt_batch* mt_tm2d_batch_create(const mt_tm2d_batch_item* items, int nvar, int variant, int stage_steps,
+                               void* dev_ws, long long ws_bytes, void* stream);
+
+/* Advance every variant k steps from global step step0 in one persistent
+ * launch pair per block of steps (interior tiles, CPML-ring tiles); monitor
+ * spectra are folded per variant; states end in the primary buffers.  With
+ * tapes set in the items, Ez after each step s of the call goes to tape[s].
+ * Replaces a Python loop of run_tm calls over variants
+ * (examples_2/y_splitter/c_y_splitter.py:154-200 design pattern). */
+int mt_tm2d_batch_run(mt_batch* b, int step0, int k, void* stream);
+
+/* Reverse-time adjoint of k steps for every variant at once (the batched form
+ * of mt_tm2d_adjoint): adj[v], tape[v] and the monitor cotangent descriptors
+ * of variant v (mons + mon_off[v] .. mons + mon_off[v+1]). */
+int mt_tm2d_batch_adjoint(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes, const mt_monitor* mons,
+                          const int* mon_off, int step0, int k, void* stream);
+
+void mt_batch_destroy(mt_batch* b);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* mt_last_error(void);
 

@@ -1,0 +1,166 @@
+// Device bodies of the 2D TM adjoint passes (shared by the single-problem
+// entry point in adjoint2d.hip and the batched one in batch2d.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace mt {
+
+constexpr int ABX = 64, ABY = 4;
+
+template <typename T>
+struct AdjK {
+  int nx, ny, y_pmc, ny_e;
+  T dt, dx;
+  const T* eps;
+  Pml<T> hy, hx, ex, ey;
+  T *aEz, *aDz, *aHx, *aHy, *apzy, *apzx, *apyx, *apxy;
+  T *gHy, *gHx, *gEx, *gEy;
+  T* eps_grad;
+};
+
+// A: Ez' cotangent -> eps gradient and Dz cotangent; curl cotangent -> D-half
+// psi cotangents and the H-difference cotangents gHy, gHx.
+template <typename T>
+__device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, int i, int j) {
+  const int nx = p.nx, ny = p.ny;
+  long long c = (long long)i * ny + j;
+  T eps = p.eps[c];
+  T ezb = p.aEz[c];
+  p.eps_grad[c] = p.eps_grad[c] + (-ezb * ez_after[c]) / eps;
+  T dzb = p.aDz[c] + ezb / eps;
+  p.aDz[c] = dzb;
+  p.aEz[c] = T(0);
+  bool live = i >= 1 && i < nx - 1 && (p.y_pmc || (j >= 1 && j < ny - 1));
+  T ghy = T(0), ghx = T(0);
+  if (live) {
+    T cb = p.dt * dzb;
+    int ii = i - 1, jj = p.y_pmc ? j : j - 1;
+    long long q = (long long)ii * p.ny_e + jj;
+    ghy = cb;
+    if (p.ex.in(ii)) {
+      T pt = p.apyx[q] + cb;
+      ghy = cb + p.ex.a[ii] * pt;
+      p.apyx[q] = p.ex.b[ii] * pt;
+    }
+    ghx = -cb;
+    if (p.ey.in(jj)) {
+      T pt = p.apxy[q] - cb;
+      ghx = -cb + p.ey.a[jj] * pt;
+      p.apxy[q] = p.ey.b[jj] * pt;
+    }
+  }
+  p.gHy[c] = ghy;
+  p.gHx[c] = ghx;
+}
+
+// B: Hy/Hx cotangents (transposed differences of gHy/gHx), B-half psi
+// cotangents, E-difference cotangents gEx, gEy.
+template <typename T>
+__device__ __forceinline__ void adj_b_cell(const AdjK<T>& p, int i, int j) {
+  const int nx = p.nx, ny = p.ny;
+  if (i < nx - 1) {
+    long long h = (long long)i * ny + j;
+    T hb = p.aHy[h] + (p.gHy[h] - p.gHy[h + ny]) / p.dx;
+    p.aHy[h] = hb;
+    T gb = p.dt * hb;
+    if (p.hx.in(i)) {
+      T pt = p.apzx[h] + p.dt * hb;
+      gb = gb + p.hx.a[i] * pt;
+      p.apzx[h] = p.hx.b[i] * pt;
+    }
+    p.gEx[h] = gb;
+  }
+  if (j < ny - 1) {
+    long long h = (long long)i * (ny - 1) + j;
+    long long c = (long long)i * ny + j;
+    T hb = p.aHx[h] + (p.gHx[c] - p.gHx[c + 1]) / p.dx;
+    p.aHx[h] = hb;
+    T gb = -p.dt * hb;
+    if (p.hy.in(j)) {
+      T pt = p.apzy[h] - p.dt * hb;
+      gb = gb + p.hy.a[j] * pt;
+      p.apzy[h] = p.hy.b[j] * pt;
+    }
+    p.gEy[h] = gb;
+  }
+}
+
+// C: Ez cotangent of x_t = transposed differences of gEx, gEy.
+template <typename T>
+__device__ __forceinline__ void adj_c_cell(const AdjK<T>& p, int i, int j) {
+  const int nx = p.nx, ny = p.ny;
+  T gxm = i >= 1 ? p.gEx[(long long)(i - 1) * ny + j] : T(0);
+  T gx0 = i < nx - 1 ? p.gEx[(long long)i * ny + j] : T(0);
+  T gym = j >= 1 ? p.gEy[(long long)i * (ny - 1) + j - 1] : T(0);
+  T gy0 = j < ny - 1 ? p.gEy[(long long)i * (ny - 1) + j] : T(0);
+  p.aEz[(long long)i * ny + j] = (gxm - gx0) / p.dx + (gym - gy0) / p.dx;
+}
+
+// Monitor cotangent of element e of term q at global step t.
+template <typename T>
+__device__ __forceinline__ void adj_inject_elem(const mt_monitor& m, int q, int e, int t, double dt_d, int t_rel) {
+  int n = m.n0 * m.n1;
+  const mt_dft_term& tm = m.term[q];
+  int i0 = e / m.n1, i1 = e - i0 * m.n1;
+  long long off = (long long)i0 * tm.s0 + (long long)i1 * tm.s1;
+  T vb;
+  if (m.kind == 1) {
+    vb = static_cast<const T*>(m.acc)[t_rel];
+  } else {
+    T dt = static_cast<T>(dt_d);
+    double tt = ((double)t + (tm.t_half ? 0.5 : 1.0)) * dt_d;
+    T tq = static_cast<T>(tt);
+    const T* om = static_cast<const T*>(m.omega);
+    const T* ac = static_cast<const T*>(m.acc);
+    vb = T(0);
+    for (int f = 0; f < m.nfreq; ++f) {
+      T ph = om[f] * tq;
+      T re = ac[((long long)f * 2 * m.n_terms + 2 * q) * n + e];
+      T im = ac[((long long)f * 2 * m.n_terms + 2 * q + 1) * n + e];
+      vb = vb + (dt * cos(ph)) * re + (-(dt * sin(ph))) * im;
+    }
+  }
+  T* a = const_cast<T*>(static_cast<const T*>(tm.a));
+  if (tm.b) {
+    T* b = const_cast<T*>(static_cast<const T*>(tm.b));
+    T h = T(0.5) * vb;
+    a[off] = a[off] + h;
+    b[off] = b[off] + h;
+  } else {
+    a[off] = a[off] + vb;
+  }
+}
+
+template <typename T>
+AdjK<T> make_adjk(const mt_tm2d* d, const mt_tm2d_adj* a) {
+  AdjK<T> p;
+  p.nx = d->nx;
+  p.ny = d->ny;
+  p.y_pmc = d->y_pmc;
+  p.ny_e = d->y_pmc ? d->ny : d->ny - 2;
+  p.dt = static_cast<T>(d->dt);
+  p.dx = static_cast<T>(d->dx);
+  p.eps = static_cast<const T*>(d->eps);
+  p.hy = pml_of<T>(d->hy);
+  p.hx = pml_of<T>(d->hx);
+  p.ex = pml_of<T>(d->ex);
+  p.ey = pml_of<T>(d->ey);
+  p.aEz = static_cast<T*>(a->aEz);
+  p.aDz = static_cast<T*>(a->aDz);
+  p.aHx = static_cast<T*>(a->aHx);
+  p.aHy = static_cast<T*>(a->aHy);
+  p.apzy = static_cast<T*>(a->apsi_Ezy);
+  p.apzx = static_cast<T*>(a->apsi_Ezx);
+  p.apyx = static_cast<T*>(a->apsi_Hyx);
+  p.apxy = static_cast<T*>(a->apsi_Hxy);
+  p.gHy = static_cast<T*>(a->gHy);
+  p.gHx = static_cast<T*>(a->gHx);
+  p.gEx = static_cast<T*>(a->gEx);
+  p.gEy = static_cast<T*>(a->gEy);
+  p.eps_grad = static_cast<T*>(a->eps_grad);
+  return p;
+}
+
+}  // namespace mt

@@ -1,0 +1,384 @@
+// Batched 2D TM design variants of uneven size: one persistent launch pair
+// (interior tiles, CPML-ring tiles) advances every variant a block of steps;
+// the batched adjoint runs each reverse pass over the cells of all variants
+// in one launch.  Per-variant descriptors live in caller-provided device
+// memory, so the forward launches depend only on scalars (graph-capturable).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "adjoint2d.h"
+#include "common.h"
+#include "tiled2d.h"
+
+namespace {
+constexpr int kMaxMonPerVar = 8;
+constexpr int kCellBlk = 256;
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct Layout {
+  size_t vars, pre_in, pre_ring, adjk, blk_pre, inj, inj_pre, tapes, dts, total;
+};
+
+template <typename T>
+Layout layout(int nvar) {
+  Layout L{};
+  size_t o = 0;
+  L.vars = o;
+  o = align256(o + nvar * sizeof(mt::TmBatchVar<T>));
+  L.pre_in = o;
+  o = align256(o + (nvar + 1) * sizeof(int));
+  L.pre_ring = o;
+  o = align256(o + (nvar + 1) * sizeof(int));
+  L.adjk = o;
+  o = align256(o + nvar * sizeof(mt::AdjK<T>));
+  L.blk_pre = o;
+  o = align256(o + (nvar + 1) * sizeof(int));
+  L.inj = o;
+  o = align256(o + (size_t)kMaxMonPerVar * nvar * sizeof(mt_monitor));
+  L.inj_pre = o;
+  o = align256(o + (size_t)kMaxMonPerVar * (nvar + 1) * sizeof(int));
+  L.tapes = o;
+  o = align256(o + nvar * sizeof(void*));
+  L.dts = o;
+  o = align256(o + nvar * sizeof(double));
+  L.total = o;
+  return L;
+}
+}  // namespace
+
+struct mt_batch {
+  int nvar, dtype, variant, S, NS;
+  int n_in, n_ring, grid_in, grid_ring, cell_blocks, max_mons;
+  char* ws;
+  Layout L;
+  std::vector<mt_tm2d> probs;
+  std::vector<std::vector<mt_monitor>> mons;
+  std::vector<void*> stage;
+  std::vector<long long> stage_bytes;
+};
+
+namespace mt {
+namespace {
+
+template <typename T, int PH>
+__global__ __launch_bounds__(kCellBlk) void adj_batch_cells(const AdjK<T>* ks, void* const* tapes,
+                                                            const int* blk_pre, int nvar, int s_rel) {
+  int b = blockIdx.x;
+  int lo = 0, hi = nvar;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (blk_pre[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const AdjK<T>& p = ks[lo];
+  long long cells = (long long)p.nx * p.ny;
+  long long cell = (long long)(b - blk_pre[lo]) * kCellBlk + threadIdx.x;
+  if (cell >= cells) return;
+  int i = (int)(cell / p.ny), j = (int)(cell % p.ny);
+  if (PH == 0) adj_a_cell(p, static_cast<const T*>(tapes[lo]) + (long long)s_rel * cells, i, j);
+  else if (PH == 1) adj_b_cell(p, i, j);
+  else adj_c_cell(p, i, j);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kCellBlk) void adj_batch_inject(const mt_monitor* inj, const int* pre, int nvar,
+                                                             int t, const double* dts, int t_rel) {
+  int b = blockIdx.x;
+  int lo = 0, hi = nvar;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (pre[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const mt_monitor& m = inj[lo];
+  int e = (b - pre[lo]) * kCellBlk + threadIdx.x;
+  if (m.n_terms == 0 || e >= m.n0 * m.n1) return;
+  for (int q = 0; q < m.n_terms; ++q) adj_inject_elem<T>(m, q, e, t, dts[lo], t_rel);
+}
+
+template <typename T>
+TmTiled<T> tiled_of(const mt_tm2d* d, const mt_source2d* src, const void* coef_tab, int variant, bool swap) {
+  TmTiled<T> p{};
+  p.nx = d->nx;
+  p.ny = d->ny;
+  p.y_pmc = d->y_pmc;
+  p.ny_e = d->y_pmc ? d->ny : d->ny - 2;
+  p.dt = static_cast<T>(d->dt);
+  p.dx = static_cast<T>(d->dx);
+  p.rdx = T(1) / p.dx;
+  p.eps = static_cast<const T*>(d->eps);
+  p.hy = pml_of<T>(d->hy);
+  p.hx = pml_of<T>(d->hx);
+  p.ex = pml_of<T>(d->ex);
+  p.ey = pml_of<T>(d->ey);
+  const int joff = d->y_pmc ? 0 : 1;
+  p.in_i0 = std::max(d->hx.lo_end, d->ex.lo_end + 1) + 1;
+  p.in_i1 = std::min(d->hx.hi_begin, d->ex.hi_begin + 1) - 1;
+  p.in_j0 = std::max(d->hy.lo_end, d->ey.lo_end + joff) + 1;
+  p.in_j1 = std::min(d->hy.hi_begin, d->ey.hi_begin + joff) - 1;
+  p.map = tiled_map(variant, d->dtype, d->nx, d->ny, p.in_i0, p.in_i1, p.in_j0, p.in_j1);
+  T* A[8] = {static_cast<T*>(d->Ez), static_cast<T*>(d->Dz), static_cast<T*>(d->Hx), static_cast<T*>(d->Hy),
+             static_cast<T*>(d->psi_Ezy), static_cast<T*>(d->psi_Ezx), static_cast<T*>(d->psi_Hyx),
+             static_cast<T*>(d->psi_Hxy)};
+  T* B[8];
+  for (int q = 0; q < 8; ++q) B[q] = static_cast<T*>(d->alt[q]);
+  T** in = swap ? B : A;
+  T** out = swap ? A : B;
+  p.Ez = in[0], p.Dz = in[1], p.Hx = in[2], p.Hy = in[3];
+  p.pEzy = in[4], p.pEzx = in[5], p.pHyx = in[6], p.pHxy = in[7];
+  p.oEz = out[0], p.oDz = out[1], p.oHx = out[2], p.oHy = out[3];
+  p.oEzy = out[4], p.oEzx = out[5], p.oHyx = out[6], p.oHxy = out[7];
+  p.tape = static_cast<T*>(d->tape);
+  if (src && src->present) p.src = *src;
+  p.coef_tab = static_cast<const T*>(coef_tab);
+  return p;
+}
+
+template <typename T>
+int create_t(mt_batch* b, const mt_tm2d_batch_item* items, hipStream_t s) {
+  const int nvar = b->nvar;
+  std::vector<TmBatchVar<T>> vars(nvar);
+  std::vector<int> pre_in(nvar + 1, 0), pre_ring(nvar + 1, 0), blk(nvar + 1, 0);
+  std::vector<double> dts(nvar);
+  for (int v = 0; v < nvar; ++v) {
+    const mt_tm2d_batch_item& it = items[v];
+    vars[v].pA = tiled_of<T>(&it.prob, &it.src, it.coef_tab, b->variant, false);
+    vars[v].pB = tiled_of<T>(&it.prob, &it.src, it.coef_tab, b->variant, true);
+    MonitorPlan mp{it.mons, it.n_mons, it.stage, it.stage_bytes, b->S, b->dtype};
+    vars[v].taps.n = 0;
+    if (it.n_mons && !build_taps(mp, 0, 0, sizeof(T), &vars[v].taps))
+      return fail(MT_EINVAL, "batched run: every monitor term needs its semantic (tile) form");
+    const TileMap& m = vars[v].pA.map;
+    int n_in = (m.r1 - m.r0) * (m.c1 - m.c0);
+    pre_in[v + 1] = pre_in[v] + n_in;
+    pre_ring[v + 1] = pre_ring[v] + (m.R * m.C - n_in);
+    blk[v + 1] = blk[v] + (int)(((long long)it.prob.nx * it.prob.ny + kCellBlk - 1) / kCellBlk);
+    dts[v] = it.prob.dt;
+  }
+  b->n_in = pre_in[nvar];
+  b->n_ring = pre_ring[nvar];
+  b->cell_blocks = blk[nvar];
+  char* ws = b->ws;
+  if (hipMemcpyAsync(ws + b->L.vars, vars.data(), nvar * sizeof(TmBatchVar<T>), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(ws + b->L.pre_in, pre_in.data(), (nvar + 1) * sizeof(int), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(ws + b->L.pre_ring, pre_ring.data(), (nvar + 1) * sizeof(int), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(ws + b->L.blk_pre, blk.data(), (nvar + 1) * sizeof(int), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(ws + b->L.dts, dts.data(), nvar * sizeof(double), hipMemcpyHostToDevice, s) ||
+      hipStreamSynchronize(s))
+    return fail(MT_EHIP, "batched run: descriptor upload failed");
+  return MT_OK;
+}
+
+template <typename T>
+int run_t(mt_batch* b, int step0, int k, hipStream_t s) {
+  const int S = b->max_mons ? b->S : k;
+  int done = 0, slot = 0, first = step0, cur = 0, rc;
+  const TmBatchVar<T>* vars = reinterpret_cast<const TmBatchVar<T>*>(b->ws + b->L.vars);
+  const int* pin = reinterpret_cast<const int*>(b->ws + b->L.pre_in);
+  const int* pring = reinterpret_cast<const int*>(b->ws + b->L.pre_ring);
+  while (done < k) {
+    int nsub = std::min(b->NS, k - done);
+    if (b->max_mons) nsub = std::min(nsub, S - slot);
+    rc = (b->dtype == MT_F64)
+             ? tm_batch_dispatch_f64(b->variant, reinterpret_cast<const TmBatchVar<double>*>(vars), pin, pring,
+                                     b->nvar, b->n_in, b->n_ring, b->grid_in, b->grid_ring, nsub, cur,
+                                     step0 + done, slot, done, s)
+             : tm_batch_dispatch_f32(b->variant, reinterpret_cast<const TmBatchVar<float>*>(vars), pin, pring,
+                                     b->nvar, b->n_in, b->n_ring, b->grid_in, b->grid_ring, nsub, cur,
+                                     step0 + done, slot, done, s);
+    if (rc) return rc;
+    cur ^= 1;
+    done += nsub;
+    slot += nsub;
+    if (b->max_mons && (slot == S || done == k)) {
+      for (int v = 0; v < b->nvar; ++v) {
+        if (b->mons[v].empty()) continue;
+        MonitorPlan mp{b->mons[v].data(), (int)b->mons[v].size(), b->stage[v], b->stage_bytes[v], b->S, b->dtype};
+        if ((rc = monitors_flush(mp, slot, first, b->probs[v].dt, s))) return rc;
+      }
+      first += slot;
+      slot = 0;
+    }
+  }
+  if (cur) {
+    for (int v = 0; v < b->nvar; ++v) {
+      const mt_tm2d& d = b->probs[v];
+      size_t nx = d.nx, ny = d.ny, nye = d.y_pmc ? d.ny : d.ny - 2;
+      size_t n[8] = {nx * ny, nx * ny, nx * (ny - 1), (nx - 1) * ny, nx * (ny - 1), (nx - 1) * ny,
+                     (nx - 2) * nye, (nx - 2) * nye};
+      void* A[8] = {d.Ez, d.Dz, d.Hx, d.Hy, d.psi_Ezy, d.psi_Ezx, d.psi_Hyx, d.psi_Hxy};
+      for (int q = 0; q < 8; ++q)
+        if (hipMemcpyAsync(A[q], d.alt[q], n[q] * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess)
+          return fail(MT_EHIP, "batched run: copy back failed");
+    }
+  }
+  return MT_OK;
+}
+
+template <typename T>
+int adjoint_t(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes, const mt_monitor* mons, const int* mon_off,
+              int step0, int k, hipStream_t s) {
+  const int nvar = b->nvar;
+  std::vector<AdjK<T>> ks(nvar);
+  for (int v = 0; v < nvar; ++v) ks[v] = make_adjk<T>(&b->probs[v], &adj[v]);
+  int max_m = 0;
+  for (int v = 0; v < nvar; ++v) max_m = std::max(max_m, mon_off[v + 1] - mon_off[v]);
+  if (max_m > kMaxMonPerVar) return fail(MT_EINVAL, "batched adjoint: too many monitors per variant");
+  std::vector<mt_monitor> inj((size_t)kMaxMonPerVar * nvar);
+  std::vector<int> ipre((size_t)kMaxMonPerVar * (nvar + 1), 0);
+  for (int m = 0; m < max_m; ++m) {
+    for (int v = 0; v < nvar; ++v) {
+      mt_monitor& dst = inj[(size_t)m * nvar + v];
+      dst = mt_monitor{};
+      int blocks = 0;
+      if (m < mon_off[v + 1] - mon_off[v]) {
+        dst = mons[mon_off[v] + m];
+        blocks = (dst.n0 * dst.n1 + kCellBlk - 1) / kCellBlk;
+      }
+      ipre[(size_t)m * (nvar + 1) + v + 1] = ipre[(size_t)m * (nvar + 1) + v] + blocks;
+    }
+  }
+  char* ws = b->ws;
+  if (hipMemcpyAsync(ws + b->L.adjk, ks.data(), nvar * sizeof(AdjK<T>), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(ws + b->L.tapes, tapes, nvar * sizeof(void*), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(ws + b->L.inj, inj.data(), inj.size() * sizeof(mt_monitor), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(ws + b->L.inj_pre, ipre.data(), ipre.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
+      hipStreamSynchronize(s))
+    return fail(MT_EHIP, "batched adjoint: descriptor upload failed");
+  const AdjK<T>* dks = reinterpret_cast<const AdjK<T>*>(ws + b->L.adjk);
+  void* const* dtp = reinterpret_cast<void* const*>(ws + b->L.tapes);
+  const int* dblk = reinterpret_cast<const int*>(ws + b->L.blk_pre);
+  const mt_monitor* dinj = reinterpret_cast<const mt_monitor*>(ws + b->L.inj);
+  const int* dipre = reinterpret_cast<const int*>(ws + b->L.inj_pre);
+  const double* ddts = reinterpret_cast<const double*>(ws + b->L.dts);
+  for (int s_ = k - 1; s_ >= 0; --s_) {
+    int t = step0 + s_;
+    for (int m = 0; m < max_m; ++m) {
+      int nb = ipre[(size_t)m * (nvar + 1) + nvar];
+      if (nb)
+        hipLaunchKernelGGL(adj_batch_inject<T>, dim3(nb), dim3(kCellBlk), 0, s, dinj + (size_t)m * nvar,
+                           dipre + (size_t)m * (nvar + 1), nvar, t, ddts, s_);
+    }
+    hipLaunchKernelGGL((adj_batch_cells<T, 0>), dim3(b->cell_blocks), dim3(kCellBlk), 0, s, dks, dtp, dblk, nvar, s_);
+    hipLaunchKernelGGL((adj_batch_cells<T, 1>), dim3(b->cell_blocks), dim3(kCellBlk), 0, s, dks, dtp, dblk, nvar, s_);
+    hipLaunchKernelGGL((adj_batch_cells<T, 2>), dim3(b->cell_blocks), dim3(kCellBlk), 0, s, dks, dtp, dblk, nvar, s_);
+    int rc = check_launch("batched tm2d adjoint step");
+    if (rc) return rc;
+  }
+  return MT_OK;
+}
+
+}  // namespace
+}  // namespace mt
+
+using namespace mt;
+
+extern "C" long long mt_tm2d_batch_workspace(int nvar, int dtype) {
+  if (nvar < 1) return 0;
+  return dtype == MT_F64 ? (long long)layout<double>(nvar).total : (long long)layout<float>(nvar).total;
+}
+
+extern "C" mt_batch* mt_tm2d_batch_create(const mt_tm2d_batch_item* items, int nvar, int variant, int stage_steps,
+                                          void* dev_ws, long long ws_bytes, void* stream) {
+  if (!items || nvar < 1) {
+    fail(MT_EINVAL, "batched run: no variants");
+    return nullptr;
+  }
+  int dtype = items[0].prob.dtype;
+  if (variant < 0 || variant >= tiled_variant_count()) {
+    fail(MT_EINVAL, "batched run: invalid tile variant");
+    return nullptr;
+  }
+  if (ws_bytes < mt_tm2d_batch_workspace(nvar, dtype) || !dev_ws) {
+    fail(MT_EINVAL, "batched run: workspace too small");
+    return nullptr;
+  }
+  int max_mons = 0;
+  for (int v = 0; v < nvar; ++v) {
+    const mt_tm2d& d = items[v].prob;
+    if (d.dtype != dtype) {
+      fail(MT_EINVAL, "batched run: all variants must share the dtype");
+      return nullptr;
+    }
+    if (d.nx < 3 || d.ny < 3 || !d.eps || !d.Ez || !d.Dz || !d.Hx || !d.Hy || !d.psi_Ezy || !d.psi_Ezx ||
+        !d.psi_Hyx || !d.psi_Hxy) {
+      fail(MT_EINVAL, "batched run: incomplete problem");
+      return nullptr;
+    }
+    for (int q = 0; q < 8; ++q)
+      if (!d.alt[q]) {
+        fail(MT_EINVAL, "batched run: missing ping-pong buffers");
+        return nullptr;
+      }
+    if (check_axis(d.hy, d.ny - 1, "hy") || check_axis(d.hx, d.nx - 1, "hx") || check_axis(d.ex, d.nx - 2, "ex") ||
+        check_axis(d.ey, d.y_pmc ? d.ny : d.ny - 2, "ey"))
+      return nullptr;
+    const mt_source2d& sr = items[v].src;
+    if (sr.present && (!items[v].coef_tab || sr.i0 < 0 || sr.j0 < 0 || sr.i1 > d.nx || sr.j1 > d.ny || !sr.w)) {
+      fail(MT_EINVAL, "batched run: bad source");
+      return nullptr;
+    }
+    for (int m = 0; m < items[v].n_mons; ++m)
+      if (items[v].mons[m].kind != 0) {
+        fail(MT_EINVAL, "batched run: probes are not supported (DFT monitors only)");
+        return nullptr;
+      }
+    if (items[v].n_mons) {
+      MonitorPlan mp{items[v].mons, items[v].n_mons, items[v].stage, items[v].stage_bytes, stage_steps, dtype};
+      if (monitors_validate(mp)) return nullptr;
+    }
+    max_mons = std::max(max_mons, items[v].n_mons);
+  }
+  mt_batch* b = new mt_batch();
+  b->nvar = nvar;
+  b->dtype = dtype;
+  b->variant = variant;
+  b->S = stage_steps > 0 ? stage_steps : 1;
+  b->NS = tiled_variant_steps(variant, dtype);
+  b->max_mons = max_mons;
+  b->ws = static_cast<char*>(dev_ws);
+  b->L = dtype == MT_F64 ? layout<double>(nvar) : layout<float>(nvar);
+  for (int v = 0; v < nvar; ++v) {
+    b->probs.push_back(items[v].prob);
+    b->mons.emplace_back(items[v].mons, items[v].mons + items[v].n_mons);
+    b->stage.push_back(items[v].stage);
+    b->stage_bytes.push_back(items[v].stage_bytes);
+  }
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  b->grid_in = 4 * cus;
+  b->grid_ring = 4 * cus;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc = dtype == MT_F64 ? create_t<double>(b, items, s) : create_t<float>(b, items, s);
+  if (rc) {
+    delete b;
+    return nullptr;
+  }
+  return b;
+}
+
+extern "C" int mt_tm2d_batch_run(mt_batch* b, int step0, int k, void* stream) {
+  if (!b) return fail(MT_EINVAL, "null batch");
+  if (k < 0) return fail(MT_EINVAL, "negative step count");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return b->dtype == MT_F64 ? run_t<double>(b, step0, k, s) : run_t<float>(b, step0, k, s);
+}
+
+extern "C" int mt_tm2d_batch_adjoint(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes, const mt_monitor* mons,
+                                     const int* mon_off, int step0, int k, void* stream) {
+  if (!b || !adj || !tapes || !mon_off) return fail(MT_EINVAL, "batched adjoint: null argument");
+  for (int v = 0; v < b->nvar; ++v) {
+    const mt_tm2d_adj& a = adj[v];
+    if (!a.aEz || !a.aDz || !a.aHx || !a.aHy || !a.apsi_Ezy || !a.apsi_Ezx || !a.apsi_Hyx || !a.apsi_Hxy ||
+        !a.gHy || !a.gHx || !a.gEx || !a.gEy || !a.eps_grad || (k > 0 && !tapes[v]))
+      return fail(MT_EINVAL, "batched adjoint: null adjoint pointer");
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return b->dtype == MT_F64 ? adjoint_t<double>(b, adj, tapes, mons, mon_off, step0, k, s)
+                            : adjoint_t<float>(b, adj, tapes, mons, mon_off, step0, k, s);
+}
+
+extern "C" void mt_batch_destroy(mt_batch* b) { delete b; }

@@ -56,6 +56,15 @@ struct TmTiled {
   T* tape;
   mt_source2d src;
   T coef[kMaxSub];
+  const T* coef_tab;  // batched runs: dt*amp for every global step (else NULL)
+};
+
+// One variant of a batched TM run: its problem in both ping-pong
+// orientations (pA reads buffer set A and writes B, pB the reverse).
+template <typename T>
+struct TmBatchVar {
+  TmTiled<T> pA, pB;
+  TapSet taps;
 };
 
 int te_tiled_dispatch_f32(int variant, const TeTiled<float>& p, const TapSet& taps, int nsub, int write_e,
@@ -66,6 +75,13 @@ int tm_tiled_dispatch_f32(int variant, const TmTiled<float>& p, const TapSet& ta
                           hipStream_t s);
 int tm_tiled_dispatch_f64(int variant, const TmTiled<double>& p, const TapSet& taps, int nsub, int tape_s0,
                           hipStream_t s);
+int tm_batch_dispatch_f32(int variant, const TmBatchVar<float>* vars, const int* pre_in, const int* pre_ring,
+                          int nvar, int n_in, int n_ring, int grid_in, int grid_ring, int nsub, int parity,
+                          int t0, int slot_base, int tape_s0, hipStream_t s);
+int tm_batch_dispatch_f64(int variant, const TmBatchVar<double>* vars, const int* pre_in, const int* pre_ring,
+                          int nvar, int n_in, int n_ring, int grid_in, int grid_ring, int nsub, int parity,
+                          int t0, int slot_base, int tape_s0, hipStream_t s);
+TileMap tiled_map(int variant, int dtype, int nx, int ny, int in_i0, int in_i1, int in_j0, int in_j1);
 int tiled_variant_count();
 int tiled_variant_steps(int variant, int dtype);
 

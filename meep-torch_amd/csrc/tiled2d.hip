@@ -52,6 +52,32 @@ __device__ __forceinline__ T div_c(T a, T b, T y) {
   return a / b;
 }
 
+// Tile (tr, tc) for linear index b of the interior rectangle or of the ring.
+__device__ __forceinline__ void tile_linear(bool pml, const TileMap& m, int b, int* tr, int* tc) {
+  if (!pml) {
+    int w = m.c1 - m.c0;
+    *tr = m.r0 + b / w;
+    *tc = m.c0 + b % w;
+    return;
+  }
+  if (b < m.r0 * m.C) {
+    *tr = b / m.C;
+    *tc = b % m.C;
+    return;
+  }
+  b -= m.r0 * m.C;
+  if (b < (m.R - m.r1) * m.C) {
+    *tr = m.r1 + b / m.C;
+    *tc = b % m.C;
+    return;
+  }
+  b -= (m.R - m.r1) * m.C;
+  int per = m.c0 + (m.C - m.c1);
+  *tr = m.r0 + b / per;
+  int k = b % per;
+  *tc = k < m.c0 ? k : m.c1 + (k - m.c0);
+}
+
 // Tile (tr, tc) of this workgroup: interior launches cover the rectangle
 // [r0, r1) x [c0, c1); PML launches enumerate the ring of tiles around it.
 __device__ __forceinline__ void tile_of(bool pml, const TileMap& m, int* tr, int* tc) {
@@ -84,11 +110,11 @@ __device__ __forceinline__ void tile_of(bool pml, const TileMap& m, int* tr, int
 // FIELD(id, li, lj) returns the LDS value of field `id` at local cell.
 template <typename T, int RX, int RY, typename F>
 __device__ void write_taps(const TapSet& taps, int s, int I0, int J0, int TXo, int TYo, int a0, int b0,
-                           F field) {
+                           F field, int slot_base = 0) {
   const int tid = threadIdx.x;
   for (int q = 0; q < taps.n; ++q) {
     const Tap& t = taps.t[q];
-    T* out = static_cast<T*>(t.out) + (long long)(t.slot0 + s) * t.slot_stride;
+    T* out = static_cast<T*>(t.out) + (long long)(t.slot0 + slot_base + s) * t.slot_stride;
     if (t.line == 1) {  // row: fixed i = fix, j = lo + e
       if (t.fix < I0 || t.fix >= I0 + TXo) continue;
       int jlo = max(t.lo, J0), jhi = min(t.lo + t.n, J0 + TYo);
@@ -298,7 +324,8 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
 
 // ------------------------------------------------------------------- TM
 template <typename T, int TX, int TY, int NS, bool PML>
-__global__ __launch_bounds__(BLK) void tm_tiled(TmTiled<T> p, TapSet taps, int nsub, int tape_s0) {
+__device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps, int nsub, int tape_s0, int tr,
+                                        int tc, int slot_base, int coef_t0) {
   constexpr int RX = TX + 2 * NS + 1, RY = TY + 2 * NS + 1, RC = RX * RY;
   constexpr int PER = (RC + BLK - 1) / BLK;
   __shared__ T sEz[RC], sDz[RC], sHx[RC], sHy[RC], sK[RC];
@@ -308,8 +335,6 @@ __global__ __launch_bounds__(BLK) void tm_tiled(TmTiled<T> p, TapSet taps, int n
   T* sPyx = sP + 2 * RC;   // psi_Hyx: Dz update, x shell (index (i-1, jj))
   T* sPxy = sP + 3 * RC;   // psi_Hxy: Dz update, y shell (index (i-1, jj))
   const int nx = p.nx, ny = p.ny, ny_e = p.ny_e;
-  int tr, tc;
-  tile_of(PML, p.map, &tr, &tc);
   const int I0 = tr * TX, J0 = tc * TY;
   const int a0 = I0 - NS - 1, b0 = J0 - NS - 1;
   const int tid = threadIdx.x;
@@ -357,7 +382,7 @@ __global__ __launch_bounds__(BLK) void tm_tiled(TmTiled<T> p, TapSet taps, int n
   }
   __syncthreads();
   for (int s = 0; s < nsub; ++s) {
-    const T coef = p.coef[s];
+    const T coef = p.coef_tab ? p.coef_tab[coef_t0 + s] : p.coef[s];
     // B half (stepper_tm.py:85-92,126-128)
     int li = tid / RY, lj = tid % RY;
 #pragma unroll
@@ -446,7 +471,7 @@ __global__ __launch_bounds__(BLK) void tm_tiled(TmTiled<T> p, TapSet taps, int n
       write_taps<T, RX, RY>(taps, s, I0, J0, TX, TY, a0, b0, [&](int f, int li, int lj) -> T {
         int c = li * RY + lj;
         return f == 0 ? sEz[c] : f == 1 ? sHx[c] : f == 2 ? sHy[c] : sDz[c];
-      });
+      }, slot_base);
       __syncthreads();
     }
   }
@@ -474,6 +499,37 @@ __global__ __launch_bounds__(BLK) void tm_tiled(TmTiled<T> p, TapSet taps, int n
         if (p.ey.in(jj)) p.oHxy[q] = sPxy[lc];
       }
     }
+  }
+}
+
+template <typename T, int TX, int TY, int NS, bool PML>
+__global__ __launch_bounds__(BLK) void tm_tiled(TmTiled<T> p, TapSet taps, int nsub, int tape_s0) {
+  int tr, tc;
+  tile_of(PML, p.map, &tr, &tc);
+  tm_tile<T, TX, TY, NS, PML>(p, taps, nsub, tape_s0, tr, tc, 0, 0);
+}
+
+// Persistent batched form: every workgroup walks the flattened (variant,
+// tile) work list of all variants.  Per-variant problems (both ping-pong
+// orientations), taps and tile maps live in device memory, so the launch
+// depends only on scalars and can be captured in a graph and replayed.
+template <typename T, int TX, int TY, int NS, bool PML>
+__global__ __launch_bounds__(BLK) void tm_batch(const TmBatchVar<T>* vars, const int* prefix, int nvar,
+                                                int nsub, int parity, int t0, int slot_base, int tape_s0) {
+  const int total = prefix[nvar];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    int lo = 0, hi = nvar;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (prefix[mid] <= item) lo = mid;
+      else hi = mid;
+    }
+    const TmBatchVar<T>& V = vars[lo];
+    const TmTiled<T>& p = parity ? V.pB : V.pA;
+    int tr, tc;
+    tile_linear(PML, p.map, item - prefix[lo], &tr, &tc);
+    tm_tile<T, TX, TY, NS, PML>(p, V.taps, nsub, tape_s0, tr, tc, slot_base, t0);
+    __syncthreads();
   }
 }
 
@@ -528,6 +584,24 @@ int launch_tm_tiled(TmTiled<T> p, const TapSet& taps, int nsub, int tape_s0, hip
   return check_launch("tm2d tiled step");
 }
 
+template <typename T, int TX, int TY, int NS>
+int launch_tm_batch(const TmBatchVar<T>* vars, const int* pre_in, const int* pre_ring, int nvar, int n_in,
+                    int n_ring, int grid_in, int grid_ring, int nsub, int parity, int t0, int slot_base,
+                    int tape_s0, hipStream_t s) {
+  if (n_in)
+    hipLaunchKernelGGL((tm_batch<T, TX, TY, NS, false>), dim3(std::min(n_in, grid_in)), dim3(BLK), 0, s, vars,
+                       pre_in, nvar, nsub, parity, t0, slot_base, tape_s0);
+  if (n_ring)
+    hipLaunchKernelGGL((tm_batch<T, TX, TY, NS, true>), dim3(std::min(n_ring, grid_ring)), dim3(BLK), 0, s, vars,
+                       pre_ring, nvar, nsub, parity, t0, slot_base, tape_s0);
+  return check_launch("tm2d batched step");
+}
+
+template <int TX, int TY, int NS>
+TileMap map_for(int nx, int ny, int in_i0, int in_i1, int in_j0, int in_j1) {
+  return make_map<TX, TY, NS>(nx, ny, in_i0, in_i1, in_j0, in_j1);
+}
+
 // Tile-shape variants the autotuner chooses from (index = variant id).
 // (TX, TY, NS): owned rows, owned columns, Yee steps per launch.  FP64 tiles
 // are smaller so LDS (fields + psi) fits one workgroup.
@@ -578,6 +652,45 @@ int tm_tiled_dispatch_f64(int variant, const TmTiled<double>& p, const TapSet& t
   MT_VARIANTS_F64(X, double)
 #undef X
   return fail(MT_EINVAL, "unknown tiled variant");
+}
+
+int tm_batch_dispatch_f32(int variant, const TmBatchVar<float>* vars, const int* pre_in, const int* pre_ring,
+                          int nvar, int n_in, int n_ring, int grid_in, int grid_ring, int nsub, int parity,
+                          int t0, int slot_base, int tape_s0, hipStream_t s) {
+  int v = 0;
+#define X(T, A, B, C)                                                                                  \
+  if (variant == v++)                                                                                  \
+    return launch_tm_batch<T, A, B, C>(vars, pre_in, pre_ring, nvar, n_in, n_ring, grid_in, grid_ring, \
+                                       nsub, parity, t0, slot_base, tape_s0, s);
+  MT_VARIANTS_F32(X, float)
+#undef X
+  return fail(MT_EINVAL, "unknown tiled variant");
+}
+
+int tm_batch_dispatch_f64(int variant, const TmBatchVar<double>* vars, const int* pre_in, const int* pre_ring,
+                          int nvar, int n_in, int n_ring, int grid_in, int grid_ring, int nsub, int parity,
+                          int t0, int slot_base, int tape_s0, hipStream_t s) {
+  int v = 0;
+#define X(T, A, B, C)                                                                                  \
+  if (variant == v++)                                                                                  \
+    return launch_tm_batch<T, A, B, C>(vars, pre_in, pre_ring, nvar, n_in, n_ring, grid_in, grid_ring, \
+                                       nsub, parity, t0, slot_base, tape_s0, s);
+  MT_VARIANTS_F64(X, double)
+#undef X
+  return fail(MT_EINVAL, "unknown tiled variant");
+}
+
+TileMap tiled_map(int variant, int dtype, int nx, int ny, int in_i0, int in_i1, int in_j0, int in_j1) {
+  int v = 0;
+#define X(T, A, B, C) \
+  if (variant == v++) return map_for<A, B, C>(nx, ny, in_i0, in_i1, in_j0, in_j1);
+  if (dtype == MT_F64) {
+    MT_VARIANTS_F64(X, double)
+  } else {
+    MT_VARIANTS_F32(X, float)
+  }
+#undef X
+  return TileMap{};
 }
 
 int tiled_variant_count() { return 7; }

@@ -1,0 +1,294 @@
+"""Batched 2D TM design variants of uneven size on one GPU.
+
+BatchedTM advances B independent TM problems (each its own grid size, eps,
+CPML, source and DFT monitors) together: every block of steps is ONE
+persistent launch pair over the (variant, tile) work list of all variants
+(mt_tm2d_batch_run), so small and large grids share the chip.  All buffers are
+allocated once; `capture()` records the whole forward run (checkpoint copies,
+stepping, spectra, per-chunk folds) in a HIP graph that every later forward
+replays, the pattern of an inverse-design loop where only eps changes.
+
+The gradient with respect to each variant's eps runs the batched reverse-time
+adjoint (mt_tm2d_batch_adjoint) chunk by chunk over the checkpoints, with the
+chunk recomputed on a per-variant Ez tape; `BatchedTM.apply(eps_list)` wraps
+forward + adjoint in torch.autograd.Function.
+
+Monitor accumulators follow run_tm's semantics with checkpoint_every = K: each
+K-step chunk is summed separately and chunk sums are added in order
+(meep_torch/core/run.py:56-63), so a variant's spectra equal run_tm's.
+"""
+import ctypes
+
+import torch
+
+from . import _engine as E
+from .core import engine2d
+from .core.monitors import omega_of
+from .core.stepper_tm import FIELDS
+
+_ORDER = engine2d.TM_ORDER
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("prob", E.TM2D), ("src", E.Source2D), ("coef_tab", E.c_void_p),
+                ("mons", ctypes.POINTER(E.Monitor)), ("n_mons", E.c_int), ("pad_", E.c_int),
+                ("stage", E.c_void_p), ("stage_bytes", E.c_ll)]
+
+
+def _bind(L):
+    if getattr(L, "_batch_bound", False):
+        return
+    P = ctypes.POINTER
+    L.mt_tm2d_batch_workspace.restype = E.c_ll
+    L.mt_tm2d_batch_workspace.argtypes = [E.c_int, E.c_int]
+    L.mt_tm2d_batch_create.restype = E.c_void_p
+    L.mt_tm2d_batch_create.argtypes = [P(BatchItem), E.c_int, E.c_int, E.c_int, E.c_void_p, E.c_ll, E.c_void_p]
+    L.mt_tm2d_batch_run.restype = E.c_int
+    L.mt_tm2d_batch_run.argtypes = [E.c_void_p, E.c_int, E.c_int, E.c_void_p]
+    L.mt_tm2d_batch_adjoint.restype = E.c_int
+    L.mt_tm2d_batch_adjoint.argtypes = [E.c_void_p, P(E.TM2DAdj), P(E.c_void_p), P(E.Monitor),
+                                        P(E.c_int), E.c_int, E.c_int, E.c_void_p]
+    L.mt_batch_destroy.restype = None
+    L.mt_batch_destroy.argtypes = [E.c_void_p]
+    L._batch_bound = True
+
+
+class BatchedTM:
+    """B TM problems stepped together.  problems: list of (coeffs, source,
+    monitors) with coeffs from build_tm_coeffs (eps_inf is replaced per call).
+    Every variant runs n_steps with checkpoints every K steps."""
+
+    def __init__(self, problems, n_steps, checkpoint_every=200, variant=None, stage_steps=64):
+        self.L = E.lib()
+        _bind(self.L)
+        self.B = len(problems)
+        self.n_steps = int(n_steps)
+        self.K = int(checkpoint_every) if checkpoint_every > 0 else self.n_steps
+        c0 = problems[0][0]
+        self.dtype = c0.eps_inf.dtype
+        self.device = c0.eps_inf.device
+        E.require_device(c0.eps_inf)
+        self.code = E.dtype_code(self.dtype)
+        if variant is None:
+            variant = 2 if self.dtype == torch.float32 else 1
+        self.variant = int(variant)
+        self.S = int(stage_steps)
+        self.var = []
+        for v, (coeffs, source, monitors) in enumerate(problems):
+            self.var.append(self._alloc(coeffs, source, list(monitors)))
+        self.ws = torch.empty(max(1, self.L.mt_tm2d_batch_workspace(self.B, self.code)), dtype=torch.uint8,
+                              device=self.device)
+        self._plans = {}
+        self.graph = None
+
+    # ------------------------------------------------------------ allocation
+    def _alloc(self, coeffs, source, monitors):
+        nx, ny = coeffs.eps_inf.shape
+        z = lambda *s: torch.zeros(*s, dtype=self.dtype, device=self.device)
+        ny_e = ny if coeffs.y_pmc else ny - 2
+        shapes = {"Ez": (nx, ny), "Dz": (nx, ny), "Hx": (nx, ny - 1), "Hy": (nx - 1, ny),
+                  "psi_Ezy": (nx, ny - 1), "psi_Ezx": (nx - 1, ny), "psi_Hyx": (nx - 2, ny_e),
+                  "psi_Hxy": (nx - 2, ny_e)}
+        V = {"coeffs": coeffs, "nx": nx, "ny": ny, "monitors": monitors}
+        V["eps"] = coeffs.eps_inf.detach().to(self.dtype).clone().contiguous()
+        V["state"] = {f: z(*shapes[f]) for f in FIELDS}
+        V["alt"] = {f: z(*shapes[f]) for f in FIELDS}
+        nck = (self.n_steps + self.K - 1) // self.K
+        V["ckpt"] = [{f: z(*shapes[f]) for f in FIELDS} for _ in range(nck)]
+        V["tape"] = z(self.K, nx, ny)
+        plan = engine2d.source_plan(source, (nx, ny), self.device, self.dtype)
+        V["plan"] = plan
+        if plan is not None and plan["box"] is not None:
+            vals = [coeffs.dt * plan["source"].amplitude(t) for t in range(self.n_steps)]
+            V["coef"] = torch.tensor(vals, dtype=torch.float64).to(self.dtype).to(self.device)
+        else:
+            V["coef"] = None
+        arr, outs, keep = engine2d.monitor_structs(V["state"], monitors, self.K, self.dtype, self.device)
+        V["mon_arr"], V["mon_keep"] = arr, keep
+        V["chunk_acc"] = [o for (_, _, o) in outs]
+        V["acc"] = [torch.zeros_like(o) for o in V["chunk_acc"]]
+        V["outs_meta"] = outs
+        if monitors:
+            nbytes = self.L.mt_stage_bytes(arr, len(monitors), self.S, self.code)
+            V["stage"] = torch.empty(max(1, nbytes), dtype=torch.uint8, device=self.device)
+            V["stage_bytes"] = nbytes
+        else:
+            V["stage"], V["stage_bytes"] = None, 0
+        return V
+
+    def _prob(self, V, tape):
+        c = V["coeffs"]
+        d = E.TM2D()
+        d.nx, d.ny, d.dtype, d.y_pmc = V["nx"], V["ny"], self.code, int(bool(c.y_pmc))
+        d.dt, d.dx = float(c.dt), float(c.dx)
+        d.eps = V["eps"].data_ptr()
+        d.hy, d.hx, d.ex, d.ey = (engine2d._axis(c, n) for n in ("hy", "hx", "ex", "ey"))
+        for f in FIELDS:
+            setattr(d, f, V["state"][f].data_ptr())
+        for q, f in enumerate(_ORDER):
+            d.alt[q] = V["alt"][f].data_ptr()
+        d.tape = V["tape"].data_ptr() if tape else 0
+        d.variant = self.variant
+        return d
+
+    def _plan(self, tape):
+        if tape in self._plans:
+            return self._plans[tape]
+        items = (BatchItem * self.B)()
+        for v, V in enumerate(self.var):
+            it = items[v]
+            it.prob = self._prob(V, tape)
+            s, _ = engine2d.source_struct(V["plan"])
+            it.src = s
+            it.coef_tab = V["coef"].data_ptr() if V["coef"] is not None else None
+            it.mons = ctypes.cast(V["mon_arr"], ctypes.POINTER(E.Monitor))
+            it.n_mons = len(V["monitors"])
+            it.stage = V["stage"].data_ptr() if V["stage"] is not None else None
+            it.stage_bytes = V["stage_bytes"]
+        h = self.L.mt_tm2d_batch_create(items, self.B, self.variant, self.S, self.ws.data_ptr(),
+                                        self.ws.numel(), E.stream_handle())
+        if not h:
+            E.check(-1, "mt_tm2d_batch_create")
+        self._plans[tape] = (h, items)
+        return self._plans[tape]
+
+    def __del__(self):
+        try:
+            for h, _ in self._plans.values():
+                self.L.mt_batch_destroy(h)
+        except Exception:
+            pass
+
+    # -------------------------------------------------------------- forward
+    def set_eps(self, eps_list):
+        with torch.no_grad():
+            for V, e in zip(self.var, eps_list):
+                V["eps"].copy_(e.detach().to(self.dtype))
+
+    def _forward_body(self):
+        h, _ = self._plan(False)
+        for V in self.var:
+            for t in V["state"].values():
+                t.zero_()
+            for a in V["acc"]:
+                a.zero_()
+        step0, ci = 0, 0
+        while step0 < self.n_steps:
+            k = min(self.K, self.n_steps - step0)
+            for V in self.var:
+                for f in FIELDS:
+                    V["ckpt"][ci][f].copy_(V["state"][f])
+                for a in V["chunk_acc"]:
+                    a.zero_()
+            rc = self.L.mt_tm2d_batch_run(h, step0, k, E.stream_handle())
+            E.check(rc, "mt_tm2d_batch_run")
+            for V in self.var:
+                for a, ca in zip(V["acc"], V["chunk_acc"]):
+                    a.add_(ca)
+            step0 += k
+            ci += 1
+
+    def capture(self):
+        """Record the forward run in a HIP graph (replayed by forward())."""
+        self._plan(False)
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self._forward_body()          # warm-up outside the graph
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._forward_body()
+        self.graph = g
+
+    def forward(self, eps_list=None):
+        """Run every variant; returns [[acc per monitor] per variant] shaped as
+        the monitors' run_tm outputs (views of the fixed buffers)."""
+        if eps_list is not None:
+            self.set_eps(eps_list)
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self._forward_body()
+        res = []
+        for V in self.var:
+            res.append([a.reshape(m.out_shape(V["state"])) for a, m in zip(V["acc"], V["monitors"])])
+        return res
+
+    # -------------------------------------------------------------- adjoint
+    def gradient(self, gaccs):
+        """dL/deps per variant given dL/dacc (list per variant of per-monitor
+        cotangents, None where unused).  Uses the checkpoints of the last forward."""
+        h, _ = self._plan(True)
+        B = self.B
+        adj, grads, mons, off, tapes = [], [], [], [0], (E.c_void_p * B)()
+        adjs = (E.TM2DAdj * B)()
+        keep = []
+        for v, V in enumerate(self.var):
+            nx, ny = V["nx"], V["ny"]
+            A = {f: torch.zeros_like(V["state"][f]) for f in FIELDS}
+            sc = {"gHy": torch.empty(nx, ny, dtype=self.dtype, device=self.device),
+                  "gHx": torch.empty(nx, ny, dtype=self.dtype, device=self.device),
+                  "gEx": torch.empty(nx - 1, ny, dtype=self.dtype, device=self.device),
+                  "gEy": torch.empty(nx, ny - 1, dtype=self.dtype, device=self.device)}
+            g = torch.zeros(nx, ny, dtype=self.dtype, device=self.device)
+            adj.append((A, sc))
+            grads.append(g)
+            a = adjs[v]
+            for f in FIELDS:
+                setattr(a, "a" + f, A[f].data_ptr())
+            for f, t in sc.items():
+                setattr(a, f, t.data_ptr())
+            a.eps_grad = g.data_ptr()
+            tapes[v] = V["tape"].data_ptr()
+            from .core.adjoint2d import _adj_monitors
+            arr, n_used, kp = _adj_monitors(A, V["monitors"], list(gaccs[v]), self.dtype, self.device)
+            keep.append(kp)
+            for i in range(n_used):
+                mons.append(arr[i])
+            off.append(off[-1] + n_used)
+        mon_arr = (E.Monitor * max(1, len(mons)))(*mons)
+        off_arr = (E.c_int * (B + 1))(*off)
+        nck = len(self.var[0]["ckpt"])
+        for ci in reversed(range(nck)):
+            step0 = ci * self.K
+            k = min(self.K, self.n_steps - step0)
+            for V in self.var:
+                for f in FIELDS:
+                    V["state"][f].copy_(V["ckpt"][ci][f])
+            rc = self.L.mt_tm2d_batch_run(h, step0, k, E.stream_handle())
+            E.check(rc, "mt_tm2d_batch_run (recompute)")
+            rc = self.L.mt_tm2d_batch_adjoint(h, adjs, tapes, mon_arr, off_arr, step0, k, E.stream_handle())
+            E.check(rc, "mt_tm2d_batch_adjoint")
+        del keep
+        return grads
+
+    def apply(self, eps_list):
+        """Differentiable forward: returns [[acc per monitor] per variant];
+        backward gives dL/deps for each variant's eps tensor."""
+        flat = _BatchFn.apply(self, *eps_list)
+        res, i = [], 0
+        for V in self.var:
+            n = len(V["monitors"])
+            res.append(list(flat[i:i + n]))
+            i += n
+        return res
+
+
+class _BatchFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, runner, *eps_list):
+        out = runner.forward(list(eps_list))
+        ctx.runner = runner
+        ctx.counts = [len(o) for o in out]
+        return tuple(a.clone() for o in out for a in o)
+
+    @staticmethod
+    def backward(ctx, *g):
+        runner = ctx.runner
+        gaccs, i = [], 0
+        for n in ctx.counts:
+            gaccs.append([x.contiguous() if x is not None else None for x in g[i:i + n]])
+            i += n
+        grads = runner.gradient(gaccs)
+        return (None, *grads)
