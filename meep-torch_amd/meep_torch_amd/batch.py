@@ -56,9 +56,11 @@ def _bind(L):
 class BatchedTM:
     """B TM problems stepped together.  problems: list of (coeffs, source,
     monitors) with coeffs from build_tm_coeffs (eps_inf is replaced per call).
-    Every variant runs n_steps with checkpoints every K steps."""
+    Every variant runs n_steps with checkpoints every K steps.  With
+    adjoint=False no checkpoints or Ez tapes are kept (forward runs only;
+    spectra are still folded per K-step chunk)."""
 
-    def __init__(self, problems, n_steps, checkpoint_every=200, variant=None, stage_steps=64):
+    def __init__(self, problems, n_steps, checkpoint_every=200, variant=None, stage_steps=64, adjoint=True):
         self.L = E.lib()
         _bind(self.L)
         self.B = len(problems)
@@ -73,6 +75,7 @@ class BatchedTM:
             variant = 2 if self.dtype == torch.float32 else 1
         self.variant = int(variant)
         self.S = int(stage_steps)
+        self.adjoint = bool(adjoint)
         self.var = []
         for v, (coeffs, source, monitors) in enumerate(problems):
             self.var.append(self._alloc(coeffs, source, list(monitors)))
@@ -94,8 +97,8 @@ class BatchedTM:
         V["state"] = {f: z(*shapes[f]) for f in FIELDS}
         V["alt"] = {f: z(*shapes[f]) for f in FIELDS}
         nck = (self.n_steps + self.K - 1) // self.K
-        V["ckpt"] = [{f: z(*shapes[f]) for f in FIELDS} for _ in range(nck)]
-        V["tape"] = z(self.K, nx, ny)
+        V["ckpt"] = [{f: z(*shapes[f]) for f in FIELDS} for _ in range(nck)] if self.adjoint else []
+        V["tape"] = z(self.K, nx, ny) if self.adjoint else None
         plan = engine2d.source_plan(source, (nx, ny), self.device, self.dtype)
         V["plan"] = plan
         if plan is not None and plan["box"] is not None:
@@ -176,8 +179,9 @@ class BatchedTM:
         while step0 < self.n_steps:
             k = min(self.K, self.n_steps - step0)
             for V in self.var:
-                for f in FIELDS:
-                    V["ckpt"][ci][f].copy_(V["state"][f])
+                if self.adjoint:
+                    for f in FIELDS:
+                        V["ckpt"][ci][f].copy_(V["state"][f])
                 for a in V["chunk_acc"]:
                     a.zero_()
             rc = self.L.mt_tm2d_batch_run(h, step0, k, E.stream_handle())
@@ -219,6 +223,8 @@ class BatchedTM:
     def gradient(self, gaccs):
         """dL/deps per variant given dL/dacc (list per variant of per-monitor
         cotangents, None where unused).  Uses the checkpoints of the last forward."""
+        if not self.adjoint:
+            raise RuntimeError("BatchedTM was built with adjoint=False (no checkpoints or tapes)")
         h, _ = self._plan(True)
         B = self.B
         adj, grads, mons, off, tapes = [], [], [], [0], (E.c_void_p * B)()

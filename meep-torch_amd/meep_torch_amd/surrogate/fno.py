@@ -1,0 +1,328 @@
+"""Spectral-convolution neural operator (FNO-style) on the matrix cores.
+
+Maps a density rho (B, n, n) to a spectrum (B, n_out) and, through its
+backward, to d(loss)/d(rho).  Architecture (channels-first, P = n*n points):
+
+    x    = [rho, (i+0.5)/n, (j+0.5)/n]                   (B, 3, P)
+    v0   = Wlift x + blift                               (B, C, P)
+    v_l+1 = act( iDFT( W_l(k) * DFT(v_l) ) + W'_l v_l + b_l )   l < L
+    h    = gelu(Wq v_L + bq);  s = mean_P h;  y = Wo s + bo
+    act = gelu for l < L-1, identity for the last layer.
+
+DFT keeps mx x my modes of the real 2D DFT (kx in [0, mx/2) U [n-mx/2, n),
+ky in [0, my), ortho scaling 1/n); iDFT is the matching irfft2 of the
+zero-padded spectrum (ky > 0 counted twice, imaginary part of the result
+dropped).  Both are dense real GEMMs against precomputed BF16 bases, and the
+per-mode complex channel mixing is a batched real GEMM against the block
+matrix [[Re W, Im W], [-Im W, Re W]], so every matrix product of the
+forward and backward runs in one kernel (mt_gemm_bf16: BF16 operands, FP32
+accumulation).  Operands are rounded to BF16 at the GEMM inputs; biases,
+activations, reductions and the optimizer stay FP32.
+
+Default size: n=128, C=64, modes 16 x 8, 4 layers, head 128, 50 outputs:
+4 x (2*64*64*128 + 64*64 + 64) + lift + head = 4,225,970 parameters.
+"""
+import math
+
+import numpy as np
+import torch
+
+from .. import _engine as E
+from . import _nn
+
+
+def mode_list(n, mx, my):
+    """(kx, ky) of the retained modes, in the order used by every layout:
+    mode m = a * my + ky with kx(a) = a for a < mx/2 else n - mx + a."""
+    if mx % 2 or mx > n or my > n // 2:
+        raise ValueError("modes: mx must be even and <= n, my <= n/2 (no Nyquist column)")
+    out = []
+    for a in range(mx):
+        kx = a if a < mx // 2 else n - mx + a
+        for ky in range(my):
+            out.append((kx, ky))
+    return out
+
+
+def dft_bases(n, mx, my):
+    """Real truncated-DFT bases in float64.
+
+    F (2Mo, P): row 2m = cos(theta)/n, row 2m+1 = -sin(theta)/n, so
+    F @ v = [Re, Im] of the ortho-normalised rfft2 coefficients.
+    G (2Mo, P): row 2m = c cos(theta)/n, row 2m+1 = -c sin(theta)/n with
+    c = 1 for ky = 0 else 2, so U @ G = irfft2 (ortho) of the zero-padded
+    spectrum U.  theta = 2 pi (kx x + ky y) / n at point p = x * n + y."""
+    modes = mode_list(n, mx, my)
+    x = np.arange(n, dtype=np.float64)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    Mo = len(modes)
+    F = np.empty((2 * Mo, n * n))
+    G = np.empty((2 * Mo, n * n))
+    for m, (kx, ky) in enumerate(modes):
+        th = (2.0 * math.pi / n) * ((kx * X + ky * Y) % n)
+        c, s = np.cos(th).reshape(-1) / n, np.sin(th).reshape(-1) / n
+        w = 1.0 if ky == 0 else 2.0
+        F[2 * m], F[2 * m + 1] = c, -s
+        G[2 * m], G[2 * m + 1] = w * c, -w * s
+    return F, G
+
+
+class SpectralOperator:
+    """Parameters live in one flat FP32 vector (`params`) with named views;
+    `grads` mirrors it.  BF16 copies of the weights (and the packed spectral
+    blocks) are refreshed by `sync_weights()` after every change."""
+
+    def __init__(self, n=128, width=64, modes=(16, 8), layers=4, n_out=50, head=128, in_ch=3, seed=0,
+                 device="cuda"):
+        self.n, self.C, self.L, self.n_out, self.H, self.in_ch = n, width, layers, n_out, head, in_ch
+        self.mx, self.my = modes
+        self.Mo = self.mx * self.my
+        self.P = n * n
+        self.device = torch.device(device)
+        C, Mo, H = width, self.Mo, head
+        spec = [("lift_w", (C, in_ch)), ("lift_b", (C,))]
+        for l in range(layers):
+            spec += [(f"s{l}_re", (C, C, Mo)), (f"s{l}_im", (C, C, Mo)), (f"w{l}", (C, C)), (f"b{l}", (C,))]
+        spec += [("q_w", (H, C)), ("q_b", (H,)), ("o_w", (n_out, H)), ("o_b", (n_out,))]
+        self.spec = spec
+        total = sum(int(np.prod(s)) for _, s in spec)
+        self.n_params = total
+        self.params = torch.zeros(total, dtype=torch.float32, device=self.device)
+        self.grads = torch.zeros_like(self.params)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.step_count = 0
+        self.p, self.g = self._views(self.params), self._views(self.grads)
+        self.init_params(seed)
+        F, G = dft_bases(n, self.mx, self.my)
+        self.F = self._bf(F)                        # (2Mo, P)
+        self.FT = self._bf(np.ascontiguousarray(F.T))  # (P, 2Mo)
+        self.G = self._bf(G)
+        self.GT = self._bf(np.ascontiguousarray(G.T))
+        self.ws = _nn.Workspace(self.device)
+        self._wbf = {}
+        self.sync_weights()
+        self._buf = {}
+
+    # ---- parameters -------------------------------------------------------
+    def _views(self, flat):
+        out, o = {}, 0
+        for name, shape in self.spec:
+            k = int(np.prod(shape))
+            out[name] = flat[o:o + k].view(shape)
+            o += k
+        return out
+
+    def _bf(self, a):
+        return torch.from_numpy(np.asarray(a, dtype=np.float64)).to(torch.float32).to(torch.bfloat16).to(
+            self.device)
+
+    def init_params(self, seed):
+        """Seeded CPU init: uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) for the
+        pointwise maps, scale * uniform[0, 1) with scale = 1/(C*C) for both
+        parts of the spectral weights."""
+        gen = torch.Generator().manual_seed(int(seed))
+        host = torch.empty(self.n_params, dtype=torch.float32)
+        hv = self._views(host)
+        for name, shape in self.spec:
+            if "_re" in name or "_im" in name:
+                hv[name].copy_(torch.rand(shape, generator=gen) / (self.C * self.C))
+            else:
+                fan_in = shape[1] if len(shape) == 2 else (self.in_ch if name == "lift_b" else
+                                                           self.H if name == "o_b" else self.C)
+                bound = 1.0 / math.sqrt(fan_in)
+                hv[name].copy_((torch.rand(shape, generator=gen) * 2 - 1) * bound)
+        self.params.copy_(host.to(self.device))
+        self.m.zero_()
+        self.v.zero_()
+        self.step_count = 0
+
+    def sync_weights(self):
+        """Refresh the BF16 weight copies from `params` (after init, load or an
+        optimizer step)."""
+        C, Mo = self.C, self.Mo
+        for name in ["lift_w", "q_w", "o_w"] + [f"w{l}" for l in range(self.L)]:
+            t = self.p[name]
+            if name not in self._wbf:
+                self._wbf[name] = torch.empty(t.shape, dtype=torch.bfloat16, device=self.device)
+            _nn.cast_bf16(t, self._wbf[name])
+        for l in range(self.L):
+            key = f"wp{l}"
+            if key not in self._wbf:
+                self._wbf[key] = torch.empty((Mo, 2 * C, 2 * C), dtype=torch.bfloat16, device=self.device)
+            _nn.pack_spectral(self.p[f"s{l}_re"], self.p[f"s{l}_im"], self._wbf[key], C, C, Mo)
+
+    def state_dict(self):
+        return {"params": self.params.detach().cpu().clone(), "step": self.step_count,
+                "config": dict(n=self.n, width=self.C, modes=(self.mx, self.my), layers=self.L,
+                               n_out=self.n_out, head=self.H, in_ch=self.in_ch)}
+
+    def load_state_dict(self, sd):
+        self.params.copy_(sd["params"].to(self.device))
+        self.step_count = int(sd.get("step", 0))
+        self.sync_weights()
+
+    # ---- buffers ----------------------------------------------------------
+    def _get(self, key, shape, dtype):
+        t = self._buf.get(key)
+        if t is None or t.shape != torch.Size(shape) or t.dtype != dtype:
+            t = torch.empty(shape, dtype=dtype, device=self.device)
+            self._buf[key] = t
+        return t
+
+    def _input(self, rho):
+        B, n = rho.shape[0], self.n
+        x = self._get("x32", (B, self.in_ch, n, n), torch.float32)
+        x[:, 0].copy_(rho)
+        if self.in_ch > 1:
+            ax = (torch.arange(n, device=self.device, dtype=torch.float32) + 0.5) / n
+            x[:, 1].copy_(ax.view(n, 1).expand(n, n))
+            x[:, 2].copy_(ax.view(1, n).expand(n, n))
+        return _nn.cast_bf16(x, self._get("x16", x.shape, torch.bfloat16))
+
+    # ---- forward ----------------------------------------------------------
+    def forward(self, rho):
+        """rho (B, n, n) float32 on the device -> spectrum (B, n_out) float32.
+        Saves what `backward` needs."""
+        E.require_device(rho)
+        if rho.dim() != 3 or rho.shape[1:] != (self.n, self.n):
+            raise ValueError(f"rho must be (B, {self.n}, {self.n})")
+        rho = rho.detach().to(torch.float32).contiguous()
+        B, C, P, Mo, H, L = rho.shape[0], self.C, self.P, self.Mo, self.H, self.L
+        ws, W, bf, f32 = self.ws, self._wbf, torch.bfloat16, torch.float32
+        CP, S = C * P, 2 * Mo * C
+        x16 = self._input(rho)
+        # lift (no activation)
+        pre = self._get("pre_lift", (B, C, P), f32)
+        _nn.gemm(W["lift_w"], (self.in_ch, 1, 0, 0), x16, (P, 1, self.in_ch * P, 0), pre, (P, 1, CP),
+                 C, P, self.in_ch, batch=B, ws=ws)
+        v = self._get("v0", (B, C, P), bf)
+        _nn.bias_act(pre, self.p["lift_b"], C, P, False, act16=v)
+        for l in range(L):
+            vh = self._get(f"vh{l}", (B, S), f32)
+            vh16 = self._get(f"vh16_{l}", (B, S), bf)
+            uh = self._get("uh", (B, S), f32)
+            uh16 = self._get("uh16", (B, S), bf)
+            u = self._get(f"pre{l}", (B, C, P), f32)
+            # truncated forward DFT: vh[b] (C x 2Mo, stored [b][col][c]) = v[b] (C x P) F^T
+            _nn.gemm(v, (P, 1, CP, 0), self.FT, (2 * Mo, 1, 0, 0), vh, (1, C, S), C, 2 * Mo, P, batch=B, ws=ws)
+            _nn.cast_bf16(vh, vh16)
+            # per-mode complex mixing: [Re|Im](B x 2C) @ Wp[m] (2C x 2C)
+            _nn.gemm(vh16, (S, 1, 2 * C, 0), W[f"wp{l}"], (2 * C, 1, 4 * C * C, 0), uh, (S, 1, 2 * C),
+                     B, 2 * C, 2 * C, batch=Mo, ws=ws)
+            _nn.cast_bf16(uh, uh16)
+            # inverse DFT: u[b] (C x P) = uh[b] (C x 2Mo) G, plus the pointwise path W'_l v
+            _nn.gemm(uh16, (1, C, S, 0), self.G, (P, 1, 0, 0), u, (P, 1, CP), C, P, 2 * Mo, batch=B, ws=ws)
+            _nn.gemm(W[f"w{l}"], (C, 1, 0, 0), v, (P, 1, CP, 0), u, (P, 1, CP), C, P, C, batch=B, beta=1.0,
+                     ws=ws)
+            v = self._get(f"v{l + 1}", (B, C, P), bf)
+            _nn.bias_act(u, self.p[f"b{l}"], C, P, l < L - 1, act16=v)
+        # head
+        hq = self._get("pre_q", (B, H, P), f32)
+        _nn.gemm(W["q_w"], (C, 1, 0, 0), v, (P, 1, CP, 0), hq, (P, 1, H * P), H, P, C, batch=B, ws=ws)
+        hact = self._get("hq_act", (B, H, P), f32)
+        _nn.bias_act(hq, self.p["q_b"], H, P, True, act32=hact)
+        hm = self._get("hm", (B, H), f32)
+        _nn.row_sum(hact, hm, B * H, P, 1.0 / P)
+        hm16 = _nn.cast_bf16(hm, self._get("hm16", (B, H), bf))
+        out = torch.empty((B, self.n_out), dtype=f32, device=self.device)
+        _nn.gemm(hm16, (H, 1, 0, 0), W["o_w"], (1, H, 0, 0), out, (self.n_out, 1, 0), B, self.n_out, H, ws=ws)
+        _nn.bias_act(out, self.p["o_b"], self.n_out, 1, False)
+        self._saved_B = B
+        return out
+
+    # ---- backward ---------------------------------------------------------
+    def backward(self, dout, param_grads=True, input_grad=True):
+        """Backward of the last forward.  Fills `grads` (if param_grads) and
+        returns d/d(rho) (B, n, n) (if input_grad, else None)."""
+        E.require_device(dout)
+        B = self._saved_B
+        if dout.shape != (B, self.n_out):
+            raise ValueError("dout shape does not match the last forward")
+        C, P, Mo, H, L, n_out = self.C, self.P, self.Mo, self.H, self.L, self.n_out
+        ws, W, bf, f32, g = self.ws, self._wbf, torch.bfloat16, torch.float32, self.g
+        CP, S = C * P, 2 * Mo * C
+        b = self._buf
+        dout = dout.detach().to(f32).contiguous()
+        d16 = _nn.cast_bf16(dout, self._get("dout16", (B, n_out), bf))
+        if param_grads:
+            _nn.channel_sum(dout, g["o_b"], B, n_out, 1)
+            _nn.gemm(d16, (1, n_out, 0, 0), b["hm16"], (H, 1, 0, 0), g["o_w"], (H, 1, 0), n_out, H, B, ws=ws)
+        dhm = self._get("dhm", (B, H), f32)
+        _nn.gemm(d16, (n_out, 1, 0, 0), W["o_w"], (H, 1, 0, 0), dhm, (H, 1, 0), B, H, n_out, ws=ws)
+        dq32 = b["hq_act"]  # reused: the FP32 activation is no longer needed
+        dq16 = self._get("dq16", (B, H, P), bf)
+        _nn.act_grad(dhm, b["pre_q"], B * H * P, True, P=P, row_bcast=True, scale=1.0 / P, d32=dq32, d16=dq16)
+        vL = b[f"v{L}"]
+        if param_grads:
+            _nn.channel_sum(dq32, g["q_b"], B, H, P)
+            _nn.gemm(dq16, (P, 1, 0, H * P), vL, (1, P, 0, CP), g["q_w"], (C, 1, 0), H, C, P, R=B, ws=ws)
+        dv = self._get("dv_a", (B, C, P), f32)
+        dv_next = self._get("dv_b", (B, C, P), f32)
+        _nn.gemm(W["q_w"], (1, C, 0, 0), dq16, (P, 1, H * P, 0), dv, (P, 1, CP), C, P, H, batch=B, ws=ws)
+        dp16 = self._get("dp16", (B, C, P), bf)
+        duh = self._get("uh", (B, S), f32)
+        duh16 = self._get("uh16", (B, S), bf)
+        dvh = self._get("dvh", (B, S), f32)
+        dvh16 = self._get("dvh16", (B, S), bf)
+        dwp = self._get("dwp", (Mo, 2 * C, 2 * C), f32)
+        for l in range(L - 1, -1, -1):
+            v = b[f"v{l}"]
+            _nn.act_grad(dv, b[f"pre{l}"], B * CP, l < L - 1, d32=dv, d16=dp16)
+            if param_grads:
+                _nn.channel_sum(dv, g[f"b{l}"], B, C, P)
+                _nn.gemm(dp16, (P, 1, 0, CP), v, (1, P, 0, CP), g[f"w{l}"], (C, 1, 0), C, C, P, R=B, ws=ws)
+            # through the inverse DFT: duh[b] (C x 2Mo) = dpre[b] (C x P) G^T
+            _nn.gemm(dp16, (P, 1, CP, 0), self.GT, (2 * Mo, 1, 0, 0), duh, (1, C, S), C, 2 * Mo, P, batch=B,
+                     ws=ws)
+            _nn.cast_bf16(duh, duh16)
+            if param_grads:
+                _nn.gemm(b[f"vh16_{l}"], (1, S, 2 * C, 0), duh16, (S, 1, 2 * C, 0), dwp, (2 * C, 1, 4 * C * C),
+                         2 * C, 2 * C, B, batch=Mo, ws=ws)
+                _nn.unpack_spectral_grad(dwp, g[f"s{l}_re"], g[f"s{l}_im"], C, C, Mo)
+            if l == 0 and not input_grad:
+                break
+            _nn.gemm(duh16, (S, 1, 2 * C, 0), W[f"wp{l}"], (1, 2 * C, 4 * C * C, 0), dvh, (S, 1, 2 * C),
+                     B, 2 * C, 2 * C, batch=Mo, ws=ws)
+            _nn.cast_bf16(dvh, dvh16)
+            # dv_l = W'^T dpre + (through the forward DFT) dvh[b] (C x 2Mo) F
+            _nn.gemm(W[f"w{l}"], (1, C, 0, 0), dp16, (P, 1, CP, 0), dv_next, (P, 1, CP), C, P, C, batch=B, ws=ws)
+            _nn.gemm(dvh16, (1, C, S, 0), self.F, (P, 1, 0, 0), dv_next, (P, 1, CP), C, P, 2 * Mo, batch=B,
+                     beta=1.0, ws=ws)
+            dv, dv_next = dv_next, dv
+        if not input_grad:
+            return None
+        # lift (identity activation): dpre_lift = dv
+        _nn.cast_bf16(dv, dp16)
+        x16 = b["x16"]
+        if param_grads:
+            _nn.channel_sum(dv, g["lift_b"], B, C, P)
+            _nn.gemm(dp16, (P, 1, 0, CP), x16, (1, P, 0, self.in_ch * P), g["lift_w"], (self.in_ch, 1, 0),
+                     C, self.in_ch, P, R=B, ws=ws)
+        drho = torch.empty((B, self.n, self.n), dtype=f32, device=self.device)
+        _nn.gemm(W["lift_w"], (1, self.in_ch, 0, 0), dp16, (P, 1, CP, 0), drho, (P, 1, P), 1, P, C, batch=B,
+                 ws=ws)
+        return drho
+
+    # ---- optimizer ----------------------------------------------------------
+    def adam_step(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+        self.step_count += 1
+        _nn.adam(self.params, self.grads, self.m, self.v, lr, b1, b2, eps, self.step_count, wd)
+        self.sync_weights()
+
+
+class _SurrogateFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rho, net):
+        ctx.net = net
+        return net.forward(rho)
+
+    @staticmethod
+    def backward(ctx, dout):
+        return ctx.net.backward(dout, param_grads=False, input_grad=True), None
+
+
+def predict(net, rho):
+    """Differentiable w.r.t. rho (gradient through the network's own backward
+    kernels; parameter gradients are not formed)."""
+    return _SurrogateFn.apply(rho, net)

@@ -1,5 +1,5 @@
 """CPU-side checks of the engine boundary: the C-ABI library loads and exports
-every entry point include/meep_fdtd.h declares; host logic (profiles, shell
+every entry point the headers under include/ declare; host logic (profiles, shell
 bounds, source boxes, feature-flag errors) behaves like the reference."""
 import ctypes
 import os
@@ -10,11 +10,11 @@ import torch
 
 from conftest import ROOT
 
-HEADER = os.path.join(ROOT, "include", "meep_fdtd.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("meep_fdtd.h", "meep_surrogate.h")]
 
 
 def _declared():
-    text = open(HEADER).read()
+    text = "\n".join(open(h).read() for h in HEADERS)
     return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w ]+?\*?\s*\b(mt_\w+)\s*\(", text, re.M)))
 
 
@@ -22,7 +22,8 @@ def test_library_exports_every_declared_symbol():
     from meep_torch_amd import _engine
     L = _engine.lib()
     names = _declared()
-    assert {"mt_tm2d_run", "mt_te2d_run", "mt_yee3d_run", "mt_last_error"} <= set(names)
+    assert {"mt_tm2d_run", "mt_te2d_run", "mt_yee3d_run", "mt_last_error", "mt_gemm_bf16"} <= set(names)
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == sorted(os.path.basename(h) for h in HEADERS)
     for n in names:
         assert hasattr(L, n), f"libmeepfdtd.so does not export {n}"
     assert L.mt_abi_version() == _engine.ABI_VERSION
