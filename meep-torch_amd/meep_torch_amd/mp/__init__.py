@@ -1,0 +1,12 @@
+"""meep-compatible facade (subset) over the HIP FDTD engine: Simulation,
+Block/Medium rasterization, PML, Gaussian/continuous line sources, flux and
+DFT-field monitors (meep_torch/mp)."""
+from .geometry import PML, Absorber, Block, GeometricObject, Medium
+from .simulation import DftFields, DftFlux, FluxRegion, Simulation, Volume, get_flux_freqs, get_fluxes
+from .sources import ContinuousSource, GaussianSource, Source
+from .vec import (ALL, Dielectric, Dz, Ex, Ey, Ez, High, Hx, Hy, Hz, Low, Vector3, X, Y, Z, inf)
+
+__all__ = ["PML", "Absorber", "Block", "GeometricObject", "Medium", "DftFields", "DftFlux", "FluxRegion",
+           "Simulation", "Volume", "get_flux_freqs", "get_fluxes", "ContinuousSource", "GaussianSource",
+           "Source", "ALL", "Dielectric", "Dz", "Ex", "Ey", "Ez", "High", "Hx", "Hy", "Hz", "Low", "Vector3",
+           "X", "Y", "Z", "inf"]

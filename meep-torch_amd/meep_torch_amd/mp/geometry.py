@@ -1,0 +1,217 @@
+"""Grid spec, materials, axis-aligned Blocks and their rasterization onto the
+Yee grid (the host-side setup of the Simulation facade).
+
+Coordinates follow meep: the cell is centred on the origin; grid cell (i, j)
+covers the corner-origin square [i dx, (i+1) dx) x [j dx, (j+1) dx)
+(meep_torch/mp/coords.py:1-125).  A Block's coverage of a cell is its exact
+area fraction (the reference computes the same fraction by clipping the
+polygon strip by strip, meep_torch/geometry/raster.py:37-67, and snaps values
+within 1e-9 of 0 or 1; both are applied here).  Objects are painted in list
+order: TM eps = cov * eps_obj + (1 - cov) * eps_before
+(meep_torch/geometry/smoothing.py:11-12, meep_torch/geometry/scene.py:53-101);
+TE uses the diagonal-projected Kottke pair at interface cells
+(meep_torch/geometry/smoothing.py:15-23; Kottke, Farjadpour & Johnson,
+Phys. Rev. E 77, 036611 (2008)).
+"""
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .vec import ALL, Vector3
+
+
+@dataclass(frozen=True)
+class GridSpec:
+    nx: int
+    ny: int
+    dx: float
+    sx: float
+    sy: float
+
+
+def grid_spec(cell, resolution):
+    """cell_size + resolution -> GridSpec: dx = 1/R, n = max(3, round(s R))
+    (meep_torch/mp/coords.py:34-50, 2D)."""
+    c = Vector3(*cell)
+    dx = 1.0 / float(resolution)
+    nx = max(3, int(round(c.x / dx))) if c.x > 0 else 3
+    ny = max(3, int(round(c.y / dx))) if c.y > 0 else 3
+    return GridSpec(nx, ny, dx, c.x if c.x > 0 else nx * dx, c.y if c.y > 0 else ny * dx)
+
+
+def _clip(v, lo, hi):
+    return max(lo, min(hi, v))
+
+
+def cell_index(x, gs):
+    """World x -> cell index round((x + sx/2) / dx) (meep_torch/mp/coords.py:79-85)."""
+    return int(round((x + gs.sx / 2.0) / gs.dx))
+
+
+def cell_index_y(y, gs):
+    return int(round((y + gs.sy / 2.0) / gs.dx))
+
+
+def span_index_range(a, b, gs, axis):
+    """Corner-origin span [a, b] -> half-open index range, at least one cell
+    (meep_torch/mp/coords.py:112-121)."""
+    n = gs.nx if axis == "x" else gs.ny
+    i0 = max(0, int(round(a / gs.dx)))
+    i1 = min(n, int(round(b / gs.dx)))
+    if i1 <= i0:
+        i1 = min(n, i0 + 1)
+    return i0, i1
+
+
+def rect_corners(center, size, gs):
+    """Centre + size (world) -> corner-origin (x0, y0, x1, y1), infinite extents
+    and overhangs clipped to the cell (meep_torch/mp/coords.py:63-76)."""
+    if math.isinf(size.x):
+        x0, x1 = 0.0, gs.sx
+    else:
+        x0 = _clip(center.x - size.x / 2.0 + gs.sx / 2.0, 0.0, gs.sx)
+        x1 = _clip(center.x + size.x / 2.0 + gs.sx / 2.0, 0.0, gs.sx)
+    if math.isinf(size.y):
+        y0, y1 = 0.0, gs.sy
+    else:
+        y0 = _clip(center.y - size.y / 2.0 + gs.sy / 2.0, 0.0, gs.sy)
+        y1 = _clip(center.y + size.y / 2.0 + gs.sy / 2.0, 0.0, gs.sy)
+    return x0, y0, x1, y1
+
+
+def pml_cells(thickness, gs):
+    return max(0, int(round(thickness / gs.dx)))
+
+
+class Medium:
+    """Isotropic, non-dispersive dielectric: epsilon, or index (eps = n^2).
+    Other meep Medium properties raise NotImplementedError by name."""
+
+    def __init__(self, epsilon=None, index=None, **unsupported):
+        for k, v in unsupported.items():
+            if v not in (None, 0, 0.0, (), [], 1, 1.0, False):
+                raise NotImplementedError(f"Medium({k}=...) is not supported by the HIP FDTD path")
+        if index is not None and epsilon is not None:
+            raise ValueError("Medium: give epsilon or index, not both")
+        self.epsilon = float(index) ** 2 if index is not None else (1.0 if epsilon is None else float(epsilon))
+
+    def resolve(self):
+        return self.epsilon
+
+
+class GeometricObject:
+    def __init__(self, material=None, center=Vector3(), epsilon_func=None):
+        if epsilon_func is not None:
+            raise NotImplementedError("epsilon_func geometry is not supported")
+        self.material = material if material is not None else Medium()
+        if isinstance(self.material, (int, float)):
+            self.material = Medium(epsilon=float(self.material))
+        self.center = Vector3(*center)
+
+    def shift(self, v):
+        raise NotImplementedError
+
+
+class Block(GeometricObject):
+    """Axis-aligned brick (rotated bases raise NotImplementedError)."""
+
+    def __init__(self, size, e1=Vector3(1, 0, 0), e2=Vector3(0, 1, 0), e3=Vector3(0, 0, 1), **kw):
+        super().__init__(**kw)
+        self.size = Vector3(*size)
+        if Vector3(*e1) != Vector3(1, 0, 0) or Vector3(*e2) != Vector3(0, 1, 0):
+            raise NotImplementedError("rotated Block is not supported")
+
+    def coverage(self, gs):
+        x0, y0, x1, y1 = rect_corners(self.center, self.size, gs)
+        return rect_coverage(x0, y0, x1, y1, gs)
+
+
+def _overlap_1d(a, b, n, dx):
+    lo = np.arange(n, dtype=np.float64) * dx
+    hi = (np.arange(n, dtype=np.float64) + 1.0) * dx
+    return np.clip(np.minimum(hi, b) - np.maximum(lo, a), 0.0, None) / dx
+
+
+def rect_coverage(x0, y0, x1, y1, gs):
+    """Area fraction of every cell covered by [x0, x1] x [y0, y1] (corner
+    coordinates), clipped to [0, 1] and snapped to 0/1 within 1e-9."""
+    cov = np.outer(_overlap_1d(x0, x1, gs.nx, gs.dx), _overlap_1d(y0, y1, gs.ny, gs.dx))
+    cov = np.clip(cov, 0.0, 1.0)
+    cov[np.abs(cov - 1.0) <= 1e-9] = 1.0
+    cov[np.abs(cov) <= 1e-9] = 0.0
+    return cov
+
+
+def interface_normals(cov):
+    """Unit normals (pointing out of the object) from central differences of
+    the coverage with edge replication, on interface cells (0 < cov < 1)
+    only; zero elsewhere."""
+    p = np.pad(cov, 1, mode="edge")
+    gx = 0.5 * (p[2:, 1:-1] - p[:-2, 1:-1])
+    gy = 0.5 * (p[1:-1, 2:] - p[1:-1, :-2])
+    length = np.sqrt(gx * gx + gy * gy)
+    ok = (cov > 0) & (cov < 1) & (length > 0)
+    safe = np.where(ok, length, 1.0)
+    return np.where(ok, -gx / safe, 0.0), np.where(ok, -gy / safe, 0.0)
+
+
+def rasterize(objects, gs, background=1.0):
+    """-> (eps_tm, eps_te_xx, eps_te_yy) float64 numpy arrays (nx, ny)."""
+    eps_tm = np.full((gs.nx, gs.ny), float(background))
+    te_xx = eps_tm.copy()
+    te_yy = eps_tm.copy()
+    for obj in objects:
+        if not isinstance(obj, Block):
+            raise NotImplementedError(f"geometry object {type(obj).__name__} is not supported (Block only)")
+        cov = obj.coverage(gs)
+        val = obj.material.resolve()
+        old = eps_tm
+        eps_tm = cov * val + (1.0 - cov) * old
+        arith = eps_tm
+        harm = 1.0 / (cov / val + (1.0 - cov) / old)
+        nxn, nyn = interface_normals(cov)
+        n2x, n2y = nxn * nxn, nyn * nyn
+        edge = (n2x + n2y) > 0.5
+        te_xx = np.where(edge, n2x * harm + (1.0 - n2x) * arith, arith)
+        te_yy = np.where(edge, n2y * harm + (1.0 - n2y) * arith, arith)
+    return eps_tm, te_xx, te_yy
+
+
+class PML:
+    """Perfectly matched layer of `thickness` on `direction` (ALL, X, Y) and
+    `side` (ALL, Low, High)."""
+
+    def __init__(self, thickness, direction=ALL, side=ALL, R_asymptotic=1e-15, mean_stretch=1.0,
+                 pml_profile=None):
+        if pml_profile is not None or mean_stretch != 1.0:
+            raise NotImplementedError("PML profile/stretch options are not supported")
+        self.thickness = float(thickness)
+        self.direction = direction
+        self.side = side
+
+
+class Absorber(PML):
+    def __init__(self, *a, **k):
+        raise NotImplementedError("Absorber layers are not supported by the HIP FDTD path (use PML)")
+
+
+def pml_sides(layers, gs):
+    """[[x_lo, x_hi], [y_lo, y_hi]] cell counts, max over layers."""
+    sides = [[0, 0], [0, 0]]
+    for bl in layers:
+        if not isinstance(bl, PML):
+            raise NotImplementedError(f"boundary layer {type(bl).__name__} is not supported")
+        n = pml_cells(bl.thickness, gs)
+        axes = (0, 1) if bl.direction == ALL else (bl.direction,)
+        for ax in axes:
+            if ax not in (0, 1):
+                raise ValueError("2D cell: PML direction must be X, Y or ALL")
+            for s in ((0, 1) if bl.side == ALL else (bl.side,)):
+                sides[ax][s] = max(sides[ax][s], n)
+    return sides
+
+
+def to_tensor(a, dtype, device):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64).to(device=device, dtype=dtype)

@@ -1,0 +1,311 @@
+"""meep-compatible Simulation facade over the HIP FDTD engine (2D TM/TE).
+
+Covers the hook-free flow of meep_torch/mp/simulation.py: construction
+(:214-381), structure rasterization (:511-573), sources (:796-880),
+add_flux/get_fluxes/get_flux_freqs (:1357-1386,1809-1870),
+add_dft_fields/get_dft_array (:1592-1641), run(until | until_after_sources |
+n_steps) (:1096-1142,1239-1302) and get_array (:2119-2150).
+
+Time stepping semantics follow the reference: every run() starts from zero
+fields and zero DFT accumulators and integrates to the accumulated clock
+(so consecutive runs reproduce one continuing run), in chunks
+[this run's steps, previous clock] exactly as its _run_until loop issues
+them (meep_torch/mp/simulation.py:1271-1295); each chunk's monitor sums are
+added in chunk order.  Step functions, stop conditions, symmetries, k-points,
+3D cells, multi-source runs and non-Block geometry raise
+NotImplementedError.
+"""
+import numpy as np
+import torch
+
+from ..core import engine2d
+from ..core.coeffs import build_te_coeffs, build_tm_coeffs
+from ..core.grid import Grid
+from ..core.monitors import DFTFieldsMonitor, FluxMonitorTE, FluxMonitorTEY, FluxMonitorTM
+from ..core.sources import LineSource
+from ..core.stepper_te import FIELDS as TE_FIELDS, init_te_state
+from ..core.stepper_tm import FIELDS as TM_FIELDS, init_tm_state
+from . import geometry as G
+from .sources import source_span
+from .vec import Vector3, pol_for
+
+_DEFAULTS = dict(eps_averaging=True, symmetries=None, force_complex_fields=False, m=0, k_point=False,
+                 bfast_scaled_k=None, material_function=None, epsilon_func=None, epsilon_input_file="",
+                 output_volume=None, force_all_components=False, geometry_center=None, extra_materials=None,
+                 kz_2d="complex", progress_interval=4, subpixel_tol=1e-4, subpixel_maxeval=100000,
+                 loop_tile_base_db=0, loop_tile_base_eh=0, ensure_periodicity=True, num_chunks=0,
+                 accurate_fields_near_cylorigin=False, filename_prefix=None, output_single_precision=False,
+                 split_chunks_evenly=True, chunk_layout=None, collect_stats=False)
+
+
+class FluxRegion:
+    def __init__(self, center=Vector3(), size=Vector3(), direction=None, weight=1.0, volume=None):
+        if volume is not None:
+            center, size = volume.center, volume.size
+        self.center, self.size = Vector3(*center), Vector3(*size)
+        self.direction = direction
+        self.weight = weight
+
+
+class Volume:
+    def __init__(self, center=Vector3(), size=Vector3()):
+        self.center, self.size = Vector3(*center), Vector3(*size)
+
+
+class _Dft:
+    def __init__(self, sim, name, freqs):
+        self.sim, self.name, self.freqs = sim, name, list(freqs)
+        self.nfreqs = len(self.freqs)
+
+
+class DftFlux(_Dft):
+    pass
+
+
+class DftFields(_Dft):
+    pass
+
+
+def _name(f):
+    return f.name if isinstance(f, _Dft) else f
+
+
+class Simulation:
+    def __init__(self, *, cell_size, resolution, geometry=None, sources=None, boundary_layers=None,
+                 default_material=None, Courant=0.5, dtype=torch.float64, device=None, dimensions=2,
+                 **options):
+        for k, v in options.items():
+            if k not in _DEFAULTS:
+                raise TypeError(f"Simulation() got an unexpected keyword argument {k!r}")
+            if k == "geometry_center" and (v is None or Vector3(*v) == Vector3()):
+                continue
+            if k == "symmetries" and not v:
+                continue
+            if k == "eps_averaging" or v == _DEFAULTS[k]:
+                continue
+            raise NotImplementedError(f"Simulation({k}=...) is not supported by the HIP FDTD facade")
+        if dimensions != 2:
+            raise NotImplementedError("the Simulation facade drives 2D cells only")
+        self.dimensions = 2
+        self.dtype = dtype
+        self.cell_size = Vector3(*cell_size)
+        self.resolution = resolution
+        self.gs = G.grid_spec(self.cell_size, resolution)
+        kw = {} if device is None else {"device": device}
+        self.grid = Grid(nx=self.gs.nx, ny=self.gs.ny, dx=self.gs.dx, dtype=dtype, courant=float(Courant), **kw)
+        self._sources = list(sources or [])
+        self._pol = pol_for(self._sources[0].component) if self._sources else "tm"
+        self._geometry = list(geometry or [])
+        self._default = default_material if default_material is not None else G.Medium(epsilon=1.0)
+        if isinstance(self._default, (int, float)):
+            self._default = G.Medium(epsilon=float(self._default))
+        self.boundary_layers = list(boundary_layers or [])
+        self._npml_sides = G.pml_sides(self.boundary_layers, self.gs)
+        self._monitors = {}        # name -> (kind, monitor, meta)
+        self._flux_parts = {}
+        self._acc = {}
+        self._t_step = 0
+        self._state = None
+        self._raster = None
+        self._coeffs = None
+        self.run_index = 0
+
+    # ---- structure -----------------------------------------------------
+    def _init_structure(self):
+        if self._coeffs is not None:
+            return
+        pol = pol_for(self._sources[0].component) if self._sources else "tm"
+        self._pol = pol
+        self._raster = G.rasterize(self._geometry, self.gs, self._default.resolve())
+        g = self.grid
+        npml = (tuple(self._npml_sides[0]), tuple(self._npml_sides[1]))
+        if pol == "tm":
+            eps = G.to_tensor(self._raster[0], g.dtype, g.device)
+            self._coeffs = build_tm_coeffs(g, eps_inf=eps, npml=npml)
+        else:
+            eps_xx = G.to_tensor(self._raster[1], torch.float64, g.device)
+            eps_yy = G.to_tensor(self._raster[2], torch.float64, g.device)
+            c = build_te_coeffs(g, eps_inf=eps_xx, npml=npml)
+            self._coeffs = c._replace(eps_y=(0.5 * (eps_yy[:, :-1] + eps_yy[:, 1:])).to(g.dtype))
+
+    def init_sim(self):
+        self._init_structure()
+
+    def _make_sources(self):
+        if len(self._sources) > 1:
+            raise NotImplementedError("several sources in one run are not supported by the HIP facade")
+        out = []
+        for spec in self._sources:
+            ix, j0, j1 = source_span(spec, self.gs)
+            src = LineSource(self.grid, ix, j0, j1, profile=spec.src.profile())
+            src.mask = src.mask * spec.amplitude
+            if self._pol == "te":
+                src.mask = src.mask[:, :-1].contiguous()
+            out.append(src)
+        return out
+
+    def _source_cutoff(self):
+        return max((float(getattr(s.src.profile(), "t_end", 0.0)) for s in self._sources), default=0.0)
+
+    # ---- monitors ------------------------------------------------------
+    def _line_monitor(self, region, freqs):
+        c, s, gs = region.center, region.size, self.gs
+        x_normal = abs(s.x) <= abs(s.y)
+        if x_normal:
+            idx = G.cell_index(c.x, gs)
+            y0, y1 = c.y - s.y / 2 + gs.sy / 2, c.y + s.y / 2 + gs.sy / 2
+            lo, hi = G.span_index_range(max(0.0, y0), min(gs.sy, y1), gs, "y")
+        else:
+            idx = max(0, min(gs.ny, G.cell_index_y(c.y, gs)))
+            x0, x1 = c.x - s.x / 2 + gs.sx / 2, c.x + s.x / 2 + gs.sx / 2
+            lo, hi = G.span_index_range(max(0.0, x0), min(gs.sx, x1), gs, "x")
+        meta = {"idx": idx, "lo": lo, "hi": hi, "wt": complex(region.weight or 1.0).real}
+        if self._pol == "tm":
+            mon = FluxMonitorTM(self.grid, freqs, x_normal=x_normal, idx=idx, lo=lo, hi=hi)
+        elif x_normal:
+            mon = FluxMonitorTE(self.grid, freqs, ix=idx, lo=lo, hi=hi)
+        else:
+            mon = FluxMonitorTEY(self.grid, freqs, iy=idx, lo=lo, hi=hi)
+        return mon, meta
+
+    def add_flux(self, fcen, df, nfreq, *regions, name=None):
+        freqs = np.linspace(fcen - df / 2.0, fcen + df / 2.0, nfreq).tolist()
+        self._init_structure()
+        name = name or f"flux{len(self._monitors)}"
+        parts = []
+        for r, region in enumerate(regions):
+            mon, meta = self._line_monitor(region, freqs)
+            key = name if len(regions) == 1 else f"{name}:r{r}"
+            self._monitors[key] = ("flux", mon, meta)
+            parts.append(key)
+        if len(regions) > 1:
+            self._flux_parts[name] = parts
+        return DftFlux(self, name, freqs)
+
+    def add_dft_fields(self, components, fcen, df, nfreq, where=None, center=None, size=None, name=None,
+                       yee_grid=False, decimation_factor=0):
+        freqs = np.linspace(fcen - df / 2.0, fcen + df / 2.0, nfreq).tolist()
+        if where is not None:
+            center, size = where.center, where.size
+        self._init_structure()
+        name = name or f"dft{len(self._monitors)}"
+        box = None
+        if center is not None or size is not None:
+            x0, y0, x1, y1 = G.rect_corners(Vector3(*(center or Vector3())), Vector3(*(size or Vector3())),
+                                            self.gs)
+            i0, i1 = G.span_index_range(x0, x1, self.gs, "x")
+            j0, j1 = G.span_index_range(y0, y1, self.gs, "y")
+            box = (i0, i1, j0, j1)
+        for comp in components:
+            mon = DFTFieldsMonitor(self.grid, freqs, component=comp)
+            self._monitors[f"{name}:{comp}"] = ("dft", mon, {"freqs": freqs, "box": box})
+        return DftFields(self, name, freqs)
+
+    # ---- running -------------------------------------------------------
+    def run(self, *step_funcs, until=None, until_after_sources=None, n_steps=None):
+        if step_funcs or callable(until) or callable(until_after_sources) or \
+                isinstance(until, (list, tuple)) or isinstance(until_after_sources, (list, tuple)):
+            raise NotImplementedError("step functions / stop conditions are not supported by the HIP facade")
+        self.run_index += 1
+        self._init_structure()
+        dt = self.grid.dt
+        if n_steps is None:
+            if until is not None:
+                T = until
+            elif until_after_sources is not None:
+                T = self._source_cutoff() + until_after_sources
+            else:
+                T = 0
+            n_steps = max(1, int(round(float(T) / dt)))
+        self._advance(int(n_steps))
+
+    def _advance(self, n_total):
+        mode = self._pol
+        names = TM_FIELDS if mode == "tm" else TE_FIELDS
+        st = init_tm_state(self.grid) if mode == "tm" else init_te_state(self.grid)
+        work = {f: getattr(st, f).contiguous() for f in names}
+        srcs = self._make_sources()
+        src = srcs[0] if srcs else None
+        target = work["Dz"].shape if mode == "tm" else work["Dy"].shape
+        plan = engine2d.source_plan(src, target, self.grid.device, self.grid.dtype)
+        keys = list(self._monitors)
+        mons = [self._monitors[k][1] for k in keys]
+        acc = [None] * len(mons)
+        start = self._t_step
+        stop_at = start + n_total
+        clock, f_t = start, 0
+        while clock < stop_at:
+            k = stop_at - clock
+            outs = engine2d.run_chunk(mode, work, self._coeffs, f_t, k, plan, mons)
+            for i, o in enumerate(outs):
+                acc[i] = o if acc[i] is None else acc[i] + o
+            f_t += k
+            clock = f_t
+        self._t_step = clock
+        self._state = st._replace(**work)
+        self._acc = {k: a for k, a in zip(keys, acc)}
+
+    def restart_fields(self):
+        self._state = None
+        self._t_step = 0
+
+    def round_time(self):
+        return self._t_step * self.grid.dt
+
+    meep_time = round_time
+
+    # ---- readout ---------------------------------------------------------
+    def get_fluxes(self, f):
+        name = _name(f)
+        parts = self._flux_parts.get(name)
+        if parts:
+            return [sum(v) for v in zip(*(self.get_fluxes(p) for p in parts))]
+        kind, mon, meta = self._monitors[name]
+        acc = self._acc.get(name)
+        if acc is None:
+            return [0.0] * int(mon.freqs.numel())
+        vals = mon.flux(acc, self.grid.dx).detach().cpu().numpy() * meta.get("wt", 1.0)
+        return vals.tolist()
+
+    def get_flux_freqs(self, f):
+        name = _name(f)
+        parts = self._flux_parts.get(name)
+        return self._monitors[parts[0] if parts else name][1].freqs.detach().cpu().numpy().tolist()
+
+    def get_dft_array(self, f, component, num_freq):
+        key = f"{_name(f)}:{component}"
+        kind, mon, meta = self._monitors[key]
+        a = self._acc[key][num_freq].detach().cpu().numpy()
+        out = a[0] + 1j * a[1]
+        if meta.get("box") is not None:
+            i0, i1, j0, j1 = meta["box"]
+            out = out[i0:i1, j0:j1]
+        return out
+
+    def get_array(self, component="Dielectric", center=None, size=None):
+        if center is not None or size is not None:
+            raise NotImplementedError("get_array sub-volumes are not supported")
+        if component == "Dielectric":
+            self._init_structure()
+            return (self._raster[0] if self._pol == "tm" else self._raster[1]).copy()
+        if self._state is None:
+            raise RuntimeError("run() before get_array(fields)")
+        if component in ("Ez", "Ex") and self._pol == "tm":
+            return self._state.Ez.detach().cpu().numpy()
+        if component in ("Hz", "Hy") and self._pol == "te":
+            return self._state.Hz.detach().cpu().numpy()
+        f = getattr(self._state, component, None)
+        if f is None:
+            raise ValueError(f"unsupported component {component!r}")
+        return f.detach().cpu().numpy()
+
+    def get_epsilon(self):
+        return self.get_array("Dielectric")
+
+
+def get_fluxes(f):
+    return f.sim.get_fluxes(f)
+
+
+def get_flux_freqs(f):
+    return f.sim.get_flux_freqs(f)

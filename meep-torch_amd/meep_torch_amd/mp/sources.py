@@ -1,0 +1,82 @@
+"""meep source specifications -> engine line sources
+(meep_torch/mp/sources.py:100-160,484-507)."""
+import math
+
+from ..core import sources as core_sources
+from .geometry import cell_index, cell_index_y, span_index_range
+from .vec import Vector3
+
+
+class _Delayed:
+    """Profile that is zero before start_time and shifted by it afterwards."""
+
+    def __init__(self, inner, start_time):
+        self._inner = inner
+        self.start_time = float(start_time)
+        self.t_end = getattr(inner, "t_end", 0.0) + self.start_time
+
+    def amplitude(self, t):
+        return self._inner.amplitude(t - self.start_time) if t >= self.start_time else 0.0
+
+
+class GaussianSource:
+    """Gaussian pulse: width = max(width, 1/fwidth), peak at
+    start_time + cutoff * width; engine profile GaussianPulse(f, 1/width, cutoff)."""
+
+    def __init__(self, frequency=None, width=0, fwidth=float("inf"), start_time=0, cutoff=5.0,
+                 is_integrated=False, wavelength=None):
+        if frequency is None and wavelength is None:
+            raise ValueError("GaussianSource requires either frequency or wavelength")
+        if is_integrated:
+            raise NotImplementedError("is_integrated=True is not supported (soft current sources only)")
+        self.frequency = 1.0 / wavelength if wavelength else float(frequency)
+        self.width = max(width, 1.0 / fwidth)
+        self.start_time = start_time
+        self.cutoff = cutoff
+
+    def profile(self):
+        if self.width == 0:
+            raise ValueError("GaussianSource needs width or fwidth")
+        p = core_sources.GaussianPulse(self.frequency, 1.0 / float(self.width), cutoff=self.cutoff)
+        return _Delayed(p, self.start_time) if self.start_time else p
+
+
+class ContinuousSource:
+    def __init__(self, frequency=None, width=0, start_time=0, end_time=float("inf"), slowness=3.0,
+                 wavelength=None, is_integrated=False):
+        if frequency is None and wavelength is None:
+            raise ValueError("ContinuousSource requires either frequency or wavelength")
+        if is_integrated or start_time or not math.isinf(end_time):
+            raise NotImplementedError("ContinuousSource start/end times and is_integrated are not supported")
+        self.frequency = 1.0 / wavelength if wavelength else float(frequency)
+        self.width = width
+        self.slowness = slowness
+
+    def profile(self):
+        return core_sources.ContinuousSource(self.frequency, width=self.width or None, slowness=self.slowness)
+
+
+class Source:
+    def __init__(self, src, component, center=Vector3(), size=Vector3(), amplitude=1.0, amp_func=None):
+        if amp_func is not None:
+            raise NotImplementedError("amp_func sources are not supported")
+        if complex(amplitude).imag != 0:
+            raise NotImplementedError("complex source amplitudes are not supported")
+        self.src, self.component = src, component
+        self.center, self.size = Vector3(*center), Vector3(*size)
+        self.amplitude = complex(amplitude).real
+
+
+def source_span(spec, gs):
+    """(ix, j0, j1): x-normal line (or point) of the source (meep_torch/mp/sources.py:484-501)."""
+    c = spec.center
+    ix = cell_index(c.x, gs)
+    if spec.size.y == 0:
+        jy = max(0, min(gs.ny, cell_index_y(c.y, gs)))
+        return ix, jy, jy + 1
+    if math.isinf(spec.size.y):
+        return ix, 0, gs.ny
+    y0 = c.y - spec.size.y / 2.0 + gs.sy / 2.0
+    y1 = c.y + spec.size.y / 2.0 + gs.sy / 2.0
+    j0, j1 = span_index_range(max(0.0, y0), min(gs.sy, y1), gs, "y")
+    return ix, j0, j1

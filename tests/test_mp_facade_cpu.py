@@ -1,0 +1,93 @@
+"""Host side of the meep-compatible Simulation facade (no GPU work): the
+GOLDEN_PLAIN scene (meep_torch tests/mp_parity/test_meep_flow.py:49-54)
+rasterizes to the reference's eps array bit-exactly, its source, monitor and
+PML indices follow meep_torch/mp/coords.py, flux frequencies equal the
+reference's (in each dtype), Block coverage is the exact area fraction, and
+unsupported options raise NotImplementedError."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden
+
+
+def _plain(dtype):
+    import meep_torch_amd.mp as mp
+    return mp, mp.Simulation(
+        cell_size=mp.Vector3(10, 6), resolution=10,
+        geometry=[mp.Block(center=mp.Vector3(), size=mp.Vector3(mp.inf, 1, mp.inf),
+                           material=mp.Medium(epsilon=12))],
+        sources=[mp.Source(mp.GaussianSource(0.15, fwidth=0.1), component=mp.Ez,
+                           center=mp.Vector3(-3, 0), size=mp.Vector3(0, 2))],
+        boundary_layers=[mp.PML(1.0)], dtype=dtype, device="cpu")
+
+
+@pytest.mark.parametrize("dtype,tag", [(torch.float32, "f32"), (torch.float64, "f64")])
+def test_golden_plain_setup_matches_reference(dtype, tag):
+    g = golden("golden_plain")
+    mp, sim = _plain(dtype)
+    eps = sim.get_array(component=mp.Dielectric)
+    assert eps.dtype == np.float64 and np.array_equal(eps, g[f"{tag}_eps"])
+    fl = sim.add_flux(0.15, 0.1, 3, mp.FluxRegion(center=mp.Vector3(3, 0), size=mp.Vector3(0, 4)))
+    assert np.array_equal(np.array(mp.get_flux_freqs(fl)), g[f"{tag}_freqs"])
+    mon = sim._monitors[fl.name][1]
+    assert (mon.idx, mon.lo, mon.hi, mon.x_normal) == (80, 10, 50, True)
+    src = sim._make_sources()[0]
+    nz = src.mask.nonzero()
+    assert nz[:, 0].unique().tolist() == [20] and nz[:, 1].tolist() == list(range(20, 40))
+    assert sim._npml_sides == [[10, 10], [10, 10]]
+    assert int(round(40 / sim.grid.dt)) == 800
+
+
+def test_block_coverage_is_exact_area_fraction():
+    from meep_torch_amd.mp import geometry as G
+    from meep_torch_amd.mp.vec import Vector3
+    gs = G.grid_spec(Vector3(2, 1), 10)
+    b = G.Block(size=Vector3(0.55, 0.33), center=Vector3(0.123, -0.07), material=G.Medium(epsilon=4))
+    cov = b.coverage(gs)
+    x0, y0, x1, y1 = G.rect_corners(b.center, b.size, gs)
+    assert abs(cov.sum() * gs.dx ** 2 - (x1 - x0) * (y1 - y0)) < 1e-12
+    # brute force: 200x200 sub-samples per cell on a few cells
+    for (i, j) in [(int(x0 / gs.dx), int(y0 / gs.dx)), (int(x1 / gs.dx), int(y1 / gs.dx)), (10, 3)]:
+        s = (np.arange(200) + 0.5) / 200 * gs.dx
+        X, Y = np.meshgrid(i * gs.dx + s, j * gs.dx + s, indexing="ij")
+        bf = ((X >= x0) & (X < x1) & (Y >= y0) & (Y < y1)).mean()
+        assert abs(cov[i, j] - bf) < 0.01
+
+
+def test_te_kottke_pair_on_interface():
+    from meep_torch_amd.mp import geometry as G
+    from meep_torch_amd.mp.vec import Vector3
+    gs = G.grid_spec(Vector3(2, 2), 10)
+    # slab with its edge inside a cell along x: normal is +-x there
+    from meep_torch_amd.mp.vec import inf
+    b = G.Block(size=Vector3(0.55, inf), center=Vector3(0.0, 0.0),
+                material=G.Medium(epsilon=9))
+    eps_tm, xx, yy = G.rasterize([b], gs, 1.0)
+    i = int(np.argmax((eps_tm[:, 10] > 1) & (eps_tm[:, 10] < 9)))
+    c = (eps_tm[i, 10] - 1.0) / 8.0
+    harm = 1.0 / (c / 9.0 + (1 - c) / 1.0)
+    assert abs(xx[i, 10] - harm) < 1e-12        # E normal to the interface: harmonic mean
+    assert abs(yy[i, 10] - eps_tm[i, 10]) < 1e-12  # tangential: arithmetic mean
+
+
+def test_unsupported_options_raise():
+    import meep_torch_amd.mp as mp
+    with pytest.raises(NotImplementedError):
+        mp.Simulation(cell_size=mp.Vector3(4, 4), resolution=10, symmetries=[object()], device="cpu")
+    with pytest.raises(NotImplementedError):
+        mp.Simulation(cell_size=mp.Vector3(4, 4, 4), resolution=10, dimensions=3, device="cpu")
+    with pytest.raises(NotImplementedError):
+        mp.Medium(epsilon=2, chi3=0.1)
+    with pytest.raises(NotImplementedError):
+        mp.Block(size=mp.Vector3(1, 1), e1=mp.Vector3(1, 1))
+    _, sim = _plain(torch.float32)
+    with pytest.raises(NotImplementedError):
+        sim.run(lambda s: None, until=1)
+
+
+def test_run_requires_hip_device():
+    """No CPU fallback: running the facade on CPU tensors raises."""
+    _, sim = _plain(torch.float32)
+    with pytest.raises(RuntimeError):
+        sim.run(until=1)
