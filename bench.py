@@ -6,7 +6,11 @@ source, 10000 Yee steps, 4 flux lines x 100 frequencies accumulated on the fly.
 One bench "step" is one full C2 run (10000 Yee steps); `value` is whole-job
 cell-updates/s with inputs resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c3|c1] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5] [--no-cpu-baseline]
+
+c4: batched forward+adjoint design iterations of 64 uneven TM variants.
+c5: surrogate-assisted inverse design -- FDTD dataset, BF16 matrix-core
+training, then plain-FDTD and surrogate-assisted Adam loops from one start.
 
 Prints one JSON line on stdout (progress goes to stderr).
 """
@@ -187,7 +191,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--c5-solves", type=int, default=20000, help="FDTD solves in the surrogate dataset")
+    ap.add_argument("--c5-epochs", type=int, default=20)
+    ap.add_argument("--c5-iters", type=int, default=500, help="Adam iterations of each design loop")
+    ap.add_argument("--c5-yee-steps", type=int, default=2500)
     ap.add_argument("--c4-variants", type=int, default=64)
     ap.add_argument("--yee-steps", type=int, default=0, help="override Yee steps per run (tuning)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -198,6 +206,8 @@ def main():
         sys.exit(2)
     if args.workload == "c4":
         return bench_c4(args)
+    if args.workload == "c5":
+        return bench_c5(args)
     builder = {"c1": build_c1, "c2": build_c2, "c3": build_c3}[args.workload]
     run, cells, ysteps, _ = builder(args.yee_steps or None)
     log(f"[bench] {args.workload}: {cells} cells x {ysteps} Yee steps per bench step")
@@ -270,6 +280,68 @@ def bench_c4(args):
                                f"[{min(sizes)},{max(sizes)}], {ysteps} steps, ckpt 200",
                    "cells_per_variant_sum": cells},
         "sweep_cell_updates_per_s": upd / wall, "fom": float(fom)}), flush=True)
+
+
+def _gemm_roofline(net, batch=64, reps=5):
+    """Matrix-core roofline of the surrogate's GEMM kernel: FLOPs of every
+    GEMM in `reps` training steps (forward + backward + Adam) at `batch`,
+    divided by the summed duration of those GEMM launches (HIP events around
+    each launch on the launch stream)."""
+    from meep_torch_amd.surrogate import _nn
+    rho = torch.rand(batch, net.n, net.n, device="cuda")
+    dout = torch.randn(batch, net.n_out, device="cuda")
+    net.forward(rho)
+    net.backward(dout)
+    torch.cuda.synchronize()
+    _nn.PROFILE = []
+    for _ in range(reps):
+        net.forward(rho)
+        net.backward(dout)
+    torch.cuda.synchronize()
+    rec, _nn.PROFILE = _nn.PROFILE, None
+    flops = sum(f for f, _, _ in rec)
+    ms = sum(a.elapsed_time(b) for _, a, b in rec)
+    achieved = flops / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s", "frac": achieved / 2500.0,
+            "traffic": None, "kernel": f"gemm_kernel (BF16 MFMA, FP32 acc), all GEMMs of a B={batch} training step",
+            "gemm_launches": len(rec) // reps, "gemm_flops_per_step": flops / reps}
+
+
+def bench_c5(args):
+    from meep_torch_amd.surrogate import design as D
+    from meep_torch_amd.surrogate.problem import TransmissionProblem
+    from meep_torch_amd.surrogate.train import build_surrogate
+    pb = TransmissionProblem(steps=args.c5_yee_steps)
+    t0 = time.perf_counter()
+    net, data, hist = build_surrogate(pb, n_train=args.c5_solves, n_val=512, epochs=args.c5_epochs, log=log)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    roof = _gemm_roofline(net)
+    log(f"[bench] c5 GEMM roofline: {roof['achieved']:.1f} TFLOP/s ({100 * roof['frac']:.1f}% of BF16 dense)")
+    theta0 = D.start_design(pb)
+    n_train = args.c5_solves
+    rf = D.fdtd_loop(pb, iters=args.c5_iters, theta0=theta0, log=log)
+    rs = D.surrogate_loop(pb, net, iters=args.c5_iters, theta0=theta0,
+                          data=(data[0][:n_train], data[1][:n_train]), log=log)
+    gap = abs(rs["final_fom"] - rf["final_fom"]) / abs(rf["final_fom"])
+    log(f"[bench] c5 plain FDTD: {rf['iters_per_s']:.2f} it/s final FOM {rf['final_fom']:.4f}; surrogate: "
+        f"{rs['iters_per_s']:.2f} it/s final FOM {rs['final_fom']:.4f} (gap {100 * gap:.2f}%), "
+        f"{rs['failed']}/{rs['verifications']} verifications failed, {rs['fdtd_iters']} FDTD iterations")
+    print(json.dumps({
+        "metric": "surrogate-assisted design iterations/s (final FOM vs plain FDTD loop)",
+        "value": rs["iters_per_s"], "unit": "design-iterations/s", "n_gpus": 1, "steps": args.c5_iters,
+        "warmup": 0, "ms_per_step": rs["wall_s"] * 1e3 / args.c5_iters, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": f"C5 surrogate-assisted design: 2D TM {pb.n}x{pb.n}, 128x128 design, "
+                               f"{pb.steps} Yee steps/solve, {args.c5_solves} training solves, FNO 4x64 "
+                               f"16x8 modes ({net.n_params} params), {args.c5_iters} Adam iterations, "
+                               f"verify every 10, fallback above 2%"},
+        "plain_fdtd_iters_per_s": rf["iters_per_s"], "speedup_vs_plain": rs["iters_per_s"] / rf["iters_per_s"],
+        "final_fom_plain": rf["final_fom"], "final_fom_surrogate": rs["final_fom"], "fom_gap": gap,
+        "verifications": rs["verifications"], "verifications_failed": rs["failed"],
+        "fdtd_iterations_in_surrogate_loop": rs["fdtd_iters"], "setup_s": setup_s,
+        "val_err_median": hist[-1].get("val_err_median"), "val_err_p90": hist[-1].get("val_err_p90"),
+        "roofline": roof}), flush=True)
 
 
 if __name__ == "__main__":

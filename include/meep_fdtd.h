@@ -99,6 +99,10 @@ typedef struct {
   /* psi, reference shapes (stepper_3d.py:47-58), in this order:
    * Ez_y Ey_z Ex_z Ez_x Ey_x Ex_y Hz_y Hy_z Hx_z Hz_x Hy_x Hx_y */
   void* psi[12];
+  /* optional device tape: when set, E after step s of a call is written to
+   * slot s = tape + s * (|Ex| + |Ey| + |Ez|) as [Ex | Ey | Ez]
+   * (reference shapes), for the adjoint's eps gradient */
+  void* tape;
 } mt_yee3d;
 
 /* 3D soft source: kind 1 = point at (i0, j0, k0), kind 2 = mask given as dense
@@ -197,6 +201,30 @@ typedef struct {
  * chunks (meep_torch/core/run.py:99-120, eps reached via stepper_tm.py:226-227). */
 int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const void* tape,
                     const mt_monitor* mons, int n_mons, int step0, int k, void* stream);
+
+/* Adjoint state of the 3D functional stepper (sigma = 0, CPML): cotangents of
+ * every state array (reference shapes, updated in place), 12 scratch arrays
+ * of difference cotangents -- the first six shaped like the E components
+ * that read them (Hz_y: Ex, Hy_z: Ex, Hx_z: Ey, Hz_x: Ey, Hy_x: Ez, Hx_y: Ez),
+ * the last six like the H components (Ez_y: Hx, Ey_z: Hx, Ex_z: Hy, Ez_x: Hy,
+ * Ey_x: Hz, Ex_y: Hz) --, three eps_c cotangents shaped like Ex, Ey, Ez, and
+ * eps_grad (nx,ny,nz) into which dL/deps is accumulated. */
+typedef struct {
+  void *aEx, *aEy, *aEz, *aHx, *aHy, *aHz;
+  void* apsi[12];        /* order of mt_yee3d.psi */
+  void* g[12];
+  void* geps[3];
+  void* eps_grad;
+} mt_yee3d_adj;
+
+/* Reverse-time adjoint of k steps of mt_yee3d_run (lean = 0) from global step
+ * step0: inject the monitor cotangents (monitor term views pointing into the
+ * adjoint arrays, acc = dL/dacc), then transpose the E and H half-steps.
+ * tape holds k + 1 slots: slot 0 = E at step0, slot s + 1 = E after step
+ * step0 + s.  Replaces the autograd reverse sweep of run_3d_adjoint
+ * (meep_torch/core/adjoint3d.py:56-115). */
+int mt_yee3d_adjoint(const mt_yee3d* p, const mt_yee3d_adj* a, const void* tape, const mt_monitor* mons,
+                     int n_mons, int step0, int k, void* stream);
 
 /* ---- Batched TM design variants (uneven grid sizes, one persistent launch) ----
  * Each item is a full 2D TM problem with its own state (primary buffers plus

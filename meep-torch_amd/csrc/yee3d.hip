@@ -97,7 +97,7 @@ __device__ __forceinline__ T inject(const K3<T>& p, const mt_source3d& s, int co
 }
 
 template <typename T>
-__global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T c) {
+__global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T c, T* tape) {
   int k = blockIdx.x * BZ + threadIdx.x;
   int j = blockIdx.y * BYY + threadIdx.y;
   int i = blockIdx.z;
@@ -123,7 +123,9 @@ __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T
     T curl = (jy && kz) ? chy - chz : T(0);
     T epsc = T(0.5) * (eps0 + p.eps[ce_ + (long long)ny * nz]);
     T E = e_update(p, p.Ex[x], epsc, curl);
-    p.Ex[x] = inject(p, src, 0, i, j, k, E, epsc, c);
+    E = inject(p, src, 0, i, j, k, E, epsc, c);
+    p.Ex[x] = E;
+    if (tape) tape[x] = E;
   }
   if (j < ny - 1) {  // Ey (nx, ny-1, nz)
     long long y = ((long long)i * (ny - 1) + j) * nz + k;
@@ -142,7 +144,9 @@ __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T
     T curl = (kz && ix) ? chz - chx : T(0);
     T epsc = T(0.5) * (eps0 + p.eps[ce_ + nz]);
     T E = e_update(p, p.Ey[y], epsc, curl);
-    p.Ey[y] = inject(p, src, 1, i, j, k, E, epsc, c);
+    E = inject(p, src, 1, i, j, k, E, epsc, c);
+    p.Ey[y] = E;
+    if (tape) tape[(long long)(nx - 1) * ny * nz + y] = E;
   }
   if (k < nz - 1) {  // Ez (nx, ny, nz-1)
     long long z = ((long long)i * ny + j) * (nz - 1) + k;
@@ -161,7 +165,9 @@ __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T
     T curl = (ix && jy) ? chx - chy : T(0);
     T epsc = T(0.5) * (eps0 + p.eps[ce_ + 1]);
     T E = e_update(p, p.Ez[z], epsc, curl);
-    p.Ez[z] = inject(p, src, 2, i, j, k, E, epsc, c);
+    E = inject(p, src, 2, i, j, k, E, epsc, c);
+    p.Ez[z] = E;
+    if (tape) tape[(long long)(nx - 1) * ny * nz + (long long)nx * (ny - 1) * nz + z] = E;
   }
 }
 
@@ -193,10 +199,13 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
   dim3 grd((p.nz + BZ - 1) / BZ, (p.ny + BYY - 1) / BYY, p.nx);
   int S = mp.stage_steps > 0 ? mp.stage_steps : 1;
   int filled = 0, first = step0;
+  const long long e_cells = (long long)(p.nx - 1) * p.ny * p.nz + (long long)p.nx * (p.ny - 1) * p.nz +
+                            (long long)p.nx * p.ny * (p.nz - 1);
   for (int s_ = 0; s_ < k; ++s_) {
     hipLaunchKernelGGL(h3_kernel<T>, grd, blk, 0, s, p);
     T c = sv.kind ? static_cast<T>(coef[s_]) : T(0);
-    hipLaunchKernelGGL(e3_kernel<T>, grd, blk, 0, s, p, sv, c);
+    T* slot = d->tape ? static_cast<T*>(d->tape) + (long long)s_ * e_cells : nullptr;
+    hipLaunchKernelGGL(e3_kernel<T>, grd, blk, 0, s, p, sv, c, slot);
     int rc = check_launch("yee3d step");
     if (rc) return rc;
     if (mp.n_mons) {

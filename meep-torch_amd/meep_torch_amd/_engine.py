@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -62,7 +62,13 @@ class Yee3D(ctypes.Structure):
                 ("h", PmlAxis * 3), ("e", PmlAxis * 3),
                 ("Ex", c_void_p), ("Ey", c_void_p), ("Ez", c_void_p),
                 ("Hx", c_void_p), ("Hy", c_void_p), ("Hz", c_void_p),
-                ("psi", c_void_p * 12)]
+                ("psi", c_void_p * 12), ("tape", c_void_p)]
+
+
+class Yee3DAdj(ctypes.Structure):
+    _fields_ = [("aEx", c_void_p), ("aEy", c_void_p), ("aEz", c_void_p), ("aHx", c_void_p),
+                ("aHy", c_void_p), ("aHz", c_void_p), ("apsi", c_void_p * 12), ("g", c_void_p * 12),
+                ("geps", c_void_p * 3), ("eps_grad", c_void_p)]
 
 
 class Source3D(ctypes.Structure):
@@ -111,6 +117,9 @@ def lib():
     L.mt_tm2d_adjoint.restype = c_int
     L.mt_tm2d_adjoint.argtypes = [P(TM2D), P(TM2DAdj), c_void_p, P(Monitor), c_int, c_int, c_int,
                                   c_void_p]
+    L.mt_yee3d_adjoint.restype = c_int
+    L.mt_yee3d_adjoint.argtypes = [P(Yee3D), P(Yee3DAdj), c_void_p, P(Monitor), c_int, c_int, c_int,
+                                   c_void_p]
     if L.mt_abi_version() != ABI_VERSION:
         raise RuntimeError("meep_torch_amd: libmeepfdtd.so ABI mismatch; rebuild")
     _lib = L

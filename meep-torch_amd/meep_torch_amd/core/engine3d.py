@@ -60,8 +60,13 @@ def _axis(c, key):
     return E.pml_axis(b, a, c.shells[key] if c.shells else None)
 
 
-def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0):
-    """Advance the field dict n_steps in place; returns monitor outputs."""
+def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=None, reuse=None, raw=False):
+    """Advance the field dict n_steps in place; returns monitor outputs.
+
+    tape: device buffer of n_steps slots [Ex | Ey | Ez] receiving E after
+    every step; reuse: raw outputs of a previous call whose accumulators keep
+    summing (one sequential sum across calls); raw: return the raw
+    (monitor, kind, buffer) outputs instead of the reshaped ones."""
     from .engine2d import finish_outputs, monitor_structs
     L = E.lib()
     if c.cpml is None:
@@ -86,13 +91,14 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0):
         setattr(d, comp, fields[comp].data_ptr())
     for q, name in enumerate(PSI_NAMES):
         d.psi[q] = fields[name].data_ptr()
+    d.tape = tape.data_ptr() if tape is not None else None
     src_t = source.src if source is not None else None
     s, wkeep = _source(src_t, fields, lean, dtype, device)
     coef = None
     if s.kind:
         vals = [c.dt * source.amplitude(t) for t in range(step0, step0 + n_steps)]
         coef = (ctypes.c_double * max(1, n_steps))(*vals)
-    arr, outs, keep = monitor_structs(fields, list(monitors), n_steps, dtype, device)
+    arr, outs, keep = monitor_structs(fields, list(monitors), n_steps, dtype, device, reuse=reuse)
     n = len(monitors)
     S, stage, nbytes = stage_alloc(L, arr, n, code, device)
     rc = L.mt_yee3d_run(ctypes.byref(d), ctypes.byref(s), coef, arr, n,
@@ -100,4 +106,4 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0):
                         E.stream_handle())
     E.check(rc, "mt_yee3d_run")
     del eps, wkeep, keep
-    return finish_outputs(outs, fields)
+    return outs if raw else finish_outputs(outs, fields)

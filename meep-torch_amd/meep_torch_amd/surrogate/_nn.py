@@ -47,6 +47,7 @@ def _p(t):
 
 
 TARGET_WG = 512     # workgroups one GEMM launch aims for (2 per CU)
+PROFILE = None      # list -> every GEMM appends (flops, start event, end event)
 MIN_TILES = 8       # fewest 32-deep reduction tiles per split
 
 
@@ -84,7 +85,13 @@ def gemm(A, a, B, b, C, c, M, N, K, *, batch=1, R=1, alpha=1.0, beta=0.0, ws=Non
         if ws is None:
             raise ValueError("gemm: a split reduction needs a workspace")
         g.work = _p(ws.get(splits * batch * M * N))
+    if PROFILE is not None:
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        ev[0].record()
     E.check(lib().mt_gemm_bf16(ctypes.byref(g), E.stream_handle()), "mt_gemm_bf16")
+    if PROFILE is not None:
+        ev[1].record()
+        PROFILE.append((2.0 * M * N * K * R * batch, ev[0], ev[1]))
     return C
 
 
