@@ -15,6 +15,7 @@ training, then plain-FDTD and surrogate-assisted Adam loops from one start.
 Prints one JSON line on stdout (progress goes to stderr).
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -30,6 +31,26 @@ import torch  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 ALG_BYTES = {"c1": 32.0, "c2": 29.0, "c3": 58.0}   # FP32 algorithmic bytes per cell-update
+
+
+DOMINANT = {
+    "c1": "tm_tiled interior-tile launch (temporally blocked TM steps), HIP events on its stream",
+    "c2": "te_tiled interior-tile launch (temporally blocked TE steps), HIP events on its stream",
+    "c3": "one 3D Yee step (fused kernel, or H + E kernels), HIP events on its stream",
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def _traffic(workload, cell_steps_per_launch):
+    """HBM bytes per launch of the dominant kernel from the committed PMC
+    measurement (FETCH_SIZE x fetch scale + WRITE_SIZE x write scale, per
+    cell-step), scaled to this launch size; None if not measured."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)[workload]
+    except (OSError, KeyError, ValueError):
+        return None
+    return t["bytes_per_cell_step"] * cell_steps_per_launch
 
 
 def log(*a):
@@ -196,6 +217,9 @@ def main():
     ap.add_argument("--c5-epochs", type=int, default=20)
     ap.add_argument("--c5-iters", type=int, default=500, help="Adam iterations of each design loop")
     ap.add_argument("--c5-yee-steps", type=int, default=2500)
+    ap.add_argument("--c3-variant", type=int, default=None, help="3D stepper: 0 two-pass, 1-3 fused")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 1 GiB float32 tensor three times (reference for PMC byte counters)")
     ap.add_argument("--c4-variants", type=int, default=64)
     ap.add_argument("--yee-steps", type=int, default=0, help="override Yee steps per run (tuning)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -204,10 +228,21 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the FDTD path shards by spatial slabs across GPUs, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        x = torch.ones(256 * 1024 * 1024, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("[bench] copied 3 x 1 GiB")
+        return
     if args.workload == "c4":
         return bench_c4(args)
     if args.workload == "c5":
         return bench_c5(args)
+    if args.c3_variant is not None:
+        from meep_torch_amd.core import engine3d
+        engine3d.DEFAULT_VARIANT = args.c3_variant
     builder = {"c1": build_c1, "c2": build_c2, "c3": build_c3}[args.workload]
     run, cells, ysteps, _ = builder(args.yee_steps or None)
     log(f"[bench] {args.workload}: {cells} cells x {ysteps} Yee steps per bench step")
@@ -220,6 +255,10 @@ def main():
         log("[bench] autotuner ms per variant (-1 = two-pass):",
             {k: round(v, 3) for k, v in tune.tune.last.items()}, "chosen:", dict(tune._CACHE))
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    from meep_torch_amd import _engine as E
+    L = E.lib()
+    L.mt_kernel_timing_read(None, None, None)
+    L.mt_kernel_timing_enable(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev0.record()
@@ -228,13 +267,24 @@ def main():
     ev1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    L.mt_kernel_timing_enable(0)
+    k_ms, k_n, k_cells = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_longlong()
+    E.check(L.mt_kernel_timing_read(ctypes.byref(k_ms), ctypes.byref(k_n), ctypes.byref(k_cells)),
+            "mt_kernel_timing_read")
     gpu_ms = ev0.elapsed_time(ev1)
-    log(f"[bench] timed {args.steps} runs: wall {wall:.3f} s, events {gpu_ms / 1e3:.3f} s")
+    log(f"[bench] timed {args.steps} runs: wall {wall:.3f} s, events {gpu_ms / 1e3:.3f} s; dominant kernel "
+        f"{k_n.value} launches, {k_ms.value:.1f} ms, {k_cells.value} cell-steps")
     updates = cells * ysteps * args.steps
     value = updates / wall
     alg = ALG_BYTES[args.workload]
-    per_step_s = gpu_ms / 1e3 / (ysteps * args.steps)
-    achieved = alg * cells / per_step_s / 1e9
+    if k_n.value:
+        per_launch_cells = k_cells.value / k_n.value
+        per_launch_s = k_ms.value * 1e-3 / k_n.value
+    else:   # no timed launches (should not happen): whole-step events
+        per_launch_cells = cells
+        per_launch_s = gpu_ms / 1e3 / (ysteps * args.steps)
+    achieved = alg * per_launch_cells / per_launch_s / 1e9
+    traffic = _traffic(args.workload, per_launch_cells)
     line = {
         "metric": "Yee cell-updates/s (FP32) with % of MI355X HBM roofline",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -246,9 +296,11 @@ def main():
                                 }[args.workload],
                    "yee_steps_per_step": ysteps, "cells": cells},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "one Yee step (all kernels of the step, HIP events)",
-                     "alg_bytes_per_cell_update": alg},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": DOMINANT[args.workload],
+                     "launch_avg_us": per_launch_s * 1e6, "cell_steps_per_launch": per_launch_cells,
+                     "alg_bytes_per_cell_update": alg,
+                     "whole_step_frac": alg * cells / (gpu_ms / 1e3 / (ysteps * args.steps)) / 1e9 / HBM_PEAK_GBS},
     }
     if not args.no_cpu_baseline and args.workload == "c2":
         log("[bench] cpu baseline (oracle) ...")

@@ -103,6 +103,14 @@ typedef struct {
    * slot s = tape + s * (|Ex| + |Ey| + |Ez|) as [Ex | Ey | Ez]
    * (reference shapes), for the adjoint's eps gradient */
   void* tape;
+  /* variant 0: two passes per step (H kernel, E kernel); 4: the same two
+   * passes with each thread streaming a (j, k) column along x; 1-3: one
+   * fused pass per step streaming along x (tile rows 8, 4, 16), which needs alt:
+   * second buffers of the 6 fields then the 12 psi arrays (same shapes;
+   * the psi ones zero-filled), used ping-pong; the state ends in the
+   * primary buffers either way */
+  int variant, pad2_;
+  void* alt[18];
 } mt_yee3d;
 
 /* 3D soft source: kind 1 = point at (i0, j0, k0), kind 2 = mask given as dense
@@ -268,6 +276,16 @@ int mt_tm2d_batch_adjoint(mt_batch* b, const mt_tm2d_adj* adj, void* const* tape
                           const int* mon_off, int step0, int k, void* stream);
 
 void mt_batch_destroy(mt_batch* b);
+
+/* Kernel timing hook.  No counterpart in the reference: a measurement aid
+ * for benchmarks.  While enabled, every launch of the dominant stencil
+ * kernel (2D: the temporally blocked interior-tile launch; 3D: one full
+ * step) is bracketed by HIP events on its own stream and its cell-steps
+ * are counted.  mt_kernel_timing_read waits for those events, returns the
+ * summed duration, launch count and cell-steps, and clears the record.
+ * Host-thread-unsafe: use from one thread. */
+int mt_kernel_timing_enable(int on);
+int mt_kernel_timing_read(double* total_ms, long long* launches, long long* cell_steps);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* mt_last_error(void);

@@ -29,6 +29,23 @@ inline Pml<T> pml_of(const mt_pml_axis& ax) {
 
 int check_axis(const mt_pml_axis& ax, int n, const char* name);
 
+// a / b for the constant grid step b with y = RN(1/b): q0 = a y, then one
+// FMA correction with the exact remainder a - q0 b gives the correctly
+// rounded quotient (Markstein's theorem: y within half an ulp of 1/b, q0
+// within one ulp of a/b, exact remainder; checked exhaustively against IEEE
+// division over |a| in [2^-100, 2^100] for several b).  Outside that range
+// the true division is used.
+template <typename T>
+__device__ __forceinline__ T div_c(T a, T b, T y) {
+  T aa = a < T(0) ? -a : a;
+  if (aa >= T(0x1p-100) && aa <= T(0x1p100)) {
+    T q0 = a * y;
+    T r = fma(-q0, b, a);
+    return fma(r, y, q0);
+  }
+  return a / b;
+}
+
 // Monitor support shared by the 2D and 3D steppers (dft.hip).
 struct MonitorPlan {
   const mt_monitor* mons;
@@ -39,6 +56,10 @@ struct MonitorPlan {
   int dtype;
 };
 int monitors_validate(const MonitorPlan& mp);
+// Kernel timing hook (mt_kernel_timing_enable): brackets launches with HIP
+// events on their stream when enabled; no-ops otherwise.
+void timing_begin(hipStream_t s);
+void timing_end(hipStream_t s, long long cell_steps);
 // Gather the values of every term for step slot `slot` (after the step).
 int monitors_sample(const MonitorPlan& mp, int slot, int t_rel, hipStream_t s);
 // Fold `n_filled` staged steps whose first global step index is t0 into the accumulators.

@@ -11,6 +11,8 @@
 // round like the reference's.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <cmath>
 #include <string>
 
@@ -294,7 +296,76 @@ int monitors_flush(const MonitorPlan& mp, int n_filled, int t0, double dt, hipSt
                             : flush_t<float>(mp, n_filled, t0, dt, s);
 }
 
+namespace {
+struct TimingRec {
+  hipEvent_t a, b;
+  long long cells;  // -1 while open
+};
+std::vector<TimingRec> g_rec;
+std::vector<hipEvent_t> g_pool;
+bool g_timing = false;
+constexpr size_t kMaxTimingRecs = 1 << 16;
+
+hipEvent_t take_event() {
+  if (!g_pool.empty()) {
+    hipEvent_t e = g_pool.back();
+    g_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+}  // namespace
+
+void timing_begin(hipStream_t s) {
+  if (!g_timing || g_rec.size() >= kMaxTimingRecs) return;
+  TimingRec r{take_event(), take_event(), -1};
+  if (!r.a || !r.b) return;
+  if (hipEventRecord(r.a, s) != hipSuccess) return;
+  g_rec.push_back(r);
+}
+
+void timing_end(hipStream_t s, long long cell_steps) {
+  if (!g_timing || g_rec.empty() || g_rec.back().cells != -1) return;
+  TimingRec& r = g_rec.back();
+  if (hipEventRecord(r.b, s) != hipSuccess) {
+    g_pool.push_back(r.a);
+    g_pool.push_back(r.b);
+    g_rec.pop_back();
+    return;
+  }
+  r.cells = cell_steps;
+}
+
 }  // namespace mt
+
+extern "C" int mt_kernel_timing_enable(int on) {
+  mt::g_timing = on != 0;
+  return MT_OK;
+}
+
+extern "C" int mt_kernel_timing_read(double* total_ms, long long* launches, long long* cell_steps) {
+  double ms = 0.0;
+  long long n = 0, cells = 0;
+  for (auto& r : mt::g_rec) {
+    if (r.cells >= 0) {
+      float t = 0.f;
+      if (hipEventSynchronize(r.b) != hipSuccess || hipEventElapsedTime(&t, r.a, r.b) != hipSuccess)
+        return mt::fail(MT_EHIP, "kernel timing: event query failed");
+      ms += t;
+      ++n;
+      cells += r.cells;
+    }
+    mt::g_pool.push_back(r.a);
+    mt::g_pool.push_back(r.b);
+  }
+  mt::g_rec.clear();
+  if (total_ms) *total_ms = ms;
+  if (launches) *launches = n;
+  if (cell_steps) *cell_steps = cells;
+  return MT_OK;
+}
 
 extern "C" long long mt_stage_bytes(const mt_monitor* mons, int n_mons, int stage_steps, int dtype) {
   long long es = dtype == MT_F64 ? 8 : 4;
@@ -311,4 +382,4 @@ extern "C" long long mt_stage_bytes(const mt_monitor* mons, int n_mons, int stag
 
 extern "C" const char* mt_last_error(void) { return mt::g_err.c_str(); }
 
-extern "C" int mt_abi_version(void) { return 5; }
+extern "C" int mt_abi_version(void) { return 7; }

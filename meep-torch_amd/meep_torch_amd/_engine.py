@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 5
+ABI_VERSION = 7
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -62,7 +62,8 @@ class Yee3D(ctypes.Structure):
                 ("h", PmlAxis * 3), ("e", PmlAxis * 3),
                 ("Ex", c_void_p), ("Ey", c_void_p), ("Ez", c_void_p),
                 ("Hx", c_void_p), ("Hy", c_void_p), ("Hz", c_void_p),
-                ("psi", c_void_p * 12), ("tape", c_void_p)]
+                ("psi", c_void_p * 12), ("tape", c_void_p), ("variant", c_int), ("pad2_", c_int),
+                ("alt", c_void_p * 18)]
 
 
 class Yee3DAdj(ctypes.Structure):
@@ -117,6 +118,10 @@ def lib():
     L.mt_tm2d_adjoint.restype = c_int
     L.mt_tm2d_adjoint.argtypes = [P(TM2D), P(TM2DAdj), c_void_p, P(Monitor), c_int, c_int, c_int,
                                   c_void_p]
+    L.mt_kernel_timing_enable.restype = c_int
+    L.mt_kernel_timing_enable.argtypes = [c_int]
+    L.mt_kernel_timing_read.restype = c_int
+    L.mt_kernel_timing_read.argtypes = [P(c_double), P(c_ll), P(c_ll)]
     L.mt_yee3d_adjoint.restype = c_int
     L.mt_yee3d_adjoint.argtypes = [P(Yee3D), P(Yee3DAdj), c_void_p, P(Monitor), c_int, c_int, c_int,
                                    c_void_p]

@@ -14,6 +14,10 @@ from .stepper_3d import COMPS, PSI_NAMES
 
 _COMP = {"Ex": 0, "Ey": 1, "Ez": 2}
 
+# 0: two kernels per step; 4: two kernels streaming along x; 1/2/3: one
+# fused pass per step (8/4/16-row tiles)
+DEFAULT_VARIANT = 0
+
 
 def _source(src, fields, lean, dtype, device):
     s = E.Source3D()
@@ -60,13 +64,15 @@ def _axis(c, key):
     return E.pml_axis(b, a, c.shells[key] if c.shells else None)
 
 
-def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=None, reuse=None, raw=False):
+def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=None, reuse=None, raw=False,
+               variant=None):
     """Advance the field dict n_steps in place; returns monitor outputs.
 
     tape: device buffer of n_steps slots [Ex | Ey | Ez] receiving E after
     every step; reuse: raw outputs of a previous call whose accumulators keep
     summing (one sequential sum across calls); raw: return the raw
-    (monitor, kind, buffer) outputs instead of the reshaped ones."""
+    (monitor, kind, buffer) outputs instead of the reshaped ones.  variant:
+    0 two-pass, 1-3 fused streaming step (default: DEFAULT_VARIANT)."""
     from .engine2d import finish_outputs, monitor_structs
     L = E.lib()
     if c.cpml is None:
@@ -92,6 +98,13 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=N
     for q, name in enumerate(PSI_NAMES):
         d.psi[q] = fields[name].data_ptr()
     d.tape = tape.data_ptr() if tape is not None else None
+    v = DEFAULT_VARIANT if variant is None else int(variant)
+    alt = []
+    if 1 <= v <= 3 and n_steps > 0:
+        alt = [torch.empty_like(fields[f]) for f in COMPS] + [torch.zeros_like(fields[f]) for f in PSI_NAMES]
+        for q, t in enumerate(alt):
+            d.alt[q] = t.data_ptr()
+    d.variant = v if n_steps > 0 else 0
     src_t = source.src if source is not None else None
     s, wkeep = _source(src_t, fields, lean, dtype, device)
     coef = None
@@ -105,5 +118,5 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=N
                         E.ptr(stage) if stage is not None else None, nbytes, S, step0, n_steps,
                         E.stream_handle())
     E.check(rc, "mt_yee3d_run")
-    del eps, wkeep, keep
+    del eps, wkeep, keep, alt
     return outs if raw else finish_outputs(outs, fields)

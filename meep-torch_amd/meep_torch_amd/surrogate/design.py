@@ -56,13 +56,17 @@ def fdtd_loop(problem, iters=500, lr=0.02, theta0=None, log=None):
 
 
 def surrogate_loop(problem, net, iters=500, lr=0.02, theta0=None, verify_every=10, tol=0.02, data=None,
-                   finetune_steps=8, finetune_batch=32, finetune_lr=2e-4, log=None):
-    """Surrogate-assisted design.  At every verification the FDTD spectrum
-    and gradient of the current design are computed; the FDTD gradient is
-    used for that step, the sample is added to a fine-tuning buffer, and
-    the mode for the next iterations is 'surrogate' if the network's
-    spectrum error is <= tol, else 'fdtd' (every iteration an FDTD forward
-    + adjoint) until a later verification passes."""
+                   finetune_steps=8, finetune_batch=32, finetune_lr=3e-4, log=None):
+    """Surrogate-assisted design.  Every `verify_every` iterations the FDTD
+    forward + adjoint of the current design gives its true spectrum and
+    gradient; that gradient drives the step, and the network's spectrum
+    error there decides the mode of the following iterations: 'surrogate'
+    (network forward + input gradient) if the error is <= tol, else 'fdtd'
+    (an FDTD forward + adjoint every iteration) until a later verification
+    passes.  Every FDTD solve of the loop is also a labelled sample: the
+    network is fine-tuned on the samples gathered along the trajectory
+    (mixed with replay from its training set), so it learns the region the
+    design is moving through."""
     theta = (theta0 if theta0 is not None else start_design(problem)).clone()
     opt = _Adam(theta, lr)
     mode = "fdtd"
@@ -73,23 +77,21 @@ def surrogate_loop(problem, net, iters=500, lr=0.02, theta0=None, verify_every=1
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for it in range(iters):
-        if it % verify_every == 0:
+        verify = it % verify_every == 0
+        if verify or mode == "fdtd":
             T, fom, g = problem.solve_with_gradient(theta)
             with torch.no_grad():
                 rho = problem.project(theta)
-                err = float(spectrum_error(net.forward(rho[None]), T[None])[0])
-            stats["verifications"] += 1
-            stats["errors"].append(err)
-            mode = "surrogate" if err <= tol else "fdtd"
-            if err > tol:
-                stats["failed"] += 1
-                new_rho.append(rho)
-                new_T.append(T)
-                _finetune(net, new_rho, new_T, data, finetune_steps, finetune_batch, finetune_lr, gen)
-            stats["fdtd_iters"] += 1
-            hist.append(fom)
-        elif mode == "fdtd":
-            T, fom, g = problem.solve_with_gradient(theta)
+            if verify:
+                with torch.no_grad():
+                    err = float(spectrum_error(net.forward(rho[None]), T[None])[0])
+                stats["verifications"] += 1
+                stats["errors"].append(err)
+                mode = "surrogate" if err <= tol else "fdtd"
+                stats["failed"] += int(err > tol)
+            new_rho.append(rho)
+            new_T.append(T)
+            _finetune(net, new_rho, new_T, data, finetune_steps, finetune_batch, finetune_lr, gen)
             stats["fdtd_iters"] += 1
             hist.append(fom)
         else:
@@ -99,7 +101,7 @@ def surrogate_loop(problem, net, iters=500, lr=0.02, theta0=None, verify_every=1
             fom_s.backward()
             g = th.grad
             stats["surrogate_iters"] += 1
-            hist.append(float(fom_s))
+            hist.append(float(fom_s.detach()))
         opt.ascend(g)
         if log and (it + 1) % 50 == 0:
             log(f"[design/surrogate] iter {it + 1}: FOM {hist[-1]:.4f} mode {mode} "
@@ -112,8 +114,8 @@ def surrogate_loop(problem, net, iters=500, lr=0.02, theta0=None, verify_every=1
 
 
 def _finetune(net, new_rho, new_T, data, steps, batch, lr, gen):
-    """A few Adam steps on the verified samples mixed with replayed dataset
-    samples (so the network does not forget the training distribution)."""
+    """A few Adam steps on the trajectory samples mixed with replayed
+    training samples (so the network keeps the training distribution)."""
     R = torch.stack(new_rho)
     T = torch.stack(new_T)
     for _ in range(steps):

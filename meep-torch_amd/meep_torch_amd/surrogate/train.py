@@ -32,18 +32,32 @@ def spectrum_error(pred, T):
     return (pred - T).norm(dim=-1) / T.norm(dim=-1).clamp_min(1e-12)
 
 
-def mse_step(net, rho, T, lr):
+def mse_step(net, rho, T, lr, wd=0.0, clip=1.0):
+    """One Adam step on the mean squared spectrum error; the gradient is
+    rescaled to global norm <= clip first."""
     pred = net.forward(rho)
     diff = pred - T
     dout = diff * (2.0 / diff.numel())
     net.backward(dout, param_grads=True, input_grad=False)
-    net.adam_step(lr)
+    if clip:
+        gn = net.grads.norm()
+        net.grads.mul_(torch.clamp(clip / (gn + 1e-12), max=1.0))
+    net.adam_step(lr, wd=wd)
     return float((diff * diff).mean())
 
 
-def train(net, rho, T, epochs=20, batch=64, lr=2e-3, lr_min=5e-5, val=None, seed=0, log=None):
-    """Adam on mean squared spectrum error, cosine-decayed learning rate.
-    Returns the per-epoch history."""
+def flip_half(rho, gen):
+    """Mirror a random half of the batch across the guide axis (y): the
+    transmission problem is symmetric under y -> n-1-y (guides, source and
+    monitor are centred), so T is unchanged and the batch gains variety."""
+    m = torch.rand(rho.shape[0], generator=gen, device=rho.device) < 0.5
+    return torch.where(m.view(-1, 1, 1), rho.flip(2), rho)
+
+
+def train(net, rho, T, epochs=20, batch=64, lr=1e-3, lr_min=3e-5, val=None, seed=0, log=None, wd=1e-5,
+          augment=True):
+    """Adam on mean squared spectrum error, cosine-decayed learning rate, L2
+    weight decay wd, y-mirror augmentation.  Returns the per-epoch history."""
     N = rho.shape[0]
     g = torch.Generator(device=rho.device).manual_seed(int(seed))
     steps = epochs * (N // batch)
@@ -54,7 +68,8 @@ def train(net, rho, T, epochs=20, batch=64, lr=2e-3, lr_min=5e-5, val=None, seed
         for s in range(0, N - batch + 1, batch):
             idx = perm[s:s + batch]
             lr_k = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * k / max(1, steps)))
-            tot += mse_step(net, rho[idx], T[idx], lr_k)
+            xb = flip_half(rho[idx], g) if augment else rho[idx]
+            tot += mse_step(net, xb, T[idx], lr_k, wd)
             k += 1
         rec = {"epoch": ep + 1, "train_mse": tot / max(1, N // batch)}
         if val is not None:
