@@ -391,6 +391,7 @@ def bench_c5(args):
         "plain_fdtd_iters_per_s": rf["iters_per_s"], "speedup_vs_plain": rs["iters_per_s"] / rf["iters_per_s"],
         "final_fom_plain": rf["final_fom"], "final_fom_surrogate": rs["final_fom"], "fom_gap": gap,
         "verifications": rs["verifications"], "verifications_failed": rs["failed"],
+        "rollbacks": rs["rollbacks"], "verified_errors": [round(e, 4) for e in rs["errors"]],
         "fdtd_iterations_in_surrogate_loop": rs["fdtd_iters"], "setup_s": setup_s,
         "val_err_median": hist[-1].get("val_err_median"), "val_err_p90": hist[-1].get("val_err_p90"),
         "roofline": roof}), flush=True)

@@ -66,13 +66,21 @@ def surrogate_loop(problem, net, iters=500, lr=0.02, theta0=None, verify_every=1
     passes.  Every FDTD solve of the loop is also a labelled sample: the
     network is fine-tuned on the samples gathered along the trajectory
     (mixed with replay from its training set), so it learns the region the
-    design is moving through."""
+    design is moving through.  A verification that follows surrogate steps
+    and finds the FDTD figure of merit below the last verified one rolls the
+    design back to that point (and restarts the optimizer moments):
+    surrogate steps never lose verified progress.  Segments driven by FDTD
+    gradients are never rolled back, so the loop is at worst the plain FDTD
+    loop plus the fine-tuning cost."""
     theta = (theta0 if theta0 is not None else start_design(problem)).clone()
     opt = _Adam(theta, lr)
     mode = "fdtd"
     new_rho, new_T = [], []
     gen = torch.Generator(device=problem.device).manual_seed(1234)
-    stats = {"verifications": 0, "failed": 0, "fdtd_iters": 0, "surrogate_iters": 0, "errors": []}
+    stats = {"verifications": 0, "failed": 0, "fdtd_iters": 0, "surrogate_iters": 0, "errors": [],
+             "rollbacks": 0}
+    best_theta, best_fom = theta.clone(), None
+    since = 0   # surrogate steps since the last verification
     hist = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -82,6 +90,7 @@ def surrogate_loop(problem, net, iters=500, lr=0.02, theta0=None, verify_every=1
             T, fom, g = problem.solve_with_gradient(theta)
             with torch.no_grad():
                 rho = problem.project(theta)
+            rolled = False
             if verify:
                 with torch.no_grad():
                     err = float(spectrum_error(net.forward(rho[None]), T[None])[0])
@@ -89,11 +98,25 @@ def surrogate_loop(problem, net, iters=500, lr=0.02, theta0=None, verify_every=1
                 stats["errors"].append(err)
                 mode = "surrogate" if err <= tol else "fdtd"
                 stats["failed"] += int(err > tol)
+                if since and best_fom is not None and fom < best_fom:
+                    theta.copy_(best_theta)
+                    opt.m.zero_()
+                    opt.v.zero_()
+                    opt.t = 0
+                    mode = "fdtd"
+                    rolled = True
+                    stats["rollbacks"] += 1
+                else:
+                    best_theta.copy_(theta)
+                    best_fom = fom
+                since = 0
             new_rho.append(rho)
             new_T.append(T)
             _finetune(net, new_rho, new_T, data, finetune_steps, finetune_batch, finetune_lr, gen)
             stats["fdtd_iters"] += 1
             hist.append(fom)
+            if rolled:
+                continue
         else:
             th = theta.detach().clone().requires_grad_(True)
             Ts = predict(net, problem.project(th)[None])[0]
@@ -101,6 +124,7 @@ def surrogate_loop(problem, net, iters=500, lr=0.02, theta0=None, verify_every=1
             fom_s.backward()
             g = th.grad
             stats["surrogate_iters"] += 1
+            since += 1
             hist.append(float(fom_s.detach()))
         opt.ascend(g)
         if log and (it + 1) % 50 == 0:

@@ -92,12 +92,14 @@ class TransmissionProblem:
             self._ref = self.monitor.flux(acc, self.dx).detach().clone()
         return self._ref
 
-    def transmission(self, rho_batch, batch=64):
+    def transmission(self, rho_batch, batch=64, log=None):
         """T (count, nfreq) for a stack of projected densities (no gradient)."""
         ref = self.reference_flux()
         out = []
         with torch.no_grad():
             for s in range(0, rho_batch.shape[0], batch):
+                if log and s and s % (batch * 64) == 0:
+                    log(f"[surrogate] {s} / {rho_batch.shape[0]} solves")
                 chunk = rho_batch[s:s + batch]
                 r = self.runner(batch, False)
                 eps = [self.eps_of(x) for x in chunk]

@@ -20,7 +20,7 @@ def generate_dataset(problem, count, seed=0, batch=64, log=None):
     theta = sample_designs(count, seed, problem.d, problem.device)
     with torch.no_grad():
         rho = problem.project(theta)
-    T = problem.transmission(rho, batch=batch)
+    T = problem.transmission(rho, batch=batch, log=log)
     torch.cuda.synchronize()
     if log:
         log(f"[surrogate] dataset: {count} FDTD solves in {time.perf_counter() - t0:.1f} s")
