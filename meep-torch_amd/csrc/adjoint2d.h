@@ -98,13 +98,29 @@ __device__ __forceinline__ void adj_c_cell(const AdjK<T>& p, int i, int j) {
   p.aEz[(long long)i * ny + j] = (gxm - gx0) / p.dx + (gym - gy0) / p.dx;
 }
 
-// Monitor cotangent of element e of term q at global step t.
+// Adds the cotangent vb of element e of monitor term tm into the adjoint
+// array(s) it samples (half to each of the two averaged arrays).
+template <typename T>
+__device__ __forceinline__ void adj_inject_add(const mt_monitor& m, const mt_dft_term& tm, int e, T vb) {
+  int i0 = e / m.n1, i1 = e - i0 * m.n1;
+  long long off = (long long)i0 * tm.s0 + (long long)i1 * tm.s1;
+  T* a = const_cast<T*>(static_cast<const T*>(tm.a));
+  if (tm.b) {
+    T* b = const_cast<T*>(static_cast<const T*>(tm.b));
+    T h = T(0.5) * vb;
+    a[off] = a[off] + h;
+    b[off] = b[off] + h;
+  } else {
+    a[off] = a[off] + vb;
+  }
+}
+
+// Monitor cotangent of element e of term q at global step t (one thread
+// sums every frequency).
 template <typename T>
 __device__ __forceinline__ void adj_inject_elem(const mt_monitor& m, int q, int e, int t, double dt_d, int t_rel) {
   int n = m.n0 * m.n1;
   const mt_dft_term& tm = m.term[q];
-  int i0 = e / m.n1, i1 = e - i0 * m.n1;
-  long long off = (long long)i0 * tm.s0 + (long long)i1 * tm.s1;
   T vb;
   if (m.kind == 1) {
     vb = static_cast<const T*>(m.acc)[t_rel];
@@ -122,14 +138,59 @@ __device__ __forceinline__ void adj_inject_elem(const mt_monitor& m, int q, int 
       vb = vb + (dt * cos(ph)) * re + (-(dt * sin(ph))) * im;
     }
   }
-  T* a = const_cast<T*>(static_cast<const T*>(tm.a));
-  if (tm.b) {
-    T* b = const_cast<T*>(static_cast<const T*>(tm.b));
-    T h = T(0.5) * vb;
-    a[off] = a[off] + h;
-    b[off] = b[off] + h;
-  } else {
-    a[off] = a[off] + vb;
+  adj_inject_add<T>(m, tm, e, vb);
+}
+
+// Block-cooperative injection: a block of kInjE * kInjS threads owns the
+// monitor elements [e0, e0 + kInjE).  The phase factors dt cos(w t),
+// -dt sin(w t) of every (term, frequency) are evaluated once per block into
+// LDS (not once per element), each of the kInjS lane groups sums every
+// kInjS-th frequency, and lane group 0 adds the partial sums in group order.
+// Monitors with more than kInjTab (term, frequency) pairs, and probe stacks,
+// take the one-thread-per-element form.
+constexpr int kInjE = 32, kInjS = 8, kInjTab = 1024;
+
+template <typename T>
+__device__ void adj_inject_block(const mt_monitor& m, int e0, int t, double dt_d, int t_rel) {
+  __shared__ T cs[2 * kInjTab];
+  __shared__ T part[kInjS][kInjE];
+  const int n = m.n0 * m.n1;
+  const int el = threadIdx.x % kInjE, sl = threadIdx.x / kInjE;
+  const int e = e0 + el;
+  const int nq = m.n_terms, nf = m.nfreq;
+  if (m.kind == 1 || nq * nf > kInjTab) {   // uniform over the block
+    if (sl == 0 && e < n)
+      for (int q = 0; q < nq; ++q) adj_inject_elem<T>(m, q, e, t, dt_d, t_rel);
+    return;
+  }
+  const T dt = static_cast<T>(dt_d);
+  const T* om = static_cast<const T*>(m.omega);
+  const T* ac = static_cast<const T*>(m.acc);
+  for (int x = threadIdx.x; x < nq * nf; x += kInjE * kInjS) {
+    int q = x / nf, f = x - q * nf;
+    double tt = ((double)t + (m.term[q].t_half ? 0.5 : 1.0)) * dt_d;
+    T ph = om[f] * static_cast<T>(tt);
+    cs[2 * x] = dt * cos(ph);
+    cs[2 * x + 1] = -(dt * sin(ph));
+  }
+  __syncthreads();
+  for (int q = 0; q < nq; ++q) {
+    T v = T(0);
+    if (e < n)
+      for (int f = sl; f < nf; f += kInjS) {
+        const T* c = cs + 2 * (q * nf + f);
+        T re = ac[((long long)f * 2 * nq + 2 * q) * n + e];
+        T im = ac[((long long)f * 2 * nq + 2 * q + 1) * n + e];
+        v = v + c[0] * re + c[1] * im;
+      }
+    part[sl][el] = v;
+    __syncthreads();
+    if (sl == 0 && e < n) {
+      T vb = part[0][el];
+      for (int g = 1; g < kInjS; ++g) vb = vb + part[g][el];
+      adj_inject_add<T>(m, m.term[q], e, vb);
+    }
+    __syncthreads();
   }
 }
 

@@ -47,11 +47,8 @@ __global__ __launch_bounds__(ABX* ABY) void adj_c_kernel(AdjK<T> p) {
 }
 
 template <typename T>
-__global__ void adj_inject_kernel(mt_monitor m, int t, double dt_d, int t_rel) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  int q = blockIdx.y;
-  if (e >= m.n0 * m.n1 || q >= m.n_terms) return;
-  adj_inject_elem<T>(m, q, e, t, dt_d, t_rel);
+__global__ __launch_bounds__(kInjE* kInjS) void adj_inject_kernel(mt_monitor m, int t, double dt_d, int t_rel) {
+  adj_inject_block<T>(m, blockIdx.x * kInjE, t, dt_d, t_rel);
 }
 
 template <typename T>
@@ -67,8 +64,8 @@ int adj_t(const mt_tm2d* d, const mt_tm2d_adj* a, const void* tape, const mt_mon
       const mt_monitor& mm = mons[m];
       int n = mm.n0 * mm.n1;
       if (n == 0) continue;
-      dim3 g((n + 255) / 256, mm.n_terms);
-      hipLaunchKernelGGL(adj_inject_kernel<T>, g, dim3(256), 0, s, mm, t, d->dt, s_);
+      hipLaunchKernelGGL(adj_inject_kernel<T>, dim3((n + kInjE - 1) / kInjE), dim3(kInjE * kInjS), 0, s, mm, t,
+                         d->dt, s_);
     }
     hipLaunchKernelGGL(adj_a_kernel<T>, grd, blk, 0, s, p, tp + (long long)s_ * cells);
     hipLaunchKernelGGL(adj_b_kernel<T>, grd, blk, 0, s, p);

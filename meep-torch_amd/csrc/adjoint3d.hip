@@ -195,11 +195,8 @@ __global__ __launch_bounds__(ABZ* ABY) void adj3_c(A3<T> p) {
 }
 
 template <typename T>
-__global__ void adj3_inject(mt_monitor m, int t, double dt_d) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  int q = blockIdx.y;
-  if (e >= m.n0 * m.n1 || q >= m.n_terms) return;
-  adj_inject_elem<T>(m, q, e, t, dt_d, 0);
+__global__ __launch_bounds__(kInjE* kInjS) void adj3_inject(mt_monitor m, int t, double dt_d) {
+  adj_inject_block<T>(m, blockIdx.x * kInjE, t, dt_d, 0);
 }
 
 template <typename T>
@@ -238,7 +235,7 @@ int adj3_t(const mt_yee3d* d, const mt_yee3d_adj* a, const void* tape, const mt_
       const mt_monitor& mm = mons[m];
       const int n = mm.n0 * mm.n1;
       if (n == 0) continue;
-      hipLaunchKernelGGL(adj3_inject<T>, dim3((n + 255) / 256, mm.n_terms), dim3(256), 0, s, mm, t, d->dt);
+      hipLaunchKernelGGL(adj3_inject<T>, dim3((n + kInjE - 1) / kInjE), dim3(kInjE * kInjS), 0, s, mm, t, d->dt);
     }
     hipLaunchKernelGGL(adj3_a<T>, grd, blk, 0, s, p, tp + (long long)s_ * slot, tp + (long long)(s_ + 1) * slot);
     hipLaunchKernelGGL(adj3_b<T>, grd, blk, 0, s, p);

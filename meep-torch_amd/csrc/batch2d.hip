@@ -94,9 +94,8 @@ __global__ __launch_bounds__(kCellBlk) void adj_batch_inject(const mt_monitor* i
     else hi = mid;
   }
   const mt_monitor& m = inj[lo];
-  int e = (b - pre[lo]) * kCellBlk + threadIdx.x;
-  if (m.n_terms == 0 || e >= m.n0 * m.n1) return;
-  for (int q = 0; q < m.n_terms; ++q) adj_inject_elem<T>(m, q, e, t, dts[lo], t_rel);
+  if (m.n_terms == 0) return;
+  adj_inject_block<T>(m, (b - pre[lo]) * kInjE, t, dts[lo], t_rel);
 }
 
 template <typename T>
@@ -236,7 +235,7 @@ int adjoint_t(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes, const mt_
       int blocks = 0;
       if (m < mon_off[v + 1] - mon_off[v]) {
         dst = mons[mon_off[v] + m];
-        blocks = (dst.n0 * dst.n1 + kCellBlk - 1) / kCellBlk;
+        blocks = (dst.n0 * dst.n1 + kInjE - 1) / kInjE;
       }
       ipre[(size_t)m * (nvar + 1) + v + 1] = ipre[(size_t)m * (nvar + 1) + v] + blocks;
     }
@@ -259,7 +258,7 @@ int adjoint_t(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes, const mt_
     for (int m = 0; m < max_m; ++m) {
       int nb = ipre[(size_t)m * (nvar + 1) + nvar];
       if (nb)
-        hipLaunchKernelGGL(adj_batch_inject<T>, dim3(nb), dim3(kCellBlk), 0, s, dinj + (size_t)m * nvar,
+        hipLaunchKernelGGL(adj_batch_inject<T>, dim3(nb), dim3(kInjE * kInjS), 0, s, dinj + (size_t)m * nvar,
                            dipre + (size_t)m * (nvar + 1), nvar, t, ddts, s_);
     }
     hipLaunchKernelGGL((adj_batch_cells<T, 0>), dim3(b->cell_blocks), dim3(kCellBlk), 0, s, dks, dtp, dblk, nvar, s_);
