@@ -32,46 +32,91 @@ struct K3 {
 
 enum { P_EZ_Y, P_EY_Z, P_EX_Z, P_EZ_X, P_EY_X, P_EX_Y, P_HZ_Y, P_HY_Z, P_HX_Z, P_HZ_X, P_HY_X, P_HX_Y };
 
-// d + psi with psi' = b psi + a d inside a shell (stepper_3d.py:368-370)
+// The CPML curl term d + psi' with psi' = b psi + a d inside a shell, d
+// outside (stepper_3d.py:368-370), split into its loads (ce_load) and its
+// arithmetic + psi store (ce_apply), so that a kernel can issue every load
+// of a cell before its first store: the field, psi and profile arrays never
+// alias, but the compiler cannot know that and would otherwise keep each
+// component's loads behind the previous component's stores.
 template <typename T>
-__device__ __forceinline__ T ce(const Pml<T>& pr, int idx, T* psi, long long q, T d) {
-  if (!pr.in(idx)) return d;
-  T ps = pr.b[idx] * psi[q] + pr.a[idx] * d;
+struct CeIn {
+  bool in;
+  T b, a, ps;
+};
+
+template <typename T>
+__device__ __forceinline__ CeIn<T> ce_load(const Pml<T>& pr, int idx, bool use, const T* psi, long long q) {
+  CeIn<T> c{use && pr.in(idx), T(0), T(0), T(0)};
+  if (c.in) {
+    c.b = pr.b[idx];
+    c.a = pr.a[idx];
+    c.ps = psi[q];
+  }
+  return c;
+}
+
+template <typename T>
+__device__ __forceinline__ T ce_apply(const CeIn<T>& c, T* psi, long long q, T d) {
+  if (!c.in) return d;
+  T ps = c.b * c.ps + c.a * d;
   psi[q] = ps;
   return d + ps;
 }
 
 template <typename T>
 __global__ __launch_bounds__(BZ* BYY) void h3_kernel(K3<T> p) {
-  int k = blockIdx.x * BZ + threadIdx.x;
-  int j = blockIdx.y * BYY + threadIdx.y;
-  int i = blockIdx.z;
+  const int k = blockIdx.x * BZ + threadIdx.x;
+  const int j = blockIdx.y * BYY + threadIdx.y;
+  const int i = blockIdx.z;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
   if (j >= ny || k >= nz) return;
   const T dx = p.dx, dt = p.dt;
-  if (j < ny - 1 && k < nz - 1) {  // Hx (nx, ny-1, nz-1)
-    long long h = ((long long)i * (ny - 1) + j) * (nz - 1) + k;
-    long long ez = ((long long)i * ny + j) * (nz - 1) + k;
-    long long ey = ((long long)i * (ny - 1) + j) * nz + k;
-    T cy = ce(p.h[1], j, p.psi[P_EZ_Y], h, (p.Ez[ez + (nz - 1)] - p.Ez[ez]) / dx);
-    T cz = ce(p.h[2], k, p.psi[P_EY_Z], h, (p.Ey[ey + 1] - p.Ey[ey]) / dx);
-    p.Hx[h] = p.Hx[h] - dt * (cy - cz);
+  const bool fx = j < ny - 1 && k < nz - 1;                    // Hx (nx, ny-1, nz-1)
+  const bool fy = i < nx - 1 && k < nz - 1;                    // Hy (nx-1, ny, nz-1)
+  const bool fz = i < nx - 1 && j < ny - 1;                    // Hz (nx-1, ny-1, nz)
+  const long long ix = ((long long)i * (ny - 1) + j) * (nz - 1) + k;  // Hx
+  const long long iy = ((long long)i * ny + j) * (nz - 1) + k;        // Hy, Ez (i, j, k)
+  const long long iz = ((long long)i * (ny - 1) + j) * nz + k;        // Hz, Ey (i, j, k)
+  const long long ie = ((long long)i * ny + j) * nz + k;              // Ex (i, j, k)
+  // ---- loads
+  T ez0 = T(0), ezj = T(0), ezx = T(0), ey0 = T(0), eyk = T(0), eyx = T(0), ex0 = T(0), exk = T(0), exj = T(0);
+  T hx = T(0), hy = T(0), hz = T(0);
+  if (fx || fy) ez0 = p.Ez[iy];
+  if (fx || fz) ey0 = p.Ey[iz];
+  if (fy || fz) ex0 = p.Ex[ie];
+  if (fx) {
+    ezj = p.Ez[iy + (nz - 1)];
+    eyk = p.Ey[iz + 1];
+    hx = p.Hx[ix];
   }
-  if (i < nx - 1 && k < nz - 1) {  // Hy (nx-1, ny, nz-1)
-    long long h = ((long long)i * ny + j) * (nz - 1) + k;
-    long long ex = ((long long)i * ny + j) * nz + k;
-    long long ez = h;  // Ez (nx, ny, nz-1) shares the index for i < nx-1
-    T cz = ce(p.h[2], k, p.psi[P_EX_Z], h, (p.Ex[ex + 1] - p.Ex[ex]) / dx);
-    T cx = ce(p.h[0], i, p.psi[P_EZ_X], h, (p.Ez[ez + (long long)ny * (nz - 1)] - p.Ez[ez]) / dx);
-    p.Hy[h] = p.Hy[h] - dt * (cz - cx);
+  if (fy) {
+    exk = p.Ex[ie + 1];
+    ezx = p.Ez[iy + (long long)ny * (nz - 1)];
+    hy = p.Hy[iy];
   }
-  if (i < nx - 1 && j < ny - 1) {  // Hz (nx-1, ny-1, nz)
-    long long h = ((long long)i * (ny - 1) + j) * nz + k;
-    long long ey = h;  // Ey (nx, ny-1, nz) shares the index for i < nx-1
-    long long ex = ((long long)i * ny + j) * nz + k;
-    T cx = ce(p.h[0], i, p.psi[P_EY_X], h, (p.Ey[ey + (long long)(ny - 1) * nz] - p.Ey[ey]) / dx);
-    T cy = ce(p.h[1], j, p.psi[P_EX_Y], h, (p.Ex[ex + nz] - p.Ex[ex]) / dx);
-    p.Hz[h] = p.Hz[h] - dt * (cx - cy);
+  if (fz) {
+    eyx = p.Ey[iz + (long long)(ny - 1) * nz];
+    exj = p.Ex[ie + nz];
+    hz = p.Hz[iz];
+  }
+  const CeIn<T> cxy = ce_load(p.h[1], j, fx, p.psi[P_EZ_Y], ix), cxz = ce_load(p.h[2], k, fx, p.psi[P_EY_Z], ix);
+  const CeIn<T> cyz = ce_load(p.h[2], k, fy, p.psi[P_EX_Z], iy), cyx = ce_load(p.h[0], i, fy, p.psi[P_EZ_X], iy);
+  const CeIn<T> czx = ce_load(p.h[0], i, fz, p.psi[P_EY_X], iz), czy = ce_load(p.h[1], j, fz, p.psi[P_EX_Y], iz);
+  // ---- arithmetic and stores (stepper_3d.py:363-390)
+  if (fx) {
+    T cy = ce_apply(cxy, p.psi[P_EZ_Y], ix, (ezj - ez0) / dx);
+    T cz = ce_apply(cxz, p.psi[P_EY_Z], ix, (eyk - ey0) / dx);
+    p.Hx[ix] = hx - dt * (cy - cz);
+  }
+  if (fy) {
+    T cz = ce_apply(cyz, p.psi[P_EX_Z], iy, (exk - ex0) / dx);
+    T cx = ce_apply(cyx, p.psi[P_EZ_X], iy, (ezx - ez0) / dx);
+    p.Hy[iy] = hy - dt * (cz - cx);
+  }
+  if (fz) {
+    T cx = ce_apply(czx, p.psi[P_EY_X], iz, (eyx - ey0) / dx);
+    T cy = ce_apply(czy, p.psi[P_EX_Y], iz, (exj - ex0) / dx);
+    p.Hz[iz] = hz - dt * (cx - cy);
   }
 }
 
@@ -101,76 +146,88 @@ __device__ __forceinline__ T inject(const P& p, const mt_source3d& s, int comp, 
 
 template <typename T>
 __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T c, T* tape) {
-  int k = blockIdx.x * BZ + threadIdx.x;
-  int j = blockIdx.y * BYY + threadIdx.y;
-  int i = blockIdx.z;
+  const int k = blockIdx.x * BZ + threadIdx.x;
+  const int j = blockIdx.y * BYY + threadIdx.y;
+  const int i = blockIdx.z;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
   if (j >= ny || k >= nz) return;
   const T dx = p.dx;
-  long long ce_ = ((long long)i * ny + j) * nz + k;  // cell index in eps
-  T eps0 = p.eps[ce_];
-  if (i < nx - 1) {  // Ex (nx-1, ny, nz)
-    long long x = ce_;
+  const bool ax = i < nx - 1, ay = j < ny - 1, az = k < nz - 1;  // Ex, Ey, Ez exist at this cell
+  const bool ix = i >= 1 && ax, jy = j >= 1 && ay, kz = k >= 1 && az;
+  const long long ce_ = ((long long)i * ny + j) * nz + k;                 // eps, Ex
+  const long long iy = ((long long)i * (ny - 1) + j) * nz + k;            // Ey, Hz (i, j, k)
+  const long long iz = ((long long)i * ny + j) * (nz - 1) + k;            // Ez, Hy (i, j, k)
+  const long long ihx = ((long long)i * (ny - 1) + j) * (nz - 1) + k;     // Hx (i, j, k)
+  // ---- loads (every one before the first store; see ce_load)
+  const T eps0 = p.eps[ce_];
+  T epx = T(0), epy = T(0), epz = T(0), ex = T(0), ey = T(0), ez = T(0);
+  T hz0 = T(0), hzj = T(0), hzx = T(0), hy0 = T(0), hyk = T(0), hyx = T(0), hx0 = T(0), hxk = T(0), hxj = T(0);
+  if (ax) {
+    epx = p.eps[ce_ + (long long)ny * nz];
+    ex = p.Ex[ce_];
+  }
+  if (ay) {
+    epy = p.eps[ce_ + nz];
+    ey = p.Ey[iy];
+  }
+  if (az) {
+    epz = p.eps[ce_ + 1];
+    ez = p.Ez[iz];
+  }
+  if (ax && ay) hz0 = p.Hz[iy];
+  if (ax && jy) hzj = p.Hz[iy - nz];
+  if (ay && ix) hzx = p.Hz[iy - (long long)(ny - 1) * nz];
+  if (ax && az) hy0 = p.Hy[iz];
+  if (ax && kz) hyk = p.Hy[iz - 1];
+  if (az && ix) hyx = p.Hy[iz - (long long)ny * (nz - 1)];
+  if (ay && az) hx0 = p.Hx[ihx];
+  if (ay && kz) hxk = p.Hx[ihx - 1];
+  if (az && jy) hxj = p.Hx[ihx - (nz - 1)];
+  const long long q_xy = ((long long)i * (ny - 2) + (j - 1)) * nz + k;
+  const long long q_xz = ((long long)i * ny + j) * (nz - 2) + (k - 1);
+  const long long q_yz = ((long long)i * (ny - 1) + j) * (nz - 2) + (k - 1);
+  const long long q_yx = ((long long)(i - 1) * (ny - 1) + j) * nz + k;
+  const long long q_zx = ((long long)(i - 1) * ny + j) * (nz - 1) + k;
+  const long long q_zy = ((long long)i * (ny - 2) + (j - 1)) * (nz - 1) + k;
+  const CeIn<T> cxy = ce_load(p.e[1], j - 1, ax && jy, p.psi[P_HZ_Y], q_xy);
+  const CeIn<T> cxz = ce_load(p.e[2], k - 1, ax && kz, p.psi[P_HY_Z], q_xz);
+  const CeIn<T> cyz = ce_load(p.e[2], k - 1, ay && kz, p.psi[P_HX_Z], q_yz);
+  const CeIn<T> cyx = ce_load(p.e[0], i - 1, ay && ix, p.psi[P_HZ_X], q_yx);
+  const CeIn<T> czx = ce_load(p.e[0], i - 1, az && ix, p.psi[P_HY_X], q_zx);
+  const CeIn<T> czy = ce_load(p.e[1], j - 1, az && jy, p.psi[P_HX_Y], q_zy);
+  // ---- arithmetic and stores (stepper_3d.py:392-445)
+  if (ax) {  // Ex (nx-1, ny, nz)
     T chy = T(0), chz = T(0);
-    bool jy = j >= 1 && j < ny - 1, kz = k >= 1 && k < nz - 1;
-    if (jy) {
-      long long hz = ((long long)i * (ny - 1) + j) * nz + k;
-      long long q = ((long long)i * (ny - 2) + (j - 1)) * nz + k;
-      chy = ce(p.e[1], j - 1, p.psi[P_HZ_Y], q, (p.Hz[hz] - p.Hz[hz - nz]) / dx);
-    }
-    if (kz) {
-      long long hy = ((long long)i * ny + j) * (nz - 1) + k;
-      long long q = ((long long)i * ny + j) * (nz - 2) + (k - 1);
-      chz = ce(p.e[2], k - 1, p.psi[P_HY_Z], q, (p.Hy[hy] - p.Hy[hy - 1]) / dx);
-    }
+    if (jy) chy = ce_apply(cxy, p.psi[P_HZ_Y], q_xy, (hz0 - hzj) / dx);
+    if (kz) chz = ce_apply(cxz, p.psi[P_HY_Z], q_xz, (hy0 - hyk) / dx);
     T curl = (jy && kz) ? chy - chz : T(0);
-    T epsc = T(0.5) * (eps0 + p.eps[ce_ + (long long)ny * nz]);
-    T E = e_update(p, p.Ex[x], epsc, curl);
+    T epsc = T(0.5) * (eps0 + epx);
+    T E = e_update(p, ex, epsc, curl);
     E = inject(p, src, 0, i, j, k, E, epsc, c);
-    p.Ex[x] = E;
-    if (tape) tape[x] = E;
+    p.Ex[ce_] = E;
+    if (tape) tape[ce_] = E;
   }
-  if (j < ny - 1) {  // Ey (nx, ny-1, nz)
-    long long y = ((long long)i * (ny - 1) + j) * nz + k;
+  if (ay) {  // Ey (nx, ny-1, nz)
     T chz = T(0), chx = T(0);
-    bool kz = k >= 1 && k < nz - 1, ix = i >= 1 && i < nx - 1;
-    if (kz) {
-      long long hx = ((long long)i * (ny - 1) + j) * (nz - 1) + k;
-      long long q = ((long long)i * (ny - 1) + j) * (nz - 2) + (k - 1);
-      chz = ce(p.e[2], k - 1, p.psi[P_HX_Z], q, (p.Hx[hx] - p.Hx[hx - 1]) / dx);
-    }
-    if (ix) {
-      long long hz = ((long long)i * (ny - 1) + j) * nz + k;
-      long long q = ((long long)(i - 1) * (ny - 1) + j) * nz + k;
-      chx = ce(p.e[0], i - 1, p.psi[P_HZ_X], q, (p.Hz[hz] - p.Hz[hz - (long long)(ny - 1) * nz]) / dx);
-    }
+    if (kz) chz = ce_apply(cyz, p.psi[P_HX_Z], q_yz, (hx0 - hxk) / dx);
+    if (ix) chx = ce_apply(cyx, p.psi[P_HZ_X], q_yx, (hz0 - hzx) / dx);
     T curl = (kz && ix) ? chz - chx : T(0);
-    T epsc = T(0.5) * (eps0 + p.eps[ce_ + nz]);
-    T E = e_update(p, p.Ey[y], epsc, curl);
+    T epsc = T(0.5) * (eps0 + epy);
+    T E = e_update(p, ey, epsc, curl);
     E = inject(p, src, 1, i, j, k, E, epsc, c);
-    p.Ey[y] = E;
-    if (tape) tape[(long long)(nx - 1) * ny * nz + y] = E;
+    p.Ey[iy] = E;
+    if (tape) tape[(long long)(nx - 1) * ny * nz + iy] = E;
   }
-  if (k < nz - 1) {  // Ez (nx, ny, nz-1)
-    long long z = ((long long)i * ny + j) * (nz - 1) + k;
+  if (az) {  // Ez (nx, ny, nz-1)
     T chx = T(0), chy = T(0);
-    bool ix = i >= 1 && i < nx - 1, jy = j >= 1 && j < ny - 1;
-    if (ix) {
-      long long hy = z;  // Hy (nx-1, ny, nz-1) at (i, j, k) for i < nx-1
-      long long q = ((long long)(i - 1) * ny + j) * (nz - 1) + k;
-      chx = ce(p.e[0], i - 1, p.psi[P_HY_X], q, (p.Hy[hy] - p.Hy[hy - (long long)ny * (nz - 1)]) / dx);
-    }
-    if (jy) {
-      long long hx = ((long long)i * (ny - 1) + j) * (nz - 1) + k;
-      long long q = ((long long)i * (ny - 2) + (j - 1)) * (nz - 1) + k;
-      chy = ce(p.e[1], j - 1, p.psi[P_HX_Y], q, (p.Hx[hx] - p.Hx[hx - (nz - 1)]) / dx);
-    }
+    if (ix) chx = ce_apply(czx, p.psi[P_HY_X], q_zx, (hy0 - hyx) / dx);
+    if (jy) chy = ce_apply(czy, p.psi[P_HX_Y], q_zy, (hx0 - hxj) / dx);
     T curl = (ix && jy) ? chx - chy : T(0);
-    T epsc = T(0.5) * (eps0 + p.eps[ce_ + 1]);
-    T E = e_update(p, p.Ez[z], epsc, curl);
+    T epsc = T(0.5) * (eps0 + epz);
+    T E = e_update(p, ez, epsc, curl);
     E = inject(p, src, 2, i, j, k, E, epsc, c);
-    p.Ez[z] = E;
-    if (tape) tape[(long long)(nx - 1) * ny * nz + (long long)nx * (ny - 1) * nz + z] = E;
+    p.Ez[iz] = E;
+    if (tape) tape[(long long)(nx - 1) * ny * nz + (long long)nx * (ny - 1) * nz + iz] = E;
   }
 }
 
@@ -203,7 +260,7 @@ struct F3 {
   T* psin[12];
 };
 
-// ce() with the psi recursion reading the old buffer and (if W) writing the new one
+// The CPML curl term with the psi recursion reading the old buffer and (if W) writing the new one
 template <typename T, bool W>
 __device__ __forceinline__ T ce2(const Pml<T>& pr, int idx, const T* ps, T* pn, long long q, T d) {
   if (!pr.in(idx)) return d;
