@@ -20,35 +20,51 @@ struct AdjK {
   T* eps_grad;
 };
 
+// Each pass below issues all of a cell's loads before its first store: the
+// adjoint, scratch and profile arrays never alias, but the compiler cannot
+// know that and would otherwise serialise every load behind the stores that
+// precede it in program order.
+
 // A: Ez' cotangent -> eps gradient and Dz cotangent; curl cotangent -> D-half
 // psi cotangents and the H-difference cotangents gHy, gHx.
 template <typename T>
 __device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, int i, int j) {
   const int nx = p.nx, ny = p.ny;
-  long long c = (long long)i * ny + j;
-  T eps = p.eps[c];
-  T ezb = p.aEz[c];
-  p.eps_grad[c] = p.eps_grad[c] + (-ezb * ez_after[c]) / eps;
-  T dzb = p.aDz[c] + ezb / eps;
+  const long long c = (long long)i * ny + j;
+  const bool live = i >= 1 && i < nx - 1 && (p.y_pmc || (j >= 1 && j < ny - 1));
+  const int ii = i - 1, jj = p.y_pmc ? j : j - 1;
+  const long long q = (long long)ii * p.ny_e + jj;
+  const T eps = p.eps[c], ezb = p.aEz[c], eg = p.eps_grad[c], ea = ez_after[c], dz0 = p.aDz[c];
+  const bool sx = live && p.ex.in(ii), sy = live && p.ey.in(jj);
+  T pyx = T(0), ax = T(0), bx = T(0), pxy = T(0), ay = T(0), by = T(0);
+  if (sx) {
+    pyx = p.apyx[q];
+    ax = p.ex.a[ii];
+    bx = p.ex.b[ii];
+  }
+  if (sy) {
+    pxy = p.apxy[q];
+    ay = p.ey.a[jj];
+    by = p.ey.b[jj];
+  }
+  p.eps_grad[c] = eg + (-ezb * ea) / eps;
+  T dzb = dz0 + ezb / eps;
   p.aDz[c] = dzb;
   p.aEz[c] = T(0);
-  bool live = i >= 1 && i < nx - 1 && (p.y_pmc || (j >= 1 && j < ny - 1));
   T ghy = T(0), ghx = T(0);
   if (live) {
     T cb = p.dt * dzb;
-    int ii = i - 1, jj = p.y_pmc ? j : j - 1;
-    long long q = (long long)ii * p.ny_e + jj;
     ghy = cb;
-    if (p.ex.in(ii)) {
-      T pt = p.apyx[q] + cb;
-      ghy = cb + p.ex.a[ii] * pt;
-      p.apyx[q] = p.ex.b[ii] * pt;
+    if (sx) {
+      T pt = pyx + cb;
+      ghy = cb + ax * pt;
+      p.apyx[q] = bx * pt;
     }
     ghx = -cb;
-    if (p.ey.in(jj)) {
-      T pt = p.apxy[q] - cb;
-      ghx = -cb + p.ey.a[jj] * pt;
-      p.apxy[q] = p.ey.b[jj] * pt;
+    if (sy) {
+      T pt = pxy - cb;
+      ghx = -cb + ay * pt;
+      p.apxy[q] = by * pt;
     }
   }
   p.gHy[c] = ghy;
@@ -60,30 +76,53 @@ __device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, 
 template <typename T>
 __device__ __forceinline__ void adj_b_cell(const AdjK<T>& p, int i, int j) {
   const int nx = p.nx, ny = p.ny;
-  if (i < nx - 1) {
-    long long h = (long long)i * ny + j;
-    T hb = p.aHy[h] + (p.gHy[h] - p.gHy[h + ny]) / p.dx;
-    p.aHy[h] = hb;
-    T gb = p.dt * hb;
-    if (p.hx.in(i)) {
-      T pt = p.apzx[h] + p.dt * hb;
-      gb = gb + p.hx.a[i] * pt;
-      p.apzx[h] = p.hx.b[i] * pt;
-    }
-    p.gEx[h] = gb;
+  const bool fy = i < nx - 1, fx = j < ny - 1;
+  const long long hy = (long long)i * ny + j;            // Hy (nx-1, ny), gHy, gEx
+  const long long hx = (long long)i * (ny - 1) + j;      // Hx (nx, ny-1), gEy
+  T ahy = T(0), gy0 = T(0), gy1 = T(0), ahx = T(0), gx0 = T(0), gx1 = T(0);
+  T pzx = T(0), a_x = T(0), b_x = T(0), pzy = T(0), a_y = T(0), b_y = T(0);
+  const bool sx = fy && p.hx.in(i), sy = fx && p.hy.in(j);
+  if (fy) {
+    ahy = p.aHy[hy];
+    gy0 = p.gHy[hy];
+    gy1 = p.gHy[hy + ny];
   }
-  if (j < ny - 1) {
-    long long h = (long long)i * (ny - 1) + j;
-    long long c = (long long)i * ny + j;
-    T hb = p.aHx[h] + (p.gHx[c] - p.gHx[c + 1]) / p.dx;
-    p.aHx[h] = hb;
-    T gb = -p.dt * hb;
-    if (p.hy.in(j)) {
-      T pt = p.apzy[h] - p.dt * hb;
-      gb = gb + p.hy.a[j] * pt;
-      p.apzy[h] = p.hy.b[j] * pt;
+  if (fx) {
+    ahx = p.aHx[hx];
+    gx0 = p.gHx[hy];
+    gx1 = p.gHx[hy + 1];
+  }
+  if (sx) {
+    pzx = p.apzx[hy];
+    a_x = p.hx.a[i];
+    b_x = p.hx.b[i];
+  }
+  if (sy) {
+    pzy = p.apzy[hx];
+    a_y = p.hy.a[j];
+    b_y = p.hy.b[j];
+  }
+  if (fy) {
+    T hb = ahy + (gy0 - gy1) / p.dx;
+    p.aHy[hy] = hb;
+    T gb = p.dt * hb;
+    if (sx) {
+      T pt = pzx + p.dt * hb;
+      gb = gb + a_x * pt;
+      p.apzx[hy] = b_x * pt;
     }
-    p.gEy[h] = gb;
+    p.gEx[hy] = gb;
+  }
+  if (fx) {
+    T hb = ahx + (gx0 - gx1) / p.dx;
+    p.aHx[hx] = hb;
+    T gb = -p.dt * hb;
+    if (sy) {
+      T pt = pzy - p.dt * hb;
+      gb = gb + a_y * pt;
+      p.apzy[hx] = b_y * pt;
+    }
+    p.gEy[hx] = gb;
   }
 }
 
