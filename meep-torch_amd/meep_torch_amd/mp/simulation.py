@@ -6,14 +6,15 @@ add_flux/get_fluxes/get_flux_freqs (:1357-1386,1809-1870),
 add_dft_fields/get_dft_array (:1592-1641), run(until | until_after_sources |
 n_steps) (:1096-1142,1239-1302) and get_array (:2119-2150).
 
-Time stepping semantics follow the reference: every run() starts from zero
+3D cells (dimensions=3) take the reference's 3D adapter path, in sim3d.py.
+
+Time stepping semantics (2D) follow the reference: every run() starts from zero
 fields and zero DFT accumulators and integrates to the accumulated clock
 (so consecutive runs reproduce one continuing run), in chunks
 [this run's steps, previous clock] exactly as its _run_until loop issues
 them (meep_torch/mp/simulation.py:1271-1295); each chunk's monitor sums are
 added in chunk order.  Step functions, stop conditions, symmetries, k-points,
-3D cells, multi-source runs and non-Block geometry raise
-NotImplementedError.
+multi-source 2D runs and non-Block geometry raise NotImplementedError.
 """
 import numpy as np
 import torch
@@ -26,6 +27,7 @@ from ..core.sources import LineSource
 from ..core.stepper_te import FIELDS as TE_FIELDS, init_te_state
 from ..core.stepper_tm import FIELDS as TM_FIELDS, init_tm_state
 from . import geometry as G
+from .sim3d import Sim3D
 from .sources import source_span
 from .vec import Vector3, pol_for
 
@@ -70,7 +72,7 @@ def _name(f):
     return f.name if isinstance(f, _Dft) else f
 
 
-class Simulation:
+class Simulation(Sim3D):
     def __init__(self, *, cell_size, resolution, geometry=None, sources=None, boundary_layers=None,
                  default_material=None, Courant=0.5, dtype=torch.float64, device=None, dimensions=2,
                  **options):
@@ -84,8 +86,12 @@ class Simulation:
             if k == "eps_averaging" or v == _DEFAULTS[k]:
                 continue
             raise NotImplementedError(f"Simulation({k}=...) is not supported by the HIP FDTD facade")
+        if dimensions == 3:
+            self._init3(cell_size, resolution, geometry, sources, boundary_layers, default_material, Courant, dtype,
+                        device)
+            return
         if dimensions != 2:
-            raise NotImplementedError("the Simulation facade drives 2D cells only")
+            raise NotImplementedError(f"the Simulation facade drives 2D and 3D cells, not dimensions={dimensions}")
         self.dimensions = 2
         self.dtype = dtype
         self.cell_size = Vector3(*cell_size)
@@ -112,6 +118,8 @@ class Simulation:
 
     # ---- structure -----------------------------------------------------
     def _init_structure(self):
+        if self.dimensions == 3:
+            return self._init_structure3()
         if self._coeffs is not None:
             return
         pol = pol_for(self._sources[0].component) if self._sources else "tm"
@@ -171,6 +179,8 @@ class Simulation:
     def add_flux(self, fcen, df, nfreq, *regions, name=None):
         freqs = np.linspace(fcen - df / 2.0, fcen + df / 2.0, nfreq).tolist()
         self._init_structure()
+        if self.dimensions == 3:
+            return DftFlux(self, self._add_flux3(freqs, list(regions), name), freqs)
         name = name or f"flux{len(self._monitors)}"
         parts = []
         for r, region in enumerate(regions):
@@ -189,6 +199,10 @@ class Simulation:
             center, size = where.center, where.size
         self._init_structure()
         name = name or f"dft{len(self._monitors)}"
+        if self.dimensions == 3:
+            region = None if center is None and size is None else (center, size)
+            self._add_dft3(freqs, list(components), name, region)
+            return DftFields(self, name, freqs)
         box = None
         if center is not None or size is not None:
             x0, y0, x1, y1 = G.rect_corners(Vector3(*(center or Vector3())), Vector3(*(size or Vector3())),
@@ -208,7 +222,7 @@ class Simulation:
             raise NotImplementedError("step functions / stop conditions are not supported by the HIP facade")
         self.run_index += 1
         self._init_structure()
-        dt = self.grid.dt
+        dt = self._coeffs.dt if self.dimensions == 3 else self.grid.dt
         if n_steps is None:
             if until is not None:
                 T = until
@@ -217,7 +231,10 @@ class Simulation:
             else:
                 T = 0
             n_steps = max(1, int(round(float(T) / dt)))
-        self._advance(int(n_steps))
+        if self.dimensions == 3:
+            self._run3(int(n_steps))
+        else:
+            self._advance(int(n_steps))
 
     def _advance(self, n_total):
         mode = self._pol
@@ -250,6 +267,9 @@ class Simulation:
         self._t_step = 0
 
     def round_time(self):
+        if self.dimensions == 3:
+            self._init_structure()
+            return self._t_step * self._coeffs.dt
         return self._t_step * self.grid.dt
 
     meep_time = round_time
@@ -264,7 +284,7 @@ class Simulation:
         acc = self._acc.get(name)
         if acc is None:
             return [0.0] * int(mon.freqs.numel())
-        vals = mon.flux(acc, self.grid.dx).detach().cpu().numpy() * meta.get("wt", 1.0)
+        vals = mon.flux(acc, self.gs.dx).detach().cpu().numpy() * meta.get("wt", 1.0)
         return vals.tolist()
 
     def get_flux_freqs(self, f):
@@ -285,6 +305,8 @@ class Simulation:
     def get_array(self, component="Dielectric", center=None, size=None):
         if center is not None or size is not None:
             raise NotImplementedError("get_array sub-volumes are not supported")
+        if self.dimensions == 3:
+            return self._get_array3(component)
         if component == "Dielectric":
             self._init_structure()
             return (self._raster[0] if self._pol == "tm" else self._raster[1]).copy()

@@ -76,7 +76,7 @@ def test_unsupported_options_raise():
     with pytest.raises(NotImplementedError):
         mp.Simulation(cell_size=mp.Vector3(4, 4), resolution=10, symmetries=[object()], device="cpu")
     with pytest.raises(NotImplementedError):
-        mp.Simulation(cell_size=mp.Vector3(4, 4, 4), resolution=10, dimensions=3, device="cpu")
+        mp.Simulation(cell_size=mp.Vector3(4, 4, 4), resolution=10, dimensions=3, Courant=0.6, device="cpu")
     with pytest.raises(NotImplementedError):
         mp.Medium(epsilon=2, chi3=0.1)
     with pytest.raises(NotImplementedError):
@@ -87,7 +87,42 @@ def test_unsupported_options_raise():
 
 
 def test_run_requires_hip_device():
-    """No CPU fallback: running the facade on CPU tensors raises."""
+    """No CPU fallback: running the facade on CPU tensors raises (2D and 3D)."""
     _, sim = _plain(torch.float32)
     with pytest.raises(RuntimeError):
         sim.run(until=1)
+    import meep_torch_amd.mp as mp
+    sim3 = mp.Simulation(cell_size=mp.Vector3(1, 1, 1), resolution=10, dimensions=3, device="cpu",
+                         sources=[mp.Source(mp.GaussianSource(1.0, fwidth=0.5), component=mp.Ez)])
+    with pytest.raises(RuntimeError):
+        sim3.run(until=1)
+
+
+def test_3d_host_setup_closed_forms():
+    """3D grid, index boxes and CPML sides by hand (meep_torch/mp/coords.py:34-111,
+    mp/simulation.py:89-116,359-379): cell 4 x 3 x 2 at resolution 10."""
+    import meep_torch_amd.mp as mp
+    from meep_torch_amd.mp.sim3d import box3d, cell_index3, grid_spec3, pml_sides3
+    gs = grid_spec3(mp.Vector3(4, 3, 2), 10)
+    assert (gs.nx, gs.ny, gs.nz) == (40, 30, 20)
+    # x in [2.0, 3.0] um from the corner, y the whole axis, z in [0.8, 1.2]
+    assert box3d(mp.Vector3(0.5, 0, 0), mp.Vector3(1, mp.inf, 0.4), gs) == (20, 30, 0, 30, 8, 12)
+    # clipped at the high x face: [3.6, 4.2] -> [3.6, 4.0]
+    assert box3d(mp.Vector3(1.9, 0, 0), mp.Vector3(0.6, 0.2, 0.2), gs)[:2] == (36, 40)
+    # a zero-thickness box keeps one cell at the low face
+    assert box3d(mp.Vector3(-2, 0, 0), mp.Vector3(0, 0.2, 0.2), gs)[:2] == (0, 1)
+    assert cell_index3(mp.Vector3(-2, 0, 0.04), gs) == (0, 15, 10)
+    assert pml_sides3([mp.PML(0.5)], gs, mp.Vector3(4, 3, 2)) == [[5, 5], [5, 5], [5, 5]]
+    sides = pml_sides3([mp.PML(0.3, direction=mp.X, side=mp.Low), mp.PML(0.2)], gs, mp.Vector3(4, 3, 2))
+    assert sides == [[3, 2], [2, 2], [2, 2]]
+    assert pml_sides3([mp.PML(0.4, direction=mp.Z, side=mp.High)], gs, mp.Vector3(4, 3, 2)) == \
+        [[0, 0], [0, 0], [0, 4]]
+
+
+def test_3d_courant_and_options_raise():
+    import pytest
+    import meep_torch_amd.mp as mp
+    with pytest.raises(NotImplementedError):
+        mp.Simulation(cell_size=mp.Vector3(1, 1, 1), resolution=10, dimensions=3, Courant=0.4)
+    with pytest.raises(NotImplementedError):
+        mp.Simulation(cell_size=mp.Vector3(1, 1, 1), resolution=10, dimensions=1)
