@@ -41,14 +41,17 @@ DOMINANT = {
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
-def _traffic(workload, cell_steps_per_launch):
+def _traffic(workload, cell_steps_per_launch, variant=None):
     """HBM bytes per launch of the dominant kernel from the committed PMC
     measurement (FETCH_SIZE x fetch scale + WRITE_SIZE x write scale, per
-    cell-step), scaled to this launch size; None if not measured."""
+    cell-step), scaled to this launch size; None if not measured (or
+    measured for another kernel variant)."""
     try:
         with open(TRAFFIC_FILE) as fh:
             t = json.load(fh)[workload]
     except (OSError, KeyError, ValueError):
+        return None
+    if "variant" in t and variant is not None and t["variant"] != variant:
         return None
     return t["bytes_per_cell_step"] * cell_steps_per_launch
 
@@ -284,7 +287,11 @@ def main():
         per_launch_cells = cells
         per_launch_s = gpu_ms / 1e3 / (ysteps * args.steps)
     achieved = alg * per_launch_cells / per_launch_s / 1e9
-    traffic = _traffic(args.workload, per_launch_cells)
+    variant = None
+    if args.workload == "c3":
+        from meep_torch_amd.core import engine3d
+        variant = engine3d.DEFAULT_VARIANT
+    traffic = _traffic(args.workload, per_launch_cells, variant)
     line = {
         "metric": "Yee cell-updates/s (FP32) with % of MI355X HBM roofline",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
