@@ -382,7 +382,7 @@ def bench_c5(args):
     rf = D.fdtd_loop(pb, iters=args.c5_iters, theta0=theta0, log=log)
     rs = D.surrogate_loop(pb, net, iters=args.c5_iters, theta0=theta0,
                           data=(data[0][:n_train], data[1][:n_train]), log=log)
-    gap = abs(rs["final_fom"] - rf["final_fom"]) / abs(rf["final_fom"])
+    gap = abs(rs["final_fom"] - rf["final_fom"]) / max(abs(rf["final_fom"]), 1e-30)
     log(f"[bench] c5 plain FDTD: {rf['iters_per_s']:.2f} it/s final FOM {rf['final_fom']:.4f}; surrogate: "
         f"{rs['iters_per_s']:.2f} it/s final FOM {rs['final_fom']:.4f} (gap {100 * gap:.2f}%), "
         f"{rs['failed']}/{rs['verifications']} verifications failed, {rs['fdtd_iters']} FDTD iterations")

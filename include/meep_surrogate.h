@@ -56,8 +56,9 @@ int mt_bias_act(float* x, const float* bias, int C, int P, long long n, int act,
 int mt_act_grad(const float* dy, const float* pre, long long n, int act, int P, int row_bcast, float scale,
                 float* d32, void* d16, void* stream);
 
-/* out[c] = scale * sum_{b<B, p<P} x[b, c, p] (fixed-order, deterministic). */
-int mt_channel_sum(const float* x, float* out, int B, int C, int P, float scale, void* stream);
+/* out[c] = scale * sum_{b<B, p<P} x[b, c, p] (fixed-order, deterministic).
+ * work: device scratch of B*C floats (the per-(b, c) row sums). */
+int mt_channel_sum(const float* x, float* out, float* work, int B, int C, int P, float scale, void* stream);
 
 /* out[r] = scale * sum_{p<P} x[r, p]. */
 int mt_row_sum(const float* x, float* out, long long rows, int P, float scale, void* stream);

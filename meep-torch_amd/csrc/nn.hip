@@ -232,22 +232,15 @@ __global__ void act_grad_kernel(const float* dy, const float* pre, long long n, 
 
 // out[c] = scale * sum_{b, p} x[b, c, p]; one block per channel, fixed
 // per-thread strides and a fixed tree, so the sum is deterministic.
-__global__ __launch_bounds__(256) void channel_sum_kernel(const float* x, float* out, int B, int C, int P,
+// out[c] = scale * sum_b rows[b, c]: the per-channel fold of the (b, c) row
+// sums, in order of b.
+__global__ __launch_bounds__(256) void channel_fold_kernel(const float* rows, float* out, int B, int C,
                                                            float scale) {
-  __shared__ float red[256];
-  const int c = blockIdx.x;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
   float s = 0.f;
-  for (int b = 0; b < B; ++b) {
-    const float* row = x + ((long long)b * C + c) * P;
-    for (int p = threadIdx.x; p < P; p += 256) s += row[p];
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[c] = red[0] * scale;
+  for (int b = 0; b < B; ++b) s += rows[(long long)b * C + c];
+  out[c] = s * scale;
 }
 
 // out[row] = scale * sum_p x[row, p]
@@ -391,11 +384,15 @@ extern "C" int mt_act_grad(const float* dy, const float* pre, long long n, int a
   return check_launch("act_grad");
 }
 
-extern "C" int mt_channel_sum(const float* x, float* out, int B, int C, int P, float scale, void* stream) {
-  if (C <= 0) return MT_OK;
-  if (!x || !out || B < 1 || P < 1) return fail(MT_EINVAL, "channel_sum: bad arguments");
-  hipLaunchKernelGGL(channel_sum_kernel, dim3(C), dim3(256), 0, static_cast<hipStream_t>(stream), x, out, B, C, P,
-                     scale);
+extern "C" int mt_channel_sum(const float* x, float* out, float* work, int B, int C, int P, float scale,
+                              void* stream) {
+  if (B <= 0 || C <= 0) return MT_OK;
+  if (!x || !out || !work || P < 1 || (long long)B * C > 0x7fffffff) return fail(MT_EINVAL, "channel_sum: bad arguments");
+  // one block per (b, c) row fills the whole GPU (a block per channel left
+  // most CUs idle), then a deterministic fold over b
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(row_sum_kernel, dim3((unsigned)(B * C)), dim3(256), 0, s, x, work, P, 1.0f);
+  hipLaunchKernelGGL(channel_fold_kernel, dim3((C + 255) / 256), dim3(256), 0, s, work, out, B, C, scale);
   return check_launch("channel_sum");
 }
 

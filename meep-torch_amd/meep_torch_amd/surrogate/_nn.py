@@ -29,7 +29,7 @@ def lib():
     L.mt_bias_act.argtypes = [c_void_p, c_void_p, c_int, c_int, c_ll, c_int, c_void_p, c_void_p, c_void_p]
     L.mt_act_grad.argtypes = [c_void_p, c_void_p, c_ll, c_int, c_int, c_int, c_float, c_void_p, c_void_p,
                               c_void_p]
-    L.mt_channel_sum.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_float, c_void_p]
+    L.mt_channel_sum.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_float, c_void_p]
     L.mt_row_sum.argtypes = [c_void_p, c_void_p, c_ll, c_int, c_float, c_void_p]
     L.mt_pack_spectral.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]
     L.mt_unpack_spectral_grad.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]
@@ -115,8 +115,12 @@ def act_grad(dy, pre, n, act, P=1, row_bcast=False, scale=1.0, d32=None, d16=Non
                               E.stream_handle()), "mt_act_grad")
 
 
-def channel_sum(x, out, B, C, P, scale=1.0):
-    E.check(lib().mt_channel_sum(_p(x), _p(out), B, C, P, float(scale), E.stream_handle()), "mt_channel_sum")
+def channel_sum(x, out, B, C, P, scale=1.0, ws=None):
+    """out[c] = scale * sum over b, p of x[b, c, p]; the B*C row sums go to
+    the workspace ws (a fresh buffer when None)."""
+    work = ws.get(B * C) if ws is not None else torch.empty(B * C, dtype=torch.float32, device=x.device)
+    E.check(lib().mt_channel_sum(_p(x), _p(out), _p(work), B, C, P, float(scale), E.stream_handle()),
+            "mt_channel_sum")
 
 
 def row_sum(x, out, rows, P, scale=1.0):

@@ -246,7 +246,7 @@ class SpectralOperator:
         dout = dout.detach().to(f32).contiguous()
         d16 = _nn.cast_bf16(dout, self._get("dout16", (B, n_out), bf))
         if param_grads:
-            _nn.channel_sum(dout, g["o_b"], B, n_out, 1)
+            _nn.channel_sum(dout, g["o_b"], B, n_out, 1, ws=ws)
             _nn.gemm(d16, (1, n_out, 0, 0), b["hm16"], (H, 1, 0, 0), g["o_w"], (H, 1, 0), n_out, H, B, ws=ws)
         dhm = self._get("dhm", (B, H), f32)
         _nn.gemm(d16, (n_out, 1, 0, 0), W["o_w"], (H, 1, 0, 0), dhm, (H, 1, 0), B, H, n_out, ws=ws)
@@ -255,7 +255,7 @@ class SpectralOperator:
         _nn.act_grad(dhm, b["pre_q"], B * H * P, True, P=P, row_bcast=True, scale=1.0 / P, d32=dq32, d16=dq16)
         vL = b[f"v{L}"]
         if param_grads:
-            _nn.channel_sum(dq32, g["q_b"], B, H, P)
+            _nn.channel_sum(dq32, g["q_b"], B, H, P, ws=ws)
             _nn.gemm(dq16, (P, 1, 0, H * P), vL, (1, P, 0, CP), g["q_w"], (C, 1, 0), H, C, P, R=B, ws=ws)
         dv = self._get("dv_a", (B, C, P), f32)
         dv_next = self._get("dv_b", (B, C, P), f32)
@@ -270,7 +270,7 @@ class SpectralOperator:
             v = b[f"v{l}"]
             _nn.act_grad(dv, b[f"pre{l}"], B * CP, l < L - 1, d32=dv, d16=dp16)
             if param_grads:
-                _nn.channel_sum(dv, g[f"b{l}"], B, C, P)
+                _nn.channel_sum(dv, g[f"b{l}"], B, C, P, ws=ws)
                 _nn.gemm(dp16, (P, 1, 0, CP), v, (1, P, 0, CP), g[f"w{l}"], (C, 1, 0), C, C, P, R=B, ws=ws)
             # through the inverse DFT: duh[b] (C x 2Mo) = dpre[b] (C x P) G^T
             _nn.gemm(dp16, (P, 1, CP, 0), self.GT, (2 * Mo, 1, 0, 0), duh, (1, C, S), C, 2 * Mo, P, batch=B,
@@ -296,7 +296,7 @@ class SpectralOperator:
         _nn.cast_bf16(dv, dp16)
         x16 = b["x16"]
         if param_grads:
-            _nn.channel_sum(dv, g["lift_b"], B, C, P)
+            _nn.channel_sum(dv, g["lift_b"], B, C, P, ws=ws)
             _nn.gemm(dp16, (P, 1, 0, CP), x16, (1, P, 0, self.in_ch * P), g["lift_w"], (self.in_ch, 1, 0),
                      C, self.in_ch, P, R=B, ws=ws)
         drho = torch.empty((B, self.n, self.n), dtype=f32, device=self.device)
