@@ -138,7 +138,7 @@ template <typename T, int TX, int TY, int NS, bool PML>
 __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int nsub, int write_e) {
   constexpr int RX = TX + 2 * NS + 1, RY = TY + 2 * NS + 1, RC = RX * RY;
   constexpr int PER = (RC + BLK - 1) / BLK;
-  __shared__ T sHz[RC], sDx[RC], sDy[RC], sEx[RC], sEy[RC], sKx[RC], sKy[RC];
+  __shared__ T sHz[RC], sDx[RC], sDy[RC], sEx[RC], sEy[RC];
   __shared__ T sP[PML ? 4 * RC : 1];
   T* sPzy = sP;            // psi_Hzy: Dx update, y shell (index (i, j-1))
   T* sPzx = sP + RC;       // psi_Hzx: Dy update, x shell (index (i-1, j))
@@ -151,8 +151,14 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
   const int a0 = I0 - NS - 1, b0 = J0 - NS - 1;
   const int tid = threadIdx.x;
   const T dt = p.dt, dx = p.dx, rdx = p.rdx;
+  // A thread updates the same PER cells of the LDS region in every
+  // sub-step, so eps and the D fields of those cells live in its registers:
+  // the update reads them without LDS traffic, and the LDS holds only what
+  // neighbours, taps and the write-back read (fewer bytes per workgroup,
+  // more workgroups per CU).
+  T vDx[PER], vDy[PER], vKx[PER], vKy[PER];
   {
-    T vHz[PER], vDx[PER], vDy[PER], vKx[PER], vKy[PER];
+    T vHz[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       int c = tid + u * BLK;
@@ -174,8 +180,6 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
         sHz[c] = vHz[u];
         sDx[c] = vDx[u];
         sDy[c] = vDy[u];
-        sKx[c] = vKx[u];
-        sKy[c] = vKy[u];
       }
     }
   }
@@ -206,7 +210,7 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
       if (c >= RC) break;
       int i = a0 + li, j = b0 + lj;
       if (!PML || (i >= 0 && i < nx - 1 && j >= 0 && j < ny)) {
-        T dxv = sDx[c];
+        T dxv = vDx[u];
         if ((!PML || (j >= 1 && j < ny - 1)) && lj >= 1) {
           T d = div_c(sHz[c] - sHz[c - 1], dx, rdx);
           T curl = d;
@@ -217,11 +221,12 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
           }
           dxv = dxv + dt * curl;
         }
+        vDx[u] = dxv;
         sDx[c] = dxv;
-        sEx[c] = dxv / sKx[c];
+        sEx[c] = dxv / vKx[u];
       }
       if (!PML || (i >= 0 && i < nx && j >= 0 && j < ny - 1)) {
-        T dyv = sDy[c];
+        T dyv = vDy[u];
         if ((!PML || (i >= 1 && i < nx - 1)) && li >= 1) {
           T d = div_c(sHz[c] - sHz[c - RY], dx, rdx);
           T curl = d;
@@ -235,8 +240,9 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
         if (p.src.present && i >= p.src.i0 && i < p.src.i1 && j >= p.src.j0 && j < p.src.j1)
           dyv = dyv - coef * static_cast<const T*>(p.src.w)[(long long)(i - p.src.i0) * (p.src.j1 - p.src.j0) +
                                                              (j - p.src.j0)];
+        vDy[u] = dyv;
         sDy[c] = dyv;
-        sEy[c] = dyv / sKy[c];
+        sEy[c] = dyv / vKy[u];
       }
     }
     __syncthreads();
@@ -312,7 +318,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
                                         int tc, int slot_base, int coef_t0) {
   constexpr int RX = TX + 2 * NS + 1, RY = TY + 2 * NS + 1, RC = RX * RY;
   constexpr int PER = (RC + BLK - 1) / BLK;
-  __shared__ T sEz[RC], sDz[RC], sHx[RC], sHy[RC], sK[RC];
+  __shared__ T sEz[RC], sDz[RC], sHx[RC], sHy[RC];
   __shared__ T sP[PML ? 4 * RC : 1];
   T* sPzy = sP;            // psi_Ezy: Hx update, y shell (index (i, j))
   T* sPzx = sP + RC;       // psi_Ezx: Hy update, x shell (index (i, j))
@@ -324,8 +330,11 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
   const int tid = threadIdx.x;
   const T dt = p.dt, dx = p.dx, rdx = p.rdx;
   const int joff = p.y_pmc ? 0 : 1;
+  // eps and Dz of the PER cells this thread updates stay in its registers
+  // across sub-steps (as in te_tiled)
+  T vDz[PER], vK[PER];
   {
-    T vEz[PER], vDz[PER], vHx[PER], vHy[PER], vK[PER];
+    T vEz[PER], vHx[PER], vHy[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       int c = tid + u * BLK;
@@ -345,7 +354,6 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
         sDz[c] = vDz[u];
         sHx[c] = vHx[u];
         sHy[c] = vHy[u];
-        sK[c] = vK[u];
       }
     }
   }
@@ -415,7 +423,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
       if (c >= RC) break;
       int i = a0 + li, j = b0 + lj;
       if (PML && (i < 0 || i >= nx || j < 0 || j >= ny)) continue;
-      T dz = sDz[c];
+      T dz = vDz[u];
       bool live = !PML || (i >= 1 && i < nx - 1 && (p.y_pmc || (j >= 1 && j < ny - 1)));
       if (live && li >= 1 && lj >= 1) {
         T hx1 = (!PML || j < ny - 1) ? sHx[c] : T(0);
@@ -440,8 +448,9 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
       if (p.src.present && i >= p.src.i0 && i < p.src.i1 && j >= p.src.j0 && j < p.src.j1)
         dz = dz - coef * static_cast<const T*>(p.src.w)[(long long)(i - p.src.i0) * (p.src.j1 - p.src.j0) +
                                                         (j - p.src.j0)];
+      vDz[u] = dz;
       sDz[c] = dz;
-      sEz[c] = dz / sK[c];
+      sEz[c] = dz / vK[u];
     }
     __syncthreads();
     if (p.tape) {
