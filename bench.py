@@ -342,10 +342,14 @@ def bench_c4(args):
 
 
 def _gemm_roofline(net, batch=64, reps=5):
-    """Matrix-core roofline of the surrogate's GEMM kernel: FLOPs of every
-    GEMM in `reps` training steps (forward + backward + Adam) at `batch`,
-    divided by the summed duration of those GEMM launches (HIP events around
-    each launch on the launch stream)."""
+    """Roofline of the surrogate's GEMM kernel over every GEMM of `reps`
+    training passes (forward + backward) at `batch`: FLOPs and algorithmic
+    bytes (BF16 operands read once, FP32 result written, and read when
+    accumulated into) divided by the summed duration of those launches (HIP
+    events around each launch on its stream).  The FNO's products are thin
+    (reductions of 64-256 over 16384-wide activations, or 16384-deep over
+    256 outputs), so the bound is whichever of the MFMA and HBM fractions is
+    higher; both are reported."""
     from meep_torch_amd.surrogate import _nn
     rho = torch.rand(batch, net.n, net.n, device="cuda")
     dout = torch.randn(batch, net.n_out, device="cuda")
@@ -358,12 +362,18 @@ def _gemm_roofline(net, batch=64, reps=5):
         net.backward(dout)
     torch.cuda.synchronize()
     rec, _nn.PROFILE = _nn.PROFILE, None
-    flops = sum(f for f, _, _ in rec)
-    ms = sum(a.elapsed_time(b) for _, a, b in rec)
-    achieved = flops / (ms * 1e-3) / 1e12
-    return {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s", "frac": achieved / 2500.0,
-            "traffic": None, "kernel": f"gemm_kernel (BF16 MFMA, FP32 acc), all GEMMs of a B={batch} training step",
-            "gemm_launches": len(rec) // reps, "gemm_flops_per_step": flops / reps}
+    flops = sum(r[0] for r in rec)
+    nbytes = sum(r[1] for r in rec)
+    sec = sum(a.elapsed_time(b) for _, _, a, b in rec) * 1e-3
+    tf, gbs = flops / sec / 1e12, nbytes / sec / 1e9
+    f_m, f_h = tf / 2500.0, gbs / HBM_PEAK_GBS
+    common = {"traffic": None, "kernel": f"gemm_kernel (BF16 MFMA, FP32 acc), all GEMMs of a B={batch} training pass",
+              "gemm_launches": len(rec) // reps, "gemm_flops_per_pass": flops / reps,
+              "gemm_alg_bytes_per_pass": nbytes / reps, "mfma_tflops": tf, "mfma_frac": f_m, "hbm_gbs": gbs,
+              "hbm_frac": f_h}
+    if f_h > f_m:
+        return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": f_h, **common}
+    return {"bound": "mfma", "achieved": tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": f_m, **common}
 
 
 def bench_c5(args):
@@ -376,7 +386,8 @@ def bench_c5(args):
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     roof = _gemm_roofline(net)
-    log(f"[bench] c5 GEMM roofline: {roof['achieved']:.1f} TFLOP/s ({100 * roof['frac']:.1f}% of BF16 dense)")
+    log(f"[bench] c5 GEMM roofline: {roof['mfma_tflops']:.1f} TFLOP/s ({100 * roof['mfma_frac']:.1f}% of BF16 dense), "
+        f"{roof['hbm_gbs']:.0f} GB/s of algorithmic bytes ({100 * roof['hbm_frac']:.1f}% of HBM): {roof['bound']}-bound")
     theta0 = D.start_design(pb)
     n_train = args.c5_solves
     rf = D.fdtd_loop(pb, iters=args.c5_iters, theta0=theta0, log=log)

@@ -47,7 +47,7 @@ def _p(t):
 
 
 TARGET_WG = 512     # workgroups one GEMM launch aims for (2 per CU)
-PROFILE = None      # list -> every GEMM appends (flops, start event, end event)
+PROFILE = None      # list -> every GEMM appends (flops, algorithmic bytes, start event, end event)
 MIN_TILES = 8       # fewest 32-deep reduction tiles per split
 
 
@@ -91,7 +91,8 @@ def gemm(A, a, B, b, C, c, M, N, K, *, batch=1, R=1, alpha=1.0, beta=0.0, ws=Non
     E.check(lib().mt_gemm_bf16(ctypes.byref(g), E.stream_handle()), "mt_gemm_bf16")
     if PROFILE is not None:
         ev[1].record()
-        PROFILE.append((2.0 * M * N * K * R * batch, ev[0], ev[1]))
+        nbytes = (2.0 * (M * K + K * N) * R + 4.0 * M * N * (2 if beta else 1)) * batch
+        PROFILE.append((2.0 * M * N * K * R * batch, nbytes, ev[0], ev[1]))
     return C
 
 

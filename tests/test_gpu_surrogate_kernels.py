@@ -28,6 +28,11 @@ GEMM_CASES = [
     (1, 300, 64, 2, 1, True, False, 1, 1.0, 0.0),
     (33, 3, 1000, 1, 4, False, True, 16, 2.0, 0.0),
     (128, 128, 2048, 1, 1, False, False, 32, 1.0, 0.0),
+    # larger ragged shapes in every orientation, splits, beta
+    (300, 260, 200, 2, 1, False, True, 1, 1.0, 0.0),
+    (256, 384, 96, 1, 2, True, False, 3, 0.5, 1.0),
+    (130, 129, 77, 2, 1, True, True, 1, 1.0, 1.0),
+    (512, 256, 4096, 1, 1, False, False, 8, 1.0, 0.0),
 ]
 
 
