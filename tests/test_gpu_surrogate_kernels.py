@@ -29,6 +29,8 @@ GEMM_CASES = [
     (33, 3, 1000, 1, 4, False, True, 16, 2.0, 0.0),
     (128, 128, 2048, 1, 1, False, False, 32, 1.0, 0.0),
     # larger ragged shapes in every orientation, splits, beta
+    (64, 1000, 96, 3, 1, True, False, 1, 1.0, 1.0),
+    (50, 512, 300, 2, 2, False, True, 2, 0.5, 0.0),
     (300, 260, 200, 2, 1, False, True, 1, 1.0, 0.0),
     (256, 384, 96, 1, 2, True, False, 3, 0.5, 1.0),
     (130, 129, 77, 2, 1, True, True, 1, 1.0, 1.0),
