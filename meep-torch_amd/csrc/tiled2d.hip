@@ -90,6 +90,27 @@ __device__ __forceinline__ void tile_of(bool pml, const TileMap& m, int* tr, int
 }
 
 // ---------------------------------------------------------------- taps
+// Whether any monitor term has cells in the owned block [I0, I0+TXo) x
+// [J0, J0+TYo) (the cells write_taps writes).  Uniform over the workgroup:
+// a tile without taps skips write_taps and the barrier after it.
+__device__ __forceinline__ bool taps_touch(const TapSet& taps, int I0, int J0, int TXo, int TYo) {
+  for (int q = 0; q < taps.n; ++q) {
+    const Tap& t = taps.t[q];
+    if (t.line == 1) {
+      if (t.fix >= I0 && t.fix < I0 + TXo && t.lo < J0 + TYo && t.lo + t.n > J0) return true;
+    } else if (t.line == 2) {
+      if (t.fix >= J0 && t.fix < J0 + TYo && t.lo < I0 + TXo && t.lo + t.n > I0) return true;
+    } else if (t.line == 3) {
+      if (I0 < t.n0 && J0 < t.n1) return true;
+    } else if (t.line == 4) {
+      if (t.fix >= I0 && t.fix < I0 + TXo && t.lo >= J0 && t.lo < J0 + TYo) return true;
+    } else {
+      return true;
+    }
+  }
+  return false;
+}
+
 // Write this tile's share of every monitor term after sub-step `s`.
 // FIELD(id, li, lj) returns the LDS value of field `id` at local cell.
 template <typename T, int RX, int RY, typename F>
@@ -196,6 +217,10 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
     }
   }
   __syncthreads();
+  // uniform per tile: does the source reach the LDS region, do taps reach the owned block
+  const bool src_tile = p.src.present && p.src.i0 < a0 + RX && p.src.i1 > a0 && p.src.j0 < b0 + RY &&
+                        p.src.j1 > b0;
+  const bool tap_tile = taps.n && taps_touch(taps, I0, J0, TX, TY);
   for (int s = 0; s < nsub; ++s) {
     const T coef = p.coef[s];
     // D half (stepper_te.py:80-103,131-134,160-168)
@@ -237,7 +262,7 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
           }
           dyv = dyv - dt * curl;
         }
-        if (p.src.present && i >= p.src.i0 && i < p.src.i1 && j >= p.src.j0 && j < p.src.j1)
+        if (src_tile && i >= p.src.i0 && i < p.src.i1 && j >= p.src.j0 && j < p.src.j1)
           dyv = dyv - coef * static_cast<const T*>(p.src.w)[(long long)(i - p.src.i0) * (p.src.j1 - p.src.j0) +
                                                              (j - p.src.j0)];
         vDy[u] = dyv;
@@ -277,7 +302,7 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
       }
     }
     __syncthreads();
-    if (taps.n) {
+    if (tap_tile) {
       write_taps<T, RX, RY>(taps, s, I0, J0, TX, TY, a0, b0, [&](int f, int li, int lj) -> T {
         int c = li * RY + lj;
         return f == 0 ? sHz[c] : f == 1 ? sEx[c] : f == 2 ? sEy[c] : f == 3 ? sDx[c] : sDy[c];
@@ -373,6 +398,10 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
     }
   }
   __syncthreads();
+  // uniform per tile: does the source reach the LDS region, do taps reach the owned block
+  const bool src_tile = p.src.present && p.src.i0 < a0 + RX && p.src.i1 > a0 && p.src.j0 < b0 + RY &&
+                        p.src.j1 > b0;
+  const bool tap_tile = taps.n && taps_touch(taps, I0, J0, TX, TY);
   for (int s = 0; s < nsub; ++s) {
     const T coef = p.coef_tab ? p.coef_tab[coef_t0 + s] : p.coef[s];
     // B half (stepper_tm.py:85-92,126-128)
@@ -445,7 +474,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
         }
         dz = dz + dt * curl;
       }
-      if (p.src.present && i >= p.src.i0 && i < p.src.i1 && j >= p.src.j0 && j < p.src.j1)
+      if (src_tile && i >= p.src.i0 && i < p.src.i1 && j >= p.src.j0 && j < p.src.j1)
         dz = dz - coef * static_cast<const T*>(p.src.w)[(long long)(i - p.src.i0) * (p.src.j1 - p.src.j0) +
                                                         (j - p.src.j0)];
       vDz[u] = dz;
@@ -460,7 +489,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
           p.tape[(long long)(tape_s0 + s) * nx * ny + (long long)i * ny + j] = sEz[(i - a0) * RY + (j - b0)];
       }
     }
-    if (taps.n) {
+    if (tap_tile) {
       write_taps<T, RX, RY>(taps, s, I0, J0, TX, TY, a0, b0, [&](int f, int li, int lj) -> T {
         int c = li * RY + lj;
         return f == 0 ? sEz[c] : f == 1 ? sHx[c] : f == 2 ? sHy[c] : sDz[c];
