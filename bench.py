@@ -291,6 +291,8 @@ def main():
     if args.workload == "c3":
         from meep_torch_amd.core import engine3d
         variant = engine3d.DEFAULT_VARIANT
+    elif tune._CACHE and len(set(tune._CACHE.values())) == 1:
+        variant = next(iter(tune._CACHE.values()))     # the tile variant the autotuner chose
     traffic = _traffic(args.workload, per_launch_cells, variant)
     line = {
         "metric": "Yee cell-updates/s (FP32) with % of MI355X HBM roofline",
