@@ -29,7 +29,13 @@
 namespace mt {
 namespace {
 
-constexpr int BLK = 512;  // threads per tile workgroup
+// Threads per tile workgroup.  TM tiles run 1024 threads (3-6 cells per
+// thread at NS = 4): the TM step is light and the design loops' grids are
+// small, so more waves per tile shorten each launch.  TE tiles run 512: on
+// 2048^2 the interior TE launch is arithmetic-bound and 1024-thread tiles
+// measured slower (98 -> 123 us per launch).
+constexpr int BLK_TE = 512;
+constexpr int BLK_TM = 1024;
 
 template <typename T>
 __device__ __forceinline__ T dget(const T* p, long long idx, bool ok) {
@@ -156,9 +162,9 @@ __device__ void write_taps(const TapSet& taps, int s, int I0, int J0, int TXo, i
 
 // ------------------------------------------------------------------- TE
 template <typename T, int TX, int TY, int NS, bool PML>
-__global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int nsub, int write_e) {
+__global__ __launch_bounds__(BLK_TE) void te_tiled(TeTiled<T> p, TapSet taps, int nsub, int write_e) {
   constexpr int RX = TX + 2 * NS + 1, RY = TY + 2 * NS + 1, RC = RX * RY;
-  constexpr int PER = (RC + BLK - 1) / BLK;
+  constexpr int PER = (RC + BLK_TE - 1) / BLK_TE;
   __shared__ T sHz[RC], sDx[RC], sDy[RC], sEx[RC], sEy[RC];
   __shared__ T sP[PML ? 4 * RC : 1];
   T* sPzy = sP;            // psi_Hzy: Dx update, y shell (index (i, j-1))
@@ -182,7 +188,7 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
     T vHz[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      int c = tid + u * BLK;
+      int c = tid + u * BLK_TE;
       int i = a0 + c / RY, j = b0 + c % RY;
       bool in = c < RC;
       bool hz = in && i >= 0 && i < nx - 1 && j >= 0 && j < ny - 1;
@@ -196,7 +202,7 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
     }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      int c = tid + u * BLK;
+      int c = tid + u * BLK_TE;
       if (c < RC) {
         sHz[c] = vHz[u];
         sDx[c] = vDx[u];
@@ -205,7 +211,7 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
     }
   }
   if (PML) {
-    for (int c = tid; c < RC; c += BLK) {
+    for (int c = tid; c < RC; c += BLK_TE) {
       int i = a0 + c / RY, j = b0 + c % RY;
       bool hz = i >= 0 && i < nx - 1 && j >= 0 && j < ny - 1;
       bool ex = i >= 0 && i < nx - 1 && j >= 0 && j < ny;
@@ -226,7 +232,7 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
     // D half (stepper_te.py:80-103,131-134,160-168)
     int li = tid / RY, lj = tid % RY;
 #pragma unroll
-    for (int u = 0; u < PER; ++u, li += BLK / RY, lj += BLK % RY) {
+    for (int u = 0; u < PER; ++u, li += BLK_TE / RY, lj += BLK_TE % RY) {
       if (lj >= RY) {
         lj -= RY;
         ++li;
@@ -275,7 +281,7 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
     li = tid / RY;
     lj = tid % RY;
 #pragma unroll
-    for (int u = 0; u < PER; ++u, li += BLK / RY, lj += BLK % RY) {
+    for (int u = 0; u < PER; ++u, li += BLK_TE / RY, lj += BLK_TE % RY) {
       if (lj >= RY) {
         lj -= RY;
         ++li;
@@ -311,7 +317,7 @@ __global__ __launch_bounds__(BLK) void te_tiled(TeTiled<T> p, TapSet taps, int n
     }
   }
   // write back the owned block
-  for (int c = tid; c < TX * TY; c += BLK) {
+  for (int c = tid; c < TX * TY; c += BLK_TE) {
     int i = I0 + c / TY, j = J0 + c % TY;
     int lc = (i - a0) * RY + (j - b0);
     if (i < nx - 1 && j < ny - 1) {
@@ -342,7 +348,7 @@ template <typename T, int TX, int TY, int NS, bool PML>
 __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps, int nsub, int tape_s0, int tr,
                                         int tc, int slot_base, int coef_t0) {
   constexpr int RX = TX + 2 * NS + 1, RY = TY + 2 * NS + 1, RC = RX * RY;
-  constexpr int PER = (RC + BLK - 1) / BLK;
+  constexpr int PER = (RC + BLK_TM - 1) / BLK_TM;
   __shared__ T sEz[RC], sDz[RC], sHx[RC], sHy[RC];
   __shared__ T sP[PML ? 4 * RC : 1];
   T* sPzy = sP;            // psi_Ezy: Hx update, y shell (index (i, j))
@@ -362,7 +368,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
     T vEz[PER], vHx[PER], vHy[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      int c = tid + u * BLK;
+      int c = tid + u * BLK_TM;
       int i = a0 + c / RY, j = b0 + c % RY;
       bool ez = c < RC && i >= 0 && i < nx && j >= 0 && j < ny;
       vEz[u] = dget(p.Ez, (long long)i * ny + j, ez);
@@ -373,7 +379,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
     }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      int c = tid + u * BLK;
+      int c = tid + u * BLK_TM;
       if (c < RC) {
         sEz[c] = vEz[u];
         sDz[c] = vDz[u];
@@ -383,7 +389,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
     }
   }
   if (PML) {
-    for (int c = tid; c < RC; c += BLK) {
+    for (int c = tid; c < RC; c += BLK_TM) {
       int i = a0 + c / RY, j = b0 + c % RY;
       bool ez = i >= 0 && i < nx && j >= 0 && j < ny;
       bool hx = ez && j < ny - 1;
@@ -407,7 +413,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
     // B half (stepper_tm.py:85-92,126-128)
     int li = tid / RY, lj = tid % RY;
 #pragma unroll
-    for (int u = 0; u < PER; ++u, li += BLK / RY, lj += BLK % RY) {
+    for (int u = 0; u < PER; ++u, li += BLK_TM / RY, lj += BLK_TM % RY) {
       if (lj >= RY) {
         lj -= RY;
         ++li;
@@ -443,7 +449,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
     li = tid / RY;
     lj = tid % RY;
 #pragma unroll
-    for (int u = 0; u < PER; ++u, li += BLK / RY, lj += BLK % RY) {
+    for (int u = 0; u < PER; ++u, li += BLK_TM / RY, lj += BLK_TM % RY) {
       if (lj >= RY) {
         lj -= RY;
         ++li;
@@ -483,7 +489,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
     }
     __syncthreads();
     if (p.tape) {
-      for (int c = tid; c < TX * TY; c += BLK) {
+      for (int c = tid; c < TX * TY; c += BLK_TM) {
         int i = I0 + c / TY, j = J0 + c % TY;
         if (i < nx && j < ny)
           p.tape[(long long)(tape_s0 + s) * nx * ny + (long long)i * ny + j] = sEz[(i - a0) * RY + (j - b0)];
@@ -497,7 +503,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
       __syncthreads();
     }
   }
-  for (int c = tid; c < TX * TY; c += BLK) {
+  for (int c = tid; c < TX * TY; c += BLK_TM) {
     int i = I0 + c / TY, j = J0 + c % TY;
     if (i >= nx || j >= ny) continue;
     int lc = (i - a0) * RY + (j - b0);
@@ -525,7 +531,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
 }
 
 template <typename T, int TX, int TY, int NS, bool PML>
-__global__ __launch_bounds__(BLK) void tm_tiled(TmTiled<T> p, TapSet taps, int nsub, int tape_s0) {
+__global__ __launch_bounds__(BLK_TM) void tm_tiled(TmTiled<T> p, TapSet taps, int nsub, int tape_s0) {
   int tr, tc;
   tile_of(PML, p.map, &tr, &tc);
   tm_tile<T, TX, TY, NS, PML>(p, taps, nsub, tape_s0, tr, tc, 0, 0);
@@ -536,7 +542,7 @@ __global__ __launch_bounds__(BLK) void tm_tiled(TmTiled<T> p, TapSet taps, int n
 // orientations), taps and tile maps live in device memory, so the launch
 // depends only on scalars and can be captured in a graph and replayed.
 template <typename T, int TX, int TY, int NS, bool PML>
-__global__ __launch_bounds__(BLK) void tm_batch(const TmBatchVar<T>* vars, const int* prefix, int nvar,
+__global__ __launch_bounds__(BLK_TM) void tm_batch(const TmBatchVar<T>* vars, const int* prefix, int nvar,
                                                 int nsub, int parity, int t0, int slot_base, int tape_s0) {
   const int total = prefix[nvar];
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
@@ -595,8 +601,8 @@ int launch_te_tiled(TeTiled<T> p, const TapSet& taps, int nsub, int write_e, hip
   p.map = make_map<TX, TY, NS>(p.nx, p.ny, p.in_i0, p.in_i1, p.in_j0, p.in_j1);
   launch_pair<T>(
       p.map,
-      [&](dim3 g) { hipLaunchKernelGGL((te_tiled<T, TX, TY, NS, false>), g, dim3(BLK), 0, s, p, taps, nsub, write_e); },
-      [&](dim3 g) { hipLaunchKernelGGL((te_tiled<T, TX, TY, NS, true>), g, dim3(BLK), 0, s, p, taps, nsub, write_e); },
+      [&](dim3 g) { hipLaunchKernelGGL((te_tiled<T, TX, TY, NS, false>), g, dim3(BLK_TE), 0, s, p, taps, nsub, write_e); },
+      [&](dim3 g) { hipLaunchKernelGGL((te_tiled<T, TX, TY, NS, true>), g, dim3(BLK_TE), 0, s, p, taps, nsub, write_e); },
       s, (long long)TX * TY, nsub);
   return check_launch("te2d tiled step");
 }
@@ -606,8 +612,8 @@ int launch_tm_tiled(TmTiled<T> p, const TapSet& taps, int nsub, int tape_s0, hip
   p.map = make_map<TX, TY, NS>(p.nx, p.ny, p.in_i0, p.in_i1, p.in_j0, p.in_j1);
   launch_pair<T>(
       p.map,
-      [&](dim3 g) { hipLaunchKernelGGL((tm_tiled<T, TX, TY, NS, false>), g, dim3(BLK), 0, s, p, taps, nsub, tape_s0); },
-      [&](dim3 g) { hipLaunchKernelGGL((tm_tiled<T, TX, TY, NS, true>), g, dim3(BLK), 0, s, p, taps, nsub, tape_s0); },
+      [&](dim3 g) { hipLaunchKernelGGL((tm_tiled<T, TX, TY, NS, false>), g, dim3(BLK_TM), 0, s, p, taps, nsub, tape_s0); },
+      [&](dim3 g) { hipLaunchKernelGGL((tm_tiled<T, TX, TY, NS, true>), g, dim3(BLK_TM), 0, s, p, taps, nsub, tape_s0); },
       s, (long long)TX * TY, nsub);
   return check_launch("tm2d tiled step");
 }
@@ -617,10 +623,10 @@ int launch_tm_batch(const TmBatchVar<T>* vars, const int* pre_in, const int* pre
                     int n_ring, int grid_in, int grid_ring, int nsub, int parity, int t0, int slot_base,
                     int tape_s0, hipStream_t s) {
   if (n_in)
-    hipLaunchKernelGGL((tm_batch<T, TX, TY, NS, false>), dim3(std::min(n_in, grid_in)), dim3(BLK), 0, s, vars,
+    hipLaunchKernelGGL((tm_batch<T, TX, TY, NS, false>), dim3(std::min(n_in, grid_in)), dim3(BLK_TM), 0, s, vars,
                        pre_in, nvar, nsub, parity, t0, slot_base, tape_s0);
   if (n_ring)
-    hipLaunchKernelGGL((tm_batch<T, TX, TY, NS, true>), dim3(std::min(n_ring, grid_ring)), dim3(BLK), 0, s, vars,
+    hipLaunchKernelGGL((tm_batch<T, TX, TY, NS, true>), dim3(std::min(n_ring, grid_ring)), dim3(BLK_TM), 0, s, vars,
                        pre_ring, nvar, nsub, parity, t0, slot_base, tape_s0);
   return check_launch("tm2d batched step");
 }
