@@ -17,7 +17,6 @@ Prints one JSON line on stdout (progress goes to stderr).
 import argparse
 import ctypes
 import json
-import math
 import os
 import sys
 import time

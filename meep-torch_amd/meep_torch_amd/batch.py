@@ -23,7 +23,6 @@ import torch
 
 from . import _engine as E
 from .core import engine2d
-from .core.monitors import omega_of
 from .core.stepper_tm import FIELDS
 
 _ORDER = engine2d.TM_ORDER

@@ -3,12 +3,11 @@
 Same constructors, result shapes and flux formula as the reference; sampling
 runs inside the engine (four cropped plane terms per flux plane, one term per
 DFT plane)."""
-import math
 
 import torch
 
 from .engine3d import run_fields
-from .stepper_3d import COMPS, PSI_NAMES, State3D
+from .stepper_3d import COMPS, PSI_NAMES
 
 
 class PlaneSource:

@@ -5,14 +5,12 @@ Lean3D keeps the reference attributes: F (fields, updated in place), psi as
 lean form (E += curl * inv_eps * dt; sources subtract), which rounds
 differently from step_3d.  ``compile`` is accepted and ignored: the step is
 already one pair of fused HIP kernels."""
-import math
 from dataclasses import dataclass, field
 
 import torch
 
 from .engine3d import run_fields
-from .pml import cpml_profiles
-from .stepper_3d import COMPS, PSI_NAMES, build_3d_coeffs
+from .stepper_3d import build_3d_coeffs
 
 __all__ = ["build_lean", "run_3d_lean", "step_lean", "Lean3D"]
 

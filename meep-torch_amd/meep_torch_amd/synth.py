@@ -14,7 +14,6 @@ C3  3D 384^3, dx=0.05, SiO2 eps=2.1 below z0, Si slab eps=12.1 (22 cells thick
     input, GaussianPulse(0.5, 0.2), 2000 steps, 2 x-normal flux planes x 64
     frequencies in [0.4, 0.6].
 """
-import math
 
 import numpy as np
 import torch
