@@ -31,11 +31,14 @@ namespace {
 
 // Threads per tile workgroup.  TM tiles run 1024 threads (3-6 cells per
 // thread at NS = 4): the TM step is light and the design loops' grids are
-// small, so more waves per tile shorten each launch.  TE tiles run 512: on
-// 2048^2 the interior TE launch is arithmetic-bound and 1024-thread tiles
-// measured slower (98 -> 123 us per launch).
-constexpr int BLK_TE = 512;
+// small, so more waves per tile shorten each launch.  TE interior tiles run
+// 512: on 2048^2 the interior TE launch is arithmetic-bound and 1024-thread
+// interior tiles measured slower (98 -> 123 us per launch).
 constexpr int BLK_TM = 1024;
+// TE: 512 threads for interior tiles; CPML ring tiles (few, each carrying
+// the psi recursions, so the ring launch is one latency chain per CU) run
+// 1024 threads, which shortens each thread's share of a tile.
+constexpr int te_block(bool pml) { return pml ? 1024 : 512; }
 
 template <typename T>
 __device__ __forceinline__ T dget(const T* p, long long idx, bool ok) {
@@ -162,7 +165,8 @@ __device__ void write_taps(const TapSet& taps, int s, int I0, int J0, int TXo, i
 
 // ------------------------------------------------------------------- TE
 template <typename T, int TX, int TY, int NS, bool PML>
-__global__ __launch_bounds__(BLK_TE) void te_tiled(TeTiled<T> p, TapSet taps, int nsub, int write_e) {
+__global__ __launch_bounds__(te_block(PML)) void te_tiled(TeTiled<T> p, TapSet taps, int nsub, int write_e) {
+  constexpr int BLK_TE = te_block(PML);
   constexpr int RX = TX + 2 * NS + 1, RY = TY + 2 * NS + 1, RC = RX * RY;
   constexpr int PER = (RC + BLK_TE - 1) / BLK_TE;
   __shared__ T sHz[RC], sDx[RC], sDy[RC], sEx[RC], sEy[RC];
@@ -601,8 +605,8 @@ int launch_te_tiled(TeTiled<T> p, const TapSet& taps, int nsub, int write_e, hip
   p.map = make_map<TX, TY, NS>(p.nx, p.ny, p.in_i0, p.in_i1, p.in_j0, p.in_j1);
   launch_pair<T>(
       p.map,
-      [&](dim3 g) { hipLaunchKernelGGL((te_tiled<T, TX, TY, NS, false>), g, dim3(BLK_TE), 0, s, p, taps, nsub, write_e); },
-      [&](dim3 g) { hipLaunchKernelGGL((te_tiled<T, TX, TY, NS, true>), g, dim3(BLK_TE), 0, s, p, taps, nsub, write_e); },
+      [&](dim3 g) { hipLaunchKernelGGL((te_tiled<T, TX, TY, NS, false>), g, dim3(te_block(false)), 0, s, p, taps, nsub, write_e); },
+      [&](dim3 g) { hipLaunchKernelGGL((te_tiled<T, TX, TY, NS, true>), g, dim3(te_block(true)), 0, s, p, taps, nsub, write_e); },
       s, (long long)TX * TY, nsub);
   return check_launch("te2d tiled step");
 }
