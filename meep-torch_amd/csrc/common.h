@@ -29,19 +29,23 @@ inline Pml<T> pml_of(const mt_pml_axis& ax) {
 
 int check_axis(const mt_pml_axis& ax, int n, const char* name);
 
-// a / b for the constant grid step b with y = RN(1/b): q0 = a y, then one
-// FMA correction with the exact remainder a - q0 b gives the correctly
+// a / b for the constant grid step b > 0 with y = RN(1/b): q0 = a y, then
+// one FMA correction with the exact remainder a - q0 b gives the correctly
 // rounded quotient (Markstein's theorem: y within half an ulp of 1/b, q0
-// within one ulp of a/b, exact remainder; checked exhaustively against IEEE
-// division over |a| in [2^-100, 2^100] for several b).  Outside that range
-// the true division is used.
+// within one ulp of a/b, exact remainder; checked against IEEE division over
+// |a| in [2^-100, 2^100] by tests/test_div_const_cpu.py).  a = +-0 gives
+// +-0 exactly (the correction would turn -0 into +0, so zero is selected
+// explicitly); the rest of the range (tiny or huge |a|) takes the true
+// division.  Zero is common (fields ahead of a wavefront), so keeping it on
+// the fast path spares whole waves the divergent division branch.
 template <typename T>
 __device__ __forceinline__ T div_c(T a, T b, T y) {
   T aa = a < T(0) ? -a : a;
-  if (aa >= T(0x1p-100) && aa <= T(0x1p100)) {
+  if ((aa >= T(0x1p-100) && aa <= T(0x1p100)) || aa == T(0)) {
     T q0 = a * y;
     T r = fma(-q0, b, a);
-    return fma(r, y, q0);
+    T q1 = fma(r, y, q0);
+    return aa == T(0) ? a : q1;
   }
   return a / b;
 }
