@@ -103,8 +103,7 @@ typedef struct {
    * slot s = tape + s * (|Ex| + |Ey| + |Ez|) as [Ex | Ey | Ez]
    * (reference shapes), for the adjoint's eps gradient */
   void* tape;
-  /* variant 0: two passes per step (H kernel, E kernel); 4: the same two
-   * passes with each thread streaming a (j, k) column along x; 1-3: one
+  /* variant 0: two passes per step (H kernel, E kernel); 1-3: one
    * fused pass per step streaming along x (tile rows 8, 4, 16), which needs alt:
    * second buffers of the 6 fields then the 12 psi arrays (same shapes;
    * the psi ones zero-filled), used ping-pong; the state ends in the
