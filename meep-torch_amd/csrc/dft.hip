@@ -97,33 +97,57 @@ __global__ void phase_kernel(const T* omega, int nfreq, int n_filled, int t0, do
   o[3] = sin(ph);
 }
 
-// One thread per (element, frequency, term): the accumulator pair is read and
-// written once per flush, the staged samples are read coalesced along the
-// element axis and the phase row (one frequency per block) is uniform across
-// the block.  Accumulator layout (nfreq, 2*n_terms, n).
-template <typename T>
+// One thread per (element, term) and group of G frequencies: the
+// accumulator pairs are read and written once per flush, each staged sample
+// is read once per group (coalesced along the element axis) and feeds G
+// independent accumulations, and the phase rows are uniform across the
+// block.  Every accumulator still adds its steps one by one in step order,
+// so the result does not depend on the grouping.  G = 4 for large monitors
+// (3D planes: the staged samples, re-read once per group, are far larger
+// than L2); G = 1 where that would leave too few threads.  Accumulator
+// layout (nfreq, 2*n_terms, n).
+template <typename T, int G>
 __global__ void flush_kernel(const T* stage, const T* table, T* acc, int n_terms, int n, int nfreq,
                              int n_filled, int half_mask, T dt) {
+  constexpr int kFlushF = G;
   int e = blockIdx.x * blockDim.x + threadIdx.x;
-  int f = blockIdx.y;
+  const int f0 = blockIdx.y * kFlushF;
   int q = blockIdx.z;
   if (e >= n) return;
+  const int nf = nfreq - f0 < kFlushF ? nfreq - f0 : kFlushF;   // uniform over the block
   const T* st = stage + (long long)q * n + e;  // slot stride = n_terms * n
   long long slot_stride = (long long)n_terms * n;
   int toff = ((half_mask >> q) & 1) ? 2 : 0;
-  T* are = acc + ((long long)f * 2 * n_terms + 2 * q) * n + e;
-  T* aim = are + n;
-  T re = *are, im = *aim;
-  const T* tb = table + (long long)f * n_filled * 4 + toff;
+  T re[kFlushF], im[kFlushF];
+#pragma unroll
+  for (int g = 0; g < kFlushF; ++g) {
+    if (g < nf) {
+      const T* a = acc + ((long long)(f0 + g) * 2 * n_terms + 2 * q) * n + e;
+      re[g] = a[0];
+      im[g] = a[n];
+    }
+  }
   for (int s = 0; s < n_filled; ++s) {
     T v = st[(long long)s * slot_stride];
-    T c = tb[s * 4 + 0];
-    T sn = tb[s * 4 + 1];
-    re = re + dt * (c * v);
-    im = im + dt * ((-sn) * v);
+#pragma unroll
+    for (int g = 0; g < kFlushF; ++g) {
+      if (g < nf) {
+        const T* tb = table + ((long long)(f0 + g) * n_filled + s) * 4 + toff;
+        T c = tb[0];
+        T sn = tb[1];
+        re[g] = re[g] + dt * (c * v);
+        im[g] = im[g] + dt * ((-sn) * v);
+      }
+    }
   }
-  *are = re;
-  *aim = im;
+#pragma unroll
+  for (int g = 0; g < kFlushF; ++g) {
+    if (g < nf) {
+      T* a = acc + ((long long)(f0 + g) * 2 * n_terms + 2 * q) * n + e;
+      a[0] = re[g];
+      a[n] = im[g];
+    }
+  }
 }
 
 template <typename T>
@@ -280,10 +304,17 @@ static int flush_t(const MonitorPlan& mp, int n_filled, int t0, double dt, hipSt
     int half_mask = 0;
     for (int q = 0; q < m.n_terms; ++q) half_mask |= (m.term[q].t_half ? 1 : 0) << q;
     int blk = n >= 256 ? 256 : 64;
-    dim3 grid((unsigned)((n + blk - 1) / blk), (unsigned)m.nfreq, (unsigned)m.n_terms);
-    hipLaunchKernelGGL((flush_kernel<T>), grid, dim3(blk), 0, s, (const T*)stage, (const T*)table,
-                       static_cast<T*>(m.acc), m.n_terms, (int)n, m.nfreq, n_filled, half_mask,
-                       static_cast<T>(dt));
+    const bool grouped = (long long)n * m.n_terms * ((m.nfreq + 3) / 4) >= (1LL << 18);
+    const int G = grouped ? 4 : 1;
+    dim3 grid((unsigned)((n + blk - 1) / blk), (unsigned)((m.nfreq + G - 1) / G), (unsigned)m.n_terms);
+    if (grouped)
+      hipLaunchKernelGGL((flush_kernel<T, 4>), grid, dim3(blk), 0, s, (const T*)stage, (const T*)table,
+                         static_cast<T*>(m.acc), m.n_terms, (int)n, m.nfreq, n_filled, half_mask,
+                         static_cast<T>(dt));
+    else
+      hipLaunchKernelGGL((flush_kernel<T, 1>), grid, dim3(blk), 0, s, (const T*)stage, (const T*)table,
+                         static_cast<T*>(m.acc), m.n_terms, (int)n, m.nfreq, n_filled, half_mask,
+                         static_cast<T>(dt));
     int rc = check_launch("monitor flush");
     if (rc) return rc;
   }
