@@ -74,6 +74,10 @@ typedef struct {
    * psi_Hzy psi_Hzx (alt[7] unused).  Ex/Ey are written at the end of the call. */
   int variant, pad2_;
   void* alt[8];
+  /* optional device tape (two-pass path only): E after step s of a call is
+   * written to slot s = tape + s * ((nx-1)*ny + nx*(ny-1)) as [Ex | Ey], for
+   * mt_te2d_adjoint's eps gradient */
+  void* tape;
 } mt_te2d;
 
 /* Soft source on a 2D field: dense weights on the bounding box
@@ -208,6 +212,25 @@ typedef struct {
  * chunks (meep_torch/core/run.py:99-120, eps reached via stepper_tm.py:226-227). */
 int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const void* tape,
                     const mt_monitor* mons, int n_mons, int step0, int k, void* stream);
+
+/* Adjoint state and scratch of one TE problem (same shapes as the forward
+ * state; psi cotangents are tracked inside the CPML shells only). */
+typedef struct {
+  void *aHz, *aEx, *aEy, *aDx, *aDy;
+  void *apsi_Eyx, *apsi_Exy, *apsi_Hzy, *apsi_Hzx;
+  void *gEy, *gEx;            /* scratch (nx-1,ny-1) each */
+  void *gDx, *gDy;            /* scratch (nx-1,ny) (nx,ny-1) */
+  void *epsx_grad, *epsy_grad;  /* (nx-1,ny) (nx,ny-1), accumulated */
+} mt_te2d_adj;
+
+/* Reverse sweep of k TE steps starting at global step step0 (the gradient of
+ * torch.utils.checkpoint'ed run_te chunks, meep_torch/core/run.py:81-118,
+ * through step_te, meep_torch/core/stepper_te.py:73-190): for s = k-1..0 the
+ * monitor cotangents of step step0+s are injected, then the exact transpose
+ * of the step runs, with E after the step read from tape slot s (as written
+ * by mt_te2d_run).  Accumulates dL/deps_x and dL/deps_y. */
+int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const void* tape, const mt_monitor* mons,
+                    int n_mons, int step0, int k, void* stream);
 
 /* Adjoint state of the 3D functional stepper (sigma = 0, CPML): cotangents of
  * every state array (reference shapes, updated in place), 12 scratch arrays

@@ -76,6 +76,202 @@ int adj_t(const mt_tm2d* d, const mt_tm2d_adj* a, const void* tape, const mt_mon
   return MT_OK;
 }
 
+// ------------------------------------------------------------------- TE
+// Reverse step of the TE Yee step (stepper_te.py:73-190): forward step t
+//   D half  gy = Dy Hz, psi_Hzy' = b psi + a gy, Dx' = Dx + dt (gy + psi')
+//           gx = Dx Hz, psi_Hzx' = b psi + a gx, Dy' = Dy - dt (gx + psi') - c_t mask
+//   E       Ex' = Dx' / eps_x, Ey' = Dy' / eps_y
+//   B half  Hz' = Hz - dt ((dEy + psi_Eyx') - dEx - psi_Exy')
+// and the monitors sample (Ex', Ey', Hz') after the step.  After the
+// monitor injection, three passes give the exact transpose:
+//   P1 (Hz cells)  curl cotangent -> B-half psi cotangents and the
+//                  E-difference cotangents gEy, gEx
+//   P2 (E/D cells) Ex', Ey' cotangents (transposed differences of gEx, gEy)
+//                  -> eps_x / eps_y gradients and Dx', Dy' cotangents; D-half
+//                  psi cotangents and the H-difference cotangents gDx, gDy
+//   P3 (Hz cells)  Hz cotangent += transposed differences of gDx, gDy
+template <typename T>
+struct AdjTe {
+  int nx, ny;
+  T dt, dx;
+  const T *eps_x, *eps_y;
+  Pml<T> hx, hy, ey, ex;
+  T *aHz, *aEx, *aEy, *aDx, *aDy, *apEyx, *apExy, *apHzy, *apHzx;
+  T *gEy, *gEx, *gDx, *gDy, *gepsx, *gepsy;
+};
+
+template <typename T>
+__global__ __launch_bounds__(ABX* ABY) void te_adj_p1(AdjTe<T> p) {
+  const int j = blockIdx.x * ABX + threadIdx.x;
+  const int i = blockIdx.y * ABY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx - 1 || j >= ny - 1) return;
+  const long long h = (long long)i * (ny - 1) + j;
+  const bool sx = p.hx.in(i), sy = p.hy.in(j);
+  const T ahz = p.aHz[h];
+  T pyx = T(0), pxy = T(0), bx = T(0), ax = T(0), by = T(0), ay = T(0);
+  if (sx) {
+    pyx = p.apEyx[h];
+    bx = p.hx.b[i];
+    ax = p.hx.a[i];
+  }
+  if (sy) {
+    pxy = p.apExy[h];
+    by = p.hy.b[j];
+    ay = p.hy.a[j];
+  }
+  const T gc = -p.dt * ahz;  // cotangent of the curl
+  T gy = gc, gx = -gc;
+  if (sx) {
+    T t = pyx + gc;
+    gy = gc + ax * t;
+    p.apEyx[h] = bx * t;
+  }
+  if (sy) {
+    T t = pxy - gc;
+    gx = -gc + ay * t;
+    p.apExy[h] = by * t;
+  }
+  p.gEy[h] = gy;
+  p.gEx[h] = gx;
+}
+
+template <typename T>
+__global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, const T* tey) {
+  const int j = blockIdx.x * ABX + threadIdx.x;
+  const int i = blockIdx.y * ABY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx || j >= ny) return;
+  if (i < nx - 1) {  // Ex / Dx cell (nx-1, ny)
+    const long long c = (long long)i * ny + j;
+    const long long h = (long long)i * (ny - 1) + j;
+    const bool live = j >= 1 && j < ny - 1;
+    const int jj = j - 1;
+    const bool sh = live && p.ey.in(jj);
+    const long long q = (long long)i * (ny - 2) + jj;
+    const T gxm = j >= 1 ? p.gEx[h - 1] : T(0);
+    const T gx0 = j < ny - 1 ? p.gEx[h] : T(0);
+    const T aex = p.aEx[c], adx = p.aDx[c], epsx = p.eps_x[c], ex = tex[c], ge = p.gepsx[c];
+    T ps = T(0), b = T(0), a = T(0);
+    if (sh) {
+      ps = p.apHzy[q];
+      b = p.ey.b[jj];
+      a = p.ey.a[jj];
+    }
+    const T exb = aex + (gxm - gx0) / p.dx;
+    p.gepsx[c] = ge + (-exb * ex) / epsx;
+    const T dxb = adx + exb / epsx;
+    p.aDx[c] = dxb;
+    p.aEx[c] = T(0);
+    T gd = T(0);
+    if (live) {
+      const T g = p.dt * dxb;
+      gd = g;
+      if (sh) {
+        T t = ps + g;
+        gd = g + a * t;
+        p.apHzy[q] = b * t;
+      }
+    }
+    p.gDx[c] = gd;
+  }
+  if (j < ny - 1) {  // Ey / Dy cell (nx, ny-1)
+    const long long c = (long long)i * (ny - 1) + j;
+    const bool live = i >= 1 && i < nx - 1;
+    const int ii = i - 1;
+    const bool sh = live && p.ex.in(ii);
+    const long long q = (long long)ii * (ny - 1) + j;
+    const T gym = i >= 1 ? p.gEy[c - (ny - 1)] : T(0);
+    const T gy0 = i < nx - 1 ? p.gEy[c] : T(0);
+    const T aey = p.aEy[c], ady = p.aDy[c], epsy = p.eps_y[c], ey = tey[c], ge = p.gepsy[c];
+    T ps = T(0), b = T(0), a = T(0);
+    if (sh) {
+      ps = p.apHzx[q];
+      b = p.ex.b[ii];
+      a = p.ex.a[ii];
+    }
+    const T eyb = aey + (gym - gy0) / p.dx;
+    p.gepsy[c] = ge + (-eyb * ey) / epsy;
+    const T dyb = ady + eyb / epsy;
+    p.aDy[c] = dyb;
+    p.aEy[c] = T(0);
+    T gd = T(0);
+    if (live) {
+      const T g = -p.dt * dyb;
+      gd = g;
+      if (sh) {
+        T t = ps + g;
+        gd = g + a * t;
+        p.apHzx[q] = b * t;
+      }
+    }
+    p.gDy[c] = gd;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(ABX* ABY) void te_adj_p3(AdjTe<T> p) {
+  const int j = blockIdx.x * ABX + threadIdx.x;
+  const int i = blockIdx.y * ABY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx - 1 || j >= ny - 1) return;
+  const long long h = (long long)i * (ny - 1) + j;
+  const T d0 = p.gDx[(long long)i * ny + j], d1 = p.gDx[(long long)i * ny + j + 1];
+  const T e0 = p.gDy[(long long)i * (ny - 1) + j], e1 = p.gDy[(long long)(i + 1) * (ny - 1) + j];
+  p.aHz[h] = p.aHz[h] + (d0 - d1) / p.dx + (e0 - e1) / p.dx;
+}
+
+template <typename T>
+int te_adj_t(const mt_te2d* d, const mt_te2d_adj* a, const void* tape, const mt_monitor* mons, int n_mons,
+             int step0, int k, hipStream_t s) {
+  AdjTe<T> p;
+  p.nx = d->nx;
+  p.ny = d->ny;
+  p.dt = static_cast<T>(d->dt);
+  p.dx = static_cast<T>(d->dx);
+  p.eps_x = static_cast<const T*>(d->eps_x);
+  p.eps_y = static_cast<const T*>(d->eps_y);
+  p.hx = pml_of<T>(d->hx);
+  p.hy = pml_of<T>(d->hy);
+  p.ey = pml_of<T>(d->ey);
+  p.ex = pml_of<T>(d->ex);
+  p.aHz = static_cast<T*>(a->aHz);
+  p.aEx = static_cast<T*>(a->aEx);
+  p.aEy = static_cast<T*>(a->aEy);
+  p.aDx = static_cast<T*>(a->aDx);
+  p.aDy = static_cast<T*>(a->aDy);
+  p.apEyx = static_cast<T*>(a->apsi_Eyx);
+  p.apExy = static_cast<T*>(a->apsi_Exy);
+  p.apHzy = static_cast<T*>(a->apsi_Hzy);
+  p.apHzx = static_cast<T*>(a->apsi_Hzx);
+  p.gEy = static_cast<T*>(a->gEy);
+  p.gEx = static_cast<T*>(a->gEx);
+  p.gDx = static_cast<T*>(a->gDx);
+  p.gDy = static_cast<T*>(a->gDy);
+  p.gepsx = static_cast<T*>(a->epsx_grad);
+  p.gepsy = static_cast<T*>(a->epsy_grad);
+  const T* tp = static_cast<const T*>(tape);
+  const long long slot = (long long)(p.nx - 1) * p.ny + (long long)p.nx * (p.ny - 1);
+  dim3 blk(ABX, ABY), grd = grid2(p.ny, p.nx, blk);
+  for (int s_ = k - 1; s_ >= 0; --s_) {
+    const int t = step0 + s_;
+    for (int m = 0; m < n_mons; ++m) {
+      const mt_monitor& mm = mons[m];
+      const int n = mm.n0 * mm.n1;
+      if (n == 0) continue;
+      hipLaunchKernelGGL(adj_inject_kernel<T>, dim3((n + kInjE - 1) / kInjE), dim3(kInjE * kInjS), 0, s, mm, t,
+                         d->dt, s_);
+    }
+    const T* ts = tp + (long long)s_ * slot;
+    hipLaunchKernelGGL(te_adj_p1<T>, grd, blk, 0, s, p);
+    hipLaunchKernelGGL(te_adj_p2<T>, grd, blk, 0, s, p, ts, ts + (long long)(p.nx - 1) * p.ny);
+    hipLaunchKernelGGL(te_adj_p3<T>, grd, blk, 0, s, p);
+    int rc = check_launch("te2d adjoint step");
+    if (rc) return rc;
+  }
+  return MT_OK;
+}
+
 }  // namespace
 }  // namespace mt
 
@@ -106,4 +302,31 @@ extern "C" int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const voi
   hipStream_t s = static_cast<hipStream_t>(stream);
   return p->dtype == MT_F64 ? adj_t<double>(p, a, tape, mons, n_mons, step0, k, s)
                             : adj_t<float>(p, a, tape, mons, n_mons, step0, k, s);
+}
+
+extern "C" int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const void* tape, const mt_monitor* mons,
+                               int n_mons, int step0, int k, void* stream) {
+  if (!p || !a) return fail(MT_EINVAL, "null problem or adjoint state");
+  if (p->nx < 3 || p->ny < 3) return fail(MT_EINVAL, "grid must be at least 3x3");
+  if (k < 0) return fail(MT_EINVAL, "negative step count");
+  if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward E tape");
+  if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");
+  if (!p->eps_x || !p->eps_y || !a->aHz || !a->aEx || !a->aEy || !a->aDx || !a->aDy || !a->apsi_Eyx ||
+      !a->apsi_Exy || !a->apsi_Hzy || !a->apsi_Hzx || !a->gEy || !a->gEx || !a->gDx || !a->gDy ||
+      !a->epsx_grad || !a->epsy_grad)
+    return fail(MT_EINVAL, "null adjoint pointer");
+  int rc;
+  if ((rc = check_axis(p->hx, p->nx - 1, "hx")) || (rc = check_axis(p->hy, p->ny - 1, "hy")) ||
+      (rc = check_axis(p->ey, p->ny - 2, "ey")) || (rc = check_axis(p->ex, p->nx - 2, "ex")))
+    return rc;
+  for (int i = 0; i < n_mons; ++i) {
+    const mt_monitor& m = mons[i];
+    if (m.n_terms < 1 || m.n_terms > MT_MAX_TERMS || !m.acc) return fail(MT_EINVAL, "bad monitor");
+    if (m.kind == 0 && (m.nfreq < 1 || !m.omega)) return fail(MT_EINVAL, "monitor: no frequencies");
+    for (int q = 0; q < m.n_terms; ++q)
+      if (!m.term[q].a) return fail(MT_EINVAL, "monitor: null adjoint view");
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return p->dtype == MT_F64 ? te_adj_t<double>(p, a, tape, mons, n_mons, step0, k, s)
+                            : te_adj_t<float>(p, a, tape, mons, n_mons, step0, k, s);
 }

@@ -114,12 +114,13 @@ struct TeK {
   const T *eps_x, *eps_y;
   Pml<T> hx, hy, ey, ex;
   T *Hz, *Ex, *Ey, *Dx, *Dy, *pEyx, *pExy, *pHzy, *pHzx;
+  T* tape;  // optional: [Ex | Ey] after each step
 };
 
 // meep_torch/core/stepper_te.py:80-103 (D half), :131-134 (source on Dy),
 // :160-168 (E = D / eps)
 template <typename T>
-__global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src, T coef) {
+__global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src, T coef, int s_rel) {
   int j = blockIdx.x * BX + threadIdx.x;
   int i = blockIdx.y * BY + threadIdx.y;
   const int nx = p.nx, ny = p.ny;
@@ -140,7 +141,9 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
       dxv = dxv + p.dt * curl;
     }
     p.Dx[c] = dxv;
-    p.Ex[c] = dxv / p.eps_x[c];
+    T ex = dxv / p.eps_x[c];
+    p.Ex[c] = ex;
+    if (p.tape) p.tape[(long long)s_rel * ((long long)(nx - 1) * ny + (long long)nx * (ny - 1)) + c] = ex;
   }
   if (j < ny - 1) {  // Dy (nx, ny-1)
     long long c = (long long)i * (ny - 1) + j;
@@ -162,7 +165,10 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
       dyv = dyv - coef * w[(long long)(i - src.i0) * (src.j1 - src.j0) + (j - src.j0)];
     }
     p.Dy[c] = dyv;
-    p.Ey[c] = dyv / p.eps_y[c];
+    T ey = dyv / p.eps_y[c];
+    p.Ey[c] = ey;
+    if (p.tape)
+      p.tape[(long long)s_rel * ((long long)(nx - 1) * ny + (long long)nx * (ny - 1)) + (long long)(nx - 1) * ny + c] = ey;
   }
 }
 
@@ -268,6 +274,7 @@ int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const
   p.pExy = static_cast<T*>(d->psi_Exy);
   p.pHzy = static_cast<T*>(d->psi_Hzy);
   p.pHzx = static_cast<T*>(d->psi_Hzx);
+  p.tape = static_cast<T*>(d->tape);
   mt_source2d sv{};
   if (src && src->present) sv = *src;
   dim3 blk(BX, BY), grd = grid2(p.ny, p.nx, blk);
@@ -275,7 +282,7 @@ int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const
   int filled = 0, first = step0;
   for (int s_ = 0; s_ < k; ++s_) {
     T c = sv.present ? static_cast<T>(coef[s_]) : T(0);
-    hipLaunchKernelGGL(te_d_kernel<T>, grd, blk, 0, s, p, sv, c);
+    hipLaunchKernelGGL(te_d_kernel<T>, grd, blk, 0, s, p, sv, c, s_);
     hipLaunchKernelGGL(te_b_kernel<T>, grd, blk, 0, s, p);
     int rc = check_launch("te2d step");
     if (rc) return rc;
@@ -504,7 +511,7 @@ extern "C" int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const doubl
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (p->variant >= 0) {
+  if (p->variant >= 0 && !p->tape) {   // the tape is written by the two-pass kernels
     if (p->variant >= tiled_variant_count()) return fail(MT_EINVAL, "unknown tiled variant");
     if (!alt_ok(p->alt, 7)) return fail(MT_EINVAL, "tiled path needs the alt buffers");
     if (taps_ok(mp))

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -48,7 +48,14 @@ class TE2D(ctypes.Structure):
                 ("Hz", c_void_p), ("Ex", c_void_p), ("Ey", c_void_p), ("Dx", c_void_p),
                 ("Dy", c_void_p), ("psi_Eyx", c_void_p), ("psi_Exy", c_void_p),
                 ("psi_Hzy", c_void_p), ("psi_Hzx", c_void_p),
-                ("variant", c_int), ("pad2_", c_int), ("alt", c_void_p * 8)]
+                ("variant", c_int), ("pad2_", c_int), ("alt", c_void_p * 8), ("tape", c_void_p)]
+
+
+class TE2DAdj(ctypes.Structure):
+    _fields_ = [("aHz", c_void_p), ("aEx", c_void_p), ("aEy", c_void_p), ("aDx", c_void_p),
+                ("aDy", c_void_p), ("apsi_Eyx", c_void_p), ("apsi_Exy", c_void_p),
+                ("apsi_Hzy", c_void_p), ("apsi_Hzx", c_void_p), ("gEy", c_void_p), ("gEx", c_void_p),
+                ("gDx", c_void_p), ("gDy", c_void_p), ("epsx_grad", c_void_p), ("epsy_grad", c_void_p)]
 
 
 class Source2D(ctypes.Structure):
@@ -117,6 +124,9 @@ def lib():
     L.mt_tiled_variants.argtypes = [c_int, P(c_int), P(c_int), P(c_int)]
     L.mt_tm2d_adjoint.restype = c_int
     L.mt_tm2d_adjoint.argtypes = [P(TM2D), P(TM2DAdj), c_void_p, P(Monitor), c_int, c_int, c_int,
+                                  c_void_p]
+    L.mt_te2d_adjoint.restype = c_int
+    L.mt_te2d_adjoint.argtypes = [P(TE2D), P(TE2DAdj), c_void_p, P(Monitor), c_int, c_int, c_int,
                                   c_void_p]
     L.mt_kernel_timing_enable.restype = c_int
     L.mt_kernel_timing_enable.argtypes = [c_int]

@@ -1,4 +1,4 @@
-"""Hand-written time-reversed adjoint for run_tm behind torch.autograd.Function.
+"""Hand-written time-reversed adjoint for run_tm / run_te behind torch.autograd.Function.
 
 Replaces the reference's autograd through torch.utils.checkpoint chunks
 (meep_torch/core/run.py:63-118).  Forward: the HIP stepper runs the whole
@@ -12,8 +12,11 @@ autograd as usual.
 
 Monitor accumulators are summed exactly as the forward-only run sums them (one
 sequential sum per reference chunk), so forward values are identical with or
-without gradients.  Supported: eps_inf as the only grad-requiring coefficient;
-TM mode; every monitor class of the 2D engine.  Gradients with respect to the
+without gradients.  Supported: eps_inf (TM) or eps_x / eps_y (TE) as the only
+grad-requiring coefficients; every monitor class of the 2D engine.  TE: the
+tape holds [Ex | Ey] after every step and mt_te2d_adjoint accumulates
+dL/deps_x and dL/deps_y, which autograd carries back through build_te_coeffs'
+face averaging to eps.  Gradients with respect to the
 initial state are returned for the fields (psi: inside the CPML shells)."""
 import ctypes
 
@@ -21,6 +24,7 @@ import torch
 
 from .. import _engine as E
 from . import engine2d
+from .stepper_te import FIELDS as TE_FIELDS
 from .stepper_tm import FIELDS
 
 DEFAULT_K = 200
@@ -60,37 +64,53 @@ def _adj_monitors(adj_fields, monitors, gaccs, dtype, device):
     return arr, len(used), keep
 
 
+def _forward_chunks(mode, names, work, cdet, cfg):
+    """Run the whole interval in place on ``work``, keeping a device copy of
+    the state at the start of every chunk; returns (accumulators, checkpoints)."""
+    plan, monitors, n_steps, ck = cfg["plan"], cfg["monitors"], cfg["n_steps"], cfg["ck"]
+    K = ck if ck > 0 else DEFAULT_K
+    acc = [None] * len(monitors)
+    ckpts = []
+    step0 = 0
+    raw = None
+    while step0 < n_steps:
+        k = min(K, n_steps - step0)
+        ckpts.append((step0, k, {f: work[f].clone() for f in names}))
+        # one sequential sum per reference chunk: keep summing into the
+        # same buffers unless checkpoint_every starts a new chunk
+        reuse = raw if (ck <= 0 and raw is not None) else None
+        raw = engine2d.run_chunk(mode, work, cdet, step0, k, plan, monitors, reuse=reuse, raw=True)
+        if ck > 0 or step0 + k >= n_steps:
+            outs = engine2d.finish_outputs(raw, work)
+            for i, (mon, out) in enumerate(zip(monitors, outs)):
+                if acc[i] is None:
+                    acc[i] = out
+                elif mon.reduce == "sum":
+                    acc[i] = acc[i] + out
+                else:
+                    acc[i] = torch.cat([acc[i], out])
+        elif any(kind == "probe" for _, kind, _ in raw):
+            raise NotImplementedError("PointProbe with gradients needs checkpoint_every > 0")
+        step0 += k
+    return acc, ckpts
+
+
+def _chunk_cotangents(monitors, gacc, step0, k):
+    """Probe cotangents are per step: slice them per chunk."""
+    out = []
+    for mon, g in zip(monitors, gacc):
+        if g is not None and mon.reduce != "sum":
+            g = g[step0:step0 + k]
+        out.append(g)
+    return out
+
+
 class _TMRun(torch.autograd.Function):
     @staticmethod
     def forward(ctx, eps, cfg, *state_in):
-        coeffs, plan, monitors, n_steps, ck = (cfg["coeffs"], cfg["plan"], cfg["monitors"],
-                                               cfg["n_steps"], cfg["ck"])
-        cdet = coeffs._replace(eps_inf=eps.detach())
+        cdet = cfg["coeffs"]._replace(eps_inf=eps.detach())
         work = {f: t.detach().clone().contiguous() for f, t in zip(FIELDS, state_in)}
-        K = ck if ck > 0 else DEFAULT_K
-        acc = [None] * len(monitors)
-        ckpts = []
-        step0 = 0
-        raw = None
-        while step0 < n_steps:
-            k = min(K, n_steps - step0)
-            ckpts.append((step0, k, {f: work[f].clone() for f in FIELDS}))
-            # one sequential sum per reference chunk: keep summing into the
-            # same buffers unless checkpoint_every starts a new chunk
-            reuse = raw if (ck <= 0 and raw is not None) else None
-            raw = engine2d.run_chunk("tm", work, cdet, step0, k, plan, monitors, reuse=reuse, raw=True)
-            if ck > 0 or step0 + k >= n_steps:
-                outs = engine2d.finish_outputs(raw, work)
-                for i, (mon, out) in enumerate(zip(monitors, outs)):
-                    if acc[i] is None:
-                        acc[i] = out
-                    elif mon.reduce == "sum":
-                        acc[i] = acc[i] + out
-                    else:
-                        acc[i] = torch.cat([acc[i], out])
-            elif any(kind == "probe" for _, kind, _ in raw):
-                raise NotImplementedError("PointProbe with gradients needs checkpoint_every > 0")
-            step0 += k
+        acc, ckpts = _forward_chunks("tm", FIELDS, work, cdet, cfg)
         ctx.cfg = cfg
         ctx.cdet = cdet
         ctx.ckpts = ckpts
@@ -115,7 +135,6 @@ class _TMRun(torch.autograd.Function):
                    "gEx": torch.empty(nx - 1, ny, dtype=dtype, device=device),
                    "gEy": torch.empty(nx, ny - 1, dtype=dtype, device=device)}
         eps_grad = torch.zeros(nx, ny, dtype=dtype, device=device)
-        # probe cotangents are per step: slice them per chunk
         L = E.lib()
         a = E.TM2DAdj()
         for f in FIELDS:
@@ -128,11 +147,7 @@ class _TMRun(torch.autograd.Function):
             work = {f: t.clone() for f, t in st.items()}
             tape = torch.empty(k, nx, ny, dtype=dtype, device=device)
             engine2d.run_chunk("tm", work, cdet, step0, k, plan, [], tape=tape)
-            g_chunk = []
-            for mon, g in zip(monitors, gacc):
-                if g is not None and mon.reduce != "sum":
-                    g = g[step0:step0 + k]
-                g_chunk.append(g)
+            g_chunk = _chunk_cotangents(monitors, gacc, step0, k)
             arr, n_used, keep = _adj_monitors(adj, monitors, g_chunk, dtype, device)
             rc = L.mt_tm2d_adjoint(ctypes.byref(d), ctypes.byref(a), E.ptr(tape), arr, n_used,
                                    step0, k, E.stream_handle())
@@ -154,20 +169,94 @@ def _descriptor(c, fields, dtype, device):
     return d, eps
 
 
+class _TERun(torch.autograd.Function):
+    """TE counterpart of _TMRun: differentiable in eps_x and eps_y (the
+    face-averaged permittivities of TECoeffs, stepper_te.py:160-168)."""
+
+    @staticmethod
+    def forward(ctx, eps_x, eps_y, cfg, *state_in):
+        cdet = cfg["coeffs"]._replace(eps_x=eps_x.detach(), eps_y=eps_y.detach())
+        work = {f: t.detach().clone().contiguous() for f, t in zip(TE_FIELDS, state_in)}
+        acc, ckpts = _forward_chunks("te", TE_FIELDS, work, cdet, cfg)
+        ctx.cfg = cfg
+        ctx.cdet = cdet
+        ctx.ckpts = ckpts
+        ctx.save_for_backward(eps_x, eps_y)
+        ctx.n_state = len(state_in)
+        return (*[work[f] for f in TE_FIELDS], *acc)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        eps_x, eps_y = ctx.saved_tensors
+        cfg, cdet = ctx.cfg, ctx.cdet
+        monitors, plan = cfg["monitors"], cfg["plan"]
+        nf = len(TE_FIELDS)
+        gstate, gacc = grads[:nf], list(grads[nf:])
+        st0 = ctx.ckpts[0][2]
+        dtype, device = st0["Hz"].dtype, st0["Hz"].device
+        nx, ny = st0["Ey"].shape[0], st0["Ex"].shape[1]
+        adj = {f: (g.detach().to(dtype).clone().contiguous() if g is not None
+                   else torch.zeros_like(st0[f])) for f, g in zip(TE_FIELDS, gstate)}
+        scratch = {"gEy": torch.empty(nx - 1, ny - 1, dtype=dtype, device=device),
+                   "gEx": torch.empty(nx - 1, ny - 1, dtype=dtype, device=device),
+                   "gDx": torch.empty(nx - 1, ny, dtype=dtype, device=device),
+                   "gDy": torch.empty(nx, ny - 1, dtype=dtype, device=device)}
+        gx = torch.zeros(nx - 1, ny, dtype=dtype, device=device)
+        gy = torch.zeros(nx, ny - 1, dtype=dtype, device=device)
+        L = E.lib()
+        a = E.TE2DAdj()
+        for f in TE_FIELDS:
+            setattr(a, "a" + f, adj[f].data_ptr())
+        for f, t in scratch.items():
+            setattr(a, f, t.data_ptr())
+        a.epsx_grad, a.epsy_grad = gx.data_ptr(), gy.data_ptr()
+        d, d_keep = _te_descriptor(cdet, nx, ny, dtype, device)
+        slot = (nx - 1) * ny + nx * (ny - 1)
+        for step0, k, st in reversed(ctx.ckpts):
+            work = {f: t.clone() for f, t in st.items()}
+            tape = torch.empty(k, slot, dtype=dtype, device=device)
+            engine2d.run_chunk("te", work, cdet, step0, k, plan, [], tape=tape)
+            g_chunk = _chunk_cotangents(monitors, gacc, step0, k)
+            arr, n_used, keep = _adj_monitors(adj, monitors, g_chunk, dtype, device)
+            rc = L.mt_te2d_adjoint(ctypes.byref(d), ctypes.byref(a), E.ptr(tape), arr, n_used,
+                                   step0, k, E.stream_handle())
+            E.check(rc, "mt_te2d_adjoint")
+            del keep, tape
+        del d_keep
+        g_state = [adj[f] for f in TE_FIELDS][:ctx.n_state]
+        return (gx.to(eps_x.dtype), gy.to(eps_y.dtype), None, *g_state)
+
+
+def _te_descriptor(c, nx, ny, dtype, device):
+    d = E.TE2D()
+    d.nx, d.ny, d.dtype = nx, ny, E.dtype_code(dtype)
+    d.dt, d.dx = float(c.dt), float(c.dx)
+    ex_ = c.eps_x.to(device=device, dtype=dtype).contiguous()
+    ey_ = c.eps_y.to(device=device, dtype=dtype).contiguous()
+    d.eps_x, d.eps_y = ex_.data_ptr(), ey_.data_ptr()
+    d.hx, d.hy, d.ey, d.ex = (engine2d._axis(c, n) for n in ("hx", "hy", "ey", "ex"))
+    return d, (ex_, ey_)
+
+
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
-    if mode != "tm":
-        raise NotImplementedError("gradients through run_te are not supported (TM adjoint only)")
+    allowed = ("eps_inf",) if mode == "tm" else ("eps_x", "eps_y")
     for name, t in coeffs._asdict().items():
-        if torch.is_tensor(t) and t.requires_grad and name != "eps_inf":
+        if torch.is_tensor(t) and t.requires_grad and name not in allowed:
             raise NotImplementedError(f"gradient with respect to coefficient '{name}' is not supported")
     from .run import _validate
-    _validate("tm", state, coeffs)
-    eps = coeffs.eps_inf
-    E.require_device(eps)
-    plan = engine2d.source_plan(source, state.Dz.shape, state.Dz.device, state.Dz.dtype)
-    cfg = {"coeffs": coeffs, "plan": plan, "monitors": list(monitors), "n_steps": n_steps,
-           "ck": checkpoint_every}
-    outs = _TMRun.apply(eps, cfg, *[getattr(state, f) for f in FIELDS])
-    nf = len(FIELDS)
-    new_state = state._replace(**dict(zip(FIELDS, outs[:nf])))
+    _validate(mode, state, coeffs)
+    cfg = {"coeffs": coeffs, "monitors": list(monitors), "n_steps": n_steps, "ck": checkpoint_every}
+    if mode == "tm":
+        names, eps = FIELDS, (coeffs.eps_inf,)
+        tgt = state.Dz
+        fn = _TMRun
+    else:
+        names, eps = TE_FIELDS, (coeffs.eps_x, coeffs.eps_y)
+        tgt = state.Dy
+        fn = _TERun
+    E.require_device(*eps)
+    cfg["plan"] = engine2d.source_plan(source, tgt.shape, tgt.device, tgt.dtype)
+    outs = fn.apply(*eps, cfg, *[getattr(state, f) for f in names])
+    nf = len(names)
+    new_state = state._replace(**dict(zip(names, outs[:nf])))
     return new_state, list(outs[nf:])

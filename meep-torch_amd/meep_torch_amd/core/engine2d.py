@@ -131,7 +131,8 @@ TE_ORDER = ("Hz", "Dx", "Dy", "psi_Eyx", "psi_Exy", "psi_Hzy", "psi_Hzx")
 def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=None,
               raw=False, variant=None, alt=None):
     """Advance ``fields`` (dict of device tensors, reference shapes) k steps.
-    tape: optional (k, nx, ny) tensor receiving Ez after every step (TM).
+    tape: optional tensor receiving E after every step: (k, nx, ny) Ez for TM,
+    (k, (nx-1)*ny + nx*(ny-1)) rows [Ex | Ey] for TE.
     reuse: raw outputs of a previous call whose accumulators keep summing.
     raw: return the raw (monitor, kind, buffer) triples instead of shaped outputs."""
     L = E.lib()
@@ -164,6 +165,11 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
         d.hx, d.hy, d.ey, d.ex = (_axis(coeffs, n) for n in ("hx", "hy", "ey", "ex"))
         for f in ("Hz", "Ex", "Ey", "Dx", "Dy", "psi_Eyx", "psi_Exy", "psi_Hzy", "psi_Hzx"):
             setattr(d, f, fields[f].data_ptr())
+        if tape is not None:
+            slot = (nx - 1) * ny + nx * (ny - 1)
+            assert tape.is_contiguous() and tape.shape == (k, slot) and tape.dtype == dtype
+            d.tape = tape.data_ptr()
+            variant = -1  # the tape is written by the two-pass path only
         keep_c = [ex_, ey_]
     for f, t in fields.items():
         if not t.is_contiguous():
