@@ -228,7 +228,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the FDTD path shards by spatial slabs across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(the FDTD path would shard by spatial slabs across GPUs)", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         x = torch.ones(256 * 1024 * 1024, device="cuda")
@@ -237,6 +237,14 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         log("[bench] copied 3 x 1 GiB")
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(10):
+            y.copy_(x)
+        ev[1].record()
+        torch.cuda.synchronize()
+        us = ev[0].elapsed_time(ev[1]) * 1e3 / 10
+        log(f"[bench] plain 1 GiB device copy: {us:.1f} us, {2 * x.numel() * 4 / us / 1e3:.0f} GB/s (read + write)")
         return
     if args.workload == "c4":
         return bench_c4(args)

@@ -4,7 +4,9 @@ Covers the hook-free flow of meep_torch/mp/simulation.py: construction
 (:214-381), structure rasterization (:511-573), sources (:796-880),
 add_flux/get_fluxes/get_flux_freqs (:1357-1386,1809-1870),
 add_dft_fields/get_dft_array (:1592-1641), run(until | until_after_sources |
-n_steps) (:1096-1142,1239-1302) and get_array (:2119-2150).
+n_steps, step functions, stop conditions) (:1096-1142,1239-1302),
+get_field_point (:983-1024), get_flux_data / load_flux_data /
+load_minus_flux_data (:1882-1926) and get_array (:2119-2150).
 
 3D cells (dimensions=3) take the reference's 3D adapter path, in sim3d.py.
 
@@ -13,8 +15,10 @@ fields and zero DFT accumulators and integrates to the accumulated clock
 (so consecutive runs reproduce one continuing run), in chunks
 [this run's steps, previous clock] exactly as its _run_until loop issues
 them (meep_torch/mp/simulation.py:1271-1295); each chunk's monitor sums are
-added in chunk order.  Step functions, stop conditions, symmetries, k-points,
-multi-source 2D runs and non-Block geometry raise NotImplementedError.
+added in chunk order.  Step functions and stop conditions (run_control.py)
+split the run into engine calls at the points where a hook must look at the
+fields.  Symmetries, k-points, multi-source 2D runs and non-Block geometry
+raise NotImplementedError.
 """
 import numpy as np
 import torch
@@ -27,6 +31,7 @@ from ..core.sources import LineSource
 from ..core.stepper_te import FIELDS as TE_FIELDS, init_te_state
 from ..core.stepper_tm import FIELDS as TM_FIELDS, init_tm_state
 from . import geometry as G
+from . import run_control as RC
 from .sim3d import Sim3D
 from .sources import source_span
 from .vec import Vector3, pol_for
@@ -217,27 +222,40 @@ class Simulation(Sim3D):
 
     # ---- running -------------------------------------------------------
     def run(self, *step_funcs, until=None, until_after_sources=None, n_steps=None):
-        if step_funcs or callable(until) or callable(until_after_sources) or \
-                isinstance(until, (list, tuple)) or isinstance(until_after_sources, (list, tuple)):
-            raise NotImplementedError("step functions / stop conditions are not supported by the HIP facade")
         self.run_index += 1
         self._init_structure()
-        dt = self._coeffs.dt if self.dimensions == 3 else self.grid.dt
-        if n_steps is None:
-            if until is not None:
-                T = until
-            elif until_after_sources is not None:
-                T = self._source_cutoff() + until_after_sources
-            else:
-                T = 0
-            n_steps = max(1, int(round(float(T) / dt)))
         if self.dimensions == 3:
+            if step_funcs or callable(until) or callable(until_after_sources) or \
+                    isinstance(until, (list, tuple)) or isinstance(until_after_sources, (list, tuple)):
+                raise NotImplementedError("step functions / stop conditions: 2D runs only")
+            if n_steps is None:
+                T = until if until is not None else (
+                    self._source_cutoff() + until_after_sources if until_after_sources is not None else 0)
+                n_steps = max(1, int(round(float(T) / self._coeffs.dt)))
             self._run3(int(n_steps))
-        else:
-            self._advance(int(n_steps))
+            return
+        self._run_until(step_funcs, until, until_after_sources, n_steps)
 
-    def _advance(self, n_total):
+    def _run_until(self, step_funcs, until, until_after_sources, n_steps):
+        """meep_torch/mp/simulation.py:1239-1302: every run starts from zero
+        fields (and the loaded flux data, if any), hooks fire between engine
+        calls sized by the hook schedule, stop conditions are fed after every
+        step, and the step counter continues from the previous run's clock
+        while the fields' own counter restarts at 0 (so consecutive runs
+        reproduce one continuing run)."""
         mode = self._pol
+        dt = self.grid.dt
+        cutoff = self._source_cutoff()
+        begin, per_step, scheduled = RC.plan_hooks(step_funcs, cutoff)
+        conds, gate, n_total = None, 0.0, n_steps
+        stop = until if until is not None else until_after_sources
+        if callable(stop) or isinstance(stop, (list, tuple)):
+            conds = list(stop) if isinstance(stop, (list, tuple)) else [stop]
+            if until is None:
+                gate = cutoff            # until_after_sources: conditions count after the cutoff
+        elif n_total is None:
+            T = float(until) if until is not None else cutoff + float(until_after_sources or 0.0)
+            n_total = max(1, int(round(T / dt)))
         names = TM_FIELDS if mode == "tm" else TE_FIELDS
         st = init_tm_state(self.grid) if mode == "tm" else init_te_state(self.grid)
         work = {f: getattr(st, f).contiguous() for f in names}
@@ -247,20 +265,36 @@ class Simulation(Sim3D):
         plan = engine2d.source_plan(src, target, self.grid.device, self.grid.dtype)
         keys = list(self._monitors)
         mons = [self._monitors[k][1] for k in keys]
-        acc = [None] * len(mons)
+        seeds = getattr(self, "_acc_seed", {})
+        acc = [seeds[k].clone() if k in seeds else None for k in keys]
+        self._state = st._replace(**work)
+        self._acc = {k: a for k, a in zip(keys, acc) if a is not None}
         start = self._t_step
-        stop_at = start + n_total
-        clock, f_t = start, 0
-        while clock < stop_at:
-            k = stop_at - clock
+        stop_at = None if n_total is None else start + int(n_total)
+        f_t = 0                          # the fields' own step counter
+        for g in begin:
+            RC.eval_step_func(self, g, "step")
+        RC.fire_scheduled(self, scheduled, start * dt)
+        while stop_at is None or self._t_step < stop_at:
+            t = self._t_step * dt
+            left = stop_at - self._t_step if stop_at is not None else 1 << 30
+            k = RC.chunk_size(t, dt, left, per_step, scheduled, conds is not None)
             outs = engine2d.run_chunk(mode, work, self._coeffs, f_t, k, plan, mons)
             for i, o in enumerate(outs):
                 acc[i] = o if acc[i] is None else acc[i] + o
             f_t += k
-            clock = f_t
-        self._t_step = clock
-        self._state = st._replace(**work)
-        self._acc = {k: a for k, a in zip(keys, acc)}
+            self._t_step = f_t
+            self._state = st._replace(**work)
+            self._acc = dict(zip(keys, acc))
+            t = self._t_step * dt
+            RC.fire_per_step(self, per_step, t, dt)
+            RC.fire_scheduled(self, scheduled, t)
+            if conds is not None:
+                hit = [bool(g(self)) for g in conds]     # every condition sees every step
+                if t >= gate - 1e-9 and any(hit):
+                    break
+        for func in step_funcs:
+            RC.eval_step_func(self, func, "finish")
 
     def restart_fields(self):
         self._state = None
@@ -291,6 +325,70 @@ class Simulation(Sim3D):
         name = _name(f)
         parts = self._flux_parts.get(name)
         return self._monitors[parts[0] if parts else name][1].freqs.detach().cpu().numpy().tolist()
+
+    def get_field_point(self, c, pt):
+        """meep_torch/mp/simulation.py:983-1024 (no symmetries): the value of
+        component c in the cell holding point pt; D components read
+        eps * E there, B components read H; a component the state does not
+        carry reads the run's primary field (Ez for TM, Hz for TE)."""
+        if self.dimensions == 3:
+            raise NotImplementedError("get_field_point: 2D runs only")
+        if self._state is None:
+            return 0.0
+        gs = self.gs
+        pt = Vector3(*pt)
+        ix = max(0, min(gs.nx - 1, G.cell_index(pt.x, gs)))
+        iy = max(0, min(gs.ny - 1, G.cell_index_y(pt.y, gs)))
+        name, scale = (c if isinstance(c, str) else ""), 1.0
+        if name.startswith("D") and name != "Dielectric":
+            name = "E" + name[1:]
+            scale = float(np.asarray(self.get_epsilon())[ix, iy])
+        elif name.startswith("B"):
+            name = "H" + name[1:]
+        f = getattr(self._state, name, None)
+        if f is None:
+            f = self._state.Hz if self._pol == "te" else self._state.Ez
+        return scale * float(f[min(ix, f.shape[0] - 1), min(iy, f.shape[1] - 1)])
+
+    def get_flux_data(self, flux):
+        """meep_torch/mp/simulation.py:1882-1898: the monitor's accumulated
+        transforms as FluxData(E, H), complex (nfreq, points), regions of a
+        multi-region monitor concatenated along the points."""
+        if self.dimensions == 3:
+            raise NotImplementedError("get_flux_data: 2D runs only")
+        name = _name(flux)
+        es, hs = [], []
+        for p in self._flux_parts.get(name) or [name]:
+            if p not in self._acc:
+                raise RuntimeError(f"get_flux_data({name!r}): no accumulated fields -- run() first")
+            a = self._acc[p].detach().cpu().numpy()
+            es.append(a[:, 0] + 1j * a[:, 1])
+            hs.append(a[:, 2] + 1j * a[:, 3])
+        return RC.FluxData(E=np.concatenate(es, axis=-1), H=np.concatenate(hs, axis=-1))
+
+    def load_flux_data(self, flux, fdata, _scale=1.0):
+        """meep_torch/mp/simulation.py:1900-1921: seed the monitor's
+        accumulators with fdata; every later run() accumulates on top."""
+        if self.dimensions == 3:
+            raise NotImplementedError("load_flux_data: 2D runs only")
+        name = _name(flux)
+        E, H = np.asarray(fdata.E), np.asarray(fdata.H)
+        seeds, off = {}, 0
+        for p in self._flux_parts.get(name) or [name]:
+            meta = self._monitors[p][2]
+            n = meta["hi"] - meta["lo"]
+            e, h = E[:, off:off + n], H[:, off:off + n]
+            off += n
+            a = _scale * np.stack([e.real, e.imag, h.real, h.imag], axis=1)
+            seeds[p] = torch.as_tensor(a, dtype=self.grid.dtype, device=self.grid.device)
+        if off != E.shape[-1]:
+            raise ValueError(f"load_flux_data({name!r}): data has {E.shape[-1]} points, monitor has {off}")
+        self._acc_seed = {**getattr(self, "_acc_seed", {}), **seeds}
+
+    def load_minus_flux_data(self, flux, fdata):
+        """meep_torch/mp/simulation.py:1923-1926: load_flux_data with the
+        transforms negated, so the next run subtracts them."""
+        self.load_flux_data(flux, fdata, _scale=-1.0)
 
     def get_dft_array(self, f, component, num_freq):
         key = f"{_name(f)}:{component}"

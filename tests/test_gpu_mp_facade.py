@@ -2,7 +2,8 @@
 (100x60 TM guide, eps 12, 10-cell PML, 800 steps, 3 flux frequencies) against
 the reference's own run, in FP32 (fields <= 1e-5 rel-L2, spectra <= 1e-4) and
 FP64 (<= 1e-10); consecutive runs integrate to the accumulated clock like one
-continuing run; DFT field arrays match a run_tm DFT monitor."""
+continuing run; DFT field arrays match a run_tm DFT monitor; step functions,
+stop conditions and flux-data loading follow the reference's run protocol."""
 import numpy as np
 import pytest
 import torch
@@ -71,3 +72,100 @@ def test_dft_fields_match_run_tm_monitor():
     i0, j0 = 50, 20
     assert arr.shape == (20, 20)
     assert np.array_equal(arr, ref[i0:i0 + 20, j0:j0 + 20])
+
+
+# The reference's flux values for the same scene run as
+# run(at_every(5.0, hook), until=40) (meep_torch tests/mp_parity/test_meep_flow.py:53-54)
+GOLDEN_STEPPED = [2.5635746995635057e-06, 6.786305104894549e-06, 4.077624008546449e-06]
+
+
+def test_stepped_run_matches_reference_golden():
+    """Hooks fire at t = 0, 5, ..., 40 (meep_torch tests/mp_parity/
+    test_meep_flow.py:111-118); the run is split into 100-step engine calls
+    at the hook times, which regroups the monitor sums only: fluxes within
+    1e-12 of the reference's stepped values, fields equal to the plain run."""
+    mp, sim, fl = _plain(torch.float64)
+    hooks = []
+    sim.run(mp.at_every(5.0, lambda s: hooks.append(s.round_time())), until=40)
+    assert hooks == [0.0, 5.0, 10.0, 15.0, 20.0, 25.0, 30.0, 35.0, 40.0]
+    got = np.array(mp.get_fluxes(fl))
+    print(f"\n[golden_stepped] flux rel {_rel(got, np.array(GOLDEN_STEPPED)):.2e}")
+    assert np.allclose(got, GOLDEN_STEPPED, rtol=1e-12, atol=0)
+    _, plain, _ = _plain(torch.float64)
+    plain.run(until=40)
+    assert np.array_equal(sim.get_array(component=mp.Ez), plain.get_array(component=mp.Ez))
+
+
+def test_stop_condition_steps_one_at_a_time():
+    """A stop condition is fed after every step (meep_torch tests/mp_parity/
+    test_meep_flow.py:98-108: seven calls, then stop); the fields equal a
+    plain 7-step run and get_field_point reads the cell holding the point."""
+    mp, sim, _ = _plain(torch.float64)
+    seen = []
+    sim.run(until=[lambda s: seen.append(s.round_time()) or len(seen) >= 7])
+    dt = sim.grid.dt
+    assert sim._t_step == 7 and np.allclose(seen, [dt * i for i in range(1, 8)], rtol=0, atol=1e-12)
+    _, ref, _ = _plain(torch.float64)
+    ref.run(n_steps=7)
+    ez = ref.get_array(component=mp.Ez)
+    assert np.array_equal(sim.get_array(component=mp.Ez), ez)
+    # point (-3, 0.2) of the 10 x 6 cell at resolution 10 -> cell (20, 32)
+    assert sim.get_field_point(mp.Ez, mp.Vector3(-3, 0.2)) == float(ez[20, 32])
+    eps = sim.get_epsilon()
+    assert sim.get_field_point("Dz", mp.Vector3(-3, 0.2)) == float(eps[20, 32]) * float(ez[20, 32])
+
+
+def test_stop_when_fields_decayed_after_sources():
+    """until_after_sources=stop_when_fields_decayed(...): conditions count only
+    after the source cutoff.  A plain run of the same length, with a per-step
+    hook recording the probe, replays the rule offline and must stop at the
+    same step."""
+    mp, sim, _ = _plain(torch.float64)
+    pt = mp.Vector3(3, 0)
+    limit = lambda s: s.round_time() > 400.0     # bounds the test; must not be what stops it
+    sim.run(until_after_sources=[mp.stop_when_fields_decayed(5.0, mp.Ez, pt, 1e-2), limit])
+    n_stop, cutoff, dt = sim._t_step, sim._source_cutoff(), sim.grid.dt
+    assert cutoff < n_stop * dt < 400.0
+    _, ref, _ = _plain(torch.float64)
+    probe = []
+    ref.run(lambda s: probe.append(s.get_field_point(mp.Ez, pt)), n_steps=n_stop)
+    cond = mp.stop_when_fields_decayed(5.0, mp.Ez, pt, 1e-2)
+
+    class Replay:
+        grid = ref.grid
+
+        def __init__(self):
+            self.i = 0
+
+        def round_time(self):
+            return self.i * dt
+
+        def get_field_point(self, c, p):
+            return probe[self.i - 1]
+
+    r = Replay()
+    stop = None
+    for i in range(1, n_stop + 1):
+        r.i = i
+        if cond(r) and i * dt >= cutoff - 1e-9:
+            stop = i
+            break
+    assert stop == n_stop
+
+
+def test_flux_data_roundtrip():
+    """load_minus_flux_data of a run's own data, then the same run: every
+    accumulator is -a + a = 0 exactly, so the flux is exactly zero;
+    load_flux_data instead doubles E and H (exact), so the flux is exactly 4x."""
+    mp, a, fa = _plain(torch.float64)
+    a.run(until=40)
+    data = a.get_flux_data(fa)
+    assert data.E.shape == (3, 40) and data.H.shape == (3, 40)
+    _, b, fb = _plain(torch.float64)
+    b.load_minus_flux_data(fb, data)
+    b.run(until=40)
+    assert mp.get_fluxes(fb) == [0.0, 0.0, 0.0]
+    _, c, fc = _plain(torch.float64)
+    c.load_flux_data(fc, data)
+    c.run(until=40)
+    assert mp.get_fluxes(fc) == [4.0 * v for v in mp.get_fluxes(fa)]
