@@ -39,6 +39,16 @@ typedef struct {
   int hi_begin;
 } mt_pml_axis;
 
+/* Soft source on a 2D field: dense weights on the bounding box
+ * [i0,i1) x [j0,j1) of the reference's mask (sources.py:49-78); at step s the
+ * field loses (coef[s] cast to dtype) * w (stepper_tm.py:197-200,
+ * stepper_te.py:131-134).  coef is a HOST array of k doubles = dt * amp(t). */
+typedef struct {
+  int present;
+  int i0, i1, j0, j1;
+  const void* w;                          /* device (i1-i0, j1-j0) */
+} mt_source2d;
+
 /* 2D TM problem + state (meep_torch/core/coeffs.py:21-74 TMCoeffs and
  * meep_torch/core/stepper_tm.py:12-34 TMState, eps/CPML/y_pmc subset).
  * Profiles follow the reference naming: hy (ny-1, half-y), hx (nx-1, half-x),
@@ -59,6 +69,13 @@ typedef struct {
    * always end in the primary buffers. */
   int variant, pad_;
   void* alt[8];
+  /* optional magnetic (TFSF) currents of the B phase (stepper_tm.py:169-193,
+   * run.py:17-23): hsrc[0] on Hx (nx, ny-1), hsrc[1] on Hy (nx-1, ny); at
+   * step s the field loses (hcoef[s] cast to dtype) * w right after its curl
+   * update.  hcoef is a HOST array of k doubles = dt * h_amplitude(t).  Runs
+   * with a magnetic current take the two-pass path. */
+  mt_source2d hsrc[2];
+  const double* hcoef;
 } mt_tm2d;
 
 /* 2D TE problem + state (coeffs.py:294-345 TECoeffs, stepper_te.py:11-24). */
@@ -78,17 +95,12 @@ typedef struct {
    * written to slot s = tape + s * ((nx-1)*ny + nx*(ny-1)) as [Ex | Ey], for
    * mt_te2d_adjoint's eps gradient */
   void* tape;
+  /* optional magnetic (TFSF) current on Hz (nx-1, ny-1) of the B half
+   * (stepper_te.py:120-128), as hsrc/hcoef of mt_tm2d */
+  mt_source2d hsrc;
+  const double* hcoef;
 } mt_te2d;
 
-/* Soft source on a 2D field: dense weights on the bounding box
- * [i0,i1) x [j0,j1) of the reference's mask (sources.py:49-78); at step s the
- * field loses (coef[s] cast to dtype) * w (stepper_tm.py:197-200,
- * stepper_te.py:131-134).  coef is a HOST array of k doubles = dt * amp(t). */
-typedef struct {
-  int present;
-  int i0, i1, j0, j1;
-  const void* w;                          /* device (i1-i0, j1-j0) */
-} mt_source2d;
 
 /* 3D problem + state (stepper_3d.py:28-67 Coeffs3D / State3D, CPML path,
  * sigma = 0).  lean = 1 selects the in-place lean stepper's E update and
@@ -314,6 +326,16 @@ const char* mt_last_error(void);
 
 /* ABI version (bumped on any struct layout change). */
 int mt_abi_version(void);
+
+/* Layout of the ABI structs as this library was compiled, for bindings to
+ * check their own declarations against (no counterpart in the reference: a
+ * binding aid).  Writes up to n values and returns how many there are:
+ * sizeof of mt_pml_axis, mt_source2d, mt_tm2d, mt_te2d, mt_yee3d,
+ * mt_source3d, mt_dft_term, mt_monitor, mt_tm2d_adj, mt_te2d_adj,
+ * mt_yee3d_adj, mt_tm2d_batch_item, then offsetof of mt_tm2d.hsrc,
+ * mt_tm2d.hcoef, mt_te2d.tape, mt_te2d.hsrc, mt_te2d.hcoef, mt_yee3d.alt,
+ * mt_monitor.acc. */
+int mt_abi_layout(long long* out, int n);
 
 #ifdef __cplusplus
 }

@@ -312,6 +312,10 @@ extern "C" mt_batch* mt_tm2d_batch_create(const mt_tm2d_batch_item* items, int n
         fail(MT_EINVAL, "batched run: missing ping-pong buffers");
         return nullptr;
       }
+    if (d.hsrc[0].present || d.hsrc[1].present) {
+      fail(MT_EINVAL, "batched run: magnetic (TFSF) sources are not supported");
+      return nullptr;
+    }
     if (check_axis(d.hy, d.ny - 1, "hy") || check_axis(d.hx, d.nx - 1, "hx") || check_axis(d.ex, d.nx - 2, "ex") ||
         check_axis(d.ey, d.y_pmc ? d.ny : d.ny - 2, "ey"))
       return nullptr;

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include <cmath>
+#include <cstddef>
 #include <string>
 
 #include "common.h"
@@ -413,4 +414,18 @@ extern "C" long long mt_stage_bytes(const mt_monitor* mons, int n_mons, int stag
 
 extern "C" const char* mt_last_error(void) { return mt::g_err.c_str(); }
 
-extern "C" int mt_abi_version(void) { return 8; }
+extern "C" int mt_abi_version(void) { return 9; }
+
+extern "C" int mt_abi_layout(long long* out, int n) {
+  const long long v[] = {
+      (long long)sizeof(mt_pml_axis),        (long long)sizeof(mt_source2d),   (long long)sizeof(mt_tm2d),
+      (long long)sizeof(mt_te2d),            (long long)sizeof(mt_yee3d),      (long long)sizeof(mt_source3d),
+      (long long)sizeof(mt_dft_term),        (long long)sizeof(mt_monitor),    (long long)sizeof(mt_tm2d_adj),
+      (long long)sizeof(mt_te2d_adj),        (long long)sizeof(mt_yee3d_adj),  (long long)sizeof(mt_tm2d_batch_item),
+      (long long)offsetof(mt_tm2d, hsrc),    (long long)offsetof(mt_tm2d, hcoef), (long long)offsetof(mt_te2d, tape),
+      (long long)offsetof(mt_te2d, hsrc),    (long long)offsetof(mt_te2d, hcoef), (long long)offsetof(mt_yee3d, alt),
+      (long long)offsetof(mt_monitor, acc)};
+  const int total = (int)(sizeof(v) / sizeof(v[0]));
+  for (int q = 0; q < n && q < total; ++q) out[q] = v[q];
+  return total;
+}

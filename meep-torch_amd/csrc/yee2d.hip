@@ -34,8 +34,20 @@ struct TmK {
 };
 
 // meep_torch/core/stepper_tm.py:85-92,126-128 (B half, H from curl E)
+// Weight of a magnetic source box at (i, j), or 0 outside it.
 template <typename T>
-__global__ __launch_bounds__(BX* BY) void tm_b_kernel(TmK<T> p) {
+__device__ __forceinline__ bool in_box(const mt_source2d& s, int i, int j) {
+  return s.present && i >= s.i0 && i < s.i1 && j >= s.j0 && j < s.j1;
+}
+
+template <typename T>
+__device__ __forceinline__ T box_w(const mt_source2d& s, int i, int j) {
+  return static_cast<const T*>(s.w)[(long long)(i - s.i0) * (s.j1 - s.j0) + (j - s.j0)];
+}
+
+// B half plus the optional TFSF magnetic currents (stepper_tm.py:169-193)
+template <typename T>
+__global__ __launch_bounds__(BX* BY) void tm_b_kernel(TmK<T> p, mt_source2d hxs, mt_source2d hys, T hc) {
   int j = blockIdx.x * BX + threadIdx.x;
   int i = blockIdx.y * BY + threadIdx.y;
   if (i >= p.nx || j >= p.ny) return;
@@ -50,7 +62,9 @@ __global__ __launch_bounds__(BX* BY) void tm_b_kernel(TmK<T> p) {
       p.pEzy[h] = ps;
       curl = d + ps;
     }
-    p.Hx[h] = p.Hx[h] - p.dt * curl;
+    T v = p.Hx[h] - p.dt * curl;
+    if (in_box<T>(hxs, i, j)) v = v - hc * box_w<T>(hxs, i, j);
+    p.Hx[h] = v;
   }
   if (i < p.nx - 1) {
     long long h = (long long)i * ny + j;
@@ -61,7 +75,9 @@ __global__ __launch_bounds__(BX* BY) void tm_b_kernel(TmK<T> p) {
       p.pEzx[h] = ps;
       curl = d + ps;
     }
-    p.Hy[h] = p.Hy[h] + p.dt * curl;
+    T v = p.Hy[h] + p.dt * curl;
+    if (in_box<T>(hys, i, j)) v = v - hc * box_w<T>(hys, i, j);
+    p.Hy[h] = v;
   }
 }
 
@@ -174,7 +190,7 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
 
 // meep_torch/core/stepper_te.py:106-117 (B half, Hz from curl E)
 template <typename T>
-__global__ __launch_bounds__(BX* BY) void te_b_kernel(TeK<T> p) {
+__global__ __launch_bounds__(BX* BY) void te_b_kernel(TeK<T> p, mt_source2d hzs, T hc) {
   int j = blockIdx.x * BX + threadIdx.x;
   int i = blockIdx.y * BY + threadIdx.y;
   const int nx = p.nx, ny = p.ny;
@@ -194,7 +210,9 @@ __global__ __launch_bounds__(BX* BY) void te_b_kernel(TeK<T> p) {
     p.pExy[h] = ps;
     curl = curl - ps;
   }
-  p.Hz[h] = p.Hz[h] - p.dt * curl;
+  T v = p.Hz[h] - p.dt * curl;
+  if (in_box<T>(hzs, i, j)) v = v - hc * box_w<T>(hzs, i, j);  // stepper_te.py:120-128
+  p.Hz[h] = v;
 }
 
 int check_src(const mt_source2d* s, int n0, int n1) {
@@ -234,7 +252,8 @@ int tm_run_t(const mt_tm2d* d, const mt_source2d* src, const double* coef, const
   int S = mp.stage_steps > 0 ? mp.stage_steps : 1;
   int filled = 0, first = step0;
   for (int s_ = 0; s_ < k; ++s_) {
-    hipLaunchKernelGGL(tm_b_kernel<T>, grd, blk, 0, s, p);
+    const T hc = d->hcoef ? static_cast<T>(d->hcoef[s_]) : T(0);
+    hipLaunchKernelGGL(tm_b_kernel<T>, grd, blk, 0, s, p, d->hsrc[0], d->hsrc[1], hc);
     T c = sv.present ? static_cast<T>(coef[s_]) : T(0);
     hipLaunchKernelGGL(tm_d_kernel<T>, grd, blk, 0, s, p, sv, c, s_);
     int rc = check_launch("tm2d step");
@@ -283,7 +302,8 @@ int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const
   for (int s_ = 0; s_ < k; ++s_) {
     T c = sv.present ? static_cast<T>(coef[s_]) : T(0);
     hipLaunchKernelGGL(te_d_kernel<T>, grd, blk, 0, s, p, sv, c, s_);
-    hipLaunchKernelGGL(te_b_kernel<T>, grd, blk, 0, s, p);
+    const T hc = d->hcoef ? static_cast<T>(d->hcoef[s_]) : T(0);
+    hipLaunchKernelGGL(te_b_kernel<T>, grd, blk, 0, s, p, d->hsrc, hc);
     int rc = check_launch("te2d step");
     if (rc) return rc;
     if (mp.n_mons) {
@@ -478,10 +498,13 @@ extern "C" int mt_tm2d_run(const mt_tm2d* p, const mt_source2d* src, const doubl
     return rc;
   if ((rc = check_src(src, p->nx, p->ny))) return rc;
   if (src && src->present && !coef && k > 0) return fail(MT_EINVAL, "source without coefficients");
+  if ((rc = check_src(&p->hsrc[0], p->nx, p->ny - 1)) || (rc = check_src(&p->hsrc[1], p->nx - 1, p->ny))) return rc;
+  const bool hsrc = p->hsrc[0].present || p->hsrc[1].present;
+  if (hsrc && !p->hcoef && k > 0) return fail(MT_EINVAL, "magnetic source without coefficients");
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (p->variant >= 0) {
+  if (p->variant >= 0 && !hsrc) {   // magnetic currents are injected by the two-pass kernels
     if (p->variant >= tiled_variant_count()) return fail(MT_EINVAL, "unknown tiled variant");
     if (!alt_ok(p->alt, 8)) return fail(MT_EINVAL, "tiled path needs the alt buffers");
     if (taps_ok(mp))
@@ -508,10 +531,13 @@ extern "C" int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const doubl
     return rc;
   if ((rc = check_src(src, p->nx, p->ny - 1))) return rc;
   if (src && src->present && !coef && k > 0) return fail(MT_EINVAL, "source without coefficients");
+  if ((rc = check_src(&p->hsrc, p->nx - 1, p->ny - 1))) return rc;
+  if (p->hsrc.present && !p->hcoef && k > 0) return fail(MT_EINVAL, "magnetic source without coefficients");
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (p->variant >= 0 && !p->tape) {   // the tape is written by the two-pass kernels
+  // the tape and magnetic currents are handled by the two-pass kernels
+  if (p->variant >= 0 && !p->tape && !p->hsrc.present) {
     if (p->variant >= tiled_variant_count()) return fail(MT_EINVAL, "unknown tiled variant");
     if (!alt_ok(p->alt, 7)) return fail(MT_EINVAL, "tiled path needs the alt buffers");
     if (taps_ok(mp))

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -24,6 +24,11 @@ class PmlAxis(ctypes.Structure):
                 ("hi_begin", c_int)]
 
 
+class Source2D(ctypes.Structure):
+    _fields_ = [("present", c_int), ("i0", c_int), ("i1", c_int), ("j0", c_int), ("j1", c_int),
+                ("w", c_void_p)]
+
+
 class TM2D(ctypes.Structure):
     _fields_ = [("nx", c_int), ("ny", c_int), ("dtype", c_int), ("y_pmc", c_int),
                 ("dt", c_double), ("dx", c_double), ("eps", c_void_p),
@@ -31,7 +36,8 @@ class TM2D(ctypes.Structure):
                 ("Ez", c_void_p), ("Dz", c_void_p), ("Hx", c_void_p), ("Hy", c_void_p),
                 ("psi_Ezy", c_void_p), ("psi_Ezx", c_void_p),
                 ("psi_Hyx", c_void_p), ("psi_Hxy", c_void_p), ("tape", c_void_p),
-                ("variant", c_int), ("pad_", c_int), ("alt", c_void_p * 8)]
+                ("variant", c_int), ("pad_", c_int), ("alt", c_void_p * 8),
+                ("hsrc", Source2D * 2), ("hcoef", ctypes.POINTER(c_double))]
 
 
 class TM2DAdj(ctypes.Structure):
@@ -48,7 +54,8 @@ class TE2D(ctypes.Structure):
                 ("Hz", c_void_p), ("Ex", c_void_p), ("Ey", c_void_p), ("Dx", c_void_p),
                 ("Dy", c_void_p), ("psi_Eyx", c_void_p), ("psi_Exy", c_void_p),
                 ("psi_Hzy", c_void_p), ("psi_Hzx", c_void_p),
-                ("variant", c_int), ("pad2_", c_int), ("alt", c_void_p * 8), ("tape", c_void_p)]
+                ("variant", c_int), ("pad2_", c_int), ("alt", c_void_p * 8), ("tape", c_void_p),
+                ("hsrc", Source2D), ("hcoef", ctypes.POINTER(c_double))]
 
 
 class TE2DAdj(ctypes.Structure):
@@ -56,11 +63,6 @@ class TE2DAdj(ctypes.Structure):
                 ("aDy", c_void_p), ("apsi_Eyx", c_void_p), ("apsi_Exy", c_void_p),
                 ("apsi_Hzy", c_void_p), ("apsi_Hzx", c_void_p), ("gEy", c_void_p), ("gEx", c_void_p),
                 ("gDx", c_void_p), ("gDy", c_void_p), ("epsx_grad", c_void_p), ("epsy_grad", c_void_p)]
-
-
-class Source2D(ctypes.Structure):
-    _fields_ = [("present", c_int), ("i0", c_int), ("i1", c_int), ("j0", c_int), ("j1", c_int),
-                ("w", c_void_p)]
 
 
 class Yee3D(ctypes.Structure):
@@ -135,6 +137,8 @@ def lib():
     L.mt_yee3d_adjoint.restype = c_int
     L.mt_yee3d_adjoint.argtypes = [P(Yee3D), P(Yee3DAdj), c_void_p, P(Monitor), c_int, c_int, c_int,
                                    c_void_p]
+    L.mt_abi_layout.restype = c_int
+    L.mt_abi_layout.argtypes = [P(c_ll), c_int]
     if L.mt_abi_version() != ABI_VERSION:
         raise RuntimeError("meep_torch_amd: libmeepfdtd.so ABI mismatch; rebuild")
     _lib = L

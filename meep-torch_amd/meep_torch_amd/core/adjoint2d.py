@@ -255,7 +255,8 @@ def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_
         tgt = state.Dy
         fn = _TERun
     E.require_device(*eps)
-    cfg["plan"] = engine2d.source_plan(source, tgt.shape, tgt.device, tgt.dtype)
+    nx, ny = (state.Ez.shape if mode == "tm" else (state.Ey.shape[0], state.Ex.shape[1]))
+    cfg["plan"] = engine2d.source_plan(source, tgt.shape, tgt.device, tgt.dtype, engine2d.h_shapes(mode, nx, ny))
     outs = fn.apply(*eps, cfg, *[getattr(state, f) for f in names])
     nf = len(names)
     new_state = state._replace(**dict(zip(names, outs[:nf])))

@@ -16,20 +16,45 @@ STAGE_STEPS = 64
 STAGE_BUDGET = 1 << 29  # bytes of staging per call
 
 
-def source_plan(source, field_shape, device, dtype):
-    """Reference source protocol (run.py:17-25): .amplitude(t), .mask,
-    optional .h_masks (TFSF, not carried)."""
+def h_shapes(mode, nx, ny):
+    """Magnetic (TFSF) mask keys of step_tm / step_te and the field each one
+    feeds: key -> (field shape, slot of mt_tm2d.hsrc / mt_te2d.hsrc)."""
+    if mode == "tm":
+        return {"src_hx_mask": ((nx, ny - 1), 0), "src_hy_mask": ((nx - 1, ny), 1)}
+    return {"src_hz_mask": ((nx - 1, ny - 1), 0)}
+
+
+def source_plan(source, field_shape, device, dtype, hshapes=None):
+    """Reference source protocol (run.py:17-25): .amplitude(t), .mask, and
+    for a TFSF source .h_amplitude(t) with .h_masks, a dict of magnetic
+    masks keyed like step_tm / step_te's arguments (``hshapes``: the keys
+    this caller accepts, from h_shapes; None refuses magnetic masks)."""
     if source is None:
         return None
-    if getattr(source, "h_masks", None):
-        raise NotImplementedError("TFSF magnetic source (h_masks) is not supported by the HIP stepper")
     if getattr(source, "per_step_mask", False):
         raise NotImplementedError(f"{type(source).__name__}: per-step masks are not supported")
-    mask = source.mask
-    if tuple(mask.shape) != tuple(field_shape):
-        raise ValueError(f"source mask shape {tuple(mask.shape)} != field shape {tuple(field_shape)}")
-    box = E.mask_box(mask.to(device=device, dtype=dtype))
-    return {"source": source, "box": box}
+    h_masks = getattr(source, "h_masks", None) or {}
+    if h_masks and hshapes is None:
+        raise NotImplementedError("TFSF magnetic source (h_masks) is not supported on this path")
+    mask = getattr(source, "mask", None)
+    box = None
+    if mask is not None:
+        if tuple(mask.shape) != tuple(field_shape):
+            raise ValueError(f"source mask shape {tuple(mask.shape)} != field shape {tuple(field_shape)}")
+        box = E.mask_box(mask.to(device=device, dtype=dtype))
+    hboxes = []
+    for key, m in h_masks.items():
+        if m is None:
+            continue
+        if key not in hshapes:
+            raise ValueError(f"unknown magnetic source mask {key!r} (expected one of {sorted(hshapes)})")
+        shape, slot = hshapes[key]
+        if tuple(m.shape) != tuple(shape):
+            raise ValueError(f"{key} shape {tuple(m.shape)} != field shape {tuple(shape)}")
+        hb = E.mask_box(m.to(device=device, dtype=dtype))
+        if hb is not None:
+            hboxes.append((slot, hb))
+    return {"source": source, "box": box, "h": hboxes}
 
 
 def coef_array(plan, dt, step0, k):
@@ -41,6 +66,26 @@ def coef_array(plan, dt, step0, k):
     return (ctypes.c_double * max(1, k))(*vals)
 
 
+def coef_arrays(plan, dt, step0, k):
+    """(dt * amplitude(t), dt * h_amplitude(t)) host arrays for the k steps
+    from step0, each None when that current is absent.  Per step the electric
+    amplitude is taken before the magnetic one, in the reference loop's order
+    (run.py:17-23), so stateful waveforms see the same call sequence."""
+    if plan is None:
+        return None, None
+    src = plan["source"]
+    want_e, want_h = plan["box"] is not None, bool(plan.get("h"))
+    ev, hv = [], []
+    for t in range(step0, step0 + k):
+        a = src.amplitude(t)
+        if want_e:
+            ev.append(dt * a)
+        if want_h:
+            hv.append(dt * src.h_amplitude(t))
+    n = max(1, k)
+    return ((ctypes.c_double * n)(*ev) if want_e else None, (ctypes.c_double * n)(*hv) if want_h else None)
+
+
 def source_struct(plan):
     s = E.Source2D()
     if plan is None or plan["box"] is None:
@@ -50,6 +95,12 @@ def source_struct(plan):
     s.present, s.i0, s.i1, s.j0, s.j1 = 1, i0, i1, j0, j1
     s.w = E.ptr(w)
     return s, w
+
+
+def _fill_source(s, box):
+    (i0, j0), (i1, j1), w = box
+    s.present, s.i0, s.i1, s.j0, s.j1 = 1, i0, i1, j0, j1
+    s.w = E.ptr(w)
 
 
 def monitor_structs(fields, monitors, k, dtype, device, reuse=None):
@@ -174,6 +225,8 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
     for f, t in fields.items():
         if not t.is_contiguous():
             raise ValueError(f"state field {f} must be contiguous")
+    if plan is not None and plan.get("h"):
+        variant = -1  # magnetic currents are injected by the two-pass kernels
     if variant is None:
         from .. import tune
         variant = tune.variant_for(mode, fields, coeffs, plan, monitors, k)
@@ -185,7 +238,13 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
         for q, f in enumerate(order):
             d.alt[q] = alt[f].data_ptr()
     src, wkeep = source_struct(plan)
-    coef = coef_array(plan, coeffs.dt, step0, k) if src.present else None
+    coef, hcoef = coef_arrays(plan, coeffs.dt, step0, k)
+    hkeep = []
+    if plan is not None and plan.get("h"):
+        for slot, hb in plan["h"]:
+            _fill_source(d.hsrc[slot] if mode == "tm" else d.hsrc, hb)
+            hkeep.append(hb[2])
+        d.hcoef = hcoef
     arr, outs, keep = monitor_structs(fields, list(monitors), k, dtype, device, reuse)
     n = len(monitors)
     S, stage, nbytes = stage_alloc(L, arr, n, code, device)
@@ -193,5 +252,5 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
     rc = fn(ctypes.byref(d), ctypes.byref(src), coef, arr, n,
             E.ptr(stage) if stage is not None else None, nbytes, S, step0, k, E.stream_handle())
     E.check(rc, f"mt_{mode}2d_run")
-    del keep_c, wkeep, keep
+    del keep_c, wkeep, keep, hkeep, hcoef
     return outs if raw else finish_outputs(outs, fields)

@@ -18,14 +18,20 @@ from .stepper_tm import TMState, _check as _check_tm, step_tm
 
 
 class _Fixed:
-    """Source with one fixed amplitude (single-step calls)."""
+    """Source with one fixed amplitude (single-step calls), optionally with
+    TFSF magnetic masks and their fixed amplitude."""
 
-    def __init__(self, amp, mask):
+    def __init__(self, amp, mask, h_amp=0.0, h_masks=None):
         self._amp = float(amp)
+        self._h_amp = float(h_amp)
         self.mask = mask
+        self.h_masks = {k: v for k, v in (h_masks or {}).items() if v is not None}
 
     def amplitude(self, t):
         return self._amp
+
+    def h_amplitude(self, t):
+        return self._h_amp
 
 
 def _mode_of(step_fn, state):
@@ -38,6 +44,11 @@ def _mode_of(step_fn, state):
 
 def _fields(mode):
     return TM_FIELDS if mode == "tm" else TE_FIELDS
+
+
+def _hshapes(mode, work):
+    nx, ny = (work["Ez"].shape if mode == "tm" else (work["Ey"].shape[0], work["Ex"].shape[1]))
+    return engine2d.h_shapes(mode, nx, ny)
 
 
 def _target_shape(mode, work):
@@ -66,7 +77,7 @@ def _run_engine(mode, state, coeffs, n_steps, source, monitors, checkpoint_every
     names = _fields(mode)
     work = {f: getattr(state, f).detach().clone().contiguous() for f in names}
     plan = engine2d.source_plan(source, _target_shape(mode, work), work[names[0]].device,
-                                work[names[0]].dtype)
+                                work[names[0]].dtype, _hshapes(mode, work))
     acc = [None] * len(monitors)
     chunk = n_steps if checkpoint_every <= 0 else checkpoint_every
     step0 = 0
@@ -112,7 +123,7 @@ def _run_chunk(step_fn, state, coeffs, step0: int, k: int, source, monitors):
     names = _fields(mode)
     work = {f: getattr(state, f).detach().clone().contiguous() for f in names}
     plan = engine2d.source_plan(source, _target_shape(mode, work), work[names[0]].device,
-                                work[names[0]].dtype)
+                                work[names[0]].dtype, _hshapes(mode, work))
     with torch.no_grad():
         outs = engine2d.run_chunk(mode, work, coeffs, step0, k, plan, list(monitors))
     return state._replace(**work), outs

@@ -47,15 +47,15 @@ def _check(state: TEState):
 def step_te(state: TEState, m, src_amp=0.0, src_mask_ey: torch.Tensor | None = None,
             src_h_amp: float = 0.0, src_hz_mask: torch.Tensor | None = None) -> TEState:
     """One TE step entered at the D half (stepper_te.py:178-190) on the engine."""
-    if src_hz_mask is not None:
-        raise NotImplementedError("TFSF magnetic source masks are not supported by the HIP stepper")
     from .run import _Fixed
     from . import engine2d
     _check(state)
     work = {f: getattr(state, f).clone() for f in FIELDS}
     plan = None
-    if src_mask_ey is not None:
-        plan = engine2d.source_plan(_Fixed(src_amp, src_mask_ey), work["Dy"].shape,
-                                    work["Dy"].device, work["Dy"].dtype)
+    if src_mask_ey is not None or src_hz_mask is not None:
+        nx, ny = work["Ey"].shape[0], work["Ex"].shape[1]
+        plan = engine2d.source_plan(_Fixed(src_amp, src_mask_ey, src_h_amp, {"src_hz_mask": src_hz_mask}),
+                                    work["Dy"].shape, work["Dy"].device, work["Dy"].dtype,
+                                    engine2d.h_shapes("te", nx, ny))
     engine2d.run_chunk("te", work, m, 0, 1, plan, [])
     return state._replace(**work)

@@ -63,15 +63,15 @@ def step_tm(state: TMState, m: TMCoeffs, src_amp=0.0, src_mask: torch.Tensor | N
             src_hx_mask: torch.Tensor | None = None) -> TMState:
     """One TM step in meep's phase order (B, source(B), eh(H), D, source(D),
     pols, eh(E)) on the HIP engine."""
-    if src_hy_mask is not None or src_hx_mask is not None:
-        raise NotImplementedError("TFSF magnetic source masks are not supported by the HIP stepper")
     from .run import _Fixed
     from . import engine2d
     _check(state, m)
     work = {f: getattr(state, f).clone() for f in FIELDS}
     plan = None
-    if src_mask is not None:
-        plan = engine2d.source_plan(_Fixed(src_amp, src_mask), work["Dz"].shape,
-                                    work["Dz"].device, work["Dz"].dtype)
+    h = {"src_hy_mask": src_hy_mask, "src_hx_mask": src_hx_mask}
+    if src_mask is not None or any(v is not None for v in h.values()):
+        nx, ny = work["Ez"].shape
+        plan = engine2d.source_plan(_Fixed(src_amp, src_mask, src_h_amp, h), work["Dz"].shape,
+                                    work["Dz"].device, work["Dz"].dtype, engine2d.h_shapes("tm", nx, ny))
     engine2d.run_chunk("tm", work, m, 0, 1, plan, [])
     return state._replace(**work)

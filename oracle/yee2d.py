@@ -147,11 +147,13 @@ def te_state(nx, ny, dtype=torch.float32):
 
 
 # ---------------------------------------------------------------------- steps
-def tm_step(s, c, coef=0.0, mask=None):
-    """One TM step: B half (stepper_tm.py:85-92,126-128), D half
-    (stepper_tm.py:134-165, PEC or y_pmc), soft source (stepper_tm.py:197-200:
-    Dz -= (dt*amp) * mask, the scalar formed in double first), then
-    Ez = Dz / eps (stepper_tm.py:226-227).  ``coef`` is dt*amp."""
+def tm_step(s, c, coef=0.0, mask=None, hcoef=0.0, hx_mask=None, hy_mask=None):
+    """One TM step: B half (stepper_tm.py:85-92,126-128), the TFSF magnetic
+    currents of the B phase (stepper_tm.py:180-193: H -= (dt*h_amp) * mask),
+    D half (stepper_tm.py:134-165, PEC or y_pmc), soft source
+    (stepper_tm.py:197-200: Dz -= (dt*amp) * mask, the scalar formed in
+    double first), then Ez = Dz / eps (stepper_tm.py:226-227).  ``coef`` is
+    dt*amp, ``hcoef`` dt*h_amp."""
     dx, dt = c.dx, c.dt
     Ez = s["Ez"]
     g = (Ez[:, 1:] - Ez[:, :-1]) / dx
@@ -160,6 +162,10 @@ def tm_step(s, c, coef=0.0, mask=None):
     g2 = (Ez[1:, :] - Ez[:-1, :]) / dx
     pzx = c.b_hx * s["psi_Ezx"] + c.a_hx * g2
     Hy = s["Hy"] + dt * (g2 + pzx)
+    if hy_mask is not None:
+        Hy = Hy - hcoef * hy_mask
+    if hx_mask is not None:
+        Hx = Hx - hcoef * hx_mask
 
     if c.y_pmc:
         Hxp = torch.nn.functional.pad(Hx, (1, 1))
@@ -180,10 +186,11 @@ def tm_step(s, c, coef=0.0, mask=None):
     return out
 
 
-def te_step(s, c, coef=0.0, mask=None):
+def te_step(s, c, coef=0.0, mask=None, hcoef=0.0, hz_mask=None):
     """One TE step entered at the D half (stepper_te.py:178-190): D from curl
     Hz (stepper_te.py:80-103), source on Dy (stepper_te.py:131-134), E = D/eps
-    (stepper_te.py:160-168), then Hz from curl E (stepper_te.py:107-117)."""
+    (stepper_te.py:160-168), then Hz from curl E (stepper_te.py:107-117) and
+    the TFSF magnetic current on Hz (stepper_te.py:126-128)."""
     dx, dt = c.dx, c.dt
     Hz = s["Hz"]
     gy = (Hz[:, 1:] - Hz[:, :-1]) / dx
@@ -203,6 +210,8 @@ def te_step(s, c, coef=0.0, mask=None):
     pyx = c.b_hx * s["psi_Eyx"] + c.a_hx * dEy
     pxy = c.b_hy * s["psi_Exy"] + c.a_hy * dEx
     Hz2 = Hz - dt * (dEy + pyx - dEx - pxy)
+    if hz_mask is not None:
+        Hz2 = Hz2 - hcoef * hz_mask
     out = dict(s)
     out.update(Dx=Dx, Dy=Dy, Ex=Ex, Ey=Ey, Hz=Hz2, psi_Hzy=phy, psi_Hzx=phx,
                psi_Eyx=pyx, psi_Exy=pxy)
@@ -284,7 +293,8 @@ def dft_field_sample(field, comp, freqs, dt, t_idx, dtype):
 
 
 # ------------------------------------------------------------------- runner
-def run2d(mode, state, c, n_steps, amps=None, mask=None, monitors=(), checkpoint_every=0):
+def run2d(mode, state, c, n_steps, amps=None, mask=None, monitors=(), checkpoint_every=0, hamps=None,
+          hmasks=None):
     """meep_torch/core/run.py:10-63,86-97: chunked forward run.  Per chunk the
     monitor samples are summed in step order starting from the first sample;
     chunk sums are added to the running total.  ``amps`` are the per-step
@@ -292,6 +302,9 @@ def run2d(mode, state, c, n_steps, amps=None, mask=None, monitors=(), checkpoint
 
     monitors: list of dicts {"type": "flux", "kind", "idx", "lo", "hi", "freqs"}
     or {"type": "dft", "comp", "freqs"} or {"type": "probe", "ix", "iy"}.
+    ``hamps`` / ``hmasks`` (optional): per-step magnetic amplitudes and the
+    TFSF masks {"hx_mask", "hy_mask"} (TM) or {"hz_mask"} (TE), fed to the
+    step as hcoef = dt * hamps[t] (run.py:17-23).
     """
     step = tm_step if mode == "tm" else te_step
     dtype = state[next(iter(state))].dtype
@@ -303,10 +316,11 @@ def run2d(mode, state, c, n_steps, amps=None, mask=None, monitors=(), checkpoint
         k = min(chunk, n_steps - t0)
         part = [None] * len(monitors)
         for t in range(t0, t0 + k):
+            hk = {} if hamps is None else dict(hmasks, hcoef=c.dt * hamps[t])
             if amps is not None:
-                s = step(s, c, c.dt * amps[t], mask)
+                s = step(s, c, c.dt * amps[t], mask, **hk)
             else:
-                s = step(s, c)
+                s = step(s, c, **hk)
             for i, mon in enumerate(monitors):
                 if mon["type"] == "flux":
                     smp = flux_line_sample(mon["kind"], s, mon["idx"], mon["lo"], mon["hi"],

@@ -29,6 +29,25 @@ def test_library_exports_every_declared_symbol():
     assert L.mt_abi_version() == _engine.ABI_VERSION
 
 
+def test_ctypes_structs_match_the_compiled_layout():
+    """Every ABI struct the Python binding declares has the size (and the
+    late fields the offsets) the library was compiled with: a mismatch would
+    hand the kernels wrong pointers."""
+    from meep_torch_amd import _engine as E
+    from meep_torch_amd.batch import BatchItem
+    L = E.lib()
+    buf = (ctypes.c_longlong * 64)()
+    n = L.mt_abi_layout(buf, 64)
+    assert n == 19
+    sizes = [E.PmlAxis, E.Source2D, E.TM2D, E.TE2D, E.Yee3D, E.Source3D, E.DftTerm, E.Monitor, E.TM2DAdj,
+             E.TE2DAdj, E.Yee3DAdj, BatchItem]
+    for q, st in enumerate(sizes):
+        assert buf[q] == ctypes.sizeof(st), (st.__name__, buf[q], ctypes.sizeof(st))
+    offs = [E.TM2D.hsrc.offset, E.TM2D.hcoef.offset, E.TE2D.tape.offset, E.TE2D.hsrc.offset,
+            E.TE2D.hcoef.offset, E.Yee3D.alt.offset, E.Monitor.acc.offset]
+    assert list(buf[12:19]) == offs
+
+
 def test_error_path_without_gpu_work():
     """Invalid descriptors are rejected on the host before any launch."""
     from meep_torch_amd import _engine as E
