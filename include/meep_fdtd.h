@@ -76,6 +76,22 @@ typedef struct {
    * with a magnetic current take the two-pass path. */
   mt_source2d hsrc[2];
   const double* hcoef;
+  /* optional media (coeffs.py:87-106,154-200; stepper_tm.py:116-121,161-163,
+   * 198-231), two-pass path only:
+   *   n_poles electric ADE poles: c1, c2, c3 and the pole stacks P_cur,
+   *     P_prev, all (n_poles, nx, ny); P(n+1) = c1 P(n) + c2 P(n-1) + c3 Ez(n),
+   *     Ez = (Dz - sum P) / eps;
+   *   kerr = 1: three fixed-point passes Ez = rhs / (eps + chi2 Ez + chi3 Ez^2)
+   *     with chi2, chi3 (nx, ny);
+   *   d1, d2 (nx, ny): Dz = d1 Dz + d2 curl (sigma_d); the electric source
+   *     is then (d2 * coef[s]) * w with coef[s] the raw amplitude, not dt*amp;
+   *   h1x, h2x (nx, ny-1), h1y, h2y (nx-1, ny): Hx = h1x Hx - h2x curl,
+   *     Hy = h1y Hy + h2y curl (sigma_b and mu). */
+  int n_poles, kerr;
+  const void *c1, *c2, *c3;
+  void *P_cur, *P_prev;
+  const void *chi2, *chi3;
+  const void *d1, *d2, *h1x, *h2x, *h1y, *h2y;
 } mt_tm2d;
 
 /* 2D TE problem + state (coeffs.py:294-345 TECoeffs, stepper_te.py:11-24). */
@@ -334,7 +350,7 @@ int mt_abi_version(void);
  * mt_source3d, mt_dft_term, mt_monitor, mt_tm2d_adj, mt_te2d_adj,
  * mt_yee3d_adj, mt_tm2d_batch_item, then offsetof of mt_tm2d.hsrc,
  * mt_tm2d.hcoef, mt_te2d.tape, mt_te2d.hsrc, mt_te2d.hcoef, mt_yee3d.alt,
- * mt_monitor.acc. */
+ * mt_monitor.acc, mt_tm2d.n_poles, mt_tm2d.h2y. */
 int mt_abi_layout(long long* out, int n);
 
 #ifdef __cplusplus

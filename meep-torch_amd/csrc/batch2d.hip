@@ -316,6 +316,10 @@ extern "C" mt_batch* mt_tm2d_batch_create(const mt_tm2d_batch_item* items, int n
       fail(MT_EINVAL, "batched run: magnetic (TFSF) sources are not supported");
       return nullptr;
     }
+    if (d.n_poles || d.kerr || d.d1 || d.h1x) {
+      fail(MT_EINVAL, "batched run: dispersive, nonlinear or conductive media are not supported");
+      return nullptr;
+    }
     if (check_axis(d.hy, d.ny - 1, "hy") || check_axis(d.hx, d.nx - 1, "hx") || check_axis(d.ex, d.nx - 2, "ex") ||
         check_axis(d.ey, d.y_pmc ? d.ny : d.ny - 2, "ey"))
       return nullptr;

@@ -31,6 +31,10 @@ struct TmK {
   T *Ez, *Dz, *Hx, *Hy, *pEzy, *pEzx, *pHyx, *pHxy;
   int ny_e;
   T* tape;  // optional (k, nx, ny) record of Ez after each step
+  // optional media (null / 0 when absent)
+  int n_poles, kerr;
+  const T *c1, *c2, *c3, *chi2, *chi3, *d1, *d2, *h1x, *h2x, *h1y, *h2y;
+  T *Pc, *Pp;
 };
 
 // meep_torch/core/stepper_tm.py:85-92,126-128 (B half, H from curl E)
@@ -62,7 +66,8 @@ __global__ __launch_bounds__(BX* BY) void tm_b_kernel(TmK<T> p, mt_source2d hxs,
       p.pEzy[h] = ps;
       curl = d + ps;
     }
-    T v = p.Hx[h] - p.dt * curl;
+    // stepper_tm.py:116-121: sigma_b / mu fold into h1, h2
+    T v = p.h1x ? p.h1x[h] * p.Hx[h] - p.h2x[h] * curl : p.Hx[h] - p.dt * curl;
     if (in_box<T>(hxs, i, j)) v = v - hc * box_w<T>(hxs, i, j);
     p.Hx[h] = v;
   }
@@ -75,7 +80,7 @@ __global__ __launch_bounds__(BX* BY) void tm_b_kernel(TmK<T> p, mt_source2d hxs,
       p.pEzx[h] = ps;
       curl = d + ps;
     }
-    T v = p.Hy[h] + p.dt * curl;
+    T v = p.h1y ? p.h1y[h] * p.Hy[h] + p.h2y[h] * curl : p.Hy[h] + p.dt * curl;
     if (in_box<T>(hys, i, j)) v = v - hc * box_w<T>(hys, i, j);
     p.Hy[h] = v;
   }
@@ -91,6 +96,7 @@ __global__ __launch_bounds__(BX* BY) void tm_d_kernel(TmK<T> p, mt_source2d src,
   long long c = (long long)i * ny + j;
   T dz = p.Dz[c];
   bool live = i >= 1 && i < nx - 1 && (p.y_pmc || (j >= 1 && j < ny - 1));
+  T curl = T(0);  // zero on the PEC rows, as the reference's padding
   if (live) {
     // H = 0 beyond a PMC wall (stepper_tm.py:135-137)
     T hx1 = (j < ny - 1) ? p.Hx[(long long)i * (ny - 1) + j] : T(0);
@@ -105,20 +111,43 @@ __global__ __launch_bounds__(BX* BY) void tm_d_kernel(TmK<T> p, mt_source2d src,
       p.pHyx[q] = ps;
       a1 = dHy + ps;
     }
-    T curl = a1 - dHx;
+    curl = a1 - dHx;
     if (p.ey.in(jj)) {
       T ps = p.ey.b[jj] * p.pHxy[q] + p.ey.a[jj] * dHx;
       p.pHxy[q] = ps;
       curl = curl - ps;
     }
-    dz = dz + p.dt * curl;
+    if (!p.d1) dz = dz + p.dt * curl;
   }
+  if (p.d1) dz = p.d1[c] * dz + p.d2[c] * curl;  // stepper_tm.py:161-163 (sigma_d)
   if (src.present && i >= src.i0 && i < src.i1 && j >= src.j0 && j < src.j1) {
     const T* w = static_cast<const T*>(src.w);
-    dz = dz - coef * w[(long long)(i - src.i0) * (src.j1 - src.j0) + (j - src.j0)];
+    const T wv = w[(long long)(i - src.i0) * (src.j1 - src.j0) + (j - src.j0)];
+    // stepper_tm.py:198-200: under sigma_d the scale is d2 and coef the raw amplitude
+    dz = p.d2 ? dz - (p.d2[c] * coef) * wv : dz - coef * wv;
   }
   p.Dz[c] = dz;
-  T ez = dz / p.eps[c];
+  T rhs = dz;
+  if (p.n_poles) {  // stepper_tm.py:203-208, 224: P(n+1) from Ez(n), then Dz - sum P
+    const T ezo = p.Ez[c];
+    const long long plane = (long long)nx * ny;
+    T sum = T(0);
+    for (int k = 0; k < p.n_poles; ++k) {
+      const long long q = k * plane + c;
+      const T pc = p.Pc[q], pp = p.Pp[q];
+      const T pn = p.c1[q] * pc + p.c2[q] * pp + p.c3[q] * ezo;
+      p.Pp[q] = pc;
+      p.Pc[q] = pn;
+      sum = k == 0 ? pn : sum + pn;
+    }
+    rhs = dz - sum;
+  }
+  const T e0 = p.eps[c];
+  T ez = rhs / e0;
+  if (p.kerr) {  // stepper_tm.py:228-231: D = eps E + chi2 E^2 + chi3 E^3 by fixed point
+    const T x2 = p.chi2[c], x3 = p.chi3[c];
+    for (int it = 0; it < 3; ++it) ez = rhs / (e0 + x2 * ez + x3 * ez * ez);
+  }
   p.Ez[c] = ez;
   if (p.tape) p.tape[(long long)s_rel * nx * ny + c] = ez;
 }
@@ -246,6 +275,21 @@ int tm_run_t(const mt_tm2d* d, const mt_source2d* src, const double* coef, const
   p.pHxy = static_cast<T*>(d->psi_Hxy);
   p.ny_e = d->y_pmc ? d->ny : d->ny - 2;
   p.tape = static_cast<T*>(d->tape);
+  p.n_poles = d->n_poles;
+  p.kerr = d->kerr;
+  p.c1 = static_cast<const T*>(d->c1);
+  p.c2 = static_cast<const T*>(d->c2);
+  p.c3 = static_cast<const T*>(d->c3);
+  p.Pc = static_cast<T*>(d->P_cur);
+  p.Pp = static_cast<T*>(d->P_prev);
+  p.chi2 = static_cast<const T*>(d->chi2);
+  p.chi3 = static_cast<const T*>(d->chi3);
+  p.d1 = static_cast<const T*>(d->d1);
+  p.d2 = static_cast<const T*>(d->d2);
+  p.h1x = static_cast<const T*>(d->h1x);
+  p.h2x = static_cast<const T*>(d->h2x);
+  p.h1y = static_cast<const T*>(d->h1y);
+  p.h2y = static_cast<const T*>(d->h2y);
   mt_source2d sv{};
   if (src && src->present) sv = *src;
   dim3 blk(BX, BY), grd = grid2(p.ny, p.nx, blk);
@@ -501,10 +545,18 @@ extern "C" int mt_tm2d_run(const mt_tm2d* p, const mt_source2d* src, const doubl
   if ((rc = check_src(&p->hsrc[0], p->nx, p->ny - 1)) || (rc = check_src(&p->hsrc[1], p->nx - 1, p->ny))) return rc;
   const bool hsrc = p->hsrc[0].present || p->hsrc[1].present;
   if (hsrc && !p->hcoef && k > 0) return fail(MT_EINVAL, "magnetic source without coefficients");
+  if (p->n_poles < 0) return fail(MT_EINVAL, "negative pole count");
+  if (p->n_poles && (!p->c1 || !p->c2 || !p->c3 || !p->P_cur || !p->P_prev))
+    return fail(MT_EINVAL, "poles without coefficients or pole stacks");
+  if (p->kerr && (!p->chi2 || !p->chi3)) return fail(MT_EINVAL, "chi2/chi3 fixed point without chi2 and chi3");
+  if (!p->d1 != !p->d2) return fail(MT_EINVAL, "sigma_d needs both d1 and d2");
+  if (!p->h1x != !p->h2x || !p->h1x != !p->h1y || !p->h1x != !p->h2y)
+    return fail(MT_EINVAL, "sigma_b/mu needs h1x, h2x, h1y and h2y");
+  const bool media = p->n_poles || p->kerr || p->d1 || p->h1x;
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (p->variant >= 0 && !hsrc) {   // magnetic currents are injected by the two-pass kernels
+  if (p->variant >= 0 && !hsrc && !media) {   // magnetic currents and media: two-pass kernels
     if (p->variant >= tiled_variant_count()) return fail(MT_EINVAL, "unknown tiled variant");
     if (!alt_ok(p->alt, 8)) return fail(MT_EINVAL, "tiled path needs the alt buffers");
     if (taps_ok(mp))

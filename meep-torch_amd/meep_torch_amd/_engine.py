@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -37,7 +37,11 @@ class TM2D(ctypes.Structure):
                 ("psi_Ezy", c_void_p), ("psi_Ezx", c_void_p),
                 ("psi_Hyx", c_void_p), ("psi_Hxy", c_void_p), ("tape", c_void_p),
                 ("variant", c_int), ("pad_", c_int), ("alt", c_void_p * 8),
-                ("hsrc", Source2D * 2), ("hcoef", ctypes.POINTER(c_double))]
+                ("hsrc", Source2D * 2), ("hcoef", ctypes.POINTER(c_double)),
+                ("n_poles", c_int), ("kerr", c_int), ("c1", c_void_p), ("c2", c_void_p), ("c3", c_void_p),
+                ("P_cur", c_void_p), ("P_prev", c_void_p), ("chi2", c_void_p), ("chi3", c_void_p),
+                ("d1", c_void_p), ("d2", c_void_p), ("h1x", c_void_p), ("h2x", c_void_p),
+                ("h1y", c_void_p), ("h2y", c_void_p)]
 
 
 class TM2DAdj(ctypes.Structure):

@@ -85,6 +85,9 @@ class BatchedTM:
 
     # ------------------------------------------------------------ allocation
     def _alloc(self, coeffs, source, monitors):
+        if (coeffs.c1.shape[0] or coeffs.has_kerr or coeffs.has_chi2 or coeffs.d1 is not None
+                or coeffs.h1x is not None or coeffs.mc1x is not None):
+            raise NotImplementedError("batched run: dispersive, nonlinear or conductive media are not supported")
         nx, ny = coeffs.eps_inf.shape
         z = lambda *s: torch.zeros(*s, dtype=self.dtype, device=self.device)
         ny_e = ny if coeffs.y_pmc else ny - 2

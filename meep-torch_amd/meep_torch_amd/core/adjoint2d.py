@@ -239,6 +239,9 @@ def _te_descriptor(c, nx, ny, dtype, device):
 
 
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
+    if mode == "tm" and (coeffs.c1.shape[0] or coeffs.has_kerr or coeffs.has_chi2 or coeffs.d1 is not None
+                         or coeffs.h1x is not None):
+        raise NotImplementedError("gradients through dispersive, nonlinear or conductive media are not supported")
     allowed = ("eps_inf",) if mode == "tm" else ("eps_x", "eps_y")
     for name, t in coeffs._asdict().items():
         if torch.is_tensor(t) and t.requires_grad and name not in allowed:

@@ -6,9 +6,10 @@ them keeps working; one extra trailing field, ``shells``, carries the CPML
 shell bounds the kernels use to skip the psi recursion in the interior.
 
 Supported by the HIP stepper: eps_inf, npml (int or ((xlo, xhi), (ylo, yhi))),
-and y_pmc for TM.  Dispersive poles, chi2/chi3, sigma_d/sigma_b/mu, magnetic
-poles, tensor eps and symmetry folds raise NotImplementedError naming the
-flag (no silent fallback)."""
+y_pmc for TM, and for TM the media terms: electric ADE poles, chi2/chi3 and
+sigma_d/sigma_b/mu.  TE media, magnetic poles, tensor eps and symmetry folds
+raise NotImplementedError naming the flag (no silent fallback)."""
+import math
 from typing import NamedTuple, Sequence
 
 import torch
@@ -125,6 +126,19 @@ def _as_field(val, grid: Grid, default: float) -> torch.Tensor:
     return torch.full((grid.nx, grid.ny), float(val), dtype=grid.dtype, device=grid.device)
 
 
+def pole_recurrence(pole, dt):
+    """coeffs.py:87-106: ADE scalars of one pole (fn, gamma, sigma[, fc]);
+    fc = fn is a Lorentz pole, fn = 0 with fc the plasma frequency a Drude
+    pole.  w0 = 2 pi fn, wc = 2 pi fc, G = 2 pi gamma, den = 1 + G dt / 2:
+    c1 = (2 - w0^2 dt^2) / den, c2 = -(1 - G dt / 2) / den,
+    c3 = sigma wc^2 dt^2 / den (times the pole's spatial mask)."""
+    fn, gamma, sigma = pole[0], pole[1], pole[2]
+    fc = pole[3] if len(pole) > 3 else fn
+    w0, wc, G = 2 * math.pi * fn, 2 * math.pi * fc, 2 * math.pi * gamma
+    den = 1.0 + G * dt / 2.0
+    return (2.0 - w0 * w0 * dt * dt) / den, -(1.0 - G * dt / 2.0) / den, sigma * wc * wc * dt * dt / den
+
+
 def _unsupported(flag):
     raise NotImplementedError(
         f"meep_torch_amd HIP stepper: '{flag}' is not supported yet (eps, CPML and y_pmc are)")
@@ -142,19 +156,38 @@ def build_tm_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
                     sigma_b=None, mu=None, m_poles: Sequence = (), m_pole_masks=None,
                     y_mirror: int = 0, x_mirror: int = 0, y_rot2: int = 0,
                     rot4: int = 0) -> TMCoeffs:
-    """coeffs.py:134-291, eps + CPML (+ y_pmc) subset."""
+    """coeffs.py:134-291: eps + CPML (+ y_pmc), electric ADE poles
+    (coeffs.py:87-106, 190-198), chi2 / chi3 (coeffs.py:154-158) and
+    sigma_d / sigma_b / mu (coeffs.py:160-175)."""
     dt, dx = grid.dt, grid.dx
     eps = _as_field(eps_inf, grid, 1.0)
     x3 = _as_field(chi3, grid, 0.0)
     x2 = _as_field(chi2, grid, 0.0)
-    if bool((x3 != 0).any()):
-        _unsupported("chi3 (Kerr)")
-    if bool((x2 != 0).any()):
-        _unsupported("chi2")
-    if len(poles):
-        _unsupported("poles (ADE dispersion)")
+    has_kerr = bool((x3 != 0).any())
+    has_chi2 = bool((x2 != 0).any())
+    cond = {}
     if sigma_d is not None or sigma_b is not None or mu is not None:
-        _unsupported("sigma_d/sigma_b/mu")
+        sd = _as_field(sigma_d, grid, 0.0)
+        sb = _as_field(sigma_b, grid, 0.0)
+        mu_ = _as_field(mu, grid, 1.0)
+        cond["d1"] = (1 - sd * dt / 2) / (1 + sd * dt / 2)
+        cond["d2"] = dt / (1 + sd * dt / 2)
+        sbx, mux = 0.5 * (sb[:, :-1] + sb[:, 1:]), 0.5 * (mu_[:, :-1] + mu_[:, 1:])   # Hx (nx, ny-1)
+        sby, muy = 0.5 * (sb[:-1, :] + sb[1:, :]), 0.5 * (mu_[:-1, :] + mu_[1:, :])   # Hy (nx-1, ny)
+        cond["h1x"] = (1 - sbx * dt / 2) / (1 + sbx * dt / 2)
+        cond["h2x"] = dt / (mux * (1 + sbx * dt / 2))
+        cond["h1y"] = (1 - sby * dt / 2) / (1 + sby * dt / 2)
+        cond["h2y"] = dt / (muy * (1 + sby * dt / 2))
+    n_poles = len(poles)
+    c1 = grid.zeros(n_poles, grid.nx, grid.ny)
+    c2 = grid.zeros(n_poles, grid.nx, grid.ny)
+    c3 = grid.zeros(n_poles, grid.nx, grid.ny)
+    for k, pole in enumerate(poles):
+        v1, v2, v3 = pole_recurrence(pole, dt)
+        mask = pole_masks[k] if pole_masks is not None else torch.ones_like(eps)
+        c1[k] = v1 * torch.ones_like(eps)
+        c2[k] = v2 * torch.ones_like(eps)
+        c3[k] = v3 * mask.to(eps)
     if len(m_poles):
         _unsupported("m_poles")
     if y_mirror or x_mirror or y_rot2 or rot4:
@@ -175,10 +208,9 @@ def build_tm_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
     shells = {"hy": shell_bounds(b_hy, a_hy), "hx": shell_bounds(b_hx, a_hx),
               "ex": shell_bounds(b_ex, a_ex), "ey": shell_bounds(b_ey, a_ey)}
     dev = lambda t: t.contiguous().to(grid.device)
-    z = grid.zeros(0, grid.nx, grid.ny)
     return TMCoeffs(
-        dt=dt, dx=dx, eps_inf=eps, chi3=x3, has_kerr=False, chi2=x2, has_chi2=False,
-        c1=z, c2=z.clone(), c3=z.clone(), y_pmc=y_pmc,
+        dt=dt, dx=dx, eps_inf=eps, chi3=x3, has_kerr=has_kerr, chi2=x2, has_chi2=has_chi2,
+        c1=c1, c2=c2, c3=c3, y_pmc=y_pmc, **cond,
         b_hy=dev(b_hy)[None, :], a_hy=dev(a_hy)[None, :], k_hy=dev(k_hy)[None, :],
         b_hx=dev(b_hx)[:, None], a_hx=dev(a_hx)[:, None], k_hx=dev(k_hx)[:, None],
         b_ex=dev(b_ex)[:, None], a_ex=dev(a_ex)[:, None], k_ex=dev(k_ex)[:, None],

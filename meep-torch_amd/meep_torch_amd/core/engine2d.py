@@ -66,11 +66,13 @@ def coef_array(plan, dt, step0, k):
     return (ctypes.c_double * max(1, k))(*vals)
 
 
-def coef_arrays(plan, dt, step0, k):
+def coef_arrays(plan, dt, step0, k, raw=False):
     """(dt * amplitude(t), dt * h_amplitude(t)) host arrays for the k steps
     from step0, each None when that current is absent.  Per step the electric
     amplitude is taken before the magnetic one, in the reference loop's order
-    (run.py:17-23), so stateful waveforms see the same call sequence."""
+    (run.py:17-23), so stateful waveforms see the same call sequence.  raw:
+    the electric coefficient is amplitude(t) itself (TM under sigma_d, where
+    the kernel scales it by d2, stepper_tm.py:198-200)."""
     if plan is None:
         return None, None
     src = plan["source"]
@@ -79,7 +81,7 @@ def coef_arrays(plan, dt, step0, k):
     for t in range(step0, step0 + k):
         a = src.amplitude(t)
         if want_e:
-            ev.append(dt * a)
+            ev.append(a if raw else dt * a)
         if want_h:
             hv.append(dt * src.h_amplitude(t))
     n = max(1, k)
@@ -175,6 +177,40 @@ def _axis(coeffs, name):
     return E.pml_axis(b, a, coeffs.shells[name] if coeffs.shells else None)
 
 
+def _tm_media(d, c, fields, dtype, device):
+    """Fill the media fields of an mt_tm2d from TMCoeffs (electric poles with
+    the state's pole stacks, chi2 / chi3, sigma_d, sigma_b / mu); returns the
+    device tensors the descriptor points to."""
+    keep = []
+
+    def put(name, t):
+        t = t.to(device=device, dtype=dtype).contiguous()
+        keep.append(t)
+        setattr(d, name, t.data_ptr())
+
+    if c.mc1x is not None:
+        raise NotImplementedError("TM magnetic poles (m_poles) are not supported by the HIP stepper")
+    n = int(c.c1.shape[0])
+    if n:
+        if "P_cur" not in fields or int(fields["P_cur"].shape[0]) != n:
+            raise ValueError(f"{n} poles need P_cur / P_prev stacks of {n} in the state")
+        d.n_poles = n
+        for name in ("c1", "c2", "c3"):
+            put(name, getattr(c, name))
+        d.P_cur, d.P_prev = fields["P_cur"].data_ptr(), fields["P_prev"].data_ptr()
+    if c.has_kerr or c.has_chi2:
+        d.kerr = 1
+        put("chi2", c.chi2 if c.chi2 is not None else torch.zeros_like(c.eps_inf))
+        put("chi3", c.chi3)
+    if c.d1 is not None:
+        put("d1", c.d1)
+        put("d2", c.d2)
+    if c.h1x is not None:
+        for name in ("h1x", "h2x", "h1y", "h2y"):
+            put(name, getattr(c, name))
+    return keep
+
+
 TM_ORDER = ("Ez", "Dz", "Hx", "Hy", "psi_Ezy", "psi_Ezx", "psi_Hyx", "psi_Hxy")
 TE_ORDER = ("Hz", "Dx", "Dy", "psi_Eyx", "psi_Exy", "psi_Hzy", "psi_Hzx")
 
@@ -204,7 +240,7 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
         if tape is not None:
             assert tape.is_contiguous() and tape.shape == (k, nx, ny) and tape.dtype == dtype
             d.tape = tape.data_ptr()
-        keep_c = [eps]
+        keep_c = [eps] + _tm_media(d, coeffs, fields, dtype, device)
     else:
         nx, ny = fields["Ey"].shape[0], fields["Ex"].shape[1]
         d = E.TE2D()
@@ -225,8 +261,9 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
     for f, t in fields.items():
         if not t.is_contiguous():
             raise ValueError(f"state field {f} must be contiguous")
-    if plan is not None and plan.get("h"):
-        variant = -1  # magnetic currents are injected by the two-pass kernels
+    media = mode == "tm" and bool(d.n_poles or d.kerr or d.d1 or d.h1x)
+    if (plan is not None and plan.get("h")) or media:
+        variant = -1  # magnetic currents and media are handled by the two-pass kernels
     if variant is None:
         from .. import tune
         variant = tune.variant_for(mode, fields, coeffs, plan, monitors, k)
@@ -238,7 +275,7 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
         for q, f in enumerate(order):
             d.alt[q] = alt[f].data_ptr()
     src, wkeep = source_struct(plan)
-    coef, hcoef = coef_arrays(plan, coeffs.dt, step0, k)
+    coef, hcoef = coef_arrays(plan, coeffs.dt, step0, k, raw=mode == "tm" and coeffs.d1 is not None)
     hkeep = []
     if plan is not None and plan.get("h"):
         for slot, hb in plan["h"]:

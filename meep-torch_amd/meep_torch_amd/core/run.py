@@ -14,7 +14,7 @@ from . import engine2d
 from .stepper_te import FIELDS as TE_FIELDS
 from .stepper_te import TEState, step_te
 from .stepper_tm import FIELDS as TM_FIELDS
-from .stepper_tm import TMState, _check as _check_tm, step_tm
+from .stepper_tm import TMState, _check as _check_tm, fields_of as tm_fields_of, step_tm
 
 
 class _Fixed:
@@ -42,8 +42,10 @@ def _mode_of(step_fn, state):
     raise NotImplementedError(f"step function {step_fn!r} has no HIP engine")
 
 
-def _fields(mode):
-    return TM_FIELDS if mode == "tm" else TE_FIELDS
+def _fields(mode, state=None):
+    if mode == "tm":
+        return tm_fields_of(state) if state is not None else TM_FIELDS
+    return TE_FIELDS
 
 
 def _hshapes(mode, work):
@@ -74,7 +76,7 @@ def _absorb(acc, monitors, outs):
 
 def _run_engine(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
     _validate(mode, state, coeffs)
-    names = _fields(mode)
+    names = _fields(mode, state)
     work = {f: getattr(state, f).detach().clone().contiguous() for f in names}
     plan = engine2d.source_plan(source, _target_shape(mode, work), work[names[0]].device,
                                 work[names[0]].dtype, _hshapes(mode, work))
@@ -120,7 +122,7 @@ def _run_chunk(step_fn, state, coeffs, step0: int, k: int, source, monitors):
     (state, [chunk output per monitor])."""
     mode = _mode_of(step_fn, state)
     _validate(mode, state, coeffs)
-    names = _fields(mode)
+    names = _fields(mode, state)
     work = {f: getattr(state, f).detach().clone().contiguous() for f in names}
     plan = engine2d.source_plan(source, _target_shape(mode, work), work[names[0]].device,
                                 work[names[0]].dtype, _hshapes(mode, work))

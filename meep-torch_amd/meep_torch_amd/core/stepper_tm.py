@@ -37,21 +37,31 @@ FIELDS = ("Ez", "Dz", "Hx", "Hy", "psi_Ezy", "psi_Ezx", "psi_Hyx", "psi_Hxy")
 def init_tm_state(grid: Grid, n_poles: int = 0, y_pmc: bool = False, n_mpoles: int = 0,
                   y_mirror: int = 0, x_mirror: int = 0, y_rot2: int = 0, rot4: int = 0) -> TMState:
     """stepper_tm.py:37-71 (psi_H rows: ny-2 with PEC walls, ny under y_pmc)."""
-    if n_poles or n_mpoles or y_mirror or x_mirror or y_rot2 or rot4:
+    if n_mpoles or y_mirror or x_mirror or y_rot2 or rot4:
         raise NotImplementedError(
-            "init_tm_state: poles / magnetic poles / symmetry folds are not supported by the HIP stepper")
+            "init_tm_state: magnetic poles / symmetry folds are not supported by the HIP stepper")
     nx, ny = grid.nx, grid.ny
     ny_e = ny if y_pmc else ny - 2
     return TMState(
         Ez=grid.zeros(nx, ny), Hx=grid.zeros(nx, ny - 1), Hy=grid.zeros(nx - 1, ny),
-        Dz=grid.zeros(nx, ny), P_cur=grid.zeros(0, nx, ny), P_prev=grid.zeros(0, nx, ny),
+        Dz=grid.zeros(nx, ny), P_cur=grid.zeros(n_poles, nx, ny), P_prev=grid.zeros(n_poles, nx, ny),
         psi_Ezy=grid.zeros(nx, ny - 1), psi_Ezx=grid.zeros(nx - 1, ny),
         psi_Hyx=grid.zeros(nx - 2, ny_e), psi_Hxy=grid.zeros(nx - 2, ny_e))
 
 
+def fields_of(state: TMState):
+    """State fields the engine advances: FIELDS plus the pole stacks when the
+    state carries poles."""
+    return FIELDS + (("P_cur", "P_prev") if state.P_cur.shape[0] else ())
+
+
 def _check(state: TMState, m: TMCoeffs):
-    if state.P_cur.shape[0] or state.Bx is not None or state.Hx0 is not None:
-        raise NotImplementedError("TM state with poles / magnetic poles / mirror rows")
+    if state.Bx is not None or state.Hx0 is not None:
+        raise NotImplementedError("TM state with magnetic poles / mirror rows")
+    if m.mc1x is not None:
+        raise NotImplementedError("TM magnetic poles (m_poles) are not supported by the HIP stepper")
+    if state.P_cur.shape[0] != m.c1.shape[0]:
+        raise ValueError(f"state has {state.P_cur.shape[0]} pole stacks, coefficients {m.c1.shape[0]} poles")
     nx, ny = state.Ez.shape
     ny_e = ny if m.y_pmc else ny - 2
     if tuple(state.psi_Hxy.shape) != (nx - 2, ny_e):
@@ -66,7 +76,7 @@ def step_tm(state: TMState, m: TMCoeffs, src_amp=0.0, src_mask: torch.Tensor | N
     from .run import _Fixed
     from . import engine2d
     _check(state, m)
-    work = {f: getattr(state, f).clone() for f in FIELDS}
+    work = {f: getattr(state, f).clone() for f in fields_of(state)}
     plan = None
     h = {"src_hy_mask": src_hy_mask, "src_hx_mask": src_hx_mask}
     if src_mask is not None or any(v is not None for v in h.values()):

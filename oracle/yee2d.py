@@ -2,9 +2,10 @@
 
 Operation order follows the reference so FP32 results round identically:
   profiles      meep_torch/core/pml.py:17-59
-  TM coeffs     meep_torch/core/coeffs.py:134-291 (eps, npml, y_pmc subset)
+  TM coeffs     meep_torch/core/coeffs.py:87-106,134-291 (eps, npml, y_pmc,
+                electric poles, chi2/chi3, sigma_d/sigma_b/mu)
   TE coeffs     meep_torch/core/coeffs.py:357-503 (eps, npml subset)
-  TM step       meep_torch/core/stepper_tm.py:78-254 (no poles/sigma/mirrors)
+  TM step       meep_torch/core/stepper_tm.py:78-254 (no magnetic poles/mirrors)
   TE step       meep_torch/core/stepper_te.py:73-190 (no poles/sigma/tensor eps)
   sources       meep_torch/core/sources.py:26-91
   monitors      meep_torch/core/monitors.py:11-155
@@ -72,10 +73,69 @@ class Coef2D:
     a_ex: torch.Tensor = None
     b_ey: torch.Tensor = None
     a_ey: torch.Tensor = None
+    # TM media (None / empty when absent): ADE poles (np, nx, ny), chi2 / chi3
+    # (nx, ny), conductivity / permeability factors
+    c1: torch.Tensor = None
+    c2: torch.Tensor = None
+    c3: torch.Tensor = None
+    chi2: torch.Tensor = None
+    chi3: torch.Tensor = None
+    kerr: bool = False
+    d1: torch.Tensor = None
+    d2: torch.Tensor = None
+    h1x: torch.Tensor = None
+    h2x: torch.Tensor = None
+    h1y: torch.Tensor = None
+    h2y: torch.Tensor = None
 
 
-def tm_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, y_pmc=False):
-    """meep_torch/core/coeffs.py:134-291 restricted to eps + CPML (+ y_pmc)."""
+def pole_recurrence(pole, dt):
+    """meep_torch/core/coeffs.py:87-106: ADE scalars of one pole
+    (fn, gamma, sigma[, fc]): w0 = 2 pi fn, wc = 2 pi fc (fc = fn: Lorentz),
+    G = 2 pi gamma, den = 1 + G dt / 2, c1 = (2 - w0^2 dt^2) / den,
+    c2 = -(1 - G dt / 2) / den, c3 = sigma wc^2 dt^2 / den."""
+    fn, gamma, sigma = pole[0], pole[1], pole[2]
+    fc = pole[3] if len(pole) > 3 else fn
+    w0, wc, G = 2 * math.pi * fn, 2 * math.pi * fc, 2 * math.pi * gamma
+    den = 1.0 + G * dt / 2.0
+    return (2.0 - w0 * w0 * dt * dt) / den, -(1.0 - G * dt / 2.0) / den, sigma * wc * wc * dt * dt / den
+
+
+def _tm_media(c, nx, ny, dtype, poles, pole_masks, chi2, chi3, sigma_d, sigma_b, mu):
+    """meep_torch/core/coeffs.py:154-200 (electric poles, chi2 / chi3,
+    sigma_d / sigma_b / mu)."""
+    dt = c.dt
+    full = lambda v, d: (torch.full((nx, ny), float(d if v is None else v), dtype=dtype)
+                         if not torch.is_tensor(v) else v.to(dtype))
+    n = len(poles)
+    c.c1 = torch.zeros(n, nx, ny, dtype=dtype)
+    c.c2 = torch.zeros(n, nx, ny, dtype=dtype)
+    c.c3 = torch.zeros(n, nx, ny, dtype=dtype)
+    for k, pole in enumerate(poles):
+        v1, v2, v3 = pole_recurrence(pole, dt)
+        mask = pole_masks[k] if pole_masks is not None else torch.ones(nx, ny, dtype=dtype)
+        c.c1[k] = v1 * torch.ones(nx, ny, dtype=dtype)
+        c.c2[k] = v2 * torch.ones(nx, ny, dtype=dtype)
+        c.c3[k] = v3 * mask.to(dtype)
+    c.chi2, c.chi3 = full(chi2, 0.0), full(chi3, 0.0)
+    c.kerr = bool((c.chi2 != 0).any()) or bool((c.chi3 != 0).any())
+    if sigma_d is not None or sigma_b is not None or mu is not None:
+        sd, sb, mu_ = full(sigma_d, 0.0), full(sigma_b, 0.0), full(mu, 1.0)
+        c.d1 = (1 - sd * dt / 2) / (1 + sd * dt / 2)
+        c.d2 = dt / (1 + sd * dt / 2)
+        sbx, mux = 0.5 * (sb[:, :-1] + sb[:, 1:]), 0.5 * (mu_[:, :-1] + mu_[:, 1:])
+        sby, muy = 0.5 * (sb[:-1, :] + sb[1:, :]), 0.5 * (mu_[:-1, :] + mu_[1:, :])
+        c.h1x = (1 - sbx * dt / 2) / (1 + sbx * dt / 2)
+        c.h2x = dt / (mux * (1 + sbx * dt / 2))
+        c.h1y = (1 - sby * dt / 2) / (1 + sby * dt / 2)
+        c.h2y = dt / (muy * (1 + sby * dt / 2))
+    return c
+
+
+def tm_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, y_pmc=False, poles=(),
+              pole_masks=None, chi2=None, chi3=None, sigma_d=None, sigma_b=None, mu=None):
+    """meep_torch/core/coeffs.py:134-291: eps + CPML (+ y_pmc), electric ADE
+    poles, chi2 / chi3 and sigma_d / sigma_b / mu."""
     dt = courant * dx
     eps = eps.to(dtype)
     if isinstance(npml, (tuple, list)):
@@ -94,11 +154,12 @@ def tm_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, y_pmc=Fal
         b_ey, a_ey = b_eyf, a_eyf
     else:
         b_ey, a_ey = b_eyf[1:-1], a_eyf[1:-1]
-    return Coef2D(dt=dt, dx=dx, eps=eps, y_pmc=y_pmc,
-                  b_hy=b_hy[None, :], a_hy=a_hy[None, :],
-                  b_hx=b_hx[:, None], a_hx=a_hx[:, None],
-                  b_ex=b_exf[1:-1][:, None], a_ex=a_exf[1:-1][:, None],
-                  b_ey=b_ey[None, :], a_ey=a_ey[None, :])
+    c = Coef2D(dt=dt, dx=dx, eps=eps, y_pmc=y_pmc,
+               b_hy=b_hy[None, :], a_hy=a_hy[None, :],
+               b_hx=b_hx[:, None], a_hx=a_hx[:, None],
+               b_ex=b_exf[1:-1][:, None], a_ex=a_exf[1:-1][:, None],
+               b_ey=b_ey[None, :], a_ey=a_ey[None, :])
+    return _tm_media(c, nx, ny, dtype, poles, pole_masks, chi2, chi3, sigma_d, sigma_b, mu)
 
 
 def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32):
@@ -128,13 +189,17 @@ def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32):
 
 
 # --------------------------------------------------------------------- state
-def tm_state(nx, ny, dtype=torch.float32, y_pmc=False):
-    """Field dict with the TMState shapes of stepper_tm.py:12-71."""
+def tm_state(nx, ny, dtype=torch.float32, y_pmc=False, n_poles=0):
+    """Field dict with the TMState shapes of stepper_tm.py:12-71 (pole
+    stacks only when n_poles > 0)."""
     z = lambda *s: torch.zeros(*s, dtype=dtype)
     ny_e = ny if y_pmc else ny - 2
-    return {"Ez": z(nx, ny), "Hx": z(nx, ny - 1), "Hy": z(nx - 1, ny), "Dz": z(nx, ny),
-            "psi_Ezy": z(nx, ny - 1), "psi_Ezx": z(nx - 1, ny),
-            "psi_Hyx": z(nx - 2, ny_e), "psi_Hxy": z(nx - 2, ny_e)}
+    out = {"Ez": z(nx, ny), "Hx": z(nx, ny - 1), "Hy": z(nx - 1, ny), "Dz": z(nx, ny),
+           "psi_Ezy": z(nx, ny - 1), "psi_Ezx": z(nx - 1, ny),
+           "psi_Hyx": z(nx - 2, ny_e), "psi_Hxy": z(nx - 2, ny_e)}
+    if n_poles:
+        out.update(P_cur=z(n_poles, nx, ny), P_prev=z(n_poles, nx, ny))
+    return out
 
 
 def te_state(nx, ny, dtype=torch.float32):
@@ -152,16 +217,21 @@ def tm_step(s, c, coef=0.0, mask=None, hcoef=0.0, hx_mask=None, hy_mask=None):
     currents of the B phase (stepper_tm.py:180-193: H -= (dt*h_amp) * mask),
     D half (stepper_tm.py:134-165, PEC or y_pmc), soft source
     (stepper_tm.py:197-200: Dz -= (dt*amp) * mask, the scalar formed in
-    double first), then Ez = Dz / eps (stepper_tm.py:226-227).  ``coef`` is
-    dt*amp, ``hcoef`` dt*h_amp."""
+    double first), the ADE poles and Ez = (Dz - sum P) / eps with the chi2 /
+    chi3 fixed point (stepper_tm.py:203-231).  ``coef`` is dt*amp (the raw
+    amp under sigma_d, scaled by d2), ``hcoef`` dt*h_amp."""
     dx, dt = c.dx, c.dt
     Ez = s["Ez"]
     g = (Ez[:, 1:] - Ez[:, :-1]) / dx
     pzy = c.b_hy * s["psi_Ezy"] + c.a_hy * g
-    Hx = s["Hx"] - dt * (g + pzy)
     g2 = (Ez[1:, :] - Ez[:-1, :]) / dx
     pzx = c.b_hx * s["psi_Ezx"] + c.a_hx * g2
-    Hy = s["Hy"] + dt * (g2 + pzx)
+    if c.h1x is not None:       # stepper_tm.py:116-121: dB/dt = -curl E - sigma_b B, mu folded in h2
+        Hx = c.h1x * s["Hx"] - c.h2x * (g + pzy)
+        Hy = c.h1y * s["Hy"] + c.h2y * (g2 + pzx)
+    else:
+        Hx = s["Hx"] - dt * (g + pzy)
+        Hy = s["Hy"] + dt * (g2 + pzx)
     if hy_mask is not None:
         Hy = Hy - hcoef * hy_mask
     if hx_mask is not None:
@@ -177,11 +247,29 @@ def tm_step(s, c, coef=0.0, mask=None, hcoef=0.0, hx_mask=None, hy_mask=None):
     pyx = c.b_ex * s["psi_Hyx"] + c.a_ex * dHy
     pxy = c.b_ey * s["psi_Hxy"] + c.a_ey * dHx
     curl = dHy + pyx - dHx - pxy
-    Dz = s["Dz"] + dt * torch.nn.functional.pad(curl, pad_y + (1, 1))
+    if c.d1 is not None:        # stepper_tm.py:161-163: dD/dt = curl H - sigma_d D
+        Dz = c.d1 * s["Dz"] + c.d2 * torch.nn.functional.pad(curl, pad_y + (1, 1))
+    else:
+        Dz = s["Dz"] + dt * torch.nn.functional.pad(curl, pad_y + (1, 1))
     if mask is not None:
-        Dz = Dz - coef * mask
+        if c.d1 is not None:    # stepper_tm.py:198-200: scale = d2, coef is then the raw amplitude
+            Dz = Dz - c.d2 * coef * mask
+        else:
+            Dz = Dz - coef * mask
     out = dict(s)
-    out.update(Hx=Hx, Hy=Hy, Dz=Dz, Ez=Dz / c.eps, psi_Ezy=pzy, psi_Ezx=pzx,
+    rhs = Dz
+    if c.c1 is not None and c.c1.shape[0]:
+        # stepper_tm.py:203-208: P(n+1) = c1 P(n) + c2 P(n-1) + c3 Ez(n)
+        P_new = c.c1 * s["P_cur"] + c.c2 * s["P_prev"] + c.c3 * Ez
+        out.update(P_cur=P_new, P_prev=s["P_cur"])
+        rhs = Dz - P_new.sum(0)
+    # stepper_tm.py:226-231: E = rhs / eps, then 3 fixed-point passes of
+    # E = rhs / (eps + chi2 E + chi3 E^2) when chi2 or chi3 is present
+    Ez_new = rhs / c.eps
+    if c.kerr:
+        for _ in range(3):
+            Ez_new = rhs / (c.eps + c.chi2 * Ez_new + c.chi3 * Ez_new * Ez_new)
+    out.update(Hx=Hx, Hy=Hy, Dz=Dz, Ez=Ez_new, psi_Ezy=pzy, psi_Ezx=pzx,
                psi_Hyx=pyx, psi_Hxy=pxy)
     return out
 
@@ -318,7 +406,9 @@ def run2d(mode, state, c, n_steps, amps=None, mask=None, monitors=(), checkpoint
         for t in range(t0, t0 + k):
             hk = {} if hamps is None else dict(hmasks, hcoef=c.dt * hamps[t])
             if amps is not None:
-                s = step(s, c, c.dt * amps[t], mask, **hk)
+                # under sigma_d the source is scaled by d2 instead of dt (stepper_tm.py:198-200)
+                a = amps[t] if getattr(c, "d1", None) is not None else c.dt * amps[t]
+                s = step(s, c, a, mask, **hk)
             else:
                 s = step(s, c, **hk)
             for i, mon in enumerate(monitors):

@@ -38,14 +38,15 @@ def test_ctypes_structs_match_the_compiled_layout():
     L = E.lib()
     buf = (ctypes.c_longlong * 64)()
     n = L.mt_abi_layout(buf, 64)
-    assert n == 19
+    assert n == 21
     sizes = [E.PmlAxis, E.Source2D, E.TM2D, E.TE2D, E.Yee3D, E.Source3D, E.DftTerm, E.Monitor, E.TM2DAdj,
              E.TE2DAdj, E.Yee3DAdj, BatchItem]
     for q, st in enumerate(sizes):
         assert buf[q] == ctypes.sizeof(st), (st.__name__, buf[q], ctypes.sizeof(st))
     offs = [E.TM2D.hsrc.offset, E.TM2D.hcoef.offset, E.TE2D.tape.offset, E.TE2D.hsrc.offset,
-            E.TE2D.hcoef.offset, E.Yee3D.alt.offset, E.Monitor.acc.offset]
-    assert list(buf[12:19]) == offs
+            E.TE2D.hcoef.offset, E.Yee3D.alt.offset, E.Monitor.acc.offset, E.TM2D.n_poles.offset,
+            E.TM2D.h2y.offset]
+    assert list(buf[12:21]) == offs
 
 
 def test_error_path_without_gpu_work():
@@ -93,14 +94,41 @@ def test_mask_box():
     assert mask_box(torch.zeros(4, 4)) is None
 
 
-@pytest.mark.parametrize("kw,flag", [({"chi3": 0.1}, "chi3"), ({"poles": [(1.0, 0.1, 1.0)]}, "poles"),
-                                     ({"sigma_d": 0.1}, "sigma"), ({"y_mirror": 1}, "symmetry")])
-def test_unsupported_flags_raise(kw, flag):
-    from meep_torch_amd.core.coeffs import build_tm_coeffs
+@pytest.mark.parametrize("te,kw,flag", [(True, {"chi3": 0.1}, "chi3"), (True, {"poles": [(1.0, 0.1, 1.0)]}, "poles"),
+                                        (True, {"sigma_d": 0.1}, "sigma"), (False, {"y_mirror": 1}, "symmetry"),
+                                        (False, {"m_poles": [(1.0, 0.1, 1.0)]}, "m_poles")])
+def test_unsupported_flags_raise(te, kw, flag):
+    """TE media, TM magnetic poles and symmetry folds still raise by name."""
+    from meep_torch_amd.core.coeffs import build_te_coeffs, build_tm_coeffs
     from meep_torch_amd.core.grid import Grid
     g = Grid(nx=16, ny=12, dx=0.05, dtype=torch.float32, device="cpu")
     with pytest.raises(NotImplementedError, match=flag):
-        build_tm_coeffs(g, npml=2, **kw)
+        (build_te_coeffs if te else build_tm_coeffs)(g, npml=2, **kw)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_tm_media_coefficients_equal_the_oracle(dtype):
+    """build_tm_coeffs's pole, chi2/chi3 and sigma/mu arrays are the oracle's,
+    bit for bit (the oracle's are pinned by tests/test_oracle_media_cpu.py)."""
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from oracle import yee2d
+    nx, ny = 24, 18
+    gen = torch.Generator().manual_seed(1)
+    eps = 1.0 + torch.rand(nx, ny, generator=gen, dtype=torch.float64)
+    m1 = (torch.rand(nx, ny, generator=gen, dtype=torch.float64) > 0.5).double()
+    kw = dict(poles=[(1.2, 0.1, 1.5), (0.0, 0.3, 2.0, 0.9)], pole_masks=[m1, 1.0 - m1],
+              chi3=0.02 * m1, chi2=0.01 * (1 - m1), sigma_d=0.5 * m1, sigma_b=0.2 * m1, mu=1.0 + m1)
+    g = Grid(nx=nx, ny=ny, dx=0.05, dtype=dtype, device="cpu")
+    m = build_tm_coeffs(g, eps_inf=eps, npml=3, **{k: (v.to(dtype) if torch.is_tensor(v) else
+                                                       [x.to(dtype) for x in v] if k == "pole_masks" else v)
+                                                    for k, v in kw.items()})
+    o = yee2d.tm_coeffs(nx, ny, 0.05, eps, 3, dtype=dtype, **{k: (v.to(dtype) if torch.is_tensor(v) else
+                                                                  [x.to(dtype) for x in v] if k == "pole_masks"
+                                                                  else v) for k, v in kw.items()})
+    assert m.has_kerr and m.has_chi2 and o.kerr
+    for name in ("c1", "c2", "c3", "chi2", "chi3", "d1", "d2", "h1x", "h2x", "h1y", "h2y"):
+        assert torch.equal(getattr(m, name), getattr(o, name)), name
 
 
 def test_run_requires_hip_device():

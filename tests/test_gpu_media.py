@@ -1,0 +1,199 @@
+"""TM media on the HIP engine (the two-pass kernels): electric ADE poles
+(Lorentz and Drude), the chi2 / chi3 fixed point and sigma_d / sigma_b / mu
+(meep_torch/core/coeffs.py:87-106,154-200, stepper_tm.py:116-121,161-163,
+198-231).
+
+* every term at once against the oracle: FP64 fields to 1e-12, FP32 within
+  the 1e-5 fields / 1e-4 spectra bars, with and without checkpoint chunks,
+  and one step of step_tm from a random state;
+* the reference's physics scenes through run_tm: a Lorentz slab against the
+  transfer matrix (3%, meep_torch tests/analytic/test_dispersion.py) and a
+  sigma_d slab against meep's transmission (amplitude rms 3%, the fixture of
+  meep_torch tests/examples/test_w5_conductivity.py);
+* gradients through media and the batched launch refuse loudly."""
+import cmath
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import has_gpu
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a HIP device")]
+HERE = os.path.dirname(os.path.abspath(__file__))
+NX, NY, NPML, DX = 96, 64, 10, 0.05
+FREQS = [0.6, 0.8, 1.0]
+
+
+def _media(dtype):
+    from meep_torch_amd import synth
+    r = torch.tensor(synth.splitmix_uniform(NX * NY, 31).reshape(NX, NY), dtype=torch.float64)
+    blk = torch.zeros(NX, NY, dtype=torch.float64)
+    blk[50:75, 16:48] = 1.0
+    eps = 1.0 + 1.5 * blk * r
+    kw = dict(poles=[(1.2, 0.1, 1.5), (0.0, 0.3, 0.5, 0.9)],        # Lorentz, Drude
+              pole_masks=[blk.to(dtype), (blk * (r > 0.5)).to(dtype)],
+              chi3=(0.05 * blk).to(dtype), chi2=(0.02 * blk * r).to(dtype),
+              sigma_d=(0.3 * blk).to(dtype), sigma_b=(0.1 * blk).to(dtype), mu=(1.0 + 0.5 * blk).to(dtype))
+    return eps.to(dtype), kw
+
+
+def _hip(dtype, ck=0, n=400):
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTM
+    from meep_torch_amd.core.run import run_tm
+    from meep_torch_amd.core.sources import GaussianPulse, LineSource
+    from meep_torch_amd.core.stepper_tm import init_tm_state
+    eps, kw = _media(dtype)
+    g = Grid(nx=NX, ny=NY, dx=DX, dtype=dtype, device="cuda")
+    m = build_tm_coeffs(g, eps_inf=eps.cuda(), npml=NPML,
+                        **{k: ([x.cuda() for x in v] if k == "pole_masks" else v.cuda() if torch.is_tensor(v) else v)
+                           for k, v in kw.items()})
+    # drive so that chi3 |E|^2 reaches about 0.07 in the block: far above
+    # rounding, mildly nonlinear
+    src = LineSource(g, ix=20, j0=16, j1=48, profile=GaussianPulse(0.8, 0.5), amp_func=lambda y: 5.0)
+    mon = FluxMonitorTM(g, FREQS, x_normal=True, idx=80, lo=12, hi=52)
+    st, (acc,) = run_tm(init_tm_state(g, n_poles=2), m, n, source=src, monitors=[mon], checkpoint_every=ck)
+    return st, acc, src
+
+
+def _oracle(dtype, src, n=400):
+    from oracle import yee2d
+    torch.set_num_threads(8)
+    eps, kw = _media(dtype)
+    c = yee2d.tm_coeffs(NX, NY, DX, eps, NPML, dtype=dtype, **kw)
+    mons = [{"type": "flux", "kind": "tm_x", "idx": 80, "lo": 12, "hi": 52, "freqs": FREQS}]
+    amps = [src.amplitude(t) for t in range(n)]
+    return yee2d.run2d("tm", yee2d.tm_state(NX, NY, dtype, n_poles=2), c, n, amps=amps,
+                       mask=src.mask.cpu(), monitors=mons)
+
+
+def rel(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    return float((a - b).norm() / b.norm())
+
+
+@pytest.mark.parametrize("dtype,tf,ts", [(torch.float64, 1e-12, 1e-12), (torch.float32, 1e-5, 1e-4)],
+                         ids=["f64", "f32"])
+@pytest.mark.parametrize("ck", [0, 150])
+def test_media_run_matches_oracle(dtype, tf, ts, ck):
+    st, acc, src = _hip(dtype, ck)
+    s, (oacc,) = _oracle(dtype, src)
+    names = ("Ez", "Dz", "Hx", "Hy", "P_cur", "P_prev")
+    errs = {f: rel(getattr(st, f), s[f]) for f in names}
+    errs["acc"] = rel(acc, oacc)
+    print(f"\n[media {dtype} ck={ck}] " + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
+    assert float(s["P_cur"].abs().max()) > 0 and float(s["Ez"].abs().max()) > 0
+    for f in names:
+        assert errs[f] <= tf, (f, errs[f])
+    assert errs["acc"] <= ts
+
+
+def test_media_step_tm_matches_oracle():
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.stepper_tm import init_tm_state, step_tm
+    from oracle import yee2d
+    dt = torch.float64
+    eps, kw = _media(dt)
+    g = Grid(nx=NX, ny=NY, dx=DX, dtype=dt, device="cuda")
+    m = build_tm_coeffs(g, eps_inf=eps.cuda(), npml=NPML,
+                        **{k: ([x.cuda() for x in v] if k == "pole_masks" else v.cuda() if torch.is_tensor(v) else v)
+                           for k, v in kw.items()})
+    gen = torch.Generator().manual_seed(4)
+    st = init_tm_state(g, n_poles=2)
+    st = st._replace(**{f: torch.randn(getattr(st, f).shape, generator=gen, dtype=dt).cuda()
+                        for f in ("Ez", "Dz", "Hx", "Hy", "P_cur", "P_prev")})
+    mask = torch.zeros(NX, NY, dtype=dt)
+    mask[20, 16:48] = 1.0 / DX
+    out = step_tm(st, m, 0.7, mask.cuda())
+    c = yee2d.tm_coeffs(NX, NY, DX, eps, NPML, dtype=dt, **kw)
+    ref = yee2d.tm_step({f: getattr(st, f).cpu() for f in ("Ez", "Dz", "Hx", "Hy", "P_cur", "P_prev", "psi_Ezy",
+                                                          "psi_Ezx", "psi_Hyx", "psi_Hxy")},
+                        c, 0.7, mask)                 # raw amplitude: sigma_d scales it by d2
+    for f in ("Ez", "Dz", "Hx", "Hy", "P_cur", "P_prev"):
+        assert rel(getattr(out, f), ref[f]) <= 1e-13, f
+
+
+def test_lorentz_slab_transmission_on_hip():
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTM
+    from meep_torch_amd.core.run import run_tm
+    from meep_torch_amd.core.sources import GaussianPulse, LineSource
+    from meep_torch_amd.core.stepper_tm import init_tm_state
+    pole, eps_inf = (2.0, 0.1, 1.5), 2.25
+    freqs = [0.50, 0.55, 0.60, 0.65, 0.70, 0.75, 0.80]
+
+    def flux(on):
+        g = Grid(nx=1400, ny=8, dx=0.02, dtype=torch.float64, device="cuda")
+        slab = torch.zeros(1400, 8, dtype=torch.float64, device="cuda")
+        slab[600:700, :] = 1.0
+        kw = dict(eps_inf=1.0 + (eps_inf - 1.0) * slab, poles=[pole], pole_masks=[slab]) if on else {}
+        m = build_tm_coeffs(g, npml=40, y_pmc=True, **kw)
+        src = LineSource(g, ix=100, j0=0, j1=8, profile=GaussianPulse(0.65, 0.35))
+        mon = FluxMonitorTM(g, freqs, x_normal=True, idx=1200, lo=0, hi=8)
+        _, (acc,) = run_tm(init_tm_state(g, n_poles=1 if on else 0, y_pmc=True), m, 9000, source=src,
+                           monitors=[mon])
+        return mon.flux(acc, g.dx).cpu()
+
+    T = flux(True) / flux(False)
+    worst = 0.0
+    for i, f in enumerate(freqs):
+        fn, gm, s = pole
+        n2 = cmath.sqrt(eps_inf + s * fn * fn / (fn * fn - f * f - 1j * f * gm))
+        delta = 2 * cmath.pi * f * n2 * 2.0
+        r12 = (1 - n2) / (1 + n2)
+        t = (2 / (1 + n2)) * (2 * n2 / (1 + n2)) * cmath.exp(1j * delta) / (1 - r12 ** 2 * cmath.exp(2j * delta))
+        e = abs(float(T[i]) - abs(t) ** 2) / abs(t) ** 2
+        worst = max(worst, e)
+        assert e < 0.03, (f, float(T[i]), abs(t) ** 2)
+    print(f"\n[lorentz slab] worst relative error vs transfer matrix {worst:.2e}")
+
+
+def test_conductive_slab_vs_meep_on_hip():
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTM
+    from meep_torch_amd.core.run import run_tm
+    from meep_torch_amd.core.sources import GaussianPulse, LineSource
+    from meep_torch_amd.core.stepper_tm import init_tm_state
+    with open(os.path.join(HERE, "golden", "meep_conductive_slab.json")) as fh:
+        ref = json.load(fh)
+    dx = 1.0 / ref["resolution"]
+    nx = int(round(30 / dx))
+
+    def flux(on):
+        g = Grid(nx=nx, ny=8, dx=dx, dtype=torch.float32, device="cuda")
+        sd = torch.zeros(nx, 8, dtype=torch.float32, device="cuda")
+        sd[int(round(14 / dx)):int(round(16 / dx)), :] = ref["sigma_d"]
+        m = build_tm_coeffs(g, npml=int(round(1.5 / dx)), y_pmc=True, **({"sigma_d": sd} if on else {}))
+        src = LineSource(g, ix=int(round(5 / dx)), j0=0, j1=8, profile=GaussianPulse(ref["fcen"], ref["df"]))
+        mon = FluxMonitorTM(g, ref["freqs"], x_normal=True, idx=int(round(21 / dx)), lo=0, hi=8)
+        _, (acc,) = run_tm(init_tm_state(g, y_pmc=True), m, 10000, source=src, monitors=[mon])
+        return mon.flux(acc, g.dx).double().cpu().numpy()
+
+    amp = np.sqrt(flux(True) / flux(False))
+    rms = float(np.sqrt(np.mean((amp / np.sqrt(ref["T_power"]) - 1.0) ** 2)))
+    print(f"\n[conductive slab fp32] amplitude rms vs meep {rms:.2e}")
+    assert rms < 0.03
+
+
+def test_media_refuse_gradients_and_batches():
+    from meep_torch_amd.batch import BatchedTM
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.run import run_tm
+    from meep_torch_amd.core.stepper_tm import init_tm_state
+    g = Grid(nx=32, ny=32, dx=0.05, dtype=torch.float64, device="cuda")
+    eps = torch.ones(32, 32, dtype=torch.float64, device="cuda", requires_grad=True)
+    m = build_tm_coeffs(g, eps_inf=eps, npml=4, sigma_d=0.1)
+    with pytest.raises(NotImplementedError):
+        run_tm(init_tm_state(g), m, 5)
+    m2 = build_tm_coeffs(g, eps_inf=2.0, npml=4, poles=[(1.0, 0.1, 1.0)])
+    with pytest.raises(NotImplementedError):
+        BatchedTM([(m2, None, [])], 10)
