@@ -85,20 +85,76 @@ def pml_cells(thickness, gs):
     return max(0, int(round(thickness / gs.dx)))
 
 
-class Medium:
-    """Isotropic, non-dispersive dielectric: epsilon, or index (eps = n^2).
-    Other meep Medium properties raise NotImplementedError by name."""
+def _iso(sigma, sigma_diag, sigma_offdiag, what):
+    """One scalar sigma from meep's sigma / sigma_diag / sigma_offdiag
+    (meep_torch/mp/material.py:63-83): anisotropic values raise."""
+    if sigma_offdiag is not None and any(float(v) for v in Vector3(*sigma_offdiag)):
+        raise NotImplementedError(f"{what}: anisotropic sigma_offdiag is not supported")
+    if sigma:
+        return float(sigma)
+    d = Vector3(*(sigma_diag if sigma_diag is not None else Vector3()))
+    if not (d.x == d.y == d.z):
+        raise NotImplementedError(f"{what}: anisotropic sigma_diag is not supported")
+    return float(d.x)
 
-    def __init__(self, epsilon=None, index=None, **unsupported):
+
+class LorentzianSusceptibility:
+    """chi(f) = sigma fn^2 / (fn^2 - f^2 - i f gamma); engine pole
+    (fn, gamma, sigma) (meep_torch/mp/material.py:89-107)."""
+
+    def __init__(self, frequency=0.0, gamma=0.0, sigma=None, sigma_diag=None, sigma_offdiag=None):
+        self.frequency, self.gamma = float(frequency), float(gamma)
+        self.sigma = _iso(sigma, sigma_diag, sigma_offdiag, "LorentzianSusceptibility")
+
+    def to_pole(self):
+        return (self.frequency, self.gamma, self.sigma)
+
+
+class DrudeSusceptibility:
+    """chi(f) = -sigma fp^2 / (f^2 + i f gamma); engine pole
+    (0, gamma, sigma, fp): no restoring force, coupling at the plasma
+    frequency (meep_torch/mp/material.py:110-132)."""
+
+    def __init__(self, frequency=0.0, gamma=0.0, sigma=None, sigma_diag=None, sigma_offdiag=None):
+        self.frequency, self.gamma = float(frequency), float(gamma)
+        self.sigma = _iso(sigma, sigma_diag, sigma_offdiag, "DrudeSusceptibility")
+
+    def to_pole(self):
+        return (0.0, self.gamma, self.sigma, self.frequency)
+
+
+class Medium:
+    """Isotropic medium: epsilon (or index, eps = n^2), electric Lorentzian /
+    Drude susceptibilities, chi3, D and B conductivities and mu (scalars).
+    The media terms reach 2D TM runs; other meep Medium properties raise
+    NotImplementedError by name."""
+
+    def __init__(self, epsilon=None, index=None, E_susceptibilities=(), chi3=None, D_conductivity=None,
+                 B_conductivity=None, mu=None, **unsupported):
         for k, v in unsupported.items():
             if v not in (None, 0, 0.0, (), [], 1, 1.0, False):
                 raise NotImplementedError(f"Medium({k}=...) is not supported by the HIP FDTD path")
         if index is not None and epsilon is not None:
             raise ValueError("Medium: give epsilon or index, not both")
         self.epsilon = float(index) ** 2 if index is not None else (1.0 if epsilon is None else float(epsilon))
+        for sus in E_susceptibilities or ():
+            if not isinstance(sus, (LorentzianSusceptibility, DrudeSusceptibility)):
+                raise NotImplementedError(f"susceptibility {type(sus).__name__} is not supported")
+        self.E_susceptibilities = list(E_susceptibilities or ())
+        self.chi3 = float(chi3 or 0.0)
+        self.D_conductivity = float(D_conductivity or 0.0)
+        self.B_conductivity = float(B_conductivity or 0.0)
+        self.mu = float(mu or 1.0)
 
     def resolve(self):
         return self.epsilon
+
+    def poles(self):
+        return tuple(s.to_pole() for s in self.E_susceptibilities)
+
+    def has_media(self):
+        return bool(self.E_susceptibilities) or self.chi3 != 0.0 or self.D_conductivity != 0.0 or \
+            self.B_conductivity != 0.0 or self.mu != 1.0
 
 
 class GeometricObject:
@@ -177,6 +233,61 @@ def rasterize(objects, gs, background=1.0):
         te_xx = np.where(edge, n2x * harm + (1.0 - n2x) * arith, arith)
         te_yy = np.where(edge, n2y * harm + (1.0 - n2y) * arith, arith)
     return eps_tm, te_xx, te_yy
+
+
+def rasterize_media(objects, gs, background):
+    """Media maps of the painter's scene (meep_torch/geometry/scene.py:53-101,
+    meep_torch/mp/geometry.py:265-286, meep_torch/mp/simulation.py:526-550),
+    float64 numpy (nx, ny):
+
+    * poles / pole masks: the background's poles cover everything; each
+      object scales every existing mask by (1 - coverage) and adds its
+      coverage to the masks of its own poles (a pole equal to an earlier one
+      shares its mask);
+    * chi3: coverage-weighted over the objects whose chi3 is nonzero, then
+      the background's chi3 (when nonzero) painted over the whole cell;
+    * sigma_d / sigma_b / mu: when any object or the background has one,
+      each is the arithmetic coverage blend of every object's value over the
+      background's (objects without the property paint its default).
+    Returns {"poles", "masks", "chi3" (or None), "cond" ((sd, sb, mu) or None)}."""
+    shape = (gs.nx, gs.ny)
+    poles = list(background.poles())
+    masks = [np.ones(shape) for _ in poles]
+    chi3, any_chi3 = np.zeros(shape), False
+    covs = []
+    for obj in objects:
+        if not isinstance(obj, Block):
+            raise NotImplementedError(f"geometry object {type(obj).__name__} is not supported (Block only)")
+        cov = obj.coverage(gs)
+        m = obj.material
+        covs.append((cov, m))
+        for k in range(len(masks)):
+            masks[k] = (1.0 - cov) * masks[k]
+        for p in m.poles():
+            if p in poles:
+                k = poles.index(p)
+                masks[k] = cov + masks[k]
+            else:
+                poles.append(p)
+                masks.append(cov.copy())
+        if m.chi3 != 0.0:
+            chi3 = cov * m.chi3 + (1.0 - cov) * chi3
+            any_chi3 = True
+    if background.chi3 != 0.0:
+        full = np.ones(shape)
+        chi3 = full * background.chi3 + (1.0 - full) * chi3
+        any_chi3 = True
+    cond = None
+    bg = (background.D_conductivity, background.B_conductivity, background.mu)
+    if any((m.D_conductivity, m.B_conductivity, m.mu) != (0.0, 0.0, 1.0) for _, m in covs) or bg != (0.0, 0.0, 1.0):
+        maps = []
+        for q, name in enumerate(("D_conductivity", "B_conductivity", "mu")):
+            v = np.full(shape, bg[q])
+            for cov, m in covs:
+                v = cov * getattr(m, name) + (1.0 - cov) * v
+            maps.append(v)
+        cond = tuple(maps)
+    return {"poles": poles, "masks": masks, "chi3": chi3 if any_chi3 else None, "cond": cond}
 
 
 class PML:

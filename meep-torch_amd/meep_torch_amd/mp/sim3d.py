@@ -135,11 +135,15 @@ class Sim3D:
         if self._coeffs is not None:
             return
         gs = self.gs
+        if self._default.has_media():
+            raise NotImplementedError("dispersive, nonlinear or conductive media in 3D cells are not supported")
         bg = float(self._default.resolve())
         eps = torch.full((gs.nx, gs.ny, gs.nz), bg, dtype=self.dtype, device=self.device)
         for obj in self._geometry:
             if not isinstance(obj, G.Block):
                 raise NotImplementedError(f"3D geometry {type(obj).__name__} is not supported (Block only)")
+            if obj.material.has_media():
+                raise NotImplementedError("dispersive, nonlinear or conductive media in 3D cells are not supported")
             i0, i1, j0, j1, k0, k1 = box3d(obj.center, obj.size, gs)
             val = float(obj.material.resolve())
             eps[i0:i1, j0:j1, k0:k1] = val

@@ -29,7 +29,7 @@ from ..core.grid import Grid
 from ..core.monitors import DFTFieldsMonitor, FluxMonitorTE, FluxMonitorTEY, FluxMonitorTM
 from ..core.sources import LineSource
 from ..core.stepper_te import FIELDS as TE_FIELDS, init_te_state
-from ..core.stepper_tm import FIELDS as TM_FIELDS, init_tm_state
+from ..core.stepper_tm import fields_of as tm_fields_of, init_tm_state
 from . import geometry as G
 from . import run_control as RC
 from .sim3d import Sim3D
@@ -132,9 +132,22 @@ class Simulation(Sim3D):
         self._raster = G.rasterize(self._geometry, self.gs, self._default.resolve())
         g = self.grid
         npml = (tuple(self._npml_sides[0]), tuple(self._npml_sides[1]))
+        media = G.rasterize_media(self._geometry, self.gs, self._default)
+        has_media = bool(media["poles"]) or media["chi3"] is not None or media["cond"] is not None
+        self._n_poles = len(media["poles"])
         if pol == "tm":
             eps = G.to_tensor(self._raster[0], g.dtype, g.device)
-            self._coeffs = build_tm_coeffs(g, eps_inf=eps, npml=npml)
+            kw = {}
+            if media["poles"]:
+                kw.update(poles=media["poles"], pole_masks=[G.to_tensor(m, g.dtype, g.device) for m in media["masks"]])
+            if media["chi3"] is not None:
+                kw["chi3"] = G.to_tensor(media["chi3"], g.dtype, g.device)
+            if media["cond"] is not None:
+                sd, sb, mu = (G.to_tensor(v, g.dtype, g.device) for v in media["cond"])
+                kw.update(sigma_d=sd, sigma_b=sb, mu=mu)
+            self._coeffs = build_tm_coeffs(g, eps_inf=eps, npml=npml, **kw)
+        elif has_media:
+            raise NotImplementedError("dispersive, nonlinear or conductive media in TE runs are not supported")
         else:
             eps_xx = G.to_tensor(self._raster[1], torch.float64, g.device)
             eps_yy = G.to_tensor(self._raster[2], torch.float64, g.device)
@@ -256,8 +269,8 @@ class Simulation(Sim3D):
         elif n_total is None:
             T = float(until) if until is not None else cutoff + float(until_after_sources or 0.0)
             n_total = max(1, int(round(T / dt)))
-        names = TM_FIELDS if mode == "tm" else TE_FIELDS
-        st = init_tm_state(self.grid) if mode == "tm" else init_te_state(self.grid)
+        st = init_tm_state(self.grid, n_poles=self._n_poles) if mode == "tm" else init_te_state(self.grid)
+        names = tm_fields_of(st) if mode == "tm" else TE_FIELDS
         work = {f: getattr(st, f).contiguous() for f in names}
         srcs = self._make_sources()
         src = srcs[0] if srcs else None

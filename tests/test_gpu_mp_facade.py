@@ -169,3 +169,47 @@ def test_flux_data_roundtrip():
     c.load_flux_data(fc, data)
     c.run(until=40)
     assert mp.get_fluxes(fc) == [4.0 * v for v in mp.get_fluxes(fa)]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_media_blocks_equal_direct_run(dtype):
+    """A TM cell with a Lorentz block (eps 2) and a Drude + chi3 + D-conductivity
+    block, both on cell boundaries so their masks are exactly 0 / 1, against
+    run_tm on hand-built coefficients: fields and fluxes identical."""
+    import meep_torch_amd.mp as mp
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.monitors import FluxMonitorTM
+    from meep_torch_amd.core.run import run_tm
+    from meep_torch_amd.core.stepper_tm import init_tm_state
+    L = mp.LorentzianSusceptibility(frequency=1.2, gamma=0.1, sigma=1.5)
+    D = mp.DrudeSusceptibility(frequency=0.9, gamma=0.3, sigma=0.5)
+    sim = mp.Simulation(
+        cell_size=mp.Vector3(10, 6), resolution=10,
+        geometry=[mp.Block(center=mp.Vector3(-2.5, 0), size=mp.Vector3(1, mp.inf),
+                           material=mp.Medium(epsilon=2, E_susceptibilities=[L])),
+                  mp.Block(center=mp.Vector3(1.5, 0), size=mp.Vector3(1, 2),
+                           material=mp.Medium(E_susceptibilities=[D], chi3=0.05, D_conductivity=0.2))],
+        sources=[mp.Source(mp.GaussianSource(0.8, fwidth=0.5), component=mp.Ez,
+                           center=mp.Vector3(-4, 0), size=mp.Vector3(0, 2))],
+        boundary_layers=[mp.PML(1.0)], dtype=dtype, device="cuda")
+    fl = sim.add_flux(0.8, 0.4, 3, mp.FluxRegion(center=mp.Vector3(4, 0), size=mp.Vector3(0, 4)))
+    sim.run(until=20)
+    # by hand: x corner [2, 3) -> i 20..29 (all y); x [6, 7), y [2, 4) -> i 60..69, j 20..39
+    g = sim.grid
+    z = lambda v=0.0: torch.full((100, 60), v, dtype=dtype, device="cuda")
+    eps, mL, mD, chi3, sd = z(1.0), z(), z(), z(), z()
+    eps[20:30, :] = 2.0
+    mL[20:30, :] = 1.0
+    mD[60:70, 20:40] = 1.0
+    chi3[60:70, 20:40] = 0.05
+    sd[60:70, 20:40] = 0.2
+    m = build_tm_coeffs(g, eps_inf=eps, npml=((10, 10), (10, 10)), poles=[L.to_pole(), D.to_pole()],
+                        pole_masks=[mL, mD], chi3=chi3, sigma_d=sd, sigma_b=z(), mu=z(1.0))
+    src = sim._make_sources()[0]
+    freqs = np.linspace(0.8 - 0.4 / 2.0, 0.8 + 0.4 / 2.0, 3).tolist()     # meep's fcen +- df/2 grid
+    mon = FluxMonitorTM(g, freqs, x_normal=True, idx=90, lo=10, hi=50)
+    st, (acc,) = run_tm(init_tm_state(g, n_poles=2), m, sim._t_step, source=src, monitors=[mon])
+    assert sim._t_step == 400
+    assert np.array_equal(sim.get_array(component=mp.Ez), st.Ez.cpu().numpy())
+    assert np.array_equal(np.asarray(mp.get_fluxes(fl)), mon.flux(acc, g.dx).cpu().numpy())
+    assert float(st.Ez.abs().max()) > 0 and float(st.P_cur.abs().max()) > 0

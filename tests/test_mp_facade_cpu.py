@@ -78,7 +78,7 @@ def test_unsupported_options_raise():
     with pytest.raises(NotImplementedError):
         mp.Simulation(cell_size=mp.Vector3(4, 4, 4), resolution=10, dimensions=3, Courant=0.6, device="cpu")
     with pytest.raises(NotImplementedError):
-        mp.Medium(epsilon=2, chi3=0.1)
+        mp.Medium(epsilon=2, H_susceptibilities=[object()])
     with pytest.raises(NotImplementedError):
         mp.Block(size=mp.Vector3(1, 1), e1=mp.Vector3(1, 1))
     sim3 = mp.Simulation(cell_size=mp.Vector3(1, 1, 1), resolution=10, dimensions=3, device="cpu")
@@ -126,3 +126,60 @@ def test_3d_courant_and_options_raise():
         mp.Simulation(cell_size=mp.Vector3(1, 1, 1), resolution=10, dimensions=3, Courant=0.4)
     with pytest.raises(NotImplementedError):
         mp.Simulation(cell_size=mp.Vector3(1, 1, 1), resolution=10, dimensions=1)
+
+
+def test_media_rasterization_compositing():
+    """Painter rules of the media maps (meep_torch/geometry/scene.py:84-96,
+    meep_torch/mp/geometry.py:265-286): masks are scaled by (1 - coverage)
+    under every later object and gain the coverage of objects sharing the
+    pole; chi3 blends over objects with nonzero chi3 and a nonzero background
+    chi3 is painted last over everything; conductivity / mu blend over every
+    object once any medium has one."""
+    import numpy as np
+    import meep_torch_amd.mp as mp
+    from meep_torch_amd.mp import geometry as G
+    gs = G.grid_spec(mp.Vector3(4, 4), 10)
+    L = mp.LorentzianSusceptibility(frequency=1.2, gamma=0.1, sigma=1.5)
+    D = mp.DrudeSusceptibility(frequency=0.9, gamma=0.3, sigma=0.5)
+    a = mp.Block(center=mp.Vector3(-0.5, 0), size=mp.Vector3(1.0, mp.inf),
+                 material=mp.Medium(epsilon=2, E_susceptibilities=[L]))
+    b = mp.Block(center=mp.Vector3(-0.275, 0), size=mp.Vector3(1.0, 2.0),    # half cells on its x edges
+                 material=mp.Medium(epsilon=3, E_susceptibilities=[L], chi3=0.1))
+    c = mp.Block(center=mp.Vector3(1.0, 0), size=mp.Vector3(0.6, 0.6),
+                 material=mp.Medium(E_susceptibilities=[D], D_conductivity=0.3, mu=2.0))
+    ca, cb, cc = (o.coverage(gs) for o in (a, b, c))
+    assert ((cb > 0) & (cb < 1)).any()                                     # partial cells exist
+    out = G.rasterize_media([a, b, c], gs, mp.Medium())
+    assert out["poles"] == [(1.2, 0.1, 1.5), (0.0, 0.3, 0.5, 0.9)]
+    mL, mD = out["masks"]
+    assert np.array_equal(mL, (1 - cc) * (cb + (1 - cb) * ca))
+    assert np.array_equal(mD, cc)
+    assert np.array_equal(out["chi3"], cb * 0.1)
+    sd, sb, mu = out["cond"]
+    assert np.array_equal(sd, cc * 0.3) and not sb.any()
+    v = np.ones((gs.nx, gs.ny))
+    for cov, val in ((ca, 1.0), (cb, 1.0), (cc, 2.0)):                  # every object paints mu
+        v = cov * val + (1.0 - cov) * v
+    assert np.array_equal(mu, v)
+    # a nonzero background chi3 is painted over everything, last
+    out2 = G.rasterize_media([a, b], gs, mp.Medium(chi3=0.05))
+    assert np.array_equal(out2["chi3"], np.full((gs.nx, gs.ny), 0.05))
+    # no media anywhere: no maps
+    out3 = G.rasterize_media([mp.Block(size=mp.Vector3(1, 1), material=mp.Medium(epsilon=4))], gs, mp.Medium())
+    assert out3["poles"] == [] and out3["chi3"] is None and out3["cond"] is None
+
+
+def test_media_outside_tm_raise():
+    import meep_torch_amd.mp as mp
+    med = mp.Medium(epsilon=2, D_conductivity=0.1)
+    te = mp.Simulation(cell_size=mp.Vector3(2, 2), resolution=10, device="cpu",
+                       geometry=[mp.Block(size=mp.Vector3(1, 1), material=med)],
+                       sources=[mp.Source(mp.GaussianSource(1.0, fwidth=0.5), component=mp.Hz)])
+    with pytest.raises(NotImplementedError):
+        te.init_sim()
+    s3 = mp.Simulation(cell_size=mp.Vector3(1, 1, 1), resolution=10, dimensions=3, device="cpu",
+                       geometry=[mp.Block(size=mp.Vector3(0.5, 0.5, 0.5), material=med)])
+    with pytest.raises(NotImplementedError):
+        s3.init_sim()
+    with pytest.raises(NotImplementedError):
+        mp.LorentzianSusceptibility(frequency=1.0, gamma=0.1, sigma_diag=mp.Vector3(1, 2, 1))
