@@ -129,7 +129,7 @@ def build_c1(steps_override=None):
     return run, 256 * 256, steps, None
 
 
-def build_c4(n_var=64, steps=4000, ck=200, seed=4):
+def build_c4(n_var=64, steps=4000, ck=200, seed=4, variant=None):
     """C4: n_var TM design variants, N ~ U{512..1536} (splitmix), 128x128
     density (U[0,1)) -> conic filter (r = 4 cells) -> tanh projection (beta 8)
     -> eps 2.1..12.1 painted at the centre of an eps-2.1 cell, 24-cell CPML,
@@ -157,7 +157,7 @@ def build_c4(n_var=64, steps=4000, ck=200, seed=4):
         mons.append(mon)
     rho = [torch.tensor(synth.splitmix_uniform(128 * 128, 100 + v).reshape(128, 128),
                         dtype=torch.float32, device="cuda") for v in range(n_var)]
-    bt = BatchedTM(probs, steps, checkpoint_every=ck)
+    bt = BatchedTM(probs, steps, checkpoint_every=ck, variant=variant)
     cells = sum(n * n for n in sizes)
 
     def eps_of(v, r):
@@ -223,6 +223,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB float32 tensor three times (reference for PMC byte counters)")
     ap.add_argument("--c4-variants", type=int, default=64)
+    ap.add_argument("--c4-tile", type=int, default=None, help="tile variant of the batched launch (mt_tiled_variants)")
     ap.add_argument("--yee-steps", type=int, default=0, help="override Yee steps per run (tuning)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -326,7 +327,7 @@ def main():
 
 def bench_c4(args):
     steps = args.yee_steps or 4000
-    it, cells, ysteps, sizes = build_c4(args.c4_variants, steps)
+    it, cells, ysteps, sizes = build_c4(args.c4_variants, steps, variant=args.c4_tile)
     log(f"[bench] c4: {len(sizes)} variants, {cells} cells, {ysteps} steps, sizes {min(sizes)}..{max(sizes)}")
     for i in range(args.warmup):
         it()

@@ -71,7 +71,8 @@ class BatchedTM:
         E.require_device(c0.eps_inf)
         self.code = E.dtype_code(self.dtype)
         if variant is None:
-            variant = 2 if self.dtype == torch.float32 else 1
+            # FP32: 32 x 64 tiles, 4 steps per launch (13% faster on C4 than 16 x 64)
+            variant = 5 if self.dtype == torch.float32 else 1
         self.variant = int(variant)
         self.S = int(stage_steps)
         self.adjoint = bool(adjoint)
