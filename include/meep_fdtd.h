@@ -115,6 +115,20 @@ typedef struct {
    * (stepper_te.py:120-128), as hsrc/hcoef of mt_tm2d */
   mt_source2d hsrc;
   const double* hcoef;
+  /* optional media (coeffs.py:357-440; stepper_te.py:96-103,110-115,
+   * 131-134,137-168), two-pass path only:
+   *   npx / npy electric ADE poles on Ex / Ey: c1x, c2x, c3x and the stacks
+   *     Px_cur, Px_prev (npx, nx-1, ny); c1y, c2y, c3y, Py_cur, Py_prev
+   *     (npy, nx, ny-1); P(n+1/2) = c1 P(n-1/2) + c2 P(n-3/2) + c3 E(n-1/2),
+   *     E = (D - sum P) / eps;
+   *   d1x, d2x (nx-1, ny), d1y, d2y (nx, ny-1): Dx = d1x Dx + d2x curl_x,
+   *     Dy = d1y Dy - d2y curl_y (sigma_d); the electric source is then
+   *     (d2y * coef[s]) * w with coef[s] the raw amplitude;
+   *   hz1, hz2 (nx-1, ny-1): Hz = hz1 Hz - hz2 curl (sigma_b and mu). */
+  int npx, npy;
+  const void *c1x, *c2x, *c3x, *c1y, *c2y, *c3y;
+  void *Px_cur, *Px_prev, *Py_cur, *Py_prev;
+  const void *d1x, *d2x, *d1y, *d2y, *hz1, *hz2;
 } mt_te2d;
 
 
@@ -350,7 +364,7 @@ int mt_abi_version(void);
  * mt_source3d, mt_dft_term, mt_monitor, mt_tm2d_adj, mt_te2d_adj,
  * mt_yee3d_adj, mt_tm2d_batch_item, then offsetof of mt_tm2d.hsrc,
  * mt_tm2d.hcoef, mt_te2d.tape, mt_te2d.hsrc, mt_te2d.hcoef, mt_yee3d.alt,
- * mt_monitor.acc, mt_tm2d.n_poles, mt_tm2d.h2y. */
+ * mt_monitor.acc, mt_tm2d.n_poles, mt_tm2d.h2y, mt_te2d.npx, mt_te2d.hz2. */
 int mt_abi_layout(long long* out, int n);
 
 #ifdef __cplusplus

@@ -160,7 +160,33 @@ struct TeK {
   Pml<T> hx, hy, ey, ex;
   T *Hz, *Ex, *Ey, *Dx, *Dy, *pEyx, *pExy, *pHzy, *pHzx;
   T* tape;  // optional: [Ex | Ey] after each step
+  // optional media (null / 0 when absent)
+  int npx, npy;
+  const T *c1x, *c2x, *c3x, *c1y, *c2y, *c3y, *d1x, *d2x, *d1y, *d2y, *hz1, *hz2;
+  T *Pxc, *Pxp, *Pyc, *Pyp;
 };
+
+// E = (D - sum P) / eps after advancing the pole stacks from the E before
+// the step (stepper_te.py:137-147, 160-168); the stacks have `np` planes of
+// `plane` cells.
+template <typename T>
+__device__ __forceinline__ T te_e_from_d(T d, T eps, T e_old, int np, long long plane, long long c, const T* c1,
+                                         const T* c2, const T* c3, T* pc, T* pp) {
+  T rhs = d;
+  if (np) {
+    T sum = T(0);
+    for (int k = 0; k < np; ++k) {
+      const long long q = k * plane + c;
+      const T a = pc[q], b = pp[q];
+      const T pn = c1[q] * a + c2[q] * b + c3[q] * e_old;
+      pp[q] = a;
+      pc[q] = pn;
+      sum = k == 0 ? pn : sum + pn;
+    }
+    rhs = d - sum;
+  }
+  return rhs / eps;
+}
 
 // meep_torch/core/stepper_te.py:80-103 (D half), :131-134 (source on Dy),
 // :160-168 (E = D / eps)
@@ -173,9 +199,10 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
   if (i < nx - 1) {  // Dx (nx-1, ny)
     long long c = (long long)i * ny + j;
     T dxv = p.Dx[c];
+    T curl = T(0);  // zero on the PEC rows, as the reference's padding
     if (j >= 1 && j < ny - 1) {
       T d = (p.Hz[(long long)i * (ny - 1) + j] - p.Hz[(long long)i * (ny - 1) + j - 1]) / p.dx;
-      T curl = d;
+      curl = d;
       int jj = j - 1;
       if (p.ey.in(jj)) {
         long long q = (long long)i * (ny - 2) + jj;
@@ -183,19 +210,22 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
         p.pHzy[q] = ps;
         curl = d + ps;
       }
-      dxv = dxv + p.dt * curl;
+      if (!p.d1x) dxv = dxv + p.dt * curl;
     }
+    if (p.d1x) dxv = p.d1x[c] * dxv + p.d2x[c] * curl;  // stepper_te.py:96-99 (sigma_d)
     p.Dx[c] = dxv;
-    T ex = dxv / p.eps_x[c];
+    const T ex = te_e_from_d(dxv, p.eps_x[c], p.npx ? p.Ex[c] : T(0), p.npx, (long long)(nx - 1) * ny, c, p.c1x,
+                             p.c2x, p.c3x, p.Pxc, p.Pxp);
     p.Ex[c] = ex;
     if (p.tape) p.tape[(long long)s_rel * ((long long)(nx - 1) * ny + (long long)nx * (ny - 1)) + c] = ex;
   }
   if (j < ny - 1) {  // Dy (nx, ny-1)
     long long c = (long long)i * (ny - 1) + j;
     T dyv = p.Dy[c];
+    T curl = T(0);
     if (i >= 1 && i < nx - 1) {
       T d = (p.Hz[(long long)i * (ny - 1) + j] - p.Hz[(long long)(i - 1) * (ny - 1) + j]) / p.dx;
-      T curl = d;
+      curl = d;
       int ii = i - 1;
       if (p.ex.in(ii)) {
         long long q = (long long)ii * (ny - 1) + j;
@@ -203,14 +233,18 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
         p.pHzx[q] = ps;
         curl = d + ps;
       }
-      dyv = dyv - p.dt * curl;
+      if (!p.d1y) dyv = dyv - p.dt * curl;
     }
+    if (p.d1y) dyv = p.d1y[c] * dyv - p.d2y[c] * curl;
     if (src.present && i >= src.i0 && i < src.i1 && j >= src.j0 && j < src.j1) {
       const T* w = static_cast<const T*>(src.w);
-      dyv = dyv - coef * w[(long long)(i - src.i0) * (src.j1 - src.j0) + (j - src.j0)];
+      const T wv = w[(long long)(i - src.i0) * (src.j1 - src.j0) + (j - src.j0)];
+      // stepper_te.py:131-134: under sigma_d the scale is d2y and coef the raw amplitude
+      dyv = p.d2y ? dyv - (p.d2y[c] * coef) * wv : dyv - coef * wv;
     }
     p.Dy[c] = dyv;
-    T ey = dyv / p.eps_y[c];
+    const T ey = te_e_from_d(dyv, p.eps_y[c], p.npy ? p.Ey[c] : T(0), p.npy, (long long)nx * (ny - 1), c, p.c1y,
+                             p.c2y, p.c3y, p.Pyc, p.Pyp);
     p.Ey[c] = ey;
     if (p.tape)
       p.tape[(long long)s_rel * ((long long)(nx - 1) * ny + (long long)nx * (ny - 1)) + (long long)(nx - 1) * ny + c] = ey;
@@ -239,7 +273,8 @@ __global__ __launch_bounds__(BX* BY) void te_b_kernel(TeK<T> p, mt_source2d hzs,
     p.pExy[h] = ps;
     curl = curl - ps;
   }
-  T v = p.Hz[h] - p.dt * curl;
+  // stepper_te.py:110-115: sigma_b / mu fold into hz1, hz2
+  T v = p.hz1 ? p.hz1[h] * p.Hz[h] - p.hz2[h] * curl : p.Hz[h] - p.dt * curl;
   if (in_box<T>(hzs, i, j)) v = v - hc * box_w<T>(hzs, i, j);  // stepper_te.py:120-128
   p.Hz[h] = v;
 }
@@ -338,6 +373,24 @@ int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const
   p.pHzy = static_cast<T*>(d->psi_Hzy);
   p.pHzx = static_cast<T*>(d->psi_Hzx);
   p.tape = static_cast<T*>(d->tape);
+  p.npx = d->npx;
+  p.npy = d->npy;
+  p.c1x = static_cast<const T*>(d->c1x);
+  p.c2x = static_cast<const T*>(d->c2x);
+  p.c3x = static_cast<const T*>(d->c3x);
+  p.c1y = static_cast<const T*>(d->c1y);
+  p.c2y = static_cast<const T*>(d->c2y);
+  p.c3y = static_cast<const T*>(d->c3y);
+  p.Pxc = static_cast<T*>(d->Px_cur);
+  p.Pxp = static_cast<T*>(d->Px_prev);
+  p.Pyc = static_cast<T*>(d->Py_cur);
+  p.Pyp = static_cast<T*>(d->Py_prev);
+  p.d1x = static_cast<const T*>(d->d1x);
+  p.d2x = static_cast<const T*>(d->d2x);
+  p.d1y = static_cast<const T*>(d->d1y);
+  p.d2y = static_cast<const T*>(d->d2y);
+  p.hz1 = static_cast<const T*>(d->hz1);
+  p.hz2 = static_cast<const T*>(d->hz2);
   mt_source2d sv{};
   if (src && src->present) sv = *src;
   dim3 blk(BX, BY), grd = grid2(p.ny, p.nx, blk);
@@ -585,11 +638,20 @@ extern "C" int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const doubl
   if (src && src->present && !coef && k > 0) return fail(MT_EINVAL, "source without coefficients");
   if ((rc = check_src(&p->hsrc, p->nx - 1, p->ny - 1))) return rc;
   if (p->hsrc.present && !p->hcoef && k > 0) return fail(MT_EINVAL, "magnetic source without coefficients");
+  if (p->npx < 0 || p->npy < 0) return fail(MT_EINVAL, "negative pole count");
+  if (p->npx && (!p->c1x || !p->c2x || !p->c3x || !p->Px_cur || !p->Px_prev))
+    return fail(MT_EINVAL, "Ex poles without coefficients or pole stacks");
+  if (p->npy && (!p->c1y || !p->c2y || !p->c3y || !p->Py_cur || !p->Py_prev))
+    return fail(MT_EINVAL, "Ey poles without coefficients or pole stacks");
+  if (!p->d1x != !p->d2x || !p->d1x != !p->d1y || !p->d1x != !p->d2y)
+    return fail(MT_EINVAL, "sigma_d needs d1x, d2x, d1y and d2y");
+  if (!p->hz1 != !p->hz2) return fail(MT_EINVAL, "sigma_b/mu needs hz1 and hz2");
+  const bool media = p->npx || p->npy || p->d1x || p->hz1;
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   // the tape and magnetic currents are handled by the two-pass kernels
-  if (p->variant >= 0 && !p->tape && !p->hsrc.present) {
+  if (p->variant >= 0 && !p->tape && !p->hsrc.present && !media) {
     if (p->variant >= tiled_variant_count()) return fail(MT_EINVAL, "unknown tiled variant");
     if (!alt_ok(p->alt, 7)) return fail(MT_EINVAL, "tiled path needs the alt buffers");
     if (taps_ok(mp))

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -59,7 +59,11 @@ class TE2D(ctypes.Structure):
                 ("Dy", c_void_p), ("psi_Eyx", c_void_p), ("psi_Exy", c_void_p),
                 ("psi_Hzy", c_void_p), ("psi_Hzx", c_void_p),
                 ("variant", c_int), ("pad2_", c_int), ("alt", c_void_p * 8), ("tape", c_void_p),
-                ("hsrc", Source2D), ("hcoef", ctypes.POINTER(c_double))]
+                ("hsrc", Source2D), ("hcoef", ctypes.POINTER(c_double)),
+                ("npx", c_int), ("npy", c_int), ("c1x", c_void_p), ("c2x", c_void_p), ("c3x", c_void_p),
+                ("c1y", c_void_p), ("c2y", c_void_p), ("c3y", c_void_p), ("Px_cur", c_void_p),
+                ("Px_prev", c_void_p), ("Py_cur", c_void_p), ("Py_prev", c_void_p), ("d1x", c_void_p),
+                ("d2x", c_void_p), ("d1y", c_void_p), ("d2y", c_void_p), ("hz1", c_void_p), ("hz2", c_void_p)]
 
 
 class TE2DAdj(ctypes.Structure):

@@ -218,6 +218,28 @@ def build_tm_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
         shells=shells)
 
 
+def _axis_poles(grid, dt, eps_ref, shared, axis_poles, shared_masks, axis_masks, shape, avg):
+    """coeffs.py:109-131: (c1, c2, c3) of one TE E axis; the shared poles
+    come first, then this axis' own; each pole's cell mask (ones when not
+    given) is Yee-averaged onto the axis by ``avg``."""
+    allp = list(shared) + list(axis_poles)
+    c1 = grid.zeros(len(allp), *shape)
+    c2 = grid.zeros(len(allp), *shape)
+    c3 = grid.zeros(len(allp), *shape)
+    ns = len(shared)
+    for k, pole in enumerate(allp):
+        v1, v2, v3 = pole_recurrence(pole, dt)
+        if k < ns:
+            m = shared_masks[k] if shared_masks is not None else None
+        else:
+            m = axis_masks[k - ns] if axis_masks is not None else None
+        mask = (m if m is not None else torch.ones_like(eps_ref)).to(eps_ref)
+        c1[k] = v1
+        c2[k] = v2
+        c3[k] = v3 * avg(mask)
+    return c1, c2, c3
+
+
 def _avg_to_ex(v):
     return 0.5 * (v[:-1, :] + v[1:, :])
 
@@ -231,20 +253,41 @@ def build_te_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
                     eps_tensor=None, poles_x: Sequence = (), poles_y: Sequence = (),
                     pole_masks_x=None, pole_masks_y=None, poles_tensor: Sequence = (),
                     pole_masks_tensor=None, y_mirror: int = 0, x_mirror: int = 0) -> TECoeffs:
-    """coeffs.py:357-503, eps + CPML subset.  eps_x / eps_y are the two-point
-    averages of the cell eps (coeffs.py:347-355,385)."""
+    """coeffs.py:357-503: eps + CPML, electric poles shared by both axes and
+    per-axis (poles_x / poles_y, coeffs.py:109-131,420-427), sigma_d /
+    sigma_b / mu (coeffs.py:399-410).  eps_x / eps_y are the two-point
+    averages of the cell eps (coeffs.py:347-355,385).  chi2 / chi3 (which
+    couple Ex and Ey through neighbour averages in the fixed point), tensor
+    eps, oriented poles and symmetry folds raise."""
     dt, dx = grid.dt, grid.dx
     eps = _as_field(eps_inf, grid, 1.0)
     x3 = _as_field(chi3, grid, 0.0)
     x2 = _as_field(chi2, grid, 0.0)
     if bool((x3 != 0).any()):
-        _unsupported("chi3 (Kerr)")
+        _unsupported("chi3 (Kerr) in TE")
     if bool((x2 != 0).any()):
-        _unsupported("chi2")
-    if len(poles) or len(poles_x) or len(poles_y) or len(poles_tensor):
-        _unsupported("poles (ADE dispersion)")
+        _unsupported("chi2 in TE")
+    if len(poles_tensor):
+        _unsupported("poles_tensor (oriented poles)")
+    cond = {}
     if sigma_d is not None or sigma_b is not None or mu is not None:
-        _unsupported("sigma_d/sigma_b/mu")
+        sd = _as_field(sigma_d, grid, 0.0)
+        sb = _as_field(sigma_b, grid, 0.0)
+        mu_ = _as_field(mu, grid, 1.0)
+        sdx, sdy = _avg_to_ex(sd), _avg_to_ey(sd)
+        cond["d1x"] = (1 - sdx * dt / 2) / (1 + sdx * dt / 2)
+        cond["d2x"] = dt / (1 + sdx * dt / 2)
+        cond["d1y"] = (1 - sdy * dt / 2) / (1 + sdy * dt / 2)
+        cond["d2y"] = dt / (1 + sdy * dt / 2)
+        # Hz (nx-1, ny-1): four-point average of the cell centres
+        sbz = 0.25 * (sb[:-1, :-1] + sb[1:, :-1] + sb[:-1, 1:] + sb[1:, 1:])
+        muz = 0.25 * (mu_[:-1, :-1] + mu_[1:, :-1] + mu_[:-1, 1:] + mu_[1:, 1:])
+        cond["hz1"] = (1 - sbz * dt / 2) / (1 + sbz * dt / 2)
+        cond["hz2"] = dt / (muz * (1 + sbz * dt / 2))
+    c1x, c2x, c3x = _axis_poles(grid, dt, eps, poles, poles_x, pole_masks, pole_masks_x,
+                                (grid.nx - 1, grid.ny), _avg_to_ex)
+    c1y, c2y, c3y = _axis_poles(grid, dt, eps, poles, poles_y, pole_masks, pole_masks_y,
+                                (grid.nx, grid.ny - 1), _avg_to_ey)
     if eps_tensor is not None:
         _unsupported("eps_tensor (anisotropic eps)")
     if y_mirror or x_mirror:
@@ -262,12 +305,10 @@ def build_te_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
     shells = {"hx": shell_bounds(b_hx, a_hx), "hy": shell_bounds(b_hy, a_hy),
               "ey": shell_bounds(b_ey, a_ey), "ex": shell_bounds(b_ex, a_ex)}
     dev = lambda t: t.contiguous().to(grid.device)
-    zx = grid.zeros(0, grid.nx - 1, grid.ny)
-    zy = grid.zeros(0, grid.nx, grid.ny - 1)
     return TECoeffs(
         dt=dt, dx=dx, eps_x=eps_x, eps_y=eps_y, chi3_x=_avg_to_ex(x3), chi3_y=_avg_to_ey(x3),
         has_kerr=False, chi2_x=_avg_to_ex(x2), chi2_y=_avg_to_ey(x2), has_chi2=False,
-        c1x=zx, c2x=zx.clone(), c3x=zx.clone(), c1y=zy, c2y=zy.clone(), c3y=zy.clone(),
+        c1x=c1x, c2x=c2x, c3x=c3x, c1y=c1y, c2y=c2y, c3y=c3y, **cond,
         b_hx=dev(b_hx)[:, None], a_hx=dev(a_hx)[:, None], k_hx=dev(k_hx)[:, None],
         b_hy=dev(b_hy)[None, :], a_hy=dev(a_hy)[None, :], k_hy=dev(k_hy)[None, :],
         b_ey=dev(b_ey)[None, :], a_ey=dev(a_ey)[None, :], k_ey=dev(k_ey)[None, :],

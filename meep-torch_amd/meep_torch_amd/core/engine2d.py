@@ -211,6 +211,38 @@ def _tm_media(d, c, fields, dtype, device):
     return keep
 
 
+def _te_media(d, c, fields, dtype, device):
+    """Fill the media fields of an mt_te2d from TECoeffs (per-axis electric
+    poles with the state's stacks, sigma_d, sigma_b / mu)."""
+    keep = []
+
+    def put(name, t):
+        t = t.to(device=device, dtype=dtype).contiguous()
+        keep.append(t)
+        setattr(d, name, t.data_ptr())
+
+    if c.has_kerr or c.has_chi2 or c.inv_xx_x is not None or c.c3xy is not None:
+        raise NotImplementedError("TE chi2/chi3, tensor eps and oriented poles are not supported")
+    for ax, n in (("x", int(c.c1x.shape[0])), ("y", int(c.c1y.shape[0]))):
+        if not n:
+            continue
+        cur, prev = f"P{ax}_cur", f"P{ax}_prev"
+        if cur not in fields or int(fields[cur].shape[0]) != n:
+            raise ValueError(f"{n} E{ax} poles need {cur} / {prev} stacks of {n} in the state")
+        setattr(d, f"np{ax}", n)
+        for name in (f"c1{ax}", f"c2{ax}", f"c3{ax}"):
+            put(name, getattr(c, name))
+        setattr(d, cur, fields[cur].data_ptr())
+        setattr(d, prev, fields[prev].data_ptr())
+    if c.d1x is not None:
+        for name in ("d1x", "d2x", "d1y", "d2y"):
+            put(name, getattr(c, name))
+    if c.hz1 is not None:
+        put("hz1", c.hz1)
+        put("hz2", c.hz2)
+    return keep
+
+
 TM_ORDER = ("Ez", "Dz", "Hx", "Hy", "psi_Ezy", "psi_Ezx", "psi_Hyx", "psi_Hxy")
 TE_ORDER = ("Hz", "Dx", "Dy", "psi_Eyx", "psi_Exy", "psi_Hzy", "psi_Hzx")
 
@@ -257,11 +289,11 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
             assert tape.is_contiguous() and tape.shape == (k, slot) and tape.dtype == dtype
             d.tape = tape.data_ptr()
             variant = -1  # the tape is written by the two-pass path only
-        keep_c = [ex_, ey_]
+        keep_c = [ex_, ey_] + _te_media(d, coeffs, fields, dtype, device)
     for f, t in fields.items():
         if not t.is_contiguous():
             raise ValueError(f"state field {f} must be contiguous")
-    media = mode == "tm" and bool(d.n_poles or d.kerr or d.d1 or d.h1x)
+    media = bool(d.n_poles or d.kerr or d.d1 or d.h1x) if mode == "tm" else bool(d.npx or d.npy or d.d1x or d.hz1)
     if (plan is not None and plan.get("h")) or media:
         variant = -1  # magnetic currents and media are handled by the two-pass kernels
     if variant is None:
@@ -275,7 +307,8 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
         for q, f in enumerate(order):
             d.alt[q] = alt[f].data_ptr()
     src, wkeep = source_struct(plan)
-    coef, hcoef = coef_arrays(plan, coeffs.dt, step0, k, raw=mode == "tm" and coeffs.d1 is not None)
+    raw_amp = (coeffs.d1 is not None) if mode == "tm" else (coeffs.d1x is not None)
+    coef, hcoef = coef_arrays(plan, coeffs.dt, step0, k, raw=raw_amp)
     hkeep = []
     if plan is not None and plan.get("h"):
         for slot, hb in plan["h"]:

@@ -12,7 +12,7 @@ import torch
 
 from . import engine2d
 from .stepper_te import FIELDS as TE_FIELDS
-from .stepper_te import TEState, step_te
+from .stepper_te import TEState, _check as _check_te, fields_of as te_fields_of, step_te
 from .stepper_tm import FIELDS as TM_FIELDS
 from .stepper_tm import TMState, _check as _check_tm, fields_of as tm_fields_of, step_tm
 
@@ -45,7 +45,7 @@ def _mode_of(step_fn, state):
 def _fields(mode, state=None):
     if mode == "tm":
         return tm_fields_of(state) if state is not None else TM_FIELDS
-    return TE_FIELDS
+    return te_fields_of(state) if state is not None else TE_FIELDS
 
 
 def _hshapes(mode, work):
@@ -60,8 +60,8 @@ def _target_shape(mode, work):
 def _validate(mode, state, coeffs):
     if mode == "tm":
         _check_tm(state, coeffs)
-    elif state.Px_cur.shape[0] or state.Py_cur.shape[0]:
-        raise NotImplementedError("TE state with poles")
+    else:
+        _check_te(state, coeffs)
 
 
 def _absorb(acc, monitors, outs):

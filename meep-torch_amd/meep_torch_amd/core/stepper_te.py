@@ -27,21 +27,30 @@ FIELDS = ("Hz", "Ex", "Ey", "Dx", "Dy", "psi_Eyx", "psi_Exy", "psi_Hzy", "psi_Hz
 
 def init_te_state(grid: Grid, n_poles: int = 0, n_poles_y: int | None = None) -> TEState:
     """stepper_te.py:27-49."""
-    if n_poles or n_poles_y:
-        raise NotImplementedError("init_te_state: poles are not supported by the HIP stepper")
+    npx = int(n_poles)
+    npy = npx if n_poles_y is None else int(n_poles_y)
     nx, ny = grid.nx, grid.ny
     return TEState(
         Hz=grid.zeros(nx - 1, ny - 1), Ex=grid.zeros(nx - 1, ny), Ey=grid.zeros(nx, ny - 1),
         Dx=grid.zeros(nx - 1, ny), Dy=grid.zeros(nx, ny - 1),
-        Px_cur=grid.zeros(0, nx - 1, ny), Px_prev=grid.zeros(0, nx - 1, ny),
-        Py_cur=grid.zeros(0, nx, ny - 1), Py_prev=grid.zeros(0, nx, ny - 1),
+        Px_cur=grid.zeros(npx, nx - 1, ny), Px_prev=grid.zeros(npx, nx - 1, ny),
+        Py_cur=grid.zeros(npy, nx, ny - 1), Py_prev=grid.zeros(npy, nx, ny - 1),
         psi_Eyx=grid.zeros(nx - 1, ny - 1), psi_Exy=grid.zeros(nx - 1, ny - 1),
         psi_Hzy=grid.zeros(nx - 1, ny - 2), psi_Hzx=grid.zeros(nx - 2, ny - 1))
 
 
-def _check(state: TEState):
-    if state.Px_cur.shape[0] or state.Py_cur.shape[0]:
-        raise NotImplementedError("TE state with poles")
+def fields_of(state: TEState):
+    """State fields the engine advances: FIELDS plus the pole stacks the
+    state carries."""
+    extra = (("Px_cur", "Px_prev") if state.Px_cur.shape[0] else ()) + \
+        (("Py_cur", "Py_prev") if state.Py_cur.shape[0] else ())
+    return FIELDS + extra
+
+
+def _check(state: TEState, m=None):
+    if m is not None and (state.Px_cur.shape[0] != m.c1x.shape[0] or state.Py_cur.shape[0] != m.c1y.shape[0]):
+        raise ValueError(f"state has {state.Px_cur.shape[0]} / {state.Py_cur.shape[0]} pole stacks, "
+                         f"coefficients {m.c1x.shape[0]} / {m.c1y.shape[0]} poles")
 
 
 def step_te(state: TEState, m, src_amp=0.0, src_mask_ey: torch.Tensor | None = None,
@@ -49,8 +58,8 @@ def step_te(state: TEState, m, src_amp=0.0, src_mask_ey: torch.Tensor | None = N
     """One TE step entered at the D half (stepper_te.py:178-190) on the engine."""
     from .run import _Fixed
     from . import engine2d
-    _check(state)
-    work = {f: getattr(state, f).clone() for f in FIELDS}
+    _check(state, m)
+    work = {f: getattr(state, f).clone() for f in fields_of(state)}
     plan = None
     if src_mask_ey is not None or src_hz_mask is not None:
         nx, ny = work["Ey"].shape[0], work["Ex"].shape[1]

@@ -28,7 +28,7 @@ from ..core.coeffs import build_te_coeffs, build_tm_coeffs
 from ..core.grid import Grid
 from ..core.monitors import DFTFieldsMonitor, FluxMonitorTE, FluxMonitorTEY, FluxMonitorTM
 from ..core.sources import LineSource
-from ..core.stepper_te import FIELDS as TE_FIELDS, init_te_state
+from ..core.stepper_te import fields_of as te_fields_of, init_te_state
 from ..core.stepper_tm import fields_of as tm_fields_of, init_tm_state
 from . import geometry as G
 from . import run_control as RC
@@ -146,12 +146,19 @@ class Simulation(Sim3D):
                 sd, sb, mu = (G.to_tensor(v, g.dtype, g.device) for v in media["cond"])
                 kw.update(sigma_d=sd, sigma_b=sb, mu=mu)
             self._coeffs = build_tm_coeffs(g, eps_inf=eps, npml=npml, **kw)
-        elif has_media:
-            raise NotImplementedError("dispersive, nonlinear or conductive media in TE runs are not supported")
         else:
+            if media["chi3"] is not None:
+                raise NotImplementedError("chi3 in TE runs is not supported")
+            kw = {}
+            if media["poles"]:
+                kw.update(poles=media["poles"],
+                          pole_masks=[G.to_tensor(m, torch.float64, g.device) for m in media["masks"]])
+            if media["cond"] is not None:
+                sd, sb, mu = (G.to_tensor(v, torch.float64, g.device) for v in media["cond"])
+                kw.update(sigma_d=sd, sigma_b=sb, mu=mu)
             eps_xx = G.to_tensor(self._raster[1], torch.float64, g.device)
             eps_yy = G.to_tensor(self._raster[2], torch.float64, g.device)
-            c = build_te_coeffs(g, eps_inf=eps_xx, npml=npml)
+            c = build_te_coeffs(g, eps_inf=eps_xx, npml=npml, **kw)
             self._coeffs = c._replace(eps_y=(0.5 * (eps_yy[:, :-1] + eps_yy[:, 1:])).to(g.dtype))
 
     def init_sim(self):
@@ -269,8 +276,12 @@ class Simulation(Sim3D):
         elif n_total is None:
             T = float(until) if until is not None else cutoff + float(until_after_sources or 0.0)
             n_total = max(1, int(round(T / dt)))
-        st = init_tm_state(self.grid, n_poles=self._n_poles) if mode == "tm" else init_te_state(self.grid)
-        names = tm_fields_of(st) if mode == "tm" else TE_FIELDS
+        if mode == "tm":
+            st = init_tm_state(self.grid, n_poles=self._n_poles)
+            names = tm_fields_of(st)
+        else:
+            st = init_te_state(self.grid, n_poles=self._n_poles)
+            names = te_fields_of(st)
         work = {f: getattr(st, f).contiguous() for f in names}
         srcs = self._make_sources()
         src = srcs[0] if srcs else None

@@ -87,6 +87,19 @@ class Coef2D:
     h2x: torch.Tensor = None
     h1y: torch.Tensor = None
     h2y: torch.Tensor = None
+    # TE media: per-axis poles (np, nx-1, ny) / (np, nx, ny-1), sigma factors
+    c1x: torch.Tensor = None
+    c2x: torch.Tensor = None
+    c3x: torch.Tensor = None
+    c1y: torch.Tensor = None
+    c2y: torch.Tensor = None
+    c3y: torch.Tensor = None
+    d1x: torch.Tensor = None
+    d2x: torch.Tensor = None
+    d1y: torch.Tensor = None
+    d2y: torch.Tensor = None
+    hz1: torch.Tensor = None
+    hz2: torch.Tensor = None
 
 
 def pole_recurrence(pole, dt):
@@ -162,9 +175,29 @@ def tm_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, y_pmc=Fal
     return _tm_media(c, nx, ny, dtype, poles, pole_masks, chi2, chi3, sigma_d, sigma_b, mu)
 
 
-def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32):
-    """meep_torch/core/coeffs.py:357-503 restricted to eps + CPML; eps_x/eps_y
-    are the two-point averages of coeffs.py:347-355."""
+def _te_axis_poles(nx, ny, dt, dtype, shared, own, shared_masks, own_masks, shape, avg):
+    """meep_torch/core/coeffs.py:109-131: shared poles first, then the
+    axis' own; each cell mask (ones if absent) Yee-averaged by ``avg``."""
+    allp = list(shared) + list(own)
+    c1, c2, c3 = (torch.zeros(len(allp), *shape, dtype=dtype) for _ in range(3))
+    for k, pole in enumerate(allp):
+        v1, v2, v3 = pole_recurrence(pole, dt)
+        if k < len(shared):
+            m = shared_masks[k] if shared_masks is not None else None
+        else:
+            m = own_masks[k - len(shared)] if own_masks is not None else None
+        mask = (m if m is not None else torch.ones(nx, ny, dtype=dtype)).to(dtype)
+        c1[k] = v1
+        c2[k] = v2
+        c3[k] = v3 * avg(mask)
+    return c1, c2, c3
+
+
+def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, poles=(), pole_masks=None, poles_x=(),
+              poles_y=(), pole_masks_x=None, pole_masks_y=None, sigma_d=None, sigma_b=None, mu=None):
+    """meep_torch/core/coeffs.py:357-503 restricted to eps + CPML, shared and
+    per-axis electric poles and sigma_d / sigma_b / mu; eps_x/eps_y are the
+    two-point averages of coeffs.py:347-355."""
     dt = courant * dx
     eps = eps.to(dtype)
     eps_x = 0.5 * (eps[:-1, :] + eps[1:, :])
@@ -181,11 +214,30 @@ def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32):
     b_hy, a_hy, _ = prof(ny - 1, 0.5, py)
     b_eyf, a_eyf, _ = prof(ny, 0.0, py)
     b_exf, a_exf, _ = prof(nx, 0.0, px)
-    return Coef2D(dt=dt, dx=dx, eps=eps, eps_x=eps_x, eps_y=eps_y,
-                  b_hx=b_hx[:, None], a_hx=a_hx[:, None],
-                  b_hy=b_hy[None, :], a_hy=a_hy[None, :],
-                  b_ey=b_eyf[1:-1][None, :], a_ey=a_eyf[1:-1][None, :],
-                  b_ex=b_exf[1:-1][:, None], a_ex=a_exf[1:-1][:, None])
+    c = Coef2D(dt=dt, dx=dx, eps=eps, eps_x=eps_x, eps_y=eps_y,
+               b_hx=b_hx[:, None], a_hx=a_hx[:, None],
+               b_hy=b_hy[None, :], a_hy=a_hy[None, :],
+               b_ey=b_eyf[1:-1][None, :], a_ey=a_eyf[1:-1][None, :],
+               b_ex=b_exf[1:-1][:, None], a_ex=a_exf[1:-1][:, None])
+    ax = lambda v: 0.5 * (v[:-1, :] + v[1:, :])
+    ay = lambda v: 0.5 * (v[:, :-1] + v[:, 1:])
+    c.c1x, c.c2x, c.c3x = _te_axis_poles(nx, ny, dt, dtype, poles, poles_x, pole_masks, pole_masks_x,
+                                         (nx - 1, ny), ax)
+    c.c1y, c.c2y, c.c3y = _te_axis_poles(nx, ny, dt, dtype, poles, poles_y, pole_masks, pole_masks_y,
+                                         (nx, ny - 1), ay)
+    if sigma_d is not None or sigma_b is not None or mu is not None:
+        # coeffs.py:399-410
+        full = lambda v, d: (torch.full((nx, ny), float(d if v is None else v), dtype=dtype)
+                             if not torch.is_tensor(v) else v.to(dtype))
+        sd, sb, mu_ = full(sigma_d, 0.0), full(sigma_b, 0.0), full(mu, 1.0)
+        sdx, sdy = ax(sd), ay(sd)
+        c.d1x, c.d2x = (1 - sdx * dt / 2) / (1 + sdx * dt / 2), dt / (1 + sdx * dt / 2)
+        c.d1y, c.d2y = (1 - sdy * dt / 2) / (1 + sdy * dt / 2), dt / (1 + sdy * dt / 2)
+        sbz = 0.25 * (sb[:-1, :-1] + sb[1:, :-1] + sb[:-1, 1:] + sb[1:, 1:])
+        muz = 0.25 * (mu_[:-1, :-1] + mu_[1:, :-1] + mu_[:-1, 1:] + mu_[1:, 1:])
+        c.hz1 = (1 - sbz * dt / 2) / (1 + sbz * dt / 2)
+        c.hz2 = dt / (muz * (1 + sbz * dt / 2))
+    return c
 
 
 # --------------------------------------------------------------------- state
@@ -202,13 +254,19 @@ def tm_state(nx, ny, dtype=torch.float32, y_pmc=False, n_poles=0):
     return out
 
 
-def te_state(nx, ny, dtype=torch.float32):
-    """Field dict with the TEState shapes of stepper_te.py:11-49."""
+def te_state(nx, ny, dtype=torch.float32, npx=0, npy=0):
+    """Field dict with the TEState shapes of stepper_te.py:11-49 (pole stacks
+    only when present)."""
     z = lambda *s: torch.zeros(*s, dtype=dtype)
-    return {"Hz": z(nx - 1, ny - 1), "Ex": z(nx - 1, ny), "Ey": z(nx, ny - 1),
-            "Dx": z(nx - 1, ny), "Dy": z(nx, ny - 1),
-            "psi_Eyx": z(nx - 1, ny - 1), "psi_Exy": z(nx - 1, ny - 1),
-            "psi_Hzy": z(nx - 1, ny - 2), "psi_Hzx": z(nx - 2, ny - 1)}
+    out = {"Hz": z(nx - 1, ny - 1), "Ex": z(nx - 1, ny), "Ey": z(nx, ny - 1),
+           "Dx": z(nx - 1, ny), "Dy": z(nx, ny - 1),
+           "psi_Eyx": z(nx - 1, ny - 1), "psi_Exy": z(nx - 1, ny - 1),
+           "psi_Hzy": z(nx - 1, ny - 2), "psi_Hzx": z(nx - 2, ny - 1)}
+    if npx:
+        out.update(Px_cur=z(npx, nx - 1, ny), Px_prev=z(npx, nx - 1, ny))
+    if npy:
+        out.update(Py_cur=z(npy, nx, ny - 1), Py_prev=z(npy, nx, ny - 1))
+    return out
 
 
 # ---------------------------------------------------------------------- steps
@@ -287,20 +345,37 @@ def te_step(s, c, coef=0.0, mask=None, hcoef=0.0, hz_mask=None):
     gx = (Hz[1:, :] - Hz[:-1, :]) / dx
     phx = c.b_ex * s["psi_Hzx"] + c.a_ex * gx
     cy = torch.nn.functional.pad(gx + phx, (0, 0, 1, 1))
-    Dx = s["Dx"] + dt * cx
-    Dy = s["Dy"] - dt * cy
+    if c.d1x is not None:       # stepper_te.py:96-99 (sigma_d)
+        Dx = c.d1x * s["Dx"] + c.d2x * cx
+        Dy = c.d1y * s["Dy"] - c.d2y * cy
+    else:
+        Dx = s["Dx"] + dt * cx
+        Dy = s["Dy"] - dt * cy
     if mask is not None:
-        Dy = Dy - coef * mask
-    Ex = Dx / c.eps_x
-    Ey = Dy / c.eps_y
+        # stepper_te.py:131-134: scale d2y under sigma_d (coef is then the raw amplitude)
+        Dy = Dy - (c.d2y * coef * mask if c.d1x is not None else coef * mask)
+    out = dict(s)
+    rx, ry = Dx, Dy
+    if c.c1x is not None and c.c1x.shape[0]:   # stepper_te.py:137-147
+        Pxn = c.c1x * s["Px_cur"] + c.c2x * s["Px_prev"] + c.c3x * s["Ex"]
+        out.update(Px_cur=Pxn, Px_prev=s["Px_cur"])
+        rx = Dx - Pxn.sum(0)
+    if c.c1y is not None and c.c1y.shape[0]:
+        Pyn = c.c1y * s["Py_cur"] + c.c2y * s["Py_prev"] + c.c3y * s["Ey"]
+        out.update(Py_cur=Pyn, Py_prev=s["Py_cur"])
+        ry = Dy - Pyn.sum(0)
+    Ex = rx / c.eps_x
+    Ey = ry / c.eps_y
     dEy = (Ey[1:, :] - Ey[:-1, :]) / dx
     dEx = (Ex[:, 1:] - Ex[:, :-1]) / dx
     pyx = c.b_hx * s["psi_Eyx"] + c.a_hx * dEy
     pxy = c.b_hy * s["psi_Exy"] + c.a_hy * dEx
-    Hz2 = Hz - dt * (dEy + pyx - dEx - pxy)
+    if c.hz1 is not None:       # stepper_te.py:110-115 (sigma_b, mu)
+        Hz2 = c.hz1 * Hz - c.hz2 * (dEy + pyx - dEx - pxy)
+    else:
+        Hz2 = Hz - dt * (dEy + pyx - dEx - pxy)
     if hz_mask is not None:
         Hz2 = Hz2 - hcoef * hz_mask
-    out = dict(s)
     out.update(Dx=Dx, Dy=Dy, Ex=Ex, Ey=Ey, Hz=Hz2, psi_Hzy=phy, psi_Hzx=phx,
                psi_Eyx=pyx, psi_Exy=pxy)
     return out
@@ -407,7 +482,8 @@ def run2d(mode, state, c, n_steps, amps=None, mask=None, monitors=(), checkpoint
             hk = {} if hamps is None else dict(hmasks, hcoef=c.dt * hamps[t])
             if amps is not None:
                 # under sigma_d the source is scaled by d2 instead of dt (stepper_tm.py:198-200)
-                a = amps[t] if getattr(c, "d1", None) is not None else c.dt * amps[t]
+                raw = getattr(c, "d1", None) is not None or getattr(c, "d1x", None) is not None
+                a = amps[t] if raw else c.dt * amps[t]
                 s = step(s, c, a, mask, **hk)
             else:
                 s = step(s, c, **hk)

@@ -197,3 +197,123 @@ def test_media_refuse_gradients_and_batches():
     m2 = build_tm_coeffs(g, eps_inf=2.0, npml=4, poles=[(1.0, 0.1, 1.0)])
     with pytest.raises(NotImplementedError):
         BatchedTM([(m2, None, [])], 10)
+
+
+def _te_media(dtype):
+    from meep_torch_amd import synth
+    r = torch.tensor(synth.splitmix_uniform(NX * NY, 37).reshape(NX, NY), dtype=torch.float64)
+    blk = torch.zeros(NX, NY, dtype=torch.float64)
+    blk[50:75, 16:48] = 1.0
+    eps = 1.0 + 1.5 * blk * r
+    kw = dict(poles=[(1.2, 0.1, 1.5)], pole_masks=[blk.to(dtype)],
+              poles_x=[(0.0, 0.3, 0.5, 0.9)], pole_masks_x=[(blk * (r > 0.5)).to(dtype)],      # Drude on Ex only
+              sigma_d=(0.3 * blk).to(dtype), sigma_b=(0.1 * blk).to(dtype), mu=(1.0 + 0.5 * blk).to(dtype))
+    return eps.to(dtype), kw
+
+
+@pytest.mark.parametrize("dtype,tf,ts", [(torch.float64, 1e-12, 1e-12), (torch.float32, 1e-5, 1e-4)],
+                         ids=["f64", "f32"])
+def test_te_media_run_matches_oracle(dtype, tf, ts):
+    """TE: a pole shared by Ex and Ey, a Drude pole on Ex only, sigma_d,
+    sigma_b and mu, against the oracle's TE step."""
+    from meep_torch_amd.core.coeffs import build_te_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTE
+    from meep_torch_amd.core.run import run_te
+    from meep_torch_amd.core.sources import GaussianPulse, LineSource
+    from meep_torch_amd.core.stepper_te import init_te_state
+    from oracle import yee2d
+    eps, kw = _te_media(dtype)
+    g = Grid(nx=NX, ny=NY, dx=DX, dtype=dtype, device="cuda")
+    dev = {k: ([x.cuda() for x in v] if k.startswith("pole_masks") else v.cuda() if torch.is_tensor(v) else v)
+           for k, v in kw.items()}
+    m = build_te_coeffs(g, eps_inf=eps.cuda(), npml=NPML, **dev)
+    src = LineSource(g, ix=20, j0=16, j1=48, profile=GaussianPulse(0.8, 0.5))
+    src.mask = src.mask[:, :-1].contiguous()
+    mon = FluxMonitorTE(g, FREQS, ix=80, lo=12, hi=52)
+    st, (acc,) = run_te(init_te_state(g, n_poles=2, n_poles_y=1), m, 400, source=src, monitors=[mon],
+                        checkpoint_every=150)
+    torch.set_num_threads(8)
+    c = yee2d.te_coeffs(NX, NY, DX, eps, NPML, dtype=dtype, **kw)
+    s, (oacc,) = yee2d.run2d("te", yee2d.te_state(NX, NY, dtype, npx=2, npy=1), c, 400,
+                             amps=[src.amplitude(t) for t in range(400)], mask=src.mask.cpu(),
+                             monitors=[{"type": "flux", "kind": "te_x", "idx": 80, "lo": 12, "hi": 52,
+                                        "freqs": FREQS}], checkpoint_every=150)
+    names = ("Hz", "Ex", "Ey", "Dx", "Dy", "Px_cur", "Px_prev", "Py_cur", "Py_prev")
+    errs = {f: rel(getattr(st, f), s[f]) for f in names}
+    errs["acc"] = rel(acc, oacc)
+    print(f"\n[te media {dtype}] " + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
+    assert float(s["Px_cur"][1].abs().max()) > 0
+    for f in names:
+        assert errs[f] <= tf, (f, errs[f])
+    assert errs["acc"] <= ts
+
+
+def test_te_lorentz_slab_on_hip():
+    """Parallel-plate TE (PEC y walls, PML along x): the Lorentz slab within
+    3% of the transfer matrix through run_te."""
+    from meep_torch_amd.core.coeffs import build_te_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTE
+    from meep_torch_amd.core.run import run_te
+    from meep_torch_amd.core.sources import GaussianPulse, LineSource
+    from meep_torch_amd.core.stepper_te import init_te_state
+    pole, eps_inf = (2.0, 0.1, 1.5), 2.25
+    freqs = [0.50, 0.60, 0.70, 0.80]
+
+    def flux(on):
+        g = Grid(nx=1400, ny=8, dx=0.02, dtype=torch.float64, device="cuda")
+        slab = torch.zeros(1400, 8, dtype=torch.float64, device="cuda")
+        slab[600:700, :] = 1.0
+        kw = dict(eps_inf=1.0 + (eps_inf - 1.0) * slab, poles=[pole], pole_masks=[slab]) if on else {}
+        m = build_te_coeffs(g, npml=((40, 40), (0, 0)), **kw)
+        src = LineSource(g, ix=100, j0=0, j1=8, profile=GaussianPulse(0.65, 0.35))
+        src.mask = src.mask[:, :-1].contiguous()
+        mon = FluxMonitorTE(g, freqs, ix=1200, lo=0, hi=7)
+        _, (acc,) = run_te(init_te_state(g, n_poles=1 if on else 0), m, 9000, source=src, monitors=[mon])
+        return mon.flux(acc, g.dx).cpu()
+
+    T = flux(True) / flux(False)
+    for i, f in enumerate(freqs):
+        fn, gm, s = pole
+        n2 = cmath.sqrt(eps_inf + s * fn * fn / (fn * fn - f * f - 1j * f * gm))
+        delta = 2 * cmath.pi * f * n2 * 2.0
+        r12 = (1 - n2) / (1 + n2)
+        t = (2 / (1 + n2)) * (2 * n2 / (1 + n2)) * cmath.exp(1j * delta) / (1 - r12 ** 2 * cmath.exp(2j * delta))
+        assert abs(float(T[i]) - abs(t) ** 2) / abs(t) ** 2 < 0.03, (f, float(T[i]), abs(t) ** 2)
+
+
+def test_te_facade_drude_block_equals_direct_run():
+    """A TE Simulation cell with a cell-aligned Drude + D-conductivity block
+    against run_te on hand-built maps (FP64): identical fields and flux."""
+    import meep_torch_amd.mp as mp
+    from meep_torch_amd.core.coeffs import build_te_coeffs
+    from meep_torch_amd.core.monitors import FluxMonitorTE
+    from meep_torch_amd.core.run import run_te
+    from meep_torch_amd.core.stepper_te import init_te_state
+    D = mp.DrudeSusceptibility(frequency=0.9, gamma=0.3, sigma=0.5)
+    sim = mp.Simulation(
+        cell_size=mp.Vector3(10, 6), resolution=10,
+        geometry=[mp.Block(center=mp.Vector3(1.5, 0), size=mp.Vector3(1, 2),
+                           material=mp.Medium(epsilon=2, E_susceptibilities=[D], D_conductivity=0.2))],
+        sources=[mp.Source(mp.GaussianSource(0.8, fwidth=0.5), component=mp.Ey,
+                           center=mp.Vector3(-4, 0), size=mp.Vector3(0, 2))],
+        boundary_layers=[mp.PML(1.0)], dtype=torch.float64, device="cuda")
+    fl = sim.add_flux(0.8, 0.4, 3, mp.FluxRegion(center=mp.Vector3(4, 0), size=mp.Vector3(0, 4)))
+    sim.run(until=20)
+    g = sim.grid
+    z = lambda v=0.0: torch.full((100, 60), v, dtype=torch.float64, device="cuda")
+    eps, mD, sd = z(1.0), z(), z()
+    eps[60:70, 20:40] = 2.0
+    mD[60:70, 20:40] = 1.0
+    sd[60:70, 20:40] = 0.2
+    m = build_te_coeffs(g, eps_inf=eps, npml=((10, 10), (10, 10)), poles=[D.to_pole()], pole_masks=[mD],
+                        sigma_d=sd, sigma_b=z(), mu=z(1.0))
+    src = sim._make_sources()[0]
+    freqs = np.linspace(0.8 - 0.4 / 2.0, 0.8 + 0.4 / 2.0, 3).tolist()
+    kind, mon, meta = sim._monitors[fl.name]
+    mon2 = FluxMonitorTE(g, freqs, ix=meta["idx"], lo=meta["lo"], hi=meta["hi"])
+    st, (acc,) = run_te(init_te_state(g, n_poles=1), m, sim._t_step, source=src, monitors=[mon2])
+    assert np.array_equal(sim.get_array(component=mp.Ey), st.Ey.cpu().numpy())
+    assert np.array_equal(np.asarray(mp.get_fluxes(fl)), mon2.flux(acc, g.dx).cpu().numpy())
+    assert float(st.Px_cur.abs().max()) > 0

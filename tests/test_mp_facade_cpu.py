@@ -170,10 +170,11 @@ def test_media_rasterization_compositing():
 
 
 def test_media_outside_tm_raise():
+    """chi3 in TE cells and any medium in 3D cells raise by name."""
     import meep_torch_amd.mp as mp
     med = mp.Medium(epsilon=2, D_conductivity=0.1)
     te = mp.Simulation(cell_size=mp.Vector3(2, 2), resolution=10, device="cpu",
-                       geometry=[mp.Block(size=mp.Vector3(1, 1), material=med)],
+                       geometry=[mp.Block(size=mp.Vector3(1, 1), material=mp.Medium(epsilon=2, chi3=0.1))],
                        sources=[mp.Source(mp.GaussianSource(1.0, fwidth=0.5), component=mp.Hz)])
     with pytest.raises(NotImplementedError):
         te.init_sim()

@@ -15,7 +15,10 @@ physics the reference's own analytic tests use:
   of meep_torch tests/examples/test_w5_conductivity.py);
 * chi3 (stepper_tm.py:228-231): third-harmonic generation scales as the cube
   of the drive (2x input -> 8x, within 5%) and vanishes without chi3
-  (meep_torch tests/analytic/test_kerr.py).
+  (meep_torch tests/analytic/test_kerr.py);
+* the TE step's per-axis poles and sigma_d on the same slabs, as a TE
+  parallel-plate wave (PEC y walls, where the y-uniform Ey / Hz wave is
+  exact).
 
 A dropped pole term, a wrong pole coefficient, a wrong conductivity factor or
 a missing cubic term moves these by far more than the bars."""
@@ -139,3 +142,69 @@ def test_kerr_third_harmonic_scales_as_cube():
     assert abs(fb / fa - 2.0) < 0.01
     assert abs(tb / ta - 8.0) < 0.4
     assert t0_ / f0_ < 1e-9 < ta / fa
+
+
+def _te_line(nx, ny, dx, c, src_ix, amps, mons, np_=0):
+    """TE parallel plate: PEC y walls, PML along x only, a line current on
+    every Ey row: the y-uniform Ey / Hz plane wave is exact (Ey is normal to
+    the walls), so 1D transmission formulas apply."""
+    mask = torch.zeros(nx, ny - 1, dtype=DT)
+    mask[src_ix, :] = 1.0 / dx
+    st = yee2d.te_state(nx, ny, DT, npx=np_, npy=np_)
+    return yee2d.run2d("te", st, c, len(amps), amps=amps, mask=mask, monitors=mons)
+
+
+def test_te_lorentz_slab_matches_transfer_matrix():
+    """The TE step's per-axis poles (stepper_te.py:137-147) on the Lorentz
+    slab of the TM test, as a parallel-plate TE wave: same 3% bar."""
+    nx, ny, dx = 1400, 8, 0.02
+    pole, eps_inf = (2.0, 0.1, 1.5), 2.25
+    freqs = [0.50, 0.60, 0.70, 0.80]
+
+    def flux(on):
+        slab = torch.zeros(nx, ny, dtype=DT)
+        slab[600:700, :] = 1.0
+        eps = 1.0 + (eps_inf - 1.0) * slab if on else torch.ones(nx, ny, dtype=DT)
+        poles = [pole] if on else []
+        c = yee2d.te_coeffs(nx, ny, dx, eps, ((40, 40), (0, 0)), dtype=DT, poles=poles,
+                            pole_masks=[slab] if on else None)
+        f = gaussian(0.65, 0.35)
+        amps = [f((t + 0.5) * c.dt) for t in range(9000)]
+        mons = [{"type": "flux", "kind": "te_x", "idx": 1200, "lo": 0, "hi": ny - 1, "freqs": freqs}]
+        _, acc = _te_line(nx, ny, dx, c, 100, amps, mons, np_=len(poles))
+        return yee2d.flux_from_acc("te_x", acc[0], dx)
+
+    T = flux(True) / flux(False)
+    for i, f in enumerate(freqs):
+        fn, g, s = pole
+        n2 = cmath.sqrt(eps_inf + s * fn * fn / (fn * fn - f * f - 1j * f * g))
+        delta = 2 * cmath.pi * f * n2 * 2.0
+        r12 = (1 - n2) / (1 + n2)
+        t = (2 / (1 + n2)) * (2 * n2 / (1 + n2)) * cmath.exp(1j * delta) / (1 - r12 ** 2 * cmath.exp(2j * delta))
+        assert abs(float(T[i]) - abs(t) ** 2) / abs(t) ** 2 < 0.03, (f, float(T[i]), abs(t) ** 2)
+
+
+def test_te_conductive_slab_matches_analytic():
+    """TE sigma_d (coeffs.py:399-404, stepper_te.py:96-99, 131-134) on the
+    sigma_d = 1, L = 2 slab: amplitude transmission within 2% of
+    exp(-alpha L) |4 n / (1 + n)^2| at f = 1."""
+    dx = 1.0 / 40
+    nx, ny = int(round(30 / dx)), 8
+
+    def flux(on):
+        slab = torch.zeros(nx, ny, dtype=DT)
+        slab[int(round(14 / dx)):int(round(16 / dx)), :] = 1.0
+        c = yee2d.te_coeffs(nx, ny, dx, torch.ones(nx, ny, dtype=DT), ((int(round(1.5 / dx)),) * 2, (0, 0)),
+                            dtype=DT, sigma_d=slab if on else None)
+        f = gaussian(1.0, 0.2)
+        amps = [f((t + 0.5) * c.dt) for t in range(10000)]
+        mons = [{"type": "flux", "kind": "te_x", "idx": int(round(21 / dx)), "lo": 0, "hi": ny - 1,
+                 "freqs": [1.0]}]
+        _, acc = _te_line(nx, ny, dx, c, int(round(5 / dx)), amps, mons)
+        return yee2d.flux_from_acc("te_x", acc[0], dx)
+
+    amp = math.sqrt(float(flux(True)[0] / flux(False)[0]))
+    w = 2 * math.pi
+    n = complex(np.sqrt(1.0 + 1j / w))
+    t_an = math.exp(-w * abs(n.imag) * 2.0) * abs(4 * n / (1 + n) ** 2)
+    assert abs(amp - t_an) / t_an < 0.02, (amp, t_an)
