@@ -129,6 +129,13 @@ typedef struct {
   const void *c1x, *c2x, *c3x, *c1y, *c2y, *c3y;
   void *Px_cur, *Px_prev, *Py_cur, *Py_prev;
   const void *d1x, *d2x, *d1y, *d2y, *hz1, *hz2;
+  /* kerr = 1: three Jacobi passes of E = rhs / (eps + chi2 E + chi3 |E|^2)
+   * with |E|^2 = E^2 + (four-point average of the other component)^2
+   * (stepper_te.py:52-61,161-175); chi2x, chi3x (nx-1, ny), chi2y, chi3y
+   * (nx, ny-1); kx, ky: device scratch of the Ex and Ey shapes. */
+  int kerr, pad3_;
+  const void *chi2x, *chi3x, *chi2y, *chi3y;
+  void *kx, *ky;
 } mt_te2d;
 
 
@@ -364,7 +371,8 @@ int mt_abi_version(void);
  * mt_source3d, mt_dft_term, mt_monitor, mt_tm2d_adj, mt_te2d_adj,
  * mt_yee3d_adj, mt_tm2d_batch_item, then offsetof of mt_tm2d.hsrc,
  * mt_tm2d.hcoef, mt_te2d.tape, mt_te2d.hsrc, mt_te2d.hcoef, mt_yee3d.alt,
- * mt_monitor.acc, mt_tm2d.n_poles, mt_tm2d.h2y, mt_te2d.npx, mt_te2d.hz2. */
+ * mt_monitor.acc, mt_tm2d.n_poles, mt_tm2d.h2y, mt_te2d.npx, mt_te2d.hz2,
+ * mt_te2d.ky. */
 int mt_abi_layout(long long* out, int n);
 
 #ifdef __cplusplus

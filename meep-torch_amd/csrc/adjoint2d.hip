@@ -312,8 +312,8 @@ extern "C" int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const voi
   if (p->nx < 3 || p->ny < 3) return fail(MT_EINVAL, "grid must be at least 3x3");
   if (k < 0) return fail(MT_EINVAL, "negative step count");
   if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward E tape");
-  if (p->npx || p->npy || p->d1x || p->hz1)
-    return fail(MT_EINVAL, "adjoint through dispersive or conductive media is not supported");
+  if (p->npx || p->npy || p->d1x || p->hz1 || p->kerr)
+    return fail(MT_EINVAL, "adjoint through dispersive, nonlinear or conductive media is not supported");
   if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");
   if (!p->eps_x || !p->eps_y || !a->aHz || !a->aEx || !a->aEy || !a->aDx || !a->aDy || !a->apsi_Eyx ||
       !a->apsi_Exy || !a->apsi_Hzy || !a->apsi_Hzx || !a->gEy || !a->gEx || !a->gDx || !a->gDy ||

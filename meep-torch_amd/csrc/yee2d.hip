@@ -164,7 +164,56 @@ struct TeK {
   int npx, npy;
   const T *c1x, *c2x, *c3x, *c1y, *c2y, *c3y, *d1x, *d2x, *d1y, *d2y, *hz1, *hz2;
   T *Pxc, *Pxp, *Pyc, *Pyp;
+  int kerr;
+  const T *chi2x, *chi3x, *chi2y, *chi3y;
+  T *kx, *ky;  // scratch of the Ex / Ey shapes for the Kerr passes
 };
+
+// D - sum P of one E cell with the pole stacks already advanced (the value
+// te_e_from_d divided by eps, recomputed in the same order).
+template <typename T>
+__device__ __forceinline__ T te_rhs(T d, int np, long long plane, long long c, const T* pc) {
+  if (!np) return d;
+  T sum = pc[c];
+  for (int k = 1; k < np; ++k) sum = sum + pc[k * plane + c];
+  return d - sum;
+}
+
+// One Jacobi pass of the TE chi2 / chi3 fixed point (stepper_te.py:161-175):
+// |E|^2 at Ex = Ex^2 + avg4(Ey)^2 and at Ey = Ey^2 + avg4(Ex)^2, the
+// four-point averages with zero padding beyond the walls
+// (stepper_te.py:52-61), all from the input pair (exi, eyi).
+template <typename T>
+__global__ __launch_bounds__(BX* BY) void te_kerr_kernel(TeK<T> p, const T* exi, const T* eyi, T* exo, T* eyo) {
+  int j = blockIdx.x * BX + threadIdx.x;
+  int i = blockIdx.y * BY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx || j >= ny) return;
+  if (i < nx - 1) {  // Ex (nx-1, ny); Ey neighbours (i, j-1) (i, j) (i+1, j-1) (i+1, j)
+    const long long c = (long long)i * ny + j;
+    const T a = j >= 1 ? eyi[(long long)i * (ny - 1) + j - 1] : T(0);
+    const T b = j < ny - 1 ? eyi[(long long)i * (ny - 1) + j] : T(0);
+    const T cc = j >= 1 ? eyi[(long long)(i + 1) * (ny - 1) + j - 1] : T(0);
+    const T d = j < ny - 1 ? eyi[(long long)(i + 1) * (ny - 1) + j] : T(0);
+    const T av = T(0.25) * (a + b + cc + d);
+    const T e = exi[c];
+    const T e2 = e * e + av * av;
+    const T rhs = te_rhs(p.Dx[c], p.npx, (long long)(nx - 1) * ny, c, p.Pxc);
+    exo[c] = rhs / (p.eps_x[c] + p.chi2x[c] * e + p.chi3x[c] * e2);
+  }
+  if (j < ny - 1) {  // Ey (nx, ny-1); Ex neighbours (i-1, j) (i-1, j+1) (i, j) (i, j+1)
+    const long long c = (long long)i * (ny - 1) + j;
+    const T a = i >= 1 ? exi[(long long)(i - 1) * ny + j] : T(0);
+    const T b = i >= 1 ? exi[(long long)(i - 1) * ny + j + 1] : T(0);
+    const T cc = i < nx - 1 ? exi[(long long)i * ny + j] : T(0);
+    const T d = i < nx - 1 ? exi[(long long)i * ny + j + 1] : T(0);
+    const T av = T(0.25) * (a + b + cc + d);
+    const T e = eyi[c];
+    const T e2 = e * e + av * av;
+    const T rhs = te_rhs(p.Dy[c], p.npy, (long long)nx * (ny - 1), c, p.Pyc);
+    eyo[c] = rhs / (p.eps_y[c] + p.chi2y[c] * e + p.chi3y[c] * e2);
+  }
+}
 
 // E = (D - sum P) / eps after advancing the pole stacks from the E before
 // the step (stepper_te.py:137-147, 160-168); the stacks have `np` planes of
@@ -216,7 +265,7 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
     p.Dx[c] = dxv;
     const T ex = te_e_from_d(dxv, p.eps_x[c], p.npx ? p.Ex[c] : T(0), p.npx, (long long)(nx - 1) * ny, c, p.c1x,
                              p.c2x, p.c3x, p.Pxc, p.Pxp);
-    p.Ex[c] = ex;
+    (p.kerr ? p.kx : p.Ex)[c] = ex;  // Kerr: the fixed point starts from the scratch copy
     if (p.tape) p.tape[(long long)s_rel * ((long long)(nx - 1) * ny + (long long)nx * (ny - 1)) + c] = ex;
   }
   if (j < ny - 1) {  // Dy (nx, ny-1)
@@ -245,7 +294,7 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
     p.Dy[c] = dyv;
     const T ey = te_e_from_d(dyv, p.eps_y[c], p.npy ? p.Ey[c] : T(0), p.npy, (long long)nx * (ny - 1), c, p.c1y,
                              p.c2y, p.c3y, p.Pyc, p.Pyp);
-    p.Ey[c] = ey;
+    (p.kerr ? p.ky : p.Ey)[c] = ey;
     if (p.tape)
       p.tape[(long long)s_rel * ((long long)(nx - 1) * ny + (long long)nx * (ny - 1)) + (long long)(nx - 1) * ny + c] = ey;
   }
@@ -391,6 +440,13 @@ int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const
   p.d2y = static_cast<const T*>(d->d2y);
   p.hz1 = static_cast<const T*>(d->hz1);
   p.hz2 = static_cast<const T*>(d->hz2);
+  p.kerr = d->kerr;
+  p.chi2x = static_cast<const T*>(d->chi2x);
+  p.chi3x = static_cast<const T*>(d->chi3x);
+  p.chi2y = static_cast<const T*>(d->chi2y);
+  p.chi3y = static_cast<const T*>(d->chi3y);
+  p.kx = static_cast<T*>(d->kx);
+  p.ky = static_cast<T*>(d->ky);
   mt_source2d sv{};
   if (src && src->present) sv = *src;
   dim3 blk(BX, BY), grd = grid2(p.ny, p.nx, blk);
@@ -399,6 +455,11 @@ int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const
   for (int s_ = 0; s_ < k; ++s_) {
     T c = sv.present ? static_cast<T>(coef[s_]) : T(0);
     hipLaunchKernelGGL(te_d_kernel<T>, grd, blk, 0, s, p, sv, c, s_);
+    if (p.kerr) {  // three passes: scratch -> E -> scratch -> E
+      hipLaunchKernelGGL(te_kerr_kernel<T>, grd, blk, 0, s, p, p.kx, p.ky, p.Ex, p.Ey);
+      hipLaunchKernelGGL(te_kerr_kernel<T>, grd, blk, 0, s, p, p.Ex, p.Ey, p.kx, p.ky);
+      hipLaunchKernelGGL(te_kerr_kernel<T>, grd, blk, 0, s, p, p.kx, p.ky, p.Ex, p.Ey);
+    }
     const T hc = d->hcoef ? static_cast<T>(d->hcoef[s_]) : T(0);
     hipLaunchKernelGGL(te_b_kernel<T>, grd, blk, 0, s, p, d->hsrc, hc);
     int rc = check_launch("te2d step");
@@ -646,7 +707,10 @@ extern "C" int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const doubl
   if (!p->d1x != !p->d2x || !p->d1x != !p->d1y || !p->d1x != !p->d2y)
     return fail(MT_EINVAL, "sigma_d needs d1x, d2x, d1y and d2y");
   if (!p->hz1 != !p->hz2) return fail(MT_EINVAL, "sigma_b/mu needs hz1 and hz2");
-  const bool media = p->npx || p->npy || p->d1x || p->hz1;
+  if (p->kerr && (!p->chi2x || !p->chi3x || !p->chi2y || !p->chi3y || !p->kx || !p->ky))
+    return fail(MT_EINVAL, "TE chi2/chi3 fixed point needs chi2x, chi3x, chi2y, chi3y and the kx, ky scratch");
+  if (p->kerr && p->tape) return fail(MT_EINVAL, "the E tape is not written under chi2/chi3");
+  const bool media = p->npx || p->npy || p->d1x || p->hz1 || p->kerr;
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);

@@ -243,8 +243,8 @@ def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_
                          or coeffs.h1x is not None):
         raise NotImplementedError("gradients through dispersive, nonlinear or conductive media are not supported")
     if mode == "te" and (coeffs.c1x.shape[0] or coeffs.c1y.shape[0] or coeffs.d1x is not None
-                         or coeffs.hz1 is not None):
-        raise NotImplementedError("gradients through dispersive or conductive media are not supported")
+                         or coeffs.hz1 is not None or coeffs.has_kerr or coeffs.has_chi2):
+        raise NotImplementedError("gradients through dispersive, nonlinear or conductive media are not supported")
     allowed = ("eps_inf",) if mode == "tm" else ("eps_x", "eps_y")
     for name, t in coeffs._asdict().items():
         if torch.is_tensor(t) and t.requires_grad and name not in allowed:

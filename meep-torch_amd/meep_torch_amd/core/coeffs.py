@@ -255,18 +255,16 @@ def build_te_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
                     pole_masks_tensor=None, y_mirror: int = 0, x_mirror: int = 0) -> TECoeffs:
     """coeffs.py:357-503: eps + CPML, electric poles shared by both axes and
     per-axis (poles_x / poles_y, coeffs.py:109-131,420-427), sigma_d /
-    sigma_b / mu (coeffs.py:399-410).  eps_x / eps_y are the two-point
-    averages of the cell eps (coeffs.py:347-355,385).  chi2 / chi3 (which
-    couple Ex and Ey through neighbour averages in the fixed point), tensor
-    eps, oriented poles and symmetry folds raise."""
+    sigma_b / mu (coeffs.py:399-410), chi2 / chi3 averaged onto the E
+    positions (coeffs.py:383-384).  eps_x / eps_y are the two-point averages
+    of the cell eps (coeffs.py:347-355,385).  Tensor eps, oriented poles and
+    symmetry folds raise."""
     dt, dx = grid.dt, grid.dx
     eps = _as_field(eps_inf, grid, 1.0)
     x3 = _as_field(chi3, grid, 0.0)
     x2 = _as_field(chi2, grid, 0.0)
-    if bool((x3 != 0).any()):
-        _unsupported("chi3 (Kerr) in TE")
-    if bool((x2 != 0).any()):
-        _unsupported("chi2 in TE")
+    has_kerr = bool((x3 != 0).any())
+    has_chi2 = bool((x2 != 0).any())
     if len(poles_tensor):
         _unsupported("poles_tensor (oriented poles)")
     cond = {}
@@ -307,7 +305,7 @@ def build_te_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
     dev = lambda t: t.contiguous().to(grid.device)
     return TECoeffs(
         dt=dt, dx=dx, eps_x=eps_x, eps_y=eps_y, chi3_x=_avg_to_ex(x3), chi3_y=_avg_to_ey(x3),
-        has_kerr=False, chi2_x=_avg_to_ex(x2), chi2_y=_avg_to_ey(x2), has_chi2=False,
+        has_kerr=has_kerr, chi2_x=_avg_to_ex(x2), chi2_y=_avg_to_ey(x2), has_chi2=has_chi2,
         c1x=c1x, c2x=c2x, c3x=c3x, c1y=c1y, c2y=c2y, c3y=c3y, **cond,
         b_hx=dev(b_hx)[:, None], a_hx=dev(a_hx)[:, None], k_hx=dev(k_hx)[:, None],
         b_hy=dev(b_hy)[None, :], a_hy=dev(a_hy)[None, :], k_hy=dev(k_hy)[None, :],

@@ -221,8 +221,15 @@ def _te_media(d, c, fields, dtype, device):
         keep.append(t)
         setattr(d, name, t.data_ptr())
 
-    if c.has_kerr or c.has_chi2 or c.inv_xx_x is not None or c.c3xy is not None:
-        raise NotImplementedError("TE chi2/chi3, tensor eps and oriented poles are not supported")
+    if c.inv_xx_x is not None or c.c3xy is not None:
+        raise NotImplementedError("TE tensor eps and oriented poles are not supported")
+    if c.has_kerr or c.has_chi2:
+        d.kerr = 1
+        for name, t in (("chi2x", c.chi2_x), ("chi3x", c.chi3_x), ("chi2y", c.chi2_y), ("chi3y", c.chi3_y)):
+            put(name, t)
+        kx, ky = torch.empty_like(fields["Ex"]), torch.empty_like(fields["Ey"])
+        keep += [kx, ky]
+        d.kx, d.ky = kx.data_ptr(), ky.data_ptr()
     for ax, n in (("x", int(c.c1x.shape[0])), ("y", int(c.c1y.shape[0]))):
         if not n:
             continue
@@ -293,7 +300,8 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
     for f, t in fields.items():
         if not t.is_contiguous():
             raise ValueError(f"state field {f} must be contiguous")
-    media = bool(d.n_poles or d.kerr or d.d1 or d.h1x) if mode == "tm" else bool(d.npx or d.npy or d.d1x or d.hz1)
+    media = bool(d.n_poles or d.kerr or d.d1 or d.h1x) if mode == "tm" else \
+        bool(d.npx or d.npy or d.d1x or d.hz1 or d.kerr)
     if (plan is not None and plan.get("h")) or media:
         variant = -1  # magnetic currents and media are handled by the two-pass kernels
     if variant is None:

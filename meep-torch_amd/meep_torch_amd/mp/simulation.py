@@ -147,9 +147,9 @@ class Simulation(Sim3D):
                 kw.update(sigma_d=sd, sigma_b=sb, mu=mu)
             self._coeffs = build_tm_coeffs(g, eps_inf=eps, npml=npml, **kw)
         else:
-            if media["chi3"] is not None:
-                raise NotImplementedError("chi3 in TE runs is not supported")
             kw = {}
+            if media["chi3"] is not None:
+                kw["chi3"] = G.to_tensor(media["chi3"], torch.float64, g.device)
             if media["poles"]:
                 kw.update(poles=media["poles"],
                           pole_masks=[G.to_tensor(m, torch.float64, g.device) for m in media["masks"]])

@@ -100,6 +100,10 @@ class Coef2D:
     d2y: torch.Tensor = None
     hz1: torch.Tensor = None
     hz2: torch.Tensor = None
+    chi2x: torch.Tensor = None
+    chi3x: torch.Tensor = None
+    chi2y: torch.Tensor = None
+    chi3y: torch.Tensor = None
 
 
 def pole_recurrence(pole, dt):
@@ -194,7 +198,8 @@ def _te_axis_poles(nx, ny, dt, dtype, shared, own, shared_masks, own_masks, shap
 
 
 def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, poles=(), pole_masks=None, poles_x=(),
-              poles_y=(), pole_masks_x=None, pole_masks_y=None, sigma_d=None, sigma_b=None, mu=None):
+              poles_y=(), pole_masks_x=None, pole_masks_y=None, sigma_d=None, sigma_b=None, mu=None, chi2=None,
+              chi3=None):
     """meep_torch/core/coeffs.py:357-503 restricted to eps + CPML, shared and
     per-axis electric poles and sigma_d / sigma_b / mu; eps_x/eps_y are the
     two-point averages of coeffs.py:347-355."""
@@ -225,6 +230,10 @@ def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, poles=(),
                                          (nx - 1, ny), ax)
     c.c1y, c.c2y, c.c3y = _te_axis_poles(nx, ny, dt, dtype, poles, poles_y, pole_masks, pole_masks_y,
                                          (nx, ny - 1), ay)
+    full = lambda v: (torch.full((nx, ny), float(v or 0.0), dtype=dtype) if not torch.is_tensor(v) else v.to(dtype))
+    x2, x3 = full(chi2), full(chi3)
+    c.kerr = bool((x2 != 0).any()) or bool((x3 != 0).any())   # coeffs.py:380-384
+    c.chi2x, c.chi2y, c.chi3x, c.chi3y = ax(x2), ay(x2), ax(x3), ay(x3)
     if sigma_d is not None or sigma_b is not None or mu is not None:
         # coeffs.py:399-410
         full = lambda v, d: (torch.full((nx, ny), float(d if v is None else v), dtype=dtype)
@@ -366,6 +375,19 @@ def te_step(s, c, coef=0.0, mask=None, hcoef=0.0, hz_mask=None):
         ry = Dy - Pyn.sum(0)
     Ex = rx / c.eps_x
     Ey = ry / c.eps_y
+    if c.kerr:
+        # stepper_te.py:161-175: Jacobi passes, |E|^2 with the four-point
+        # average of the other component, zero-padded beyond the walls
+        # (stepper_te.py:52-61)
+        for _ in range(3):
+            eyp = torch.nn.functional.pad(Ey, (1, 1))
+            exp_ = torch.nn.functional.pad(Ex, (0, 0, 1, 1))
+            ay4 = 0.25 * (eyp[:-1, :-1] + eyp[:-1, 1:] + eyp[1:, :-1] + eyp[1:, 1:])
+            ax4 = 0.25 * (exp_[:-1, :-1] + exp_[:-1, 1:] + exp_[1:, :-1] + exp_[1:, 1:])
+            e2x = Ex * Ex + ay4 ** 2
+            e2y = Ey * Ey + ax4 ** 2
+            Ex = rx / (c.eps_x + c.chi2x * Ex + c.chi3x * e2x)
+            Ey = ry / (c.eps_y + c.chi2y * Ey + c.chi3y * e2y)
     dEy = (Ey[1:, :] - Ey[:-1, :]) / dx
     dEx = (Ex[:, 1:] - Ex[:, :-1]) / dx
     pyx = c.b_hx * s["psi_Eyx"] + c.a_hx * dEy

@@ -38,15 +38,15 @@ def test_ctypes_structs_match_the_compiled_layout():
     L = E.lib()
     buf = (ctypes.c_longlong * 64)()
     n = L.mt_abi_layout(buf, 64)
-    assert n == 23
+    assert n == 24
     sizes = [E.PmlAxis, E.Source2D, E.TM2D, E.TE2D, E.Yee3D, E.Source3D, E.DftTerm, E.Monitor, E.TM2DAdj,
              E.TE2DAdj, E.Yee3DAdj, BatchItem]
     for q, st in enumerate(sizes):
         assert buf[q] == ctypes.sizeof(st), (st.__name__, buf[q], ctypes.sizeof(st))
     offs = [E.TM2D.hsrc.offset, E.TM2D.hcoef.offset, E.TE2D.tape.offset, E.TE2D.hsrc.offset,
             E.TE2D.hcoef.offset, E.Yee3D.alt.offset, E.Monitor.acc.offset, E.TM2D.n_poles.offset,
-            E.TM2D.h2y.offset, E.TE2D.npx.offset, E.TE2D.hz2.offset]
-    assert list(buf[12:23]) == offs
+            E.TM2D.h2y.offset, E.TE2D.npx.offset, E.TE2D.hz2.offset, E.TE2D.ky.offset]
+    assert list(buf[12:24]) == offs
 
 
 def test_error_path_without_gpu_work():
@@ -94,14 +94,13 @@ def test_mask_box():
     assert mask_box(torch.zeros(4, 4)) is None
 
 
-@pytest.mark.parametrize("te,kw,flag", [(True, {"chi3": 0.1}, "chi3"), (True, {"chi2": 0.1}, "chi2"),
-                                        (True, {"poles_tensor": [(1.0, 0.1, (1.0, 0.2, 1.0))]}, "poles_tensor"),
+@pytest.mark.parametrize("te,kw,flag", [(True, {"poles_tensor": [(1.0, 0.1, (1.0, 0.2, 1.0))]}, "poles_tensor"),
                                         (True, {"eps_tensor": (2.0, 0.1, 2.0)}, "eps_tensor"),
                                         (False, {"y_mirror": 1}, "symmetry"),
                                         (False, {"m_poles": [(1.0, 0.1, 1.0)]}, "m_poles")])
 def test_unsupported_flags_raise(te, kw, flag):
-    """TE chi2/chi3, oriented poles and tensor eps, TM magnetic poles and
-    symmetry folds still raise by name."""
+    """TE oriented poles and tensor eps, TM magnetic poles and symmetry
+    folds still raise by name."""
     from meep_torch_amd.core.coeffs import build_te_coeffs, build_tm_coeffs
     from meep_torch_amd.core.grid import Grid
     g = Grid(nx=16, ny=12, dx=0.05, dtype=torch.float32, device="cpu")

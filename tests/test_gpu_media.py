@@ -207,7 +207,8 @@ def _te_media(dtype):
     eps = 1.0 + 1.5 * blk * r
     kw = dict(poles=[(1.2, 0.1, 1.5)], pole_masks=[blk.to(dtype)],
               poles_x=[(0.0, 0.3, 0.5, 0.9)], pole_masks_x=[(blk * (r > 0.5)).to(dtype)],      # Drude on Ex only
-              sigma_d=(0.3 * blk).to(dtype), sigma_b=(0.1 * blk).to(dtype), mu=(1.0 + 0.5 * blk).to(dtype))
+              sigma_d=(0.3 * blk).to(dtype), sigma_b=(0.1 * blk).to(dtype), mu=(1.0 + 0.5 * blk).to(dtype),
+              chi3=(0.05 * blk).to(dtype), chi2=(0.02 * blk * r).to(dtype))
     return eps.to(dtype), kw
 
 
@@ -215,7 +216,8 @@ def _te_media(dtype):
                          ids=["f64", "f32"])
 def test_te_media_run_matches_oracle(dtype, tf, ts):
     """TE: a pole shared by Ex and Ey, a Drude pole on Ex only, sigma_d,
-    sigma_b and mu, against the oracle's TE step."""
+    sigma_b, mu, chi2 and chi3 (three Jacobi passes coupling Ex and Ey),
+    against the oracle's TE step."""
     from meep_torch_amd.core.coeffs import build_te_coeffs
     from meep_torch_amd.core.grid import Grid
     from meep_torch_amd.core.monitors import FluxMonitorTE
@@ -228,7 +230,7 @@ def test_te_media_run_matches_oracle(dtype, tf, ts):
     dev = {k: ([x.cuda() for x in v] if k.startswith("pole_masks") else v.cuda() if torch.is_tensor(v) else v)
            for k, v in kw.items()}
     m = build_te_coeffs(g, eps_inf=eps.cuda(), npml=NPML, **dev)
-    src = LineSource(g, ix=20, j0=16, j1=48, profile=GaussianPulse(0.8, 0.5))
+    src = LineSource(g, ix=20, j0=16, j1=48, profile=GaussianPulse(0.8, 0.5), amp_func=lambda y: 5.0)
     src.mask = src.mask[:, :-1].contiguous()
     mon = FluxMonitorTE(g, FREQS, ix=80, lo=12, hi=52)
     st, (acc,) = run_te(init_te_state(g, n_poles=2, n_poles_y=1), m, 400, source=src, monitors=[mon],
@@ -284,8 +286,8 @@ def test_te_lorentz_slab_on_hip():
 
 
 def test_te_facade_drude_block_equals_direct_run():
-    """A TE Simulation cell with a cell-aligned Drude + D-conductivity block
-    against run_te on hand-built maps (FP64): identical fields and flux."""
+    """A TE Simulation cell with a cell-aligned Drude + D-conductivity + chi3
+    block against run_te on hand-built maps (FP64): identical fields and flux."""
     import meep_torch_amd.mp as mp
     from meep_torch_amd.core.coeffs import build_te_coeffs
     from meep_torch_amd.core.monitors import FluxMonitorTE
@@ -295,7 +297,8 @@ def test_te_facade_drude_block_equals_direct_run():
     sim = mp.Simulation(
         cell_size=mp.Vector3(10, 6), resolution=10,
         geometry=[mp.Block(center=mp.Vector3(1.5, 0), size=mp.Vector3(1, 2),
-                           material=mp.Medium(epsilon=2, E_susceptibilities=[D], D_conductivity=0.2))],
+                           material=mp.Medium(epsilon=2, E_susceptibilities=[D], D_conductivity=0.2,
+                                              chi3=0.05))],
         sources=[mp.Source(mp.GaussianSource(0.8, fwidth=0.5), component=mp.Ey,
                            center=mp.Vector3(-4, 0), size=mp.Vector3(0, 2))],
         boundary_layers=[mp.PML(1.0)], dtype=torch.float64, device="cuda")
@@ -303,12 +306,13 @@ def test_te_facade_drude_block_equals_direct_run():
     sim.run(until=20)
     g = sim.grid
     z = lambda v=0.0: torch.full((100, 60), v, dtype=torch.float64, device="cuda")
-    eps, mD, sd = z(1.0), z(), z()
+    eps, mD, sd, x3 = z(1.0), z(), z(), z()
     eps[60:70, 20:40] = 2.0
     mD[60:70, 20:40] = 1.0
     sd[60:70, 20:40] = 0.2
+    x3[60:70, 20:40] = 0.05
     m = build_te_coeffs(g, eps_inf=eps, npml=((10, 10), (10, 10)), poles=[D.to_pole()], pole_masks=[mD],
-                        sigma_d=sd, sigma_b=z(), mu=z(1.0))
+                        sigma_d=sd, sigma_b=z(), mu=z(1.0), chi3=x3)
     src = sim._make_sources()[0]
     freqs = np.linspace(0.8 - 0.4 / 2.0, 0.8 + 0.4 / 2.0, 3).tolist()
     kind, mon, meta = sim._monitors[fl.name]

@@ -169,15 +169,10 @@ def test_media_rasterization_compositing():
     assert out3["poles"] == [] and out3["chi3"] is None and out3["cond"] is None
 
 
-def test_media_outside_tm_raise():
-    """chi3 in TE cells and any medium in 3D cells raise by name."""
+def test_media_in_3d_raise():
+    """Any medium in a 3D cell raises by name."""
     import meep_torch_amd.mp as mp
     med = mp.Medium(epsilon=2, D_conductivity=0.1)
-    te = mp.Simulation(cell_size=mp.Vector3(2, 2), resolution=10, device="cpu",
-                       geometry=[mp.Block(size=mp.Vector3(1, 1), material=mp.Medium(epsilon=2, chi3=0.1))],
-                       sources=[mp.Source(mp.GaussianSource(1.0, fwidth=0.5), component=mp.Hz)])
-    with pytest.raises(NotImplementedError):
-        te.init_sim()
     s3 = mp.Simulation(cell_size=mp.Vector3(1, 1, 1), resolution=10, dimensions=3, device="cpu",
                        geometry=[mp.Block(size=mp.Vector3(0.5, 0.5, 0.5), material=med)])
     with pytest.raises(NotImplementedError):

@@ -208,3 +208,41 @@ def test_te_conductive_slab_matches_analytic():
     n = complex(np.sqrt(1.0 + 1j / w))
     t_an = math.exp(-w * abs(n.imag) * 2.0) * abs(4 * n / (1 + n) ** 2)
     assert abs(amp - t_an) / t_an < 0.02, (amp, t_an)
+
+
+def test_te_kerr_third_harmonic_scales_as_cube():
+    """The TE chi3 fixed point (stepper_te.py:161-175) on the parallel-plate
+    wave: twice the drive gives eight times the third harmonic, and none
+    without chi3."""
+    f0 = 0.25
+
+    def peaks(amp, chi3):
+        nx, ny, dx = 1000, 8, 0.02
+        c = yee2d.te_coeffs(nx, ny, dx, torch.full((nx, ny), 2.25, dtype=DT), ((40, 40), (0, 0)), dtype=DT,
+                            chi3=chi3)
+
+        def a(t):
+            ramp = 1.0 if t >= 40.0 else 0.5 * (1 - math.cos(math.pi * t / 40.0))
+            return amp * ramp * math.sin(2 * math.pi * f0 * t)
+
+        mask = torch.zeros(nx, ny - 1, dtype=DT)
+        mask[100, :] = 1.0 / dx
+        st, series = yee2d.te_state(nx, ny, DT), []
+        for t in range(16000):
+            st = yee2d.te_step(st, c, c.dt * a((t + 0.5) * c.dt), mask)
+            series.append(float(st["Ey"][700, 3]))
+        steady = torch.tensor(series[10000:], dtype=DT)   # past the ramp and the transit
+        spec = torch.fft.rfft(steady).abs()
+        fr = torch.fft.rfftfreq(len(steady), c.dt)
+
+        def pk(f):
+            i = int((fr - f).abs().argmin())
+            return float(spec[i - 2:i + 3].max())
+        return pk(f0), pk(3 * f0)
+
+    fa, ta = peaks(0.02, 0.02)
+    fb, tb = peaks(0.04, 0.02)
+    f0_, t0_ = peaks(0.02, 0.0)
+    assert abs(fb / fa - 2.0) < 0.01
+    assert abs(tb / ta - 8.0) < 0.4
+    assert t0_ / f0_ < 1e-9 < ta / fa
