@@ -139,9 +139,10 @@ typedef struct {
 } mt_te2d;
 
 
-/* 3D problem + state (stepper_3d.py:28-67 Coeffs3D / State3D, CPML path,
- * sigma = 0).  lean = 1 selects the in-place lean stepper's E update and
- * source sign (stepper_3d_lean.py:234-296). */
+/* 3D problem + state (stepper_3d.py:28-67 Coeffs3D / State3D, CPML path;
+ * material conductivity and ADE poles through the media fields at the end).
+ * lean = 1 selects the in-place lean stepper's E update and source sign
+ * (stepper_3d_lean.py:234-296). */
 typedef struct {
   int nx, ny, nz, dtype, lean, pad_;
   double dt, dx;
@@ -163,6 +164,24 @@ typedef struct {
    * primary buffers either way */
   int variant, pad2_;
   void* alt[18];
+  /* Media of the functional stepper (stepper_3d.py:217-266,295-301); two-pass
+   * variant 0 with lean = 0 only.  damp: all NULL (sigma = 0, no poles) or 12
+   * device arrays, (d1, d2) of ex, ey, ez on the E component grids then of hx,
+   * hy, hz on the H component grids, from the Yee-averaged sigma_d / sigma_b:
+   * d1 = (1 - s dt/2) / (1 + s dt/2), d2 = dt / (1 + s dt/2), and
+   *   H = d1 H - d2 curlE,   E = (d1 E eps_c + d2 curlH) / eps_c.
+   * n_poles > 0 (needs damp): c1, c2 device arrays of n_poles scalars, and per
+   * E component a = x, y, z the stacks c3[a], P_cur[a], P_prev[a] of shape
+   * (n_poles, component grid):
+   *   P' = c1 P + c2 P_prev + c3 E(n),  P_prev' = P,
+   *   E = (d1 (E eps_c + sum P) + d2 curlH - sum P') / eps_c,
+   * the sums taken in pole order. */
+  const void* damp[12];
+  int n_poles, pad3_;
+  const void *c1, *c2;
+  const void* c3[3];
+  void* P_cur[3];
+  void* P_prev[3];
 } mt_yee3d;
 
 /* 3D soft source: kind 1 = point at (i0, j0, k0), kind 2 = mask given as dense
@@ -372,7 +391,7 @@ int mt_abi_version(void);
  * mt_yee3d_adj, mt_tm2d_batch_item, then offsetof of mt_tm2d.hsrc,
  * mt_tm2d.hcoef, mt_te2d.tape, mt_te2d.hsrc, mt_te2d.hcoef, mt_yee3d.alt,
  * mt_monitor.acc, mt_tm2d.n_poles, mt_tm2d.h2y, mt_te2d.npx, mt_te2d.hz2,
- * mt_te2d.ky. */
+ * mt_te2d.ky, mt_yee3d.damp, mt_yee3d.P_prev. */
 int mt_abi_layout(long long* out, int n);
 
 #ifdef __cplusplus

@@ -258,6 +258,7 @@ extern "C" int mt_yee3d_adjoint(const mt_yee3d* p, const mt_yee3d_adj* a, const 
   if ((long long)p->ny * p->nz > (1LL << 31) || p->nx > 65535)
     return fail(MT_EINVAL, "grid too large for the launch shape");
   if (p->lean) return fail(MT_EINVAL, "the 3D adjoint transposes the functional stepper (lean = 0)");
+  if (p->damp[0] || p->n_poles) return fail(MT_ENOTIMPL, "the 3D adjoint has no media (sigma or poles)");
   if (k < 0) return fail(MT_EINVAL, "negative step count");
   if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward E tape");
   if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");

@@ -414,7 +414,7 @@ extern "C" long long mt_stage_bytes(const mt_monitor* mons, int n_mons, int stag
 
 extern "C" const char* mt_last_error(void) { return mt::g_err.c_str(); }
 
-extern "C" int mt_abi_version(void) { return 12; }
+extern "C" int mt_abi_version(void) { return 13; }
 
 extern "C" int mt_abi_layout(long long* out, int n) {
   const long long v[] = {
@@ -426,7 +426,7 @@ extern "C" int mt_abi_layout(long long* out, int n) {
       (long long)offsetof(mt_te2d, hsrc),    (long long)offsetof(mt_te2d, hcoef), (long long)offsetof(mt_yee3d, alt),
       (long long)offsetof(mt_monitor, acc),  (long long)offsetof(mt_tm2d, n_poles), (long long)offsetof(mt_tm2d, h2y),
       (long long)offsetof(mt_te2d, npx),     (long long)offsetof(mt_te2d, hz2),
-      (long long)offsetof(mt_te2d, ky)};
+      (long long)offsetof(mt_te2d, ky),      (long long)offsetof(mt_yee3d, damp), (long long)offsetof(mt_yee3d, P_prev)};
   const int total = (int)(sizeof(v) / sizeof(v[0]));
   for (int q = 0; q < n && q < total; ++q) out[q] = v[q];
   return total;

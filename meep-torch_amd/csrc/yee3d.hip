@@ -1,4 +1,5 @@
-// 3D Yee leapfrog with CPML shells (sigma = 0), point / masked soft sources.
+// 3D Yee leapfrog with CPML shells, point / masked soft sources, and (two-pass
+// path) material conductivity and ADE poles.
 //
 // Two passes per step: H from curl E, then E from curl H.  One thread per
 // cell (i, j, k) of the nx*ny*nz lattice updates every component whose grid
@@ -28,6 +29,14 @@ struct K3 {
   Pml<T> h[3], e[3];
   T *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;
   T* psi[12];
+  // media (two-pass kernels with M = true): (d1, d2) of ex ey ez hx hy hz,
+  // np poles with scalars c1[n], c2[n] and per E component the stacks
+  // c3[a], Pc[a] (P), Pp[a] (P_prev) of np component grids each
+  const T* dmp[12];
+  int np;
+  const T *c1, *c2;
+  const T* c3[3];
+  T *Pc[3], *Pp[3];
 };
 
 enum { P_EZ_Y, P_EY_Z, P_EX_Z, P_EZ_X, P_EY_X, P_EX_Y, P_HZ_Y, P_HY_Z, P_HX_Z, P_HZ_X, P_HY_X, P_HX_Y };
@@ -63,7 +72,7 @@ __device__ __forceinline__ T ce_apply(const CeIn<T>& c, T* psi, long long q, T d
   return d + ps;
 }
 
-template <typename T>
+template <typename T, bool M>
 __global__ __launch_bounds__(BZ* BYY) void h3_kernel(K3<T> p) {
   const int k = blockIdx.x * BZ + threadIdx.x;
   const int j = blockIdx.y * BYY + threadIdx.y;
@@ -102,21 +111,29 @@ __global__ __launch_bounds__(BZ* BYY) void h3_kernel(K3<T> p) {
   const CeIn<T> cxy = ce_load(p.h[1], j, fx, p.psi[P_EZ_Y], ix), cxz = ce_load(p.h[2], k, fx, p.psi[P_EY_Z], ix);
   const CeIn<T> cyz = ce_load(p.h[2], k, fy, p.psi[P_EX_Z], iy), cyx = ce_load(p.h[0], i, fy, p.psi[P_EZ_X], iy);
   const CeIn<T> czx = ce_load(p.h[0], i, fz, p.psi[P_EY_X], iz), czy = ce_load(p.h[1], j, fz, p.psi[P_EX_Y], iz);
+  // conductivity pairs: H = d1 H - d2 curl (stepper_3d.py:375,382,389); at
+  // sigma = 0 the reference's pairs are exactly (1, dt)
+  T d1x = T(1), d2x = dt, d1y = T(1), d2y = dt, d1z = T(1), d2z = dt;
+  if constexpr (M) {
+    if (fx) d1x = p.dmp[6][ix], d2x = p.dmp[7][ix];
+    if (fy) d1y = p.dmp[8][iy], d2y = p.dmp[9][iy];
+    if (fz) d1z = p.dmp[10][iz], d2z = p.dmp[11][iz];
+  }
   // ---- arithmetic and stores (stepper_3d.py:363-390)
   if (fx) {
     T cy = ce_apply(cxy, p.psi[P_EZ_Y], ix, (ezj - ez0) / dx);
     T cz = ce_apply(cxz, p.psi[P_EY_Z], ix, (eyk - ey0) / dx);
-    p.Hx[ix] = hx - dt * (cy - cz);
+    p.Hx[ix] = M ? d1x * hx - d2x * (cy - cz) : hx - dt * (cy - cz);
   }
   if (fy) {
     T cz = ce_apply(cyz, p.psi[P_EX_Z], iy, (exk - ex0) / dx);
     T cx = ce_apply(cyx, p.psi[P_EZ_X], iy, (ezx - ez0) / dx);
-    p.Hy[iy] = hy - dt * (cz - cx);
+    p.Hy[iy] = M ? d1y * hy - d2y * (cz - cx) : hy - dt * (cz - cx);
   }
   if (fz) {
     T cx = ce_apply(czx, p.psi[P_EY_X], iz, (eyx - ey0) / dx);
     T cy = ce_apply(czy, p.psi[P_EX_Y], iz, (exj - ex0) / dx);
-    p.Hz[iz] = hz - dt * (cx - cy);
+    p.Hz[iz] = M ? d1z * hz - d2z * (cx - cy) : hz - dt * (cx - cy);
   }
 }
 
@@ -127,6 +144,28 @@ __device__ __forceinline__ T e_update(const P& p, T E, T eps_c, T curl) {
     return E + (curl * inv) * p.dt;
   }
   return (E * eps_c + p.dt * curl) / eps_c;
+}
+
+// E update with media (stepper_3d.py:295-301,318-329,402-437): the poles
+// advance from the incoming E(n), P' = (c1 P + c2 P_prev) + c3 E(n), then
+//   E = (d1 (E eps_c + sum P) + d2 curl - sum P') / eps_c
+// (without poles E = (d1 E eps_c + d2 curl) / eps_c); q is the cell's index
+// in component a's grid of nq cells.
+template <typename T>
+__device__ __forceinline__ T e_media(const K3<T>& p, int a, long long q, long long nq, T E, T eps_c, T curl) {
+  const T d1 = p.dmp[2 * a][q], d2 = p.dmp[2 * a + 1][q];
+  if (p.np == 0) return (d1 * E * eps_c + d2 * curl) / eps_c;
+  T sc = T(0), sn = T(0);
+  for (int n = 0; n < p.np; ++n) {
+    const long long o = (long long)n * nq + q;
+    const T pc = p.Pc[a][o], pp = p.Pp[a][o];
+    const T pn = (p.c1[n] * pc + p.c2[n] * pp) + p.c3[a][o] * E;
+    p.Pp[a][o] = pc;
+    p.Pc[a][o] = pn;
+    sc = n ? sc + pc : pc;
+    sn = n ? sn + pn : pn;
+  }
+  return (d1 * (E * eps_c + sc) + d2 * curl - sn) / eps_c;
 }
 
 template <typename T, typename P>
@@ -144,7 +183,7 @@ __device__ __forceinline__ T inject(const P& p, const mt_source3d& s, int comp, 
   return E - (c * m) / eps_c;
 }
 
-template <typename T>
+template <typename T, bool M>
 __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T c, T* tape) {
   const int k = blockIdx.x * BZ + threadIdx.x;
   const int j = blockIdx.y * BYY + threadIdx.y;
@@ -202,7 +241,7 @@ __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T
     if (kz) chz = ce_apply(cxz, p.psi[P_HY_Z], q_xz, (hy0 - hyk) / dx);
     T curl = (jy && kz) ? chy - chz : T(0);
     T epsc = T(0.5) * (eps0 + epx);
-    T E = e_update(p, ex, epsc, curl);
+    T E = M ? e_media(p, 0, ce_, (long long)(nx - 1) * ny * nz, ex, epsc, curl) : e_update(p, ex, epsc, curl);
     E = inject(p, src, 0, i, j, k, E, epsc, c);
     p.Ex[ce_] = E;
     if (tape) tape[ce_] = E;
@@ -213,7 +252,7 @@ __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T
     if (ix) chx = ce_apply(cyx, p.psi[P_HZ_X], q_yx, (hz0 - hzx) / dx);
     T curl = (kz && ix) ? chz - chx : T(0);
     T epsc = T(0.5) * (eps0 + epy);
-    T E = e_update(p, ey, epsc, curl);
+    T E = M ? e_media(p, 1, iy, (long long)nx * (ny - 1) * nz, ey, epsc, curl) : e_update(p, ey, epsc, curl);
     E = inject(p, src, 1, i, j, k, E, epsc, c);
     p.Ey[iy] = E;
     if (tape) tape[(long long)(nx - 1) * ny * nz + iy] = E;
@@ -224,7 +263,7 @@ __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T
     if (jy) chy = ce_apply(czy, p.psi[P_HX_Y], q_zy, (hx0 - hxj) / dx);
     T curl = (ix && jy) ? chx - chy : T(0);
     T epsc = T(0.5) * (eps0 + epz);
-    T E = e_update(p, ez, epsc, curl);
+    T E = M ? e_media(p, 2, iz, (long long)nx * ny * (nz - 1), ez, epsc, curl) : e_update(p, ez, epsc, curl);
     E = inject(p, src, 2, i, j, k, E, epsc, c);
     p.Ez[iz] = E;
     if (tape) tape[(long long)(nx - 1) * ny * nz + (long long)nx * (ny - 1) * nz + iz] = E;
@@ -699,6 +738,16 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
   p.Hy = static_cast<T*>(d->Hy);
   p.Hz = static_cast<T*>(d->Hz);
   for (int q = 0; q < 12; ++q) p.psi[q] = static_cast<T*>(d->psi[q]);
+  const bool media = d->damp[0] != nullptr;
+  for (int q = 0; q < 12; ++q) p.dmp[q] = static_cast<const T*>(d->damp[q]);
+  p.np = media ? d->n_poles : 0;
+  p.c1 = static_cast<const T*>(d->c1);
+  p.c2 = static_cast<const T*>(d->c2);
+  for (int a = 0; a < 3; ++a) {
+    p.c3[a] = static_cast<const T*>(d->c3[a]);
+    p.Pc[a] = static_cast<T*>(d->P_cur[a]);
+    p.Pp[a] = static_cast<T*>(d->P_prev[a]);
+  }
   mt_source3d sv{};
   if (src && src->kind) sv = *src;
   if (d->variant == 1) return run3_fused<T, 8>(d, sv, coef, mp, step0, k, s);
@@ -712,10 +761,15 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
                             (long long)p.nx * p.ny * (p.nz - 1);
   for (int s_ = 0; s_ < k; ++s_) {
     timing_begin(s);
-    hipLaunchKernelGGL(h3_kernel<T>, grd, blk, 0, s, p);
     T c = sv.kind ? static_cast<T>(coef[s_]) : T(0);
     T* slot = d->tape ? static_cast<T*>(d->tape) + (long long)s_ * e_cells : nullptr;
-    hipLaunchKernelGGL(e3_kernel<T>, grd, blk, 0, s, p, sv, c, slot);
+    if (media) {
+      hipLaunchKernelGGL((h3_kernel<T, true>), grd, blk, 0, s, p);
+      hipLaunchKernelGGL((e3_kernel<T, true>), grd, blk, 0, s, p, sv, c, slot);
+    } else {
+      hipLaunchKernelGGL((h3_kernel<T, false>), grd, blk, 0, s, p);
+      hipLaunchKernelGGL((e3_kernel<T, false>), grd, blk, 0, s, p, sv, c, slot);
+    }
     timing_end(s, (long long)p.nx * p.ny * p.nz);
     int rc = check_launch("yee3d step");
     if (rc) return rc;
@@ -771,6 +825,23 @@ extern "C" int mt_yee3d_run(const mt_yee3d* p, const mt_source3d* src, const dou
     for (int q = 0; q < 18; ++q)
       if (!p->alt[q]) return fail(MT_EINVAL, "fused 3D variant needs all 18 alt buffers");
     if (p->dtype == MT_F64 && p->variant == 3) return fail(MT_EINVAL, "3D variant 3 (16-row tiles) is FP32 only");
+  }
+  bool any_damp = false, all_damp = true;
+  for (int q = 0; q < 12; ++q) {
+    any_damp = any_damp || p->damp[q];
+    all_damp = all_damp && p->damp[q];
+  }
+  if (any_damp && !all_damp) return fail(MT_EINVAL, "3D media need all 12 damping arrays");
+  if (p->n_poles < 0 || p->n_poles > 64) return fail(MT_EINVAL, "3D n_poles must be 0..64");
+  if (p->n_poles > 0 && !any_damp) return fail(MT_EINVAL, "3D poles need the damping arrays (d1 = 1, d2 = dt)");
+  if (any_damp) {
+    if (p->variant != 0) return fail(MT_EINVAL, "3D media run on the two-pass variant 0 only");
+    if (p->lean) return fail(MT_EINVAL, "3D media are not supported by the lean stepper");
+    if (p->n_poles > 0) {
+      if (!p->c1 || !p->c2) return fail(MT_EINVAL, "3D poles without c1 / c2");
+      for (int a = 0; a < 3; ++a)
+        if (!p->c3[a] || !p->P_cur[a] || !p->P_prev[a]) return fail(MT_EINVAL, "3D poles: null c3 or P stack");
+    }
   }
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;

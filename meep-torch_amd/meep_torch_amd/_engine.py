@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 12
+ABI_VERSION = 13
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -82,7 +82,9 @@ class Yee3D(ctypes.Structure):
                 ("Ex", c_void_p), ("Ey", c_void_p), ("Ez", c_void_p),
                 ("Hx", c_void_p), ("Hy", c_void_p), ("Hz", c_void_p),
                 ("psi", c_void_p * 12), ("tape", c_void_p), ("variant", c_int), ("pad2_", c_int),
-                ("alt", c_void_p * 18)]
+                ("alt", c_void_p * 18), ("damp", c_void_p * 12), ("n_poles", c_int), ("pad3_", c_int),
+                ("c1", c_void_p), ("c2", c_void_p), ("c3", c_void_p * 3), ("P_cur", c_void_p * 3),
+                ("P_prev", c_void_p * 3)]
 
 
 class Yee3DAdj(ctypes.Structure):

@@ -13,8 +13,8 @@ whatever graph produced it), like the reference.  Scheme:
    the chunk backwards (monitor cotangents injected at every step) and
    accumulates dL/deps.
 
-Functional stepper (step_3d, sigma = 0, CPML) only; the lean stepper and
-ADE poles raise NotImplementedError.
+Functional stepper (step_3d, sigma = 0, CPML) only; the lean stepper,
+material conductivity and ADE poles raise NotImplementedError.
 """
 import ctypes
 
@@ -39,6 +39,9 @@ def _fields_of(state):
 
 def run_3d_adjoint(c, n_steps: int, chunk: int, source, monitors, loss_fn, state0):
     from .adjoint2d import _adj_monitors
+    from .stepper_3d import has_media
+    if has_media(c):
+        raise NotImplementedError("3D adjoint through conductive or dispersive media is not supported")
     if chunk < 1:
         raise ValueError("chunk must be >= 1")
     L = E.lib()

@@ -10,7 +10,7 @@ import torch
 
 from .. import _engine as E
 from .engine2d import stage_alloc
-from .stepper_3d import COMPS, PSI_NAMES
+from .stepper_3d import COMPS, PSI_NAMES, has_media
 
 _COMP = {"Ex": 0, "Ey": 1, "Ez": 2}
 
@@ -64,6 +64,43 @@ def _axis(c, key):
     return E.pml_axis(b, a, c.shells[key] if c.shells else None)
 
 
+def _media(d, c, fields, dtype, device):
+    """Fill the descriptor's media fields from Coeffs3D (the (d1, d2) pairs of
+    c.damp and the pole scalars / c3 stacks of c.pols, stepper_3d.py:217-266);
+    returns the device tensors the call must keep alive."""
+    keep = []
+
+    def dev(t):
+        t = t.to(device=device, dtype=dtype).contiguous()
+        keep.append(t)
+        return t.data_ptr()
+
+    shapes = {"ex": fields["Ex"].shape, "ey": fields["Ey"].shape, "ez": fields["Ez"].shape,
+              "hx": fields["Hx"].shape, "hy": fields["Hy"].shape, "hz": fields["Hz"].shape}
+    for q, key in enumerate(("ex", "ey", "ez", "hx", "hy", "hz")):
+        for r, t in enumerate(c.damp[key]):
+            if tuple(t.shape) != tuple(shapes[key]):
+                raise ValueError(f"damping pair {key} of shape {tuple(t.shape)} != {tuple(shapes[key])}")
+            d.damp[2 * q + r] = dev(t)
+    if c.pols is not None:
+        n = int(c.pols["x"][0].numel())
+        d.n_poles = n
+        d.c1 = dev(c.pols["x"][0].reshape(-1))
+        d.c2 = dev(c.pols["x"][1].reshape(-1))
+        for a, ax in enumerate("xyz"):
+            want = (n,) + tuple(fields["E" + ax].shape)
+            for name in (f"P{ax}_cur", f"P{ax}_prev"):
+                t = fields.get(name)
+                if t is None or tuple(t.shape) != want:
+                    raise ValueError(f"{n} 3D poles need {name} of shape {want} in the state")
+            if tuple(c.pols[ax][2].shape) != want:
+                raise ValueError(f"c3 stack for E{ax} of shape {tuple(c.pols[ax][2].shape)} != {want}")
+            d.c3[a] = dev(c.pols[ax][2])
+            d.P_cur[a] = fields[f"P{ax}_cur"].data_ptr()
+            d.P_prev[a] = fields[f"P{ax}_prev"].data_ptr()
+    return keep
+
+
 def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=None, reuse=None, raw=False,
                variant=None):
     """Advance the field dict n_steps in place; returns monitor outputs.
@@ -99,6 +136,11 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=N
         d.psi[q] = fields[name].data_ptr()
     d.tape = tape.data_ptr() if tape is not None else None
     v = DEFAULT_VARIANT if variant is None else int(variant)
+    media = _media(d, c, fields, dtype, device) if has_media(c) else []
+    if media:
+        if lean:
+            raise NotImplementedError("3D media (sigma, poles) are not supported by the lean stepper")
+        v = 0
     alt = []
     if 1 <= v <= 3 and n_steps > 0:
         alt = [torch.empty_like(fields[f]) for f in COMPS] + [torch.zeros_like(fields[f]) for f in PSI_NAMES]
@@ -118,5 +160,5 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=N
                         E.ptr(stage) if stage is not None else None, nbytes, S, step0, n_steps,
                         E.stream_handle())
     E.check(rc, "mt_yee3d_run")
-    del eps, wkeep, keep, alt
+    del eps, wkeep, keep, alt, media
     return outs if raw else finish_outputs(outs, fields)

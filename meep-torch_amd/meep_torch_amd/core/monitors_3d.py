@@ -7,7 +7,7 @@ DFT plane)."""
 import torch
 
 from .engine3d import run_fields
-from .stepper_3d import COMPS, PSI_NAMES
+from .stepper_3d import state_names
 
 
 class PlaneSource:
@@ -139,9 +139,7 @@ def run_3d(state, coeffs, n_steps: int, *, source=None, monitors=()):
     cloned)."""
     if state.psi_Ez_y is None:
         raise NotImplementedError("3D state without CPML memory (absorber path) is not supported")
-    if state.Px_cur is not None:
-        raise NotImplementedError("3D state with ADE poles is not supported")
     with torch.no_grad():
-        work = {f: getattr(state, f).detach().clone().contiguous() for f in COMPS + PSI_NAMES}
+        work = {f: getattr(state, f).detach().clone().contiguous() for f in state_names(state)}
         outs = run_fields(work, coeffs, n_steps, source, list(monitors), lean=False)
     return state._replace(**work), outs

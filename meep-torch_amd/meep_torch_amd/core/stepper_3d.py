@@ -2,9 +2,9 @@
 
 Coeffs3D / State3D keep the reference fields and shapes (State3D psi arrays are
 full-grid; the kernels touch them only inside the CPML shells).  The HIP
-stepper runs the CPML path with sigma = 0 (the configuration the reference's
-Simulation and lean stepper use); the matched-absorber path, material
-conductivity, ADE poles and PMC walls raise NotImplementedError."""
+stepper runs the CPML path, with material conductivity (sigma_d, sigma_b) and
+Lorentz / Drude ADE poles on the two-pass kernels; the matched-absorber path
+and PMC walls raise NotImplementedError."""
 import math
 from collections.abc import Mapping
 from typing import NamedTuple
@@ -76,15 +76,22 @@ def _axis_sides3(v):
 
 
 class _Damp(Mapping):
-    """Per-component (d1, d2) conductivity pairs of stepper_3d.py:226-235,
-    built on first access (all-ones / all-dt at sigma = 0)."""
+    """Per-component (d1, d2) conductivity pairs of stepper_3d.py:217-235,
+    built on first access: d1 = (1 - s dt/2) / (1 + s dt/2),
+    d2 = dt / (1 + s dt/2) of sigma_d averaged along the E component's axis
+    and of sigma_b averaged along the two axes across an H component
+    (all-ones / all-dt at sigma = 0).  ``active`` tells whether material
+    conductivity was given."""
 
-    def __init__(self, eps, dt):
+    def __init__(self, eps, dt, sigma_d=None, sigma_b=None):
         self._eps, self._dt, self._cache = eps, dt, {}
+        self._sd, self._sb = sigma_d, sigma_b
+        self.active = sigma_d is not None or sigma_b is not None
 
     def _pair(self, key):
         e = self._eps
-        s = torch.zeros_like(e)
+        src = self._sd if key[0] == "e" else self._sb
+        s = (src.to(dtype=e.dtype, device=e.device) if src is not None else torch.zeros_like(e))
         dt = self._dt
         avg = {"ex": lambda v: _avg(v, 0), "ey": lambda v: _avg(v, 1), "ez": lambda v: _avg(v, 2),
                "hx": lambda v: _avg(_avg(v, 1), 2), "hy": lambda v: _avg(_avg(v, 0), 2),
@@ -109,13 +116,14 @@ def build_3d_coeffs(nx: int, ny: int, nz: int, dx: float, eps=None, npml=0, n_ab
                     absorber_R0: float = 1e-15, pmc_axes=()) -> Coeffs3D:
     """stepper_3d.py:150-266, CPML subset: dt = 0.5 dx / sqrt(3); per axis an
     H-side profile of length n-1 at offset 1/2 and an E-side profile of
-    length n at offset 0 trimmed to [1:-1]."""
+    length n at offset 0 trimmed to [1:-1].  sigma_d / sigma_b: cell-centred
+    (nx, ny, nz) conductivities (stepper_3d.py:217-235); poles: Lorentz /
+    Drude (fn, gamma, sigma[, fc]) with cell-centred pole_masks, giving per
+    E component c1, c2 of shape (np, 1, 1, 1) and c3 = v3 * (mask averaged
+    along the component's axis) (stepper_3d.py:237-266)."""
+    from .coeffs import pole_recurrence
     if any(l or h for l, h in _axis_sides3(n_abs)):
         raise NotImplementedError("3D matched-sigma absorber (n_abs) is not supported by the HIP stepper")
-    if sigma_d is not None or sigma_b is not None:
-        raise NotImplementedError("3D sigma_d/sigma_b are not supported by the HIP stepper")
-    if len(poles):
-        raise NotImplementedError("3D ADE poles are not supported by the HIP stepper")
     if len(pmc_axes):
         raise NotImplementedError("3D PMC walls are not supported by the HIP stepper")
     npml_ax = _axis_sides3(npml)
@@ -135,15 +143,38 @@ def build_3d_coeffs(nx: int, ny: int, nz: int, dx: float, eps=None, npml=0, n_ab
         b, a, k = b[1:-1].contiguous(), a[1:-1].contiguous(), k[1:-1].contiguous()
         shells[ke] = shell_bounds(b, a)
         cpml[ke] = tuple(_bc(v.to(device), ax) for v in (b, a, k))
-    return Coeffs3D(dt=dt, dx=dx, eps=e, damp=_Damp(e, dt), cpml=cpml, pols=None, pmc=(),
-                    shells=shells)
+    for name, v in (("sigma_d", sigma_d), ("sigma_b", sigma_b)):
+        if v is not None and (not torch.is_tensor(v) or tuple(v.shape) != (nx, ny, nz)):
+            raise ValueError(f"{name} must be a ({nx}, {ny}, {nz}) tensor")
+    pols = None
+    if len(poles):
+        if pole_masks is not None and len(pole_masks) != len(poles):
+            raise ValueError("pole_masks needs one mask per pole")
+        pols = {}
+        rec = [pole_recurrence(tuple(p), dt) for p in poles]
+        n = len(poles)
+        c1 = torch.tensor([r[0] for r in rec], dtype=dtype, device=device).reshape(n, 1, 1, 1)
+        c2 = torch.tensor([r[1] for r in rec], dtype=dtype, device=device).reshape(n, 1, 1, 1)
+        for key, shape, ax in (("x", (nx - 1, ny, nz), 0), ("y", (nx, ny - 1, nz), 1),
+                               ("z", (nx, ny, nz - 1), 2)):
+            c3 = torch.zeros(n, *shape, dtype=dtype, device=device)
+            for q in range(n):
+                m = (pole_masks[q] if pole_masks is not None else
+                     torch.ones(nx, ny, nz, dtype=dtype, device=device))
+                c3[q] = rec[q][2] * _avg(m.to(dtype=dtype, device=device), ax)
+            pols[key] = (c1, c2, c3)
+    return Coeffs3D(dt=dt, dx=dx, eps=e, damp=_Damp(e, dt, sigma_d, sigma_b), cpml=cpml, pols=pols,
+                    pmc=(), shells=shells)
+
+
+def has_media(c: Coeffs3D) -> bool:
+    """True when the coefficients carry conductivity or ADE poles."""
+    return c.pols is not None or bool(getattr(c.damp, "active", False))
 
 
 def init_3d_state(nx: int, ny: int, nz: int, npml: int = 0, n_poles: int = 0,
                   dtype=torch.float64, device="cuda") -> State3D:
-    """stepper_3d.py:269-292."""
-    if n_poles:
-        raise NotImplementedError("3D ADE poles are not supported by the HIP stepper")
+    """stepper_3d.py:269-292 (pole stacks (n_poles, *E component shape))."""
     z = lambda *s: torch.zeros(*s, dtype=dtype, device=device)
     psi = {}
     if npml > 0:
@@ -154,8 +185,20 @@ def init_3d_state(nx: int, ny: int, nz: int, npml: int = 0, n_poles: int = 0,
             psi_Hz_y=z(nx - 1, ny - 2, nz), psi_Hy_z=z(nx - 1, ny, nz - 2),
             psi_Hx_z=z(nx, ny - 1, nz - 2), psi_Hz_x=z(nx - 2, ny - 1, nz),
             psi_Hy_x=z(nx - 2, ny, nz - 1), psi_Hx_y=z(nx, ny - 2, nz - 1))
+    if n_poles:
+        psi.update(Px_cur=z(n_poles, nx - 1, ny, nz), Px_prev=z(n_poles, nx - 1, ny, nz),
+                   Py_cur=z(n_poles, nx, ny - 1, nz), Py_prev=z(n_poles, nx, ny - 1, nz),
+                   Pz_cur=z(n_poles, nx, ny, nz - 1), Pz_prev=z(n_poles, nx, ny, nz - 1))
     return State3D(z(nx - 1, ny, nz), z(nx, ny - 1, nz), z(nx, ny, nz - 1),
                    z(nx, ny - 1, nz - 1), z(nx - 1, ny, nz - 1), z(nx - 1, ny - 1, nz), **psi)
+
+
+POLE_NAMES = ("Px_cur", "Px_prev", "Py_cur", "Py_prev", "Pz_cur", "Pz_prev")
+
+
+def state_names(state: State3D):
+    """Field names a run advances: components, psi and, with poles, the stacks."""
+    return COMPS + PSI_NAMES + (POLE_NAMES if state.Px_cur is not None else ())
 
 
 class _OneStep:
@@ -171,6 +214,6 @@ def step_3d(state: State3D, c: Coeffs3D, src_amp: float = 0.0, src=None) -> Stat
     reference's point-source path, which writes into the new state's tensor in
     place (stepper_3d.py:476-481), the input state is never modified."""
     from .engine3d import run_fields
-    work = {f: getattr(state, f).clone() for f in COMPS + PSI_NAMES}
+    work = {f: getattr(state, f).clone() for f in state_names(state)}
     run_fields(work, c, 1, _OneStep(src_amp, src) if src is not None else None, [], lean=False)
     return state._replace(**work)

@@ -3,9 +3,12 @@
 Follows the reference's 3D adapter, meep_torch/mp/simulation.py:630-766 and
 its 3D dispatch points (:1116-1137, :1372-1375, :1609-1629, :1860, :2119-2127):
 
-* structure: eps of the default material, axis-aligned Blocks voxel-filled
-  over their index box (coords.box3d, meep_torch/mp/coords.py:97-111; no
-  subpixel averaging in 3D), build_3d_coeffs with per-side CPML counts;
+* structure: eps, D_conductivity and B_conductivity of the default material,
+  axis-aligned Blocks voxel-filled over their index box (coords.box3d,
+  meep_torch/mp/coords.py:97-111; no subpixel averaging in 3D),
+  build_3d_coeffs with per-side CPML counts; Lorentz / Drude
+  susceptibilities as ADE poles (the reference's 3D adapter raises on
+  them); chi3 and mu raise;
 * sources: the first source only (:678-714): a zero-size source is a
   PointSource3D, a plane source is a PlaneSource along the axis of its
   smallest extent, full-plane when a transverse extent is infinite,
@@ -131,25 +134,52 @@ class Sim3D:
         self.run_index = 0
 
     # ---- structure -----------------------------------------------------
+    @staticmethod
+    def _check_medium3(m):
+        if m.chi3 != 0.0 or m.mu != 1.0:
+            raise NotImplementedError("chi3 and mu in 3D cells are not supported (the 3D stepper has neither)")
+
     def _init_structure3(self):
+        """Voxel-filled Blocks in painter order (mp/simulation.py:634-677):
+        eps, D_conductivity and B_conductivity volumes, conductivity passed
+        only when some cell has it.  Beyond the reference's 3D adapter, which
+        raises on them, Lorentz / Drude susceptibilities become ADE poles
+        with one mask per (medium, pole): the cells that medium paints."""
         if self._coeffs is not None:
             return
         gs = self.gs
-        if self._default.has_media():
-            raise NotImplementedError("dispersive, nonlinear or conductive media in 3D cells are not supported")
-        bg = float(self._default.resolve())
-        eps = torch.full((gs.nx, gs.ny, gs.nz), bg, dtype=self.dtype, device=self.device)
-        for obj in self._geometry:
+        shape = (gs.nx, gs.ny, gs.nz)
+        bgm = self._default
+        self._check_medium3(bgm)
+        eps = torch.full(shape, float(bgm.resolve()), dtype=self.dtype, device=self.device)
+        sd = torch.full(shape, bgm.D_conductivity, dtype=self.dtype, device=self.device)
+        sb = torch.full(shape, bgm.B_conductivity, dtype=self.dtype, device=self.device)
+        owner = torch.full(shape, -1, dtype=torch.int32, device=self.device)
+        for q, obj in enumerate(self._geometry):
             if not isinstance(obj, G.Block):
                 raise NotImplementedError(f"3D geometry {type(obj).__name__} is not supported (Block only)")
-            if obj.material.has_media():
-                raise NotImplementedError("dispersive, nonlinear or conductive media in 3D cells are not supported")
+            m = obj.material
+            self._check_medium3(m)
             i0, i1, j0, j1, k0, k1 = box3d(obj.center, obj.size, gs)
-            val = float(obj.material.resolve())
-            eps[i0:i1, j0:j1, k0:k1] = val
+            eps[i0:i1, j0:j1, k0:k1] = float(m.resolve())
+            sd[i0:i1, j0:j1, k0:k1] = m.D_conductivity
+            sb[i0:i1, j0:j1, k0:k1] = m.B_conductivity
+            owner[i0:i1, j0:j1, k0:k1] = q
+        has_cond = bool((sd.abs().sum() + sb.abs().sum()).item())
+        poles, masks = [], []
+        for q, m in [(-1, bgm)] + list(enumerate(o.material for o in self._geometry)):
+            if m.E_susceptibilities:
+                cells = (owner == q).to(self.dtype)
+                if bool(cells.any()):
+                    for pole in m.poles():
+                        poles.append(pole)
+                        masks.append(cells)
         self._eps3d = eps
+        self._n_poles = len(poles)
         self._coeffs = build_3d_coeffs(gs.nx, gs.ny, gs.nz, gs.dx, eps=eps,
                                        npml=tuple(tuple(p) for p in self._npml_sides),
+                                       sigma_d=sd if has_cond else None, sigma_b=sb if has_cond else None,
+                                       poles=poles, pole_masks=masks or None,
                                        dtype=self.dtype, device=self.device)
 
     def _make_source3(self):
@@ -214,7 +244,8 @@ class Sim3D:
     # ---- running -------------------------------------------------------
     def _run3(self, n_steps):
         gs = self.gs
-        st = init_3d_state(gs.nx, gs.ny, gs.nz, npml=self._npml, dtype=self.dtype, device=self.device)
+        st = init_3d_state(gs.nx, gs.ny, gs.nz, npml=self._npml, n_poles=self._n_poles, dtype=self.dtype,
+                           device=self.device)
         keys = list(self._monitors)
         mons = [self._monitors[k][1] for k in keys]
         st, outs = run_3d(st, self._coeffs, n_steps, source=self._make_source3(), monitors=mons)

@@ -1,14 +1,16 @@
 """3D Yee restatement (CPU, torch eager) -- test infrastructure only.
 
-Follows meep_torch/core/stepper_3d.py:150-266 (CPML coefficients, sigma = 0),
-stepper_3d.py:363-445 (CPML step), stepper_3d.py:448-481 (sources),
+Follows meep_torch/core/stepper_3d.py:150-266 (CPML coefficients; material
+sigma_d / sigma_b damping pairs and ADE poles, stepper_3d.py:217-266),
+stepper_3d.py:295-301,363-445 (CPML step, folded-D pole update),
+stepper_3d.py:448-481 (sources),
 stepper_3d_lean.py:174-296 (in-place lean form) and monitors_3d.py:18-210.
 """
 import math
 
 import torch
 
-from .yee2d import cpml_profiles
+from .yee2d import cpml_profiles, pole_recurrence
 
 
 def avg(v, ax):
@@ -29,9 +31,41 @@ def _along(v, ax):
     return v.reshape(sh)
 
 
-def coeffs3d(nx, ny, nz, dx, eps, npml, dtype=torch.float32):
+def _media3d(c, nx, ny, nz, dtype, sigma_d, sigma_b, poles, pole_masks):
+    """stepper_3d.py:217-266.  Damping pairs of the Yee-averaged conductivity
+    (E: averaged along the component's own axis; H: along the two others):
+      d1 = (1 - s dt/2) / (1 + s dt/2),  d2 = dt / (1 + s dt/2);
+    poles: per E component (c1, c2) scalars shaped (np, 1, 1, 1) and
+    c3 = v3 * (mask averaged along the component axis), (np, *comp shape)."""
+    dt = c["dt"]
+    z = torch.zeros(nx, ny, nz, dtype=dtype)
+    se = sigma_d.to(dtype) if sigma_d is not None else z
+    sh = sigma_b.to(dtype) if sigma_b is not None else z
+    pair = lambda s: ((1 - s * dt / 2) / (1 + s * dt / 2), dt / (1 + s * dt / 2))
+    c["damp"] = {"Ex": pair(avg(se, 0)), "Ey": pair(avg(se, 1)), "Ez": pair(avg(se, 2)),
+                 "Hx": pair(avg(avg(sh, 1), 2)), "Hy": pair(avg(avg(sh, 0), 2)),
+                 "Hz": pair(avg(avg(sh, 0), 1))}
+    c["pols"] = None
+    if len(poles):
+        rec = [pole_recurrence(p, dt) for p in poles]
+        n = len(poles)
+        c1 = torch.tensor([r[0] for r in rec], dtype=dtype).reshape(n, 1, 1, 1)
+        c2 = torch.tensor([r[1] for r in rec], dtype=dtype).reshape(n, 1, 1, 1)
+        c["pols"] = {}
+        for comp, ax in (("Ex", 0), ("Ey", 1), ("Ez", 2)):
+            c3 = torch.stack([rec[q][2] * avg((pole_masks[q] if pole_masks is not None
+                                              else torch.ones(nx, ny, nz)).to(dtype), ax)
+                              for q in range(n)])
+            c["pols"][comp] = (c1, c2, c3)
+
+
+def coeffs3d(nx, ny, nz, dx, eps, npml, dtype=torch.float32, sigma_d=None, sigma_b=None, poles=(),
+             pole_masks=None):
     """Per-axis CPML profiles: H side length n-1 at offset 1/2, E side length n
-    at offset 0 trimmed to [1:-1] (stepper_3d.py:182-195); dt = dx/(2 sqrt 3)."""
+    at offset 0 trimmed to [1:-1] (stepper_3d.py:182-195); dt = dx/(2 sqrt 3).
+    npml may be 0 (b = 1, a = 0: psi stays zero, PEC box).  With any of
+    sigma_d, sigma_b (cell-centred (nx, ny, nz)) or poles the media
+    coefficients of _media3d are added."""
     dt = 0.5 * dx / math.sqrt(3.0)
     pro = {}
     for ax, n in enumerate((nx, ny, nz)):
@@ -40,9 +74,13 @@ def coeffs3d(nx, ny, nz, dx, eps, npml, dtype=torch.float32):
         b, a, _ = cpml_profiles(n, npml, 0.0, dt, dx, dtype)
         pro[("E", ax)] = (_along(b[1:-1], ax), _along(a[1:-1], ax))
     eps = eps.to(dtype)
-    return {"dt": dt, "dx": dx, "eps": eps, "pro": pro,
-            "eps_c": [avg(eps, 0), avg(eps, 1), avg(eps, 2)],
-            "inv_eps": [1.0 / avg(eps, 0), 1.0 / avg(eps, 1), 1.0 / avg(eps, 2)]}
+    c = {"dt": dt, "dx": dx, "eps": eps, "pro": pro,
+         "eps_c": [avg(eps, 0), avg(eps, 1), avg(eps, 2)],
+         "inv_eps": [1.0 / avg(eps, 0), 1.0 / avg(eps, 1), 1.0 / avg(eps, 2)],
+         "damp": None, "pols": None}
+    if sigma_d is not None or sigma_b is not None or len(poles):
+        _media3d(c, nx, ny, nz, dtype, sigma_d, sigma_b, poles, pole_masks)
+    return c
 
 
 PSI = {  # name: (field differentiated, axis, side)  -- stepper_3d.py:47-58
@@ -57,7 +95,9 @@ COMP_SHAPE = {"Ex": (-1, 0, 0), "Ey": (0, -1, 0), "Ez": (0, 0, -1),
               "Hx": (0, -1, -1), "Hy": (-1, 0, -1), "Hz": (-1, -1, 0)}
 
 
-def state3d(nx, ny, nz, dtype=torch.float32):
+def state3d(nx, ny, nz, dtype=torch.float32, n_poles=0):
+    """Fields and psi (stepper_3d.py:269-292); with n_poles the per-component
+    P_cur / P_prev stacks (n_poles, *E component shape)."""
     z = lambda sh: torch.zeros(*sh, dtype=dtype)
     st = {}
     for c, d in COMP_SHAPE.items():
@@ -66,6 +106,10 @@ def state3d(nx, ny, nz, dtype=torch.float32):
         sh = list(st[f].shape)
         sh[ax] -= 1
         st[name] = z(sh)
+    if n_poles:
+        for a in "xyz":
+            st[f"P{a}_cur"] = z((n_poles,) + tuple(st["E" + a].shape))
+            st[f"P{a}_prev"] = z((n_poles,) + tuple(st["E" + a].shape))
     return st
 
 
@@ -101,9 +145,23 @@ def step3d(st, c, amp=0.0, src=None, lean=False):
     (stepper_3d.py:452-461) / E -= mask*(dt*amp)*inv_eps (lean)."""
     s = dict(st)
     dt = c["dt"]
-    s["Hx"] = st["Hx"] - dt * (_ce(s, c, "psi_Ez_y") - _ce(s, c, "psi_Ey_z"))
-    s["Hy"] = st["Hy"] - dt * (_ce(s, c, "psi_Ex_z") - _ce(s, c, "psi_Ez_x"))
-    s["Hz"] = st["Hz"] - dt * (_ce(s, c, "psi_Ey_x") - _ce(s, c, "psi_Ex_y"))
+    dm, pols = c.get("damp"), c.get("pols")
+    if (dm is not None or pols is not None) and lean:
+        raise ValueError("the lean form has no media")
+    pn = {}
+    if pols is not None:       # stepper_3d.py:318-329: P(n+1) from the incoming E(n)
+        for comp in ("Ex", "Ey", "Ez"):
+            c1, c2, c3 = pols[comp]
+            a = comp[1]
+            pn[comp] = c1 * st[f"P{a}_cur"] + c2 * st[f"P{a}_prev"] + c3 * st[comp]
+            s[f"P{a}_cur"], s[f"P{a}_prev"] = pn[comp], st[f"P{a}_cur"]
+    for comp, pa, pb in (("Hx", "psi_Ez_y", "psi_Ey_z"), ("Hy", "psi_Ex_z", "psi_Ez_x"),
+                         ("Hz", "psi_Ey_x", "psi_Ex_y")):
+        curl = _ce(s, c, pa) - _ce(s, c, pb)
+        if dm is not None:     # stepper_3d.py:375,382,389: H = d1 H - d2 curl E
+            s[comp] = dm[comp][0] * st[comp] - dm[comp][1] * curl
+        else:
+            s[comp] = st[comp] - dt * curl
     # E: curl terms (each psi updated on its full extent, then trimmed)
     for comp, (pa, fa), (pb, fb), axes in (
             ("Ex", ("psi_Hz_y", 2), ("psi_Hy_z", 1), (1, 2)),
@@ -115,6 +173,15 @@ def step3d(st, c, amp=0.0, src=None, lean=False):
         k = "xyz".index(comp[1])
         if lean:
             s[comp] = st[comp] + curl * c["inv_eps"][k] * dt
+        elif pols is not None:  # stepper_3d.py:295-301 (_e_pole)
+            e = c["eps_c"][k]
+            d1, d2 = dm[comp]
+            s[comp] = (d1 * (st[comp] * e + st[f"P{comp[1]}_cur"].sum(0)) + d2 * curl
+                       - pn[comp].sum(0)) / e
+        elif dm is not None:    # stepper_3d.py:402-405: E = (d1 E eps_c + d2 curl) / eps_c
+            e = c["eps_c"][k]
+            d1, d2 = dm[comp]
+            s[comp] = (d1 * st[comp] * e + d2 * curl) / e
         else:
             e = c["eps_c"][k]
             s[comp] = (st[comp] * e + dt * curl) / e

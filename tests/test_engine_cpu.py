@@ -38,15 +38,16 @@ def test_ctypes_structs_match_the_compiled_layout():
     L = E.lib()
     buf = (ctypes.c_longlong * 64)()
     n = L.mt_abi_layout(buf, 64)
-    assert n == 24
+    assert n == 26
     sizes = [E.PmlAxis, E.Source2D, E.TM2D, E.TE2D, E.Yee3D, E.Source3D, E.DftTerm, E.Monitor, E.TM2DAdj,
              E.TE2DAdj, E.Yee3DAdj, BatchItem]
     for q, st in enumerate(sizes):
         assert buf[q] == ctypes.sizeof(st), (st.__name__, buf[q], ctypes.sizeof(st))
     offs = [E.TM2D.hsrc.offset, E.TM2D.hcoef.offset, E.TE2D.tape.offset, E.TE2D.hsrc.offset,
             E.TE2D.hcoef.offset, E.Yee3D.alt.offset, E.Monitor.acc.offset, E.TM2D.n_poles.offset,
-            E.TM2D.h2y.offset, E.TE2D.npx.offset, E.TE2D.hz2.offset, E.TE2D.ky.offset]
-    assert list(buf[12:24]) == offs
+            E.TM2D.h2y.offset, E.TE2D.npx.offset, E.TE2D.hz2.offset, E.TE2D.ky.offset,
+            E.Yee3D.damp.offset, E.Yee3D.P_prev.offset]
+    assert list(buf[12:26]) == offs
 
 
 def test_error_path_without_gpu_work():

@@ -98,3 +98,41 @@ def test_3d_facade_plane_source_and_dft_fields():
     got = sim.get_dft_array(dft, mp.Ey, 0)
     assert np.array_equal(got, a[0] + 1j * a[1])
     assert float(np.abs(got).max()) > 0
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_3d_facade_media_equal_direct_run(dtype):
+    """A Block of conductive Drude metal (D_conductivity plus one Drude pole)
+    and a slab with B_conductivity: the facade's run equals run_3d on
+    sigma_d / sigma_b maps and a pole mask built from the painted cells."""
+    import meep_torch_amd.mp as mp
+    from meep_torch_amd.core.monitors_3d import FluxMonitor3D, PointSource3D, run_3d
+    from meep_torch_amd.core.stepper_3d import build_3d_coeffs, init_3d_state
+    gsrc = mp.GaussianSource(0.8, fwidth=0.4)
+    drude = mp.DrudeSusceptibility(frequency=1.2, gamma=0.2, sigma=2.0)
+    metal = mp.Medium(epsilon=2, D_conductivity=0.5, E_susceptibilities=[drude])
+    lossy = mp.Medium(epsilon=3, B_conductivity=0.3)
+    sim = mp.Simulation(cell_size=mp.Vector3(3.2, 2.4, 2.0), resolution=10, dimensions=3,
+                        geometry=[mp.Block(center=mp.Vector3(0.2, 0, 0), size=mp.Vector3(1.0, 0.8, 0.6),
+                                           material=metal),
+                                  mp.Block(center=mp.Vector3(0.6, 0, 0), size=mp.Vector3(0.2, 1.2, 1.0),
+                                           material=lossy)],
+                        sources=[mp.Source(gsrc, component=mp.Ez, center=mp.Vector3(-1.0, 0, 0), amplitude=2.0)],
+                        boundary_layers=[mp.PML(0.4)], dtype=dtype, device="cuda")
+    fl = sim.add_flux(0.8, 0.4, 5, mp.FluxRegion(center=mp.Vector3(1.0, 0, 0), size=mp.Vector3(0, mp.inf, mp.inf)))
+    sim.run(until=5)
+
+    eps = torch.from_numpy(sim.get_array(component=mp.Dielectric)).to(device="cuda", dtype=dtype)
+    block, slab = (eps == 2.0).to(dtype), (eps == 3.0).to(dtype)
+    assert float(block.sum()) > 0 and float(slab.sum()) > 0
+    c = build_3d_coeffs(32, 24, 20, 0.1, eps=eps, npml=4, sigma_d=0.5 * block, sigma_b=0.3 * slab,
+                        poles=[drude.to_pole()], pole_masks=[block], dtype=dtype, device="cuda")
+    n_steps = int(round(5 / c.dt))
+    src = _Scaled(PointSource3D(c.dt, "Ez", 6, 12, 10, gsrc.profile()), 2.0)
+    freqs = np.linspace(0.8 - 0.4 / 2.0, 0.8 + 0.4 / 2.0, 5).tolist()
+    mon = FluxMonitor3D(c.dt, freqs, normal="x", idx=26, ap=5, dtype=dtype)
+    st, (acc,) = run_3d(init_3d_state(32, 24, 20, npml=4, n_poles=1, dtype=dtype, device="cuda"), c, n_steps,
+                        source=src, monitors=[mon])
+    assert np.array_equal(sim.get_array(component=mp.Ez), st.Ez.cpu().numpy())
+    assert np.array_equal(np.asarray(mp.get_fluxes(fl)), mon.flux(acc, 0.1).detach().cpu().numpy())
+    assert float(st.Px_cur.abs().max()) > 0

@@ -169,13 +169,43 @@ def test_media_rasterization_compositing():
     assert out3["poles"] == [] and out3["chi3"] is None and out3["cond"] is None
 
 
-def test_media_in_3d_raise():
-    """Any medium in a 3D cell raises by name."""
+def test_media_in_3d_cells():
+    """3D cells take D / B conductivity (painted like eps, as the reference's
+    3D adapter does) and Lorentz / Drude poles (one mask per medium and
+    pole, the cells that medium paints last); chi3 and mu raise by name."""
     import meep_torch_amd.mp as mp
-    med = mp.Medium(epsilon=2, D_conductivity=0.1)
-    s3 = mp.Simulation(cell_size=mp.Vector3(1, 1, 1), resolution=10, dimensions=3, device="cpu",
-                       geometry=[mp.Block(size=mp.Vector3(0.5, 0.5, 0.5), material=med)])
-    with pytest.raises(NotImplementedError):
-        s3.init_sim()
+    for med in (mp.Medium(epsilon=2, chi3=0.1), mp.Medium(epsilon=2, mu=2.0)):
+        s3 = mp.Simulation(cell_size=mp.Vector3(1, 1, 1), resolution=10, dimensions=3, device="cpu",
+                           geometry=[mp.Block(size=mp.Vector3(0.5, 0.5, 0.5), material=med)])
+        with pytest.raises(NotImplementedError, match="chi3 and mu"):
+            s3.init_sim()
     with pytest.raises(NotImplementedError):
         mp.LorentzianSusceptibility(frequency=1.0, gamma=0.1, sigma_diag=mp.Vector3(1, 2, 1))
+    drude = mp.DrudeSusceptibility(frequency=2.0, gamma=0.3, sigma=1.5)
+    metal = mp.Medium(epsilon=2, D_conductivity=0.4, E_susceptibilities=[drude])
+    lossy = mp.Medium(epsilon=3, B_conductivity=0.2)
+    s3 = mp.Simulation(cell_size=mp.Vector3(1, 1, 1), resolution=10, dimensions=3, device="cpu",
+                       dtype=torch.float64, boundary_layers=[mp.PML(0.2)],
+                       geometry=[mp.Block(size=mp.Vector3(0.6, 0.6, 0.6), material=metal),
+                                 mp.Block(center=mp.Vector3(0.2, 0, 0), size=mp.Vector3(0.2, 1, 1),
+                                          material=lossy)])
+    s3.init_sim()
+    c = s3._coeffs
+    assert c.pols is not None and c.pols["x"][0].numel() == 1
+    # cell-centred maps recovered from the pole mask and eps: the Drude block minus the slab
+    eps = s3._eps3d
+    slab = eps == 3.0
+    block = (eps == 2.0)
+    assert int(block.sum()) > 0 and int(slab.sum()) > 0
+    dt = c.dt
+    v3 = drude.to_pole()
+    from meep_torch_amd.core.coeffs import pole_recurrence
+    r3 = pole_recurrence(v3, dt)[2]
+    assert torch.equal(c.pols["z"][2][0], r3 * (0.5 * (block.double()[:, :, :-1] + block.double()[:, :, 1:])))
+    sd = block.double() * 0.4
+    sb = slab.double() * 0.2
+    a = 0.5 * (sd[:-1] + sd[1:])
+    assert torch.equal(c.damp["ex"][0], (1 - a * dt / 2) / (1 + a * dt / 2))
+    b = 0.5 * (sb[:, :-1] + sb[:, 1:])
+    b = 0.5 * (b[:, :, :-1] + b[:, :, 1:])
+    assert torch.equal(c.damp["hx"][1], dt / (1 + b * dt / 2))
