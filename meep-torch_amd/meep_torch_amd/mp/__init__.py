@@ -4,8 +4,9 @@ DFT-field monitors, step functions and stop conditions (meep_torch/mp)."""
 from .geometry import PML, Absorber, Block, DrudeSusceptibility, GeometricObject, LorentzianSusceptibility, Medium
 from .run_control import (FluxData, after_sources, after_time, at_beginning, at_end, at_every, at_time,
                           before_time, during_sources, stop_after_walltime, stop_when_fields_decayed)
-from .simulation import DftFields, DftFlux, FluxRegion, Simulation, Volume, get_flux_freqs, get_fluxes
-from .sources import ContinuousSource, GaussianSource, Source
+from .simulation import (DftFields, DftFlux, EigCoeffsResult, FluxRegion, Simulation, Volume, get_flux_freqs,
+                         get_fluxes)
+from .sources import AUTOMATIC, ContinuousSource, EigenModeSource, GaussianSource, Source
 from .vec import (ALL, Dielectric, Dz, Ex, Ey, Ez, High, Hx, Hy, Hz, Low, Vector3, X, Y, Z, inf)
 
 __all__ = ["PML", "Absorber", "Block", "DrudeSusceptibility", "GeometricObject", "LorentzianSusceptibility", "Medium", "DftFields", "DftFlux", "FluxRegion",
@@ -13,4 +14,4 @@ __all__ = ["PML", "Absorber", "Block", "DrudeSusceptibility", "GeometricObject",
            "Source", "ALL", "Dielectric", "Dz", "Ex", "Ey", "Ez", "High", "Hx", "Hy", "Hz", "Low", "Vector3",
            "X", "Y", "Z", "inf", "FluxData", "after_sources", "after_time", "at_beginning", "at_end",
            "at_every", "at_time", "before_time", "during_sources", "stop_after_walltime",
-           "stop_when_fields_decayed"]
+           "stop_when_fields_decayed", "AUTOMATIC", "EigenModeSource", "EigCoeffsResult"]

@@ -20,6 +20,8 @@ split the run into engine calls at the points where a hook must look at the
 fields.  Symmetries, k-points, multi-source 2D runs and non-Block geometry
 raise NotImplementedError.
 """
+from collections import namedtuple
+
 import numpy as np
 import torch
 
@@ -33,7 +35,7 @@ from ..core.stepper_tm import fields_of as tm_fields_of, init_tm_state
 from . import geometry as G
 from . import run_control as RC
 from .sim3d import Sim3D
-from .sources import source_span
+from .sources import AUTOMATIC, EigenModeSource, port_line, source_span
 from .vec import Vector3, pol_for
 
 _DEFAULTS = dict(eps_averaging=True, symmetries=None, force_complex_fields=False, m=0, k_point=False,
@@ -75,6 +77,10 @@ class DftFields(_Dft):
 
 def _name(f):
     return f.name if isinstance(f, _Dft) else f
+
+
+# meep's get_eigenmode_coefficients result (meep_torch/mp/run_control.py:193-195)
+EigCoeffsResult = namedtuple("EigCoeffsResult", ["alpha", "vgrp", "kpoints", "kdom", "cscale"])
 
 
 class Simulation(Sim3D):
@@ -120,6 +126,12 @@ class Simulation(Sim3D):
         self._raster = None
         self._coeffs = None
         self.run_index = 0
+        # S-parameter path (mp/simulation.py:353-355,1305-1338,1766-1789)
+        self._eig_source = next((q for q in self._sources if isinstance(q, EigenModeSource)), None)
+        self._mode_ports = []
+        self._mode_freqs = None
+        self._port_index = {}
+        self._S = None
 
     # ---- structure -----------------------------------------------------
     def _init_structure(self):
@@ -217,6 +229,80 @@ class Simulation(Sim3D):
             self._flux_parts[name] = parts
         return DftFlux(self, name, freqs)
 
+    def add_mode_monitor(self, fcen, df, nfreq, *regions, name=None):
+        """meep add_mode_monitor (mp/simulation.py:1766-1789): with an
+        EigenModeSource a mode port of the S-parameter run, otherwise a flux
+        monitor."""
+        if self.dimensions == 3:
+            raise NotImplementedError("3D EigenModeSource / S-parameters are not supported")
+        if len(regions) != 1:
+            raise NotImplementedError("add_mode_monitor takes exactly one region")
+        if self._eig_source is None:
+            return self.add_flux(fcen, df, nfreq, *regions, name=name)
+        freqs = np.linspace(fcen - df / 2.0, fcen + df / 2.0, nfreq).tolist()
+        region = regions[0]
+        name = name or f"mode{len(self._mode_ports)}"
+        self._mode_ports.append((name,) + port_line(region.center, region.size, self.gs) + (region.direction,))
+        self._mode_freqs = freqs
+        return DftFlux(self, name, freqs)
+
+    def _run_modes(self, n_steps):
+        """mp/simulation.py:1304-1338: the EigenModeSource's port is port 0;
+        each mode monitor becomes a port (one on the source's own line maps
+        to port 0, the reflection), and the run is compute_smatrix's
+        per-port launch sequence, kept as S."""
+        from ..ports import Port, compute_smatrix
+        es = self._eig_source
+        if self._pol != "tm":
+            raise NotImplementedError("EigenModeSource in a TE cell is not supported by the facade")
+        if self._n_poles or self._coeffs.d1 is not None or self._coeffs.has_kerr:
+            raise NotImplementedError("EigenModeSource S-parameters in a cell with dispersive, nonlinear or "
+                                      "conductive media are not supported")
+        if es.eig_band != 1:
+            raise NotImplementedError("EigenModeSource eig_band > 1: the S-parameter path launches band 1")
+        axis, idx, lo, hi = port_line(es.center, es.size, self.gs)
+        ports = [Port(axis=axis, index=idx, lo=lo, hi=hi, direction=1 if es.direction == AUTOMATIC else es.direction)]
+        self._port_index = {}
+        for name, ax, i, l, h, d in self._mode_ports:
+            if (ax, i, l, h) == (axis, idx, lo, hi):
+                self._port_index[name] = 0
+                continue
+            self._port_index[name] = len(ports)
+            ports.append(Port(axis=ax, index=i, lo=l, hi=h, direction=d))
+        src = es.src
+        fcen, fwidth = float(src.frequency), 1.0 / float(getattr(src, "width", 0) or 10.0)
+        eps = G.to_tensor(self._raster[0], self.grid.dtype, self.grid.device)
+        npml = max(max(p) for p in self._npml_sides)
+        self._S = compute_smatrix(eps, self.grid, ports, pol="tm", fcen=fcen, fwidth=fwidth,
+                                  freqs=self._mode_freqs or [fcen], n_steps=n_steps, npml=npml).s
+
+    def get_eigenmode_coefficients(self, flux, bands, eig_parity=0, eig_vol=None, eig_resolution=0,
+                                   eig_tolerance=1e-12, kpoint_func=None, direction=AUTOMATIC):
+        """meep get_eigenmode_coefficients on a mode port of an EigenModeSource
+        run (mp/simulation.py:1928-1985): alpha (1, nfreq, 2) whose forward
+        slot holds the S-parameter column S[:, port, 0] (the source port's
+        own line reports the reflection) and whose backward slot is zero.
+        The plane-wave decomposition of Bloch-periodic flux planes is not
+        built."""
+        for nm, v, ok in (("eig_parity", eig_parity, 0), ("eig_vol", eig_vol, None),
+                          ("kpoint_func", kpoint_func, None)):
+            if v != ok:
+                raise NotImplementedError(f"get_eigenmode_coefficients {nm} is not supported")
+        if direction not in (AUTOMATIC, None):
+            raise NotImplementedError("get_eigenmode_coefficients direction: the monitor normal only")
+        if not isinstance(bands, (list, tuple, range)):
+            raise TypeError("get_eigenmode_coefficients: bands must be a list")
+        name = _name(flux)
+        if self._S is None or name not in self._port_index:
+            raise NotImplementedError(f"get_eigenmode_coefficients({name!r}): only mode ports of an "
+                                      "EigenModeSource run are decomposed (no plane-wave branch)")
+        if list(bands) != [1]:
+            raise NotImplementedError(f"get_eigenmode_coefficients(bands={list(bands)}): mode ports carry band 1")
+        col = self._S[:, self._port_index[name], 0].detach().cpu().numpy()
+        alpha = np.zeros((1, col.shape[0], 2), dtype=np.complex128)
+        alpha[0, :, 0] = col
+        return EigCoeffsResult(alpha, np.zeros(col.shape[0]), [], [], np.zeros(col.shape[0]))
+
     def add_dft_fields(self, components, fcen, df, nfreq, where=None, center=None, size=None, name=None,
                        yee_grid=False, decimation_factor=0):
         freqs = np.linspace(fcen - df / 2.0, fcen + df / 2.0, nfreq).tolist()
@@ -253,6 +339,16 @@ class Simulation(Sim3D):
                     self._source_cutoff() + until_after_sources if until_after_sources is not None else 0)
                 n_steps = max(1, int(round(float(T) / self._coeffs.dt)))
             self._run3(int(n_steps))
+            return
+        if self._eig_source is not None:
+            if step_funcs or callable(until) or callable(until_after_sources) or \
+                    isinstance(until, (list, tuple)) or isinstance(until_after_sources, (list, tuple)):
+                raise NotImplementedError("step functions / stop conditions with an EigenModeSource")
+            if n_steps is None:
+                T = until if until is not None else self._source_cutoff() + float(until_after_sources or 0.0)
+                n_steps = max(1, int(round(float(T) / self.grid.dt)))
+            self._run_modes(int(n_steps))
+            self._t_step += int(n_steps)
             return
         self._run_until(step_funcs, until, until_after_sources, n_steps)
 
@@ -347,6 +443,8 @@ class Simulation(Sim3D):
 
     def get_flux_freqs(self, f):
         name = _name(f)
+        if any(p[0] == name for p in getattr(self, "_mode_ports", ())):   # a mode port of the S-parameter path
+            return list(self._mode_freqs)
         parts = self._flux_parts.get(name)
         return self._monitors[parts[0] if parts else name][1].freqs.detach().cpu().numpy().tolist()
 

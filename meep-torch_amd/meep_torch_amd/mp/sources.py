@@ -67,6 +67,49 @@ class Source:
         self.amplitude = complex(amplitude).real
 
 
+AUTOMATIC = -1      # meep: launch / decompose along the region normal, forward
+
+
+class EigenModeSource(Source):
+    """meep EigenModeSource (meep_torch/mp/sources.py:424-470): a port mode
+    launched by the S-parameter path (EigenModeSource + add_mode_monitor ->
+    ports.compute_smatrix).  The propagation axis is the axis of the region's
+    smallest extent; eig_band is 1-based as in meep.  Knobs with no engine
+    path raise by name."""
+
+    def __init__(self, src, center=Vector3(), size=Vector3(0, 1, 0), component="Ez", direction=AUTOMATIC,
+                 eig_band=1, amplitude=1.0, eig_kpoint=None, eig_match_freq=True, eig_parity=0, eig_vol=None,
+                 eig_resolution=0, eig_tolerance=1e-12, eig_lattice_size=None, eig_lattice_center=None,
+                 volume=None):
+        if eig_band < 1:
+            raise ValueError(f"eig_band is 1-based like meep (1 = fundamental mode), got {eig_band}")
+        for name, v, ok in (("eig_kpoint", eig_kpoint, None), ("eig_vol", eig_vol, None),
+                            ("eig_lattice_size", eig_lattice_size, None),
+                            ("eig_lattice_center", eig_lattice_center, None), ("eig_parity", eig_parity, 0)):
+            if v != ok:
+                raise NotImplementedError(f"EigenModeSource {name} is not supported (modes are solved on the "
+                                          "port line along its normal)")
+        if eig_match_freq is not True:
+            raise NotImplementedError("EigenModeSource eig_match_freq=False (fixed-k mode) is not supported")
+        if volume is not None:
+            center, size = volume.center, volume.size
+        super().__init__(src, component, center=center, size=size, amplitude=amplitude)
+        self.direction, self.eig_band = direction, int(eig_band)
+
+
+def port_line(center, size, gs):
+    """(axis, idx, lo, hi) of a port or mode-monitor line (meep_torch/mp/
+    sources.py:510-529): normal to the region's smallest extent, spanning
+    the other extent clipped to the cell."""
+    c, s = center, size
+    if abs(s.x) <= abs(s.y):
+        y0, y1 = c.y - s.y / 2 + gs.sy / 2, c.y + s.y / 2 + gs.sy / 2
+        return ("x", cell_index(c.x, gs)) + tuple(span_index_range(max(0.0, y0), min(gs.sy, y1), gs, "y"))
+    x0, x1 = c.x - s.x / 2 + gs.sx / 2, c.x + s.x / 2 + gs.sx / 2
+    return (("y", max(0, min(gs.ny, cell_index_y(c.y, gs))))
+            + tuple(span_index_range(max(0.0, x0), min(gs.sx, x1), gs, "x")))
+
+
 def source_span(spec, gs):
     """(ix, j0, j1): x-normal line (or point) of the source (meep_torch/mp/sources.py:484-501)."""
     c = spec.center

@@ -151,3 +151,33 @@ def test_te_straight_guide():
     for i, (t21, t11) in enumerate(((5e-2, 1e-1), (2e-2, 5e-2), (2e-2, 5e-2))):
         assert abs(abs(st.s[i, 1, 0]) - 1.0) < t21 and abs(st.s[i, 0, 0]) < t11
         assert abs(abs(st.s[i, 1, 0]) - abs(st.s[i, 0, 1])) < 1e-6
+
+
+def test_facade_eigenmode_source_equals_compute_smatrix():
+    """meep-syntax S-parameters (EigenModeSource + add_mode_monitor +
+    get_eigenmode_coefficients, mp/simulation.py:1304-1338,1928-1985) on a
+    Block-built straight guide equal compute_smatrix on the same eps and
+    ports, and match meep's |S21|."""
+    import meep_torch_amd.mp as mp
+    from meep_torch_amd.ports import Port, compute_smatrix
+    src = mp.GaussianSource(0.15, fwidth=0.1)
+    sim = mp.Simulation(cell_size=mp.Vector3(20, 8), resolution=10, device="cuda",
+                        geometry=[mp.Block(size=mp.Vector3(mp.inf, 1.0), material=mp.Medium(epsilon=12))],
+                        sources=[mp.EigenModeSource(src, center=mp.Vector3(-6, 0), size=mp.Vector3(0, 4),
+                                                    direction=1)],
+                        boundary_layers=[mp.PML(1.0)])
+    m_in = sim.add_mode_monitor(0.15, 0.04, 3, mp.FluxRegion(center=mp.Vector3(-6, 0), size=mp.Vector3(0, 4)))
+    m_out = sim.add_mode_monitor(0.15, 0.04, 3, mp.FluxRegion(center=mp.Vector3(6, 0), size=mp.Vector3(0, 4),
+                                                              direction=-1))
+    sim.run(n_steps=3000)
+    t = sim.get_eigenmode_coefficients(m_out, [1]).alpha
+    r = sim.get_eigenmode_coefficients(m_in, [1]).alpha
+    assert t.shape == (1, 3, 2) and not t[..., 1].any()
+    g, eps = _guide(200, 80)
+    assert torch.equal(torch.from_numpy(sim.get_array(component=mp.Dielectric)).cuda(), eps)
+    sm = compute_smatrix(eps, g, [Port("x", 40, 20, 60, 1), Port("x", 160, 20, 60, -1)], pol="tm", fcen=0.15,
+                         fwidth=0.1, freqs=mp.get_flux_freqs(m_out), n_steps=3000, npml=10)
+    assert torch.equal(torch.from_numpy(t[0, :, 0]), sm.s[:, 1, 0])
+    assert torch.equal(torch.from_numpy(r[0, :, 0]), sm.s[:, 0, 0])
+    for i in range(3):
+        assert abs(abs(t[0, i, 0]) - MEEP["straight"]["S21_mag"][i]) < 0.03

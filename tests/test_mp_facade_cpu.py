@@ -209,3 +209,42 @@ def test_media_in_3d_cells():
     b = 0.5 * (sb[:, :-1] + sb[:, 1:])
     b = 0.5 * (b[:, :, :-1] + b[:, :, 1:])
     assert torch.equal(c.damp["hx"][1], dt / (1 + b * dt / 2))
+
+
+def test_eigenmode_source_port_mapping_and_refusals():
+    """EigenModeSource / add_mode_monitor set-up (mp/sources.py:510-529,
+    mp/simulation.py:1766-1789): a line normal to the region's smallest
+    extent; a monitor on the source's line is the reflection port; options
+    with no engine path raise by name."""
+    import meep_torch_amd.mp as mp
+    from meep_torch_amd.mp.sources import port_line
+    gs_sim = mp.Simulation(cell_size=mp.Vector3(20, 8), resolution=10, device="cpu")
+    gs = gs_sim.gs
+    assert port_line(mp.Vector3(-6, 0), mp.Vector3(0, 4), gs) == ("x", 40, 20, 60)
+    assert port_line(mp.Vector3(0, 2), mp.Vector3(4, 0), gs) == ("y", 60, 80, 120)
+    src = mp.GaussianSource(0.15, fwidth=0.1)
+    with pytest.raises(ValueError, match="1-based"):
+        mp.EigenModeSource(src, eig_band=0)
+    with pytest.raises(NotImplementedError, match="eig_kpoint"):
+        mp.EigenModeSource(src, eig_kpoint=mp.Vector3(1, 0))
+    with pytest.raises(NotImplementedError, match="eig_match_freq"):
+        mp.EigenModeSource(src, eig_match_freq=False)
+    sim = mp.Simulation(cell_size=mp.Vector3(20, 8), resolution=10, device="cpu",
+                        sources=[mp.EigenModeSource(src, center=mp.Vector3(-6, 0), size=mp.Vector3(0, 4),
+                                                    direction=1)],
+                        boundary_layers=[mp.PML(1.0)])
+    f = sim.add_mode_monitor(0.15, 0.04, 3, mp.FluxRegion(center=mp.Vector3(-6, 0), size=mp.Vector3(0, 4)))
+    sim.add_mode_monitor(0.15, 0.04, 3, mp.FluxRegion(center=mp.Vector3(6, 0), size=mp.Vector3(0, 4),
+                                                      direction=-1))
+    assert sim._mode_ports == [("mode0", "x", 40, 20, 60, None), ("mode1", "x", 160, 20, 60, -1)]
+    assert mp.get_flux_freqs(f) == pytest.approx([0.13, 0.15, 0.17])
+    with pytest.raises(NotImplementedError, match="mode ports"):
+        sim.get_eigenmode_coefficients(f, [1])
+    with pytest.raises(NotImplementedError, match="step functions"):
+        sim.run(mp.at_end(lambda s: None), n_steps=10)
+    # without an EigenModeSource a mode monitor is a flux monitor
+    plain = mp.Simulation(cell_size=mp.Vector3(20, 8), resolution=10, device="cpu",
+                          sources=[mp.Source(src, component=mp.Ez, center=mp.Vector3(-6, 0))],
+                          boundary_layers=[mp.PML(1.0)])
+    m = plain.add_mode_monitor(0.15, 0.04, 3, mp.FluxRegion(center=mp.Vector3(6, 0), size=mp.Vector3(0, 4)))
+    assert plain._monitors[m.name][0] == "flux"
