@@ -253,9 +253,8 @@ class Simulation(Sim3D):
         per-port launch sequence, kept as S."""
         from ..ports import Port, compute_smatrix
         es = self._eig_source
-        if self._pol != "tm":
-            raise NotImplementedError("EigenModeSource in a TE cell is not supported by the facade")
-        if self._n_poles or self._coeffs.d1 is not None or self._coeffs.has_kerr:
+        c = self._coeffs
+        if self._n_poles or (c.d1 if self._pol == "tm" else c.d1x) is not None or c.has_kerr:
             raise NotImplementedError("EigenModeSource S-parameters in a cell with dispersive, nonlinear or "
                                       "conductive media are not supported")
         if es.eig_band != 1:
@@ -271,15 +270,18 @@ class Simulation(Sim3D):
             ports.append(Port(axis=ax, index=i, lo=l, hi=h, direction=d))
         src = es.src
         fcen, fwidth = float(src.frequency), 1.0 / float(getattr(src, "width", 0) or 10.0)
-        eps = G.to_tensor(self._raster[0], self.grid.dtype, self.grid.device)
+        # TM: the rasterised eps; TE: its xx map, which compute_smatrix
+        # averages onto the Ey nodes itself (mp/simulation.py:1335)
+        eps = G.to_tensor(self._raster[0 if self._pol == "tm" else 1], self.grid.dtype, self.grid.device)
         npml = max(max(p) for p in self._npml_sides)
-        self._S = compute_smatrix(eps, self.grid, ports, pol="tm", fcen=fcen, fwidth=fwidth,
+        self._S = compute_smatrix(eps, self.grid, ports, pol=self._pol, fcen=fcen, fwidth=fwidth,
                                   freqs=self._mode_freqs or [fcen], n_steps=n_steps, npml=npml).s
 
     def get_eigenmode_coefficients(self, flux, bands, eig_parity=0, eig_vol=None, eig_resolution=0,
                                    eig_tolerance=1e-12, kpoint_func=None, direction=AUTOMATIC):
         """meep get_eigenmode_coefficients on a mode port of an EigenModeSource
-        run (mp/simulation.py:1928-1985): alpha (1, nfreq, 2) whose forward
+        run (mp/simulation.py:1928-1985), TM or TE (x-normal ports) cells:
+        alpha (1, nfreq, 2) whose forward
         slot holds the S-parameter column S[:, port, 0] (the source port's
         own line reports the reflection) and whose backward slot is zero.
         The plane-wave decomposition of Bloch-periodic flux planes is not

@@ -181,3 +181,26 @@ def test_facade_eigenmode_source_equals_compute_smatrix():
     assert torch.equal(torch.from_numpy(r[0, :, 0]), sm.s[:, 0, 0])
     for i in range(3):
         assert abs(abs(t[0, i, 0]) - MEEP["straight"]["S21_mag"][i]) < 0.03
+
+
+def test_facade_eigenmode_source_te_cell():
+    """A TE cell (Hz source component): the facade hands compute_smatrix its
+    xx eps map, which for a cell-aligned guide is the plain eps map."""
+    import meep_torch_amd.mp as mp
+    from meep_torch_amd.ports import Port, compute_smatrix
+    src = mp.GaussianSource(0.15, fwidth=0.1)
+    sim = mp.Simulation(cell_size=mp.Vector3(20, 8), resolution=10, device="cuda",
+                        geometry=[mp.Block(size=mp.Vector3(mp.inf, 1.0), material=mp.Medium(epsilon=12))],
+                        sources=[mp.EigenModeSource(src, component=mp.Hz, center=mp.Vector3(-6, 0),
+                                                    size=mp.Vector3(0, 4), direction=1)],
+                        boundary_layers=[mp.PML(1.0)])
+    sim.add_mode_monitor(0.15, 0.04, 3, mp.FluxRegion(center=mp.Vector3(-6, 0), size=mp.Vector3(0, 4)))
+    m_out = sim.add_mode_monitor(0.15, 0.04, 3, mp.FluxRegion(center=mp.Vector3(6, 0), size=mp.Vector3(0, 4),
+                                                              direction=-1))
+    sim.run(n_steps=3000)
+    t = sim.get_eigenmode_coefficients(m_out, [1]).alpha
+    g, eps = _guide(200, 80)
+    sm = compute_smatrix(eps, g, [Port("x", 40, 20, 60, 1), Port("x", 160, 20, 60, -1)], pol="te", fcen=0.15,
+                         fwidth=0.1, freqs=mp.get_flux_freqs(m_out), n_steps=3000, npml=10)
+    assert torch.equal(torch.from_numpy(t[0, :, 0]), sm.s[:, 1, 0])
+    assert abs(abs(t[0, 1, 0]) - 1.0) < 2e-2 and abs(abs(t[0, 2, 0]) - 1.0) < 2e-2
