@@ -12,10 +12,10 @@ import torch
 
 from ..core.coeffs import build_te_coeffs, build_tm_coeffs
 from ..core.monitors import FluxMonitorTE, FluxMonitorTM
-from ..core.run import run_te, run_tm
+from ..core.run import run_te, run_tm, run_until_decayed
 from ..core.sources import GaussianPulse
-from ..core.stepper_te import init_te_state
-from ..core.stepper_tm import init_tm_state
+from ..core.stepper_te import init_te_state, step_te
+from ..core.stepper_tm import init_tm_state, step_tm
 from .mode_monitor import decompose
 from .mode_source import ModeSourceTE, ModeSourceTM
 from .port import validate_ports
@@ -30,9 +30,10 @@ class SMatrix(NamedTuple):
 def compute_smatrix(eps, grid, ports, *, pol, fcen, fwidth, freqs, n_steps, npml, checkpoint_every=0,
                     decay_probe=None, decay_ratio=1e-6, n_decay=50):
     """eps: cell-centred (nx, ny) map; a Gaussian pulse (fcen, fwidth) drives
-    every run for n_steps steps."""
-    if decay_probe is not None:
-        raise NotImplementedError("compute_smatrix(decay_probe=...): fixed n_steps runs only")
+    every run for n_steps steps, or, with decay_probe = (ix, iy), until the
+    squared field (Ez for TM, Hz for TE) at that cell stays below
+    decay_ratio times its peak for n_decay consecutive steps, n_steps at
+    most (core/run.py:run_until_decayed)."""
     validate_ports(ports, grid, npml)
     if pol == "tm":
         coeffs = build_tm_coeffs(grid, eps_inf=eps, npml=npml)
@@ -55,8 +56,15 @@ def compute_smatrix(eps, grid, ports, *, pol, fcen, fwidth, freqs, n_steps, npml
     s = torch.zeros(len(freqs), len(index), len(index), dtype=torch.complex128)
     for col, (j, n) in enumerate(index):
         src = src_cls(grid, coeffs, ports[j], GaussianPulse(fcen, fwidth), fcen, mode=n)
-        _, accs = run(init(grid), coeffs, n_steps, source=src, monitors=[monitor(p) for p in ports],
-                      checkpoint_every=checkpoint_every)
+        mons = [monitor(p) for p in ports]
+        if decay_probe is not None:
+            ix, iy = decay_probe
+            f = "Ez" if pol == "tm" else "Hz"
+            _, accs = run_until_decayed(step_tm if pol == "tm" else step_te, init(grid), coeffs, src, mons,
+                                        lambda st: getattr(st, f)[ix, iy] * getattr(st, f)[ix, iy],
+                                        decay_ratio=decay_ratio, n_decay=n_decay, max_steps=n_steps)
+        else:
+            _, accs = run(init(grid), coeffs, n_steps, source=src, monitors=mons, checkpoint_every=checkpoint_every)
         amps = [decompose(acc, line(p), grid.dx, freqs, pol, p.n_modes, axis_sign=sign(p))
                 for p, acc in zip(ports, accs)]
         a_p, a_m = amps[j]

@@ -218,3 +218,23 @@ def test_te_near_cutoff_shortfall_is_the_port_line_truncation():
     print("\n[te wide] |S21|", st.s[:, 1, 0].abs().tolist(), "|S11|", st.s[:, 0, 0].abs().tolist())
     for i in range(3):
         assert abs(abs(st.s[i, 1, 0]) - 1.0) < 2e-2 and abs(st.s[i, 0, 0]) < 5e-2
+
+
+def test_smatrix_until_decayed_wiring():
+    """decay_probe runs each launch through run_until_decayed, one engine
+    call per step with the TFSF currents and the monitors' global clock.
+    With a decay ratio of 0 the stop never triggers, so the S-matrix must be
+    the fixed-length run's (the DFT sums run in the same step order).  A
+    probe placed in the guide can stop a launch early, in the quiet between
+    the source's fast radiation and the guided pulse, as the reference's
+    criterion does; that case is not compared."""
+    from meep_torch_amd.ports import Port, compute_smatrix
+    g, eps = _guide(200, 80)
+    ports = [Port("x", 40, 20, 60, 1), Port("x", 160, 20, 60, -1)]
+    fixed = compute_smatrix(eps, g, ports, pol="tm", fcen=FCEN, fwidth=FWIDTH, freqs=FREQS, n_steps=1500,
+                            npml=10)
+    dec = compute_smatrix(eps, g, ports, pol="tm", fcen=FCEN, fwidth=FWIDTH, freqs=FREQS, n_steps=1500,
+                          npml=10, decay_probe=(170, 40), decay_ratio=0.0)
+    err = float((dec.s - fixed.s).abs().max())
+    print(f"\n[decay wiring] max |S_decay - S_fixed| {err:.2e}")
+    assert err < 1e-12
