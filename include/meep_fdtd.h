@@ -276,8 +276,10 @@ typedef struct {
  * run with term views pointing into the ADJOINT fields and `acc` holding the
  * accumulator cotangent dL/dacc; each step injects
  *   dt*(cos(w t) dacc_re - sin(w t) dacc_im)  (halved onto both rows of an
- * averaged term).  Replaces PyTorch autograd through run_tm's checkpointed
- * chunks (meep_torch/core/run.py:99-120, eps reached via stepper_tm.py:226-227). */
+ * averaged term).  Conductivity (d1, d2, h1x, h2x, h1y, h2y of p, all or none)
+ * is transposed with the recurrences; poles and Kerr are refused.  Replaces
+ * PyTorch autograd through run_tm's checkpointed chunks
+ * (meep_torch/core/run.py:99-120, eps reached via stepper_tm.py:226-227). */
 int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const void* tape,
                     const mt_monitor* mons, int n_mons, int step0, int k, void* stream);
 

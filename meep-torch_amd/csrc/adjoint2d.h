@@ -18,6 +18,10 @@ struct AdjK {
   T *aEz, *aDz, *aHx, *aHy, *apzy, *apzx, *apyx, *apxy;
   T *gHy, *gHx, *gEx, *gEy;
   T* eps_grad;
+  // conductivity (sigma_d, sigma_b, mu): Dz' = d1 Dz + d2 curl,
+  // Hx' = h1x Hx - h2x curl, Hy' = h1y Hy + h2y curl; NULL: d1 = h1 = 1,
+  // d2 = h2 = dt
+  const T *d1, *d2, *h1x, *h2x, *h1y, *h2y;
 };
 
 // Each pass below issues all of a cell's loads before its first store: the
@@ -35,6 +39,7 @@ __device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, 
   const int ii = i - 1, jj = p.y_pmc ? j : j - 1;
   const long long q = (long long)ii * p.ny_e + jj;
   const T eps = p.eps[c], ezb = p.aEz[c], eg = p.eps_grad[c], ea = ez_after[c], dz0 = p.aDz[c];
+  const T d1 = p.d1 ? p.d1[c] : T(1), d2 = p.d1 ? p.d2[c] : p.dt;
   const bool sx = live && p.ex.in(ii), sy = live && p.ey.in(jj);
   T pyx = T(0), ax = T(0), bx = T(0), pxy = T(0), ay = T(0), by = T(0);
   if (sx) {
@@ -49,11 +54,11 @@ __device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, 
   }
   p.eps_grad[c] = eg + (-ezb * ea) / eps;
   T dzb = dz0 + ezb / eps;
-  p.aDz[c] = dzb;
+  p.aDz[c] = p.d1 ? d1 * dzb : dzb;          // Dz' = d1 Dz + d2 curl
   p.aEz[c] = T(0);
   T ghy = T(0), ghx = T(0);
   if (live) {
-    T cb = p.dt * dzb;
+    T cb = d2 * dzb;
     ghy = cb;
     if (sx) {
       T pt = pyx + cb;
@@ -102,12 +107,19 @@ __device__ __forceinline__ void adj_b_cell(const AdjK<T>& p, int i, int j) {
     a_y = p.hy.a[j];
     b_y = p.hy.b[j];
   }
+  // Hy' = h1y Hy + h2y curl, Hx' = h1x Hx - h2x curl (h1 = 1, h2 = dt without
+  // conductivity)
+  T h1y = T(1), h2y = p.dt, h1x = T(1), h2x = p.dt;
+  if (p.h1y) {
+    if (fy) h1y = p.h1y[hy], h2y = p.h2y[hy];
+    if (fx) h1x = p.h1x[hx], h2x = p.h2x[hx];
+  }
   if (fy) {
     T hb = ahy + (gy0 - gy1) / p.dx;
-    p.aHy[hy] = hb;
-    T gb = p.dt * hb;
+    p.aHy[hy] = p.h1y ? h1y * hb : hb;
+    T gb = h2y * hb;
     if (sx) {
-      T pt = pzx + p.dt * hb;
+      T pt = pzx + h2y * hb;
       gb = gb + a_x * pt;
       p.apzx[hy] = b_x * pt;
     }
@@ -115,10 +127,10 @@ __device__ __forceinline__ void adj_b_cell(const AdjK<T>& p, int i, int j) {
   }
   if (fx) {
     T hb = ahx + (gx0 - gx1) / p.dx;
-    p.aHx[hx] = hb;
-    T gb = -p.dt * hb;
+    p.aHx[hx] = p.h1y ? h1x * hb : hb;
+    T gb = -h2x * hb;
     if (sy) {
-      T pt = pzy - p.dt * hb;
+      T pt = pzy - h2x * hb;
       gb = gb + a_y * pt;
       p.apzy[hx] = b_y * pt;
     }
@@ -260,6 +272,12 @@ AdjK<T> make_adjk(const mt_tm2d* d, const mt_tm2d_adj* a) {
   p.gEx = static_cast<T*>(a->gEx);
   p.gEy = static_cast<T*>(a->gEy);
   p.eps_grad = static_cast<T*>(a->eps_grad);
+  p.d1 = static_cast<const T*>(d->d1);
+  p.d2 = static_cast<const T*>(d->d2);
+  p.h1x = static_cast<const T*>(d->h1x);
+  p.h2x = static_cast<const T*>(d->h2x);
+  p.h1y = static_cast<const T*>(d->h1y);
+  p.h2y = static_cast<const T*>(d->h2y);
   return p;
 }
 

@@ -284,8 +284,10 @@ extern "C" int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const voi
   if (k < 0) return fail(MT_EINVAL, "negative step count");
   if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward Ez tape");
   if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");
-  if (p->n_poles || p->kerr || p->d1 || p->h1x)
-    return fail(MT_EINVAL, "adjoint through dispersive, nonlinear or conductive media is not supported");
+  if (p->n_poles || p->kerr)
+    return fail(MT_EINVAL, "adjoint through dispersive or nonlinear media is not supported");
+  if (!p->d1 != !p->d2 || !p->h1x != !p->h2x || !p->h1y != !p->h2y || !p->d1 != !p->h1x || !p->d1 != !p->h1y)
+    return fail(MT_EINVAL, "conductivity needs all of d1, d2, h1x, h2x, h1y, h2y");
   if (!p->eps || !a->aEz || !a->aDz || !a->aHx || !a->aHy || !a->apsi_Ezy || !a->apsi_Ezx ||
       !a->apsi_Hyx || !a->apsi_Hxy || !a->gHy || !a->gHx || !a->gEx || !a->gEy || !a->eps_grad)
     return fail(MT_EINVAL, "null adjoint pointer");

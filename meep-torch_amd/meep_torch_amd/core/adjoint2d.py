@@ -166,7 +166,13 @@ def _descriptor(c, fields, dtype, device):
     eps = c.eps_inf.to(device=device, dtype=dtype).contiguous()
     d.eps = eps.data_ptr()
     d.hy, d.hx, d.ex, d.ey = (engine2d._axis(c, n) for n in ("hy", "hx", "ex", "ey"))
-    return d, eps
+    keep = [eps]
+    if c.d1 is not None or c.h1x is not None:     # conductivity (sigma_d, sigma_b, mu)
+        for name in ("d1", "d2", "h1x", "h2x", "h1y", "h2y"):
+            t = getattr(c, name).to(device=device, dtype=dtype).contiguous()
+            keep.append(t)
+            setattr(d, name, t.data_ptr())
+    return d, keep
 
 
 class _TERun(torch.autograd.Function):
@@ -239,9 +245,8 @@ def _te_descriptor(c, nx, ny, dtype, device):
 
 
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
-    if mode == "tm" and (coeffs.c1.shape[0] or coeffs.has_kerr or coeffs.has_chi2 or coeffs.d1 is not None
-                         or coeffs.h1x is not None):
-        raise NotImplementedError("gradients through dispersive, nonlinear or conductive media are not supported")
+    if mode == "tm" and (coeffs.c1.shape[0] or coeffs.has_kerr or coeffs.has_chi2):
+        raise NotImplementedError("gradients through dispersive or nonlinear media are not supported")
     if mode == "te" and (coeffs.c1x.shape[0] or coeffs.c1y.shape[0] or coeffs.d1x is not None
                          or coeffs.hz1 is not None or coeffs.has_kerr or coeffs.has_chi2):
         raise NotImplementedError("gradients through dispersive, nonlinear or conductive media are not supported")
