@@ -298,7 +298,9 @@ typedef struct {
  * through step_te, meep_torch/core/stepper_te.py:73-190): for s = k-1..0 the
  * monitor cotangents of step step0+s are injected, then the exact transpose
  * of the step runs, with E after the step read from tape slot s (as written
- * by mt_te2d_run).  Accumulates dL/deps_x and dL/deps_y. */
+ * by mt_te2d_run).  Accumulates dL/deps_x and dL/deps_y.  Conductivity
+ * (d1x, d2x, d1y, d2y, hz1, hz2 of p, all or none) is transposed with the
+ * recurrences; poles and Kerr are refused. */
 int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const void* tape, const mt_monitor* mons,
                     int n_mons, int step0, int k, void* stream);
 

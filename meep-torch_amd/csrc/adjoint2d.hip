@@ -98,6 +98,9 @@ struct AdjTe {
   Pml<T> hx, hy, ey, ex;
   T *aHz, *aEx, *aEy, *aDx, *aDy, *apEyx, *apExy, *apHzy, *apHzx;
   T *gEy, *gEx, *gDx, *gDy, *gepsx, *gepsy;
+  // conductivity (sigma_d, sigma_b, mu): Dx' = d1x Dx + d2x cx,
+  // Dy' = d1y Dy - d2y cy, Hz' = hz1 Hz - hz2 curl; NULL: 1 and dt
+  const T *d1x, *d2x, *d1y, *d2y, *hz1, *hz2;
 };
 
 template <typename T>
@@ -120,7 +123,9 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p1(AdjTe<T> p) {
     by = p.hy.b[j];
     ay = p.hy.a[j];
   }
-  const T gc = -p.dt * ahz;  // cotangent of the curl
+  const T h1 = p.hz1 ? p.hz1[h] : T(1), h2 = p.hz1 ? p.hz2[h] : p.dt;
+  if (p.hz1) p.aHz[h] = h1 * ahz;  // Hz' = hz1 Hz - hz2 curl
+  const T gc = -h2 * ahz;          // cotangent of the curl
   T gy = gc, gx = -gc;
   if (sx) {
     T t = pyx + gc;
@@ -152,6 +157,7 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
     const T gxm = j >= 1 ? p.gEx[h - 1] : T(0);
     const T gx0 = j < ny - 1 ? p.gEx[h] : T(0);
     const T aex = p.aEx[c], adx = p.aDx[c], epsx = p.eps_x[c], ex = tex[c], ge = p.gepsx[c];
+    const T c1 = p.d1x ? p.d1x[c] : T(1), c2 = p.d1x ? p.d2x[c] : p.dt;
     T ps = T(0), b = T(0), a = T(0);
     if (sh) {
       ps = p.apHzy[q];
@@ -161,11 +167,11 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
     const T exb = aex + (gxm - gx0) / p.dx;
     p.gepsx[c] = ge + (-exb * ex) / epsx;
     const T dxb = adx + exb / epsx;
-    p.aDx[c] = dxb;
+    p.aDx[c] = p.d1x ? c1 * dxb : dxb;
     p.aEx[c] = T(0);
     T gd = T(0);
     if (live) {
-      const T g = p.dt * dxb;
+      const T g = c2 * dxb;
       gd = g;
       if (sh) {
         T t = ps + g;
@@ -184,6 +190,7 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
     const T gym = i >= 1 ? p.gEy[c - (ny - 1)] : T(0);
     const T gy0 = i < nx - 1 ? p.gEy[c] : T(0);
     const T aey = p.aEy[c], ady = p.aDy[c], epsy = p.eps_y[c], ey = tey[c], ge = p.gepsy[c];
+    const T c1 = p.d1x ? p.d1y[c] : T(1), c2 = p.d1x ? p.d2y[c] : p.dt;
     T ps = T(0), b = T(0), a = T(0);
     if (sh) {
       ps = p.apHzx[q];
@@ -193,11 +200,11 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
     const T eyb = aey + (gym - gy0) / p.dx;
     p.gepsy[c] = ge + (-eyb * ey) / epsy;
     const T dyb = ady + eyb / epsy;
-    p.aDy[c] = dyb;
+    p.aDy[c] = p.d1x ? c1 * dyb : dyb;
     p.aEy[c] = T(0);
     T gd = T(0);
     if (live) {
-      const T g = -p.dt * dyb;
+      const T g = -c2 * dyb;
       gd = g;
       if (sh) {
         T t = ps + g;
@@ -250,6 +257,12 @@ int te_adj_t(const mt_te2d* d, const mt_te2d_adj* a, const void* tape, const mt_
   p.gDy = static_cast<T*>(a->gDy);
   p.gepsx = static_cast<T*>(a->epsx_grad);
   p.gepsy = static_cast<T*>(a->epsy_grad);
+  p.d1x = static_cast<const T*>(d->d1x);
+  p.d2x = static_cast<const T*>(d->d2x);
+  p.d1y = static_cast<const T*>(d->d1y);
+  p.d2y = static_cast<const T*>(d->d2y);
+  p.hz1 = static_cast<const T*>(d->hz1);
+  p.hz2 = static_cast<const T*>(d->hz2);
   const T* tp = static_cast<const T*>(tape);
   const long long slot = (long long)(p.nx - 1) * p.ny + (long long)p.nx * (p.ny - 1);
   dim3 blk(ABX, ABY), grd = grid2(p.ny, p.nx, blk);
@@ -314,8 +327,10 @@ extern "C" int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const voi
   if (p->nx < 3 || p->ny < 3) return fail(MT_EINVAL, "grid must be at least 3x3");
   if (k < 0) return fail(MT_EINVAL, "negative step count");
   if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward E tape");
-  if (p->npx || p->npy || p->d1x || p->hz1 || p->kerr)
-    return fail(MT_EINVAL, "adjoint through dispersive, nonlinear or conductive media is not supported");
+  if (p->npx || p->npy || p->kerr)
+    return fail(MT_EINVAL, "adjoint through dispersive or nonlinear media is not supported");
+  if (!p->d1x != !p->d2x || !p->d1x != !p->d1y || !p->d1x != !p->d2y || !p->d1x != !p->hz1 || !p->d1x != !p->hz2)
+    return fail(MT_EINVAL, "conductivity needs all of d1x, d2x, d1y, d2y, hz1, hz2");
   if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");
   if (!p->eps_x || !p->eps_y || !a->aHz || !a->aEx || !a->aEy || !a->aDx || !a->aDy || !a->apsi_Eyx ||
       !a->apsi_Exy || !a->apsi_Hzy || !a->apsi_Hzx || !a->gEy || !a->gEx || !a->gDx || !a->gDy ||

@@ -241,15 +241,20 @@ def _te_descriptor(c, nx, ny, dtype, device):
     ey_ = c.eps_y.to(device=device, dtype=dtype).contiguous()
     d.eps_x, d.eps_y = ex_.data_ptr(), ey_.data_ptr()
     d.hx, d.hy, d.ey, d.ex = (engine2d._axis(c, n) for n in ("hx", "hy", "ey", "ex"))
-    return d, (ex_, ey_)
+    keep = [ex_, ey_]
+    if c.d1x is not None or c.hz1 is not None:    # conductivity (sigma_d, sigma_b, mu)
+        for name in ("d1x", "d2x", "d1y", "d2y", "hz1", "hz2"):
+            t = getattr(c, name).to(device=device, dtype=dtype).contiguous()
+            keep.append(t)
+            setattr(d, name, t.data_ptr())
+    return d, keep
 
 
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
     if mode == "tm" and (coeffs.c1.shape[0] or coeffs.has_kerr or coeffs.has_chi2):
         raise NotImplementedError("gradients through dispersive or nonlinear media are not supported")
-    if mode == "te" and (coeffs.c1x.shape[0] or coeffs.c1y.shape[0] or coeffs.d1x is not None
-                         or coeffs.hz1 is not None or coeffs.has_kerr or coeffs.has_chi2):
-        raise NotImplementedError("gradients through dispersive, nonlinear or conductive media are not supported")
+    if mode == "te" and (coeffs.c1x.shape[0] or coeffs.c1y.shape[0] or coeffs.has_kerr or coeffs.has_chi2):
+        raise NotImplementedError("gradients through dispersive or nonlinear media are not supported")
     allowed = ("eps_inf",) if mode == "tm" else ("eps_x", "eps_y")
     for name, t in coeffs._asdict().items():
         if torch.is_tensor(t) and t.requires_grad and name not in allowed:
