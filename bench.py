@@ -348,7 +348,7 @@ def bench_c4(args):
         "config": {"workload": f"C4 batched TM fwd+adjoint, {len(sizes)} variants N in "
                                f"[{min(sizes)},{max(sizes)}], {ysteps} steps, ckpt 200",
                    "cells_per_variant_sum": cells},
-        "sweep_cell_updates_per_s": upd / wall, "fom": float(fom)}), flush=True)
+        "sweep_cell_updates_per_s": upd / wall, "fom": float(fom.detach())}), flush=True)
 
 
 def _gemm_roofline(net, batch=64, reps=5):
