@@ -266,6 +266,9 @@ typedef struct {
   void *gHy, *gHx;                        /* scratch (nx, ny) each */
   void *gEx, *gEy;                        /* scratch (nx-1, ny) and (nx, ny-1) */
   void* eps_grad;                         /* device (nx, ny), accumulated */
+  /* cotangents of the pole stacks P_cur, P_prev (n_poles, nx, ny) when the
+   * problem has ADE poles; NULL otherwise */
+  void *aP_cur, *aP_prev;
 } mt_tm2d_adj;
 
 /* Reverse-time adjoint of k TM steps [step0, step0+k): on entry the mt_tm2d_adj
@@ -277,7 +280,8 @@ typedef struct {
  * accumulator cotangent dL/dacc; each step injects
  *   dt*(cos(w t) dacc_re - sin(w t) dacc_im)  (halved onto both rows of an
  * averaged term).  Conductivity (d1, d2, h1x, h2x, h1y, h2y of p, all or none)
- * is transposed with the recurrences; poles and Kerr are refused.  Replaces
+ * and ADE poles (c1, c2, c3 of p with a->aP_cur, a->aP_prev) are transposed
+ * with the recurrences; Kerr is refused.  Replaces
  * PyTorch autograd through run_tm's checkpointed chunks
  * (meep_torch/core/run.py:99-120, eps reached via stepper_tm.py:226-227). */
 int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const void* tape,

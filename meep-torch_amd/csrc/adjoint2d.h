@@ -22,6 +22,11 @@ struct AdjK {
   // Hx' = h1x Hx - h2x curl, Hy' = h1y Hy + h2y curl; NULL: d1 = h1 = 1,
   // d2 = h2 = dt
   const T *d1, *d2, *h1x, *h2x, *h1y, *h2y;
+  // np ADE poles: per-cell scalars c1, c2, c3 and the cotangents of the
+  // stacks P (aPc) and P_prev (aPp), all (np, nx, ny)
+  int np;
+  const T *c1, *c2, *c3;
+  T *aPc, *aPp;
 };
 
 // Each pass below issues all of a cell's loads before its first store: the
@@ -55,7 +60,23 @@ __device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, 
   p.eps_grad[c] = eg + (-ezb * ea) / eps;
   T dzb = dz0 + ezb / eps;
   p.aDz[c] = p.d1 ? d1 * dzb : dzb;          // Dz' = d1 Dz + d2 curl
-  p.aEz[c] = T(0);
+  // poles: Ez' = (Dz' - sum P') / eps and P' = c1 P + c2 P_prev + c3 Ez, so
+  // P' gains -Ez'b / eps; P then carries c1 P'b + P_prev'b (P_prev' = P),
+  // P_prev carries c2 P'b, and the old Ez gains c3 P'b (phase C adds the
+  // curl part to it)
+  T ez_old = T(0);
+  if (p.np) {
+    const long long ncell = (long long)nx * ny;
+    const T r = ezb / eps;
+    for (int n = 0; n < p.np; ++n) {
+      const long long o = (long long)n * ncell + c;
+      const T apn = p.aPc[o] - r, app = p.aPp[o];
+      p.aPc[o] = p.c1[o] * apn + app;
+      p.aPp[o] = p.c2[o] * apn;
+      ez_old = ez_old + p.c3[o] * apn;
+    }
+  }
+  p.aEz[c] = ez_old;
   T ghy = T(0), ghx = T(0);
   if (live) {
     T cb = d2 * dzb;
@@ -146,7 +167,9 @@ __device__ __forceinline__ void adj_c_cell(const AdjK<T>& p, int i, int j) {
   T gx0 = i < nx - 1 ? p.gEx[(long long)i * ny + j] : T(0);
   T gym = j >= 1 ? p.gEy[(long long)i * (ny - 1) + j - 1] : T(0);
   T gy0 = j < ny - 1 ? p.gEy[(long long)i * (ny - 1) + j] : T(0);
-  p.aEz[(long long)i * ny + j] = (gxm - gx0) / p.dx + (gym - gy0) / p.dx;
+  const long long c = (long long)i * ny + j;
+  const T v = (gxm - gx0) / p.dx + (gym - gy0) / p.dx;
+  p.aEz[c] = p.np ? p.aEz[c] + v : v;
 }
 
 // Adds the cotangent vb of element e of monitor term tm into the adjoint
@@ -278,6 +301,12 @@ AdjK<T> make_adjk(const mt_tm2d* d, const mt_tm2d_adj* a) {
   p.h2x = static_cast<const T*>(d->h2x);
   p.h1y = static_cast<const T*>(d->h1y);
   p.h2y = static_cast<const T*>(d->h2y);
+  p.np = (d->n_poles > 0 && a->aP_cur) ? d->n_poles : 0;
+  p.c1 = static_cast<const T*>(d->c1);
+  p.c2 = static_cast<const T*>(d->c2);
+  p.c3 = static_cast<const T*>(d->c3);
+  p.aPc = static_cast<T*>(a->aP_cur);
+  p.aPp = static_cast<T*>(a->aP_prev);
   return p;
 }
 

@@ -25,7 +25,7 @@ import torch
 from .. import _engine as E
 from . import engine2d
 from .stepper_te import FIELDS as TE_FIELDS
-from .stepper_tm import FIELDS
+from .stepper_tm import fields_of as tm_fields_of
 
 DEFAULT_K = 200
 
@@ -108,28 +108,29 @@ def _chunk_cotangents(monitors, gacc, step0, k):
 class _TMRun(torch.autograd.Function):
     @staticmethod
     def forward(ctx, eps, cfg, *state_in):
+        names = cfg["names"]
         cdet = cfg["coeffs"]._replace(eps_inf=eps.detach())
-        work = {f: t.detach().clone().contiguous() for f, t in zip(FIELDS, state_in)}
-        acc, ckpts = _forward_chunks("tm", FIELDS, work, cdet, cfg)
+        work = {f: t.detach().clone().contiguous() for f, t in zip(names, state_in)}
+        acc, ckpts = _forward_chunks("tm", names, work, cdet, cfg)
         ctx.cfg = cfg
         ctx.cdet = cdet
         ctx.ckpts = ckpts
         ctx.save_for_backward(eps)
         ctx.n_state = len(state_in)
-        return (*[work[f] for f in FIELDS], *acc)
+        return (*[work[f] for f in names], *acc)
 
     @staticmethod
     def backward(ctx, *grads):
         (eps,) = ctx.saved_tensors
         cfg, cdet = ctx.cfg, ctx.cdet
-        monitors, plan = cfg["monitors"], cfg["plan"]
-        nf = len(FIELDS)
+        monitors, plan, names = cfg["monitors"], cfg["plan"], cfg["names"]
+        nf = len(names)
         gstate, gacc = grads[:nf], list(grads[nf:])
         ref = ctx.ckpts[0][2]["Ez"]
         dtype, device = ref.dtype, ref.device
         nx, ny = ref.shape
         adj = {f: (g.detach().to(dtype).clone().contiguous() if g is not None
-                   else torch.zeros_like(ctx.ckpts[0][2][f])) for f, g in zip(FIELDS, gstate)}
+                   else torch.zeros_like(ctx.ckpts[0][2][f])) for f, g in zip(names, gstate)}
         scratch = {"gHy": torch.empty(nx, ny, dtype=dtype, device=device),
                    "gHx": torch.empty(nx, ny, dtype=dtype, device=device),
                    "gEx": torch.empty(nx - 1, ny, dtype=dtype, device=device),
@@ -137,7 +138,7 @@ class _TMRun(torch.autograd.Function):
         eps_grad = torch.zeros(nx, ny, dtype=dtype, device=device)
         L = E.lib()
         a = E.TM2DAdj()
-        for f in FIELDS:
+        for f in names:
             setattr(a, "a" + f, adj[f].data_ptr())
         for f, t in scratch.items():
             setattr(a, f, t.data_ptr())
@@ -154,7 +155,7 @@ class _TMRun(torch.autograd.Function):
             E.check(rc, "mt_tm2d_adjoint")
             del keep, tape
         del d_keep
-        g_state = [adj[f] for f in FIELDS][:ctx.n_state]
+        g_state = [adj[f] for f in names][:ctx.n_state]
         return (eps_grad.to(eps.dtype), None, *g_state)
 
 
@@ -167,11 +168,16 @@ def _descriptor(c, fields, dtype, device):
     d.eps = eps.data_ptr()
     d.hy, d.hx, d.ex, d.ey = (engine2d._axis(c, n) for n in ("hy", "hx", "ex", "ey"))
     keep = [eps]
+    media = []
     if c.d1 is not None or c.h1x is not None:     # conductivity (sigma_d, sigma_b, mu)
-        for name in ("d1", "d2", "h1x", "h2x", "h1y", "h2y"):
-            t = getattr(c, name).to(device=device, dtype=dtype).contiguous()
-            keep.append(t)
-            setattr(d, name, t.data_ptr())
+        media += ["d1", "d2", "h1x", "h2x", "h1y", "h2y"]
+    if c.c1.shape[0]:                              # ADE poles
+        d.n_poles = int(c.c1.shape[0])
+        media += ["c1", "c2", "c3"]
+    for name in media:
+        t = getattr(c, name).to(device=device, dtype=dtype).contiguous()
+        keep.append(t)
+        setattr(d, name, t.data_ptr())
     return d, keep
 
 
@@ -251,8 +257,8 @@ def _te_descriptor(c, nx, ny, dtype, device):
 
 
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
-    if mode == "tm" and (coeffs.c1.shape[0] or coeffs.has_kerr or coeffs.has_chi2):
-        raise NotImplementedError("gradients through dispersive or nonlinear media are not supported")
+    if mode == "tm" and (coeffs.has_kerr or coeffs.has_chi2):
+        raise NotImplementedError("gradients through nonlinear (chi2 / chi3) media are not supported")
     if mode == "te" and (coeffs.c1x.shape[0] or coeffs.c1y.shape[0] or coeffs.has_kerr or coeffs.has_chi2):
         raise NotImplementedError("gradients through dispersive or nonlinear media are not supported")
     allowed = ("eps_inf",) if mode == "tm" else ("eps_x", "eps_y")
@@ -263,7 +269,7 @@ def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_
     _validate(mode, state, coeffs)
     cfg = {"coeffs": coeffs, "monitors": list(monitors), "n_steps": n_steps, "ck": checkpoint_every}
     if mode == "tm":
-        names, eps = FIELDS, (coeffs.eps_inf,)
+        names, eps = tuple(tm_fields_of(state)), (coeffs.eps_inf,)
         tgt = state.Dz
         fn = _TMRun
     else:
@@ -272,6 +278,7 @@ def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_
         fn = _TERun
     E.require_device(*eps)
     nx, ny = (state.Ez.shape if mode == "tm" else (state.Ey.shape[0], state.Ex.shape[1]))
+    cfg["names"] = names
     cfg["plan"] = engine2d.source_plan(source, tgt.shape, tgt.device, tgt.dtype, engine2d.h_shapes(mode, nx, ny))
     outs = fn.apply(*eps, cfg, *[getattr(state, f) for f in names])
     nf = len(names)

@@ -10,9 +10,9 @@
   transfer matrix (3%, meep_torch tests/analytic/test_dispersion.py) and a
   sigma_d slab against meep's transmission (amplitude rms 3%, the fixture of
   meep_torch tests/examples/test_w5_conductivity.py);
-* eps gradients through conductive media (sigma_d, sigma_b, mu) against CPU
-  autograd through the oracle; gradients through poles and Kerr, and the
-  batched launch, refuse loudly."""
+* eps gradients through conductive media (sigma_d, sigma_b, mu, TM and TE)
+  and TM ADE poles against CPU autograd through the oracle; gradients
+  through Kerr media and TE poles, and the batched launch, refuse loudly."""
 import cmath
 import json
 import os
@@ -193,9 +193,9 @@ def test_media_refuse_gradients_and_batches():
     from meep_torch_amd.core.stepper_tm import init_tm_state
     g = Grid(nx=32, ny=32, dx=0.05, dtype=torch.float64, device="cuda")
     eps = torch.ones(32, 32, dtype=torch.float64, device="cuda", requires_grad=True)
-    m = build_tm_coeffs(g, eps_inf=eps, npml=4, poles=[(1.0, 0.1, 1.0)])
-    with pytest.raises(NotImplementedError, match="dispersive"):
-        run_tm(init_tm_state(g, n_poles=1), m, 5)
+    m = build_tm_coeffs(g, eps_inf=eps, npml=4, chi3=0.01)
+    with pytest.raises(NotImplementedError, match="nonlinear"):
+        run_tm(init_tm_state(g), m, 5)
     m2 = build_tm_coeffs(g, eps_inf=2.0, npml=4, poles=[(1.0, 0.1, 1.0)])
     with pytest.raises(NotImplementedError):
         BatchedTM([(m2, None, [])], 10)
@@ -393,4 +393,44 @@ def test_te_eps_gradient_through_conductive_media():
     g_ref, = torch.autograd.grad(yee2d.flux_from_acc("te_x", oacc, DX).sum(), e)
     err = rel(eps.grad, g_ref)
     print(f"\n[te cond grad] rel-L2 {err:.2e}")
+    assert float(g_ref[50:75, 16:48].abs().max()) > 0 and err <= 1e-9
+
+
+@pytest.mark.parametrize("cond", [False, True], ids=["poles", "poles+sigma"])
+def test_tm_eps_gradient_through_poles(cond):
+    """The TM adjoint transposes the ADE pole recurrence (P' = c1 P + c2 P_prev
+    + c3 Ez, Ez' = (Dz' - sum P') / eps) with the pole stacks' cotangents
+    carried between steps: the eps gradient of a flux objective through a
+    Lorentz + Drude block (alone and with sigma_d / sigma_b / mu) equals CPU
+    autograd through the oracle's steps (FP64), across checkpoint chunks."""
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTM
+    from meep_torch_amd.core.run import run_tm
+    from meep_torch_amd.core.sources import GaussianPulse, LineSource
+    from meep_torch_amd.core.stepper_tm import init_tm_state
+    from oracle import yee2d
+    dt = torch.float64
+    eps0, kw0 = _media(dt)
+    names = ("poles", "pole_masks") + (("sigma_d", "sigma_b", "mu") if cond else ())
+    kw = {k: kw0[k] for k in names}
+    dev = {k: ([x.cuda() for x in v] if k == "pole_masks" else v.cuda() if torch.is_tensor(v) else v)
+           for k, v in kw.items()}
+    n = 300
+    g = Grid(nx=NX, ny=NY, dx=DX, dtype=dt, device="cuda")
+    eps = eps0.clone().cuda().requires_grad_(True)
+    m = build_tm_coeffs(g, eps_inf=eps, npml=NPML, **dev)
+    src = LineSource(g, ix=20, j0=16, j1=48, profile=GaussianPulse(0.8, 0.5))
+    mon = FluxMonitorTM(g, FREQS, x_normal=True, idx=80, lo=12, hi=52)
+    _, (acc,) = run_tm(init_tm_state(g, n_poles=2), m, n, source=src, monitors=[mon], checkpoint_every=100)
+    mon.flux(acc, DX).sum().backward()
+    torch.set_num_threads(8)
+    e = eps0.clone().requires_grad_(True)
+    c = yee2d.tm_coeffs(NX, NY, DX, e, NPML, dtype=dt, **kw)
+    mons = [{"type": "flux", "kind": "tm_x", "idx": 80, "lo": 12, "hi": 52, "freqs": FREQS}]
+    _, oacc = yee2d.run2d("tm", yee2d.tm_state(NX, NY, dt, n_poles=2), c, n,
+                          amps=[src.amplitude(t) for t in range(n)], mask=src.mask.cpu(), monitors=mons)
+    g_ref, = torch.autograd.grad(yee2d.flux_from_acc("tm_x", oacc[0], DX).sum(), e)
+    err = rel(eps.grad, g_ref)
+    print(f"\n[pole grad cond={cond}] rel-L2 {err:.2e}")
     assert float(g_ref[50:75, 16:48].abs().max()) > 0 and err <= 1e-9
