@@ -295,6 +295,9 @@ typedef struct {
   void *gEy, *gEx;            /* scratch (nx-1,ny-1) each */
   void *gDx, *gDy;            /* scratch (nx-1,ny) (nx,ny-1) */
   void *epsx_grad, *epsy_grad;  /* (nx-1,ny) (nx,ny-1), accumulated */
+  /* cotangents of the pole stacks Px_cur, Px_prev (npx, nx-1, ny) and
+   * Py_cur, Py_prev (npy, nx, ny-1) when the problem has them; else NULL */
+  void *aPx_cur, *aPx_prev, *aPy_cur, *aPy_prev;
 } mt_te2d_adj;
 
 /* Reverse sweep of k TE steps starting at global step step0 (the gradient of
@@ -303,8 +306,9 @@ typedef struct {
  * monitor cotangents of step step0+s are injected, then the exact transpose
  * of the step runs, with E after the step read from tape slot s (as written
  * by mt_te2d_run).  Accumulates dL/deps_x and dL/deps_y.  Conductivity
- * (d1x, d2x, d1y, d2y, hz1, hz2 of p, all or none) is transposed with the
- * recurrences; poles and Kerr are refused. */
+ * (d1x, d2x, d1y, d2y, hz1, hz2 of p, all or none) and ADE poles (c1x..c3y
+ * of p with the aP* cotangents of a) are transposed with the recurrences;
+ * Kerr is refused. */
 int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const void* tape, const mt_monitor* mons,
                     int n_mons, int step0, int k, void* stream);
 

@@ -101,7 +101,30 @@ struct AdjTe {
   // conductivity (sigma_d, sigma_b, mu): Dx' = d1x Dx + d2x cx,
   // Dy' = d1y Dy - d2y cy, Hz' = hz1 Hz - hz2 curl; NULL: 1 and dt
   const T *d1x, *d2x, *d1y, *d2y, *hz1, *hz2;
+  // ADE poles per axis: per-cell c1, c2, c3 stacks and the cotangents of the
+  // P / P_prev stacks (npx on the Ex grid, npy on the Ey grid)
+  int npx, npy;
+  const T *c1x, *c2x, *c3x, *c1y, *c2y, *c3y;
+  T *aPxc, *aPxp, *aPyc, *aPyp;
 };
+
+// Transpose of one axis's poles at a cell: P' = c1 P + c2 P_prev + c3 E(old)
+// and E' = (D' - sum P') / eps, with r = E'b / eps the cotangent that E'
+// passes to every P'.  Updates the stacks' cotangents in place and returns
+// the old E's cotangent (the old E feeds nothing else in a TE step).
+template <typename T>
+__device__ __forceinline__ T te_pole_adj(int np, const T* c1, const T* c2, const T* c3, T* apc, T* app,
+                                         long long c, long long ncell, T r) {
+  T e_old = T(0);
+  for (int n = 0; n < np; ++n) {
+    const long long o = (long long)n * ncell + c;
+    const T apn = apc[o] - r, prev = app[o];
+    apc[o] = c1[o] * apn + prev;
+    app[o] = c2[o] * apn;
+    e_old = e_old + c3[o] * apn;
+  }
+  return e_old;
+}
 
 template <typename T>
 __global__ __launch_bounds__(ABX* ABY) void te_adj_p1(AdjTe<T> p) {
@@ -168,7 +191,9 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
     p.gepsx[c] = ge + (-exb * ex) / epsx;
     const T dxb = adx + exb / epsx;
     p.aDx[c] = p.d1x ? c1 * dxb : dxb;
-    p.aEx[c] = T(0);
+    p.aEx[c] = p.npx ? te_pole_adj(p.npx, p.c1x, p.c2x, p.c3x, p.aPxc, p.aPxp, c, (long long)(nx - 1) * ny,
+                                   exb / epsx)
+                     : T(0);
     T gd = T(0);
     if (live) {
       const T g = c2 * dxb;
@@ -201,7 +226,9 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
     p.gepsy[c] = ge + (-eyb * ey) / epsy;
     const T dyb = ady + eyb / epsy;
     p.aDy[c] = p.d1x ? c1 * dyb : dyb;
-    p.aEy[c] = T(0);
+    p.aEy[c] = p.npy ? te_pole_adj(p.npy, p.c1y, p.c2y, p.c3y, p.aPyc, p.aPyp, c, (long long)nx * (ny - 1),
+                                   eyb / epsy)
+                     : T(0);
     T gd = T(0);
     if (live) {
       const T g = -c2 * dyb;
@@ -263,6 +290,18 @@ int te_adj_t(const mt_te2d* d, const mt_te2d_adj* a, const void* tape, const mt_
   p.d2y = static_cast<const T*>(d->d2y);
   p.hz1 = static_cast<const T*>(d->hz1);
   p.hz2 = static_cast<const T*>(d->hz2);
+  p.npx = (d->npx > 0 && a->aPx_cur) ? d->npx : 0;
+  p.npy = (d->npy > 0 && a->aPy_cur) ? d->npy : 0;
+  p.c1x = static_cast<const T*>(d->c1x);
+  p.c2x = static_cast<const T*>(d->c2x);
+  p.c3x = static_cast<const T*>(d->c3x);
+  p.c1y = static_cast<const T*>(d->c1y);
+  p.c2y = static_cast<const T*>(d->c2y);
+  p.c3y = static_cast<const T*>(d->c3y);
+  p.aPxc = static_cast<T*>(a->aPx_cur);
+  p.aPxp = static_cast<T*>(a->aPx_prev);
+  p.aPyc = static_cast<T*>(a->aPy_cur);
+  p.aPyp = static_cast<T*>(a->aPy_prev);
   const T* tp = static_cast<const T*>(tape);
   const long long slot = (long long)(p.nx - 1) * p.ny + (long long)p.nx * (p.ny - 1);
   dim3 blk(ABX, ABY), grd = grid2(p.ny, p.nx, blk);
@@ -328,8 +367,11 @@ extern "C" int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const voi
   if (p->nx < 3 || p->ny < 3) return fail(MT_EINVAL, "grid must be at least 3x3");
   if (k < 0) return fail(MT_EINVAL, "negative step count");
   if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward E tape");
-  if (p->npx || p->npy || p->kerr)
-    return fail(MT_EINVAL, "adjoint through dispersive or nonlinear media is not supported");
+  if (p->kerr) return fail(MT_EINVAL, "adjoint through nonlinear (chi2 / chi3) media is not supported");
+  if (p->npx < 0 || p->npy < 0 ||
+      (p->npx && (!a->aPx_cur || !a->aPx_prev || !p->c1x || !p->c2x || !p->c3x)) ||
+      (p->npy && (!a->aPy_cur || !a->aPy_prev || !p->c1y || !p->c2y || !p->c3y)))
+    return fail(MT_EINVAL, "adjoint with poles needs c1, c2, c3 and the pole-stack cotangents");
   if (!p->d1x != !p->d2x || !p->d1x != !p->d1y || !p->d1x != !p->d2y || !p->d1x != !p->hz1 || !p->d1x != !p->hz2)
     return fail(MT_EINVAL, "conductivity needs all of d1x, d2x, d1y, d2y, hz1, hz2");
   if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");

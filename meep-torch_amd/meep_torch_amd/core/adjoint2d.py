@@ -24,7 +24,7 @@ import torch
 
 from .. import _engine as E
 from . import engine2d
-from .stepper_te import FIELDS as TE_FIELDS
+from .stepper_te import fields_of as te_fields_of
 from .stepper_tm import fields_of as tm_fields_of
 
 DEFAULT_K = 200
@@ -188,27 +188,29 @@ class _TERun(torch.autograd.Function):
     @staticmethod
     def forward(ctx, eps_x, eps_y, cfg, *state_in):
         cdet = cfg["coeffs"]._replace(eps_x=eps_x.detach(), eps_y=eps_y.detach())
-        work = {f: t.detach().clone().contiguous() for f, t in zip(TE_FIELDS, state_in)}
-        acc, ckpts = _forward_chunks("te", TE_FIELDS, work, cdet, cfg)
+        names = cfg["names"]
+        work = {f: t.detach().clone().contiguous() for f, t in zip(names, state_in)}
+        acc, ckpts = _forward_chunks("te", names, work, cdet, cfg)
         ctx.cfg = cfg
         ctx.cdet = cdet
         ctx.ckpts = ckpts
         ctx.save_for_backward(eps_x, eps_y)
         ctx.n_state = len(state_in)
-        return (*[work[f] for f in TE_FIELDS], *acc)
+        return (*[work[f] for f in names], *acc)
 
     @staticmethod
     def backward(ctx, *grads):
         eps_x, eps_y = ctx.saved_tensors
         cfg, cdet = ctx.cfg, ctx.cdet
         monitors, plan = cfg["monitors"], cfg["plan"]
-        nf = len(TE_FIELDS)
+        names = cfg["names"]
+        nf = len(names)
         gstate, gacc = grads[:nf], list(grads[nf:])
         st0 = ctx.ckpts[0][2]
         dtype, device = st0["Hz"].dtype, st0["Hz"].device
         nx, ny = st0["Ey"].shape[0], st0["Ex"].shape[1]
         adj = {f: (g.detach().to(dtype).clone().contiguous() if g is not None
-                   else torch.zeros_like(st0[f])) for f, g in zip(TE_FIELDS, gstate)}
+                   else torch.zeros_like(st0[f])) for f, g in zip(names, gstate)}
         scratch = {"gEy": torch.empty(nx - 1, ny - 1, dtype=dtype, device=device),
                    "gEx": torch.empty(nx - 1, ny - 1, dtype=dtype, device=device),
                    "gDx": torch.empty(nx - 1, ny, dtype=dtype, device=device),
@@ -217,7 +219,7 @@ class _TERun(torch.autograd.Function):
         gy = torch.zeros(nx, ny - 1, dtype=dtype, device=device)
         L = E.lib()
         a = E.TE2DAdj()
-        for f in TE_FIELDS:
+        for f in names:
             setattr(a, "a" + f, adj[f].data_ptr())
         for f, t in scratch.items():
             setattr(a, f, t.data_ptr())
@@ -235,7 +237,7 @@ class _TERun(torch.autograd.Function):
             E.check(rc, "mt_te2d_adjoint")
             del keep, tape
         del d_keep
-        g_state = [adj[f] for f in TE_FIELDS][:ctx.n_state]
+        g_state = [adj[f] for f in names][:ctx.n_state]
         return (gx.to(eps_x.dtype), gy.to(eps_y.dtype), None, *g_state)
 
 
@@ -248,19 +250,27 @@ def _te_descriptor(c, nx, ny, dtype, device):
     d.eps_x, d.eps_y = ex_.data_ptr(), ey_.data_ptr()
     d.hx, d.hy, d.ey, d.ex = (engine2d._axis(c, n) for n in ("hx", "hy", "ey", "ex"))
     keep = [ex_, ey_]
+    media = []
     if c.d1x is not None or c.hz1 is not None:    # conductivity (sigma_d, sigma_b, mu)
-        for name in ("d1x", "d2x", "d1y", "d2y", "hz1", "hz2"):
-            t = getattr(c, name).to(device=device, dtype=dtype).contiguous()
-            keep.append(t)
-            setattr(d, name, t.data_ptr())
+        media += ["d1x", "d2x", "d1y", "d2y", "hz1", "hz2"]
+    if c.c1x.shape[0]:                             # ADE poles on Ex
+        d.npx = int(c.c1x.shape[0])
+        media += ["c1x", "c2x", "c3x"]
+    if c.c1y.shape[0]:                             # ADE poles on Ey
+        d.npy = int(c.c1y.shape[0])
+        media += ["c1y", "c2y", "c3y"]
+    for name in media:
+        t = getattr(c, name).to(device=device, dtype=dtype).contiguous()
+        keep.append(t)
+        setattr(d, name, t.data_ptr())
     return d, keep
 
 
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
     if mode == "tm" and (coeffs.has_kerr or coeffs.has_chi2):
         raise NotImplementedError("gradients through nonlinear (chi2 / chi3) media are not supported")
-    if mode == "te" and (coeffs.c1x.shape[0] or coeffs.c1y.shape[0] or coeffs.has_kerr or coeffs.has_chi2):
-        raise NotImplementedError("gradients through dispersive or nonlinear media are not supported")
+    if mode == "te" and (coeffs.has_kerr or coeffs.has_chi2):
+        raise NotImplementedError("gradients through nonlinear (chi2 / chi3) media are not supported")
     allowed = ("eps_inf",) if mode == "tm" else ("eps_x", "eps_y")
     for name, t in coeffs._asdict().items():
         if torch.is_tensor(t) and t.requires_grad and name not in allowed:
@@ -273,7 +283,7 @@ def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_
         tgt = state.Dz
         fn = _TMRun
     else:
-        names, eps = TE_FIELDS, (coeffs.eps_x, coeffs.eps_y)
+        names, eps = tuple(te_fields_of(state)), (coeffs.eps_x, coeffs.eps_y)
         tgt = state.Dy
         fn = _TERun
     E.require_device(*eps)

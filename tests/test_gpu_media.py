@@ -10,9 +10,9 @@
   transfer matrix (3%, meep_torch tests/analytic/test_dispersion.py) and a
   sigma_d slab against meep's transmission (amplitude rms 3%, the fixture of
   meep_torch tests/examples/test_w5_conductivity.py);
-* eps gradients through conductive media (sigma_d, sigma_b, mu, TM and TE)
-  and TM ADE poles against CPU autograd through the oracle; gradients
-  through Kerr media and TE poles, and the batched launch, refuse loudly."""
+* eps gradients through conductive media (sigma_d, sigma_b, mu) and ADE
+  poles, TM and TE, against CPU autograd through the oracle; gradients
+  through Kerr media, and the batched launch, refuse loudly."""
 import cmath
 import json
 import os
@@ -433,4 +433,44 @@ def test_tm_eps_gradient_through_poles(cond):
     g_ref, = torch.autograd.grad(yee2d.flux_from_acc("tm_x", oacc[0], DX).sum(), e)
     err = rel(eps.grad, g_ref)
     print(f"\n[pole grad cond={cond}] rel-L2 {err:.2e}")
+    assert float(g_ref[50:75, 16:48].abs().max()) > 0 and err <= 1e-9
+
+
+def test_te_eps_gradient_through_poles():
+    """TE poles (one shared by Ex and Ey, a Drude pole on Ex only) with
+    conductivity: the TE adjoint carries the Px / Py stacks' cotangents, and
+    the eps gradient of a flux objective equals CPU autograd through the
+    oracle's TE steps (FP64)."""
+    from meep_torch_amd.core.coeffs import build_te_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTE
+    from meep_torch_amd.core.run import run_te
+    from meep_torch_amd.core.sources import GaussianPulse, LineSource
+    from meep_torch_amd.core.stepper_te import init_te_state
+    from oracle import yee2d
+    dt = torch.float64
+    eps0, kw0 = _te_media(dt)
+    kw = {k: v for k, v in kw0.items() if k not in ("chi3", "chi2")}
+    dev = {k: ([x.cuda() for x in v] if k.startswith("pole_masks") else v.cuda() if torch.is_tensor(v) else v)
+           for k, v in kw.items()}
+    n = 300
+    g = Grid(nx=NX, ny=NY, dx=DX, dtype=dt, device="cuda")
+    eps = eps0.clone().cuda().requires_grad_(True)
+    m = build_te_coeffs(g, eps_inf=eps, npml=NPML, **dev)
+    src = LineSource(g, ix=20, j0=16, j1=48, profile=GaussianPulse(0.8, 0.5))
+    src.mask = src.mask[:, :-1].contiguous()
+    mon = FluxMonitorTE(g, FREQS, ix=80, lo=12, hi=52)
+    _, (acc,) = run_te(init_te_state(g, n_poles=2, n_poles_y=1), m, n, source=src, monitors=[mon],
+                       checkpoint_every=100)
+    mon.flux(acc, DX).sum().backward()
+    torch.set_num_threads(8)
+    e = eps0.clone().requires_grad_(True)
+    c = yee2d.te_coeffs(NX, NY, DX, e, NPML, dtype=dt, **kw)
+    _, (oacc,) = yee2d.run2d("te", yee2d.te_state(NX, NY, dt, npx=2, npy=1), c, n,
+                             amps=[src.amplitude(t) for t in range(n)], mask=src.mask.cpu(),
+                             monitors=[{"type": "flux", "kind": "te_x", "idx": 80, "lo": 12, "hi": 52,
+                                        "freqs": FREQS}])
+    g_ref, = torch.autograd.grad(yee2d.flux_from_acc("te_x", oacc, DX).sum(), e)
+    err = rel(eps.grad, g_ref)
+    print(f"\n[te pole grad] rel-L2 {err:.2e}")
     assert float(g_ref[50:75, 16:48].abs().max()) > 0 and err <= 1e-9
