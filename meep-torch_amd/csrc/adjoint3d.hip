@@ -1,5 +1,5 @@
 // Reverse-time discrete adjoint of the 3D functional Yee step (CPML shells,
-// sigma = 0, soft sources, DFT monitors), for dL/deps.
+// material conductivity, soft sources, DFT monitors), for dL/deps.
 //
 // Forward step t (yee3d.hip) maps x_t -> x_{t+1}:
 //   H half  for each H component, two differences of the old E (÷ dx), each
@@ -21,6 +21,10 @@
 //           dL/deps += 1/2 of the eps_c cotangents of the components that
 //           average this cell
 // Identity parts (dE'/dE = 1, dH'/dH = 1) leave the adjoint arrays in place.
+// With conductivity the step is E' = d1 E + (d2 curl + source) / eps_c and
+// H' = h1 H - h2 curl: A scales dL/dE' by d1 in place, uses d2 for dt and
+// -(dL/dE') (E' - d1 E) / eps_c for the eps_c cotangent; B scales dL/dH' by
+// h1 and uses h2 for dt.
 #include <hip/hip_runtime.h>
 
 #include "adjoint2d.h"
@@ -47,6 +51,8 @@ struct A3 {
   T* g[12];
   T* geps[3];
   T* eps_grad;
+  // conductivity pairs (d1, d2) of ex ey ez hx hy hz, or all NULL
+  const T* dmp[12];
 };
 
 // Transpose of c = d + psi', psi' = b psi + a d (psi' only inside a shell):
@@ -73,9 +79,11 @@ __global__ __launch_bounds__(ABZ* ABY) void adj3_a(A3<T> p, const T* e0, const T
     const long long x = c_;
     const T ab = p.aEx[x];
     const T epsc = T(0.5) * (eps0 + p.eps[c_ + (long long)ny * nz]);
-    p.geps[0][x] = -ab * (e1[x] - e0[x]) / epsc;
+    const T d1 = p.dmp[0] ? p.dmp[0][x] : T(1), d2 = p.dmp[0] ? p.dmp[1][x] : dt;
+    p.geps[0][x] = -ab * (e1[x] - (p.dmp[0] ? d1 * e0[x] : e0[x])) / epsc;
+    if (p.dmp[0]) p.aEx[x] = d1 * ab;
     const bool jy = j >= 1 && j < ny - 1, kz = k >= 1 && k < nz - 1;
-    const T cb = (jy && kz) ? ab * dt / epsc : T(0);
+    const T cb = (jy && kz) ? ab * d2 / epsc : T(0);
     T gy = T(0), gz = T(0);
     if (jy) gy = ce_t(p.e[1], j - 1, p.apsi[P_HZ_Y], ((long long)i * (ny - 2) + (j - 1)) * nz + k, cb);
     if (kz) gz = ce_t(p.e[2], k - 1, p.apsi[P_HY_Z], ((long long)i * ny + j) * (nz - 2) + (k - 1), -cb);
@@ -86,9 +94,11 @@ __global__ __launch_bounds__(ABZ* ABY) void adj3_a(A3<T> p, const T* e0, const T
     const long long y = ((long long)i * (ny - 1) + j) * nz + k;
     const T ab = p.aEy[y];
     const T epsc = T(0.5) * (eps0 + p.eps[c_ + nz]);
-    p.geps[1][y] = -ab * (e1[nEx + y] - e0[nEx + y]) / epsc;
+    const T d1 = p.dmp[0] ? p.dmp[2][y] : T(1), d2 = p.dmp[0] ? p.dmp[3][y] : dt;
+    p.geps[1][y] = -ab * (e1[nEx + y] - (p.dmp[0] ? d1 * e0[nEx + y] : e0[nEx + y])) / epsc;
+    if (p.dmp[0]) p.aEy[y] = d1 * ab;
     const bool kz = k >= 1 && k < nz - 1, ix = i >= 1 && i < nx - 1;
-    const T cb = (kz && ix) ? ab * dt / epsc : T(0);
+    const T cb = (kz && ix) ? ab * d2 / epsc : T(0);
     T gz = T(0), gx = T(0);
     if (kz) gz = ce_t(p.e[2], k - 1, p.apsi[P_HX_Z], ((long long)i * (ny - 1) + j) * (nz - 2) + (k - 1), cb);
     if (ix) gx = ce_t(p.e[0], i - 1, p.apsi[P_HZ_X], ((long long)(i - 1) * (ny - 1) + j) * nz + k, -cb);
@@ -99,9 +109,11 @@ __global__ __launch_bounds__(ABZ* ABY) void adj3_a(A3<T> p, const T* e0, const T
     const long long z = ((long long)i * ny + j) * (nz - 1) + k;
     const T ab = p.aEz[z];
     const T epsc = T(0.5) * (eps0 + p.eps[c_ + 1]);
-    p.geps[2][z] = -ab * (e1[nEx + nEy + z] - e0[nEx + nEy + z]) / epsc;
+    const T d1 = p.dmp[0] ? p.dmp[4][z] : T(1), d2 = p.dmp[0] ? p.dmp[5][z] : dt;
+    p.geps[2][z] = -ab * (e1[nEx + nEy + z] - (p.dmp[0] ? d1 * e0[nEx + nEy + z] : e0[nEx + nEy + z])) / epsc;
+    if (p.dmp[0]) p.aEz[z] = d1 * ab;
     const bool ix = i >= 1 && i < nx - 1, jy = j >= 1 && j < ny - 1;
-    const T cb = (ix && jy) ? ab * dt / epsc : T(0);
+    const T cb = (ix && jy) ? ab * d2 / epsc : T(0);
     T gx = T(0), gy = T(0);
     if (ix) gx = ce_t(p.e[0], i - 1, p.apsi[P_HY_X], ((long long)(i - 1) * ny + j) * (nz - 1) + k, cb);
     if (jy) gy = ce_t(p.e[1], j - 1, p.apsi[P_HX_Y], ((long long)i * (ny - 2) + (j - 1)) * (nz - 1) + k, -cb);
@@ -124,28 +136,31 @@ __global__ __launch_bounds__(ABZ* ABY) void adj3_b(A3<T> p) {
     const long long h = ((long long)i * (ny - 1) + j) * (nz - 1) + k;
     T a = p.aHx[h] + (p.g[G_HX_Z][ey(i, j, k)] - p.g[G_HX_Z][ey(i, j, k + 1)]) +
           (p.g[G_HX_Y][ez(i, j, k)] - p.g[G_HX_Y][ez(i, j + 1, k)]);
-    p.aHx[h] = a;
-    // Hx' = Hx - dt (cy - cz): cy from Ez y-diff, cz from Ey z-diff
-    p.g[G_EZ_Y][h] = ce_t(p.h[1], j, p.apsi[P_EZ_Y], h, -dt * a) * rdx;
-    p.g[G_EY_Z][h] = ce_t(p.h[2], k, p.apsi[P_EY_Z], h, dt * a) * rdx;
+    const T h2 = p.dmp[0] ? p.dmp[7][h] : dt;
+    p.aHx[h] = p.dmp[0] ? p.dmp[6][h] * a : a;
+    // Hx' = h1 Hx - h2 (cy - cz): cy from Ez y-diff, cz from Ey z-diff
+    p.g[G_EZ_Y][h] = ce_t(p.h[1], j, p.apsi[P_EZ_Y], h, -h2 * a) * rdx;
+    p.g[G_EY_Z][h] = ce_t(p.h[2], k, p.apsi[P_EY_Z], h, h2 * a) * rdx;
   }
   if (i < nx - 1 && k < nz - 1) {  // Hy (nx-1, ny, nz-1)
     const long long h = ((long long)i * ny + j) * (nz - 1) + k;
     T a = p.aHy[h] + (p.g[G_HY_Z][ex(i, j, k)] - p.g[G_HY_Z][ex(i, j, k + 1)]) +
           (p.g[G_HY_X][ez(i, j, k)] - p.g[G_HY_X][ez(i + 1, j, k)]);
-    p.aHy[h] = a;
-    // Hy' = Hy - dt (cz - cx): cz from Ex z-diff, cx from Ez x-diff
-    p.g[G_EX_Z][h] = ce_t(p.h[2], k, p.apsi[P_EX_Z], h, -dt * a) * rdx;
-    p.g[G_EZ_X][h] = ce_t(p.h[0], i, p.apsi[P_EZ_X], h, dt * a) * rdx;
+    const T h2 = p.dmp[0] ? p.dmp[9][h] : dt;
+    p.aHy[h] = p.dmp[0] ? p.dmp[8][h] * a : a;
+    // Hy' = h1 Hy - h2 (cz - cx): cz from Ex z-diff, cx from Ez x-diff
+    p.g[G_EX_Z][h] = ce_t(p.h[2], k, p.apsi[P_EX_Z], h, -h2 * a) * rdx;
+    p.g[G_EZ_X][h] = ce_t(p.h[0], i, p.apsi[P_EZ_X], h, h2 * a) * rdx;
   }
   if (i < nx - 1 && j < ny - 1) {  // Hz (nx-1, ny-1, nz)
     const long long h = ((long long)i * (ny - 1) + j) * nz + k;
     T a = p.aHz[h] + (p.g[G_HZ_Y][ex(i, j, k)] - p.g[G_HZ_Y][ex(i, j + 1, k)]) +
           (p.g[G_HZ_X][ey(i, j, k)] - p.g[G_HZ_X][ey(i + 1, j, k)]);
-    p.aHz[h] = a;
-    // Hz' = Hz - dt (cx - cy): cx from Ey x-diff, cy from Ex y-diff
-    p.g[G_EY_X][h] = ce_t(p.h[0], i, p.apsi[P_EY_X], h, -dt * a) * rdx;
-    p.g[G_EX_Y][h] = ce_t(p.h[1], j, p.apsi[P_EX_Y], h, dt * a) * rdx;
+    const T h2 = p.dmp[0] ? p.dmp[11][h] : dt;
+    p.aHz[h] = p.dmp[0] ? p.dmp[10][h] * a : a;
+    // Hz' = h1 Hz - h2 (cx - cy): cx from Ey x-diff, cy from Ex y-diff
+    p.g[G_EY_X][h] = ce_t(p.h[0], i, p.apsi[P_EY_X], h, -h2 * a) * rdx;
+    p.g[G_EX_Y][h] = ce_t(p.h[1], j, p.apsi[P_EX_Y], h, h2 * a) * rdx;
   }
 }
 
@@ -225,6 +240,7 @@ int adj3_t(const mt_yee3d* d, const mt_yee3d_adj* a, const void* tape, const mt_
   }
   for (int q = 0; q < 3; ++q) p.geps[q] = static_cast<T*>(a->geps[q]);
   p.eps_grad = static_cast<T*>(a->eps_grad);
+  for (int q = 0; q < 12; ++q) p.dmp[q] = static_cast<const T*>(d->damp[q]);
   const long long slot = (long long)(p.nx - 1) * p.ny * p.nz + (long long)p.nx * (p.ny - 1) * p.nz +
                          (long long)p.nx * p.ny * (p.nz - 1);
   const T* tp = static_cast<const T*>(tape);
@@ -258,7 +274,9 @@ extern "C" int mt_yee3d_adjoint(const mt_yee3d* p, const mt_yee3d_adj* a, const 
   if ((long long)p->ny * p->nz > (1LL << 31) || p->nx > 65535)
     return fail(MT_EINVAL, "grid too large for the launch shape");
   if (p->lean) return fail(MT_EINVAL, "the 3D adjoint transposes the functional stepper (lean = 0)");
-  if (p->damp[0] || p->n_poles) return fail(MT_ENOTIMPL, "the 3D adjoint has no media (sigma or poles)");
+  if (p->n_poles) return fail(MT_ENOTIMPL, "the 3D adjoint has no ADE poles");
+  for (int q = 1; q < 12; ++q)
+    if (!p->damp[q] != !p->damp[0]) return fail(MT_EINVAL, "3D conductivity needs all 12 damping arrays");
   if (k < 0) return fail(MT_EINVAL, "negative step count");
   if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward E tape");
   if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");

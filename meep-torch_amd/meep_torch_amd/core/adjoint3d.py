@@ -13,8 +13,8 @@ whatever graph produced it), like the reference.  Scheme:
    the chunk backwards (monitor cotangents injected at every step) and
    accumulates dL/deps.
 
-Functional stepper (step_3d, sigma = 0, CPML) only; the lean stepper,
-material conductivity and ADE poles raise NotImplementedError.
+Functional stepper (step_3d, CPML, material conductivity) only; the lean
+stepper and ADE poles raise NotImplementedError.
 """
 import ctypes
 
@@ -39,9 +39,8 @@ def _fields_of(state):
 
 def run_3d_adjoint(c, n_steps: int, chunk: int, source, monitors, loss_fn, state0):
     from .adjoint2d import _adj_monitors
-    from .stepper_3d import has_media
-    if has_media(c):
-        raise NotImplementedError("3D adjoint through conductive or dispersive media is not supported")
+    if c.pols is not None:
+        raise NotImplementedError("3D adjoint through dispersive (ADE pole) media is not supported")
     if chunk < 1:
         raise ValueError("chunk must be >= 1")
     L = E.lib()
@@ -97,6 +96,13 @@ def run_3d_adjoint(c, n_steps: int, chunk: int, source, monitors, loss_fn, state
     for a, (kh, ke) in enumerate((("hx", "ex"), ("hy", "ey"), ("hz", "ez"))):
         d.h[a] = _axis(cd, kh)
         d.e[a] = _axis(cd, ke)
+    damp_keep = []
+    if getattr(c.damp, "active", False):          # conductivity (sigma_d, sigma_b)
+        for q, key in enumerate(("ex", "ey", "ez", "hx", "hy", "hz")):
+            for r, t in enumerate(c.damp[key]):
+                t = t.to(device=device, dtype=dtype).contiguous()
+                damp_keep.append(t)
+                d.damp[2 * q + r] = t.data_ptr()
     for t0, k, st in reversed(ckpts):
         for f in work:
             work[f].copy_(st[f])

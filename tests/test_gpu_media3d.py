@@ -186,5 +186,8 @@ def test_media_refuse_lean_and_adjoint():
     with pytest.raises(NotImplementedError, match="lean"):
         run_fields(work, c, 3, None, [], lean=True)
     from meep_torch_amd.core.adjoint3d import run_3d_adjoint
-    with pytest.raises(NotImplementedError, match="media"):
-        run_3d_adjoint(c, 10, 5, None, [], None, st)
+    mpp = _maps(sigma=False)
+    cp = build_3d_coeffs(NX, NY, NZ, DX, npml=NPML, dtype=torch.float64, device="cuda", eps=mpp["eps"].cuda(),
+                         poles=POLES, pole_masks=[m.cuda() for m in mpp["pole_masks"]])
+    with pytest.raises(NotImplementedError, match="pole"):
+        run_3d_adjoint(cp, 10, 5, None, [], None, st)
