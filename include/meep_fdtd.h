@@ -325,6 +325,10 @@ typedef struct {
   void* g[12];
   void* geps[3];
   void* eps_grad;
+  /* cotangents of the pole stacks P_cur[a], P_prev[a] of mt_yee3d when it
+   * has ADE poles (a = x, y, z); NULL otherwise */
+  void* aP_cur[3];
+  void* aP_prev[3];
 } mt_yee3d_adj;
 
 /* Reverse-time adjoint of k steps of mt_yee3d_run (lean = 0) from global step

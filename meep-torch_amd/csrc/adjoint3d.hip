@@ -21,6 +21,7 @@
 //           dL/deps += 1/2 of the eps_c cotangents of the components that
 //           average this cell
 // Identity parts (dE'/dE = 1, dH'/dH = 1) leave the adjoint arrays in place.
+// With poles, A also transposes the pole recurrence (pole3_adj).
 // With conductivity the step is E' = d1 E + (d2 curl + source) / eps_c and
 // H' = h1 H - h2 curl: A scales dL/dE' by d1 in place, uses d2 for dt and
 // -(dL/dE') (E' - d1 E) / eps_c for the eps_c cotangent; B scales dL/dH' by
@@ -53,7 +54,30 @@ struct A3 {
   T* eps_grad;
   // conductivity pairs (d1, d2) of ex ey ez hx hy hz, or all NULL
   const T* dmp[12];
+  // np ADE poles: scalars c1[n], c2[n]; per E component the c3 stack and the
+  // cotangents of the P / P_prev stacks (np component grids each)
+  int np;
+  const T *c1, *c2;
+  const T* c3[3];
+  T *aPc[3], *aPp[3];
 };
+
+// Transpose of one E cell's poles: P' = (c1 P + c2 P_prev) + c3 E and
+// E' = (d1 (E eps_c + sum P) + d2 curl - sum P') / eps_c, with r = E'b / eps_c.
+// P' gains -r, P gains d1 r + c1 P'b + P_prev'b (P_prev' = P), P_prev gains
+// c2 P'b; returns the old E's cotangent from the poles, sum c3 P'b.
+template <typename T>
+__device__ __forceinline__ T pole3_adj(const A3<T>& p, int a, long long q, long long nq, T d1, T r) {
+  T e_old = T(0);
+  for (int n = 0; n < p.np; ++n) {
+    const long long o = (long long)n * nq + q;
+    const T apn = p.aPc[a][o] - r, prev = p.aPp[a][o];
+    p.aPc[a][o] = d1 * r + p.c1[n] * apn + prev;
+    p.aPp[a][o] = p.c2[n] * apn;
+    e_old = e_old + p.c3[a][o] * apn;
+  }
+  return e_old;
+}
 
 // Transpose of c = d + psi', psi' = b psi + a d (psi' only inside a shell):
 // given the cotangent cb of c, returns the cotangent of d and updates the
@@ -81,7 +105,8 @@ __global__ __launch_bounds__(ABZ* ABY) void adj3_a(A3<T> p, const T* e0, const T
     const T epsc = T(0.5) * (eps0 + p.eps[c_ + (long long)ny * nz]);
     const T d1 = p.dmp[0] ? p.dmp[0][x] : T(1), d2 = p.dmp[0] ? p.dmp[1][x] : dt;
     p.geps[0][x] = -ab * (e1[x] - (p.dmp[0] ? d1 * e0[x] : e0[x])) / epsc;
-    if (p.dmp[0]) p.aEx[x] = d1 * ab;
+    if (p.np) p.aEx[x] = d1 * ab + pole3_adj(p, 0, x, nEx, d1, ab / epsc);
+    else if (p.dmp[0]) p.aEx[x] = d1 * ab;
     const bool jy = j >= 1 && j < ny - 1, kz = k >= 1 && k < nz - 1;
     const T cb = (jy && kz) ? ab * d2 / epsc : T(0);
     T gy = T(0), gz = T(0);
@@ -96,7 +121,8 @@ __global__ __launch_bounds__(ABZ* ABY) void adj3_a(A3<T> p, const T* e0, const T
     const T epsc = T(0.5) * (eps0 + p.eps[c_ + nz]);
     const T d1 = p.dmp[0] ? p.dmp[2][y] : T(1), d2 = p.dmp[0] ? p.dmp[3][y] : dt;
     p.geps[1][y] = -ab * (e1[nEx + y] - (p.dmp[0] ? d1 * e0[nEx + y] : e0[nEx + y])) / epsc;
-    if (p.dmp[0]) p.aEy[y] = d1 * ab;
+    if (p.np) p.aEy[y] = d1 * ab + pole3_adj(p, 1, y, nEy, d1, ab / epsc);
+    else if (p.dmp[0]) p.aEy[y] = d1 * ab;
     const bool kz = k >= 1 && k < nz - 1, ix = i >= 1 && i < nx - 1;
     const T cb = (kz && ix) ? ab * d2 / epsc : T(0);
     T gz = T(0), gx = T(0);
@@ -111,7 +137,8 @@ __global__ __launch_bounds__(ABZ* ABY) void adj3_a(A3<T> p, const T* e0, const T
     const T epsc = T(0.5) * (eps0 + p.eps[c_ + 1]);
     const T d1 = p.dmp[0] ? p.dmp[4][z] : T(1), d2 = p.dmp[0] ? p.dmp[5][z] : dt;
     p.geps[2][z] = -ab * (e1[nEx + nEy + z] - (p.dmp[0] ? d1 * e0[nEx + nEy + z] : e0[nEx + nEy + z])) / epsc;
-    if (p.dmp[0]) p.aEz[z] = d1 * ab;
+    if (p.np) p.aEz[z] = d1 * ab + pole3_adj(p, 2, z, (long long)nx * ny * (nz - 1), d1, ab / epsc);
+    else if (p.dmp[0]) p.aEz[z] = d1 * ab;
     const bool ix = i >= 1 && i < nx - 1, jy = j >= 1 && j < ny - 1;
     const T cb = (ix && jy) ? ab * d2 / epsc : T(0);
     T gx = T(0), gy = T(0);
@@ -241,6 +268,14 @@ int adj3_t(const mt_yee3d* d, const mt_yee3d_adj* a, const void* tape, const mt_
   for (int q = 0; q < 3; ++q) p.geps[q] = static_cast<T*>(a->geps[q]);
   p.eps_grad = static_cast<T*>(a->eps_grad);
   for (int q = 0; q < 12; ++q) p.dmp[q] = static_cast<const T*>(d->damp[q]);
+  p.np = (d->n_poles > 0 && a->aP_cur[0]) ? d->n_poles : 0;
+  p.c1 = static_cast<const T*>(d->c1);
+  p.c2 = static_cast<const T*>(d->c2);
+  for (int q = 0; q < 3; ++q) {
+    p.c3[q] = static_cast<const T*>(d->c3[q]);
+    p.aPc[q] = static_cast<T*>(a->aP_cur[q]);
+    p.aPp[q] = static_cast<T*>(a->aP_prev[q]);
+  }
   const long long slot = (long long)(p.nx - 1) * p.ny * p.nz + (long long)p.nx * (p.ny - 1) * p.nz +
                          (long long)p.nx * p.ny * (p.nz - 1);
   const T* tp = static_cast<const T*>(tape);
@@ -274,7 +309,13 @@ extern "C" int mt_yee3d_adjoint(const mt_yee3d* p, const mt_yee3d_adj* a, const 
   if ((long long)p->ny * p->nz > (1LL << 31) || p->nx > 65535)
     return fail(MT_EINVAL, "grid too large for the launch shape");
   if (p->lean) return fail(MT_EINVAL, "the 3D adjoint transposes the functional stepper (lean = 0)");
-  if (p->n_poles) return fail(MT_ENOTIMPL, "the 3D adjoint has no ADE poles");
+  if (p->n_poles < 0) return fail(MT_EINVAL, "negative pole count");
+  if (p->n_poles) {
+    if (!p->damp[0] || !p->c1 || !p->c2) return fail(MT_EINVAL, "3D adjoint with poles needs damp, c1 and c2");
+    for (int q = 0; q < 3; ++q)
+      if (!p->c3[q] || !a->aP_cur[q] || !a->aP_prev[q])
+        return fail(MT_EINVAL, "3D adjoint with poles needs c3 and the pole-stack cotangents");
+  }
   for (int q = 1; q < 12; ++q)
     if (!p->damp[q] != !p->damp[0]) return fail(MT_EINVAL, "3D conductivity needs all 12 damping arrays");
   if (k < 0) return fail(MT_EINVAL, "negative step count");

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 15
+ABI_VERSION = 16
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -91,7 +91,8 @@ class Yee3D(ctypes.Structure):
 class Yee3DAdj(ctypes.Structure):
     _fields_ = [("aEx", c_void_p), ("aEy", c_void_p), ("aEz", c_void_p), ("aHx", c_void_p),
                 ("aHy", c_void_p), ("aHz", c_void_p), ("apsi", c_void_p * 12), ("g", c_void_p * 12),
-                ("geps", c_void_p * 3), ("eps_grad", c_void_p)]
+                ("geps", c_void_p * 3), ("eps_grad", c_void_p), ("aP_cur", c_void_p * 3),
+                ("aP_prev", c_void_p * 3)]
 
 
 class Source3D(ctypes.Structure):

@@ -13,8 +13,8 @@ whatever graph produced it), like the reference.  Scheme:
    the chunk backwards (monitor cotangents injected at every step) and
    accumulates dL/deps.
 
-Functional stepper (step_3d, CPML, material conductivity) only; the lean
-stepper and ADE poles raise NotImplementedError.
+Functional stepper (step_3d, CPML, material conductivity and ADE poles)
+only; the lean stepper raises NotImplementedError.
 """
 import ctypes
 
@@ -22,7 +22,7 @@ import torch
 
 from .. import _engine as E
 from .engine3d import _axis, run_fields
-from .stepper_3d import COMPS, PSI_NAMES
+from .stepper_3d import COMPS, PSI_NAMES, state_names
 
 __all__ = ["run_3d_adjoint"]
 
@@ -32,15 +32,11 @@ _G_SHAPES = ("Ex", "Ex", "Ey", "Ey", "Ez", "Ez", "Hx", "Hx", "Hy", "Hy", "Hz", "
 def _fields_of(state):
     if state.psi_Ez_y is None:
         raise NotImplementedError("3D adjoint needs the CPML state (psi arrays)")
-    if state.Px_cur is not None:
-        raise NotImplementedError("3D adjoint with ADE poles is not supported")
-    return {f: getattr(state, f).detach().clone().contiguous() for f in COMPS + PSI_NAMES}
+    return {f: getattr(state, f).detach().clone().contiguous() for f in state_names(state)}
 
 
 def run_3d_adjoint(c, n_steps: int, chunk: int, source, monitors, loss_fn, state0):
     from .adjoint2d import _adj_monitors
-    if c.pols is not None:
-        raise NotImplementedError("3D adjoint through dispersive (ADE pole) media is not supported")
     if chunk < 1:
         raise ValueError("chunk must be >= 1")
     L = E.lib()
@@ -85,6 +81,12 @@ def run_3d_adjoint(c, n_steps: int, chunk: int, source, monitors, loss_fn, state
     for q in range(3):
         adj.geps[q] = geps[q].data_ptr()
     adj.eps_grad = eps_grad.data_ptr()
+    if c.pols is not None:
+        if "Px_cur" not in A:
+            raise ValueError("3D poles need the P_cur / P_prev stacks in the state")
+        for q, ax in enumerate("xyz"):
+            adj.aP_cur[q] = A[f"P{ax}_cur"].data_ptr()
+            adj.aP_prev[q] = A[f"P{ax}_prev"].data_ptr()
     mon_arr, n_used, keep = _adj_monitors(A, monitors, list(gaccs), dtype, device)
     slot = sum(work[f].numel() for f in ("Ex", "Ey", "Ez"))
     tape = torch.empty((chunk + 1) * slot, dtype=dtype, device=device)
@@ -97,12 +99,22 @@ def run_3d_adjoint(c, n_steps: int, chunk: int, source, monitors, loss_fn, state
         d.h[a] = _axis(cd, kh)
         d.e[a] = _axis(cd, ke)
     damp_keep = []
-    if getattr(c.damp, "active", False):          # conductivity (sigma_d, sigma_b)
+    if getattr(c.damp, "active", False) or c.pols is not None:   # conductivity pairs (ones / dt without)
         for q, key in enumerate(("ex", "ey", "ez", "hx", "hy", "hz")):
             for r, t in enumerate(c.damp[key]):
                 t = t.to(device=device, dtype=dtype).contiguous()
                 damp_keep.append(t)
                 d.damp[2 * q + r] = t.data_ptr()
+    if c.pols is not None:                                       # ADE poles
+        d.n_poles = int(c.pols["x"][0].numel())
+        for name, t in (("c1", c.pols["x"][0].reshape(-1)), ("c2", c.pols["x"][1].reshape(-1))):
+            t = t.to(device=device, dtype=dtype).contiguous()
+            damp_keep.append(t)
+            setattr(d, name, t.data_ptr())
+        for q, ax in enumerate("xyz"):
+            t = c.pols[ax][2].to(device=device, dtype=dtype).contiguous()
+            damp_keep.append(t)
+            d.c3[q] = t.data_ptr()
     for t0, k, st in reversed(ckpts):
         for f in work:
             work[f].copy_(st[f])

@@ -87,10 +87,13 @@ def test_gradient_reaches_upstream_graph():
     assert abs(float(w.grad) - float(ref)) <= 1e-10 * abs(float(ref))
 
 
-def test_conductive_media_gradient_matches_oracle_autograd():
+@pytest.mark.parametrize("poles", [False, True], ids=["sigma", "sigma+poles"])
+def test_media_gradient_matches_oracle_autograd(poles):
     """With sigma_d and sigma_b (E' = d1 E + (d2 curl + source) / eps_c,
-    H' = h1 H - h2 curl) the 3D adjoint's eps gradient equals CPU autograd
-    through the oracle's 3D steps (FP64), across checkpoint chunks."""
+    H' = h1 H - h2 curl), and with a Lorentz and a Drude pole on top (the
+    pole stacks' cotangents carried between steps), the 3D adjoint's eps
+    gradient equals CPU autograd through the oracle's 3D steps (FP64),
+    across checkpoint chunks."""
     from meep_torch_amd.core.adjoint3d import run_3d_adjoint
     from meep_torch_amd.core.monitors_3d import DFTFieldsMonitor3D, PointSource3D
     from meep_torch_amd.core.sources import GaussianPulse
@@ -103,22 +106,25 @@ def test_conductive_media_gradient_matches_oracle_autograd():
     eps0 = 1.0 + 2.0 * torch.rand(nx, ny, nz, generator=gen, dtype=torch.float64) * inner
     sd = 2.0 * torch.rand(nx, ny, nz, generator=gen, dtype=torch.float64) * inner
     sb = 1.0 * torch.rand(nx, ny, nz, generator=gen, dtype=torch.float64) * inner
+    half = (torch.rand(nx, ny, nz, generator=gen, dtype=torch.float64) > 0.5).double() * inner
+    pk = dict(poles=[(1.1, 0.15, 1.3), (0.0, 0.4, 2.0, 0.9)], pole_masks=[half, inner - half]) if poles else {}
     eps = eps0.clone().cuda().requires_grad_(True)
     c = build_3d_coeffs(nx, ny, nz, 0.05, eps=eps, npml=6, sigma_d=sd.cuda(), sigma_b=sb.cuda(),
-                        dtype=torch.float64, device="cuda")
+                        dtype=torch.float64, device="cuda",
+                        **({k: ([m.cuda() for m in v] if k == "pole_masks" else v) for k, v in pk.items()}))
     prof = GaussianPulse(0.6, 0.25)
     src = PointSource3D(c.dt, "Ez", nx // 2, ny // 2, nz // 2 - 3, prof)
     mon = DFTFieldsMonitor3D(c.dt, [0.6], "Ez", plane=(0, nx - 10), dtype=torch.float64)
-    st0 = init_3d_state(nx, ny, nz, npml=6, dtype=torch.float64, device="cuda")
+    st0 = init_3d_state(nx, ny, nz, npml=6, n_poles=2 if poles else 0, dtype=torch.float64, device="cuda")
     run_3d_adjoint(c, steps, 16, src, [mon], _loss, st0)
     torch.set_num_threads(8)
     e = eps0.clone().requires_grad_(True)
-    oc = yee3d.coeffs3d(nx, ny, nz, 0.05, e, 6, torch.float64, sigma_d=sd, sigma_b=sb)
+    oc = yee3d.coeffs3d(nx, ny, nz, 0.05, e, 6, torch.float64, sigma_d=sd, sigma_b=sb, **pk)
     amps = [prof.amplitude((t + 0.5) * oc["dt"]) for t in range(steps)]
-    _, oacc = yee3d.run3d(yee3d.state3d(nx, ny, nz, torch.float64), oc, steps, amps,
+    _, oacc = yee3d.run3d(yee3d.state3d(nx, ny, nz, torch.float64, n_poles=2 if poles else 0), oc, steps, amps,
                           ("Ez", nx // 2, ny // 2, nz // 2 - 3),
                           [{"type": "dft", "comp": "Ez", "plane": (0, nx - 10), "freq": 0.6}])
     g_ref, = torch.autograd.grad(_loss(oacc), e)
     err = _rel(eps.grad.cpu(), g_ref)
-    print(f"\n[adjoint3d sigma] grad rel-L2 vs oracle autograd {err:.2e}")
+    print(f"\n[adjoint3d media poles={poles}] grad rel-L2 vs oracle autograd {err:.2e}")
     assert float(g_ref[9:21, 8:16, 7:13].abs().max()) > 0 and err <= 1e-9

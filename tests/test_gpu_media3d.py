@@ -175,7 +175,7 @@ def test_step_wrapper_with_media_from_a_random_state():
     assert torch.equal(st.Ex.cpu(), rnd["Ex"])          # the input state is left alone
 
 
-def test_media_refuse_lean_and_adjoint():
+def test_media_refuse_the_lean_stepper():
     from meep_torch_amd.core.engine3d import run_fields
     from meep_torch_amd.core.stepper_3d import build_3d_coeffs, init_3d_state, state_names
     mp = _maps(poles=False)
@@ -185,9 +185,3 @@ def test_media_refuse_lean_and_adjoint():
     work = {f: getattr(st, f).clone() for f in state_names(st)}
     with pytest.raises(NotImplementedError, match="lean"):
         run_fields(work, c, 3, None, [], lean=True)
-    from meep_torch_amd.core.adjoint3d import run_3d_adjoint
-    mpp = _maps(sigma=False)
-    cp = build_3d_coeffs(NX, NY, NZ, DX, npml=NPML, dtype=torch.float64, device="cuda", eps=mpp["eps"].cuda(),
-                         poles=POLES, pole_masks=[m.cuda() for m in mpp["pole_masks"]])
-    with pytest.raises(NotImplementedError, match="pole"):
-        run_3d_adjoint(cp, 10, 5, None, [], None, st)
