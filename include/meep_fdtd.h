@@ -42,11 +42,17 @@ typedef struct {
 /* Soft source on a 2D field: dense weights on the bounding box
  * [i0,i1) x [j0,j1) of the reference's mask (sources.py:49-78); at step s the
  * field loses (coef[s] cast to dtype) * w (stepper_tm.py:197-200,
- * stepper_te.py:131-134).  coef is a HOST array of k doubles = dt * amp(t). */
+ * stepper_te.py:131-134).  coef is a HOST array of k doubles = dt * amp(t).
+ * wstep = 0: one weight box for every step; wstep > 0 (electric sources of
+ * mt_tm2d_run / mt_te2d_run only): step s of the call uses the box at
+ * w + s * wstep elements -- the per-step sum of several sources'
+ * amp_i(t) * mask_i that the reference injects as one mask
+ * (mp/cpp/core.py:584-589), with coef[s] = dt. */
 typedef struct {
   int present;
   int i0, i1, j0, j1;
-  const void* w;                          /* device (i1-i0, j1-j0) */
+  const void* w;                          /* device (i1-i0, j1-j0) [per step] */
+  long long wstep;
 } mt_source2d;
 
 /* 2D TM problem + state (meep_torch/core/coeffs.py:21-74 TMCoeffs and
@@ -157,11 +163,13 @@ typedef struct {
    * slot s = tape + s * (|Ex| + |Ey| + |Ez|) as [Ex | Ey | Ez]
    * (reference shapes), for the adjoint's eps gradient */
   void* tape;
-  /* variant 0: two passes per step (H kernel, E kernel); 1-3: one
-   * fused pass per step streaming along x (tile rows 8, 4, 16), which needs alt:
-   * second buffers of the 6 fields then the 12 psi arrays (same shapes;
-   * the psi ones zero-filled), used ping-pong; the state ends in the
-   * primary buffers either way */
+  /* variant 0: two passes per step (H kernel, E kernel), in place; 1-3:
+   * split step -- one pass streaming along x over the psi-free interior box
+   * (tile rows 6, 14, 4; FP64 uses 6 for variant 2) plus the per-cell H and
+   * E updates over the CPML shell pieces -- which needs alt[0..5]: second
+   * buffers of Ex Ey Ez Hx Hy Hz (same shapes), used ping-pong (psi stay in
+   * place; alt[6..17] are unused); the state ends in the primary buffers
+   * either way.  Media run the two-pass kernels whatever the variant. */
   int variant, pad2_;
   void* alt[18];
   /* Media of the functional stepper (stepper_3d.py:217-266,295-301); two-pass
@@ -232,6 +240,21 @@ typedef struct {
  * launch) and tx[v], ty[v] (owned tile rows/columns) when the arrays are
  * non-NULL.  Used by the host-side autotuner. */
 int mt_tiled_variants(int dtype, int* steps, int* tx, int* ty);
+
+/* Tile counts of the temporally blocked launch that mt_tm2d_run /
+ * mt_te2d_run (and the batched runner) make for p->variant on this problem:
+ * interior tiles (no CPML psi in their LDS region) and ring tiles.  Host
+ * only, no device work.  No counterpart in the reference: an introspection
+ * aid so tests can show that a grid exercises the interior kernels. */
+int mt_tm2d_tiles(const mt_tm2d* p, int* n_interior, int* n_ring);
+int mt_te2d_tiles(const mt_te2d* p, int* n_interior, int* n_ring);
+
+/* RN(1/b) if the tiled kernels' constant-divisor sequence (q0 = a*y, one
+ * FMA correction) equals IEEE a / b for every FP32 dividend, checked
+ * exhaustively on the host (cached per b); 0 otherwise, in which case the
+ * kernels divide.  Host only.  No counterpart in the reference, whose steps
+ * divide by dx with torch's true division (stepper_tm.py:86-89). */
+float mt_div_recip_f32(float b);
 
 /* Bytes of staging workspace one call needs for `stage_steps` buffered steps. */
 long long mt_stage_bytes(const mt_monitor* mons, int n_mons, int stage_steps, int dtype);
@@ -356,6 +379,10 @@ typedef struct {
   int n_mons, pad_;
   void* stage;
   long long stage_bytes;
+  /* magnetic (TFSF) currents prob.hsrc[0..1] (ports/mode_source.py:60-81):
+   * device table hcoef_tab[t] = dt*h_amplitude(t) for every global step t
+   * (dtype of the problem); NULL without them */
+  const void* hcoef_tab;
 } mt_tm2d_batch_item;
 
 typedef struct mt_batch mt_batch;       /* opaque host-side plan */

@@ -99,7 +99,8 @@ __global__ __launch_bounds__(kCellBlk) void adj_batch_inject(const mt_monitor* i
 }
 
 template <typename T>
-TmTiled<T> tiled_of(const mt_tm2d* d, const mt_source2d* src, const void* coef_tab, int variant, bool swap) {
+TmTiled<T> tiled_of(const mt_tm2d* d, const mt_source2d* src, const void* coef_tab, const void* hcoef_tab,
+                    int variant, bool swap) {
   TmTiled<T> p{};
   p.nx = d->nx;
   p.ny = d->ny;
@@ -107,7 +108,7 @@ TmTiled<T> tiled_of(const mt_tm2d* d, const mt_source2d* src, const void* coef_t
   p.ny_e = d->y_pmc ? d->ny : d->ny - 2;
   p.dt = static_cast<T>(d->dt);
   p.dx = static_cast<T>(d->dx);
-  p.rdx = T(1) / p.dx;
+  p.rdx = const_div_recip<T>(p.dx);
   p.eps = static_cast<const T*>(d->eps);
   p.hy = pml_of<T>(d->hy);
   p.hx = pml_of<T>(d->hx);
@@ -133,6 +134,9 @@ TmTiled<T> tiled_of(const mt_tm2d* d, const mt_source2d* src, const void* coef_t
   p.tape = static_cast<T*>(d->tape);
   if (src && src->present) p.src = *src;
   p.coef_tab = static_cast<const T*>(coef_tab);
+  p.hsrc[0] = d->hsrc[0];
+  p.hsrc[1] = d->hsrc[1];
+  p.hcoef_tab = static_cast<const T*>(hcoef_tab);
   return p;
 }
 
@@ -144,8 +148,8 @@ int create_t(mt_batch* b, const mt_tm2d_batch_item* items, hipStream_t s) {
   std::vector<double> dts(nvar);
   for (int v = 0; v < nvar; ++v) {
     const mt_tm2d_batch_item& it = items[v];
-    vars[v].pA = tiled_of<T>(&it.prob, &it.src, it.coef_tab, b->variant, false);
-    vars[v].pB = tiled_of<T>(&it.prob, &it.src, it.coef_tab, b->variant, true);
+    vars[v].pA = tiled_of<T>(&it.prob, &it.src, it.coef_tab, it.hcoef_tab, b->variant, false);
+    vars[v].pB = tiled_of<T>(&it.prob, &it.src, it.coef_tab, it.hcoef_tab, b->variant, true);
     MonitorPlan mp{it.mons, it.n_mons, it.stage, it.stage_bytes, b->S, b->dtype};
     vars[v].taps.n = 0;
     if (it.n_mons && !build_taps(mp, 0, 0, sizeof(T), &vars[v].taps))
@@ -312,10 +316,11 @@ extern "C" mt_batch* mt_tm2d_batch_create(const mt_tm2d_batch_item* items, int n
         fail(MT_EINVAL, "batched run: missing ping-pong buffers");
         return nullptr;
       }
-    if (d.hsrc[0].present || d.hsrc[1].present) {
-      fail(MT_EINVAL, "batched run: magnetic (TFSF) sources are not supported");
+    if ((d.hsrc[0].present || d.hsrc[1].present) && !items[v].hcoef_tab) {
+      fail(MT_EINVAL, "batched run: magnetic (TFSF) source without its coefficient table");
       return nullptr;
     }
+    if (check_src(&d.hsrc[0], d.nx, d.ny - 1) || check_src(&d.hsrc[1], d.nx - 1, d.ny)) return nullptr;
     if (d.n_poles || d.kerr || d.d1 || d.h1x) {
       fail(MT_EINVAL, "batched run: dispersive, nonlinear or conductive media are not supported");
       return nullptr;
@@ -324,7 +329,8 @@ extern "C" mt_batch* mt_tm2d_batch_create(const mt_tm2d_batch_item* items, int n
         check_axis(d.ey, d.y_pmc ? d.ny : d.ny - 2, "ey"))
       return nullptr;
     const mt_source2d& sr = items[v].src;
-    if (sr.present && (!items[v].coef_tab || sr.i0 < 0 || sr.j0 < 0 || sr.i1 > d.nx || sr.j1 > d.ny || !sr.w)) {
+    if (sr.present && (!items[v].coef_tab || sr.i0 < 0 || sr.j0 < 0 || sr.i1 > d.nx || sr.j1 > d.ny || !sr.w ||
+                       sr.wstep != 0)) {
       fail(MT_EINVAL, "batched run: bad source");
       return nullptr;
     }

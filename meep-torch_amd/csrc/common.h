@@ -28,18 +28,33 @@ inline Pml<T> pml_of(const mt_pml_axis& ax) {
 }
 
 int check_axis(const mt_pml_axis& ax, int n, const char* name);
+// A 2D source box inside an n0 x n1 field (or absent).
+int check_src(const mt_source2d* s, int n0, int n1);
+
+// RN(1/b) when div_c's fast sequence equals IEEE division for every FP32
+// dividend with divisor b (checked exhaustively on the host once per b and
+// cached), else 0; div_c then divides.
+float div_recip_f32(float b);
+template <typename T>
+inline T const_div_recip(T b);
+template <>
+inline float const_div_recip<float>(float b) { return div_recip_f32(b); }
+template <>
+inline double const_div_recip<double>(double) { return 0.0; }
 
 // a / b for the constant grid step b > 0 with y = RN(1/b): q0 = a y, then
-// one FMA correction with the exact remainder a - q0 b gives the correctly
-// rounded quotient (Markstein's theorem: y within half an ulp of 1/b, q0
-// within one ulp of a/b, exact remainder; checked against IEEE division over
-// |a| in [2^-100, 2^100] by tests/test_div_const_cpu.py).  a = +-0 gives
-// +-0 exactly (the correction would turn -0 into +0, so zero is selected
-// explicitly); the rest of the range (tiny or huge |a|) takes the true
+// one FMA correction with the exact remainder a - q0 b (Markstein's
+// sequence).  That this equals the IEEE quotient is not assumed from a
+// theorem: the host passes y only for FP32 divisors whose sequence it has
+// checked against IEEE division for every significand (const_div_recip),
+// and y = 0 otherwise (FP64 included), which selects true division for
+// every operand.  a = +-0 gives +-0 exactly (the correction would turn -0
+// into +0, so zero is selected explicitly); tiny or huge |a| takes the true
 // division.  Zero is common (fields ahead of a wavefront), so keeping it on
 // the fast path spares whole waves the divergent division branch.
 template <typename T>
 __device__ __forceinline__ T div_c(T a, T b, T y) {
+  if (y == T(0)) return a / b;
   T aa = a < T(0) ? -a : a;
   if ((aa >= T(0x1p-100) && aa <= T(0x1p100)) || aa == T(0)) {
     T q0 = a * y;

@@ -15,6 +15,10 @@
 
 #include <cmath>
 #include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "common.h"
@@ -34,6 +38,48 @@ int fail(int code, const std::string& msg) {
 int check_launch(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(MT_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return MT_OK;
+}
+
+// Exhaustive host check of div_c's quotient sequence for one FP32 divisor b:
+// every significand of a in [1, 2).  The sequence scales exactly with a's
+// exponent while q0, r and q1 stay normal, which the range guards of div_c
+// (|a| in [2^-100, 2^100]) and of this function (b in [2^-16, 2^16]) ensure,
+// so passing here covers every operand div_c sends down the fast path.
+static bool div_const_exact_f32(float b) {
+  if (!(b >= 0x1p-16f && b <= 0x1p16f)) return false;
+  volatile float vb = b;
+  const float y = 1.0f / vb;
+  for (uint32_t m = 0; m < (1u << 23); ++m) {
+    uint32_t w = (127u << 23) | m;
+    float a;
+    std::memcpy(&a, &w, 4);
+    volatile float va = a;
+    const float q = va / vb;
+    const float q0 = a * y;
+    const float r = std::fma(-q0, b, a);
+    const float q1 = std::fma(r, y, q0);
+    if (std::memcmp(&q, &q1, 4) != 0) return false;
+  }
+  return true;
+}
+
+float div_recip_f32(float b) {
+  static std::mutex mu;
+  static std::map<uint32_t, bool> seen;
+  uint32_t key;
+  std::memcpy(&key, &b, 4);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = seen.find(key);
+  if (it == seen.end()) it = seen.emplace(key, div_const_exact_f32(b)).first;
+  return it->second ? 1.0f / b : 0.0f;
+}
+
+int check_src(const mt_source2d* s, int n0, int n1) {
+  if (!s || !s->present) return MT_OK;
+  if (s->wstep < 0) return fail(MT_EINVAL, "source: negative weight step");
+  if (s->i0 < 0 || s->j0 < 0 || s->i1 > n0 || s->j1 > n1 || s->i0 >= s->i1 || s->j0 >= s->j1 || !s->w)
+    return fail(MT_EINVAL, "source box outside the field");
   return MT_OK;
 }
 
@@ -414,7 +460,7 @@ extern "C" long long mt_stage_bytes(const mt_monitor* mons, int n_mons, int stag
 
 extern "C" const char* mt_last_error(void) { return mt::g_err.c_str(); }
 
-extern "C" int mt_abi_version(void) { return 16; }
+extern "C" int mt_abi_version(void) { return 18; }
 
 extern "C" int mt_abi_layout(long long* out, int n) {
   const long long v[] = {

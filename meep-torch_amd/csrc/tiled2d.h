@@ -57,6 +57,11 @@ struct TmTiled {
   mt_source2d src;
   T coef[kMaxSub];
   const T* coef_tab;  // batched runs: dt*amp for every global step (else NULL)
+  // magnetic (TFSF) currents of the B half on Hx (hsrc[0]) and Hy (hsrc[1]):
+  // H -= hc * w after the curl update, hc = hcoef[substep] or hcoef_tab[step]
+  mt_source2d hsrc[2];
+  T hcoef[kMaxSub];
+  const T* hcoef_tab;
 };
 
 // One variant of a batched TM run: its problem in both ping-pong

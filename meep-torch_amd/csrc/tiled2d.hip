@@ -273,7 +273,7 @@ __global__ __launch_bounds__(te_block(PML)) void te_tiled(TeTiled<T> p, TapSet t
           dyv = dyv - dt * curl;
         }
         if (src_tile && i >= p.src.i0 && i < p.src.i1 && j >= p.src.j0 && j < p.src.j1)
-          dyv = dyv - coef * static_cast<const T*>(p.src.w)[(long long)(i - p.src.i0) * (p.src.j1 - p.src.j0) +
+          dyv = dyv - coef * static_cast<const T*>(p.src.w)[(long long)s * p.src.wstep + (long long)(i - p.src.i0) * (p.src.j1 - p.src.j0) +
                                                              (j - p.src.j0)];
         vDy[u] = dyv;
         sDy[c] = dyv;
@@ -411,9 +411,14 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
   // uniform per tile: does the source reach the LDS region, do taps reach the owned block
   const bool src_tile = p.src.present && p.src.i0 < a0 + RX && p.src.i1 > a0 && p.src.j0 < b0 + RY &&
                         p.src.j1 > b0;
+  const bool hx_tile = p.hsrc[0].present && p.hsrc[0].i0 < a0 + RX && p.hsrc[0].i1 > a0 &&
+                       p.hsrc[0].j0 < b0 + RY && p.hsrc[0].j1 > b0;
+  const bool hy_tile = p.hsrc[1].present && p.hsrc[1].i0 < a0 + RX && p.hsrc[1].i1 > a0 &&
+                       p.hsrc[1].j0 < b0 + RY && p.hsrc[1].j1 > b0;
   const bool tap_tile = taps.n && taps_touch(taps, I0, J0, TX, TY);
   for (int s = 0; s < nsub; ++s) {
     const T coef = p.coef_tab ? p.coef_tab[coef_t0 + s] : p.coef[s];
+    const T hc = p.hcoef_tab ? p.hcoef_tab[coef_t0 + s] : p.hcoef[s];
     // B half (stepper_tm.py:85-92,126-128)
     int li = tid / RY, lj = tid % RY;
 #pragma unroll
@@ -435,7 +440,12 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
           sPzy[c] = ps;
           curl = d + ps;
         }
-        sHx[c] = sHx[c] - dt * curl;
+        T hxv = sHx[c] - dt * curl;
+        if (hx_tile && i >= p.hsrc[0].i0 && i < p.hsrc[0].i1 && j >= p.hsrc[0].j0 && j < p.hsrc[0].j1)
+          hxv = hxv - hc * static_cast<const T*>(p.hsrc[0].w)[(long long)(i - p.hsrc[0].i0) *
+                                                                   (p.hsrc[0].j1 - p.hsrc[0].j0) +
+                                                               (j - p.hsrc[0].j0)];  // stepper_tm.py:190-191
+        sHx[c] = hxv;
       }
       if ((!PML || i < nx - 1) && li < RX - 1) {
         T d = div_c(sEz[c + RY] - ez, dx, rdx);
@@ -445,7 +455,12 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
           sPzx[c] = ps;
           curl = d + ps;
         }
-        sHy[c] = sHy[c] + dt * curl;
+        T hyv = sHy[c] + dt * curl;
+        if (hy_tile && i >= p.hsrc[1].i0 && i < p.hsrc[1].i1 && j >= p.hsrc[1].j0 && j < p.hsrc[1].j1)
+          hyv = hyv - hc * static_cast<const T*>(p.hsrc[1].w)[(long long)(i - p.hsrc[1].i0) *
+                                                                   (p.hsrc[1].j1 - p.hsrc[1].j0) +
+                                                               (j - p.hsrc[1].j0)];  // stepper_tm.py:188-189
+        sHy[c] = hyv;
       }
     }
     __syncthreads();
@@ -485,7 +500,7 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
         dz = dz + dt * curl;
       }
       if (src_tile && i >= p.src.i0 && i < p.src.i1 && j >= p.src.j0 && j < p.src.j1)
-        dz = dz - coef * static_cast<const T*>(p.src.w)[(long long)(i - p.src.i0) * (p.src.j1 - p.src.j0) +
+        dz = dz - coef * static_cast<const T*>(p.src.w)[(long long)s * p.src.wstep + (long long)(i - p.src.i0) * (p.src.j1 - p.src.j0) +
                                                         (j - p.src.j0)];
       vDz[u] = dz;
       sDz[c] = dz;

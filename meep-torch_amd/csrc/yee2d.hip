@@ -49,6 +49,15 @@ __device__ __forceinline__ T box_w(const mt_source2d& s, int i, int j) {
   return static_cast<const T*>(s.w)[(long long)(i - s.i0) * (s.j1 - s.j0) + (j - s.j0)];
 }
 
+// The source of step s of a call: per-step weight boxes (wstep > 0) advance
+// the weight pointer; a static box is used as is.
+template <typename T>
+mt_source2d step_source(const mt_source2d& s, int step) {
+  mt_source2d v = s;
+  if (v.present && v.wstep) v.w = static_cast<const T*>(v.w) + (long long)step * v.wstep;
+  return v;
+}
+
 // B half plus the optional TFSF magnetic currents (stepper_tm.py:169-193)
 template <typename T>
 __global__ __launch_bounds__(BX* BY) void tm_b_kernel(TmK<T> p, mt_source2d hxs, mt_source2d hys, T hc) {
@@ -328,13 +337,6 @@ __global__ __launch_bounds__(BX* BY) void te_b_kernel(TeK<T> p, mt_source2d hzs,
   p.Hz[h] = v;
 }
 
-int check_src(const mt_source2d* s, int n0, int n1) {
-  if (!s || !s->present) return MT_OK;
-  if (s->i0 < 0 || s->j0 < 0 || s->i1 > n0 || s->j1 > n1 || s->i0 >= s->i1 || s->j0 >= s->j1 || !s->w)
-    return fail(MT_EINVAL, "source box outside the field");
-  return MT_OK;
-}
-
 template <typename T>
 int tm_run_t(const mt_tm2d* d, const mt_source2d* src, const double* coef, const MonitorPlan& mp,
              int step0, int k, hipStream_t s) {
@@ -383,7 +385,7 @@ int tm_run_t(const mt_tm2d* d, const mt_source2d* src, const double* coef, const
     const T hc = d->hcoef ? static_cast<T>(d->hcoef[s_]) : T(0);
     hipLaunchKernelGGL(tm_b_kernel<T>, grd, blk, 0, s, p, d->hsrc[0], d->hsrc[1], hc);
     T c = sv.present ? static_cast<T>(coef[s_]) : T(0);
-    hipLaunchKernelGGL(tm_d_kernel<T>, grd, blk, 0, s, p, sv, c, s_);
+    hipLaunchKernelGGL(tm_d_kernel<T>, grd, blk, 0, s, p, step_source<T>(sv, s_), c, s_);
     int rc = check_launch("tm2d step");
     if (rc) return rc;
     if (mp.n_mons) {
@@ -454,7 +456,7 @@ int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const
   int filled = 0, first = step0;
   for (int s_ = 0; s_ < k; ++s_) {
     T c = sv.present ? static_cast<T>(coef[s_]) : T(0);
-    hipLaunchKernelGGL(te_d_kernel<T>, grd, blk, 0, s, p, sv, c, s_);
+    hipLaunchKernelGGL(te_d_kernel<T>, grd, blk, 0, s, p, step_source<T>(sv, s_), c, s_);
     if (p.kerr) {  // three passes: scratch -> E -> scratch -> E
       hipLaunchKernelGGL(te_kerr_kernel<T>, grd, blk, 0, s, p, p.kx, p.ky, p.Ex, p.Ey);
       hipLaunchKernelGGL(te_kerr_kernel<T>, grd, blk, 0, s, p, p.Ex, p.Ey, p.kx, p.ky);
@@ -476,6 +478,31 @@ int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const
   return MT_OK;
 }
 
+// Cells whose tiles carry no CPML psi: rows [i0, i1) x columns [j0, j1).
+void tm_interior(const mt_tm2d* d, int* i0, int* i1, int* j0, int* j1) {
+  const int joff = d->y_pmc ? 0 : 1;
+  *i0 = std::max(d->hx.lo_end, d->ex.lo_end + 1) + 1;
+  *i1 = std::min(d->hx.hi_begin, d->ex.hi_begin + 1) - 1;
+  *j0 = std::max(d->hy.lo_end, d->ey.lo_end + joff) + 1;
+  *j1 = std::min(d->hy.hi_begin, d->ey.hi_begin + joff) - 1;
+}
+
+void te_interior(const mt_te2d* d, int* i0, int* i1, int* j0, int* j1) {
+  *i0 = std::max(d->hx.lo_end, d->ex.lo_end + 1) + 1;
+  *i1 = std::min(d->hx.hi_begin, d->ex.hi_begin + 1) - 1;
+  *j0 = std::max(d->hy.lo_end, d->ey.lo_end + 1) + 1;
+  *j1 = std::min(d->hy.hi_begin, d->ey.hi_begin + 1) - 1;
+}
+
+int tile_counts(int variant, int dtype, int nx, int ny, int i0, int i1, int j0, int j1, int* n_in, int* n_ring) {
+  if (variant < 0 || variant >= tiled_variant_count()) return fail(MT_EINVAL, "unknown tiled variant");
+  const TileMap m = tiled_map(variant, dtype, nx, ny, i0, i1, j0, j1);
+  const int in = (m.r1 - m.r0) * (m.c1 - m.c0);
+  if (n_in) *n_in = in;
+  if (n_ring) *n_ring = m.R * m.C - in;
+  return MT_OK;
+}
+
 template <typename T, typename P>
 int tm_run_tiled(const mt_tm2d* d, const mt_source2d* src, const double* coef, const MonitorPlan& mp,
                  int step0, int k, hipStream_t s) {
@@ -486,17 +513,13 @@ int tm_run_tiled(const mt_tm2d* d, const mt_source2d* src, const double* coef, c
   p.ny_e = d->y_pmc ? d->ny : d->ny - 2;
   p.dt = static_cast<T>(d->dt);
   p.dx = static_cast<T>(d->dx);
-  p.rdx = T(1) / p.dx;
+  p.rdx = const_div_recip<T>(p.dx);
   p.eps = static_cast<const T*>(d->eps);
   p.hy = pml_of<T>(d->hy);
   p.hx = pml_of<T>(d->hx);
   p.ex = pml_of<T>(d->ex);
   p.ey = pml_of<T>(d->ey);
-  const int joff = d->y_pmc ? 0 : 1;
-  p.in_i0 = std::max(d->hx.lo_end, d->ex.lo_end + 1) + 1;
-  p.in_i1 = std::min(d->hx.hi_begin, d->ex.hi_begin + 1) - 1;
-  p.in_j0 = std::max(d->hy.lo_end, d->ey.lo_end + joff) + 1;
-  p.in_j1 = std::min(d->hy.hi_begin, d->ey.hi_begin + joff) - 1;
+  tm_interior(d, &p.in_i0, &p.in_i1, &p.in_j0, &p.in_j1);
   T* A[8] = {static_cast<T*>(d->Ez), static_cast<T*>(d->Dz), static_cast<T*>(d->Hx), static_cast<T*>(d->Hy),
              static_cast<T*>(d->psi_Ezy), static_cast<T*>(d->psi_Ezx), static_cast<T*>(d->psi_Hyx),
              static_cast<T*>(d->psi_Hxy)};
@@ -507,6 +530,9 @@ int tm_run_tiled(const mt_tm2d* d, const mt_source2d* src, const double* coef, c
                       (size_t)(d->nx - 2) * p.ny_e, (size_t)(d->nx - 2) * p.ny_e};
   p.tape = static_cast<T*>(d->tape);
   if (src && src->present) p.src = *src;
+  p.hsrc[0] = d->hsrc[0];
+  p.hsrc[1] = d->hsrc[1];
+  const bool hsrc = d->hsrc[0].present || d->hsrc[1].present;
   const int NS = tiled_variant_steps(d->variant, d->dtype);
   const int S = mp.n_mons ? mp.stage_steps : k;
   int done = 0, slot = 0, first = step0, cur = 0, rc;
@@ -520,6 +546,8 @@ int tm_run_tiled(const mt_tm2d* d, const mt_source2d* src, const double* coef, c
     p.oEz = out[0], p.oDz = out[1], p.oHx = out[2], p.oHy = out[3];
     p.oEzy = out[4], p.oEzx = out[5], p.oHyx = out[6], p.oHxy = out[7];
     for (int u = 0; u < kMaxSub; ++u) p.coef[u] = (p.src.present && u < nsub) ? static_cast<T>(coef[done + u]) : T(0);
+    for (int u = 0; u < kMaxSub; ++u) p.hcoef[u] = (hsrc && u < nsub) ? static_cast<T>(d->hcoef[done + u]) : T(0);
+    if (src && src->present) p.src = step_source<T>(*src, done);
     TapSet taps;
     taps.n = 0;
     if (mp.n_mons && !build_taps(mp, slot, done, sizeof(T), &taps)) return fail(MT_EINVAL, "tap build failed");
@@ -549,17 +577,14 @@ int te_run_tiled(const mt_te2d* d, const mt_source2d* src, const double* coef, c
   p.ny = d->ny;
   p.dt = static_cast<T>(d->dt);
   p.dx = static_cast<T>(d->dx);
-  p.rdx = T(1) / p.dx;
+  p.rdx = const_div_recip<T>(p.dx);
   p.eps_x = static_cast<const T*>(d->eps_x);
   p.eps_y = static_cast<const T*>(d->eps_y);
   p.hx = pml_of<T>(d->hx);
   p.hy = pml_of<T>(d->hy);
   p.ey = pml_of<T>(d->ey);
   p.ex = pml_of<T>(d->ex);
-  p.in_i0 = std::max(d->hx.lo_end, d->ex.lo_end + 1) + 1;
-  p.in_i1 = std::min(d->hx.hi_begin, d->ex.hi_begin + 1) - 1;
-  p.in_j0 = std::max(d->hy.lo_end, d->ey.lo_end + 1) + 1;
-  p.in_j1 = std::min(d->hy.hi_begin, d->ey.hi_begin + 1) - 1;
+  te_interior(d, &p.in_i0, &p.in_i1, &p.in_j0, &p.in_j1);
   T* A[7] = {static_cast<T*>(d->Hz), static_cast<T*>(d->Dx), static_cast<T*>(d->Dy),
              static_cast<T*>(d->psi_Eyx), static_cast<T*>(d->psi_Exy), static_cast<T*>(d->psi_Hzy),
              static_cast<T*>(d->psi_Hzx)};
@@ -583,6 +608,7 @@ int te_run_tiled(const mt_te2d* d, const mt_source2d* src, const double* coef, c
     p.oHz = out[0], p.oDx = out[1], p.oDy = out[2], p.oEyx = out[3], p.oExy = out[4], p.oHzy = out[5],
     p.oHzx = out[6];
     for (int u = 0; u < kMaxSub; ++u) p.coef[u] = (p.src.present && u < nsub) ? static_cast<T>(coef[done + u]) : T(0);
+    if (src && src->present) p.src = step_source<T>(*src, done);
     TapSet taps;
     taps.n = 0;
     if (mp.n_mons && !build_taps(mp, slot, done, sizeof(T), &taps)) return fail(MT_EINVAL, "tap build failed");
@@ -670,7 +696,7 @@ extern "C" int mt_tm2d_run(const mt_tm2d* p, const mt_source2d* src, const doubl
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (p->variant >= 0 && !hsrc && !media) {   // magnetic currents and media: two-pass kernels
+  if (p->variant >= 0 && !media) {   // media: two-pass kernels
     if (p->variant >= tiled_variant_count()) return fail(MT_EINVAL, "unknown tiled variant");
     if (!alt_ok(p->alt, 8)) return fail(MT_EINVAL, "tiled path needs the alt buffers");
     if (taps_ok(mp))
@@ -725,3 +751,19 @@ extern "C" int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const doubl
   return p->dtype == MT_F64 ? te_run_t<double>(p, src, coef, mp, step0, k, s)
                             : te_run_t<float>(p, src, coef, mp, step0, k, s);
 }
+
+extern "C" int mt_tm2d_tiles(const mt_tm2d* p, int* n_interior, int* n_ring) {
+  if (!p) return fail(MT_EINVAL, "null problem");
+  int i0, i1, j0, j1;
+  tm_interior(p, &i0, &i1, &j0, &j1);
+  return tile_counts(p->variant, p->dtype, p->nx, p->ny, i0, i1, j0, j1, n_interior, n_ring);
+}
+
+extern "C" int mt_te2d_tiles(const mt_te2d* p, int* n_interior, int* n_ring) {
+  if (!p) return fail(MT_EINVAL, "null problem");
+  int i0, i1, j0, j1;
+  te_interior(p, &i0, &i1, &j0, &j1);
+  return tile_counts(p->variant, p->dtype, p->nx, p->ny, i0, i1, j0, j1, n_interior, n_ring);
+}
+
+extern "C" float mt_div_recip_f32(float b) { return div_recip_f32(b); }

@@ -1,9 +1,12 @@
 // 3D Yee leapfrog with CPML shells, point / masked soft sources, and (two-pass
 // path) material conductivity and ADE poles.
 //
-// Two passes per step: H from curl E, then E from curl H.  One thread per
-// cell (i, j, k) of the nx*ny*nz lattice updates every component whose grid
-// covers that cell; z is the contiguous axis and runs along the wave.
+// Variant 0, two passes per step: H from curl E, then E from curl H, in
+// place.  One thread per cell (i, j, k) of the nx*ny*nz lattice updates every
+// component whose grid covers that cell; z is the contiguous axis and runs
+// along the wave.  Variants 1-3 split each step into a one-pass interior
+// kernel (no psi) and the same per-cell updates over the CPML shell, on
+// ping-pong field buffers.
 // Arithmetic follows meep_torch/core/stepper_3d.py:363-445 op by op (lean:
 // stepper_3d_lean.py:234-296), so FP32 rounds exactly like the reference.
 // psi arrays keep the reference's full-grid shapes; only cells inside a CPML
@@ -27,7 +30,8 @@ struct K3 {
   T dt, dx;
   const T* eps;
   Pml<T> h[3], e[3];
-  T *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;
+  T *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;      // read: E, and old H
+  T *oEx, *oEy, *oEz, *oHx, *oHy, *oHz;  // written: new E, new H (== the read set in place)
   T* psi[12];
   // media (two-pass kernels with M = true): (d1, d2) of ex ey ez hx hy hz,
   // np poles with scalars c1[n], c2[n] and per E component the stacks
@@ -72,13 +76,13 @@ __device__ __forceinline__ T ce_apply(const CeIn<T>& c, T* psi, long long q, T d
   return d + ps;
 }
 
+// H half-step of one cell (i, j, k): every H component whose grid covers the
+// cell.  Reads E and the old H from Ex..Hz and writes the new H to oHx..oHz
+// (the same buffers for the in-place two-pass step; the ping-pong buffers of
+// the split step).
 template <typename T, bool M>
-__global__ __launch_bounds__(BZ* BYY) void h3_kernel(K3<T> p) {
-  const int k = blockIdx.x * BZ + threadIdx.x;
-  const int j = blockIdx.y * BYY + threadIdx.y;
-  const int i = blockIdx.z;
+__device__ __forceinline__ void h3_cell(const K3<T>& p, int i, int j, int k) {
   const int nx = p.nx, ny = p.ny, nz = p.nz;
-  if (j >= ny || k >= nz) return;
   const T dx = p.dx, dt = p.dt;
   const bool fx = j < ny - 1 && k < nz - 1;                    // Hx (nx, ny-1, nz-1)
   const bool fy = i < nx - 1 && k < nz - 1;                    // Hy (nx-1, ny, nz-1)
@@ -123,18 +127,26 @@ __global__ __launch_bounds__(BZ* BYY) void h3_kernel(K3<T> p) {
   if (fx) {
     T cy = ce_apply(cxy, p.psi[P_EZ_Y], ix, (ezj - ez0) / dx);
     T cz = ce_apply(cxz, p.psi[P_EY_Z], ix, (eyk - ey0) / dx);
-    p.Hx[ix] = M ? d1x * hx - d2x * (cy - cz) : hx - dt * (cy - cz);
+    p.oHx[ix] = M ? d1x * hx - d2x * (cy - cz) : hx - dt * (cy - cz);
   }
   if (fy) {
     T cz = ce_apply(cyz, p.psi[P_EX_Z], iy, (exk - ex0) / dx);
     T cx = ce_apply(cyx, p.psi[P_EZ_X], iy, (ezx - ez0) / dx);
-    p.Hy[iy] = M ? d1y * hy - d2y * (cz - cx) : hy - dt * (cz - cx);
+    p.oHy[iy] = M ? d1y * hy - d2y * (cz - cx) : hy - dt * (cz - cx);
   }
   if (fz) {
     T cx = ce_apply(czx, p.psi[P_EY_X], iz, (eyx - ey0) / dx);
     T cy = ce_apply(czy, p.psi[P_EX_Y], iz, (exj - ex0) / dx);
-    p.Hz[iz] = M ? d1z * hz - d2z * (cx - cy) : hz - dt * (cx - cy);
+    p.oHz[iz] = M ? d1z * hz - d2z * (cx - cy) : hz - dt * (cx - cy);
   }
+}
+
+template <typename T, bool M>
+__global__ __launch_bounds__(BZ* BYY) void h3_kernel(K3<T> p) {
+  const int k = blockIdx.x * BZ + threadIdx.x;
+  const int j = blockIdx.y * BYY + threadIdx.y;
+  if (j >= p.ny || k >= p.nz) return;
+  h3_cell<T, M>(p, blockIdx.z, j, k);
 }
 
 template <typename T, typename P>
@@ -183,13 +195,12 @@ __device__ __forceinline__ T inject(const P& p, const mt_source3d& s, int comp, 
   return E - (c * m) / eps_c;
 }
 
+// E half-step of one cell: reads the new H from oHx..oHz and the old E from
+// Ex..Ez, writes the new E to oEx..oEz (same buffers when in place).
 template <typename T, bool M>
-__global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T c, T* tape) {
-  const int k = blockIdx.x * BZ + threadIdx.x;
-  const int j = blockIdx.y * BYY + threadIdx.y;
-  const int i = blockIdx.z;
+__device__ __forceinline__ void e3_cell(const K3<T>& p, const mt_source3d& src, T c, T* tape, int i, int j,
+                                        int k) {
   const int nx = p.nx, ny = p.ny, nz = p.nz;
-  if (j >= ny || k >= nz) return;
   const T dx = p.dx;
   const bool ax = i < nx - 1, ay = j < ny - 1, az = k < nz - 1;  // Ex, Ey, Ez exist at this cell
   const bool ix = i >= 1 && ax, jy = j >= 1 && ay, kz = k >= 1 && az;
@@ -197,6 +208,7 @@ __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T
   const long long iy = ((long long)i * (ny - 1) + j) * nz + k;            // Ey, Hz (i, j, k)
   const long long iz = ((long long)i * ny + j) * (nz - 1) + k;            // Ez, Hy (i, j, k)
   const long long ihx = ((long long)i * (ny - 1) + j) * (nz - 1) + k;     // Hx (i, j, k)
+  const T *Hx = p.oHx, *Hy = p.oHy, *Hz = p.oHz;
   // ---- loads (every one before the first store; see ce_load)
   const T eps0 = p.eps[ce_];
   T epx = T(0), epy = T(0), epz = T(0), ex = T(0), ey = T(0), ez = T(0);
@@ -213,15 +225,15 @@ __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T
     epz = p.eps[ce_ + 1];
     ez = p.Ez[iz];
   }
-  if (ax && ay) hz0 = p.Hz[iy];
-  if (ax && jy) hzj = p.Hz[iy - nz];
-  if (ay && ix) hzx = p.Hz[iy - (long long)(ny - 1) * nz];
-  if (ax && az) hy0 = p.Hy[iz];
-  if (ax && kz) hyk = p.Hy[iz - 1];
-  if (az && ix) hyx = p.Hy[iz - (long long)ny * (nz - 1)];
-  if (ay && az) hx0 = p.Hx[ihx];
-  if (ay && kz) hxk = p.Hx[ihx - 1];
-  if (az && jy) hxj = p.Hx[ihx - (nz - 1)];
+  if (ax && ay) hz0 = Hz[iy];
+  if (ax && jy) hzj = Hz[iy - nz];
+  if (ay && ix) hzx = Hz[iy - (long long)(ny - 1) * nz];
+  if (ax && az) hy0 = Hy[iz];
+  if (ax && kz) hyk = Hy[iz - 1];
+  if (az && ix) hyx = Hy[iz - (long long)ny * (nz - 1)];
+  if (ay && az) hx0 = Hx[ihx];
+  if (ay && kz) hxk = Hx[ihx - 1];
+  if (az && jy) hxj = Hx[ihx - (nz - 1)];
   const long long q_xy = ((long long)i * (ny - 2) + (j - 1)) * nz + k;
   const long long q_xz = ((long long)i * ny + j) * (nz - 2) + (k - 1);
   const long long q_yz = ((long long)i * (ny - 1) + j) * (nz - 2) + (k - 1);
@@ -243,7 +255,7 @@ __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T
     T epsc = T(0.5) * (eps0 + epx);
     T E = M ? e_media(p, 0, ce_, (long long)(nx - 1) * ny * nz, ex, epsc, curl) : e_update(p, ex, epsc, curl);
     E = inject(p, src, 0, i, j, k, E, epsc, c);
-    p.Ex[ce_] = E;
+    p.oEx[ce_] = E;
     if (tape) tape[ce_] = E;
   }
   if (ay) {  // Ey (nx, ny-1, nz)
@@ -254,7 +266,7 @@ __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T
     T epsc = T(0.5) * (eps0 + epy);
     T E = M ? e_media(p, 1, iy, (long long)nx * (ny - 1) * nz, ey, epsc, curl) : e_update(p, ey, epsc, curl);
     E = inject(p, src, 1, i, j, k, E, epsc, c);
-    p.Ey[iy] = E;
+    p.oEy[iy] = E;
     if (tape) tape[(long long)(nx - 1) * ny * nz + iy] = E;
   }
   if (az) {  // Ez (nx, ny, nz-1)
@@ -265,343 +277,363 @@ __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T
     T epsc = T(0.5) * (eps0 + epz);
     T E = M ? e_media(p, 2, iz, (long long)nx * ny * (nz - 1), ez, epsc, curl) : e_update(p, ez, epsc, curl);
     E = inject(p, src, 2, i, j, k, E, epsc, c);
-    p.Ez[iz] = E;
+    p.oEz[iz] = E;
     if (tape) tape[(long long)(nx - 1) * ny * nz + (long long)nx * (ny - 1) * nz + iz] = E;
   }
 }
 
+template <typename T, bool M>
+__global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T c, T* tape) {
+  const int k = blockIdx.x * BZ + threadIdx.x;
+  const int j = blockIdx.y * BYY + threadIdx.y;
+  if (j >= p.ny || k >= p.nz) return;
+  e3_cell<T, M>(p, src, c, tape, blockIdx.z, j, k);
+}
+
 // ---------------------------------------------------------------------------
-// Fused single-pass step, streaming along x.
-//
-// A workgroup of 64 x TJ threads owns the (j, k) column block
-// [j0, j0+TJ) x [k0, k0+64) of one x segment and marches over its planes.
-// Per plane i it computes H'(i) for its cells plus the low halo row
-// (j0-1: Hx', Hz') and column (k0-1: Hx', Hy') that its E' update reads,
-// puts them in LDS, then computes E'(i).  The x differences of E' use the
-// thread's own H'(i-1), kept in registers.  Old E and eps planes i, i+1 are
-// staged in LDS (three rotating buffers, halo included), so each value
-// comes from memory once per workgroup.  Fields and psi are read from the
-// old buffers and written to the new ones (ping-pong): halo values are
-// recomputed, never read from a neighbour's output, so workgroups need no
-// ordering.  A segment starting at i_begin > 0 first recomputes H'(i_begin-1)
-// without writing it.  The arithmetic of every update is the two-pass
-// kernels' expression for expression, so results are bit-identical.
+// Split step (variants 1-3): a one-pass interior kernel over the box of cells
+// whose H and E updates carry no CPML psi, and the two-pass cell updates over
+// the six shell pieces around it.  Fields are ping-ponged: every kernel of a
+// step reads E and H from buffer set A and writes set B (psi arrays are
+// updated in place: each psi value is read and written by its own cell
+// only).  Order per step: interior kernel, shell H pass (both read only A),
+// then shell E pass (reads the new H in B, written by both before it).
 // ---------------------------------------------------------------------------
-template <typename T>
-struct F3 {
-  int nx, ny, nz, lean, i_seg;
-  T dt, dx;
-  const T* eps;
-  Pml<T> h[3], e[3];
-  const T *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;
-  T *Exn, *Eyn, *Ezn, *Hxn, *Hyn, *Hzn;
-  const T* psi[12];
-  T* psin[12];
+
+// One shell piece: the cell box [i0, i0+ni) x [j0, j0+nj) x [k0, k0+nk),
+// covered by blocks of 256 threads laid out 64 (k) x 4 (j) ("wide") or
+// 16 x 16 ("narrow", for pieces thin along k); b0 = its first block.
+struct Piece {
+  int i0, j0, k0, ni, nj, nk, narrow, gk, gj, b0;
 };
 
-// The CPML curl term with the psi recursion reading the old buffer and (if W) writing the new one
-template <typename T, bool W>
-__device__ __forceinline__ T ce2(const Pml<T>& pr, int idx, const T* ps, T* pn, long long q, T d) {
-  if (!pr.in(idx)) return d;
-  T v = pr.b[idx] * ps[q] + pr.a[idx] * d;
-  if (W) pn[q] = v;
-  return d + v;
-}
-
-constexpr int FK = 64;
-
-template <typename T, int TJ>
-struct FusedSmem {
-  T e[3][3][TJ + 2][FK + 2];  // [buffer][Ex,Ey,Ez][j - (j0-1)][k - (k0-1)]
-  T eps[3][TJ + 1][FK + 1];   // [buffer][j - j0][k - k0]
-  T h[3][TJ + 1][FK + 1];     // H' of the plane: [Hx,Hy,Hz][j - (j0-1)][k - (k0-1)]
+struct Pieces {
+  Piece p[6];
+  int n, total;
 };
 
-// Plane staging: each thread owns NQ fixed slots of the (TJ+2) x (FK+2) E
-// region and of the (TJ+1) x (FK+1) eps region; loads go to registers first
-// (issued a plane ahead) and are stored to LDS after the current plane's
-// compute, so memory latency overlaps the arithmetic.
-constexpr int kPlaneQ = 2;
-
-template <typename T, int TJ>
-struct PlaneRegs {
-  T e[kPlaneQ][3];
-  T eps[kPlaneQ];
-};
-
-template <typename T, int TJ>
-__device__ __forceinline__ void fused_fetch_plane(const F3<T>& p, PlaneRegs<T, TJ>& pr, int ip, int j0, int k0) {
-  const int nt = FK * TJ, tid = threadIdx.y * FK + threadIdx.x;
-  const int nx = p.nx, ny = p.ny, nz = p.nz;
-  static_assert((TJ + 2) * (FK + 2) <= kPlaneQ * FK * TJ, "plane slots");
-  static_assert((TJ + 1) * (FK + 1) <= kPlaneQ * FK * TJ, "eps slots");
+__device__ __forceinline__ Piece piece_of(const Pieces& P, int b) {
+  // uniform selection without dynamic indexing of the argument block
+  Piece q = P.p[0];
 #pragma unroll
-  for (int q = 0; q < kPlaneQ; ++q) {
-    const int e = tid + q * nt;
-    T a = T(0), b = T(0), c = T(0), d = T(0);
-    if (e < (TJ + 2) * (FK + 2)) {
-      const int r = e / (FK + 2), cc = e - r * (FK + 2);
-      const int j = j0 - 1 + r, k = k0 - 1 + cc;
-      if (j >= 0 && j < ny && k >= 0 && k < nz) {
-        if (ip < nx - 1) a = p.Ex[((long long)ip * ny + j) * nz + k];
-        if (j < ny - 1) b = p.Ey[((long long)ip * (ny - 1) + j) * nz + k];
-        if (k < nz - 1) c = p.Ez[((long long)ip * ny + j) * (nz - 1) + k];
-      }
-    }
-    if (e < (TJ + 1) * (FK + 1)) {
-      const int r = e / (FK + 1), cc = e - r * (FK + 1);
-      const int j = j0 + r, k = k0 + cc;
-      if (j < ny && k < nz) d = p.eps[((long long)ip * ny + j) * nz + k];
-    }
-    pr.e[q][0] = a;
-    pr.e[q][1] = b;
-    pr.e[q][2] = c;
-    pr.eps[q] = d;
-  }
-}
-
-template <typename T, int TJ>
-__device__ __forceinline__ void fused_store_plane(FusedSmem<T, TJ>& sm, const PlaneRegs<T, TJ>& pr, int buf) {
-  const int nt = FK * TJ, tid = threadIdx.y * FK + threadIdx.x;
-#pragma unroll
-  for (int q = 0; q < kPlaneQ; ++q) {
-    const int e = tid + q * nt;
-    if (e < (TJ + 2) * (FK + 2)) {
-      const int r = e / (FK + 2), cc = e - r * (FK + 2);
-      sm.e[buf][0][r][cc] = pr.e[q][0];
-      sm.e[buf][1][r][cc] = pr.e[q][1];
-      sm.e[buf][2][r][cc] = pr.e[q][2];
-    }
-    if (e < (TJ + 1) * (FK + 1)) {
-      const int r = e / (FK + 1), cc = e - r * (FK + 1);
-      sm.eps[buf][r][cc] = pr.eps[q];
-    }
-  }
-}
-
-// Per-plane views: base pointers of plane ip of every array (uniform across
-// the workgroup) so that per-thread offsets stay 32-bit.  Valid for arrays
-// below 2^31 elements per plane set (checked on the host).
-template <typename T>
-struct PlaneP {
-  const T *hx, *hy, *hz;        // old H, plane ip
-  T *hxn, *hyn, *hzn;           // new H, plane ip
-  const T *ps[12];              // old psi at the plane the update indexes
-  T* pn[12];
-};
-
-template <typename T>
-__device__ __forceinline__ PlaneP<T> plane_ptrs(const F3<T>& p, int ip) {
-  const long long nx = p.nx, ny = p.ny, nz = p.nz;
-  PlaneP<T> q;
-  const long long sHx = (ny - 1) * (nz - 1), sHy = ny * (nz - 1), sHz = (ny - 1) * nz;
-  q.hx = p.Hx + ip * sHx;
-  q.hy = p.Hy + ip * sHy;
-  q.hz = p.Hz + ip * sHz;
-  q.hxn = p.Hxn + ip * sHx;
-  q.hyn = p.Hyn + ip * sHy;
-  q.hzn = p.Hzn + ip * sHz;
-  // H-side psi share the H component's plane stride
-  const long long sh[6] = {sHx, sHx, sHy, sHy, sHz, sHz};
-  for (int a = 0; a < 6; ++a) {
-    q.ps[a] = p.psi[a] + ip * sh[a];
-    q.pn[a] = p.psin[a] + ip * sh[a];
-  }
-  // E-side psi: plane ip (y/z shells) or ip-1 (x shells)
-  const long long se[6] = {(ny - 2) * nz, ny * (nz - 2), (ny - 1) * (nz - 2), (ny - 1) * nz, ny * (nz - 1),
-                           (ny - 2) * (nz - 1)};
-  const long long pl[6] = {ip, ip, ip, ip - 1, ip - 1, ip};
-  for (int a = 0; a < 6; ++a) {
-    q.ps[6 + a] = p.psi[6 + a] + pl[a] * se[a];
-    q.pn[6 + a] = p.psin[6 + a] + pl[a] * se[a];
-  }
+  for (int t = 1; t < 6; ++t)
+    if (t < P.n && b >= P.p[t].b0) q = P.p[t];
   return q;
 }
 
-// H' components of cell (ip, j, k) from its old value hold; sm.e planes ip
-// (cur) and ip+1 (nxt) in LDS coordinates (r, c) = (j - (j0-1), k - (k0-1)).
-template <typename T, int TJ, bool W>
-__device__ __forceinline__ T fused_hx(const F3<T>& p, const PlaneP<T>& q, const FusedSmem<T, TJ>& sm, int cur,
-                                      int j, int k, int r, int c, T hold) {
-  const int h = j * (p.nz - 1) + k;
-  T cy = ce2<T, W>(p.h[1], j, q.ps[0], q.pn[0], h, (sm.e[cur][2][r + 1][c] - sm.e[cur][2][r][c]) / p.dx);
-  T cz = ce2<T, W>(p.h[2], k, q.ps[1], q.pn[1], h, (sm.e[cur][1][r][c + 1] - sm.e[cur][1][r][c]) / p.dx);
-  T v = hold - p.dt * (cy - cz);
-  if (W) q.hxn[h] = v;
-  return v;
+__device__ __forceinline__ bool piece_cell(const Pieces& P, int* i, int* j, int* k) {
+  const Piece q = piece_of(P, blockIdx.x);
+  const int b = blockIdx.x - q.b0;
+  const int bk = b % q.gk, r = b / q.gk, bj = r % q.gj, bi = r / q.gj;
+  const int tk = q.narrow ? (threadIdx.x & 15) : (threadIdx.x & 63);
+  const int tj = q.narrow ? (threadIdx.x >> 4) : (threadIdx.x >> 6);
+  const int kk = bk * (q.narrow ? 16 : 64) + tk, jj = bj * (q.narrow ? 16 : 4) + tj;
+  if (kk >= q.nk || jj >= q.nj) return false;
+  *i = q.i0 + bi;
+  *j = q.j0 + jj;
+  *k = q.k0 + kk;
+  return true;
 }
 
-template <typename T, int TJ, bool W>
-__device__ __forceinline__ T fused_hy(const F3<T>& p, const PlaneP<T>& q, const FusedSmem<T, TJ>& sm, int cur,
-                                      int nxt, int ip, int j, int k, int r, int c, T hold) {
-  const int h = j * (p.nz - 1) + k;
-  T cz = ce2<T, W>(p.h[2], k, q.ps[2], q.pn[2], h, (sm.e[cur][0][r][c + 1] - sm.e[cur][0][r][c]) / p.dx);
-  T cx = ce2<T, W>(p.h[0], ip, q.ps[3], q.pn[3], h, (sm.e[nxt][2][r][c] - sm.e[cur][2][r][c]) / p.dx);
-  T v = hold - p.dt * (cz - cx);
-  if (W) q.hyn[h] = v;
-  return v;
-}
-
-template <typename T, int TJ, bool W>
-__device__ __forceinline__ T fused_hz(const F3<T>& p, const PlaneP<T>& q, const FusedSmem<T, TJ>& sm, int cur,
-                                      int nxt, int ip, int j, int k, int r, int c, T hold) {
-  const int h = j * p.nz + k;
-  T cx = ce2<T, W>(p.h[0], ip, q.ps[4], q.pn[4], h, (sm.e[nxt][1][r][c] - sm.e[cur][1][r][c]) / p.dx);
-  T cy = ce2<T, W>(p.h[1], j, q.ps[5], q.pn[5], h, (sm.e[cur][0][r + 1][c] - sm.e[cur][0][r][c]) / p.dx);
-  T v = hold - p.dt * (cx - cy);
-  if (W) q.hzn[h] = v;
-  return v;
-}
-
-// Old H values one thread needs for plane ip: its own cell, and (halo
-// lanes) the low-row / low-column cells it computes.
 template <typename T>
-struct HoldRegs {
-  T x, y, z, a, b;  // own Hx, Hy, Hz; halo: (row) Hx, Hz or (column) Hx, Hy
+__global__ __launch_bounds__(256) void h3_pieces(K3<T> p, Pieces P) {
+  int i, j, k;
+  if (piece_cell(P, &i, &j, &k)) h3_cell<T, false>(p, i, j, k);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void e3_pieces(K3<T> p, Pieces P, mt_source3d src, T c, T* tape) {
+  int i, j, k;
+  if (piece_cell(P, &i, &j, &k)) e3_cell<T, false>(p, src, c, tape, i, j, k);
+}
+
+// Interior kernel.  A workgroup owns a tile of TJ rows (j) x 64 columns (k)
+// and one x segment [ib, ie) of the interior box, and streams along x.  It
+// has TJ + 2 waves, one per tile row from j0 - 1 to j0 + TJ, lane = k - k0:
+// the first row computes the halo Hx', Hz' that the tile's E' reads, the last
+// one only supplies E and eps; lane 0 also holds column k0 - 1 (halo Hx', Hy')
+// and lane 63 column k0 + 64 (Ex, Ey, eps).  Per plane i, after one barrier:
+// H'(i) from E(i) (LDS) and E(i+1) (registers), then E'(i-1) from H'(i-1)
+// (LDS) and H'(i-2) (registers).  E planes are double-buffered in LDS, eps
+// triple-buffered, H' double-buffered, so one barrier per plane orders every
+// LDS write after the reads of the same buffer.  Loads run two planes ahead
+// (E) or one (H, eps).  Halo values are recomputed from the read-only set A,
+// never read from another workgroup's output, so workgroups are independent.
+// Arithmetic is the two-pass cell updates' expression for expression (the box
+// is psi-free and away from the PEC walls), so results are bit-identical.
+template <typename T>
+struct I3 {
+  int nx, ny, nz, lean;
+  int I0, I1, J0, J1, K0, K1;   // interior box
+  int kbase, njt, nkt, iseg;    // k tiles from kbase (64-aligned); j tiles from J0; x segments of iseg planes
+  int ntiles, per_xcd;
+  T dt, dx;
+  const T* eps;
+  const T *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;   // set A (read)
+  T *oEx, *oEy, *oEz, *oHx, *oHy, *oHz;   // set B (written)
+};
+
+template <typename T>
+struct Col {          // one field column value set
+  T x, y, z;
 };
 
 template <typename T, int TJ>
-__device__ __forceinline__ void fused_fetch_h(const F3<T>& p, HoldRegs<T>& h, int ip, int j0, int k0, int wcol) {
+__global__ __launch_bounds__(64 * (TJ + 2)) void y3_interior(I3<T> p, mt_source3d src, T c_src, T* tape) {
+  constexpr int NR = TJ + 2;
+  __shared__ T sE[2][3][NR][66];
+  __shared__ T sEps[3][TJ + 1][65];
+  __shared__ T sH[2][3][TJ + 1][65];
+  // XCD-aware tile order: consecutive workgroups go to different XCDs
+  // (round robin), so XCD x takes the contiguous tile range
+  // [x * per_xcd, (x + 1) * per_xcd), j fastest: j-neighbour tiles, which
+  // share halo rows, run on the same XCD (same L2) at about the same time.
+  const int t = (blockIdx.x & 7) * p.per_xcd + (blockIdx.x >> 3);
+  if (t >= p.ntiles) return;
+  const int jt = t % p.njt, r2 = t / p.njt, kt = r2 % p.nkt, sg = r2 / p.nkt;
+  const int j0 = p.J0 + jt * TJ, k0 = p.kbase + kt * 64;
+  const int ib = p.I0 + sg * p.iseg, ie = min(p.I1, ib + p.iseg);
+  const int lane = threadIdx.x, wr = (int)threadIdx.y - 1;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
-  const int tx = threadIdx.x, ty = threadIdx.y, j = j0 + ty, k = k0 + tx;
-  h = HoldRegs<T>{T(0), T(0), T(0), T(0), T(0)};
-  if (ip >= nx) return;
-  const T* Hx = p.Hx + (long long)ip * (ny - 1) * (nz - 1);
-  const T* Hy = p.Hy + (long long)ip * ny * (nz - 1);
-  const T* Hz = p.Hz + (long long)ip * (ny - 1) * nz;
-  if (j < ny && k < nz) {
-    if (j < ny - 1 && k < nz - 1) h.x = Hx[j * (nz - 1) + k];
-    if (ip < nx - 1 && k < nz - 1) h.y = Hy[j * (nz - 1) + k];
-    if (ip < nx - 1 && j < ny - 1) h.z = Hz[j * nz + k];
-  }
-  if (ty == 0 && j0 >= 1 && k < nz) {
-    const int jh = j0 - 1;
-    if (k < nz - 1) h.a = Hx[jh * (nz - 1) + k];
-    if (ip < nx - 1) h.b = Hz[jh * nz + k];
-  } else if (ty == wcol && tx < TJ && k0 >= 1 && j0 + tx < ny) {
-    const int jc = j0 + tx, kh = k0 - 1;
-    if (jc < ny - 1) h.a = Hx[jc * (nz - 1) + kh];
-    if (ip < nx - 1) h.b = Hy[jc * (nz - 1) + kh];
+  const int j = j0 + wr, k = k0 + lane;
+  const bool own = wr >= 0 && wr < TJ;
+  const bool jv = j >= 0 && j < ny, kv = k < nz;
+  const bool active = own && j < p.J1 && k >= p.K0 && k < p.K1;
+  const bool haloL = lane == 0 && k0 >= p.K0;           // column k0-1 feeds active cells
+  const bool haloR = lane == 63 && k0 + 64 < nz;        // column k0+64
+  const bool wantH = own || wr == -1;
+  // plane strides and row offsets
+  const long long sEx = (long long)ny * nz, sEy = (long long)(ny - 1) * nz, sEz = (long long)ny * (nz - 1);
+  const long long sHx = (long long)(ny - 1) * (nz - 1), sHy = sEz, sHz = sEy;
+  const int oX = j * nz + k, oY = j * nz + k, oZ = j * (nz - 1) + k, oHX = j * (nz - 1) + k;
+  const T dx = p.dx, dt = p.dt;
+
+  auto loadE = [&](int ip, int kk, bool ok) {
+    Col<T> v{T(0), T(0), T(0)};
+    if (ok && jv && kk >= 0 && kk < nz) {
+      if (ip < nx - 1) v.x = p.Ex[ip * sEx + j * nz + kk];
+      if (j < ny - 1) v.y = p.Ey[ip * sEy + j * nz + kk];
+      if (kk < nz - 1) v.z = p.Ez[ip * sEz + j * (nz - 1) + kk];
+    }
+    return v;
+  };
+  auto loadEps = [&](int ip, int kk, bool ok) {
+    return (ok && wr >= 0 && jv && kk < nz) ? p.eps[ip * sEx + j * nz + kk] : T(0);
+  };
+  auto loadH = [&](int ip) {
+    Col<T> v{T(0), T(0), T(0)};
+    if (wantH && jv && kv && ip < nx - 1) {
+      if (j < ny - 1 && k < nz - 1) v.x = p.Hx[ip * sHx + oHX];
+      if (own && k < nz - 1) v.y = p.Hy[ip * sHy + oZ];
+      if (j < ny - 1) v.z = p.Hz[ip * sHz + oY];
+    }
+    return v;
+  };
+  auto loadHL = [&](int ip) {   // Hx, Hy at (ip, j, k0-1) for the halo column
+    Col<T> v{T(0), T(0), T(0)};
+    if (haloL && own && jv && ip < nx - 1) {
+      const int kk = k0 - 1;
+      if (j < ny - 1) v.x = p.Hx[ip * sHx + j * (nz - 1) + kk];
+      v.y = p.Hy[ip * sHy + j * (nz - 1) + kk];
+    }
+    return v;
+  };
+
+  // registers: E at planes i (e0), i+1 (e1), i+2 (in flight); eps at i-1, i;
+  // H'(i-1), H'(i-2) of the own cell; old H of plane i
+  const int ip0 = ib - 1;
+  Col<T> e0 = loadE(ip0, k, true), e1 = loadE(ip0 + 1, k, true), em{T(0), T(0), T(0)};
+  Col<T> l0 = loadE(ip0, k0 - 1, lane == 0), l1 = loadE(ip0 + 1, k0 - 1, lane == 0);
+  Col<T> r0 = loadE(ip0, k0 + 64, haloR), r1 = loadE(ip0 + 1, k0 + 64, haloR);
+  T eps0 = loadEps(ip0, k, true), epsR0 = loadEps(ip0, k0 + 64, haloR && wr < TJ), epsM = T(0);
+  Col<T> hold = loadH(ip0), hl = loadHL(ip0);
+  Col<T> hp{T(0), T(0), T(0)}, hpp{T(0), T(0), T(0)};
+
+  for (int i = ip0; i <= ie; ++i) {
+    const int bE = i & 1, bS = i % 3;
+    // ---- plane i into LDS
+    sE[bE][0][wr + 1][lane + 1] = e0.x;
+    sE[bE][1][wr + 1][lane + 1] = e0.y;
+    sE[bE][2][wr + 1][lane + 1] = e0.z;
+    if (lane == 0) {
+      sE[bE][0][wr + 1][0] = l0.x;
+      sE[bE][1][wr + 1][0] = l0.y;
+      sE[bE][2][wr + 1][0] = l0.z;
+    }
+    if (lane == 63) {
+      sE[bE][0][wr + 1][65] = r0.x;
+      sE[bE][1][wr + 1][65] = r0.y;
+    }
+    if (wr >= 0) {
+      sEps[bS][wr][lane] = eps0;
+      if (lane == 63 && wr < TJ) sEps[bS][wr][64] = epsR0;
+    }
+    // ---- prefetch: E(i+2), old H(i+1), eps(i+1)
+    const bool moreE = i + 2 <= ie;
+    const bool moreH = i + 1 < ie;
+    Col<T> eN{T(0), T(0), T(0)}, lN{T(0), T(0), T(0)}, rN{T(0), T(0), T(0)}, hN{T(0), T(0), T(0)},
+        hlN{T(0), T(0), T(0)};
+    T epsN = T(0), epsRN = T(0);
+    if (moreE) {
+      eN = loadE(i + 2, k, true);
+      lN = loadE(i + 2, k0 - 1, lane == 0);
+      rN = loadE(i + 2, k0 + 64, haloR);
+    }
+    if (moreH) {
+      hN = loadH(i + 1);
+      hlN = loadHL(i + 1);
+    }
+    if (i + 1 <= ie) {
+      epsN = loadEps(i + 1, k, true);
+      epsRN = loadEps(i + 1, k0 + 64, haloR && wr < TJ);
+    }
+    __syncthreads();
+    // ---- H'(i): own cells, halo row j0-1 (Hx', Hz'), halo column k0-1 (Hx', Hy')
+    Col<T> hn{T(0), T(0), T(0)};
+    if (i < ie && wantH) {
+      const int r = wr + 1, c = lane + 1;
+      const T ezj = sE[bE][2][r + 1][c], eyk = sE[bE][1][r][c + 1];
+      const T exk = sE[bE][0][r][c + 1], exj = sE[bE][0][r + 1][c];
+      {  // Hx' (stepper_3d.py:371-375)
+        T cy = (ezj - e0.z) / dx;
+        T cz = (eyk - e0.y) / dx;
+        hn.x = hold.x - dt * (cy - cz);
+      }
+      if (own) {  // Hy'
+        T cz = (exk - e0.x) / dx;
+        T cx = (e1.z - e0.z) / dx;
+        hn.y = hold.y - dt * (cz - cx);
+      }
+      {  // Hz'
+        T cx = (e1.y - e0.y) / dx;
+        T cy = (exj - e0.x) / dx;
+        hn.z = hold.z - dt * (cx - cy);
+      }
+      const int bH = i & 1;
+      sH[bH][0][r][c] = hn.x;
+      sH[bH][1][r][c] = hn.y;
+      sH[bH][2][r][c] = hn.z;
+      if (haloL && own) {  // column k0-1
+        const T ez0 = sE[bE][2][r][0], ey0 = sE[bE][1][r][0], ex0 = sE[bE][0][r][0];
+        T cy = (sE[bE][2][r + 1][0] - ez0) / dx;
+        T cz = (sE[bE][1][r][1] - ey0) / dx;
+        sH[bH][0][r][0] = hl.x - dt * (cy - cz);
+        T cz2 = (sE[bE][0][r][1] - ex0) / dx;
+        T cx2 = (l1.z - ez0) / dx;
+        sH[bH][1][r][0] = hl.y - dt * (cz2 - cx2);
+      }
+      if (active && i >= ib) {
+        p.oHx[i * sHx + oHX] = hn.x;
+        p.oHy[i * sHy + oZ] = hn.y;
+        p.oHz[i * sHz + oY] = hn.z;
+      }
+    }
+    // ---- E'(i-1) of the own cell (stepper_3d.py:392-437)
+    if (i > ib && active) {
+      const int im = i - 1, bH = im & 1, bP = im % 3, r = wr + 1, c = lane + 1;
+      T ex, ey, ez;
+      {
+        T chy = (hp.z - sH[bH][2][r - 1][c]) / dx;
+        T chz = (hp.y - sH[bH][1][r][c - 1]) / dx;
+        T curl = chy - chz;
+        T epsc = T(0.5) * (epsM + eps0);
+        ex = e_update(p, em.x, epsc, curl);
+        ex = inject(p, src, 0, im, j, k, ex, epsc, c_src);
+      }
+      {
+        T chz = (hp.x - sH[bH][0][r][c - 1]) / dx;
+        T chx = (hp.z - hpp.z) / dx;
+        T curl = chz - chx;
+        T epsc = T(0.5) * (epsM + sEps[bP][wr + 1][lane]);
+        ey = e_update(p, em.y, epsc, curl);
+        ey = inject(p, src, 1, im, j, k, ey, epsc, c_src);
+      }
+      {
+        T chx = (hp.y - hpp.y) / dx;
+        T chy = (hp.x - sH[bH][0][r - 1][c]) / dx;
+        T curl = chx - chy;
+        T epsc = T(0.5) * (epsM + sEps[bP][wr][lane + 1]);
+        ez = e_update(p, em.z, epsc, curl);
+        ez = inject(p, src, 2, im, j, k, ez, epsc, c_src);
+      }
+      p.oEx[im * sEx + oX] = ex;
+      p.oEy[im * sEy + oY] = ey;
+      p.oEz[im * sEz + oZ] = ez;
+      if (tape) {
+        tape[im * sEx + oX] = ex;
+        tape[(long long)(nx - 1) * sEx + im * sEy + oY] = ey;
+        tape[(long long)(nx - 1) * sEx + (long long)nx * sEy + im * sEz + oZ] = ez;
+      }
+    }
+    // ---- rotate
+    em = e0;
+    e0 = e1;
+    e1 = eN;
+    l0 = l1;
+    l1 = lN;
+    r0 = r1;
+    r1 = rN;
+    epsM = eps0;
+    eps0 = epsN;
+    epsR0 = epsRN;
+    hpp = hp;
+    hp = hn;
+    hold = hN;
+    hl = hlN;
   }
 }
 
-template <typename T, int TJ>
-__global__ __launch_bounds__(FK* TJ, 4) void fused3_kernel(F3<T> p, mt_source3d src, T c_src, T* tape) {
-  __shared__ FusedSmem<T, TJ> sm;
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int k0 = blockIdx.x * FK, j0 = blockIdx.y * TJ;
-  const int k = k0 + tx, j = j0 + ty;
-  const int nx = p.nx, ny = p.ny, nz = p.nz;
-  const int i_begin = blockIdx.z * p.i_seg;
-  const int i_end = min(nx, i_begin + p.i_seg);
-  const int i_first = i_begin > 0 ? i_begin - 1 : 0;
-  const bool own = j < ny && k < nz;
-  const int r = ty + 1, c = tx + 1;
-  const int wcol = TJ >= 2 ? 1 : 0;  // wave computing the halo column
-  const T dx = p.dx;
-  T hyp = T(0), hzp = T(0);  // own H'(i-1)
+// The interior box: cells whose H (and the H of their low neighbours, which
+// the interior kernel recomputes) and E carry no psi, away from the walls.
+struct Box3 {
+  int I0, I1, J0, J1, K0, K1;
+};
 
-  PlaneRegs<T, TJ> pre;
-  fused_fetch_plane<T, TJ>(p, pre, i_first, j0, k0);
-  fused_store_plane<T, TJ>(sm, pre, i_first % 3);
-  if (i_first + 1 < nx) {
-    fused_fetch_plane<T, TJ>(p, pre, i_first + 1, j0, k0);
-    fused_store_plane<T, TJ>(sm, pre, (i_first + 1) % 3);
+inline Box3 interior_box(const mt_yee3d* d) {
+  const int n[3] = {d->nx, d->ny, d->nz};
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    // H at index i is psi-free on [h.lo_end, h.hi_begin); E at i uses the E
+    // profile at i-1: psi-free on [e.lo_end + 1, e.hi_begin + 1); the halo
+    // H'(i-1) must be psi-free too
+    lo[a] = std::max({1, d->h[a].lo_end + 1, d->e[a].lo_end + 1});
+    hi[a] = std::min({n[a] - 1, d->h[a].hi_begin, d->e[a].hi_begin + 1});
   }
-  HoldRegs<T> hold;
-  fused_fetch_h<T, TJ>(p, hold, i_first, j0, k0, wcol);
-  __syncthreads();
-  for (int ip = i_first; ip < i_end; ++ip) {
-    const int cur = ip % 3, nxt = (ip + 1) % 3;
-    const bool wr = ip >= i_begin;
-    const bool more = ip + 2 < nx;
-    const PlaneP<T> q = plane_ptrs(p, ip);
-    // ---- H'(ip): own cell, low halo row (j0-1), low halo column (k0-1)
-    T hx = T(0), hy = T(0), hz = T(0);
-    if (own) {
-      if (j < ny - 1 && k < nz - 1)
-        hx = wr ? fused_hx<T, TJ, true>(p, q, sm, cur, j, k, r, c, hold.x)
-                : fused_hx<T, TJ, false>(p, q, sm, cur, j, k, r, c, hold.x);
-      if (ip < nx - 1 && k < nz - 1)
-        hy = wr ? fused_hy<T, TJ, true>(p, q, sm, cur, nxt, ip, j, k, r, c, hold.y)
-                : fused_hy<T, TJ, false>(p, q, sm, cur, nxt, ip, j, k, r, c, hold.y);
-      if (ip < nx - 1 && j < ny - 1)
-        hz = wr ? fused_hz<T, TJ, true>(p, q, sm, cur, nxt, ip, j, k, r, c, hold.z)
-                : fused_hz<T, TJ, false>(p, q, sm, cur, nxt, ip, j, k, r, c, hold.z);
-      sm.h[0][r][c] = hx;
-      sm.h[1][r][c] = hy;
-      sm.h[2][r][c] = hz;
-    }
-    if (ty == 0 && j0 >= 1 && k < nz) {  // halo row j0-1: Hx', Hz'
-      const int jh = j0 - 1;
-      if (k < nz - 1) sm.h[0][0][c] = fused_hx<T, TJ, false>(p, q, sm, cur, jh, k, 0, c, hold.a);
-      if (ip < nx - 1) sm.h[2][0][c] = fused_hz<T, TJ, false>(p, q, sm, cur, nxt, ip, jh, k, 0, c, hold.b);
-    } else if (ty == wcol && tx < TJ && k0 >= 1 && j0 + tx < ny) {  // halo column k0-1: Hx', Hy'
-      const int jc = j0 + tx, kh = k0 - 1, rc = tx + 1;
-      if (jc < ny - 1) sm.h[0][rc][0] = fused_hx<T, TJ, false>(p, q, sm, cur, jc, kh, rc, 0, hold.a);
-      if (ip < nx - 1) sm.h[1][rc][0] = fused_hy<T, TJ, false>(p, q, sm, cur, nxt, ip, jc, kh, rc, 0, hold.b);
-    }
-    __syncthreads();
-    // prefetch the old H of the next plane and the E / eps plane after it
-    fused_fetch_h<T, TJ>(p, hold, ip + 1, j0, k0, wcol);
-    if (more) fused_fetch_plane<T, TJ>(p, pre, ip + 2, j0, k0);
-    // ---- E'(ip) of the own cell
-    if (own && wr) {
-      const T eps0 = sm.eps[cur][ty][tx];
-      if (ip < nx - 1) {  // Ex
-        const int x = j * nz + k;
-        T chy = T(0), chz = T(0);
-        const bool jy = j >= 1 && j < ny - 1, kz = k >= 1 && k < nz - 1;
-        if (jy) chy = ce2<T, true>(p.e[1], j - 1, q.ps[6], q.pn[6], (j - 1) * nz + k, (sm.h[2][r][c] - sm.h[2][r - 1][c]) / dx);
-        if (kz) chz = ce2<T, true>(p.e[2], k - 1, q.ps[7], q.pn[7], j * (nz - 2) + (k - 1), (sm.h[1][r][c] - sm.h[1][r][c - 1]) / dx);
-        T curl = (jy && kz) ? chy - chz : T(0);
-        T epsc = T(0.5) * (eps0 + sm.eps[nxt][ty][tx]);
-        T E = e_update(p, sm.e[cur][0][r][c], epsc, curl);
-        E = inject(p, src, 0, ip, j, k, E, epsc, c_src);
-        const long long o = (long long)ip * ny * nz + x;
-        p.Exn[o] = E;
-        if (tape) tape[o] = E;
-      }
-      if (j < ny - 1) {  // Ey
-        const int y = j * nz + k;
-        T chz = T(0), chx = T(0);
-        const bool kz = k >= 1 && k < nz - 1, ix = ip >= 1 && ip < nx - 1;
-        if (kz) chz = ce2<T, true>(p.e[2], k - 1, q.ps[8], q.pn[8], j * (nz - 2) + (k - 1), (sm.h[0][r][c] - sm.h[0][r][c - 1]) / dx);
-        if (ix) chx = ce2<T, true>(p.e[0], ip - 1, q.ps[9], q.pn[9], y, (hz - hzp) / dx);
-        T curl = (kz && ix) ? chz - chx : T(0);
-        T epsc = T(0.5) * (eps0 + sm.eps[cur][ty + 1][tx]);
-        T E = e_update(p, sm.e[cur][1][r][c], epsc, curl);
-        E = inject(p, src, 1, ip, j, k, E, epsc, c_src);
-        const long long o = (long long)ip * (ny - 1) * nz + y;
-        p.Eyn[o] = E;
-        if (tape) tape[(long long)(nx - 1) * ny * nz + o] = E;
-      }
-      if (k < nz - 1) {  // Ez
-        const int z = j * (nz - 1) + k;
-        T chx = T(0), chy = T(0);
-        const bool ix = ip >= 1 && ip < nx - 1, jy = j >= 1 && j < ny - 1;
-        if (ix) chx = ce2<T, true>(p.e[0], ip - 1, q.ps[10], q.pn[10], z, (hy - hyp) / dx);
-        if (jy) chy = ce2<T, true>(p.e[1], j - 1, q.ps[11], q.pn[11], (j - 1) * (nz - 1) + k, (sm.h[0][r][c] - sm.h[0][r - 1][c]) / dx);
-        T curl = (ix && jy) ? chx - chy : T(0);
-        T epsc = T(0.5) * (eps0 + sm.eps[cur][ty][tx + 1]);
-        T E = e_update(p, sm.e[cur][2][r][c], epsc, curl);
-        E = inject(p, src, 2, ip, j, k, E, epsc, c_src);
-        const long long o = (long long)ip * ny * (nz - 1) + z;
-        p.Ezn[o] = E;
-        if (tape) tape[(long long)(nx - 1) * ny * nz + (long long)nx * (ny - 1) * nz + o] = E;
-      }
-    }
-    hyp = hy;
-    hzp = hz;
-    if (more) fused_store_plane<T, TJ>(sm, pre, (ip + 2) % 3);
-    __syncthreads();
+  return Box3{lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]};
+}
+
+inline Piece make_piece(int i0, int i1, int j0, int j1, int k0, int k1, int narrow) {
+  Piece q{};
+  q.i0 = i0, q.j0 = j0, q.k0 = k0;
+  q.ni = std::max(0, i1 - i0), q.nj = std::max(0, j1 - j0), q.nk = std::max(0, k1 - k0);
+  q.narrow = narrow;
+  q.gk = (q.nk + (narrow ? 15 : 63)) / (narrow ? 16 : 64);
+  q.gj = (q.nj + (narrow ? 15 : 3)) / (narrow ? 16 : 4);
+  return q;
+}
+
+inline Pieces shell_pieces(const Box3& b, int nx, int ny, int nz) {
+  Piece all[6] = {make_piece(0, b.I0, 0, ny, 0, nz, 0),          make_piece(b.I1, nx, 0, ny, 0, nz, 0),
+                  make_piece(b.I0, b.I1, 0, b.J0, 0, nz, 0),     make_piece(b.I0, b.I1, b.J1, ny, 0, nz, 0),
+                  make_piece(b.I0, b.I1, b.J0, b.J1, 0, b.K0, 1), make_piece(b.I0, b.I1, b.J0, b.J1, b.K1, nz, 1)};
+  Pieces P{};
+  int blocks = 0;
+  for (int t = 0; t < 6; ++t) {
+    const Piece& q = all[t];
+    if (q.ni == 0 || q.nj == 0 || q.nk == 0) continue;
+    P.p[P.n] = q;
+    P.p[P.n].b0 = blocks;
+    blocks += q.gk * q.gj * q.ni;
+    ++P.n;
   }
+  P.total = blocks;
+  return P;
 }
 
 // Monitor descriptors with every field term moved from the primary buffers
-// to the matching alt buffers (same offsets).
+// to the matching alternate buffers (same offsets).
 void retarget(const mt_monitor* in, int n, mt_monitor* out, void* const* prim, const long long* bytes,
-              void* const* alt) {
+              void* const* alt, int n_arrays) {
   for (int m = 0; m < n; ++m) {
     out[m] = in[m];
     for (int q = 0; q < in[m].n_terms; ++q) {
@@ -609,7 +641,7 @@ void retarget(const mt_monitor* in, int n, mt_monitor* out, void* const* prim, c
       for (int w = 0; w < 2; ++w) {
         if (!ptrs[w]) continue;
         const char* a = static_cast<const char*>(ptrs[w]);
-        for (int f = 0; f < 18; ++f) {
+        for (int f = 0; f < n_arrays; ++f) {
           const char* b0 = static_cast<const char*>(prim[f]);
           if (a >= b0 && a < b0 + bytes[f]) {
             const void* moved = static_cast<const char*>(alt[f]) + (a - b0);
@@ -624,80 +656,68 @@ void retarget(const mt_monitor* in, int n, mt_monitor* out, void* const* prim, c
 }
 
 template <typename T, int TJ>
-int run3_fused(const mt_yee3d* d, const mt_source3d& sv, const double* coef, const MonitorPlan& mp, int step0,
-               int k, hipStream_t s) {
+int run3_split(const mt_yee3d* d, K3<T> p, const mt_source3d& sv, const double* coef, const MonitorPlan& mp,
+               int step0, int k, hipStream_t s) {
   const int nx = d->nx, ny = d->ny, nz = d->nz;
-  void* prim[18] = {d->Ex, d->Ey, d->Ez, d->Hx, d->Hy, d->Hz};
-  for (int q = 0; q < 12; ++q) prim[6 + q] = d->psi[q];
-  long long n[18];
+  const Box3 b = interior_box(d);
+  void* prim[6] = {d->Ex, d->Ey, d->Ez, d->Hx, d->Hy, d->Hz};
   const long long nE[3] = {(long long)(nx - 1) * ny * nz, (long long)nx * (ny - 1) * nz, (long long)nx * ny * (nz - 1)};
   const long long nH[3] = {(long long)nx * (ny - 1) * (nz - 1), (long long)(nx - 1) * ny * (nz - 1),
                            (long long)(nx - 1) * (ny - 1) * nz};
-  const long long nP[12] = {nH[0], nH[0], nH[1], nH[1], nH[2], nH[2],
-                            (long long)(nx - 1) * (ny - 2) * nz, (long long)(nx - 1) * ny * (nz - 2),
-                            (long long)nx * (ny - 1) * (nz - 2), (long long)(nx - 2) * (ny - 1) * nz,
-                            (long long)(nx - 2) * ny * (nz - 1), (long long)nx * (ny - 2) * (nz - 1)};
+  long long bytes[6];
   for (int q = 0; q < 3; ++q) {
-    n[q] = nE[q];
-    n[3 + q] = nH[q];
+    bytes[q] = nE[q] * (long long)sizeof(T);
+    bytes[3 + q] = nH[q] * (long long)sizeof(T);
   }
-  for (int q = 0; q < 12; ++q) n[6 + q] = nP[q];
-  long long bytes[18];
-  for (int q = 0; q < 18; ++q) bytes[q] = n[q] * (long long)sizeof(T);
   std::vector<mt_monitor> alt_mons(mp.n_mons > 0 ? mp.n_mons : 1);
-  if (mp.n_mons) retarget(mp.mons, mp.n_mons, alt_mons.data(), prim, bytes, d->alt);
+  if (mp.n_mons) retarget(mp.mons, mp.n_mons, alt_mons.data(), prim, bytes, d->alt, 6);
   MonitorPlan mpa = mp;
   mpa.mons = alt_mons.data();
 
-  F3<T> p;
-  p.nx = nx;
-  p.ny = ny;
-  p.nz = nz;
-  p.lean = d->lean;
-  p.dt = static_cast<T>(d->dt);
-  p.dx = static_cast<T>(d->dx);
-  p.eps = static_cast<const T*>(d->eps);
-  for (int a = 0; a < 3; ++a) {
-    p.h[a] = pml_of<T>(d->h[a]);
-    p.e[a] = pml_of<T>(d->e[a]);
-  }
-  const int gk = (nz + FK - 1) / FK, gj = (ny + TJ - 1) / TJ;
+  I3<T> q{};
+  q.nx = nx, q.ny = ny, q.nz = nz, q.lean = d->lean;
+  q.I0 = b.I0, q.I1 = b.I1, q.J0 = b.J0, q.J1 = b.J1, q.K0 = b.K0, q.K1 = b.K1;
+  q.kbase = (b.K0 / 64) * 64;
+  q.njt = (b.J1 - b.J0 + TJ - 1) / TJ;
+  q.nkt = (b.K1 - q.kbase + 63) / 64;
   int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     cus = 256;
-  int nseg = (4 * cus + gk * gj - 1) / (gk * gj);
-  nseg = std::max(1, std::min(nseg, nx / 16));
-  p.i_seg = (nx + nseg - 1) / nseg;
-  nseg = (nx + p.i_seg - 1) / p.i_seg;
-  dim3 blk(FK, TJ, 1), grd(gk, gj, nseg);
+  // enough x segments for about three workgroups per CU, each at least 16 planes
+  const int tiles2d = q.njt * q.nkt;
+  int nseg = std::max(1, std::min((3 * cus + tiles2d - 1) / tiles2d, (b.I1 - b.I0) / 16));
+  q.iseg = (b.I1 - b.I0 + nseg - 1) / nseg;
+  nseg = (b.I1 - b.I0 + q.iseg - 1) / q.iseg;
+  q.ntiles = tiles2d * nseg;
+  q.per_xcd = (q.ntiles + 7) / 8;
+  q.dt = static_cast<T>(d->dt);
+  q.dx = static_cast<T>(d->dx);
+  q.eps = static_cast<const T*>(d->eps);
+  const Pieces P = shell_pieces(b, nx, ny, nz);
+  const long long box_cells = (long long)(b.I1 - b.I0) * (b.J1 - b.J0) * (b.K1 - b.K0);
   const long long e_cells = nE[0] + nE[1] + nE[2];
   int S = mp.stage_steps > 0 ? mp.stage_steps : 1;
   int filled = 0, first = step0;
   for (int s_ = 0; s_ < k; ++s_) {
-    void* const* src_b = (s_ % 2 == 0) ? prim : d->alt;
-    void* const* dst_b = (s_ % 2 == 0) ? d->alt : prim;
-    p.Ex = static_cast<const T*>(src_b[0]);
-    p.Ey = static_cast<const T*>(src_b[1]);
-    p.Ez = static_cast<const T*>(src_b[2]);
-    p.Hx = static_cast<const T*>(src_b[3]);
-    p.Hy = static_cast<const T*>(src_b[4]);
-    p.Hz = static_cast<const T*>(src_b[5]);
-    p.Exn = static_cast<T*>(dst_b[0]);
-    p.Eyn = static_cast<T*>(dst_b[1]);
-    p.Ezn = static_cast<T*>(dst_b[2]);
-    p.Hxn = static_cast<T*>(dst_b[3]);
-    p.Hyn = static_cast<T*>(dst_b[4]);
-    p.Hzn = static_cast<T*>(dst_b[5]);
-    for (int q = 0; q < 12; ++q) {
-      p.psi[q] = static_cast<const T*>(src_b[6 + q]);
-      p.psin[q] = static_cast<T*>(dst_b[6 + q]);
-    }
-    T c = sv.kind ? static_cast<T>(coef[s_]) : T(0);
+    void* const* A = (s_ % 2 == 0) ? prim : d->alt;
+    void* const* B = (s_ % 2 == 0) ? d->alt : prim;
+    q.Ex = static_cast<const T*>(A[0]), q.Ey = static_cast<const T*>(A[1]), q.Ez = static_cast<const T*>(A[2]);
+    q.Hx = static_cast<const T*>(A[3]), q.Hy = static_cast<const T*>(A[4]), q.Hz = static_cast<const T*>(A[5]);
+    q.oEx = static_cast<T*>(B[0]), q.oEy = static_cast<T*>(B[1]), q.oEz = static_cast<T*>(B[2]);
+    q.oHx = static_cast<T*>(B[3]), q.oHy = static_cast<T*>(B[4]), q.oHz = static_cast<T*>(B[5]);
+    p.Ex = static_cast<T*>(A[0]), p.Ey = static_cast<T*>(A[1]), p.Ez = static_cast<T*>(A[2]);
+    p.Hx = static_cast<T*>(A[3]), p.Hy = static_cast<T*>(A[4]), p.Hz = static_cast<T*>(A[5]);
+    p.oEx = q.oEx, p.oEy = q.oEy, p.oEz = q.oEz, p.oHx = q.oHx, p.oHy = q.oHy, p.oHz = q.oHz;
+    const T c = sv.kind ? static_cast<T>(coef[s_]) : T(0);
     T* slot = d->tape ? static_cast<T*>(d->tape) + (long long)s_ * e_cells : nullptr;
     timing_begin(s);
-    hipLaunchKernelGGL((fused3_kernel<T, TJ>), grd, blk, 0, s, p, sv, c, slot);
-    timing_end(s, (long long)nx * ny * nz);
-    int rc = check_launch("yee3d fused step");
+    hipLaunchKernelGGL((y3_interior<T, TJ>), dim3(8 * q.per_xcd), dim3(64, TJ + 2), 0, s, q, sv, c, slot);
+    timing_end(s, box_cells);
+    if (P.total) {
+      hipLaunchKernelGGL(h3_pieces<T>, dim3(P.total), dim3(256), 0, s, p, P);
+      hipLaunchKernelGGL(e3_pieces<T>, dim3(P.total), dim3(256), 0, s, p, P, sv, c, slot);
+    }
+    int rc = check_launch("yee3d split step");
     if (rc) return rc;
     if (mp.n_mons) {
       if ((rc = monitors_sample(s_ % 2 == 0 ? mpa : mp, filled, s_, s))) return rc;
@@ -709,9 +729,9 @@ int run3_fused(const mt_yee3d* d, const mt_source3d& sv, const double* coef, con
     }
   }
   if (k % 2 == 1) {
-    for (int q = 0; q < 18; ++q)
-      if (hipMemcpyAsync(prim[q], d->alt[q], bytes[q], hipMemcpyDeviceToDevice, s) != hipSuccess)
-        return fail(MT_EHIP, "yee3d fused: copy-back failed");
+    for (int q6 = 0; q6 < 6; ++q6)
+      if (hipMemcpyAsync(prim[q6], d->alt[q6], bytes[q6], hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return fail(MT_EHIP, "yee3d split: copy-back failed");
   }
   return MT_OK;
 }
@@ -731,12 +751,12 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
     p.h[a] = pml_of<T>(d->h[a]);
     p.e[a] = pml_of<T>(d->e[a]);
   }
-  p.Ex = static_cast<T*>(d->Ex);
-  p.Ey = static_cast<T*>(d->Ey);
-  p.Ez = static_cast<T*>(d->Ez);
-  p.Hx = static_cast<T*>(d->Hx);
-  p.Hy = static_cast<T*>(d->Hy);
-  p.Hz = static_cast<T*>(d->Hz);
+  p.Ex = p.oEx = static_cast<T*>(d->Ex);
+  p.Ey = p.oEy = static_cast<T*>(d->Ey);
+  p.Ez = p.oEz = static_cast<T*>(d->Ez);
+  p.Hx = p.oHx = static_cast<T*>(d->Hx);
+  p.Hy = p.oHy = static_cast<T*>(d->Hy);
+  p.Hz = p.oHz = static_cast<T*>(d->Hz);
   for (int q = 0; q < 12; ++q) p.psi[q] = static_cast<T*>(d->psi[q]);
   const bool media = d->damp[0] != nullptr;
   for (int q = 0; q < 12; ++q) p.dmp[q] = static_cast<const T*>(d->damp[q]);
@@ -750,9 +770,14 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
   }
   mt_source3d sv{};
   if (src && src->kind) sv = *src;
-  if (d->variant == 1) return run3_fused<T, 8>(d, sv, coef, mp, step0, k, s);
-  if (d->variant == 2) return run3_fused<T, 4>(d, sv, coef, mp, step0, k, s);
-  if (d->variant == 3) return run3_fused<T, 16>(d, sv, coef, mp, step0, k, s);
+  const Box3 box = interior_box(d);
+  const bool split = d->variant >= 1 && !media && box.I1 > box.I0 && box.J1 > box.J0 && box.K1 > box.K0;
+  if (split && d->variant == 1) return run3_split<T, 6>(d, p, sv, coef, mp, step0, k, s);
+  if (split && d->variant == 2) {
+    if constexpr (sizeof(T) == 4) return run3_split<T, 14>(d, p, sv, coef, mp, step0, k, s);
+    else return run3_split<T, 6>(d, p, sv, coef, mp, step0, k, s);
+  }
+  if (split && d->variant == 3) return run3_split<T, 4>(d, p, sv, coef, mp, step0, k, s);
   dim3 blk(BZ, BYY, 1);
   dim3 grd((p.nz + BZ - 1) / BZ, (p.ny + BYY - 1) / BYY, p.nx);
   int S = mp.stage_steps > 0 ? mp.stage_steps : 1;
@@ -822,9 +847,8 @@ extern "C" int mt_yee3d_run(const mt_yee3d* p, const mt_source3d* src, const dou
   }
   if (p->variant < 0 || p->variant > 3) return fail(MT_EINVAL, "3D variant must be 0..3");
   if (p->variant >= 1 && p->variant <= 3) {
-    for (int q = 0; q < 18; ++q)
-      if (!p->alt[q]) return fail(MT_EINVAL, "fused 3D variant needs all 18 alt buffers");
-    if (p->dtype == MT_F64 && p->variant == 3) return fail(MT_EINVAL, "3D variant 3 (16-row tiles) is FP32 only");
+    for (int q = 0; q < 6; ++q)
+      if (!p->alt[q]) return fail(MT_EINVAL, "split 3D variant needs the 6 alt field buffers");
   }
   bool any_damp = false, all_damp = true;
   for (int q = 0; q < 12; ++q) {
@@ -835,7 +859,6 @@ extern "C" int mt_yee3d_run(const mt_yee3d* p, const mt_source3d* src, const dou
   if (p->n_poles < 0 || p->n_poles > 64) return fail(MT_EINVAL, "3D n_poles must be 0..64");
   if (p->n_poles > 0 && !any_damp) return fail(MT_EINVAL, "3D poles need the damping arrays (d1 = 1, d2 = dt)");
   if (any_damp) {
-    if (p->variant != 0) return fail(MT_EINVAL, "3D media run on the two-pass variant 0 only");
     if (p->lean) return fail(MT_EINVAL, "3D media are not supported by the lean stepper");
     if (p->n_poles > 0) {
       if (!p->c1 || !p->c2) return fail(MT_EINVAL, "3D poles without c1 / c2");

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 16
+ABI_VERSION = 18
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -26,7 +26,7 @@ class PmlAxis(ctypes.Structure):
 
 class Source2D(ctypes.Structure):
     _fields_ = [("present", c_int), ("i0", c_int), ("i1", c_int), ("j0", c_int), ("j1", c_int),
-                ("w", c_void_p)]
+                ("w", c_void_p), ("wstep", c_ll)]
 
 
 class TM2D(ctypes.Structure):
@@ -153,6 +153,12 @@ def lib():
                                    c_void_p]
     L.mt_abi_layout.restype = c_int
     L.mt_abi_layout.argtypes = [P(c_ll), c_int]
+    L.mt_tm2d_tiles.restype = c_int
+    L.mt_tm2d_tiles.argtypes = [P(TM2D), P(c_int), P(c_int)]
+    L.mt_te2d_tiles.restype = c_int
+    L.mt_te2d_tiles.argtypes = [P(TE2D), P(c_int), P(c_int)]
+    L.mt_div_recip_f32.restype = ctypes.c_float
+    L.mt_div_recip_f32.argtypes = [ctypes.c_float]
     if L.mt_abi_version() != ABI_VERSION:
         raise RuntimeError("meep_torch_amd: libmeepfdtd.so ABI mismatch; rebuild")
     _lib = L

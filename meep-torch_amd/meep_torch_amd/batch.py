@@ -31,7 +31,7 @@ _ORDER = engine2d.TM_ORDER
 class BatchItem(ctypes.Structure):
     _fields_ = [("prob", E.TM2D), ("src", E.Source2D), ("coef_tab", E.c_void_p),
                 ("mons", ctypes.POINTER(E.Monitor)), ("n_mons", E.c_int), ("pad_", E.c_int),
-                ("stage", E.c_void_p), ("stage_bytes", E.c_ll)]
+                ("stage", E.c_void_p), ("stage_bytes", E.c_ll), ("hcoef_tab", E.c_void_p)]
 
 
 def _bind(L):
@@ -102,13 +102,15 @@ class BatchedTM:
         nck = (self.n_steps + self.K - 1) // self.K
         V["ckpt"] = [{f: z(*shapes[f]) for f in FIELDS} for _ in range(nck)] if self.adjoint else []
         V["tape"] = z(self.K, nx, ny) if self.adjoint else None
-        plan = engine2d.source_plan(source, (nx, ny), self.device, self.dtype)
+        plan = engine2d.source_plan(source, (nx, ny), self.device, self.dtype,
+                                    hshapes=engine2d.h_shapes("tm", nx, ny))
         V["plan"] = plan
-        if plan is not None and plan["box"] is not None:
-            vals = [coeffs.dt * plan["source"].amplitude(t) for t in range(self.n_steps)]
-            V["coef"] = torch.tensor(vals, dtype=torch.float64).to(self.dtype).to(self.device)
-        else:
-            V["coef"] = None
+        ev, hv = engine2d.coef_arrays(plan, coeffs.dt, 0, self.n_steps)
+        # per-step dt * amplitude tables for the whole run, cast like the
+        # single-problem path casts each step's coefficient
+        tab = lambda arr: torch.tensor(list(arr)[:self.n_steps], dtype=torch.float64).to(self.dtype).to(self.device)
+        V["coef"] = tab(ev) if ev is not None else None
+        V["hcoef"] = tab(hv) if hv is not None else None
         arr, outs, keep = engine2d.monitor_structs(V["state"], monitors, self.K, self.dtype, self.device)
         V["mon_arr"], V["mon_keep"] = arr, keep
         V["chunk_acc"] = [o for (_, _, o) in outs]
@@ -147,6 +149,10 @@ class BatchedTM:
             s, _ = engine2d.source_struct(V["plan"])
             it.src = s
             it.coef_tab = V["coef"].data_ptr() if V["coef"] is not None else None
+            if V["hcoef"] is not None:
+                for slot, hb in V["plan"]["h"]:
+                    engine2d._fill_source(it.prob.hsrc[slot], hb)
+                it.hcoef_tab = V["hcoef"].data_ptr()
             it.mons = ctypes.cast(V["mon_arr"], ctypes.POINTER(E.Monitor))
             it.n_mons = len(V["monitors"])
             it.stage = V["stage"].data_ptr() if V["stage"] is not None else None

@@ -35,10 +35,26 @@ def _fields_of(state):
     return {f: getattr(state, f).detach().clone().contiguous() for f in state_names(state)}
 
 
+def _refuse_media_grads(c):
+    """The reverse sweep differentiates with respect to eps only; a material
+    tensor (sigma_d, sigma_b, pole masks / c3) that asks for a gradient
+    raises instead of silently receiving none."""
+    damp = c.damp
+    for name, t in (("sigma_d", getattr(damp, "_sd", None)), ("sigma_b", getattr(damp, "_sb", None))):
+        if torch.is_tensor(t) and t.requires_grad:
+            raise NotImplementedError(f"gradient with respect to {name} is not supported in 3D")
+    if c.pols is not None:
+        for ax, trip in c.pols.items():
+            for t in trip:
+                if torch.is_tensor(t) and t.requires_grad:
+                    raise NotImplementedError("gradient with respect to 3D pole masks / coefficients is not supported")
+
+
 def run_3d_adjoint(c, n_steps: int, chunk: int, source, monitors, loss_fn, state0):
     from .adjoint2d import _adj_monitors
     if chunk < 1:
         raise ValueError("chunk must be >= 1")
+    _refuse_media_grads(c)
     L = E.lib()
     monitors = list(monitors)
     work = _fields_of(state0)

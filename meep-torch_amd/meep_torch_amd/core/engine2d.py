@@ -31,6 +31,8 @@ def source_plan(source, field_shape, device, dtype, hshapes=None):
     this caller accepts, from h_shapes; None refuses magnetic masks)."""
     if source is None:
         return None
+    if getattr(source, "parts", None) is not None:
+        return _multi_plan(source, field_shape, device, dtype)
     if getattr(source, "per_step_mask", False):
         raise NotImplementedError(f"{type(source).__name__}: per-step masks are not supported")
     h_masks = getattr(source, "h_masks", None) or {}
@@ -55,6 +57,73 @@ def source_plan(source, field_shape, device, dtype, hshapes=None):
         if hb is not None:
             hboxes.append((slot, hb))
     return {"source": source, "box": box, "h": hboxes}
+
+
+def _multi_plan(source, field_shape, device, dtype):
+    """Several electric sources injected as one: the reference sums
+    amp_i(t) * mask_i into one mask every step and injects it with amplitude
+    1 (mp/cpp/core.py:584-589).  The plan keeps every part's mask cropped to
+    the union box of their nonzeros; run_chunk builds the per-step sums."""
+    parts = list(source.parts)
+    for p in parts:
+        if getattr(p, "h_masks", None):
+            raise NotImplementedError("several sources with TFSF magnetic currents are not supported")
+        if getattr(p, "per_step_mask", False):
+            raise NotImplementedError(f"{type(p).__name__}: per-step masks are not supported")
+        if tuple(p.mask.shape) != tuple(field_shape):
+            raise ValueError(f"source mask shape {tuple(p.mask.shape)} != field shape {tuple(field_shape)}")
+    masks = [p.mask.detach().to(device=device, dtype=dtype) for p in parts]
+    nz = torch.zeros(field_shape, dtype=torch.bool, device=device)
+    for m in masks:
+        nz |= m.ne(0)
+    if not bool(nz.any()):
+        return None
+    idx = nz.nonzero()
+    lo = idx.min(0).values.tolist()
+    hi = (idx.max(0).values + 1).tolist()
+    w = torch.stack([m[lo[0]:hi[0], lo[1]:hi[1]] for m in masks]).contiguous()
+    return {"source": source, "parts": parts, "multi": w, "ubox": (lo, hi), "box": None, "h": []}
+
+
+class _SummedMask:
+    """Source protocol of one engine call over a per-step weight stack: the
+    stack already holds sum_i amp_i(t) * mask_i, injected with amplitude 1."""
+
+    @staticmethod
+    def amplitude(t):
+        return 1.0
+
+
+MULTI_STACK_BYTES = 1 << 27      # per-step weight stack of one engine call
+
+
+def _run_chunk_multi(mode, fields, coeffs, step0, k, plan, monitors, tape, reuse, raw, variant, alt):
+    """run_chunk for a several-source plan: the k steps go to the engine in
+    calls of at most MULTI_STACK_BYTES of weight stack, monitors summing on
+    in one sequence across the calls.  Per step the parts' amplitudes are
+    taken in source order (the reference's call order); each is rounded to
+    the grid dtype and multiplied into its mask, and the products are summed
+    left to right, as the reference's `amp_i * mask_i` tensor ops do."""
+    parts, W = plan["parts"], plan["multi"]
+    lo, hi = plan["ubox"]
+    n, bi, bj = W.shape
+    per = max(1, MULTI_STACK_BYTES // (bi * bj * W.element_size()))
+    out, done = reuse, 0
+    while True:
+        m = min(per, k - done)
+        amps = [[p.amplitude(t) for p in parts] for t in range(step0 + done, step0 + done + m)]
+        A = torch.tensor(amps, dtype=torch.float64).reshape(m, n).to(W.dtype).to(W.device)
+        stack = A[:, 0, None, None] * W[0]
+        for i in range(1, n):
+            stack = stack + A[:, i, None, None] * W[i]
+        sub = {"source": _SummedMask(), "box": (lo, hi, stack.contiguous()), "wstep": bi * bj, "h": []}
+        out = run_chunk(mode, fields, coeffs, step0 + done, m, sub, monitors,
+                        tape=tape[done:done + m] if tape is not None else None, reuse=out, raw=True,
+                        variant=variant, alt=alt)
+        done += m
+        if done >= k:
+            break
+    return out if raw else finish_outputs(out, fields)
 
 
 def coef_array(plan, dt, step0, k):
@@ -96,6 +165,7 @@ def source_struct(plan):
     (i0, j0), (i1, j1), w = plan["box"]
     s.present, s.i0, s.i1, s.j0, s.j1 = 1, i0, i1, j0, j1
     s.w = E.ptr(w)
+    s.wstep = int(plan.get("wstep", 0))
     return s, w
 
 
@@ -250,6 +320,23 @@ def _te_media(d, c, fields, dtype, device):
     return keep
 
 
+def tile_counts(mode, nx, ny, coeffs, dtype, variant):
+    """(interior tiles, ring tiles) of the temporally blocked launch that a
+    run of this grid and CPML makes with tile `variant` (host-only query)."""
+    L = E.lib()
+    d = E.TM2D() if mode == "tm" else E.TE2D()
+    d.nx, d.ny, d.dtype, d.variant = nx, ny, E.dtype_code(dtype), int(variant)
+    names = ("hy", "hx", "ex", "ey") if mode == "tm" else ("hx", "hy", "ey", "ex")
+    for n in names:
+        setattr(d, n, _axis(coeffs, n))
+    if mode == "tm":
+        d.y_pmc = int(bool(coeffs.y_pmc))
+    n_in, n_ring = ctypes.c_int(), ctypes.c_int()
+    fn = L.mt_tm2d_tiles if mode == "tm" else L.mt_te2d_tiles
+    E.check(fn(ctypes.byref(d), ctypes.byref(n_in), ctypes.byref(n_ring)), f"mt_{mode}2d_tiles")
+    return n_in.value, n_ring.value
+
+
 TM_ORDER = ("Ez", "Dz", "Hx", "Hy", "psi_Ezy", "psi_Ezx", "psi_Hyx", "psi_Hxy")
 TE_ORDER = ("Hz", "Dx", "Dy", "psi_Eyx", "psi_Exy", "psi_Hzy", "psi_Hzx")
 
@@ -261,6 +348,8 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
     (k, (nx-1)*ny + nx*(ny-1)) rows [Ex | Ey] for TE.
     reuse: raw outputs of a previous call whose accumulators keep summing.
     raw: return the raw (monitor, kind, buffer) triples instead of shaped outputs."""
+    if plan is not None and plan.get("multi") is not None:
+        return _run_chunk_multi(mode, fields, coeffs, step0, k, plan, monitors, tape, reuse, raw, variant, alt)
     L = E.lib()
     ref = fields["Ez"] if mode == "tm" else fields["Hz"]
     dtype, device = ref.dtype, ref.device
@@ -302,8 +391,8 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
             raise ValueError(f"state field {f} must be contiguous")
     media = bool(d.n_poles or d.kerr or d.d1 or d.h1x) if mode == "tm" else \
         bool(d.npx or d.npy or d.d1x or d.hz1 or d.kerr)
-    if (plan is not None and plan.get("h")) or media:
-        variant = -1  # magnetic currents and media are handled by the two-pass kernels
+    if media or (mode == "te" and plan is not None and plan.get("h")):
+        variant = -1  # media and TE magnetic currents are handled by the two-pass kernels
     if variant is None:
         from .. import tune
         variant = tune.variant_for(mode, fields, coeffs, plan, monitors, k)

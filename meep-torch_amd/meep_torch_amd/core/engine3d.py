@@ -109,7 +109,8 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=N
     every step; reuse: raw outputs of a previous call whose accumulators keep
     summing (one sequential sum across calls); raw: return the raw
     (monitor, kind, buffer) outputs instead of the reshaped ones.  variant:
-    0 two-pass, 1-3 fused streaming step (default: DEFAULT_VARIANT)."""
+    0 two-pass, 1-3 split step: one-pass interior kernel (tile rows 6, 14,
+    4) + shell cells (default: DEFAULT_VARIANT)."""
     from .engine2d import finish_outputs, monitor_structs
     L = E.lib()
     if c.cpml is None:
@@ -143,7 +144,7 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=N
         v = 0
     alt = []
     if 1 <= v <= 3 and n_steps > 0:
-        alt = [torch.empty_like(fields[f]) for f in COMPS] + [torch.zeros_like(fields[f]) for f in PSI_NAMES]
+        alt = [torch.empty_like(fields[f]) for f in COMPS]   # ping-pong field set; psi stay in place
         for q, t in enumerate(alt):
             d.alt[q] = t.data_ptr()
     d.variant = v if n_steps > 0 else 0

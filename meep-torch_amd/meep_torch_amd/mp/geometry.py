@@ -1,5 +1,8 @@
-"""Grid spec, materials, axis-aligned Blocks and their rasterization onto the
-Yee grid (the host-side setup of the Simulation facade).
+"""Grid spec, materials, geometric objects (Block, Ellipsoid, Cylinder,
+Sphere, Prism) and their rasterization onto the Yee grid (the host-side setup
+of the Simulation facade).  Axis-aligned Blocks take the exact rectangle
+coverage; every other object is the polygon of its 2D cross-section on the
+exact polygon coverage of geometry.raster.
 
 Coordinates follow meep: the cell is centred on the origin; grid cell (i, j)
 covers the corner-origin square [i dx, (i+1) dx) x [j dx, (j+1) dx)
@@ -170,18 +173,125 @@ class GeometricObject:
         raise NotImplementedError
 
 
+def to_corner(v, gs):
+    """World (centre-origin) point -> corner-origin (x, y), clipped to the
+    cell, mp.inf mapped to the far edge (meep_torch/mp/coords.py:55-61)."""
+    x = gs.sx if math.isinf(v.x) else v.x + gs.sx / 2.0
+    y = gs.sy if math.isinf(v.y) else v.y + gs.sy / 2.0
+    return _clip(x, 0.0, gs.sx), _clip(y, 0.0, gs.sy)
+
+
+def circle_vertices(radius, cx, cy, gs, n=None):
+    """The polygon that stands for a circle: n vertices from angle 0, n =
+    round(2 pi r / dx) clamped to [16, 256] unless given (the reference's
+    resolution rule, meep_torch/mp/geometry.py:47-51, kept so that the eps
+    maps agree)."""
+    if n is None:
+        n = min(256, max(16, int(round(2 * math.pi * radius / gs.dx))))
+    ang = np.linspace(0.0, 2 * math.pi, n, endpoint=False)
+    return np.stack([cx + radius * np.cos(ang), cy + radius * np.sin(ang)], axis=1)
+
+
+def polygon_coverage(vertices, gs):
+    from ..geometry.polygons import Polygon
+    from ..geometry.raster import coverage_array
+    return coverage_array(Polygon(np.asarray(vertices)).vertices, gs.nx, gs.ny, gs.dx)
+
+
+def _inplane_pair(e1, e2, e3):
+    e1, e2, e3 = Vector3(*e1), Vector3(*e2), Vector3(*e3)
+    if e1.z != 0.0 or e2.z != 0.0 or e3.x != 0.0 or e3.y != 0.0:
+        raise NotImplementedError("Block basis must keep e1/e2 in the x-y plane (2D raster)")
+    return e1.unit(), e2.unit()
+
+
 class Block(GeometricObject):
-    """Axis-aligned brick (rotated bases raise NotImplementedError)."""
+    """Brick with basis e1 / e2 (in-plane rotation allowed): axis-aligned
+    blocks take the exact rectangle coverage, rotated ones the polygon path."""
 
     def __init__(self, size, e1=Vector3(1, 0, 0), e2=Vector3(0, 1, 0), e3=Vector3(0, 0, 1), **kw):
         super().__init__(**kw)
         self.size = Vector3(*size)
-        if Vector3(*e1) != Vector3(1, 0, 0) or Vector3(*e2) != Vector3(0, 1, 0):
-            raise NotImplementedError("rotated Block is not supported")
+        self.e1, self.e2, self.e3 = Vector3(*e1), Vector3(*e2), Vector3(*e3)
 
     def coverage(self, gs):
-        x0, y0, x1, y1 = rect_corners(self.center, self.size, gs)
-        return rect_coverage(x0, y0, x1, y1, gs)
+        u1, u2 = _inplane_pair(self.e1, self.e2, self.e3)
+        if u1 == Vector3(1.0, 0.0, 0.0) and u2 == Vector3(0.0, 1.0, 0.0):
+            x0, y0, x1, y1 = rect_corners(self.center, self.size, gs)
+            return rect_coverage(x0, y0, x1, y1, gs)
+        if math.isinf(self.size.x) or math.isinf(self.size.y):
+            raise NotImplementedError("rotated Block with mp.inf size not supported")
+        x, y = to_corner(self.center, gs)
+        hx, hy = self.size.x / 2.0, self.size.y / 2.0
+        return polygon_coverage([[x + a * u1.x + b * u2.x, y + a * u1.y + b * u2.y]
+                                 for a, b in ((-hx, -hy), (hx, -hy), (hx, hy), (-hx, hy))], gs)
+
+
+class Ellipsoid(Block):
+    """Ellipse inscribed in its (possibly rotated) block, as a polygon of
+    round(2 pi max(a, b) / dx) vertices clamped to [16, 256]."""
+
+    def coverage(self, gs):
+        u1, u2 = _inplane_pair(self.e1, self.e2, self.e3)
+        x, y = to_corner(self.center, gs)
+        a, b = self.size.x / 2.0, self.size.y / 2.0
+        n = min(256, max(16, int(round(2 * math.pi * max(a, b) / gs.dx))))
+        ang = np.linspace(0.0, 2 * math.pi, n, endpoint=False)
+        ca, sb = a * np.cos(ang), b * np.sin(ang)
+        return polygon_coverage(np.stack([x + ca * u1.x + sb * u2.x, y + ca * u1.y + sb * u2.y], axis=1), gs)
+
+
+class Cylinder(GeometricObject):
+    """Cylinder along z: its circular cross-section (n_pts vertices when
+    given, else the circle rule of circle_vertices)."""
+
+    def __init__(self, radius, axis=Vector3(0, 0, 1), height=1e20, n_pts=None, **kw):
+        super().__init__(**kw)
+        self.radius, self.axis, self.height, self.n_pts = float(radius), Vector3(*axis), float(height), n_pts
+
+    def coverage(self, gs):
+        if self.axis.x != 0.0 or self.axis.y != 0.0:
+            raise NotImplementedError("Cylinder axis off z not supported (2D raster)")
+        x, y = to_corner(self.center, gs)
+        n = max(3, int(self.n_pts)) if self.n_pts is not None else None
+        return polygon_coverage(circle_vertices(self.radius, x, y, gs, n), gs)
+
+
+class Sphere(GeometricObject):
+    """Sphere: its x-y cross-section through the centre."""
+
+    def __init__(self, radius, **kw):
+        super().__init__(**kw)
+        self.radius = float(radius)
+
+    def coverage(self, gs):
+        x, y = to_corner(self.center, gs)
+        return polygon_coverage(circle_vertices(self.radius, x, y, gs), gs)
+
+
+class Prism(GeometricObject):
+    """Prism along z from world-coordinate vertices; a given centre moves the
+    vertices' centroid onto it (meep's rule).  Vertices map to the cell as
+    points do (clipped to it)."""
+
+    def __init__(self, vertices, height=1e20, axis=Vector3(0, 0, 1), center=None, sidewall_angle=0, **kw):
+        if sidewall_angle != 0:
+            raise NotImplementedError("Prism sidewall_angle not supported (2D raster)")
+        ax = Vector3(*axis)
+        if ax.x != 0.0 or ax.y != 0.0:
+            raise NotImplementedError("Prism axis must be along z (2D raster)")
+        vs = [Vector3(*v) for v in vertices]
+        centroid = Vector3(sum(v.x for v in vs) / len(vs), sum(v.y for v in vs) / len(vs),
+                           sum(v.z for v in vs) / len(vs))
+        if center is not None:
+            d = Vector3(*center) - centroid
+            vs = [v + d for v in vs]
+            centroid = Vector3(*center)
+        super().__init__(center=centroid, **kw)
+        self.vertices, self.height, self.axis = vs, float(height), ax
+
+    def coverage(self, gs):
+        return polygon_coverage([to_corner(v, gs) for v in self.vertices], gs)
 
 
 def _overlap_1d(a, b, n, dx):
@@ -192,12 +302,11 @@ def _overlap_1d(a, b, n, dx):
 
 def rect_coverage(x0, y0, x1, y1, gs):
     """Area fraction of every cell covered by [x0, x1] x [y0, y1] (corner
-    coordinates), clipped to [0, 1] and snapped to 0/1 within 1e-9."""
+    coordinates), clipped to [0, 1] and snapped as the reference's polygon
+    coverage snaps (geometry.raster.snap)."""
+    from ..geometry.raster import snap
     cov = np.outer(_overlap_1d(x0, x1, gs.nx, gs.dx), _overlap_1d(y0, y1, gs.ny, gs.dx))
-    cov = np.clip(cov, 0.0, 1.0)
-    cov[np.abs(cov - 1.0) <= 1e-9] = 1.0
-    cov[np.abs(cov) <= 1e-9] = 0.0
-    return cov
+    return snap(np.clip(cov, 0.0, 1.0))
 
 
 def interface_normals(cov):
@@ -219,8 +328,8 @@ def rasterize(objects, gs, background=1.0):
     te_xx = eps_tm.copy()
     te_yy = eps_tm.copy()
     for obj in objects:
-        if not isinstance(obj, Block):
-            raise NotImplementedError(f"geometry object {type(obj).__name__} is not supported (Block only)")
+        if not hasattr(obj, "coverage"):
+            raise NotImplementedError(f"geometry object {type(obj).__name__} is not supported")
         cov = obj.coverage(gs)
         val = obj.material.resolve()
         old = eps_tm
@@ -256,8 +365,8 @@ def rasterize_media(objects, gs, background):
     chi3, any_chi3 = np.zeros(shape), False
     covs = []
     for obj in objects:
-        if not isinstance(obj, Block):
-            raise NotImplementedError(f"geometry object {type(obj).__name__} is not supported (Block only)")
+        if not hasattr(obj, "coverage"):
+            raise NotImplementedError(f"geometry object {type(obj).__name__} is not supported")
         cov = obj.coverage(gs)
         m = obj.material
         covs.append((cov, m))

@@ -17,8 +17,8 @@ fields and zero DFT accumulators and integrates to the accumulated clock
 them (meep_torch/mp/simulation.py:1271-1295); each chunk's monitor sums are
 added in chunk order.  Step functions and stop conditions (run_control.py)
 split the run into engine calls at the points where a hook must look at the
-fields.  Symmetries, k-points, multi-source 2D runs and non-Block geometry
-raise NotImplementedError.
+fields.  Symmetries, k-points, multi-source 2D runs and geometry without a
+2D cross-section raise NotImplementedError.
 """
 from collections import namedtuple
 

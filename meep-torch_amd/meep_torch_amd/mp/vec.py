@@ -45,6 +45,10 @@ class Vector3:
     def norm(self):
         return math.sqrt(self.x ** 2 + self.y ** 2 + self.z ** 2)
 
+    def unit(self):
+        n = self.norm()
+        return Vector3(self.x / n, self.y / n, self.z / n)
+
     def __repr__(self):
         return f"Vector3({self.x}, {self.y}, {self.z})"
 

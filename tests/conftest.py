@@ -33,3 +33,16 @@ def has_gpu():
         return torch.cuda.is_available()
     except Exception:
         return False
+
+
+def record(case, **values):
+    """Append one JSON line of measured errors to $MEEP_PARITY_REPORT (if
+    set), so that errors a passing test computes -- e.g. against an FP64
+    run -- are kept, not only printed."""
+    import json
+    print(f"[{case}] " + ", ".join(f"{k} {v:.3e}" if isinstance(v, float) else f"{k} {v}"
+                                   for k, v in values.items()))
+    path = os.environ.get("MEEP_PARITY_REPORT")
+    if path:
+        with open(path, "a") as fh:
+            fh.write(json.dumps({"case": case, **values}) + "\n")

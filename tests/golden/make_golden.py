@@ -27,6 +27,25 @@ Cases (all CPU, torch threads = 8):
   grad_3d    run_3d_adjoint eps gradient (30x24x20, 90 steps, chunk 16), f64
   filters    conic/gaussian filter, tanh projection, DesignRegion.paint, f64
   golden_plain  mp.Simulation GOLDEN_PLAIN cell (100x60 TM guide), f32 + f64
+  media_tm   2D TM 48x40, npml 6, 240 steps, f32 + f64: (all) Lorentz + Drude
+             poles with masks, chi2 and chi3 regions, sigma_d, sigma_b and mu,
+             line source; (nl) chi2 + chi3 only under a strong drive
+  media_te   2D TE analogue: shared and per-axis poles, chi2 / chi3, sigma_d,
+             sigma_b, mu; (nl) chi2 + chi3 only
+  media_3d   3D 22x20x18, npml 4, 120 steps, sigma_d, sigma_b and a Lorentz +
+             Drude pole pair with masks, point source, flux plane, f32 + f64
+  tfsf_tm    ModeSourceTM (TFSF eigenmode source, h_masks) on a 2D TM guide:
+             masks, the electric / magnetic amplitude sequences, fields and
+             flux after 600 steps, f64
+  sim3d      mp.Simulation in 3D (dimensions from a 3-vector cell): a point
+             and a plane (finite aperture) GaussianSource, flux plane, f64
+  raster     polygon coverage (triangle, rotated square, concave L, a polygon
+             partly outside the grid, a clockwise pentagon, a 48-gon) on a
+             40x36 grid, a three-polygon painter's scene (TM eps, Kottke TE
+             pair, chi3), and the layout plugin's eps map of a two-polygon
+             component
+  shapes     mp.Simulation 2D TM cell with a Cylinder, Sphere, Prism, rotated
+             Block and Ellipsoid: eps map, Ez and a flux after t = 12, f64
 """
 import hashlib
 import json
@@ -393,17 +412,266 @@ def case_golden_plain():
     save("golden_plain", **out)
 
 
+# --------------------------------------------------------------- media pins
+def _media_maps(nx, ny):
+    x = torch.arange(nx, dtype=torch.float64)[:, None]
+    y = torch.arange(ny, dtype=torch.float64)[None, :]
+    eps = 1.0 + 2.0 * torch.exp(-((x - 30.0) ** 2 + (y - 18.0) ** 2) / 30.0)
+    r = torch.tensor(splitmix_uniform(nx * ny, 7).reshape(nx, ny))
+    inner = torch.zeros(nx, ny, dtype=torch.float64)
+    inner[8:nx - 8, 8:ny - 8] = 1.0
+    left = inner.clone()
+    left[nx // 2:] = 0.0
+    right = inner - left
+    return {"eps": eps, "m1": (r > 0.5).double() * left, "m2": (r <= 0.5).double() * left,
+            "chi2": 0.3 * right * (y > ny / 2).double(), "chi3": 0.05 * right * (y <= ny / 2).double(),
+            "sigma_d": 2.0 * r * right, "sigma_b": 1.0 * (1.0 - r) * left, "mu": 1.0 + 0.5 * r * inner}
+
+
+POLES2 = [(1.1, 0.15, 1.3), (0.0, 0.4, 2.0, 0.9)]
+
+
+def case_media_tm():
+    from meep_torch.core.coeffs import build_tm_coeffs
+    from meep_torch.core.grid import Grid
+    from meep_torch.core.monitors import FluxMonitorTM
+    from meep_torch.core.run import run_tm
+    from meep_torch.core.sources import GaussianPulse, LineSource
+    from meep_torch.core.stepper_tm import init_tm_state
+    nx, ny, n = 48, 40, 240
+    mp_ = _media_maps(nx, ny)
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        g = Grid(nx=nx, ny=ny, dx=0.05, courant=0.5, dtype=dtype, device="cpu")
+        cv = {k: v.to(dtype) for k, v in mp_.items()}
+        out = {k: v for k, v in cv.items()}
+        freqs = [0.6, 0.8, 1.0]
+        for case, kw, drive in (
+                ("all", dict(poles=POLES2, pole_masks=[cv["m1"], cv["m2"]], chi2=cv["chi2"], chi3=cv["chi3"],
+                             sigma_d=cv["sigma_d"], sigma_b=cv["sigma_b"], mu=cv["mu"]), 20.0),
+                ("nl", dict(chi2=cv["chi2"], chi3=cv["chi3"]), 60.0)):
+            m = build_tm_coeffs(g, eps_inf=cv["eps"], npml=6, **kw)
+            src = LineSource(g, ix=10, j0=8, j1=ny - 8, profile=GaussianPulse(0.8, 0.6, cutoff=3.0))
+            src.mask = src.mask * drive
+            mon = FluxMonitorTM(g, freqs, x_normal=True, idx=38, lo=7, hi=ny - 7)
+            st, (acc,) = run_tm(init_tm_state(g, n_poles=len(kw.get("poles", ()))), m, n, source=src,
+                                monitors=[mon])
+            out[f"{case}_mask"] = src.mask
+            out[f"{case}_amp"] = np.array([src.amplitude(t) for t in range(n)])
+            for f in st._fields:
+                v = getattr(st, f)
+                if isinstance(v, torch.Tensor) and v.numel():
+                    out[f"{case}_{f}"] = v
+            out[f"{case}_acc"] = acc
+        save(f"media_tm_{tag}", **out)
+
+
+def case_media_te():
+    from meep_torch.core.coeffs import build_te_coeffs
+    from meep_torch.core.grid import Grid
+    from meep_torch.core.monitors import FluxMonitorTE
+    from meep_torch.core.run import run_te
+    from meep_torch.core.sources import GaussianPulse, LineSource
+    from meep_torch.core.stepper_te import init_te_state
+    nx, ny, n = 48, 40, 240
+    mp_ = _media_maps(nx, ny)
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        g = Grid(nx=nx, ny=ny, dx=0.05, courant=0.5, dtype=dtype, device="cpu")
+        cv = {k: v.to(dtype) for k, v in mp_.items()}
+        out = {k: v for k, v in cv.items()}
+        freqs = [0.6, 0.8, 1.0]
+        for case, kw, drive in (
+                ("all", dict(poles=POLES2[:1], pole_masks=[cv["m1"]], poles_x=POLES2[1:], pole_masks_x=[cv["m2"]],
+                             poles_y=[(1.6, 0.2, 0.7)], pole_masks_y=[cv["m2"]], chi2=cv["chi2"], chi3=cv["chi3"],
+                             sigma_d=cv["sigma_d"], sigma_b=cv["sigma_b"], mu=cv["mu"]), 20.0),
+                ("nl", dict(chi2=cv["chi2"], chi3=cv["chi3"]), 60.0)):
+            m = build_te_coeffs(g, eps_inf=cv["eps"], npml=6, **kw)
+            src = LineSource(g, ix=10, j0=8, j1=ny - 8, profile=GaussianPulse(0.8, 0.6, cutoff=3.0))
+            src.mask = (src.mask[:, :-1] * drive).contiguous()
+            mon = FluxMonitorTE(g, freqs, ix=38, lo=7, hi=ny - 7)
+            st0 = init_te_state(g, n_poles=int(m.c1x.shape[0]), n_poles_y=int(m.c1y.shape[0]))
+            st, (acc,) = run_te(st0, m, n, source=src, monitors=[mon])
+            out[f"{case}_mask"] = src.mask
+            out[f"{case}_amp"] = np.array([src.amplitude(t) for t in range(n)])
+            for f in st._fields:
+                v = getattr(st, f)
+                if isinstance(v, torch.Tensor) and v.numel():
+                    out[f"{case}_{f}"] = v
+            out[f"{case}_acc"] = acc
+        save(f"media_te_{tag}", **out)
+
+
+def case_media_3d():
+    from meep_torch.core.monitors_3d import FluxMonitor3D, PointSource3D, run_3d
+    from meep_torch.core.sources import GaussianPulse
+    from meep_torch.core.stepper_3d import build_3d_coeffs, init_3d_state
+    nx, ny, nz, npml, n = 22, 20, 18, 4, 120
+    r = torch.tensor(splitmix_uniform(nx * ny * nz, 9).reshape(nx, ny, nz))
+    inner = torch.zeros(nx, ny, nz, dtype=torch.float64)
+    inner[npml + 2:nx - npml - 2, npml + 2:ny - npml - 2, npml + 2:nz - npml - 2] = 1.0
+    eps = 1.0 + 2.0 * r * inner
+    half = (r > 0.5).double() * inner
+    maps = {"eps": eps, "sigma_d": 3.0 * r * inner, "sigma_b": 1.5 * (1.0 - r) * inner,
+            "m1": half, "m2": inner - half}
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        cv = {k: v.to(dtype) for k, v in maps.items()}
+        c = build_3d_coeffs(nx, ny, nz, 0.05, eps=cv["eps"], npml=npml, sigma_d=cv["sigma_d"],
+                            sigma_b=cv["sigma_b"], poles=POLES2, pole_masks=[cv["m1"], cv["m2"]],
+                            dtype=dtype, device="cpu")
+        src = PointSource3D(c.dt, "Ez", 7, ny // 2, nz // 2, GaussianPulse(1.0, 2.0, cutoff=2.0))
+        mon = FluxMonitor3D(c.dt, [0.8, 1.0, 1.2], normal="x", idx=15, ap=npml + 1, dtype=dtype)
+        st, (acc,) = run_3d(init_3d_state(nx, ny, nz, npml=npml, n_poles=2, dtype=dtype, device="cpu"), c, n,
+                            source=src, monitors=[mon])
+        out = dict(cv)
+        out["amp"] = np.array([src.amplitude(t) for t in range(n)])
+        for f in st._fields:
+            v = getattr(st, f)
+            if isinstance(v, torch.Tensor) and v.numel():
+                out[f] = v
+        out["acc"] = acc
+        out["flux"] = mon.flux(acc, 0.05)
+        save(f"media_3d_{tag}", **out)
+
+
+def case_tfsf_tm():
+    from meep_torch.core.coeffs import build_tm_coeffs
+    from meep_torch.core.grid import Grid
+    from meep_torch.core.monitors import FluxMonitorTM
+    from meep_torch.core.run import run_tm
+    from meep_torch.core.sources import GaussianPulse
+    from meep_torch.core.stepper_tm import init_tm_state
+    from meep_torch.ports.mode_source import ModeSourceTM
+    from meep_torch.ports.port import Port
+    nx, ny, n = 160, 64, 600
+    dtype = torch.float64
+    g = Grid(nx=nx, ny=ny, dx=0.05, courant=0.5, dtype=dtype, device="cpu")
+    eps = torch.ones(nx, ny, dtype=dtype)
+    eps[:, 28:36] = 12.0
+    m = build_tm_coeffs(g, eps_inf=eps, npml=12)
+    port = Port(axis="x", index=40, lo=14, hi=ny - 14, direction=+1)
+    src = ModeSourceTM(g, m, port, GaussianPulse(0.6, 0.3), 0.6)
+    mons = [FluxMonitorTM(g, [0.5, 0.6, 0.7], x_normal=True, idx=120, lo=14, hi=ny - 14),
+            FluxMonitorTM(g, [0.5, 0.6, 0.7], x_normal=True, idx=20, lo=14, hi=ny - 14)]
+    st, acc = run_tm(init_tm_state(g), m, n, source=src, monitors=mons)
+    out = {"eps": eps, "mask": src.mask, "neff": np.array(src.neff),
+           "amp": np.array([src.amplitude(t) for t in range(n)]),
+           "hamp": np.array([src.h_amplitude(t) for t in range(n)])}
+    for k, v in src.h_masks.items():
+        out[k] = v
+    for f in ("Ez", "Hx", "Hy", "Dz"):
+        out[f] = getattr(st, f)
+    for i, a in enumerate(acc):
+        out[f"acc{i}"] = a
+    save("tfsf_tm_f64", **out)
+
+
+def case_sim3d():
+    import meep_torch.mp as mp
+    out = {}
+    for kind in ("point", "plane"):
+        cell = mp.Vector3(1.6, 1.4, 1.2)
+        geom = [mp.Block(center=mp.Vector3(0.1, 0, -0.1), size=mp.Vector3(mp.inf, 0.3, 0.25),
+                         material=mp.Medium(epsilon=6.0))]
+        if kind == "point":
+            srcs = [mp.Source(mp.GaussianSource(1.0, fwidth=1.0), component=mp.Ez,
+                              center=mp.Vector3(-0.4, 0.05, 0.1))]
+        else:
+            srcs = [mp.Source(mp.GaussianSource(1.0, fwidth=1.0), component=mp.Ey,
+                              center=mp.Vector3(-0.4, 0, 0), size=mp.Vector3(0, 0.6, 0.5))]
+        sim = mp.Simulation(cell_size=cell, resolution=20, geometry=geom, sources=srcs,
+                            boundary_layers=[mp.PML(0.2)], dimensions=3)
+        fl = sim.add_flux(1.0, 0.4, 3, mp.FluxRegion(center=mp.Vector3(0.4, 0, 0), size=mp.Vector3(0, 1.4, 1.2)))
+        sim.run(until=6.0)
+        out[f"{kind}_flux"] = np.array(mp.get_fluxes(fl))
+        for comp, name in ((mp.Ex, "Ex"), (mp.Ey, "Ey"), (mp.Ez, "Ez")):
+            out[f"{kind}_{name}"] = np.asarray(sim.get_array(component=comp))
+    save("sim3d_f64", **out)
+
+
+# ------------------------------------------------------------- polygon raster
+RASTER_POLYS = {
+    "triangle": [[0.11, 0.23], [1.37, 0.41], [0.52, 1.63]],
+    "rot_square": [[0.8, 0.05], [1.55, 0.8], [0.8, 1.55], [0.05, 0.8]],
+    "concave_l": [[0.2, 0.2], [1.4, 0.2], [1.4, 0.6], [0.6, 0.6], [0.6, 1.5], [0.2, 1.5]],
+    "outside": [[-0.3, 0.9], [1.1, -0.25], [2.4, 1.3], [0.7, 2.2]],
+    "cw_pentagon": [[0.9, 1.6], [1.5, 1.1], [1.3, 0.3], [0.5, 0.3], [0.3, 1.1]],
+}
+
+
+def case_raster():
+    from meep_torch.core.grid import Grid
+    from meep_torch.geometry.polygons import Polygon
+    from meep_torch.geometry.raster import coverage
+    from meep_torch.geometry.scene import rasterize_scene
+    g = Grid(nx=40, ny=36, dx=0.05, device="cpu")
+    out = {}
+    for name, v in RASTER_POLYS.items():
+        out[f"cov_{name}"] = coverage(Polygon(np.array(v)), g)
+    ring = np.stack([0.95 + 0.55 * np.cos(np.linspace(0, 2 * np.pi, 48, endpoint=False)),
+                     0.9 + 0.55 * np.sin(np.linspace(0, 2 * np.pi, 48, endpoint=False))], axis=1)
+    out["ngon"] = ring
+    out["cov_ngon"] = coverage(Polygon(ring), g)
+    items = [(Polygon(np.array(RASTER_POLYS["concave_l"])), 12.0), (Polygon(ring), 4.0),
+             (Polygon(np.array(RASTER_POLYS["triangle"])), 2.25)]
+    rm = rasterize_scene(items, g, background=1.44, chi3_items=[(Polygon(ring), 0.02)])
+    out.update(scene_eps_tm=rm.eps_tm, scene_te_xx=rm.eps_te_xx, scene_te_yy=rm.eps_te_yy, scene_chi3=rm.chi3)
+    # the layout plugin's steps (plugins/gdsfactory.py:19-42) on a duck-typed
+    # component: a straight guide and a bent taper on layer 1, padded 0.4 um
+    polys = [np.array([[0.0, 0.0], [2.0, 0.0], [2.0, 0.5], [0.0, 0.5]]),
+             np.array([[2.0, 0.0], [3.1, 0.4], [3.3, 1.2], [2.9, 1.25], [2.6, 0.7], [2.0, 0.5]])]
+    res, pad, n_core, n_clad = 20, 0.4, 3.48, 1.44
+    xs = np.concatenate([p[:, 0] for p in polys])
+    ys = np.concatenate([p[:, 1] for p in polys])
+    x0, y0 = xs.min() - pad, ys.min() - pad
+    sx, sy = (xs.max() + pad) - x0, (ys.max() + pad) - y0
+    pg = Grid(nx=int(round(sx * res)), ny=int(round(sy * res)), dx=1.0 / res, device="cpu")
+    rm = rasterize_scene([(Polygon(p - [x0, y0]), n_core ** 2) for p in polys], pg, background=n_clad ** 2)
+    out.update(plugin_eps=rm.eps_tm, plugin_nx=np.array(pg.nx), plugin_ny=np.array(pg.ny))
+    save("raster", **out)
+
+
+# ------------------------------------------------------- facade 2D shapes
+def case_shapes():
+    import meep_torch.mp as mp
+    out = {}
+    geom = [mp.Cylinder(radius=0.7, center=mp.Vector3(-1.5, 0.3), material=mp.Medium(epsilon=6.0)),
+            mp.Sphere(radius=0.5, center=mp.Vector3(1.2, -0.8), material=mp.Medium(epsilon=3.0)),
+            mp.Prism(vertices=[mp.Vector3(0, 0.5), mp.Vector3(1.5, 1.2), mp.Vector3(0.8, 1.8)],
+                     material=mp.Medium(epsilon=9.0)),
+            mp.Block(size=mp.Vector3(1.2, 0.4), center=mp.Vector3(-0.2, -1.0), e1=mp.Vector3(1, 1),
+                     e2=mp.Vector3(-1, 1), material=mp.Medium(epsilon=4.0)),
+            mp.Ellipsoid(size=mp.Vector3(1.6, 0.8), center=mp.Vector3(2.0, 1.0), material=mp.Medium(epsilon=2.5))]
+    sim = mp.Simulation(cell_size=mp.Vector3(6, 4), resolution=10, geometry=geom,
+                        sources=[mp.Source(mp.GaussianSource(0.8, fwidth=0.6), component=mp.Ez,
+                                           center=mp.Vector3(-2.4, 0.1))],
+                        boundary_layers=[mp.PML(0.5)])
+    fl = sim.add_flux(0.8, 0.4, 3, mp.FluxRegion(center=mp.Vector3(2.6, 0), size=mp.Vector3(0, 3)))
+    sim.run(until=12)
+    out["eps"] = np.asarray(sim.get_array(component=mp.Dielectric))
+    out["Ez"] = np.asarray(sim.get_array(component=mp.Ez))
+    out["flux"] = np.array(mp.get_fluxes(fl))
+    save("shapes_f64", **out)
+
+
 def main():
     which = sys.argv[1:] or ["pml", "tm", "te", "c1", "3d", "grad_tm", "grad_3d",
-                             "filters", "golden_plain"]
+                             "filters", "golden_plain", "media_tm", "media_te", "media_3d",
+                             "tfsf_tm", "sim3d", "raster", "shapes"]
     t_all = time.perf_counter()
     timing = {}
+    try:
+        with open(os.path.join(HERE, "MANIFEST.json")) as fh:
+            timing = json.load(fh).get("timing", {})
+    except (OSError, ValueError):
+        pass
     for w in which:
         t0 = time.perf_counter()
         {"pml": case_pml, "tm": case_tm_small, "te": case_te_small,
          "c1": lambda: timing.update(c1=case_c1()), "3d": case_3d_small,
          "grad_tm": case_grad_tm, "grad_3d": case_grad_3d,
-         "filters": case_filters, "golden_plain": case_golden_plain}[w]()
+         "filters": case_filters, "golden_plain": case_golden_plain,
+         "media_tm": case_media_tm, "media_te": case_media_te, "media_3d": case_media_3d,
+         "tfsf_tm": case_tfsf_tm, "sim3d": case_sim3d, "raster": case_raster,
+         "shapes": case_shapes}[w]()
         print(f"[golden] {w}: {time.perf_counter() - t0:.1f}s", flush=True)
     man = {}
     for fn in sorted(os.listdir(HERE)):

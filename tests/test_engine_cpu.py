@@ -152,3 +152,26 @@ def test_grid_validation():
     with pytest.raises(ValueError):
         Grid(nx=8, ny=8, dx=0.1, courant=0.8)
     assert Grid(nx=8, ny=8, dx=0.1, courant=0.5, device="cpu").dt == 0.05
+
+
+@pytest.mark.parametrize("what", ["sigma_d", "sigma_b", "pole_masks"])
+def test_3d_adjoint_refuses_material_gradients(what):
+    """run_3d_adjoint differentiates with respect to eps only: a sigma_d,
+    sigma_b or pole-mask tensor that requires grad raises by name (as the 2D
+    adjoint does for non-eps coefficients) before any device work."""
+    from meep_torch_amd.core.adjoint3d import run_3d_adjoint
+    from meep_torch_amd.core.stepper_3d import build_3d_coeffs, init_3d_state
+    n = 12
+    m = torch.zeros(n, n, n, dtype=torch.float64)
+    m[4:8, 4:8, 4:8] = 1.0
+    kw = {}
+    if what == "pole_masks":
+        kw = dict(poles=[(1.0, 0.1, 1.0)], pole_masks=[m.clone().requires_grad_(True)])
+    else:
+        kw = {what: m.clone().requires_grad_(True)}
+    c = build_3d_coeffs(n, n, n, 0.05, eps=torch.ones(n, n, n, dtype=torch.float64), npml=2,
+                        dtype=torch.float64, device="cpu", **kw)
+    st = init_3d_state(n, n, n, npml=2, n_poles=1 if what == "pole_masks" else 0, dtype=torch.float64,
+                       device="cpu")
+    with pytest.raises(NotImplementedError, match="sigma_d|sigma_b|pole"):
+        run_3d_adjoint(c, 4, 2, None, [], lambda accs: 0.0, st)

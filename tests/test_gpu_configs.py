@@ -1,0 +1,167 @@
+"""The configurations' own workloads on the HIP engine against the CPU oracle
+(the reference algorithm restated, tests/test_oracle_golden.py pins it to the
+reference's golden vectors), with the kernels those workloads run:
+
+C2  2D TE 2048 x 2048, seeded random-density eps (Gaussian-smoothed) on the
+    centred 512 x 512, 32-cell CPML, Jy line source, four 100-frequency flux
+    lines: the full grid for 3000 of its 10000 steps on the FP32 tile variant
+    the benchmark runs (32 x 64 tiles, 4 steps per launch), whose
+    interior-tile kernel carries most of the grid.
+C3  3D slab waveguide on substrate (eps 12.1 on 2.1, 22 cells thick in z,
+    50 wide in y) at 96^3 with C3's 12-cell CPML, Ez dipole and two x-normal
+    64-frequency flux planes, large enough that the DFT flush takes its
+    grouped (4 frequencies per thread) path.
+
+Bars: fields relative L2 <= 1e-5, spectra <= 1e-4 against the FP32 oracle.
+The errors of the FP32 engine and of the FP32 oracle against the FP64 oracle
+are recorded (conftest.record)."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import has_gpu, record
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a HIP device")]
+
+
+def rel(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    n = float(b.norm())
+    return float((a - b).norm()) / (n if n > 0 else 1.0)
+
+
+C2_STEPS = 3000
+
+
+def _c2_oracle(dtype, steps):
+    from meep_torch_amd import synth
+    from oracle import yee2d
+    import os
+    torch.set_num_threads(min(16, os.cpu_count() or 1))
+    P = synth.C2
+    n = P["n"]
+    eps = synth.c2_eps(device="cpu", dtype=torch.float64)
+    c = yee2d.te_coeffs(n, n, P["dx"], eps, P["npml"], dtype=dtype)
+    mask = yee2d.line_mask(n, n, P["src_ix"], P["src_lo"], P["src_hi"], P["dx"], dtype=dtype)[:, :-1].contiguous()
+    amp = yee2d.gaussian_pulse(P["fcen"], P["fwidth"])
+    fr = synth.c2_freqs()
+    mons = [{"type": "flux", "kind": "te_x", "idx": i, "lo": P["line_lo"], "hi": P["line_hi"], "freqs": fr}
+            for i in P["lines_x"]]
+    mons += [{"type": "flux", "kind": "te_y", "idx": j, "lo": P["line_lo"], "hi": P["line_hi"], "freqs": fr}
+             for j in P["lines_y"]]
+    amps = [amp((t + 0.5) * c.dt) for t in range(steps)]
+    return yee2d.run2d("te", yee2d.te_state(n, n, dtype), c, steps, amps=amps, mask=mask, monitors=mons)
+
+
+@pytest.mark.timeout(600)
+def test_c2_workload_matches_oracle(monkeypatch):
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    from meep_torch_amd.core import engine2d
+    from meep_torch_amd.core.coeffs import build_te_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd import synth
+    monkeypatch.setenv("MEEP_TORCH_AMD_VARIANT", "5")
+    P = synth.C2
+    n = P["n"]
+    g = Grid(nx=n, ny=n, dx=P["dx"], dtype=torch.float32, device="cpu")
+    n_in, n_ring = engine2d.tile_counts("te", n, n, build_te_coeffs(g, eps_inf=2.1, npml=P["npml"]),
+                                        torch.float32, 5)
+    assert n_in > 4 * n_ring, (n_in, n_ring)
+    run, _, _, _ = bench.build_c2(C2_STEPS)
+    st, accs = run()
+    torch.cuda.synchronize()
+    s32, a32 = _c2_oracle(torch.float32, C2_STEPS)
+    errs = {f: rel(getattr(st, f), s32[f]) for f in ("Hz", "Ex", "Ey", "Dx", "Dy", "psi_Eyx", "psi_Hzx")}
+    for i, (a, b) in enumerate(zip(accs, a32)):
+        errs[f"acc{i}"] = rel(a, b)
+    s64, a64 = _c2_oracle(torch.float64, C2_STEPS)
+    e64 = {"engine_f32_vs_f64_Hz": rel(st.Hz, s64["Hz"]), "oracle_f32_vs_f64_Hz": rel(s32["Hz"], s64["Hz"]),
+           "engine_f32_vs_f64_acc0": rel(accs[0], a64[0]), "oracle_f32_vs_f64_acc0": rel(a32[0], a64[0]),
+           "engine_f32_vs_f64_acc2": rel(accs[2], a64[2]), "oracle_f32_vs_f64_acc2": rel(a32[2], a64[2])}
+    record("C2 2048^2 TE 3000 steps, tile variant 5", **errs, **e64)
+    for f in ("Hz", "Ex", "Ey", "Dx", "Dy", "psi_Eyx", "psi_Hzx"):
+        assert errs[f] <= 1e-5, (f, errs[f])
+    for i in range(4):
+        assert errs[f"acc{i}"] <= 1e-4, (i, errs[f"acc{i}"])
+
+
+C3N, C3_STEPS = 96, 600
+
+
+def _c3_setup():
+    from meep_torch_amd import synth
+    P = synth.C3
+    fr = [P["fcen"] - 0.1 + 0.2 * i / (P["nfreq"] - 1) for i in range(P["nfreq"])]
+    return P, fr, synth.c3_eps(C3N, dtype=torch.float64), (30, C3N - 20)
+
+
+def _c3_oracle(dtype):
+    import os
+    from oracle import yee2d, yee3d
+    torch.set_num_threads(min(16, os.cpu_count() or 1))
+    P, fr, eps, planes = _c3_setup()
+    c = yee3d.coeffs3d(C3N, C3N, C3N, P["dx"], eps, P["npml"], dtype)
+    amp = yee2d.gaussian_pulse(P["fcen"], 1.0, cutoff=3.0)
+    amps = [amp((t + 0.5) * c["dt"]) for t in range(C3_STEPS)]
+    mons = [{"type": "flux", "normal": "x", "idx": i, "freqs": fr, "ap": P["npml"] + 1} for i in planes]
+    return yee3d.run3d(yee3d.state3d(C3N, C3N, C3N, dtype), c, C3_STEPS, amps,
+                       ("Ez", P["npml"] + 8, C3N // 2, C3N // 2), mons)
+
+
+def _c3_engine(variant=None):
+    from meep_torch_amd.core import engine3d
+    from meep_torch_amd.core.monitors_3d import FluxMonitor3D, PointSource3D, run_3d
+    from meep_torch_amd.core.sources import GaussianPulse
+    from meep_torch_amd.core.stepper_3d import build_3d_coeffs, init_3d_state
+    P, fr, eps, planes = _c3_setup()
+    dtype = torch.float32
+    c = build_3d_coeffs(C3N, C3N, C3N, P["dx"], eps=eps.to(dtype).cuda(), npml=P["npml"], dtype=dtype,
+                        device="cuda")
+    src = PointSource3D(c.dt, "Ez", P["npml"] + 8, C3N // 2, C3N // 2, GaussianPulse(P["fcen"], 1.0, cutoff=3.0))
+    mons = [FluxMonitor3D(c.dt, fr, normal="x", idx=i, ap=P["npml"] + 1, dtype=dtype) for i in planes]
+    old = engine3d.DEFAULT_VARIANT
+    if variant is not None:
+        engine3d.DEFAULT_VARIANT = variant
+    try:
+        st, acc = run_3d(init_3d_state(C3N, C3N, C3N, npml=P["npml"], dtype=dtype, device="cuda"), c, C3_STEPS,
+                         source=src, monitors=mons)
+    finally:
+        engine3d.DEFAULT_VARIANT = old
+    torch.cuda.synchronize()
+    return st, acc, mons
+
+
+_ORACLE = {}
+
+
+def _c3_oracle_cached(dtype):
+    if dtype not in _ORACLE:
+        _ORACLE[dtype] = _c3_oracle(dtype)
+    return _ORACLE[dtype]
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_c3_geometry_matches_oracle(variant):
+    """variant 0: two passes per step; 1: split step (interior kernel + shell)."""
+    P, fr, _, _ = _c3_setup()
+    plane = C3N - 2 * (P["npml"] + 1)
+    assert plane * plane * 8 * ((len(fr) + 3) // 4) >= 1 << 18   # the grouped flush path
+    st, acc, mons = _c3_engine(variant)
+    s32, a32 = _c3_oracle_cached(torch.float32)
+    fields = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "psi_Ez_x", "psi_Hy_z", "psi_Hx_y")
+    errs = {f: rel(getattr(st, f), s32[f]) for f in fields}
+    for i, (a, b) in enumerate(zip(acc, a32)):
+        errs[f"acc{i}"] = rel(a, b)
+        errs[f"flux{i}"] = rel(mons[i].flux(a, P["dx"]), mons[i].flux(b.cuda(), P["dx"]))
+    s64, a64 = _c3_oracle_cached(torch.float64)
+    e64 = {"engine_f32_vs_f64_Ez": rel(st.Ez, s64["Ez"]), "oracle_f32_vs_f64_Ez": rel(s32["Ez"], s64["Ez"]),
+           "engine_f32_vs_f64_acc1": rel(acc[1], a64[1]), "oracle_f32_vs_f64_acc1": rel(a32[1], a64[1])}
+    record(f"C3 geometry {C3N}^3 {C3_STEPS} steps, 3D variant {variant}", **errs, **e64)
+    for f in fields:
+        assert errs[f] <= 1e-5, (f, errs[f])
+    for i in range(2):
+        assert errs[f"acc{i}"] <= 1e-4 and errs[f"flux{i}"] <= 1e-4, (i, errs)

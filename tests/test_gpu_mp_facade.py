@@ -213,3 +213,20 @@ def test_media_blocks_equal_direct_run(dtype):
     assert np.array_equal(sim.get_array(component=mp.Ez), st.Ez.cpu().numpy())
     assert np.array_equal(np.asarray(mp.get_fluxes(fl)), mon.flux(acc, g.dx).cpu().numpy())
     assert float(st.Ez.abs().max()) > 0 and float(st.P_cur.abs().max()) > 0
+
+
+def test_shapes_run_matches_reference():
+    """The facade cell with a Cylinder, Sphere, Prism, rotated Block and
+    Ellipsoid, run to t = 12 on the HIP engine: Ez and the flux spectrum
+    equal the reference Simulation's (golden shapes_f64) within FP64
+    rounding."""
+    import meep_torch_amd.mp as mp
+    from conftest import golden
+    from test_mp_facade_cpu import _shapes_sim
+    G = golden("shapes_f64")
+    sim, fl = _shapes_sim("cuda")
+    sim.run(until=12)
+    ez = np.asarray(sim.get_array(component=mp.Ez))
+    assert np.linalg.norm(ez - G["Ez"]) <= 1e-10 * np.linalg.norm(G["Ez"])
+    fx = np.asarray(mp.get_fluxes(fl))
+    assert np.abs(fx - G["flux"]).max() <= 1e-9 * np.abs(G["flux"]).max()

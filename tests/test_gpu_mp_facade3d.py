@@ -136,3 +136,37 @@ def test_3d_facade_media_equal_direct_run(dtype):
     assert np.array_equal(sim.get_array(component=mp.Ez), st.Ez.cpu().numpy())
     assert np.array_equal(np.asarray(mp.get_fluxes(fl)), mon.flux(acc, 0.1).detach().cpu().numpy())
     assert float(st.Px_cur.abs().max()) > 0
+
+
+@pytest.mark.parametrize("kind", ["point", "plane"])
+def test_3d_facade_matches_reference_simulation(kind):
+    """The facade's 3D source mapping against the reference's own 3D
+    Simulation (golden sim3d_f64, written by the reference's mp.Simulation,
+    mp/simulation.py:630-766): a point Ez dipole, and a finite-aperture Ey
+    plane source (whose sign and aperture differ from the point form,
+    stepper_3d.py:452-481), each with a voxel Block and a flux plane; E
+    fields and fluxes within FP64 rounding."""
+    import meep_torch_amd.mp as mp
+    from conftest import golden
+    G = golden("sim3d_f64")
+    cell = mp.Vector3(1.6, 1.4, 1.2)
+    geom = [mp.Block(center=mp.Vector3(0.1, 0, -0.1), size=mp.Vector3(mp.inf, 0.3, 0.25),
+                     material=mp.Medium(epsilon=6.0))]
+    if kind == "point":
+        srcs = [mp.Source(mp.GaussianSource(1.0, fwidth=1.0), component=mp.Ez, center=mp.Vector3(-0.4, 0.05, 0.1))]
+    else:
+        srcs = [mp.Source(mp.GaussianSource(1.0, fwidth=1.0), component=mp.Ey, center=mp.Vector3(-0.4, 0, 0),
+                          size=mp.Vector3(0, 0.6, 0.5))]
+    sim = mp.Simulation(cell_size=cell, resolution=20, geometry=geom, sources=srcs,
+                        boundary_layers=[mp.PML(0.2)], dimensions=3, dtype=torch.float64, device="cuda")
+    fl = sim.add_flux(1.0, 0.4, 3, mp.FluxRegion(center=mp.Vector3(0.4, 0, 0), size=mp.Vector3(0, 1.4, 1.2)))
+    sim.run(until=6.0)
+    for name, comp in (("Ex", mp.Ex), ("Ey", mp.Ey), ("Ez", mp.Ez)):
+        ref = G[f"{kind}_{name}"]
+        got = np.asarray(sim.get_array(component=comp))
+        assert got.shape == ref.shape, (name, got.shape, ref.shape)
+        assert np.linalg.norm(got - ref) <= 1e-12 * max(np.linalg.norm(ref), 1e-300), name
+    assert float(np.abs(G[f"{kind}_Ey" if kind == "plane" else f"{kind}_Ez"]).max()) > 0
+    ref_flux = G[f"{kind}_flux"]
+    got_flux = np.asarray(mp.get_fluxes(fl))
+    assert np.abs(got_flux - ref_flux).max() <= 1e-10 * np.abs(ref_flux).max(), (got_flux, ref_flux)

@@ -201,11 +201,27 @@ def test_tfsf_eps_gradient_matches_oracle_autograd(mode):
     assert e <= 1e-9
 
 
-def test_batched_path_refuses_magnetic_sources():
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_batched_path_takes_magnetic_sources(dtype):
+    """TFSF magnetic currents in the batched launch (per-step h-coefficient
+    tables, the sheets injected by the tiled kernel): two variants, one with
+    the Huygens pair and one without, equal run_tm of each alone bit for
+    bit."""
     from meep_torch_amd.batch import BatchedTM
     from meep_torch_amd.core.coeffs import build_tm_coeffs
     from meep_torch_amd.core.grid import Grid
-    g = Grid(nx=NX, ny=NY, dx=DX, dtype=torch.float32, device="cuda")
-    m = build_tm_coeffs(g, eps_inf=torch.ones(NX, NY, device="cuda"), npml=NPML)
-    with pytest.raises(NotImplementedError):
-        BatchedTM([(m, Huygens("tm", g.dt, torch.float32, "cuda"), [])], 10)
+    from meep_torch_amd.core.monitors import FluxMonitorTM
+    from meep_torch_amd.core.run import run_tm
+    from meep_torch_amd.core.stepper_tm import init_tm_state
+    g = Grid(nx=NX, ny=NY, dx=DX, dtype=dtype, device="cuda")
+    m = build_tm_coeffs(g, eps_inf=torch.ones(NX, NY, dtype=dtype, device="cuda"), npml=NPML)
+    srcs = [Huygens("tm", g.dt, dtype, "cuda"), Huygens("tm", g.dt, dtype, "cuda")]
+    srcs[1].h_masks = {}
+    mk = lambda: [FluxMonitorTM(g, [0.8, 1.0, 1.2], x_normal=True, idx=150, lo=NPML, hi=NY - NPML)]
+    n = 400
+    bt = BatchedTM([(m, s, mk()) for s in srcs], n, checkpoint_every=0, adjoint=False)
+    out = bt.forward([m.eps_inf, m.eps_inf])
+    for s, o in zip(srcs, out):
+        _, (ref,) = run_tm(init_tm_state(g), m, n, source=s, monitors=mk())
+        assert torch.equal(o[0], ref), float((o[0] - ref).abs().max())
+    assert not torch.equal(out[0][0], out[1][0])

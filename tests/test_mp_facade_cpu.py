@@ -79,8 +79,9 @@ def test_unsupported_options_raise():
         mp.Simulation(cell_size=mp.Vector3(4, 4, 4), resolution=10, dimensions=3, Courant=0.6, device="cpu")
     with pytest.raises(NotImplementedError):
         mp.Medium(epsilon=2, H_susceptibilities=[object()])
-    with pytest.raises(NotImplementedError):
-        mp.Block(size=mp.Vector3(1, 1), e1=mp.Vector3(1, 1))
+    with pytest.raises(NotImplementedError):   # a cylinder whose axis leaves z has no 2D cross-section here
+        mp.Simulation(cell_size=mp.Vector3(4, 4), resolution=10, device="cpu",
+                      geometry=[mp.Cylinder(radius=0.5, axis=mp.Vector3(1, 0, 0))]).get_array(component=mp.Dielectric)
     sim3 = mp.Simulation(cell_size=mp.Vector3(1, 1, 1), resolution=10, dimensions=3, device="cpu")
     with pytest.raises(NotImplementedError):
         sim3.run(lambda s: None, until=1)        # step functions drive 2D runs only
@@ -248,3 +249,34 @@ def test_eigenmode_source_port_mapping_and_refusals():
                           boundary_layers=[mp.PML(1.0)])
     m = plain.add_mode_monitor(0.15, 0.04, 3, mp.FluxRegion(center=mp.Vector3(6, 0), size=mp.Vector3(0, 4)))
     assert plain._monitors[m.name][0] == "flux"
+
+
+def _shapes_sim(device):
+    import meep_torch_amd.mp as mp
+    geom = [mp.Cylinder(radius=0.7, center=mp.Vector3(-1.5, 0.3), material=mp.Medium(epsilon=6.0)),
+            mp.Sphere(radius=0.5, center=mp.Vector3(1.2, -0.8), material=mp.Medium(epsilon=3.0)),
+            mp.Prism(vertices=[mp.Vector3(0, 0.5), mp.Vector3(1.5, 1.2), mp.Vector3(0.8, 1.8)],
+                     material=mp.Medium(epsilon=9.0)),
+            mp.Block(size=mp.Vector3(1.2, 0.4), center=mp.Vector3(-0.2, -1.0), e1=mp.Vector3(1, 1),
+                     e2=mp.Vector3(-1, 1), material=mp.Medium(epsilon=4.0)),
+            mp.Ellipsoid(size=mp.Vector3(1.6, 0.8), center=mp.Vector3(2.0, 1.0), material=mp.Medium(epsilon=2.5))]
+    sim = mp.Simulation(cell_size=mp.Vector3(6, 4), resolution=10, geometry=geom,
+                        sources=[mp.Source(mp.GaussianSource(0.8, fwidth=0.6), component=mp.Ez,
+                                           center=mp.Vector3(-2.4, 0.1))],
+                        boundary_layers=[mp.PML(0.5)], dtype=torch.float64, device=device)
+    fl = sim.add_flux(0.8, 0.4, 3, mp.FluxRegion(center=mp.Vector3(2.6, 0), size=mp.Vector3(0, 3)))
+    return sim, fl
+
+
+def test_shapes_eps_matches_reference():
+    """Cylinder, Sphere, Prism, rotated Block and Ellipsoid (mp/geometry.py:
+    95-240 of the reference) rasterised by the facade: the eps map equals
+    the reference Simulation's (golden shapes_f64) to 1e-12."""
+    import meep_torch_amd.mp as mp
+    from conftest import golden
+    sim, _ = _shapes_sim("cpu")
+    eps = np.asarray(sim.get_array(component=mp.Dielectric))
+    ref = golden("shapes_f64")["eps"]
+    assert eps.shape == ref.shape
+    assert np.abs(eps - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert len(np.unique(np.round(ref, 6))) > 6     # every shape leaves its own values
