@@ -408,6 +408,17 @@ int mt_tm2d_batch_run(mt_batch* b, int step0, int k, void* stream);
 int mt_tm2d_batch_adjoint(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes, const mt_monitor* mons,
                           const int* mon_off, int step0, int k, void* stream);
 
+/* The same adjoint in two parts, for callers that replay it from a graph:
+ * _prepare uploads the descriptors (adj, tapes, monitor cotangents) into the
+ * plan's workspace and waits for the upload; _sweep then runs the reverse
+ * steps [step0, step0+k) with launches only (capturable).  The buffers the
+ * descriptors point to must stay in place between prepare and every sweep.
+ * Same math as mt_tm2d_batch_adjoint (core/run.py:99-120 checkpointed
+ * backward, batched). */
+int mt_tm2d_batch_adjoint_prepare(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes, const mt_monitor* mons,
+                                  const int* mon_off, void* stream);
+int mt_tm2d_batch_adjoint_sweep(mt_batch* b, int step0, int k, void* stream);
+
 void mt_batch_destroy(mt_batch* b);
 
 /* Kernel timing hook.  No counterpart in the reference: a measurement aid

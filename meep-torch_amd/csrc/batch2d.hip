@@ -58,6 +58,10 @@ struct mt_batch {
   std::vector<std::vector<mt_monitor>> mons;
   std::vector<void*> stage;
   std::vector<long long> stage_bytes;
+  // adjoint descriptors uploaded by mt_tm2d_batch_adjoint_prepare
+  bool adj_ready = false;
+  int adj_max_m = 0;
+  std::vector<int> adj_inj_blocks;
 };
 
 namespace mt {
@@ -221,9 +225,12 @@ int run_t(mt_batch* b, int step0, int k, hipStream_t s) {
   return MT_OK;
 }
 
+// Upload the adjoint descriptors of every variant (cotangent fields, tapes,
+// monitor injection terms) into the plan's device workspace; the sweeps that
+// follow only pass scalars, so they can be captured in a graph.
 template <typename T>
-int adjoint_t(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes, const mt_monitor* mons, const int* mon_off,
-              int step0, int k, hipStream_t s) {
+int adjoint_prepare_t(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes, const mt_monitor* mons,
+                      const int* mon_off, hipStream_t s) {
   const int nvar = b->nvar;
   std::vector<AdjK<T>> ks(nvar);
   for (int v = 0; v < nvar; ++v) ks[v] = make_adjk<T>(&b->probs[v], &adj[v]);
@@ -251,6 +258,21 @@ int adjoint_t(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes, const mt_
       hipMemcpyAsync(ws + b->L.inj_pre, ipre.data(), ipre.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
       hipStreamSynchronize(s))
     return fail(MT_EHIP, "batched adjoint: descriptor upload failed");
+  b->adj_max_m = max_m;
+  b->adj_inj_blocks.assign(max_m, 0);
+  for (int m = 0; m < max_m; ++m) b->adj_inj_blocks[m] = ipre[(size_t)m * (nvar + 1) + nvar];
+  b->adj_ready = true;
+  return MT_OK;
+}
+
+// Reverse-time sweep over steps [step0, step0 + k) with the prepared
+// descriptors: per step, the monitor injection then the three transpose
+// passes over the cells of every variant.  Launches only.
+template <typename T>
+int adjoint_sweep_t(mt_batch* b, int step0, int k, hipStream_t s) {
+  if (!b->adj_ready) return fail(MT_EINVAL, "batched adjoint: descriptors not prepared");
+  const int nvar = b->nvar;
+  char* ws = b->ws;
   const AdjK<T>* dks = reinterpret_cast<const AdjK<T>*>(ws + b->L.adjk);
   void* const* dtp = reinterpret_cast<void* const*>(ws + b->L.tapes);
   const int* dblk = reinterpret_cast<const int*>(ws + b->L.blk_pre);
@@ -259,8 +281,8 @@ int adjoint_t(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes, const mt_
   const double* ddts = reinterpret_cast<const double*>(ws + b->L.dts);
   for (int s_ = k - 1; s_ >= 0; --s_) {
     int t = step0 + s_;
-    for (int m = 0; m < max_m; ++m) {
-      int nb = ipre[(size_t)m * (nvar + 1) + nvar];
+    for (int m = 0; m < b->adj_max_m; ++m) {
+      int nb = b->adj_inj_blocks[m];
       if (nb)
         hipLaunchKernelGGL(adj_batch_inject<T>, dim3(nb), dim3(kInjE * kInjS), 0, s, dinj + (size_t)m * nvar,
                            dipre + (size_t)m * (nvar + 1), nvar, t, ddts, s_);
@@ -390,8 +412,31 @@ extern "C" int mt_tm2d_batch_adjoint(mt_batch* b, const mt_tm2d_adj* adj, void* 
       return fail(MT_EINVAL, "batched adjoint: null adjoint pointer");
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
-  return b->dtype == MT_F64 ? adjoint_t<double>(b, adj, tapes, mons, mon_off, step0, k, s)
-                            : adjoint_t<float>(b, adj, tapes, mons, mon_off, step0, k, s);
+  int rc = b->dtype == MT_F64 ? adjoint_prepare_t<double>(b, adj, tapes, mons, mon_off, s)
+                              : adjoint_prepare_t<float>(b, adj, tapes, mons, mon_off, s);
+  if (rc) return rc;
+  return b->dtype == MT_F64 ? adjoint_sweep_t<double>(b, step0, k, s) : adjoint_sweep_t<float>(b, step0, k, s);
+}
+
+extern "C" int mt_tm2d_batch_adjoint_prepare(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes,
+                                             const mt_monitor* mons, const int* mon_off, void* stream) {
+  if (!b || !adj || !tapes || !mon_off) return fail(MT_EINVAL, "batched adjoint: null argument");
+  for (int v = 0; v < b->nvar; ++v) {
+    const mt_tm2d_adj& a = adj[v];
+    if (!a.aEz || !a.aDz || !a.aHx || !a.aHy || !a.apsi_Ezy || !a.apsi_Ezx || !a.apsi_Hyx || !a.apsi_Hxy ||
+        !a.gHy || !a.gHx || !a.gEx || !a.gEy || !a.eps_grad || !tapes[v])
+      return fail(MT_EINVAL, "batched adjoint: null adjoint pointer");
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return b->dtype == MT_F64 ? adjoint_prepare_t<double>(b, adj, tapes, mons, mon_off, s)
+                            : adjoint_prepare_t<float>(b, adj, tapes, mons, mon_off, s);
+}
+
+extern "C" int mt_tm2d_batch_adjoint_sweep(mt_batch* b, int step0, int k, void* stream) {
+  if (!b) return fail(MT_EINVAL, "batched adjoint: null plan");
+  if (k < 0) return fail(MT_EINVAL, "negative step count");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return b->dtype == MT_F64 ? adjoint_sweep_t<double>(b, step0, k, s) : adjoint_sweep_t<float>(b, step0, k, s);
 }
 
 extern "C" void mt_batch_destroy(mt_batch* b) { delete b; }

@@ -641,9 +641,12 @@ template <typename T, int TX, int TY, int NS>
 int launch_tm_batch(const TmBatchVar<T>* vars, const int* pre_in, const int* pre_ring, int nvar, int n_in,
                     int n_ring, int grid_in, int grid_ring, int nsub, int parity, int t0, int slot_base,
                     int tape_s0, hipStream_t s) {
-  if (n_in)
+  if (n_in) {
+    timing_begin(s);
     hipLaunchKernelGGL((tm_batch<T, TX, TY, NS, false>), dim3(std::min(n_in, grid_in)), dim3(BLK_TM), 0, s, vars,
                        pre_in, nvar, nsub, parity, t0, slot_base, tape_s0);
+    timing_end(s, (long long)n_in * TX * TY * nsub);
+  }
   if (n_ring)
     hipLaunchKernelGGL((tm_batch<T, TX, TY, NS, true>), dim3(std::min(n_ring, grid_ring)), dim3(BLK_TM), 0, s, vars,
                        pre_ring, nvar, nsub, parity, t0, slot_base, tape_s0);

@@ -356,11 +356,59 @@ __global__ __launch_bounds__(256) void e3_pieces(K3<T> p, Pieces P, mt_source3d 
 // H'(i) from E(i) (LDS) and E(i+1) (registers), then E'(i-1) from H'(i-1)
 // (LDS) and H'(i-2) (registers).  E planes are double-buffered in LDS, eps
 // triple-buffered, H' double-buffered, so one barrier per plane orders every
-// LDS write after the reads of the same buffer.  Loads run two planes ahead
-// (E) or one (H, eps).  Halo values are recomputed from the read-only set A,
+// LDS write after the reads of the same buffer; the barrier orders LDS only
+// (lds_barrier), so global loads, issued two planes ahead, stay in flight.  Halo values are recomputed from the read-only set A,
 // never read from another workgroup's output, so workgroups are independent.
 // Arithmetic is the two-pass cell updates' expression for expression (the box
 // is psi-free and away from the PEC walls), so results are bit-identical.
+// Workgroup barrier that orders LDS only: this wave's LDS reads and writes
+// complete (lgkmcnt 0), then s_barrier.  __syncthreads() would also wait for
+// every outstanding global load and store (its workgroup fences wait on
+// vmcnt), draining the prefetch of the streaming kernel at every plane; here
+// global loads land only in the issuing lane's registers and the stores go
+// to buffers no workgroup reads, so they stay in flight across the barrier.
+// The empty asm statements keep the compiler from moving memory accesses
+// across it.
+__device__ __forceinline__ void lds_barrier() {
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);   // vmcnt 63 (any), expcnt 7 (any), lgkmcnt 0
+  __builtin_amdgcn_s_barrier();
+  __asm__ volatile("" ::: "memory");
+}
+
+// Raw buffer access to one x plane of a field (n elements): a load whose
+// offset is out of range reads 0 and such a store is dropped, so edge cells
+// and halo lanes need no branch around their loads and stores, and the
+// compiler counts the loads in flight exactly (no wait for all of them at a
+// join of paths that issued different numbers).
+constexpr int kOutOfRange = static_cast<int>(0x80000000u);
+
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const T* plane, long long n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(plane), 0, static_cast<int>(n * (long long)sizeof(T)),
+                                           0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T plane_load(const T* plane, long long n, int idx, bool ok) {
+  const auto r = plane_rsrc(plane, n);
+  const int off = ok ? idx * (int)sizeof(T) : kOutOfRange;
+  if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+template <typename T>
+__device__ __forceinline__ void plane_store(T* plane, long long n, int idx, bool ok, T v) {
+  const auto r = plane_rsrc(plane, n);
+  const int off = ok ? idx * (int)sizeof(T) : kOutOfRange;
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+  } else {
+    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, 0);
+  }
+}
+
 template <typename T>
 struct I3 {
   int nx, ny, nz, lean;
@@ -381,9 +429,9 @@ struct Col {          // one field column value set
 template <typename T, int TJ>
 __global__ __launch_bounds__(64 * (TJ + 2)) void y3_interior(I3<T> p, mt_source3d src, T c_src, T* tape) {
   constexpr int NR = TJ + 2;
-  __shared__ T sE[2][3][NR][66];
-  __shared__ T sEps[3][TJ + 1][65];
-  __shared__ T sH[2][3][TJ + 1][65];
+  __shared__ T sE[4][3][NR][66];      // E planes i-1 .. i+2 (ring of 4)
+  __shared__ T sEps[3][TJ + 1][65];   // eps planes i-1 .. i+1 (ring of 3)
+  __shared__ T sH[2][3][TJ + 1][65];  // H' planes i-1, i
   // XCD-aware tile order: consecutive workgroups go to different XCDs
   // (round robin), so XCD x takes the contiguous tile range
   // [x * per_xcd, (x + 1) * per_xcd), j fastest: j-neighbour tiles, which
@@ -402,6 +450,7 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void y3_interior(I3<T> p, mt_source3
   const bool haloL = lane == 0 && k0 >= p.K0;           // column k0-1 feeds active cells
   const bool haloR = lane == 63 && k0 + 64 < nz;        // column k0+64
   const bool wantH = own || wr == -1;
+  const int r = wr + 1, c = lane + 1;                   // LDS coordinates of the own cell
   // plane strides and row offsets
   const long long sEx = (long long)ny * nz, sEy = (long long)(ny - 1) * nz, sEz = (long long)ny * (nz - 1);
   const long long sHx = (long long)(ny - 1) * (nz - 1), sHy = sEz, sHz = sEy;
@@ -409,177 +458,176 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void y3_interior(I3<T> p, mt_source3
   const T dx = p.dx, dt = p.dt;
 
   auto loadE = [&](int ip, int kk, bool ok) {
-    Col<T> v{T(0), T(0), T(0)};
-    if (ok && jv && kk >= 0 && kk < nz) {
-      if (ip < nx - 1) v.x = p.Ex[ip * sEx + j * nz + kk];
-      if (j < ny - 1) v.y = p.Ey[ip * sEy + j * nz + kk];
-      if (kk < nz - 1) v.z = p.Ez[ip * sEz + j * (nz - 1) + kk];
-    }
-    return v;
+    const bool v = ok && jv && kk >= 0 && kk < nz;
+    return Col<T>{plane_load(p.Ex + ip * sEx, sEx, j * nz + kk, v && ip < nx - 1),
+                  plane_load(p.Ey + ip * sEy, sEy, j * nz + kk, v && j < ny - 1),
+                  plane_load(p.Ez + ip * sEz, sEz, j * (nz - 1) + kk, v && kk < nz - 1)};
   };
   auto loadEps = [&](int ip, int kk, bool ok) {
-    return (ok && wr >= 0 && jv && kk < nz) ? p.eps[ip * sEx + j * nz + kk] : T(0);
+    return plane_load(p.eps + ip * sEx, sEx, j * nz + kk, ok && wr >= 0 && jv && kk < nz);
   };
-  auto loadH = [&](int ip) {
-    Col<T> v{T(0), T(0), T(0)};
-    if (wantH && jv && kv && ip < nx - 1) {
-      if (j < ny - 1 && k < nz - 1) v.x = p.Hx[ip * sHx + oHX];
-      if (own && k < nz - 1) v.y = p.Hy[ip * sHy + oZ];
-      if (j < ny - 1) v.z = p.Hz[ip * sHz + oY];
-    }
-    return v;
+  auto loadH = [&](int ip, bool ok) {
+    const bool v = ok && wantH && jv && kv && ip < nx - 1;
+    return Col<T>{plane_load(p.Hx + ip * sHx, sHx, oHX, v && j < ny - 1 && k < nz - 1),
+                  plane_load(p.Hy + ip * sHy, sHy, oZ, v && own && k < nz - 1),
+                  plane_load(p.Hz + ip * sHz, sHz, oY, v && j < ny - 1)};
   };
-  auto loadHL = [&](int ip) {   // Hx, Hy at (ip, j, k0-1) for the halo column
-    Col<T> v{T(0), T(0), T(0)};
-    if (haloL && own && jv && ip < nx - 1) {
-      const int kk = k0 - 1;
-      if (j < ny - 1) v.x = p.Hx[ip * sHx + j * (nz - 1) + kk];
-      v.y = p.Hy[ip * sHy + j * (nz - 1) + kk];
-    }
-    return v;
+  auto loadHL = [&](int ip, bool ok) {   // Hx, Hy at (ip, j, k0-1) for the halo column
+    const bool v = ok && haloL && own && jv && ip < nx - 1;
+    const int q = j * (nz - 1) + k0 - 1;
+    return Col<T>{plane_load(p.Hx + ip * sHx, sHx, q, v && j < ny - 1), plane_load(p.Hy + ip * sHy, sHy, q, v),
+                  T(0)};
   };
-
-  // registers: E at planes i (e0), i+1 (e1), i+2 (in flight); eps at i-1, i;
-  // H'(i-1), H'(i-2) of the own cell; old H of plane i
-  const int ip0 = ib - 1;
-  Col<T> e0 = loadE(ip0, k, true), e1 = loadE(ip0 + 1, k, true), em{T(0), T(0), T(0)};
-  Col<T> l0 = loadE(ip0, k0 - 1, lane == 0), l1 = loadE(ip0 + 1, k0 - 1, lane == 0);
-  Col<T> r0 = loadE(ip0, k0 + 64, haloR), r1 = loadE(ip0 + 1, k0 + 64, haloR);
-  T eps0 = loadEps(ip0, k, true), epsR0 = loadEps(ip0, k0 + 64, haloR && wr < TJ), epsM = T(0);
-  Col<T> hold = loadH(ip0), hl = loadHL(ip0);
-  Col<T> hp{T(0), T(0), T(0)}, hpp{T(0), T(0), T(0)};
-
-  for (int i = ip0; i <= ie; ++i) {
-    const int bE = i & 1, bS = i % 3;
-    // ---- plane i into LDS
-    sE[bE][0][wr + 1][lane + 1] = e0.x;
-    sE[bE][1][wr + 1][lane + 1] = e0.y;
-    sE[bE][2][wr + 1][lane + 1] = e0.z;
+  // E of plane ip (own cell, lane 0's column k0-1, lane 63's column k0+64) into ring slot ip & 3
+  auto putE = [&](int ip, const Col<T>& e, const Col<T>& l, const Col<T>& rr) {
+    const int b = ip & 3;
+    sE[b][0][r][c] = e.x;
+    sE[b][1][r][c] = e.y;
+    sE[b][2][r][c] = e.z;
     if (lane == 0) {
-      sE[bE][0][wr + 1][0] = l0.x;
-      sE[bE][1][wr + 1][0] = l0.y;
-      sE[bE][2][wr + 1][0] = l0.z;
+      sE[b][0][r][0] = l.x;
+      sE[b][1][r][0] = l.y;
+      sE[b][2][r][0] = l.z;
     }
     if (lane == 63) {
-      sE[bE][0][wr + 1][65] = r0.x;
-      sE[bE][1][wr + 1][65] = r0.y;
+      sE[b][0][r][65] = rr.x;
+      sE[b][1][r][65] = rr.y;
     }
+  };
+
+  // Pipeline (two planes of load latency for every stream): at iteration i
+  // plane i+1 of E and plane i of eps go into LDS from registers loaded two
+  // iterations earlier, and E(i+3), eps(i+2) and the old H(i+2) are loaded.
+  // Three register stages rotate by name (the loop is unrolled by three), so
+  // no value is copied out of a load still in flight: a copy would wait for it.
+  struct Stage {
+    Col<T> e, l, rr, h, hl;
+    T eps, epsR;
+  };
+  auto fetch = [&](Stage& st, int i) {  // the stage that iteration i consumes
+    const bool e_ok = i + 1 <= ie, eps_ok = i <= ie, h_ok = i < ie;
+    st.e = loadE(i + 1, k, e_ok);
+    st.l = loadE(i + 1, k0 - 1, e_ok && lane == 0);
+    st.rr = loadE(i + 1, k0 + 64, e_ok && haloR);
+    st.eps = loadEps(i, k, eps_ok);
+    st.epsR = loadEps(i, k0 + 64, eps_ok && haloR && wr < TJ);
+    st.h = loadH(i, h_ok);
+    st.hl = loadHL(i, h_ok);
+  };
+  const int ip0 = ib - 1;
+  putE(ip0, loadE(ip0, k, true), loadE(ip0, k0 - 1, lane == 0), loadE(ip0, k0 + 64, haloR));
+  Stage S0, S1, S2;
+  fetch(S0, ip0);
+  fetch(S1, ip0 + 1);
+  Col<T> hp{T(0), T(0), T(0)}, hpp{T(0), T(0), T(0)};
+
+  auto body = [&](int i, const Stage& cur, Stage& ahead) {
+    // ---- into LDS: E(i+1), eps(i)
+    putE(i + 1, cur.e, cur.l, cur.rr);
     if (wr >= 0) {
-      sEps[bS][wr][lane] = eps0;
-      if (lane == 63 && wr < TJ) sEps[bS][wr][64] = epsR0;
+      sEps[i % 3][wr][lane] = cur.eps;
+      if (lane == 63 && wr < TJ) sEps[i % 3][wr][64] = cur.epsR;
     }
-    // ---- prefetch: E(i+2), old H(i+1), eps(i+1)
-    const bool moreE = i + 2 <= ie;
-    const bool moreH = i + 1 < ie;
-    Col<T> eN{T(0), T(0), T(0)}, lN{T(0), T(0), T(0)}, rN{T(0), T(0), T(0)}, hN{T(0), T(0), T(0)},
-        hlN{T(0), T(0), T(0)};
-    T epsN = T(0), epsRN = T(0);
-    if (moreE) {
-      eN = loadE(i + 2, k, true);
-      lN = loadE(i + 2, k0 - 1, lane == 0);
-      rN = loadE(i + 2, k0 + 64, haloR);
-    }
-    if (moreH) {
-      hN = loadH(i + 1);
-      hlN = loadHL(i + 1);
-    }
-    if (i + 1 <= ie) {
-      epsN = loadEps(i + 1, k, true);
-      epsRN = loadEps(i + 1, k0 + 64, haloR && wr < TJ);
-    }
-    __syncthreads();
+    // ---- loads two planes ahead
+    fetch(ahead, i + 2);
+    lds_barrier();
     // ---- H'(i): own cells, halo row j0-1 (Hx', Hz'), halo column k0-1 (Hx', Hy')
     Col<T> hn{T(0), T(0), T(0)};
     if (i < ie && wantH) {
-      const int r = wr + 1, c = lane + 1;
-      const T ezj = sE[bE][2][r + 1][c], eyk = sE[bE][1][r][c + 1];
-      const T exk = sE[bE][0][r][c + 1], exj = sE[bE][0][r + 1][c];
+      const int b0 = i & 3, b1 = (i + 1) & 3;
+      const T ex0 = sE[b0][0][r][c], ey0 = sE[b0][1][r][c], ez0 = sE[b0][2][r][c];
+      const T ezj = sE[b0][2][r + 1][c], eyk = sE[b0][1][r][c + 1];
+      const T exk = sE[b0][0][r][c + 1], exj = sE[b0][0][r + 1][c];
+      const T eyx = sE[b1][1][r][c], ezx = sE[b1][2][r][c];
       {  // Hx' (stepper_3d.py:371-375)
-        T cy = (ezj - e0.z) / dx;
-        T cz = (eyk - e0.y) / dx;
-        hn.x = hold.x - dt * (cy - cz);
+        T cy = (ezj - ez0) / dx;
+        T cz = (eyk - ey0) / dx;
+        hn.x = cur.h.x - dt * (cy - cz);
       }
       if (own) {  // Hy'
-        T cz = (exk - e0.x) / dx;
-        T cx = (e1.z - e0.z) / dx;
-        hn.y = hold.y - dt * (cz - cx);
+        T cz = (exk - ex0) / dx;
+        T cx = (ezx - ez0) / dx;
+        hn.y = cur.h.y - dt * (cz - cx);
       }
       {  // Hz'
-        T cx = (e1.y - e0.y) / dx;
-        T cy = (exj - e0.x) / dx;
-        hn.z = hold.z - dt * (cx - cy);
+        T cx = (eyx - ey0) / dx;
+        T cy = (exj - ex0) / dx;
+        hn.z = cur.h.z - dt * (cx - cy);
       }
       const int bH = i & 1;
       sH[bH][0][r][c] = hn.x;
       sH[bH][1][r][c] = hn.y;
       sH[bH][2][r][c] = hn.z;
       if (haloL && own) {  // column k0-1
-        const T ez0 = sE[bE][2][r][0], ey0 = sE[bE][1][r][0], ex0 = sE[bE][0][r][0];
-        T cy = (sE[bE][2][r + 1][0] - ez0) / dx;
-        T cz = (sE[bE][1][r][1] - ey0) / dx;
-        sH[bH][0][r][0] = hl.x - dt * (cy - cz);
-        T cz2 = (sE[bE][0][r][1] - ex0) / dx;
-        T cx2 = (l1.z - ez0) / dx;
-        sH[bH][1][r][0] = hl.y - dt * (cz2 - cx2);
-      }
-      if (active && i >= ib) {
-        p.oHx[i * sHx + oHX] = hn.x;
-        p.oHy[i * sHy + oZ] = hn.y;
-        p.oHz[i * sHz + oY] = hn.z;
+        const T hez0 = sE[b0][2][r][0], hey0 = sE[b0][1][r][0], hex0 = sE[b0][0][r][0];
+        T cy = (sE[b0][2][r + 1][0] - hez0) / dx;
+        T cz = (sE[b0][1][r][1] - hey0) / dx;
+        sH[bH][0][r][0] = cur.hl.x - dt * (cy - cz);
+        T cz2 = (sE[b0][0][r][1] - hex0) / dx;
+        T cx2 = (sE[b1][2][r][0] - hez0) / dx;
+        sH[bH][1][r][0] = cur.hl.y - dt * (cz2 - cx2);
       }
     }
+    {
+      const bool st = active && i >= ib && i < ie;
+      plane_store(p.oHx + i * sHx, sHx, oHX, st, hn.x);
+      plane_store(p.oHy + i * sHy, sHy, oZ, st, hn.y);
+      plane_store(p.oHz + i * sHz, sHz, oY, st, hn.z);
+    }
     // ---- E'(i-1) of the own cell (stepper_3d.py:392-437)
+    const int im = i - 1;
+    T ex = T(0), ey = T(0), ez = T(0);
     if (i > ib && active) {
-      const int im = i - 1, bH = im & 1, bP = im % 3, r = wr + 1, c = lane + 1;
-      T ex, ey, ez;
+      const int bH = im & 1, bP = im % 3, bE = im & 3;
+      const T epsm = sEps[bP][wr][lane], eps1 = sEps[i % 3][wr][lane];
       {
         T chy = (hp.z - sH[bH][2][r - 1][c]) / dx;
         T chz = (hp.y - sH[bH][1][r][c - 1]) / dx;
         T curl = chy - chz;
-        T epsc = T(0.5) * (epsM + eps0);
-        ex = e_update(p, em.x, epsc, curl);
+        T epsc = T(0.5) * (epsm + eps1);
+        ex = e_update(p, sE[bE][0][r][c], epsc, curl);
         ex = inject(p, src, 0, im, j, k, ex, epsc, c_src);
       }
       {
         T chz = (hp.x - sH[bH][0][r][c - 1]) / dx;
         T chx = (hp.z - hpp.z) / dx;
         T curl = chz - chx;
-        T epsc = T(0.5) * (epsM + sEps[bP][wr + 1][lane]);
-        ey = e_update(p, em.y, epsc, curl);
+        T epsc = T(0.5) * (epsm + sEps[bP][wr + 1][lane]);
+        ey = e_update(p, sE[bE][1][r][c], epsc, curl);
         ey = inject(p, src, 1, im, j, k, ey, epsc, c_src);
       }
       {
         T chx = (hp.y - hpp.y) / dx;
         T chy = (hp.x - sH[bH][0][r - 1][c]) / dx;
         T curl = chx - chy;
-        T epsc = T(0.5) * (epsM + sEps[bP][wr][lane + 1]);
-        ez = e_update(p, em.z, epsc, curl);
+        T epsc = T(0.5) * (epsm + sEps[bP][wr][lane + 1]);
+        ez = e_update(p, sE[bE][2][r][c], epsc, curl);
         ez = inject(p, src, 2, im, j, k, ez, epsc, c_src);
       }
-      p.oEx[im * sEx + oX] = ex;
-      p.oEy[im * sEy + oY] = ey;
-      p.oEz[im * sEz + oZ] = ez;
+    }
+    {
+      const bool st = i > ib && active;
+      plane_store(p.oEx + im * sEx, sEx, oX, st, ex);
+      plane_store(p.oEy + im * sEy, sEy, oY, st, ey);
+      plane_store(p.oEz + im * sEz, sEz, oZ, st, ez);
       if (tape) {
-        tape[im * sEx + oX] = ex;
-        tape[(long long)(nx - 1) * sEx + im * sEy + oY] = ey;
-        tape[(long long)(nx - 1) * sEx + (long long)nx * sEy + im * sEz + oZ] = ez;
+        T* ty = tape + (long long)(nx - 1) * sEx;
+        T* tz = ty + (long long)nx * sEy;
+        plane_store(tape + im * sEx, sEx, oX, st, ex);
+        plane_store(ty + im * sEy, sEy, oY, st, ey);
+        plane_store(tz + im * sEz, sEz, oZ, st, ez);
       }
     }
-    // ---- rotate
-    em = e0;
-    e0 = e1;
-    e1 = eN;
-    l0 = l1;
-    l1 = lN;
-    r0 = r1;
-    r1 = rN;
-    epsM = eps0;
-    eps0 = epsN;
-    epsR0 = epsRN;
     hpp = hp;
     hp = hn;
-    hold = hN;
-    hl = hlN;
+  };
+  for (int i = ip0;;) {
+    body(i, S0, S2);
+    if (++i > ie) break;
+    body(i, S1, S0);
+    if (++i > ie) break;
+    body(i, S2, S1);
+    if (++i > ie) break;
   }
 }
 
@@ -680,12 +728,10 @@ int run3_split(const mt_yee3d* d, K3<T> p, const mt_source3d& sv, const double* 
   q.kbase = (b.K0 / 64) * 64;
   q.njt = (b.J1 - b.J0 + TJ - 1) / TJ;
   q.nkt = (b.K1 - q.kbase + 63) / 64;
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-    cus = 256;
-  // enough x segments for about three workgroups per CU, each at least 16 planes
+  // x segments of about 32 planes: many workgroups (small tail), and the
+  // one recomputed halo plane per segment costs about 3%
   const int tiles2d = q.njt * q.nkt;
-  int nseg = std::max(1, std::min((3 * cus + tiles2d - 1) / tiles2d, (b.I1 - b.I0) / 16));
+  int nseg = std::max(1, (b.I1 - b.I0 + 31) / 32);
   q.iseg = (b.I1 - b.I0 + nseg - 1) / nseg;
   nseg = (b.I1 - b.I0 + q.iseg - 1) / q.iseg;
   q.ntiles = tiles2d * nseg;
@@ -772,12 +818,14 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
   if (src && src->kind) sv = *src;
   const Box3 box = interior_box(d);
   const bool split = d->variant >= 1 && !media && box.I1 > box.I0 && box.J1 > box.J0 && box.K1 > box.K0;
-  if (split && d->variant == 1) return run3_split<T, 6>(d, p, sv, coef, mp, step0, k, s);
-  if (split && d->variant == 2) {
-    if constexpr (sizeof(T) == 4) return run3_split<T, 14>(d, p, sv, coef, mp, step0, k, s);
-    else return run3_split<T, 6>(d, p, sv, coef, mp, step0, k, s);
+  // tile rows: FP32 6 / 10 / 4, FP64 4 (the LDS rings must fit 64 KB)
+  if (split) {
+    if constexpr (sizeof(T) == 4) {
+      if (d->variant == 1) return run3_split<T, 6>(d, p, sv, coef, mp, step0, k, s);
+      if (d->variant == 2) return run3_split<T, 10>(d, p, sv, coef, mp, step0, k, s);
+    }
+    return run3_split<T, 4>(d, p, sv, coef, mp, step0, k, s);
   }
-  if (split && d->variant == 3) return run3_split<T, 4>(d, p, sv, coef, mp, step0, k, s);
   dim3 blk(BZ, BYY, 1);
   dim3 grd((p.nz + BZ - 1) / BZ, (p.ny + BYY - 1) / BYY, p.nx);
   int S = mp.stage_steps > 0 ? mp.stage_steps : 1;

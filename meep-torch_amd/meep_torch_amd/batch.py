@@ -47,6 +47,11 @@ def _bind(L):
     L.mt_tm2d_batch_adjoint.restype = E.c_int
     L.mt_tm2d_batch_adjoint.argtypes = [E.c_void_p, P(E.TM2DAdj), P(E.c_void_p), P(E.Monitor),
                                         P(E.c_int), E.c_int, E.c_int, E.c_void_p]
+    L.mt_tm2d_batch_adjoint_prepare.restype = E.c_int
+    L.mt_tm2d_batch_adjoint_prepare.argtypes = [E.c_void_p, P(E.TM2DAdj), P(E.c_void_p), P(E.Monitor), P(E.c_int),
+                                                E.c_void_p]
+    L.mt_tm2d_batch_adjoint_sweep.restype = E.c_int
+    L.mt_tm2d_batch_adjoint_sweep.argtypes = [E.c_void_p, E.c_int, E.c_int, E.c_void_p]
     L.mt_batch_destroy.restype = None
     L.mt_batch_destroy.argtypes = [E.c_void_p]
     L._batch_bound = True
@@ -79,10 +84,13 @@ class BatchedTM:
         self.var = []
         for v, (coeffs, source, monitors) in enumerate(problems):
             self.var.append(self._alloc(coeffs, source, list(monitors)))
-        self.ws = torch.empty(max(1, self.L.mt_tm2d_batch_workspace(self.B, self.code)), dtype=torch.uint8,
-                              device=self.device)
+        # one descriptor workspace per plan (forward, and recompute with Ez tapes)
+        self._ws_bytes = max(1, self.L.mt_tm2d_batch_workspace(self.B, self.code))
+        self.ws = {}
         self._plans = {}
         self.graph = None
+        self.adj_graph = None
+        self._adj = None
 
     # ------------------------------------------------------------ allocation
     def _alloc(self, coeffs, source, monitors):
@@ -157,8 +165,9 @@ class BatchedTM:
             it.n_mons = len(V["monitors"])
             it.stage = V["stage"].data_ptr() if V["stage"] is not None else None
             it.stage_bytes = V["stage_bytes"]
-        h = self.L.mt_tm2d_batch_create(items, self.B, self.variant, self.S, self.ws.data_ptr(),
-                                        self.ws.numel(), E.stream_handle())
+        ws = self.ws.setdefault(tape, torch.empty(self._ws_bytes, dtype=torch.uint8, device=self.device))
+        h = self.L.mt_tm2d_batch_create(items, self.B, self.variant, self.S, ws.data_ptr(), ws.numel(),
+                                        E.stream_handle())
         if not h:
             E.check(-1, "mt_tm2d_batch_create")
         self._plans[tape] = (h, items)
@@ -214,12 +223,13 @@ class BatchedTM:
             self._forward_body()
         self.graph = g
 
-    def forward(self, eps_list=None):
+    def forward(self, eps_list=None, use_graph=True):
         """Run every variant; returns [[acc per monitor] per variant] shaped as
-        the monitors' run_tm outputs (views of the fixed buffers)."""
+        the monitors' run_tm outputs (views of the fixed buffers).  With a
+        captured graph the run replays it unless use_graph is False."""
         if eps_list is not None:
             self.set_eps(eps_list)
-        if self.graph is not None:
+        if self.graph is not None and use_graph:
             self.graph.replay()
         else:
             self._forward_body()
@@ -229,16 +239,20 @@ class BatchedTM:
         return res
 
     # -------------------------------------------------------------- adjoint
-    def gradient(self, gaccs):
-        """dL/deps per variant given dL/dacc (list per variant of per-monitor
-        cotangents, None where unused).  Uses the checkpoints of the last forward."""
+    def _adj_setup(self):
+        """Persistent adjoint buffers (cotangent fields, transposed-difference
+        scratch, eps gradients, monitor cotangents) and their descriptors,
+        uploaded once (mt_tm2d_batch_adjoint_prepare): every later sweep
+        only launches, so it can be replayed from a graph."""
+        if self._adj is not None:
+            return self._adj
         if not self.adjoint:
             raise RuntimeError("BatchedTM was built with adjoint=False (no checkpoints or tapes)")
+        from .core.adjoint2d import _adj_monitors
         h, _ = self._plan(True)
         B = self.B
-        adj, grads, mons, off, tapes = [], [], [], [0], (E.c_void_p * B)()
-        adjs = (E.TM2DAdj * B)()
-        keep = []
+        adjs, tapes = (E.TM2DAdj * B)(), (E.c_void_p * B)()
+        per, mons, off, keep = [], [], [0], []
         for v, V in enumerate(self.var):
             nx, ny = V["nx"], V["ny"]
             A = {f: torch.zeros_like(V["state"][f]) for f in FIELDS}
@@ -247,8 +261,7 @@ class BatchedTM:
                   "gEx": torch.empty(nx - 1, ny, dtype=self.dtype, device=self.device),
                   "gEy": torch.empty(nx, ny - 1, dtype=self.dtype, device=self.device)}
             g = torch.zeros(nx, ny, dtype=self.dtype, device=self.device)
-            adj.append((A, sc))
-            grads.append(g)
+            gacc = [torch.zeros_like(a) for a in V["acc"]]
             a = adjs[v]
             for f in FIELDS:
                 setattr(a, "a" + f, A[f].data_ptr())
@@ -256,14 +269,26 @@ class BatchedTM:
                 setattr(a, f, t.data_ptr())
             a.eps_grad = g.data_ptr()
             tapes[v] = V["tape"].data_ptr()
-            from .core.adjoint2d import _adj_monitors
-            arr, n_used, kp = _adj_monitors(A, V["monitors"], list(gaccs[v]), self.dtype, self.device)
+            arr, n_used, kp = _adj_monitors(A, V["monitors"], gacc, self.dtype, self.device)
             keep.append(kp)
             for i in range(n_used):
                 mons.append(arr[i])
             off.append(off[-1] + n_used)
+            per.append({"A": A, "sc": sc, "g": g, "gacc": gacc})
         mon_arr = (E.Monitor * max(1, len(mons)))(*mons)
         off_arr = (E.c_int * (B + 1))(*off)
+        rc = self.L.mt_tm2d_batch_adjoint_prepare(h, adjs, tapes, mon_arr, off_arr, E.stream_handle())
+        E.check(rc, "mt_tm2d_batch_adjoint_prepare")
+        self._adj = {"h": h, "per": per, "keep": (adjs, tapes, mon_arr, off_arr, keep)}
+        return self._adj
+
+    def _gradient_body(self):
+        adj = self._adj
+        h = adj["h"]
+        for P in adj["per"]:
+            for t in P["A"].values():
+                t.zero_()
+            P["g"].zero_()
         nck = len(self.var[0]["ckpt"])
         for ci in reversed(range(nck)):
             step0 = ci * self.K
@@ -273,10 +298,42 @@ class BatchedTM:
                     V["state"][f].copy_(V["ckpt"][ci][f])
             rc = self.L.mt_tm2d_batch_run(h, step0, k, E.stream_handle())
             E.check(rc, "mt_tm2d_batch_run (recompute)")
-            rc = self.L.mt_tm2d_batch_adjoint(h, adjs, tapes, mon_arr, off_arr, step0, k, E.stream_handle())
-            E.check(rc, "mt_tm2d_batch_adjoint")
-        del keep
-        return grads
+            rc = self.L.mt_tm2d_batch_adjoint_sweep(h, step0, k, E.stream_handle())
+            E.check(rc, "mt_tm2d_batch_adjoint_sweep")
+
+    def gradient(self, gaccs):
+        """dL/deps per variant given dL/dacc (list per variant of per-monitor
+        cotangents, None where unused).  Uses the checkpoints of the last
+        forward; replays the captured adjoint graph when there is one."""
+        adj = self._adj_setup()
+        with torch.no_grad():
+            for P, gv in zip(adj["per"], gaccs):
+                for buf, g in zip(P["gacc"], gv):
+                    if g is None:
+                        buf.zero_()
+                    else:
+                        buf.copy_(g.detach().reshape(buf.shape))
+        if self.adj_graph is not None:
+            self.adj_graph.replay()
+        else:
+            self._gradient_body()
+        return [P["g"].clone() for P in adj["per"]]
+
+    def capture_adjoint(self):
+        """Record the reverse sweep (checkpoint restores, chunk recomputes on
+        the Ez tapes, batched adjoint steps) in a HIP graph that gradient()
+        replays; the cotangents are copied into fixed buffers before each
+        replay."""
+        self._adj_setup()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self._gradient_body()         # warm-up outside the graph
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._gradient_body()
+        self.adj_graph = g
 
     def apply(self, eps_list):
         """Differentiable forward: returns [[acc per monitor] per variant];

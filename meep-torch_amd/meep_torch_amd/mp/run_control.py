@@ -163,14 +163,28 @@ class stop_when_fields_decayed:
         self._window = 0.0
 
     def __call__(self, sim):
-        self._window = max(self._window, abs(sim.get_field_point(self.c, self.pt)) ** 2)
-        t = sim.round_time()
+        return self.observe(sim.get_field_point(self.c, self.pt), sim.round_time())
+
+    def observe(self, value, t):
+        """One step's field value at the end time t of the step."""
+        self._window = max(self._window, abs(value) ** 2)
         if t <= self._t_window + self.dt:
             return False
         w, self._window = self._window, 0.0
         self._t_window = t
         self._peak = max(self._peak, w)
         return self._peak > 0 and w <= self.decay_by * self._peak
+
+    def steps_to_check(self, step, dt):
+        """Steps s >= 1 from step `step` until the first one whose end time
+        (step + s) * dt closes the current window (the next comparison)."""
+        limit = self._t_window + self.dt
+        s = max(1, int(math.floor(limit / dt)) - step)
+        while (step + s) * dt <= limit:
+            s += 1
+        while s > 1 and (step + s - 1) * dt > limit:
+            s -= 1
+        return s
 
 
 def stop_after_walltime(seconds):

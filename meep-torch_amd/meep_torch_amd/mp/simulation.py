@@ -17,8 +17,8 @@ fields and zero DFT accumulators and integrates to the accumulated clock
 them (meep_torch/mp/simulation.py:1271-1295); each chunk's monitor sums are
 added in chunk order.  Step functions and stop conditions (run_control.py)
 split the run into engine calls at the points where a hook must look at the
-fields.  Symmetries, k-points, multi-source 2D runs and geometry without a
-2D cross-section raise NotImplementedError.
+fields.  Symmetries, k-points and geometry without a 2D cross-section
+raise NotImplementedError.
 """
 from collections import namedtuple
 
@@ -81,6 +81,28 @@ def _name(f):
 
 # meep's get_eigenmode_coefficients result (meep_torch/mp/run_control.py:193-195)
 EigCoeffsResult = namedtuple("EigCoeffsResult", ["alpha", "vgrp", "kpoints", "kdom", "cscale"])
+
+
+class _FieldProbe:
+    """Engine point probe of one state component (reduce 'stack')."""
+    reduce = "stack"
+
+    def __init__(self, name, ix, iy):
+        self.name, self.ix, self.iy = name, ix, iy
+
+    def engine_terms(self, f):
+        from ..core.monitors import _ids
+        fid = _ids(f).get(self.name)
+        sem = (fid, 4, self.ix, self.iy) if fid is not None else None
+        return "probe", [(f[self.name][self.ix, self.iy].reshape(1, 1), None, 0, sem)]
+
+
+class _SourceSum:
+    """Several facade sources driving one run (the engine sums
+    amp_i(t) * mask_i every step, engine2d's multi-source plan)."""
+
+    def __init__(self, parts):
+        self.parts = list(parts)
 
 
 class Simulation(Sim3D):
@@ -177,8 +199,6 @@ class Simulation(Sim3D):
         self._init_structure()
 
     def _make_sources(self):
-        if len(self._sources) > 1:
-            raise NotImplementedError("several sources in one run are not supported by the HIP facade")
         out = []
         for spec in self._sources:
             ix, j0, j1 = source_span(spec, self.gs)
@@ -382,13 +402,30 @@ class Simulation(Sim3D):
             names = te_fields_of(st)
         work = {f: getattr(st, f).contiguous() for f in names}
         srcs = self._make_sources()
-        src = srcs[0] if srcs else None
+        # several sources: one injection of sum_i amp_i(t) mask_i per step,
+        # and the monitors fold every step into one running sum, as the
+        # reference's per-step loop does (mp/cpp/core.py:584-589,648-664,802-820)
+        multi = len(srcs) > 1
+        src = _SourceSum(srcs) if multi else (srcs[0] if srcs else None)
         target = work["Dz"].shape if mode == "tm" else work["Dy"].shape
         plan = engine2d.source_plan(src, target, self.grid.device, self.grid.dtype)
         keys = list(self._monitors)
         mons = [self._monitors[k][1] for k in keys]
         seeds = getattr(self, "_acc_seed", {})
         acc = [seeds[k].clone() if k in seeds else None for k in keys]
+        # stop_when_fields_decayed only compares at the end of its dt window:
+        # a device probe records its field every step and the engine runs a
+        # whole window per call (same stop step as feeding it every step)
+        probes = self._decay_probes(conds, work) if conds is not None else None
+        raw = None
+        if (multi or probes is not None) and mons:
+            # one running sum per monitor, started from the loaded flux data:
+            # the reference folds every step into the total on these paths
+            # (per-step calls), not chunk partials
+            raw = engine2d.run_chunk(mode, work, self._coeffs, 0, 0, None, mons, raw=True)
+            for (_, _, buf), a in zip(raw, acc):
+                if a is not None:
+                    buf.copy_(a.reshape(buf.shape))
         self._state = st._replace(**work)
         self._acc = {k: a for k, a in zip(keys, acc) if a is not None}
         start = self._t_step
@@ -400,10 +437,19 @@ class Simulation(Sim3D):
         while stop_at is None or self._t_step < stop_at:
             t = self._t_step * dt
             left = stop_at - self._t_step if stop_at is not None else 1 << 30
-            k = RC.chunk_size(t, dt, left, per_step, scheduled, conds is not None)
-            outs = engine2d.run_chunk(mode, work, self._coeffs, f_t, k, plan, mons)
-            for i, o in enumerate(outs):
-                acc[i] = o if acc[i] is None else acc[i] + o
+            k = RC.chunk_size(t, dt, left, per_step, scheduled, conds is not None and probes is None)
+            if probes is not None:
+                k = min([k] + [g.steps_to_check(self._t_step, dt) for g in conds])
+            pm = [p for p, _ in probes] if probes is not None else []
+            if raw is not None:
+                raw = engine2d.run_chunk(mode, work, self._coeffs, f_t, k, plan, mons + pm, reuse=raw, raw=True)
+                outs = engine2d.finish_outputs(raw, work)
+                acc = [a.clone() for a in outs[:len(mons)]]
+            else:
+                outs = engine2d.run_chunk(mode, work, self._coeffs, f_t, k, plan, mons + pm)
+                for i, o in enumerate(outs[:len(mons)]):
+                    acc[i] = o if acc[i] is None else acc[i] + o
+            step_before = self._t_step
             f_t += k
             self._t_step = f_t
             self._state = st._replace(**work)
@@ -411,12 +457,46 @@ class Simulation(Sim3D):
             t = self._t_step * dt
             RC.fire_per_step(self, per_step, t, dt)
             RC.fire_scheduled(self, scheduled, t)
-            if conds is not None:
+            if probes is not None:
+                vals = [(o.detach().double().cpu() * scale).tolist() for o, (_, scale) in zip(outs[len(mons):], probes)]
+                hit = [False] * len(conds)
+                for q in range(k):
+                    t_q = (step_before + q + 1) * dt
+                    hit = [bool(g.observe(v[q], t_q)) for g, v in zip(conds, vals)]
+                if t >= gate - 1e-9 and any(hit):
+                    break
+            elif conds is not None:
                 hit = [bool(g(self)) for g in conds]     # every condition sees every step
                 if t >= gate - 1e-9 and any(hit):
                     break
         for func in step_funcs:
             RC.eval_step_func(self, func, "finish")
+
+    def _decay_probes(self, conds, work):
+        """(probe monitor, scale) per stop_when_fields_decayed condition, or
+        None when some condition is another callable (those are fed every
+        step).  The probe reads the component get_field_point reads (D as
+        eps E, B as H, a component the state lacks as the primary field) at
+        the same cell."""
+        if not conds or not all(isinstance(g, RC.stop_when_fields_decayed) for g in conds):
+            return None
+        out = []
+        gs = self.gs
+        for g in conds:
+            pt = Vector3(*g.pt)
+            ix = max(0, min(gs.nx - 1, G.cell_index(pt.x, gs)))
+            iy = max(0, min(gs.ny - 1, G.cell_index_y(pt.y, gs)))
+            name, scale = (g.c if isinstance(g.c, str) else ""), 1.0
+            if name.startswith("D") and name != "Dielectric":
+                name = "E" + name[1:]
+                scale = float(np.asarray(self.get_epsilon())[ix, iy])
+            elif name.startswith("B"):
+                name = "H" + name[1:]
+            if name not in work:
+                name = "Hz" if self._pol == "te" else "Ez"
+            f = work[name]
+            out.append((_FieldProbe(name, min(ix, f.shape[0] - 1), min(iy, f.shape[1] - 1)), scale))
+        return out
 
     def restart_fields(self):
         self._state = None

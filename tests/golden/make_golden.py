@@ -46,6 +46,9 @@ Cases (all CPU, torch threads = 8):
              component
   shapes     mp.Simulation 2D TM cell with a Cylinder, Sphere, Prism, rotated
              Block and Ellipsoid: eps map, Ez and a flux after t = 12, f64
+  multisrc   mp.Simulation 2D TM cell with two sources (point and line, own
+             pulses and amplitudes): Ez and a flux after t = 10, plain and
+             with an at_every step function splitting the run, f64
 """
 import hashlib
 import json
@@ -652,10 +655,38 @@ def case_shapes():
     save("shapes_f64", **out)
 
 
+# ---------------------------------------------------------- several sources
+def case_multisrc():
+    import meep_torch.mp as mp
+    out = {}
+    for hook in (False, True):
+        sim = mp.Simulation(cell_size=mp.Vector3(6, 4), resolution=10,
+                            geometry=[mp.Block(size=mp.Vector3(mp.inf, 0.6), material=mp.Medium(epsilon=8.0))],
+                            sources=[mp.Source(mp.GaussianSource(0.8, fwidth=0.6), component=mp.Ez,
+                                               center=mp.Vector3(-2.0, 0.0)),
+                                     mp.Source(mp.GaussianSource(1.1, fwidth=0.5), component=mp.Ez,
+                                               center=mp.Vector3(-1.4, 0.5), size=mp.Vector3(0, 0.8),
+                                               amplitude=-0.7)],
+                            boundary_layers=[mp.PML(0.5)])
+        fl = sim.add_flux(0.9, 0.5, 4, mp.FluxRegion(center=mp.Vector3(2.2, 0), size=mp.Vector3(0, 3)))
+        seen = []
+        if hook:
+            sim.run(mp.at_every(1.5, lambda s: seen.append(float(s.get_field_point(mp.Ez, mp.Vector3(1.0, 0))))),
+                    until=10)
+        else:
+            sim.run(until=10)
+        tag = "hook" if hook else "plain"
+        out[f"{tag}_Ez"] = np.asarray(sim.get_array(component=mp.Ez))
+        out[f"{tag}_flux"] = np.array(mp.get_fluxes(fl))
+        if hook:
+            out["hook_probe"] = np.array(seen)
+    save("multisrc_f64", **out)
+
+
 def main():
     which = sys.argv[1:] or ["pml", "tm", "te", "c1", "3d", "grad_tm", "grad_3d",
                              "filters", "golden_plain", "media_tm", "media_te", "media_3d",
-                             "tfsf_tm", "sim3d", "raster", "shapes"]
+                             "tfsf_tm", "sim3d", "raster", "shapes", "multisrc"]
     t_all = time.perf_counter()
     timing = {}
     try:
@@ -671,7 +702,7 @@ def main():
          "filters": case_filters, "golden_plain": case_golden_plain,
          "media_tm": case_media_tm, "media_te": case_media_te, "media_3d": case_media_3d,
          "tfsf_tm": case_tfsf_tm, "sim3d": case_sim3d, "raster": case_raster,
-         "shapes": case_shapes}[w]()
+         "shapes": case_shapes, "multisrc": case_multisrc}[w]()
         print(f"[golden] {w}: {time.perf_counter() - t0:.1f}s", flush=True)
     man = {}
     for fn in sorted(os.listdir(HERE)):

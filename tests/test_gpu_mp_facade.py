@@ -230,3 +230,76 @@ def test_shapes_run_matches_reference():
     assert np.linalg.norm(ez - G["Ez"]) <= 1e-10 * np.linalg.norm(G["Ez"])
     fx = np.asarray(mp.get_fluxes(fl))
     assert np.abs(fx - G["flux"]).max() <= 1e-9 * np.abs(G["flux"]).max()
+
+
+@pytest.mark.parametrize("hook", [False, True])
+def test_several_sources_match_reference(hook):
+    """Two sources in one facade run (a point and a line current with their
+    own pulses and amplitudes): the engine injects sum_i amp_i(t) mask_i each
+    step from a per-step weight stack (mp/cpp/core.py:584-589), and with a
+    step function splitting the run the spectra keep one running sum across
+    the engine calls, as the reference's per-step loop does.  Ez, the flux
+    spectrum and the probed values equal the reference Simulation's (golden
+    multisrc_f64) within FP64 rounding."""
+    import meep_torch_amd.mp as mp
+    G = golden("multisrc_f64")
+    sim = mp.Simulation(cell_size=mp.Vector3(6, 4), resolution=10,
+                        geometry=[mp.Block(size=mp.Vector3(mp.inf, 0.6), material=mp.Medium(epsilon=8.0))],
+                        sources=[mp.Source(mp.GaussianSource(0.8, fwidth=0.6), component=mp.Ez,
+                                           center=mp.Vector3(-2.0, 0.0)),
+                                 mp.Source(mp.GaussianSource(1.1, fwidth=0.5), component=mp.Ez,
+                                           center=mp.Vector3(-1.4, 0.5), size=mp.Vector3(0, 0.8), amplitude=-0.7)],
+                        boundary_layers=[mp.PML(0.5)], dtype=torch.float64, device="cuda")
+    fl = sim.add_flux(0.9, 0.5, 4, mp.FluxRegion(center=mp.Vector3(2.2, 0), size=mp.Vector3(0, 3)))
+    seen = []
+    if hook:
+        sim.run(mp.at_every(1.5, lambda s: seen.append(float(s.get_field_point(mp.Ez, mp.Vector3(1.0, 0))))),
+                until=10)
+    else:
+        sim.run(until=10)
+    tag = "hook" if hook else "plain"
+    ez = np.asarray(sim.get_array(component=mp.Ez))
+    assert np.linalg.norm(ez - G[f"{tag}_Ez"]) <= 1e-12 * np.linalg.norm(G[f"{tag}_Ez"])
+    fx = np.asarray(mp.get_fluxes(fl))
+    assert np.abs(fx - G[f"{tag}_flux"]).max() <= 1e-10 * np.abs(G[f"{tag}_flux"]).max()
+    if hook:
+        assert np.allclose(seen, G["hook_probe"], rtol=1e-10, atol=1e-16)
+
+
+@pytest.mark.parametrize("pol", ["tm", "te"])
+def test_decay_stop_in_window_sized_calls(pol, monkeypatch):
+    """stop_when_fields_decayed compares only at the end of its dt window
+    (mp/run_control.py:323-351 of the reference): the facade runs a whole
+    window per engine call with a device probe on the watched component, and
+    must stop at the same step, with the same fields and spectra, as the
+    same condition fed after every step (a wrapping callable forces that
+    path), in far fewer engine calls."""
+    import meep_torch_amd.mp as mp
+    from meep_torch_amd.core import engine2d
+    calls = {"n": 0}
+    real = engine2d.run_chunk
+
+    def counting(*a, **k):
+        calls["n"] += 1
+        return real(*a, **k)
+
+    monkeypatch.setattr(engine2d, "run_chunk", counting)
+    comp = mp.Ez if pol == "tm" else mp.Hz
+
+    def run(wrapped):
+        sim = mp.Simulation(cell_size=mp.Vector3(8, 4), resolution=10,
+                            geometry=[mp.Block(size=mp.Vector3(mp.inf, 0.6), material=mp.Medium(epsilon=9.0))],
+                            sources=[mp.Source(mp.GaussianSource(0.7, fwidth=0.4), component=comp,
+                                               center=mp.Vector3(-3.0, 0))],
+                            boundary_layers=[mp.PML(0.8)], dtype=torch.float64, device="cuda")
+        fl = sim.add_flux(0.7, 0.3, 3, mp.FluxRegion(center=mp.Vector3(3.0, 0), size=mp.Vector3(0, 3)))
+        cond = mp.stop_when_fields_decayed(4.0, comp, mp.Vector3(3.0, 0), 1e-3)
+        calls["n"] = 0
+        sim.run(until_after_sources=(lambda s: cond(s)) if wrapped else cond)
+        return sim.round_time(), np.asarray(sim.get_array(component=comp)), np.asarray(mp.get_fluxes(fl)), calls["n"]
+
+    t_w, f_w, x_w, n_w = run(True)
+    t_f, f_f, x_f, n_f = run(False)
+    assert t_f == t_w
+    assert np.array_equal(f_f, f_w) and np.array_equal(x_f, x_w)
+    assert n_f * 20 < n_w, (n_f, n_w)

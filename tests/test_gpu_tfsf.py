@@ -217,7 +217,7 @@ def test_batched_path_takes_magnetic_sources(dtype):
     m = build_tm_coeffs(g, eps_inf=torch.ones(NX, NY, dtype=dtype, device="cuda"), npml=NPML)
     srcs = [Huygens("tm", g.dt, dtype, "cuda"), Huygens("tm", g.dt, dtype, "cuda")]
     srcs[1].h_masks = {}
-    mk = lambda: [FluxMonitorTM(g, [0.8, 1.0, 1.2], x_normal=True, idx=150, lo=NPML, hi=NY - NPML)]
+    mk = lambda: [FluxMonitorTM(g, [0.8, 1.0, 1.2], x_normal=True, idx=90, lo=NPML, hi=NY - NPML)]
     n = 400
     bt = BatchedTM([(m, s, mk()) for s in srcs], n, checkpoint_every=0, adjoint=False)
     out = bt.forward([m.eps_inf, m.eps_inf])
