@@ -268,6 +268,22 @@ int mt_tm2d_run(const mt_tm2d* p, const mt_source2d* src, const double* coef,
                 const mt_monitor* mons, int n_mons, void* stage, long long stage_bytes,
                 int stage_steps, int step0, int k, void* stream);
 
+/* Allocation-free variant of mt_tm2d_run.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as mt_tm2d_run without the staging workspace: the
+ * DFT staging ring lives in the library's static device array of
+ * mt_static_stage_bytes() bytes (16 MiB), one per GPU, held until the
+ * process ends; a call whose monitors need more than that for one step fails
+ * with MT_EINVAL.  Nothing is allocated (the tiled variants still step
+ * through the caller's ping-pong field copies in p->alt, as mt_tm2d_run
+ * does).  Calls that overlap on one GPU share the array and race. */
+int mt_tm2d_run_static(const mt_tm2d* p, const mt_source2d* src, const double* coef,
+                       const mt_monitor* mons, int n_mons, int step0, int k, void* stream);
+
+/* Bytes of the static staging array behind mt_tm2d_run_static. */
+long long mt_static_stage_bytes(void);
+
 /* Same for 2D TE: _run_chunk driving step_te (meep_torch/core/stepper_te.py:178-190)
  * with FluxMonitorTE/FluxMonitorTEY/DFTFieldsMonitor (monitors.py:24-155). */
 int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const double* coef,

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "common.h"
@@ -767,3 +768,63 @@ extern "C" int mt_te2d_tiles(const mt_te2d* p, int* n_interior, int* n_ring) {
 }
 
 extern "C" float mt_div_recip_f32(float b) { return div_recip_f32(b); }
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant of mt_tm2d_run: the DFT staging ring lives in a
+// file-scope device array instead of a caller workspace.  Every GPU the
+// library runs on holds one copy of the array until the process ends; its
+// device address is looked up once per device (hipGetSymbolAddress) and
+// used for every launch -- the array's name is a host-side stand-in and is
+// never handed to a kernel.  The ring needs no clearing: each slot is
+// written by the gather kernel before the flush reads it, and the phase
+// table is rebuilt at every flush.  Calls that overlap on one GPU share the
+// array and race.
+// ---------------------------------------------------------------------------
+namespace mt {
+namespace {
+constexpr long long kStaticStageBytes = 16LL << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) char scratch[kStaticStageBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+
+int scratch_address(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(MT_EHIP, "static staging: no current device");
+  if (dev < 0 || dev >= kMaxDevices) return fail(MT_EINVAL, "static staging: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(MT_EHIP, std::string("static staging: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return MT_OK;
+}
+}  // namespace
+}  // namespace mt
+
+extern "C" long long mt_static_stage_bytes(void) { return mt::kStaticStageBytes; }
+
+extern "C" int mt_tm2d_run_static(const mt_tm2d* p, const mt_source2d* src, const double* coef,
+                                  const mt_monitor* mons, int n_mons, int step0, int k, void* stream) {
+  if (!p) return fail(MT_EINVAL, "null problem");
+  if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");
+  void* stage = nullptr;
+  long long bytes = 0;
+  int steps = 1;
+  if (n_mons > 0) {
+    if (!mons) return fail(MT_EINVAL, "null monitor array");
+    const long long per_step = mt_stage_bytes(mons, n_mons, 1, p->dtype);
+    if (per_step > kStaticStageBytes)
+      return fail(MT_EINVAL, "monitors need " + std::to_string(per_step) + " bytes of staging per step, above the " +
+                                 std::to_string(kStaticStageBytes) + "-byte static staging array");
+    steps = per_step > 0 ? (int)std::min<long long>(64, kStaticStageBytes / per_step) : 64;
+    bytes = mt_stage_bytes(mons, n_mons, steps, p->dtype);
+    int rc = scratch_address(&stage);
+    if (rc) return rc;
+  }
+  return mt_tm2d_run(p, src, coef, mons, n_mons, stage, bytes, steps, step0, k, stream);
+}

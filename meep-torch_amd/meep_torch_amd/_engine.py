@@ -157,6 +157,9 @@ def lib():
     L.mt_tm2d_tiles.argtypes = [P(TM2D), P(c_int), P(c_int)]
     L.mt_te2d_tiles.restype = c_int
     L.mt_te2d_tiles.argtypes = [P(TE2D), P(c_int), P(c_int)]
+    L.mt_tm2d_run_static.restype = c_int
+    L.mt_tm2d_run_static.argtypes = [P(TM2D), P(Source2D), P(c_double), P(Monitor), c_int, c_int, c_int, c_void_p]
+    L.mt_static_stage_bytes.restype = c_ll
     L.mt_div_recip_f32.restype = ctypes.c_float
     L.mt_div_recip_f32.argtypes = [ctypes.c_float]
     if L.mt_abi_version() != ABI_VERSION:

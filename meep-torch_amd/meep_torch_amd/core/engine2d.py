@@ -342,7 +342,7 @@ TE_ORDER = ("Hz", "Dx", "Dy", "psi_Eyx", "psi_Exy", "psi_Hzy", "psi_Hzx")
 
 
 def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=None,
-              raw=False, variant=None, alt=None):
+              raw=False, variant=None, alt=None, static_stage=False):
     """Advance ``fields`` (dict of device tensors, reference shapes) k steps.
     tape: optional tensor receiving E after every step: (k, nx, ny) Ez for TM,
     (k, (nx-1)*ny + nx*(ny-1)) rows [Ex | Ey] for TE.
@@ -414,6 +414,14 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
         d.hcoef = hcoef
     arr, outs, keep = monitor_structs(fields, list(monitors), k, dtype, device, reuse)
     n = len(monitors)
+    if static_stage:
+        # staging from the library's static device array (mt_tm2d_run_static)
+        if mode != "tm":
+            raise NotImplementedError("static staging is provided for TM runs")
+        rc = L.mt_tm2d_run_static(ctypes.byref(d), ctypes.byref(src), coef, arr, n, step0, k, E.stream_handle())
+        E.check(rc, "mt_tm2d_run_static")
+        del keep_c, wkeep, keep, hkeep, hcoef
+        return outs if raw else finish_outputs(outs, fields)
     S, stage, nbytes = stage_alloc(L, arr, n, code, device)
     fn = L.mt_tm2d_run if mode == "tm" else L.mt_te2d_run
     rc = fn(ctypes.byref(d), ctypes.byref(src), coef, arr, n,
