@@ -383,7 +383,9 @@ extern "C" mt_batch* mt_tm2d_batch_create(const mt_tm2d_batch_item* items, int n
     b->stage_bytes.push_back(items[v].stage_bytes);
   }
   int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
   b->grid_in = 4 * cus;
   b->grid_ring = 4 * cus;
   hipStream_t s = static_cast<hipStream_t>(stream);

@@ -31,9 +31,9 @@ int check_axis(const mt_pml_axis& ax, int n, const char* name);
 // A 2D source box inside an n0 x n1 field (or absent).
 int check_src(const mt_source2d* s, int n0, int n1);
 
-// RN(1/b) when div_c's fast sequence equals IEEE division for every FP32
-// dividend with divisor b (checked exhaustively on the host once per b and
-// cached), else 0; div_c then divides.
+// RN(1/b) when div_c's fast sequence equals IEEE division for every finite
+// FP32 dividend with divisor b (checked exhaustively on the host once per b
+// and cached), else 0; div_c then divides.
 float div_recip_f32(float b);
 template <typename T>
 inline T const_div_recip(T b);
@@ -44,26 +44,56 @@ inline double const_div_recip<double>(double) { return 0.0; }
 
 // a / b for the constant grid step b > 0 with y = RN(1/b): q0 = a y, then
 // one FMA correction with the exact remainder a - q0 b (Markstein's
-// sequence).  That this equals the IEEE quotient is not assumed from a
-// theorem: the host passes y only for FP32 divisors whose sequence it has
-// checked against IEEE division for every significand (const_div_recip),
-// and y = 0 otherwise (FP64 included), which selects true division for
-// every operand.  a = +-0 gives +-0 exactly (the correction would turn -0
-// into +0, so zero is selected explicitly); tiny or huge |a| takes the true
-// division.  Zero is common (fields ahead of a wavefront), so keeping it on
-// the fast path spares whole waves the divergent division branch.
+// sequence), the sign of a restored with copysign (a zero quotient takes a's
+// sign, as IEEE division gives it).  That this equals the IEEE quotient is
+// not assumed from a theorem: the host passes y only for FP32 divisors whose
+// sequence it has checked against IEEE division for every dividend of
+// magnitude in [2^-100, 2^100] and for zero (const_div_recip), and y = 0
+// otherwise (FP64 included), which selects true division.  Other dividends
+// (tiny values in the tail of a wavefront, huge ones) take true division.
+// Four instructions plus the exponent test, against about ten for IEEE
+// division (a reciprocal among them).
+__device__ __forceinline__ bool dx_div_covered(float a) {
+  return (unsigned)(__builtin_amdgcn_frexp_expf(a) + 99) <= 200u;  // 0 or 2^-100 <= |a| <= 2^100
+}
+
 template <typename T>
 __device__ __forceinline__ T div_c(T a, T b, T y) {
   if (y == T(0)) return a / b;
-  T aa = a < T(0) ? -a : a;
-  if ((aa >= T(0x1p-100) && aa <= T(0x1p100)) || aa == T(0)) {
-    T q0 = a * y;
-    T r = fma(-q0, b, a);
-    T q1 = fma(r, y, q0);
-    return aa == T(0) ? a : q1;
+  const T q0 = a * y;
+  const T r = fma(-q0, b, a);
+  T q = copysign(fma(r, y, q0), a);
+  if (!dx_div_covered(static_cast<float>(a))) {
+    __asm__ volatile("");  // a real branch: keeps the division off the common path
+    q = a / b;
   }
-  return a / b;
+  return q;
 }
+
+// The same for a group of quotients that the caller checks once: each call
+// takes the fast sequence and records its dividend's exponent; when exact()
+// is false for a lane, the caller recomputes that lane's whole group with
+// IeeeDiv (one branch per group instead of one per quotient).  Fast = false
+// (no verified y) divides every quotient.
+template <typename T, bool Fast>
+struct DxDiv {
+  T b, y;
+  unsigned worst;
+  __device__ __forceinline__ T operator()(T a) {
+    if constexpr (!Fast) return a / b;
+    const T q0 = a * y;
+    const T r = fma(-q0, b, a);
+    worst = max(worst, (unsigned)(__builtin_amdgcn_frexp_expf(static_cast<float>(a)) + 99));
+    return copysign(fma(r, y, q0), a);
+  }
+  __device__ __forceinline__ bool exact() const { return !Fast || worst <= 200u; }
+};
+
+template <typename T>
+struct IeeeDiv {
+  T b;
+  __device__ __forceinline__ T operator()(T a) const { return a / b; }
+};
 
 // Monitor support shared by the 2D and 3D steppers (dft.hip).
 struct MonitorPlan {

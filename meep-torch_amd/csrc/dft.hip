@@ -17,9 +17,14 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
+
+#include <xmmintrin.h>
 
 #include "common.h"
 #include "tiled2d.h"
@@ -42,26 +47,85 @@ int check_launch(const char* what) {
 }
 
 // Exhaustive host check of div_c's quotient sequence for one FP32 divisor b:
-// every significand of a in [1, 2).  The sequence scales exactly with a's
-// exponent while q0, r and q1 stay normal, which the range guards of div_c
-// (|a| in [2^-100, 2^100]) and of this function (b in [2^-16, 2^16]) ensure,
-// so passing here covers every operand div_c sends down the fast path.
+// with y = RN(1/b), q0 = RN(a y), r = fma(-q0, b, a), q = copysign(fma(r, y,
+// q0), a) must equal the IEEE quotient a / b for every dividend of magnitude
+// in [2^-100, 2^100] (all 1.68e9 positive bit patterns there) and for zero;
+// the sequence and the quotient are odd in a (copysign gives a zero quotient
+// a's sign), so the negative patterns follow.  The kernels send every other
+// dividend to true division.  Run once per divisor (cached), split over host
+// threads, vectorised with AVX2/FMA where the host has them; the threads
+// clear the FTZ/DAZ flags so that the check sees IEEE arithmetic.
+namespace {
+
+inline void ieee_denormals() { _mm_setcsr(_mm_getcsr() & ~0x8040u); }
+
+bool markstein_range_generic(uint32_t lo, uint32_t hi, float b, float y) {
+  ieee_denormals();
+  for (uint32_t w = lo; w < hi; ++w) {
+    float a;
+    std::memcpy(&a, &w, 4);
+    const float q = a / b;
+    const float q0 = a * y;
+    const float r = std::fma(-q0, b, a);
+    const float q1 = std::copysign(std::fma(r, y, q0), a);
+    if (std::memcmp(&q, &q1, 4) != 0) return false;
+  }
+  return true;
+}
+
+__attribute__((target("avx2,fma"))) bool markstein_range_fma(uint32_t lo, uint32_t hi, float b, float y) {
+  ieee_denormals();
+  constexpr uint32_t kBlock = 1u << 16;
+  for (uint32_t w0 = lo; w0 < hi; w0 += kBlock) {
+    const uint32_t w1 = std::min(hi, w0 + kBlock);
+    uint32_t bad = 0;
+    for (uint32_t w = w0; w < w1; ++w) {
+      float a;
+      std::memcpy(&a, &w, 4);
+      const float q = a / b;
+      const float q0 = a * y;
+      const float r = __builtin_fmaf(-q0, b, a);
+      const float q1 = __builtin_copysignf(__builtin_fmaf(r, y, q0), a);
+      uint32_t u, v;
+      std::memcpy(&u, &q, 4);
+      std::memcpy(&v, &q1, 4);
+      bad |= u ^ v;
+    }
+    if (bad) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
 static bool div_const_exact_f32(float b) {
   if (!(b >= 0x1p-16f && b <= 0x1p16f)) return false;
   volatile float vb = b;
   const float y = 1.0f / vb;
-  for (uint32_t m = 0; m < (1u << 23); ++m) {
-    uint32_t w = (127u << 23) | m;
-    float a;
-    std::memcpy(&a, &w, 4);
-    volatile float va = a;
-    const float q = va / vb;
-    const float q0 = a * y;
-    const float r = std::fma(-q0, b, a);
-    const float q1 = std::fma(r, y, q0);
-    if (std::memcmp(&q, &q1, 4) != 0) return false;
-  }
-  return true;
+  const bool hw = __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
+  constexpr uint32_t kLo = 27u << 23, kHi = (227u << 23) + 1;  // bit patterns of 2^-100 .. 2^100 inclusive
+  const bool zero_ok = hw ? markstein_range_fma(0, 1, b, y) : markstein_range_generic(0, 1, b, y);
+  if (!zero_ok) return false;
+  unsigned nt = std::thread::hardware_concurrency();
+  nt = std::max(1u, std::min(16u, nt));
+  std::atomic<bool> ok{true};
+  std::atomic<uint32_t> next{0};
+  constexpr uint32_t kChunk = 1u << 24;
+  auto worker = [&]() {
+    for (;;) {
+      const uint32_t c = next.fetch_add(1);
+      const uint64_t lo = kLo + (uint64_t)c * kChunk;
+      if (lo >= kHi || !ok.load()) return;
+      const uint32_t hi = (uint32_t)std::min<uint64_t>(kHi, lo + kChunk);
+      const bool pass = hw ? markstein_range_fma((uint32_t)lo, hi, b, y) : markstein_range_generic((uint32_t)lo, hi, b, y);
+      if (!pass) ok.store(false);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  return ok.load();
 }
 
 float div_recip_f32(float b) {

@@ -28,6 +28,7 @@ template <typename T>
 struct K3 {
   int nx, ny, nz, lean;
   T dt, dx;
+  T rdx;  // RN(1/dx) when division by dx may take div_c's fast sequence, else 0
   const T* eps;
   Pml<T> h[3], e[3];
   T *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;      // read: E, and old H
@@ -83,7 +84,7 @@ __device__ __forceinline__ T ce_apply(const CeIn<T>& c, T* psi, long long q, T d
 template <typename T, bool M>
 __device__ __forceinline__ void h3_cell(const K3<T>& p, int i, int j, int k) {
   const int nx = p.nx, ny = p.ny, nz = p.nz;
-  const T dx = p.dx, dt = p.dt;
+  const T dx = p.dx, dt = p.dt, rdx = p.rdx;
   const bool fx = j < ny - 1 && k < nz - 1;                    // Hx (nx, ny-1, nz-1)
   const bool fy = i < nx - 1 && k < nz - 1;                    // Hy (nx-1, ny, nz-1)
   const bool fz = i < nx - 1 && j < ny - 1;                    // Hz (nx-1, ny-1, nz)
@@ -125,18 +126,18 @@ __device__ __forceinline__ void h3_cell(const K3<T>& p, int i, int j, int k) {
   }
   // ---- arithmetic and stores (stepper_3d.py:363-390)
   if (fx) {
-    T cy = ce_apply(cxy, p.psi[P_EZ_Y], ix, (ezj - ez0) / dx);
-    T cz = ce_apply(cxz, p.psi[P_EY_Z], ix, (eyk - ey0) / dx);
+    T cy = ce_apply(cxy, p.psi[P_EZ_Y], ix, div_c(ezj - ez0, dx, rdx));
+    T cz = ce_apply(cxz, p.psi[P_EY_Z], ix, div_c(eyk - ey0, dx, rdx));
     p.oHx[ix] = M ? d1x * hx - d2x * (cy - cz) : hx - dt * (cy - cz);
   }
   if (fy) {
-    T cz = ce_apply(cyz, p.psi[P_EX_Z], iy, (exk - ex0) / dx);
-    T cx = ce_apply(cyx, p.psi[P_EZ_X], iy, (ezx - ez0) / dx);
+    T cz = ce_apply(cyz, p.psi[P_EX_Z], iy, div_c(exk - ex0, dx, rdx));
+    T cx = ce_apply(cyx, p.psi[P_EZ_X], iy, div_c(ezx - ez0, dx, rdx));
     p.oHy[iy] = M ? d1y * hy - d2y * (cz - cx) : hy - dt * (cz - cx);
   }
   if (fz) {
-    T cx = ce_apply(czx, p.psi[P_EY_X], iz, (eyx - ey0) / dx);
-    T cy = ce_apply(czy, p.psi[P_EX_Y], iz, (exj - ex0) / dx);
+    T cx = ce_apply(czx, p.psi[P_EY_X], iz, div_c(eyx - ey0, dx, rdx));
+    T cy = ce_apply(czy, p.psi[P_EX_Y], iz, div_c(exj - ex0, dx, rdx));
     p.oHz[iz] = M ? d1z * hz - d2z * (cx - cy) : hz - dt * (cx - cy);
   }
 }
@@ -201,7 +202,7 @@ template <typename T, bool M>
 __device__ __forceinline__ void e3_cell(const K3<T>& p, const mt_source3d& src, T c, T* tape, int i, int j,
                                         int k) {
   const int nx = p.nx, ny = p.ny, nz = p.nz;
-  const T dx = p.dx;
+  const T dx = p.dx, rdx = p.rdx;
   const bool ax = i < nx - 1, ay = j < ny - 1, az = k < nz - 1;  // Ex, Ey, Ez exist at this cell
   const bool ix = i >= 1 && ax, jy = j >= 1 && ay, kz = k >= 1 && az;
   const long long ce_ = ((long long)i * ny + j) * nz + k;                 // eps, Ex
@@ -249,8 +250,8 @@ __device__ __forceinline__ void e3_cell(const K3<T>& p, const mt_source3d& src, 
   // ---- arithmetic and stores (stepper_3d.py:392-445)
   if (ax) {  // Ex (nx-1, ny, nz)
     T chy = T(0), chz = T(0);
-    if (jy) chy = ce_apply(cxy, p.psi[P_HZ_Y], q_xy, (hz0 - hzj) / dx);
-    if (kz) chz = ce_apply(cxz, p.psi[P_HY_Z], q_xz, (hy0 - hyk) / dx);
+    if (jy) chy = ce_apply(cxy, p.psi[P_HZ_Y], q_xy, div_c(hz0 - hzj, dx, rdx));
+    if (kz) chz = ce_apply(cxz, p.psi[P_HY_Z], q_xz, div_c(hy0 - hyk, dx, rdx));
     T curl = (jy && kz) ? chy - chz : T(0);
     T epsc = T(0.5) * (eps0 + epx);
     T E = M ? e_media(p, 0, ce_, (long long)(nx - 1) * ny * nz, ex, epsc, curl) : e_update(p, ex, epsc, curl);
@@ -260,8 +261,8 @@ __device__ __forceinline__ void e3_cell(const K3<T>& p, const mt_source3d& src, 
   }
   if (ay) {  // Ey (nx, ny-1, nz)
     T chz = T(0), chx = T(0);
-    if (kz) chz = ce_apply(cyz, p.psi[P_HX_Z], q_yz, (hx0 - hxk) / dx);
-    if (ix) chx = ce_apply(cyx, p.psi[P_HZ_X], q_yx, (hz0 - hzx) / dx);
+    if (kz) chz = ce_apply(cyz, p.psi[P_HX_Z], q_yz, div_c(hx0 - hxk, dx, rdx));
+    if (ix) chx = ce_apply(cyx, p.psi[P_HZ_X], q_yx, div_c(hz0 - hzx, dx, rdx));
     T curl = (kz && ix) ? chz - chx : T(0);
     T epsc = T(0.5) * (eps0 + epy);
     T E = M ? e_media(p, 1, iy, (long long)nx * (ny - 1) * nz, ey, epsc, curl) : e_update(p, ey, epsc, curl);
@@ -271,8 +272,8 @@ __device__ __forceinline__ void e3_cell(const K3<T>& p, const mt_source3d& src, 
   }
   if (az) {  // Ez (nx, ny, nz-1)
     T chx = T(0), chy = T(0);
-    if (ix) chx = ce_apply(czx, p.psi[P_HY_X], q_zx, (hy0 - hyx) / dx);
-    if (jy) chy = ce_apply(czy, p.psi[P_HX_Y], q_zy, (hx0 - hxj) / dx);
+    if (ix) chx = ce_apply(czx, p.psi[P_HY_X], q_zx, div_c(hy0 - hyx, dx, rdx));
+    if (jy) chy = ce_apply(czy, p.psi[P_HX_Y], q_zy, div_c(hx0 - hxj, dx, rdx));
     T curl = (ix && jy) ? chx - chy : T(0);
     T epsc = T(0.5) * (eps0 + epz);
     T E = M ? e_media(p, 2, iz, (long long)nx * ny * (nz - 1), ez, epsc, curl) : e_update(p, ez, epsc, curl);
@@ -376,46 +377,13 @@ __device__ __forceinline__ void lds_barrier() {
   __asm__ volatile("" ::: "memory");
 }
 
-// Raw buffer access to one x plane of a field (n elements): a load whose
-// offset is out of range reads 0 and such a store is dropped, so edge cells
-// and halo lanes need no branch around their loads and stores, and the
-// compiler counts the loads in flight exactly (no wait for all of them at a
-// join of paths that issued different numbers).
-constexpr int kOutOfRange = static_cast<int>(0x80000000u);
-
-template <typename T>
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const T* plane, long long n) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(plane), 0, static_cast<int>(n * (long long)sizeof(T)),
-                                           0x00020000);
-}
-
-template <typename T>
-__device__ __forceinline__ T plane_load(const T* plane, long long n, int idx, bool ok) {
-  const auto r = plane_rsrc(plane, n);
-  const int off = ok ? idx * (int)sizeof(T) : kOutOfRange;
-  if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-  else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-
-template <typename T>
-__device__ __forceinline__ void plane_store(T* plane, long long n, int idx, bool ok, T v) {
-  const auto r = plane_rsrc(plane, n);
-  const int off = ok ? idx * (int)sizeof(T) : kOutOfRange;
-  if constexpr (sizeof(T) == 4) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
-  } else {
-    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, 0);
-  }
-}
-
 template <typename T>
 struct I3 {
   int nx, ny, nz, lean;
   int I0, I1, J0, J1, K0, K1;   // interior box
   int kbase, njt, nkt, iseg;    // k tiles from kbase (64-aligned); j tiles from J0; x segments of iseg planes
   int ntiles, per_xcd;
-  T dt, dx;
+  T dt, dx, rdx;
   const T* eps;
   const T *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;   // set A (read)
   T *oEx, *oEy, *oEz, *oHx, *oHy, *oHz;   // set B (written)
@@ -426,7 +394,7 @@ struct Col {          // one field column value set
   T x, y, z;
 };
 
-template <typename T, int TJ>
+template <typename T, int TJ, bool FD>
 __global__ __launch_bounds__(64 * (TJ + 2)) void y3_interior(I3<T> p, mt_source3d src, T c_src, T* tape) {
   constexpr int NR = TJ + 2;
   __shared__ T sE[4][3][NR][66];      // E planes i-1 .. i+2 (ring of 4)
@@ -441,57 +409,80 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void y3_interior(I3<T> p, mt_source3
   const int jt = t % p.njt, r2 = t / p.njt, kt = r2 % p.nkt, sg = r2 / p.nkt;
   const int j0 = p.J0 + jt * TJ, k0 = p.kbase + kt * 64;
   const int ib = p.I0 + sg * p.iseg, ie = min(p.I1, ib + p.iseg);
-  const int lane = threadIdx.x, wr = (int)threadIdx.y - 1;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
+  // One wave per row, so the row and everything derived from it is uniform
+  // across the wave (scalar registers and branches); only k varies by lane.
+  const int lane = threadIdx.x;
+  const int wr = __builtin_amdgcn_readfirstlane((int)threadIdx.y) - 1;   // -1 .. TJ
   const int j = j0 + wr, k = k0 + lane;
-  const bool own = wr >= 0 && wr < TJ;
-  const bool jv = j >= 0 && j < ny, kv = k < nz;
+  const bool own = wr >= 0 && wr < TJ;                        // rows j0 .. j0+TJ-1
+  const bool wantH = wr < TJ;                                 // own rows and the halo row j0-1
   const bool active = own && j < p.J1 && k >= p.K0 && k < p.K1;
-  const bool haloL = lane == 0 && k0 >= p.K0;           // column k0-1 feeds active cells
-  const bool haloR = lane == 63 && k0 + 64 < nz;        // column k0+64
-  const bool wantH = own || wr == -1;
-  const int r = wr + 1, c = lane + 1;                   // LDS coordinates of the own cell
-  // plane strides and row offsets
-  const long long sEx = (long long)ny * nz, sEy = (long long)(ny - 1) * nz, sEz = (long long)ny * (nz - 1);
-  const long long sHx = (long long)(ny - 1) * (nz - 1), sHy = sEz, sHz = sEy;
-  const int oX = j * nz + k, oY = j * nz + k, oZ = j * (nz - 1) + k, oHX = j * (nz - 1) + k;
-  const T dx = p.dx, dt = p.dt;
+  const bool haloL = k0 >= p.K0;                              // column k0-1 feeds active cells
+  const int r = wr + 1, c = lane + 1;                         // LDS coordinates of the own cell
+  const T dx = p.dx, dt = p.dt, rdx = p.rdx;
 
-  auto loadE = [&](int ip, int kk, bool ok) {
-    const bool v = ok && jv && kk >= 0 && kk < nz;
-    return Col<T>{plane_load(p.Ex + ip * sEx, sEx, j * nz + kk, v && ip < nx - 1),
-                  plane_load(p.Ey + ip * sEy, sEy, j * nz + kk, v && j < ny - 1),
-                  plane_load(p.Ez + ip * sEz, sEz, j * (nz - 1) + kk, v && kk < nz - 1)};
+  // Every load is in bounds without a mask: row, column and plane indices are
+  // clamped into the grid.  The interior box keeps every neighbour an active
+  // cell reads inside the grid, so a clamped (duplicate) value only ever
+  // reaches cells that are not stored.  Lanes 0 and 63 read the halo columns
+  // k0-1 and k0+64 with the same instruction that reads the other lanes' own
+  // column.  Offsets are byte offsets within one x plane.
+  const int jE = min(max(j, 0), ny - 1), jY = min(max(j, 0), ny - 2);
+  const int kE = min(k, nz - 1), kZ = min(k, nz - 2);
+  const int kh = lane == 0 ? max(k0 - 1, 0) : (lane == 63 ? min(k0 + 64, nz - 1) : kE);
+  const int khZ = min(kh, nz - 2);
+  constexpr unsigned S = sizeof(T);
+  const unsigned oEx = (unsigned)(jE * nz + kE) * S, oEy = (unsigned)(jY * nz + kE) * S;  // also eps; Hz
+  const unsigned oEz = (unsigned)(jE * (nz - 1) + kZ) * S;                                // also Hy
+  const unsigned oHx = (unsigned)(jY * (nz - 1) + kZ) * S;
+  const unsigned hEx = (unsigned)(jE * nz + kh) * S, hEy = (unsigned)(jY * nz + kh) * S;  // also eps
+  const unsigned hEz = (unsigned)(jE * (nz - 1) + khZ) * S;                               // also Hy
+  const unsigned hHx = (unsigned)(jY * (nz - 1) + khZ) * S;
+  const long long pE = (long long)ny * nz, pY = (long long)(ny - 1) * nz, pZ = (long long)ny * (nz - 1),
+                  pX = (long long)(ny - 1) * (nz - 1);
+  auto at = [](const T* base, long long plane, int ip, unsigned off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base + ip * plane) + off);
   };
-  auto loadEps = [&](int ip, int kk, bool ok) {
-    return plane_load(p.eps + ip * sEx, sEx, j * nz + kk, ok && wr >= 0 && jv && kk < nz);
+  auto put = [](T* base, long long plane, int ip, unsigned off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base + ip * plane) + off) = v;
   };
-  auto loadH = [&](int ip, bool ok) {
-    const bool v = ok && wantH && jv && kv && ip < nx - 1;
-    return Col<T>{plane_load(p.Hx + ip * sHx, sHx, oHX, v && j < ny - 1 && k < nz - 1),
-                  plane_load(p.Hy + ip * sHy, sHy, oZ, v && own && k < nz - 1),
-                  plane_load(p.Hz + ip * sHz, sHz, oY, v && j < ny - 1)};
+
+  struct Stage {
+    Col<T> e, eh, h;   // E own column, E halo columns, old H own column
+    T eps, epsh;       // eps own column, eps halo column (lane 63)
+    T hlx, hly;        // old Hx, Hy of column k0-1 (lane 0)
   };
-  auto loadHL = [&](int ip, bool ok) {   // Hx, Hy at (ip, j, k0-1) for the halo column
-    const bool v = ok && haloL && own && jv && ip < nx - 1;
-    const int q = j * (nz - 1) + k0 - 1;
-    return Col<T>{plane_load(p.Hx + ip * sHx, sHx, q, v && j < ny - 1), plane_load(p.Hy + ip * sHy, sHy, q, v),
-                  T(0)};
+  // the stage iteration i consumes: E(i+1), eps(i), H(i); planes past the
+  // segment are clamped into the grid and never used
+  auto fetch = [&](Stage& st, int i) {
+    const int pe = min(i + 1, nx - 1), pex = min(i + 1, nx - 2), pp = min(i, nx - 1), ph = min(i, nx - 2);
+    st.e.x = at(p.Ex, pE, pex, oEx);
+    st.e.y = at(p.Ey, pY, pe, oEy);
+    st.e.z = at(p.Ez, pZ, pe, oEz);
+    st.eh.x = at(p.Ex, pE, pex, hEx);
+    st.eh.y = at(p.Ey, pY, pe, hEy);
+    st.eh.z = at(p.Ez, pZ, pe, hEz);
+    st.eps = at(p.eps, pE, pp, oEx);
+    st.epsh = at(p.eps, pE, pp, hEx);
+    st.h.x = at(p.Hx, pX, ph, oHx);
+    st.h.y = at(p.Hy, pZ, ph, oEz);
+    st.h.z = at(p.Hz, pY, ph, oEy);
+    st.hlx = at(p.Hx, pX, ph, hHx);
+    st.hly = at(p.Hy, pZ, ph, hEz);
   };
-  // E of plane ip (own cell, lane 0's column k0-1, lane 63's column k0+64) into ring slot ip & 3
-  auto putE = [&](int ip, const Col<T>& e, const Col<T>& l, const Col<T>& rr) {
+  auto putE = [&](int ip, const Col<T>& e, const Col<T>& eh) {
     const int b = ip & 3;
     sE[b][0][r][c] = e.x;
     sE[b][1][r][c] = e.y;
     sE[b][2][r][c] = e.z;
     if (lane == 0) {
-      sE[b][0][r][0] = l.x;
-      sE[b][1][r][0] = l.y;
-      sE[b][2][r][0] = l.z;
-    }
-    if (lane == 63) {
-      sE[b][0][r][65] = rr.x;
-      sE[b][1][r][65] = rr.y;
+      sE[b][0][r][0] = eh.x;
+      sE[b][1][r][0] = eh.y;
+      sE[b][2][r][0] = eh.z;
+    } else if (lane == 63) {
+      sE[b][0][r][65] = eh.x;
+      sE[b][1][r][65] = eh.y;
     }
   };
 
@@ -500,22 +491,12 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void y3_interior(I3<T> p, mt_source3
   // iterations earlier, and E(i+3), eps(i+2) and the old H(i+2) are loaded.
   // Three register stages rotate by name (the loop is unrolled by three), so
   // no value is copied out of a load still in flight: a copy would wait for it.
-  struct Stage {
-    Col<T> e, l, rr, h, hl;
-    T eps, epsR;
-  };
-  auto fetch = [&](Stage& st, int i) {  // the stage that iteration i consumes
-    const bool e_ok = i + 1 <= ie, eps_ok = i <= ie, h_ok = i < ie;
-    st.e = loadE(i + 1, k, e_ok);
-    st.l = loadE(i + 1, k0 - 1, e_ok && lane == 0);
-    st.rr = loadE(i + 1, k0 + 64, e_ok && haloR);
-    st.eps = loadEps(i, k, eps_ok);
-    st.epsR = loadEps(i, k0 + 64, eps_ok && haloR && wr < TJ);
-    st.h = loadH(i, h_ok);
-    st.hl = loadHL(i, h_ok);
-  };
   const int ip0 = ib - 1;
-  putE(ip0, loadE(ip0, k, true), loadE(ip0, k0 - 1, lane == 0), loadE(ip0, k0 + 64, haloR));
+  {
+    const int pex = min(ip0, nx - 2);
+    putE(ip0, Col<T>{at(p.Ex, pE, pex, oEx), at(p.Ey, pY, ip0, oEy), at(p.Ez, pZ, ip0, oEz)},
+         Col<T>{at(p.Ex, pE, pex, hEx), at(p.Ey, pY, ip0, hEy), at(p.Ez, pZ, ip0, hEz)});
+  }
   Stage S0, S1, S2;
   fetch(S0, ip0);
   fetch(S1, ip0 + 1);
@@ -523,10 +504,10 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void y3_interior(I3<T> p, mt_source3
 
   auto body = [&](int i, const Stage& cur, Stage& ahead) {
     // ---- into LDS: E(i+1), eps(i)
-    putE(i + 1, cur.e, cur.l, cur.rr);
+    putE(i + 1, cur.e, cur.eh);
     if (wr >= 0) {
       sEps[i % 3][wr][lane] = cur.eps;
-      if (lane == 63 && wr < TJ) sEps[i % 3][wr][64] = cur.epsR;
+      if (lane == 63 && wr < TJ) sEps[i % 3][wr][64] = cur.epsh;
     }
     // ---- loads two planes ahead
     fetch(ahead, i + 2);
@@ -539,83 +520,72 @@ __global__ __launch_bounds__(64 * (TJ + 2)) void y3_interior(I3<T> p, mt_source3
       const T ezj = sE[b0][2][r + 1][c], eyk = sE[b0][1][r][c + 1];
       const T exk = sE[b0][0][r][c + 1], exj = sE[b0][0][r + 1][c];
       const T eyx = sE[b1][1][r][c], ezx = sE[b1][2][r][c];
-      {  // Hx' (stepper_3d.py:371-375)
-        T cy = (ezj - ez0) / dx;
-        T cz = (eyk - ey0) / dx;
-        hn.x = cur.h.x - dt * (cy - cz);
-      }
-      if (own) {  // Hy'
-        T cz = (exk - ex0) / dx;
-        T cx = (ezx - ez0) / dx;
-        hn.y = cur.h.y - dt * (cz - cx);
-      }
-      {  // Hz'
-        T cx = (eyx - ey0) / dx;
-        T cy = (exj - ex0) / dx;
-        hn.z = cur.h.z - dt * (cx - cy);
+      // stepper_3d.py:371-375 (Hx'), and the same for Hy', Hz'
+      auto hcalc = [&](auto& D) {
+        Col<T> h;
+        h.x = cur.h.x - dt * (D(ezj - ez0) - D(eyk - ey0));
+        h.y = own ? cur.h.y - dt * (D(exk - ex0) - D(ezx - ez0)) : T(0);
+        h.z = cur.h.z - dt * (D(eyx - ey0) - D(exj - ex0));
+        return h;
+      };
+      DxDiv<T, FD> D{dx, rdx, 0u};
+      hn = hcalc(D);
+      if (!D.exact()) {
+        __asm__ volatile("");
+        IeeeDiv<T> I{dx};
+        hn = hcalc(I);
       }
       const int bH = i & 1;
       sH[bH][0][r][c] = hn.x;
       sH[bH][1][r][c] = hn.y;
       sH[bH][2][r][c] = hn.z;
-      if (haloL && own) {  // column k0-1
+      if (own && haloL && lane == 0) {  // column k0-1
         const T hez0 = sE[b0][2][r][0], hey0 = sE[b0][1][r][0], hex0 = sE[b0][0][r][0];
-        T cy = (sE[b0][2][r + 1][0] - hez0) / dx;
-        T cz = (sE[b0][1][r][1] - hey0) / dx;
-        sH[bH][0][r][0] = cur.hl.x - dt * (cy - cz);
-        T cz2 = (sE[b0][0][r][1] - hex0) / dx;
-        T cx2 = (sE[b1][2][r][0] - hez0) / dx;
-        sH[bH][1][r][0] = cur.hl.y - dt * (cz2 - cx2);
+        const T a0 = sE[b0][2][r + 1][0] - hez0, a1 = sE[b0][1][r][1] - hey0;
+        const T a2 = sE[b0][0][r][1] - hex0, a3 = sE[b1][2][r][0] - hez0;
+        sH[bH][0][r][0] = cur.hlx - dt * (div_c(a0, dx, rdx) - div_c(a1, dx, rdx));
+        sH[bH][1][r][0] = cur.hly - dt * (div_c(a2, dx, rdx) - div_c(a3, dx, rdx));
       }
-    }
-    {
-      const bool st = active && i >= ib && i < ie;
-      plane_store(p.oHx + i * sHx, sHx, oHX, st, hn.x);
-      plane_store(p.oHy + i * sHy, sHy, oZ, st, hn.y);
-      plane_store(p.oHz + i * sHz, sHz, oY, st, hn.z);
+      if (active && i >= ib) {
+        put(p.oHx, pX, i, oHx, hn.x);
+        put(p.oHy, pZ, i, oEz, hn.y);
+        put(p.oHz, pY, i, oEy, hn.z);
+      }
     }
     // ---- E'(i-1) of the own cell (stepper_3d.py:392-437)
-    const int im = i - 1;
-    T ex = T(0), ey = T(0), ez = T(0);
     if (i > ib && active) {
-      const int bH = im & 1, bP = im % 3, bE = im & 3;
+      const int im = i - 1, bH = im & 1, bP = im % 3, bE = im & 3;
       const T epsm = sEps[bP][wr][lane], eps1 = sEps[i % 3][wr][lane];
-      {
-        T chy = (hp.z - sH[bH][2][r - 1][c]) / dx;
-        T chz = (hp.y - sH[bH][1][r][c - 1]) / dx;
-        T curl = chy - chz;
-        T epsc = T(0.5) * (epsm + eps1);
-        ex = e_update(p, sE[bE][0][r][c], epsc, curl);
-        ex = inject(p, src, 0, im, j, k, ex, epsc, c_src);
+      const T hzj = sH[bH][2][r - 1][c], hyk = sH[bH][1][r][c - 1];
+      const T hxk = sH[bH][0][r][c - 1], hxj = sH[bH][0][r - 1][c];
+      auto curls = [&](auto& D) {
+        return Col<T>{D(hp.z - hzj) - D(hp.y - hyk), D(hp.x - hxk) - D(hp.z - hpp.z),
+                      D(hp.y - hpp.y) - D(hp.x - hxj)};
+      };
+      DxDiv<T, FD> D{dx, rdx, 0u};
+      Col<T> cv = curls(D);
+      if (!D.exact()) {
+        __asm__ volatile("");
+        IeeeDiv<T> I{dx};
+        cv = curls(I);
       }
-      {
-        T chz = (hp.x - sH[bH][0][r][c - 1]) / dx;
-        T chx = (hp.z - hpp.z) / dx;
-        T curl = chz - chx;
-        T epsc = T(0.5) * (epsm + sEps[bP][wr + 1][lane]);
-        ey = e_update(p, sE[bE][1][r][c], epsc, curl);
-        ey = inject(p, src, 1, im, j, k, ey, epsc, c_src);
-      }
-      {
-        T chx = (hp.y - hpp.y) / dx;
-        T chy = (hp.x - sH[bH][0][r - 1][c]) / dx;
-        T curl = chx - chy;
-        T epsc = T(0.5) * (epsm + sEps[bP][wr][lane + 1]);
-        ez = e_update(p, sE[bE][2][r][c], epsc, curl);
-        ez = inject(p, src, 2, im, j, k, ez, epsc, c_src);
-      }
-    }
-    {
-      const bool st = i > ib && active;
-      plane_store(p.oEx + im * sEx, sEx, oX, st, ex);
-      plane_store(p.oEy + im * sEy, sEy, oY, st, ey);
-      plane_store(p.oEz + im * sEz, sEz, oZ, st, ez);
-      if (tape) {
-        T* ty = tape + (long long)(nx - 1) * sEx;
-        T* tz = ty + (long long)nx * sEy;
-        plane_store(tape + im * sEx, sEx, oX, st, ex);
-        plane_store(ty + im * sEy, sEy, oY, st, ey);
-        plane_store(tz + im * sEz, sEz, oZ, st, ez);
+      T epsc = T(0.5) * (epsm + eps1);
+      T ex = e_update(p, sE[bE][0][r][c], epsc, cv.x);
+      ex = inject(p, src, 0, im, j, k, ex, epsc, c_src);
+      epsc = T(0.5) * (epsm + sEps[bP][wr + 1][lane]);
+      T ey = e_update(p, sE[bE][1][r][c], epsc, cv.y);
+      ey = inject(p, src, 1, im, j, k, ey, epsc, c_src);
+      epsc = T(0.5) * (epsm + sEps[bP][wr][lane + 1]);
+      T ez = e_update(p, sE[bE][2][r][c], epsc, cv.z);
+      ez = inject(p, src, 2, im, j, k, ez, epsc, c_src);
+      put(p.oEx, pE, im, oEx, ex);
+      put(p.oEy, pY, im, oEy, ey);
+      put(p.oEz, pZ, im, oEz, ez);
+      if (tape) {  // [Ex | Ey | Ez] of this step
+        T* ty = tape + (long long)(nx - 1) * pE;
+        put(tape, pE, im, oEx, ex);
+        put(ty, pY, im, oEy, ey);
+        put(ty + (long long)nx * pY, pZ, im, oEz, ez);
       }
     }
     hpp = hp;
@@ -738,6 +708,7 @@ int run3_split(const mt_yee3d* d, K3<T> p, const mt_source3d& sv, const double* 
   q.per_xcd = (q.ntiles + 7) / 8;
   q.dt = static_cast<T>(d->dt);
   q.dx = static_cast<T>(d->dx);
+  q.rdx = const_div_recip<T>(q.dx);
   q.eps = static_cast<const T*>(d->eps);
   const Pieces P = shell_pieces(b, nx, ny, nz);
   const long long box_cells = (long long)(b.I1 - b.I0) * (b.J1 - b.J0) * (b.K1 - b.K0);
@@ -757,7 +728,10 @@ int run3_split(const mt_yee3d* d, K3<T> p, const mt_source3d& sv, const double* 
     const T c = sv.kind ? static_cast<T>(coef[s_]) : T(0);
     T* slot = d->tape ? static_cast<T*>(d->tape) + (long long)s_ * e_cells : nullptr;
     timing_begin(s);
-    hipLaunchKernelGGL((y3_interior<T, TJ>), dim3(8 * q.per_xcd), dim3(64, TJ + 2), 0, s, q, sv, c, slot);
+    if (q.rdx != T(0))
+      hipLaunchKernelGGL((y3_interior<T, TJ, true>), dim3(8 * q.per_xcd), dim3(64, TJ + 2), 0, s, q, sv, c, slot);
+    else
+      hipLaunchKernelGGL((y3_interior<T, TJ, false>), dim3(8 * q.per_xcd), dim3(64, TJ + 2), 0, s, q, sv, c, slot);
     timing_end(s, box_cells);
     if (P.total) {
       hipLaunchKernelGGL(h3_pieces<T>, dim3(P.total), dim3(256), 0, s, p, P);
@@ -792,6 +766,7 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
   p.lean = d->lean;
   p.dt = static_cast<T>(d->dt);
   p.dx = static_cast<T>(d->dx);
+  p.rdx = const_div_recip<T>(p.dx);
   p.eps = static_cast<const T*>(d->eps);
   for (int a = 0; a < 3; ++a) {
     p.h[a] = pml_of<T>(d->h[a]);
@@ -817,7 +792,8 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
   mt_source3d sv{};
   if (src && src->kind) sv = *src;
   const Box3 box = interior_box(d);
-  const bool split = d->variant >= 1 && !media && box.I1 > box.I0 && box.J1 > box.J0 && box.K1 > box.K0;
+  const bool small = (long long)d->ny * d->nz * (long long)sizeof(T) < (1LL << 31);  // 32-bit in-plane offsets
+  const bool split = d->variant >= 1 && !media && small && box.I1 > box.I0 && box.J1 > box.J0 && box.K1 > box.K0;
   // tile rows: FP32 6 / 10 / 4, FP64 4 (the LDS rings must fit 64 KB)
   if (split) {
     if constexpr (sizeof(T) == 4) {

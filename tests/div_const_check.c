@@ -38,6 +38,16 @@ int main(int argc, char** argv) {
       }
     }
   }
+  /* below 2^-100 the sequence is not exact for every dividend (the kernels
+     send those to true division): b = 0.05 fails at a = 0x1.999a78p-128 */
+  {
+    volatile float b = 0.05f, a = 0x1.999a78p-128f;
+    float y = 1.0f / b, q = a / b, q0 = a * y, r = fmaf(-q0, b, a), q1 = fmaf(r, y, q0);
+    if (memcmp(&q, &q1, 4) == 0) {
+      printf("expected a mismatch below 2^-100\n");
+      return 2;
+    }
+  }
   printf("tried %llu, mismatches %llu\n", tried, bad);
   return bad != 0;
 }

@@ -2,15 +2,18 @@
 meep-torch_amd/csrc/common.h) against IEEE float division, on the CPU.
 
 The library passes the kernels y = RN(1/dx) only for an FP32 divisor whose
-quotient sequence it has checked against IEEE division for every significand
-(mt_div_recip_f32, cached per divisor), and 0 otherwise, which makes the
-kernels divide.  Checked here: the library's answer for the grid steps the
-configurations and the facade's usual resolutions give, its refusal outside
-the range where the check is complete, and -- independently -- a C program
-(tests/div_const_check.c, gcc, libm's FMA) that tries every significand at 23
-exponents across [2^-100, 2^100] and both signs.  The torch FP32 reference
-divides by dx with true division, so bit-parity of the tiled kernels rests
-on this."""
+quotient sequence it has checked against IEEE division for every dividend of
+magnitude in [2^-100, 2^100] and for zero (mt_div_recip_f32: all 1.68e9
+positive bit patterns, threaded, cached per divisor), and 0 otherwise, which
+makes the kernels divide; the kernels send any other dividend (tiny tails of
+a wavefront, huge values) to true division.  Checked here: the library's
+answer for the grid steps the configurations and the facade's usual
+resolutions give, its refusal outside the divisor range, and --
+independently -- a C program (tests/div_const_check.c, gcc, libm's FMA) that
+tries every significand at 23 exponents across [2^-100, 2^100] and both
+signs, and shows that the sequence does fail below 2^-100 (why the kernels
+test the exponent).  The torch FP32 reference divides by dx with true
+division, so bit-parity of the kernels rests on this."""
 import os
 import shutil
 import subprocess
