@@ -498,6 +498,7 @@ def bench_c5(args):
         "final_fom_plain": rf["final_fom"], "final_fom_surrogate": rs["final_fom"], "fom_gap": gap,
         "verifications": rs["verifications"], "verifications_failed": rs["failed"],
         "rollbacks": rs["rollbacks"], "verified_errors": [round(e, 4) for e in rs["errors"]],
+        "median_verified_error": (sorted(rs["errors"])[len(rs["errors"]) // 2] if rs["errors"] else None),
         "fdtd_iterations_in_surrogate_loop": rs["fdtd_iters"], "setup_s": setup_s,
         "val_err_median": hist[-1].get("val_err_median"), "val_err_p90": hist[-1].get("val_err_p90"),
         "roofline": roof}), flush=True)

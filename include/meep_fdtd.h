@@ -435,6 +435,18 @@ int mt_tm2d_batch_adjoint_prepare(mt_batch* b, const mt_tm2d_adj* adj, void* con
                                   const int* mon_off, void* stream);
 int mt_tm2d_batch_adjoint_sweep(mt_batch* b, int step0, int k, void* stream);
 
+/* Optional second cotangent set for the batched sweep (call after
+ * mt_tm2d_batch_adjoint_prepare): alt[v] has the same shapes as adj[v] and
+ * the same eps_grad; mons are the monitor injections into that set (same
+ * order and offsets).  When no variant has conductivity or ADE poles
+ * (*fused = 1) every later sweep runs each reverse step as one fused pass
+ * over tiles that reads one set and writes the other (the transposed
+ * differences stay in LDS), and leaves the result in the first set; the
+ * gradient is bit-identical to the three-pass sweep.  Same math as
+ * mt_tm2d_batch_adjoint (the checkpointed backward of core/run.py:99-120). */
+int mt_tm2d_batch_adjoint_pingpong(mt_batch* b, const mt_tm2d_adj* alt, const mt_monitor* mons, const int* mon_off,
+                                   int* fused, void* stream);
+
 void mt_batch_destroy(mt_batch* b);
 
 /* Kernel timing hook.  No counterpart in the reference: a measurement aid

@@ -172,6 +172,115 @@ __device__ __forceinline__ void adj_c_cell(const AdjK<T>& p, int i, int j) {
   p.aEz[c] = p.np ? p.aEz[c] + v : v;
 }
 
+// ---------------------------------------------------------------------------
+// One-pass form of A, B and C for problems without conductivity or poles
+// (the batched fused adjoint step): the same arithmetic, split so that a
+// tile can evaluate A and B on its cells and on a one-cell halo, keep gHy,
+// gHx, gEx, gEy in LDS, and write the new cotangents of its own cells to a
+// second set (r reads the cotangents of step t+1, w receives those of step
+// t; eps, profiles and eps_grad are shared).
+
+// A at (i, j): returns gHy, gHx; with own, also writes eps_grad, w.aDz and
+// the D-half psi cotangents of the cell.  (aEz is not written: without
+// poles C overwrites it.)
+template <typename T>
+__device__ __forceinline__ void adj_a_fused(const AdjK<T>& r, const AdjK<T>& w, const T* ez_after, int i, int j,
+                                            bool own, T& ghy, T& ghx) {
+  const int nx = r.nx, ny = r.ny;
+  const long long c = (long long)i * ny + j;
+  const bool live = i >= 1 && i < nx - 1 && (r.y_pmc || (j >= 1 && j < ny - 1));
+  const int ii = i - 1, jj = r.y_pmc ? j : j - 1;
+  const long long q = (long long)ii * r.ny_e + jj;
+  const T eps = r.eps[c], ezb = r.aEz[c], dz0 = r.aDz[c];
+  const bool sx = live && r.ex.in(ii), sy = live && r.ey.in(jj);
+  T pyx = T(0), ax = T(0), bx = T(0), pxy = T(0), ay = T(0), by = T(0);
+  if (sx) {
+    pyx = r.apyx[q];
+    ax = r.ex.a[ii];
+    bx = r.ex.b[ii];
+  }
+  if (sy) {
+    pxy = r.apxy[q];
+    ay = r.ey.a[jj];
+    by = r.ey.b[jj];
+  }
+  if (own) {
+    const T eg = r.eps_grad[c], ea = ez_after[c];
+    w.eps_grad[c] = eg + (-ezb * ea) / eps;
+  }
+  const T dzb = dz0 + ezb / eps;
+  if (own) w.aDz[c] = dzb;
+  ghy = T(0);
+  ghx = T(0);
+  if (live) {
+    const T cb = r.dt * dzb;
+    ghy = cb;
+    if (sx) {
+      const T pt = pyx + cb;
+      ghy = cb + ax * pt;
+      if (own) w.apyx[q] = bx * pt;
+    }
+    ghx = -cb;
+    if (sy) {
+      const T pt = pxy - cb;
+      ghx = -cb + ay * pt;
+      if (own) w.apxy[q] = by * pt;
+    }
+  }
+}
+
+// Hy part of B at (i, j) (needs i < nx-1): gy0 = gHy(i, j), gy1 = gHy(i+1, j);
+// returns gEx, with own writes the Hy cotangent and its psi cotangent.
+template <typename T>
+__device__ __forceinline__ T adj_b_hy_fused(const AdjK<T>& r, const AdjK<T>& w, int i, int j, T gy0, T gy1,
+                                            bool own) {
+  const long long hy = (long long)i * r.ny + j;
+  const bool sx = r.hx.in(i);
+  const T ahy = r.aHy[hy];
+  T pzx = T(0), a_x = T(0), b_x = T(0);
+  if (sx) {
+    pzx = r.apzx[hy];
+    a_x = r.hx.a[i];
+    b_x = r.hx.b[i];
+  }
+  const T h2y = r.dt;
+  const T hb = ahy + (gy0 - gy1) / r.dx;
+  if (own) w.aHy[hy] = hb;
+  T gb = h2y * hb;
+  if (sx) {
+    const T pt = pzx + h2y * hb;
+    gb = gb + a_x * pt;
+    if (own) w.apzx[hy] = b_x * pt;
+  }
+  return gb;
+}
+
+// Hx part of B at (i, j) (needs j < ny-1): gx0 = gHx(i, j), gx1 = gHx(i, j+1);
+// returns gEy, with own writes the Hx cotangent and its psi cotangent.
+template <typename T>
+__device__ __forceinline__ T adj_b_hx_fused(const AdjK<T>& r, const AdjK<T>& w, int i, int j, T gx0, T gx1,
+                                            bool own) {
+  const long long hx = (long long)i * (r.ny - 1) + j;
+  const bool sy = r.hy.in(j);
+  const T ahx = r.aHx[hx];
+  T pzy = T(0), a_y = T(0), b_y = T(0);
+  if (sy) {
+    pzy = r.apzy[hx];
+    a_y = r.hy.a[j];
+    b_y = r.hy.b[j];
+  }
+  const T h2x = r.dt;
+  const T hb = ahx + (gx0 - gx1) / r.dx;
+  if (own) w.aHx[hx] = hb;
+  T gb = -h2x * hb;
+  if (sy) {
+    const T pt = pzy - h2x * hb;
+    gb = gb + a_y * pt;
+    if (own) w.apzy[hx] = b_y * pt;
+  }
+  return gb;
+}
+
 // Adds the cotangent vb of element e of monitor term tm into the adjoint
 // array(s) it samples (half to each of the two averaged arrays).
 template <typename T>

@@ -19,7 +19,7 @@ constexpr int kCellBlk = 256;
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
-  size_t vars, pre_in, pre_ring, adjk, blk_pre, inj, inj_pre, tapes, dts, total;
+  size_t vars, pre_in, pre_ring, adjk, adjk2, blk_pre, tile_pre, inj, inj2, inj_pre, tapes, dts, total;
 };
 
 template <typename T>
@@ -34,9 +34,15 @@ Layout layout(int nvar) {
   o = align256(o + (nvar + 1) * sizeof(int));
   L.adjk = o;
   o = align256(o + nvar * sizeof(mt::AdjK<T>));
+  L.adjk2 = o;
+  o = align256(o + nvar * sizeof(mt::AdjK<T>));
   L.blk_pre = o;
   o = align256(o + (nvar + 1) * sizeof(int));
+  L.tile_pre = o;
+  o = align256(o + (nvar + 1) * sizeof(int));
   L.inj = o;
+  o = align256(o + (size_t)kMaxMonPerVar * nvar * sizeof(mt_monitor));
+  L.inj2 = o;
   o = align256(o + (size_t)kMaxMonPerVar * nvar * sizeof(mt_monitor));
   L.inj_pre = o;
   o = align256(o + (size_t)kMaxMonPerVar * (nvar + 1) * sizeof(int));
@@ -62,6 +68,10 @@ struct mt_batch {
   bool adj_ready = false;
   int adj_max_m = 0;
   std::vector<int> adj_inj_blocks;
+  // second cotangent set of the fused adjoint step (mt_tm2d_batch_adjoint_pingpong)
+  bool adj_fused = false;
+  int fused_tiles = 0;
+  std::vector<mt_tm2d_adj> adj0, adj1;
 };
 
 namespace mt {
@@ -100,6 +110,85 @@ __global__ __launch_bounds__(kCellBlk) void adj_batch_inject(const mt_monitor* i
   const mt_monitor& m = inj[lo];
   if (m.n_terms == 0) return;
   adj_inject_block<T>(m, (b - pre[lo]) * kInjE, t, dts[lo], t_rel);
+}
+
+// Fused reverse step (no conductivity, no poles): one pass per tile of
+// FTI x FTJ cells of one variant.  A runs on the tile and a one-cell halo
+// (gHy on the rows above and below, gHx on the columns left and right),
+// B's Hy part on the tile and the row above, its Hx part on the tile and
+// the column to the left, C on the tile; gHy, gHx, gEx, gEy stay in LDS.
+// Every cotangent is read from set r (step t+1) and the tile's own cells
+// are written to set w (step t), so the halo reads never race with a
+// neighbour tile's writes; eps_grad (own cells only) is updated in place.
+// Same arithmetic as adj_a_cell / adj_b_cell / adj_c_cell.
+constexpr int FTI = 8, FTJ = 64;
+
+template <typename T>
+__global__ __launch_bounds__(FTI * FTJ) void adj_batch_fused(const AdjK<T>* rs, const AdjK<T>* wsd,
+                                                             void* const* tapes, const int* tile_pre, int nvar,
+                                                             int s_rel) {
+  __shared__ T sGHy[FTI + 2][FTJ];   // rows i0-1 .. i0+FTI
+  __shared__ T sGHx[FTI][FTJ + 2];   // columns j0-1 .. j0+FTJ
+  __shared__ T sGEx[FTI + 1][FTJ];   // rows i0-1 .. i0+FTI-1
+  __shared__ T sGEy[FTI][FTJ + 1];   // columns j0-1 .. j0+FTJ-1
+  const int b = blockIdx.x;
+  int lo = 0, hi = nvar;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (tile_pre[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const AdjK<T>& r = rs[lo];
+  const AdjK<T>& w = wsd[lo];
+  const int nx = r.nx, ny = r.ny;
+  const int ntj = (ny + FTJ - 1) / FTJ, t = b - tile_pre[lo];
+  const int i0 = (t / ntj) * FTI, j0 = (t % ntj) * FTJ;
+  const int tx = threadIdx.x, ty = threadIdx.y, q = ty * FTJ + tx;
+  const int i = i0 + ty, j = j0 + tx;
+  const bool own = i < nx && j < ny;
+  const T* ez_after = static_cast<const T*>(tapes[lo]) + (long long)s_rel * nx * ny;
+  // ---- A
+  T ghy = T(0), ghx = T(0);
+  if (own) adj_a_fused(r, w, ez_after, i, j, true, ghy, ghx);
+  sGHy[ty + 1][tx] = ghy;
+  sGHx[ty][tx + 1] = ghx;
+  if (q < 2 * FTJ) {                      // rows i0-1 and i0+FTI: gHy
+    const int hi_row = q >= FTJ, jj = j0 + (q % FTJ), ii = hi_row ? i0 + FTI : i0 - 1;
+    T gy = T(0), gx;
+    if (ii >= 0 && ii < nx && jj < ny) adj_a_fused(r, w, ez_after, ii, jj, false, gy, gx);
+    sGHy[hi_row ? FTI + 1 : 0][q % FTJ] = gy;
+  } else if (q < 2 * FTJ + 2 * FTI) {     // columns j0-1 and j0+FTJ: gHx
+    const int q2 = q - 2 * FTJ, hi_col = q2 >= FTI, ii = i0 + (q2 % FTI), jj = hi_col ? j0 + FTJ : j0 - 1;
+    T gy, gx = T(0);
+    if (jj >= 0 && jj < ny && ii < nx) adj_a_fused(r, w, ez_after, ii, jj, false, gy, gx);
+    sGHx[q2 % FTI][hi_col ? FTJ + 1 : 0] = gx;
+  }
+  __syncthreads();
+  // ---- B
+  T gex = T(0), gey = T(0);
+  if (own && i < nx - 1) gex = adj_b_hy_fused(r, w, i, j, sGHy[ty + 1][tx], sGHy[ty + 2][tx], true);
+  if (own && j < ny - 1) gey = adj_b_hx_fused(r, w, i, j, sGHx[ty][tx + 1], sGHx[ty][tx + 2], true);
+  sGEx[ty + 1][tx] = gex;
+  sGEy[ty][tx + 1] = gey;
+  if (q < FTJ) {                          // row i0-1: gEx
+    T g = T(0);
+    if (i0 >= 1 && j0 + q < ny) g = adj_b_hy_fused(r, w, i0 - 1, j0 + q, sGHy[0][q], sGHy[1][q], false);
+    sGEx[0][q] = g;
+  } else if (q < FTJ + FTI) {             // column j0-1: gEy
+    const int rr = q - FTJ;
+    T g = T(0);
+    if (j0 >= 1 && i0 + rr < nx) g = adj_b_hx_fused(r, w, i0 + rr, j0 - 1, sGHx[rr][0], sGHx[rr][1], false);
+    sGEy[rr][0] = g;
+  }
+  __syncthreads();
+  // ---- C
+  if (own) {
+    const T gxm = i >= 1 ? sGEx[ty][tx] : T(0);
+    const T gx0 = i < nx - 1 ? sGEx[ty + 1][tx] : T(0);
+    const T gym = j >= 1 ? sGEy[ty][tx] : T(0);
+    const T gy0 = j < ny - 1 ? sGEy[ty][tx + 1] : T(0);
+    w.aEz[(long long)i * ny + j] = (gxm - gx0) / r.dx + (gym - gy0) / r.dx;
+  }
 }
 
 template <typename T>
@@ -259,18 +348,106 @@ int adjoint_prepare_t(mt_batch* b, const mt_tm2d_adj* adj, void* const* tapes, c
       hipStreamSynchronize(s))
     return fail(MT_EHIP, "batched adjoint: descriptor upload failed");
   b->adj_max_m = max_m;
+  b->adj0.assign(adj, adj + nvar);
+  b->adj_fused = false;
   b->adj_inj_blocks.assign(max_m, 0);
   for (int m = 0; m < max_m; ++m) b->adj_inj_blocks[m] = ipre[(size_t)m * (nvar + 1) + nvar];
   b->adj_ready = true;
   return MT_OK;
 }
 
+// Second cotangent set for the fused reverse step: descriptors of set 1
+// (same scratch and eps_grad as set 0) and monitor injections into it, plus
+// the tile prefix.  Applies when no variant has conductivity or poles;
+// *fused says whether the sweeps will use it.
+template <typename T>
+int adjoint_pingpong_t(mt_batch* b, const mt_tm2d_adj* alt, const mt_monitor* mons, const int* mon_off, int* fused,
+                       hipStream_t s) {
+  if (!b->adj_ready) return fail(MT_EINVAL, "batched adjoint: descriptors not prepared");
+  const int nvar = b->nvar;
+  *fused = 0;
+  b->adj_fused = false;
+  for (int v = 0; v < nvar; ++v) {
+    const mt_tm2d& d = b->probs[v];
+    if (d.d1 || d.h1x || d.h1y || d.n_poles > 0) return MT_OK;
+  }
+  std::vector<AdjK<T>> ks(nvar);
+  std::vector<int> tpre(nvar + 1, 0);
+  for (int v = 0; v < nvar; ++v) {
+    ks[v] = make_adjk<T>(&b->probs[v], &alt[v]);
+    const mt_tm2d& d = b->probs[v];
+    tpre[v + 1] = tpre[v] + ((d.nx + FTI - 1) / FTI) * ((d.ny + FTJ - 1) / FTJ);
+  }
+  std::vector<mt_monitor> inj((size_t)kMaxMonPerVar * nvar);
+  for (int m = 0; m < b->adj_max_m; ++m)
+    for (int v = 0; v < nvar; ++v) {
+      mt_monitor& dst = inj[(size_t)m * nvar + v];
+      dst = mt_monitor{};
+      if (m < mon_off[v + 1] - mon_off[v]) dst = mons[mon_off[v] + m];
+    }
+  char* ws = b->ws;
+  if (hipMemcpyAsync(ws + b->L.adjk2, ks.data(), nvar * sizeof(AdjK<T>), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(ws + b->L.tile_pre, tpre.data(), (nvar + 1) * sizeof(int), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(ws + b->L.inj2, inj.data(), inj.size() * sizeof(mt_monitor), hipMemcpyHostToDevice, s) ||
+      hipStreamSynchronize(s))
+    return fail(MT_EHIP, "batched adjoint: second-set upload failed");
+  b->adj1.assign(alt, alt + nvar);
+  b->fused_tiles = tpre[nvar];
+  b->adj_fused = true;
+  *fused = 1;
+  return MT_OK;
+}
+
 // Reverse-time sweep over steps [step0, step0 + k) with the prepared
-// descriptors: per step, the monitor injection then the three transpose
-// passes over the cells of every variant.  Launches only.
+// descriptors: per step, the monitor injection then either the fused pass
+// (alternating the two cotangent sets, set 0 again at the end) or the three
+// transpose passes over the cells of every variant.  Launches only.
 template <typename T>
 int adjoint_sweep_t(mt_batch* b, int step0, int k, hipStream_t s) {
   if (!b->adj_ready) return fail(MT_EINVAL, "batched adjoint: descriptors not prepared");
+  if (b->adj_fused) {
+    const int nvar = b->nvar;
+    char* ws = b->ws;
+    const AdjK<T>* k0 = reinterpret_cast<const AdjK<T>*>(ws + b->L.adjk);
+    const AdjK<T>* k1 = reinterpret_cast<const AdjK<T>*>(ws + b->L.adjk2);
+    void* const* dtp = reinterpret_cast<void* const*>(ws + b->L.tapes);
+    const int* tpre = reinterpret_cast<const int*>(ws + b->L.tile_pre);
+    const mt_monitor* inj0 = reinterpret_cast<const mt_monitor*>(ws + b->L.inj);
+    const mt_monitor* inj1 = reinterpret_cast<const mt_monitor*>(ws + b->L.inj2);
+    const int* dipre = reinterpret_cast<const int*>(ws + b->L.inj_pre);
+    const double* ddts = reinterpret_cast<const double*>(ws + b->L.dts);
+    int par = 0;
+    for (int s_ = k - 1; s_ >= 0; --s_) {
+      const int t = step0 + s_;
+      for (int m = 0; m < b->adj_max_m; ++m) {
+        const int nb = b->adj_inj_blocks[m];
+        if (nb)
+          hipLaunchKernelGGL(adj_batch_inject<T>, dim3(nb), dim3(kInjE * kInjS), 0, s,
+                             (par ? inj1 : inj0) + (size_t)m * nvar, dipre + (size_t)m * (nvar + 1), nvar, t, ddts,
+                             s_);
+      }
+      hipLaunchKernelGGL(adj_batch_fused<T>, dim3(b->fused_tiles), dim3(FTJ, FTI), 0, s, par ? k1 : k0,
+                         par ? k0 : k1, dtp, tpre, nvar, s_);
+      par ^= 1;
+      int rc = check_launch("batched tm2d fused adjoint step");
+      if (rc) return rc;
+    }
+    if (par) {  // odd step count: the cotangents are in set 1
+      for (int v = 0; v < nvar; ++v) {
+        const mt_tm2d& d = b->probs[v];
+        const mt_tm2d_adj &a0 = b->adj0[v], &a1 = b->adj1[v];
+        size_t nx = d.nx, ny = d.ny, nye = d.y_pmc ? d.ny : d.ny - 2;
+        size_t n[8] = {nx * ny, nx * ny, nx * (ny - 1), (nx - 1) * ny, nx * (ny - 1), (nx - 1) * ny,
+                       (nx - 2) * nye, (nx - 2) * nye};
+        void* dst[8] = {a0.aEz, a0.aDz, a0.aHx, a0.aHy, a0.apsi_Ezy, a0.apsi_Ezx, a0.apsi_Hyx, a0.apsi_Hxy};
+        void* src[8] = {a1.aEz, a1.aDz, a1.aHx, a1.aHy, a1.apsi_Ezy, a1.apsi_Ezx, a1.apsi_Hyx, a1.apsi_Hxy};
+        for (int q = 0; q < 8; ++q)
+          if (hipMemcpyAsync(dst[q], src[q], n[q] * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess)
+            return fail(MT_EHIP, "batched adjoint: copy back failed");
+      }
+    }
+    return MT_OK;
+  }
   const int nvar = b->nvar;
   char* ws = b->ws;
   const AdjK<T>* dks = reinterpret_cast<const AdjK<T>*>(ws + b->L.adjk);
@@ -432,6 +609,24 @@ extern "C" int mt_tm2d_batch_adjoint_prepare(mt_batch* b, const mt_tm2d_adj* adj
   hipStream_t s = static_cast<hipStream_t>(stream);
   return b->dtype == MT_F64 ? adjoint_prepare_t<double>(b, adj, tapes, mons, mon_off, s)
                             : adjoint_prepare_t<float>(b, adj, tapes, mons, mon_off, s);
+}
+
+extern "C" int mt_tm2d_batch_adjoint_pingpong(mt_batch* b, const mt_tm2d_adj* alt, const mt_monitor* mons,
+                                              const int* mon_off, int* fused, void* stream) {
+  if (!b || !alt || !mon_off || !fused) return fail(MT_EINVAL, "batched adjoint: null argument");
+  if (!b->adj_ready || (int)b->adj0.size() != b->nvar)
+    return fail(MT_EINVAL, "batched adjoint: descriptors not prepared");
+  for (int v = 0; v < b->nvar; ++v) {
+    const mt_tm2d_adj& a = alt[v];
+    if (!a.aEz || !a.aDz || !a.aHx || !a.aHy || !a.apsi_Ezy || !a.apsi_Ezx || !a.apsi_Hyx || !a.apsi_Hxy ||
+        !a.eps_grad)
+      return fail(MT_EINVAL, "batched adjoint: null pointer in the second cotangent set");
+    if (a.eps_grad != b->adj0[v].eps_grad)
+      return fail(MT_EINVAL, "batched adjoint: both cotangent sets must share eps_grad");
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return b->dtype == MT_F64 ? adjoint_pingpong_t<double>(b, alt, mons, mon_off, fused, s)
+                            : adjoint_pingpong_t<float>(b, alt, mons, mon_off, fused, s);
 }
 
 extern "C" int mt_tm2d_batch_adjoint_sweep(mt_batch* b, int step0, int k, void* stream) {

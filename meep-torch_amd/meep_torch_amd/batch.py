@@ -52,6 +52,9 @@ def _bind(L):
                                                 E.c_void_p]
     L.mt_tm2d_batch_adjoint_sweep.restype = E.c_int
     L.mt_tm2d_batch_adjoint_sweep.argtypes = [E.c_void_p, E.c_int, E.c_int, E.c_void_p]
+    L.mt_tm2d_batch_adjoint_pingpong.restype = E.c_int
+    L.mt_tm2d_batch_adjoint_pingpong.argtypes = [E.c_void_p, P(E.TM2DAdj), P(E.Monitor), P(E.c_int), P(E.c_int),
+                                                  E.c_void_p]
     L.mt_batch_destroy.restype = None
     L.mt_batch_destroy.argtypes = [E.c_void_p]
     L._batch_bound = True
@@ -64,8 +67,12 @@ class BatchedTM:
     adjoint=False no checkpoints or Ez tapes are kept (forward runs only;
     spectra are still folded per K-step chunk)."""
 
-    def __init__(self, problems, n_steps, checkpoint_every=200, variant=None, stage_steps=64, adjoint=True):
+    def __init__(self, problems, n_steps, checkpoint_every=200, variant=None, stage_steps=64, adjoint=True,
+                 fused_adjoint=True):
         self.L = E.lib()
+        # reverse steps as one fused pass over tiles with two cotangent sets
+        # (False: the three transpose passes; the gradient is the same bit for bit)
+        self.fused_adjoint = bool(fused_adjoint)
         _bind(self.L)
         self.B = len(problems)
         self.n_steps = int(n_steps)
@@ -279,7 +286,30 @@ class BatchedTM:
         off_arr = (E.c_int * (B + 1))(*off)
         rc = self.L.mt_tm2d_batch_adjoint_prepare(h, adjs, tapes, mon_arr, off_arr, E.stream_handle())
         E.check(rc, "mt_tm2d_batch_adjoint_prepare")
-        self._adj = {"h": h, "per": per, "keep": (adjs, tapes, mon_arr, off_arr, keep)}
+        # second cotangent set: the fused reverse step reads one set and
+        # writes the other (batched variants carry no conductivity or poles)
+        fused = E.c_int(0)
+        adjs2, mons2, keep2 = (E.TM2DAdj * B)(), [], []
+        if self.fused_adjoint:
+            for v, (V, P_) in enumerate(zip(self.var, per)):
+                A2 = {f: torch.zeros_like(t) for f, t in P_["A"].items()}
+                a2 = adjs2[v]
+                for f in FIELDS:
+                    setattr(a2, "a" + f, A2[f].data_ptr())
+                for f, t in P_["sc"].items():
+                    setattr(a2, f, t.data_ptr())
+                a2.eps_grad = P_["g"].data_ptr()
+                arr, n_used, kp = _adj_monitors(A2, V["monitors"], P_["gacc"], self.dtype, self.device)
+                keep2.append(kp)
+                mons2.extend(arr[i] for i in range(n_used))
+                P_["A2"] = A2
+            mon_arr2 = (E.Monitor * max(1, len(mons2)))(*mons2)
+            rc = self.L.mt_tm2d_batch_adjoint_pingpong(h, adjs2, mon_arr2, off_arr, ctypes.byref(fused),
+                                                       E.stream_handle())
+            E.check(rc, "mt_tm2d_batch_adjoint_pingpong")
+            keep2.append((adjs2, mon_arr2))
+        self._adj = {"h": h, "per": per, "fused": bool(fused.value),
+                     "keep": (adjs, tapes, mon_arr, off_arr, keep, keep2)}
         return self._adj
 
     def _gradient_body(self):

@@ -175,3 +175,29 @@ def test_adjoint_graph_replays_match_eager():
         for a, b in zip(ge, gg):
             assert torch.equal(a, b), (it, float((a - b).abs().max()))
         assert float(ge[0].abs().max()) > 0
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("ck", [CK, 75])
+def test_fused_adjoint_step_equals_three_passes(dtype, ck):
+    """The fused reverse step (one tiled pass per step, two cotangent sets,
+    transposed differences in LDS) against the three transpose passes: eps
+    gradients bit for bit, with segments of an even (100) and an odd (75,
+    the result copied back from the second set) number of steps, on grids
+    that end in partial tiles in both directions."""
+    from meep_torch_amd.batch import BatchedTM
+    sizes = SIZES + [(83, 131)]
+    probs = [_problem(nx, ny, dtype, 60 + i) for i, (nx, ny) in enumerate(sizes)]
+    grads = []
+    for fused in (False, True):
+        bt = BatchedTM([(m, src, [mon]) for (g, m, src, mon, eps) in probs], N_STEPS, checkpoint_every=ck,
+                       fused_adjoint=fused)
+        eps_b = [p[4].clone().requires_grad_(True) for p in probs]
+        out = bt.apply(eps_b)
+        J = sum(mon.flux(o[0], 0.05).sum() for (_, _, _, mon, _), o in zip(probs, out))
+        J.backward()
+        assert bt._adj["fused"] == fused
+        grads.append([e.grad.clone() for e in eps_b])
+    for a, b in zip(*grads):
+        assert float(a.abs().max()) > 0
+        assert torch.equal(a, b), float((a - b).abs().max())
