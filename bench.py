@@ -40,15 +40,17 @@ HBM_PEAK_GBS = 8000.0
 # FP32 algorithmic bytes per cell-update: whole step (CPML shells included)
 ALG_BYTES = {"c1": 32.0, "c2": 29.0, "c3": 58.0}
 # ... and for the dominant kernel's cells: the 2D interior tiles are counted
-# at the whole-step figure (their tiles run the same fields); the 3D interior
-# box has no psi terms: 48 B of field read + write and 4 B of eps
-KERNEL_ALG_BYTES = {"c1": 32.0, "c2": 29.0, "c3": 52.0}
+# at the whole-step figure (their tiles run the same fields), as is the
+# two-pass 3D step (the whole grid); the interior box of the split 3D step
+# has no psi terms: 48 B of field read + write and 4 B of eps
+KERNEL_ALG_BYTES = {"c1": 32.0, "c2": 29.0, "c3": 58.0, "c3_split": 52.0}
 
 
 DOMINANT = {
     "c1": "tm_tiled interior-tile launch (temporally blocked TM steps), HIP events on its stream",
     "c2": "te_tiled interior-tile launch (temporally blocked TE steps), HIP events on its stream",
-    "c3": "y3_interior (one-pass interior box of the split 3D step), HIP events on its stream",
+    "c3": "h3_kernel + e3_kernel (one two-pass 3D step over the whole grid), HIP events on its stream",
+    "c3_split": "y3_interior (one-pass interior box of the split 3D step), HIP events on its stream",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
@@ -344,13 +346,16 @@ def main():
     else:   # no timed launches (should not happen): whole-step events
         per_launch_cells = cells
         per_launch_s = gpu_ms / 1e3 / (ysteps * args.steps)
-    achieved = k_alg * per_launch_cells / per_launch_s / 1e9
     variant = None
+    dominant = DOMINANT[args.workload]
     if args.workload == "c3":
         from meep_torch_amd.core import engine3d
         variant = engine3d.DEFAULT_VARIANT
+        if variant >= 1:
+            k_alg, dominant = KERNEL_ALG_BYTES["c3_split"], DOMINANT["c3_split"]
     elif tune._CACHE and len(set(tune._CACHE.values())) == 1:
         variant = next(iter(tune._CACHE.values()))     # the tile variant the autotuner chose
+    achieved = k_alg * per_launch_cells / per_launch_s / 1e9
     traffic = _traffic(args.workload, per_launch_cells, variant)
     line = {
         "metric": "Yee cell-updates/s (FP32) with % of MI355X HBM roofline",
@@ -365,7 +370,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_gbs": traffic / per_launch_s / 1e9 if traffic else None,
-                     "kernel": DOMINANT[args.workload],
+                     "kernel": dominant,
                      "launch_avg_us": per_launch_s * 1e6, "cell_steps_per_launch": per_launch_cells,
                      "alg_bytes_per_cell_update": k_alg,
                      "whole_step_alg_bytes_per_cell_update": alg,
@@ -405,18 +410,38 @@ def bench_c4(args):
     k_ms, k_n, k_cells = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_longlong()
     E.check(L.mt_kernel_timing_read(ctypes.byref(k_ms), ctypes.byref(k_n), ctypes.byref(k_cells)),
             "mt_kernel_timing_read")
+    # ... and the fused reverse step on one more adjoint sweep issued eagerly
+    L.mt_kernel_timing_enable(2)
+    it.runner._gradient_body()
+    torch.cuda.synchronize()
+    L.mt_kernel_timing_enable(0)
+    a_ms, a_n, a_cells = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_longlong()
+    E.check(L.mt_kernel_timing_read(ctypes.byref(a_ms), ctypes.byref(a_n), ctypes.byref(a_cells)),
+            "mt_kernel_timing_read")
     upd = cells * ysteps * 3 * args.steps   # forward + recompute + adjoint sweeps
-    roof = None
+    roof = adj_roof = None
     if k_n.value:
         per_s = k_ms.value * 1e-3 / k_n.value
         per_cells = k_cells.value / k_n.value
         alg = 29.0      # FP32 TM forward: Ez, Hx, Hy read + write and eps (interior tiles: no psi)
         gbs = alg * per_cells / per_s / 1e9
         roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                "traffic": None, "kernel": "tm_batch interior-tile launch (forward and recompute sweeps), HIP "
-                                            "events on its stream", "launch_avg_us": per_s * 1e6,
-                "cell_steps_per_launch": per_cells, "alg_bytes_per_cell_update": alg,
-                "eager_forward_ms": k_ms.value}
+                "traffic": _traffic("c4", per_cells), "kernel": "tm_batch interior-tile launch (forward and "
+                                                               "recompute sweeps), HIP events on its stream",
+                "launch_avg_us": per_s * 1e6, "cell_steps_per_launch": per_cells, "alg_bytes_per_cell_update": alg,
+                "interior_launches_total_ms": k_ms.value}
+    if a_n.value:
+        per_s = a_ms.value * 1e-3 / a_n.value
+        per_cells = a_cells.value / a_n.value
+        # reads eps, Ez/Dz/Hx/Hy cotangents, eps_grad and the Ez tape; writes the
+        # four cotangents and eps_grad (CPML psi cotangents not counted)
+        alg = 48.0
+        gbs = alg * per_cells / per_s / 1e9
+        adj_roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": gbs / HBM_PEAK_GBS, "traffic": _traffic("c4_adjoint", per_cells),
+                    "kernel": "adj_batch_fused (one reverse step of every variant), HIP events on its stream",
+                    "launch_avg_us": per_s * 1e6, "cell_steps_per_launch": per_cells,
+                    "alg_bytes_per_cell_update": alg}
     print(json.dumps({
         "metric": "batched design iterations/s (forward + adjoint, all variants)",
         "value": args.steps / wall, "unit": "design-iterations/s", "n_gpus": 1, "steps": args.steps,
@@ -426,7 +451,7 @@ def bench_c4(args):
                                f"[{min(sizes)},{max(sizes)}], {ysteps} steps, ckpt 200",
                    "cells_per_variant_sum": cells},
         "sweep_cell_updates_per_s": upd / wall, "fom": float(fom.detach()), "graphs": "forward + adjoint",
-        "roofline": roof}), flush=True)
+        "roofline": roof, "adjoint_roofline": adj_roof}), flush=True)
 
 
 def _gemm_roofline(net, batch=64, reps=5):

@@ -450,10 +450,10 @@ int mt_tm2d_batch_adjoint_pingpong(mt_batch* b, const mt_tm2d_adj* alt, const mt
 void mt_batch_destroy(mt_batch* b);
 
 /* Kernel timing hook.  No counterpart in the reference: a measurement aid
- * for benchmarks.  While enabled, every launch of the dominant stencil
- * kernel (2D: the temporally blocked interior-tile launch; 3D: one full
- * step) is bracketed by HIP events on its own stream and its cell-steps
- * are counted.  mt_kernel_timing_read waits for those events, returns the
+ * for benchmarks.  on = 1: every launch of the dominant stencil kernel (2D:
+ * the temporally blocked interior-tile launch; 3D: one full step) is
+ * bracketed by HIP events on its own stream and its cell-steps are counted;
+ * on = 2: the same for every fused batched adjoint step; 3: both; 0: off.  mt_kernel_timing_read waits for those events, returns the
  * summed duration, launch count and cell-steps, and clears the record.
  * Host-thread-unsafe: use from one thread. */
 int mt_kernel_timing_enable(int on);

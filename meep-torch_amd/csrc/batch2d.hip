@@ -71,6 +71,7 @@ struct mt_batch {
   // second cotangent set of the fused adjoint step (mt_tm2d_batch_adjoint_pingpong)
   bool adj_fused = false;
   int fused_tiles = 0;
+  long long total_cells = 0;
   std::vector<mt_tm2d_adj> adj0, adj1;
 };
 
@@ -393,6 +394,8 @@ int adjoint_pingpong_t(mt_batch* b, const mt_tm2d_adj* alt, const mt_monitor* mo
     return fail(MT_EHIP, "batched adjoint: second-set upload failed");
   b->adj1.assign(alt, alt + nvar);
   b->fused_tiles = tpre[nvar];
+  b->total_cells = 0;
+  for (int v = 0; v < nvar; ++v) b->total_cells += (long long)b->probs[v].nx * b->probs[v].ny;
   b->adj_fused = true;
   *fused = 1;
   return MT_OK;
@@ -426,8 +429,10 @@ int adjoint_sweep_t(mt_batch* b, int step0, int k, hipStream_t s) {
                              (par ? inj1 : inj0) + (size_t)m * nvar, dipre + (size_t)m * (nvar + 1), nvar, t, ddts,
                              s_);
       }
+      timing_begin(s, 2);
       hipLaunchKernelGGL(adj_batch_fused<T>, dim3(b->fused_tiles), dim3(FTJ, FTI), 0, s, par ? k1 : k0,
                          par ? k0 : k1, dtp, tpre, nvar, s_);
+      timing_end(s, b->total_cells, 2);
       par ^= 1;
       int rc = check_launch("batched tm2d fused adjoint step");
       if (rc) return rc;

@@ -105,10 +105,11 @@ struct MonitorPlan {
   int dtype;
 };
 int monitors_validate(const MonitorPlan& mp);
-// Kernel timing hook (mt_kernel_timing_enable): brackets launches with HIP
-// events on their stream when enabled; no-ops otherwise.
-void timing_begin(hipStream_t s);
-void timing_end(hipStream_t s, long long cell_steps);
+// Kernel timing hook (mt_kernel_timing_enable): brackets launches of the
+// given kind (1: forward stencil, 2: batched adjoint step) with HIP events
+// on their stream when that kind is enabled; no-ops otherwise.
+void timing_begin(hipStream_t s, int kind = 1);
+void timing_end(hipStream_t s, long long cell_steps, int kind = 1);
 // Gather the values of every term for step slot `slot` (after the step).
 int monitors_sample(const MonitorPlan& mp, int slot, int t_rel, hipStream_t s);
 // Fold `n_filled` staged steps whose first global step index is t0 into the accumulators.

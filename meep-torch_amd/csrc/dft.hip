@@ -445,7 +445,7 @@ struct TimingRec {
 };
 std::vector<TimingRec> g_rec;
 std::vector<hipEvent_t> g_pool;
-bool g_timing = false;
+int g_timing = 0;  // bit 0: forward stencil launches, bit 1: batched adjoint steps
 constexpr size_t kMaxTimingRecs = 1 << 16;
 
 hipEvent_t take_event() {
@@ -460,16 +460,16 @@ hipEvent_t take_event() {
 }
 }  // namespace
 
-void timing_begin(hipStream_t s) {
-  if (!g_timing || g_rec.size() >= kMaxTimingRecs) return;
+void timing_begin(hipStream_t s, int kind) {
+  if (!(g_timing & kind) || g_rec.size() >= kMaxTimingRecs) return;
   TimingRec r{take_event(), take_event(), -1};
   if (!r.a || !r.b) return;
   if (hipEventRecord(r.a, s) != hipSuccess) return;
   g_rec.push_back(r);
 }
 
-void timing_end(hipStream_t s, long long cell_steps) {
-  if (!g_timing || g_rec.empty() || g_rec.back().cells != -1) return;
+void timing_end(hipStream_t s, long long cell_steps, int kind) {
+  if (!(g_timing & kind) || g_rec.empty() || g_rec.back().cells != -1) return;
   TimingRec& r = g_rec.back();
   if (hipEventRecord(r.b, s) != hipSuccess) {
     g_pool.push_back(r.a);
@@ -483,7 +483,7 @@ void timing_end(hipStream_t s, long long cell_steps) {
 }  // namespace mt
 
 extern "C" int mt_kernel_timing_enable(int on) {
-  mt::g_timing = on != 0;
+  mt::g_timing = on < 0 ? 0 : on;
   return MT_OK;
 }
 
