@@ -67,7 +67,9 @@ typedef struct {
   void *Ez, *Dz, *Hx, *Hy;               /* device (nx,ny) (nx,ny) (nx,ny-1) (nx-1,ny) */
   void *psi_Ezy, *psi_Ezx;               /* device (nx,ny-1) (nx-1,ny) */
   void *psi_Hyx, *psi_Hxy;               /* device (nx-2, ny_e), ny_e = ny-2 or ny (y_pmc) */
-  void* tape;                            /* NULL, or device (k, nx, ny): Ez after each step of the call */
+  void* tape;                            /* NULL, or device (k, nx, ny): Ez after each step of the call
+                                          * (with kerr = 1: rhs = Dz - sum P after each step, from which
+                                          * the adjoint recomputes the fixed-point iterates) */
   /* Temporally blocked path: variant >= 0 selects a tile shape / steps per
    * launch (mt_tiled_variants); -1 runs two passes per step.  alt[] are
    * second buffers of the same shapes as Ez Dz Hx Hy psi_Ezy psi_Ezx psi_Hyx

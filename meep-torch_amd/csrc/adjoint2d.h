@@ -27,6 +27,9 @@ struct AdjK {
   int np;
   const T *c1, *c2, *c3;
   T *aPc, *aPp;
+  // chi2/chi3 fixed point (kerr = 1): the tape holds rhs = Dz - sum P
+  int kerr;
+  const T *chi2, *chi3;
 };
 
 // Each pass below issues all of a cell's loads before its first store: the
@@ -57,8 +60,38 @@ __device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, 
     ay = p.ey.a[jj];
     by = p.ey.b[jj];
   }
-  p.eps_grad[c] = eg + (-ezb * ea) / eps;
-  T dzb = dz0 + ezb / eps;
+  // cotangents of rhs = Dz - sum P and of eps through E = rhs / eps, or
+  // through the fixed point E(k+1) = rhs / den(k), den(k) = eps + chi2 E(k)
+  // + chi3 E(k)^2 from E(0) = rhs / eps (stepper_tm.py:224-231): the
+  // iterates are recomputed from the taped rhs, then the three passes are
+  // transposed last to first
+  T rhs_b, eps_b;
+  if (p.kerr) {
+    const T x2 = p.chi2[c], x3 = p.chi3[c], rhs = ea;
+    T E[4], den[3];
+    E[0] = rhs / eps;
+    for (int k = 0; k < 3; ++k) {
+      den[k] = eps + x2 * E[k] + x3 * E[k] * E[k];
+      E[k + 1] = rhs / den[k];
+    }
+    T eb = ezb;
+    rhs_b = T(0);
+    eps_b = T(0);
+    for (int k = 2; k >= 0; --k) {
+      const T g = eb / den[k];      // dE(k+1)/drhs
+      const T t = -g * E[k + 1];    // dE(k+1)/dden(k)
+      rhs_b = rhs_b + g;
+      eps_b = eps_b + t;
+      eb = t * (x2 + T(2) * x3 * E[k]);
+    }
+    rhs_b = rhs_b + eb / eps;
+    eps_b = eps_b + (-eb * E[0]) / eps;
+  } else {
+    rhs_b = ezb / eps;
+    eps_b = (-ezb * ea) / eps;
+  }
+  p.eps_grad[c] = eg + eps_b;
+  T dzb = dz0 + rhs_b;
   p.aDz[c] = p.d1 ? d1 * dzb : dzb;          // Dz' = d1 Dz + d2 curl
   // poles: Ez' = (Dz' - sum P') / eps and P' = c1 P + c2 P_prev + c3 Ez, so
   // P' gains -Ez'b / eps; P then carries c1 P'b + P_prev'b (P_prev' = P),
@@ -67,7 +100,7 @@ __device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, 
   T ez_old = T(0);
   if (p.np) {
     const long long ncell = (long long)nx * ny;
-    const T r = ezb / eps;
+    const T r = rhs_b;
     for (int n = 0; n < p.np; ++n) {
       const long long o = (long long)n * ncell + c;
       const T apn = p.aPc[o] - r, app = p.aPp[o];
@@ -416,6 +449,9 @@ AdjK<T> make_adjk(const mt_tm2d* d, const mt_tm2d_adj* a) {
   p.c3 = static_cast<const T*>(d->c3);
   p.aPc = static_cast<T*>(a->aP_cur);
   p.aPp = static_cast<T*>(a->aP_prev);
+  p.kerr = d->kerr;
+  p.chi2 = static_cast<const T*>(d->chi2);
+  p.chi3 = static_cast<const T*>(d->chi3);
   return p;
 }
 

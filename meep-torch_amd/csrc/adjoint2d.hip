@@ -336,7 +336,8 @@ extern "C" int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const voi
   if (k < 0) return fail(MT_EINVAL, "negative step count");
   if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward Ez tape");
   if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");
-  if (p->kerr) return fail(MT_EINVAL, "adjoint through nonlinear (chi2 / chi3) media is not supported");
+  if (p->kerr && (!p->chi2 || !p->chi3))
+    return fail(MT_EINVAL, "adjoint through chi2/chi3 needs chi2 and chi3 (and a tape of Dz - sum P)");
   if (p->n_poles < 0 || (p->n_poles && (!a->aP_cur || !a->aP_prev || !p->c1 || !p->c2 || !p->c3)))
     return fail(MT_EINVAL, "adjoint with poles needs c1, c2, c3 and the aP_cur / aP_prev cotangents");
   if (!p->d1 != !p->d2 || !p->h1x != !p->h2x || !p->h1y != !p->h2y || !p->d1 != !p->h1x || !p->d1 != !p->h1y)

@@ -159,7 +159,8 @@ __global__ __launch_bounds__(BX* BY) void tm_d_kernel(TmK<T> p, mt_source2d src,
     for (int it = 0; it < 3; ++it) ez = rhs / (e0 + x2 * ez + x3 * ez * ez);
   }
   p.Ez[c] = ez;
-  if (p.tape) p.tape[(long long)s_rel * nx * ny + c] = ez;
+  // under chi2/chi3 the tape keeps rhs: the adjoint recomputes the iterates from it
+  if (p.tape) p.tape[(long long)s_rel * nx * ny + c] = p.kerr ? rhs : ez;
 }
 
 template <typename T>

@@ -178,6 +178,13 @@ def _descriptor(c, fields, dtype, device):
         t = getattr(c, name).to(device=device, dtype=dtype).contiguous()
         keep.append(t)
         setattr(d, name, t.data_ptr())
+    if c.has_kerr or c.has_chi2:                   # chi2 / chi3 fixed point (the tape holds Dz - sum P)
+        d.kerr = 1
+        for name, t in (("chi2", c.chi2 if c.chi2 is not None else torch.zeros_like(c.eps_inf)),
+                        ("chi3", c.chi3)):
+            t = t.to(device=device, dtype=dtype).contiguous()
+            keep.append(t)
+            setattr(d, name, t.data_ptr())
     return d, keep
 
 
@@ -267,8 +274,6 @@ def _te_descriptor(c, nx, ny, dtype, device):
 
 
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
-    if mode == "tm" and (coeffs.has_kerr or coeffs.has_chi2):
-        raise NotImplementedError("gradients through nonlinear (chi2 / chi3) media are not supported")
     if mode == "te" and (coeffs.has_kerr or coeffs.has_chi2):
         raise NotImplementedError("gradients through nonlinear (chi2 / chi3) media are not supported")
     allowed = ("eps_inf",) if mode == "tm" else ("eps_x", "eps_y")
