@@ -310,6 +310,14 @@ typedef struct {
   /* cotangents of the pole stacks P_cur, P_prev (n_poles, nx, ny) when the
    * problem has ADE poles; NULL otherwise */
   void *aP_cur, *aP_prev;
+  /* optional gradients of the other differentiable coefficients, accumulated
+   * like eps_grad (NULL: not wanted): chi2_grad, chi3_grad (nx, ny) of the
+   * chi2 / chi3 maps, c3_grad (n_poles, nx, ny) of the poles' c3 (the pole
+   * masks enter the step through it).  c3_grad needs ez_init, the Ez before
+   * the first step of the call (the tape holds the Ez, or under chi2/chi3
+   * the Dz - sum P, after each step). */
+  void *chi2_grad, *chi3_grad, *c3_grad;
+  const void* ez_init;
 } mt_tm2d_adj;
 
 /* Reverse-time adjoint of k TM steps [step0, step0+k): on entry the mt_tm2d_adj

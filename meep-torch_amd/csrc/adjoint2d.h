@@ -30,7 +30,25 @@ struct AdjK {
   // chi2/chi3 fixed point (kerr = 1): the tape holds rhs = Dz - sum P
   int kerr;
   const T *chi2, *chi3;
+  // optional gradients of chi2, chi3 and the poles' c3; ez_init = Ez before
+  // the first step of the call (c3's gradient needs the Ez each step starts from)
+  T *chi2_grad, *chi3_grad, *c3_grad;
+  const T* ez_init;
 };
+
+// Ez before step s_rel of a call: ez_init for the first step, else the tape
+// slot of the step before (under chi2/chi3 that slot is Dz - sum P, and Ez
+// is the forward's three fixed-point passes from it).
+template <typename T>
+__device__ __forceinline__ T adj_ez_before(const AdjK<T>& p, const T* ez_after, int s_rel, long long c) {
+  if (s_rel == 0) return p.ez_init[c];
+  const T v = ez_after[c - (long long)p.nx * p.ny];
+  if (!p.kerr) return v;
+  const T e0 = p.eps[c], x2 = p.chi2[c], x3 = p.chi3[c];
+  T ez = v / e0;
+  for (int it = 0; it < 3; ++it) ez = v / (e0 + x2 * ez + x3 * ez * ez);
+  return ez;
+}
 
 // Each pass below issues all of a cell's loads before its first store: the
 // adjoint, scratch and profile arrays never alias, but the compiler cannot
@@ -40,7 +58,7 @@ struct AdjK {
 // A: Ez' cotangent -> eps gradient and Dz cotangent; curl cotangent -> D-half
 // psi cotangents and the H-difference cotangents gHy, gHx.
 template <typename T>
-__device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, int i, int j) {
+__device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, int i, int j, int s_rel = 0) {
   const int nx = p.nx, ny = p.ny;
   const long long c = (long long)i * ny + j;
   const bool live = i >= 1 && i < nx - 1 && (p.y_pmc || (j >= 1 && j < ny - 1));
@@ -77,15 +95,20 @@ __device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, 
     T eb = ezb;
     rhs_b = T(0);
     eps_b = T(0);
+    T x2_b = T(0), x3_b = T(0);
     for (int k = 2; k >= 0; --k) {
       const T g = eb / den[k];      // dE(k+1)/drhs
       const T t = -g * E[k + 1];    // dE(k+1)/dden(k)
       rhs_b = rhs_b + g;
       eps_b = eps_b + t;
+      x2_b = x2_b + t * E[k];
+      x3_b = x3_b + t * (E[k] * E[k]);
       eb = t * (x2 + T(2) * x3 * E[k]);
     }
     rhs_b = rhs_b + eb / eps;
     eps_b = eps_b + (-eb * E[0]) / eps;
+    if (p.chi2_grad) p.chi2_grad[c] = p.chi2_grad[c] + x2_b;
+    if (p.chi3_grad) p.chi3_grad[c] = p.chi3_grad[c] + x3_b;
   } else {
     rhs_b = ezb / eps;
     eps_b = (-ezb * ea) / eps;
@@ -101,9 +124,12 @@ __device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, 
   if (p.np) {
     const long long ncell = (long long)nx * ny;
     const T r = rhs_b;
+    // P' = c1 P + c2 P_prev + c3 Ez(n): c3 gains P'b Ez(n)
+    const T ezn = p.c3_grad ? adj_ez_before(p, ez_after, s_rel, c) : T(0);
     for (int n = 0; n < p.np; ++n) {
       const long long o = (long long)n * ncell + c;
       const T apn = p.aPc[o] - r, app = p.aPp[o];
+      if (p.c3_grad) p.c3_grad[o] = p.c3_grad[o] + apn * ezn;
       p.aPc[o] = p.c1[o] * apn + app;
       p.aPp[o] = p.c2[o] * apn;
       ez_old = ez_old + p.c3[o] * apn;
@@ -452,6 +478,10 @@ AdjK<T> make_adjk(const mt_tm2d* d, const mt_tm2d_adj* a) {
   p.kerr = d->kerr;
   p.chi2 = static_cast<const T*>(d->chi2);
   p.chi3 = static_cast<const T*>(d->chi3);
+  p.chi2_grad = static_cast<T*>(a->chi2_grad);
+  p.chi3_grad = static_cast<T*>(a->chi3_grad);
+  p.c3_grad = p.np ? static_cast<T*>(a->c3_grad) : nullptr;
+  p.ez_init = static_cast<const T*>(a->ez_init);
   return p;
 }
 

@@ -26,10 +26,10 @@ namespace mt {
 namespace {
 
 template <typename T>
-__global__ __launch_bounds__(ABX* ABY) void adj_a_kernel(AdjK<T> p, const T* ez_after) {
+__global__ __launch_bounds__(ABX* ABY) void adj_a_kernel(AdjK<T> p, const T* ez_after, int s_rel) {
   int j = blockIdx.x * ABX + threadIdx.x;
   int i = blockIdx.y * ABY + threadIdx.y;
-  if (i < p.nx && j < p.ny) adj_a_cell(p, ez_after, i, j);
+  if (i < p.nx && j < p.ny) adj_a_cell(p, ez_after, i, j, s_rel);
 }
 
 template <typename T>
@@ -67,7 +67,7 @@ int adj_t(const mt_tm2d* d, const mt_tm2d_adj* a, const void* tape, const mt_mon
       hipLaunchKernelGGL(adj_inject_kernel<T>, dim3((n + kInjE - 1) / kInjE), dim3(kInjE * kInjS), 0, s, mm, t,
                          d->dt, s_);
     }
-    hipLaunchKernelGGL(adj_a_kernel<T>, grd, blk, 0, s, p, tp + (long long)s_ * cells);
+    hipLaunchKernelGGL(adj_a_kernel<T>, grd, blk, 0, s, p, tp + (long long)s_ * cells, s_);
     hipLaunchKernelGGL(adj_b_kernel<T>, grd, blk, 0, s, p);
     hipLaunchKernelGGL(adj_c_kernel<T>, grd, blk, 0, s, p);
     int rc = check_launch("tm2d adjoint step");
@@ -338,6 +338,10 @@ extern "C" int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const voi
   if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");
   if (p->kerr && (!p->chi2 || !p->chi3))
     return fail(MT_EINVAL, "adjoint through chi2/chi3 needs chi2 and chi3 (and a tape of Dz - sum P)");
+  if ((a->chi2_grad || a->chi3_grad) && !p->kerr)
+    return fail(MT_EINVAL, "chi2/chi3 gradients need the chi2/chi3 fixed point (kerr = 1)");
+  if (a->c3_grad && p->n_poles > 0 && k > 0 && !a->ez_init)
+    return fail(MT_EINVAL, "the c3 gradient needs ez_init (Ez before the first step)");
   if (p->n_poles < 0 || (p->n_poles && (!a->aP_cur || !a->aP_prev || !p->c1 || !p->c2 || !p->c3)))
     return fail(MT_EINVAL, "adjoint with poles needs c1, c2, c3 and the aP_cur / aP_prev cotangents");
   if (!p->d1 != !p->d2 || !p->h1x != !p->h2x || !p->h1y != !p->h2y || !p->d1 != !p->h1x || !p->d1 != !p->h1y)

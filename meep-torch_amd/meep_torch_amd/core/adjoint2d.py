@@ -106,10 +106,17 @@ def _chunk_cotangents(monitors, gacc, step0, k):
 
 
 class _TMRun(torch.autograd.Function):
+    """Differentiable in eps_inf and, when given, the chi2 / chi3 maps and the
+    poles' c3 (through which pole masks enter, coeffs.py:184-190)."""
+
     @staticmethod
-    def forward(ctx, eps, cfg, *state_in):
+    def forward(ctx, eps, chi2, chi3, c3, cfg, *state_in):
         names = cfg["names"]
-        cdet = cfg["coeffs"]._replace(eps_inf=eps.detach())
+        rep = {"eps_inf": eps.detach()}
+        for name, t in (("chi2", chi2), ("chi3", chi3), ("c3", c3)):
+            if t is not None:
+                rep[name] = t.detach()
+        cdet = cfg["coeffs"]._replace(**rep)
         work = {f: t.detach().clone().contiguous() for f, t in zip(names, state_in)}
         acc, ckpts = _forward_chunks("tm", names, work, cdet, cfg)
         ctx.cfg = cfg
@@ -117,6 +124,7 @@ class _TMRun(torch.autograd.Function):
         ctx.ckpts = ckpts
         ctx.save_for_backward(eps)
         ctx.n_state = len(state_in)
+        ctx.extra = (chi2 is not None, chi3 is not None, c3 is not None)
         return (*[work[f] for f in names], *acc)
 
     @staticmethod
@@ -144,8 +152,24 @@ class _TMRun(torch.autograd.Function):
             setattr(a, f, t.data_ptr())
         a.eps_grad = eps_grad.data_ptr()
         d, d_keep = _descriptor(cdet, adj, dtype, device)
+        kerr = bool(cdet.has_kerr or cdet.has_chi2)
+        want2, want3, want_c3 = (ctx.extra[q] and ctx.needs_input_grad[q + 1] for q in range(3))
+        extra = {}
+        if want2:
+            extra["chi2"] = torch.zeros(nx, ny, dtype=dtype, device=device)
+            if kerr:
+                a.chi2_grad = extra["chi2"].data_ptr()
+        if want3:
+            extra["chi3"] = torch.zeros(nx, ny, dtype=dtype, device=device)
+            if kerr:
+                a.chi3_grad = extra["chi3"].data_ptr()
+        if want_c3:
+            extra["c3"] = torch.zeros(cdet.c3.shape, dtype=dtype, device=device)
+            if cdet.c3.shape[0]:
+                a.c3_grad = extra["c3"].data_ptr()
         for step0, k, st in reversed(ctx.ckpts):
             work = {f: t.clone() for f, t in st.items()}
+            a.ez_init = st["Ez"].data_ptr()
             tape = torch.empty(k, nx, ny, dtype=dtype, device=device)
             engine2d.run_chunk("tm", work, cdet, step0, k, plan, [], tape=tape)
             g_chunk = _chunk_cotangents(monitors, gacc, step0, k)
@@ -156,7 +180,7 @@ class _TMRun(torch.autograd.Function):
             del keep, tape
         del d_keep
         g_state = [adj[f] for f in names][:ctx.n_state]
-        return (eps_grad.to(eps.dtype), None, *g_state)
+        return (eps_grad.to(eps.dtype), extra.get("chi2"), extra.get("chi3"), extra.get("c3"), None, *g_state)
 
 
 def _descriptor(c, fields, dtype, device):
@@ -276,7 +300,7 @@ def _te_descriptor(c, nx, ny, dtype, device):
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
     if mode == "te" and (coeffs.has_kerr or coeffs.has_chi2):
         raise NotImplementedError("gradients through nonlinear (chi2 / chi3) media are not supported")
-    allowed = ("eps_inf",) if mode == "tm" else ("eps_x", "eps_y")
+    allowed = ("eps_inf", "chi2", "chi3", "c3") if mode == "tm" else ("eps_x", "eps_y")
     for name, t in coeffs._asdict().items():
         if torch.is_tensor(t) and t.requires_grad and name not in allowed:
             raise NotImplementedError(f"gradient with respect to coefficient '{name}' is not supported")
@@ -295,6 +319,11 @@ def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_
     nx, ny = (state.Ez.shape if mode == "tm" else (state.Ey.shape[0], state.Ex.shape[1]))
     cfg["names"] = names
     cfg["plan"] = engine2d.source_plan(source, tgt.shape, tgt.device, tgt.dtype, engine2d.h_shapes(mode, nx, ny))
+    if mode == "tm":
+        # the other differentiable coefficients, when something upstream wants their gradient
+        extra = [t if (torch.is_tensor(t) and t.requires_grad) else None
+                 for t in (coeffs.chi2, coeffs.chi3, coeffs.c3)]
+        eps = (*eps, *extra)
     outs = fn.apply(*eps, cfg, *[getattr(state, f) for f in names])
     nf = len(names)
     new_state = state._replace(**dict(zip(names, outs[:nf])))
