@@ -127,12 +127,15 @@ constexpr int FTI = 8, FTJ = 64;
 template <typename T>
 __global__ __launch_bounds__(FTI * FTJ) void adj_batch_fused(const AdjK<T>* rs, const AdjK<T>* wsd,
                                                              void* const* tapes, const int* tile_pre, int nvar,
-                                                             int s_rel) {
+                                                             int s_rel, int per_xcd) {
   __shared__ T sGHy[FTI + 2][FTJ];   // rows i0-1 .. i0+FTI
   __shared__ T sGHx[FTI][FTJ + 2];   // columns j0-1 .. j0+FTJ
   __shared__ T sGEx[FTI + 1][FTJ];   // rows i0-1 .. i0+FTI-1
   __shared__ T sGEy[FTI][FTJ + 1];   // columns j0-1 .. j0+FTJ-1
-  const int b = blockIdx.x;
+  // XCD-aware order (as in tm_batch): XCD x takes the contiguous tile range
+  // [x * per_xcd, (x + 1) * per_xcd), so the tiles whose halos overlap share an L2
+  const int b = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  if (b >= tile_pre[nvar]) return;
   int lo = 0, hi = nvar;
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
@@ -430,8 +433,9 @@ int adjoint_sweep_t(mt_batch* b, int step0, int k, hipStream_t s) {
                              s_);
       }
       timing_begin(s, 2);
-      hipLaunchKernelGGL(adj_batch_fused<T>, dim3(b->fused_tiles), dim3(FTJ, FTI), 0, s, par ? k1 : k0,
-                         par ? k0 : k1, dtp, tpre, nvar, s_);
+      const int per_xcd = (b->fused_tiles + 7) / 8;
+      hipLaunchKernelGGL(adj_batch_fused<T>, dim3(8 * per_xcd), dim3(FTJ, FTI), 0, s, par ? k1 : k0,
+                         par ? k0 : k1, dtp, tpre, nvar, s_, per_xcd);
       timing_end(s, b->total_cells, 2);
       par ^= 1;
       int rc = check_launch("batched tm2d fused adjoint step");

@@ -564,7 +564,13 @@ template <typename T, int TX, int TY, int NS, bool PML>
 __global__ __launch_bounds__(BLK_TM) void tm_batch(const TmBatchVar<T>* vars, const int* prefix, int nvar,
                                                 int nsub, int parity, int t0, int slot_base, int tape_s0) {
   const int total = prefix[nvar];
-  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+  // XCD-aware order: workgroups are dealt to the eight XCDs round robin, so
+  // the blocks of XCD x (blockIdx % 8 == x) share the contiguous item range
+  // [x * chunk, (x + 1) * chunk): neighbouring tiles, whose halos overlap,
+  // run on the same XCD (same L2) at about the same time
+  const int x = blockIdx.x & 7, nx8 = ((int)gridDim.x - x + 7) >> 3;
+  const int chunk = (total + 7) >> 3, end = min(total, (x + 1) * chunk);
+  for (int item = x * chunk + (int)(blockIdx.x >> 3); item < end; item += nx8) {
     int lo = 0, hi = nvar;
     while (hi - lo > 1) {
       int mid = (lo + hi) >> 1;
