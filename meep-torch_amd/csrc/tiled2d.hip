@@ -556,13 +556,25 @@ __global__ __launch_bounds__(BLK_TM) void tm_tiled(TmTiled<T> p, TapSet taps, in
   tm_tile<T, TX, TY, NS, PML>(p, taps, nsub, tape_s0, tr, tc, 0, 0);
 }
 
+// FP32 interior tiles held to 64 VGPRs (eight waves per SIMD): two
+// 1024-thread workgroups fit a CU (their LDS allows it), so one hides the
+// other's barrier and load latency.  Ring tiles (CPML psi) and FP64 keep the
+// plain bound.
+template <typename T, int TX, int TY, int NS>
+__global__ __launch_bounds__(BLK_TM) __attribute__((amdgpu_waves_per_eu(8, 8))) void tm_tiled_in2(
+    TmTiled<T> p, TapSet taps, int nsub, int tape_s0) {
+  int tr, tc;
+  tile_of(false, p.map, &tr, &tc);
+  tm_tile<T, TX, TY, NS, false>(p, taps, nsub, tape_s0, tr, tc, 0, 0);
+}
+
 // Persistent batched form: every workgroup walks the flattened (variant,
 // tile) work list of all variants.  Per-variant problems (both ping-pong
 // orientations), taps and tile maps live in device memory, so the launch
 // depends only on scalars and can be captured in a graph and replayed.
 template <typename T, int TX, int TY, int NS, bool PML>
-__global__ __launch_bounds__(BLK_TM) void tm_batch(const TmBatchVar<T>* vars, const int* prefix, int nvar,
-                                                int nsub, int parity, int t0, int slot_base, int tape_s0) {
+__device__ __forceinline__ void tm_batch_body(const TmBatchVar<T>* vars, const int* prefix, int nvar, int nsub,
+                                              int parity, int t0, int slot_base, int tape_s0) {
   const int total = prefix[nvar];
   // XCD-aware order: workgroups are dealt to the eight XCDs round robin, so
   // the blocks of XCD x (blockIdx % 8 == x) share the contiguous item range
@@ -584,6 +596,20 @@ __global__ __launch_bounds__(BLK_TM) void tm_batch(const TmBatchVar<T>* vars, co
     tm_tile<T, TX, TY, NS, PML>(p, V.taps, nsub, tape_s0, tr, tc, slot_base, t0);
     __syncthreads();
   }
+}
+
+template <typename T, int TX, int TY, int NS, bool PML>
+__global__ __launch_bounds__(BLK_TM) void tm_batch(const TmBatchVar<T>* vars, const int* prefix, int nvar,
+                                                int nsub, int parity, int t0, int slot_base, int tape_s0) {
+  tm_batch_body<T, TX, TY, NS, PML>(vars, prefix, nvar, nsub, parity, t0, slot_base, tape_s0);
+}
+
+// the interior form held to 64 VGPRs, as tm_tiled_in2
+template <typename T, int TX, int TY, int NS>
+__global__ __launch_bounds__(BLK_TM) __attribute__((amdgpu_waves_per_eu(8, 8))) void tm_batch_in2(
+    const TmBatchVar<T>* vars, const int* prefix, int nvar, int nsub, int parity, int t0, int slot_base,
+    int tape_s0) {
+  tm_batch_body<T, TX, TY, NS, false>(vars, prefix, nvar, nsub, parity, t0, slot_base, tape_s0);
 }
 
 }  // namespace
@@ -637,7 +663,12 @@ int launch_tm_tiled(TmTiled<T> p, const TapSet& taps, int nsub, int tape_s0, hip
   p.map = make_map<TX, TY, NS>(p.nx, p.ny, p.in_i0, p.in_i1, p.in_j0, p.in_j1);
   launch_pair<T>(
       p.map,
-      [&](dim3 g) { hipLaunchKernelGGL((tm_tiled<T, TX, TY, NS, false>), g, dim3(BLK_TM), 0, s, p, taps, nsub, tape_s0); },
+      [&](dim3 g) {
+        if constexpr (sizeof(T) == 4)
+          hipLaunchKernelGGL((tm_tiled_in2<T, TX, TY, NS>), g, dim3(BLK_TM), 0, s, p, taps, nsub, tape_s0);
+        else
+          hipLaunchKernelGGL((tm_tiled<T, TX, TY, NS, false>), g, dim3(BLK_TM), 0, s, p, taps, nsub, tape_s0);
+      },
       [&](dim3 g) { hipLaunchKernelGGL((tm_tiled<T, TX, TY, NS, true>), g, dim3(BLK_TM), 0, s, p, taps, nsub, tape_s0); },
       s, (long long)TX * TY, nsub);
   return check_launch("tm2d tiled step");
@@ -649,7 +680,11 @@ int launch_tm_batch(const TmBatchVar<T>* vars, const int* pre_in, const int* pre
                     int tape_s0, hipStream_t s) {
   if (n_in) {
     timing_begin(s);
-    hipLaunchKernelGGL((tm_batch<T, TX, TY, NS, false>), dim3(std::min(n_in, grid_in)), dim3(BLK_TM), 0, s, vars,
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL((tm_batch_in2<T, TX, TY, NS>), dim3(std::min(n_in, grid_in)), dim3(BLK_TM), 0, s, vars,
+                         pre_in, nvar, nsub, parity, t0, slot_base, tape_s0);
+    else
+      hipLaunchKernelGGL((tm_batch<T, TX, TY, NS, false>), dim3(std::min(n_in, grid_in)), dim3(BLK_TM), 0, s, vars,
                        pre_in, nvar, nsub, parity, t0, slot_base, tape_s0);
     timing_end(s, (long long)n_in * TX * TY * nsub);
   }
