@@ -14,9 +14,10 @@
 //
 // Arithmetic per cell is the per-half-step kernels' (yee2d.hip), op for op,
 // so results are bit-identical to them and to the reference's FP32 stepper.
-// Tiles whose halo touches a CPML shell also carry the four psi arrays in LDS
-// (template PML = true); interior tiles carry none and run in a separate
-// launch.  Monitor samples are taken inside the tile after each step by the
+// Tiles whose halo touches a CPML shell also run the four psi recursions
+// (template PML = true), with each cell's psi values in the registers of
+// the thread that updates it (the recursion is pointwise, so psi needs no
+// LDS); interior tiles carry none and run in a separate launch.  Monitor samples are taken inside the tile after each step by the
 // tile that owns the sampled cell and written to the staging ring consumed by
 // the DFT flush (dft.hip).
 #include <hip/hip_runtime.h>
@@ -170,11 +171,12 @@ __global__ __launch_bounds__(te_block(PML)) void te_tiled(TeTiled<T> p, TapSet t
   constexpr int RX = TX + 2 * NS + 1, RY = TY + 2 * NS + 1, RC = RX * RY;
   constexpr int PER = (RC + BLK_TE - 1) / BLK_TE;
   __shared__ T sHz[RC], sDx[RC], sDy[RC], sEx[RC], sEy[RC];
-  __shared__ T sP[PML ? 4 * RC : 1];
-  T* sPzy = sP;            // psi_Hzy: Dx update, y shell (index (i, j-1))
-  T* sPzx = sP + RC;       // psi_Hzx: Dy update, x shell (index (i-1, j))
-  T* sPyx = sP + 2 * RC;   // psi_Eyx: Hz update, x shell (index (i, j))
-  T* sPxy = sP + 3 * RC;   // psi_Exy: Hz update, y shell (index (i, j))
+  // The psi recursions are pointwise (a cell's psi feeds only that cell's
+  // update), so the thread that updates a cell keeps its four psi values in
+  // registers and writes back those of its owned cells at the end.
+  constexpr int PP = PML ? PER : 1;
+  T vPzy[PP], vPzx[PP], vPyx[PP], vPxy[PP];  // psi_Hzy (Dx, y shell, (i, j-1)), psi_Hzx (Dy, x shell,
+                                             // (i-1, j)), psi_Eyx / psi_Exy (Hz, x / y shell, (i, j))
   const int nx = p.nx, ny = p.ny;
   int tr, tc;
   tile_of(PML, p.map, &tr, &tc);
@@ -214,16 +216,19 @@ __global__ __launch_bounds__(te_block(PML)) void te_tiled(TeTiled<T> p, TapSet t
       }
     }
   }
-  if (PML) {
-    for (int c = tid; c < RC; c += BLK_TE) {
+  if constexpr (PML) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      int c = tid + u * BLK_TE;
       int i = a0 + c / RY, j = b0 + c % RY;
-      bool hz = i >= 0 && i < nx - 1 && j >= 0 && j < ny - 1;
-      bool ex = i >= 0 && i < nx - 1 && j >= 0 && j < ny;
-      bool ey = i >= 0 && i < nx && j >= 0 && j < ny - 1;
-      sPzy[c] = dget(p.pHzy, (long long)i * (ny - 2) + (j - 1), ex && j >= 1 && j < ny - 1 && p.ey.in(j - 1));
-      sPzx[c] = dget(p.pHzx, (long long)(i - 1) * (ny - 1) + j, ey && i >= 1 && i < nx - 1 && p.ex.in(i - 1));
-      sPyx[c] = dget(p.pEyx, (long long)i * (ny - 1) + j, hz && p.hx.in(i));
-      sPxy[c] = dget(p.pExy, (long long)i * (ny - 1) + j, hz && p.hy.in(j));
+      bool in = c < RC;
+      bool hz = in && i >= 0 && i < nx - 1 && j >= 0 && j < ny - 1;
+      bool ex = in && i >= 0 && i < nx - 1 && j >= 0 && j < ny;
+      bool ey = in && i >= 0 && i < nx && j >= 0 && j < ny - 1;
+      vPzy[u] = dget(p.pHzy, (long long)i * (ny - 2) + (j - 1), ex && j >= 1 && j < ny - 1 && p.ey.in(j - 1));
+      vPzx[u] = dget(p.pHzx, (long long)(i - 1) * (ny - 1) + j, ey && i >= 1 && i < nx - 1 && p.ex.in(i - 1));
+      vPyx[u] = dget(p.pEyx, (long long)i * (ny - 1) + j, hz && p.hx.in(i));
+      vPxy[u] = dget(p.pExy, (long long)i * (ny - 1) + j, hz && p.hy.in(j));
     }
   }
   __syncthreads();
@@ -250,8 +255,8 @@ __global__ __launch_bounds__(te_block(PML)) void te_tiled(TeTiled<T> p, TapSet t
           T d = div_c(sHz[c] - sHz[c - 1], dx, rdx);
           T curl = d;
           if (PML && p.ey.in(j - 1)) {
-            T ps = p.ey.b[j - 1] * sPzy[c] + p.ey.a[j - 1] * d;
-            sPzy[c] = ps;
+            T ps = p.ey.b[j - 1] * vPzy[PML ? u : 0] + p.ey.a[j - 1] * d;
+            vPzy[PML ? u : 0] = ps;
             curl = d + ps;
           }
           dxv = dxv + dt * curl;
@@ -266,8 +271,8 @@ __global__ __launch_bounds__(te_block(PML)) void te_tiled(TeTiled<T> p, TapSet t
           T d = div_c(sHz[c] - sHz[c - RY], dx, rdx);
           T curl = d;
           if (PML && p.ex.in(i - 1)) {
-            T ps = p.ex.b[i - 1] * sPzx[c] + p.ex.a[i - 1] * d;
-            sPzx[c] = ps;
+            T ps = p.ex.b[i - 1] * vPzx[PML ? u : 0] + p.ex.a[i - 1] * d;
+            vPzx[PML ? u : 0] = ps;
             curl = d + ps;
           }
           dyv = dyv - dt * curl;
@@ -298,14 +303,14 @@ __global__ __launch_bounds__(te_block(PML)) void te_tiled(TeTiled<T> p, TapSet t
         T dEx = div_c(sEx[c + 1] - sEx[c], dx, rdx);
         T a1 = dEy;
         if (PML && p.hx.in(i)) {
-          T ps = p.hx.b[i] * sPyx[c] + p.hx.a[i] * dEy;
-          sPyx[c] = ps;
+          T ps = p.hx.b[i] * vPyx[PML ? u : 0] + p.hx.a[i] * dEy;
+          vPyx[PML ? u : 0] = ps;
           a1 = dEy + ps;
         }
         T curl = a1 - dEx;
         if (PML && p.hy.in(j)) {
-          T ps = p.hy.b[j] * sPxy[c] + p.hy.a[j] * dEx;
-          sPxy[c] = ps;
+          T ps = p.hy.b[j] * vPxy[PML ? u : 0] + p.hy.a[j] * dEx;
+          vPxy[PML ? u : 0] = ps;
           curl = curl - ps;
         }
         sHz[c] = sHz[c] - dt * curl;
@@ -327,22 +332,33 @@ __global__ __launch_bounds__(te_block(PML)) void te_tiled(TeTiled<T> p, TapSet t
     if (i < nx - 1 && j < ny - 1) {
       long long h = (long long)i * (ny - 1) + j;
       p.oHz[h] = sHz[lc];
-      if (PML) {
-        if (p.hx.in(i)) p.oEyx[h] = sPyx[lc];
-        if (p.hy.in(j)) p.oExy[h] = sPxy[lc];
-      }
     }
     if (i < nx - 1 && j < ny) {
       long long h = (long long)i * ny + j;
       p.oDx[h] = sDx[lc];
       if (write_e) p.Ex[h] = sEx[lc];
-      if (PML && j >= 1 && j < ny - 1 && p.ey.in(j - 1)) p.oHzy[(long long)i * (ny - 2) + (j - 1)] = sPzy[lc];
     }
     if (i < nx && j < ny - 1) {
       long long h = (long long)i * (ny - 1) + j;
       p.oDy[h] = sDy[lc];
       if (write_e) p.Ey[h] = sEy[lc];
-      if (PML && i >= 1 && i < nx - 1 && p.ex.in(i - 1)) p.oHzx[(long long)(i - 1) * (ny - 1) + j] = sPzx[lc];
+    }
+  }
+  if constexpr (PML) {  // psi of the owned cells, from the registers of the threads that updated them
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      int c = tid + u * BLK_TE;
+      int i = a0 + c / RY, j = b0 + c % RY;
+      if (c >= RC || i < I0 || i >= I0 + TX || j < J0 || j >= J0 + TY) continue;
+      if (i < nx - 1 && j < ny - 1) {
+        long long h = (long long)i * (ny - 1) + j;
+        if (p.hx.in(i)) p.oEyx[h] = vPyx[u];
+        if (p.hy.in(j)) p.oExy[h] = vPxy[u];
+      }
+      if (i < nx - 1 && j < ny && j >= 1 && j < ny - 1 && p.ey.in(j - 1))
+        p.oHzy[(long long)i * (ny - 2) + (j - 1)] = vPzy[u];
+      if (i < nx && j < ny - 1 && i >= 1 && i < nx - 1 && p.ex.in(i - 1))
+        p.oHzx[(long long)(i - 1) * (ny - 1) + j] = vPzx[u];
     }
   }
 }
@@ -354,11 +370,10 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
   constexpr int RX = TX + 2 * NS + 1, RY = TY + 2 * NS + 1, RC = RX * RY;
   constexpr int PER = (RC + BLK_TM - 1) / BLK_TM;
   __shared__ T sEz[RC], sDz[RC], sHx[RC], sHy[RC];
-  __shared__ T sP[PML ? 4 * RC : 1];
-  T* sPzy = sP;            // psi_Ezy: Hx update, y shell (index (i, j))
-  T* sPzx = sP + RC;       // psi_Ezx: Hy update, x shell (index (i, j))
-  T* sPyx = sP + 2 * RC;   // psi_Hyx: Dz update, x shell (index (i-1, jj))
-  T* sPxy = sP + 3 * RC;   // psi_Hxy: Dz update, y shell (index (i-1, jj))
+  // psi in the registers of the thread that updates the cell (as in te_tiled)
+  constexpr int PP = PML ? PER : 1;
+  T vPzy[PP], vPzx[PP], vPyx[PP], vPxy[PP];  // psi_Ezy (Hx, y shell, (i, j)), psi_Ezx (Hy, x shell, (i, j)),
+                                             // psi_Hyx / psi_Hxy (Dz, x / y shell, (i-1, jj))
   const int nx = p.nx, ny = p.ny, ny_e = p.ny_e;
   const int I0 = tr * TX, J0 = tc * TY;
   const int a0 = I0 - NS - 1, b0 = J0 - NS - 1;
@@ -392,19 +407,21 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
       }
     }
   }
-  if (PML) {
-    for (int c = tid; c < RC; c += BLK_TM) {
+  if constexpr (PML) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      int c = tid + u * BLK_TM;
       int i = a0 + c / RY, j = b0 + c % RY;
-      bool ez = i >= 0 && i < nx && j >= 0 && j < ny;
+      bool ez = c < RC && i >= 0 && i < nx && j >= 0 && j < ny;
       bool hx = ez && j < ny - 1;
       bool hy = ez && i < nx - 1;
-      sPzy[c] = dget(p.pEzy, (long long)i * (ny - 1) + j, hx && p.hy.in(j));
-      sPzx[c] = dget(p.pEzx, (long long)i * ny + j, hy && p.hx.in(i));
+      vPzy[u] = dget(p.pEzy, (long long)i * (ny - 1) + j, hx && p.hy.in(j));
+      vPzx[u] = dget(p.pEzx, (long long)i * ny + j, hy && p.hx.in(i));
       int jj = j - joff;
       bool live = ez && i >= 1 && i < nx - 1 && jj >= 0 && jj < ny_e;
       long long q = (long long)(i - 1) * ny_e + jj;
-      sPyx[c] = dget(p.pHyx, q, live && p.ex.in(i - 1));
-      sPxy[c] = dget(p.pHxy, q, live && p.ey.in(jj));
+      vPyx[u] = dget(p.pHyx, q, live && p.ex.in(i - 1));
+      vPxy[u] = dget(p.pHxy, q, live && p.ey.in(jj));
     }
   }
   __syncthreads();
@@ -436,8 +453,8 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
         T d = div_c(sEz[c + 1] - ez, dx, rdx);
         T curl = d;
         if (PML && p.hy.in(j)) {
-          T ps = p.hy.b[j] * sPzy[c] + p.hy.a[j] * d;
-          sPzy[c] = ps;
+          T ps = p.hy.b[j] * vPzy[PML ? u : 0] + p.hy.a[j] * d;
+          vPzy[PML ? u : 0] = ps;
           curl = d + ps;
         }
         T hxv = sHx[c] - dt * curl;
@@ -451,8 +468,8 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
         T d = div_c(sEz[c + RY] - ez, dx, rdx);
         T curl = d;
         if (PML && p.hx.in(i)) {
-          T ps = p.hx.b[i] * sPzx[c] + p.hx.a[i] * d;
-          sPzx[c] = ps;
+          T ps = p.hx.b[i] * vPzx[PML ? u : 0] + p.hx.a[i] * d;
+          vPzx[PML ? u : 0] = ps;
           curl = d + ps;
         }
         T hyv = sHy[c] + dt * curl;
@@ -487,14 +504,14 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
         int ii = i - 1, jj = j - joff;
         T a1 = dHy;
         if (PML && p.ex.in(ii)) {
-          T ps = p.ex.b[ii] * sPyx[c] + p.ex.a[ii] * dHy;
-          sPyx[c] = ps;
+          T ps = p.ex.b[ii] * vPyx[PML ? u : 0] + p.ex.a[ii] * dHy;
+          vPyx[PML ? u : 0] = ps;
           a1 = dHy + ps;
         }
         T curl = a1 - dHx;
         if (PML && p.ey.in(jj)) {
-          T ps = p.ey.b[jj] * sPxy[c] + p.ey.a[jj] * dHx;
-          sPxy[c] = ps;
+          T ps = p.ey.b[jj] * vPxy[PML ? u : 0] + p.ey.a[jj] * dHx;
+          vPxy[PML ? u : 0] = ps;
           curl = curl - ps;
         }
         dz = dz + dt * curl;
@@ -529,21 +546,22 @@ __device__ __forceinline__ void tm_tile(const TmTiled<T>& p, const TapSet& taps,
     long long e = (long long)i * ny + j;
     p.oEz[e] = sEz[lc];
     p.oDz[e] = sDz[lc];
-    if (j < ny - 1) {
-      long long h = (long long)i * (ny - 1) + j;
-      p.oHx[h] = sHx[lc];
-      if (PML && p.hy.in(j)) p.oEzy[h] = sPzy[lc];
-    }
-    if (i < nx - 1) {
-      p.oHy[e] = sHy[lc];
-      if (PML && p.hx.in(i)) p.oEzx[e] = sPzx[lc];
-    }
-    if (PML) {
+    if (j < ny - 1) p.oHx[(long long)i * (ny - 1) + j] = sHx[lc];
+    if (i < nx - 1) p.oHy[e] = sHy[lc];
+  }
+  if constexpr (PML) {  // psi of the owned cells, from the registers of the threads that updated them
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      int c = tid + u * BLK_TM;
+      int i = a0 + c / RY, j = b0 + c % RY;
+      if (c >= RC || i < I0 || i >= I0 + TX || j < J0 || j >= J0 + TY || i >= nx || j >= ny) continue;
+      if (j < ny - 1 && p.hy.in(j)) p.oEzy[(long long)i * (ny - 1) + j] = vPzy[u];
+      if (i < nx - 1 && p.hx.in(i)) p.oEzx[(long long)i * ny + j] = vPzx[u];
       int jj = j - joff;
       if (i >= 1 && i < nx - 1 && jj >= 0 && jj < ny_e) {
         long long q = (long long)(i - 1) * ny_e + jj;
-        if (p.ex.in(i - 1)) p.oHyx[q] = sPyx[lc];
-        if (p.ey.in(jj)) p.oHxy[q] = sPxy[lc];
+        if (p.ex.in(i - 1)) p.oHyx[q] = vPyx[u];
+        if (p.ey.in(jj)) p.oHxy[q] = vPxy[u];
       }
     }
   }
