@@ -150,9 +150,17 @@ typedef struct {
 /* 3D problem + state (stepper_3d.py:28-67 Coeffs3D / State3D, CPML path;
  * material conductivity and ADE poles through the media fields at the end).
  * lean = 1 selects the in-place lean stepper's E update and source sign
- * (stepper_3d_lean.py:234-296). */
+ * (stepper_3d_lean.py:234-296).  absorber = 1 selects the path of a run
+ * without CPML (stepper_3d.py:328-361: Coeffs3D.cpml is None, the boundary
+ * is the matched-sigma absorber folded into the damping pairs, or plain PEC):
+ *   H = d1 H - (d2 (dE_a - dE_b)) / dx,  E = (d1 E eps_c + d2 (dH_a - dH_b) / dx) / eps_c
+ * (poles as below); it needs the 12 damping arrays, lean = 0, every profile
+ * with lo_end = 0 and hi_begin = n (b, a and psi may be NULL), and runs the
+ * two-pass kernels.  A CPML run whose absorber axes carry the reference's
+ * identity profile (b = a = 0, k = 1) is an ordinary absorber = 0 run whose
+ * shells cover those axes. */
 typedef struct {
-  int nx, ny, nz, dtype, lean, pad_;
+  int nx, ny, nz, dtype, lean, absorber;
   double dt, dx;
   const void* eps;                        /* device (nx,ny,nz) cell-centred */
   mt_pml_axis h[3];                       /* per axis, length n-1 (offset 1/2) */

@@ -309,6 +309,7 @@ extern "C" int mt_yee3d_adjoint(const mt_yee3d* p, const mt_yee3d_adj* a, const 
   if ((long long)p->ny * p->nz > (1LL << 31) || p->nx > 65535)
     return fail(MT_EINVAL, "grid too large for the launch shape");
   if (p->lean) return fail(MT_EINVAL, "the 3D adjoint transposes the functional stepper (lean = 0)");
+  if (p->absorber) return fail(MT_EINVAL, "the 3D adjoint transposes the CPML path (absorber = 0)");
   if (p->n_poles < 0) return fail(MT_EINVAL, "negative pole count");
   if (p->n_poles) {
     if (!p->damp[0] || !p->c1 || !p->c2) return fail(MT_EINVAL, "3D adjoint with poles needs damp, c1 and c2");

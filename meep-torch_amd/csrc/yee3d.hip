@@ -81,7 +81,10 @@ __device__ __forceinline__ T ce_apply(const CeIn<T>& c, T* psi, long long q, T d
 // cell.  Reads E and the old H from Ex..Hz and writes the new H to oHx..oHz
 // (the same buffers for the in-place two-pass step; the ping-pong buffers of
 // the split step).
-template <typename T, bool M>
+// A (with M): the matched-absorber path of a run without CPML
+// (stepper_3d.py:329-335): H = d1 H - (d2 (dE_a - dE_b)) / dx, one division
+// per component, and no psi.
+template <typename T, bool M, bool A = false>
 __device__ __forceinline__ void h3_cell(const K3<T>& p, int i, int j, int k) {
   const int nx = p.nx, ny = p.ny, nz = p.nz;
   const T dx = p.dx, dt = p.dt, rdx = p.rdx;
@@ -113,9 +116,6 @@ __device__ __forceinline__ void h3_cell(const K3<T>& p, int i, int j, int k) {
     exj = p.Ex[ie + nz];
     hz = p.Hz[iz];
   }
-  const CeIn<T> cxy = ce_load(p.h[1], j, fx, p.psi[P_EZ_Y], ix), cxz = ce_load(p.h[2], k, fx, p.psi[P_EY_Z], ix);
-  const CeIn<T> cyz = ce_load(p.h[2], k, fy, p.psi[P_EX_Z], iy), cyx = ce_load(p.h[0], i, fy, p.psi[P_EZ_X], iy);
-  const CeIn<T> czx = ce_load(p.h[0], i, fz, p.psi[P_EY_X], iz), czy = ce_load(p.h[1], j, fz, p.psi[P_EX_Y], iz);
   // conductivity pairs: H = d1 H - d2 curl (stepper_3d.py:375,382,389); at
   // sigma = 0 the reference's pairs are exactly (1, dt)
   T d1x = T(1), d2x = dt, d1y = T(1), d2y = dt, d1z = T(1), d2z = dt;
@@ -124,6 +124,16 @@ __device__ __forceinline__ void h3_cell(const K3<T>& p, int i, int j, int k) {
     if (fy) d1y = p.dmp[8][iy], d2y = p.dmp[9][iy];
     if (fz) d1z = p.dmp[10][iz], d2z = p.dmp[11][iz];
   }
+  if constexpr (A) {
+    static_assert(M, "the absorber path carries the damping pairs");
+    if (fx) p.oHx[ix] = d1x * hx - div_c(d2x * ((ezj - ez0) - (eyk - ey0)), dx, rdx);
+    if (fy) p.oHy[iy] = d1y * hy - div_c(d2y * ((exk - ex0) - (ezx - ez0)), dx, rdx);
+    if (fz) p.oHz[iz] = d1z * hz - div_c(d2z * ((eyx - ey0) - (exj - ex0)), dx, rdx);
+    return;
+  }
+  const CeIn<T> cxy = ce_load(p.h[1], j, fx, p.psi[P_EZ_Y], ix), cxz = ce_load(p.h[2], k, fx, p.psi[P_EY_Z], ix);
+  const CeIn<T> cyz = ce_load(p.h[2], k, fy, p.psi[P_EX_Z], iy), cyx = ce_load(p.h[0], i, fy, p.psi[P_EZ_X], iy);
+  const CeIn<T> czx = ce_load(p.h[0], i, fz, p.psi[P_EY_X], iz), czy = ce_load(p.h[1], j, fz, p.psi[P_EX_Y], iz);
   // ---- arithmetic and stores (stepper_3d.py:363-390)
   if (fx) {
     T cy = ce_apply(cxy, p.psi[P_EZ_Y], ix, div_c(ezj - ez0, dx, rdx));
@@ -142,12 +152,12 @@ __device__ __forceinline__ void h3_cell(const K3<T>& p, int i, int j, int k) {
   }
 }
 
-template <typename T, bool M>
+template <typename T, bool M, bool A = false>
 __global__ __launch_bounds__(BZ* BYY) void h3_kernel(K3<T> p) {
   const int k = blockIdx.x * BZ + threadIdx.x;
   const int j = blockIdx.y * BYY + threadIdx.y;
   if (j >= p.ny || k >= p.nz) return;
-  h3_cell<T, M>(p, blockIdx.z, j, k);
+  h3_cell<T, M, A>(p, blockIdx.z, j, k);
 }
 
 template <typename T, typename P>
@@ -197,8 +207,10 @@ __device__ __forceinline__ T inject(const P& p, const mt_source3d& s, int comp, 
 }
 
 // E half-step of one cell: reads the new H from oHx..oHz and the old E from
-// Ex..Ez, writes the new E to oEx..oEz (same buffers when in place).
-template <typename T, bool M>
+// Ex..Ez, writes the new E to oEx..oEz (same buffers when in place).  A: the
+// absorber path's curl, (dH_a - dH_b) / dx with one division and no psi
+// (stepper_3d.py:336-360).
+template <typename T, bool M, bool A = false>
 __device__ __forceinline__ void e3_cell(const K3<T>& p, const mt_source3d& src, T c, T* tape, int i, int j,
                                         int k) {
   const int nx = p.nx, ny = p.ny, nz = p.nz;
@@ -235,6 +247,31 @@ __device__ __forceinline__ void e3_cell(const K3<T>& p, const mt_source3d& src, 
   if (ay && az) hx0 = Hx[ihx];
   if (ay && kz) hxk = Hx[ihx - 1];
   if (az && jy) hxj = Hx[ihx - (nz - 1)];
+  if constexpr (A) {
+    static_assert(M, "the absorber path carries the damping pairs");
+    if (ax) {
+      T curl = (jy && kz) ? div_c((hz0 - hzj) - (hy0 - hyk), dx, rdx) : T(0);
+      T epsc = T(0.5) * (eps0 + epx);
+      T E = inject(p, src, 0, i, j, k, e_media(p, 0, ce_, (long long)(nx - 1) * ny * nz, ex, epsc, curl), epsc, c);
+      p.oEx[ce_] = E;
+      if (tape) tape[ce_] = E;
+    }
+    if (ay) {
+      T curl = (kz && ix) ? div_c((hx0 - hxk) - (hz0 - hzx), dx, rdx) : T(0);
+      T epsc = T(0.5) * (eps0 + epy);
+      T E = inject(p, src, 1, i, j, k, e_media(p, 1, iy, (long long)nx * (ny - 1) * nz, ey, epsc, curl), epsc, c);
+      p.oEy[iy] = E;
+      if (tape) tape[(long long)(nx - 1) * ny * nz + iy] = E;
+    }
+    if (az) {
+      T curl = (ix && jy) ? div_c((hy0 - hyx) - (hx0 - hxj), dx, rdx) : T(0);
+      T epsc = T(0.5) * (eps0 + epz);
+      T E = inject(p, src, 2, i, j, k, e_media(p, 2, iz, (long long)nx * ny * (nz - 1), ez, epsc, curl), epsc, c);
+      p.oEz[iz] = E;
+      if (tape) tape[(long long)(nx - 1) * ny * nz + (long long)nx * (ny - 1) * nz + iz] = E;
+    }
+    return;
+  }
   const long long q_xy = ((long long)i * (ny - 2) + (j - 1)) * nz + k;
   const long long q_xz = ((long long)i * ny + j) * (nz - 2) + (k - 1);
   const long long q_yz = ((long long)i * (ny - 1) + j) * (nz - 2) + (k - 1);
@@ -283,12 +320,12 @@ __device__ __forceinline__ void e3_cell(const K3<T>& p, const mt_source3d& src, 
   }
 }
 
-template <typename T, bool M>
+template <typename T, bool M, bool A = false>
 __global__ __launch_bounds__(BZ* BYY) void e3_kernel(K3<T> p, mt_source3d src, T c, T* tape) {
   const int k = blockIdx.x * BZ + threadIdx.x;
   const int j = blockIdx.y * BYY + threadIdx.y;
   if (j >= p.ny || k >= p.nz) return;
-  e3_cell<T, M>(p, src, c, tape, blockIdx.z, j, k);
+  e3_cell<T, M, A>(p, src, c, tape, blockIdx.z, j, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -793,7 +830,8 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
   if (src && src->kind) sv = *src;
   const Box3 box = interior_box(d);
   const bool small = (long long)d->ny * d->nz * (long long)sizeof(T) < (1LL << 31);  // 32-bit in-plane offsets
-  const bool split = d->variant >= 1 && !media && small && box.I1 > box.I0 && box.J1 > box.J0 && box.K1 > box.K0;
+  const bool split = d->variant >= 1 && !media && !d->absorber && small && box.I1 > box.I0 && box.J1 > box.J0 &&
+                     box.K1 > box.K0;
   // tile rows: FP32 6 / 10 / 4, FP64 4 (the LDS rings must fit 64 KB)
   if (split) {
     if constexpr (sizeof(T) == 4) {
@@ -812,7 +850,10 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
     timing_begin(s);
     T c = sv.kind ? static_cast<T>(coef[s_]) : T(0);
     T* slot = d->tape ? static_cast<T*>(d->tape) + (long long)s_ * e_cells : nullptr;
-    if (media) {
+    if (d->absorber) {
+      hipLaunchKernelGGL((h3_kernel<T, true, true>), grd, blk, 0, s, p);
+      hipLaunchKernelGGL((e3_kernel<T, true, true>), grd, blk, 0, s, p, sv, c, slot);
+    } else if (media) {
       hipLaunchKernelGGL((h3_kernel<T, true>), grd, blk, 0, s, p);
       hipLaunchKernelGGL((e3_kernel<T, true>), grd, blk, 0, s, p, sv, c, slot);
     } else {
@@ -850,13 +891,21 @@ extern "C" int mt_yee3d_run(const mt_yee3d* p, const mt_source3d* src, const dou
   if (p->dtype != MT_F32 && p->dtype != MT_F64) return fail(MT_EINVAL, "dtype must be f32 or f64");
   if (!p->eps || !p->Ex || !p->Ey || !p->Ez || !p->Hx || !p->Hy || !p->Hz)
     return fail(MT_EINVAL, "null field pointer");
-  for (int q = 0; q < 12; ++q)
-    if (!p->psi[q]) return fail(MT_EINVAL, "null psi pointer");
+  if (p->absorber != 0 && p->absorber != 1) return fail(MT_EINVAL, "absorber must be 0 or 1");
   int n[3] = {p->nx, p->ny, p->nz};
   int rc;
-  for (int a = 0; a < 3; ++a) {
-    if ((rc = check_axis(p->h[a], n[a] - 1, "h"))) return rc;
-    if ((rc = check_axis(p->e[a], n[a] - 2, "e"))) return rc;
+  if (p->absorber) {  // no CPML: psi unused, every axis without shells
+    for (int a = 0; a < 3; ++a)
+      if (p->h[a].n != n[a] - 1 || p->e[a].n != n[a] - 2 || p->h[a].lo_end != 0 || p->e[a].lo_end != 0 ||
+          p->h[a].hi_begin != p->h[a].n || p->e[a].hi_begin != p->e[a].n)
+        return fail(MT_EINVAL, "absorber run: every profile must have its length and no shell");
+  } else {
+    for (int q = 0; q < 12; ++q)
+      if (!p->psi[q]) return fail(MT_EINVAL, "null psi pointer");
+    for (int a = 0; a < 3; ++a) {
+      if ((rc = check_axis(p->h[a], n[a] - 1, "h"))) return rc;
+      if ((rc = check_axis(p->e[a], n[a] - 2, "e"))) return rc;
+    }
   }
   if (src && src->kind) {
     if (src->kind != 1 && src->kind != 2) return fail(MT_EINVAL, "source kind must be 1 or 2");
@@ -882,6 +931,8 @@ extern "C" int mt_yee3d_run(const mt_yee3d* p, const mt_source3d* src, const dou
   if (any_damp && !all_damp) return fail(MT_EINVAL, "3D media need all 12 damping arrays");
   if (p->n_poles < 0 || p->n_poles > 64) return fail(MT_EINVAL, "3D n_poles must be 0..64");
   if (p->n_poles > 0 && !any_damp) return fail(MT_EINVAL, "3D poles need the damping arrays (d1 = 1, d2 = dt)");
+  if (p->absorber && (!any_damp || p->lean))
+    return fail(MT_EINVAL, "absorber run needs the 12 damping arrays and the functional stepper");
   if (any_damp) {
     if (p->lean) return fail(MT_EINVAL, "3D media are not supported by the lean stepper");
     if (p->n_poles > 0) {

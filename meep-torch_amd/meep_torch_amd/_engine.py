@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 20
+ABI_VERSION = 21
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -79,7 +79,7 @@ class TE2DAdj(ctypes.Structure):
 
 class Yee3D(ctypes.Structure):
     _fields_ = [("nx", c_int), ("ny", c_int), ("nz", c_int), ("dtype", c_int), ("lean", c_int),
-                ("pad_", c_int), ("dt", c_double), ("dx", c_double), ("eps", c_void_p),
+                ("absorber", c_int), ("dt", c_double), ("dx", c_double), ("eps", c_void_p),
                 ("h", PmlAxis * 3), ("e", PmlAxis * 3),
                 ("Ex", c_void_p), ("Ey", c_void_p), ("Ez", c_void_p),
                 ("Hx", c_void_p), ("Hy", c_void_p), ("Hz", c_void_p),

@@ -54,6 +54,9 @@ def run_3d_adjoint(c, n_steps: int, chunk: int, source, monitors, loss_fn, state
     from .adjoint2d import _adj_monitors
     if chunk < 1:
         raise ValueError("chunk must be >= 1")
+    if c.cpml is None:
+        raise NotImplementedError("the hand-written 3D adjoint transposes the CPML path; a run without CPML "
+                                  "(matched-absorber path) has none")
     _refuse_media_grads(c)
     L = E.lib()
     monitors = list(monitors)

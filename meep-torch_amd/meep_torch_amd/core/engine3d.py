@@ -64,6 +64,11 @@ def _axis(c, key):
     return E.pml_axis(b, a, c.shells[key] if c.shells else None)
 
 
+def _no_shell(n):
+    """Profile of an axis without CPML (absorber runs): no arrays, no shell."""
+    return E.PmlAxis(None, None, n, 0, n)
+
+
 def _media(d, c, fields, dtype, device):
     """Fill the descriptor's media fields from Coeffs3D (the (d1, d2) pairs of
     c.damp and the pole scalars / c3 stacks of c.pols, stepper_3d.py:217-266);
@@ -113,8 +118,9 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=N
     4) + shell cells (default: DEFAULT_VARIANT)."""
     from .engine2d import finish_outputs, monitor_structs
     L = E.lib()
-    if c.cpml is None:
-        raise NotImplementedError("3D run without CPML is not supported by the HIP stepper")
+    absorber = c.cpml is None   # stepper_3d.py:328-361: the matched-absorber path
+    if absorber and lean:
+        raise NotImplementedError("the lean stepper runs the CPML path only")
     ref = fields["Ex"]
     dtype, device = ref.dtype, ref.device
     code = E.dtype_code(dtype)
@@ -125,19 +131,24 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=N
     nx, ny, nz = fields["Ey"].shape[0], fields["Ex"].shape[1], fields["Ex"].shape[2]
     d = E.Yee3D()
     d.nx, d.ny, d.nz, d.dtype, d.lean = nx, ny, nz, code, int(bool(lean))
+    d.absorber = int(absorber)
     d.dt, d.dx = float(c.dt), float(c.dx)
     eps = c.eps.to(device=device, dtype=dtype).contiguous()
     d.eps = eps.data_ptr()
     for a, (kh, ke) in enumerate((("hx", "ex"), ("hy", "ey"), ("hz", "ez"))):
-        d.h[a] = _axis(c, kh)
-        d.e[a] = _axis(c, ke)
+        n = (nx, ny, nz)[a]
+        d.h[a] = _no_shell(n - 1) if absorber else _axis(c, kh)
+        d.e[a] = _no_shell(n - 2) if absorber else _axis(c, ke)
     for comp in COMPS:
         setattr(d, comp, fields[comp].data_ptr())
-    for q, name in enumerate(PSI_NAMES):
-        d.psi[q] = fields[name].data_ptr()
+    if not absorber:
+        for q, name in enumerate(PSI_NAMES):
+            if fields.get(name) is None:
+                raise ValueError(f"a CPML run needs the psi array {name} in the state")
+            d.psi[q] = fields[name].data_ptr()
     d.tape = tape.data_ptr() if tape is not None else None
     v = DEFAULT_VARIANT if variant is None else int(variant)
-    media = _media(d, c, fields, dtype, device) if has_media(c) else []
+    media = _media(d, c, fields, dtype, device) if (has_media(c) or absorber) else []
     if media:
         if lean:
             raise NotImplementedError("3D media (sigma, poles) are not supported by the lean stepper")

@@ -137,8 +137,8 @@ class FluxMonitor3D:
 def run_3d(state, coeffs, n_steps: int, *, source=None, monitors=()):
     """monitors_3d.py:192-210 on the HIP engine (functional: the input state is
     cloned)."""
-    if state.psi_Ez_y is None:
-        raise NotImplementedError("3D state without CPML memory (absorber path) is not supported")
+    if state.psi_Ez_y is None and coeffs.cpml is not None:
+        raise ValueError("a CPML run needs a state with the CPML memory (init_3d_state(npml > 0))")
     with torch.no_grad():
         work = {f: getattr(state, f).detach().clone().contiguous() for f in state_names(state)}
         outs = run_fields(work, coeffs, n_steps, source, list(monitors), lean=False)

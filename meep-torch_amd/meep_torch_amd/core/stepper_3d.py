@@ -2,16 +2,17 @@
 
 Coeffs3D / State3D keep the reference fields and shapes (State3D psi arrays are
 full-grid; the kernels touch them only inside the CPML shells).  The HIP
-stepper runs the CPML path, with material conductivity (sigma_d, sigma_b) and
-Lorentz / Drude ADE poles on the two-pass kernels; the matched-absorber path
-and PMC walls raise NotImplementedError."""
+stepper runs the CPML path, the matched-sigma absorber path of a run without
+CPML, and the mixed CPML + absorber mode, with material conductivity
+(sigma_d, sigma_b) and Lorentz / Drude ADE poles on the two-pass kernels; PMC
+walls raise NotImplementedError."""
 import math
 from collections.abc import Mapping
 from typing import NamedTuple
 
 import torch
 
-from .pml import cpml_profiles, side_counts
+from .pml import absorber_sigma, cpml_profiles, side_counts
 from .._engine import shell_bounds
 
 
@@ -83,15 +84,17 @@ class _Damp(Mapping):
     (all-ones / all-dt at sigma = 0).  ``active`` tells whether material
     conductivity was given."""
 
-    def __init__(self, eps, dt, sigma_d=None, sigma_b=None):
+    def __init__(self, eps, dt, sigma_d=None, sigma_b=None, sig_bound=None):
         self._eps, self._dt, self._cache = eps, dt, {}
-        self._sd, self._sb = sigma_d, sigma_b
-        self.active = sigma_d is not None or sigma_b is not None
+        self._sd, self._sb, self._bound = sigma_d, sigma_b, sig_bound
+        self.active = sigma_d is not None or sigma_b is not None or sig_bound is not None
 
     def _pair(self, key):
         e = self._eps
         src = self._sd if key[0] == "e" else self._sb
         s = (src.to(dtype=e.dtype, device=e.device) if src is not None else torch.zeros_like(e))
+        if self._bound is not None:   # boundary absorber sigma + material sigma (stepper_3d.py:224-225)
+            s = self._bound.to(dtype=e.dtype, device=e.device) + s
         dt = self._dt
         avg = {"ex": lambda v: _avg(v, 0), "ey": lambda v: _avg(v, 1), "ez": lambda v: _avg(v, 2),
                "hx": lambda v: _avg(_avg(v, 1), 2), "hy": lambda v: _avg(_avg(v, 0), 2),
@@ -114,35 +117,63 @@ def build_3d_coeffs(nx: int, ny: int, nz: int, dx: float, eps=None, npml=0, n_ab
                     absorber_axes=(), sigma_d=None, sigma_b=None, poles=(), pole_masks=None,
                     dtype=torch.float64, device="cuda", absorber_m: int = 2,
                     absorber_R0: float = 1e-15, pmc_axes=()) -> Coeffs3D:
-    """stepper_3d.py:150-266, CPML subset: dt = 0.5 dx / sqrt(3); per axis an
-    H-side profile of length n-1 at offset 1/2 and an E-side profile of
-    length n at offset 0 trimmed to [1:-1].  sigma_d / sigma_b: cell-centred
-    (nx, ny, nz) conductivities (stepper_3d.py:217-235); poles: Lorentz /
-    Drude (fn, gamma, sigma[, fc]) with cell-centred pole_masks, giving per
-    E component c1, c2 of shape (np, 1, 1, 1) and c3 = v3 * (mask averaged
-    along the component's axis) (stepper_3d.py:237-266)."""
+    """stepper_3d.py:150-266: dt = 0.5 dx / sqrt(3).  Boundary modes
+    (stepper_3d.py:174-215): CPML (npml > 0, n_abs = 0): per axis an H-side
+    profile of length n-1 at offset 1/2 and an E-side profile of length n at
+    offset 0 trimmed to [1:-1]; matched-sigma absorber (npml = 0): no CPML
+    (cpml None, the absorber path of step_3d) and the conductivity
+    sig_bound = max over axes of the absorber_sigma ramps (zero with
+    n_abs = 0: a plain PEC box); mixed (both): CPML on the axes outside
+    absorber_axes, the identity profile (b = a = 0, k = 1: psi stays zero, a
+    plain derivative) on the axes in it, and sig_bound = max of their ramps.
+    sig_bound adds to the material sigma_d and sigma_b.  npml / n_abs: an int
+    for every face, or per axis an int or (lo, hi) pair.  sigma_d / sigma_b:
+    cell-centred (nx, ny, nz) conductivities (stepper_3d.py:217-235); poles:
+    Lorentz / Drude (fn, gamma, sigma[, fc]) with cell-centred pole_masks,
+    giving per E component c1, c2 of shape (np, 1, 1, 1) and c3 = v3 * (mask
+    averaged along the component's axis) (stepper_3d.py:237-266)."""
     from .coeffs import pole_recurrence
-    if any(l or h for l, h in _axis_sides3(n_abs)):
-        raise NotImplementedError("3D matched-sigma absorber (n_abs) is not supported by the HIP stepper")
     if len(pmc_axes):
         raise NotImplementedError("3D PMC walls are not supported by the HIP stepper")
-    npml_ax = _axis_sides3(npml)
-    if not any(l or h for l, h in npml_ax):
-        raise NotImplementedError("3D run without CPML (absorber-path arithmetic) is not supported")
+    npml_ax, nabs_ax = _axis_sides3(npml), _axis_sides3(n_abs)
+    has_pml = any(l or h for l, h in npml_ax)
+    has_abs = any(l or h for l, h in nabs_ax)
+    mixed = has_pml and has_abs
+    ident = set(int(a) for a in absorber_axes) if mixed else set()
+    if not ident <= {0, 1, 2}:
+        raise ValueError(f"absorber_axes must name axes 0, 1, 2, got {tuple(absorber_axes)}")
     dt = 0.5 * dx / math.sqrt(3.0)
     e = (eps if isinstance(eps, torch.Tensor) else torch.ones(nx, ny, nz, dtype=dtype))
     e = e.to(dtype=dtype, device=device)
     dims = (nx, ny, nz)
-    cpml, shells = {}, {}
-    for ax, kh, ke in ((0, "hx", "ex"), (1, "hy", "ey"), (2, "hz", "ez")):
-        n = dims[ax]
-        b, a, k = cpml_profiles(n - 1, npml_ax[ax], 0.5, dt, dx, dtype=dtype)
-        shells[kh] = shell_bounds(b, a)
-        cpml[kh] = tuple(_bc(v.to(device), ax) for v in (b, a, k))
-        b, a, k = cpml_profiles(n, npml_ax[ax], 0.0, dt, dx, dtype=dtype)
-        b, a, k = b[1:-1].contiguous(), a[1:-1].contiguous(), k[1:-1].contiguous()
-        shells[ke] = shell_bounds(b, a)
-        cpml[ke] = tuple(_bc(v.to(device), ax) for v in (b, a, k))
+    ramp = lambda ax: absorber_sigma(dims[ax], nabs_ax[ax], dx, R0=absorber_R0, m=absorber_m, dtype=dtype)
+    cpml = shells = sig_bound = None
+    if has_pml:
+        cpml, shells = {}, {}
+        for ax, kh, ke in ((0, "hx", "ex"), (1, "hy", "ey"), (2, "hz", "ez")):
+            n = dims[ax]
+            if ax in ident:   # one shell over the whole axis: psi' = 0 psi + 0 d, curl d + psi'
+                for key, m in ((kh, n - 1), (ke, n - 2)):
+                    z = torch.zeros(m, dtype=dtype)
+                    cpml[key] = tuple(_bc(v.to(device), ax) for v in (z, z.clone(), torch.ones(m, dtype=dtype)))
+                    shells[key] = (m, m)
+                continue
+            b, a, k = cpml_profiles(n - 1, npml_ax[ax], 0.5, dt, dx, dtype=dtype)
+            shells[kh] = shell_bounds(b, a)
+            cpml[kh] = tuple(_bc(v.to(device), ax) for v in (b, a, k))
+            b, a, k = cpml_profiles(n, npml_ax[ax], 0.0, dt, dx, dtype=dtype)
+            b, a, k = b[1:-1].contiguous(), a[1:-1].contiguous(), k[1:-1].contiguous()
+            shells[ke] = shell_bounds(b, a)
+            cpml[ke] = tuple(_bc(v.to(device), ax) for v in (b, a, k))
+        if mixed:
+            sig_bound = torch.zeros(nx, ny, nz, dtype=dtype)
+            for ax in sorted(ident):
+                sig_bound = torch.maximum(sig_bound, _bc(ramp(ax), ax).expand(nx, ny, nz))
+    elif has_abs:
+        sx, sy, sz = ramp(0), ramp(1), ramp(2)
+        sig_bound = torch.maximum(torch.maximum(sx[:, None, None], sy[None, :, None]), sz[None, None, :])
+    else:
+        sig_bound = torch.zeros(nx, ny, nz, dtype=dtype)
     for name, v in (("sigma_d", sigma_d), ("sigma_b", sigma_b)):
         if v is not None and (not torch.is_tensor(v) or tuple(v.shape) != (nx, ny, nz)):
             raise ValueError(f"{name} must be a ({nx}, {ny}, {nz}) tensor")
@@ -163,7 +194,7 @@ def build_3d_coeffs(nx: int, ny: int, nz: int, dx: float, eps=None, npml=0, n_ab
                      torch.ones(nx, ny, nz, dtype=dtype, device=device))
                 c3[q] = rec[q][2] * _avg(m.to(dtype=dtype, device=device), ax)
             pols[key] = (c1, c2, c3)
-    return Coeffs3D(dt=dt, dx=dx, eps=e, damp=_Damp(e, dt, sigma_d, sigma_b), cpml=cpml, pols=pols,
+    return Coeffs3D(dt=dt, dx=dx, eps=e, damp=_Damp(e, dt, sigma_d, sigma_b, sig_bound), cpml=cpml, pols=pols,
                     pmc=(), shells=shells)
 
 
@@ -197,8 +228,10 @@ POLE_NAMES = ("Px_cur", "Px_prev", "Py_cur", "Py_prev", "Pz_cur", "Pz_prev")
 
 
 def state_names(state: State3D):
-    """Field names a run advances: components, psi and, with poles, the stacks."""
-    return COMPS + PSI_NAMES + (POLE_NAMES if state.Px_cur is not None else ())
+    """Field names a run advances: components, psi (a CPML state) and, with
+    poles, the stacks."""
+    return (COMPS + (PSI_NAMES if state.psi_Ez_y is not None else ()) +
+            (POLE_NAMES if state.Px_cur is not None else ()))
 
 
 class _OneStep:

@@ -413,16 +413,21 @@ class PML:
 
 
 class Absorber(PML):
-    def __init__(self, *a, **k):
-        raise NotImplementedError("Absorber layers are not supported by the HIP FDTD path (use PML)")
+    """meep mp.Absorber: the same edge selection as PML, realised as a
+    matched sigma_d = sigma_b quadratic conductivity ramp (R 1e-15) instead
+    of a CPML (meep_torch/mp/material.py:502-505, mp/simulation.py:539-550,
+    615-627)."""
 
 
-def pml_sides(layers, gs):
-    """[[x_lo, x_hi], [y_lo, y_hi]] cell counts, max over layers."""
+def pml_sides(layers, gs, absorber=False):
+    """[[x_lo, x_hi], [y_lo, y_hi]] cell counts, max over the PML layers (or,
+    absorber=True, over the Absorber layers)."""
     sides = [[0, 0], [0, 0]]
     for bl in layers:
         if not isinstance(bl, PML):
             raise NotImplementedError(f"boundary layer {type(bl).__name__} is not supported")
+        if isinstance(bl, Absorber) != bool(absorber):
+            continue
         n = pml_cells(bl.thickness, gs)
         axes = (0, 1) if bl.direction == ALL else (bl.direction,)
         for ax in axes:
@@ -431,6 +436,16 @@ def pml_sides(layers, gs):
             for s in ((0, 1) if bl.side == ALL else (bl.side,)):
                 sides[ax][s] = max(sides[ax][s], n)
     return sides
+
+
+def absorber_ramp(nabs, nx, ny, dx, dtype):
+    """The 2D Absorber conductivity (mp/simulation.py:615-627), float64 numpy
+    (nx, ny): max of the x and y absorber_sigma ramps, each computed in the
+    grid's dtype."""
+    from ..core.pml import absorber_sigma
+    rx = absorber_sigma(nx, tuple(nabs[0]), dx, dtype=dtype)[:, None].expand(nx, ny)
+    ry = absorber_sigma(ny, tuple(nabs[1]), dx, dtype=dtype)[None, :].expand(nx, ny)
+    return torch.maximum(rx, ry).to(torch.float64).numpy()
 
 
 def to_tensor(a, dtype, device):

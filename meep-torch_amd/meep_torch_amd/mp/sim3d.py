@@ -69,14 +69,17 @@ def box3d(center, size, gs):
     return tuple(out)
 
 
-def pml_sides3(layers, gs, cell):
-    """[[lo, hi]] CPML cell counts per axis; direction ALL covers the axes the
-    cell spans (meep_torch/mp/simulation.py:89-116, :359-379)."""
+def pml_sides3(layers, gs, cell, absorber=False):
+    """[[lo, hi]] CPML cell counts per axis (absorber=True: of the Absorber
+    layers); direction ALL covers the axes the cell spans
+    (meep_torch/mp/simulation.py:89-116, :359-379)."""
     sides = [[0, 0], [0, 0], [0, 0]]
     extent = (cell.x, cell.y, cell.z)
     for bl in layers:
         if not isinstance(bl, G.PML):
             raise NotImplementedError(f"boundary layer {type(bl).__name__} is not supported")
+        if isinstance(bl, G.Absorber) != bool(absorber):
+            continue
         n = G.pml_cells(bl.thickness, gs)
         if bl.direction == ALL:
             axes = tuple(a for a in range(3) if extent[a]) or (0, 1, 2)
@@ -123,6 +126,7 @@ class Sim3D:
             self._default = G.Medium(epsilon=float(self._default))
         self.boundary_layers = list(boundary_layers or [])
         self._npml_sides = pml_sides3(self.boundary_layers, self.gs, self.cell_size)
+        self._nabs_sides = pml_sides3(self.boundary_layers, self.gs, self.cell_size, absorber=True)
         self._npml = max(max(p) for p in self._npml_sides)
         self._monitors = {}
         self._flux_parts = {}
@@ -178,6 +182,7 @@ class Sim3D:
         self._n_poles = len(poles)
         self._coeffs = build_3d_coeffs(gs.nx, gs.ny, gs.nz, gs.dx, eps=eps,
                                        npml=tuple(tuple(p) for p in self._npml_sides),
+                                       n_abs=tuple(tuple(p) for p in self._nabs_sides),
                                        sigma_d=sd if has_cond else None, sigma_b=sb if has_cond else None,
                                        poles=poles, pole_masks=masks or None,
                                        dtype=self.dtype, device=self.device)

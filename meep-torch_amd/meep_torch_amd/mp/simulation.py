@@ -167,6 +167,12 @@ class Simulation(Sim3D):
         g = self.grid
         npml = (tuple(self._npml_sides[0]), tuple(self._npml_sides[1]))
         media = G.rasterize_media(self._geometry, self.gs, self._default)
+        nabs = G.pml_sides(self.boundary_layers, self.gs, absorber=True)
+        if any(any(p) for p in nabs):   # Absorber: the ramp joins sigma_d and sigma_b (mp/simulation.py:539-550)
+            ramp = G.absorber_ramp(nabs, g.nx, g.ny, g.dx, g.dtype)
+            sd, sb, mu = media["cond"] if media["cond"] is not None else (
+                np.zeros((g.nx, g.ny)), np.zeros((g.nx, g.ny)), np.ones((g.nx, g.ny)))
+            media["cond"] = (sd + ramp, sb + ramp, mu)
         has_media = bool(media["poles"]) or media["chi3"] is not None or media["cond"] is not None
         self._n_poles = len(media["poles"])
         if pol == "tm":

@@ -1,8 +1,10 @@
 """3D Yee restatement (CPU, torch eager) -- test infrastructure only.
 
-Follows meep_torch/core/stepper_3d.py:150-266 (CPML coefficients; material
-sigma_d / sigma_b damping pairs and ADE poles, stepper_3d.py:217-266),
-stepper_3d.py:295-301,363-445 (CPML step, folded-D pole update),
+Follows meep_torch/core/stepper_3d.py:150-266 (CPML coefficients, the
+matched-sigma absorber and mixed boundary modes; material sigma_d / sigma_b
+damping pairs and ADE poles, stepper_3d.py:217-266), pml.py:62-80 (absorber
+ramp), stepper_3d.py:295-301,328-361,363-445 (absorber and CPML steps,
+folded-D pole update),
 stepper_3d.py:448-481 (sources),
 stepper_3d_lean.py:174-296 (in-place lean form) and monitors_3d.py:18-210.
 """
@@ -10,7 +12,7 @@ import math
 
 import torch
 
-from .yee2d import cpml_profiles, pole_recurrence
+from .yee2d import _sides, cpml_profiles, pole_recurrence
 
 
 def avg(v, ax):
@@ -31,7 +33,25 @@ def _along(v, ax):
     return v.reshape(sh)
 
 
-def _media3d(c, nx, ny, nz, dtype, sigma_d, sigma_b, poles, pole_masks):
+def absorber_sigma(n, nabs, dx, dtype, R0=1e-15, m=2):
+    """Matched-absorber ramp (pml.py:62-80): on each absorbing edge of nc
+    cells, sigma_i = smax ((nc - i - 1/2) / nc)^m, i = 0 at the wall, with
+    smax = -(m + 1) ln(R0) / (2 nc dx); zero elsewhere."""
+    lo, hi = _sides(nabs)
+    p = torch.zeros(n, dtype=dtype)
+
+    def ramp(nc):
+        smax = -(m + 1) * math.log(R0) / (2.0 * nc * dx)
+        return smax * ((nc - torch.arange(nc, dtype=dtype) - 0.5) / nc) ** m
+
+    if lo > 0:
+        p[:lo] = ramp(lo)
+    if hi > 0:
+        p[n - hi:] = torch.flip(ramp(hi), [0])
+    return p
+
+
+def _media3d(c, nx, ny, nz, dtype, sigma_d, sigma_b, poles, pole_masks, sig_bound=None):
     """stepper_3d.py:217-266.  Damping pairs of the Yee-averaged conductivity
     (E: averaged along the component's own axis; H: along the two others):
       d1 = (1 - s dt/2) / (1 + s dt/2),  d2 = dt / (1 + s dt/2);
@@ -41,6 +61,8 @@ def _media3d(c, nx, ny, nz, dtype, sigma_d, sigma_b, poles, pole_masks):
     z = torch.zeros(nx, ny, nz, dtype=dtype)
     se = sigma_d.to(dtype) if sigma_d is not None else z
     sh = sigma_b.to(dtype) if sigma_b is not None else z
+    if sig_bound is not None:   # stepper_3d.py:224-225: boundary sigma + material sigma
+        se, sh = sig_bound + se, sig_bound + sh
     pair = lambda s: ((1 - s * dt / 2) / (1 + s * dt / 2), dt / (1 + s * dt / 2))
     c["damp"] = {"Ex": pair(avg(se, 0)), "Ey": pair(avg(se, 1)), "Ez": pair(avg(se, 2)),
                  "Hx": pair(avg(avg(sh, 1), 2)), "Hy": pair(avg(avg(sh, 0), 2)),
@@ -60,26 +82,61 @@ def _media3d(c, nx, ny, nz, dtype, sigma_d, sigma_b, poles, pole_masks):
 
 
 def coeffs3d(nx, ny, nz, dx, eps, npml, dtype=torch.float32, sigma_d=None, sigma_b=None, poles=(),
-             pole_masks=None):
+             pole_masks=None, n_abs=0, absorber_axes=(), absorber_m=2, absorber_R0=1e-15, absorber=None):
     """Per-axis CPML profiles: H side length n-1 at offset 1/2, E side length n
     at offset 0 trimmed to [1:-1] (stepper_3d.py:182-195); dt = dx/(2 sqrt 3).
-    npml may be 0 (b = 1, a = 0: psi stays zero, PEC box).  With any of
-    sigma_d, sigma_b (cell-centred (nx, ny, nz)) or poles the media
-    coefficients of _media3d are added."""
+    npml / n_abs: an int for every face or per axis an int or (lo, hi) pair.
+    Boundary modes (stepper_3d.py:174-215): no absorber -> CPML profiles
+    (npml 0 on an axis gives b = 1, a = 0 there: psi stays zero, so npml = 0
+    everywhere is a PEC box in CPML arithmetic); n_abs with no npml -> the
+    absorber path (c["absorber"], no CPML) with
+    sig_bound = max(sx[i], sy[j], sz[k]) of the absorber ramps; both ->
+    mixed: the identity profile (b = a = 0) on absorber_axes and
+    sig_bound = max of their ramps.  absorber=True forces the absorber path
+    with sig_bound = 0 (a run with neither, the reference's cpml=None
+    branch).  With any of sigma_d, sigma_b (cell-centred (nx, ny, nz)),
+    poles or a boundary sigma the media coefficients of _media3d are
+    added."""
     dt = 0.5 * dx / math.sqrt(3.0)
-    pro = {}
-    for ax, n in enumerate((nx, ny, nz)):
-        b, a, _ = cpml_profiles(n - 1, npml, 0.5, dt, dx, dtype)
-        pro[("H", ax)] = (_along(b, ax), _along(a, ax))
-        b, a, _ = cpml_profiles(n, npml, 0.0, dt, dx, dtype)
-        pro[("E", ax)] = (_along(b[1:-1], ax), _along(a[1:-1], ax))
+    per = lambda v: tuple(_sides(x) for x in v) if isinstance(v, (tuple, list)) else (_sides(v),) * 3
+    npml_ax, nabs_ax = per(npml), per(n_abs)
+    has_pml = any(l or h for l, h in npml_ax)
+    has_abs = any(l or h for l, h in nabs_ax)
+    dims = (nx, ny, nz)
+    ramp = lambda ax: absorber_sigma(dims[ax], nabs_ax[ax], dx, dtype, R0=absorber_R0, m=absorber_m)
     eps = eps.to(dtype)
-    c = {"dt": dt, "dx": dx, "eps": eps, "pro": pro,
+    c = {"dt": dt, "dx": dx, "eps": eps, "pro": None, "absorber": False,
          "eps_c": [avg(eps, 0), avg(eps, 1), avg(eps, 2)],
          "inv_eps": [1.0 / avg(eps, 0), 1.0 / avg(eps, 1), 1.0 / avg(eps, 2)],
          "damp": None, "pols": None}
-    if sigma_d is not None or sigma_b is not None or len(poles):
-        _media3d(c, nx, ny, nz, dtype, sigma_d, sigma_b, poles, pole_masks)
+    sig_bound = None
+    if not absorber and (has_pml or not has_abs):
+        ident = set(absorber_axes) if has_abs else set()
+        pro = {}
+        for ax, n in enumerate(dims):
+            if ax in ident:      # stepper_3d.py:132-145: b = 0, a = 0, k = 1
+                z = lambda m: torch.zeros(m, dtype=dtype)
+                pro[("H", ax)] = (_along(z(n - 1), ax), _along(z(n - 1), ax))
+                pro[("E", ax)] = (_along(z(n - 2), ax), _along(z(n - 2), ax))
+                continue
+            b, a, _ = cpml_profiles(n - 1, npml_ax[ax], 0.5, dt, dx, dtype)
+            pro[("H", ax)] = (_along(b, ax), _along(a, ax))
+            b, a, _ = cpml_profiles(n, npml_ax[ax], 0.0, dt, dx, dtype)
+            pro[("E", ax)] = (_along(b[1:-1], ax), _along(a[1:-1], ax))
+        c["pro"] = pro
+        if has_abs:
+            sig_bound = torch.zeros(nx, ny, nz, dtype=dtype)
+            for ax in ident:
+                sig_bound = torch.maximum(sig_bound, _along(ramp(ax), ax).expand(nx, ny, nz))
+    else:
+        c["absorber"] = True
+        if has_abs:
+            sig_bound = torch.maximum(torch.maximum(ramp(0)[:, None, None], ramp(1)[None, :, None]),
+                                      ramp(2)[None, None, :])
+        else:
+            sig_bound = torch.zeros(nx, ny, nz, dtype=dtype)
+    if sigma_d is not None or sigma_b is not None or len(poles) or sig_bound is not None:
+        _media3d(c, nx, ny, nz, dtype, sigma_d, sigma_b, poles, pole_masks, sig_bound)
     return c
 
 
@@ -155,6 +212,8 @@ def step3d(st, c, amp=0.0, src=None, lean=False):
             a = comp[1]
             pn[comp] = c1 * st[f"P{a}_cur"] + c2 * st[f"P{a}_prev"] + c3 * st[comp]
             s[f"P{a}_cur"], s[f"P{a}_prev"] = pn[comp], st[f"P{a}_cur"]
+    if c.get("absorber"):
+        return _absorber_step(st, s, c, pn, amp, src)
     for comp, pa, pb in (("Hx", "psi_Ez_y", "psi_Ey_z"), ("Hy", "psi_Ex_z", "psi_Ez_x"),
                          ("Hz", "psi_Ey_x", "psi_Ex_y")):
         curl = _ce(s, c, pa) - _ce(s, c, pb)
@@ -185,6 +244,36 @@ def step3d(st, c, amp=0.0, src=None, lean=False):
         else:
             e = c["eps_c"][k]
             s[comp] = (st[comp] * e + dt * curl) / e
+    return _source(s, c, amp, src, lean)
+
+
+def _absorber_step(st, s, c, pn, amp, src):
+    """stepper_3d.py:328-361 (no CPML): H = d1 H - d2 (dE_a - dE_b) / dx
+    (Python order: (d2 * diff) / dx), E curl = pad((dH_a - dH_b) / dx) over
+    the interior, then the same E updates as the CPML path."""
+    dx, dm, pols = c["dx"], c["damp"], c.get("pols")
+    for comp, (fa, aa), (fb, ab) in (("Hx", ("Ez", 1), ("Ey", 2)), ("Hy", ("Ex", 2), ("Ez", 0)),
+                                     ("Hz", ("Ey", 0), ("Ex", 1))):
+        d1, d2 = dm[comp]
+        s[comp] = d1 * st[comp] - d2 * (diff(st[fa], aa) - diff(st[fb], ab)) / dx
+    for comp, (fa, aa, ta), (fb, ab, tb), axes in (
+            ("Ex", ("Hz", 1, 2), ("Hy", 2, 1), (1, 2)),
+            ("Ey", ("Hx", 2, 0), ("Hz", 0, 2), (0, 2)),
+            ("Ez", ("Hy", 0, 1), ("Hx", 1, 0), (0, 1))):
+        curl = _pad((_trim(diff(s[fa], aa), ta) - _trim(diff(s[fb], ab), tb)) / dx, axes)
+        k = "xyz".index(comp[1])
+        e = c["eps_c"][k]
+        d1, d2 = dm[comp]
+        if pols is not None:
+            s[comp] = (d1 * (st[comp] * e + st[f"P{comp[1]}_cur"].sum(0)) + d2 * curl - pn[comp].sum(0)) / e
+        else:
+            s[comp] = (d1 * st[comp] * e + d2 * curl) / e
+    return _source(s, c, amp, src, False)
+
+
+def _source(s, c, amp, src, lean):
+    """Soft sources of step3d (stepper_3d.py:448-481, stepper_3d_lean.py:294-296)."""
+    dt = c["dt"]
     if src is not None and amp != 0.0:
         comp = src[0]
         k = "xyz".index(comp[1])

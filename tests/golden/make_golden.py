@@ -34,6 +34,11 @@ Cases (all CPU, torch threads = 8):
              sigma_b, mu; (nl) chi2 + chi3 only
   media_3d   3D 22x20x18, npml 4, 120 steps, sigma_d, sigma_b and a Lorentz +
              Drude pole pair with masks, point source, flux plane, f32 + f64
+  absorber_3d  3D 22x20x18, 120 steps, f32 + f64, boundary modes without a
+             full CPML: (abs) matched-sigma absorber n_abs 4 with sigma_d,
+             sigma_b and the pole pair; (mixed) npml 4 with the absorber on
+             the z axis (n_abs 4, absorber_axes (2,)); (pec) neither (the
+             absorber-path arithmetic in a PEC box); point source, flux plane
   tfsf_tm    ModeSourceTM (TFSF eigenmode source, h_masks) on a 2D TM guide:
              masks, the electric / magnetic amplitude sequences, fields and
              flux after 600 steps, f64
@@ -46,6 +51,9 @@ Cases (all CPU, torch threads = 8):
              component
   shapes     mp.Simulation 2D TM cell with a Cylinder, Sphere, Prism, rotated
              Block and Ellipsoid: eps map, Ez and a flux after t = 12, f64
+  sim_absorber  mp.Simulation with mp.Absorber layers: a 2D TM and a 2D TE
+             cell (Absorber on x, PML on y) and a 3D cell (Absorber all
+             round): Ez / Hz / Ey and a flux, f64
   multisrc   mp.Simulation 2D TM cell with two sources (point and line, own
              pulses and amplitudes): Ez and a flux after t = 10, plain and
              with an at_every step function splitting the run, f64
@@ -535,6 +543,42 @@ def case_media_3d():
         save(f"media_3d_{tag}", **out)
 
 
+def case_absorber_3d():
+    from meep_torch.core.monitors_3d import FluxMonitor3D, PointSource3D, run_3d
+    from meep_torch.core.sources import GaussianPulse
+    from meep_torch.core.stepper_3d import build_3d_coeffs, init_3d_state
+    nx, ny, nz, n = 22, 20, 18, 120
+    r = torch.tensor(splitmix_uniform(nx * ny * nz, 21).reshape(nx, ny, nz))
+    inner = torch.zeros(nx, ny, nz, dtype=torch.float64)
+    inner[6:nx - 6, 6:ny - 6, 6:nz - 6] = 1.0
+    half = (r > 0.5).double() * inner
+    maps = {"eps": 1.0 + 2.0 * r * inner, "sigma_d": 2.0 * r * inner, "sigma_b": 1.0 * (1.0 - r) * inner,
+            "m1": half, "m2": inner - half}
+    cases = {"abs": dict(npml=0, n_abs=4, media=True), "mixed": dict(npml=4, n_abs=4, axes=(2,), media=False),
+             "pec": dict(npml=0, n_abs=0, media=False)}
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        cv = {k: v.to(dtype) for k, v in maps.items()}
+        out = dict(cv)
+        for case, o in cases.items():
+            kw = dict(sigma_d=cv["sigma_d"], sigma_b=cv["sigma_b"], poles=POLES2,
+                      pole_masks=[cv["m1"], cv["m2"]]) if o["media"] else {}
+            c = build_3d_coeffs(nx, ny, nz, 0.05, eps=cv["eps"], npml=o["npml"], n_abs=o["n_abs"],
+                                absorber_axes=o.get("axes", ()), dtype=dtype, device="cpu", **kw)
+            src = PointSource3D(c.dt, "Ez", 9, ny // 2, nz // 2, GaussianPulse(1.0, 2.0, cutoff=2.0))
+            mon = FluxMonitor3D(c.dt, [0.8, 1.0, 1.2], normal="x", idx=14, ap=2, dtype=dtype)
+            st0 = init_3d_state(nx, ny, nz, npml=o["npml"], n_poles=2 if o["media"] else 0, dtype=dtype,
+                                device="cpu")
+            st, (acc,) = run_3d(st0, c, n, source=src, monitors=[mon])
+            for f in st._fields:
+                v = getattr(st, f)
+                if isinstance(v, torch.Tensor) and v.numel():
+                    out[f"{case}_{f}"] = v
+            out[f"{case}_acc"] = acc
+            out[f"{case}_flux"] = mon.flux(acc, 0.05)
+        out["amp"] = np.array([src.amplitude(t) for t in range(n)])
+        save(f"absorber_3d_{tag}", **out)
+
+
 def case_tfsf_tm():
     from meep_torch.core.coeffs import build_tm_coeffs
     from meep_torch.core.grid import Grid
@@ -683,10 +727,36 @@ def case_multisrc():
     save("multisrc_f64", **out)
 
 
+def case_sim_absorber():
+    import meep_torch.mp as mp
+    out = {}
+    for pol, comp in (("tm", mp.Ez), ("te", mp.Hz)):
+        sim = mp.Simulation(cell_size=mp.Vector3(6, 4), resolution=10,
+                            geometry=[mp.Block(size=mp.Vector3(mp.inf, 0.6), material=mp.Medium(epsilon=8.0))],
+                            sources=[mp.Source(mp.GaussianSource(0.8, fwidth=0.6), component=comp,
+                                               center=mp.Vector3(-1.8, 0.0), size=mp.Vector3(0, 0.8))],
+                            boundary_layers=[mp.Absorber(0.8, direction=mp.X), mp.PML(0.5, direction=mp.Y)])
+        fl = sim.add_flux(0.8, 0.5, 4, mp.FluxRegion(center=mp.Vector3(1.6, 0), size=mp.Vector3(0, 3)))
+        sim.run(until=12)
+        out[f"{pol}_field"] = np.asarray(sim.get_array(component=comp))
+        out[f"{pol}_flux"] = np.array(mp.get_fluxes(fl))
+    sim = mp.Simulation(cell_size=mp.Vector3(1.6, 1.4, 1.2), resolution=20,
+                        geometry=[mp.Block(center=mp.Vector3(0.1, 0, -0.1), size=mp.Vector3(mp.inf, 0.3, 0.25),
+                                           material=mp.Medium(epsilon=6.0))],
+                        sources=[mp.Source(mp.GaussianSource(1.0, fwidth=1.0), component=mp.Ey,
+                                           center=mp.Vector3(-0.4, 0, 0), size=mp.Vector3(0, 0.6, 0.5))],
+                        boundary_layers=[mp.Absorber(0.25)], dimensions=3)
+    fl = sim.add_flux(1.0, 0.4, 3, mp.FluxRegion(center=mp.Vector3(0.4, 0, 0), size=mp.Vector3(0, 1.0, 0.8)))
+    sim.run(until=6.0)
+    out["d3_flux"] = np.array(mp.get_fluxes(fl))
+    out["d3_Ey"] = np.asarray(sim.get_array(component=mp.Ey))
+    save("sim_absorber_f64", **out)
+
+
 def main():
     which = sys.argv[1:] or ["pml", "tm", "te", "c1", "3d", "grad_tm", "grad_3d",
                              "filters", "golden_plain", "media_tm", "media_te", "media_3d",
-                             "tfsf_tm", "sim3d", "raster", "shapes", "multisrc"]
+                             "absorber_3d", "tfsf_tm", "sim3d", "raster", "shapes", "multisrc", "sim_absorber"]
     t_all = time.perf_counter()
     timing = {}
     try:
@@ -701,8 +771,9 @@ def main():
          "grad_tm": case_grad_tm, "grad_3d": case_grad_3d,
          "filters": case_filters, "golden_plain": case_golden_plain,
          "media_tm": case_media_tm, "media_te": case_media_te, "media_3d": case_media_3d,
+         "absorber_3d": case_absorber_3d,
          "tfsf_tm": case_tfsf_tm, "sim3d": case_sim3d, "raster": case_raster,
-         "shapes": case_shapes, "multisrc": case_multisrc}[w]()
+         "shapes": case_shapes, "multisrc": case_multisrc, "sim_absorber": case_sim_absorber}[w]()
         print(f"[golden] {w}: {time.perf_counter() - t0:.1f}s", flush=True)
     man = {}
     for fn in sorted(os.listdir(HERE)):

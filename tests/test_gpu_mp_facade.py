@@ -303,3 +303,28 @@ def test_decay_stop_in_window_sized_calls(pol, monkeypatch):
     assert t_f == t_w
     assert np.array_equal(f_f, f_w) and np.array_equal(x_f, x_w)
     assert n_f * 20 < n_w, (n_f, n_w)
+
+
+@pytest.mark.parametrize("pol", ["tm", "te"])
+def test_absorber_layers_match_reference(pol):
+    """mp.Absorber on x with a PML on y: the facade folds the matched
+    absorber ramp (max of the x and y absorber_sigma ramps) into sigma_d and
+    sigma_b (mp/simulation.py:539-550,615-627) and runs the conductive
+    kernels; the field and the flux equal the reference Simulation's
+    (golden sim_absorber_f64) within FP64 rounding."""
+    import meep_torch_amd.mp as mp
+    G = golden("sim_absorber_f64")
+    comp = mp.Ez if pol == "tm" else mp.Hz
+    sim = mp.Simulation(cell_size=mp.Vector3(6, 4), resolution=10,
+                        geometry=[mp.Block(size=mp.Vector3(mp.inf, 0.6), material=mp.Medium(epsilon=8.0))],
+                        sources=[mp.Source(mp.GaussianSource(0.8, fwidth=0.6), component=comp,
+                                           center=mp.Vector3(-1.8, 0.0), size=mp.Vector3(0, 0.8))],
+                        boundary_layers=[mp.Absorber(0.8, direction=mp.X), mp.PML(0.5, direction=mp.Y)],
+                        dtype=torch.float64, device="cuda")
+    fl = sim.add_flux(0.8, 0.5, 4, mp.FluxRegion(center=mp.Vector3(1.6, 0), size=mp.Vector3(0, 3)))
+    sim.run(until=12)
+    f = np.asarray(sim.get_array(component=comp))
+    assert f.shape == G[f"{pol}_field"].shape
+    assert np.linalg.norm(f - G[f"{pol}_field"]) <= 1e-12 * np.linalg.norm(G[f"{pol}_field"])
+    fx = np.asarray(mp.get_fluxes(fl))
+    assert np.abs(fx - G[f"{pol}_flux"]).max() <= 1e-10 * np.abs(G[f"{pol}_flux"]).max()

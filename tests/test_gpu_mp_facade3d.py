@@ -170,3 +170,27 @@ def test_3d_facade_matches_reference_simulation(kind):
     ref_flux = G[f"{kind}_flux"]
     got_flux = np.asarray(mp.get_fluxes(fl))
     assert np.abs(got_flux - ref_flux).max() <= 1e-10 * np.abs(ref_flux).max(), (got_flux, ref_flux)
+
+
+def test_3d_facade_absorber_matches_reference_simulation():
+    """mp.Absorber all round a 3D cell: no CPML, the matched-sigma absorber
+    path (build_3d_coeffs(npml=0, n_abs=...), stepper_3d.py:328-361) with a
+    plane source and a flux plane; Ey and the flux equal the reference
+    Simulation's (golden sim_absorber_f64) within FP64 rounding."""
+    import meep_torch_amd.mp as mp
+    from conftest import golden
+    G = golden("sim_absorber_f64")
+    sim = mp.Simulation(cell_size=mp.Vector3(1.6, 1.4, 1.2), resolution=20,
+                        geometry=[mp.Block(center=mp.Vector3(0.1, 0, -0.1), size=mp.Vector3(mp.inf, 0.3, 0.25),
+                                           material=mp.Medium(epsilon=6.0))],
+                        sources=[mp.Source(mp.GaussianSource(1.0, fwidth=1.0), component=mp.Ey,
+                                           center=mp.Vector3(-0.4, 0, 0), size=mp.Vector3(0, 0.6, 0.5))],
+                        boundary_layers=[mp.Absorber(0.25)], dimensions=3, dtype=torch.float64, device="cuda")
+    fl = sim.add_flux(1.0, 0.4, 3, mp.FluxRegion(center=mp.Vector3(0.4, 0, 0), size=mp.Vector3(0, 1.0, 0.8)))
+    sim.run(until=6.0)
+    assert sim._coeffs.cpml is None
+    got, ref = np.asarray(sim.get_array(component=mp.Ey)), G["d3_Ey"]
+    assert got.shape == ref.shape
+    assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+    fx = np.asarray(mp.get_fluxes(fl))
+    assert np.abs(fx - G["d3_flux"]).max() <= 1e-10 * np.abs(G["d3_flux"]).max()

@@ -9,7 +9,8 @@ in FP32 and FP64.
   under a strong drive ("nl"), where the three-pass fixed point
   (stepper_tm.py:228-231, stepper_te.py:161-175) changes the fields by far
   more than rounding;
-* 3D 22 x 20 x 18 with sigma_d, sigma_b and two poles under a point source.
+* 3D 22 x 20 x 18 with sigma_d, sigma_b and two poles under a point source,
+  and the boundary modes without a full CPML (absorber_3d).
 
 Bit-exact equality pins every coefficient, sign and operation order of the
 nonlinear and dispersive updates (a wrong chi2 / chi3 factor, a dropped pole
@@ -114,6 +115,52 @@ def test_3d_media_bitexact(tag, dtype):
         _eq(s[f], g[f], f)
     _eq(acc[0], g["acc"], "acc")
     _eq(yee3d.flux_plane(acc[0], 0.05), g["flux"], "flux")
+
+
+ABS_CASES = {"abs": dict(npml=0, n_abs=4, media=True), "mixed": dict(npml=4, n_abs=4, axes=(2,), media=False),
+             "pec": dict(npml=0, n_abs=0, media=False)}
+
+
+@pytest.mark.parametrize("case", sorted(ABS_CASES))
+@pytest.mark.parametrize("tag,dtype", [("f32", torch.float32), ("f64", torch.float64)])
+def test_3d_absorber_modes_bitexact(case, tag, dtype):
+    """Boundary modes without a full CPML (stepper_3d.py:174-215,328-361),
+    reproduced bit for bit from the reference's own run: the matched-sigma
+    absorber (one division per curl, sig_bound added to the material sigma,
+    poles), the mixed mode (identity profile on the absorber axis) and the
+    absorber-path arithmetic with no boundary sigma at all."""
+    g = golden(f"absorber_3d_{tag}")
+    o = ABS_CASES[case]
+    nx, ny, nz, n = 22, 20, 18, 120
+    kw = dict(sigma_d=_t(g["sigma_d"]), sigma_b=_t(g["sigma_b"]), poles=POLES2,
+              pole_masks=[_t(g["m1"]), _t(g["m2"])]) if o["media"] else {}
+    c = yee3d.coeffs3d(nx, ny, nz, 0.05, _t(g["eps"]), o["npml"], dtype, n_abs=o["n_abs"],
+                       absorber_axes=o.get("axes", ()), absorber=(o["npml"] == 0), **kw)
+    assert c["absorber"] == (o["npml"] == 0)
+    mons = [{"type": "flux", "normal": "x", "idx": 14, "freqs": [0.8, 1.0, 1.2], "ap": 2}]
+    s, acc = yee3d.run3d(yee3d.state3d(nx, ny, nz, dtype, n_poles=2 if o["media"] else 0), c, n,
+                         list(g["amp"]), ("Ez", 9, ny // 2, nz // 2), mons)
+    for f in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz") + (("Px_cur", "Pz_prev") if o["media"] else ()):
+        _eq(s[f], g[f"{case}_{f}"], f"{case} {f}")
+    _eq(acc[0], g[f"{case}_acc"], f"{case} acc")
+    _eq(yee3d.flux_plane(acc[0], 0.05), g[f"{case}_flux"], f"{case} flux")
+
+
+def test_3d_absorber_is_not_the_cpml_arithmetic():
+    """The absorber path's single division per curl is a different rounding
+    from the CPML path's per-derivative division: the same PEC box run with
+    CPML arithmetic (npml 0: b = 1, a = 0, psi zero) differs from the
+    reference's absorber-path fixture in FP32, so the bit-exact test above
+    sees the arithmetic, not only the physics."""
+    g = golden("absorber_3d_f32")
+    nx, ny, nz, n = 22, 20, 18, 120
+    c = yee3d.coeffs3d(nx, ny, nz, 0.05, _t(g["eps"]), 0, torch.float32)
+    assert not c["absorber"]
+    s, _ = yee3d.run3d(yee3d.state3d(nx, ny, nz, torch.float32), c, n, list(g["amp"]),
+                       ("Ez", 9, ny // 2, nz // 2), [])
+    ref = _t(g["pec_Ez"])
+    assert not torch.equal(s["Ez"], ref)
+    assert float((s["Ez"] - ref).norm() / ref.norm()) < 1e-4   # same physics, other rounding
 
 
 def test_tfsf_mode_source_run_bitexact():
