@@ -100,6 +100,16 @@ typedef struct {
   void *P_cur, *P_prev;
   const void *chi2, *chi3;
   const void *d1, *d2, *h1x, *h2x, *h1y, *h2y;
+  /* optional magnetic Lorentz poles (coeffs.py:177-196, stepper_tm.py:103-106,
+   * 180-193,209-224), two-pass path only, not with h1x..h2y: B integrates the
+   * curl, Bx = Bx - dt curl, By = By + dt curl, the TFSF currents go into B,
+   * then per pole M' = (mc1 M + mc2 M_prev) + mc3 H(n-1/2), M_prev' = M, and
+   * H = B - sum M' (pole order).  mc1x, mc2x, mc3x, Mx_cur, Mx_prev are
+   * (n_mpoles, nx, ny-1), the y ones (n_mpoles, nx-1, ny); Bx (nx, ny-1),
+   * By (nx-1, ny). */
+  int n_mpoles, pad4_;
+  const void *mc1x, *mc2x, *mc3x, *mc1y, *mc2y, *mc3y;
+  void *Bx, *By, *Mx_cur, *Mx_prev, *My_cur, *My_prev;
 } mt_tm2d;
 
 /* 2D TE problem + state (coeffs.py:294-345 TECoeffs, stepper_te.py:11-24). */

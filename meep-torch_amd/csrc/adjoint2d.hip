@@ -344,6 +344,7 @@ extern "C" int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const voi
     return fail(MT_EINVAL, "the c3 gradient needs ez_init (Ez before the first step)");
   if (p->n_poles < 0 || (p->n_poles && (!a->aP_cur || !a->aP_prev || !p->c1 || !p->c2 || !p->c3)))
     return fail(MT_EINVAL, "adjoint with poles needs c1, c2, c3 and the aP_cur / aP_prev cotangents");
+  if (p->n_mpoles) return fail(MT_EINVAL, "the TM adjoint does not transpose magnetic poles (n_mpoles = 0)");
   if (!p->d1 != !p->d2 || !p->h1x != !p->h2x || !p->h1y != !p->h2y || !p->d1 != !p->h1x || !p->d1 != !p->h1y)
     return fail(MT_EINVAL, "conductivity needs all of d1, d2, h1x, h2x, h1y, h2y");
   if (!p->eps || !a->aEz || !a->aDz || !a->aHx || !a->aHy || !a->apsi_Ezy || !a->apsi_Ezx ||

@@ -36,7 +36,28 @@ struct TmK {
   int n_poles, kerr;
   const T *c1, *c2, *c3, *chi2, *chi3, *d1, *d2, *h1x, *h2x, *h1y, *h2y;
   T *Pc, *Pp;
+  // magnetic Lorentz poles (n_mpoles > 0): B, the M stacks and their scalars
+  int n_mpoles;
+  const T *mc1x, *mc2x, *mc3x, *mc1y, *mc2y, *mc3y;
+  T *Bx, *By, *Mxc, *Mxp, *Myc, *Myp;
 };
+
+// H = B - sum M' with M' = (mc1 M + mc2 M_prev) + mc3 H_old per pole, in pole
+// order (stepper_tm.py:209-224); q is the cell's index in a grid of nq cells.
+template <typename T>
+__device__ __forceinline__ T h_from_b(int nm, const T* mc1, const T* mc2, const T* mc3, T* Mc, T* Mp, long long q,
+                                      long long nq, T b, T h_old) {
+  T sum = T(0);
+  for (int k = 0; k < nm; ++k) {
+    const long long o = (long long)k * nq + q;
+    const T mc = Mc[o];
+    const T mn = (mc1[o] * mc + mc2[o] * Mp[o]) + mc3[o] * h_old;
+    Mp[o] = mc;
+    Mc[o] = mn;
+    sum = k ? sum + mn : mn;
+  }
+  return b - sum;
+}
 
 // meep_torch/core/stepper_tm.py:85-92,126-128 (B half, H from curl E)
 // Weight of a magnetic source box at (i, j), or 0 outside it.
@@ -76,10 +97,17 @@ __global__ __launch_bounds__(BX* BY) void tm_b_kernel(TmK<T> p, mt_source2d hxs,
       p.pEzy[h] = ps;
       curl = d + ps;
     }
-    // stepper_tm.py:116-121: sigma_b / mu fold into h1, h2
-    T v = p.h1x ? p.h1x[h] * p.Hx[h] - p.h2x[h] * curl : p.Hx[h] - p.dt * curl;
-    if (in_box<T>(hxs, i, j)) v = v - hc * box_w<T>(hxs, i, j);
-    p.Hx[h] = v;
+    if (p.n_mpoles) {  // stepper_tm.py:103-106,187-193,214-219: B integrates the curl and takes the current
+      T b = p.Bx[h] - p.dt * curl;
+      if (in_box<T>(hxs, i, j)) b = b - hc * box_w<T>(hxs, i, j);
+      p.Bx[h] = b;
+      p.Hx[h] = h_from_b(p.n_mpoles, p.mc1x, p.mc2x, p.mc3x, p.Mxc, p.Mxp, h, (long long)p.nx * (ny - 1), b, p.Hx[h]);
+    } else {
+      // stepper_tm.py:116-121: sigma_b / mu fold into h1, h2
+      T v = p.h1x ? p.h1x[h] * p.Hx[h] - p.h2x[h] * curl : p.Hx[h] - p.dt * curl;
+      if (in_box<T>(hxs, i, j)) v = v - hc * box_w<T>(hxs, i, j);
+      p.Hx[h] = v;
+    }
   }
   if (i < p.nx - 1) {
     long long h = (long long)i * ny + j;
@@ -90,9 +118,16 @@ __global__ __launch_bounds__(BX* BY) void tm_b_kernel(TmK<T> p, mt_source2d hxs,
       p.pEzx[h] = ps;
       curl = d + ps;
     }
-    T v = p.h1y ? p.h1y[h] * p.Hy[h] + p.h2y[h] * curl : p.Hy[h] + p.dt * curl;
-    if (in_box<T>(hys, i, j)) v = v - hc * box_w<T>(hys, i, j);
-    p.Hy[h] = v;
+    if (p.n_mpoles) {
+      T b = p.By[h] + p.dt * curl;
+      if (in_box<T>(hys, i, j)) b = b - hc * box_w<T>(hys, i, j);
+      p.By[h] = b;
+      p.Hy[h] = h_from_b(p.n_mpoles, p.mc1y, p.mc2y, p.mc3y, p.Myc, p.Myp, h, (long long)(p.nx - 1) * ny, b, p.Hy[h]);
+    } else {
+      T v = p.h1y ? p.h1y[h] * p.Hy[h] + p.h2y[h] * curl : p.Hy[h] + p.dt * curl;
+      if (in_box<T>(hys, i, j)) v = v - hc * box_w<T>(hys, i, j);
+      p.Hy[h] = v;
+    }
   }
 }
 
@@ -378,6 +413,19 @@ int tm_run_t(const mt_tm2d* d, const mt_source2d* src, const double* coef, const
   p.h2x = static_cast<const T*>(d->h2x);
   p.h1y = static_cast<const T*>(d->h1y);
   p.h2y = static_cast<const T*>(d->h2y);
+  p.n_mpoles = d->n_mpoles;
+  p.mc1x = static_cast<const T*>(d->mc1x);
+  p.mc2x = static_cast<const T*>(d->mc2x);
+  p.mc3x = static_cast<const T*>(d->mc3x);
+  p.mc1y = static_cast<const T*>(d->mc1y);
+  p.mc2y = static_cast<const T*>(d->mc2y);
+  p.mc3y = static_cast<const T*>(d->mc3y);
+  p.Bx = static_cast<T*>(d->Bx);
+  p.By = static_cast<T*>(d->By);
+  p.Mxc = static_cast<T*>(d->Mx_cur);
+  p.Mxp = static_cast<T*>(d->Mx_prev);
+  p.Myc = static_cast<T*>(d->My_cur);
+  p.Myp = static_cast<T*>(d->My_prev);
   mt_source2d sv{};
   if (src && src->present) sv = *src;
   dim3 blk(BX, BY), grd = grid2(p.ny, p.nx, blk);
@@ -694,7 +742,14 @@ extern "C" int mt_tm2d_run(const mt_tm2d* p, const mt_source2d* src, const doubl
   if (!p->d1 != !p->d2) return fail(MT_EINVAL, "sigma_d needs both d1 and d2");
   if (!p->h1x != !p->h2x || !p->h1x != !p->h1y || !p->h1x != !p->h2y)
     return fail(MT_EINVAL, "sigma_b/mu needs h1x, h2x, h1y and h2y");
-  const bool media = p->n_poles || p->kerr || p->d1 || p->h1x;
+  if (p->n_mpoles < 0) return fail(MT_EINVAL, "negative magnetic pole count");
+  if (p->n_mpoles) {
+    if (p->h1x) return fail(MT_EINVAL, "magnetic poles are not combined with sigma_b/mu (h1x..h2y)");
+    if (!p->mc1x || !p->mc2x || !p->mc3x || !p->mc1y || !p->mc2y || !p->mc3y || !p->Bx || !p->By ||
+        !p->Mx_cur || !p->Mx_prev || !p->My_cur || !p->My_prev)
+      return fail(MT_EINVAL, "magnetic poles need mc1..mc3 x/y, Bx, By and the M stacks");
+  }
+  const bool media = p->n_poles || p->kerr || p->d1 || p->h1x || p->n_mpoles;
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 21
+ABI_VERSION = 22
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -41,7 +41,11 @@ class TM2D(ctypes.Structure):
                 ("n_poles", c_int), ("kerr", c_int), ("c1", c_void_p), ("c2", c_void_p), ("c3", c_void_p),
                 ("P_cur", c_void_p), ("P_prev", c_void_p), ("chi2", c_void_p), ("chi3", c_void_p),
                 ("d1", c_void_p), ("d2", c_void_p), ("h1x", c_void_p), ("h2x", c_void_p),
-                ("h1y", c_void_p), ("h2y", c_void_p)]
+                ("h1y", c_void_p), ("h2y", c_void_p),
+                ("n_mpoles", c_int), ("pad4_", c_int), ("mc1x", c_void_p), ("mc2x", c_void_p),
+                ("mc3x", c_void_p), ("mc1y", c_void_p), ("mc2y", c_void_p), ("mc3y", c_void_p),
+                ("Bx", c_void_p), ("By", c_void_p), ("Mx_cur", c_void_p), ("Mx_prev", c_void_p),
+                ("My_cur", c_void_p), ("My_prev", c_void_p)]
 
 
 class TM2DAdj(ctypes.Structure):

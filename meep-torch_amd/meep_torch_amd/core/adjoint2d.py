@@ -300,6 +300,9 @@ def _te_descriptor(c, nx, ny, dtype, device):
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
     if mode == "te" and (coeffs.has_kerr or coeffs.has_chi2):
         raise NotImplementedError("gradients through nonlinear (chi2 / chi3) media are not supported")
+    if mode == "tm" and coeffs.mc1x is not None:
+        raise NotImplementedError("gradients through magnetic poles (m_poles) are not supported: the "
+                                  "hand-written TM adjoint does not transpose the B / M recurrence")
     allowed = ("eps_inf", "chi2", "chi3", "c3") if mode == "tm" else ("eps_x", "eps_y")
     for name, t in coeffs._asdict().items():
         if torch.is_tensor(t) and t.requires_grad and name not in allowed:

@@ -157,8 +157,10 @@ def build_tm_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
                     y_mirror: int = 0, x_mirror: int = 0, y_rot2: int = 0,
                     rot4: int = 0) -> TMCoeffs:
     """coeffs.py:134-291: eps + CPML (+ y_pmc), electric ADE poles
-    (coeffs.py:87-106, 190-198), chi2 / chi3 (coeffs.py:154-158) and
-    sigma_d / sigma_b / mu (coeffs.py:160-175)."""
+    (coeffs.py:87-106, 190-198), chi2 / chi3 (coeffs.py:154-158),
+    sigma_d / sigma_b / mu (coeffs.py:160-175) and magnetic Lorentz poles
+    m_poles (fn, gamma, sigma) with cell masks m_pole_masks
+    (coeffs.py:177-196)."""
     dt, dx = grid.dt, grid.dx
     eps = _as_field(eps_inf, grid, 1.0)
     x3 = _as_field(chi3, grid, 0.0)
@@ -189,7 +191,24 @@ def build_tm_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
         c2[k] = v2 * torch.ones_like(eps)
         c3[k] = v3 * mask.to(eps)
     if len(m_poles):
-        _unsupported("m_poles")
+        # coeffs.py:177-196: magnetic Lorentz poles on the Hx / Hy grids, the
+        # per-pole scalars broadcast and mc3 scaled by the mask averaged onto
+        # the H component (not combined with sigma_b / mu)
+        if cond:
+            raise NotImplementedError("m_poles with sigma_b/mu not combined (YAGNI)")
+        nm = len(m_poles)
+        for suf, shape, avg in (("x", (grid.nx, grid.ny - 1), lambda v: 0.5 * (v[:, :-1] + v[:, 1:])),
+                                ("y", (grid.nx - 1, grid.ny), lambda v: 0.5 * (v[:-1, :] + v[1:, :]))):
+            c1m, c2m, c3m = grid.zeros(nm, *shape), grid.zeros(nm, *shape), grid.zeros(nm, *shape)
+            for k, pole in enumerate(m_poles):
+                w0, G = 2 * math.pi * pole[0], 2 * math.pi * pole[1]
+                den = 1.0 + G * dt / 2.0
+                mask = (m_pole_masks[k] if m_pole_masks is not None
+                        else torch.ones(grid.nx, grid.ny, dtype=grid.dtype, device=grid.device))
+                c1m[k] = (2.0 - w0 * w0 * dt * dt) / den
+                c2m[k] = -(1.0 - G * dt / 2.0) / den
+                c3m[k] = (pole[2] * w0 * w0 * dt * dt / den) * avg(mask.to(device=grid.device, dtype=grid.dtype))
+            cond[f"mc1{suf}"], cond[f"mc2{suf}"], cond[f"mc3{suf}"] = c1m, c2m, c3m
     if y_mirror or x_mirror or y_rot2 or rot4:
         _unsupported("symmetry folds (y_mirror/x_mirror/y_rot2/rot4)")
     px, py = _axis_sides(npml)

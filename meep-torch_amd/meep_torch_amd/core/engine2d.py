@@ -258,8 +258,15 @@ def _tm_media(d, c, fields, dtype, device):
         keep.append(t)
         setattr(d, name, t.data_ptr())
 
-    if c.mc1x is not None:
-        raise NotImplementedError("TM magnetic poles (m_poles) are not supported by the HIP stepper")
+    if c.mc1x is not None:   # magnetic Lorentz poles (coeffs.py:177-196): B, the M stacks and their scalars
+        nm = int(c.mc1x.shape[0])
+        if "Bx" not in fields or int(fields["Mx_cur"].shape[0]) != nm:
+            raise ValueError(f"{nm} magnetic poles need Bx, By and M stacks of {nm} in the state")
+        d.n_mpoles = nm
+        for name in ("mc1x", "mc2x", "mc3x", "mc1y", "mc2y", "mc3y"):
+            put(name, getattr(c, name))
+        for name in ("Bx", "By", "Mx_cur", "Mx_prev", "My_cur", "My_prev"):
+            setattr(d, name, fields[name].data_ptr())
     n = int(c.c1.shape[0])
     if n:
         if "P_cur" not in fields or int(fields["P_cur"].shape[0]) != n:
@@ -389,7 +396,7 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
     for f, t in fields.items():
         if not t.is_contiguous():
             raise ValueError(f"state field {f} must be contiguous")
-    media = bool(d.n_poles or d.kerr or d.d1 or d.h1x) if mode == "tm" else \
+    media = bool(d.n_poles or d.kerr or d.d1 or d.h1x or d.n_mpoles) if mode == "tm" else \
         bool(d.npx or d.npy or d.d1x or d.hz1 or d.kerr)
     if media or (mode == "te" and plan is not None and plan.get("h")):
         variant = -1  # media and TE magnetic currents are handled by the two-pass kernels

@@ -46,8 +46,11 @@ def cpml_profiles(n, npml, offset, dt, dx, m=3, R0=1e-6, alpha_max=0.05,
 
 
 def absorber_sigma(n, nabs, dx, *, R0=1e-15, m=2, dtype=torch.float64, device="cpu"):
-    """Matched-absorber sigma ramp (pml.py:62-80).  Built for API parity; the
-    HIP stepper does not run absorber boundaries (run_3d raises)."""
+    """Matched-absorber sigma ramp (pml.py:62-80): on each absorbing edge of
+    nc cells, smax ((nc - i - 1/2) / nc)^m from the wall inwards, with
+    smax = -(m + 1) ln(R0) / (2 nc dx); zero elsewhere.  Feeds the 3D
+    absorber boundary (build_3d_coeffs n_abs) and the facade's Absorber
+    layers."""
     n_lo, n_hi = side_counts(nabs)
     p = torch.zeros(n, dtype=dtype)
 

@@ -32,34 +32,43 @@ class TMState(NamedTuple):
 
 
 FIELDS = ("Ez", "Dz", "Hx", "Hy", "psi_Ezy", "psi_Ezx", "psi_Hyx", "psi_Hxy")
+MAG_FIELDS = ("Bx", "By", "Mx_cur", "Mx_prev", "My_cur", "My_prev")
 
 
 def init_tm_state(grid: Grid, n_poles: int = 0, y_pmc: bool = False, n_mpoles: int = 0,
                   y_mirror: int = 0, x_mirror: int = 0, y_rot2: int = 0, rot4: int = 0) -> TMState:
-    """stepper_tm.py:37-71 (psi_H rows: ny-2 with PEC walls, ny under y_pmc)."""
-    if n_mpoles or y_mirror or x_mirror or y_rot2 or rot4:
-        raise NotImplementedError(
-            "init_tm_state: magnetic poles / symmetry folds are not supported by the HIP stepper")
+    """stepper_tm.py:37-71 (psi_H rows: ny-2 with PEC walls, ny under y_pmc;
+    with n_mpoles the B fields and the magnetic pole stacks)."""
+    if y_mirror or x_mirror or y_rot2 or rot4:
+        raise NotImplementedError("init_tm_state: symmetry folds are not supported by the HIP stepper")
     nx, ny = grid.nx, grid.ny
     ny_e = ny if y_pmc else ny - 2
+    mag = {}
+    if n_mpoles:
+        mag = dict(Bx=grid.zeros(nx, ny - 1), By=grid.zeros(nx - 1, ny),
+                   Mx_cur=grid.zeros(n_mpoles, nx, ny - 1), Mx_prev=grid.zeros(n_mpoles, nx, ny - 1),
+                   My_cur=grid.zeros(n_mpoles, nx - 1, ny), My_prev=grid.zeros(n_mpoles, nx - 1, ny))
     return TMState(
         Ez=grid.zeros(nx, ny), Hx=grid.zeros(nx, ny - 1), Hy=grid.zeros(nx - 1, ny),
         Dz=grid.zeros(nx, ny), P_cur=grid.zeros(n_poles, nx, ny), P_prev=grid.zeros(n_poles, nx, ny),
         psi_Ezy=grid.zeros(nx, ny - 1), psi_Ezx=grid.zeros(nx - 1, ny),
-        psi_Hyx=grid.zeros(nx - 2, ny_e), psi_Hxy=grid.zeros(nx - 2, ny_e))
+        psi_Hyx=grid.zeros(nx - 2, ny_e), psi_Hxy=grid.zeros(nx - 2, ny_e), **mag)
 
 
 def fields_of(state: TMState):
     """State fields the engine advances: FIELDS plus the pole stacks when the
-    state carries poles."""
-    return FIELDS + (("P_cur", "P_prev") if state.P_cur.shape[0] else ())
+    state carries poles, and B with the magnetic pole stacks."""
+    return (FIELDS + (("P_cur", "P_prev") if state.P_cur.shape[0] else ()) +
+            (MAG_FIELDS if state.Bx is not None else ()))
 
 
 def _check(state: TMState, m: TMCoeffs):
-    if state.Bx is not None or state.Hx0 is not None:
-        raise NotImplementedError("TM state with magnetic poles / mirror rows")
-    if m.mc1x is not None:
-        raise NotImplementedError("TM magnetic poles (m_poles) are not supported by the HIP stepper")
+    if state.Hx0 is not None:
+        raise NotImplementedError("TM state with mirror rows (symmetry folds)")
+    nm = 0 if m.mc1x is None else int(m.mc1x.shape[0])
+    if (state.Bx is not None) != bool(nm) or (nm and int(state.Mx_cur.shape[0]) != nm):
+        raise ValueError(f"magnetic poles: the coefficients have {nm}, the state "
+                         f"{0 if state.Mx_cur is None else int(state.Mx_cur.shape[0])} (init_tm_state(n_mpoles=...))")
     if state.P_cur.shape[0] != m.c1.shape[0]:
         raise ValueError(f"state has {state.P_cur.shape[0]} pole stacks, coefficients {m.c1.shape[0]} poles")
     nx, ny = state.Ez.shape

@@ -87,6 +87,13 @@ class Coef2D:
     h2x: torch.Tensor = None
     h1y: torch.Tensor = None
     h2y: torch.Tensor = None
+    # TM magnetic Lorentz poles: (nm, nx, ny-1) on Hx, (nm, nx-1, ny) on Hy
+    mc1x: torch.Tensor = None
+    mc2x: torch.Tensor = None
+    mc3x: torch.Tensor = None
+    mc1y: torch.Tensor = None
+    mc2y: torch.Tensor = None
+    mc3y: torch.Tensor = None
     # TE media: per-axis poles (np, nx-1, ny) / (np, nx, ny-1), sigma factors
     c1x: torch.Tensor = None
     c2x: torch.Tensor = None
@@ -149,10 +156,33 @@ def _tm_media(c, nx, ny, dtype, poles, pole_masks, chi2, chi3, sigma_d, sigma_b,
     return c
 
 
+def _tm_mpoles(c, nx, ny, dtype, m_poles, m_pole_masks):
+    """meep_torch/core/coeffs.py:177-196: per magnetic pole (fn, gamma,
+    sigma), w0 = 2 pi fn, G = 2 pi gamma, den = 1 + G dt / 2:
+    mc1 = (2 - w0^2 dt^2) / den, mc2 = -(1 - G dt / 2) / den and
+    mc3 = (sigma w0^2 dt^2 / den) * (mask averaged onto the H component)."""
+    dt, nm = c.dt, len(m_poles)
+    for suf, shape, avg in (("x", (nx, ny - 1), lambda v: 0.5 * (v[:, :-1] + v[:, 1:])),
+                            ("y", (nx - 1, ny), lambda v: 0.5 * (v[:-1, :] + v[1:, :]))):
+        m1, m2, m3 = (torch.zeros(nm, *shape, dtype=dtype) for _ in range(3))
+        for k, (fn, gamma, sigma) in enumerate(p[:3] for p in m_poles):
+            w0, G = 2 * math.pi * fn, 2 * math.pi * gamma
+            den = 1.0 + G * dt / 2.0
+            mask = m_pole_masks[k] if m_pole_masks is not None else torch.ones(nx, ny, dtype=dtype)
+            m1[k] = (2.0 - w0 * w0 * dt * dt) / den
+            m2[k] = -(1.0 - G * dt / 2.0) / den
+            m3[k] = (sigma * w0 * w0 * dt * dt / den) * avg(mask.to(dtype))
+        setattr(c, "mc1" + suf, m1)
+        setattr(c, "mc2" + suf, m2)
+        setattr(c, "mc3" + suf, m3)
+
+
 def tm_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, y_pmc=False, poles=(),
-              pole_masks=None, chi2=None, chi3=None, sigma_d=None, sigma_b=None, mu=None):
+              pole_masks=None, chi2=None, chi3=None, sigma_d=None, sigma_b=None, mu=None, m_poles=(),
+              m_pole_masks=None):
     """meep_torch/core/coeffs.py:134-291: eps + CPML (+ y_pmc), electric ADE
-    poles, chi2 / chi3 and sigma_d / sigma_b / mu."""
+    poles, chi2 / chi3, sigma_d / sigma_b / mu and magnetic Lorentz poles
+    (not combined with sigma_b / mu, as in the reference)."""
     dt = courant * dx
     eps = eps.to(dtype)
     if isinstance(npml, (tuple, list)):
@@ -176,7 +206,12 @@ def tm_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, y_pmc=Fal
                b_hx=b_hx[:, None], a_hx=a_hx[:, None],
                b_ex=b_exf[1:-1][:, None], a_ex=a_exf[1:-1][:, None],
                b_ey=b_ey[None, :], a_ey=a_ey[None, :])
-    return _tm_media(c, nx, ny, dtype, poles, pole_masks, chi2, chi3, sigma_d, sigma_b, mu)
+    c = _tm_media(c, nx, ny, dtype, poles, pole_masks, chi2, chi3, sigma_d, sigma_b, mu)
+    if len(m_poles):
+        if c.h1x is not None:
+            raise NotImplementedError("m_poles with sigma_b/mu")
+        _tm_mpoles(c, nx, ny, dtype, m_poles, m_pole_masks)
+    return c
 
 
 def _te_axis_poles(nx, ny, dt, dtype, shared, own, shared_masks, own_masks, shape, avg):
@@ -250,9 +285,10 @@ def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, poles=(),
 
 
 # --------------------------------------------------------------------- state
-def tm_state(nx, ny, dtype=torch.float32, y_pmc=False, n_poles=0):
+def tm_state(nx, ny, dtype=torch.float32, y_pmc=False, n_poles=0, n_mpoles=0):
     """Field dict with the TMState shapes of stepper_tm.py:12-71 (pole
-    stacks only when n_poles > 0)."""
+    stacks only when n_poles > 0; B and the magnetic pole stacks only when
+    n_mpoles > 0)."""
     z = lambda *s: torch.zeros(*s, dtype=dtype)
     ny_e = ny if y_pmc else ny - 2
     out = {"Ez": z(nx, ny), "Hx": z(nx, ny - 1), "Hy": z(nx - 1, ny), "Dz": z(nx, ny),
@@ -260,6 +296,10 @@ def tm_state(nx, ny, dtype=torch.float32, y_pmc=False, n_poles=0):
            "psi_Hyx": z(nx - 2, ny_e), "psi_Hxy": z(nx - 2, ny_e)}
     if n_poles:
         out.update(P_cur=z(n_poles, nx, ny), P_prev=z(n_poles, nx, ny))
+    if n_mpoles:
+        out.update(Bx=z(nx, ny - 1), By=z(nx - 1, ny), Mx_cur=z(n_mpoles, nx, ny - 1),
+                   Mx_prev=z(n_mpoles, nx, ny - 1), My_cur=z(n_mpoles, nx - 1, ny),
+                   My_prev=z(n_mpoles, nx - 1, ny))
     return out
 
 
@@ -293,16 +333,30 @@ def tm_step(s, c, coef=0.0, mask=None, hcoef=0.0, hx_mask=None, hy_mask=None):
     pzy = c.b_hy * s["psi_Ezy"] + c.a_hy * g
     g2 = (Ez[1:, :] - Ez[:-1, :]) / dx
     pzx = c.b_hx * s["psi_Ezx"] + c.a_hx * g2
-    if c.h1x is not None:       # stepper_tm.py:116-121: dB/dt = -curl E - sigma_b B, mu folded in h2
-        Hx = c.h1x * s["Hx"] - c.h2x * (g + pzy)
-        Hy = c.h1y * s["Hy"] + c.h2y * (g2 + pzx)
+    mag = {}
+    if c.mc1x is not None:      # stepper_tm.py:103-106: B integrates the curl
+        Bx = s["Bx"] - dt * (g + pzy)
+        By = s["By"] + dt * (g2 + pzx)
+        if hy_mask is not None:     # stepper_tm.py:185-193: the TFSF currents go into B
+            By = By - hcoef * hy_mask
+        if hx_mask is not None:
+            Bx = Bx - hcoef * hx_mask
+        # stepper_tm.py:214-224: M' = mc1 M + mc2 M_prev + mc3 H(n-1/2), H = B - sum M'
+        Mx = c.mc1x * s["Mx_cur"] + c.mc2x * s["Mx_prev"] + c.mc3x * s["Hx"]
+        My = c.mc1y * s["My_cur"] + c.mc2y * s["My_prev"] + c.mc3y * s["Hy"]
+        Hx, Hy = Bx - Mx.sum(0), By - My.sum(0)
+        mag = dict(Bx=Bx, By=By, Mx_cur=Mx, Mx_prev=s["Mx_cur"], My_cur=My, My_prev=s["My_cur"])
     else:
-        Hx = s["Hx"] - dt * (g + pzy)
-        Hy = s["Hy"] + dt * (g2 + pzx)
-    if hy_mask is not None:
-        Hy = Hy - hcoef * hy_mask
-    if hx_mask is not None:
-        Hx = Hx - hcoef * hx_mask
+        if c.h1x is not None:       # stepper_tm.py:116-121: dB/dt = -curl E - sigma_b B, mu folded in h2
+            Hx = c.h1x * s["Hx"] - c.h2x * (g + pzy)
+            Hy = c.h1y * s["Hy"] + c.h2y * (g2 + pzx)
+        else:
+            Hx = s["Hx"] - dt * (g + pzy)
+            Hy = s["Hy"] + dt * (g2 + pzx)
+        if hy_mask is not None:
+            Hy = Hy - hcoef * hy_mask
+        if hx_mask is not None:
+            Hx = Hx - hcoef * hx_mask
 
     if c.y_pmc:
         Hxp = torch.nn.functional.pad(Hx, (1, 1))
@@ -324,6 +378,7 @@ def tm_step(s, c, coef=0.0, mask=None, hcoef=0.0, hx_mask=None, hy_mask=None):
         else:
             Dz = Dz - coef * mask
     out = dict(s)
+    out.update(mag)
     rhs = Dz
     if c.c1 is not None and c.c1.shape[0]:
         # stepper_tm.py:203-208: P(n+1) = c1 P(n) + c2 P(n-1) + c3 Ez(n)

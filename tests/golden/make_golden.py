@@ -30,6 +30,11 @@ Cases (all CPU, torch threads = 8):
   media_tm   2D TM 48x40, npml 6, 240 steps, f32 + f64: (all) Lorentz + Drude
              poles with masks, chi2 and chi3 regions, sigma_d, sigma_b and mu,
              line source; (nl) chi2 + chi3 only under a strong drive
+  mag_tm     2D TM 48x40, npml 6, 240 steps, f32 + f64, magnetic Lorentz poles
+             (m_poles with cell masks): (mag) with two electric poles and
+             chi3 under a line source, flux line; (magh) alone under a line
+             source plus a TFSF-style Hy current (h_masks / h_amplitude),
+             which the B phase injects into B
   media_te   2D TE analogue: shared and per-axis poles, chi2 / chi3, sigma_d,
              sigma_b, mu; (nl) chi2 + chi3 only
   media_3d   3D 22x20x18, npml 4, 120 steps, sigma_d, sigma_b and a Lorentz +
@@ -476,6 +481,68 @@ def case_media_tm():
         save(f"media_tm_{tag}", **out)
 
 
+MPOLES2 = [(1.0, 0.2, 1.5), (0.7, 0.1, 0.8)]
+
+
+class _HSource:
+    """A run_tm source with an electric line mask and one magnetic (Hy)
+    current mask, each with its own pulse (the h_masks protocol of
+    core/run.py:17-23)."""
+
+    def __init__(self, mask, amp, hy_mask, h_amp):
+        self.mask, self._a, self.h_masks, self._h = mask, amp, {"src_hy_mask": hy_mask}, h_amp
+
+    def amplitude(self, t):
+        return self._a(t)
+
+    def h_amplitude(self, t):
+        return self._h(t)
+
+
+def case_mag_tm():
+    from meep_torch.core.coeffs import build_tm_coeffs
+    from meep_torch.core.grid import Grid
+    from meep_torch.core.monitors import FluxMonitorTM
+    from meep_torch.core.run import run_tm
+    from meep_torch.core.sources import GaussianPulse, LineSource
+    from meep_torch.core.stepper_tm import init_tm_state
+    nx, ny, n = 48, 40, 240
+    mp_ = _media_maps(nx, ny)
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        g = Grid(nx=nx, ny=ny, dx=0.05, courant=0.5, dtype=dtype, device="cpu")
+        cv = {k: v.to(dtype) for k, v in mp_.items()}
+        out = {k: v for k, v in cv.items()}
+        mm = [cv["m2"] + 0.5 * cv["m1"], cv["m1"]]
+        out["mm1"], out["mm2"] = mm
+        line = LineSource(g, ix=10, j0=8, j1=ny - 8, profile=GaussianPulse(0.8, 0.6, cutoff=3.0))
+        mon = FluxMonitorTM(g, [0.6, 0.8, 1.0], x_normal=True, idx=38, lo=7, hi=ny - 7)
+        m = build_tm_coeffs(g, eps_inf=cv["eps"], npml=6, poles=POLES2, pole_masks=[cv["m1"], cv["m2"]],
+                            chi3=cv["chi3"], m_poles=MPOLES2, m_pole_masks=mm)
+        line.mask = line.mask * 20.0
+        st, (acc,) = run_tm(init_tm_state(g, n_poles=2, n_mpoles=2), m, n, source=line, monitors=[mon])
+        out["mag_mask"] = line.mask
+        out["mag_amp"] = np.array([line.amplitude(t) for t in range(n)])
+        for f in st._fields:
+            v = getattr(st, f)
+            if isinstance(v, torch.Tensor) and v.numel():
+                out[f"mag_{f}"] = v
+        out["mag_acc"] = acc
+        hy = torch.zeros(nx - 1, ny, dtype=dtype)
+        hy[14, 10:ny - 10] = 1.0 / 0.05
+        hp = GaussianPulse(1.0, 0.8, cutoff=3.0)
+        src = _HSource(line.mask, line.amplitude, hy, lambda t: 0.7 * hp.amplitude((t + 1.0) * g.dt))
+        m = build_tm_coeffs(g, eps_inf=cv["eps"], npml=6, m_poles=MPOLES2, m_pole_masks=mm)
+        st, (acc,) = run_tm(init_tm_state(g, n_mpoles=2), m, n, source=src, monitors=[mon])
+        out["magh_hy_mask"] = hy
+        out["magh_h_amp"] = np.array([src.h_amplitude(t) for t in range(n)])
+        for f in st._fields:
+            v = getattr(st, f)
+            if isinstance(v, torch.Tensor) and v.numel():
+                out[f"magh_{f}"] = v
+        out["magh_acc"] = acc
+        save(f"mag_tm_{tag}", **out)
+
+
 def case_media_te():
     from meep_torch.core.coeffs import build_te_coeffs
     from meep_torch.core.grid import Grid
@@ -756,7 +823,8 @@ def case_sim_absorber():
 def main():
     which = sys.argv[1:] or ["pml", "tm", "te", "c1", "3d", "grad_tm", "grad_3d",
                              "filters", "golden_plain", "media_tm", "media_te", "media_3d",
-                             "absorber_3d", "tfsf_tm", "sim3d", "raster", "shapes", "multisrc", "sim_absorber"]
+                             "absorber_3d", "tfsf_tm", "sim3d", "raster", "shapes", "multisrc", "sim_absorber",
+                             "mag_tm"]
     t_all = time.perf_counter()
     timing = {}
     try:
@@ -773,7 +841,8 @@ def main():
          "media_tm": case_media_tm, "media_te": case_media_te, "media_3d": case_media_3d,
          "absorber_3d": case_absorber_3d,
          "tfsf_tm": case_tfsf_tm, "sim3d": case_sim3d, "raster": case_raster,
-         "shapes": case_shapes, "multisrc": case_multisrc, "sim_absorber": case_sim_absorber}[w]()
+         "shapes": case_shapes, "multisrc": case_multisrc, "sim_absorber": case_sim_absorber,
+         "mag_tm": case_mag_tm}[w]()
         print(f"[golden] {w}: {time.perf_counter() - t0:.1f}s", flush=True)
     man = {}
     for fn in sorted(os.listdir(HERE)):

@@ -98,15 +98,35 @@ def test_mask_box():
 @pytest.mark.parametrize("te,kw,flag", [(True, {"poles_tensor": [(1.0, 0.1, (1.0, 0.2, 1.0))]}, "poles_tensor"),
                                         (True, {"eps_tensor": (2.0, 0.1, 2.0)}, "eps_tensor"),
                                         (False, {"y_mirror": 1}, "symmetry"),
-                                        (False, {"m_poles": [(1.0, 0.1, 1.0)]}, "m_poles")])
+                                        (False, {"m_poles": [(1.0, 0.1, 1.0)], "mu": 2.0}, "m_poles")])
 def test_unsupported_flags_raise(te, kw, flag):
-    """TE oriented poles and tensor eps, TM magnetic poles and symmetry
-    folds still raise by name."""
+    """TE oriented poles and tensor eps, symmetry folds, and TM magnetic
+    poles combined with sigma_b / mu (which the reference refuses too,
+    coeffs.py:178-179) raise by name."""
     from meep_torch_amd.core.coeffs import build_te_coeffs, build_tm_coeffs
     from meep_torch_amd.core.grid import Grid
     g = Grid(nx=16, ny=12, dx=0.05, dtype=torch.float32, device="cpu")
     with pytest.raises(NotImplementedError, match=flag):
         (build_te_coeffs if te else build_tm_coeffs)(g, npml=2, **kw)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_tm_magnetic_pole_coefficients_equal_the_oracle(dtype):
+    """build_tm_coeffs's magnetic pole arrays (mc1..mc3 on the Hx / Hy grids)
+    are the oracle's bit for bit (the oracle's are pinned to the reference's
+    runs by tests/test_oracle_media_golden.py)."""
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from oracle import yee2d
+    gm = torch.Generator().manual_seed(5)
+    masks = [torch.rand(16, 12, generator=gm, dtype=torch.float64).to(dtype) for _ in range(2)]
+    poles = [(1.0, 0.2, 1.5), (0.7, 0.1, 0.8)]
+    g = Grid(nx=16, ny=12, dx=0.05, dtype=dtype, device="cpu")
+    c = build_tm_coeffs(g, npml=2, m_poles=poles, m_pole_masks=masks)
+    oc = yee2d.tm_coeffs(16, 12, 0.05, torch.ones(16, 12), 2, dtype=dtype, m_poles=poles, m_pole_masks=masks)
+    for name in ("mc1x", "mc2x", "mc3x", "mc1y", "mc2y", "mc3y"):
+        assert torch.equal(getattr(c, name), getattr(oc, name)), name
+    assert tuple(c.mc3x.shape) == (2, 16, 11) and tuple(c.mc3y.shape) == (2, 15, 12)
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])

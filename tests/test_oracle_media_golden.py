@@ -9,6 +9,8 @@ in FP32 and FP64.
   under a strong drive ("nl"), where the three-pass fixed point
   (stepper_tm.py:228-231, stepper_te.py:161-175) changes the fields by far
   more than rounding;
+* TM magnetic Lorentz poles, with electric poles and chi3 and under a TFSF
+  magnetic current (mag_tm);
 * 3D 22 x 20 x 18 with sigma_d, sigma_b and two poles under a point source,
   and the boundary modes without a full CPML (absorber_3d).
 
@@ -115,6 +117,36 @@ def test_3d_media_bitexact(tag, dtype):
         _eq(s[f], g[f], f)
     _eq(acc[0], g["acc"], "acc")
     _eq(yee3d.flux_plane(acc[0], 0.05), g["flux"], "flux")
+
+
+MPOLES2 = [(1.0, 0.2, 1.5), (0.7, 0.1, 0.8)]
+MAG_FIELDS = ("Ez", "Dz", "Hx", "Hy", "Bx", "By", "Mx_cur", "Mx_prev", "My_cur", "My_prev", "psi_Ezy", "psi_Hyx")
+
+
+@pytest.mark.parametrize("case", ["mag", "magh"])
+@pytest.mark.parametrize("tag,dtype", [("f32", torch.float32), ("f64", torch.float64)])
+def test_tm_magnetic_poles_bitexact(case, tag, dtype):
+    """Magnetic Lorentz poles in 2D TM (coeffs.py:177-196, stepper_tm.py:
+    103-106,180-193,209-224), reproduced bit for bit from the reference's
+    runs: B integrates the curl, the TFSF Hy current goes into B, and
+    H = B - sum M' with M' = mc1 M + mc2 M_prev + mc3 H(n-1/2); with
+    electric poles and chi3 in the same cell (mag) and alone under a
+    magnetic current (magh).  A dropped mc3 term, H(n+1/2) in place of
+    H(n-1/2) in the recurrence, or the current into H instead of B fails
+    here."""
+    g = golden(f"mag_tm_{tag}")
+    nx, ny, n = 48, 40, 240
+    kw = dict(m_poles=MPOLES2, m_pole_masks=[_t(g["mm1"]), _t(g["mm2"])])
+    if case == "mag":
+        kw.update(poles=POLES2, pole_masks=[_t(g["m1"]), _t(g["m2"])], chi3=_t(g["chi3"]))
+    c = yee2d.tm_coeffs(nx, ny, 0.05, _t(g["eps"]), 6, dtype=dtype, **kw)
+    mons = [{"type": "flux", "kind": "tm_x", "idx": 38, "lo": 7, "hi": ny - 7, "freqs": FREQS}]
+    st = yee2d.tm_state(nx, ny, dtype, n_poles=2 if case == "mag" else 0, n_mpoles=2)
+    hk = {} if case == "mag" else dict(hamps=g["magh_h_amp"], hmasks={"hy_mask": _t(g["magh_hy_mask"])})
+    s, acc = yee2d.run2d("tm", st, c, n, amps=g["mag_amp"], mask=_t(g["mag_mask"]), monitors=mons, **hk)
+    for f in MAG_FIELDS + (("P_cur",) if case == "mag" else ()):
+        _eq(s[f], g[f"{case}_{f}"], f"{case} {f}")
+    _eq(acc[0], g[f"{case}_acc"], f"{case} acc")
 
 
 ABS_CASES = {"abs": dict(npml=0, n_abs=4, media=True), "mixed": dict(npml=4, n_abs=4, axes=(2,), media=False),
