@@ -154,6 +154,16 @@ typedef struct {
   int kerr, pad3_;
   const void *chi2x, *chi3x, *chi2y, *chi3y;
   void *kx, *ky;
+  /* optional in-plane tensor eps (coeffs.py:390-400, stepper_te.py:158-160):
+   * inv_xx_x, inv_xy_x (nx-1, ny), inv_yy_y, inv_xy_y (nx, ny-1), all four or
+   * none; E = inv_xx_x rhs_x + inv_xy_x avg4(rhs_y) (and the y analogue)
+   * with rhs = D - sum P, before any chi2/chi3 passes.  Optional oriented
+   * poles (coeffs.py:425-455, stepper_te.py:137-147): c3xy (npx, nx-1, ny)
+   * and c3yx (npy, nx, ny-1), both or none: P' += c3xy avg4(Ey(n-1/2)) on Ex
+   * (c3yx avg4(Ex) on Ey).  avg4 is the four-point average with zeros
+   * beyond the walls.  Either needs kx, ky. */
+  const void *inv_xx_x, *inv_xy_x, *inv_yy_y, *inv_xy_y;
+  const void *c3xy, *c3yx;
 } mt_te2d;
 
 

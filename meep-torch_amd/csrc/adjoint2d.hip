@@ -374,6 +374,7 @@ extern "C" int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const voi
   if (k < 0) return fail(MT_EINVAL, "negative step count");
   if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward E tape");
   if (p->kerr) return fail(MT_EINVAL, "adjoint through nonlinear (chi2 / chi3) media is not supported");
+  if (p->inv_xx_x || p->c3xy) return fail(MT_EINVAL, "the TE adjoint does not transpose tensor eps or oriented poles");
   if (p->npx < 0 || p->npy < 0 ||
       (p->npx && (!a->aPx_cur || !a->aPx_prev || !p->c1x || !p->c2x || !p->c3x)) ||
       (p->npy && (!a->aPy_cur || !a->aPy_prev || !p->c1y || !p->c2y || !p->c3y)))

@@ -212,11 +212,35 @@ struct TeK {
   T *Pxc, *Pxp, *Pyc, *Pyp;
   int kerr;
   const T *chi2x, *chi3x, *chi2y, *chi3y;
-  T *kx, *ky;  // scratch of the Ex / Ey shapes for the Kerr passes
+  T *kx, *ky;  // scratch of the Ex / Ey shapes for the Kerr passes, tensor eps and oriented poles
+  // optional tensor eps (inverse components on the E grids) and oriented poles
+  const T *ixx, *ixyx, *iyy, *ixyy, *c3xy, *c3yx;
+  int scratch;  // the D half writes E (or, with tensor eps, rhs) to kx / ky instead of Ex / Ey
 };
 
+// Four-point averages with zeros beyond the walls (stepper_te.py:52-61): of
+// an Ey-shaped field at Ex (i, j) (i < nx-1), and of an Ex-shaped field at
+// Ey (i, j) (j < ny-1), summed in the reference's order.
+template <typename T>
+__device__ __forceinline__ T avg4_to_ex(const T* ey, int nx, int ny, int i, int j) {
+  const T a = j >= 1 ? ey[(long long)i * (ny - 1) + j - 1] : T(0);
+  const T b = j < ny - 1 ? ey[(long long)i * (ny - 1) + j] : T(0);
+  const T c = j >= 1 ? ey[(long long)(i + 1) * (ny - 1) + j - 1] : T(0);
+  const T d = j < ny - 1 ? ey[(long long)(i + 1) * (ny - 1) + j] : T(0);
+  return T(0.25) * (a + b + c + d);
+}
+
+template <typename T>
+__device__ __forceinline__ T avg4_to_ey(const T* ex, int nx, int ny, int i, int j) {
+  const T a = i >= 1 ? ex[(long long)(i - 1) * ny + j] : T(0);
+  const T b = i >= 1 ? ex[(long long)(i - 1) * ny + j + 1] : T(0);
+  const T c = i < nx - 1 ? ex[(long long)i * ny + j] : T(0);
+  const T d = i < nx - 1 ? ex[(long long)i * ny + j + 1] : T(0);
+  return T(0.25) * (a + b + c + d);
+}
+
 // D - sum P of one E cell with the pole stacks already advanced (the value
-// te_e_from_d divided by eps, recomputed in the same order).
+// te_rhs_from_d returned, recomputed in the same order).
 template <typename T>
 __device__ __forceinline__ T te_rhs(T d, int np, long long plane, long long c, const T* pc) {
   if (!np) return d;
@@ -237,11 +261,7 @@ __global__ __launch_bounds__(BX* BY) void te_kerr_kernel(TeK<T> p, const T* exi,
   if (i >= nx || j >= ny) return;
   if (i < nx - 1) {  // Ex (nx-1, ny); Ey neighbours (i, j-1) (i, j) (i+1, j-1) (i+1, j)
     const long long c = (long long)i * ny + j;
-    const T a = j >= 1 ? eyi[(long long)i * (ny - 1) + j - 1] : T(0);
-    const T b = j < ny - 1 ? eyi[(long long)i * (ny - 1) + j] : T(0);
-    const T cc = j >= 1 ? eyi[(long long)(i + 1) * (ny - 1) + j - 1] : T(0);
-    const T d = j < ny - 1 ? eyi[(long long)(i + 1) * (ny - 1) + j] : T(0);
-    const T av = T(0.25) * (a + b + cc + d);
+    const T av = avg4_to_ex(eyi, nx, ny, i, j);
     const T e = exi[c];
     const T e2 = e * e + av * av;
     const T rhs = te_rhs(p.Dx[c], p.npx, (long long)(nx - 1) * ny, c, p.Pxc);
@@ -249,11 +269,7 @@ __global__ __launch_bounds__(BX* BY) void te_kerr_kernel(TeK<T> p, const T* exi,
   }
   if (j < ny - 1) {  // Ey (nx, ny-1); Ex neighbours (i-1, j) (i-1, j+1) (i, j) (i, j+1)
     const long long c = (long long)i * (ny - 1) + j;
-    const T a = i >= 1 ? exi[(long long)(i - 1) * ny + j] : T(0);
-    const T b = i >= 1 ? exi[(long long)(i - 1) * ny + j + 1] : T(0);
-    const T cc = i < nx - 1 ? exi[(long long)i * ny + j] : T(0);
-    const T d = i < nx - 1 ? exi[(long long)i * ny + j + 1] : T(0);
-    const T av = T(0.25) * (a + b + cc + d);
+    const T av = avg4_to_ey(exi, nx, ny, i, j);
     const T e = eyi[c];
     const T e2 = e * e + av * av;
     const T rhs = te_rhs(p.Dy[c], p.npy, (long long)nx * (ny - 1), c, p.Pyc);
@@ -261,26 +277,46 @@ __global__ __launch_bounds__(BX* BY) void te_kerr_kernel(TeK<T> p, const T* exi,
   }
 }
 
-// E = (D - sum P) / eps after advancing the pole stacks from the E before
-// the step (stepper_te.py:137-147, 160-168); the stacks have `np` planes of
-// `plane` cells.
+// rhs = D - sum P after advancing the pole stacks from the E before the step
+// (stepper_te.py:137-147, 156-157); with oriented poles each P' also gains
+// c3o * av, the other component's four-point average (c3o = 0 for the
+// scalar poles); the stacks have `np` planes of `plane` cells.
 template <typename T>
-__device__ __forceinline__ T te_e_from_d(T d, T eps, T e_old, int np, long long plane, long long c, const T* c1,
-                                         const T* c2, const T* c3, T* pc, T* pp) {
+__device__ __forceinline__ T te_rhs_from_d(T d, T e_old, int np, long long plane, long long c, const T* c1,
+                                           const T* c2, const T* c3, T* pc, T* pp, const T* c3o, T av) {
   T rhs = d;
   if (np) {
     T sum = T(0);
     for (int k = 0; k < np; ++k) {
       const long long q = k * plane + c;
       const T a = pc[q], b = pp[q];
-      const T pn = c1[q] * a + c2[q] * b + c3[q] * e_old;
+      T pn = c1[q] * a + c2[q] * b + c3[q] * e_old;
+      if (c3o) pn = pn + c3o[q] * av;
       pp[q] = a;
       pc[q] = pn;
       sum = k == 0 ? pn : sum + pn;
     }
     rhs = d - sum;
   }
-  return rhs / eps;
+  return rhs;
+}
+
+// Tensor eps (stepper_te.py:158-160): E = inv_xx rhs_x + inv_xy avg4(rhs_y)
+// and E_y = inv_yy rhs_y + inv_xy avg4(rhs_x), from the rhs pair (rx, ry).
+template <typename T>
+__global__ __launch_bounds__(BX* BY) void te_tensor_kernel(TeK<T> p, const T* rx, const T* ry, T* exo, T* eyo) {
+  int j = blockIdx.x * BX + threadIdx.x;
+  int i = blockIdx.y * BY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx || j >= ny) return;
+  if (i < nx - 1) {
+    const long long c = (long long)i * ny + j;
+    exo[c] = p.ixx[c] * rx[c] + p.ixyx[c] * avg4_to_ex(ry, nx, ny, i, j);
+  }
+  if (j < ny - 1) {
+    const long long c = (long long)i * (ny - 1) + j;
+    eyo[c] = p.iyy[c] * ry[c] + p.ixyy[c] * avg4_to_ey(rx, nx, ny, i, j);
+  }
 }
 
 // meep_torch/core/stepper_te.py:80-103 (D half), :131-134 (source on Dy),
@@ -309,9 +345,11 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
     }
     if (p.d1x) dxv = p.d1x[c] * dxv + p.d2x[c] * curl;  // stepper_te.py:96-99 (sigma_d)
     p.Dx[c] = dxv;
-    const T ex = te_e_from_d(dxv, p.eps_x[c], p.npx ? p.Ex[c] : T(0), p.npx, (long long)(nx - 1) * ny, c, p.c1x,
-                             p.c2x, p.c3x, p.Pxc, p.Pxp);
-    (p.kerr ? p.kx : p.Ex)[c] = ex;  // Kerr: the fixed point starts from the scratch copy
+    const T av = p.c3xy ? avg4_to_ex(p.Ey, nx, ny, i, j) : T(0);  // E(n-1/2): the D half writes E to scratch
+    const T rhs = te_rhs_from_d(dxv, p.npx ? p.Ex[c] : T(0), p.npx, (long long)(nx - 1) * ny, c, p.c1x, p.c2x,
+                                p.c3x, p.Pxc, p.Pxp, p.c3xy, av);
+    const T ex = p.ixx ? rhs : rhs / p.eps_x[c];  // tensor eps: E is formed from the rhs pair afterwards
+    (p.scratch ? p.kx : p.Ex)[c] = ex;
     if (p.tape) p.tape[(long long)s_rel * ((long long)(nx - 1) * ny + (long long)nx * (ny - 1)) + c] = ex;
   }
   if (j < ny - 1) {  // Dy (nx, ny-1)
@@ -338,9 +376,11 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
       dyv = p.d2y ? dyv - (p.d2y[c] * coef) * wv : dyv - coef * wv;
     }
     p.Dy[c] = dyv;
-    const T ey = te_e_from_d(dyv, p.eps_y[c], p.npy ? p.Ey[c] : T(0), p.npy, (long long)nx * (ny - 1), c, p.c1y,
-                             p.c2y, p.c3y, p.Pyc, p.Pyp);
-    (p.kerr ? p.ky : p.Ey)[c] = ey;
+    const T av = p.c3yx ? avg4_to_ey(p.Ex, nx, ny, i, j) : T(0);
+    const T rhs = te_rhs_from_d(dyv, p.npy ? p.Ey[c] : T(0), p.npy, (long long)nx * (ny - 1), c, p.c1y, p.c2y,
+                                p.c3y, p.Pyc, p.Pyp, p.c3yx, av);
+    const T ey = p.iyy ? rhs : rhs / p.eps_y[c];
+    (p.scratch ? p.ky : p.Ey)[c] = ey;
     if (p.tape)
       p.tape[(long long)s_rel * ((long long)(nx - 1) * ny + (long long)nx * (ny - 1)) + (long long)(nx - 1) * ny + c] = ey;
   }
@@ -499,6 +539,14 @@ int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const
   p.chi3y = static_cast<const T*>(d->chi3y);
   p.kx = static_cast<T*>(d->kx);
   p.ky = static_cast<T*>(d->ky);
+  p.ixx = static_cast<const T*>(d->inv_xx_x);
+  p.ixyx = static_cast<const T*>(d->inv_xy_x);
+  p.iyy = static_cast<const T*>(d->inv_yy_y);
+  p.ixyy = static_cast<const T*>(d->inv_xy_y);
+  p.c3xy = static_cast<const T*>(d->c3xy);
+  p.c3yx = static_cast<const T*>(d->c3yx);
+  p.scratch = p.kerr || p.ixx || p.c3xy;
+  const size_t bx = (size_t)(p.nx - 1) * p.ny * sizeof(T), by = (size_t)p.nx * (p.ny - 1) * sizeof(T);
   mt_source2d sv{};
   if (src && src->present) sv = *src;
   dim3 blk(BX, BY), grd = grid2(p.ny, p.nx, blk);
@@ -507,6 +555,16 @@ int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const
   for (int s_ = 0; s_ < k; ++s_) {
     T c = sv.present ? static_cast<T>(coef[s_]) : T(0);
     hipLaunchKernelGGL(te_d_kernel<T>, grd, blk, 0, s, p, step_source<T>(sv, s_), c, s_);
+    if (p.ixx) {  // tensor eps: E from the rhs pair in scratch; the chi2/chi3 passes start from it
+      hipLaunchKernelGGL(te_tensor_kernel<T>, grd, blk, 0, s, p, p.kx, p.ky, p.Ex, p.Ey);
+      if (p.kerr && (hipMemcpyAsync(p.kx, p.Ex, bx, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                     hipMemcpyAsync(p.ky, p.Ey, by, hipMemcpyDeviceToDevice, s) != hipSuccess))
+        return fail(MT_EHIP, "te2d: scratch copy failed");
+    } else if (p.scratch && !p.kerr) {  // oriented poles only: E was staged in scratch
+      if (hipMemcpyAsync(p.Ex, p.kx, bx, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+          hipMemcpyAsync(p.Ey, p.ky, by, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return fail(MT_EHIP, "te2d: scratch copy failed");
+    }
     if (p.kerr) {  // three passes: scratch -> E -> scratch -> E
       hipLaunchKernelGGL(te_kerr_kernel<T>, grd, blk, 0, s, p, p.kx, p.ky, p.Ex, p.Ey);
       hipLaunchKernelGGL(te_kerr_kernel<T>, grd, blk, 0, s, p, p.Ex, p.Ey, p.kx, p.ky);
@@ -793,7 +851,15 @@ extern "C" int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const doubl
   if (p->kerr && (!p->chi2x || !p->chi3x || !p->chi2y || !p->chi3y || !p->kx || !p->ky))
     return fail(MT_EINVAL, "TE chi2/chi3 fixed point needs chi2x, chi3x, chi2y, chi3y and the kx, ky scratch");
   if (p->kerr && p->tape) return fail(MT_EINVAL, "the E tape is not written under chi2/chi3");
-  const bool media = p->npx || p->npy || p->d1x || p->hz1 || p->kerr;
+  const bool tensor = p->inv_xx_x || p->inv_xy_x || p->inv_yy_y || p->inv_xy_y;
+  if (tensor && !(p->inv_xx_x && p->inv_xy_x && p->inv_yy_y && p->inv_xy_y))
+    return fail(MT_EINVAL, "tensor eps needs inv_xx_x, inv_xy_x, inv_yy_y and inv_xy_y");
+  if (!p->c3xy != !p->c3yx) return fail(MT_EINVAL, "oriented poles need both c3xy and c3yx");
+  if (p->c3xy && (!p->npx || !p->npy)) return fail(MT_EINVAL, "oriented poles need Ex and Ey pole stacks");
+  if ((tensor || p->c3xy) && (!p->kx || !p->ky))
+    return fail(MT_EINVAL, "tensor eps and oriented poles need the kx, ky scratch");
+  if ((tensor || p->c3xy) && p->tape) return fail(MT_EINVAL, "the E tape is not written with tensor eps or oriented poles");
+  const bool media = p->npx || p->npy || p->d1x || p->hz1 || p->kerr || tensor || p->c3xy;
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);

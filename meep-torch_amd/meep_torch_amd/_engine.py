@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 22
+ABI_VERSION = 23
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -70,7 +70,9 @@ class TE2D(ctypes.Structure):
                 ("Px_prev", c_void_p), ("Py_cur", c_void_p), ("Py_prev", c_void_p), ("d1x", c_void_p),
                 ("d2x", c_void_p), ("d1y", c_void_p), ("d2y", c_void_p), ("hz1", c_void_p), ("hz2", c_void_p),
                 ("kerr", c_int), ("pad3_", c_int), ("chi2x", c_void_p), ("chi3x", c_void_p), ("chi2y", c_void_p),
-                ("chi3y", c_void_p), ("kx", c_void_p), ("ky", c_void_p)]
+                ("chi3y", c_void_p), ("kx", c_void_p), ("ky", c_void_p),
+                ("inv_xx_x", c_void_p), ("inv_xy_x", c_void_p), ("inv_yy_y", c_void_p), ("inv_xy_y", c_void_p),
+                ("c3xy", c_void_p), ("c3yx", c_void_p)]
 
 
 class TE2DAdj(ctypes.Structure):

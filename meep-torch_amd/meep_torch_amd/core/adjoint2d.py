@@ -300,6 +300,8 @@ def _te_descriptor(c, nx, ny, dtype, device):
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
     if mode == "te" and (coeffs.has_kerr or coeffs.has_chi2):
         raise NotImplementedError("gradients through nonlinear (chi2 / chi3) media are not supported")
+    if mode == "te" and (coeffs.inv_xx_x is not None or coeffs.c3xy is not None):
+        raise NotImplementedError("gradients through TE tensor eps or oriented poles are not supported")
     if mode == "tm" and coeffs.mc1x is not None:
         raise NotImplementedError("gradients through magnetic poles (m_poles) are not supported: the "
                                   "hand-written TM adjoint does not transpose the B / M recurrence")

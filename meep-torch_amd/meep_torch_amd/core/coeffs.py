@@ -276,17 +276,29 @@ def build_te_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
     per-axis (poles_x / poles_y, coeffs.py:109-131,420-427), sigma_d /
     sigma_b / mu (coeffs.py:399-410), chi2 / chi3 averaged onto the E
     positions (coeffs.py:383-384).  eps_x / eps_y are the two-point averages
-    of the cell eps (coeffs.py:347-355,385).  Tensor eps, oriented poles and
-    symmetry folds raise."""
+    of the cell eps (coeffs.py:347-355,385).  eps_tensor = (exx, exy, eyy)
+    at cell centres gives the inverse components Yee-averaged onto the E
+    positions (coeffs.py:390-400); poles_tensor (fn, gamma, (sxx, sxy, syy)
+    [, fc]) append one pole to each axis stack, with the off-diagonal drive
+    c3xy / c3yx (coeffs.py:425-455).  Symmetry folds raise."""
     dt, dx = grid.dt, grid.dx
     eps = _as_field(eps_inf, grid, 1.0)
     x3 = _as_field(chi3, grid, 0.0)
     x2 = _as_field(chi2, grid, 0.0)
     has_kerr = bool((x3 != 0).any())
     has_chi2 = bool((x2 != 0).any())
-    if len(poles_tensor):
-        _unsupported("poles_tensor (oriented poles)")
     cond = {}
+    if eps_tensor is not None:
+        # symmetric in-plane tensor at cell centres -> its inverse components
+        # averaged onto the Ex / Ey positions
+        exx = _as_field(eps_tensor[0], grid, 1.0)
+        exy = _as_field(eps_tensor[1], grid, 0.0)
+        eyy = _as_field(eps_tensor[2], grid, 1.0)
+        det = exx * eyy - exy * exy
+        cond["inv_xx_x"] = _avg_to_ex(eyy / det)
+        cond["inv_xy_x"] = _avg_to_ex(-exy / det)
+        cond["inv_yy_y"] = _avg_to_ey(exx / det)
+        cond["inv_xy_y"] = _avg_to_ey(-exy / det)
     if sigma_d is not None or sigma_b is not None or mu is not None:
         sd = _as_field(sigma_d, grid, 0.0)
         sb = _as_field(sigma_b, grid, 0.0)
@@ -305,8 +317,28 @@ def build_te_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
                                 (grid.nx - 1, grid.ny), _avg_to_ex)
     c1y, c2y, c3y = _axis_poles(grid, dt, eps, poles, poles_y, pole_masks, pole_masks_y,
                                 (grid.nx, grid.ny - 1), _avg_to_ey)
-    if eps_tensor is not None:
-        _unsupported("eps_tensor (anisotropic eps)")
+    if len(poles_tensor):
+        # oriented poles: each adds one pole to BOTH axis stacks (after the
+        # scalar ones) with c3 = v3 sxx / syy times the averaged mask, and an
+        # off-diagonal c3xy / c3yx = v3 sxy times the mask (zero for the
+        # scalar poles) driving Px from Ey and Py from Ex
+        npx0, npy0, nt = c1x.shape[0], c1y.shape[0], len(poles_tensor)
+        tc1x, tc2x, tc3x, tc3xy = (grid.zeros(nt, grid.nx - 1, grid.ny) for _ in range(4))
+        tc1y, tc2y, tc3y, tc3yx = (grid.zeros(nt, grid.nx, grid.ny - 1) for _ in range(4))
+        for k, pole in enumerate(poles_tensor):
+            fn, gamma, (sxx, sxy, syy) = pole[0], pole[1], pole[2]
+            fc = pole[3] if len(pole) > 3 else fn
+            v1, v2, v3 = pole_recurrence((fn, gamma, 1.0, fc), dt)
+            m = (pole_masks_tensor[k] if pole_masks_tensor is not None else torch.ones_like(eps)).to(eps)
+            mx, my = _avg_to_ex(m), _avg_to_ey(m)
+            tc1x[k], tc2x[k] = v1, v2
+            tc1y[k], tc2y[k] = v1, v2
+            tc3x[k], tc3xy[k] = v3 * sxx * mx, v3 * sxy * mx
+            tc3y[k], tc3yx[k] = v3 * syy * my, v3 * sxy * my
+        c1x, c2x, c3x = torch.cat([c1x, tc1x]), torch.cat([c2x, tc2x]), torch.cat([c3x, tc3x])
+        c1y, c2y, c3y = torch.cat([c1y, tc1y]), torch.cat([c2y, tc2y]), torch.cat([c3y, tc3y])
+        cond["c3xy"] = torch.cat([grid.zeros(npx0, grid.nx - 1, grid.ny), tc3xy])
+        cond["c3yx"] = torch.cat([grid.zeros(npy0, grid.nx, grid.ny - 1), tc3yx])
     if y_mirror or x_mirror:
         _unsupported("symmetry folds (y_mirror/x_mirror)")
     eps_x, eps_y = _avg_to_ex(eps), _avg_to_ey(eps)

@@ -298,12 +298,17 @@ def _te_media(d, c, fields, dtype, device):
         keep.append(t)
         setattr(d, name, t.data_ptr())
 
-    if c.inv_xx_x is not None or c.c3xy is not None:
-        raise NotImplementedError("TE tensor eps and oriented poles are not supported")
     if c.has_kerr or c.has_chi2:
         d.kerr = 1
         for name, t in (("chi2x", c.chi2_x), ("chi3x", c.chi3_x), ("chi2y", c.chi2_y), ("chi3y", c.chi3_y)):
             put(name, t)
+    if c.inv_xx_x is not None:   # tensor eps (coeffs.py:390-400)
+        for name in ("inv_xx_x", "inv_xy_x", "inv_yy_y", "inv_xy_y"):
+            put(name, getattr(c, name))
+    if c.c3xy is not None:       # oriented poles (coeffs.py:425-455)
+        put("c3xy", c.c3xy)
+        put("c3yx", c.c3yx)
+    if d.kerr or c.inv_xx_x is not None or c.c3xy is not None:
         kx, ky = torch.empty_like(fields["Ex"]), torch.empty_like(fields["Ey"])
         keep += [kx, ky]
         d.kx, d.ky = kx.data_ptr(), ky.data_ptr()
@@ -397,7 +402,7 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
         if not t.is_contiguous():
             raise ValueError(f"state field {f} must be contiguous")
     media = bool(d.n_poles or d.kerr or d.d1 or d.h1x or d.n_mpoles) if mode == "tm" else \
-        bool(d.npx or d.npy or d.d1x or d.hz1 or d.kerr)
+        bool(d.npx or d.npy or d.d1x or d.hz1 or d.kerr or d.inv_xx_x or d.c3xy)
     if media or (mode == "te" and plan is not None and plan.get("h")):
         variant = -1  # media and TE magnetic currents are handled by the two-pass kernels
     if variant is None:

@@ -111,6 +111,13 @@ class Coef2D:
     chi3x: torch.Tensor = None
     chi2y: torch.Tensor = None
     chi3y: torch.Tensor = None
+    # TE tensor eps (inverse components on the E grids) and oriented poles
+    inv_xx_x: torch.Tensor = None
+    inv_xy_x: torch.Tensor = None
+    inv_yy_y: torch.Tensor = None
+    inv_xy_y: torch.Tensor = None
+    c3xy: torch.Tensor = None
+    c3yx: torch.Tensor = None
 
 
 def pole_recurrence(pole, dt):
@@ -234,10 +241,14 @@ def _te_axis_poles(nx, ny, dt, dtype, shared, own, shared_masks, own_masks, shap
 
 def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, poles=(), pole_masks=None, poles_x=(),
               poles_y=(), pole_masks_x=None, pole_masks_y=None, sigma_d=None, sigma_b=None, mu=None, chi2=None,
-              chi3=None):
-    """meep_torch/core/coeffs.py:357-503 restricted to eps + CPML, shared and
-    per-axis electric poles and sigma_d / sigma_b / mu; eps_x/eps_y are the
-    two-point averages of coeffs.py:347-355."""
+              chi3=None, eps_tensor=None, poles_tensor=(), pole_masks_tensor=None):
+    """meep_torch/core/coeffs.py:357-503: eps + CPML, shared and per-axis
+    electric poles, sigma_d / sigma_b / mu, chi2 / chi3, the in-plane tensor
+    eps (exx, exy, eyy) -> inverse components averaged onto Ex / Ey
+    (coeffs.py:390-400) and oriented poles (fn, gamma, (sxx, sxy, syy)[, fc])
+    appended to both axis stacks with the off-diagonal c3xy / c3yx
+    (coeffs.py:425-455); eps_x/eps_y are the two-point averages of
+    coeffs.py:347-355."""
     dt = courant * dx
     eps = eps.to(dtype)
     eps_x = 0.5 * (eps[:-1, :] + eps[1:, :])
@@ -265,6 +276,29 @@ def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, poles=(),
                                          (nx - 1, ny), ax)
     c.c1y, c.c2y, c.c3y = _te_axis_poles(nx, ny, dt, dtype, poles, poles_y, pole_masks, pole_masks_y,
                                          (nx, ny - 1), ay)
+    if eps_tensor is not None:
+        fl = lambda v, d: (torch.full((nx, ny), float(d if v is None else v), dtype=dtype)
+                           if not torch.is_tensor(v) else v.to(dtype))
+        exx, exy, eyy = fl(eps_tensor[0], 1.0), fl(eps_tensor[1], 0.0), fl(eps_tensor[2], 1.0)
+        det = exx * eyy - exy * exy
+        c.inv_xx_x, c.inv_xy_x = ax(eyy / det), ax(-exy / det)
+        c.inv_yy_y, c.inv_xy_y = ay(exx / det), ay(-exy / det)
+    if len(poles_tensor):
+        nt, npx0, npy0 = len(poles_tensor), c.c1x.shape[0], c.c1y.shape[0]
+        t1x, t2x, t3x, t3xy = (torch.zeros(nt, nx - 1, ny, dtype=dtype) for _ in range(4))
+        t1y, t2y, t3y, t3yx = (torch.zeros(nt, nx, ny - 1, dtype=dtype) for _ in range(4))
+        for k, pole in enumerate(poles_tensor):
+            fn, gamma, (sxx, sxy, syy) = pole[0], pole[1], pole[2]
+            v1, v2, v3 = pole_recurrence((fn, gamma, 1.0, pole[3] if len(pole) > 3 else fn), dt)
+            m = (pole_masks_tensor[k] if pole_masks_tensor is not None else torch.ones(nx, ny)).to(dtype)
+            mx, my = ax(m), ay(m)
+            t1x[k], t2x[k], t1y[k], t2y[k] = v1, v2, v1, v2
+            t3x[k], t3xy[k] = v3 * sxx * mx, v3 * sxy * mx
+            t3y[k], t3yx[k] = v3 * syy * my, v3 * sxy * my
+        c.c1x, c.c2x, c.c3x = torch.cat([c.c1x, t1x]), torch.cat([c.c2x, t2x]), torch.cat([c.c3x, t3x])
+        c.c1y, c.c2y, c.c3y = torch.cat([c.c1y, t1y]), torch.cat([c.c2y, t2y]), torch.cat([c.c3y, t3y])
+        c.c3xy = torch.cat([torch.zeros(npx0, nx - 1, ny, dtype=dtype), t3xy])
+        c.c3yx = torch.cat([torch.zeros(npy0, nx, ny - 1, dtype=dtype), t3yx])
     full = lambda v: (torch.full((nx, ny), float(v or 0.0), dtype=dtype) if not torch.is_tensor(v) else v.to(dtype))
     x2, x3 = full(chi2), full(chi3)
     c.kerr = bool((x2 != 0).any()) or bool((x3 != 0).any())   # coeffs.py:380-384
@@ -420,16 +454,29 @@ def te_step(s, c, coef=0.0, mask=None, hcoef=0.0, hz_mask=None):
         Dy = Dy - (c.d2y * coef * mask if c.d1x is not None else coef * mask)
     out = dict(s)
     rx, ry = Dx, Dy
+    # four-point averages of the other component, zeros beyond the walls (stepper_te.py:52-61)
+    a4x = lambda v: (lambda q: 0.25 * (q[:-1, :-1] + q[:-1, 1:] + q[1:, :-1] + q[1:, 1:]))(
+        torch.nn.functional.pad(v, (1, 1)))
+    a4y = lambda v: (lambda q: 0.25 * (q[:-1, :-1] + q[:-1, 1:] + q[1:, :-1] + q[1:, 1:]))(
+        torch.nn.functional.pad(v, (0, 0, 1, 1)))
     if c.c1x is not None and c.c1x.shape[0]:   # stepper_te.py:137-147
         Pxn = c.c1x * s["Px_cur"] + c.c2x * s["Px_prev"] + c.c3x * s["Ex"]
+        if c.c3xy is not None:                  # oriented poles: Px driven by E(n-1/2) of the other axis
+            Pxn = Pxn + c.c3xy * a4x(s["Ey"])
         out.update(Px_cur=Pxn, Px_prev=s["Px_cur"])
         rx = Dx - Pxn.sum(0)
     if c.c1y is not None and c.c1y.shape[0]:
         Pyn = c.c1y * s["Py_cur"] + c.c2y * s["Py_prev"] + c.c3y * s["Ey"]
+        if c.c3yx is not None:
+            Pyn = Pyn + c.c3yx * a4y(s["Ex"])
         out.update(Py_cur=Pyn, Py_prev=s["Py_cur"])
         ry = Dy - Pyn.sum(0)
-    Ex = rx / c.eps_x
-    Ey = ry / c.eps_y
+    if c.inv_xx_x is not None:                  # stepper_te.py:158-160: E = eps^-1 (D - P)
+        Ex = c.inv_xx_x * rx + c.inv_xy_x * a4x(ry)
+        Ey = c.inv_yy_y * ry + c.inv_xy_y * a4y(rx)
+    else:
+        Ex = rx / c.eps_x
+        Ey = ry / c.eps_y
     if c.kerr:
         # stepper_te.py:161-175: Jacobi passes, |E|^2 with the four-point
         # average of the other component, zero-padded beyond the walls

@@ -37,6 +37,10 @@ Cases (all CPU, torch threads = 8):
              which the B phase injects into B
   media_te   2D TE analogue: shared and per-axis poles, chi2 / chi3, sigma_d,
              sigma_b, mu; (nl) chi2 + chi3 only
+  tensor_te  2D TE 48x40, npml 6, 240 steps, f32 + f64, anisotropic media: (ten)
+             an in-plane tensor eps (exx, exy, eyy) with an oriented pole
+             (fn, gamma, (sxx, sxy, syy)) and a shared scalar pole; (tenk)
+             the tensor eps under chi3; line source, flux line
   media_3d   3D 22x20x18, npml 4, 120 steps, sigma_d, sigma_b and a Lorentz +
              Drude pole pair with masks, point source, flux plane, f32 + f64
   absorber_3d  3D 22x20x18, 120 steps, f32 + f64, boundary modes without a
@@ -578,6 +582,45 @@ def case_media_te():
         save(f"media_te_{tag}", **out)
 
 
+def case_tensor_te():
+    from meep_torch.core.coeffs import build_te_coeffs
+    from meep_torch.core.grid import Grid
+    from meep_torch.core.monitors import FluxMonitorTE
+    from meep_torch.core.run import run_te
+    from meep_torch.core.sources import GaussianPulse, LineSource
+    from meep_torch.core.stepper_te import init_te_state
+    nx, ny, n = 48, 40, 240
+    mp_ = _media_maps(nx, ny)
+    x = torch.arange(nx, dtype=torch.float64)[:, None]
+    y = torch.arange(ny, dtype=torch.float64)[None, :]
+    blob = torch.exp(-((x - 26.0) ** 2 + (y - 20.0) ** 2) / 60.0)
+    ten = (mp_["eps"] + 1.5 * blob, 0.8 * blob, mp_["eps"] + 0.5 * blob)
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        g = Grid(nx=nx, ny=ny, dx=0.05, courant=0.5, dtype=dtype, device="cpu")
+        cv = {k: v.to(dtype) for k, v in mp_.items()}
+        tv = tuple(t.to(dtype) for t in ten)
+        out = {k: v for k, v in cv.items()}
+        out["exx"], out["exy"], out["eyy"] = tv
+        for case, kw, drive in (
+                ("ten", dict(eps_tensor=tv, poles=POLES2[:1], pole_masks=[cv["m2"]],
+                             poles_tensor=[(1.3, 0.2, (0.9, 0.4, 0.5))], pole_masks_tensor=[cv["m1"]]), 20.0),
+                ("tenk", dict(eps_tensor=tv, chi3=cv["chi3"]), 60.0)):
+            m = build_te_coeffs(g, eps_inf=cv["eps"], npml=6, **kw)
+            src = LineSource(g, ix=10, j0=8, j1=ny - 8, profile=GaussianPulse(0.8, 0.6, cutoff=3.0))
+            src.mask = (src.mask[:, :-1] * drive).contiguous()
+            mon = FluxMonitorTE(g, [0.6, 0.8, 1.0], ix=38, lo=7, hi=ny - 7)
+            st0 = init_te_state(g, n_poles=int(m.c1x.shape[0]), n_poles_y=int(m.c1y.shape[0]))
+            st, (acc,) = run_te(st0, m, n, source=src, monitors=[mon])
+            out[f"{case}_mask"] = src.mask
+            out[f"{case}_amp"] = np.array([src.amplitude(t) for t in range(n)])
+            for f in st._fields:
+                v = getattr(st, f)
+                if isinstance(v, torch.Tensor) and v.numel():
+                    out[f"{case}_{f}"] = v
+            out[f"{case}_acc"] = acc
+        save(f"tensor_te_{tag}", **out)
+
+
 def case_media_3d():
     from meep_torch.core.monitors_3d import FluxMonitor3D, PointSource3D, run_3d
     from meep_torch.core.sources import GaussianPulse
@@ -824,7 +867,7 @@ def main():
     which = sys.argv[1:] or ["pml", "tm", "te", "c1", "3d", "grad_tm", "grad_3d",
                              "filters", "golden_plain", "media_tm", "media_te", "media_3d",
                              "absorber_3d", "tfsf_tm", "sim3d", "raster", "shapes", "multisrc", "sim_absorber",
-                             "mag_tm"]
+                             "mag_tm", "tensor_te"]
     t_all = time.perf_counter()
     timing = {}
     try:
@@ -842,7 +885,7 @@ def main():
          "absorber_3d": case_absorber_3d,
          "tfsf_tm": case_tfsf_tm, "sim3d": case_sim3d, "raster": case_raster,
          "shapes": case_shapes, "multisrc": case_multisrc, "sim_absorber": case_sim_absorber,
-         "mag_tm": case_mag_tm}[w]()
+         "mag_tm": case_mag_tm, "tensor_te": case_tensor_te}[w]()
         print(f"[golden] {w}: {time.perf_counter() - t0:.1f}s", flush=True)
     man = {}
     for fn in sorted(os.listdir(HERE)):

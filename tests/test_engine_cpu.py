@@ -95,19 +95,40 @@ def test_mask_box():
     assert mask_box(torch.zeros(4, 4)) is None
 
 
-@pytest.mark.parametrize("te,kw,flag", [(True, {"poles_tensor": [(1.0, 0.1, (1.0, 0.2, 1.0))]}, "poles_tensor"),
-                                        (True, {"eps_tensor": (2.0, 0.1, 2.0)}, "eps_tensor"),
+@pytest.mark.parametrize("te,kw,flag", [(True, {"x_mirror": 1}, "symmetry"),
                                         (False, {"y_mirror": 1}, "symmetry"),
                                         (False, {"m_poles": [(1.0, 0.1, 1.0)], "mu": 2.0}, "m_poles")])
 def test_unsupported_flags_raise(te, kw, flag):
-    """TE oriented poles and tensor eps, symmetry folds, and TM magnetic
-    poles combined with sigma_b / mu (which the reference refuses too,
-    coeffs.py:178-179) raise by name."""
+    """Symmetry folds, and TM magnetic poles combined with sigma_b / mu
+    (which the reference refuses too, coeffs.py:178-179) raise by name."""
     from meep_torch_amd.core.coeffs import build_te_coeffs, build_tm_coeffs
     from meep_torch_amd.core.grid import Grid
     g = Grid(nx=16, ny=12, dx=0.05, dtype=torch.float32, device="cpu")
     with pytest.raises(NotImplementedError, match=flag):
         (build_te_coeffs if te else build_tm_coeffs)(g, npml=2, **kw)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_te_tensor_coefficients_equal_the_oracle(dtype):
+    """build_te_coeffs's tensor-eps inverse components and oriented-pole
+    stacks (c1..c3 with the appended poles, c3xy / c3yx) are the oracle's
+    bit for bit."""
+    from meep_torch_amd.core.coeffs import build_te_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from oracle import yee2d
+    gm = torch.Generator().manual_seed(9)
+    r = lambda: torch.rand(16, 12, generator=gm, dtype=torch.float64).to(dtype)
+    ten = (2.0 + r(), 0.3 * r(), 1.5 + r())
+    kw = dict(eps_tensor=ten, poles=[(1.1, 0.15, 1.3)], pole_masks=[r()],
+              poles_tensor=[(1.3, 0.2, (0.9, 0.4, 0.5)), (0.8, 0.1, (0.2, -0.3, 0.6), 0.7)],
+              pole_masks_tensor=[r(), r()])
+    g = Grid(nx=16, ny=12, dx=0.05, dtype=dtype, device="cpu")
+    c = build_te_coeffs(g, npml=2, **kw)
+    oc = yee2d.te_coeffs(16, 12, 0.05, torch.ones(16, 12), 2, dtype=dtype, **kw)
+    for name in ("inv_xx_x", "inv_xy_x", "inv_yy_y", "inv_xy_y", "c1x", "c2x", "c3x", "c1y", "c2y", "c3y",
+                 "c3xy", "c3yx"):
+        assert torch.equal(getattr(c, name), getattr(oc, name)), name
+    assert tuple(c.c3xy.shape) == (3, 15, 12) and not c.c3xy[0].any()
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])

@@ -11,6 +11,7 @@ in FP32 and FP64.
   more than rounding;
 * TM magnetic Lorentz poles, with electric poles and chi3 and under a TFSF
   magnetic current (mag_tm);
+* TE tensor eps with an oriented pole, and under chi3 (tensor_te);
 * 3D 22 x 20 x 18 with sigma_d, sigma_b and two poles under a point source,
   and the boundary modes without a full CPML (absorber_3d).
 
@@ -88,6 +89,34 @@ def test_te_media_bitexact(tag, dtype, case):
         names += ["Px_cur", "Px_prev", "Py_cur", "Py_prev"]
     for f in names:
         _eq(s[f], g[f"{case}_{f}"], f)
+    _eq(acc[0], g[f"{case}_acc"], "acc")
+
+
+@pytest.mark.parametrize("case", ["ten", "tenk"])
+@pytest.mark.parametrize("tag,dtype", [("f32", torch.float32), ("f64", torch.float64)])
+def test_te_tensor_eps_and_oriented_poles_bitexact(tag, dtype, case):
+    """TE anisotropy (coeffs.py:390-400,425-455, stepper_te.py:137-160):
+    E = eps^-1 (D - sum P) with the off-diagonal inverse applied to the
+    four-point average of the other component's rhs, an oriented pole whose
+    P is also driven by the other component's E(n-1/2), and the tensor eps
+    under the chi3 passes (which start from the tensor E), reproduced bit
+    for bit from the reference's runs.  A transposed inverse, a dropped
+    cross term or the cross drive from E(n+1/2) fails here."""
+    g = golden(f"tensor_te_{tag}")
+    ten = (_t(g["exx"]), _t(g["exy"]), _t(g["eyy"]))
+    if case == "ten":
+        kw = dict(eps_tensor=ten, poles=POLES2[:1], pole_masks=[_t(g["m2"])],
+                  poles_tensor=[(1.3, 0.2, (0.9, 0.4, 0.5))], pole_masks_tensor=[_t(g["m1"])])
+    else:
+        kw = dict(eps_tensor=ten, chi3=_t(g["chi3"]))
+    c = yee2d.te_coeffs(NX, NY, 0.05, _t(g["eps"]), 6, dtype=dtype, **kw)
+    npx = 2 if case == "ten" else 0
+    mons = [{"type": "flux", "kind": "te_x", "idx": 38, "lo": 7, "hi": NY - 7, "freqs": FREQS}]
+    s, acc = yee2d.run2d("te", yee2d.te_state(NX, NY, dtype, npx=npx, npy=npx), c, N, amps=g[f"{case}_amp"],
+                         mask=_t(g[f"{case}_mask"]), monitors=mons)
+    names = ["Hz", "Ex", "Ey", "Dx", "Dy", "psi_Eyx", "psi_Hzx"] + (["Px_cur", "Py_prev"] if npx else [])
+    for f in names:
+        _eq(s[f], g[f"{case}_{f}"], f"{case} {f}")
     _eq(acc[0], g[f"{case}_acc"], "acc")
 
 
