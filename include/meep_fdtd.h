@@ -110,6 +110,17 @@ typedef struct {
   int n_mpoles, pad4_;
   const void *mc1x, *mc2x, *mc3x, *mc1y, *mc2y, *mc3y;
   void *Bx, *By, *Mx_cur, *Mx_prev, *My_cur, *My_prev;
+  /* symmetry cuts at the low edges (coeffs.py:211-281, stepper_tm.py:
+   * 94-155), two-pass path only, not with y_pmc: y_mirror / x_mirror parity
+   * +-1, y_rot2 (Rotate2 phase at the low-y cut) and rot4 (Rotate4 phase at
+   * both cuts, square grids) +-1, 0 = off.  A cut makes the Dz row / column
+   * on the cut live: profile ey (ny-1) / ex (nx-1), psi_Hxy / psi_Hyx one
+   * node longer on that axis, and the H just beyond the cut is 0 (even
+   * mirror) or the on-plane row Hx0 (nx) under an odd y mirror, Rotate2 or
+   * Rotate4, Hy0 (ny) under an odd x mirror or Rotate4, each advanced in
+   * the B phase from a ghost Ez: H0 -/+= (dt (Ez_cut - ghost)) / dx. */
+  int y_mirror, x_mirror, y_rot2, rot4;
+  void *Hx0, *Hy0;
 } mt_tm2d;
 
 /* 2D TE problem + state (coeffs.py:294-345 TECoeffs, stepper_te.py:11-24). */
@@ -164,6 +175,12 @@ typedef struct {
    * beyond the walls.  Either needs kx, ky. */
   const void *inv_xx_x, *inv_xy_x, *inv_yy_y, *inv_xy_y;
   const void *c3xy, *c3yx;
+  /* mirror cuts at the low edges (coeffs.py:457-469, stepper_te.py:80-97),
+   * two-pass path only: parity +-1, 0 = off; an even cut makes the on-plane
+   * tangential E row live with curl (2 Hz[0]) / dx (Dx row 0 under y_mirror
+   * = +1, Dy column 0 under x_mirror = +1); the profiles carry no low-side
+   * shell. */
+  int y_mirror, x_mirror;
 } mt_te2d;
 
 

@@ -345,6 +345,8 @@ extern "C" int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const voi
   if (p->n_poles < 0 || (p->n_poles && (!a->aP_cur || !a->aP_prev || !p->c1 || !p->c2 || !p->c3)))
     return fail(MT_EINVAL, "adjoint with poles needs c1, c2, c3 and the aP_cur / aP_prev cotangents");
   if (p->n_mpoles) return fail(MT_EINVAL, "the TM adjoint does not transpose magnetic poles (n_mpoles = 0)");
+  if (p->y_mirror || p->x_mirror || p->y_rot2 || p->rot4)
+    return fail(MT_EINVAL, "the TM adjoint does not transpose symmetry cuts");
   if (!p->d1 != !p->d2 || !p->h1x != !p->h2x || !p->h1y != !p->h2y || !p->d1 != !p->h1x || !p->d1 != !p->h1y)
     return fail(MT_EINVAL, "conductivity needs all of d1, d2, h1x, h2x, h1y, h2y");
   if (!p->eps || !a->aEz || !a->aDz || !a->aHx || !a->aHy || !a->apsi_Ezy || !a->apsi_Ezx ||
@@ -375,6 +377,7 @@ extern "C" int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const voi
   if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward E tape");
   if (p->kerr) return fail(MT_EINVAL, "adjoint through nonlinear (chi2 / chi3) media is not supported");
   if (p->inv_xx_x || p->c3xy) return fail(MT_EINVAL, "the TE adjoint does not transpose tensor eps or oriented poles");
+  if (p->y_mirror || p->x_mirror) return fail(MT_EINVAL, "the TE adjoint does not transpose symmetry cuts");
   if (p->npx < 0 || p->npy < 0 ||
       (p->npx && (!a->aPx_cur || !a->aPx_prev || !p->c1x || !p->c2x || !p->c3x)) ||
       (p->npy && (!a->aPy_cur || !a->aPy_prev || !p->c1y || !p->c2y || !p->c3y)))

@@ -529,8 +529,8 @@ extern "C" mt_batch* mt_tm2d_batch_create(const mt_tm2d_batch_item* items, int n
       return nullptr;
     }
     if (check_src(&d.hsrc[0], d.nx, d.ny - 1) || check_src(&d.hsrc[1], d.nx - 1, d.ny)) return nullptr;
-    if (d.n_poles || d.kerr || d.d1 || d.h1x || d.n_mpoles) {
-      fail(MT_EINVAL, "batched run: dispersive, nonlinear, magnetic or conductive media are not supported");
+    if (d.n_poles || d.kerr || d.d1 || d.h1x || d.n_mpoles || d.y_mirror || d.x_mirror || d.y_rot2 || d.rot4) {
+      fail(MT_EINVAL, "batched run: media and symmetry cuts are not supported");
       return nullptr;
     }
     if (check_axis(d.hy, d.ny - 1, "hy") || check_axis(d.hx, d.nx - 1, "hx") || check_axis(d.ex, d.nx - 2, "ex") ||

@@ -40,6 +40,10 @@ struct TmK {
   int n_mpoles;
   const T *mc1x, *mc2x, *mc3x, *mc1y, *mc2y, *mc3y;
   T *Bx, *By, *Mxc, *Mxp, *Myc, *Myp;
+  // symmetry cuts at the low edges: ycut / xcut 0 none, 1 zero ghost H (even
+  // mirror), 2 on-plane row Hx0 / Hy0; parities of the ghost Ez
+  int ycut, xcut, ymir, y_rot2, rot4;
+  T *Hx0, *Hy0;
 };
 
 // H = B - sum M' with M' = (mc1 M + mc2 M_prev) + mc3 H_old per pole, in pole
@@ -88,6 +92,18 @@ __global__ __launch_bounds__(BX* BY) void tm_b_kernel(TmK<T> p, mt_source2d hxs,
   if (i >= p.nx || j >= p.ny) return;
   const int ny = p.ny;
   T ez = p.Ez[(long long)i * ny + j];
+  // on-plane H rows of the odd cuts (stepper_tm.py:94-115): the ghost Ez
+  // beyond the cut is -Ez_cut (odd mirror), the C2 partner phase *
+  // Ez[nx-1-i, 0] (Rotate2) or the other cut's line phase * Ez[0, i]
+  // (Rotate4); H0 -/+= (dt (Ez_cut - ghost)) / dx from Ez at time n
+  if (j == 0 && p.ycut == 2) {
+    const T g = p.rot4 ? T(p.rot4) * p.Ez[i] : p.y_rot2 ? T(p.y_rot2) * p.Ez[(long long)(p.nx - 1 - i) * ny] : -ez;
+    p.Hx0[i] = p.Hx0[i] - (p.dt * (ez - g)) / p.dx;
+  }
+  if (i == 0 && p.xcut == 2) {
+    const T g = p.rot4 ? T(p.rot4) * p.Ez[(long long)j * ny] : -ez;
+    p.Hy0[j] = p.Hy0[j] + (p.dt * (ez - g)) / p.dx;
+  }
   if (j < ny - 1) {
     long long h = (long long)i * (ny - 1) + j;
     T d = (p.Ez[(long long)i * ny + j + 1] - ez) / p.dx;
@@ -140,15 +156,19 @@ __global__ __launch_bounds__(BX* BY) void tm_d_kernel(TmK<T> p, mt_source2d src,
   const int nx = p.nx, ny = p.ny;
   long long c = (long long)i * ny + j;
   T dz = p.Dz[c];
-  bool live = i >= 1 && i < nx - 1 && (p.y_pmc || (j >= 1 && j < ny - 1));
+  // live Dz: the interior, all rows under y_pmc, and from the cut row /
+  // column on under a symmetry cut (stepper_tm.py:134-155)
+  bool live = (p.xcut ? i < nx - 1 : (i >= 1 && i < nx - 1)) &&
+              (p.y_pmc || (p.ycut ? j < ny - 1 : (j >= 1 && j < ny - 1)));
   T curl = T(0);  // zero on the PEC rows, as the reference's padding
   if (live) {
-    // H = 0 beyond a PMC wall (stepper_tm.py:135-137)
+    // H beyond the wall: 0 (PMC, even mirror) or the on-plane row (odd / Rotate cuts)
     T hx1 = (j < ny - 1) ? p.Hx[(long long)i * (ny - 1) + j] : T(0);
-    T hx0 = (j >= 1) ? p.Hx[(long long)i * (ny - 1) + j - 1] : T(0);
+    T hx0 = (j >= 1) ? p.Hx[(long long)i * (ny - 1) + j - 1] : (p.ycut == 2 ? p.Hx0[i] : T(0));
+    T hy0 = (i >= 1) ? p.Hy[(long long)(i - 1) * ny + j] : (p.xcut == 2 ? p.Hy0[j] : T(0));
     T dHx = (hx1 - hx0) / p.dx;
-    T dHy = (p.Hy[(long long)i * ny + j] - p.Hy[(long long)(i - 1) * ny + j]) / p.dx;
-    int ii = i - 1, jj = p.y_pmc ? j : j - 1;
+    T dHy = (p.Hy[(long long)i * ny + j] - hy0) / p.dx;
+    int ii = p.xcut ? i : i - 1, jj = (p.y_pmc || p.ycut) ? j : j - 1;
     long long q = (long long)ii * p.ny_e + jj;
     T a1 = dHy;
     if (p.ex.in(ii)) {
@@ -216,6 +236,7 @@ struct TeK {
   // optional tensor eps (inverse components on the E grids) and oriented poles
   const T *ixx, *ixyx, *iyy, *ixyy, *c3xy, *c3yx;
   int scratch;  // the D half writes E (or, with tensor eps, rhs) to kx / ky instead of Ex / Ey
+  int ymir, xmir;  // mirror parities of the low-edge cuts
 };
 
 // Four-point averages with zeros beyond the walls (stepper_te.py:52-61): of
@@ -342,6 +363,9 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
         curl = d + ps;
       }
       if (!p.d1x) dxv = dxv + p.dt * curl;
+    } else if (j == 0 && p.ymir == 1) {  // even y mirror: on-plane Ex live, odd Hz ghost (stepper_te.py:88-93)
+      curl = (T(2) * p.Hz[(long long)i * (ny - 1)]) / p.dx;
+      if (!p.d1x) dxv = dxv + p.dt * curl;
     }
     if (p.d1x) dxv = p.d1x[c] * dxv + p.d2x[c] * curl;  // stepper_te.py:96-99 (sigma_d)
     p.Dx[c] = dxv;
@@ -366,6 +390,9 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
         p.pHzx[q] = ps;
         curl = d + ps;
       }
+      if (!p.d1y) dyv = dyv - p.dt * curl;
+    } else if (i == 0 && p.xmir == 1) {  // even x mirror: on-plane Ey live (stepper_te.py:94-96)
+      curl = (T(2) * p.Hz[j]) / p.dx;
       if (!p.d1y) dyv = dyv - p.dt * curl;
     }
     if (p.d1y) dyv = p.d1y[c] * dyv - p.d2y[c] * curl;
@@ -466,6 +493,14 @@ int tm_run_t(const mt_tm2d* d, const mt_source2d* src, const double* coef, const
   p.Mxp = static_cast<T*>(d->Mx_prev);
   p.Myc = static_cast<T*>(d->My_cur);
   p.Myp = static_cast<T*>(d->My_prev);
+  p.ycut = d->y_mirror == 1 ? 1 : (d->y_mirror == -1 || d->y_rot2 || d->rot4) ? 2 : 0;
+  p.xcut = d->x_mirror == 1 ? 1 : (d->x_mirror == -1 || d->rot4) ? 2 : 0;
+  p.ymir = d->y_mirror;
+  p.y_rot2 = d->y_rot2;
+  p.rot4 = d->rot4;
+  p.Hx0 = static_cast<T*>(d->Hx0);
+  p.Hy0 = static_cast<T*>(d->Hy0);
+  p.ny_e = d->y_pmc ? d->ny : (p.ycut ? d->ny - 1 : d->ny - 2);
   mt_source2d sv{};
   if (src && src->present) sv = *src;
   dim3 blk(BX, BY), grd = grid2(p.ny, p.nx, blk);
@@ -546,6 +581,8 @@ int te_run_t(const mt_te2d* d, const mt_source2d* src, const double* coef, const
   p.c3xy = static_cast<const T*>(d->c3xy);
   p.c3yx = static_cast<const T*>(d->c3yx);
   p.scratch = p.kerr || p.ixx || p.c3xy;
+  p.ymir = d->y_mirror;
+  p.xmir = d->x_mirror;
   const size_t bx = (size_t)(p.nx - 1) * p.ny * sizeof(T), by = (size_t)p.nx * (p.ny - 1) * sizeof(T);
   mt_source2d sv{};
   if (src && src->present) sv = *src;
@@ -784,9 +821,21 @@ extern "C" int mt_tm2d_run(const mt_tm2d* p, const mt_source2d* src, const doubl
       !p->psi_Hxy)
     return fail(MT_EINVAL, "null field pointer");
   int rc;
+  const int cuts[4] = {p->y_mirror, p->x_mirror, p->y_rot2, p->rot4};
+  for (int q = 0; q < 4; ++q)
+    if (cuts[q] < -1 || cuts[q] > 1) return fail(MT_EINVAL, "symmetry parity must be 0/+1/-1");
+  if (p->y_mirror && p->y_rot2) return fail(MT_EINVAL, "y_mirror and y_rot2 both cut the low-y edge");
+  if (p->rot4 && (p->y_mirror || p->x_mirror || p->y_rot2)) return fail(MT_EINVAL, "rot4 cuts both low edges on its own");
+  if (p->rot4 && p->nx != p->ny) return fail(MT_EINVAL, "rot4 needs a square grid");
+  const bool ycut = p->y_mirror || p->y_rot2 || p->rot4, xcut = p->x_mirror || p->rot4;
+  if (p->y_pmc && (ycut || xcut)) return fail(MT_EINVAL, "symmetry cuts and y_pmc are mutually exclusive");
+  if ((p->y_mirror == -1 || p->y_rot2 || p->rot4) && !p->Hx0) return fail(MT_EINVAL, "odd / Rotate y cut needs Hx0");
+  if ((p->x_mirror == -1 || p->rot4) && !p->Hy0) return fail(MT_EINVAL, "odd / Rotate4 x cut needs Hy0");
+  if ((p->y_mirror == -1 || p->x_mirror == -1 || p->y_rot2 || p->rot4) && (p->h1x || p->n_mpoles))
+    return fail(MT_EINVAL, "odd mirror / Rotate2 / Rotate4 with sigma_b/mu/magnetic poles");
   if ((rc = check_axis(p->hy, p->ny - 1, "hy")) || (rc = check_axis(p->hx, p->nx - 1, "hx")) ||
-      (rc = check_axis(p->ex, p->nx - 2, "ex")) ||
-      (rc = check_axis(p->ey, p->y_pmc ? p->ny : p->ny - 2, "ey")))
+      (rc = check_axis(p->ex, xcut ? p->nx - 1 : p->nx - 2, "ex")) ||
+      (rc = check_axis(p->ey, p->y_pmc ? p->ny : (ycut ? p->ny - 1 : p->ny - 2), "ey")))
     return rc;
   if ((rc = check_src(src, p->nx, p->ny))) return rc;
   if (src && src->present && !coef && k > 0) return fail(MT_EINVAL, "source without coefficients");
@@ -807,7 +856,7 @@ extern "C" int mt_tm2d_run(const mt_tm2d* p, const mt_source2d* src, const doubl
         !p->Mx_cur || !p->Mx_prev || !p->My_cur || !p->My_prev)
       return fail(MT_EINVAL, "magnetic poles need mc1..mc3 x/y, Bx, By and the M stacks");
   }
-  const bool media = p->n_poles || p->kerr || p->d1 || p->h1x || p->n_mpoles;
+  const bool media = p->n_poles || p->kerr || p->d1 || p->h1x || p->n_mpoles || ycut || xcut;  // two-pass kernels
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -859,7 +908,10 @@ extern "C" int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const doubl
   if ((tensor || p->c3xy) && (!p->kx || !p->ky))
     return fail(MT_EINVAL, "tensor eps and oriented poles need the kx, ky scratch");
   if ((tensor || p->c3xy) && p->tape) return fail(MT_EINVAL, "the E tape is not written with tensor eps or oriented poles");
-  const bool media = p->npx || p->npy || p->d1x || p->hz1 || p->kerr || tensor || p->c3xy;
+  if (p->y_mirror < -1 || p->y_mirror > 1 || p->x_mirror < -1 || p->x_mirror > 1)
+    return fail(MT_EINVAL, "mirror parity must be 0/+1/-1");
+  const bool media = p->npx || p->npy || p->d1x || p->hz1 || p->kerr || tensor || p->c3xy || p->y_mirror ||
+                     p->x_mirror;  // two-pass kernels
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);

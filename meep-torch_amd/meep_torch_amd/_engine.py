@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 23
+ABI_VERSION = 24
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -45,7 +45,9 @@ class TM2D(ctypes.Structure):
                 ("n_mpoles", c_int), ("pad4_", c_int), ("mc1x", c_void_p), ("mc2x", c_void_p),
                 ("mc3x", c_void_p), ("mc1y", c_void_p), ("mc2y", c_void_p), ("mc3y", c_void_p),
                 ("Bx", c_void_p), ("By", c_void_p), ("Mx_cur", c_void_p), ("Mx_prev", c_void_p),
-                ("My_cur", c_void_p), ("My_prev", c_void_p)]
+                ("My_cur", c_void_p), ("My_prev", c_void_p),
+                ("y_mirror", c_int), ("x_mirror", c_int), ("y_rot2", c_int), ("rot4", c_int),
+                ("Hx0", c_void_p), ("Hy0", c_void_p)]
 
 
 class TM2DAdj(ctypes.Structure):
@@ -72,7 +74,7 @@ class TE2D(ctypes.Structure):
                 ("kerr", c_int), ("pad3_", c_int), ("chi2x", c_void_p), ("chi3x", c_void_p), ("chi2y", c_void_p),
                 ("chi3y", c_void_p), ("kx", c_void_p), ("ky", c_void_p),
                 ("inv_xx_x", c_void_p), ("inv_xy_x", c_void_p), ("inv_yy_y", c_void_p), ("inv_xy_y", c_void_p),
-                ("c3xy", c_void_p), ("c3yx", c_void_p)]
+                ("c3xy", c_void_p), ("c3yx", c_void_p), ("y_mirror", c_int), ("x_mirror", c_int)]
 
 
 class TE2DAdj(ctypes.Structure):

@@ -104,6 +104,8 @@ class BatchedTM:
         if (coeffs.c1.shape[0] or coeffs.has_kerr or coeffs.has_chi2 or coeffs.d1 is not None
                 or coeffs.h1x is not None or coeffs.mc1x is not None):
             raise NotImplementedError("batched run: dispersive, nonlinear or conductive media are not supported")
+        if coeffs.y_mirror_lo or coeffs.x_mirror_lo or coeffs.y_rot2_lo or coeffs.rot4_lo:
+            raise NotImplementedError("batched run: symmetry cuts are not supported")
         nx, ny = coeffs.eps_inf.shape
         z = lambda *s: torch.zeros(*s, dtype=self.dtype, device=self.device)
         ny_e = ny if coeffs.y_pmc else ny - 2

@@ -300,6 +300,9 @@ def _te_descriptor(c, nx, ny, dtype, device):
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
     if mode == "te" and (coeffs.has_kerr or coeffs.has_chi2):
         raise NotImplementedError("gradients through nonlinear (chi2 / chi3) media are not supported")
+    if coeffs.y_mirror_lo or coeffs.x_mirror_lo or getattr(coeffs, "y_rot2_lo", 0) or getattr(coeffs, "rot4_lo", 0):
+        raise NotImplementedError("gradients through symmetry-folded runs (mirrors, Rotate2, Rotate4) "
+                                  "are not supported")
     if mode == "te" and (coeffs.inv_xx_x is not None or coeffs.c3xy is not None):
         raise NotImplementedError("gradients through TE tensor eps or oriented poles are not supported")
     if mode == "tm" and coeffs.mc1x is not None:

@@ -6,9 +6,10 @@ them keeps working; one extra trailing field, ``shells``, carries the CPML
 shell bounds the kernels use to skip the psi recursion in the interior.
 
 Supported by the HIP stepper: eps_inf, npml (int or ((xlo, xhi), (ylo, yhi))),
-y_pmc for TM, and for TM the media terms: electric ADE poles, chi2/chi3 and
-sigma_d/sigma_b/mu.  TE media, magnetic poles, tensor eps and symmetry folds
-raise NotImplementedError naming the flag (no silent fallback)."""
+y_pmc for TM, the media terms (electric ADE poles, chi2/chi3, sigma_d/sigma_b/
+mu; TM magnetic poles; TE per-axis poles, tensor eps and oriented poles) and
+the symmetry cuts (TM mirrors, Rotate2, Rotate4; TE mirrors).  Combinations
+the reference refuses raise as it does."""
 import math
 from typing import NamedTuple, Sequence
 
@@ -156,7 +157,8 @@ def build_tm_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
                     sigma_b=None, mu=None, m_poles: Sequence = (), m_pole_masks=None,
                     y_mirror: int = 0, x_mirror: int = 0, y_rot2: int = 0,
                     rot4: int = 0) -> TMCoeffs:
-    """coeffs.py:134-291: eps + CPML (+ y_pmc), electric ADE poles
+    """coeffs.py:134-291: eps + CPML (+ y_pmc or symmetry cuts at the low
+    edges: y_mirror / x_mirror parity, y_rot2 and rot4 phases), electric ADE poles
     (coeffs.py:87-106, 190-198), chi2 / chi3 (coeffs.py:154-158),
     sigma_d / sigma_b / mu (coeffs.py:160-175) and magnetic Lorentz poles
     m_poles (fn, gamma, sigma) with cell masks m_pole_masks
@@ -209,19 +211,45 @@ def build_tm_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
                 c2m[k] = -(1.0 - G * dt / 2.0) / den
                 c3m[k] = (pole[2] * w0 * w0 * dt * dt / den) * avg(mask.to(device=grid.device, dtype=grid.dtype))
             cond[f"mc1{suf}"], cond[f"mc2{suf}"], cond[f"mc3{suf}"] = c1m, c2m, c3m
-    if y_mirror or x_mirror or y_rot2 or rot4:
-        _unsupported("symmetry folds (y_mirror/x_mirror/y_rot2/rot4)")
+    # symmetry cuts (coeffs.py:211-281): a cut replaces the low-side CPML of
+    # its axis, and the D-update profile keeps nodes 0..n-2 (the cut node is live)
+    ycut, xcut = bool(y_mirror or y_rot2 or rot4), bool(x_mirror or rot4)
     px, py = _axis_sides(npml)
+    if ycut:
+        py = (0, py[1])
+    if xcut:
+        px = (0, px[1])
     if sum(px) >= grid.nx or (sum(py) >= grid.ny and not y_pmc):
         raise ValueError(f"npml={npml} too large for grid {grid.nx}x{grid.ny}")
+    if (y_mirror not in (0, +1, -1) or x_mirror not in (0, +1, -1)
+            or y_rot2 not in (0, +1, -1) or rot4 not in (0, +1, -1)):
+        raise ValueError(f"symmetry parity must be 0/+1/-1, got y={y_mirror} "
+                         f"x={x_mirror} rot2={y_rot2} rot4={rot4}")
+    if y_mirror and y_rot2:
+        raise ValueError("y_mirror and y_rot2 both cut the low-y edge")
+    if rot4 and (y_mirror or x_mirror or y_rot2):
+        raise ValueError("rot4 cuts both low edges on its own")
+    if rot4 and grid.nx != grid.ny:
+        raise ValueError(f"rot4 needs a square grid, got {grid.nx}x{grid.ny}")
+    if y_pmc and (ycut or xcut):
+        raise ValueError("symmetry cuts and y_pmc are mutually exclusive")
+    if (y_mirror == -1 or x_mirror == -1 or y_rot2 or rot4) and ("h1x" in cond or m_poles):
+        raise NotImplementedError(
+            "odd mirror / Rotate2 / Rotate4 with sigma_b/mu/m_poles: the on-plane H row "
+            "has no conductivity/mu coefficients (standard H-grid shapes only)")
 
     b_hy, a_hy, k_hy = _profile(grid, grid.ny - 1, 0.5, (0, 0) if y_pmc else py)
     b_hx, a_hx, k_hx = _profile(grid, grid.nx - 1, 0.5, px)
     b_exf, a_exf, k_exf = _profile(grid, grid.nx, 0.0, px)
-    b_ex, a_ex, k_ex = b_exf[1:-1], a_exf[1:-1], k_exf[1:-1]
+    if xcut:
+        b_ex, a_ex, k_ex = b_exf[:-1], a_exf[:-1], k_exf[:-1]
+    else:
+        b_ex, a_ex, k_ex = b_exf[1:-1], a_exf[1:-1], k_exf[1:-1]
     b_eyf, a_eyf, k_eyf = _profile(grid, grid.ny, 0.0, (0, 0) if y_pmc else py)
     if y_pmc:
         b_ey, a_ey, k_ey = b_eyf, a_eyf, k_eyf
+    elif ycut:
+        b_ey, a_ey, k_ey = b_eyf[:-1], a_eyf[:-1], k_eyf[:-1]
     else:
         b_ey, a_ey, k_ey = b_eyf[1:-1], a_eyf[1:-1], k_eyf[1:-1]
     shells = {"hy": shell_bounds(b_hy, a_hy), "hx": shell_bounds(b_hx, a_hx),
@@ -229,7 +257,8 @@ def build_tm_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
     dev = lambda t: t.contiguous().to(grid.device)
     return TMCoeffs(
         dt=dt, dx=dx, eps_inf=eps, chi3=x3, has_kerr=has_kerr, chi2=x2, has_chi2=has_chi2,
-        c1=c1, c2=c2, c3=c3, y_pmc=y_pmc, **cond,
+        c1=c1, c2=c2, c3=c3, y_pmc=y_pmc, y_mirror_lo=y_mirror, x_mirror_lo=x_mirror, y_rot2_lo=y_rot2,
+        rot4_lo=rot4, **cond,
         b_hy=dev(b_hy)[None, :], a_hy=dev(a_hy)[None, :], k_hy=dev(k_hy)[None, :],
         b_hx=dev(b_hx)[:, None], a_hx=dev(a_hx)[:, None], k_hx=dev(k_hx)[:, None],
         b_ex=dev(b_ex)[:, None], a_ex=dev(a_ex)[:, None], k_ex=dev(k_ex)[:, None],
@@ -280,7 +309,8 @@ def build_te_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
     at cell centres gives the inverse components Yee-averaged onto the E
     positions (coeffs.py:390-400); poles_tensor (fn, gamma, (sxx, sxy, syy)
     [, fc]) append one pole to each axis stack, with the off-diagonal drive
-    c3xy / c3yx (coeffs.py:425-455).  Symmetry folds raise."""
+    c3xy / c3yx (coeffs.py:425-455).  Mirror cuts (y_mirror / x_mirror +-1)
+    drop the low-side CPML of their axis (coeffs.py:457-469)."""
     dt, dx = grid.dt, grid.dx
     eps = _as_field(eps_inf, grid, 1.0)
     x3 = _as_field(chi3, grid, 0.0)
@@ -339,10 +369,14 @@ def build_te_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
         c1y, c2y, c3y = torch.cat([c1y, tc1y]), torch.cat([c2y, tc2y]), torch.cat([c3y, tc3y])
         cond["c3xy"] = torch.cat([grid.zeros(npx0, grid.nx - 1, grid.ny), tc3xy])
         cond["c3yx"] = torch.cat([grid.zeros(npy0, grid.nx, grid.ny - 1), tc3yx])
-    if y_mirror or x_mirror:
-        _unsupported("symmetry folds (y_mirror/x_mirror)")
+    if y_mirror not in (0, +1, -1) or x_mirror not in (0, +1, -1):
+        raise ValueError(f"mirror parity must be 0/+1/-1, got y={y_mirror} x={x_mirror}")
     eps_x, eps_y = _avg_to_ex(eps), _avg_to_ey(eps)
     px, py = _axis_sides(npml)
+    if y_mirror:    # a mirror wall replaces the low-side CPML (coeffs.py:461-469)
+        py = (0, py[1])
+    if x_mirror:
+        px = (0, px[1])
     if sum(px) >= grid.nx or sum(py) >= grid.ny:
         raise ValueError(f"npml={npml} too large for grid {grid.nx}x{grid.ny}")
     b_hx, a_hx, k_hx = _profile(grid, grid.nx - 1, 0.5, px)
@@ -362,4 +396,5 @@ def build_te_coeffs(grid: Grid, eps_inf=None, poles: Sequence = (), pole_masks=N
         b_hy=dev(b_hy)[None, :], a_hy=dev(a_hy)[None, :], k_hy=dev(k_hy)[None, :],
         b_ey=dev(b_ey)[None, :], a_ey=dev(a_ey)[None, :], k_ey=dev(k_ey)[None, :],
         b_ex=dev(b_ex)[:, None], a_ex=dev(a_ex)[:, None], k_ex=dev(k_ex)[:, None],
+        y_mirror_lo=y_mirror, x_mirror_lo=x_mirror,
         shells=shells)

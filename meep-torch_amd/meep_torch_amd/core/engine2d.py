@@ -267,6 +267,12 @@ def _tm_media(d, c, fields, dtype, device):
             put(name, getattr(c, name))
         for name in ("Bx", "By", "Mx_cur", "Mx_prev", "My_cur", "My_prev"):
             setattr(d, name, fields[name].data_ptr())
+    d.y_mirror, d.x_mirror = int(c.y_mirror_lo), int(c.x_mirror_lo)   # symmetry cuts
+    d.y_rot2, d.rot4 = int(c.y_rot2_lo), int(c.rot4_lo)
+    if "Hx0" in fields:
+        d.Hx0 = fields["Hx0"].data_ptr()
+    if "Hy0" in fields:
+        d.Hy0 = fields["Hy0"].data_ptr()
     n = int(c.c1.shape[0])
     if n:
         if "P_cur" not in fields or int(fields["P_cur"].shape[0]) != n:
@@ -298,6 +304,7 @@ def _te_media(d, c, fields, dtype, device):
         keep.append(t)
         setattr(d, name, t.data_ptr())
 
+    d.y_mirror, d.x_mirror = int(c.y_mirror_lo), int(c.x_mirror_lo)   # mirror cuts
     if c.has_kerr or c.has_chi2:
         d.kerr = 1
         for name, t in (("chi2x", c.chi2_x), ("chi3x", c.chi3_x), ("chi2y", c.chi2_y), ("chi3y", c.chi3_y)):
@@ -401,8 +408,9 @@ def run_chunk(mode, fields, coeffs, step0, k, plan, monitors, tape=None, reuse=N
     for f, t in fields.items():
         if not t.is_contiguous():
             raise ValueError(f"state field {f} must be contiguous")
-    media = bool(d.n_poles or d.kerr or d.d1 or d.h1x or d.n_mpoles) if mode == "tm" else \
-        bool(d.npx or d.npy or d.d1x or d.hz1 or d.kerr or d.inv_xx_x or d.c3xy)
+    media = bool(d.n_poles or d.kerr or d.d1 or d.h1x or d.n_mpoles or d.y_mirror or d.x_mirror or d.y_rot2
+                 or d.rot4) if mode == "tm" else \
+        bool(d.npx or d.npy or d.d1x or d.hz1 or d.kerr or d.inv_xx_x or d.c3xy or d.y_mirror or d.x_mirror)
     if media or (mode == "te" and plan is not None and plan.get("h")):
         variant = -1  # media and TE magnetic currents are handled by the two-pass kernels
     if variant is None:

@@ -64,6 +64,12 @@ class Coef2D:
     eps_x: torch.Tensor = None  # TE: (nx-1, ny)
     eps_y: torch.Tensor = None  # TE: (nx, ny-1)
     y_pmc: bool = False
+    # symmetry cuts at the low edges (coeffs.py:60-74,340-344): mirror parity
+    # +-1, Rotate2 phase at the low-y cut, Rotate4 phase at both
+    y_mirror: int = 0
+    x_mirror: int = 0
+    y_rot2: int = 0
+    rot4: int = 0
     # profiles as broadcastable tensors (same slicing as the reference)
     b_hy: torch.Tensor = None
     a_hy: torch.Tensor = None
@@ -186,16 +192,24 @@ def _tm_mpoles(c, nx, ny, dtype, m_poles, m_pole_masks):
 
 def tm_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, y_pmc=False, poles=(),
               pole_masks=None, chi2=None, chi3=None, sigma_d=None, sigma_b=None, mu=None, m_poles=(),
-              m_pole_masks=None):
+              m_pole_masks=None, y_mirror=0, x_mirror=0, y_rot2=0, rot4=0):
     """meep_torch/core/coeffs.py:134-291: eps + CPML (+ y_pmc), electric ADE
     poles, chi2 / chi3, sigma_d / sigma_b / mu and magnetic Lorentz poles
-    (not combined with sigma_b / mu, as in the reference)."""
+    (not combined with sigma_b / mu, as in the reference).  Symmetry cuts
+    (coeffs.py:211-281): a cut at the low edge of an axis drops that side's
+    CPML, and the D-update profile of that axis keeps nodes 0..n-2 (the cut
+    node is live) instead of the interior 1..n-2."""
     dt = courant * dx
     eps = eps.to(dtype)
     if isinstance(npml, (tuple, list)):
         px, py = _sides(npml[0]), _sides(npml[1])
     else:
         px = py = (int(npml), int(npml))
+    ycut, xcut = bool(y_mirror or y_rot2 or rot4), bool(x_mirror or rot4)
+    if ycut:
+        py = (0, py[1])
+    if xcut:
+        px = (0, px[1])
 
     def prof(n, off, pair):
         return cpml_profiles(n, pair, off, dt, dx, dtype)
@@ -206,12 +220,15 @@ def tm_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, y_pmc=Fal
     b_eyf, a_eyf, _ = prof(ny, 0.0, (0, 0) if y_pmc else py)
     if y_pmc:
         b_ey, a_ey = b_eyf, a_eyf
+    elif ycut:
+        b_ey, a_ey = b_eyf[:-1], a_eyf[:-1]
     else:
         b_ey, a_ey = b_eyf[1:-1], a_eyf[1:-1]
-    c = Coef2D(dt=dt, dx=dx, eps=eps, y_pmc=y_pmc,
-               b_hy=b_hy[None, :], a_hy=a_hy[None, :],
+    b_ex, a_ex = (b_exf[:-1], a_exf[:-1]) if xcut else (b_exf[1:-1], a_exf[1:-1])
+    c = Coef2D(dt=dt, dx=dx, eps=eps, y_pmc=y_pmc, y_mirror=y_mirror, x_mirror=x_mirror, y_rot2=y_rot2,
+               rot4=rot4, b_hy=b_hy[None, :], a_hy=a_hy[None, :],
                b_hx=b_hx[:, None], a_hx=a_hx[:, None],
-               b_ex=b_exf[1:-1][:, None], a_ex=a_exf[1:-1][:, None],
+               b_ex=b_ex[:, None], a_ex=a_ex[:, None],
                b_ey=b_ey[None, :], a_ey=a_ey[None, :])
     c = _tm_media(c, nx, ny, dtype, poles, pole_masks, chi2, chi3, sigma_d, sigma_b, mu)
     if len(m_poles):
@@ -241,14 +258,15 @@ def _te_axis_poles(nx, ny, dt, dtype, shared, own, shared_masks, own_masks, shap
 
 def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, poles=(), pole_masks=None, poles_x=(),
               poles_y=(), pole_masks_x=None, pole_masks_y=None, sigma_d=None, sigma_b=None, mu=None, chi2=None,
-              chi3=None, eps_tensor=None, poles_tensor=(), pole_masks_tensor=None):
+              chi3=None, eps_tensor=None, poles_tensor=(), pole_masks_tensor=None, y_mirror=0, x_mirror=0):
     """meep_torch/core/coeffs.py:357-503: eps + CPML, shared and per-axis
     electric poles, sigma_d / sigma_b / mu, chi2 / chi3, the in-plane tensor
     eps (exx, exy, eyy) -> inverse components averaged onto Ex / Ey
     (coeffs.py:390-400) and oriented poles (fn, gamma, (sxx, sxy, syy)[, fc])
     appended to both axis stacks with the off-diagonal c3xy / c3yx
     (coeffs.py:425-455); eps_x/eps_y are the two-point averages of
-    coeffs.py:347-355."""
+    coeffs.py:347-355.  A mirror cut at the low edge of an axis drops that
+    side's CPML (coeffs.py:457-469)."""
     dt = courant * dx
     eps = eps.to(dtype)
     eps_x = 0.5 * (eps[:-1, :] + eps[1:, :])
@@ -257,6 +275,10 @@ def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, poles=(),
         px, py = _sides(npml[0]), _sides(npml[1])
     else:
         px = py = (int(npml), int(npml))
+    if y_mirror:
+        py = (0, py[1])
+    if x_mirror:
+        px = (0, px[1])
 
     def prof(n, off, pair):
         return cpml_profiles(n, pair, off, dt, dx, dtype)
@@ -265,7 +287,7 @@ def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, poles=(),
     b_hy, a_hy, _ = prof(ny - 1, 0.5, py)
     b_eyf, a_eyf, _ = prof(ny, 0.0, py)
     b_exf, a_exf, _ = prof(nx, 0.0, px)
-    c = Coef2D(dt=dt, dx=dx, eps=eps, eps_x=eps_x, eps_y=eps_y,
+    c = Coef2D(dt=dt, dx=dx, eps=eps, eps_x=eps_x, eps_y=eps_y, y_mirror=y_mirror, x_mirror=x_mirror,
                b_hx=b_hx[:, None], a_hx=a_hx[:, None],
                b_hy=b_hy[None, :], a_hy=a_hy[None, :],
                b_ey=b_eyf[1:-1][None, :], a_ey=a_eyf[1:-1][None, :],
@@ -319,15 +341,23 @@ def te_coeffs(nx, ny, dx, eps, npml, courant=0.5, dtype=torch.float32, poles=(),
 
 
 # --------------------------------------------------------------------- state
-def tm_state(nx, ny, dtype=torch.float32, y_pmc=False, n_poles=0, n_mpoles=0):
+def tm_state(nx, ny, dtype=torch.float32, y_pmc=False, n_poles=0, n_mpoles=0, y_mirror=0, x_mirror=0, y_rot2=0,
+             rot4=0):
     """Field dict with the TMState shapes of stepper_tm.py:12-71 (pole
     stacks only when n_poles > 0; B and the magnetic pole stacks only when
-    n_mpoles > 0)."""
+    n_mpoles > 0; the on-plane H rows Hx0 (nx,) under an odd y mirror,
+    Rotate2 or Rotate4 and Hy0 (ny,) under an odd x mirror or Rotate4; a cut
+    makes the D-update psi one node longer on its axis)."""
     z = lambda *s: torch.zeros(*s, dtype=dtype)
-    ny_e = ny if y_pmc else ny - 2
+    ny_e = ny if y_pmc else (ny - 1 if (y_mirror or y_rot2 or rot4) else ny - 2)
+    nx_e = nx - 1 if (x_mirror or rot4) else nx - 2
     out = {"Ez": z(nx, ny), "Hx": z(nx, ny - 1), "Hy": z(nx - 1, ny), "Dz": z(nx, ny),
            "psi_Ezy": z(nx, ny - 1), "psi_Ezx": z(nx - 1, ny),
-           "psi_Hyx": z(nx - 2, ny_e), "psi_Hxy": z(nx - 2, ny_e)}
+           "psi_Hyx": z(nx_e, ny_e), "psi_Hxy": z(nx_e, ny_e)}
+    if y_mirror == -1 or y_rot2 or rot4:
+        out["Hx0"] = z(nx)
+    if x_mirror == -1 or rot4:
+        out["Hy0"] = z(ny)
     if n_poles:
         out.update(P_cur=z(n_poles, nx, ny), P_prev=z(n_poles, nx, ny))
     if n_mpoles:
@@ -368,6 +398,17 @@ def tm_step(s, c, coef=0.0, mask=None, hcoef=0.0, hx_mask=None, hy_mask=None):
     g2 = (Ez[1:, :] - Ez[:-1, :]) / dx
     pzx = c.b_hx * s["psi_Ezx"] + c.a_hx * g2
     mag = {}
+    # on-plane H rows of the odd cuts (stepper_tm.py:94-115): a ghost Ez
+    # beyond the cut gives dEz/dn = (Ez_cut - ghost) / dx; the update is
+    # H0 -/+ (dt * (Ez_cut - ghost)) / dx with Ez at time n
+    if "Hx0" in s:
+        ez0 = Ez[:, 0]
+        ghost = c.rot4 * Ez[0, :] if c.rot4 else (c.y_rot2 * ez0.flip(0) if c.y_rot2 else -ez0)
+        mag["Hx0"] = s["Hx0"] - dt * (ez0 - ghost) / dx
+    if "Hy0" in s:
+        ez0x = Ez[0, :]
+        ghost_x = c.rot4 * Ez[:, 0] if c.rot4 else -ez0x
+        mag["Hy0"] = s["Hy0"] + dt * (ez0x - ghost_x) / dx
     if c.mc1x is not None:      # stepper_tm.py:103-106: B integrates the curl
         Bx = s["Bx"] - dt * (g + pzy)
         By = s["By"] + dt * (g2 + pzx)
@@ -379,7 +420,7 @@ def tm_step(s, c, coef=0.0, mask=None, hcoef=0.0, hx_mask=None, hy_mask=None):
         Mx = c.mc1x * s["Mx_cur"] + c.mc2x * s["Mx_prev"] + c.mc3x * s["Hx"]
         My = c.mc1y * s["My_cur"] + c.mc2y * s["My_prev"] + c.mc3y * s["Hy"]
         Hx, Hy = Bx - Mx.sum(0), By - My.sum(0)
-        mag = dict(Bx=Bx, By=By, Mx_cur=Mx, Mx_prev=s["Mx_cur"], My_cur=My, My_prev=s["My_cur"])
+        mag.update(Bx=Bx, By=By, Mx_cur=Mx, Mx_prev=s["Mx_cur"], My_cur=My, My_prev=s["My_cur"])
     else:
         if c.h1x is not None:       # stepper_tm.py:116-121: dB/dt = -curl E - sigma_b B, mu folded in h2
             Hx = c.h1x * s["Hx"] - c.h2x * (g + pzy)
@@ -392,20 +433,33 @@ def tm_step(s, c, coef=0.0, mask=None, hcoef=0.0, hx_mask=None, hy_mask=None):
         if hx_mask is not None:
             Hx = Hx - hcoef * hx_mask
 
+    # stepper_tm.py:134-155: the H beyond each wall (PMC: zero both sides;
+    # even mirror: a zero ghost at the cut; odd / Rotate cuts: the on-plane
+    # row), then differences over the live Ez region
     if c.y_pmc:
         Hxp = torch.nn.functional.pad(Hx, (1, 1))
         ys, pad_y = slice(None), (0, 0)
+    elif c.y_mirror == 1:
+        Hxp, ys, pad_y = torch.nn.functional.pad(Hx, (1, 0)), slice(None, -1), (0, 1)
+    elif c.y_mirror == -1 or c.y_rot2 or c.rot4:
+        Hxp, ys, pad_y = torch.cat([mag["Hx0"][:, None], Hx], dim=1), slice(None, -1), (0, 1)
     else:
         Hxp, ys, pad_y = Hx, slice(1, -1), (1, 1)
-    dHx = (Hxp[1:-1, 1:] - Hxp[1:-1, :-1]) / dx
-    dHy = (Hy[1:, ys] - Hy[:-1, ys]) / dx
+    if c.x_mirror == 1:
+        Hyp, xs, pad_x = torch.nn.functional.pad(Hy, (0, 0, 1, 0)), slice(None, -1), (0, 1)
+    elif c.x_mirror == -1 or c.rot4:
+        Hyp, xs, pad_x = torch.cat([mag["Hy0"][None, :], Hy], dim=0), slice(None, -1), (0, 1)
+    else:
+        Hyp, xs, pad_x = Hy, slice(1, -1), (1, 1)
+    dHx = (Hxp[xs, 1:] - Hxp[xs, :-1]) / dx
+    dHy = (Hyp[1:, ys] - Hyp[:-1, ys]) / dx
     pyx = c.b_ex * s["psi_Hyx"] + c.a_ex * dHy
     pxy = c.b_ey * s["psi_Hxy"] + c.a_ey * dHx
     curl = dHy + pyx - dHx - pxy
     if c.d1 is not None:        # stepper_tm.py:161-163: dD/dt = curl H - sigma_d D
-        Dz = c.d1 * s["Dz"] + c.d2 * torch.nn.functional.pad(curl, pad_y + (1, 1))
+        Dz = c.d1 * s["Dz"] + c.d2 * torch.nn.functional.pad(curl, pad_y + pad_x)
     else:
-        Dz = s["Dz"] + dt * torch.nn.functional.pad(curl, pad_y + (1, 1))
+        Dz = s["Dz"] + dt * torch.nn.functional.pad(curl, pad_y + pad_x)
     if mask is not None:
         if c.d1 is not None:    # stepper_tm.py:198-200: scale = d2, coef is then the raw amplitude
             Dz = Dz - c.d2 * coef * mask
@@ -443,6 +497,12 @@ def te_step(s, c, coef=0.0, mask=None, hcoef=0.0, hz_mask=None):
     gx = (Hz[1:, :] - Hz[:-1, :]) / dx
     phx = c.b_ex * s["psi_Hzx"] + c.a_ex * gx
     cy = torch.nn.functional.pad(gx + phx, (0, 0, 1, 1))
+    # even mirror (stepper_te.py:88-97): the on-plane tangential E row is
+    # live, with the odd Hz ghost giving dHz/dn = 2 Hz[0] / dx there
+    if c.y_mirror == 1:
+        cx = torch.cat([2.0 * Hz[:, :1] / dx, cx[:, 1:]], dim=1)
+    if c.x_mirror == 1:
+        cy = torch.cat([2.0 * Hz[:1, :] / dx, cy[1:, :]], dim=0)
     if c.d1x is not None:       # stepper_te.py:96-99 (sigma_d)
         Dx = c.d1x * s["Dx"] + c.d2x * cx
         Dy = c.d1y * s["Dy"] - c.d2y * cy

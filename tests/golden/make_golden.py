@@ -41,6 +41,11 @@ Cases (all CPU, torch threads = 8):
              an in-plane tensor eps (exx, exy, eyy) with an oriented pole
              (fn, gamma, (sxx, sxy, syy)) and a shared scalar pole; (tenk)
              the tensor eps under chi3; line source, flux line
+  fold_2d    symmetry-folded 2D runs, 36x36, npml 6 on the far sides, 200
+             steps, f32 + f64: TM with an even / odd y mirror, even / odd x
+             mirror, both mirrors, Rotate2 (+-1) and Rotate4 (+-1) cuts; TE
+             with even / odd y and x mirrors; point-like line source, flux
+             line; all fields including the on-plane H rows
   media_3d   3D 22x20x18, npml 4, 120 steps, sigma_d, sigma_b and a Lorentz +
              Drude pole pair with masks, point source, flux plane, f32 + f64
   absorber_3d  3D 22x20x18, 120 steps, f32 + f64, boundary modes without a
@@ -621,6 +626,71 @@ def case_tensor_te():
         save(f"tensor_te_{tag}", **out)
 
 
+FOLD_TM = {"ym+": dict(y_mirror=1), "ym-": dict(y_mirror=-1), "xm+": dict(x_mirror=1),
+           "xm-": dict(x_mirror=-1), "xy": dict(x_mirror=-1, y_mirror=1), "r2+": dict(y_rot2=1),
+           "r2-": dict(y_rot2=-1), "r4+": dict(rot4=1), "r4-": dict(rot4=-1)}
+FOLD_TE = {"ym+": dict(y_mirror=1), "ym-": dict(y_mirror=-1), "xm+": dict(x_mirror=1),
+           "xm-": dict(x_mirror=-1)}
+
+
+def case_fold_2d():
+    from meep_torch.core.coeffs import build_te_coeffs, build_tm_coeffs
+    from meep_torch.core.grid import Grid
+    from meep_torch.core.monitors import FluxMonitorTE, FluxMonitorTM
+    from meep_torch.core.run import run_te, run_tm
+    from meep_torch.core.sources import GaussianPulse
+    from meep_torch.core.stepper_te import init_te_state
+    from meep_torch.core.stepper_tm import init_tm_state
+    n, steps = 36, 200
+    x = torch.arange(n, dtype=torch.float64)[:, None]
+    y = torch.arange(n, dtype=torch.float64)[None, :]
+    eps = 1.0 + 1.5 * torch.exp(-((x - 14.0) ** 2 + (y - 11.0) ** 2) / 40.0)
+    amp = GaussianPulse(0.9, 0.7, cutoff=3.0)
+
+    class Src:
+        def __init__(self, mask, dt):
+            self.mask, self._dt = mask, dt
+
+        def amplitude(self, t):
+            return amp.amplitude((t + 0.5) * self._dt)
+
+    for dtype, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        g = Grid(nx=n, ny=n, dx=0.05, courant=0.5, dtype=dtype, device="cpu")
+        out = {"eps": eps.to(dtype)}
+        for name, kw in FOLD_TM.items():
+            m = build_tm_coeffs(g, eps_inf=eps.to(dtype), npml=((0 if kw.get("x_mirror") or kw.get("rot4") else 6, 6),
+                                                                (0 if kw.get("y_mirror") or kw.get("y_rot2") or
+                                                                 kw.get("rot4") else 6, 6)), **kw)
+            mask = torch.zeros(n, n, dtype=dtype)
+            mask[9, 4:14] = 20.0
+            mask[4:12, 9] += 10.0
+            st0 = init_tm_state(g, **kw)
+            mon = FluxMonitorTM(g, [0.7, 0.9, 1.1], x_normal=True, idx=26, lo=0, hi=28)
+            st, (acc,) = run_tm(st0, m, steps, source=Src(mask, g.dt), monitors=[mon])
+            out[f"tm{name}_mask"] = mask
+            for f in st._fields:
+                v = getattr(st, f)
+                if isinstance(v, torch.Tensor) and v.numel():
+                    out[f"tm{name}_{f}"] = v
+            out[f"tm{name}_acc"] = acc
+        for name, kw in FOLD_TE.items():
+            m = build_te_coeffs(g, eps_inf=eps.to(dtype), npml=((0 if kw.get("x_mirror") else 6, 6),
+                                                                (0 if kw.get("y_mirror") else 6, 6)), **kw)
+            mask = torch.zeros(n, n - 1, dtype=dtype)
+            mask[9, 0:12] = 20.0
+            mask[0:12, 9] += 10.0
+            mon = FluxMonitorTE(g, [0.7, 0.9, 1.1], ix=26, lo=0, hi=28)
+            st, (acc,) = run_te(init_te_state(g), m, steps, source=Src(mask, g.dt), monitors=[mon])
+            out[f"te{name}_mask"] = mask
+            for f in st._fields:
+                v = getattr(st, f)
+                if isinstance(v, torch.Tensor) and v.numel():
+                    out[f"te{name}_{f}"] = v
+            out[f"te{name}_acc"] = acc
+        out["amp"] = np.array([Src(None, g.dt).amplitude(t) for t in range(steps)])
+        save(f"fold_2d_{tag}", **out)
+
+
 def case_media_3d():
     from meep_torch.core.monitors_3d import FluxMonitor3D, PointSource3D, run_3d
     from meep_torch.core.sources import GaussianPulse
@@ -867,7 +937,7 @@ def main():
     which = sys.argv[1:] or ["pml", "tm", "te", "c1", "3d", "grad_tm", "grad_3d",
                              "filters", "golden_plain", "media_tm", "media_te", "media_3d",
                              "absorber_3d", "tfsf_tm", "sim3d", "raster", "shapes", "multisrc", "sim_absorber",
-                             "mag_tm", "tensor_te"]
+                             "mag_tm", "tensor_te", "fold_2d"]
     t_all = time.perf_counter()
     timing = {}
     try:
@@ -885,7 +955,7 @@ def main():
          "absorber_3d": case_absorber_3d,
          "tfsf_tm": case_tfsf_tm, "sim3d": case_sim3d, "raster": case_raster,
          "shapes": case_shapes, "multisrc": case_multisrc, "sim_absorber": case_sim_absorber,
-         "mag_tm": case_mag_tm, "tensor_te": case_tensor_te}[w]()
+         "mag_tm": case_mag_tm, "tensor_te": case_tensor_te, "fold_2d": case_fold_2d}[w]()
         print(f"[golden] {w}: {time.perf_counter() - t0:.1f}s", flush=True)
     man = {}
     for fn in sorted(os.listdir(HERE)):

@@ -95,12 +95,14 @@ def test_mask_box():
     assert mask_box(torch.zeros(4, 4)) is None
 
 
-@pytest.mark.parametrize("te,kw,flag", [(True, {"x_mirror": 1}, "symmetry"),
-                                        (False, {"y_mirror": 1}, "symmetry"),
+@pytest.mark.parametrize("te,kw,flag", [(False, {"y_mirror": -1, "mu": 2.0}, "odd mirror"),
+                                        (False, {"y_rot2": 1, "sigma_b": 0.5}, "Rotate2"),
                                         (False, {"m_poles": [(1.0, 0.1, 1.0)], "mu": 2.0}, "m_poles")])
 def test_unsupported_flags_raise(te, kw, flag):
-    """Symmetry folds, and TM magnetic poles combined with sigma_b / mu
-    (which the reference refuses too, coeffs.py:178-179) raise by name."""
+    """The combinations the reference refuses raise by name: an odd mirror or
+    Rotate cut with sigma_b / mu (no coefficients on the on-plane H row,
+    coeffs.py:239-243) and magnetic poles with sigma_b / mu
+    (coeffs.py:178-179)."""
     from meep_torch_amd.core.coeffs import build_te_coeffs, build_tm_coeffs
     from meep_torch_amd.core.grid import Grid
     g = Grid(nx=16, ny=12, dx=0.05, dtype=torch.float32, device="cpu")
