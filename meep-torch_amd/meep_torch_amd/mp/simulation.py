@@ -36,6 +36,7 @@ from . import geometry as G
 from . import run_control as RC
 from .sim3d import Sim3D
 from .sources import AUTOMATIC, EigenModeSource, port_line, source_span
+from .symmetry import Folding
 from .vec import Vector3, pol_for
 
 _DEFAULTS = dict(eps_averaging=True, symmetries=None, force_complex_fields=False, m=0, k_point=False,
@@ -114,18 +115,23 @@ class Simulation(Sim3D):
                 raise TypeError(f"Simulation() got an unexpected keyword argument {k!r}")
             if k == "geometry_center" and (v is None or Vector3(*v) == Vector3()):
                 continue
-            if k == "symmetries" and not v:
+            if k == "symmetries":
                 continue
             if k == "eps_averaging" or v == _DEFAULTS[k]:
                 continue
             raise NotImplementedError(f"Simulation({k}=...) is not supported by the HIP FDTD facade")
         if dimensions == 3:
+            if options.get("symmetries"):
+                raise NotImplementedError("Mirror / Rotate2 / Rotate4 symmetries are 2D-only")
             self._init3(cell_size, resolution, geometry, sources, boundary_layers, default_material, Courant, dtype,
                         device)
             return
         if dimensions != 2:
             raise NotImplementedError(f"the Simulation facade drives 2D and 3D cells, not dimensions={dimensions}")
         self.dimensions = 2
+        self.symmetries = list(options.get("symmetries") or [])   # meep-mutable until the structure is built
+        self._fold = Folding((), None, "tm", None, None)           # replaced when the structure is built
+        self._cut = {}
         self.dtype = dtype
         self.cell_size = Vector3(*cell_size)
         self.resolution = resolution
@@ -173,6 +179,21 @@ class Simulation(Sim3D):
             sd, sb, mu = media["cond"] if media["cond"] is not None else (
                 np.zeros((g.nx, g.ny)), np.zeros((g.nx, g.ny)), np.ones((g.nx, g.ny)))
             media["cond"] = (sd + ramp, sb + ramp, mu)
+        # symmetries (mp/simulation.py:383-494,551-570): every map is
+        # rasterised on the full cell, then the run keeps the part at and
+        # above the cut plane(s) on a smaller grid closed by the cut walls
+        self._fold = Folding(self.symmetries, self.gs, pol, self._npml_sides, nabs)
+        cut = {}
+        if self._fold.active:
+            cx, cy = self._fold.cx, self._fold.cy
+            half = lambda a: None if a is None else np.ascontiguousarray(np.asarray(a)[cx:, cy:])
+            self._raster = tuple(half(a) for a in self._raster)
+            media = dict(media, masks=[half(m) for m in media["masks"]], chi3=half(media["chi3"]),
+                         cond=None if media["cond"] is None else tuple(half(a) for a in media["cond"]))
+            self.grid = g = Grid(nx=self.gs.nx - cx, ny=self.gs.ny - cy, dx=self.gs.dx, dtype=self.dtype,
+                                 courant=g.courant, device=g.device)
+            cut = self._fold.cut_kwargs(pol)
+        self._cut = cut
         has_media = bool(media["poles"]) or media["chi3"] is not None or media["cond"] is not None
         self._n_poles = len(media["poles"])
         if pol == "tm":
@@ -185,7 +206,7 @@ class Simulation(Sim3D):
             if media["cond"] is not None:
                 sd, sb, mu = (G.to_tensor(v, g.dtype, g.device) for v in media["cond"])
                 kw.update(sigma_d=sd, sigma_b=sb, mu=mu)
-            self._coeffs = build_tm_coeffs(g, eps_inf=eps, npml=npml, **kw)
+            self._coeffs = build_tm_coeffs(g, eps_inf=eps, npml=npml, **kw, **cut)
         else:
             kw = {}
             if media["chi3"] is not None:
@@ -198,7 +219,7 @@ class Simulation(Sim3D):
                 kw.update(sigma_d=sd, sigma_b=sb, mu=mu)
             eps_xx = G.to_tensor(self._raster[1], torch.float64, g.device)
             eps_yy = G.to_tensor(self._raster[2], torch.float64, g.device)
-            c = build_te_coeffs(g, eps_inf=eps_xx, npml=npml, **kw)
+            c = build_te_coeffs(g, eps_inf=eps_xx, npml=npml, **kw, **cut)
             self._coeffs = c._replace(eps_y=(0.5 * (eps_yy[:, :-1] + eps_yy[:, 1:])).to(g.dtype))
 
     def init_sim(self):
@@ -208,6 +229,14 @@ class Simulation(Sim3D):
         out = []
         for spec in self._sources:
             ix, j0, j1 = source_span(spec, self.gs)
+            if self._fold.active:
+                # a source below a cut is dropped, not folded: the cut wall
+                # supplies its image (mp/simulation.py:822-837); a span that
+                # crosses the plane keeps its part above it
+                ix, mirx = self._fold.foldpt(ix, "x")
+                j0, j1, _, miry = self._fold.fold(j0, j1, "y")
+                if mirx or miry:
+                    continue
             src = LineSource(self.grid, ix, j0, j1, profile=spec.src.profile())
             src.mask = src.mask * spec.amplitude
             if self._pol == "te":
@@ -231,6 +260,17 @@ class Simulation(Sim3D):
             x0, x1 = c.x - s.x / 2 + gs.sx / 2, c.x + s.x / 2 + gs.sx / 2
             lo, hi = G.span_index_range(max(0.0, x0), min(gs.sx, x1), gs, "x")
         meta = {"idx": idx, "lo": lo, "hi": hi, "wt": complex(region.weight or 1.0).real}
+        if self._fold.active:
+            # fold the line onto the kept part (mp/simulation.py:1829-1843): a
+            # span crossing the plane keeps its upper part and doubles back
+            # (mir2); a line below a cut maps onto its image, where the
+            # mirrored tangential H flips against E (msign)
+            idx, pos_mir = self._fold.foldpt(idx, "x" if x_normal else "y")
+            lo, hi, meta["mir2"], span_mir = self._fold.fold(lo, hi, "y" if x_normal else "x")
+            self._fold.rot_guard("a flux/mode region", pos_mir, meta["mir2"], span_mir)
+            if pos_mir:
+                meta["msign"] = -1.0
+            meta.update(idx=idx, lo=lo, hi=hi)
         if self._pol == "tm":
             mon = FluxMonitorTM(self.grid, freqs, x_normal=x_normal, idx=idx, lo=lo, hi=hi)
         elif x_normal:
@@ -278,6 +318,8 @@ class Simulation(Sim3D):
         to port 0, the reflection), and the run is compute_smatrix's
         per-port launch sequence, kept as S."""
         from ..ports import Port, compute_smatrix
+        if self._fold.active:
+            raise NotImplementedError("EigenModeSource S-parameters with a symmetry are not supported")
         es = self._eig_source
         c = self._coeffs
         if self._n_poles or (c.d1 if self._pol == "tm" else c.d1x) is not None or c.has_kerr:
@@ -350,6 +392,9 @@ class Simulation(Sim3D):
             j0, j1 = G.span_index_range(y0, y1, self.gs, "y")
             box = (i0, i1, j0, j1)
         for comp in components:
+            if self._fold.active and comp not in ("Ez", "Dz"):
+                raise NotImplementedError(f"add_dft_fields({comp}) with a symmetry: staggered components do not "
+                                          f"de-mirror onto the full cell")
             mon = DFTFieldsMonitor(self.grid, freqs, component=comp)
             self._monitors[f"{name}:{comp}"] = ("dft", mon, {"freqs": freqs, "box": box})
         return DftFields(self, name, freqs)
@@ -401,7 +446,7 @@ class Simulation(Sim3D):
             T = float(until) if until is not None else cutoff + float(until_after_sources or 0.0)
             n_total = max(1, int(round(T / dt)))
         if mode == "tm":
-            st = init_tm_state(self.grid, n_poles=self._n_poles)
+            st = init_tm_state(self.grid, n_poles=self._n_poles, **self._cut)
             names = tm_fields_of(st)
         else:
             st = init_te_state(self.grid, n_poles=self._n_poles)
@@ -487,15 +532,12 @@ class Simulation(Sim3D):
         if not conds or not all(isinstance(g, RC.stop_when_fields_decayed) for g in conds):
             return None
         out = []
-        gs = self.gs
         for g in conds:
-            pt = Vector3(*g.pt)
-            ix = max(0, min(gs.nx - 1, G.cell_index(pt.x, gs)))
-            iy = max(0, min(gs.ny - 1, G.cell_index_y(pt.y, gs)))
-            name, scale = (g.c if isinstance(g.c, str) else ""), 1.0
+            ix, iy, sgn = self._probe_cell(Vector3(*g.pt))
+            name, scale = (g.c if isinstance(g.c, str) else ""), sgn
             if name.startswith("D") and name != "Dielectric":
                 name = "E" + name[1:]
-                scale = float(np.asarray(self.get_epsilon())[ix, iy])
+                scale = sgn * float(np.asarray(self.get_epsilon())[ix, iy])
             elif name.startswith("B"):
                 name = "H" + name[1:]
             if name not in work:
@@ -526,7 +568,9 @@ class Simulation(Sim3D):
         acc = self._acc.get(name)
         if acc is None:
             return [0.0] * int(mon.freqs.numel())
-        vals = mon.flux(acc, self.gs.dx).detach().cpu().numpy() * meta.get("wt", 1.0)
+        vals = mon.flux(acc, self.gs.dx).detach().cpu().numpy() * meta.get("wt", 1.0) * meta.get("msign", 1.0)
+        if meta.get("mir2"):
+            vals = 2.0 * vals    # the span crossed the mirror plane: the kept part holds half the flux
         return vals.tolist()
 
     def get_flux_freqs(self, f):
@@ -545,10 +589,7 @@ class Simulation(Sim3D):
             raise NotImplementedError("get_field_point: 2D runs only")
         if self._state is None:
             return 0.0
-        gs = self.gs
-        pt = Vector3(*pt)
-        ix = max(0, min(gs.nx - 1, G.cell_index(pt.x, gs)))
-        iy = max(0, min(gs.ny - 1, G.cell_index_y(pt.y, gs)))
+        ix, iy, sgn = self._probe_cell(Vector3(*pt))
         name, scale = (c if isinstance(c, str) else ""), 1.0
         if name.startswith("D") and name != "Dielectric":
             name = "E" + name[1:]
@@ -558,7 +599,21 @@ class Simulation(Sim3D):
         f = getattr(self._state, name, None)
         if f is None:
             f = self._state.Hz if self._pol == "te" else self._state.Ez
-        return scale * float(f[min(ix, f.shape[0] - 1), min(iy, f.shape[1] - 1)])
+        return sgn * scale * float(f[min(ix, f.shape[0] - 1), min(iy, f.shape[1] - 1)])
+
+    def _probe_cell(self, pt):
+        """(ix, iy, sign) of the engine cell holding meep point pt: folded
+        onto the kept part under a symmetry, with the Ez phase of the fold
+        (mp/simulation.py:995-1001)."""
+        gs = self.gs
+        ix = max(0, min(gs.nx - 1, G.cell_index(pt.x, gs)))
+        iy = max(0, min(gs.ny - 1, G.cell_index_y(pt.y, gs)))
+        if not self._fold.active:
+            return ix, iy, 1.0
+        ix, mirx = self._fold.foldpt(ix, "x")
+        iy, miry = self._fold.foldpt(iy, "y")
+        px, py = self._fold.parity("Ez")
+        return ix, iy, (px if mirx else 1.0) * (py if miry else 1.0)
 
     def get_flux_data(self, flux):
         """meep_torch/mp/simulation.py:1882-1898: the monitor's accumulated
@@ -605,6 +660,8 @@ class Simulation(Sim3D):
         kind, mon, meta = self._monitors[key]
         a = self._acc[key][num_freq].detach().cpu().numpy()
         out = a[0] + 1j * a[1]
+        if self._fold.active:
+            out = self._fold.demirror(out, self._fold.parity(component))
         if meta.get("box") is not None:
             i0, i1, j0, j1 = meta["box"]
             out = out[i0:i1, j0:j1]
@@ -617,16 +674,25 @@ class Simulation(Sim3D):
             return self._get_array3(component)
         if component == "Dielectric":
             self._init_structure()
-            return (self._raster[0] if self._pol == "tm" else self._raster[1]).copy()
+            a = (self._raster[0] if self._pol == "tm" else self._raster[1]).copy()
+            return self._fold.demirror(a) if self._fold.active else a
         if self._state is None:
             raise RuntimeError("run() before get_array(fields)")
+        fold = self._fold if self._fold.active else None
         if component in ("Ez", "Ex") and self._pol == "tm":
-            return self._state.Ez.detach().cpu().numpy()
+            a = self._state.Ez.detach().cpu().numpy()
+            return fold.demirror(a, fold.parity("Ez")) if fold else a
         if component in ("Hz", "Hy") and self._pol == "te":
-            return self._state.Hz.detach().cpu().numpy()
+            a = self._state.Hz.detach().cpu().numpy()
+            return fold.demirror_te_hz(a) if fold else a
         f = getattr(self._state, component, None)
         if f is None:
             raise ValueError(f"unsupported component {component!r}")
+        if fold:
+            if self._pol != "tm" or component not in ("Dz", "Hx", "Hy"):
+                raise NotImplementedError(f"get_array({component!r}) with a symmetry: the staggered component "
+                                          f"does not rebuild onto the full cell")
+            return fold.demirror(f.detach().cpu().numpy(), fold.parity(component))
         return f.detach().cpu().numpy()
 
     def get_epsilon(self):

@@ -68,6 +68,11 @@ Cases (all CPU, torch threads = 8):
   sim_absorber  mp.Simulation with mp.Absorber layers: a 2D TM and a 2D TE
              cell (Absorber on x, PML on y) and a 3D cell (Absorber all
              round): Ez / Hz / Ey and a flux, f64
+  sim_sym    mp.Simulation 2D cells with symmetries, f64: Mirror(Y) on a
+             guide (decay stop at a point below the plane), Mirror(X, -1) +
+             Mirror(Y) with a source pair, Rotate2, Rotate4 and a TE
+             Mirror(Y, -1): full-cell field, fluxes (lines crossing and not
+             crossing the planes) and probed points
   multisrc   mp.Simulation 2D TM cell with two sources (point and line, own
              pulses and amplitudes): Ez and a flux after t = 10, plain and
              with an at_every step function splitting the run, f64
@@ -933,11 +938,71 @@ def case_sim_absorber():
     save("sim_absorber_f64", **out)
 
 
+def _sym_cases(mp):
+    """(name, Simulation kwargs, flux regions, probe points, run kwargs) of the
+    symmetry golden; also used by the tests through the recorded values."""
+    guide = [mp.Block(size=mp.Vector3(mp.inf, 0.6), material=mp.Medium(epsilon=8.0))]
+    pulse = lambda f: mp.GaussianSource(f, fwidth=0.6)
+    return [
+        ("my", dict(cell_size=mp.Vector3(6, 4), geometry=guide,
+                    sources=[mp.Source(pulse(0.8), component=mp.Ez, center=mp.Vector3(-2.0, 0.0),
+                                       size=mp.Vector3(0, 0.8))],
+                    symmetries=[mp.Mirror(mp.Y)]),
+         [mp.FluxRegion(center=mp.Vector3(2.0, 0), size=mp.Vector3(0, 3)),
+          mp.FluxRegion(center=mp.Vector3(0.5, -1.0), size=mp.Vector3(2.0, 0))],
+         [(1.0, 0.3), (1.0, -0.3)], dict(until_after_sources=mp.stop_when_fields_decayed(2.0, mp.Ez,
+                                                                                        mp.Vector3(1.0, -0.3), 1e-2))),
+        ("mxy", dict(cell_size=mp.Vector3(5, 4), geometry=[],
+                     sources=[mp.Source(pulse(0.9), component=mp.Ez, center=mp.Vector3(0.8, 0.0)),
+                              mp.Source(pulse(0.9), component=mp.Ez, center=mp.Vector3(-0.8, 0.0), amplitude=-1.0)],
+                     symmetries=[mp.Mirror(mp.X, phase=-1), mp.Mirror(mp.Y)]),
+         [mp.FluxRegion(center=mp.Vector3(0, 1.2), size=mp.Vector3(4.0, 0)),
+          mp.FluxRegion(center=mp.Vector3(1.8, 0), size=mp.Vector3(0, 3.0))],
+         [(0.5, 0.5), (-0.5, -0.5), (-1.2, 0.7)], dict(until=8)),
+        ("r2", dict(cell_size=mp.Vector3(4, 4), geometry=[mp.Block(size=mp.Vector3(1.2, 0.4),
+                                                                   material=mp.Medium(epsilon=6.0))],
+                    sources=[mp.Source(pulse(0.9), component=mp.Ez, center=mp.Vector3(0.5, 0.5)),
+                             mp.Source(pulse(0.9), component=mp.Ez, center=mp.Vector3(-0.5, -0.5))],
+                    symmetries=[mp.Rotate2(mp.Z)]),
+         [mp.FluxRegion(center=mp.Vector3(1.2, 0.9), size=mp.Vector3(0, 1.4))],
+         [(0.6, 0.4), (-0.6, -0.4), (0.9, -0.8)], dict(until=8)),
+        ("r4", dict(cell_size=mp.Vector3(4, 4), geometry=[mp.Block(size=mp.Vector3(0.8, 0.8),
+                                                                   material=mp.Medium(epsilon=4.0))],
+                    sources=[mp.Source(pulse(1.0), component=mp.Ez, center=mp.Vector3(0.6, 0.7))],
+                    symmetries=[mp.Rotate4(mp.Z)]),
+         [mp.FluxRegion(center=mp.Vector3(1.4, 0.9), size=mp.Vector3(0, 1.4))],
+         [(0.6, 0.7), (-0.7, 0.6), (-0.6, -0.7)], dict(until=8)),
+        ("te", dict(cell_size=mp.Vector3(6, 4), geometry=guide,
+                    sources=[mp.Source(pulse(0.8), component=mp.Ey, center=mp.Vector3(-2.0, 0.0),
+                                       size=mp.Vector3(0, 0.8))],
+                    symmetries=[mp.Mirror(mp.Y, phase=-1)]),
+         [mp.FluxRegion(center=mp.Vector3(2.0, 0), size=mp.Vector3(0, 3))],
+         [(1.0, 0.3)], dict(until=10)),
+    ]
+
+
+def case_sim_sym():
+    import meep_torch.mp as mp
+    out = {}
+    for name, kw, regions, pts, runkw in _sym_cases(mp):
+        sim = mp.Simulation(resolution=10, boundary_layers=[mp.PML(0.5)], **kw)
+        fls = [sim.add_flux(0.8, 0.4, 3, r) for r in regions]
+        seen = []
+        comp = mp.Ez if name != "te" else mp.Hz
+        sim.run(mp.at_every(1.0, lambda s: seen.append([float(np.real(s.get_field_point(comp, mp.Vector3(*p))))
+                                                          for p in pts])), **runkw)
+        out[f"{name}_field"] = np.asarray(sim.get_array(component=comp))
+        out[f"{name}_flux"] = np.array([mp.get_fluxes(f) for f in fls])
+        out[f"{name}_probe"] = np.array(seen)
+        out[f"{name}_t"] = np.array(sim.meep_time())
+    save("sim_sym_f64", **out)
+
+
 def main():
     which = sys.argv[1:] or ["pml", "tm", "te", "c1", "3d", "grad_tm", "grad_3d",
                              "filters", "golden_plain", "media_tm", "media_te", "media_3d",
                              "absorber_3d", "tfsf_tm", "sim3d", "raster", "shapes", "multisrc", "sim_absorber",
-                             "mag_tm", "tensor_te", "fold_2d"]
+                             "mag_tm", "tensor_te", "fold_2d", "sim_sym"]
     t_all = time.perf_counter()
     timing = {}
     try:
@@ -955,7 +1020,7 @@ def main():
          "absorber_3d": case_absorber_3d,
          "tfsf_tm": case_tfsf_tm, "sim3d": case_sim3d, "raster": case_raster,
          "shapes": case_shapes, "multisrc": case_multisrc, "sim_absorber": case_sim_absorber,
-         "mag_tm": case_mag_tm, "tensor_te": case_tensor_te, "fold_2d": case_fold_2d}[w]()
+         "mag_tm": case_mag_tm, "tensor_te": case_tensor_te, "fold_2d": case_fold_2d, "sim_sym": case_sim_sym}[w]()
         print(f"[golden] {w}: {time.perf_counter() - t0:.1f}s", flush=True)
     man = {}
     for fn in sorted(os.listdir(HERE)):
