@@ -314,6 +314,75 @@ __device__ __forceinline__ T adj_b_hy_fused(const AdjK<T>& r, const AdjK<T>& w, 
   return gb;
 }
 
+// The B-phase inputs of one cell (the H cotangents and their psi
+// cotangents with the profile coefficients), loaded ahead: they do not
+// depend on the A phase, so a kernel can issue them before its first barrier.
+template <typename T>
+struct BIn {
+  T ahy, pzx, a_x, b_x, ahx, pzy, a_y, b_y;
+  bool sx, sy;
+};
+
+template <typename T>
+__device__ __forceinline__ BIn<T> adj_b_load(const AdjK<T>& r, int i, int j, bool hy_ok, bool hx_ok) {
+  BIn<T> v{T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), false, false};
+  if (hy_ok) {
+    const long long hy = (long long)i * r.ny + j;
+    v.sx = r.hx.in(i);
+    v.ahy = r.aHy[hy];
+    if (v.sx) {
+      v.pzx = r.apzx[hy];
+      v.a_x = r.hx.a[i];
+      v.b_x = r.hx.b[i];
+    }
+  }
+  if (hx_ok) {
+    const long long hx = (long long)i * (r.ny - 1) + j;
+    v.sy = r.hy.in(j);
+    v.ahx = r.aHx[hx];
+    if (v.sy) {
+      v.pzy = r.apzy[hx];
+      v.a_y = r.hy.a[j];
+      v.b_y = r.hy.b[j];
+    }
+  }
+  return v;
+}
+
+// adj_b_hy_fused / adj_b_hx_fused from preloaded inputs (same arithmetic);
+// Store = false for halo cells, whose cotangents another tile owns
+template <typename T, bool Store = true>
+__device__ __forceinline__ T adj_b_hy_pre(const AdjK<T>& r, const AdjK<T>& w, int i, int j, const BIn<T>& v,
+                                          T gy0, T gy1) {
+  const long long hy = (long long)i * r.ny + j;
+  const T h2y = r.dt;
+  const T hb = v.ahy + (gy0 - gy1) / r.dx;
+  if (Store) w.aHy[hy] = hb;
+  T gb = h2y * hb;
+  if (v.sx) {
+    const T pt = v.pzx + h2y * hb;
+    gb = gb + v.a_x * pt;
+    if (Store) w.apzx[hy] = v.b_x * pt;
+  }
+  return gb;
+}
+
+template <typename T, bool Store = true>
+__device__ __forceinline__ T adj_b_hx_pre(const AdjK<T>& r, const AdjK<T>& w, int i, int j, const BIn<T>& v,
+                                          T gx0, T gx1) {
+  const long long hx = (long long)i * (r.ny - 1) + j;
+  const T h2x = r.dt;
+  const T hb = v.ahx + (gx0 - gx1) / r.dx;
+  if (Store) w.aHx[hx] = hb;
+  T gb = -h2x * hb;
+  if (v.sy) {
+    const T pt = v.pzy - h2x * hb;
+    gb = gb + v.a_y * pt;
+    if (Store) w.apzy[hx] = v.b_y * pt;
+  }
+  return gb;
+}
+
 // Hx part of B at (i, j) (needs j < ny-1): gx0 = gHx(i, j), gx1 = gHx(i, j+1);
 // returns gEy, with own writes the Hx cotangent and its psi cotangent.
 template <typename T>

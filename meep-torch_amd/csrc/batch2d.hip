@@ -151,11 +151,16 @@ __global__ __launch_bounds__(FTI * FTJ) void adj_batch_fused(const AdjK<T>* rs, 
   const int i = i0 + ty, j = j0 + tx;
   const bool own = i < nx && j < ny;
   const T* ez_after = static_cast<const T*>(tapes[lo]) + (long long)s_rel * nx * ny;
+  // Every load of the A and B phases that does not depend on the A results
+  // is issued before the first store and barrier: the B inputs of the own
+  // cell and of this thread's B halo cell, then the A halo cell (which
+  // stores nothing), then the own A cell.
+  const bool hrow = q < FTJ, hcol = !hrow && q < FTJ + FTI;
+  const int rr = q - FTJ;
+  const BIn<T> pre = adj_b_load(r, i, j, own && i < nx - 1, own && j < ny - 1);
+  const BIn<T> preh = adj_b_load(r, hrow ? i0 - 1 : i0 + rr, hrow ? j0 + q : j0 - 1,
+                                 hrow && i0 >= 1 && j0 + q < ny, hcol && j0 >= 1 && i0 + rr < nx);
   // ---- A
-  T ghy = T(0), ghx = T(0);
-  if (own) adj_a_fused(r, w, ez_after, i, j, true, ghy, ghx);
-  sGHy[ty + 1][tx] = ghy;
-  sGHx[ty][tx + 1] = ghx;
   if (q < 2 * FTJ) {                      // rows i0-1 and i0+FTI: gHy
     const int hi_row = q >= FTJ, jj = j0 + (q % FTJ), ii = hi_row ? i0 + FTI : i0 - 1;
     T gy = T(0), gx;
@@ -167,21 +172,24 @@ __global__ __launch_bounds__(FTI * FTJ) void adj_batch_fused(const AdjK<T>* rs, 
     if (jj >= 0 && jj < ny && ii < nx) adj_a_fused(r, w, ez_after, ii, jj, false, gy, gx);
     sGHx[q2 % FTI][hi_col ? FTJ + 1 : 0] = gx;
   }
+  T ghy = T(0), ghx = T(0);
+  if (own) adj_a_fused(r, w, ez_after, i, j, true, ghy, ghx);
+  sGHy[ty + 1][tx] = ghy;
+  sGHx[ty][tx + 1] = ghx;
   __syncthreads();
   // ---- B
   T gex = T(0), gey = T(0);
-  if (own && i < nx - 1) gex = adj_b_hy_fused(r, w, i, j, sGHy[ty + 1][tx], sGHy[ty + 2][tx], true);
-  if (own && j < ny - 1) gey = adj_b_hx_fused(r, w, i, j, sGHx[ty][tx + 1], sGHx[ty][tx + 2], true);
+  if (own && i < nx - 1) gex = adj_b_hy_pre(r, w, i, j, pre, sGHy[ty + 1][tx], sGHy[ty + 2][tx]);
+  if (own && j < ny - 1) gey = adj_b_hx_pre(r, w, i, j, pre, sGHx[ty][tx + 1], sGHx[ty][tx + 2]);
   sGEx[ty + 1][tx] = gex;
   sGEy[ty][tx + 1] = gey;
-  if (q < FTJ) {                          // row i0-1: gEx
+  if (hrow) {                             // row i0-1: gEx
     T g = T(0);
-    if (i0 >= 1 && j0 + q < ny) g = adj_b_hy_fused(r, w, i0 - 1, j0 + q, sGHy[0][q], sGHy[1][q], false);
+    if (i0 >= 1 && j0 + q < ny) g = adj_b_hy_pre<T, false>(r, w, i0 - 1, j0 + q, preh, sGHy[0][q], sGHy[1][q]);
     sGEx[0][q] = g;
-  } else if (q < FTJ + FTI) {             // column j0-1: gEy
-    const int rr = q - FTJ;
+  } else if (hcol) {                      // column j0-1: gEy
     T g = T(0);
-    if (j0 >= 1 && i0 + rr < nx) g = adj_b_hx_fused(r, w, i0 + rr, j0 - 1, sGHx[rr][0], sGHx[rr][1], false);
+    if (j0 >= 1 && i0 + rr < nx) g = adj_b_hx_pre<T, false>(r, w, i0 + rr, j0 - 1, preh, sGHx[rr][0], sGHx[rr][1]);
     sGEy[rr][0] = g;
   }
   __syncthreads();
