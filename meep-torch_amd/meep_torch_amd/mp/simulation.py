@@ -120,6 +120,8 @@ class Simulation(Sim3D):
             if k == "eps_averaging" or v == _DEFAULTS[k]:
                 continue
             raise NotImplementedError(f"Simulation({k}=...) is not supported by the HIP FDTD facade")
+        self._fold = Folding((), None, "tm", None, None)           # replaced when a 2D structure is built
+        self._cut = {}
         if dimensions == 3:
             if options.get("symmetries"):
                 raise NotImplementedError("Mirror / Rotate2 / Rotate4 symmetries are 2D-only")
@@ -130,8 +132,6 @@ class Simulation(Sim3D):
             raise NotImplementedError(f"the Simulation facade drives 2D and 3D cells, not dimensions={dimensions}")
         self.dimensions = 2
         self.symmetries = list(options.get("symmetries") or [])   # meep-mutable until the structure is built
-        self._fold = Folding((), None, "tm", None, None)           # replaced when the structure is built
-        self._cut = {}
         self.dtype = dtype
         self.cell_size = Vector3(*cell_size)
         self.resolution = resolution
