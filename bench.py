@@ -40,10 +40,15 @@ HBM_PEAK_GBS = 8000.0
 # FP32 algorithmic bytes per cell-update: whole step (CPML shells included)
 ALG_BYTES = {"c1": 32.0, "c2": 29.0, "c3": 58.0}
 # ... and for the dominant kernel's cells: the 2D interior tiles are counted
-# at the whole-step figure (their tiles run the same fields), as is the
-# two-pass 3D step (the whole grid); the interior box of the split 3D step
-# has no psi terms: 48 B of field read + write and 4 B of eps
-KERNEL_ALG_BYTES = {"c1": 32.0, "c2": 29.0, "c3": 58.0, "c3_split": 52.0}
+# at the whole-step figure (their tiles run the same fields), as are the
+# two-pass 3D step (the whole grid) and the two-step 3D pass (the slab inside
+# the x and y CPML, z CPML included; two cell-updates per cell and launch);
+# the interior box of the split 3D step has no psi terms: 48 B of field read
+# + write and 4 B of eps
+KERNEL_ALG_BYTES = {"c1": 32.0, "c2": 29.0, "c3": 58.0, "c3_split": 52.0, "c3_tb2": 58.0}
+# 3D stepper variant of the C3 line: the two-pass step (the two-step pass,
+# variant 5, is bit-identical and measured equal at the full run length)
+C3_VARIANT = 0
 
 
 DOMINANT = {
@@ -51,6 +56,8 @@ DOMINANT = {
     "c2": "te_tiled interior-tile launch (temporally blocked TE steps), HIP events on its stream",
     "c3": "h3_kernel + e3_kernel (one two-pass 3D step over the whole grid), HIP events on its stream",
     "c3_split": "y3_interior (one-pass interior box of the split 3D step), HIP events on its stream",
+    "c3_tb2": "y3_tb2 psi-free-column and z-edge-column launches (two Yee steps per pass over the slab "
+              "inside the x and y CPML), HIP events on its stream",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
@@ -60,9 +67,10 @@ def _traffic(workload, cell_steps_per_launch, variant=None):
     measurement (FETCH_SIZE x fetch scale + WRITE_SIZE x write scale, per
     cell-step), scaled to this launch size; None if not measured (or
     measured for another kernel variant)."""
+    key = "c3_tb2" if workload == "c3" and variant is not None and variant >= 4 else workload
     try:
         with open(TRAFFIC_FILE) as fh:
-            t = json.load(fh)[workload]
+            t = json.load(fh)[key]
     except (OSError, KeyError, ValueError):
         return None
     if "variant" in t and variant is not None and t["variant"] != variant:
@@ -270,7 +278,8 @@ def main():
     ap.add_argument("--c5-iters", type=int, default=500, help="Adam iterations of each design loop")
     ap.add_argument("--c5-yee-steps", type=int, default=2500)
     ap.add_argument("--c3-variant", type=int, default=None,
-                    help="3D stepper: 0 two-pass, 1-3 split (interior kernel with 6, 14, 4 tile rows + shell)")
+                    help="3D stepper: 0 two-pass, 1-3 split (interior kernel with 6, 14, 4 tile rows + shell), "
+                         f"4-5 two steps per pass (tile rows 10, 8); default {C3_VARIANT}")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB float32 tensor three times (reference for PMC byte counters)")
     ap.add_argument("--c4-variants", type=int, default=64)
@@ -302,9 +311,9 @@ def main():
         return bench_c4(args)
     if args.workload == "c5":
         return bench_c5(args)
-    if args.c3_variant is not None:
+    if args.workload == "c3":
         from meep_torch_amd.core import engine3d
-        engine3d.DEFAULT_VARIANT = args.c3_variant
+        engine3d.DEFAULT_VARIANT = C3_VARIANT if args.c3_variant is None else args.c3_variant
     builder = {"c1": build_c1, "c2": build_c2, "c3": build_c3}[args.workload]
     run, cells, ysteps, _ = builder(args.yee_steps or None)
     log(f"[bench] {args.workload}: {cells} cells x {ysteps} Yee steps per bench step")
@@ -351,8 +360,10 @@ def main():
     if args.workload == "c3":
         from meep_torch_amd.core import engine3d
         variant = engine3d.DEFAULT_VARIANT
-        if variant >= 1:
+        if 1 <= variant <= 3:
             k_alg, dominant = KERNEL_ALG_BYTES["c3_split"], DOMINANT["c3_split"]
+        elif variant >= 4:
+            k_alg, dominant = KERNEL_ALG_BYTES["c3_tb2"], DOMINANT["c3_tb2"]
     elif tune._CACHE and len(set(tune._CACHE.values())) == 1:
         variant = next(iter(tune._CACHE.values()))     # the tile variant the autotuner chose
     achieved = k_alg * per_launch_cells / per_launch_s / 1e9

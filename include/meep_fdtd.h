@@ -215,8 +215,13 @@ typedef struct {
    * (tile rows 6, 14, 4; FP64 uses 6 for variant 2) plus the per-cell H and
    * E updates over the CPML shell pieces -- which needs alt[0..5]: second
    * buffers of Ex Ey Ez Hx Hy Hz (same shapes), used ping-pong (psi stay in
-   * place; alt[6..17] are unused); the state ends in the primary buffers
-   * either way.  Media run the two-pass kernels whatever the variant. */
+   * place); 4, 5: two steps per pass over the slab inside the x and y CPML
+   * (every z; tile rows 10, 8) plus two per-cell steps of the x and y CPML
+   * slabs, which needs alt[0..5] as above, alt[6..11] (a third field set)
+   * and alt[12..15] (scratch copies of psi Ey_z Ex_z Hy_z Hx_z); FP32 runs
+   * without tape whose monitors each read one x plane only, other runs take
+   * variant 0.  The state ends in the primary buffers either way.  Media
+   * run the two-pass kernels whatever the variant. */
   int variant, pad2_;
   void* alt[18];
   /* Media of the functional stepper (stepper_3d.py:217-266,295-301); two-pass

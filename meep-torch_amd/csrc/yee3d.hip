@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -793,6 +794,667 @@ int run3_split(const mt_yee3d* d, K3<T> p, const mt_source3d& sv, const double* 
   return MT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Variants 4-6: two Yee steps per pass (temporal blocking) over the slab I
+// of cells whose two-step dependency cone is psi-free along x and y and away
+// from the x and y walls (the interior box shrunk by two cells along x and
+// y; every z index, z walls and z CPML shells included), and the two steps
+// of the rest of the grid (the "ring": the x and y CPML slabs) with the
+// per-cell updates.  Per pair of steps, with A the current field set, B the
+// other one and C a third (alt[6..11]):
+//   y3_tb2:  I cells, steps 1 and 2, A -> B; the z-CPML memory of I cells
+//            read from the psi arrays, written to scratch copies
+//            (alt[12..15]) that a copy returns after the pass, so no wave
+//            reads a value another wave has already advanced; step-1 fields
+//            of the listed x planes (the monitors' planes) and of the cells
+//            on I's x and y faces -> C;
+//   ring 1:  step 1 of every cell outside I, A -> C (psi in place), reading
+//            the H' of I's high faces from C;
+//   ring 2:  step 2 of every cell outside I, C -> B (psi in place), reading
+//            the E' of I's low faces from C and the H'' of the I cells next
+//            to the ring from B.
+// Every value of B, of the psi arrays and of the monitored planes of C is
+// the two-pass step's value bit for bit: each cell runs the two-pass cell
+// update's expressions.
+// ---------------------------------------------------------------------------
+constexpr int TB_PLANES = 8;
+
+template <typename T>
+struct TB3 {
+  int nx, ny, nz, lean;
+  int I0, I1, J0, J1;                 // the slab I (all of z)
+  int njt, nkl, kt_lo, kt_hi, iseg, ntiles, per_xcd, nsp;
+  int sp[TB_PLANES];                  // x planes whose step-1 fields go to C
+  T dt, dx, rdx, c1, c2;              // source coefficients of steps 1 and 2
+  const T* eps;
+  Pml<T> hz, ez;                      // z profiles of the H and E updates
+  const T *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;   // A (read)
+  T *oEx, *oEy, *oEz, *oHx, *oHy, *oHz;   // B (step 2)
+  T *cEx, *cEy, *cEz, *cHx, *cHy, *cHz;   // C (step 1, listed planes and faces)
+  const T *psEYZ, *psEXZ, *psHYZ, *psHXZ; // z-CPML memory (read)
+  T *osEYZ, *osEXZ, *osHYZ, *osHXZ;       // its scratch copies (written)
+};
+
+// Cross-lane moves for the k neighbours of the two-step kernel: DPP
+// wavefront shifts (every lane active), the value held by lane l + 1 (next)
+// or l - 1 (prev); lane 63 / lane 0 receive 0.
+__device__ __forceinline__ float from_next_lane(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_prev_lane(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+// One wave per tile, no LDS and no barrier: lane l holds column
+// k = k0 - 2 + l and every lane holds the R rows j0 - 2 .. j0 + R - 3 in
+// registers, so j neighbours are registers of the same lane and k neighbours
+// come from the next / previous lane.  Outputs: rows j0 .. j0 + R - 5, lanes
+// 2 .. 61 (columns k0 .. k0 + 59), planes [ib, ie).  The wave streams along
+// x; at plane i it advances step 1 at plane i (H'(i) from E(i), E(i+1);
+// E'(i) from H'(i), H'(i-1)) and step 2 at plane i-1 (H''(i-1) from E'(i-1),
+// E'(i); E''(i-1) from H''(i-1), H''(i-2)), so each plane of A is read once
+// and each plane of B written once.  Halo rows, lanes and the two planes
+// before ib are recomputed, never read from another wave's output.
+// ZP: the tile's columns reach the z walls or the z CPML shells (the curl
+// terms along z vanish at the walls; the z-CPML memory of step 1 stays in
+// registers for step 2); else every column is psi-free and off the walls.
+// Issuing the next plane's loads before this plane's arithmetic (one more
+// register set) and a three-waves-per-SIMD register budget (spills) were
+// both measured slower on MI355X than this form at two waves per SIMD.
+// Division by dx: FD, the verified constant-divisor sequence (div_c's fast
+// path) with the dividends' exponents recorded; a row of a stage in which
+// any lane of the wave met a dividend outside the verified range is
+// recomputed with IEEE division by the whole wave (a uniform branch), which
+// is div_c's result for every quotient.
+template <int R, bool FD, bool ZP>
+__global__ __launch_bounds__(64) void y3_tb2(TB3<float> p, mt_source3d src) {
+  using T = float;
+  constexpr int TR = R - 4;
+  const int t = (blockIdx.x & 7) * p.per_xcd + (blockIdx.x >> 3);   // XCD-aware, j tiles fastest
+  if (t >= p.ntiles) return;
+  const int jt = t % p.njt, r2 = t / p.njt, e = r2 % p.nkl, sg = r2 / p.nkl;
+  // k tiles kt_lo .. kt_hi-1 are the psi-free ones; the others take ZP
+  const int kt = ZP ? (e < p.kt_lo ? e : p.kt_hi + (e - p.kt_lo)) : p.kt_lo + e;
+  const int j0 = p.J0 + jt * TR, k0 = kt * 60;
+  const int ib = p.I0 + sg * p.iseg, ie = min(p.I1, ib + p.iseg);
+  const int ny = p.ny, nz = p.nz;
+  const int lane = threadIdx.x;
+  const int kk = k0 - 2 + lane;
+  const bool lane_own = lane >= 2 && lane < 62 && kk < nz;
+  const bool hasZ = kk < nz - 1;    // Ez, Hx, Hy exist in this column
+  // Indices past the grid are clamped: they feed only cells that no output
+  // reads (halo cells beyond a wall, or past I's last row).
+  const int kA = min(max(kk, 0), nz - 1), kB = min(max(kk, 0), nz - 2), kC = min(max(kk - 1, 0), nz - 3);
+  // z walls and z CPML (ZP only): Ex and Ey carry no curl on the wall
+  // columns (stepper_3d.py:392-445 masks), the z curl terms of Hx, Hy (H
+  // profile at k) and Ex, Ey (E profile at k-1) carry psi inside the shells
+  const bool kz = !ZP || (kk >= 1 && kk < nz - 1);
+  const bool hin = ZP && kk >= 0 && kk < nz - 1 && p.hz.in(kk);
+  const bool ein = ZP && kz && p.ez.in(kk - 1);
+  T hb = T(0), ha = T(0), eb = T(0), ea = T(0);
+  if (hin) hb = p.hz.b[kk], ha = p.hz.a[kk];
+  if (ein) eb = p.ez.b[kk - 1], ea = p.ez.a[kk - 1];
+  const T dx = p.dx, dt = p.dt;
+  // byte offsets within a plane of the nz- / (nz-1)- / (nz-2)-wide grids:
+  // a wave-uniform row part (scalar registers) plus this lane's column part
+  unsigned rA[R], rB[R], rC[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int jr = min(j0 - 2 + r, ny - 2);
+    rA[r] = (unsigned)(jr * nz) * 4u;
+    rB[r] = (unsigned)(jr * (nz - 1)) * 4u;
+    rC[r] = (unsigned)(jr * (nz - 2)) * 4u;
+  }
+  const unsigned lA = (unsigned)kA * 4u, lB = (unsigned)kB * 4u, lC = (unsigned)kC * 4u;
+  // plane strides in bytes: Ex, eps (ny, nz); Ey, Hz (ny-1, nz); Ez, Hy (ny, nz-1); Hx (ny-1, nz-1);
+  // psi Hy_z (ny, nz-2), Hx_z (ny-1, nz-2)
+  const long long sA = 4LL * ny * nz, sY = 4LL * (ny - 1) * nz, sZ = 4LL * ny * (nz - 1),
+                  sX = 4LL * (ny - 1) * (nz - 1), sQ = 4LL * ny * (nz - 2), sW = 4LL * (ny - 1) * (nz - 2);
+  auto at = [](const void* base, long long plane_bytes, int ip) {
+    return static_cast<const char*>(base) + ip * plane_bytes;
+  };
+  auto ld = [](const char* plane, unsigned off) { return *reinterpret_cast<const T*>(plane + off); };
+  auto st = [](T* base, long long plane_bytes, int ip, unsigned off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + ip * plane_bytes + off) = v;
+  };
+  // The source term only where the wave's rows, columns and planes meet
+  // the source box (a uniform test per plane).
+  const bool src_rows = src.kind != 0 && j0 - 2 < src.j1 && j0 + R - 2 > src.j0 && k0 - 2 < src.k1 &&
+                        k0 + 62 > src.k0;
+  auto inj = [&](int comp, int ip, int j, T E, T epsc, T c) {
+    return inject(p, src, comp, ip, j, kk, E, epsc, c);
+  };
+
+  T e0x[R], e0y[R], e0z[R], epc[R], epp[R];   // E(i), eps(i), eps(i-1)
+  T h1px[R], h1py[R], h1pz[R];                // H'(i-1)
+  T e1px[R], e1py[R], e1pz[R];                // E'(i-1)
+  T h2py[R], h2pz[R];                         // H''(i-2)
+  T q1EYZ[R], q1EXZ[R], q1HYZ[R], q1HXZ[R];   // step-1 z-CPML memory of plane i-1 (ZP)
+  int i = ib - 2;
+  {
+    const char *pEx = at(p.Ex, sA, i), *pEy = at(p.Ey, sY, i), *pEz = at(p.Ez, sZ, i), *pe = at(p.eps, sA, i);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      e0x[r] = ld(pEx + rA[r], lA);
+      e0y[r] = ld(pEy + rA[r], lA);
+      e0z[r] = ld(pEz + rB[r], lB);
+      epc[r] = ld(pe + rA[r], lA);
+      epp[r] = h1px[r] = h1py[r] = h1pz[r] = e1px[r] = e1py[r] = e1pz[r] = h2py[r] = h2pz[r] = T(0);
+      q1EYZ[r] = q1EXZ[r] = q1HYZ[r] = q1HXZ[r] = T(0);
+    }
+  }
+  // the loads iteration i consumes: E(i+1), eps(i+1), H(i) (planes past the
+  // grid are clamped and never used)
+  T nEx[R], nEy[R], nEz[R], epn[R], hx0[R], hy0[R], hz0[R];
+  auto fetch = [&](int ip, T* fEx, T* fEy, T* fEz, T* fep, T* fHx, T* fHy, T* fHz) {
+    const int pe = min(ip + 1, p.nx - 1), pex = min(ip + 1, p.nx - 2), ph = min(ip, p.nx - 2);
+    const char *pEx = at(p.Ex, sA, pex), *pEy = at(p.Ey, sY, pe), *pEz = at(p.Ez, sZ, pe), *pp = at(p.eps, sA, pe),
+               *pHx = at(p.Hx, sX, ph), *pHy = at(p.Hy, sZ, ph), *pHz = at(p.Hz, sY, ph);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      fEx[r] = ld(pEx + rA[r], lA);
+      fEy[r] = ld(pEy + rA[r], lA);
+      fEz[r] = ld(pEz + rB[r], lB);
+      fep[r] = ld(pp + rA[r], lA);
+      if (r < R - 1) {
+        fHx[r] = ld(pHx + rB[r], lB);
+        fHy[r] = ld(pHy + rB[r], lB);
+        fHz[r] = ld(pHz + rA[r], lA);
+      }
+    }
+  };
+  for (; i <= ie; ++i) {
+    fetch(i, nEx, nEy, nEz, epn, hx0, hy0, hz0);
+    // z-CPML memory of plane i (step 0), shell columns only
+    T q0EYZ[R], q0EXZ[R], q0HYZ[R], q0HXZ[R];
+    if constexpr (ZP) {
+      const char *a1 = at(p.psEYZ, sX, i), *a2 = at(p.psEXZ, sZ, min(i, p.nx - 2)),
+                 *a3 = at(p.psHYZ, sQ, min(i, p.nx - 2)), *a4 = at(p.psHXZ, sW, i);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        q0EYZ[r] = hin ? ld(a1 + rB[r], lB) : T(0);
+        q0EXZ[r] = hin ? ld(a2 + rB[r], lB) : T(0);
+        q0HYZ[r] = ein ? ld(a3 + rC[r], lC) : T(0);
+        q0HXZ[r] = ein ? ld(a4 + rC[r], lC) : T(0);
+      }
+    }
+    const bool src_here = src_rows && i >= src.i0 - 1 && i < src.i1 + 1;
+    // ---- step 1 at plane i: H'(i) on rows 0 .. R-2 (stepper_3d.py:363-390),
+    // E'(i) on rows 1 .. R-2 (stepper_3d.py:392-445); each row's quotients
+    // are one group of the division check
+    T h1x[R], h1y[R], h1z[R], e1x[R], e1y[R], e1z[R];
+    T q1nEYZ[R], q1nEXZ[R], q1nHYZ[R], q1nHXZ[R];
+#pragma unroll
+    for (int r = 0; r < R - 1; ++r) {
+      const T eyk = from_next_lane(e0y[r]), exk = from_next_lane(e0x[r]);
+      auto row = [&](auto& dv) {
+        T cz = dv(eyk - e0y[r]);
+        if constexpr (ZP) {   // ce_apply: psi' = b psi + a d, d + psi'
+          const T ps = hb * q0EYZ[r] + ha * cz;
+          q1nEYZ[r] = ps;
+          cz = hin ? cz + ps : cz;
+        }
+        h1x[r] = hx0[r] - dt * (dv(e0z[r + 1] - e0z[r]) - cz);
+        T cz2 = dv(exk - e0x[r]);
+        if constexpr (ZP) {
+          const T ps = hb * q0EXZ[r] + ha * cz2;
+          q1nEXZ[r] = ps;
+          cz2 = hin ? cz2 + ps : cz2;
+        }
+        h1y[r] = hy0[r] - dt * (cz2 - dv(nEz[r] - e0z[r]));
+        h1z[r] = hz0[r] - dt * (dv(nEy[r] - e0y[r]) - dv(e0x[r + 1] - e0x[r]));
+      };
+      DxDiv<T, FD> dv{dx, p.rdx, 0u};
+      row(dv);
+      if (FD && __any(!dv.exact())) {
+        DxDiv<T, false> dq{dx, p.rdx, 0u};
+        row(dq);
+      }
+    }
+#pragma unroll
+    for (int r = 1; r < R - 1; ++r) {
+      const int j = j0 - 2 + r;
+      const T hyk = from_prev_lane(h1y[r]), hxk = from_prev_lane(h1x[r]), epk = from_next_lane(epc[r]);
+      auto row = [&](auto& dv) {
+        T chz = dv(h1y[r] - hyk);
+        if constexpr (ZP) {
+          const T ps = eb * q0HYZ[r] + ea * chz;
+          q1nHYZ[r] = ps;
+          chz = ein ? chz + ps : chz;
+        }
+        T curl = dv(h1z[r] - h1z[r - 1]) - chz;
+        if constexpr (ZP) curl = kz ? curl : T(0);
+        T epsc = T(0.5) * (epc[r] + epn[r]);
+        e1x[r] = e_update(p, e0x[r], epsc, curl);
+        if (src_here) e1x[r] = inj(0, i, j, e1x[r], epsc, p.c1);
+        T chz2 = dv(h1x[r] - hxk);
+        if constexpr (ZP) {
+          const T ps = eb * q0HXZ[r] + ea * chz2;
+          q1nHXZ[r] = ps;
+          chz2 = ein ? chz2 + ps : chz2;
+        }
+        curl = chz2 - dv(h1z[r] - h1pz[r]);
+        if constexpr (ZP) curl = kz ? curl : T(0);
+        epsc = T(0.5) * (epc[r] + epc[r + 1]);
+        e1y[r] = e_update(p, e0y[r], epsc, curl);
+        if (src_here) e1y[r] = inj(1, i, j, e1y[r], epsc, p.c1);
+        curl = dv(h1y[r] - h1py[r]) - dv(h1x[r] - h1x[r - 1]);
+        epsc = T(0.5) * (epc[r] + epk);
+        e1z[r] = e_update(p, e0z[r], epsc, curl);
+        if (src_here) e1z[r] = inj(2, i, j, e1z[r], epsc, p.c1);
+      };
+      DxDiv<T, FD> dv{dx, p.rdx, 0u};
+      row(dv);
+      if (FD && __any(!dv.exact())) {
+        DxDiv<T, false> dq{dx, p.rdx, 0u};
+        row(dq);
+      }
+    }
+    // step-1 fields into C: whole monitored planes, and the cells on I's x
+    // and y faces (the ring's step 1 reads H' of the high faces, its step 2
+    // reads E' of the low faces)
+    if (i >= ib && i < ie && lane_own) {
+      bool plane = i == p.I0 || i == p.I1 - 1;
+      for (int q = 0; q < p.nsp; ++q) plane = plane || p.sp[q] == i;
+#pragma unroll
+      for (int r = 2; r < R - 2; ++r) {
+        const int j = j0 - 2 + r;
+        if (j >= p.J1) break;
+        if (plane || j == p.J0 || j == p.J1 - 1) {
+          st(p.cEx, sA, i, rA[r] + lA, e1x[r]);
+          st(p.cEy, sY, i, rA[r] + lA, e1y[r]);
+          st(p.cHz, sY, i, rA[r] + lA, h1z[r]);
+          if (hasZ) {
+            st(p.cEz, sZ, i, rB[r] + lB, e1z[r]);
+            st(p.cHx, sX, i, rB[r] + lB, h1x[r]);
+            st(p.cHy, sZ, i, rB[r] + lB, h1y[r]);
+          }
+        }
+      }
+    }
+    // ---- step 2 at plane i-1: H''(i-1) on rows 1 .. R-3, E''(i-1) on the
+    // output rows 2 .. R-3
+    if (i >= ib) {
+      const int ip = i - 1;
+      const bool src2 = src_rows && ip >= src.i0 - 1 && ip < src.i1 + 1;
+      T h2x[R], h2y[R], h2z[R], ex2[R], ey2[R], ez2[R];
+      T q2EYZ[R], q2EXZ[R], q2HYZ[R], q2HXZ[R];
+#pragma unroll
+      for (int r = 1; r < R - 2; ++r) {
+        const T eyk = from_next_lane(e1py[r]), exk = from_next_lane(e1px[r]);
+        auto row = [&](auto& dv) {
+          T cz = dv(eyk - e1py[r]);
+          if constexpr (ZP) {
+            const T ps = hb * q1EYZ[r] + ha * cz;
+            q2EYZ[r] = ps;
+            cz = hin ? cz + ps : cz;
+          }
+          h2x[r] = h1px[r] - dt * (dv(e1pz[r + 1] - e1pz[r]) - cz);
+          T cz2 = dv(exk - e1px[r]);
+          if constexpr (ZP) {
+            const T ps = hb * q1EXZ[r] + ha * cz2;
+            q2EXZ[r] = ps;
+            cz2 = hin ? cz2 + ps : cz2;
+          }
+          h2y[r] = h1py[r] - dt * (cz2 - dv(e1z[r] - e1pz[r]));
+          h2z[r] = h1pz[r] - dt * (dv(e1y[r] - e1py[r]) - dv(e1px[r + 1] - e1px[r]));
+        };
+        DxDiv<T, FD> dv{dx, p.rdx, 0u};
+        row(dv);
+        if (FD && __any(!dv.exact())) {
+          DxDiv<T, false> dq{dx, p.rdx, 0u};
+          row(dq);
+        }
+      }
+#pragma unroll
+      for (int r = 2; r < R - 2; ++r) {
+        const int j = j0 - 2 + r;
+        const T hyk = from_prev_lane(h2y[r]), hxk = from_prev_lane(h2x[r]), epk = from_next_lane(epp[r]);
+        auto row = [&](auto& dv) {
+          T chz = dv(h2y[r] - hyk);
+          if constexpr (ZP) {
+            const T ps = eb * q1HYZ[r] + ea * chz;
+            q2HYZ[r] = ps;
+            chz = ein ? chz + ps : chz;
+          }
+          T curl = dv(h2z[r] - h2z[r - 1]) - chz;
+          if constexpr (ZP) curl = kz ? curl : T(0);
+          T epsc = T(0.5) * (epp[r] + epc[r]);
+          ex2[r] = e_update(p, e1px[r], epsc, curl);
+          if (src2) ex2[r] = inj(0, ip, j, ex2[r], epsc, p.c2);
+          T chz2 = dv(h2x[r] - hxk);
+          if constexpr (ZP) {
+            const T ps = eb * q1HXZ[r] + ea * chz2;
+            q2HXZ[r] = ps;
+            chz2 = ein ? chz2 + ps : chz2;
+          }
+          curl = chz2 - dv(h2z[r] - h2pz[r]);
+          if constexpr (ZP) curl = kz ? curl : T(0);
+          epsc = T(0.5) * (epp[r] + epp[r + 1]);
+          ey2[r] = e_update(p, e1py[r], epsc, curl);
+          if (src2) ey2[r] = inj(1, ip, j, ey2[r], epsc, p.c2);
+          curl = dv(h2y[r] - h2py[r]) - dv(h2x[r] - h2x[r - 1]);
+          epsc = T(0.5) * (epp[r] + epk);
+          ez2[r] = e_update(p, e1pz[r], epsc, curl);
+          if (src2) ez2[r] = inj(2, ip, j, ez2[r], epsc, p.c2);
+        };
+        DxDiv<T, FD> dv{dx, p.rdx, 0u};
+        row(dv);
+        if (FD && __any(!dv.exact())) {
+          DxDiv<T, false> dq{dx, p.rdx, 0u};
+          row(dq);
+        }
+      }
+      if (ip >= ib && lane_own) {
+#pragma unroll
+        for (int r = 2; r < R - 2; ++r) {
+          if (j0 - 2 + r >= p.J1) break;
+          st(p.oEx, sA, ip, rA[r] + lA, ex2[r]);
+          st(p.oEy, sY, ip, rA[r] + lA, ey2[r]);
+          st(p.oHz, sY, ip, rA[r] + lA, h2z[r]);
+          if (hasZ) {
+            st(p.oEz, sZ, ip, rB[r] + lB, ez2[r]);
+            st(p.oHx, sX, ip, rB[r] + lB, h2x[r]);
+            st(p.oHy, sZ, ip, rB[r] + lB, h2y[r]);
+          }
+          if constexpr (ZP) {
+            if (hin) {
+              st(p.osEYZ, sX, ip, rB[r] + lB, q2EYZ[r]);
+              st(p.osEXZ, sZ, ip, rB[r] + lB, q2EXZ[r]);
+            }
+            if (ein) {
+              st(p.osHYZ, sQ, ip, rC[r] + lC, q2HYZ[r]);
+              st(p.osHXZ, sW, ip, rC[r] + lC, q2HXZ[r]);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 1; r < R - 2; ++r) {
+        h2py[r] = h2y[r];
+        h2pz[r] = h2z[r];
+      }
+    }
+    // rotate the planes (each array over the rows it holds)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (r < R - 1) h1px[r] = h1x[r], h1py[r] = h1y[r], h1pz[r] = h1z[r];
+      if (r >= 1 && r < R - 1) e1px[r] = e1x[r], e1py[r] = e1y[r], e1pz[r] = e1z[r];
+      if constexpr (ZP) {
+        if (r < R - 1) q1EYZ[r] = q1nEYZ[r], q1EXZ[r] = q1nEXZ[r];
+        if (r >= 1 && r < R - 1) q1HYZ[r] = q1nHYZ[r], q1HXZ[r] = q1nHXZ[r];
+      }
+      e0x[r] = nEx[r], e0y[r] = nEy[r], e0z[r] = nEz[r];
+      epp[r] = epc[r], epc[r] = epn[r];
+    }
+  }
+}
+
+// Returns the z-CPML memory the two-step kernel advanced (scratch copies) to
+// the psi arrays: for each of the four z-psi arrays, the rows [j0, j1) of
+// the planes [i0, i1) over its shell index ranges [0, lo) and [hi, nk).
+struct ZCopy {
+  const float* src[4];
+  float* dst[4];
+  long long plane[4];
+  int row[4], lo[4], hi[4], nk[4];
+  int i0, i1, j0, j1;
+};
+
+__global__ __launch_bounds__(256) void zpsi_copy(ZCopy c) {
+  const int a = blockIdx.y;
+  const int nlo = c.lo[a], nhi = c.nk[a] - c.hi[a], w = nlo + nhi;
+  const long long n = (long long)(c.i1 - c.i0) * (c.j1 - c.j0) * w;
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
+    const int q = (int)(e % w);
+    const long long rj = e / w;
+    const int j = c.j0 + (int)(rj % (c.j1 - c.j0)), i = c.i0 + (int)(rj / (c.j1 - c.j0));
+    const int k = q < nlo ? q : c.hi[a] + (q - nlo);
+    const long long o = i * c.plane[a] + (long long)j * c.row[a] + k;
+    c.dst[a][o] = c.src[a][o];
+  }
+}
+
+inline Box3 shrink(const Box3& b, int m) {
+  return Box3{b.I0 + m, b.I1 - m, b.J0 + m, b.J1 - m, b.K0 + m, b.K1 - m};
+}
+
+inline bool box_ok(const Box3& b) { return b.I1 > b.I0 && b.J1 > b.J0 && b.K1 > b.K0; }
+
+// The x planes the monitors read (each term must lie in one x plane), into
+// planes[0 .. *n); false when a term spans several planes or more than
+// TB_PLANES planes are read.
+inline bool monitor_planes(const MonitorPlan& mp, void* const* prim, const long long* bytes,
+                           const long long* plane_elems, int elem, int* planes, int* n) {
+  *n = 0;
+  for (int m = 0; m < mp.n_mons; ++m) {
+    const mt_monitor& M = mp.mons[m];
+    for (int q = 0; q < M.n_terms; ++q) {
+      const void* ptrs[2] = {M.term[q].a, M.term[q].b};
+      for (int w = 0; w < 2; ++w) {
+        if (!ptrs[w]) continue;
+        const char* a = static_cast<const char*>(ptrs[w]);
+        int f = -1;
+        for (int g = 0; g < 6; ++g) {
+          const char* b0 = static_cast<const char*>(prim[g]);
+          if (a >= b0 && a < b0 + bytes[g]) f = g;
+        }
+        if (f < 0) return false;
+        const long long o = (a - static_cast<const char*>(prim[f])) / elem;
+        const long long n0 = M.kind == 1 ? 1 : M.n0, n1 = M.kind == 1 ? 1 : M.n1;
+        const long long last = o + (n0 - 1) * M.term[q].s0 + (n1 - 1) * M.term[q].s1;
+        const long long p0 = o / plane_elems[f];
+        if (last / plane_elems[f] != p0 || M.term[q].s0 < 0 || M.term[q].s1 < 0) return false;
+        bool have = false;
+        for (int u = 0; u < *n; ++u) have = have || planes[u] == (int)p0;
+        if (!have) {
+          if (*n == TB_PLANES) return false;
+          planes[(*n)++] = (int)p0;
+        }
+      }
+    }
+  }
+  return true;
+}
+
+// Variant 4-6 host loop (RI: tile rows of the psi-free columns, RZ: of the
+// z-edge columns); returns 1 (nothing launched) when the run does not fit
+// it, and the caller takes the two-pass path.
+template <typename T, int RI, int RZ>
+int run3_tb2(const mt_yee3d* d, K3<T> p, const mt_source3d& sv, const double* coef, const MonitorPlan& mp,
+             int step0, int k, hipStream_t s) {
+  static_assert(sizeof(T) == 4, "the two-step pass is FP32");
+  const int nx = d->nx, ny = d->ny, nz = d->nz;
+  const Box3 P = interior_box(d);
+  const Box3 I{P.I0 + 2, P.I1 - 2, P.J0 + 2, P.J1 - 2, 0, nz};
+  if (!box_ok(I) || nz < 4 || d->tape || d->lean) return 1;
+  void* prim[6] = {d->Ex, d->Ey, d->Ez, d->Hx, d->Hy, d->Hz};
+  const long long nE[3] = {(long long)(nx - 1) * ny * nz, (long long)nx * (ny - 1) * nz, (long long)nx * ny * (nz - 1)};
+  const long long nH[3] = {(long long)nx * (ny - 1) * (nz - 1), (long long)(nx - 1) * ny * (nz - 1),
+                           (long long)(nx - 1) * (ny - 1) * nz};
+  const long long plane[6] = {(long long)ny * nz, (long long)(ny - 1) * nz, (long long)ny * (nz - 1),
+                              (long long)(ny - 1) * (nz - 1), (long long)ny * (nz - 1), (long long)(ny - 1) * nz};
+  long long bytes[6];
+  for (int q = 0; q < 3; ++q) {
+    bytes[q] = nE[q] * (long long)sizeof(T);
+    bytes[3 + q] = nH[q] * (long long)sizeof(T);
+  }
+  TB3<T> q{};
+  if (mp.n_mons && !monitor_planes(mp, prim, bytes, plane, (int)sizeof(T), q.sp, &q.nsp)) return 1;
+  void* const* A0 = prim;
+  void* const* B0 = d->alt;
+  void* const* C = d->alt + 6;
+  std::vector<mt_monitor> alt_mons(mp.n_mons > 0 ? mp.n_mons : 1), c_mons(mp.n_mons > 0 ? mp.n_mons : 1);
+  if (mp.n_mons) {
+    retarget(mp.mons, mp.n_mons, alt_mons.data(), prim, bytes, d->alt, 6);
+    retarget(mp.mons, mp.n_mons, c_mons.data(), prim, bytes, d->alt + 6, 6);
+  }
+  MonitorPlan mpa = mp, mpc = mp;
+  mpa.mons = alt_mons.data();
+  mpc.mons = c_mons.data();
+
+  q.nx = nx, q.ny = ny, q.nz = nz, q.lean = 0;
+  q.I0 = I.I0, q.I1 = I.I1, q.J0 = I.J0, q.J1 = I.J1;
+  q.hz = p.h[2], q.ez = p.e[2];
+  q.psEYZ = static_cast<const T*>(d->psi[P_EY_Z]), q.psEXZ = static_cast<const T*>(d->psi[P_EX_Z]);
+  q.psHYZ = static_cast<const T*>(d->psi[P_HY_Z]), q.psHXZ = static_cast<const T*>(d->psi[P_HX_Z]);
+  q.osEYZ = static_cast<T*>(d->alt[12]), q.osEXZ = static_cast<T*>(d->alt[13]);
+  q.osHYZ = static_cast<T*>(d->alt[14]), q.osHXZ = static_cast<T*>(d->alt[15]);
+  // k tiles of 60 columns from 0; the ones whose 64 lanes stay inside the
+  // psi-free, wall-free z range take the plain kernel
+  const int nkt = (nz + 59) / 60;
+  int kt_lo = nkt, kt_hi = nkt;
+  for (int t = 0; t < nkt; ++t)
+    if (60 * t - 2 >= P.K0 && 60 * t + 62 <= P.K1) {
+      if (kt_lo == nkt) kt_lo = t;
+      kt_hi = t + 1;
+    }
+  if (kt_lo == nkt) kt_lo = kt_hi = 0;
+  q.kt_lo = kt_lo, q.kt_hi = kt_hi;
+  q.dt = p.dt, q.dx = p.dx, q.rdx = p.rdx;
+  q.eps = p.eps;
+  // launch shapes of the two kernels: x segments, each three planes longer
+  // than its output; the count minimises rounds x (planes + 3), a round
+  // being the waves the chip holds at once, so the last round is no sliver
+  const int nxI = I.I1 - I.I0;
+  auto shape = [&](TB3<T>& x, int R, int nkl, const void* kern) -> int {
+    x.nkl = nkl;
+    x.njt = (I.J1 - I.J0 + (R - 4) - 1) / (R - 4);
+    const int tiles2d = x.njt * nkl;
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0) != hipSuccess)
+      return fail(MT_EHIP, "yee3d two-step: occupancy query failed");
+    const long long cap = std::max(1, per_cu * cus);
+    int nseg = 1;
+    long long best = -1;
+    for (int ns = 1; ns <= std::min(nxI, 64); ++ns) {
+      const int len = (nxI + ns - 1) / ns;
+      const int used = (nxI + len - 1) / len;
+      const long long cost = (((long long)tiles2d * used + cap - 1) / cap) * (len + 3);
+      if (best < 0 || cost < best) best = cost, nseg = used;
+    }
+    x.iseg = (nxI + nseg - 1) / nseg;
+    nseg = (nxI + x.iseg - 1) / x.iseg;
+    x.ntiles = tiles2d * nseg;
+    x.per_xcd = (x.ntiles + 7) / 8;
+    return MT_OK;
+  };
+  const bool fd = q.rdx != T(0);
+  TB3<T> qi = q, qz = q;
+  int rc;
+  const void* ki_fast = (const void*)y3_tb2<RI, true, false>;
+  const void* ki_ieee = (const void*)y3_tb2<RI, false, false>;
+  if (kt_hi > kt_lo && (rc = shape(qi, RI, kt_hi - kt_lo, fd ? ki_fast : ki_ieee))) return rc;
+  if (nkt - (kt_hi - kt_lo) > 0 &&
+      (rc = shape(qz, RZ, nkt - (kt_hi - kt_lo),
+                  fd ? (const void*)y3_tb2<RZ, true, true> : (const void*)y3_tb2<RZ, false, true>)))
+    return rc;
+  const bool run_i = kt_hi > kt_lo, run_z = nkt - (kt_hi - kt_lo) > 0;
+  // the z-psi cells the edge kernel advances
+  ZCopy zc{};
+  {
+    const int src_q[4] = {P_EY_Z, P_EX_Z, P_HY_Z, P_HX_Z};
+    const long long pl[4] = {(long long)(ny - 1) * (nz - 1), (long long)ny * (nz - 1), (long long)ny * (nz - 2),
+                             (long long)(ny - 1) * (nz - 2)};
+    const int rw[4] = {nz - 1, nz - 1, nz - 2, nz - 2};
+    for (int a = 0; a < 4; ++a) {
+      zc.src[a] = static_cast<const float*>(d->alt[12 + a]);
+      zc.dst[a] = static_cast<float*>(d->psi[src_q[a]]);
+      zc.plane[a] = pl[a];
+      zc.row[a] = rw[a];
+      zc.nk[a] = rw[a];
+      const Pml<T>& pr = a < 2 ? q.hz : q.ez;
+      zc.lo[a] = std::min(std::max(pr.lo_end, 0), rw[a]);
+      zc.hi[a] = std::max(std::min(pr.hi_begin, rw[a]), zc.lo[a]);
+    }
+    zc.i0 = I.I0, zc.i1 = I.I1, zc.j0 = I.J0, zc.j1 = I.J1;
+  }
+  const bool zcopy = run_z && (zc.lo[0] + zc.nk[0] - zc.hi[0] + zc.lo[2] + zc.nk[2] - zc.hi[2]) > 0;
+  const Pieces P2 = shell_pieces(I, nx, ny, nz);   // the ring: every cell outside I
+  const long long box_cells = (long long)(I.I1 - I.I0) * (I.J1 - I.J0) * nz;
+  auto set_io = [](auto& x, void* const* rd, void* const* wr) {
+    using U = std::remove_const_t<std::remove_pointer_t<decltype(x.oEx)>>;
+    x.Ex = static_cast<U*>(rd[0]), x.Ey = static_cast<U*>(rd[1]), x.Ez = static_cast<U*>(rd[2]);
+    x.Hx = static_cast<U*>(rd[3]), x.Hy = static_cast<U*>(rd[4]), x.Hz = static_cast<U*>(rd[5]);
+    x.oEx = static_cast<U*>(wr[0]), x.oEy = static_cast<U*>(wr[1]), x.oEz = static_cast<U*>(wr[2]);
+    x.oHx = static_cast<U*>(wr[3]), x.oHy = static_cast<U*>(wr[4]), x.oHz = static_cast<U*>(wr[5]);
+  };
+  for (TB3<T>* x : {&qi, &qz}) {
+    x->cEx = static_cast<T*>(C[0]), x->cEy = static_cast<T*>(C[1]), x->cEz = static_cast<T*>(C[2]);
+    x->cHx = static_cast<T*>(C[3]), x->cHy = static_cast<T*>(C[4]), x->cHz = static_cast<T*>(C[5]);
+  }
+  int S = mp.stage_steps > 0 ? mp.stage_steps : 1;
+  int filled = 0, first = step0;
+  auto sample = [&](const MonitorPlan& X, int step) -> int {
+    if (!mp.n_mons) return 0;
+    int e;
+    if ((e = monitors_sample(X, filled, step, s))) return e;
+    if (++filled == S || step == k - 1) {
+      if ((e = monitors_flush(mp, filled, first, d->dt, s))) return e;
+      first += filled;
+      filled = 0;
+    }
+    return 0;
+  };
+  bool in_alt = false;   // the current set: primary or alt[0..5]
+  int s_ = 0;
+  for (; s_ + 1 < k; s_ += 2) {
+    void* const* A = in_alt ? B0 : A0;
+    void* const* B = in_alt ? A0 : B0;
+    const T c1 = sv.kind ? static_cast<T>(coef[s_]) : T(0), c2 = sv.kind ? static_cast<T>(coef[s_ + 1]) : T(0);
+    qi.c1 = qz.c1 = c1;
+    qi.c2 = qz.c2 = c2;
+    set_io(qi, A, B);
+    set_io(qz, A, B);
+    timing_begin(s);
+    if (run_i) {
+      void* args[] = {&qi, const_cast<mt_source3d*>(&sv)};
+      if (hipLaunchKernel(fd ? ki_fast : ki_ieee, dim3(8 * qi.per_xcd), dim3(64), args, 0, s) != hipSuccess)
+        return fail(MT_EHIP, "yee3d two-step: launch failed");
+    }
+    if (run_z) {
+      if (fd)
+        hipLaunchKernelGGL((y3_tb2<RZ, true, true>), dim3(8 * qz.per_xcd), dim3(64), 0, s, qz, sv);
+      else
+        hipLaunchKernelGGL((y3_tb2<RZ, false, true>), dim3(8 * qz.per_xcd), dim3(64), 0, s, qz, sv);
+    }
+    timing_end(s, 2 * box_cells);
+    if (zcopy) hipLaunchKernelGGL(zpsi_copy, dim3(512, 4), dim3(256), 0, s, zc);
+    // ring, step 1: A -> C
+    set_io(p, A, C);
+    hipLaunchKernelGGL(h3_pieces<T>, dim3(P2.total), dim3(256), 0, s, p, P2);
+    hipLaunchKernelGGL(e3_pieces<T>, dim3(P2.total), dim3(256), 0, s, p, P2, sv, c1, (T*)nullptr);
+    if ((rc = check_launch("yee3d two-step pass"))) return rc;
+    if ((rc = sample(mpc, s_))) return rc;
+    // ring, step 2: C -> B
+    set_io(p, C, B);
+    hipLaunchKernelGGL(h3_pieces<T>, dim3(P2.total), dim3(256), 0, s, p, P2);
+    hipLaunchKernelGGL(e3_pieces<T>, dim3(P2.total), dim3(256), 0, s, p, P2, sv, c2, (T*)nullptr);
+    if ((rc = check_launch("yee3d two-step ring"))) return rc;
+    in_alt = !in_alt;
+    if ((rc = sample(in_alt ? mpa : mp, s_ + 1))) return rc;
+  }
+  if (s_ < k) {   // odd count: the last step in place, two-pass
+    void* const* A = in_alt ? B0 : A0;
+    set_io(p, A, A);
+    dim3 blk(BZ, BYY, 1);
+    dim3 grd((nz + BZ - 1) / BZ, (ny + BYY - 1) / BYY, nx);
+    const T c = sv.kind ? static_cast<T>(coef[s_]) : T(0);
+    hipLaunchKernelGGL((h3_kernel<T, false>), grd, blk, 0, s, p);
+    hipLaunchKernelGGL((e3_kernel<T, false>), grd, blk, 0, s, p, sv, c, (T*)nullptr);
+    if ((rc = check_launch("yee3d step"))) return rc;
+    if ((rc = sample(in_alt ? mpa : mp, s_))) return rc;
+  }
+  if (in_alt) {
+    for (int q6 = 0; q6 < 6; ++q6)
+      if (hipMemcpyAsync(prim[q6], d->alt[q6], bytes[q6], hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return fail(MT_EHIP, "yee3d two-step: copy-back failed");
+  }
+  return MT_OK;
+}
+
 template <typename T>
 int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const MonitorPlan& mp,
            int step0, int k, hipStream_t s) {
@@ -830,8 +1492,18 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
   if (src && src->kind) sv = *src;
   const Box3 box = interior_box(d);
   const bool small = (long long)d->ny * d->nz * (long long)sizeof(T) < (1LL << 31);  // 32-bit in-plane offsets
-  const bool split = d->variant >= 1 && !media && !d->absorber && small && box.I1 > box.I0 && box.J1 > box.J0 &&
+  const bool split = d->variant >= 1 && d->variant <= 3 && !media && !d->absorber && small && box.I1 > box.I0 && box.J1 > box.J0 &&
                      box.K1 > box.K0;
+  // two steps per pass (variants 4, 5: tile rows 10 / 8 on the psi-free
+  // columns, 8 on the z-edge columns), FP32 CPML runs
+  // without media; runs it does not fit take the two-pass path
+  if constexpr (sizeof(T) == 4) {
+    if (d->variant >= 4 && !media && !d->absorber && small) {
+      const int rc = d->variant == 4 ? run3_tb2<T, 10, 8>(d, p, sv, coef, mp, step0, k, s)
+                                     : run3_tb2<T, 8, 8>(d, p, sv, coef, mp, step0, k, s);
+      if (rc != 1) return rc;
+    }
+  }
   // tile rows: FP32 6 / 10 / 4, FP64 4 (the LDS rings must fit 64 KB)
   if (split) {
     if constexpr (sizeof(T) == 4) {
@@ -918,10 +1590,14 @@ extern "C" int mt_yee3d_run(const mt_yee3d* p, const mt_source3d* src, const dou
     if (src->kind == 2 && !src->w) return fail(MT_EINVAL, "mask source without weights");
     if (!coef && k > 0) return fail(MT_EINVAL, "source without coefficients");
   }
-  if (p->variant < 0 || p->variant > 3) return fail(MT_EINVAL, "3D variant must be 0..3");
+  if (p->variant < 0 || p->variant > 5) return fail(MT_EINVAL, "3D variant must be 0..5");
   if (p->variant >= 1 && p->variant <= 3) {
     for (int q = 0; q < 6; ++q)
       if (!p->alt[q]) return fail(MT_EINVAL, "split 3D variant needs the 6 alt field buffers");
+  }
+  if (p->variant >= 4) {
+    for (int q = 0; q < 16; ++q)
+      if (!p->alt[q]) return fail(MT_EINVAL, "two-step 3D variant needs alt[0..15] (two field sets, four z-psi)");
   }
   bool any_damp = false, all_damp = true;
   for (int q = 0; q < 12; ++q) {

@@ -115,7 +115,10 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=N
     summing (one sequential sum across calls); raw: return the raw
     (monitor, kind, buffer) outputs instead of the reshaped ones.  variant:
     0 two-pass, 1-3 split step: one-pass interior kernel (tile rows 6, 14,
-    4) + shell cells (default: DEFAULT_VARIANT)."""
+    4) + shell cells, 4, 5 two steps per pass over the slab inside the x and
+    y CPML (all of z; FP32 runs without media or tape whose monitors read x
+    planes, else two-pass) + the x and y CPML cells (default:
+    DEFAULT_VARIANT)."""
     from .engine2d import finish_outputs, monitor_structs
     L = E.lib()
     absorber = c.cpml is None   # stepper_3d.py:328-361: the matched-absorber path
@@ -154,8 +157,12 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=N
             raise NotImplementedError("3D media (sigma, poles) are not supported by the lean stepper")
         v = 0
     alt = []
-    if 1 <= v <= 3 and n_steps > 0:
-        alt = [torch.empty_like(fields[f]) for f in COMPS]   # ping-pong field set; psi stay in place
+    if 1 <= v <= 5 and n_steps > 0:
+        # ping-pong field set (psi stay in place); the two-step variants 4, 5
+        # add a third set for the ring's first step
+        alt = [torch.empty_like(fields[f]) for f in COMPS * (2 if v >= 4 else 1)]
+        if v >= 4 and not absorber:   # scratch copies of the four z-CPML memory arrays
+            alt += [torch.empty_like(fields[f]) for f in ("psi_Ey_z", "psi_Ex_z", "psi_Hy_z", "psi_Hx_z")]
         for q, t in enumerate(alt):
             d.alt[q] = t.data_ptr()
     d.variant = v if n_steps > 0 else 0

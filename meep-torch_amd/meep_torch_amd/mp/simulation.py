@@ -36,7 +36,7 @@ from . import geometry as G
 from . import run_control as RC
 from .sim3d import Sim3D
 from .sources import AUTOMATIC, EigenModeSource, port_line, source_span
-from .symmetry import Folding
+from .symmetry import Folding, check_symmetry
 from .vec import Vector3, pol_for
 
 _DEFAULTS = dict(eps_averaging=True, symmetries=None, force_complex_fields=False, m=0, k_point=False,
@@ -132,6 +132,8 @@ class Simulation(Sim3D):
             raise NotImplementedError(f"the Simulation facade drives 2D and 3D cells, not dimensions={dimensions}")
         self.dimensions = 2
         self.symmetries = list(options.get("symmetries") or [])   # meep-mutable until the structure is built
+        for sym in self.symmetries:
+            check_symmetry(sym)
         self.dtype = dtype
         self.cell_size = Vector3(*cell_size)
         self.resolution = resolution

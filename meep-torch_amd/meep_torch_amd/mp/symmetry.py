@@ -45,6 +45,12 @@ class Identity(Symmetry):
 _HINT = "Mirror(X/Y, phase=+-1), Rotate2(Z, phase=+-1) and Rotate4(Z, phase=+-1) in 2D cells"
 
 
+def check_symmetry(s):
+    """A symmetry the facade can fold: an instance of the classes above."""
+    if not isinstance(s, Symmetry):
+        raise NotImplementedError(f"symmetry {s!r} is not supported: {_HINT}")
+
+
 class Folding:
     """The cut state of one run: cx / cy = full-cell columns / rows below the
     cut (0 = no cut on that axis), px / py = the mirror phases, rot2 / rot4 =
@@ -56,6 +62,7 @@ class Folding:
         self.rot2 = self.rot4 = 0
         mirrors, rot2, rot4 = {}, 0, 0
         for s in symmetries or ():
+            check_symmetry(s)
             real = s.phase in (1 + 0j, -1 + 0j)
             if isinstance(s, Identity):
                 continue

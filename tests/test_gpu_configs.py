@@ -144,9 +144,10 @@ def _c3_oracle_cached(dtype):
     return _ORACLE[dtype]
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 5])
 def test_c3_geometry_matches_oracle(variant):
-    """variant 0: two passes per step; 1: split step (interior kernel + shell)."""
+    """variant 0: two passes per step; 1: split step (interior kernel + shell);
+    5: two steps per pass (the C3 bench's path)."""
     P, fr, _, _ = _c3_setup()
     plane = C3N - 2 * (P["npml"] + 1)
     assert plane * plane * 8 * ((len(fr) + 3) // 4) >= 1 << 18   # the grouped flush path
