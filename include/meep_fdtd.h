@@ -225,7 +225,7 @@ typedef struct {
   int variant, pad2_;
   void* alt[18];
   /* Media of the functional stepper (stepper_3d.py:217-266,295-301); two-pass
-   * variant 0 with lean = 0 only.  damp: all NULL (sigma = 0, no poles) or 12
+   * variant 0 (other variants take it).  damp: all NULL (sigma = 0, no poles) or 12
    * device arrays, (d1, d2) of ex, ey, ez on the E component grids then of hx,
    * hy, hz on the H component grids, from the Yee-averaged sigma_d / sigma_b:
    * d1 = (1 - s dt/2) / (1 + s dt/2), d2 = dt / (1 + s dt/2), and
@@ -235,7 +235,11 @@ typedef struct {
    * (n_poles, component grid):
    *   P' = c1 P + c2 P_prev + c3 E(n),  P_prev' = P,
    *   E = (d1 (E eps_c + sum P) + d2 curlH - sum P') / eps_c,
-   * the sums taken in pole order. */
+   * the sums taken in pole order.  With lean = 1 (stepper_3d_lean.py:143-165,
+   * 207-231) damp stays NULL (the lean form has no conductivity) and poles
+   * take the lean form:
+   *   P' = fma(c3, E(n), fma(c1, P, c2 P_prev)),
+   *   E = E + ((curlH dt + sum P) - sum P') * (1 / eps_c). */
   const void* damp[12];
   int n_poles, pad3_;
   const void *c1, *c2;

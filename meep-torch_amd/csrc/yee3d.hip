@@ -175,8 +175,25 @@ __device__ __forceinline__ T e_update(const P& p, T E, T eps_c, T curl) {
 //   E = (d1 (E eps_c + sum P) + d2 curl - sum P') / eps_c
 // (without poles E = (d1 E eps_c + d2 curl) / eps_c); q is the cell's index
 // in component a's grid of nq cells.
+// Lean form (stepper_3d_lean.py:207-231, no conductivity): the in-place
+// recurrence P' = fma(c3, E, fma(c1, P, c2 P_prev)) (torch's addcmul_ rounds
+// each multiply-add once), then E + ((curl dt + sum P) - sum P') / eps_c as
+// a product with the reciprocal.
 template <typename T>
 __device__ __forceinline__ T e_media(const K3<T>& p, int a, long long q, long long nq, T E, T eps_c, T curl) {
+  if (p.lean) {
+    T sc = T(0), sn = T(0);
+    for (int n = 0; n < p.np; ++n) {
+      const long long o = (long long)n * nq + q;
+      const T pc = p.Pc[a][o], pp = p.Pp[a][o];
+      const T pn = fma(p.c3[a][o], E, fma(p.c1[n], pc, p.c2[n] * pp));
+      p.Pp[a][o] = pc;
+      p.Pc[a][o] = pn;
+      sc = n ? sc + pc : pc;
+      sn = n ? sn + pn : pn;
+    }
+    return E + (((curl * p.dt) + sc) - sn) * (T(1) / eps_c);
+  }
   const T d1 = p.dmp[2 * a][q], d2 = p.dmp[2 * a + 1][q];
   if (p.np == 0) return (d1 * E * eps_c + d2 * curl) / eps_c;
   T sc = T(0), sn = T(0);
@@ -1479,8 +1496,9 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
   p.Hz = p.oHz = static_cast<T*>(d->Hz);
   for (int q = 0; q < 12; ++q) p.psi[q] = static_cast<T*>(d->psi[q]);
   const bool media = d->damp[0] != nullptr;
+  const bool lean_poles = d->lean && d->n_poles > 0;   // poles of the lean form (no damping arrays)
   for (int q = 0; q < 12; ++q) p.dmp[q] = static_cast<const T*>(d->damp[q]);
-  p.np = media ? d->n_poles : 0;
+  p.np = (media || lean_poles) ? d->n_poles : 0;
   p.c1 = static_cast<const T*>(d->c1);
   p.c2 = static_cast<const T*>(d->c2);
   for (int a = 0; a < 3; ++a) {
@@ -1492,7 +1510,8 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
   if (src && src->kind) sv = *src;
   const Box3 box = interior_box(d);
   const bool small = (long long)d->ny * d->nz * (long long)sizeof(T) < (1LL << 31);  // 32-bit in-plane offsets
-  const bool split = d->variant >= 1 && d->variant <= 3 && !media && !d->absorber && small && box.I1 > box.I0 && box.J1 > box.J0 &&
+  const bool split = d->variant >= 1 && d->variant <= 3 && !media && !lean_poles && !d->absorber && small &&
+                     box.I1 > box.I0 && box.J1 > box.J0 &&
                      box.K1 > box.K0;
   // two steps per pass (variants 4, 5: tile rows 10 / 8 on the psi-free
   // columns, 8 on the z-edge columns), FP32 CPML runs
@@ -1527,6 +1546,9 @@ int run3_t(const mt_yee3d* d, const mt_source3d* src, const double* coef, const 
       hipLaunchKernelGGL((e3_kernel<T, true, true>), grd, blk, 0, s, p, sv, c, slot);
     } else if (media) {
       hipLaunchKernelGGL((h3_kernel<T, true>), grd, blk, 0, s, p);
+      hipLaunchKernelGGL((e3_kernel<T, true>), grd, blk, 0, s, p, sv, c, slot);
+    } else if (lean_poles) {   // H without damping, E through the lean pole form
+      hipLaunchKernelGGL((h3_kernel<T, false>), grd, blk, 0, s, p);
       hipLaunchKernelGGL((e3_kernel<T, true>), grd, blk, 0, s, p, sv, c, slot);
     } else {
       hipLaunchKernelGGL((h3_kernel<T, false>), grd, blk, 0, s, p);
@@ -1606,16 +1628,15 @@ extern "C" int mt_yee3d_run(const mt_yee3d* p, const mt_source3d* src, const dou
   }
   if (any_damp && !all_damp) return fail(MT_EINVAL, "3D media need all 12 damping arrays");
   if (p->n_poles < 0 || p->n_poles > 64) return fail(MT_EINVAL, "3D n_poles must be 0..64");
-  if (p->n_poles > 0 && !any_damp) return fail(MT_EINVAL, "3D poles need the damping arrays (d1 = 1, d2 = dt)");
+  if (p->n_poles > 0 && !any_damp && !p->lean)
+    return fail(MT_EINVAL, "3D poles need the damping arrays (d1 = 1, d2 = dt)");
   if (p->absorber && (!any_damp || p->lean))
     return fail(MT_EINVAL, "absorber run needs the 12 damping arrays and the functional stepper");
-  if (any_damp) {
-    if (p->lean) return fail(MT_EINVAL, "3D media are not supported by the lean stepper");
-    if (p->n_poles > 0) {
-      if (!p->c1 || !p->c2) return fail(MT_EINVAL, "3D poles without c1 / c2");
-      for (int a = 0; a < 3; ++a)
-        if (!p->c3[a] || !p->P_cur[a] || !p->P_prev[a]) return fail(MT_EINVAL, "3D poles: null c3 or P stack");
-    }
+  if (any_damp && p->lean) return fail(MT_EINVAL, "3D conductivity is not supported by the lean stepper");
+  if (p->n_poles > 0) {
+    if (!p->c1 || !p->c2) return fail(MT_EINVAL, "3D poles without c1 / c2");
+    for (int a = 0; a < 3; ++a)
+      if (!p->c3[a] || !p->P_cur[a] || !p->P_prev[a]) return fail(MT_EINVAL, "3D poles: null c3 or P stack");
   }
   MonitorPlan mp{mons, n_mons, stage, stage_bytes, stage_steps, p->dtype};
   if (n_mons && (rc = monitors_validate(mp))) return rc;

@@ -69,10 +69,11 @@ def _no_shell(n):
     return E.PmlAxis(None, None, n, 0, n)
 
 
-def _media(d, c, fields, dtype, device):
+def _media(d, c, fields, dtype, device, damping=True):
     """Fill the descriptor's media fields from Coeffs3D (the (d1, d2) pairs of
-    c.damp and the pole scalars / c3 stacks of c.pols, stepper_3d.py:217-266);
-    returns the device tensors the call must keep alive."""
+    c.damp and the pole scalars / c3 stacks of c.pols, stepper_3d.py:217-266;
+    damping=False: the poles alone, for the lean form, stepper_3d_lean.py:
+    143-165); returns the device tensors the call must keep alive."""
     keep = []
 
     def dev(t):
@@ -82,7 +83,7 @@ def _media(d, c, fields, dtype, device):
 
     shapes = {"ex": fields["Ex"].shape, "ey": fields["Ey"].shape, "ez": fields["Ez"].shape,
               "hx": fields["Hx"].shape, "hy": fields["Hy"].shape, "hz": fields["Hz"].shape}
-    for q, key in enumerate(("ex", "ey", "ez", "hx", "hy", "hz")):
+    for q, key in enumerate(("ex", "ey", "ez", "hx", "hy", "hz") if damping else ()):
         for r, t in enumerate(c.damp[key]):
             if tuple(t.shape) != tuple(shapes[key]):
                 raise ValueError(f"damping pair {key} of shape {tuple(t.shape)} != {tuple(shapes[key])}")
@@ -151,10 +152,10 @@ def run_fields(fields, c, n_steps, source, monitors, lean=False, step0=0, tape=N
             d.psi[q] = fields[name].data_ptr()
     d.tape = tape.data_ptr() if tape is not None else None
     v = DEFAULT_VARIANT if variant is None else int(variant)
-    media = _media(d, c, fields, dtype, device) if (has_media(c) or absorber) else []
+    if lean and bool(getattr(c.damp, "active", False)):
+        raise NotImplementedError("3D conductivity (sigma) is not supported by the lean stepper")
+    media = _media(d, c, fields, dtype, device, damping=not lean) if (has_media(c) or absorber) else []
     if media:
-        if lean:
-            raise NotImplementedError("3D media (sigma, poles) are not supported by the lean stepper")
         v = 0
     alt = []
     if 1 <= v <= 5 and n_steps > 0:

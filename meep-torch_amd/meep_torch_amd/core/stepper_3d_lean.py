@@ -1,7 +1,7 @@
 """Lean in-place forward 3D stepper (meep_torch/core/stepper_3d_lean.py).
 
 Lean3D keeps the reference attributes: F (fields, updated in place), psi as
-(lo, hi) shell views, inv_eps.  The E update and the source sign follow the
+(lo, hi) shell views, inv_eps, and with ADE poles pol / P.  The E update and the source sign follow the
 lean form (E += curl * inv_eps * dt; sources subtract), which rounds
 differently from step_3d.  ``compile`` is accepted and ignored: the step is
 already one pair of fused HIP kernels."""
@@ -32,14 +32,18 @@ class Lean3D:
     F: dict = field(default_factory=dict)
     psi: dict = field(default_factory=dict)
     full_psi: dict = field(default_factory=dict)
+    # ADE poles (stepper_3d_lean.py:59-64): pol[key] = (c1, c2, c3), P[key] =
+    # [P(n+1), P(n)] (the state's P?_cur / P?_prev stacks, advanced in place)
+    pol: dict = field(default_factory=dict)
+    P: dict = field(default_factory=dict)
 
 
 def build_lean(nx, ny, nz, dx, eps, npml, poles=(), pole_masks=None, dtype=torch.float32,
                device="cuda") -> Lean3D:
-    """stepper_3d_lean.py:90-165."""
-    if len(poles):
-        raise NotImplementedError("lean stepper with ADE poles is not supported by the HIP stepper")
-    c = build_3d_coeffs(nx, ny, nz, dx, eps=eps, npml=npml, dtype=dtype, device=device)
+    """stepper_3d_lean.py:90-165; poles with cell-centred masks as in
+    build_3d_coeffs (the same c1, c2 scalars and Yee-averaged c3 stacks)."""
+    c = build_3d_coeffs(nx, ny, nz, dx, eps=eps, npml=npml, poles=poles, pole_masks=pole_masks, dtype=dtype,
+                        device=device)
     e = c.eps
     inv = {k: 1.0 / (0.5 * (e.narrow(ax, 0, e.shape[ax] - 1) + e.narrow(ax, 1, e.shape[ax] - 1)))
            for k, ax in (("x", 0), ("y", 1), ("z", 2))}
@@ -61,8 +65,15 @@ def build_lean(nx, ny, nz, dx, eps, npml, poles=(), pole_masks=None, dtype=torch
         lo, hi = c.shells[key]
         m = t.shape[ax]
         psi[n[4:]] = (t.narrow(ax, 0, max(lo, 1)), t.narrow(ax, min(hi, m - 1), m - min(hi, m - 1)))
-    return Lean3D(dt=c.dt, dx=dx, nx=nx, ny=ny, nz=nz, npml=npml, inv_eps=inv, coeffs=c,
+    lean = Lean3D(dt=c.dt, dx=dx, nx=nx, ny=ny, nz=nz, npml=npml, inv_eps=inv, coeffs=c,
                   F=F, psi=psi, full_psi=full)
+    if c.pols is not None:
+        n = len(poles)
+        for key in "xyz":
+            lean.pol[key] = c.pols[key]
+            shape = (n,) + tuple(F["E" + key].shape)
+            lean.P[key] = [z(*shape), z(*shape)]
+    return lean
 
 
 class _StateView:
@@ -83,6 +94,8 @@ class _OneStep:
 def _fields(c: Lean3D):
     d = dict(c.F)
     d.update(c.full_psi)
+    for key, (p_new, p_old) in c.P.items():
+        d[f"P{key}_cur"], d[f"P{key}_prev"] = p_new, p_old
     return d
 
 

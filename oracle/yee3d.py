@@ -68,6 +68,7 @@ def _media3d(c, nx, ny, nz, dtype, sigma_d, sigma_b, poles, pole_masks, sig_boun
                  "Hx": pair(avg(avg(sh, 1), 2)), "Hy": pair(avg(avg(sh, 0), 2)),
                  "Hz": pair(avg(avg(sh, 0), 1))}
     c["pols"] = None
+    c["sigma"] = sigma_d is not None or sigma_b is not None
     if len(poles):
         rec = [pole_recurrence(p, dt) for p in poles]
         n = len(poles)
@@ -196,6 +197,10 @@ def step3d(st, c, amp=0.0, src=None, lean=False):
     PEC tangential trim (stepper_3d.py:363-437).  E update:
       functional form  E = (E*eps_c + dt*curl) / eps_c   (d1 = 1, d2 = dt)
       lean form        E = E + (curl * inv_eps) * dt      (stepper_3d_lean.py:260)
+      lean with poles  E = E + ((curl*dt + sum P(n)) - sum P(n+1)) * inv_eps,
+                       P(n+1) = fma(c3, E, fma(c1, P(n), c2 P(n-1))) (torch's
+                       addcmul_ rounds each multiply-add once)
+                       (stepper_3d_lean.py:207-231; H takes the (1, dt) pair)
     Point source (comp, i, j, k): functional ADDs (dt*amp) * (1/eps_c) at the
     node (stepper_3d.py:475-481); lean SUBTRACTS (dt*amp) * inv_eps
     (stepper_3d_lean.py:294-296).  Plane (comp, mask): E -= (dt*amp)*mask/eps_c
@@ -203,14 +208,17 @@ def step3d(st, c, amp=0.0, src=None, lean=False):
     s = dict(st)
     dt = c["dt"]
     dm, pols = c.get("damp"), c.get("pols")
-    if (dm is not None or pols is not None) and lean:
-        raise ValueError("the lean form has no media")
+    if lean and (c.get("sigma") or c.get("absorber")):
+        raise ValueError("the lean form has no conductivity and no absorber")
     pn = {}
     if pols is not None:       # stepper_3d.py:318-329: P(n+1) from the incoming E(n)
         for comp in ("Ex", "Ey", "Ez"):
             c1, c2, c3 = pols[comp]
             a = comp[1]
-            pn[comp] = c1 * st[f"P{a}_cur"] + c2 * st[f"P{a}_prev"] + c3 * st[comp]
+            if lean:   # stepper_3d_lean.py:207-216: in-place mul_ / addcmul_, which fuse each multiply-add
+                pn[comp] = (c2 * st[f"P{a}_prev"]).addcmul_(c1, st[f"P{a}_cur"]).addcmul_(c3, st[comp])
+            else:
+                pn[comp] = c1 * st[f"P{a}_cur"] + c2 * st[f"P{a}_prev"] + c3 * st[comp]
             s[f"P{a}_cur"], s[f"P{a}_prev"] = pn[comp], st[f"P{a}_cur"]
     if c.get("absorber"):
         return _absorber_step(st, s, c, pn, amp, src)
@@ -230,7 +238,10 @@ def step3d(st, c, amp=0.0, src=None, lean=False):
         tb = _trim(_ce(s, c, pb), fb)
         curl = _pad(ta - tb, axes)
         k = "xyz".index(comp[1])
-        if lean:
+        if lean and pols is not None:   # stepper_3d_lean.py:207-231 (_pole_step, _pole_e_lean)
+            a = comp[1]
+            s[comp] = st[comp] + (((curl * dt) + st[f"P{a}_cur"].sum(0)) - pn[comp].sum(0)) * c["inv_eps"][k]
+        elif lean:
             s[comp] = st[comp] + curl * c["inv_eps"][k] * dt
         elif pols is not None:  # stepper_3d.py:295-301 (_e_pole)
             e = c["eps_c"][k]

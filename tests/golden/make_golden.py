@@ -728,6 +728,35 @@ def case_media_3d():
         save(f"media_3d_{tag}", **out)
 
 
+def case_lean_poles_3d():
+    """The lean in-place stepper with two ADE poles on masks
+    (stepper_3d_lean.py:143-165,207-231): fields, the P(n+1) / P(n)
+    buffers and a flux spectrum after 120 FP32 steps."""
+    from meep_torch.core.monitors_3d import FluxMonitor3D, PointSource3D
+    from meep_torch.core.sources import GaussianPulse
+    from meep_torch.core.stepper_3d_lean import build_lean, run_3d_lean
+    nx, ny, nz, npml, n, dx = 22, 20, 18, 4, 120, 0.05
+    r = torch.tensor(splitmix_uniform(nx * ny * nz, 33).reshape(nx, ny, nz))
+    inner = torch.zeros(nx, ny, nz, dtype=torch.float64)
+    inner[npml + 2:nx - npml - 2, npml + 2:ny - npml - 2, npml + 2:nz - npml - 2] = 1.0
+    half = (r > 0.5).double() * inner
+    dtype = torch.float32
+    cv = {"eps": (1.0 + 2.0 * r * inner).to(dtype), "m1": half.to(dtype), "m2": (inner - half).to(dtype)}
+    c = build_lean(nx, ny, nz, dx, cv["eps"], npml, poles=POLES2, pole_masks=[cv["m1"], cv["m2"]],
+                   dtype=dtype, device="cpu")
+    src = PointSource3D(c.dt, "Ez", 7, ny // 2, nz // 2, GaussianPulse(1.0, 2.0, cutoff=2.0))
+    mon = FluxMonitor3D(c.dt, [0.8, 1.0, 1.2], normal="x", idx=15, ap=npml + 1, dtype=dtype)
+    view, (acc,) = run_3d_lean(c, n, source=src, monitors=[mon])
+    out = dict(cv)
+    out["amp"] = np.array([src.amplitude(t) for t in range(n)])
+    for f in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        out[f] = getattr(view, f)
+    for key in ("x", "y", "z"):
+        out[f"P{key}_new"], out[f"P{key}_old"] = c.P[key][0], c.P[key][1]
+    out["acc"] = acc
+    save("lean_poles_3d_f32", **out)
+
+
 def case_absorber_3d():
     from meep_torch.core.monitors_3d import FluxMonitor3D, PointSource3D, run_3d
     from meep_torch.core.sources import GaussianPulse
@@ -1002,7 +1031,7 @@ def main():
     which = sys.argv[1:] or ["pml", "tm", "te", "c1", "3d", "grad_tm", "grad_3d",
                              "filters", "golden_plain", "media_tm", "media_te", "media_3d",
                              "absorber_3d", "tfsf_tm", "sim3d", "raster", "shapes", "multisrc", "sim_absorber",
-                             "mag_tm", "tensor_te", "fold_2d", "sim_sym"]
+                             "mag_tm", "tensor_te", "fold_2d", "sim_sym", "lean_poles_3d"]
     t_all = time.perf_counter()
     timing = {}
     try:
@@ -1018,6 +1047,7 @@ def main():
          "filters": case_filters, "golden_plain": case_golden_plain,
          "media_tm": case_media_tm, "media_te": case_media_te, "media_3d": case_media_3d,
          "absorber_3d": case_absorber_3d,
+         "lean_poles_3d": case_lean_poles_3d,
          "tfsf_tm": case_tfsf_tm, "sim3d": case_sim3d, "raster": case_raster,
          "shapes": case_shapes, "multisrc": case_multisrc, "sim_absorber": case_sim_absorber,
          "mag_tm": case_mag_tm, "tensor_te": case_tensor_te, "fold_2d": case_fold_2d, "sim_sym": case_sim_sym}[w]()

@@ -148,6 +148,36 @@ def test_3d_media_bitexact(tag, dtype):
     _eq(yee3d.flux_plane(acc[0], 0.05), g["flux"], "flux")
 
 
+def test_3d_lean_poles_bitexact():
+    """The lean in-place stepper with two ADE poles on masks (stepper_3d_lean.py:
+    143-165,207-231), reproduced bit for bit from the reference's run: the pole
+    advances from E(n) before the E update and E = E + ((curl dt + sum P(n))
+    - sum P(n+1)) inv_eps, the recurrence taking the reference's in-place
+    mul_ / addcmul_ (each multiply-add rounded once).  The functional pole
+    form (d1 (E eps + sum P) + d2 curl - sum P') / eps, the lean form without
+    the pole sums, or an unfused recurrence fails here."""
+    g = golden("lean_poles_3d_f32")
+    nx, ny, nz, npml, n = 22, 20, 18, 4, 120
+    dtype = torch.float32
+    c = yee3d.coeffs3d(nx, ny, nz, 0.05, _t(g["eps"]), npml, dtype, poles=POLES2,
+                       pole_masks=[_t(g["m1"]), _t(g["m2"])])
+    mons = [{"type": "flux", "normal": "x", "idx": 15, "freqs": [0.8, 1.0, 1.2], "ap": npml + 1}]
+    s, acc = yee3d.run3d(yee3d.state3d(nx, ny, nz, dtype, n_poles=2), c, n, list(g["amp"]),
+                         ("Ez", 7, ny // 2, nz // 2), mons, lean=True)
+    for f in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        _eq(s[f], g[f], f)
+    for a in "xyz":
+        _eq(s[f"P{a}_cur"], g[f"P{a}_new"], f"P{a} new")
+        _eq(s[f"P{a}_prev"], g[f"P{a}_old"], f"P{a} old")
+    _eq(acc[0], g["acc"], "acc")
+    # the functional pole form differs from the lean one on the same run
+    s2, _ = yee3d.run3d(yee3d.state3d(nx, ny, nz, dtype, n_poles=2), c, 40, list(g["amp"])[:40],
+                        ("Ez", 7, ny // 2, nz // 2), [], lean=False)
+    s3, _ = yee3d.run3d(yee3d.state3d(nx, ny, nz, dtype, n_poles=2), c, 40, list(g["amp"])[:40],
+                        ("Ez", 7, ny // 2, nz // 2), [], lean=True)
+    assert not torch.equal(s2["Ez"], s3["Ez"])
+
+
 MPOLES2 = [(1.0, 0.2, 1.5), (0.7, 0.1, 0.8)]
 MAG_FIELDS = ("Ez", "Dz", "Hx", "Hy", "Bx", "By", "Mx_cur", "Mx_prev", "My_cur", "My_prev", "psi_Ezy", "psi_Hyx")
 
