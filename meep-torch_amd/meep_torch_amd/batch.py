@@ -60,12 +60,25 @@ def _bind(L):
     L._batch_bound = True
 
 
+def _has_media(c):
+    return bool(c.c1.shape[0] or c.has_kerr or c.has_chi2 or c.d1 is not None or c.h1x is not None
+                or c.mc1x is not None)
+
+
 class BatchedTM:
     """B TM problems stepped together.  problems: list of (coeffs, source,
     monitors) with coeffs from build_tm_coeffs (eps_inf is replaced per call).
     Every variant runs n_steps with checkpoints every K steps.  With
     adjoint=False no checkpoints or Ez tapes are kept (forward runs only;
-    spectra are still folded per K-step chunk)."""
+    spectra are still folded per K-step chunk).
+
+    Variants with dispersive, nonlinear or conductive media (ADE poles,
+    chi2 / chi3, sigma_d, sigma_b / mu, magnetic poles) run one after the
+    other through the single-problem path (its two-pass media kernels and,
+    for gradients, its hand-written adjoint with checkpoints every K steps),
+    with the same forward / gradient / apply interface; a batch holding
+    such a variant takes that path for every variant.  Graph capture is for
+    the batched launch only."""
 
     def __init__(self, problems, n_steps, checkpoint_every=200, variant=None, stage_steps=64, adjoint=True,
                  fused_adjoint=True):
@@ -88,22 +101,27 @@ class BatchedTM:
         self.variant = int(variant)
         self.S = int(stage_steps)
         self.adjoint = bool(adjoint)
+        self.graph = None
+        self.adj_graph = None
+        self._adj = None
+        self._plans = {}
+        self._seq = any(_has_media(p[0]) for p in problems)
+        if self._seq:   # media: the single-problem path, variant by variant
+            for coeffs, _, _ in problems:
+                if coeffs.y_mirror_lo or coeffs.x_mirror_lo or coeffs.y_rot2_lo or coeffs.rot4_lo:
+                    raise NotImplementedError("batched run: symmetry cuts are not supported")
+            self.var = [{"coeffs": c, "source": s_, "monitors": list(m),
+                         "eps": c.eps_inf.detach().to(self.dtype).clone().contiguous()} for c, s_, m in problems]
+            return
         self.var = []
         for v, (coeffs, source, monitors) in enumerate(problems):
             self.var.append(self._alloc(coeffs, source, list(monitors)))
         # one descriptor workspace per plan (forward, and recompute with Ez tapes)
         self._ws_bytes = max(1, self.L.mt_tm2d_batch_workspace(self.B, self.code))
         self.ws = {}
-        self._plans = {}
-        self.graph = None
-        self.adj_graph = None
-        self._adj = None
 
     # ------------------------------------------------------------ allocation
     def _alloc(self, coeffs, source, monitors):
-        if (coeffs.c1.shape[0] or coeffs.has_kerr or coeffs.has_chi2 or coeffs.d1 is not None
-                or coeffs.h1x is not None or coeffs.mc1x is not None):
-            raise NotImplementedError("batched run: dispersive, nonlinear or conductive media are not supported")
         if coeffs.y_mirror_lo or coeffs.x_mirror_lo or coeffs.y_rot2_lo or coeffs.rot4_lo:
             raise NotImplementedError("batched run: symmetry cuts are not supported")
         nx, ny = coeffs.eps_inf.shape
@@ -189,6 +207,23 @@ class BatchedTM:
         except Exception:
             pass
 
+    # ------------------------------------------------ media, variant by variant
+    def _seq_run(self, V, eps):
+        """One media variant through run_tm (differentiable in eps when eps
+        requires grad and grad mode is on); returns its accumulators."""
+        from .core.grid import Grid
+        from .core.run import run_tm
+        from .core.stepper_tm import init_tm_state
+        c = V["coeffs"]._replace(eps_inf=eps)
+        nx, ny = eps.shape
+        g = Grid(nx=nx, ny=ny, dx=float(V["coeffs"].dx), dtype=self.dtype, device=self.device)
+        mc = V["coeffs"].mc1x
+        st = init_tm_state(g, n_poles=int(V["coeffs"].c1.shape[0]), y_pmc=bool(V["coeffs"].y_pmc),
+                           n_mpoles=int(mc.shape[0]) if mc is not None else 0)
+        _, accs = run_tm(st, c, self.n_steps, source=V["source"], monitors=V["monitors"],
+                         checkpoint_every=self.K if self.adjoint else 0)
+        return list(accs)
+
     # -------------------------------------------------------------- forward
     def set_eps(self, eps_list):
         with torch.no_grad():
@@ -221,6 +256,8 @@ class BatchedTM:
 
     def capture(self):
         """Record the forward run in a HIP graph (replayed by forward())."""
+        if self._seq:
+            raise NotImplementedError("graph capture covers the batched launch, not variants with media")
         self._plan(False)
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
@@ -238,6 +275,9 @@ class BatchedTM:
         captured graph the run replays it unless use_graph is False."""
         if eps_list is not None:
             self.set_eps(eps_list)
+        if self._seq:
+            with torch.no_grad():
+                return [self._seq_run(V, V["eps"]) for V in self.var]
         if self.graph is not None and use_graph:
             self.graph.replay()
         else:
@@ -336,7 +376,25 @@ class BatchedTM:
     def gradient(self, gaccs):
         """dL/deps per variant given dL/dacc (list per variant of per-monitor
         cotangents, None where unused).  Uses the checkpoints of the last
-        forward; replays the captured adjoint graph when there is one."""
+        forward; replays the captured adjoint graph when there is one.
+        Media variants run their forward again through the single-problem
+        adjoint (run_tm's checkpointed reverse sweep)."""
+        if self._seq:
+            if not self.adjoint:
+                raise RuntimeError("BatchedTM was built with adjoint=False (no checkpoints or tapes)")
+            out = []
+            for V, gv in zip(self.var, gaccs):
+                e = V["eps"].clone().requires_grad_(True)
+                with torch.enable_grad():
+                    accs = self._seq_run(V, e)
+                    pairs = [(a, g_) for a, g_ in zip(accs, gv) if g_ is not None]
+                    if not pairs:
+                        out.append(torch.zeros_like(e))
+                        continue
+                    (ge,) = torch.autograd.grad([a for a, _ in pairs],
+                                                [e], [g_.reshape(a.shape).to(a.dtype) for a, g_ in pairs])
+                out.append(ge.detach())
+            return out
         adj = self._adj_setup()
         with torch.no_grad():
             for P, gv in zip(adj["per"], gaccs):
@@ -356,6 +414,8 @@ class BatchedTM:
         the Ez tapes, batched adjoint steps) in a HIP graph that gradient()
         replays; the cotangents are copied into fixed buffers before each
         replay."""
+        if self._seq:
+            raise NotImplementedError("graph capture covers the batched launch, not variants with media")
         self._adj_setup()
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
@@ -370,6 +430,9 @@ class BatchedTM:
     def apply(self, eps_list):
         """Differentiable forward: returns [[acc per monitor] per variant];
         backward gives dL/deps for each variant's eps tensor."""
+        if self._seq:
+            self.set_eps(eps_list)
+            return [self._seq_run(V, e) for V, e in zip(self.var, eps_list)]
         flat = _BatchFn.apply(self, *eps_list)
         res, i = [], 0
         for V in self.var:

@@ -187,7 +187,8 @@ def test_conductive_slab_vs_meep_on_hip():
 
 def test_media_refuse_gradients_and_batches():
     """TE gradients through chi2/chi3 (the Jacobi passes couple Ex and Ey
-    through four-point averages) and batched media runs raise."""
+    through four-point averages) raise; so does graph capture of a batch
+    whose variants carry media (they run one by one)."""
     from meep_torch_amd.batch import BatchedTM
     from meep_torch_amd.core.coeffs import build_te_coeffs, build_tm_coeffs
     from meep_torch_amd.core.grid import Grid
@@ -199,8 +200,8 @@ def test_media_refuse_gradients_and_batches():
     with pytest.raises(NotImplementedError, match="nonlinear"):
         run_te(init_te_state(g), m, 5)
     m2 = build_tm_coeffs(g, eps_inf=2.0, npml=4, poles=[(1.0, 0.1, 1.0)])
-    with pytest.raises(NotImplementedError):
-        BatchedTM([(m2, None, [])], 10)
+    with pytest.raises(NotImplementedError, match="graph"):
+        BatchedTM([(m2, None, [])], 10).capture()
 
 
 def _te_media(dtype):
