@@ -76,12 +76,18 @@ def test_magnetic_poles_match_reference(case, tag, dtype, chunk):
     assert float(st.My_cur.abs().max()) > 0
 
 
-def test_magnetic_poles_refuse_gradients_and_batches():
+def test_magnetic_poles_refuse_gradients_and_batch_gradients():
     from meep_torch_amd.batch import BatchedTM
     from meep_torch_amd.core.run import run_tm
     g, m, src, mon, st = _setup("magh", "f64", torch.float64)
     eps = m.eps_inf.clone().requires_grad_(True)
     with pytest.raises(NotImplementedError, match="magnetic poles"):
         run_tm(st, m._replace(eps_inf=eps), 10, source=src, monitors=[mon], checkpoint_every=5)
-    with pytest.raises(NotImplementedError, match="media"):
-        BatchedTM([(m, None, [mon])], 10, checkpoint_every=0, adjoint=False)
+    # a batch runs the variant forward through the single-problem path (the
+    # spectrum equals run_tm's); its gradient raises like run_tm's
+    bt = BatchedTM([(m, src, [mon])], 10, checkpoint_every=5)
+    (acc,), = bt.forward([m.eps_inf])
+    _, (ref,) = run_tm(_setup("magh", "f64", torch.float64)[4], m, 10, source=src, monitors=[mon])
+    assert torch.equal(acc, ref)
+    with pytest.raises(NotImplementedError, match="magnetic poles"):
+        bt.gradient([[torch.ones_like(acc)]])
