@@ -138,7 +138,9 @@ typedef struct {
   void* alt[8];
   /* optional device tape (two-pass path only): E after step s of a call is
    * written to slot s = tape + s * ((nx-1)*ny + nx*(ny-1)) as [Ex | Ey], for
-   * mt_te2d_adjoint's eps gradient */
+   * mt_te2d_adjoint's eps gradient; under chi2/chi3 (kerr) the slot holds
+   * [Dx - sum Px | Dy - sum Py], from which the adjoint recomputes the
+   * fixed-point iterates */
   void* tape;
   /* optional magnetic (TFSF) current on Hz (nx-1, ny-1) of the B half
    * (stepper_te.py:120-128), as hsrc/hcoef of mt_tm2d */
@@ -401,6 +403,10 @@ typedef struct {
   /* cotangents of the pole stacks Px_cur, Px_prev (npx, nx-1, ny) and
    * Py_cur, Py_prev (npy, nx, ny-1) when the problem has them; else NULL */
   void *aPx_cur, *aPx_prev, *aPy_cur, *aPy_prev;
+  /* chi2/chi3 problems (p->kerr): workspace of 6 * ((nx-1)*ny + nx*(ny-1))
+   * elements for the recomputed fixed-point iterates and their cotangents
+   * (the forward tape then holds rhs = D - sum P, not E); else NULL */
+  void* kerr_ws;
 } mt_te2d_adj;
 
 /* Reverse sweep of k TE steps starting at global step step0 (the gradient of
@@ -410,8 +416,11 @@ typedef struct {
  * of the step runs, with E after the step read from tape slot s (as written
  * by mt_te2d_run).  Accumulates dL/deps_x and dL/deps_y.  Conductivity
  * (d1x, d2x, d1y, d2y, hz1, hz2 of p, all or none) and ADE poles (c1x..c3y
- * of p with the aP* cotangents of a) are transposed with the recurrences;
- * Kerr is refused. */
+ * of p with the aP* cotangents of a) are transposed with the recurrences.
+ * chi2/chi3 (kerr = 1, with a->kerr_ws and the rhs tape): the three Jacobi
+ * passes of stepper_te.py:169-175 are recomputed from the taped rhs and
+ * transposed last to first, their four-point averages scattered back with
+ * the other average (each is the other's transpose). */
 int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const void* tape, const mt_monitor* mons,
                     int n_mons, int step0, int k, void* stream);
 

@@ -106,6 +106,14 @@ struct AdjTe {
   int npx, npy;
   const T *c1x, *c2x, *c3x, *c1y, *c2y, *c3y;
   T *aPxc, *aPxp, *aPyc, *aPyp;
+  // chi2/chi3 fixed point (kerr; the tape then holds rhs = D - sum P): the
+  // recomputed iterates kx[k] (Ex grid) and ky[k] (Ey grid), k = 0..2, the
+  // cotangents kgx / kgy of the iterate being transposed, those of its
+  // four-point averages (kga on the Ex grid, kgb on the Ey grid) and the
+  // rhs cotangents krx / kry
+  int kerr;
+  const T *chi2x, *chi3x, *chi2y, *chi3y;
+  T *kx[3], *ky[3], *kgx, *kgy, *kga, *kgb, *krx, *kry;
 };
 
 // Transpose of one axis's poles at a cell: P' = c1 P + c2 P_prev + c3 E(old)
@@ -164,7 +172,114 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p1(AdjTe<T> p) {
   p.gEx[h] = gx;
 }
 
+// P2 under chi2/chi3, first half: the cotangents of the final iterates
+// (E after the step) go to kgx / kgy for the fixed point's transpose.
 template <typename T>
+__global__ __launch_bounds__(ABX* ABY) void te_adj_p2a(AdjTe<T> p) {
+  const int j = blockIdx.x * ABX + threadIdx.x;
+  const int i = blockIdx.y * ABY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx || j >= ny) return;
+  if (i < nx - 1) {
+    const long long c = (long long)i * ny + j, h = (long long)i * (ny - 1) + j;
+    const T gxm = j >= 1 ? p.gEx[h - 1] : T(0), gx0 = j < ny - 1 ? p.gEx[h] : T(0);
+    p.kgx[c] = p.aEx[c] + (gxm - gx0) / p.dx;
+  }
+  if (j < ny - 1) {
+    const long long c = (long long)i * (ny - 1) + j;
+    const T gym = i >= 1 ? p.gEy[c - (ny - 1)] : T(0), gy0 = i < nx - 1 ? p.gEy[c] : T(0);
+    p.kgy[c] = p.aEy[c] + (gym - gy0) / p.dx;
+  }
+}
+
+// The fixed point's iterates from the taped rhs (stepper_te.py:164-175):
+// k = 0 the first iterate rhs / eps, k = 1, 2 one Jacobi pass from the
+// previous pair, in the forward's arithmetic.
+template <typename T>
+__global__ __launch_bounds__(ABX* ABY) void te_kerr_iter(AdjTe<T> p, const T* tex, const T* tey, int k) {
+  const int j = blockIdx.x * ABX + threadIdx.x;
+  const int i = blockIdx.y * ABY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx || j >= ny) return;
+  if (i < nx - 1) {
+    const long long c = (long long)i * ny + j;
+    if (k == 0) {
+      p.kx[0][c] = tex[c] / p.eps_x[c];
+    } else {
+      const T av = avg4_to_ex(p.ky[k - 1], nx, ny, i, j), e = p.kx[k - 1][c];
+      const T e2 = e * e + av * av;
+      p.kx[k][c] = tex[c] / (p.eps_x[c] + p.chi2x[c] * e + p.chi3x[c] * e2);
+    }
+  }
+  if (j < ny - 1) {
+    const long long c = (long long)i * (ny - 1) + j;
+    if (k == 0) {
+      p.ky[0][c] = tey[c] / p.eps_y[c];
+    } else {
+      const T av = avg4_to_ey(p.kx[k - 1], nx, ny, i, j), e = p.ky[k - 1][c];
+      const T e2 = e * e + av * av;
+      p.ky[k][c] = tey[c] / (p.eps_y[c] + p.chi2y[c] * e + p.chi3y[c] * e2);
+    }
+  }
+}
+
+// Transpose of Jacobi pass k (iterate k+1 from iterate k), pointwise part:
+// E' = rhs / den, den = eps + chi2 E + chi3 (E^2 + av^2), so with g the
+// cotangent of E' and t = -g E' / den: rhs gains g / den, eps gains t, E
+// gains t (chi2 + 2 chi3 E) (into kgx / kgy) and the average av gains
+// t chi3 2 av (kga / kgb, scattered by te_kerr_scatter).
+template <typename T>
+__global__ __launch_bounds__(ABX* ABY) void te_kerr_back(AdjTe<T> p, const T* tex, const T* tey, int k) {
+  const int j = blockIdx.x * ABX + threadIdx.x;
+  const int i = blockIdx.y * ABY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx || j >= ny) return;
+  if (i < nx - 1) {
+    const long long c = (long long)i * ny + j;
+    const T av = avg4_to_ex(p.ky[k], nx, ny, i, j), e = p.kx[k][c];
+    const T x2 = p.chi2x[c], x3 = p.chi3x[c];
+    const T den = p.eps_x[c] + x2 * e + x3 * (e * e + av * av);
+    const T en = tex[c] / den, g = p.kgx[c];
+    const T t = (-g * en) / den;
+    p.krx[c] = (k == 2 ? T(0) : p.krx[c]) + g / den;
+    p.gepsx[c] = p.gepsx[c] + t;
+    p.kgx[c] = t * (x2 + T(2) * x3 * e);
+    p.kga[c] = t * (x3 * (T(2) * av));
+  }
+  if (j < ny - 1) {
+    const long long c = (long long)i * (ny - 1) + j;
+    const T av = avg4_to_ey(p.kx[k], nx, ny, i, j), e = p.ky[k][c];
+    const T x2 = p.chi2y[c], x3 = p.chi3y[c];
+    const T den = p.eps_y[c] + x2 * e + x3 * (e * e + av * av);
+    const T en = tey[c] / den, g = p.kgy[c];
+    const T t = (-g * en) / den;
+    p.kry[c] = (k == 2 ? T(0) : p.kry[c]) + g / den;
+    p.gepsy[c] = p.gepsy[c] + t;
+    p.kgy[c] = t * (x2 + T(2) * x3 * e);
+    p.kgb[c] = t * (x3 * (T(2) * av));
+  }
+}
+
+// ... and the scatter of the averages' cotangents: the Ey-grid average of an
+// Ex-grid field and the Ex-grid average of an Ey-grid field are each other's
+// transposes, so Ex gains avg4_to_ex(kgb) and Ey gains avg4_to_ey(kga).
+template <typename T>
+__global__ __launch_bounds__(ABX* ABY) void te_kerr_scatter(AdjTe<T> p) {
+  const int j = blockIdx.x * ABX + threadIdx.x;
+  const int i = blockIdx.y * ABY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx || j >= ny) return;
+  if (i < nx - 1) {
+    const long long c = (long long)i * ny + j;
+    p.kgx[c] = p.kgx[c] + avg4_to_ex(p.kgb, nx, ny, i, j);
+  }
+  if (j < ny - 1) {
+    const long long c = (long long)i * (ny - 1) + j;
+    p.kgy[c] = p.kgy[c] + avg4_to_ey(p.kga, nx, ny, i, j);
+  }
+}
+
+template <typename T, bool KERR>
 __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, const T* tey) {
   const int j = blockIdx.x * ABX + threadIdx.x;
   const int i = blockIdx.y * ABY + threadIdx.y;
@@ -187,12 +302,19 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
       b = p.ey.b[jj];
       a = p.ey.a[jj];
     }
-    const T exb = aex + (gxm - gx0) / p.dx;
-    p.gepsx[c] = ge + (-exb * ex) / epsx;
-    const T dxb = adx + exb / epsx;
+    T rb;   // cotangent of rhs = Dx' - sum P'
+    if constexpr (KERR) {   // the first iterate rhs / eps_x; tex holds rhs
+      const T g0 = p.kgx[c];
+      rb = p.krx[c] + g0 / epsx;
+      p.gepsx[c] = ge + (-g0 * p.kx[0][c]) / epsx;
+    } else {
+      const T exb = aex + (gxm - gx0) / p.dx;
+      p.gepsx[c] = ge + (-exb * ex) / epsx;
+      rb = exb / epsx;
+    }
+    const T dxb = adx + rb;
     p.aDx[c] = p.d1x ? c1 * dxb : dxb;
-    p.aEx[c] = p.npx ? te_pole_adj(p.npx, p.c1x, p.c2x, p.c3x, p.aPxc, p.aPxp, c, (long long)(nx - 1) * ny,
-                                   exb / epsx)
+    p.aEx[c] = p.npx ? te_pole_adj(p.npx, p.c1x, p.c2x, p.c3x, p.aPxc, p.aPxp, c, (long long)(nx - 1) * ny, rb)
                      : T(0);
     T gd = T(0);
     if (live) {
@@ -222,12 +344,19 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
       b = p.ex.b[ii];
       a = p.ex.a[ii];
     }
-    const T eyb = aey + (gym - gy0) / p.dx;
-    p.gepsy[c] = ge + (-eyb * ey) / epsy;
-    const T dyb = ady + eyb / epsy;
+    T rb;
+    if constexpr (KERR) {
+      const T g0 = p.kgy[c];
+      rb = p.kry[c] + g0 / epsy;
+      p.gepsy[c] = ge + (-g0 * p.ky[0][c]) / epsy;
+    } else {
+      const T eyb = aey + (gym - gy0) / p.dx;
+      p.gepsy[c] = ge + (-eyb * ey) / epsy;
+      rb = eyb / epsy;
+    }
+    const T dyb = ady + rb;
     p.aDy[c] = p.d1x ? c1 * dyb : dyb;
-    p.aEy[c] = p.npy ? te_pole_adj(p.npy, p.c1y, p.c2y, p.c3y, p.aPyc, p.aPyp, c, (long long)nx * (ny - 1),
-                                   eyb / epsy)
+    p.aEy[c] = p.npy ? te_pole_adj(p.npy, p.c1y, p.c2y, p.c3y, p.aPyc, p.aPyp, c, (long long)nx * (ny - 1), rb)
                      : T(0);
     T gd = T(0);
     if (live) {
@@ -302,6 +431,19 @@ int te_adj_t(const mt_te2d* d, const mt_te2d_adj* a, const void* tape, const mt_
   p.aPxp = static_cast<T*>(a->aPx_prev);
   p.aPyc = static_cast<T*>(a->aPy_cur);
   p.aPyp = static_cast<T*>(a->aPy_prev);
+  p.kerr = d->kerr;
+  p.chi2x = static_cast<const T*>(d->chi2x);
+  p.chi3x = static_cast<const T*>(d->chi3x);
+  p.chi2y = static_cast<const T*>(d->chi2y);
+  p.chi3y = static_cast<const T*>(d->chi3y);
+  if (p.kerr) {   // workspace: six Ex-grid arrays, then six Ey-grid arrays
+    const long long nex = (long long)(p.nx - 1) * p.ny, ney = (long long)p.nx * (p.ny - 1);
+    T* w = static_cast<T*>(a->kerr_ws);
+    T* wy = w + 6 * nex;
+    for (int q = 0; q < 3; ++q) p.kx[q] = w + q * nex, p.ky[q] = wy + q * ney;
+    p.kgx = w + 3 * nex, p.kga = w + 4 * nex, p.krx = w + 5 * nex;
+    p.kgy = wy + 3 * ney, p.kgb = wy + 4 * ney, p.kry = wy + 5 * ney;
+  }
   const T* tp = static_cast<const T*>(tape);
   const long long slot = (long long)(p.nx - 1) * p.ny + (long long)p.nx * (p.ny - 1);
   dim3 blk(ABX, ABY), grd = grid2(p.ny, p.nx, blk);
@@ -315,8 +457,19 @@ int te_adj_t(const mt_te2d* d, const mt_te2d_adj* a, const void* tape, const mt_
                          d->dt, s_);
     }
     const T* ts = tp + (long long)s_ * slot;
+    const T* tsy = ts + (long long)(p.nx - 1) * p.ny;
     hipLaunchKernelGGL(te_adj_p1<T>, grd, blk, 0, s, p);
-    hipLaunchKernelGGL(te_adj_p2<T>, grd, blk, 0, s, p, ts, ts + (long long)(p.nx - 1) * p.ny);
+    if (p.kerr) {   // E' = three Jacobi passes from rhs: recompute, then transpose last to first
+      hipLaunchKernelGGL(te_adj_p2a<T>, grd, blk, 0, s, p);
+      for (int q = 0; q < 3; ++q) hipLaunchKernelGGL(te_kerr_iter<T>, grd, blk, 0, s, p, ts, tsy, q);
+      for (int q = 2; q >= 0; --q) {
+        hipLaunchKernelGGL(te_kerr_back<T>, grd, blk, 0, s, p, ts, tsy, q);
+        hipLaunchKernelGGL(te_kerr_scatter<T>, grd, blk, 0, s, p);
+      }
+      hipLaunchKernelGGL((te_adj_p2<T, true>), grd, blk, 0, s, p, ts, tsy);
+    } else {
+      hipLaunchKernelGGL((te_adj_p2<T, false>), grd, blk, 0, s, p, ts, tsy);
+    }
     hipLaunchKernelGGL(te_adj_p3<T>, grd, blk, 0, s, p);
     int rc = check_launch("te2d adjoint step");
     if (rc) return rc;
@@ -375,7 +528,8 @@ extern "C" int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const voi
   if (p->nx < 3 || p->ny < 3) return fail(MT_EINVAL, "grid must be at least 3x3");
   if (k < 0) return fail(MT_EINVAL, "negative step count");
   if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward E tape");
-  if (p->kerr) return fail(MT_EINVAL, "adjoint through nonlinear (chi2 / chi3) media is not supported");
+  if (p->kerr && (!a->kerr_ws || !p->chi2x || !p->chi3x || !p->chi2y || !p->chi3y))
+    return fail(MT_EINVAL, "adjoint through chi2/chi3 needs the four chi maps and kerr_ws (and a tape of D - sum P)");
   if (p->inv_xx_x || p->c3xy) return fail(MT_EINVAL, "the TE adjoint does not transpose tensor eps or oriented poles");
   if (p->y_mirror || p->x_mirror) return fail(MT_EINVAL, "the TE adjoint does not transpose symmetry cuts");
   if (p->npx < 0 || p->npy < 0 ||

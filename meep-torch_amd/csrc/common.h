@@ -27,6 +27,29 @@ inline Pml<T> pml_of(const mt_pml_axis& ax) {
   return Pml<T>{static_cast<const T*>(ax.b), static_cast<const T*>(ax.a), ax.lo_end, ax.hi_begin};
 }
 
+// Four-point averages of the TE fixed point (stepper_te.py:52-61): the value
+// of an Ey-shaped field at Ex (i, j) (i < nx-1), and of an Ex-shaped field at
+// Ey (i, j) (j < ny-1), summed in the reference's order, zero past the
+// walls.  Each is also the transpose of the other (the adjoint of the
+// fixed point scatters cotangents with them).
+template <typename T>
+__device__ __forceinline__ T avg4_to_ex(const T* ey, int nx, int ny, int i, int j) {
+  const T a = j >= 1 ? ey[(long long)i * (ny - 1) + j - 1] : T(0);
+  const T b = j < ny - 1 ? ey[(long long)i * (ny - 1) + j] : T(0);
+  const T c = j >= 1 ? ey[(long long)(i + 1) * (ny - 1) + j - 1] : T(0);
+  const T d = j < ny - 1 ? ey[(long long)(i + 1) * (ny - 1) + j] : T(0);
+  return T(0.25) * (a + b + c + d);
+}
+
+template <typename T>
+__device__ __forceinline__ T avg4_to_ey(const T* ex, int nx, int ny, int i, int j) {
+  const T a = i >= 1 ? ex[(long long)(i - 1) * ny + j] : T(0);
+  const T b = i >= 1 ? ex[(long long)(i - 1) * ny + j + 1] : T(0);
+  const T c = i < nx - 1 ? ex[(long long)i * ny + j] : T(0);
+  const T d = i < nx - 1 ? ex[(long long)i * ny + j + 1] : T(0);
+  return T(0.25) * (a + b + c + d);
+}
+
 int check_axis(const mt_pml_axis& ax, int n, const char* name);
 // A 2D source box inside an n0 x n1 field (or absent).
 int check_src(const mt_source2d* s, int n0, int n1);

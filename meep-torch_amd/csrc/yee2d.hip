@@ -239,27 +239,6 @@ struct TeK {
   int ymir, xmir;  // mirror parities of the low-edge cuts
 };
 
-// Four-point averages with zeros beyond the walls (stepper_te.py:52-61): of
-// an Ey-shaped field at Ex (i, j) (i < nx-1), and of an Ex-shaped field at
-// Ey (i, j) (j < ny-1), summed in the reference's order.
-template <typename T>
-__device__ __forceinline__ T avg4_to_ex(const T* ey, int nx, int ny, int i, int j) {
-  const T a = j >= 1 ? ey[(long long)i * (ny - 1) + j - 1] : T(0);
-  const T b = j < ny - 1 ? ey[(long long)i * (ny - 1) + j] : T(0);
-  const T c = j >= 1 ? ey[(long long)(i + 1) * (ny - 1) + j - 1] : T(0);
-  const T d = j < ny - 1 ? ey[(long long)(i + 1) * (ny - 1) + j] : T(0);
-  return T(0.25) * (a + b + c + d);
-}
-
-template <typename T>
-__device__ __forceinline__ T avg4_to_ey(const T* ex, int nx, int ny, int i, int j) {
-  const T a = i >= 1 ? ex[(long long)(i - 1) * ny + j] : T(0);
-  const T b = i >= 1 ? ex[(long long)(i - 1) * ny + j + 1] : T(0);
-  const T c = i < nx - 1 ? ex[(long long)i * ny + j] : T(0);
-  const T d = i < nx - 1 ? ex[(long long)i * ny + j + 1] : T(0);
-  return T(0.25) * (a + b + c + d);
-}
-
 // D - sum P of one E cell with the pole stacks already advanced (the value
 // te_rhs_from_d returned, recomputed in the same order).
 template <typename T>
@@ -374,7 +353,8 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
                                 p.c3x, p.Pxc, p.Pxp, p.c3xy, av);
     const T ex = p.ixx ? rhs : rhs / p.eps_x[c];  // tensor eps: E is formed from the rhs pair afterwards
     (p.scratch ? p.kx : p.Ex)[c] = ex;
-    if (p.tape) p.tape[(long long)s_rel * ((long long)(nx - 1) * ny + (long long)nx * (ny - 1)) + c] = ex;
+    // under chi2/chi3 the tape keeps rhs: the adjoint recomputes the iterates from it
+    if (p.tape) p.tape[(long long)s_rel * ((long long)(nx - 1) * ny + (long long)nx * (ny - 1)) + c] = p.kerr ? rhs : ex;
   }
   if (j < ny - 1) {  // Dy (nx, ny-1)
     long long c = (long long)i * (ny - 1) + j;
@@ -409,7 +389,8 @@ __global__ __launch_bounds__(BX* BY) void te_d_kernel(TeK<T> p, mt_source2d src,
     const T ey = p.iyy ? rhs : rhs / p.eps_y[c];
     (p.scratch ? p.ky : p.Ey)[c] = ey;
     if (p.tape)
-      p.tape[(long long)s_rel * ((long long)(nx - 1) * ny + (long long)nx * (ny - 1)) + (long long)(nx - 1) * ny + c] = ey;
+      p.tape[(long long)s_rel * ((long long)(nx - 1) * ny + (long long)nx * (ny - 1)) + (long long)(nx - 1) * ny + c] =
+          p.kerr ? rhs : ey;
   }
 }
 
@@ -899,7 +880,6 @@ extern "C" int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const doubl
   if (!p->hz1 != !p->hz2) return fail(MT_EINVAL, "sigma_b/mu needs hz1 and hz2");
   if (p->kerr && (!p->chi2x || !p->chi3x || !p->chi2y || !p->chi3y || !p->kx || !p->ky))
     return fail(MT_EINVAL, "TE chi2/chi3 fixed point needs chi2x, chi3x, chi2y, chi3y and the kx, ky scratch");
-  if (p->kerr && p->tape) return fail(MT_EINVAL, "the E tape is not written under chi2/chi3");
   const bool tensor = p->inv_xx_x || p->inv_xy_x || p->inv_yy_y || p->inv_xy_y;
   if (tensor && !(p->inv_xx_x && p->inv_xy_x && p->inv_yy_y && p->inv_xy_y))
     return fail(MT_EINVAL, "tensor eps needs inv_xx_x, inv_xy_x, inv_yy_y and inv_xy_y");

@@ -214,7 +214,8 @@ def _descriptor(c, fields, dtype, device):
 
 class _TERun(torch.autograd.Function):
     """TE counterpart of _TMRun: differentiable in eps_x and eps_y (the
-    face-averaged permittivities of TECoeffs, stepper_te.py:160-168)."""
+    face-averaged permittivities of TECoeffs, stepper_te.py:160-168), also
+    through the chi2/chi3 fixed point (stepper_te.py:169-175)."""
 
     @staticmethod
     def forward(ctx, eps_x, eps_y, cfg, *state_in):
@@ -256,6 +257,10 @@ class _TERun(torch.autograd.Function):
             setattr(a, f, t.data_ptr())
         a.epsx_grad, a.epsy_grad = gx.data_ptr(), gy.data_ptr()
         d, d_keep = _te_descriptor(cdet, nx, ny, dtype, device)
+        if d.kerr:   # iterates and cotangents of the fixed point's transpose
+            kws = torch.empty(6 * ((nx - 1) * ny + nx * (ny - 1)), dtype=dtype, device=device)
+            d_keep.append(kws)
+            a.kerr_ws = kws.data_ptr()
         slot = (nx - 1) * ny + nx * (ny - 1)
         for step0, k, st in reversed(ctx.ckpts):
             work = {f: t.clone() for f, t in st.items()}
@@ -290,6 +295,12 @@ def _te_descriptor(c, nx, ny, dtype, device):
     if c.c1y.shape[0]:                             # ADE poles on Ey
         d.npy = int(c.c1y.shape[0])
         media += ["c1y", "c2y", "c3y"]
+    if c.has_kerr or c.has_chi2:                   # the chi2/chi3 fixed point (stepper_te.py:169-175)
+        d.kerr = 1
+        for name, attr in (("chi2x", "chi2_x"), ("chi3x", "chi3_x"), ("chi2y", "chi2_y"), ("chi3y", "chi3_y")):
+            t = getattr(c, attr).to(device=device, dtype=dtype).contiguous()
+            keep.append(t)
+            setattr(d, name, t.data_ptr())
     for name in media:
         t = getattr(c, name).to(device=device, dtype=dtype).contiguous()
         keep.append(t)
@@ -298,8 +309,6 @@ def _te_descriptor(c, nx, ny, dtype, device):
 
 
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
-    if mode == "te" and (coeffs.has_kerr or coeffs.has_chi2):
-        raise NotImplementedError("gradients through nonlinear (chi2 / chi3) media are not supported")
     if coeffs.y_mirror_lo or coeffs.x_mirror_lo or getattr(coeffs, "y_rot2_lo", 0) or getattr(coeffs, "rot4_lo", 0):
         raise NotImplementedError("gradients through symmetry-folded runs (mirrors, Rotate2, Rotate4) "
                                   "are not supported")

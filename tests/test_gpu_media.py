@@ -186,18 +186,18 @@ def test_conductive_slab_vs_meep_on_hip():
 
 
 def test_media_refuse_gradients_and_batches():
-    """TE gradients through chi2/chi3 (the Jacobi passes couple Ex and Ey
-    through four-point averages) raise; so does graph capture of a batch
-    whose variants carry media (they run one by one)."""
+    """TE gradients with respect to chi3 itself (only eps_x / eps_y are
+    differentiated) raise; so does graph capture of a batch whose variants
+    carry media (they run one by one)."""
     from meep_torch_amd.batch import BatchedTM
     from meep_torch_amd.core.coeffs import build_te_coeffs, build_tm_coeffs
     from meep_torch_amd.core.grid import Grid
     from meep_torch_amd.core.run import run_te
     from meep_torch_amd.core.stepper_te import init_te_state
     g = Grid(nx=32, ny=32, dx=0.05, dtype=torch.float64, device="cuda")
-    eps = torch.ones(32, 32, dtype=torch.float64, device="cuda", requires_grad=True)
-    m = build_te_coeffs(g, eps_inf=eps, npml=4, chi3=0.01)
-    with pytest.raises(NotImplementedError, match="nonlinear"):
+    chi3 = torch.full((32, 32), 0.01, dtype=torch.float64, device="cuda", requires_grad=True)
+    m = build_te_coeffs(g, eps_inf=2.0, npml=4, chi3=chi3)
+    with pytest.raises(NotImplementedError, match="coefficient"):
         run_te(init_te_state(g), m, 5)
     m2 = build_tm_coeffs(g, eps_inf=2.0, npml=4, poles=[(1.0, 0.1, 1.0)])
     with pytest.raises(NotImplementedError, match="graph"):
@@ -477,6 +477,62 @@ def test_te_eps_gradient_through_poles():
     err = rel(eps.grad, g_ref)
     print(f"\n[te pole grad] rel-L2 {err:.2e}")
     assert float(g_ref[50:75, 16:48].abs().max()) > 0 and err <= 1e-9
+
+
+@pytest.mark.parametrize("with_poles", [False, True], ids=["kerr", "kerr+poles+sigma"])
+def test_te_eps_gradient_through_kerr(with_poles):
+    """The TE adjoint transposes the chi2/chi3 fixed point: the tape keeps
+    rhs = D - sum P, the three Jacobi passes (coupled through the four-point
+    averages of the other component) are recomputed from it and transposed
+    last to first, the averages' cotangents scattered back with the other
+    average.  The eps gradient of a flux objective equals CPU autograd
+    through the oracle's TE steps (FP64), with chi2 and chi3 alone and with
+    poles and conductivity in the same cells."""
+    from meep_torch_amd.core.coeffs import build_te_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTE
+    from meep_torch_amd.core.run import run_te
+    from meep_torch_amd.core.sources import GaussianPulse, LineSource
+    from meep_torch_amd.core.stepper_te import init_te_state
+    from oracle import yee2d
+    dt = torch.float64
+    eps0, kw0 = _te_media(dt)
+    keep = ("chi3", "chi2") + (("poles", "pole_masks", "poles_x", "pole_masks_x", "sigma_d", "sigma_b", "mu")
+                               if with_poles else ())
+    kw = {k: v for k, v in kw0.items() if k in keep}
+    kw["chi3"] = kw["chi3"] * 40.0      # a strong nonlinearity at this drive
+    dev = {k: ([x.cuda() for x in v] if k.startswith("pole_masks") else v.cuda() if torch.is_tensor(v) else v)
+           for k, v in kw.items()}
+    npx, npy = (2, 1) if with_poles else (0, 0)
+    n = 300
+    g = Grid(nx=NX, ny=NY, dx=DX, dtype=dt, device="cuda")
+    eps = eps0.clone().cuda().requires_grad_(True)
+    m = build_te_coeffs(g, eps_inf=eps, npml=NPML, **dev)
+    src = LineSource(g, ix=20, j0=16, j1=48, profile=GaussianPulse(0.8, 0.5), amp_func=lambda y: 30.0)
+    src.mask = src.mask[:, :-1].contiguous()
+    mon = FluxMonitorTE(g, FREQS, ix=80, lo=12, hi=52)
+    _, (acc,) = run_te(init_te_state(g, n_poles=npx, n_poles_y=npy), m, n, source=src, monitors=[mon],
+                       checkpoint_every=100)
+    mon.flux(acc, DX).sum().backward()
+    torch.set_num_threads(8)
+    e = eps0.clone().requires_grad_(True)
+    c = yee2d.te_coeffs(NX, NY, DX, e, NPML, dtype=dt, **kw)
+    _, (oacc,) = yee2d.run2d("te", yee2d.te_state(NX, NY, dt, npx=npx, npy=npy), c, n,
+                             amps=[src.amplitude(t) for t in range(n)], mask=src.mask.cpu(),
+                             monitors=[{"type": "flux", "kind": "te_x", "idx": 80, "lo": 12, "hi": 52,
+                                        "freqs": FREQS}])
+    g_ref, = torch.autograd.grad(yee2d.flux_from_acc("te_x", oacc, DX).sum(), e)
+    err = rel(eps.grad, g_ref)
+    # the fixed point matters here: dropping it changes the gradient by far more than the bar
+    c_lin = yee2d.te_coeffs(NX, NY, DX, e, NPML, dtype=dt, **{k: v for k, v in kw.items() if k not in ("chi3", "chi2")})
+    _, (lacc,) = yee2d.run2d("te", yee2d.te_state(NX, NY, dt, npx=npx, npy=npy), c_lin, n,
+                             amps=[src.amplitude(t) for t in range(n)], mask=src.mask.cpu(),
+                             monitors=[{"type": "flux", "kind": "te_x", "idx": 80, "lo": 12, "hi": 52,
+                                        "freqs": FREQS}])
+    g_lin, = torch.autograd.grad(yee2d.flux_from_acc("te_x", lacc, DX).sum(), e)
+    print(f"\n[te kerr grad] rel-L2 {err:.2e} (linear-media gradient differs by {rel(g_lin, g_ref):.2e})")
+    assert float(g_ref[50:75, 16:48].abs().max()) > 0 and err <= 1e-9
+    assert float(rel(g_lin, g_ref)) > 1e-4
 
 
 @pytest.mark.parametrize("with_poles", [False, True], ids=["kerr", "kerr+poles+sigma"])
