@@ -374,6 +374,10 @@ typedef struct {
    * the Dz - sum P, after each step). */
   void *chi2_grad, *chi3_grad, *c3_grad;
   const void* ez_init;
+  /* magnetic Lorentz poles (p->n_mpoles > 0): cotangents of Bx (nx, ny-1),
+   * By (nx-1, ny) and of the M stacks Mx_cur, Mx_prev (n_mpoles, nx, ny-1),
+   * My_cur, My_prev (n_mpoles, nx-1, ny); NULL otherwise */
+  void *aBx, *aBy, *aMx_cur, *aMx_prev, *aMy_cur, *aMy_prev;
 } mt_tm2d_adj;
 
 /* Reverse-time adjoint of k TM steps [step0, step0+k): on entry the mt_tm2d_adj
@@ -386,7 +390,9 @@ typedef struct {
  *   dt*(cos(w t) dacc_re - sin(w t) dacc_im)  (halved onto both rows of an
  * averaged term).  Conductivity (d1, d2, h1x, h2x, h1y, h2y of p, all or none)
  * and ADE poles (c1, c2, c3 of p with a->aP_cur, a->aP_prev) are transposed
- * with the recurrences; Kerr is refused.  Replaces
+ * with the recurrences; so are magnetic poles (mc1x..mc3y of p with the aB* /
+ * aM* cotangents of a: B' = B +- dt curl, M' = (mc1 M + mc2 M_prev) + mc3 H,
+ * H' = B' - sum M').  Replaces
  * PyTorch autograd through run_tm's checkpointed chunks
  * (meep_torch/core/run.py:99-120, eps reached via stepper_tm.py:226-227). */
 int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const void* tape,

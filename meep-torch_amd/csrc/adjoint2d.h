@@ -34,7 +34,36 @@ struct AdjK {
   // the first step of the call (c3's gradient needs the Ez each step starts from)
   T *chi2_grad, *chi3_grad, *c3_grad;
   const T* ez_init;
+  // magnetic Lorentz poles (nm > 0; no conductivity): per-cell stacks mc1..mc3
+  // of the Hx and Hy grids and the cotangents of B and of the M stacks
+  int nm;
+  const T *mc1x, *mc2x, *mc3x, *mc1y, *mc2y, *mc3y;
+  T *aBx, *aBy, *aMxc, *aMxp, *aMyc, *aMyp;
 };
+
+// Transpose of one H component's magnetic poles at a cell (stepper_tm.py:
+// 103-106,187-193,209-224): H' = B' - sum M', M' = (mc1 M + mc2 M_prev) +
+// mc3 H, B' = B +- dt curl.  hb is the cotangent of H'; the B cotangent
+// (updated in place) becomes that of B' (= of the old B), every M' gains
+// -hb, the stacks take the recurrence's transpose, and the old H's
+// cotangent (it feeds only the M') is returned; *bb gets B''s cotangent,
+// which the curl receives times +-dt.
+template <typename T>
+__device__ __forceinline__ T mpole_adj(int nm, const T* mc1, const T* mc2, const T* mc3, T* aB, T* aMc, T* aMp,
+                                       long long q, long long nq, T hb, T* bb) {
+  const T b = aB[q] + hb;
+  aB[q] = b;
+  *bb = b;
+  T h_old = T(0);
+  for (int k = 0; k < nm; ++k) {
+    const long long o = (long long)k * nq + q;
+    const T mb = aMc[o] - hb, prev = aMp[o];
+    aMc[o] = mc1[o] * mb + prev;
+    aMp[o] = mc2[o] * mb;
+    h_old = h_old + mc3[o] * mb;
+  }
+  return h_old;
+}
 
 // Ez before step s_rel of a call: ez_init for the first step, else the tape
 // slot of the step before (under chi2/chi3 that slot is Dz - sum P, and Ez
@@ -196,7 +225,14 @@ __device__ __forceinline__ void adj_b_cell(const AdjK<T>& p, int i, int j) {
   }
   if (fy) {
     T hb = ahy + (gy0 - gy1) / p.dx;
-    p.aHy[hy] = p.h1y ? h1y * hb : hb;
+    if (p.nm) {   // the curl reaches H' through B'
+      T bb;
+      p.aHy[hy] = mpole_adj(p.nm, p.mc1y, p.mc2y, p.mc3y, p.aBy, p.aMyc, p.aMyp, hy, (long long)(nx - 1) * ny, hb,
+                            &bb);
+      hb = bb;
+    } else {
+      p.aHy[hy] = p.h1y ? h1y * hb : hb;
+    }
     T gb = h2y * hb;
     if (sx) {
       T pt = pzx + h2y * hb;
@@ -207,7 +243,14 @@ __device__ __forceinline__ void adj_b_cell(const AdjK<T>& p, int i, int j) {
   }
   if (fx) {
     T hb = ahx + (gx0 - gx1) / p.dx;
-    p.aHx[hx] = p.h1y ? h1x * hb : hb;
+    if (p.nm) {
+      T bb;
+      p.aHx[hx] = mpole_adj(p.nm, p.mc1x, p.mc2x, p.mc3x, p.aBx, p.aMxc, p.aMxp, hx, (long long)nx * (ny - 1), hb,
+                            &bb);
+      hb = bb;
+    } else {
+      p.aHx[hx] = p.h1y ? h1x * hb : hb;
+    }
     T gb = -h2x * hb;
     if (sy) {
       T pt = pzy - h2x * hb;
@@ -551,6 +594,19 @@ AdjK<T> make_adjk(const mt_tm2d* d, const mt_tm2d_adj* a) {
   p.chi3_grad = static_cast<T*>(a->chi3_grad);
   p.c3_grad = p.np ? static_cast<T*>(a->c3_grad) : nullptr;
   p.ez_init = static_cast<const T*>(a->ez_init);
+  p.nm = (d->n_mpoles > 0 && a->aBx) ? d->n_mpoles : 0;
+  p.mc1x = static_cast<const T*>(d->mc1x);
+  p.mc2x = static_cast<const T*>(d->mc2x);
+  p.mc3x = static_cast<const T*>(d->mc3x);
+  p.mc1y = static_cast<const T*>(d->mc1y);
+  p.mc2y = static_cast<const T*>(d->mc2y);
+  p.mc3y = static_cast<const T*>(d->mc3y);
+  p.aBx = static_cast<T*>(a->aBx);
+  p.aBy = static_cast<T*>(a->aBy);
+  p.aMxc = static_cast<T*>(a->aMx_cur);
+  p.aMxp = static_cast<T*>(a->aMx_prev);
+  p.aMyc = static_cast<T*>(a->aMy_cur);
+  p.aMyp = static_cast<T*>(a->aMy_prev);
   return p;
 }
 

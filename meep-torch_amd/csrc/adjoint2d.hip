@@ -497,7 +497,11 @@ extern "C" int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const voi
     return fail(MT_EINVAL, "the c3 gradient needs ez_init (Ez before the first step)");
   if (p->n_poles < 0 || (p->n_poles && (!a->aP_cur || !a->aP_prev || !p->c1 || !p->c2 || !p->c3)))
     return fail(MT_EINVAL, "adjoint with poles needs c1, c2, c3 and the aP_cur / aP_prev cotangents");
-  if (p->n_mpoles) return fail(MT_EINVAL, "the TM adjoint does not transpose magnetic poles (n_mpoles = 0)");
+  if (p->n_mpoles < 0 ||
+      (p->n_mpoles && (!a->aBx || !a->aBy || !a->aMx_cur || !a->aMx_prev || !a->aMy_cur || !a->aMy_prev ||
+                       !p->mc1x || !p->mc2x || !p->mc3x || !p->mc1y || !p->mc2y || !p->mc3y)))
+    return fail(MT_EINVAL, "adjoint with magnetic poles needs mc1x..mc3y and the B / M cotangents");
+  if (p->n_mpoles && p->h1x) return fail(MT_EINVAL, "magnetic poles do not combine with sigma_b / mu");
   if (p->y_mirror || p->x_mirror || p->y_rot2 || p->rot4)
     return fail(MT_EINVAL, "the TM adjoint does not transpose symmetry cuts");
   if (!p->d1 != !p->d2 || !p->h1x != !p->h2x || !p->h1y != !p->h2y || !p->d1 != !p->h1x || !p->d1 != !p->h1y)

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 25
+ABI_VERSION = 26
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -55,7 +55,9 @@ class TM2DAdj(ctypes.Structure):
                 ("apsi_Ezy", c_void_p), ("apsi_Ezx", c_void_p), ("apsi_Hyx", c_void_p),
                 ("apsi_Hxy", c_void_p), ("gHy", c_void_p), ("gHx", c_void_p), ("gEx", c_void_p),
                 ("gEy", c_void_p), ("eps_grad", c_void_p), ("aP_cur", c_void_p), ("aP_prev", c_void_p),
-                ("chi2_grad", c_void_p), ("chi3_grad", c_void_p), ("c3_grad", c_void_p), ("ez_init", c_void_p)]
+                ("chi2_grad", c_void_p), ("chi3_grad", c_void_p), ("c3_grad", c_void_p), ("ez_init", c_void_p),
+                ("aBx", c_void_p), ("aBy", c_void_p), ("aMx_cur", c_void_p), ("aMx_prev", c_void_p),
+                ("aMy_cur", c_void_p), ("aMy_prev", c_void_p)]
 
 
 class TE2D(ctypes.Structure):

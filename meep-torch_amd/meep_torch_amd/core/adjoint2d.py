@@ -107,7 +107,8 @@ def _chunk_cotangents(monitors, gacc, step0, k):
 
 class _TMRun(torch.autograd.Function):
     """Differentiable in eps_inf and, when given, the chi2 / chi3 maps and the
-    poles' c3 (through which pole masks enter, coeffs.py:184-190)."""
+    poles' c3 (through which pole masks enter, coeffs.py:184-190); also
+    through magnetic poles (the B / M recurrence is transposed)."""
 
     @staticmethod
     def forward(ctx, eps, chi2, chi3, c3, cfg, *state_in):
@@ -198,6 +199,9 @@ def _descriptor(c, fields, dtype, device):
     if c.c1.shape[0]:                              # ADE poles
         d.n_poles = int(c.c1.shape[0])
         media += ["c1", "c2", "c3"]
+    if c.mc1x is not None:                         # magnetic Lorentz poles
+        d.n_mpoles = int(c.mc1x.shape[0])
+        media += ["mc1x", "mc2x", "mc3x", "mc1y", "mc2y", "mc3y"]
     for name in media:
         t = getattr(c, name).to(device=device, dtype=dtype).contiguous()
         keep.append(t)
@@ -314,9 +318,6 @@ def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_
                                   "are not supported")
     if mode == "te" and (coeffs.inv_xx_x is not None or coeffs.c3xy is not None):
         raise NotImplementedError("gradients through TE tensor eps or oriented poles are not supported")
-    if mode == "tm" and coeffs.mc1x is not None:
-        raise NotImplementedError("gradients through magnetic poles (m_poles) are not supported: the "
-                                  "hand-written TM adjoint does not transpose the B / M recurrence")
     allowed = ("eps_inf", "chi2", "chi3", "c3") if mode == "tm" else ("eps_x", "eps_y")
     for name, t in coeffs._asdict().items():
         if torch.is_tensor(t) and t.requires_grad and name not in allowed:

@@ -11,8 +11,9 @@ tests/test_oracle_media_golden.py):
 Fields, B, the M and P stacks and CPML memory are compared with torch.equal
 (FP32 and FP64); the flux accumulator within 1e-4 (FP32) / 1e-12 (FP64)
 relative, as the staged DFT flush sums in its own order.  Also: chunked runs
-carry B and the M stacks across calls, and the paths without a magnetic-pole
-form (the hand-written adjoint, batches) refuse by name."""
+carry B and the M stacks across calls, and the eps gradient through the
+magnetic poles (the hand-written TM adjoint transposes the B / M recurrence)
+equals CPU autograd through the oracle."""
 import numpy as np
 import pytest
 import torch
@@ -76,18 +77,47 @@ def test_magnetic_poles_match_reference(case, tag, dtype, chunk):
     assert float(st.My_cur.abs().max()) > 0
 
 
-def test_magnetic_poles_refuse_gradients_and_batch_gradients():
+@pytest.mark.parametrize("case", ["mag", "magh"])
+def test_magnetic_pole_gradients_match_oracle_autograd(case):
+    """The TM adjoint transposes the magnetic poles' B / M recurrence
+    (H' = B' - sum M', M' = (mc1 M + mc2 M_prev) + mc3 H, B' = B +- dt curl):
+    the eps gradient of a flux objective equals CPU autograd through the
+    oracle's steps (FP64), with electric poles and chi3 in the same cells
+    (mag) and under a TFSF Hy current (magh); dropping the M recurrence's
+    H term changes the gradient by far more than the bar.  A batch runs the
+    variant through the same path (its gradient equals run_tm's)."""
     from meep_torch_amd.batch import BatchedTM
     from meep_torch_amd.core.run import run_tm
-    g, m, src, mon, st = _setup("magh", "f64", torch.float64)
+    from oracle import yee2d
+    g, m, src, mon, st = _setup(case, "f64", torch.float64)
+    n = 160
     eps = m.eps_inf.clone().requires_grad_(True)
-    with pytest.raises(NotImplementedError, match="magnetic poles"):
-        run_tm(st, m._replace(eps_inf=eps), 10, source=src, monitors=[mon], checkpoint_every=5)
-    # a batch runs the variant forward through the single-problem path (the
-    # spectrum equals run_tm's); its gradient raises like run_tm's
-    bt = BatchedTM([(m, src, [mon])], 10, checkpoint_every=5)
-    (acc,), = bt.forward([m.eps_inf])
-    _, (ref,) = run_tm(_setup("magh", "f64", torch.float64)[4], m, 10, source=src, monitors=[mon])
-    assert torch.equal(acc, ref)
-    with pytest.raises(NotImplementedError, match="magnetic poles"):
-        bt.gradient([[torch.ones_like(acc)]])
+    _, (acc,) = run_tm(st, m._replace(eps_inf=eps), n, source=src, monitors=[mon], checkpoint_every=60)
+    w = torch.linspace(0.5, 1.5, acc.numel(), dtype=acc.dtype, device=acc.device).reshape(acc.shape)
+    (ge,) = torch.autograd.grad((acc * w).sum(), [eps])
+    t = lambda k: torch.from_numpy(np.ascontiguousarray(g[k]))
+    torch.set_num_threads(8)
+
+    def oracle_grad(mpoles):
+        e = t("eps").clone().requires_grad_(True)
+        kw = dict(m_poles=mpoles, m_pole_masks=[t("mm1"), t("mm2")])
+        if case == "mag":
+            kw.update(poles=POLES2, pole_masks=[t("m1"), t("m2")], chi3=t("chi3"))
+        c = yee2d.tm_coeffs(NX, NY, 0.05, e, 6, dtype=torch.float64, **kw)
+        ost = yee2d.tm_state(NX, NY, torch.float64, n_poles=2 if case == "mag" else 0, n_mpoles=2)
+        mons = [{"type": "flux", "kind": "tm_x", "idx": 38, "lo": 7, "hi": NY - 7, "freqs": [0.6, 0.8, 1.0]}]
+        hk = {} if case == "mag" else dict(hamps=g["magh_h_amp"][:n], hmasks={"hy_mask": t("magh_hy_mask")})
+        _, (oacc,) = yee2d.run2d("tm", ost, c, n, amps=g["mag_amp"][:n], mask=t("mag_mask"), monitors=mons, **hk)
+        (gr,) = torch.autograd.grad((oacc * w.cpu().reshape(oacc.shape)).sum(), e)
+        return gr
+
+    g_ref = oracle_grad(MPOLES2)
+    err = float((ge.cpu() - g_ref).norm() / g_ref.norm())
+    g_off = oracle_grad([(fn, gam, 0.0) for fn, gam, _ in MPOLES2])   # no H drive of the M stacks
+    off = float((g_off - g_ref).norm() / g_ref.norm())
+    print(f"\n[magnetic pole grad {case}] rel-L2 {err:.2e} (without the M drive {off:.2e})")
+    assert err <= 1e-9 and off > 1e-4
+    bt = BatchedTM([(m, src, [mon])], n, checkpoint_every=60)
+    bt.forward([m.eps_inf])
+    (gb,) = bt.gradient([[w]])
+    assert torch.equal(gb, ge)
