@@ -138,7 +138,7 @@ typedef struct {
   void* alt[8];
   /* optional device tape (two-pass path only): E after step s of a call is
    * written to slot s = tape + s * ((nx-1)*ny + nx*(ny-1)) as [Ex | Ey], for
-   * mt_te2d_adjoint's eps gradient; under chi2/chi3 (kerr) the slot holds
+   * mt_te2d_adjoint's eps gradient; under chi2/chi3 (kerr) or tensor eps the slot holds
    * [Dx - sum Px | Dy - sum Py], from which the adjoint recomputes the
    * fixed-point iterates */
   void* tape;
@@ -413,6 +413,10 @@ typedef struct {
    * elements for the recomputed fixed-point iterates and their cotangents
    * (the forward tape then holds rhs = D - sum P, not E); else NULL */
   void* kerr_ws;
+  /* tensor eps (p->inv_xx_x set; needs kerr_ws too): gradients of the four
+   * inverse maps inv_xx_x, inv_xy_x (nx-1, ny), inv_yy_y, inv_xy_y (nx, ny-1),
+   * accumulated (the tape then holds D - sum P); else NULL */
+  void *ixx_grad, *ixy_x_grad, *iyy_grad, *ixy_y_grad;
 } mt_te2d_adj;
 
 /* Reverse sweep of k TE steps starting at global step step0 (the gradient of

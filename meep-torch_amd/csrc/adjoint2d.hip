@@ -114,7 +114,25 @@ struct AdjTe {
   int kerr;
   const T *chi2x, *chi3x, *chi2y, *chi3y;
   T *kx[3], *ky[3], *kgx, *kgy, *kga, *kgb, *krx, *kry;
+  // tensor eps (stepper_te.py:160-166): E = ixx rhs_x + ixyx avg(rhs_y),
+  // E_y = iyy rhs_y + ixyy avg(rhs_x), and the gradients of the four maps
+  const T *ixx, *ixyx, *iyy, *ixyy;
+  T *gixx, *gixyx, *giyy, *gixyy;
 };
+
+// Four-point averages of the product a * b (same neighbours and order as
+// avg4_to_ex / avg4_to_ey), for the transposes of the tensor-eps cross terms.
+template <typename T>
+__device__ __forceinline__ T avg4_prod_to_ex(const T* a, const T* b, int nx, int ny, int i, int j) {
+  auto at = [&](int ii, int jj, bool ok) { const long long q = (long long)ii * (ny - 1) + jj; return ok ? a[q] * b[q] : T(0); };
+  return T(0.25) * (at(i, j - 1, j >= 1) + at(i, j, j < ny - 1) + at(i + 1, j - 1, j >= 1) + at(i + 1, j, j < ny - 1));
+}
+
+template <typename T>
+__device__ __forceinline__ T avg4_prod_to_ey(const T* a, const T* b, int nx, int ny, int i, int j) {
+  auto at = [&](int ii, int jj, bool ok) { const long long q = (long long)ii * ny + jj; return ok ? a[q] * b[q] : T(0); };
+  return T(0.25) * (at(i - 1, j, i >= 1) + at(i - 1, j + 1, i >= 1) + at(i, j, i < nx - 1) + at(i, j + 1, i < nx - 1));
+}
 
 // Transpose of one axis's poles at a cell: P' = c1 P + c2 P_prev + c3 E(old)
 // and E' = (D' - sum P') / eps, with r = E'b / eps the cotangent that E'
@@ -203,8 +221,8 @@ __global__ __launch_bounds__(ABX* ABY) void te_kerr_iter(AdjTe<T> p, const T* te
   if (i >= nx || j >= ny) return;
   if (i < nx - 1) {
     const long long c = (long long)i * ny + j;
-    if (k == 0) {
-      p.kx[0][c] = tex[c] / p.eps_x[c];
+    if (k == 0) {   // the first iterate: rhs / eps_x, or the tensor eps product
+      p.kx[0][c] = p.ixx ? p.ixx[c] * tex[c] + p.ixyx[c] * avg4_to_ex(tey, nx, ny, i, j) : tex[c] / p.eps_x[c];
     } else {
       const T av = avg4_to_ex(p.ky[k - 1], nx, ny, i, j), e = p.kx[k - 1][c];
       const T e2 = e * e + av * av;
@@ -214,7 +232,7 @@ __global__ __launch_bounds__(ABX* ABY) void te_kerr_iter(AdjTe<T> p, const T* te
   if (j < ny - 1) {
     const long long c = (long long)i * (ny - 1) + j;
     if (k == 0) {
-      p.ky[0][c] = tey[c] / p.eps_y[c];
+      p.ky[0][c] = p.iyy ? p.iyy[c] * tey[c] + p.ixyy[c] * avg4_to_ey(tex, nx, ny, i, j) : tey[c] / p.eps_y[c];
     } else {
       const T av = avg4_to_ey(p.kx[k - 1], nx, ny, i, j), e = p.ky[k - 1][c];
       const T e2 = e * e + av * av;
@@ -279,7 +297,36 @@ __global__ __launch_bounds__(ABX* ABY) void te_kerr_scatter(AdjTe<T> p) {
   }
 }
 
-template <typename T, bool KERR>
+// Transpose of the tensor-eps product with kgx / kgy the cotangents of the E
+// it forms (after the fixed point's transpose under chi2/chi3): the rhs
+// cotangents gain ixx g + the averaged ixy_y g of the Ey neighbours (and the
+// mirror for Ey), the maps' gradients gain g times their factor.
+template <typename T>
+__global__ __launch_bounds__(ABX* ABY) void te_tensor_back(AdjTe<T> p, const T* tex, const T* tey) {
+  const int j = blockIdx.x * ABX + threadIdx.x;
+  const int i = blockIdx.y * ABY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx || j >= ny) return;
+  if (i < nx - 1) {
+    const long long c = (long long)i * ny + j;
+    const T g = p.kgx[c];
+    p.krx[c] = (p.kerr ? p.krx[c] : T(0)) + (p.ixx[c] * g + avg4_prod_to_ex(p.ixyy, p.kgy, nx, ny, i, j));
+    p.gixx[c] = p.gixx[c] + g * tex[c];
+    p.gixyx[c] = p.gixyx[c] + g * avg4_to_ex(tey, nx, ny, i, j);
+  }
+  if (j < ny - 1) {
+    const long long c = (long long)i * (ny - 1) + j;
+    const T g = p.kgy[c];
+    p.kry[c] = (p.kerr ? p.kry[c] : T(0)) + (p.iyy[c] * g + avg4_prod_to_ey(p.ixyx, p.kgx, nx, ny, i, j));
+    p.giyy[c] = p.giyy[c] + g * tey[c];
+    p.gixyy[c] = p.gixyy[c] + g * avg4_to_ey(tex, nx, ny, i, j);
+  }
+}
+
+// MODE 0: E = rhs / eps from the E' cotangent; 1: after the chi2/chi3
+// transpose (krx holds the passes' rhs cotangent, kgx that of rhs / eps);
+// 2: tensor eps (krx complete).
+template <typename T, int MODE>
 __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, const T* tey) {
   const int j = blockIdx.x * ABX + threadIdx.x;
   const int i = blockIdx.y * ABY + threadIdx.y;
@@ -303,7 +350,9 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
       a = p.ey.a[jj];
     }
     T rb;   // cotangent of rhs = Dx' - sum P'
-    if constexpr (KERR) {   // the first iterate rhs / eps_x; tex holds rhs
+    if constexpr (MODE == 2) {
+      rb = p.krx[c];
+    } else if constexpr (MODE == 1) {   // the first iterate rhs / eps_x; tex holds rhs
       const T g0 = p.kgx[c];
       rb = p.krx[c] + g0 / epsx;
       p.gepsx[c] = ge + (-g0 * p.kx[0][c]) / epsx;
@@ -345,7 +394,9 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
       a = p.ex.a[ii];
     }
     T rb;
-    if constexpr (KERR) {
+    if constexpr (MODE == 2) {
+      rb = p.kry[c];
+    } else if constexpr (MODE == 1) {
       const T g0 = p.kgy[c];
       rb = p.kry[c] + g0 / epsy;
       p.gepsy[c] = ge + (-g0 * p.ky[0][c]) / epsy;
@@ -436,7 +487,15 @@ int te_adj_t(const mt_te2d* d, const mt_te2d_adj* a, const void* tape, const mt_
   p.chi3x = static_cast<const T*>(d->chi3x);
   p.chi2y = static_cast<const T*>(d->chi2y);
   p.chi3y = static_cast<const T*>(d->chi3y);
-  if (p.kerr) {   // workspace: six Ex-grid arrays, then six Ey-grid arrays
+  p.ixx = static_cast<const T*>(d->inv_xx_x);
+  p.ixyx = static_cast<const T*>(d->inv_xy_x);
+  p.iyy = static_cast<const T*>(d->inv_yy_y);
+  p.ixyy = static_cast<const T*>(d->inv_xy_y);
+  p.gixx = static_cast<T*>(a->ixx_grad);
+  p.gixyx = static_cast<T*>(a->ixy_x_grad);
+  p.giyy = static_cast<T*>(a->iyy_grad);
+  p.gixyy = static_cast<T*>(a->ixy_y_grad);
+  if (p.kerr || p.ixx) {   // workspace: six Ex-grid arrays, then six Ey-grid arrays
     const long long nex = (long long)(p.nx - 1) * p.ny, ney = (long long)p.nx * (p.ny - 1);
     T* w = static_cast<T*>(a->kerr_ws);
     T* wy = w + 6 * nex;
@@ -459,16 +518,23 @@ int te_adj_t(const mt_te2d* d, const mt_te2d_adj* a, const void* tape, const mt_
     const T* ts = tp + (long long)s_ * slot;
     const T* tsy = ts + (long long)(p.nx - 1) * p.ny;
     hipLaunchKernelGGL(te_adj_p1<T>, grd, blk, 0, s, p);
-    if (p.kerr) {   // E' = three Jacobi passes from rhs: recompute, then transpose last to first
+    if (p.kerr || p.ixx) {   // E' from rhs through the tensor product and / or the three Jacobi passes
       hipLaunchKernelGGL(te_adj_p2a<T>, grd, blk, 0, s, p);
-      for (int q = 0; q < 3; ++q) hipLaunchKernelGGL(te_kerr_iter<T>, grd, blk, 0, s, p, ts, tsy, q);
-      for (int q = 2; q >= 0; --q) {
-        hipLaunchKernelGGL(te_kerr_back<T>, grd, blk, 0, s, p, ts, tsy, q);
-        hipLaunchKernelGGL(te_kerr_scatter<T>, grd, blk, 0, s, p);
+      if (p.kerr) {   // recompute the iterates, then transpose the passes last to first
+        for (int q = 0; q < 3; ++q) hipLaunchKernelGGL(te_kerr_iter<T>, grd, blk, 0, s, p, ts, tsy, q);
+        for (int q = 2; q >= 0; --q) {
+          hipLaunchKernelGGL(te_kerr_back<T>, grd, blk, 0, s, p, ts, tsy, q);
+          hipLaunchKernelGGL(te_kerr_scatter<T>, grd, blk, 0, s, p);
+        }
       }
-      hipLaunchKernelGGL((te_adj_p2<T, true>), grd, blk, 0, s, p, ts, tsy);
+      if (p.ixx) {
+        hipLaunchKernelGGL(te_tensor_back<T>, grd, blk, 0, s, p, ts, tsy);
+        hipLaunchKernelGGL((te_adj_p2<T, 2>), grd, blk, 0, s, p, ts, tsy);
+      } else {
+        hipLaunchKernelGGL((te_adj_p2<T, 1>), grd, blk, 0, s, p, ts, tsy);
+      }
     } else {
-      hipLaunchKernelGGL((te_adj_p2<T, false>), grd, blk, 0, s, p, ts, tsy);
+      hipLaunchKernelGGL((te_adj_p2<T, 0>), grd, blk, 0, s, p, ts, tsy);
     }
     hipLaunchKernelGGL(te_adj_p3<T>, grd, blk, 0, s, p);
     int rc = check_launch("te2d adjoint step");
@@ -534,7 +600,10 @@ extern "C" int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const voi
   if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward E tape");
   if (p->kerr && (!a->kerr_ws || !p->chi2x || !p->chi3x || !p->chi2y || !p->chi3y))
     return fail(MT_EINVAL, "adjoint through chi2/chi3 needs the four chi maps and kerr_ws (and a tape of D - sum P)");
-  if (p->inv_xx_x || p->c3xy) return fail(MT_EINVAL, "the TE adjoint does not transpose tensor eps or oriented poles");
+  if (p->c3xy) return fail(MT_EINVAL, "the TE adjoint does not transpose oriented poles");
+  if (p->inv_xx_x && (!p->inv_xy_x || !p->inv_yy_y || !p->inv_xy_y || !a->kerr_ws || !a->ixx_grad ||
+                      !a->ixy_x_grad || !a->iyy_grad || !a->ixy_y_grad))
+    return fail(MT_EINVAL, "adjoint with tensor eps needs the four inverse maps, their gradients and kerr_ws");
   if (p->y_mirror || p->x_mirror) return fail(MT_EINVAL, "the TE adjoint does not transpose symmetry cuts");
   if (p->npx < 0 || p->npy < 0 ||
       (p->npx && (!a->aPx_cur || !a->aPx_prev || !p->c1x || !p->c2x || !p->c3x)) ||

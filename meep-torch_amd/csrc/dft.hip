@@ -524,7 +524,7 @@ extern "C" long long mt_stage_bytes(const mt_monitor* mons, int n_mons, int stag
 
 extern "C" const char* mt_last_error(void) { return mt::g_err.c_str(); }
 
-extern "C" int mt_abi_version(void) { return 26; }
+extern "C" int mt_abi_version(void) { return 27; }
 
 extern "C" int mt_abi_layout(long long* out, int n) {
   const long long v[] = {

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libmeepfdtd.so")
 
 MT_F32, MT_F64 = 0, 1
 MAX_TERMS = 4
-ABI_VERSION = 26
+ABI_VERSION = 27
 
 c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 
@@ -85,7 +85,8 @@ class TE2DAdj(ctypes.Structure):
                 ("apsi_Hzy", c_void_p), ("apsi_Hzx", c_void_p), ("gEy", c_void_p), ("gEx", c_void_p),
                 ("gDx", c_void_p), ("gDy", c_void_p), ("epsx_grad", c_void_p), ("epsy_grad", c_void_p),
                 ("aPx_cur", c_void_p), ("aPx_prev", c_void_p), ("aPy_cur", c_void_p), ("aPy_prev", c_void_p),
-                ("kerr_ws", c_void_p)]
+                ("kerr_ws", c_void_p), ("ixx_grad", c_void_p), ("ixy_x_grad", c_void_p),
+                ("iyy_grad", c_void_p), ("ixy_y_grad", c_void_p)]
 
 
 class Yee3D(ctypes.Structure):

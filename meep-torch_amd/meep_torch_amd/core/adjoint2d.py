@@ -216,14 +216,21 @@ def _descriptor(c, fields, dtype, device):
     return d, keep
 
 
+_INV = ("inv_xx_x", "inv_xy_x", "inv_yy_y", "inv_xy_y")
+
+
 class _TERun(torch.autograd.Function):
     """TE counterpart of _TMRun: differentiable in eps_x and eps_y (the
     face-averaged permittivities of TECoeffs, stepper_te.py:160-168), also
     through the chi2/chi3 fixed point (stepper_te.py:169-175)."""
 
     @staticmethod
-    def forward(ctx, eps_x, eps_y, cfg, *state_in):
-        cdet = cfg["coeffs"]._replace(eps_x=eps_x.detach(), eps_y=eps_y.detach())
+    def forward(ctx, eps_x, eps_y, ixx, ixyx, iyy, ixyy, cfg, *state_in):
+        rep = {"eps_x": eps_x.detach(), "eps_y": eps_y.detach()}
+        inv = None if ixx is None else (ixx, ixyx, iyy, ixyy)
+        if inv is not None:   # tensor eps: the four inverse maps
+            rep.update(zip(_INV, (t.detach() for t in inv)))
+        cdet = cfg["coeffs"]._replace(**rep)
         names = cfg["names"]
         work = {f: t.detach().clone().contiguous() for f, t in zip(names, state_in)}
         acc, ckpts = _forward_chunks("te", names, work, cdet, cfg)
@@ -232,6 +239,7 @@ class _TERun(torch.autograd.Function):
         ctx.ckpts = ckpts
         ctx.save_for_backward(eps_x, eps_y)
         ctx.n_state = len(state_in)
+        ctx.tensor = inv is not None
         return (*[work[f] for f in names], *acc)
 
     @staticmethod
@@ -261,10 +269,15 @@ class _TERun(torch.autograd.Function):
             setattr(a, f, t.data_ptr())
         a.epsx_grad, a.epsy_grad = gx.data_ptr(), gy.data_ptr()
         d, d_keep = _te_descriptor(cdet, nx, ny, dtype, device)
-        if d.kerr:   # iterates and cotangents of the fixed point's transpose
+        if d.kerr or ctx.tensor:   # iterates and cotangents of the fixed point's / tensor product's transpose
             kws = torch.empty(6 * ((nx - 1) * ny + nx * (ny - 1)), dtype=dtype, device=device)
             d_keep.append(kws)
             a.kerr_ws = kws.data_ptr()
+        ginv = None
+        if ctx.tensor:
+            ginv = [torch.zeros(nx - 1, ny, dtype=dtype, device=device), torch.zeros(nx - 1, ny, dtype=dtype, device=device),
+                    torch.zeros(nx, ny - 1, dtype=dtype, device=device), torch.zeros(nx, ny - 1, dtype=dtype, device=device)]
+            a.ixx_grad, a.ixy_x_grad, a.iyy_grad, a.ixy_y_grad = (t.data_ptr() for t in ginv)
         slot = (nx - 1) * ny + nx * (ny - 1)
         for step0, k, st in reversed(ctx.ckpts):
             work = {f: t.clone() for f, t in st.items()}
@@ -278,7 +291,8 @@ class _TERun(torch.autograd.Function):
             del keep, tape
         del d_keep
         g_state = [adj[f] for f in names][:ctx.n_state]
-        return (gx.to(eps_x.dtype), gy.to(eps_y.dtype), None, *g_state)
+        gi = tuple(ginv) if ginv is not None else (None,) * 4
+        return (gx.to(eps_x.dtype), gy.to(eps_y.dtype), *gi, None, *g_state)
 
 
 def _te_descriptor(c, nx, ny, dtype, device):
@@ -299,6 +313,11 @@ def _te_descriptor(c, nx, ny, dtype, device):
     if c.c1y.shape[0]:                             # ADE poles on Ey
         d.npy = int(c.c1y.shape[0])
         media += ["c1y", "c2y", "c3y"]
+    if c.inv_xx_x is not None:                     # tensor eps (stepper_te.py:160-166)
+        for name in _INV:
+            t = getattr(c, name).to(device=device, dtype=dtype).contiguous()
+            keep.append(t)
+            setattr(d, name, t.data_ptr())
     if c.has_kerr or c.has_chi2:                   # the chi2/chi3 fixed point (stepper_te.py:169-175)
         d.kerr = 1
         for name, attr in (("chi2x", "chi2_x"), ("chi3x", "chi3_x"), ("chi2y", "chi2_y"), ("chi3y", "chi3_y")):
@@ -316,9 +335,9 @@ def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_
     if coeffs.y_mirror_lo or coeffs.x_mirror_lo or getattr(coeffs, "y_rot2_lo", 0) or getattr(coeffs, "rot4_lo", 0):
         raise NotImplementedError("gradients through symmetry-folded runs (mirrors, Rotate2, Rotate4) "
                                   "are not supported")
-    if mode == "te" and (coeffs.inv_xx_x is not None or coeffs.c3xy is not None):
-        raise NotImplementedError("gradients through TE tensor eps or oriented poles are not supported")
-    allowed = ("eps_inf", "chi2", "chi3", "c3") if mode == "tm" else ("eps_x", "eps_y")
+    if mode == "te" and coeffs.c3xy is not None:
+        raise NotImplementedError("gradients through TE oriented poles are not supported")
+    allowed = ("eps_inf", "chi2", "chi3", "c3") if mode == "tm" else ("eps_x", "eps_y") + _INV
     for name, t in coeffs._asdict().items():
         if torch.is_tensor(t) and t.requires_grad and name not in allowed:
             raise NotImplementedError(f"gradient with respect to coefficient '{name}' is not supported")
@@ -330,10 +349,11 @@ def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_
         tgt = state.Dz
         fn = _TMRun
     else:
-        names, eps = tuple(te_fields_of(state)), (coeffs.eps_x, coeffs.eps_y)
+        inv = tuple(getattr(coeffs, n) for n in _INV) if coeffs.inv_xx_x is not None else (None,) * 4
+        names, eps = tuple(te_fields_of(state)), (coeffs.eps_x, coeffs.eps_y, *inv)
         tgt = state.Dy
         fn = _TERun
-    E.require_device(*eps)
+    E.require_device(*[t for t in eps if t is not None])
     nx, ny = (state.Ez.shape if mode == "tm" else (state.Ey.shape[0], state.Ex.shape[1]))
     cfg["names"] = names
     cfg["plan"] = engine2d.source_plan(source, tgt.shape, tgt.device, tgt.dtype, engine2d.h_shapes(mode, nx, ny))

@@ -13,7 +13,8 @@ tests/test_oracle_media_golden.py):
 Fields, D and the pole stacks are compared with torch.equal in FP32 and
 FP64; the flux accumulator within 1e-4 (FP32) / 1e-12 (FP64) relative, as
 the staged DFT flush sums in its own order.  The hand-written adjoint
-refuses these media by name."""
+transposes the tensor product (and the chi2/chi3 passes after it); it
+refuses oriented poles by name."""
 import numpy as np
 import pytest
 import torch
@@ -73,9 +74,48 @@ def test_tensor_eps_matches_reference(case, tag, dtype, chunk):
     assert float((acc.cpu() - ra).norm() / ra.norm()) <= tol
 
 
-def test_tensor_eps_refuses_gradients():
+def test_oriented_poles_refuse_gradients():
     from meep_torch_amd.core.run import run_te
     g, m, src, mon, st = _setup("ten", "f64", torch.float64)
     ex = m.eps_x.clone().requires_grad_(True)
-    with pytest.raises(NotImplementedError, match="tensor eps"):
+    with pytest.raises(NotImplementedError, match="oriented poles"):
         run_te(st, m._replace(eps_x=ex), 10, source=src, monitors=[mon], checkpoint_every=5)
+
+
+@pytest.mark.parametrize("kerr", [False, True], ids=["tensor", "tensor+chi3"])
+def test_tensor_eps_gradients_match_oracle_autograd(kerr):
+    """The TE adjoint transposes the tensor-eps product E = ixx rhs_x +
+    ixy_x avg(rhs_y) (its cross terms scattered back with the other
+    average), after the chi2/chi3 passes when there are some: the gradients
+    of a flux objective with respect to the tensor's exx, exy and eyy maps
+    (through the inverse maps build_te_coeffs forms) equal CPU autograd
+    through the oracle's steps (FP64)."""
+    from meep_torch_amd.core.coeffs import build_te_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTE
+    from meep_torch_amd.core.run import run_te
+    from meep_torch_amd.core.stepper_te import init_te_state
+    from oracle import yee2d
+    g = golden("tensor_te_f64")
+    t = lambda k: torch.from_numpy(np.ascontiguousarray(g[k]))
+    dt = torch.float64
+    n = 160
+    grid = Grid(nx=NX, ny=NY, dx=0.05, dtype=dt, device="cuda")
+    ten = [t(k).cuda().requires_grad_(True) for k in ("exx", "exy", "eyy")]
+    kw = dict(chi3=t("chi3").cuda()) if kerr else {}
+    m = build_te_coeffs(grid, eps_inf=t("eps").cuda(), npml=6, eps_tensor=tuple(ten), **kw)
+    mon = FluxMonitorTE(grid, [0.6, 0.8, 1.0], ix=38, lo=7, hi=NY - 7)
+    src = _Src(t("tenk_mask").cuda(), g["tenk_amp"])
+    _, (acc,) = run_te(init_te_state(grid), m, n, source=src, monitors=[mon], checkpoint_every=60)
+    mon.flux(acc, 0.05).sum().backward()
+    torch.set_num_threads(8)
+    oten = [t(k).clone().requires_grad_(True) for k in ("exx", "exy", "eyy")]
+    okw = dict(chi3=t("chi3")) if kerr else {}
+    c = yee2d.te_coeffs(NX, NY, 0.05, t("eps"), 6, dtype=dt, eps_tensor=tuple(oten), **okw)
+    _, (oacc,) = yee2d.run2d("te", yee2d.te_state(NX, NY, dt), c, n, amps=g["tenk_amp"][:n], mask=t("tenk_mask"),
+                             monitors=[{"type": "flux", "kind": "te_x", "idx": 38, "lo": 7, "hi": NY - 7,
+                                        "freqs": [0.6, 0.8, 1.0]}])
+    refs = torch.autograd.grad(yee2d.flux_from_acc("te_x", oacc, 0.05).sum(), oten)
+    errs = [float((a.grad.cpu() - r).norm() / r.norm()) for a, r in zip(ten, refs)]
+    print(f"\n[tensor grad kerr={kerr}] rel-L2 exx/exy/eyy {errs}")
+    assert all(e <= 1e-9 for e in errs) and all(float(r.abs().max()) > 0 for r in refs)
