@@ -409,9 +409,10 @@ typedef struct {
   /* cotangents of the pole stacks Px_cur, Px_prev (npx, nx-1, ny) and
    * Py_cur, Py_prev (npy, nx, ny-1) when the problem has them; else NULL */
   void *aPx_cur, *aPx_prev, *aPy_cur, *aPy_prev;
-  /* chi2/chi3 problems (p->kerr): workspace of 6 * ((nx-1)*ny + nx*(ny-1))
-   * elements for the recomputed fixed-point iterates and their cotangents
-   * (the forward tape then holds rhs = D - sum P, not E); else NULL */
+  /* chi2/chi3 problems (p->kerr), tensor eps and oriented poles (p->c3xy):
+   * workspace of 6 * ((nx-1)*ny + nx*(ny-1)) elements for the recomputed
+   * fixed-point iterates and the cotangents of the four-point averages (under
+   * chi2/chi3 the forward tape holds rhs = D - sum P, not E); else NULL */
   void* kerr_ws;
   /* tensor eps (p->inv_xx_x set; needs kerr_ws too): gradients of the four
    * inverse maps inv_xx_x, inv_xy_x (nx-1, ny), inv_yy_y, inv_xy_y (nx, ny-1),
@@ -430,7 +431,10 @@ typedef struct {
  * chi2/chi3 (kerr = 1, with a->kerr_ws and the rhs tape): the three Jacobi
  * passes of stepper_te.py:169-175 are recomputed from the taped rhs and
  * transposed last to first, their four-point averages scattered back with
- * the other average (each is the other's transpose). */
+ * the other average (each is the other's transpose).  Oriented poles (c3xy,
+ * c3yx, with a->kerr_ws): each pole's cotangent times c3xy is summed per Ex
+ * cell and scattered onto the old Ey by the same transposed average (and
+ * c3yx onto the old Ex). */
 int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const void* tape, const mt_monitor* mons,
                     int n_mons, int step0, int k, void* stream);
 

@@ -118,6 +118,10 @@ struct AdjTe {
   // E_y = iyy rhs_y + ixyy avg(rhs_x), and the gradients of the four maps
   const T *ixx, *ixyx, *iyy, *ixyy;
   T *gixx, *gixyx, *giyy, *gixyy;
+  // oriented poles (stepper_te.py:140-145): each Px' also gains c3xy avg(Ey),
+  // each Py' c3yx avg(Ex), with the old E; their cross cotangents are summed
+  // into kga (Ex grid) / kgb (Ey grid) and scattered by te_oriented_scatter
+  const T *c3xy, *c3yx;
 };
 
 // Four-point averages of the product a * b (same neighbours and order as
@@ -138,18 +142,43 @@ __device__ __forceinline__ T avg4_prod_to_ey(const T* a, const T* b, int nx, int
 // and E' = (D' - sum P') / eps, with r = E'b / eps the cotangent that E'
 // passes to every P'.  Updates the stacks' cotangents in place and returns
 // the old E's cotangent (the old E feeds nothing else in a TE step).
+// With oriented poles (c3o) *cross gets sum c3o P'b, the cotangent of the
+// other component's averaged old E at this cell.
 template <typename T>
 __device__ __forceinline__ T te_pole_adj(int np, const T* c1, const T* c2, const T* c3, T* apc, T* app,
-                                         long long c, long long ncell, T r) {
-  T e_old = T(0);
+                                         long long c, long long ncell, T r, const T* c3o = nullptr,
+                                         T* cross = nullptr) {
+  T e_old = T(0), x = T(0);
   for (int n = 0; n < np; ++n) {
     const long long o = (long long)n * ncell + c;
     const T apn = apc[o] - r, prev = app[o];
     apc[o] = c1[o] * apn + prev;
     app[o] = c2[o] * apn;
     e_old = e_old + c3[o] * apn;
+    if (c3o) x = x + c3o[o] * apn;
   }
+  if (cross) *cross = x;
   return e_old;
+}
+
+// The oriented poles' cross terms: the old Ey was averaged onto every Ex
+// cell's Px' and the old Ex onto every Ey cell's Py', so Ey gains the
+// Ey-grid average of kga and Ex the Ex-grid average of kgb (each average is
+// the other's transpose).
+template <typename T>
+__global__ __launch_bounds__(ABX* ABY) void te_oriented_scatter(AdjTe<T> p) {
+  const int j = blockIdx.x * ABX + threadIdx.x;
+  const int i = blockIdx.y * ABY + threadIdx.y;
+  const int nx = p.nx, ny = p.ny;
+  if (i >= nx || j >= ny) return;
+  if (i < nx - 1) {
+    const long long c = (long long)i * ny + j;
+    p.aEx[c] = p.aEx[c] + avg4_to_ex(p.kgb, nx, ny, i, j);
+  }
+  if (j < ny - 1) {
+    const long long c = (long long)i * (ny - 1) + j;
+    p.aEy[c] = p.aEy[c] + avg4_to_ey(p.kga, nx, ny, i, j);
+  }
 }
 
 template <typename T>
@@ -363,8 +392,11 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
     }
     const T dxb = adx + rb;
     p.aDx[c] = p.d1x ? c1 * dxb : dxb;
-    p.aEx[c] = p.npx ? te_pole_adj(p.npx, p.c1x, p.c2x, p.c3x, p.aPxc, p.aPxp, c, (long long)(nx - 1) * ny, rb)
+    T cr = T(0);
+    p.aEx[c] = p.npx ? te_pole_adj(p.npx, p.c1x, p.c2x, p.c3x, p.aPxc, p.aPxp, c, (long long)(nx - 1) * ny, rb,
+                                   p.c3xy, &cr)
                      : T(0);
+    if (p.c3xy) p.kga[c] = cr;
     T gd = T(0);
     if (live) {
       const T g = c2 * dxb;
@@ -407,8 +439,11 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
     }
     const T dyb = ady + rb;
     p.aDy[c] = p.d1x ? c1 * dyb : dyb;
-    p.aEy[c] = p.npy ? te_pole_adj(p.npy, p.c1y, p.c2y, p.c3y, p.aPyc, p.aPyp, c, (long long)nx * (ny - 1), rb)
+    T cr = T(0);
+    p.aEy[c] = p.npy ? te_pole_adj(p.npy, p.c1y, p.c2y, p.c3y, p.aPyc, p.aPyp, c, (long long)nx * (ny - 1), rb,
+                                   p.c3yx, &cr)
                      : T(0);
+    if (p.c3yx) p.kgb[c] = cr;
     T gd = T(0);
     if (live) {
       const T g = -c2 * dyb;
@@ -495,7 +530,9 @@ int te_adj_t(const mt_te2d* d, const mt_te2d_adj* a, const void* tape, const mt_
   p.gixyx = static_cast<T*>(a->ixy_x_grad);
   p.giyy = static_cast<T*>(a->iyy_grad);
   p.gixyy = static_cast<T*>(a->ixy_y_grad);
-  if (p.kerr || p.ixx) {   // workspace: six Ex-grid arrays, then six Ey-grid arrays
+  p.c3xy = static_cast<const T*>(d->c3xy);
+  p.c3yx = static_cast<const T*>(d->c3yx);
+  if (p.kerr || p.ixx || p.c3xy) {   // workspace: six Ex-grid arrays, then six Ey-grid arrays
     const long long nex = (long long)(p.nx - 1) * p.ny, ney = (long long)p.nx * (p.ny - 1);
     T* w = static_cast<T*>(a->kerr_ws);
     T* wy = w + 6 * nex;
@@ -536,6 +573,7 @@ int te_adj_t(const mt_te2d* d, const mt_te2d_adj* a, const void* tape, const mt_
     } else {
       hipLaunchKernelGGL((te_adj_p2<T, 0>), grd, blk, 0, s, p, ts, tsy);
     }
+    if (p.c3xy) hipLaunchKernelGGL(te_oriented_scatter<T>, grd, blk, 0, s, p);
     hipLaunchKernelGGL(te_adj_p3<T>, grd, blk, 0, s, p);
     int rc = check_launch("te2d adjoint step");
     if (rc) return rc;
@@ -600,7 +638,8 @@ extern "C" int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const voi
   if (k > 0 && !tape) return fail(MT_EINVAL, "adjoint needs the forward E tape");
   if (p->kerr && (!a->kerr_ws || !p->chi2x || !p->chi3x || !p->chi2y || !p->chi3y))
     return fail(MT_EINVAL, "adjoint through chi2/chi3 needs the four chi maps and kerr_ws (and a tape of D - sum P)");
-  if (p->c3xy) return fail(MT_EINVAL, "the TE adjoint does not transpose oriented poles");
+  if ((p->c3xy || p->c3yx) && (!p->c3xy || !p->c3yx || !a->kerr_ws))
+    return fail(MT_EINVAL, "adjoint with oriented poles needs c3xy, c3yx and kerr_ws");
   if (p->inv_xx_x && (!p->inv_xy_x || !p->inv_yy_y || !p->inv_xy_y || !a->kerr_ws || !a->ixx_grad ||
                       !a->ixy_x_grad || !a->iyy_grad || !a->ixy_y_grad))
     return fail(MT_EINVAL, "adjoint with tensor eps needs the four inverse maps, their gradients and kerr_ws");

@@ -887,7 +887,6 @@ extern "C" int mt_te2d_run(const mt_te2d* p, const mt_source2d* src, const doubl
   if (p->c3xy && (!p->npx || !p->npy)) return fail(MT_EINVAL, "oriented poles need Ex and Ey pole stacks");
   if ((tensor || p->c3xy) && (!p->kx || !p->ky))
     return fail(MT_EINVAL, "tensor eps and oriented poles need the kx, ky scratch");
-  if (p->c3xy && p->tape) return fail(MT_EINVAL, "the E tape is not written with oriented poles");
   if (p->y_mirror < -1 || p->y_mirror > 1 || p->x_mirror < -1 || p->x_mirror > 1)
     return fail(MT_EINVAL, "mirror parity must be 0/+1/-1");
   const bool media = p->npx || p->npy || p->d1x || p->hz1 || p->kerr || tensor || p->c3xy || p->y_mirror ||

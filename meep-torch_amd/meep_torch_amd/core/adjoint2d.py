@@ -269,7 +269,7 @@ class _TERun(torch.autograd.Function):
             setattr(a, f, t.data_ptr())
         a.epsx_grad, a.epsy_grad = gx.data_ptr(), gy.data_ptr()
         d, d_keep = _te_descriptor(cdet, nx, ny, dtype, device)
-        if d.kerr or ctx.tensor:   # iterates and cotangents of the fixed point's / tensor product's transpose
+        if d.kerr or ctx.tensor or cdet.c3xy is not None:   # workspace of the fixed point's / tensor / cross transposes
             kws = torch.empty(6 * ((nx - 1) * ny + nx * (ny - 1)), dtype=dtype, device=device)
             d_keep.append(kws)
             a.kerr_ws = kws.data_ptr()
@@ -313,6 +313,11 @@ def _te_descriptor(c, nx, ny, dtype, device):
     if c.c1y.shape[0]:                             # ADE poles on Ey
         d.npy = int(c.c1y.shape[0])
         media += ["c1y", "c2y", "c3y"]
+    if c.c3xy is not None:                         # oriented poles (stepper_te.py:140-145)
+        for name in ("c3xy", "c3yx"):
+            t = getattr(c, name).to(device=device, dtype=dtype).contiguous()
+            keep.append(t)
+            setattr(d, name, t.data_ptr())
     if c.inv_xx_x is not None:                     # tensor eps (stepper_te.py:160-166)
         for name in _INV:
             t = getattr(c, name).to(device=device, dtype=dtype).contiguous()
@@ -335,8 +340,6 @@ def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_
     if coeffs.y_mirror_lo or coeffs.x_mirror_lo or getattr(coeffs, "y_rot2_lo", 0) or getattr(coeffs, "rot4_lo", 0):
         raise NotImplementedError("gradients through symmetry-folded runs (mirrors, Rotate2, Rotate4) "
                                   "are not supported")
-    if mode == "te" and coeffs.c3xy is not None:
-        raise NotImplementedError("gradients through TE oriented poles are not supported")
     allowed = ("eps_inf", "chi2", "chi3", "c3") if mode == "tm" else ("eps_x", "eps_y") + _INV
     for name, t in coeffs._asdict().items():
         if torch.is_tensor(t) and t.requires_grad and name not in allowed:

@@ -13,8 +13,8 @@ tests/test_oracle_media_golden.py):
 Fields, D and the pole stacks are compared with torch.equal in FP32 and
 FP64; the flux accumulator within 1e-4 (FP32) / 1e-12 (FP64) relative, as
 the staged DFT flush sums in its own order.  The hand-written adjoint
-transposes the tensor product (and the chi2/chi3 passes after it); it
-refuses oriented poles by name."""
+transposes the tensor product (and the chi2/chi3 passes after it) and the
+oriented poles' cross terms."""
 import numpy as np
 import pytest
 import torch
@@ -74,15 +74,7 @@ def test_tensor_eps_matches_reference(case, tag, dtype, chunk):
     assert float((acc.cpu() - ra).norm() / ra.norm()) <= tol
 
 
-def test_oriented_poles_refuse_gradients():
-    from meep_torch_amd.core.run import run_te
-    g, m, src, mon, st = _setup("ten", "f64", torch.float64)
-    ex = m.eps_x.clone().requires_grad_(True)
-    with pytest.raises(NotImplementedError, match="oriented poles"):
-        run_te(st, m._replace(eps_x=ex), 10, source=src, monitors=[mon], checkpoint_every=5)
-
-
-@pytest.mark.parametrize("kerr", [False, True], ids=["tensor", "tensor+chi3"])
+@pytest.mark.parametrize("kerr", [False, True, "oriented"], ids=["tensor", "tensor+chi3", "tensor+oriented"])
 def test_tensor_eps_gradients_match_oracle_autograd(kerr):
     """The TE adjoint transposes the tensor-eps product E = ixx rhs_x +
     ixy_x avg(rhs_y) (its cross terms scattered back with the other
@@ -102,17 +94,23 @@ def test_tensor_eps_gradients_match_oracle_autograd(kerr):
     n = 160
     grid = Grid(nx=NX, ny=NY, dx=0.05, dtype=dt, device="cuda")
     ten = [t(k).cuda().requires_grad_(True) for k in ("exx", "exy", "eyy")]
-    kw = dict(chi3=t("chi3").cuda()) if kerr else {}
+    ori = kerr == "oriented"   # the "ten" case's oriented pole and scalar pole
+    okw_cpu = (dict(poles=POLES2[:1], pole_masks=[t("m2")], poles_tensor=[(1.3, 0.2, (0.9, 0.4, 0.5))],
+                    pole_masks_tensor=[t("m1")]) if ori else dict(chi3=t("chi3")) if kerr else {})
+    kw = {k: ([x.cuda() for x in v] if isinstance(v, list) and k.startswith("pole_masks") else
+              v.cuda() if torch.is_tensor(v) else v) for k, v in okw_cpu.items()}
     m = build_te_coeffs(grid, eps_inf=t("eps").cuda(), npml=6, eps_tensor=tuple(ten), **kw)
     mon = FluxMonitorTE(grid, [0.6, 0.8, 1.0], ix=38, lo=7, hi=NY - 7)
-    src = _Src(t("tenk_mask").cuda(), g["tenk_amp"])
-    _, (acc,) = run_te(init_te_state(grid), m, n, source=src, monitors=[mon], checkpoint_every=60)
+    case = "ten" if ori else "tenk"
+    src = _Src(t(f"{case}_mask").cuda(), g[f"{case}_amp"])
+    st0 = init_te_state(grid, n_poles=int(m.c1x.shape[0]), n_poles_y=int(m.c1y.shape[0]))
+    _, (acc,) = run_te(st0, m, n, source=src, monitors=[mon], checkpoint_every=60)
     mon.flux(acc, 0.05).sum().backward()
     torch.set_num_threads(8)
     oten = [t(k).clone().requires_grad_(True) for k in ("exx", "exy", "eyy")]
-    okw = dict(chi3=t("chi3")) if kerr else {}
-    c = yee2d.te_coeffs(NX, NY, 0.05, t("eps"), 6, dtype=dt, eps_tensor=tuple(oten), **okw)
-    _, (oacc,) = yee2d.run2d("te", yee2d.te_state(NX, NY, dt), c, n, amps=g["tenk_amp"][:n], mask=t("tenk_mask"),
+    c = yee2d.te_coeffs(NX, NY, 0.05, t("eps"), 6, dtype=dt, eps_tensor=tuple(oten), **okw_cpu)
+    ost = yee2d.te_state(NX, NY, dt, npx=int(m.c1x.shape[0]), npy=int(m.c1y.shape[0]))
+    _, (oacc,) = yee2d.run2d("te", ost, c, n, amps=g[f"{case}_amp"][:n], mask=t(f"{case}_mask"),
                              monitors=[{"type": "flux", "kind": "te_x", "idx": 38, "lo": 7, "hi": NY - 7,
                                         "freqs": [0.6, 0.8, 1.0]}])
     refs = torch.autograd.grad(yee2d.flux_from_acc("te_x", oacc, 0.05).sum(), oten)
