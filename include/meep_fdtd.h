@@ -434,7 +434,9 @@ typedef struct {
  * the other average (each is the other's transpose).  Oriented poles (c3xy,
  * c3yx, with a->kerr_ws): each pole's cotangent times c3xy is summed per Ex
  * cell and scattered onto the old Ey by the same transposed average (and
- * c3yx onto the old Ex). */
+ * c3yx onto the old Ex).  Even mirror cuts (y_mirror / x_mirror = +1): the
+ * on-plane Dx row / Dy column took 2 Hz[0] / dx, whose cotangent returns to
+ * Hz[0]; odd cuts keep the zero row. */
 int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const void* tape, const mt_monitor* mons,
                     int n_mons, int step0, int k, void* stream);
 

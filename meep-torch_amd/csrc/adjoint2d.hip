@@ -122,6 +122,9 @@ struct AdjTe {
   // each Py' c3yx avg(Ex), with the old E; their cross cotangents are summed
   // into kga (Ex grid) / kgb (Ey grid) and scattered by te_oriented_scatter
   const T *c3xy, *c3yx;
+  // even mirror cuts at the low edges (stepper_te.py:87-97): the on-plane Dx
+  // row (ymir == 1) / Dy column (xmir == 1) took c2 * 2 Hz[0] / dx
+  int ymir, xmir;
 };
 
 // Four-point averages of the product a * b (same neighbours and order as
@@ -406,6 +409,8 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
         gd = g + a * t;
         p.apHzy[q] = b * t;
       }
+    } else if (j == 0 && p.ymir == 1) {   // the odd Hz ghost: curl = 2 Hz[i, 0] / dx, no psi at the cut
+      gd = T(2) * (c2 * dxb);
     }
     p.gDx[c] = gd;
   }
@@ -453,6 +458,8 @@ __global__ __launch_bounds__(ABX* ABY) void te_adj_p2(AdjTe<T> p, const T* tex, 
         gd = g + a * t;
         p.apHzx[q] = b * t;
       }
+    } else if (i == 0 && p.xmir == 1) {
+      gd = T(2) * (-c2 * dyb);
     }
     p.gDy[c] = gd;
   }
@@ -532,6 +539,8 @@ int te_adj_t(const mt_te2d* d, const mt_te2d_adj* a, const void* tape, const mt_
   p.gixyy = static_cast<T*>(a->ixy_y_grad);
   p.c3xy = static_cast<const T*>(d->c3xy);
   p.c3yx = static_cast<const T*>(d->c3yx);
+  p.ymir = d->y_mirror;
+  p.xmir = d->x_mirror;
   if (p.kerr || p.ixx || p.c3xy) {   // workspace: six Ex-grid arrays, then six Ey-grid arrays
     const long long nex = (long long)(p.nx - 1) * p.ny, ney = (long long)p.nx * (p.ny - 1);
     T* w = static_cast<T*>(a->kerr_ws);
@@ -643,7 +652,8 @@ extern "C" int mt_te2d_adjoint(const mt_te2d* p, const mt_te2d_adj* a, const voi
   if (p->inv_xx_x && (!p->inv_xy_x || !p->inv_yy_y || !p->inv_xy_y || !a->kerr_ws || !a->ixx_grad ||
                       !a->ixy_x_grad || !a->iyy_grad || !a->ixy_y_grad))
     return fail(MT_EINVAL, "adjoint with tensor eps needs the four inverse maps, their gradients and kerr_ws");
-  if (p->y_mirror || p->x_mirror) return fail(MT_EINVAL, "the TE adjoint does not transpose symmetry cuts");
+  if (p->y_mirror < -1 || p->y_mirror > 1 || p->x_mirror < -1 || p->x_mirror > 1)
+    return fail(MT_EINVAL, "mirror parity must be 0/+1/-1");
   if (p->npx < 0 || p->npy < 0 ||
       (p->npx && (!a->aPx_cur || !a->aPx_prev || !p->c1x || !p->c2x || !p->c3x)) ||
       (p->npy && (!a->aPy_cur || !a->aPy_prev || !p->c1y || !p->c2y || !p->c3y)))

@@ -303,6 +303,7 @@ def _te_descriptor(c, nx, ny, dtype, device):
     ey_ = c.eps_y.to(device=device, dtype=dtype).contiguous()
     d.eps_x, d.eps_y = ex_.data_ptr(), ey_.data_ptr()
     d.hx, d.hy, d.ey, d.ex = (engine2d._axis(c, n) for n in ("hx", "hy", "ey", "ex"))
+    d.y_mirror, d.x_mirror = int(c.y_mirror_lo), int(c.x_mirror_lo)   # mirror cuts (stepper_te.py:87-97)
     keep = [ex_, ey_]
     media = []
     if c.d1x is not None or c.hz1 is not None:    # conductivity (sigma_d, sigma_b, mu)
@@ -337,8 +338,8 @@ def _te_descriptor(c, nx, ny, dtype, device):
 
 
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
-    if coeffs.y_mirror_lo or coeffs.x_mirror_lo or getattr(coeffs, "y_rot2_lo", 0) or getattr(coeffs, "rot4_lo", 0):
-        raise NotImplementedError("gradients through symmetry-folded runs (mirrors, Rotate2, Rotate4) "
+    if mode == "tm" and (coeffs.y_mirror_lo or coeffs.x_mirror_lo or coeffs.y_rot2_lo or coeffs.rot4_lo):
+        raise NotImplementedError("gradients through symmetry-folded TM runs (mirrors, Rotate2, Rotate4) "
                                   "are not supported")
     allowed = ("eps_inf", "chi2", "chi3", "c3") if mode == "tm" else ("eps_x", "eps_y") + _INV
     for name, t in coeffs._asdict().items():

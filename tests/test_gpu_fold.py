@@ -7,7 +7,8 @@ phase), TE cells cut by even / odd y and x mirrors.  Every field, the
 on-plane H rows Hx0 / Hy0 and the CPML memory are compared with torch.equal
 (FP32 and FP64), the flux accumulator within 1e-4 (FP32) / 1e-12 (FP64)
 relative, as the staged DFT flush sums in its own order; a chunked run
-carries the on-plane rows across calls."""
+carries the on-plane rows across calls.  TE eps gradients through the cuts
+match CPU autograd through the oracle; TM gradients through folds raise."""
 import numpy as np
 import pytest
 import torch
@@ -75,6 +76,42 @@ def test_te_folds_match_reference(name, tag, dtype):
     st, (acc,) = run_te(init_te_state(grid), m, STEPS, source=src, monitors=[mon])
     _check(g, f"te{name}", st, acc, ("Hz", "Ex", "Ey", "Dx", "Dy", "psi_Eyx", "psi_Exy", "psi_Hzy", "psi_Hzx"),
            dtype)
+
+
+@pytest.mark.parametrize("name", sorted(FOLD_TE))
+def test_te_fold_eps_gradients_match_oracle_autograd(name):
+    """The TE adjoint transposes the even cut's on-plane row (Dx row 0 under
+    y_mirror = +1, Dy column 0 under x_mirror = +1, both driven by
+    2 Hz[0] / dx with no psi at the cut); odd cuts keep the zero row.  The
+    eps gradient of the flux objective equals CPU autograd through the
+    oracle's folded TE run (FP64)."""
+    from meep_torch_amd.core.coeffs import build_te_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTE
+    from meep_torch_amd.core.run import run_te
+    from meep_torch_amd.core.stepper_te import init_te_state
+    from oracle import yee2d
+    g = golden("fold_2d_f64")
+    kw = FOLD_TE[name]
+    grid = Grid(nx=N, ny=N, dx=0.05, dtype=torch.float64, device="cuda")
+    eps = torch.from_numpy(g["eps"]).cuda().requires_grad_(True)
+    m = build_te_coeffs(grid, eps_inf=eps, npml=te_npml(kw), **kw)
+    mon = FluxMonitorTE(grid, FREQS, ix=26, lo=0, hi=28)
+    print(f"[fold] te gradient {name}", flush=True)
+    mask = torch.from_numpy(g[f"te{name}_mask"])
+    _, (acc,) = run_te(init_te_state(grid), m, STEPS, source=_Src(mask.cuda(), g["amp"]), monitors=[mon],
+                       checkpoint_every=70)
+    mon.flux(acc, 0.05).sum().backward()
+    torch.set_num_threads(8)
+    e = torch.from_numpy(g["eps"]).clone().requires_grad_(True)
+    c = yee2d.te_coeffs(N, N, 0.05, e, te_npml(kw), dtype=torch.float64, **kw)
+    _, (oacc,) = yee2d.run2d("te", yee2d.te_state(N, N, torch.float64), c, STEPS, amps=g["amp"], mask=mask,
+                             monitors=[{"type": "flux", "kind": "te_x", "idx": 26, "lo": 0, "hi": 28,
+                                        "freqs": FREQS}])
+    ref, = torch.autograd.grad(yee2d.flux_from_acc("te_x", oacc, 0.05).sum(), e)
+    err = float((eps.grad.cpu() - ref).norm() / ref.norm())
+    print(f"[fold] te gradient {name}: rel-L2 {err:.2e}", flush=True)
+    assert err <= 1e-10, (name, err)
 
 
 def test_folds_refuse_gradients_and_batches():
