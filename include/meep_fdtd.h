@@ -394,7 +394,9 @@ typedef struct {
  * aM* cotangents of a: B' = B +- dt curl, M' = (mc1 M + mc2 M_prev) + mc3 H,
  * H' = B' - sum M').  Replaces
  * PyTorch autograd through run_tm's checkpointed chunks
- * (meep_torch/core/run.py:99-120, eps reached via stepper_tm.py:226-227). */
+ * (meep_torch/core/run.py:99-120, eps reached via stepper_tm.py:226-227).
+ * Even mirror cuts (y_mirror / x_mirror = +1) are transposed with the
+ * forward's live Dz set and psi indexing; odd and Rotate cuts are refused. */
 int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const void* tape,
                     const mt_monitor* mons, int n_mons, int step0, int k, void* stream);
 

@@ -12,6 +12,7 @@ constexpr int ABX = 64, ABY = 4;
 template <typename T>
 struct AdjK {
   int nx, ny, y_pmc, ny_e;
+  int ycut, xcut;  // even mirror cuts at the low edges: the cut row / column of Dz is live
   T dt, dx;
   const T* eps;
   Pml<T> hy, hx, ex, ey;
@@ -90,8 +91,10 @@ template <typename T>
 __device__ __forceinline__ void adj_a_cell(const AdjK<T>& p, const T* ez_after, int i, int j, int s_rel = 0) {
   const int nx = p.nx, ny = p.ny;
   const long long c = (long long)i * ny + j;
-  const bool live = i >= 1 && i < nx - 1 && (p.y_pmc || (j >= 1 && j < ny - 1));
-  const int ii = i - 1, jj = p.y_pmc ? j : j - 1;
+  // live Dz as in the forward (stepper_tm.py:134-155); beyond an even cut H is 0
+  const bool live = (p.xcut ? i < nx - 1 : (i >= 1 && i < nx - 1)) &&
+                    (p.y_pmc || (p.ycut ? j < ny - 1 : (j >= 1 && j < ny - 1)));
+  const int ii = p.xcut ? i : i - 1, jj = (p.y_pmc || p.ycut) ? j : j - 1;
   const long long q = (long long)ii * p.ny_e + jj;
   const T eps = p.eps[c], ezb = p.aEz[c], eg = p.eps_grad[c], ea = ez_after[c], dz0 = p.aDz[c];
   const T d1 = p.d1 ? p.d1[c] : T(1), d2 = p.d1 ? p.d2[c] : p.dt;
@@ -554,7 +557,9 @@ AdjK<T> make_adjk(const mt_tm2d* d, const mt_tm2d_adj* a) {
   p.nx = d->nx;
   p.ny = d->ny;
   p.y_pmc = d->y_pmc;
-  p.ny_e = d->y_pmc ? d->ny : d->ny - 2;
+  p.ycut = d->y_mirror == 1;
+  p.xcut = d->x_mirror == 1;
+  p.ny_e = d->y_pmc ? d->ny : (p.ycut ? d->ny - 1 : d->ny - 2);
   p.dt = static_cast<T>(d->dt);
   p.dx = static_cast<T>(d->dx);
   p.eps = static_cast<const T*>(d->eps);

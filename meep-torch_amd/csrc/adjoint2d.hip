@@ -615,8 +615,10 @@ extern "C" int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const voi
                        !p->mc1x || !p->mc2x || !p->mc3x || !p->mc1y || !p->mc2y || !p->mc3y)))
     return fail(MT_EINVAL, "adjoint with magnetic poles needs mc1x..mc3y and the B / M cotangents");
   if (p->n_mpoles && p->h1x) return fail(MT_EINVAL, "magnetic poles do not combine with sigma_b / mu");
-  if (p->y_mirror || p->x_mirror || p->y_rot2 || p->rot4)
-    return fail(MT_EINVAL, "the TM adjoint does not transpose symmetry cuts");
+  if (p->y_mirror == -1 || p->x_mirror == -1 || p->y_rot2 || p->rot4)
+    return fail(MT_EINVAL, "the TM adjoint transposes even mirror cuts only (no odd or Rotate cuts)");
+  if (p->y_mirror < -1 || p->y_mirror > 1 || p->x_mirror < -1 || p->x_mirror > 1 || (p->y_mirror && p->y_pmc))
+    return fail(MT_EINVAL, "mirror parity must be 0/+1/-1, without y_pmc");
   if (!p->d1 != !p->d2 || !p->h1x != !p->h2x || !p->h1y != !p->h2y || !p->d1 != !p->h1x || !p->d1 != !p->h1y)
     return fail(MT_EINVAL, "conductivity needs all of d1, d2, h1x, h2x, h1y, h2y");
   if (!p->eps || !a->aEz || !a->aDz || !a->aHx || !a->aHy || !a->apsi_Ezy || !a->apsi_Ezx ||
@@ -624,8 +626,8 @@ extern "C" int mt_tm2d_adjoint(const mt_tm2d* p, const mt_tm2d_adj* a, const voi
     return fail(MT_EINVAL, "null adjoint pointer");
   int rc;
   if ((rc = check_axis(p->hy, p->ny - 1, "hy")) || (rc = check_axis(p->hx, p->nx - 1, "hx")) ||
-      (rc = check_axis(p->ex, p->nx - 2, "ex")) ||
-      (rc = check_axis(p->ey, p->y_pmc ? p->ny : p->ny - 2, "ey")))
+      (rc = check_axis(p->ex, p->x_mirror == 1 ? p->nx - 1 : p->nx - 2, "ex")) ||
+      (rc = check_axis(p->ey, p->y_pmc ? p->ny : (p->y_mirror == 1 ? p->ny - 1 : p->ny - 2), "ey")))
     return rc;
   for (int i = 0; i < n_mons; ++i) {
     const mt_monitor& m = mons[i];

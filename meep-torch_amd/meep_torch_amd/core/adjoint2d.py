@@ -188,6 +188,7 @@ def _descriptor(c, fields, dtype, device):
     d = E.TM2D()
     nx, ny = fields["Ez"].shape
     d.nx, d.ny, d.dtype, d.y_pmc = nx, ny, E.dtype_code(dtype), int(bool(c.y_pmc))
+    d.y_mirror, d.x_mirror = int(c.y_mirror_lo), int(c.x_mirror_lo)   # even cuts (stepper_tm.py:134-155)
     d.dt, d.dx = float(c.dt), float(c.dx)
     eps = c.eps_inf.to(device=device, dtype=dtype).contiguous()
     d.eps = eps.data_ptr()
@@ -338,8 +339,9 @@ def _te_descriptor(c, nx, ny, dtype, device):
 
 
 def run_with_adjoint(mode, state, coeffs, n_steps, source, monitors, checkpoint_every):
-    if mode == "tm" and (coeffs.y_mirror_lo or coeffs.x_mirror_lo or coeffs.y_rot2_lo or coeffs.rot4_lo):
-        raise NotImplementedError("gradients through symmetry-folded TM runs (mirrors, Rotate2, Rotate4) "
+    if mode == "tm" and (coeffs.y_mirror_lo == -1 or coeffs.x_mirror_lo == -1 or coeffs.y_rot2_lo
+                         or coeffs.rot4_lo):
+        raise NotImplementedError("gradients through odd-mirror or Rotate2 / Rotate4 symmetry cuts of TM runs "
                                   "are not supported")
     allowed = ("eps_inf", "chi2", "chi3", "c3") if mode == "tm" else ("eps_x", "eps_y") + _INV
     for name, t in coeffs._asdict().items():

@@ -8,13 +8,14 @@ on-plane H rows Hx0 / Hy0 and the CPML memory are compared with torch.equal
 (FP32 and FP64), the flux accumulator within 1e-4 (FP32) / 1e-12 (FP64)
 relative, as the staged DFT flush sums in its own order; a chunked run
 carries the on-plane rows across calls.  TE eps gradients through the cuts
-match CPU autograd through the oracle; TM gradients through folds raise."""
+match CPU autograd through the oracle, and so do TM gradients through even
+mirror cuts; TM gradients through odd or Rotate cuts raise."""
 import numpy as np
 import pytest
 import torch
 
 from conftest import golden, has_gpu
-from test_oracle_fold_golden import FOLD_TE, FOLD_TM, FREQS, N, STEPS, te_npml, tm_npml
+from test_oracle_fold_golden import FOLD_TE, FOLD_TM, FREQS, N, STEPS, te_npml, tm_npml  # noqa: F401
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a HIP device")]
 
@@ -111,6 +112,41 @@ def test_te_fold_eps_gradients_match_oracle_autograd(name):
     ref, = torch.autograd.grad(yee2d.flux_from_acc("te_x", oacc, 0.05).sum(), e)
     err = float((eps.grad.cpu() - ref).norm() / ref.norm())
     print(f"[fold] te gradient {name}: rel-L2 {err:.2e}", flush=True)
+    assert err <= 1e-10, (name, err)
+
+
+@pytest.mark.parametrize("name", ["ym+", "xm+"])
+def test_tm_even_fold_eps_gradients_match_oracle_autograd(name):
+    """Under an even TM cut the cut row / column of Dz is live with zero H
+    beyond it and its psi arrays start at the cut; the adjoint's A phase
+    follows the same live set and psi indexing.  The eps gradient of the
+    flux objective equals CPU autograd through the oracle (FP64)."""
+    from meep_torch_amd.core.coeffs import build_tm_coeffs
+    from meep_torch_amd.core.grid import Grid
+    from meep_torch_amd.core.monitors import FluxMonitorTM
+    from meep_torch_amd.core.run import run_tm
+    from meep_torch_amd.core.stepper_tm import init_tm_state
+    from oracle import yee2d
+    g = golden("fold_2d_f64")
+    kw = FOLD_TM[name]
+    grid = Grid(nx=N, ny=N, dx=0.05, dtype=torch.float64, device="cuda")
+    eps = torch.from_numpy(g["eps"]).cuda().requires_grad_(True)
+    m = build_tm_coeffs(grid, eps_inf=eps, npml=tm_npml(kw), **kw)
+    mon = FluxMonitorTM(grid, FREQS, x_normal=True, idx=26, lo=0, hi=28)
+    print(f"[fold] tm gradient {name}", flush=True)
+    mask = torch.from_numpy(g[f"tm{name}_mask"])
+    _, (acc,) = run_tm(init_tm_state(grid, **kw), m, STEPS, source=_Src(mask.cuda(), g["amp"]), monitors=[mon],
+                       checkpoint_every=70)
+    mon.flux(acc, 0.05).sum().backward()
+    torch.set_num_threads(8)
+    e = torch.from_numpy(g["eps"]).clone().requires_grad_(True)
+    c = yee2d.tm_coeffs(N, N, 0.05, e, tm_npml(kw), dtype=torch.float64, **kw)
+    _, (oacc,) = yee2d.run2d("tm", yee2d.tm_state(N, N, torch.float64, **kw), c, STEPS, amps=g["amp"], mask=mask,
+                             monitors=[{"type": "flux", "kind": "tm_x", "idx": 26, "lo": 0, "hi": 28,
+                                        "freqs": FREQS}])
+    ref, = torch.autograd.grad(yee2d.flux_from_acc("tm_x", oacc, 0.05).sum(), e)
+    err = float((eps.grad.cpu() - ref).norm() / ref.norm())
+    print(f"[fold] tm gradient {name}: rel-L2 {err:.2e}", flush=True)
     assert err <= 1e-10, (name, err)
 
 
